@@ -1,0 +1,1213 @@
+/*
+ * jfem_oracle.c -- CPU restatement of OpenJFEM's SOL 101 hot path (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP implementation.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * Each function names the reference file:line it restates (paths relative to
+ * the reference repository root).  Arithmetic follows the reference's statement
+ * order "as written" (accumulation order of ts_mul!/ts_mul_At_add!), built with
+ * -ffp-contract=off.  Defaults are those of an unset JFEM_* environment.
+ */
+#include "jfem_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#define M(A, i, j, ld) (A)[(i) + (ld) * (j)]
+static const double EPS = 2.220446049250313e-16; /* eps(Float64) */
+
+void or_free(void *p) { free(p); }
+
+/* ------------------------------------------------------------------------- */
+/* small dense helpers                                                        */
+/* ------------------------------------------------------------------------- */
+
+/* shape_derivs_quad, src/FEMKernels.jl:762-766 */
+static void shape_derivs_quad(double r, double s, double dNr[4], double dNs[4]) {
+    dNr[0] = -0.25 * (1 - s); dNr[1] = 0.25 * (1 - s);
+    dNr[2] = 0.25 * (1 + s);  dNr[3] = -0.25 * (1 + s);
+    dNs[0] = -0.25 * (1 - r); dNs[1] = -0.25 * (1 + r);
+    dNs[2] = 0.25 * (1 + r);  dNs[3] = 0.25 * (1 - r);
+}
+
+static void jac4(const double dNr[4], const double dNs[4], const double lc[8],
+                 double *J11, double *J12, double *J21, double *J22) {
+    /* coords[k,1] = lc[2k], coords[k,2] = lc[2k+1] */
+    *J11 = dNr[0] * lc[0] + dNr[1] * lc[2] + dNr[2] * lc[4] + dNr[3] * lc[6];
+    *J12 = dNr[0] * lc[1] + dNr[1] * lc[3] + dNr[2] * lc[5] + dNr[3] * lc[7];
+    *J21 = dNs[0] * lc[0] + dNs[1] * lc[2] + dNs[2] * lc[4] + dNs[3] * lc[6];
+    *J22 = dNs[0] * lc[1] + dNs[1] * lc[3] + dNs[2] * lc[5] + dNs[3] * lc[7];
+}
+
+/* _quad4_jacobian_measure, src/FEMKernels.jl:2862-2867 */
+static int jac_measure(double detj, double j11, double j12, double j21, double j22, double *out) {
+    double scale = fabs(j11 * j22) + fabs(j12 * j21);
+    if (!(isfinite(detj) && isfinite(scale) && fabs(detj) > 64 * EPS * scale)) return OR_ERR_DEGENERATE;
+    *out = fabs(detj);
+    return OR_OK;
+}
+
+/* C = A*B, ts_mul! order (src/FEMKernels.jl:664-675): C[i,j] += A[i,l]*B[l,j], l ascending */
+static void ts_mul(double *C, const double *A, const double *B, int m, int k, int n) {
+    for (int q = 0; q < m * n; ++q) C[q] = 0.0;
+    for (int j = 0; j < n; ++j)
+        for (int l = 0; l < k; ++l) {
+            double v = B[l + k * j];
+            for (int i = 0; i < m; ++i) C[i + m * j] += A[i + m * l] * v;
+        }
+}
+/* C += alpha*A'*B, ts_mul_At_add! order (src/FEMKernels.jl:649-661); A is k x m */
+static void ts_mul_At_add(double *C, const double *A, const double *B, double alpha,
+                          int k, int m, int n) {
+    for (int j = 0; j < n; ++j)
+        for (int l = 0; l < k; ++l) {
+            double v = alpha * B[l + k * j];
+            for (int i = 0; i < m; ++i) C[i + m * j] += A[l + k * i] * v;
+        }
+}
+
+/* 4x4 inverse by cofactors (stands in for StaticArrays inv(SMatrix{4,4})) */
+static int inv4(const double *a, double *out) {
+    double inv[16];
+    inv[0] = a[5]*a[10]*a[15] - a[5]*a[11]*a[14] - a[9]*a[6]*a[15] + a[9]*a[7]*a[14] + a[13]*a[6]*a[11] - a[13]*a[7]*a[10];
+    inv[4] = -a[4]*a[10]*a[15] + a[4]*a[11]*a[14] + a[8]*a[6]*a[15] - a[8]*a[7]*a[14] - a[12]*a[6]*a[11] + a[12]*a[7]*a[10];
+    inv[8] = a[4]*a[9]*a[15] - a[4]*a[11]*a[13] - a[8]*a[5]*a[15] + a[8]*a[7]*a[13] + a[12]*a[5]*a[11] - a[12]*a[7]*a[9];
+    inv[12] = -a[4]*a[9]*a[14] + a[4]*a[10]*a[13] + a[8]*a[5]*a[14] - a[8]*a[6]*a[13] - a[12]*a[5]*a[10] + a[12]*a[6]*a[9];
+    inv[1] = -a[1]*a[10]*a[15] + a[1]*a[11]*a[14] + a[9]*a[2]*a[15] - a[9]*a[3]*a[14] - a[13]*a[2]*a[11] + a[13]*a[3]*a[10];
+    inv[5] = a[0]*a[10]*a[15] - a[0]*a[11]*a[14] - a[8]*a[2]*a[15] + a[8]*a[3]*a[14] + a[12]*a[2]*a[11] - a[12]*a[3]*a[10];
+    inv[9] = -a[0]*a[9]*a[15] + a[0]*a[11]*a[13] + a[8]*a[1]*a[15] - a[8]*a[3]*a[13] - a[12]*a[1]*a[11] + a[12]*a[3]*a[9];
+    inv[13] = a[0]*a[9]*a[14] - a[0]*a[10]*a[13] - a[8]*a[1]*a[14] + a[8]*a[2]*a[13] + a[12]*a[1]*a[10] - a[12]*a[2]*a[9];
+    inv[2] = a[1]*a[6]*a[15] - a[1]*a[7]*a[14] - a[5]*a[2]*a[15] + a[5]*a[3]*a[14] + a[13]*a[2]*a[7] - a[13]*a[3]*a[6];
+    inv[6] = -a[0]*a[6]*a[15] + a[0]*a[7]*a[14] + a[4]*a[2]*a[15] - a[4]*a[3]*a[14] - a[12]*a[2]*a[7] + a[12]*a[3]*a[6];
+    inv[10] = a[0]*a[5]*a[15] - a[0]*a[7]*a[13] - a[4]*a[1]*a[15] + a[4]*a[3]*a[13] + a[12]*a[1]*a[7] - a[12]*a[3]*a[5];
+    inv[14] = -a[0]*a[5]*a[14] + a[0]*a[6]*a[13] + a[4]*a[1]*a[14] - a[4]*a[2]*a[13] - a[12]*a[1]*a[6] + a[12]*a[2]*a[5];
+    inv[3] = -a[1]*a[6]*a[11] + a[1]*a[7]*a[10] + a[5]*a[2]*a[11] - a[5]*a[3]*a[10] - a[9]*a[2]*a[7] + a[9]*a[3]*a[6];
+    inv[7] = a[0]*a[6]*a[11] - a[0]*a[7]*a[10] - a[4]*a[2]*a[11] + a[4]*a[3]*a[10] + a[8]*a[2]*a[7] - a[8]*a[3]*a[6];
+    inv[11] = -a[0]*a[5]*a[11] + a[0]*a[7]*a[9] + a[4]*a[1]*a[11] - a[4]*a[3]*a[9] - a[8]*a[1]*a[7] + a[8]*a[3]*a[5];
+    inv[15] = a[0]*a[5]*a[10] - a[0]*a[6]*a[9] - a[4]*a[1]*a[10] + a[4]*a[2]*a[9] + a[8]*a[1]*a[6] - a[8]*a[2]*a[5];
+    double det = a[0] * inv[0] + a[1] * inv[4] + a[2] * inv[8] + a[3] * inv[12];
+    if (det == 0.0 || !isfinite(det)) return OR_ERR_DEGENERATE;
+    for (int i = 0; i < 16; ++i) out[i] = inv[i] / det;
+    return OR_OK;
+}
+
+/* LU with partial pivoting, solve A X = B (n x n, nrhs columns); stands in for the
+ * LAPACK getrf/getrs replicas of src/FEMKernels.jl:450-561 */
+static int lu_solve(const double *Ain, int n, double *B, int nrhs) {
+    double A[64]; int piv[8];
+    memcpy(A, Ain, sizeof(double) * n * n);
+    for (int k = 0; k < n; ++k) {
+        int p = k; double mx = fabs(M(A, k, k, n));
+        for (int i = k + 1; i < n; ++i) if (fabs(M(A, i, k, n)) > mx) { mx = fabs(M(A, i, k, n)); p = i; }
+        piv[k] = p;
+        if (mx == 0.0) return OR_ERR_DEGENERATE;
+        if (p != k) for (int j = 0; j < n; ++j) { double t = M(A, k, j, n); M(A, k, j, n) = M(A, p, j, n); M(A, p, j, n) = t; }
+        for (int i = k + 1; i < n; ++i) {
+            M(A, i, k, n) /= M(A, k, k, n);
+            for (int j = k + 1; j < n; ++j) M(A, i, j, n) -= M(A, i, k, n) * M(A, k, j, n);
+        }
+    }
+    for (int c = 0; c < nrhs; ++c) {
+        double *b = B + n * c;
+        for (int k = 0; k < n; ++k) if (piv[k] != k) { double t = b[k]; b[k] = b[piv[k]]; b[piv[k]] = t; }
+        for (int i = 0; i < n; ++i) for (int j = 0; j < i; ++j) b[i] -= M(A, i, j, n) * b[j];
+        for (int i = n - 1; i >= 0; --i) { for (int j = i + 1; j < n; ++j) b[i] -= M(A, i, j, n) * b[j]; b[i] /= M(A, i, i, n); }
+    }
+    return OR_OK;
+}
+
+/* symmetric 3x3 inverse via adjugate (stands in for inv(Symmetric(...))) */
+static int inv3(const double *a, double *o) {
+    double c00 = a[4] * a[8] - a[7] * a[5];
+    double c01 = a[7] * a[2] - a[1] * a[8];
+    double c02 = a[1] * a[5] - a[4] * a[2];
+    double det = a[0] * c00 + a[3] * c01 + a[6] * c02;
+    if (det == 0.0 || !isfinite(det)) return OR_ERR_DEGENERATE;
+    o[0] = c00 / det; o[3] = (a[6] * a[5] - a[3] * a[8]) / det; o[6] = (a[3] * a[7] - a[6] * a[4]) / det;
+    o[1] = c01 / det; o[4] = (a[0] * a[8] - a[6] * a[2]) / det; o[7] = (a[6] * a[1] - a[0] * a[7]) / det;
+    o[2] = c02 / det; o[5] = (a[3] * a[2] - a[0] * a[5]) / det; o[8] = (a[0] * a[4] - a[3] * a[1]) / det;
+    return OR_OK;
+}
+
+static double maxabs(const double *a, int n) { double m = 0; for (int i = 0; i < n; ++i) if (fabs(a[i]) > m) m = fabs(a[i]); return m; }
+
+/* ------------------------------------------------------------------------- */
+/* CQUAD4: MacNeal transverse shear                                           */
+/* add_quad4_macneal_shear_rbf! default slice, src/FEMKernels.jl:4481-4834,   */
+/* 6188-6285; quad4_natural_shear_flexibility!, :4223-4352                    */
+/* ------------------------------------------------------------------------- */
+int or_q4_macneal_parts(const double lc[8], const double Cb[9], const double Cs[4],
+                        int rigid, double D[48], double Zb[16], double Zs[16],
+                        double Z[16], double Kp[144]) {
+    const double pt = 1.0 / sqrt(3.0);
+    /* sample points (xi, eta, comp) :4509; edge rows at (0,-+1), (-+1,0) :4688-4712 */
+    const double sxi[4] = {0.0, 0.0, -pt, pt}, seta[4] = {-pt, pt, 0.0, 0.0};
+    const int comp[4] = {1, 1, 2, 2};
+    double Ce[4 * 12], Je[4][4];
+    for (int sp = 0; sp < 4; ++sp) {
+        double xi = sxi[sp], eta = seta[sp];
+        double xr = xi == 0.0 ? 0.0 : copysign(1.0, xi);
+        double er = eta == 0.0 ? 0.0 : copysign(1.0, eta);
+        double dNre[4], dNse[4], J11e, J12e, J21e, J22e;
+        shape_derivs_quad(xr, er, dNre, dNse);
+        jac4(dNre, dNse, lc, &J11e, &J12e, &J21e, &J22e);
+        double Ne[4] = {0.25 * (1 - xr) * (1 - er), 0.25 * (1 + xr) * (1 - er),
+                        0.25 * (1 + xr) * (1 + er), 0.25 * (1 - xr) * (1 + er)};
+        Je[sp][0] = J11e; Je[sp][1] = J12e; Je[sp][2] = J21e; Je[sp][3] = J22e;
+        double txe = comp[sp] == 1 ? J11e : J21e, tye = comp[sp] == 1 ? J12e : J22e;
+        for (int k = 0; k < 4; ++k) {
+            int col = 3 * k;
+            M(Ce, sp, col + 0, 4) = comp[sp] == 1 ? dNre[k] : dNse[k];
+            M(Ce, sp, col + 1, 4) = -Ne[k] * tye;
+            M(Ce, sp, col + 2, 4) = Ne[k] * txe;
+        }
+    }
+    /* interpolate covariant rows and map with the edge-point inverse Jacobian :4758-4797 */
+    for (int sp = 0; sp < 4; ++sp) {
+        double xi = sxi[sp], eta = seta[sp];
+        double J11 = Je[sp][0], J12 = Je[sp][1], J21 = Je[sp][2], J22 = Je[sp][3];
+        double detJ = J11 * J22 - J12 * J21;
+        double adet = fabs(detJ) < 1e-14 ? (detJ < 0 ? -1e-14 : 1e-14) : detJ;
+        double i11 = J22 / adet, i12 = -J12 / adet, i21 = -J21 / adet, i22 = J11 / adet;
+        double w1 = 0.5 * (1.0 - eta / pt), w2 = 0.5 * (1.0 + eta / pt);
+        double w3 = 0.5 * (1.0 - xi / pt), w4 = 0.5 * (1.0 + xi / pt);
+        for (int j = 0; j < 12; ++j) {
+            double gxie = w1 * M(Ce, 0, j, 4) + w2 * M(Ce, 1, j, 4);
+            double getae = w3 * M(Ce, 2, j, 4) + w4 * M(Ce, 3, j, 4);
+            M(D, sp, j, 4) = sp < 2 ? (i11 * gxie + i12 * getae) : (i21 * gxie + i22 * getae);
+        }
+    }
+    /* natural flexibility */
+    const double rn[4] = {-1.0, 1.0, 1.0, -1.0}, sn[4] = {-1.0, -1.0, 1.0, 1.0};
+    double tr[16] = {0};
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k) {
+            int base = 3 * k;
+            M(tr, i, 0, 4) += M(D, i, base + 2, 4);
+            M(tr, i, 1, 4) += sn[k] * M(D, i, base + 2, 4);
+            M(tr, i, 2, 4) -= M(D, i, base + 1, 4);
+            M(tr, i, 3, 4) -= rn[k] * M(D, i, base + 1, 4);
+        }
+    const double x1 = lc[0], x2 = lc[2], x3 = lc[4], x4 = lc[6];
+    const double y1 = lc[1], y2 = lc[3], y3 = lc[5], y4 = lc[7];
+    double ax = 0.25 * (-x1 + x2 + x3 - x4), ay = 0.25 * (-y1 + y2 + y3 - y4);
+    double bx = 0.25 * (-x1 - x2 + x3 + x4), by = 0.25 * (-y1 - y2 + y3 + y4);
+    double cx = 0.25 * (x1 - x2 + x3 - x4), cy = 0.25 * (y1 - y2 + y3 - y4);
+    double det0 = ax * by - ay * bx;
+    double det_scale = hypot(ax, ay) * hypot(bx, by);
+    if (det_scale < EPS) det_scale = EPS;
+    if (!(fabs(det0) > 64 * EPS * det_scale)) return OR_ERR_DEGENERATE;
+    double orient = det0 > 0 ? 1.0 : (det0 < 0 ? -1.0 : 0.0);
+    const double cr[4] = {-1, 1, 1, -1}, cs_[4] = {-1, -1, 1, 1};
+    for (int q = 0; q < 4; ++q) {
+        double r = cr[q], s = cs_[q];
+        double dj = (ax + cx * s) * (by + cy * r) - (ay + cy * s) * (bx + cx * r);
+        if (!(orient * dj > 64 * EPS * det_scale)) return OR_ERR_DEGENERATE;
+    }
+    double area = 4 * fabs(det0);
+    double dx2 = (2 * ax) * (2 * ax), dy2 = (2 * by) * (2 * by);
+    double dxr = M(Cb, 0, 0, 3), dyr = M(Cb, 1, 1, 3);
+    double fx = fabs(2 * ax) > 1e-30 ? (cx / (2 * ax)) * (cx / (2 * ax)) : 0.0;
+    double fy = fabs(2 * by) > 1e-30 ? (cy / (2 * by)) * (cy / (2 * by)) : 0.0;
+    if (fx + fy > 1e-24 && ((dxr == 0.0) != (dyr == 0.0))) return OR_ERR_BAD_ARG;
+    double ux = dx2 / ((12 * area) * fmax(fabs(dxr), 1e-30));
+    double uy = dy2 / ((12 * area) * fmax(fabs(dyr), 1e-30));
+    double denx = dy2 + 39 * dx2, deny = dx2 + 39 * dy2;
+    if (!(denx > 0 && deny > 0)) return OR_ERR_DEGENERATE;
+    double zb[4] = {ux, ((3 * dy2) / denx) * ux, uy, ((3 * dx2) / deny) * uy};
+    for (int q = 0; q < 16; ++q) { Zb[q] = 0; Zs[q] = 0; }
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i)
+            for (int k = 0; k < 4; ++k) M(Zb, i, j, 4) += (M(tr, i, k, 4) * zb[k]) * M(tr, j, k, 4);
+    if (!rigid) {
+        double gp = 1.0 / sqrt(3.0);
+#define JAC(r, s) fabs((ax + cx * (s)) * (by + cy * (r)) - (ay + cy * (s)) * (bx + cx * (r)))
+        double va = sqrt(2 * JAC(0.0, -gp)), vb = sqrt(2 * JAC(0.0, gp));
+        double vc = sqrt(2 * JAC(-gp, 0.0)), vd = sqrt(2 * JAC(gp, 0.0));
+#undef JAC
+        double pxa[4] = {va, -gp * va, 0, 0}, pxb[4] = {vb, gp * vb, 0, 0};
+        double pyc[4] = {0, 0, vc, -gp * vc}, pyd[4] = {0, 0, vd, gp * vd};
+        double H[16], Hi[16];
+        for (int j = 0; j < 4; ++j)
+            for (int i = 0; i < 4; ++i)
+                M(H, i, j, 4) = M(Cs, 0, 0, 2) * (pxa[i] * pxa[j] + pxb[i] * pxb[j]) +
+                                M(Cs, 1, 1, 2) * (pyc[i] * pyc[j] + pyd[i] * pyd[j]) +
+                                M(Cs, 0, 1, 2) * (pxa[i] + pxb[i]) * (pyc[j] + pyd[j]) / 2 +
+                                M(Cs, 1, 0, 2) * (pyc[i] + pyd[i]) * (pxa[j] + pxb[j]) / 2;
+        for (int q = 0; q < 16; ++q) Hi[q] = (q % 5 == 0) ? 1.0 : 0.0;
+        if (lu_solve(H, 4, Hi, 4) != OR_OK) return OR_ERR_DEGENERATE;
+        double t4[16];
+        ts_mul(t4, tr, Hi, 4, 4, 4);
+        for (int j = 0; j < 4; ++j)
+            for (int i = 0; i < 4; ++i) {
+                double s = 0; for (int k = 0; k < 4; ++k) s += M(t4, i, k, 4) * M(tr, j, k, 4);
+                M(Zs, i, j, 4) = s;
+            }
+    }
+    for (int q = 0; q < 16; ++q) Z[q] = Zb[q] + Zs[q];
+    /* K_plate = sym(D' * (Z \ D)) :6190-6200 */
+    double sol[48];
+    memcpy(sol, D, sizeof(sol));
+    if (lu_solve(Z, 4, sol, 12) != OR_OK) return OR_ERR_DEGENERATE;
+    double raw[144];
+    for (int j = 0; j < 12; ++j)
+        for (int i = 0; i < 12; ++i) {
+            double s = 0; for (int k = 0; k < 4; ++k) s += M(D, k, i, 4) * M(sol, k, j, 4);
+            M(raw, i, j, 12) = s;
+        }
+    for (int j = 0; j < 12; ++j)
+        for (int i = 0; i < 12; ++i) M(Kp, i, j, 12) = 0.5 * (M(raw, i, j, 12) + M(raw, j, i, 12));
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* CQUAD4 core, _stiffness_quad4_core! default branch set,                    */
+/* src/FEMKernels.jl:2874-4160 (see the per-step citations below)             */
+/* ------------------------------------------------------------------------- */
+/* inverse of K[active, active] scattered into a 4x4 (zeros elsewhere):
+ * q4_internal_mode_compliance with binary weights, src/FEMKernels.jl:1918-1955 */
+static int active_inverse(const double *K, int mask, double *C) {
+    int act[4], na = 0;
+    for (int i = 0; i < 4; ++i) if (mask & (1 << i)) act[na++] = i;
+    for (int q = 0; q < 16; ++q) C[q] = 0.0;
+    if (na == 0) return OR_OK;
+    if (na == 4) return inv4(K, C);
+    double sub[16], inv[16];
+    for (int j = 0; j < na; ++j) for (int i = 0; i < na; ++i) sub[i + na * j] = M(K, act[i], act[j], 4);
+    for (int j = 0; j < na; ++j) for (int i = 0; i < na; ++i) inv[i + na * j] = (i == j) ? 1.0 : 0.0;
+    if (lu_solve(sub, na, inv, na) != OR_OK) return OR_ERR_DEGENERATE;
+    for (int j = 0; j < na; ++j) for (int i = 0; i < na; ++i) M(C, act[i], act[j], 4) = inv[i + na * j];
+    return OR_OK;
+}
+
+int or_q4_local(const double lc[8], const double Cm[9], const double Cb[9],
+                const double Cs[4], double h, double E_ref, double k6rot,
+                double drill_scale, int rigid, double Ke[576]) {
+    return or_q4_local_modes(lc, Cm, Cb, Cs, h, E_ref, k6rot, drill_scale, rigid, 15, Ke);
+}
+
+int or_q4_local_modes(const double lc[8], const double Cm[9], const double Cb[9],
+                      const double Cs[4], double h, double E_ref, double k6rot,
+                      double drill_scale, int rigid, int membrane_modes, double Ke[576]) {
+    (void)E_ref;
+    double K_ab[96] = {0}, K_bb[16] = {0}, K_abb[96] = {0}, K_bbb[16] = {0};
+    double K_abs[96] = {0}, K_bbs[16] = {0};
+    for (int q = 0; q < 576; ++q) Ke[q] = 0.0;
+    /* centre Jacobian :3136-3163 */
+    const double dNr_c[4] = {-0.25, 0.25, 0.25, -0.25}, dNs_c[4] = {-0.25, -0.25, 0.25, 0.25};
+    double J11c, J12c, J21c, J22c;
+    jac4(dNr_c, dNs_c, lc, &J11c, &J12c, &J21c, &J22c);
+    double detJc = J11c * J22c - J12c * J21c, abs_detJc = fabs(detJc);
+    double inv_detc = 1.0 / detJc;
+    double iJ11c = J22c * inv_detc, iJ12c = -J12c * inv_detc, iJ21c = -J21c * inv_detc, iJ22c = J11c * inv_detc;
+    const double pt = 1.0 / sqrt(3.0);
+    const double gr[4] = {-pt, pt, pt, -pt}, gs[4] = {-pt, -pt, pt, pt};
+    double Bm[72], Bb[72], Bi[12], Bib[12], tmp[72], t34[12];
+    const int shape_cm = maxabs(Cm, 9) > 0;
+    for (int gp = 0; gp < 4; ++gp) {  /* :3230-3621 */
+        double r = gr[gp], s = gs[gp];
+        double dNr[4], dNs[4], J11, J12, J21, J22, w;
+        shape_derivs_quad(r, s, dNr, dNs);
+        jac4(dNr, dNs, lc, &J11, &J12, &J21, &J22);
+        double detJ = J11 * J22 - J12 * J21;
+        if (jac_measure(detJ, J11, J12, J21, J22, &w) != OR_OK) return OR_ERR_DEGENERATE;
+        double inv_det = 1.0 / detJ;
+        double iJ11 = J22 * inv_det, iJ12 = -J12 * inv_det, iJ21 = -J21 * inv_det, iJ22 = J11 * inv_det;
+        memset(Bm, 0, sizeof(Bm)); memset(Bb, 0, sizeof(Bb));
+        for (int k = 0; k < 4; ++k) {  /* :3299-3352 */
+            double dx = iJ11 * dNr[k] + iJ12 * dNs[k];
+            double dy = iJ21 * dNr[k] + iJ22 * dNs[k];
+            int idx = 6 * k;
+            M(Bm, 0, idx + 0, 3) = dx; M(Bm, 1, idx + 1, 3) = dy;
+            M(Bm, 2, idx + 0, 3) = dy; M(Bm, 2, idx + 1, 3) = dx;
+            M(Bb, 0, idx + 4, 3) = dx; M(Bb, 1, idx + 3, 3) = -dy;
+            M(Bb, 2, idx + 4, 3) = dy; M(Bb, 2, idx + 3, 3) = -dx;
+        }
+        /* membrane internal modes, patch-consistent Jacobian :1847-1896 */
+        double ratio = (iJ11 * iJ22 - iJ12 * iJ21) / (iJ11c * iJ22c - iJ12c * iJ21c);
+        double m11 = ratio * iJ11c, m12 = ratio * iJ12c, m21 = ratio * iJ21c, m22 = ratio * iJ22c;
+        double p1x = m11 * (-2.0 * r), p1y = m21 * (-2.0 * r), p2x = m12 * (-2.0 * s), p2y = m22 * (-2.0 * s);
+        memset(Bi, 0, sizeof(Bi));
+        M(Bi, 0, 0, 3) = p1x; M(Bi, 2, 0, 3) = p1y; M(Bi, 1, 1, 3) = p1y; M(Bi, 2, 1, 3) = p1x;
+        M(Bi, 0, 2, 3) = p2x; M(Bi, 2, 2, 3) = p2y; M(Bi, 1, 3, 3) = p2y; M(Bi, 2, 3, 3) = p2x;
+        /* bending internal modes, qm6 Jacobian :3478-3488 */
+        double g = detJc / detJ;
+        double b1x = (iJ11c * (-2.0 * r)) * g, b1y = (iJ21c * (-2.0 * r)) * g;
+        double b2x = (iJ12c * (-2.0 * s)) * g, b2y = (iJ22c * (-2.0 * s)) * g;
+        memset(Bib, 0, sizeof(Bib));
+        M(Bib, 1, 0, 3) = -b1y; M(Bib, 1, 1, 3) = -b2y;
+        M(Bib, 0, 2, 3) = b1x;  M(Bib, 0, 3, 3) = b2x;
+        M(Bib, 2, 0, 3) = -b1x; M(Bib, 2, 1, 3) = -b2x;
+        M(Bib, 2, 2, 3) = b1y;  M(Bib, 2, 3, 3) = b2y;
+        /* accumulations :3528-3610 */
+        ts_mul(tmp, Cm, Bm, 3, 3, 24); ts_mul_At_add(Ke, Bm, tmp, w, 3, 24, 24);
+        ts_mul(tmp, Cb, Bb, 3, 3, 24); ts_mul_At_add(Ke, Bb, tmp, w, 3, 24, 24);
+        ts_mul(t34, Cm, Bi, 3, 3, 4);
+        ts_mul_At_add(K_ab, Bm, t34, w, 3, 24, 4);
+        ts_mul_At_add(K_bb, Bi, t34, w, 3, 4, 4);
+        ts_mul(t34, shape_cm ? Cm : Cb, Bib, 3, 3, 4);
+        ts_mul_At_add(K_abs, Bb, t34, w, 3, 24, 4);
+        ts_mul_At_add(K_bbs, Bib, t34, w, 3, 4, 4);
+        ts_mul(t34, Cb, Bib, 3, 3, 4);
+        ts_mul_At_add(K_abb, Bb, t34, w, 3, 24, 4);
+        ts_mul_At_add(K_bbb, Bib, t34, w, 3, 4, 4);
+    }
+    /* MacNeal shear into plate DOFs :3737-3747, skip rule :4496-4498 */
+    if (!(h < 1e-30 || (!rigid && maxabs(Cs, 4) < 1e-30))) {
+        double D[48], Zb[16], Zs[16], Z[16], Kp[144];
+        int st = or_q4_macneal_parts(lc, Cb, Cs, rigid, D, Zb, Zs, Z, Kp);
+        if (st != OR_OK) return st;
+        static const int pd[12] = {2, 3, 4, 8, 9, 10, 14, 15, 16, 20, 21, 22};
+        for (int j = 0; j < 12; ++j)
+            for (int i = 0; i < 12; ++i) M(Ke, pd[i], pd[j], 24) += M(Kp, i, j, 12);
+    }
+    /* lumped K6ROT drilling with corner Jacobian weights :3772-3817 */
+    {
+        double A_drill = 4.0 * abs_detJc;
+        double alpha_L = ((drill_scale * (k6rot * 1e-6)) * M(Cm, 2, 2, 3)) * A_drill;
+        const double cr[4] = {-1, 1, 1, -1}, cs_[4] = {-1, -1, 1, 1};
+        double Bd[24];
+        for (int i = 0; i < 4; ++i) {
+            double dNr[4], dNs[4], J11, J12, J21, J22, wdum;
+            shape_derivs_quad(cr[i], cs_[i], dNr, dNs);
+            jac4(dNr, dNs, lc, &J11, &J12, &J21, &J22);
+            double detJ = J11 * J22 - J12 * J21;
+            if (jac_measure(detJ, J11, J12, J21, J22, &wdum) != OR_OK) return OR_ERR_DEGENERATE;
+            double inv_det = 1.0 / detJ;
+            double iJ11 = J22 * inv_det, iJ12 = -J12 * inv_det, iJ21 = -J21 * inv_det, iJ22 = J11 * inv_det;
+            memset(Bd, 0, sizeof(Bd));
+            for (int k = 0; k < 4; ++k) {
+                double dx = iJ11 * dNr[k] + iJ12 * dNs[k];
+                double dy = iJ21 * dNr[k] + iJ22 * dNs[k];
+                Bd[6 * k + 0] = 0.5 * dy; Bd[6 * k + 1] = -0.5 * dx;
+            }
+            Bd[6 * i + 5] = 1.0;
+            double wc = fabs(detJ) / fmax(abs_detJc, 1e-30);
+            ts_mul_At_add(Ke, Bd, Bd, alpha_L * wc, 1, 24, 24);
+        }
+    }
+    /* section kinematic enrichment :1989-2024 (binary unit weights :1939-1967) */
+    double Pm[96] = {0}, Pb[96] = {0};
+    if (maxabs(K_bb, 16) > 0) {
+        double C[16];
+        if (active_inverse(K_bb, membrane_modes, C) != OR_OK) return OR_ERR_DEGENERATE;
+        for (int j = 0; j < 24; ++j) for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k)
+            M(Pm, i, j, 4) -= M(C, i, k, 4) * M(K_ab, j, k, 24);
+    }
+    if (maxabs(K_bbs, 16) > 0) {
+        double C[16];
+        if (inv4(K_bbs, C) != OR_OK) return OR_ERR_DEGENERATE;
+        for (int j = 0; j < 24; ++j) for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k)
+            M(Pb, i, j, 4) -= M(C, i, k, 4) * M(K_abs, j, k, 24);
+    }
+    for (int j = 0; j < 24; ++j)
+        for (int i = 0; i < 24; ++i) {
+            double v = 0.0;
+            for (int k = 0; k < 4; ++k) {
+                double qim = M(K_ab, i, k, 24), qjm = M(K_ab, j, k, 24);
+                double qib = M(K_abb, i, k, 24), qjb = M(K_abb, j, k, 24);
+                v += qim * M(Pm, k, j, 4) + M(Pm, k, i, 4) * qjm + qib * M(Pb, k, j, 4) + M(Pb, k, i, 4) * qjb;
+                for (int l = 0; l < 4; ++l)
+                    v += M(Pm, k, i, 4) * M(K_bb, k, l, 4) * M(Pm, l, j, 4) +
+                         M(Pb, k, i, 4) * M(K_bbb, k, l, 4) * M(Pb, l, j, 4);
+            }
+            M(Ke, i, j, 24) += v;
+        }
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* shell frames, constitutive, transform                                      */
+/* ------------------------------------------------------------------------- */
+static void v3sub(const double *a, const double *b, double *o) { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; }
+static double v3dot(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static void v3cross(const double *a, const double *b, double *o) {
+    o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static void v3normalize(double *a) { double n = sqrt(v3dot(a, a)); a[0] /= n; a[1] /= n; a[2] /= n; }
+
+/* shell_element_frame_quad4 (diag mode), src/solver/helpers.jl:1422-1481 */
+static void frame_quad4(const double *p1, const double *p2, const double *p3, const double *p4,
+                        double *v1, double *v2, double *v3) {
+    double gxi[3], geta[3];
+    for (int c = 0; c < 3; ++c) {
+        gxi[c] = 0.25 * (((-p1[c]) + p2[c]) + p3[c] - p4[c]);
+        geta[c] = 0.25 * (((-p1[c]) - p2[c]) + p3[c] + p4[c]);
+    }
+    double vr[3];
+    v3cross(gxi, geta, vr);
+    double d13[3], d24[3];
+    v3sub(p3, p1, d13); v3sub(p4, p2, d24);
+    if (v3dot(vr, vr) <= 1e-24) v3cross(d13, d24, vr);
+    v3[0] = vr[0]; v3[1] = vr[1]; v3[2] = vr[2]; v3normalize(v3);
+    double d13n[3] = {d13[0], d13[1], d13[2]}, d24n[3] = {d24[0], d24[1], d24[2]};
+    v3normalize(d13n); v3normalize(d24n);
+    double xr[3] = {d13n[0] - d24n[0], d13n[1] - d24n[1], d13n[2] - d24n[2]};
+    if (v3dot(xr, xr) < 1e-20) { xr[0] = d13n[0] + d24n[0]; xr[1] = d13n[1] + d24n[1]; xr[2] = d13n[2] + d24n[2]; }
+    double dd = v3dot(xr, v3);
+    double xp[3] = {xr[0] - dd * v3[0], xr[1] - dd * v3[1], xr[2] - dd * v3[2]};
+    if (v3dot(xp, xp) <= 1e-24) {
+        double de = v3dot(geta, v3);
+        double xa[3] = {geta[0] - de * v3[0], geta[1] - de * v3[1], geta[2] - de * v3[2]};
+        if (v3dot(xa, xa) <= 1e-24) {
+            double e[3]; v3sub(p2, p1, e); double d2 = v3dot(e, v3);
+            xa[0] = e[0] - d2 * v3[0]; xa[1] = e[1] - d2 * v3[1]; xa[2] = e[2] - d2 * v3[2];
+        }
+        xp[0] = xa[0]; xp[1] = xa[1]; xp[2] = xa[2];
+    }
+    v1[0] = xp[0]; v1[1] = xp[1]; v1[2] = xp[2]; v3normalize(v1);
+    v3cross(v3, v1, v2);
+}
+
+/* isotropic PSHELL section, src/solver/assembly.jl:3380-3404, 2085-2129, 2174-2233 */
+static void shell_section(const double prop[8], double Cm[9], double Cb[9], double Cs[4],
+                          double *Gref, int *membrane_only) {
+    double h = prop[OR_SP_H], E = prop[OR_SP_E], nu = prop[OR_SP_NU], G = prop[OR_SP_G];
+    double br = prop[OR_SP_BR], tst = prop[OR_SP_TST];
+    if (!(G > 0.0)) G = E > 0.0 ? E / (2 * (1 + nu)) : 0.0;
+    double cm = (E * h) / (1 - nu * nu);
+    double Mi[9] = {1.0, nu, 0.0, nu, 1.0, 0.0, 0.0, 0.0, (1 - nu) / 2};
+    for (int q = 0; q < 9; ++q) Cm[q] = cm * Mi[q];
+    if (br > 1e-12) {
+        double pc = E / fmax(1.0 - nu * nu, 1e-30);
+        double Mb[9] = {1.0, nu, 0.0, nu, 1.0, 0.0, 0.0, 0.0, (1.0 - nu) / 2.0};
+        double f = ((br * (h * h * h)) / 12.0);
+        for (int q = 0; q < 9; ++q) Cb[q] = (pc * Mb[q]) * f;
+        double ts = tst * h;
+        Cs[0] = ts * G; Cs[1] = 0.0; Cs[2] = 0.0; Cs[3] = ts * G;
+        *membrane_only = 0;
+    } else {
+        for (int q = 0; q < 9; ++q) Cb[q] = 0.0;
+        for (int q = 0; q < 4; ++q) Cs[q] = 0.0;
+        *membrane_only = 1;
+    }
+    *Gref = G;
+}
+
+/* _rotate_constitutive_3x3!, src/solver/assembly.jl:2440-2462 (C <- T' C T) */
+static void rot3(double *C, double T11, double T12, double T13, double T21, double T22, double T23,
+                 double T31, double T32, double T33) {
+    double c11 = C[0], c21 = C[1], c31 = C[2], c12 = C[3], c22 = C[4], c32 = C[5], c13 = C[6], c23 = C[7], c33 = C[8];
+    double t11 = c11 * T11 + c12 * T21 + c13 * T31, t12 = c11 * T12 + c12 * T22 + c13 * T32, t13 = c11 * T13 + c12 * T23 + c13 * T33;
+    double t21 = c21 * T11 + c22 * T21 + c23 * T31, t22 = c21 * T12 + c22 * T22 + c23 * T32, t23 = c21 * T13 + c22 * T23 + c23 * T33;
+    double t31 = c31 * T11 + c32 * T21 + c33 * T31, t32 = c31 * T12 + c32 * T22 + c33 * T32, t33 = c31 * T13 + c32 * T23 + c33 * T33;
+    C[0] = T11 * t11 + T21 * t21 + T31 * t31; C[3] = T11 * t12 + T21 * t22 + T31 * t32; C[6] = T11 * t13 + T21 * t23 + T31 * t33;
+    C[1] = T12 * t11 + T22 * t21 + T32 * t31; C[4] = T12 * t12 + T22 * t22 + T32 * t32; C[7] = T12 * t13 + T22 * t23 + T32 * t33;
+    C[2] = T13 * t11 + T23 * t21 + T33 * t31; C[5] = T13 * t12 + T23 * t22 + T33 * t32; C[8] = T13 * t13 + T23 * t23 + T33 * t33;
+}
+
+/* R' * blk * R for every 3x3 block (_quad4_transform_emit!, src/solver/assembly.jl:359-440) */
+static void transform_blocks(const double *Kl, int n, const double R[9] /* rows v1,v2,v3: R[r*3+c] */, double *Kg) {
+    int nb = n / 3;
+    for (int B = 0; B < nb; ++B)
+        for (int A = 0; A < nb; ++A) {
+            double t[9];
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    double s = 0;
+                    for (int l = 0; l < 3; ++l) s += M(Kl, 3 * A + i, 3 * B + l, n) * R[l * 3 + j];
+                    t[i + 3 * j] = s;
+                }
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    double s = 0;
+                    for (int l = 0; l < 3; ++l) s += R[l * 3 + i] * t[l + 3 * j];
+                    M(Kg, 3 * A + i, 3 * B + j, n) = s;
+                }
+        }
+}
+
+/* quad4_local_z_from_coords3d + warp gate, src/FEMKernels.jl:853-865, 1100-1102 */
+static int quad4_is_warped(const double *p, const double lc[8]) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6, *p4 = p + 9;
+    double c[3], a[3], b[3], n[3];
+    for (int k = 0; k < 3; ++k) c[k] = (((p1[k] + p2[k]) + p3[k]) + p4[k]) / 4;
+    v3sub(p3, p1, a); v3sub(p4, p2, b); v3cross(a, b, n);
+    double nr = sqrt(v3dot(n, n));
+    double v3[3] = {0, 0, 1};
+    if (nr > 1e-30) { v3[0] = n[0] / nr; v3[1] = n[1] / nr; v3[2] = n[2] / nr; }
+    double zmax = 0, xmax = 0;
+    for (int k = 0; k < 4; ++k) {
+        double d[3]; v3sub(p + 3 * k, c, d);
+        double z = fabs(v3dot(d, v3)); if (z > zmax) zmax = z;
+        if (fabs(lc[2 * k]) > xmax) xmax = fabs(lc[2 * k]);
+    }
+    if (xmax < 1e-30) xmax = 1e-30;
+    return zmax > 1e-12 * xmax;
+}
+
+int or_q4_global(const double p[12], const double prop[8], double k6rot, double Ke[576]) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6, *p4 = p + 9;
+    double v1[3], v2[3], v3[3];
+    frame_quad4(p1, p2, p3, p4, v1, v2, v3);
+    /* local coordinates about the centroid, src/solver/assembly.jl:3686-3691 */
+    double c[3], lc[8];
+    for (int k = 0; k < 3; ++k) c[k] = (((p1[k] + p2[k]) + p3[k]) + p4[k]) / 4.0;
+    for (int k = 0; k < 4; ++k) {
+        double d[3]; v3sub(p + 3 * k, c, d);
+        lc[2 * k] = v3dot(d, v1); lc[2 * k + 1] = v3dot(d, v2);
+    }
+    if (quad4_is_warped(p, lc)) return OR_ERR_WARPED;
+    double Cm[9], Cb[9], Cs[4], G; int mo;
+    shell_section(prop, Cm, Cb, Cs, &G, &mo);
+    /* PSHELL material axis from edge 1-2, helpers.jl:1749-1759, assembly.jl:4400-4425 */
+    {
+        double e[3]; v3sub(p2, p1, e);
+        double de = v3dot(e, v3);
+        double ep[3] = {e[0] - de * v3[0], e[1] - de * v3[1], e[2] - de * v3[2]};
+        double l2 = v3dot(ep, ep);
+        if (l2 > 1e-24) {
+            double l = sqrt(l2); double xm[3] = {ep[0] / l, ep[1] / l, ep[2] / l};
+            double beta = 0.0 + atan2(v3dot(xm, v2), v3dot(xm, v1));
+            if (fabs(beta) > 1e-10) {
+                double cb = cos(beta), sb = sin(beta);
+                double c2 = cb * cb, s2 = sb * sb, cs = cb * sb;
+                rot3(Cm, c2, s2, cs, s2, c2, -cs, -2 * cs, 2 * cs, c2 - s2);
+                rot3(Cb, c2, s2, cs, s2, c2, -cs, -2 * cs, 2 * cs, c2 - s2);
+                double a11 = Cs[0], a12 = Cs[2], a22 = Cs[3];
+                Cs[0] = cb * cb * a11 - 2 * cb * sb * a12 + sb * sb * a22;
+                Cs[2] = cb * sb * a11 + (cb * cb - sb * sb) * a12 - cb * sb * a22;
+                Cs[1] = Cs[2];
+                Cs[3] = sb * sb * a11 + 2 * cb * sb * a12 + cb * cb * a22;
+            }
+        }
+    }
+    int rigid = (prop[OR_SP_RIGID] != 0.0) && !mo;
+    double k6 = mo ? 0.0 : k6rot, ds = mo ? 0.0 : 1.0;
+    double Kl[576];
+    int modes = (int)prop[OR_SP_MODES];
+    if (modes <= 0 || modes > 15) modes = 15;
+    int st = or_q4_local_modes(lc, Cm, Cb, Cs, prop[OR_SP_H], prop[OR_SP_E], k6, ds, rigid, modes, Kl);
+    if (st != OR_OK) return st;
+    double R[9] = {v1[0], v1[1], v1[2], v2[0], v2[1], v2[2], v3[0], v3[1], v3[2]};
+    transform_blocks(Kl, 24, R, Ke);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* CTRIA3, stiffness_tria3_matrices_generic default branches,                 */
+/* src/FEMKernels.jl:9375-9809, guard :9281-9312                               */
+/* ------------------------------------------------------------------------- */
+static void tria3_guard(double *Zb, double La, double Lb, double c, double s, double Dn) {
+    double scale = 6 * (La / Lb) * s * Dn;
+    if (!(scale > 0 && M(Zb, 2, 2, 3) > 0)) return;
+    double z33 = M(Zb, 2, 2, 3);
+    double h11 = scale * (M(Zb, 0, 0, 3) - M(Zb, 0, 2, 3) * M(Zb, 0, 2, 3) / z33) / (La * La);
+    double h22 = scale * (M(Zb, 1, 1, 3) - M(Zb, 1, 2, 3) * M(Zb, 1, 2, 3) / z33) / (Lb * Lb);
+    double h12 = scale * (M(Zb, 0, 1, 3) - M(Zb, 0, 2, 3) * M(Zb, 1, 2, 3) / z33) / (La * Lb);
+    double p11 = h11, p12 = (h12 - c * h11) / s, p22 = (h22 - 2 * c * h12 + c * c * h11) / (s * s);
+    double a = fmax(p11, 1.0 / 10000), q = fmax(p22, a / 10000);
+    double limit = sqrt((85.0 / 100) * a * q);
+    double b = p12 < -limit ? -limit : (p12 > limit ? limit : p12);
+    if (a == p11 && b == p12 && q == p22) return;
+    double n11 = a, n12 = c * a + s * b, n22 = c * c * a + 2 * c * s * b + s * s * q;
+    M(Zb, 0, 0, 3) += (n11 - h11) * La * La / scale;
+    M(Zb, 0, 1, 3) += (n12 - h12) * La * Lb / scale;
+    M(Zb, 1, 0, 3) = M(Zb, 0, 1, 3);
+    M(Zb, 1, 1, 3) += (n22 - h22) * Lb * Lb / scale;
+}
+
+int or_t3_local(const double lc[6], const double Dm[9], const double Db[9],
+                const double Ds[4], double h, double G_ref, double k6rot, double Ke[324]) {
+    (void)h; (void)G_ref;
+    for (int q = 0; q < 324; ++q) Ke[q] = 0;
+    double x1 = lc[0], y1 = lc[1], x2 = lc[2], y2 = lc[3], x3 = lc[4], y3 = lc[5];
+    double A2 = x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2);
+    double A = 0.5 * fabs(A2);
+    if (A < 1e-12) return OR_OK; /* zero matrix, :9381 */
+    double bv[3] = {(y2 - y3) / (2 * A), (y3 - y1) / (2 * A), (y1 - y2) / (2 * A)};
+    double cv[3] = {(x3 - x2) / (2 * A), (x1 - x3) / (2 * A), (x2 - x1) / (2 * A)};
+    /* CST membrane :9390-9396 */
+    double Bm[18] = {0};
+    for (int i = 0; i < 3; ++i) {
+        M(Bm, 0, 2 * i, 3) = bv[i]; M(Bm, 1, 2 * i + 1, 3) = cv[i];
+        M(Bm, 2, 2 * i, 3) = cv[i]; M(Bm, 2, 2 * i + 1, 3) = bv[i];
+    }
+    static const int mi[6] = {0, 1, 6, 7, 12, 13};
+    for (int j = 0; j < 6; ++j)
+        for (int i = 0; i < 6; ++i) {
+            double s = 0;
+            for (int a = 0; a < 3; ++a) { double t = 0; for (int b = 0; b < 3; ++b) t += M(Bm, b, i, 3) * M(Dm, b, a, 3); s += t * M(Bm, a, j, 3); }
+            M(Ke, mi[i], mi[j], 18) = s * A;
+        }
+    /* constant-curvature bending :9399-9408 */
+    double Bb[27] = {0};
+    for (int i = 0; i < 3; ++i) {
+        M(Bb, 0, 3 * i + 2, 3) = bv[i]; M(Bb, 1, 3 * i + 1, 3) = -cv[i];
+        M(Bb, 2, 3 * i + 1, 3) = -bv[i]; M(Bb, 2, 3 * i + 2, 3) = cv[i];
+    }
+    double Kb[81];
+    for (int j = 0; j < 9; ++j)
+        for (int i = 0; i < 9; ++i) {
+            double s = 0;
+            for (int a = 0; a < 3; ++a) { double t = 0; for (int b = 0; b < 3; ++b) t += M(Bb, b, i, 3) * M(Db, b, a, 3); s += t * M(Bb, a, j, 3); }
+            M(Kb, i, j, 9) = s * A;
+        }
+    /* MITC3 tying rows :9484-9508 */
+    double Jm11 = x2 - x1, Jm12 = y2 - y1, Jm21 = x3 - x1, Jm22 = y3 - y1;
+    double dj = Jm11 * Jm22 - Jm12 * Jm21;
+    double Ji11 = Jm22 / dj, Ji12 = -Jm12 / dj, Ji21 = -Jm21 / dj, Ji22 = Jm11 / dj;
+    double grA[9], gsB[9], grC[9], gsC[9], tmp9[9];
+    {
+        const double rr[3] = {0.5, 0.0, 0.5}, ss[3] = {0.0, 0.5, 0.5};
+        const double dwdr[3] = {-1, 1, 0}, dwds[3] = {-1, 0, 1};
+        for (int pnt = 0; pnt < 3; ++pnt) {
+            double N[3] = {1 - rr[pnt] - ss[pnt], rr[pnt], ss[pnt]};
+            double Br[9], Bs[9];
+            for (int i = 0; i < 3; ++i) {
+                Br[3 * i] = dwdr[i]; Bs[3 * i] = dwds[i];
+                Br[3 * i + 2] = N[i] * Jm11; Bs[3 * i + 2] = N[i] * Jm21;
+                Br[3 * i + 1] = -N[i] * Jm12; Bs[3 * i + 1] = -N[i] * Jm22;
+            }
+            if (pnt == 0) memcpy(grA, Br, sizeof(grA));
+            else if (pnt == 1) memcpy(gsB, Bs, sizeof(gsB));
+            else { memcpy(grC, Br, sizeof(grC)); memcpy(gsC, Bs, sizeof(gsC)); }
+        }
+        (void)tmp9;
+    }
+    double cvec[9];
+    for (int i = 0; i < 9; ++i) cvec[i] = (grC[i] - grA[i]) - (gsC[i] - gsB[i]);
+    double La = hypot(x2 - x1, y2 - y1), Lb = hypot(x3 - x1, y3 - y1), Lc = hypot(x3 - x2, y3 - y2);
+    double P = La + Lb + Lc;
+    double cth = ((x2 - x1) * (x3 - x1) + (y2 - y1) * (y3 - y1)) / fmax(La * Lb, 1e-30);
+    if (cth < -1) cth = -1; if (cth > 1) cth = 1;
+    double sth = sqrt(fmax(1 - cth * cth, 0.0));
+    /* reference tying rigidity :9541-9550; G = Jinv' Ds Jinv */
+    double G11, G12, G22;
+    {
+        /* t = Ds * Jinv ; G = Jinv' * t */
+        double t11 = M(Ds, 0, 0, 2) * Ji11 + M(Ds, 0, 1, 2) * Ji21, t12 = M(Ds, 0, 0, 2) * Ji12 + M(Ds, 0, 1, 2) * Ji22;
+        double t21 = M(Ds, 1, 0, 2) * Ji11 + M(Ds, 1, 1, 2) * Ji21, t22 = M(Ds, 1, 0, 2) * Ji12 + M(Ds, 1, 1, 2) * Ji22;
+        G11 = Ji11 * t11 + Ji21 * t21; G12 = Ji11 * t12 + Ji21 * t22; G22 = Ji12 * t12 + Ji22 * t22;
+    }
+    double uu = 3 * La / P, vv = 3 * Lb / P;
+    double W[9];
+    M(W, 0, 0, 3) = G11 * A; M(W, 0, 1, 3) = G12 * A; M(W, 1, 0, 3) = M(W, 0, 1, 3); M(W, 1, 1, 3) = G22 * A;
+    M(W, 0, 2, 3) = (A / 3) * (uu * G11 - vv * G12); M(W, 2, 0, 3) = M(W, 0, 2, 3);
+    M(W, 1, 2, 3) = (A / 3) * (uu * G12 - vv * G22); M(W, 2, 1, 3) = M(W, 1, 2, 3);
+    M(W, 2, 2, 3) = uu * uu * G11 * (A / 6) - 2 * uu * vv * G12 * (A * (2 - cth) / 18) + vv * vv * G22 * (A / 6);
+    /* residual bending flexibility Zb3 with directional pairing :9578-9728 */
+    double Dbend = fabs(M(Db, 0, 0, 3));
+    if (sth > 1e-9 && Dbend > 1e-30 && La > 1e-30 && Lb > 1e-30) {
+        double S1 = La, S2 = Lb, S3 = P, s6 = 1.0 / 6;
+        double D11e = Dbend, D22e = fabs(M(Db, 1, 1, 3));
+        int dir_pair = D11e > 1e-30 && D22e > 1e-30;
+        double rho = La / Lb, Rr2 = rho + 1 / rho - 2, Tt2 = rho - 1 / rho;
+        double fr = ((1 - cth) * (1 - cth) - (rho - cth)) / (6 * sth) + Rr2 * (1 + cth * cth) / (12 * sth) + Tt2 * sth / 12;
+        double fri = ((1 - cth) * (1 - cth) - (1 / rho - cth)) / (6 * sth) + Rr2 * (1 + cth * cth) / (12 * sth) - Tt2 * sth / 12;
+        double w12 = (1 / rho - cth) / (6 * sth), w13 = (rho - cth) / (6 * sth), w23 = cth / (6 * sth);
+        double o12 = (Rr2 * cth - (1 - cth) * (1 - cth) - Tt2 * cth) / (12 * sth);
+        double o13 = (Rr2 * cth - (1 - cth) * (1 - cth) + Tt2 * cth) / (12 * sth);
+        double o23 = (1 + cth * cth) / (12 * sth);
+        double Rr = (La - Lb) * (La - Lb) / (La * Lb), Tt = (La * La - Lb * Lb) / (La * Lb);
+        double sum_d = 2 * ((1 - cth + cth * cth) / (6 * sth) + 1.0 / 6) + Rr * (1 + cth * cth) / (6 * sth);
+        double dif_d = Tt * sth / 6;
+        double a11 = (sum_d + dif_d) / 2, a22 = (sum_d - dif_d) / 2;
+        double a12 = (-1 + 4 * cth - cth * cth) / (12 * sth) - 1.0 / 6 + Rr * cth / (6 * sth);
+        double ev[3][2] = {{x2 - x1, y2 - y1}, {x3 - x1, y3 - y1}, {x3 - x2, y3 - y2}};
+        double Dn[3];
+        for (int e = 0; e < 3; ++e) {
+            double vx = ev[e][0], vy = ev[e][1], n = hypot(vx, vy);
+            if (!dir_pair) { Dn[e] = Dbend; continue; }
+            if (n < 1e-30) { Dn[e] = Dbend; continue; }
+            double nx = -vy / n, ny = vx / n;
+            double wv[3] = {nx * nx, ny * ny, 2 * nx * ny};
+            double d = 0;
+            for (int ii = 0; ii < 3; ++ii) for (int jj = 0; jj < 3; ++jj) d += wv[ii] * M(Db, ii, jj, 3) * wv[jj];
+            Dn[e] = fabs(d) > 1e-30 ? d : Dbend;
+        }
+        double Gd = dir_pair ? sqrt(D11e * D22e) : Dbend;
+        double c11 = dir_pair ? s6 / Gd + fr / Dn[0] + w13 / Dn[1] + w23 / Dn[2] : a11 / Dbend;
+        double c22 = dir_pair ? s6 / Gd + w12 / Dn[0] + fri / Dn[1] + w23 / Dn[2] : a22 / Dbend;
+        double c12 = dir_pair ? -s6 / Gd + o12 / Dn[0] + o13 / Dn[1] + o23 / Dn[2] : a12 / Dbend;
+        double Zb[9];
+        M(Zb, 0, 0, 3) = c11 * S1 * S1; M(Zb, 0, 1, 3) = c12 * S1 * S2;
+        M(Zb, 1, 0, 3) = M(Zb, 0, 1, 3); M(Zb, 1, 1, 3) = c22 * S2 * S2;
+        M(Zb, 0, 2, 3) = -s6 / Gd * S1 * S3; M(Zb, 2, 0, 3) = M(Zb, 0, 2, 3);
+        M(Zb, 1, 2, 3) = s6 / Gd * S2 * S3; M(Zb, 2, 1, 3) = M(Zb, 1, 2, 3);
+        M(Zb, 2, 2, 3) = s6 / Gd * S3 * S3;
+        tria3_guard(Zb, La, Lb, cth, sth, dir_pair ? Dn[0] : Dbend);
+        double Wi[9], Zt[9], Zs[9];
+        if (inv3(W, Wi) != OR_OK) return OR_ERR_DEGENERATE;
+        for (int q = 0; q < 9; ++q) Zt[q] = Wi[q] + Zb[q];
+        for (int j = 0; j < 3; ++j) for (int i = 0; i < 3; ++i) Zs[i + 3 * j] = 0.5 * (Zt[i + 3 * j] + Zt[j + 3 * i]);
+        if (inv3(Zs, W) != OR_OK) return OR_ERR_DEGENERATE;
+    }
+    /* Ks = D3' W D3, D3 rows grA, gsB, cvec :9731-9733 */
+    const double *D3[3] = {grA, gsB, cvec};
+    static const int bi[9] = {2, 3, 4, 8, 9, 10, 14, 15, 16};
+    for (int j = 0; j < 9; ++j)
+        for (int i = 0; i < 9; ++i) {
+            double s = 0;
+            for (int a = 0; a < 3; ++a) { double t = 0; for (int b = 0; b < 3; ++b) t += D3[b][i] * M(W, b, a, 3); s += t * D3[a][j]; }
+            M(Ke, bi[i], bi[j], 18) += M(Kb, i, j, 9) + s;
+        }
+    /* lumped drilling :9780-9795 */
+    double alpha_L = (k6rot * 1e-6) * M(Dm, 2, 2, 3) * (2 * A);
+    for (int i = 0; i < 3; ++i) {
+        double Bd[18] = {0};
+        for (int j = 0; j < 3; ++j) { Bd[6 * j] = 0.5 * cv[j]; Bd[6 * j + 1] = -0.5 * bv[j]; }
+        Bd[6 * i + 5] = 1.0;
+        for (int b = 0; b < 18; ++b) for (int a = 0; a < 18; ++a) M(Ke, a, b, 18) += alpha_L * (Bd[a] * Bd[b]);
+    }
+    return OR_OK;
+}
+
+int or_t3_global(const double p[9], const double prop[8], double k6rot, double Ke[324]) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6;
+    /* shell_element_frame_fast n=3, src/solver/helpers.jl:1486-1490 */
+    double v1[3], v2[3], v3[3], e13[3];
+    v3sub(p2, p1, v1); v3normalize(v1);
+    v3sub(p3, p1, e13); v3cross(v1, e13, v3); v3normalize(v3);
+    v3cross(v3, v1, v2);
+    double c[3], lc[6];
+    for (int k = 0; k < 3; ++k) c[k] = ((p1[k] + p2[k]) + p3[k]) / 3.0;
+    for (int k = 0; k < 3; ++k) { double d[3]; v3sub(p + 3 * k, c, d); lc[2 * k] = v3dot(d, v1); lc[2 * k + 1] = v3dot(d, v2); }
+    double Cm[9], Cb[9], Cs[4], G; int mo;
+    shell_section(prop, Cm, Cb, Cs, &G, &mo);
+    double Kl[324];
+    int st = or_t3_local(lc, Cm, Cb, Cs, prop[OR_SP_H], G, mo ? 0.0 : k6rot, Kl);
+    if (st != OR_OK) return st;
+    double R[9] = {v1[0], v1[1], v1[2], v2[0], v2[1], v2[2], v3[0], v3[1], v3[2]};
+    transform_blocks(Kl, 18, R, Ke);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* CTETRA10, src/tetra10.jl:6-79; iso_3d_constitutive src/FEMKernels.jl:12056 */
+/* ------------------------------------------------------------------------- */
+int or_tet10(const double X[30], double E, double nu, double Ke[900]) {
+    if (!(isfinite(E) && E > 0 && isfinite(nu) && nu > -1 && nu < 0.5)) return OR_ERR_BAD_ARG;
+    const double a = (5.0 + 3.0 * sqrt(5.0)) / 20.0, b = (5.0 - sqrt(5.0)) / 20.0;
+    const double pts[4][3] = {{b, b, b}, {a, b, b}, {b, a, b}, {b, b, a}};
+    static const int edges[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    double cc = E / ((1.0 + nu) * (1.0 - 2.0 * nu)), dd = 1.0 - nu, ss = (1.0 - 2.0 * nu) / 2.0;
+    double D[36] = {0};
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) M(D, i, j, 6) = (i == j) ? cc * dd : cc * nu;
+    for (int i = 3; i < 6; ++i) M(D, i, i, 6) = cc * ss;
+    double K[900] = {0};
+    for (int q = 0; q < 4; ++q) {
+        double L[4] = {1.0 - pts[q][0] - pts[q][1] - pts[q][2], pts[q][0], pts[q][1], pts[q][2]};
+        const double dL[3][4] = {{-1, 1, 0, 0}, {-1, 0, 1, 0}, {-1, 0, 0, 1}};
+        double dN[3][10];
+        for (int i = 0; i < 4; ++i) for (int k = 0; k < 3; ++k) dN[k][i] = (4.0 * L[i] - 1.0) * dL[k][i];
+        for (int e = 0; e < 6; ++e) {
+            int i = edges[e][0], j = edges[e][1];
+            for (int k = 0; k < 3; ++k) dN[k][e + 4] = 4.0 * (L[j] * dL[k][i] + L[i] * dL[k][j]);
+        }
+        double J[3][3];
+        for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) {
+            double s = 0; for (int n = 0; n < 10; ++n) s += dN[r][n] * M(X, n, c, 10); J[r][c] = s;
+        }
+        /* det(J) = col1 . (col2 x col3) */
+        double c1[3] = {J[0][0], J[1][0], J[2][0]}, c2[3] = {J[0][1], J[1][1], J[2][1]}, c3[3] = {J[0][2], J[1][2], J[2][2]};
+        double cr[3]; v3cross(c2, c3, cr);
+        double det = v3dot(c1, cr);
+        if (!(isfinite(det) && det > 0.0)) return OR_ERR_INVERTED;
+        double w = det / 24.0;
+        /* J \ dN via adjugate */
+        double Ji[9], Jc[9];
+        for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Jc[r + 3 * c] = J[r][c];
+        Ji[0] = (Jc[4] * Jc[8] - Jc[7] * Jc[5]) / det; Ji[3] = (Jc[6] * Jc[5] - Jc[3] * Jc[8]) / det; Ji[6] = (Jc[3] * Jc[7] - Jc[6] * Jc[4]) / det;
+        Ji[1] = (Jc[7] * Jc[2] - Jc[1] * Jc[8]) / det; Ji[4] = (Jc[0] * Jc[8] - Jc[6] * Jc[2]) / det; Ji[7] = (Jc[6] * Jc[1] - Jc[0] * Jc[7]) / det;
+        Ji[2] = (Jc[1] * Jc[5] - Jc[4] * Jc[2]) / det; Ji[5] = (Jc[3] * Jc[2] - Jc[0] * Jc[5]) / det; Ji[8] = (Jc[0] * Jc[4] - Jc[3] * Jc[1]) / det;
+        double g[3][10];
+        for (int r = 0; r < 3; ++r) for (int n = 0; n < 10; ++n)
+            g[r][n] = Ji[r] * dN[0][n] + Ji[r + 3] * dN[1][n] + Ji[r + 6] * dN[2][n];
+        double B[180] = {0};
+        for (int i = 0; i < 10; ++i) {
+            int c = 3 * i; double dx = g[0][i], dy = g[1][i], dz = g[2][i];
+            M(B, 0, c, 6) = dx; M(B, 1, c + 1, 6) = dy; M(B, 2, c + 2, 6) = dz;
+            M(B, 3, c, 6) = dy; M(B, 3, c + 1, 6) = dx;
+            M(B, 4, c + 1, 6) = dz; M(B, 4, c + 2, 6) = dy;
+            M(B, 5, c, 6) = dz; M(B, 5, c + 2, 6) = dx;
+        }
+        double DB[180];
+        for (int j = 0; j < 30; ++j) for (int i = 0; i < 6; ++i) {
+            double s = 0; for (int l = 0; l < 6; ++l) s += M(D, i, l, 6) * M(B, l, j, 6); M(DB, i, j, 6) = s;
+        }
+        for (int j = 0; j < 30; ++j) for (int i = 0; i < 30; ++i) {
+            double s = 0; for (int l = 0; l < 6; ++l) s += M(B, l, i, 6) * M(DB, l, j, 6);
+            M(K, i, j, 30) += w * s;
+        }
+    }
+    /* Matrix(Symmetric(K)): upper triangle mirrored */
+    for (int j = 0; j < 30; ++j) for (int i = 0; i < 30; ++i) M(Ke, i, j, 30) = i <= j ? M(K, i, j, 30) : M(K, j, i, 30);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* assembly: triplets + sparse(I,J,V) (src/solver/assembly.jl:432-438,          */
+/* 5506-5508, 5882-5884, 5942)                                                 */
+/* ------------------------------------------------------------------------- */
+typedef struct { int64_t row; int32_t col; int64_t seq; double v; } trip_t;
+static int trip_cmp(const void *a, const void *b) {
+    const trip_t *x = a, *y = b;
+    if (x->row != y->row) return x->row < y->row ? -1 : 1;
+    if (x->col != y->col) return x->col < y->col ? -1 : 1;
+    return x->seq < y->seq ? -1 : (x->seq > y->seq);
+}
+
+int or_assemble(int64_t n_nodes, const double *x, const double *y, const double *z,
+                int64_t nq, const int32_t *qc, const double *qp,
+                int64_t nt, const int32_t *tc, const double *tp,
+                int64_t ntet, const int32_t *ec, const double *ep,
+                double k6rot, int64_t *out_nnz, int64_t **row_ptr, int32_t **col,
+                double **val, int64_t *bad_elem) {
+    int64_t ntrip = nq * 576 + nt * 324 + ntet * 900;
+    trip_t *T = (trip_t *)malloc(sizeof(trip_t) * (ntrip > 0 ? ntrip : 1));
+    if (!T) return OR_ERR_OOM;
+    int64_t seq = 0;
+    *bad_elem = -1;
+    for (int64_t e = 0; e < nq; ++e) {
+        double p[12], prop[8], Ke[576]; int nd[4];
+        for (int k = 0; k < 4; ++k) {
+            nd[k] = qc[k * nq + e];
+            p[3 * k] = x[nd[k]]; p[3 * k + 1] = y[nd[k]]; p[3 * k + 2] = z[nd[k]];
+        }
+        for (int c = 0; c < 8; ++c) prop[c] = qp[c * nq + e];
+        int st = or_q4_global(p, prop, k6rot, Ke);
+        if (st != OR_OK) { free(T); *bad_elem = e; return st; }
+        for (int cc = 0; cc < 24; ++cc)
+            for (int rr = 0; rr < 24; ++rr) {
+                T[seq].row = 6 * (int64_t)nd[rr / 6] + rr % 6; T[seq].col = 6 * nd[cc / 6] + cc % 6;
+                T[seq].v = Ke[rr + 24 * cc]; T[seq].seq = seq; ++seq;
+            }
+    }
+    for (int64_t e = 0; e < nt; ++e) {
+        double p[9], prop[8], Ke[324]; int nd[3];
+        for (int k = 0; k < 3; ++k) {
+            nd[k] = tc[k * nt + e];
+            p[3 * k] = x[nd[k]]; p[3 * k + 1] = y[nd[k]]; p[3 * k + 2] = z[nd[k]];
+        }
+        for (int c = 0; c < 8; ++c) prop[c] = tp[c * nt + e];
+        int st = or_t3_global(p, prop, k6rot, Ke);
+        if (st != OR_OK) { free(T); *bad_elem = nq + e; return st; }
+        for (int cc = 0; cc < 18; ++cc)
+            for (int rr = 0; rr < 18; ++rr) {
+                T[seq].row = 6 * (int64_t)nd[rr / 6] + rr % 6; T[seq].col = 6 * nd[cc / 6] + cc % 6;
+                T[seq].v = Ke[rr + 18 * cc]; T[seq].seq = seq; ++seq;
+            }
+    }
+    for (int64_t e = 0; e < ntet; ++e) {
+        double X[30], Ke[900]; int nd[10];
+        for (int k = 0; k < 10; ++k) {
+            nd[k] = ec[k * ntet + e];
+            X[k] = x[nd[k]]; X[k + 10] = y[nd[k]]; X[k + 20] = z[nd[k]];
+        }
+        int st = or_tet10(X, ep[e], ep[ntet + e], Ke);
+        if (st != OR_OK) { free(T); *bad_elem = nq + nt + e; return st; }
+        for (int cc = 0; cc < 30; ++cc)
+            for (int rr = 0; rr < 30; ++rr) {
+                T[seq].row = 6 * (int64_t)nd[rr / 3] + rr % 3; T[seq].col = 6 * nd[cc / 3] + cc % 3;
+                T[seq].v = Ke[rr + 30 * cc]; T[seq].seq = seq; ++seq;
+            }
+    }
+    qsort(T, (size_t)ntrip, sizeof(trip_t), trip_cmp);
+    int64_t ndof = 6 * n_nodes, nnz = 0;
+    for (int64_t i = 0; i < ntrip; ++i)
+        if (i == 0 || T[i].row != T[i - 1].row || T[i].col != T[i - 1].col) ++nnz;
+    int64_t *rp = (int64_t *)calloc((size_t)ndof + 1, sizeof(int64_t));
+    int32_t *ci = (int32_t *)malloc(sizeof(int32_t) * (nnz ? nnz : 1));
+    double *cv = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+    if (!rp || !ci || !cv) { free(T); free(rp); free(ci); free(cv); return OR_ERR_OOM; }
+    int64_t k = -1;
+    for (int64_t i = 0; i < ntrip; ++i) {
+        if (i == 0 || T[i].row != T[i - 1].row || T[i].col != T[i - 1].col) {
+            ++k; ci[k] = T[i].col; cv[k] = T[i].v; rp[T[i].row + 1]++;
+        } else {
+            cv[k] += T[i].v; /* duplicates summed in triplet order */
+        }
+    }
+    for (int64_t r = 0; r < ndof; ++r) rp[r + 1] += rp[r];
+    free(T);
+    *out_nnz = nnz; *row_ptr = rp; *col = ci; *val = cv;
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* BC partition: SPC + AUTOSPC (zero/negative diagonal) + grid-point            */
+/* singularity test (src/solver/boundary_conditions.jl:1028-1138, 745-845)     */
+/* ------------------------------------------------------------------------- */
+static double csr_get(const int64_t *rp, const int32_t *ci, const double *v, int64_t i, int64_t j) {
+    int64_t lo = rp[i], hi = rp[i + 1] - 1;
+    while (lo <= hi) {
+        int64_t m = (lo + hi) / 2;
+        if (ci[m] == j) return v[m];
+        if (ci[m] < j) lo = m + 1; else hi = m - 1;
+    }
+    return 0.0;
+}
+
+/* cyclic Jacobi for symmetric 3x3 (A column-major), eigenvalues ascending */
+void or_eig3(const double A[9], double w[3], double V[9]) {
+    double a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) a[i][j] = A[i + 3 * j];
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+        double sc = fabs(a[0][0]) + fabs(a[1][1]) + fabs(a[2][2]);
+        if (off <= 1e-300 || off <= 1e-18 * sc) break;
+        for (int p = 0; p < 2; ++p) for (int q = p + 1; q < 3; ++q) {
+            if (a[p][q] == 0.0) continue;
+            double th = (a[q][q] - a[p][p]) / (2 * a[p][q]);
+            double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+            double c = 1 / sqrt(t * t + 1), s = t * c;
+            for (int k = 0; k < 3; ++k) { double akp = a[k][p], akq = a[k][q]; a[k][p] = c * akp - s * akq; a[k][q] = s * akp + c * akq; }
+            for (int k = 0; k < 3; ++k) { double apk = a[p][k], aqk = a[q][k]; a[p][k] = c * apk - s * aqk; a[q][k] = s * apk + c * aqk; }
+            for (int k = 0; k < 3; ++k) { double vkp = v[k][p], vkq = v[k][q]; v[k][p] = c * vkp - s * vkq; v[k][q] = s * vkp + c * vkq; }
+        }
+    }
+    int idx[3] = {0, 1, 2};
+    for (int i = 0; i < 3; ++i) for (int j = i + 1; j < 3; ++j) if (a[idx[j]][idx[j]] < a[idx[i]][idx[i]]) { int t = idx[i]; idx[i] = idx[j]; idx[j] = t; }
+    for (int k = 0; k < 3; ++k) { w[k] = a[idx[k]][idx[k]]; for (int r = 0; r < 3; ++r) V[r + 3 * k] = v[r][idx[k]]; }
+}
+
+int64_t or_bc_partition(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *v,
+                        uint8_t *fixed, double eps_zero) {
+    int64_t added = 0;
+    /* diagonal AUTOSPC with grid-point processing on: only exact-zero and negative
+     * diagonals are removed here (local_relative_* = false) */
+    for (int64_t i = 0; i < ndof; ++i) {
+        double d = csr_get(rp, ci, v, i, i);
+        if (!fixed[i] && (d == 0.0 || d < 0)) { fixed[i] = 1; ++added; }
+    }
+    /* grid point singularity on 3x3 translational and rotational blocks */
+    int64_t nn = ndof / 6;
+    for (int64_t n = 0; n < nn; ++n) {
+        for (int off = 0; off <= 3; off += 3) {
+            int fl[3], m = 0;
+            for (int a = 0; a < 3; ++a) if (!fixed[6 * n + off + a]) fl[m++] = a;
+            if (m < 1) continue;
+            double S[9] = {0}, w[3], V[9];
+            for (int ii = 0; ii < m; ++ii) for (int jj = 0; jj < m; ++jj) {
+                double kij = csr_get(rp, ci, v, 6 * n + off + fl[ii], 6 * n + off + fl[jj]);
+                double kji = csr_get(rp, ci, v, 6 * n + off + fl[jj], 6 * n + off + fl[ii]);
+                S[ii + 3 * jj] = (kij + kji) / 2;
+            }
+            double vals[3], vecs[9];
+            if (m == 3) { or_eig3(S, w, V); memcpy(vals, w, sizeof(w)); memcpy(vecs, V, sizeof(V)); }
+            else if (m == 2) {
+                double a = S[0], b = S[3], d = S[4];
+                double tr = 0.5 * (a + d), df = 0.5 * (a - d), r = sqrt(df * df + b * b);
+                vals[0] = tr - r; vals[1] = tr + r;
+                for (int k = 0; k < 2; ++k) {
+                    double vx, vy;
+                    if (b != 0.0) { vx = b; vy = vals[k] - a; }
+                    else { vx = (k == 0) == (a <= d) ? 1.0 : 0.0; vy = 1.0 - vx; }
+                    double nr = hypot(vx, vy); vecs[0 + 3 * k] = vx / nr; vecs[1 + 3 * k] = vy / nr;
+                }
+            } else { vals[0] = S[0]; vecs[0] = 1.0; }
+            double lmax = 0; for (int k = 0; k < m; ++k) if (fabs(vals[k]) > lmax) lmax = fabs(vals[k]);
+            if (!(lmax > 0.0)) continue;
+            int claimed[3] = {0, 0, 0};
+            for (int k = 0; k < m; ++k) {
+                double ratio = fabs(vals[k]) / lmax;
+                if (!(ratio < eps_zero)) continue;
+                int best_a = -1; double best_w = -1.0;
+                for (int ii = 0; ii < m; ++ii) {
+                    double wv = fabs(vecs[ii + 3 * k]); int a = fl[ii];
+                    if (claimed[a]) continue;
+                    if (wv > best_w + 1e-12) { best_w = wv; best_a = a; }
+                }
+                if (best_a < 0) continue;
+                int64_t g = 6 * n + off + best_a;
+                if (fixed[g]) continue;
+                claimed[best_a] = 1; fixed[g] = 1; ++added;
+            }
+        }
+    }
+    return added;
+}
+
+/* ------------------------------------------------------------------------- */
+/* reference sparse Cholesky: orderings + up-looking factor (textbook,          */
+/* stands in for CHOLMOD; src/solver/boundary_conditions.jl:1656-1714)          */
+/* ------------------------------------------------------------------------- */
+typedef struct { int64_t n; int64_t *p; int32_t *i; double *x; } csc_t;
+
+/* reverse Cuthill-McKee on a symmetric pattern (adjacency lists, no diagonal) */
+static void rcm_order(int64_t n, const int64_t *ap, const int32_t *ai, int32_t *perm) {
+    int64_t *deg = malloc(sizeof(int64_t) * n); uint8_t *vis = calloc(n, 1);
+    int32_t *q = malloc(sizeof(int32_t) * n);
+    for (int64_t v = 0; v < n; ++v) deg[v] = ap[v + 1] - ap[v];
+    int64_t head = 0, tail = 0;
+    for (;;) {
+        int64_t start = -1;
+        for (int64_t v = 0; v < n; ++v) if (!vis[v] && (start < 0 || deg[v] < deg[start])) start = v;
+        if (start < 0) break;
+        vis[start] = 1; q[tail++] = (int32_t)start;
+        while (head < tail) {
+            int32_t u = q[head++];
+            int64_t b = tail;
+            for (int64_t p = ap[u]; p < ap[u + 1]; ++p) { int32_t w = ai[p]; if (!vis[w]) { vis[w] = 1; q[tail++] = w; } }
+            /* sort new ones by degree (insertion sort) */
+            for (int64_t s = b + 1; s < tail; ++s) { int32_t t = q[s]; int64_t r = s - 1; while (r >= b && deg[q[r]] > deg[t]) { q[r + 1] = q[r]; --r; } q[r + 1] = t; }
+        }
+    }
+    for (int64_t k = 0; k < n; ++k) perm[k] = q[n - 1 - k];
+    free(deg); free(vis); free(q);
+}
+
+/* geometric nested dissection on supervariables (grid points) */
+typedef struct { const double *xyz; const int64_t *ap; const int32_t *ai; int32_t *mark; int32_t *out; int64_t nout; } nd_ctx;
+static int nd_cmp_axis; static const double *nd_xyz;
+static int nd_cmp(const void *a, const void *b) {
+    double u = nd_xyz[3 * *(const int32_t *)a + nd_cmp_axis], w = nd_xyz[3 * *(const int32_t *)b + nd_cmp_axis];
+    return u < w ? -1 : (u > w);
+}
+static void nd_rec(nd_ctx *c, int32_t *set, int64_t n, int32_t tag) {
+    if (n <= 64) { for (int64_t k = 0; k < n; ++k) c->out[c->nout++] = set[k]; return; }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t k = 0; k < n; ++k) for (int a = 0; a < 3; ++a) {
+        double v = c->xyz[3 * set[k] + a]; if (v < lo[a]) lo[a] = v; if (v > hi[a]) hi[a] = v;
+    }
+    int ax = 0; for (int a = 1; a < 3; ++a) if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
+    nd_cmp_axis = ax; nd_xyz = c->xyz;
+    qsort(set, (size_t)n, sizeof(int32_t), nd_cmp);
+    int64_t h = n / 2;
+    /* tag: side A = tag*2+1, side B = tag*2+2 (within this call only) */
+    for (int64_t k = 0; k < n; ++k) c->mark[set[k]] = k < h ? 1 : 2;
+    int32_t *A = malloc(sizeof(int32_t) * n), *B = malloc(sizeof(int32_t) * n), *S = malloc(sizeof(int32_t) * n);
+    int64_t na = 0, nb = 0, ns = 0;
+    for (int64_t k = 0; k < h; ++k) {
+        int32_t v = set[k]; int sep = 0;
+        for (int64_t p = c->ap[v]; p < c->ap[v + 1]; ++p) if (c->mark[c->ai[p]] == 2) { sep = 1; break; }
+        if (sep) S[ns++] = v; else A[na++] = v;
+    }
+    for (int64_t k = h; k < n; ++k) B[nb++] = set[k];
+    for (int64_t k = 0; k < n; ++k) c->mark[set[k]] = 0;
+    if (na == 0 || nb == 0) { for (int64_t k = 0; k < n; ++k) c->out[c->nout++] = set[k]; free(A); free(B); free(S); return; }
+    nd_rec(c, A, na, tag); nd_rec(c, B, nb, tag);
+    for (int64_t k = 0; k < ns; ++k) c->out[c->nout++] = S[k];
+    free(A); free(B); free(S);
+}
+
+int or_solve(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *val,
+             const uint8_t *fixed, int order_kind, const double *xyz_nodes,
+             int nrhs, const double *b, double *u, double *relres,
+             int64_t *nnz_L_out, double *flops_out, int64_t *n_small) {
+    /* free DOF numbering (ascending global DOF) */
+    int64_t *loc = malloc(sizeof(int64_t) * ndof); int64_t n = 0;
+    for (int64_t i = 0; i < ndof; ++i) loc[i] = fixed[i] ? -1 : n++;
+    int64_t *glob = malloc(sizeof(int64_t) * (n ? n : 1));
+    for (int64_t i = 0; i < ndof; ++i) if (loc[i] >= 0) glob[loc[i]] = i;
+    /* symmetric adjacency (off-diagonal) of K_ff */
+    int64_t *ap = calloc(n + 1, sizeof(int64_t));
+    for (int64_t r = 0; r < n; ++r) { int64_t g = glob[r]; for (int64_t p = rp[g]; p < rp[g + 1]; ++p) { int64_t c = loc[ci[p]]; if (c >= 0 && c != r) ap[r + 1]++; } }
+    for (int64_t r = 0; r < n; ++r) ap[r + 1] += ap[r];
+    int32_t *ai = malloc(sizeof(int32_t) * (ap[n] ? ap[n] : 1));
+    { int64_t *fill = malloc(sizeof(int64_t) * (n ? n : 1)); for (int64_t r = 0; r < n; ++r) fill[r] = ap[r];
+      for (int64_t r = 0; r < n; ++r) { int64_t g = glob[r]; for (int64_t p = rp[g]; p < rp[g + 1]; ++p) { int64_t c = loc[ci[p]]; if (c >= 0 && c != r) ai[fill[r]++] = (int32_t)c; } }
+      free(fill); }
+    /* ordering */
+    int32_t *perm = malloc(sizeof(int32_t) * (n ? n : 1));
+    if (order_kind == 1 && xyz_nodes) {
+        /* supervariables = grid points: order grid points by ND, expand to their free DOFs */
+        int64_t nn = ndof / 6;
+        int64_t *np_ = calloc(nn + 1, sizeof(int64_t));
+        /* node adjacency from DOF adjacency */
+        int32_t *mark = calloc(nn, sizeof(int32_t));
+        int64_t cap = ap[n] / 2 + nn + 16, used = 0;
+        int32_t *na = malloc(sizeof(int32_t) * cap);
+        for (int64_t v = 0; v < nn; ++v) {
+            np_[v] = used;
+            int32_t stamp = (int32_t)v + 1;
+            mark[v] = stamp;
+            for (int a = 0; a < 6; ++a) { int64_t r = loc[6 * v + a]; if (r < 0) continue;
+                for (int64_t p = ap[r]; p < ap[r + 1]; ++p) { int64_t w = glob[ai[p]] / 6; if (mark[w] != stamp) { mark[w] = stamp;
+                    if (used >= cap) { cap *= 2; na = realloc(na, sizeof(int32_t) * cap); } na[used++] = (int32_t)w; } } }
+        }
+        np_[nn] = used;
+        int32_t *nodes = malloc(sizeof(int32_t) * (nn ? nn : 1)), *out = malloc(sizeof(int32_t) * (nn ? nn : 1));
+        int64_t cnt = 0;
+        for (int64_t v = 0; v < nn; ++v) { int any = 0; for (int a = 0; a < 6; ++a) if (loc[6 * v + a] >= 0) any = 1; if (any) nodes[cnt++] = (int32_t)v; }
+        for (int64_t v = 0; v < nn; ++v) mark[v] = 0;
+        nd_ctx c = {xyz_nodes, np_, na, mark, out, 0};
+        nd_rec(&c, nodes, cnt, 0);
+        int64_t k = 0;
+        for (int64_t t = 0; t < c.nout; ++t) for (int a = 0; a < 6; ++a) { int64_t r = loc[6 * (int64_t)out[t] + a]; if (r >= 0) perm[k++] = (int32_t)r; }
+        free(np_); free(mark); free(na); free(nodes); free(out);
+    } else {
+        rcm_order(n, ap, ai, perm);
+    }
+    int32_t *pinv = malloc(sizeof(int32_t) * (n ? n : 1));
+    for (int64_t k = 0; k < n; ++k) pinv[perm[k]] = (int32_t)k;
+    /* C = P A P' upper triangle in CSC, values from the original upper triangle
+     * (Symmetric(K_ff) reads the upper triangle) */
+    int64_t *cp = calloc(n + 1, sizeof(int64_t));
+    for (int64_t r = 0; r < n; ++r) { int64_t g = glob[r];
+        for (int64_t p = rp[g]; p < rp[g + 1]; ++p) { int64_t c = loc[ci[p]]; if (c < r) continue; /* original upper incl diag */
+            int64_t pr = pinv[r], pc = pinv[c]; int64_t col = pr > pc ? pr : pc; cp[col + 1]++; } }
+    for (int64_t j = 0; j < n; ++j) cp[j + 1] += cp[j];
+    int32_t *cix = malloc(sizeof(int32_t) * (cp[n] ? cp[n] : 1)); double *cx = malloc(sizeof(double) * (cp[n] ? cp[n] : 1));
+    { int64_t *fill = malloc(sizeof(int64_t) * (n ? n : 1)); for (int64_t j = 0; j < n; ++j) fill[j] = cp[j];
+      for (int64_t r = 0; r < n; ++r) { int64_t g = glob[r];
+        for (int64_t p = rp[g]; p < rp[g + 1]; ++p) { int64_t c = loc[ci[p]]; if (c < r) continue;
+            int64_t pr = pinv[r], pc = pinv[c]; int64_t col = pr > pc ? pr : pc, row = pr > pc ? pc : pr;
+            cix[fill[col]] = (int32_t)row; cx[fill[col]++] = val[p]; } }
+      free(fill); }
+    /* elimination tree (Liu) on C upper */
+    int32_t *parent = malloc(sizeof(int32_t) * (n ? n : 1)), *anc = malloc(sizeof(int32_t) * (n ? n : 1));
+    for (int64_t k = 0; k < n; ++k) { parent[k] = -1; anc[k] = -1;
+        for (int64_t p = cp[k]; p < cp[k + 1]; ++p) { int32_t i = cix[p];
+            for (; i != -1 && i < k; ) { int32_t nx = anc[i]; anc[i] = (int32_t)k; if (nx == -1) { parent[i] = (int32_t)k; break; } i = nx; } } }
+    /* column counts via row subtrees (ereach) */
+    int64_t *cc = calloc(n + 1, sizeof(int64_t)); int32_t *flag = malloc(sizeof(int32_t) * (n ? n : 1)), *stk = malloc(sizeof(int32_t) * (n ? n : 1));
+    for (int64_t k = 0; k < n; ++k) flag[k] = -1;
+    for (int64_t k = 0; k < n; ++k) { flag[k] = (int32_t)k; cc[k]++;
+        for (int64_t p = cp[k]; p < cp[k + 1]; ++p) { int32_t i = cix[p]; if (i >= k) continue;
+            for (; flag[i] != k; i = parent[i]) { flag[i] = (int32_t)k; cc[i]++; } } }
+    int64_t *Lp = malloc(sizeof(int64_t) * (n + 1)); Lp[0] = 0;
+    double fl = 0;
+    for (int64_t k = 0; k < n; ++k) { Lp[k + 1] = Lp[k] + cc[k]; fl += (double)cc[k] * (double)cc[k]; }
+    int64_t nzL = Lp[n];
+    int32_t *Li = malloc(sizeof(int32_t) * (nzL ? nzL : 1)); double *Lx = malloc(sizeof(double) * (nzL ? nzL : 1));
+    double *xw = calloc(n ? n : 1, sizeof(double)); int64_t *cnext = malloc(sizeof(int64_t) * (n ? n : 1));
+    for (int64_t k = 0; k < n; ++k) { cnext[k] = Lp[k]; flag[k] = -1; }
+    int st = OR_OK;
+    for (int64_t k = 0; k < n && st == OR_OK; ++k) {
+        /* nonzero pattern of row k of L: ereach, topological order */
+        int64_t top = n; flag[k] = (int32_t)k;
+        for (int64_t p = cp[k]; p < cp[k + 1]; ++p) { int32_t i = cix[p]; if (i > k) continue; xw[i] += cx[p]; if (i == k) continue;
+            int64_t len = 0; for (; flag[i] != k; i = parent[i]) { stk[len++] = i; flag[i] = (int32_t)k; }
+            while (len > 0) stk[--top] = stk[--len]; }
+        double d = xw[k]; xw[k] = 0;
+        for (; top < n; ++top) { int32_t i = stk[top]; double lki = xw[i] / Lx[Lp[i]]; xw[i] = 0;
+            for (int64_t p = Lp[i] + 1; p < cnext[i]; ++p) xw[Li[p]] -= Lx[p] * lki;
+            d -= lki * lki; int64_t p2 = cnext[i]++; Li[p2] = (int32_t)k; Lx[p2] = lki; }
+        if (!(d > 0)) { st = OR_ERR_NOT_SPD; break; }
+        int64_t p = cnext[k]++; Li[p] = (int32_t)k; Lx[p] = sqrt(d);
+    }
+    /* column i of L is filled at cnext[i]; its first write is the diagonal (step
+     * k == i precedes every step k > i), so Lx[Lp[i]] is L(i,i) */
+    int64_t smallp = 0;
+    if (st == OR_OK) {
+        for (int64_t k = 0; k < n; ++k) { double lkk = Lx[Lp[k]]; int64_t g = glob[perm[k]];
+            double kd = fabs(csr_get(rp, ci, val, g, g));
+            double ratio = kd > 1e-30 ? lkk * lkk / kd : 1.0; if (ratio < 1e-7) ++smallp; }
+        double *y = malloc(sizeof(double) * (n ? n : 1));
+        for (int r = 0; r < nrhs; ++r) {
+            const double *br = b + (size_t)ndof * r; double *ur = u + (size_t)ndof * r;
+            for (int64_t k = 0; k < n; ++k) y[k] = br[glob[perm[k]]];
+            for (int64_t j = 0; j < n; ++j) { y[j] /= Lx[Lp[j]]; for (int64_t p = Lp[j] + 1; p < Lp[j + 1]; ++p) y[Li[p]] -= Lx[p] * y[j]; }
+            for (int64_t j = n - 1; j >= 0; --j) { for (int64_t p = Lp[j] + 1; p < Lp[j + 1]; ++p) y[j] -= Lx[p] * y[Li[p]]; y[j] /= Lx[Lp[j]]; }
+            for (int64_t i = 0; i < ndof; ++i) ur[i] = 0.0;
+            for (int64_t k = 0; k < n; ++k) ur[glob[perm[k]]] = y[k];
+            /* relative residual ||K_ff u - F_ff|| / max(||F_ff||, 1e-30), :1859-1868 */
+            double rn = 0, fn = 0;
+            for (int64_t r2 = 0; r2 < n; ++r2) { int64_t g = glob[r2]; double s = 0;
+                for (int64_t p = rp[g]; p < rp[g + 1]; ++p) if (loc[ci[p]] >= 0) s += val[p] * ur[ci[p]];
+                double e = s - br[g]; rn += e * e; fn += br[g] * br[g]; }
+            relres[r] = sqrt(rn) / fmax(sqrt(fn), 1e-30);
+        }
+        free(y);
+    }
+    if (nnz_L_out) *nnz_L_out = nzL;
+    if (flops_out) *flops_out = fl;
+    if (n_small) *n_small = smallp;
+    free(loc); free(glob); free(ap); free(ai); free(perm); free(pinv); free(cp); free(cix); free(cx);
+    free(parent); free(anc); free(cc); free(flag); free(stk); free(Lp); free(Li); free(Lx); free(xw); free(cnext);
+    return st;
+}

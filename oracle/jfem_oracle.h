@@ -1,0 +1,89 @@
+/*
+ * jfem_oracle.h -- CPU restatement of OpenJFEM's linear-static (SOL 101) hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * implementation in jf_2026_05_25_amd/csrc.  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py may load it.  The product path never calls
+ * it and never falls back to it.
+ *
+ * Conventions (shared with include/jfem_hip.h):
+ *   - matrices are column-major (Julia layout), 0-based;
+ *   - node i owns global DOFs 6*i .. 6*i+5 (u,v,w,rx,ry,rz);
+ *   - connectivity arrays are column-major n_elem x nn (conn[k*n_elem + e]);
+ *   - shell properties are column-major n_elem x 8 (see JFEM_SP_* below).
+ */
+#ifndef JFEM_ORACLE_H
+#define JFEM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* shell property columns (same as include/jfem_hip.h) */
+enum { OR_SP_H = 0, OR_SP_E = 1, OR_SP_NU = 2, OR_SP_G = 3, OR_SP_BR = 4,
+       OR_SP_TST = 5, OR_SP_RIGID = 6, OR_SP_MODES = 7, OR_SP_NCOL = 8 };
+
+/* status codes (same numbering as include/jfem_hip.h) */
+enum { OR_OK = 0, OR_ERR_DEGENERATE = 2, OR_ERR_INVERTED = 3, OR_ERR_WARPED = 4,
+       OR_ERR_BAD_ARG = 5, OR_ERR_NOT_SPD = 6, OR_ERR_OOM = 7 };
+
+/* ---------- element kernels (local frame) ---------- */
+int or_q4_local(const double lc[8], const double Cm[9], const double Cb[9],
+                const double Cs[4], double h, double E_ref, double k6rot,
+                double drill_scale, int rigid_shear, double Ke[576]);
+/* same with an explicit membrane internal-mode mask (bit i = mode i+1 active;
+ * 15 = all four, the default; 6 = modes 2,3 = JFEM_SOL101_Q4_CROSS_MEMBRANE_WEIGHTS) */
+int or_q4_local_modes(const double lc[8], const double Cm[9], const double Cb[9],
+                      const double Cs[4], double h, double E_ref, double k6rot,
+                      double drill_scale, int rigid_shear, int membrane_modes, double Ke[576]);
+/* MacNeal shear pieces for the closed-form pins: D (4x12), Zb, Zs, Z (4x4), Kp (12x12) */
+int or_q4_macneal_parts(const double lc[8], const double Cb[9], const double Cs[4],
+                        int rigid_shear, double D[48], double Zb[16], double Zs[16],
+                        double Z[16], double Kp[144]);
+int or_t3_local(const double lc[6], const double Dm[9], const double Db[9],
+                const double Ds[4], double h, double G_ref, double k6rot, double Ke[324]);
+int or_tet10(const double X[30] /* 10x3 column-major */, double E, double nu, double Ke[900]);
+
+/* ---------- element kernels (global frame, as assembled) ---------- */
+/* p: 4 nodes xyz, p[3*k + c]; prop: 8 values (JFEM_SP_*). Ke: global 24x24. */
+int or_q4_global(const double p[12], const double prop[8], double k6rot, double Ke[576]);
+int or_t3_global(const double p[9], const double prop[8], double k6rot, double Ke[324]);
+
+/* ---------- assembly with sparse(I,J,V) semantics ---------- */
+/* Triplet order: all CQUAD4 (element order, column-major 24x24), then CTRIA3, then
+ * CTETRA10.  Duplicates summed in triplet order.  Output CSR (row_ptr n+1, col, val)
+ * allocated with malloc; caller frees with or_free. */
+int or_assemble(int64_t n_nodes, const double *x, const double *y, const double *z,
+                int64_t nq, const int32_t *q_conn, const double *q_prop,
+                int64_t nt, const int32_t *t_conn, const double *t_prop,
+                int64_t ntet, const int32_t *tet_conn, const double *tet_prop,
+                double k6rot, int64_t *out_nnz, int64_t **row_ptr, int32_t **col,
+                double **val, int64_t *bad_elem);
+void or_free(void *p);
+
+/* ---------- boundary-condition partition ---------- */
+/* fixed[ndof] in/out: 1 = SPC'd on input; on output also AUTOSPC (zero/negative
+ * diagonal) and grid-point-singularity DOFs.  Returns number of DOFs added. */
+int64_t or_bc_partition(int64_t ndof, const int64_t *row_ptr, const int32_t *col,
+                        const double *val, uint8_t *fixed, double eps_zero);
+
+/* ---------- reference sparse Cholesky (up-looking, scalar) ---------- */
+/* Factor K_ff (rows/cols with fixed==0) with the given ordering kind
+ * (0 = reverse Cuthill-McKee, 1 = geometric nested dissection on node xyz),
+ * solve nrhs right-hand sides (b: ndof x nrhs column-major, full DOF numbering,
+ * fixed rows ignored), write u (ndof x nrhs; fixed rows = 0) and the relative
+ * residual ||K_ff u - F_ff|| / max(||F_ff||, 1e-30) per RHS.
+ * Also returns the post-factor pivot check count (L_ii^2/K_ii < 1e-7). */
+int or_solve(int64_t ndof, const int64_t *row_ptr, const int32_t *col, const double *val,
+             const uint8_t *fixed, int order_kind, const double *xyz_nodes,
+             int nrhs, const double *b, double *u, double *relres,
+             int64_t *nnz_L, double *flops, int64_t *n_small_pivots);
+
+/* dense helper exposed for tests: symmetric 3x3 Jacobi eigen (ascending) */
+void or_eig3(const double A[9], double w[3], double V[9]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,205 @@
+"""ctypes front end of the CPU oracle (libjfem_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product path never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import scipy.sparse as sp
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+ip32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+ip64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "libjfem_oracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libjfem_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.or_q4_local.argtypes = [dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, dp]
+        L.or_q4_macneal_parts.argtypes = [dp, dp, dp, C.c_int, dp, dp, dp, dp, dp]
+        L.or_t3_local.argtypes = [dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, dp]
+        L.or_tet10.argtypes = [dp, C.c_double, C.c_double, dp]
+        L.or_q4_global.argtypes = [dp, dp, C.c_double, dp]
+        L.or_t3_global.argtypes = [dp, dp, C.c_double, dp]
+        L.or_assemble.argtypes = [C.c_int64, dp, dp, dp, C.c_int64, C.c_void_p, C.c_void_p,
+                                  C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                  C.c_double, C.POINTER(C.c_int64), C.POINTER(C.POINTER(C.c_int64)),
+                                  C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_double)),
+                                  C.POINTER(C.c_int64)]
+        L.or_free.argtypes = [C.c_void_p]
+        L.or_bc_partition.argtypes = [C.c_int64, ip64, ip32, dp, u8p, C.c_double]
+        L.or_bc_partition.restype = C.c_int64
+        L.or_solve.argtypes = [C.c_int64, ip64, ip32, dp, u8p, C.c_int, C.c_void_p, C.c_int, dp, dp, dp,
+                               C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.or_eig3.argtypes = [dp, dp, dp]
+        _LIB = L
+    return _LIB
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, what=""):
+        super().__init__(f"oracle status {code} {what}")
+        self.code = code
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _colmajor(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).T).ravel()
+
+
+def q4_local(lc, Cm, Cb, Cs, h=1.0, E_ref=1.0, k6rot=100.0, drill_scale=1.0, rigid=False):
+    Ke = np.zeros(576)
+    st = lib().or_q4_local(_f(np.asarray(lc, float).reshape(4, 2).ravel()), _colmajor(Cm), _colmajor(Cb),
+                           _colmajor(Cs), h, E_ref, k6rot, drill_scale, int(rigid), Ke)
+    if st:
+        raise OracleError(st)
+    return Ke.reshape(24, 24, order="F")
+
+
+def q4_macneal_parts(lc, Cb, Cs, rigid=False):
+    D = np.zeros(48); Zb = np.zeros(16); Zs = np.zeros(16); Z = np.zeros(16); Kp = np.zeros(144)
+    st = lib().or_q4_macneal_parts(_f(np.asarray(lc, float).reshape(4, 2).ravel()), _colmajor(Cb),
+                                   _colmajor(Cs), int(rigid), D, Zb, Zs, Z, Kp)
+    if st:
+        raise OracleError(st)
+    return {"D": D.reshape(4, 12, order="F"), "Zb": Zb.reshape(4, 4, order="F"),
+            "Zs": Zs.reshape(4, 4, order="F"), "Z": Z.reshape(4, 4, order="F"),
+            "Kp": Kp.reshape(12, 12, order="F")}
+
+
+def t3_local(lc, Dm, Db, Ds, h=1.0, G_ref=1.0, k6rot=100.0):
+    Ke = np.zeros(324)
+    st = lib().or_t3_local(_f(np.asarray(lc, float).reshape(3, 2).ravel()), _colmajor(Dm), _colmajor(Db),
+                           _colmajor(Ds), h, G_ref, k6rot, Ke)
+    if st:
+        raise OracleError(st)
+    return Ke.reshape(18, 18, order="F")
+
+
+def tet10(X, E, nu):
+    Ke = np.zeros(900)
+    st = lib().or_tet10(_colmajor(np.asarray(X, float).reshape(10, 3)), E, nu, Ke)
+    if st:
+        raise OracleError(st)
+    return Ke.reshape(30, 30, order="F")
+
+
+def q4_global(p, prop, k6rot=100.0):
+    Ke = np.zeros(576)
+    st = lib().or_q4_global(_f(np.asarray(p, float).reshape(4, 3).ravel()), _f(prop), k6rot, Ke)
+    if st:
+        raise OracleError(st)
+    return Ke.reshape(24, 24, order="F")
+
+
+def t3_global(p, prop, k6rot=100.0):
+    Ke = np.zeros(324)
+    st = lib().or_t3_global(_f(np.asarray(p, float).reshape(3, 3).ravel()), _f(prop), k6rot, Ke)
+    if st:
+        raise OracleError(st)
+    return Ke.reshape(18, 18, order="F")
+
+
+def _cm_i32(a):
+    a = np.asarray(a, dtype=np.int32)
+    return np.ascontiguousarray(a.T).ravel() if a.size else np.zeros(1, np.int32)
+
+
+def _cm_f64(a):
+    a = np.asarray(a, dtype=np.float64)
+    return np.ascontiguousarray(a.T).ravel() if a.size else np.zeros(1)
+
+
+def assemble(model):
+    """Global K (scipy CSR, ndof x ndof) with sparse(I,J,V) duplicate summation."""
+    L = lib()
+    x = _f(model.xyz[:, 0]); y = _f(model.xyz[:, 1]); z = _f(model.xyz[:, 2])
+    qc, qp = _cm_i32(model.quads), _cm_f64(model.quad_props)
+    tc, tp = _cm_i32(model.trias), _cm_f64(model.tria_props)
+    ec, ep = _cm_i32(model.tets), _cm_f64(model.tet_props)
+    nnz = C.c_int64(); rp = C.POINTER(C.c_int64)(); ci = C.POINTER(C.c_int32)(); cv = C.POINTER(C.c_double)()
+    bad = C.c_int64(-1)
+    st = L.or_assemble(model.n_nodes, x, y, z, len(model.quads), qc.ctypes.data, qp.ctypes.data,
+                       len(model.trias), tc.ctypes.data, tp.ctypes.data,
+                       len(model.tets), ec.ctypes.data, ep.ctypes.data, model.k6rot,
+                       C.byref(nnz), C.byref(rp), C.byref(ci), C.byref(cv), C.byref(bad))
+    if st:
+        raise OracleError(st, f"element {bad.value}")
+    n = model.ndof
+    indptr = np.ctypeslib.as_array(rp, shape=(n + 1,)).copy()
+    indices = np.ctypeslib.as_array(ci, shape=(max(nnz.value, 1),))[:nnz.value].copy()
+    data = np.ctypeslib.as_array(cv, shape=(max(nnz.value, 1),))[:nnz.value].copy()
+    L.or_free(rp); L.or_free(ci); L.or_free(cv)
+    return sp.csr_matrix((data, indices, indptr), shape=(n, n))
+
+
+def bc_partition(K, spc, eps_zero=1e-8):
+    """Fixed-DOF mask after SPC + AUTOSPC + grid-point singularity processing."""
+    K = sp.csr_matrix(K)
+    K.sort_indices()
+    fixed = np.ascontiguousarray(np.asarray(spc, dtype=np.uint8).copy())
+    lib().or_bc_partition(K.shape[0], K.indptr.astype(np.int64), K.indices.astype(np.int32),
+                          _f(K.data), fixed, eps_zero)
+    return fixed
+
+
+def solve(K, fixed, B, order="nd", xyz=None):
+    """Reference sparse Cholesky solve of K_ff u = F_ff for each column of B.
+    Returns (u (ndof, nrhs), relres (nrhs,), info dict)."""
+    K = sp.csr_matrix(K)
+    K.sort_indices()
+    B = np.asarray(B, dtype=np.float64)
+    if B.ndim == 1:
+        B = B[:, None]
+    n = K.shape[0]
+    nrhs = B.shape[1]
+    Bc = np.ascontiguousarray(B.T).ravel()
+    U = np.zeros(n * nrhs)
+    rr = np.zeros(nrhs)
+    nzl = C.c_int64(); fl = C.c_double(); ns = C.c_int64()
+    xyzc = _f(np.asarray(xyz, float).ravel()) if xyz is not None else None
+    st = lib().or_solve(n, K.indptr.astype(np.int64), K.indices.astype(np.int32), _f(K.data),
+                        np.ascontiguousarray(np.asarray(fixed, np.uint8)),
+                        1 if (order == "nd" and xyz is not None) else 0,
+                        xyzc.ctypes.data if xyzc is not None else None, nrhs, Bc, U, rr,
+                        C.byref(nzl), C.byref(fl), C.byref(ns))
+    if st:
+        raise OracleError(st)
+    return U.reshape(nrhs, n).T, rr, {"nnz_L": nzl.value, "flops_colcount2": fl.value,
+                                      "small_pivots": ns.value}
+
+
+def eig3(A):
+    w = np.zeros(3); V = np.zeros(9)
+    lib().or_eig3(_colmajor(A), w, V)
+    return w, V.reshape(3, 3, order="F")
+
+
+def sol101(model, order="nd"):
+    """assemble -> BC partition -> factor -> solve, all on the CPU oracle."""
+    K = assemble(model)
+    fixed = bc_partition(K, model.spc)
+    u, rr, info = solve(K, fixed, model.loads, order=order, xyz=model.xyz)
+    info["free_dofs"] = int((fixed == 0).sum())
+    return K, fixed, u, rr, info

@@ -1,0 +1,135 @@
+"""Extract the reference's own SOL 101 golden data into small JSON fixtures.
+
+Reads (as text/JSON data only) the flat-shell validation decks of the reference
+and their recorded displacement vectors ``results/*.JU.JSON``:
+
+* validation/cases/MYSTRAN_sol101_simple  (1 CQUAD4, FORCE and PLOAD4 subcases)
+* validation/cases/MH_curved_beam_in_plane (6 tapered CQUAD4, in-plane FORCE)
+* validation/cases/MH_curved_beam_refined  (96 CQUAD4, in-plane FORCE)
+
+Curved and twisted decks (warped quads) and the TACS-backend uCRM-9 vector are
+not on this path and are not extracted.  Output: tests/golden/reference_ju.json.
+Run from the repository root: ``python tests/golden/make_fixtures.py``.
+"""
+import json
+import os
+import sys
+
+REF = "/root/reference/validation/cases"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_ju.json")
+
+CASES = [
+    ("MYSTRAN_sol101_simple", "results/sol101_simple.JU.JSON", {1: 1992, 3: 1994}, 101),
+    ("MH_curved_beam_in_plane", "results/curved_beam.JU.JSON", {1: 1}, 1),
+    ("MH_curved_beam_refined", "results/curved_beam_refined.JU.JSON", {1: 1}, 1),
+]
+
+
+def num(s):
+    s = s.strip()
+    if not s:
+        return None
+    try:
+        return float(s)
+    except ValueError:
+        pass
+    # Nastran exponent shorthand: 1.0+7, 10.-3
+    body = s
+    for i in range(len(body) - 1, 0, -1):
+        if body[i] in "+-" and body[i - 1] not in "eE":
+            return float(body[:i] + "e" + body[i:])
+    raise ValueError(s)
+
+
+def cards(path):
+    """Bulk cards as lists of string fields (free or small fixed field, '+' continuations)."""
+    lines = open(path).read().splitlines()
+    out = []
+    in_bulk = False
+    for raw in lines:
+        if raw.strip().upper().startswith("BEGIN BULK"):
+            in_bulk = True
+            continue
+        if not in_bulk or raw.startswith("$") or not raw.strip():
+            continue
+        if raw.strip().upper().startswith("ENDDATA"):
+            break
+        line = raw.rstrip()
+        if "," in line:
+            f = [x.strip() for x in line.split(",")]
+        else:
+            f = [line[i:i + 8].strip() for i in range(0, min(len(line), 72), 8)]
+        if f[0].startswith("+") and out:
+            out[-1].extend(f[1:])
+        else:
+            out.append(f)
+    return out
+
+
+def parse(case, loads_by_subcase, spc_sid):
+    deck = os.path.join(REF, case, "model.bdf")
+    grids, quads, pshell, mat1, spc, forces, pload4 = {}, [], {}, {}, [], {}, {}
+    k6rot = 100.0
+    for f in cards(deck):
+        name = f[0].upper()
+        g = lambda i: f[i] if i < len(f) else ""
+        if name == "PARAM" and g(1).upper() == "K6ROT":
+            k6rot = num(g(2))
+        elif name == "GRID":
+            grids[int(g(1))] = [num(g(3)) or 0.0, num(g(4)) or 0.0, num(g(5)) or 0.0]
+        elif name == "CQUAD4":
+            quads.append({"eid": int(g(1)), "pid": int(g(2)), "nodes": [int(g(k)) for k in range(3, 7)]})
+        elif name == "PSHELL":
+            pshell[int(g(1))] = {"mid1": int(g(2)), "t": num(g(3)),
+                                 "mid2": int(g(4)) if g(4) else 0,
+                                 "bend_ratio": num(g(5)) if g(5) else 1.0,
+                                 "mid3": int(g(6)) if g(6) else 0,
+                                 "ts_t": num(g(7)) if g(7) else 5.0 / 6.0}
+        elif name == "MAT1":
+            mat1[int(g(1))] = {"E": num(g(2)), "G": num(g(3)) or 0.0, "nu": num(g(4)) or 0.0}
+        elif name == "SPC1":
+            sid = int(g(1))
+            comps = [int(c) for c in g(2)]
+            for k in range(3, len(f)):
+                if f[k]:
+                    spc.append((sid, int(f[k]), comps))
+        elif name == "FORCE":
+            sid = int(g(1))
+            forces.setdefault(sid, []).append({"grid": int(g(2)), "mag": num(g(4)),
+                                               "dir": [num(g(5)) or 0.0, num(g(6)) or 0.0, num(g(7)) or 0.0]})
+        elif name == "PLOAD4":
+            pload4.setdefault(int(g(1)), []).append({"eid": int(g(2)), "p": num(g(3))})
+    ju = json.load(open(os.path.join(REF, case, CASE_JU[case])))
+    disp = ju["displacements"]
+    n_sub = len(ju["solver_diagnostics"])
+    per = len(disp) // n_sub
+    subcases = []
+    for k, diag in enumerate(ju["solver_diagnostics"]):
+        sid = int(diag.get("sid", k + 1))
+        if sid not in loads_by_subcase:
+            continue
+        lsid = loads_by_subcase[sid]
+        rows = disp[k * per:(k + 1) * per]
+        subcases.append({
+            "subcase": sid, "load_sid": lsid,
+            "force": forces.get(lsid, []), "pload4": pload4.get(lsid, []),
+            "expected": {str(r["grid_id"]): [r["t1"], r["t2"], r["t3"], r["r1"], r["r2"], r["r3"]] for r in rows},
+            "reference_free_dofs": diag["details"]["bc_partition"]["free_dofs"],
+            "reference_relative_residual": diag["details"]["linear_solver"]["relative_residual"],
+        })
+    return {"case": case, "k6rot": k6rot, "grids": {str(k): v for k, v in grids.items()},
+            "cquad4": quads, "pshell": {str(k): v for k, v in pshell.items()},
+            "mat1": {str(k): v for k, v in mat1.items()},
+            "spc1": [{"grid": n, "comps": c} for (s, n, c) in spc if s == spc_sid],
+            "subcases": subcases,
+            "provenance": f"validation/cases/{case}/model.bdf + {CASE_JU[case]}"}
+
+
+CASE_JU = {c: j for c, j, _, _ in CASES}
+
+if __name__ == "__main__":
+    if not os.path.isdir(REF):
+        sys.exit("reference validation cases not present")
+    data = [parse(c, l, s) for c, _, l, s in CASES]
+    json.dump(data, open(OUT, "w"), indent=1)
+    print("wrote", OUT, os.path.getsize(OUT), "bytes")
