@@ -1,0 +1,195 @@
+/*
+ * jfem_hip.h -- C ABI of the MI355X (gfx950) implementation of OpenJFEM's
+ * linear-static hot path: batched element stiffness (CQUAD4, CTRIA3, CTETRA10),
+ * deterministic scatter-assembly into a device-resident CSR matrix, and a
+ * supernodal sparse Cholesky factor (FP64, or FP32 with FP64 iterative
+ * refinement) with its triangular solves.
+ *
+ * Conventions
+ *   - plain pointers and sizes only; host pointers unless a name says "dev";
+ *   - matrices column-major (Julia layout), 0-based indices;
+ *   - grid point i owns global DOFs 6*i .. 6*i+5 (u, v, w, rx, ry, rz), i = rank of
+ *     the GRID ID (src/solver/assembly.jl:2653-2657);
+ *   - connectivity is column-major n_elem x nn: conn[k*n_elem + e];
+ *   - shell section records are column-major n_elem x 8 (JFEM_SP_*);
+ *   - every entry point returns a jfem_status; jfem_last_error(ctx) has the text.
+ *   - the library owns all device memory behind opaque handles; handles may be used
+ *     from any context of the same device; calls on one context are ordered on that
+ *     context's HIP stream; different contexts may run concurrently.
+ */
+#ifndef JFEM_HIP_H
+#define JFEM_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    JFEM_OK = 0,
+    JFEM_ERR_HIP = 1,             /* HIP runtime error */
+    JFEM_ERR_DEGENERATE = 2,      /* degenerate/folded shell: ArgumentError, src/FEMKernels.jl:2862-2867, 4244-4251 */
+    JFEM_ERR_INVERTED = 3,        /* non-positive solid Jacobian: DomainError, src/tetra10.jl:31-35 */
+    JFEM_ERR_WARPED = 4,          /* warped CQUAD4: finite-warp congruence not built here */
+    JFEM_ERR_BAD_ARG = 5,         /* bad sizes / material: ArgumentError, src/tetra10.jl:55-58 */
+    JFEM_ERR_NOT_SPD = 6,         /* factorization hit a non-positive pivot: PosDefException */
+    JFEM_ERR_OOM = 7,             /* device allocation failed */
+    JFEM_ERR_NOT_CONVERGED = 8    /* refinement did not reach the requested residual */
+} jfem_status;
+
+/* shell section record columns (per element) */
+enum { JFEM_SP_H = 0, JFEM_SP_E = 1, JFEM_SP_NU = 2, JFEM_SP_G = 3, JFEM_SP_BEND_RATIO = 4,
+       JFEM_SP_TS_T = 5, JFEM_SP_RIGID_SHEAR = 6, JFEM_SP_MEMBRANE_MODES = 7, JFEM_SP_NCOL = 8 };
+
+enum { JFEM_KIND_CQUAD4 = 0, JFEM_KIND_CTRIA3 = 1, JFEM_KIND_CTETRA10 = 2 };
+enum { JFEM_FP64 = 64, JFEM_FP32 = 32 };
+
+typedef struct jfem_ctx jfem_ctx;
+typedef struct jfem_mesh jfem_mesh;
+typedef struct jfem_pattern jfem_pattern;
+typedef struct jfem_matrix jfem_matrix;
+typedef struct jfem_symbolic jfem_symbolic;
+typedef struct jfem_factor jfem_factor;
+
+/* ---- contexts ----------------------------------------------------------- */
+/* One per host thread or request slot: a HIP stream on `device` plus scratch.
+ * (No reference counterpart: the reference is single-process Julia.) */
+int jfem_ctx_create(int device, jfem_ctx **out);
+int jfem_ctx_destroy(jfem_ctx *ctx);
+int jfem_ctx_synchronize(jfem_ctx *ctx);
+const char *jfem_last_error(jfem_ctx *ctx);
+/* free / total device memory in bytes (admission control) */
+int jfem_device_memory(jfem_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+/* milliseconds accumulated by the named kernel family since the last reset
+ * (timed with HIP events on the context's stream); family: 0 element generation,
+ * 1 assembly, 2 numeric factor, 3 triangular solves, 4 residual SpMV. */
+int jfem_ctx_kernel_time(jfem_ctx *ctx, int family, double *ms, int64_t *launches);
+int jfem_ctx_reset_kernel_time(jfem_ctx *ctx);
+int jfem_ctx_enable_timing(jfem_ctx *ctx, int on);
+
+/* ---- element kernels (batched FEM.stiffness_* over a device mesh) -------- */
+/* Replaces FEM.stiffness_quad4_matrices (src/FEMKernels.jl:2839),
+ * FEM.stiffness_tria3_matrices (:9811) and FEM.stiffness_tetra10
+ * (src/tetra10.jl:68) as called from assemble_stiffness
+ * (src/solver/assembly.jl:5220-5244, 5474, 5842) INCLUDING the per-element frame,
+ * section and T'KT transform (src/solver/assembly.jl:359-440, 5484-5499, 5858-5870):
+ * writes every element's GLOBAL-frame matrix (24x24 / 18x18 / 30x30 column-major,
+ * element-major) of the given kind into host buffer Ke_out.  bad_element receives
+ * the first failing element index (or -1). */
+int jfem_element_matrices(jfem_ctx *ctx, const jfem_mesh *mesh, int kind,
+                          double *Ke_out, int64_t *bad_element);
+
+/* Local-frame CQUAD4 kernel over n elements, the direct counterpart of
+ * stiffness_quad4_matrices(coords, Cm, Cb, Cs, h, E_ref; k6rot, drill_scale, ...)
+ * (src/FEMKernels.jl:2839).  lc: n x 8 (x1,y1,...,x4,y4 per element, element-major),
+ * Cm, Cb: n x 9, Cs: n x 4 (each element's matrix column-major, element-major);
+ * h, k6rot, drill_scale: n; flags: n (bit0 rigid shear, bits 4-7 membrane mode mask,
+ * 0 = all four).  Ke_out: n x 576. */
+int jfem_q4_local_batch(jfem_ctx *ctx, int64_t n, const double *lc, const double *Cm,
+                        const double *Cb, const double *Cs, const double *h,
+                        const double *k6rot, const double *drill_scale, const int32_t *flags,
+                        double *Ke_out, int64_t *bad_element);
+
+/* ---- mesh --------------------------------------------------------------- */
+/* Upload a model (coordinates SoA, connectivity, sections) to the device.
+ * Replaces the per-call model Dict walk of assemble_stiffness
+ * (src/solver/assembly.jl:2653-2674, 3224-3473). */
+int jfem_mesh_create(jfem_ctx *ctx, int64_t n_nodes, const double *x, const double *y,
+                     const double *z, int64_t n_quad, const int32_t *quad_conn,
+                     const double *quad_prop, int64_t n_tria, const int32_t *tria_conn,
+                     const double *tria_prop, int64_t n_tet, const int32_t *tet_conn,
+                     const double *tet_prop, double k6rot, jfem_mesh **out);
+/* Replace the section records (same topology): design variants. */
+int jfem_mesh_update_props(jfem_ctx *ctx, jfem_mesh *mesh, const double *quad_prop,
+                           const double *tria_prop, const double *tet_prop);
+int jfem_mesh_free(jfem_mesh *mesh);
+
+/* ---- pattern + assembly -------------------------------------------------- */
+/* Symbolic assembly: the CSR pattern of the global K (all 6*n_nodes DOFs) and the
+ * element-DOF -> CSR-slot gather lists; the reference analogue is the slot-map
+ * CSC refill (src/solver/assembly.jl:5975-6033).  Depends on topology only. */
+int jfem_pattern_create(jfem_ctx *ctx, const jfem_mesh *mesh, jfem_pattern **out, int64_t *nnz);
+int jfem_pattern_free(jfem_pattern *pattern);
+
+/* Numeric assembly: element generation + deterministic segmented reduction.
+ * Replaces assemble_stiffness -> sparse(I,J,V) (src/solver/assembly.jl:2630, 5942):
+ * duplicates are summed in triplet order (CQUAD4s, then CTRIA3s, then CTETRA10s,
+ * each in element order), so values are reproducible bit for bit run to run. */
+int jfem_assemble(jfem_ctx *ctx, const jfem_mesh *mesh, const jfem_pattern *pattern,
+                  jfem_matrix **out, int64_t *bad_element);
+/* re-assemble into an existing matrix (same pattern), e.g. after update_props */
+int jfem_reassemble(jfem_ctx *ctx, const jfem_mesh *mesh, jfem_matrix *K, int64_t *bad_element);
+int jfem_matrix_download(jfem_ctx *ctx, const jfem_matrix *K, int64_t *row_ptr, int32_t *col,
+                         double *val);
+int jfem_matrix_free(jfem_matrix *K);
+
+/* ---- boundary conditions -------------------------------------------------- */
+/* SPC + AUTOSPC (zero/negative diagonal) + grid-point singularity processing:
+ * src/solver/boundary_conditions.jl:1480-1607, 1028-1138, 745-845.
+ * spc: ndof (1 = SPC1); fixed_out: ndof; n_free: number of free DOFs. */
+int jfem_bc_partition(jfem_ctx *ctx, const jfem_matrix *K, const uint8_t *spc,
+                      double eps_zero, uint8_t *fixed_out, int64_t *n_free);
+
+/* ---- factor / solve -------------------------------------------------------- */
+typedef struct {
+    int64_t n_free;          /* order of K_ff */
+    int64_t n_supernodes;
+    int64_t n_levels;        /* elimination-tree levels (batched launch groups) */
+    int64_t nnz_L;           /* stored entries of L (supernodal, incl. explicit zeros) */
+    double flops;            /* sum_j (eta_j + 1)^2 over columns of L (counted per column) */
+    int64_t max_front;       /* largest front order */
+    int64_t update_entries;  /* stored update-matrix entries */
+} jfem_symbolic_info;
+
+/* Fill-reducing ordering (geometric nested dissection over grid points),
+ * elimination tree, supernodes, front maps for K_ff = K[free, free].  Replaces the
+ * symbolic half of cholesky(Symmetric(K_ff)) (src/solver/boundary_conditions.jl:1660).
+ * xyz: n_nodes x 3 column-major (may be NULL to use the mesh coordinates). */
+int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *pattern, const jfem_mesh *mesh,
+                         const uint8_t *fixed, jfem_symbolic **out, jfem_symbolic_info *info);
+int jfem_symbolic_free(jfem_symbolic *S);
+
+/* Numeric supernodal Cholesky of K_ff in FP64 or FP32 (precision = 64 | 32).
+ * Reads the upper triangle of K (Symmetric(K_ff) semantics).  On a non-positive
+ * pivot returns JFEM_ERR_NOT_SPD with bad_column = the free-DOF index (global DOF).
+ * Replaces _timed_factorization(cholesky, Symmetric(K_ff))
+ * (src/solver/boundary_conditions.jl:1660,1709). */
+int jfem_factor_create(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K,
+                       int precision, jfem_factor **out, int64_t *bad_column);
+/* refactor in place with new values (same symbolic, same precision) */
+int jfem_refactor(jfem_ctx *ctx, jfem_factor *F, const jfem_matrix *K, int64_t *bad_column);
+/* post-factor pivot check L_jj^2 / K_jj < threshold
+ * (src/solver/boundary_conditions.jl:1662-1710): writes the global DOFs that fail
+ * into dofs_out (capacity cap) and their count. */
+int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K,
+                             double threshold, int64_t *dofs_out, int64_t cap, int64_t *count);
+int jfem_factor_free(jfem_factor *F);
+
+/* Solve K_ff x = b for nrhs right-hand sides given in FULL DOF numbering
+ * (b: ndof x nrhs column-major, fixed rows ignored; x: ndof x nrhs, fixed rows 0).
+ * Replaces F \ F_ff (src/solver/boundary_conditions.jl:1442, 1714) and the
+ * residual (:1859-1868).  Runs triangular solves in the factor's precision and, while
+ * ||K_ff x - b|| / max(||b||, 1e-30) > tol and iterations < max_refine, refines in FP64:
+ * r = b - K_ff x (FP64 SpMV), x += F \ r.  relres[k] and iters[k] per RHS.
+ * b / x on the host: *_host; device-resident variants take device pointers. */
+int jfem_solve(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs,
+               const double *b_host, double *x_host, double tol, int max_refine,
+               int32_t *iters, double *relres);
+int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs,
+                   const double *b_dev, double *x_dev, double tol, int max_refine,
+                   int32_t *iters, double *relres);
+
+/* device buffers for the _dev entry points (owned by the caller) */
+int jfem_dev_alloc(jfem_ctx *ctx, int64_t bytes, void **dev_ptr);
+int jfem_dev_free(jfem_ctx *ctx, void *dev_ptr);
+int jfem_memcpy_h2d(jfem_ctx *ctx, void *dev_dst, const void *host_src, int64_t bytes);
+int jfem_memcpy_d2h(jfem_ctx *ctx, void *host_dst, const void *dev_src, int64_t bytes);
+
+/* memory held by a handle (device bytes) */
+int64_t jfem_factor_device_bytes(const jfem_factor *F);
+int64_t jfem_matrix_device_bytes(const jfem_matrix *K);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,545 @@
+// C ABI: contexts, meshes, assembly pattern, element matrices, assembly and the
+// boundary-condition partition.  Factor/solve entry points live in factor.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "jfem_internal.h"
+
+namespace jfem {
+
+int fail(jfem_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+void timer_start(jfem_ctx *ctx) {
+    if (ctx->timing) (void)hipEventRecord(ctx->ev0, ctx->stream);
+}
+
+void timer_stop(jfem_ctx *ctx, int family, int64_t launches) {
+    if (!ctx->timing) return;
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    (void)hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    ctx->ms[family] += ms;
+    ctx->launches[family] += launches;
+}
+
+template <class T>
+static hipError_t upload(T **dst, const T *src, size_t n, hipStream_t s) {
+    *dst = nullptr;
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc((void **)dst, n * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+// symmetric 3x3 eigen-decomposition by cyclic Jacobi (ascending eigenvalues)
+static void eig3(const double A[9], double w[3], double V[9]) {
+    double a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) a[i][j] = A[i + 3 * j];
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = std::fabs(a[0][1]) + std::fabs(a[0][2]) + std::fabs(a[1][2]);
+        double sc = std::fabs(a[0][0]) + std::fabs(a[1][1]) + std::fabs(a[2][2]);
+        if (off <= 1e-300 || off <= 1e-18 * sc) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (a[p][q] == 0.0) continue;
+                double th = (a[q][q] - a[p][p]) / (2 * a[p][q]);
+                double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
+                double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 3; ++k) { double akp = a[k][p], akq = a[k][q]; a[k][p] = c * akp - s * akq; a[k][q] = s * akp + c * akq; }
+                for (int k = 0; k < 3; ++k) { double apk = a[p][k], aqk = a[q][k]; a[p][k] = c * apk - s * aqk; a[q][k] = s * apk + c * aqk; }
+                for (int k = 0; k < 3; ++k) { double vkp = v[k][p], vkq = v[k][q]; v[k][p] = c * vkp - s * vkq; v[k][q] = s * vkp + c * vkq; }
+            }
+    }
+    int idx[3] = {0, 1, 2};
+    for (int i = 0; i < 3; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (a[idx[j]][idx[j]] < a[idx[i]][idx[i]]) std::swap(idx[i], idx[j]);
+    for (int k = 0; k < 3; ++k) {
+        w[k] = a[idx[k]][idx[k]];
+        for (int r = 0; r < 3; ++r) V[r + 3 * k] = v[r][idx[k]];
+    }
+}
+
+}  // namespace jfem
+
+using namespace jfem;
+
+extern "C" {
+
+int jfem_ctx_create(int device, jfem_ctx **out) {
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return JFEM_ERR_HIP;
+    if (device < 0 || device >= n) return JFEM_ERR_BAD_ARG;
+    jfem_ctx *c = new jfem_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipMalloc((void **)&c->d_status, 2 * sizeof(int64_t)) != hipSuccess) {
+        delete c;
+        return JFEM_ERR_HIP;
+    }
+    *out = c;
+    return JFEM_OK;
+}
+
+int jfem_ctx_destroy(jfem_ctx *ctx) {
+    if (!ctx) return JFEM_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    (void)hipFree(ctx->d_status);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return JFEM_OK;
+}
+
+int jfem_ctx_synchronize(jfem_ctx *ctx) {
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return JFEM_OK;
+}
+
+const char *jfem_last_error(jfem_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int jfem_device_memory(jfem_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes) {
+    size_t f = 0, t = 0;
+    JFEM_CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    JFEM_CHECK_HIP(ctx, hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+    return JFEM_OK;
+}
+
+int jfem_ctx_kernel_time(jfem_ctx *ctx, int family, double *ms, int64_t *launches) {
+    if (family < 0 || family >= FAM_COUNT) return JFEM_ERR_BAD_ARG;
+    *ms = ctx->ms[family];
+    *launches = ctx->launches[family];
+    return JFEM_OK;
+}
+
+int jfem_ctx_reset_kernel_time(jfem_ctx *ctx) {
+    for (int i = 0; i < FAM_COUNT; ++i) { ctx->ms[i] = 0; ctx->launches[i] = 0; }
+    return JFEM_OK;
+}
+
+int jfem_ctx_enable_timing(jfem_ctx *ctx, int on) {
+    ctx->timing = on != 0;
+    return JFEM_OK;
+}
+
+int jfem_dev_alloc(jfem_ctx *ctx, int64_t bytes, void **p) {
+    JFEM_CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    JFEM_CHECK_HIP(ctx, hipMalloc(p, (size_t)std::max<int64_t>(bytes, 1)));
+    return JFEM_OK;
+}
+int jfem_dev_free(jfem_ctx *ctx, void *p) {
+    JFEM_CHECK_HIP(ctx, hipFree(p));
+    return JFEM_OK;
+}
+int jfem_memcpy_h2d(jfem_ctx *ctx, void *d, const void *h, int64_t bytes) {
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(d, h, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return JFEM_OK;
+}
+int jfem_memcpy_d2h(jfem_ctx *ctx, void *h, const void *d, int64_t bytes) {
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(h, d, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return JFEM_OK;
+}
+
+// ---------------------------------------------------------------------------- mesh
+int jfem_mesh_create(jfem_ctx *ctx, int64_t n_nodes, const double *x, const double *y, const double *z,
+                     int64_t nq, const int32_t *qc, const double *qp, int64_t nt, const int32_t *tc,
+                     const double *tp, int64_t ntet, const int32_t *ec, const double *ep, double k6rot,
+                     jfem_mesh **out) {
+    *out = nullptr;
+    if (n_nodes <= 0 || nq < 0 || nt < 0 || ntet < 0) return fail(ctx, JFEM_ERR_BAD_ARG, "bad mesh sizes");
+    if (n_nodes > (int64_t)INT32_MAX / 6) return fail(ctx, JFEM_ERR_BAD_ARG, "too many grid points");
+    auto check_conn = [&](int64_t n, int nn, const int32_t *c) {
+        for (int64_t i = 0; i < n * nn; ++i)
+            if (c[i] < 0 || c[i] >= n_nodes) return false;
+        return true;
+    };
+    if ((nq && !check_conn(nq, 4, qc)) || (nt && !check_conn(nt, 3, tc)) || (ntet && !check_conn(ntet, 10, ec)))
+        return fail(ctx, JFEM_ERR_BAD_ARG, "connectivity references a grid point out of range");
+    JFEM_CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    jfem_mesh *m = new jfem_mesh();
+    m->device = ctx->device;
+    m->n_nodes = n_nodes; m->nq = nq; m->nt = nt; m->ntet = ntet; m->k6rot = k6rot;
+    m->xyz.resize(3 * n_nodes);
+    for (int64_t i = 0; i < n_nodes; ++i) { m->xyz[3 * i] = x[i]; m->xyz[3 * i + 1] = y[i]; m->xyz[3 * i + 2] = z[i]; }
+    m->q.assign(qc, qc + 4 * nq);
+    m->t.assign(tc, tc + 3 * nt);
+    m->tet.assign(ec, ec + 10 * ntet);
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = upload(&m->d_x, x, n_nodes, s);
+    if (e == hipSuccess) e = upload(&m->d_y, y, n_nodes, s);
+    if (e == hipSuccess) e = upload(&m->d_z, z, n_nodes, s);
+    if (e == hipSuccess) e = upload(&m->d_q, qc, 4 * nq, s);
+    if (e == hipSuccess) e = upload(&m->d_qp, qp, 8 * nq, s);
+    if (e == hipSuccess) e = upload(&m->d_t, tc, 3 * nt, s);
+    if (e == hipSuccess) e = upload(&m->d_tp, tp, 8 * nt, s);
+    if (e == hipSuccess) e = upload(&m->d_tet, ec, 10 * ntet, s);
+    if (e == hipSuccess) e = upload(&m->d_tetp, ep, 2 * ntet, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        jfem_mesh_free(m);
+        return fail(ctx, e == hipErrorOutOfMemory ? JFEM_ERR_OOM : JFEM_ERR_HIP, hipGetErrorString(e));
+    }
+    *out = m;
+    return JFEM_OK;
+}
+
+int jfem_mesh_update_props(jfem_ctx *ctx, jfem_mesh *m, const double *qp, const double *tp, const double *ep) {
+    hipStream_t s = ctx->stream;
+    if (qp && m->nq) JFEM_CHECK_HIP(ctx, hipMemcpyAsync(m->d_qp, qp, 8 * m->nq * sizeof(double), hipMemcpyHostToDevice, s));
+    if (tp && m->nt) JFEM_CHECK_HIP(ctx, hipMemcpyAsync(m->d_tp, tp, 8 * m->nt * sizeof(double), hipMemcpyHostToDevice, s));
+    if (ep && m->ntet) JFEM_CHECK_HIP(ctx, hipMemcpyAsync(m->d_tetp, ep, 2 * m->ntet * sizeof(double), hipMemcpyHostToDevice, s));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    return JFEM_OK;
+}
+
+int jfem_mesh_free(jfem_mesh *m) {
+    if (!m) return JFEM_OK;
+    (void)hipSetDevice(m->device);
+    for (void *p : {(void *)m->d_x, (void *)m->d_y, (void *)m->d_z, (void *)m->d_q, (void *)m->d_t, (void *)m->d_tet,
+                    (void *)m->d_qp, (void *)m->d_tp, (void *)m->d_tetp})
+        if (p) (void)hipFree(p);
+    delete m;
+    return JFEM_OK;
+}
+
+// ------------------------------------------------------------------- element output
+static int run_elements(jfem_ctx *ctx, const jfem_mesh *m, double *d_Ke, int64_t *bad, int kinds_mask) {
+    hipStream_t s = ctx->stream;
+    JFEM_CHECK_HIP(ctx, launch_reset_status(s, ctx->d_status));
+    timer_start(ctx);
+    int64_t off_t = m->nq * 576, off_e = off_t + m->nt * 324;
+    int64_t n = 0;
+    if (kinds_mask & 1) { JFEM_CHECK_HIP(ctx, launch_q4_global(s, m, d_Ke, ctx->d_status)); ++n; }
+    if (kinds_mask & 2) { JFEM_CHECK_HIP(ctx, launch_t3_global(s, m, d_Ke + ((kinds_mask & 1) ? off_t : 0), ctx->d_status)); ++n; }
+    if (kinds_mask & 4) { JFEM_CHECK_HIP(ctx, launch_tet10(s, m, d_Ke + ((kinds_mask & 1) ? off_e : 0), ctx->d_status)); ++n; }
+    timer_stop(ctx, FAM_ELEM, n);
+    int64_t st[2];
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, s));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    if (st[1] != 0) {
+        if (bad) *bad = st[0];
+        return fail(ctx, (int)st[1], "element " + std::to_string(st[0]) + " rejected (status " + std::to_string(st[1]) + ")");
+    }
+    if (bad) *bad = -1;
+    return JFEM_OK;
+}
+
+int jfem_element_matrices(jfem_ctx *ctx, const jfem_mesh *m, int kind, double *Ke_out, int64_t *bad) {
+    int64_t n = kind == JFEM_KIND_CQUAD4 ? m->nq : (kind == JFEM_KIND_CTRIA3 ? m->nt : m->ntet);
+    int64_t per = kind == JFEM_KIND_CQUAD4 ? 576 : (kind == JFEM_KIND_CTRIA3 ? 324 : 900);
+    if (kind < 0 || kind > 2) return fail(ctx, JFEM_ERR_BAD_ARG, "bad kind");
+    if (bad) *bad = -1;
+    if (n == 0) return JFEM_OK;
+    double *d = nullptr;
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d, n * per * sizeof(double)));
+    int st = run_elements(ctx, m, d, bad, 1 << kind);
+    if (st == JFEM_OK) {
+        hipError_t e = hipMemcpyAsync(Ke_out, d, n * per * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = fail(ctx, JFEM_ERR_HIP, hipGetErrorString(e));
+    }
+    (void)hipFree(d);
+    return st;
+}
+
+int jfem_q4_local_batch(jfem_ctx *ctx, int64_t n, const double *lc, const double *Cm, const double *Cb,
+                        const double *Cs, const double *h, const double *k6, const double *ds,
+                        const int32_t *flags, double *Ke_out, int64_t *bad) {
+    if (bad) *bad = -1;
+    if (n <= 0) return JFEM_OK;
+    hipStream_t s = ctx->stream;
+    double *d_lc, *d_Cm, *d_Cb, *d_Cs, *d_h, *d_k6, *d_ds, *d_Ke;
+    int32_t *d_f;
+    JFEM_CHECK_HIP(ctx, upload(&d_lc, lc, 8 * n, s));
+    JFEM_CHECK_HIP(ctx, upload(&d_Cm, Cm, 9 * n, s));
+    JFEM_CHECK_HIP(ctx, upload(&d_Cb, Cb, 9 * n, s));
+    JFEM_CHECK_HIP(ctx, upload(&d_Cs, Cs, 4 * n, s));
+    JFEM_CHECK_HIP(ctx, upload(&d_h, h, n, s));
+    JFEM_CHECK_HIP(ctx, upload(&d_k6, k6, n, s));
+    JFEM_CHECK_HIP(ctx, upload(&d_ds, ds, n, s));
+    JFEM_CHECK_HIP(ctx, upload(&d_f, flags, n, s));
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_Ke, 576 * n * sizeof(double)));
+    JFEM_CHECK_HIP(ctx, launch_reset_status(s, ctx->d_status));
+    JFEM_CHECK_HIP(ctx, launch_q4_local(s, n, d_lc, d_Cm, d_Cb, d_Cs, d_h, d_k6, d_ds, d_f, d_Ke, ctx->d_status));
+    int64_t st[2];
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, s));
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(Ke_out, d_Ke, 576 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    for (void *p : {(void *)d_lc, (void *)d_Cm, (void *)d_Cb, (void *)d_Cs, (void *)d_h, (void *)d_k6, (void *)d_ds,
+                    (void *)d_f, (void *)d_Ke})
+        (void)hipFree(p);
+    if (st[1] != 0) {
+        if (bad) *bad = st[0];
+        return fail(ctx, (int)st[1], "element " + std::to_string(st[0]) + " rejected");
+    }
+    return JFEM_OK;
+}
+
+// ------------------------------------------------------------------------- pattern
+int jfem_pattern_create(jfem_ctx *ctx, const jfem_mesh *m, jfem_pattern **out, int64_t *nnz_out) {
+    *out = nullptr;
+    const int64_t nn = m->n_nodes, ndof = 6 * nn;
+    const int64_t ntrip = m->nq * 576 + m->nt * 324 + m->ntet * 900;
+    if (ntrip >= (int64_t)UINT32_MAX) return fail(ctx, JFEM_ERR_BAD_ARG, "more than 2^32 element entries");
+    // node pairs with coupling type: 2 = all six DOFs (shell), 1 = translations only (solid)
+    std::vector<uint64_t> key;
+    key.reserve((size_t)(m->nq * 16 + m->nt * 9 + m->ntet * 100));
+    auto add_elem = [&](const std::vector<int32_t> &conn, int64_t n, int k, int64_t e, int type) {
+        for (int a = 0; a < k; ++a)
+            for (int b = 0; b < k; ++b) {
+                uint64_t na = (uint64_t)conn[a * n + e], nb = (uint64_t)conn[b * n + e];
+                key.push_back((na << 33) | (nb << 2) | (uint64_t)type);
+            }
+    };
+    for (int64_t e = 0; e < m->nq; ++e) add_elem(m->q, m->nq, 4, e, 2);
+    for (int64_t e = 0; e < m->nt; ++e) add_elem(m->t, m->nt, 3, e, 2);
+    for (int64_t e = 0; e < m->ntet; ++e) add_elem(m->tet, m->ntet, 10, e, 1);
+    std::sort(key.begin(), key.end());
+    // keep max type per (a,b)
+    std::vector<int64_t> nptr(nn + 1, 0);
+    std::vector<int32_t> nadj;
+    std::vector<uint8_t> ntype;
+    nadj.reserve(key.size() / 4);
+    for (size_t i = 0; i < key.size(); ++i) {
+        uint64_t a = key[i] >> 33, b = (key[i] >> 2) & 0x7FFFFFFFull;
+        uint8_t t = (uint8_t)(key[i] & 3);
+        if (i + 1 < key.size() && (key[i + 1] >> 2) == (key[i] >> 2)) continue;  // later entry has larger type
+        nadj.push_back((int32_t)b);
+        ntype.push_back(t);
+        nptr[a + 1]++;
+    }
+    key.clear();
+    key.shrink_to_fit();
+    for (int64_t a = 0; a < nn; ++a) nptr[a + 1] += nptr[a];
+    jfem_pattern *P = new jfem_pattern();
+    P->device = ctx->device;
+    P->ndof = ndof; P->n_nodes = nn; P->ntrip = ntrip; P->nq = m->nq; P->nt = m->nt; P->ntet = m->ntet;
+    P->rowptr.assign(ndof + 1, 0);
+    for (int64_t a = 0; a < nn; ++a)
+        for (int i = 0; i < 6; ++i) {
+            int64_t cnt = 0;
+            for (int64_t p = nptr[a]; p < nptr[a + 1]; ++p) cnt += ntype[p] == 2 ? 6 : (i < 3 ? 3 : 0);
+            P->rowptr[6 * a + i + 1] = cnt;
+        }
+    for (int64_t r = 0; r < ndof; ++r) P->rowptr[r + 1] += P->rowptr[r];
+    P->nnz = P->rowptr[ndof];
+    if (P->nnz >= (int64_t)INT32_MAX) { delete P; return fail(ctx, JFEM_ERR_BAD_ARG, "nnz exceeds 2^31"); }
+    P->col.resize(P->nnz);
+    for (int64_t a = 0; a < nn; ++a)
+        for (int i = 0; i < 6; ++i) {
+            int64_t w = P->rowptr[6 * a + i];
+            for (int64_t p = nptr[a]; p < nptr[a + 1]; ++p) {
+                int jm = ntype[p] == 2 ? 6 : (i < 3 ? 3 : 0);
+                for (int j = 0; j < jm; ++j) P->col[w++] = 6 * nadj[p] + j;
+            }
+        }
+    auto find = [&](int64_t r, int64_t c) -> int64_t {
+        const int32_t *b = P->col.data() + P->rowptr[r], *e = P->col.data() + P->rowptr[r + 1];
+        const int32_t *it = std::lower_bound(b, e, (int32_t)c);
+        return (it != e && *it == c) ? (int64_t)(it - P->col.data()) : -1;
+    };
+    // slot of every triplet, in triplet order
+    std::vector<int32_t> slot(ntrip);
+    int64_t t = 0;
+    auto emit = [&](const std::vector<int32_t> &conn, int64_t n, int k, int dpn, int64_t e) {
+        int ndl = k * dpn;
+        for (int cc = 0; cc < ndl; ++cc)
+            for (int rr = 0; rr < ndl; ++rr) {
+                int64_t r = 6 * (int64_t)conn[(rr / dpn) * n + e] + rr % dpn;
+                int64_t c = 6 * (int64_t)conn[(cc / dpn) * n + e] + cc % dpn;
+                slot[t++] = (int32_t)find(r, c);
+            }
+    };
+    for (int64_t e = 0; e < m->nq; ++e) emit(m->q, m->nq, 4, 6, e);
+    for (int64_t e = 0; e < m->nt; ++e) emit(m->t, m->nt, 3, 6, e);
+    for (int64_t e = 0; e < m->ntet; ++e) emit(m->tet, m->ntet, 10, 3, e);
+    std::vector<int64_t> seg_ptr(P->nnz + 1, 0);
+    for (int64_t i = 0; i < ntrip; ++i) seg_ptr[slot[i] + 1]++;
+    for (int64_t s = 0; s < P->nnz; ++s) seg_ptr[s + 1] += seg_ptr[s];
+    std::vector<int32_t> seg_src(ntrip);
+    {
+        std::vector<int64_t> fill(seg_ptr.begin(), seg_ptr.end() - 1);
+        for (int64_t i = 0; i < ntrip; ++i) seg_src[fill[slot[i]]++] = (int32_t)(uint32_t)i;
+    }
+    P->nodeblk.assign(nn * 36, -1);
+    P->diagpos.assign(ndof, -1);
+    for (int64_t a = 0; a < nn; ++a)
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) P->nodeblk[36 * a + 6 * i + j] = find(6 * a + i, 6 * a + j);
+    for (int64_t r = 0; r < ndof; ++r) P->diagpos[r] = (int32_t)find(r, r);
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = upload(&P->d_rowptr, P->rowptr.data(), ndof + 1, s);
+    if (e == hipSuccess) e = upload(&P->d_col, P->col.data(), P->nnz, s);
+    if (e == hipSuccess) e = upload(&P->d_seg_ptr, seg_ptr.data(), P->nnz + 1, s);
+    if (e == hipSuccess) e = upload(&P->d_seg_src, seg_src.data(), ntrip, s);
+    if (e == hipSuccess) e = upload(&P->d_nodeblk, P->nodeblk.data(), nn * 36, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        jfem_pattern_free(P);
+        return fail(ctx, e == hipErrorOutOfMemory ? JFEM_ERR_OOM : JFEM_ERR_HIP, hipGetErrorString(e));
+    }
+    if (nnz_out) *nnz_out = P->nnz;
+    *out = P;
+    return JFEM_OK;
+}
+
+int jfem_pattern_free(jfem_pattern *P) {
+    if (!P) return JFEM_OK;
+    (void)hipSetDevice(P->device);
+    for (void *p : {(void *)P->d_rowptr, (void *)P->d_col, (void *)P->d_seg_ptr, (void *)P->d_seg_src, (void *)P->d_nodeblk})
+        if (p) (void)hipFree(p);
+    delete P;
+    return JFEM_OK;
+}
+
+// ------------------------------------------------------------------------ assembly
+int jfem_reassemble(jfem_ctx *ctx, const jfem_mesh *m, jfem_matrix *K, int64_t *bad) {
+    const jfem_pattern *P = K->P;
+    if (P->nq != m->nq || P->nt != m->nt || P->ntet != m->ntet) return fail(ctx, JFEM_ERR_BAD_ARG, "mesh/pattern mismatch");
+    int st = run_elements(ctx, m, K->d_Ke, bad, 7);
+    if (st != JFEM_OK) return st;
+    timer_start(ctx);
+    JFEM_CHECK_HIP(ctx, launch_assemble(ctx->stream, P, K->d_Ke, K->d_val));
+    timer_stop(ctx, FAM_ASSEMBLY, 1);
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return JFEM_OK;
+}
+
+int jfem_assemble(jfem_ctx *ctx, const jfem_mesh *m, const jfem_pattern *P, jfem_matrix **out, int64_t *bad) {
+    *out = nullptr;
+    jfem_matrix *K = new jfem_matrix();
+    K->device = ctx->device;
+    K->P = P;
+    hipError_t e = hipMalloc((void **)&K->d_val, std::max<int64_t>(P->nnz, 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&K->d_Ke, std::max<int64_t>(P->ntrip, 1) * sizeof(double));
+    if (e != hipSuccess) {
+        jfem_matrix_free(K);
+        return fail(ctx, JFEM_ERR_OOM, "assembly buffers");
+    }
+    int st = jfem_reassemble(ctx, m, K, bad);
+    if (st != JFEM_OK) {
+        jfem_matrix_free(K);
+        return st;
+    }
+    *out = K;
+    return JFEM_OK;
+}
+
+int jfem_matrix_download(jfem_ctx *ctx, const jfem_matrix *K, int64_t *row_ptr, int32_t *col, double *val) {
+    const jfem_pattern *P = K->P;
+    if (row_ptr) std::memcpy(row_ptr, P->rowptr.data(), (P->ndof + 1) * sizeof(int64_t));
+    if (col) std::memcpy(col, P->col.data(), P->nnz * sizeof(int32_t));
+    if (val) {
+        JFEM_CHECK_HIP(ctx, hipMemcpyAsync(val, K->d_val, P->nnz * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return JFEM_OK;
+}
+
+int jfem_matrix_free(jfem_matrix *K) {
+    if (!K) return JFEM_OK;
+    (void)hipSetDevice(K->device);
+    if (K->d_val) (void)hipFree(K->d_val);
+    if (K->d_Ke) (void)hipFree(K->d_Ke);
+    delete K;
+    return JFEM_OK;
+}
+
+int64_t jfem_matrix_device_bytes(const jfem_matrix *K) {
+    return K ? (K->P->nnz + K->P->ntrip) * (int64_t)sizeof(double) : 0;
+}
+
+// ------------------------------------------------------------------ BC partition
+// src/solver/boundary_conditions.jl:1480-1607 (SPC), 1028-1138 (AUTOSPC with grid
+// point processing on: only exact-zero / negative diagonals), 745-845 (GPST).
+int jfem_bc_partition(jfem_ctx *ctx, const jfem_matrix *K, const uint8_t *spc, double eps_zero, uint8_t *fixed,
+                      int64_t *n_free) {
+    const jfem_pattern *P = K->P;
+    const int64_t nn = P->n_nodes, ndof = P->ndof;
+    std::vector<double> blk(36 * nn);
+    double *d_blk = nullptr;
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_blk, std::max<int64_t>(36 * nn, 1) * sizeof(double)));
+    JFEM_CHECK_HIP(ctx, launch_gather_nodeblk(ctx->stream, P, K->d_val, d_blk));
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(blk.data(), d_blk, 36 * nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(d_blk);
+    for (int64_t i = 0; i < ndof; ++i) fixed[i] = spc ? (spc[i] ? 1 : 0) : 0;
+    auto kv = [&](int64_t node, int a, int b) { return blk[36 * node + 6 * a + b]; };
+    for (int64_t i = 0; i < ndof; ++i) {
+        double d = kv(i / 6, (int)(i % 6), (int)(i % 6));
+        if (!fixed[i] && (d == 0.0 || d < 0)) fixed[i] = 1;
+    }
+    for (int64_t n = 0; n < nn; ++n) {
+        for (int off = 0; off <= 3; off += 3) {
+            int fl[3], m = 0;
+            for (int a = 0; a < 3; ++a)
+                if (!fixed[6 * n + off + a]) fl[m++] = a;
+            if (m < 1) continue;
+            double S[9] = {0}, vals[3] = {0}, vecs[9] = {0};
+            for (int ii = 0; ii < m; ++ii)
+                for (int jj = 0; jj < m; ++jj)
+                    S[ii + 3 * jj] = (kv(n, off + fl[ii], off + fl[jj]) + kv(n, off + fl[jj], off + fl[ii])) / 2;
+            if (m == 3) {
+                eig3(S, vals, vecs);
+            } else if (m == 2) {
+                double a = S[0], b = S[3], d = S[4];
+                double tr = 0.5 * (a + d), df = 0.5 * (a - d), r = std::sqrt(df * df + b * b);
+                vals[0] = tr - r; vals[1] = tr + r;
+                for (int k = 0; k < 2; ++k) {
+                    double vx, vy;
+                    if (b != 0.0) { vx = b; vy = vals[k] - a; }
+                    else { vx = (k == 0) == (a <= d) ? 1.0 : 0.0; vy = 1.0 - vx; }
+                    double nr = std::hypot(vx, vy);
+                    vecs[0 + 3 * k] = vx / nr; vecs[1 + 3 * k] = vy / nr;
+                }
+            } else {
+                vals[0] = S[0]; vecs[0] = 1.0;
+            }
+            double lmax = 0;
+            for (int k = 0; k < m; ++k) lmax = std::max(lmax, std::fabs(vals[k]));
+            if (!(lmax > 0.0)) continue;
+            int claimed[3] = {0, 0, 0};
+            for (int k = 0; k < m; ++k) {
+                double ratio = std::fabs(vals[k]) / lmax;
+                if (!(ratio < eps_zero)) continue;
+                int best_a = -1;
+                double best_w = -1.0;
+                for (int ii = 0; ii < m; ++ii) {
+                    double wv = std::fabs(vecs[ii + 3 * k]);
+                    int a = fl[ii];
+                    if (claimed[a]) continue;
+                    if (wv > best_w + 1e-12) { best_w = wv; best_a = a; }
+                }
+                if (best_a < 0) continue;
+                int64_t g = 6 * n + off + best_a;
+                if (fixed[g]) continue;
+                claimed[best_a] = 1;
+                fixed[g] = 1;
+            }
+        }
+    }
+    int64_t nf = 0;
+    for (int64_t i = 0; i < ndof; ++i) nf += fixed[i] ? 0 : 1;
+    if (n_free) *n_free = nf;
+    return JFEM_OK;
+}
+
+}  // extern "C"

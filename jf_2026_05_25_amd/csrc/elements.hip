@@ -1,0 +1,1188 @@
+// Batched element stiffness generation for gfx950 and the deterministic
+// segmented-reduction assembly.
+//
+// One 64-lane wavefront owns one element.  Every lane owns one 3x3 block of the
+// element matrix (CQUAD4: 8x8 blocks of a 24x24; CTRIA3: 6x6 of 18x18; CTETRA10:
+// 10x10 of 30x30, two blocks per lane).  Gauss-point derivative rows, internal-mode
+// couplings, the MacNeal plate block and the drilling rows are built once per
+// element into LDS by a few lanes, then every lane forms its block, applies the
+// frame rotation R' * blk * R and stages the global block in LDS so the wavefront
+// writes the whole element matrix with coalesced stores.
+//
+// The per-entry arithmetic follows the statement order of the reference kernels
+// (ts_mul! / ts_mul_At_add!, src/FEMKernels.jl:649-675); citations per step below.
+#include "jfem_internal.h"
+
+#include <cfloat>
+#include <climits>
+
+namespace {
+
+constexpr double kEps = 2.220446049250313e-16;
+constexpr int kWave = 64;
+constexpr int kElemsPerBlock = 2;  // wavefronts (= elements) per workgroup
+
+__device__ inline void report(int64_t *status, int64_t elem, int code) {
+    // first failing element wins (smallest index); code stored by that element
+    unsigned long long old = atomicMin((unsigned long long *)&status[0], (unsigned long long)elem);
+    if ((long long)old > elem) atomicExch((unsigned long long *)&status[1], (unsigned long long)code);
+}
+
+__device__ inline void shape_derivs(double r, double s, double *dNr, double *dNs) {
+    dNr[0] = -0.25 * (1 - s); dNr[1] = 0.25 * (1 - s); dNr[2] = 0.25 * (1 + s); dNr[3] = -0.25 * (1 + s);
+    dNs[0] = -0.25 * (1 - r); dNs[1] = -0.25 * (1 + r); dNs[2] = 0.25 * (1 + r); dNs[3] = 0.25 * (1 - r);
+}
+
+__device__ inline void jac4(const double *dNr, const double *dNs, const double *lc, double &J11, double &J12,
+                            double &J21, double &J22) {
+    J11 = dNr[0] * lc[0] + dNr[1] * lc[2] + dNr[2] * lc[4] + dNr[3] * lc[6];
+    J12 = dNr[0] * lc[1] + dNr[1] * lc[3] + dNr[2] * lc[5] + dNr[3] * lc[7];
+    J21 = dNs[0] * lc[0] + dNs[1] * lc[2] + dNs[2] * lc[4] + dNs[3] * lc[6];
+    J22 = dNs[0] * lc[1] + dNs[1] * lc[3] + dNs[2] * lc[5] + dNs[3] * lc[7];
+}
+
+// _quad4_jacobian_measure, src/FEMKernels.jl:2862-2867
+__device__ inline bool jac_ok(double detj, double j11, double j12, double j21, double j22) {
+    double scale = fabs(j11 * j22) + fabs(j12 * j21);
+    return isfinite(detj) && isfinite(scale) && fabs(detj) > 64 * kEps * scale;
+}
+
+// ---- small dense helpers (single lane) ----
+__device__ bool lu_solve_dev(const double *Ain, int n, double *B, int nrhs) {
+    double A[16];
+    int piv[4];
+    for (int q = 0; q < n * n; ++q) A[q] = Ain[q];
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double mx = fabs(A[k + n * k]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(A[i + n * k]) > mx) { mx = fabs(A[i + n * k]); p = i; }
+        piv[k] = p;
+        if (mx == 0.0) return false;
+        if (p != k)
+            for (int j = 0; j < n; ++j) { double t = A[k + n * j]; A[k + n * j] = A[p + n * j]; A[p + n * j] = t; }
+        for (int i = k + 1; i < n; ++i) {
+            A[i + n * k] /= A[k + n * k];
+            for (int j = k + 1; j < n; ++j) A[i + n * j] -= A[i + n * k] * A[k + n * j];
+        }
+    }
+    for (int c = 0; c < nrhs; ++c) {
+        double *b = B + n * c;
+        for (int k = 0; k < n; ++k)
+            if (piv[k] != k) { double t = b[k]; b[k] = b[piv[k]]; b[piv[k]] = t; }
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) b[i] -= A[i + n * j] * b[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) b[i] -= A[i + n * j] * b[j];
+            b[i] /= A[i + n * i];
+        }
+    }
+    return true;
+}
+
+__device__ bool inv4_dev(const double *a, double *out) {
+    double inv[16];
+    inv[0] = a[5]*a[10]*a[15] - a[5]*a[11]*a[14] - a[9]*a[6]*a[15] + a[9]*a[7]*a[14] + a[13]*a[6]*a[11] - a[13]*a[7]*a[10];
+    inv[4] = -a[4]*a[10]*a[15] + a[4]*a[11]*a[14] + a[8]*a[6]*a[15] - a[8]*a[7]*a[14] - a[12]*a[6]*a[11] + a[12]*a[7]*a[10];
+    inv[8] = a[4]*a[9]*a[15] - a[4]*a[11]*a[13] - a[8]*a[5]*a[15] + a[8]*a[7]*a[13] + a[12]*a[5]*a[11] - a[12]*a[7]*a[9];
+    inv[12] = -a[4]*a[9]*a[14] + a[4]*a[10]*a[13] + a[8]*a[5]*a[14] - a[8]*a[6]*a[13] - a[12]*a[5]*a[10] + a[12]*a[6]*a[9];
+    inv[1] = -a[1]*a[10]*a[15] + a[1]*a[11]*a[14] + a[9]*a[2]*a[15] - a[9]*a[3]*a[14] - a[13]*a[2]*a[11] + a[13]*a[3]*a[10];
+    inv[5] = a[0]*a[10]*a[15] - a[0]*a[11]*a[14] - a[8]*a[2]*a[15] + a[8]*a[3]*a[14] + a[12]*a[2]*a[11] - a[12]*a[3]*a[10];
+    inv[9] = -a[0]*a[9]*a[15] + a[0]*a[11]*a[13] + a[8]*a[1]*a[15] - a[8]*a[3]*a[13] - a[12]*a[1]*a[11] + a[12]*a[3]*a[9];
+    inv[13] = a[0]*a[9]*a[14] - a[0]*a[10]*a[13] - a[8]*a[1]*a[14] + a[8]*a[2]*a[13] + a[12]*a[1]*a[10] - a[12]*a[2]*a[9];
+    inv[2] = a[1]*a[6]*a[15] - a[1]*a[7]*a[14] - a[5]*a[2]*a[15] + a[5]*a[3]*a[14] + a[13]*a[2]*a[7] - a[13]*a[3]*a[6];
+    inv[6] = -a[0]*a[6]*a[15] + a[0]*a[7]*a[14] + a[4]*a[2]*a[15] - a[4]*a[3]*a[14] - a[12]*a[2]*a[7] + a[12]*a[3]*a[6];
+    inv[10] = a[0]*a[5]*a[15] - a[0]*a[7]*a[13] - a[4]*a[1]*a[15] + a[4]*a[3]*a[13] + a[12]*a[1]*a[7] - a[12]*a[3]*a[5];
+    inv[14] = -a[0]*a[5]*a[14] + a[0]*a[6]*a[13] + a[4]*a[1]*a[14] - a[4]*a[2]*a[13] - a[12]*a[1]*a[6] + a[12]*a[2]*a[5];
+    inv[3] = -a[1]*a[6]*a[11] + a[1]*a[7]*a[10] + a[5]*a[2]*a[11] - a[5]*a[3]*a[10] - a[9]*a[2]*a[7] + a[9]*a[3]*a[6];
+    inv[7] = a[0]*a[6]*a[11] - a[0]*a[7]*a[10] - a[4]*a[2]*a[11] + a[4]*a[3]*a[10] + a[8]*a[2]*a[7] - a[8]*a[3]*a[6];
+    inv[11] = -a[0]*a[5]*a[11] + a[0]*a[7]*a[9] + a[4]*a[1]*a[11] - a[4]*a[3]*a[9] - a[8]*a[1]*a[7] + a[8]*a[3]*a[5];
+    inv[15] = a[0]*a[5]*a[10] - a[0]*a[6]*a[9] - a[4]*a[1]*a[10] + a[4]*a[2]*a[9] + a[8]*a[1]*a[6] - a[8]*a[2]*a[5];
+    double det = a[0] * inv[0] + a[1] * inv[4] + a[2] * inv[8] + a[3] * inv[12];
+    if (det == 0.0 || !isfinite(det)) return false;
+    for (int i = 0; i < 16; ++i) out[i] = inv[i] / det;
+    return true;
+}
+
+// inverse of K[active, active] scattered into a 4x4 (binary internal-mode weights,
+// q4_internal_mode_compliance, src/FEMKernels.jl:1918-1955)
+__device__ bool active_inverse_dev(const double *K, int mask, double *C) {
+    int act[4], na = 0;
+    for (int i = 0; i < 4; ++i)
+        if (mask & (1 << i)) act[na++] = i;
+    for (int q = 0; q < 16; ++q) C[q] = 0.0;
+    if (na == 0) return true;
+    if (na == 4) return inv4_dev(K, C);
+    double sub[16], inv[16];
+    for (int j = 0; j < na; ++j)
+        for (int i = 0; i < na; ++i) {
+            sub[i + na * j] = K[act[i] + 4 * act[j]];
+            inv[i + na * j] = (i == j) ? 1.0 : 0.0;
+        }
+    if (!lu_solve_dev(sub, na, inv, na)) return false;
+    for (int j = 0; j < na; ++j)
+        for (int i = 0; i < na; ++i) C[act[i] + 4 * act[j]] = inv[i + na * j];
+    return true;
+}
+
+__device__ bool inv3_dev(const double *a, double *o) {
+    double c00 = a[4] * a[8] - a[7] * a[5];
+    double c01 = a[7] * a[2] - a[1] * a[8];
+    double c02 = a[1] * a[5] - a[4] * a[2];
+    double det = a[0] * c00 + a[3] * c01 + a[6] * c02;
+    if (det == 0.0 || !isfinite(det)) return false;
+    o[0] = c00 / det; o[3] = (a[6] * a[5] - a[3] * a[8]) / det; o[6] = (a[3] * a[7] - a[6] * a[4]) / det;
+    o[1] = c01 / det; o[4] = (a[0] * a[8] - a[6] * a[2]) / det; o[7] = (a[6] * a[1] - a[0] * a[7]) / det;
+    o[2] = c02 / det; o[5] = (a[3] * a[2] - a[0] * a[5]) / det; o[8] = (a[0] * a[4] - a[3] * a[1]) / det;
+    return true;
+}
+
+__device__ inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ inline void cross3(const double *a, const double *b, double *o) {
+    o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ inline void normalize3(double *a) {
+    double n = sqrt(dot3(a, a));
+    a[0] /= n; a[1] /= n; a[2] /= n;
+}
+
+// isotropic PSHELL section (src/solver/assembly.jl:3380-3404, 2085-2129, 2174-2233)
+__device__ void shell_section(const double *prop, double *Cm, double *Cb, double *Cs, double &G, bool &mo) {
+    double h = prop[JFEM_SP_H], E = prop[JFEM_SP_E], nu = prop[JFEM_SP_NU];
+    double br = prop[JFEM_SP_BEND_RATIO], tst = prop[JFEM_SP_TS_T];
+    G = prop[JFEM_SP_G];
+    if (!(G > 0.0)) G = E > 0.0 ? E / (2 * (1 + nu)) : 0.0;
+    double cm = (E * h) / (1 - nu * nu);
+    const double Mi[9] = {1.0, nu, 0.0, nu, 1.0, 0.0, 0.0, 0.0, (1 - nu) / 2};
+    for (int q = 0; q < 9; ++q) Cm[q] = cm * Mi[q];
+    if (br > 1e-12) {
+        double pc = E / fmax(1.0 - nu * nu, 1e-30);
+        double f = ((br * (h * h * h)) / 12.0);
+        for (int q = 0; q < 9; ++q) Cb[q] = (pc * Mi[q]) * f;
+        double ts = tst * h;
+        Cs[0] = ts * G; Cs[1] = 0.0; Cs[2] = 0.0; Cs[3] = ts * G;
+        mo = false;
+    } else {
+        for (int q = 0; q < 9; ++q) Cb[q] = 0.0;
+        for (int q = 0; q < 4; ++q) Cs[q] = 0.0;
+        mo = true;
+    }
+}
+
+// _rotate_constitutive_3x3!, src/solver/assembly.jl:2440-2462
+__device__ void rot3(double *C, double T11, double T12, double T13, double T21, double T22, double T23,
+                     double T31, double T32, double T33) {
+    double c11 = C[0], c21 = C[1], c31 = C[2], c12 = C[3], c22 = C[4], c32 = C[5], c13 = C[6], c23 = C[7], c33 = C[8];
+    double t11 = c11 * T11 + c12 * T21 + c13 * T31, t12 = c11 * T12 + c12 * T22 + c13 * T32, t13 = c11 * T13 + c12 * T23 + c13 * T33;
+    double t21 = c21 * T11 + c22 * T21 + c23 * T31, t22 = c21 * T12 + c22 * T22 + c23 * T32, t23 = c21 * T13 + c22 * T23 + c23 * T33;
+    double t31 = c31 * T11 + c32 * T21 + c33 * T31, t32 = c31 * T12 + c32 * T22 + c33 * T32, t33 = c31 * T13 + c32 * T23 + c33 * T33;
+    C[0] = T11 * t11 + T21 * t21 + T31 * t31; C[3] = T11 * t12 + T21 * t22 + T31 * t32; C[6] = T11 * t13 + T21 * t23 + T31 * t33;
+    C[1] = T12 * t11 + T22 * t21 + T32 * t31; C[4] = T12 * t12 + T22 * t22 + T32 * t32; C[7] = T12 * t13 + T22 * t23 + T32 * t33;
+    C[2] = T13 * t11 + T23 * t21 + T33 * t31; C[5] = T13 * t12 + T23 * t22 + T33 * t32; C[8] = T13 * t13 + T23 * t23 + T33 * t33;
+}
+
+// ------------------------------------------------------------------------------
+// CQUAD4
+// ------------------------------------------------------------------------------
+struct Q4Shared {
+    double lc[8], Cm[9], Cb[9], Cs[4], R[9];
+    double h, k6, ds;
+    int rigid, modes, do_shear, status;
+    double gdx[4][4], gdy[4][4], gw[4];
+    double Bi[4][12], Bib[4][12];      // per Gauss point, 3x4 column-major
+    double Kab[96], Kbb[16], Kabs[96], Kbbs[16], Kabb[96], Kbbb[16];
+    double Cim[16], Cib[16];
+    double Pm[96], Pb[96];
+    double Dm[48], Xm[48];             // MacNeal D (4x12) and Z \ D
+    double Kp[144];
+    double Bd[4][24], dcoef[4];
+    double out[576];
+};
+
+// B-matrix entries as functions of the local DOF d = 6k + c
+__device__ inline double bm_entry(const Q4Shared &S, int g, int l, int d) {
+    int k = d / 6, c = d % 6;
+    double dx = S.gdx[g][k], dy = S.gdy[g][k];
+    if (c == 0) return l == 0 ? dx : (l == 1 ? 0.0 : dy);
+    if (c == 1) return l == 0 ? 0.0 : (l == 1 ? dy : dx);
+    return 0.0;
+}
+__device__ inline double bb_entry(const Q4Shared &S, int g, int l, int d) {
+    int k = d / 6, c = d % 6;
+    double dx = S.gdx[g][k], dy = S.gdy[g][k];
+    if (c == 3) return l == 0 ? 0.0 : (l == 1 ? -dy : -dx);
+    if (c == 4) return l == 0 ? dx : (l == 1 ? 0.0 : dy);
+    return 0.0;
+}
+
+// MacNeal transverse-shear block (one lane): add_quad4_macneal_shear_rbf! default slice,
+// src/FEMKernels.jl:4495-4834, 6188-6200; quad4_natural_shear_flexibility! :4223-4352
+__device__ bool q4_macneal(Q4Shared &S) {
+    const double *lc = S.lc;
+    const double pt = 1.0 / sqrt(3.0);
+    const double sxi[4] = {0.0, 0.0, -pt, pt}, seta[4] = {-pt, pt, 0.0, 0.0};
+    const int comp[4] = {1, 1, 2, 2};
+    double Ce[48], Je[4][4];
+    for (int sp = 0; sp < 4; ++sp) {
+        double xi = sxi[sp], eta = seta[sp];
+        double xr = xi == 0.0 ? 0.0 : copysign(1.0, xi);
+        double er = eta == 0.0 ? 0.0 : copysign(1.0, eta);
+        double dNre[4], dNse[4], J11e, J12e, J21e, J22e;
+        shape_derivs(xr, er, dNre, dNse);
+        jac4(dNre, dNse, lc, J11e, J12e, J21e, J22e);
+        double Ne[4] = {0.25 * (1 - xr) * (1 - er), 0.25 * (1 + xr) * (1 - er), 0.25 * (1 + xr) * (1 + er),
+                        0.25 * (1 - xr) * (1 + er)};
+        Je[sp][0] = J11e; Je[sp][1] = J12e; Je[sp][2] = J21e; Je[sp][3] = J22e;
+        double txe = comp[sp] == 1 ? J11e : J21e, tye = comp[sp] == 1 ? J12e : J22e;
+        for (int k = 0; k < 4; ++k) {
+            Ce[sp + 4 * (3 * k + 0)] = comp[sp] == 1 ? dNre[k] : dNse[k];
+            Ce[sp + 4 * (3 * k + 1)] = -Ne[k] * tye;
+            Ce[sp + 4 * (3 * k + 2)] = Ne[k] * txe;
+        }
+    }
+    double *D = S.Dm;
+    for (int sp = 0; sp < 4; ++sp) {
+        double xi = sxi[sp], eta = seta[sp];
+        double J11 = Je[sp][0], J12 = Je[sp][1], J21 = Je[sp][2], J22 = Je[sp][3];
+        double detJ = J11 * J22 - J12 * J21;
+        double adet = fabs(detJ) < 1e-14 ? (detJ < 0 ? -1e-14 : 1e-14) : detJ;
+        double i11 = J22 / adet, i12 = -J12 / adet, i21 = -J21 / adet, i22 = J11 / adet;
+        double w1 = 0.5 * (1.0 - eta / pt), w2 = 0.5 * (1.0 + eta / pt);
+        double w3 = 0.5 * (1.0 - xi / pt), w4 = 0.5 * (1.0 + xi / pt);
+        for (int j = 0; j < 12; ++j) {
+            double gxie = w1 * Ce[0 + 4 * j] + w2 * Ce[1 + 4 * j];
+            double getae = w3 * Ce[2 + 4 * j] + w4 * Ce[3 + 4 * j];
+            D[sp + 4 * j] = sp < 2 ? (i11 * gxie + i12 * getae) : (i21 * gxie + i22 * getae);
+        }
+    }
+    const double rn[4] = {-1.0, 1.0, 1.0, -1.0}, sn[4] = {-1.0, -1.0, 1.0, 1.0};
+    double tr[16];
+    for (int q = 0; q < 16; ++q) tr[q] = 0.0;
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k) {
+            int base = 3 * k;
+            tr[i + 0] += D[i + 4 * (base + 2)];
+            tr[i + 4] += sn[k] * D[i + 4 * (base + 2)];
+            tr[i + 8] -= D[i + 4 * (base + 1)];
+            tr[i + 12] -= rn[k] * D[i + 4 * (base + 1)];
+        }
+    double x1 = lc[0], x2 = lc[2], x3 = lc[4], x4 = lc[6];
+    double y1 = lc[1], y2 = lc[3], y3 = lc[5], y4 = lc[7];
+    double ax = 0.25 * (-x1 + x2 + x3 - x4), ay = 0.25 * (-y1 + y2 + y3 - y4);
+    double bx = 0.25 * (-x1 - x2 + x3 + x4), by = 0.25 * (-y1 - y2 + y3 + y4);
+    double cx = 0.25 * (x1 - x2 + x3 - x4), cy = 0.25 * (y1 - y2 + y3 - y4);
+    double det0 = ax * by - ay * bx;
+    double det_scale = hypot(ax, ay) * hypot(bx, by);
+    if (det_scale < kEps) det_scale = kEps;
+    if (!(fabs(det0) > 64 * kEps * det_scale)) return false;
+    double orient = det0 > 0 ? 1.0 : (det0 < 0 ? -1.0 : 0.0);
+    const double cr[4] = {-1, 1, 1, -1}, cs_[4] = {-1, -1, 1, 1};
+    for (int q = 0; q < 4; ++q) {
+        double r = cr[q], s = cs_[q];
+        double dj = (ax + cx * s) * (by + cy * r) - (ay + cy * s) * (bx + cx * r);
+        if (!(orient * dj > 64 * kEps * det_scale)) return false;
+    }
+    double area = 4 * fabs(det0);
+    double dx2 = (2 * ax) * (2 * ax), dy2 = (2 * by) * (2 * by);
+    double dxr = S.Cb[0], dyr = S.Cb[4];
+    double fx = fabs(2 * ax) > 1e-30 ? (cx / (2 * ax)) * (cx / (2 * ax)) : 0.0;
+    double fy = fabs(2 * by) > 1e-30 ? (cy / (2 * by)) * (cy / (2 * by)) : 0.0;
+    if (fx + fy > 1e-24 && ((dxr == 0.0) != (dyr == 0.0))) return false;
+    double ux = dx2 / ((12 * area) * fmax(fabs(dxr), 1e-30));
+    double uy = dy2 / ((12 * area) * fmax(fabs(dyr), 1e-30));
+    double denx = dy2 + 39 * dx2, deny = dx2 + 39 * dy2;
+    if (!(denx > 0 && deny > 0)) return false;
+    double zb[4] = {ux, ((3 * dy2) / denx) * ux, uy, ((3 * dx2) / deny) * uy};
+    double Z[16];
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) {
+            double v = 0.0;
+            for (int k = 0; k < 4; ++k) v += (tr[i + 4 * k] * zb[k]) * tr[j + 4 * k];
+            Z[i + 4 * j] = v;
+        }
+    if (!S.rigid) {
+        double gp = 1.0 / sqrt(3.0);
+        auto jac = [&](double r, double s) {
+            return fabs((ax + cx * s) * (by + cy * r) - (ay + cy * s) * (bx + cx * r));
+        };
+        double va = sqrt(2 * jac(0.0, -gp)), vb = sqrt(2 * jac(0.0, gp));
+        double vc = sqrt(2 * jac(-gp, 0.0)), vd = sqrt(2 * jac(gp, 0.0));
+        double pxa[4] = {va, -gp * va, 0, 0}, pxb[4] = {vb, gp * vb, 0, 0};
+        double pyc[4] = {0, 0, vc, -gp * vc}, pyd[4] = {0, 0, vd, gp * vd};
+        double H[16], Hi[16];
+        const double *Cs = S.Cs;
+        for (int j = 0; j < 4; ++j)
+            for (int i = 0; i < 4; ++i) {
+                H[i + 4 * j] = Cs[0] * (pxa[i] * pxa[j] + pxb[i] * pxb[j]) + Cs[3] * (pyc[i] * pyc[j] + pyd[i] * pyd[j]) +
+                               Cs[2] * (pxa[i] + pxb[i]) * (pyc[j] + pyd[j]) / 2 +
+                               Cs[1] * (pyc[i] + pyd[i]) * (pxa[j] + pxb[j]) / 2;
+                Hi[i + 4 * j] = (i == j) ? 1.0 : 0.0;
+            }
+        if (!lu_solve_dev(H, 4, Hi, 4)) return false;
+        double t4[16];
+        for (int q = 0; q < 16; ++q) t4[q] = 0.0;
+        for (int j = 0; j < 4; ++j)
+            for (int l = 0; l < 4; ++l) {
+                double v = Hi[l + 4 * j];
+                for (int i = 0; i < 4; ++i) t4[i + 4 * j] += tr[i + 4 * l] * v;
+            }
+        for (int j = 0; j < 4; ++j)
+            for (int i = 0; i < 4; ++i) {
+                double s = 0;
+                for (int k = 0; k < 4; ++k) s += t4[i + 4 * k] * tr[j + 4 * k];
+                Z[i + 4 * j] = Z[i + 4 * j] + s;  // Z_total = Zb + Zs
+            }
+    }
+    for (int q = 0; q < 48; ++q) S.Xm[q] = D[q];
+    return lu_solve_dev(Z, 4, S.Xm, 12);
+}
+
+__device__ inline int plate_index(int d) {  // local DOF -> MacNeal plate index (w, rx, ry) or -1
+    int k = d / 6, c = d % 6;
+    return (c >= 2 && c <= 4) ? 3 * k + (c - 2) : -1;
+}
+
+// Per-wavefront CQUAD4 kernel body.  p: 4 node xyz (p[3k+c]); prop: section row.
+// global_frame: compute frame + section from p/prop; otherwise use lc/C* given.
+__device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int64_t *status, double *Ke_out) {
+    // Phase 1: Gauss-point rows (lanes 0..3), MacNeal (lane 63), drilling rows (lanes 59..62)
+    const double pt = 1.0 / sqrt(3.0);
+    const double gr[4] = {-pt, pt, pt, -pt}, gs[4] = {-pt, -pt, pt, pt};
+    const double dNr_c[4] = {-0.25, 0.25, 0.25, -0.25}, dNs_c[4] = {-0.25, -0.25, 0.25, 0.25};
+    double J11c, J12c, J21c, J22c;
+    jac4(dNr_c, dNs_c, S.lc, J11c, J12c, J21c, J22c);
+    double detJc = J11c * J22c - J12c * J21c, abs_detJc = fabs(detJc);
+    double inv_detc = 1.0 / detJc;
+    double iJ11c = J22c * inv_detc, iJ12c = -J12c * inv_detc, iJ21c = -J21c * inv_detc, iJ22c = J11c * inv_detc;
+    if (valid && lane < 4) {
+        int g = lane;
+        double r = gr[g], s = gs[g];
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(r, s, dNr, dNs);
+        jac4(dNr, dNs, S.lc, J11, J12, J21, J22);
+        double detJ = J11 * J22 - J12 * J21;
+        if (!jac_ok(detJ, J11, J12, J21, J22)) S.status = JFEM_ERR_DEGENERATE;
+        double inv_det = 1.0 / detJ;
+        double iJ11 = J22 * inv_det, iJ12 = -J12 * inv_det, iJ21 = -J21 * inv_det, iJ22 = J11 * inv_det;
+        for (int k = 0; k < 4; ++k) {
+            S.gdx[g][k] = iJ11 * dNr[k] + iJ12 * dNs[k];
+            S.gdy[g][k] = iJ21 * dNr[k] + iJ22 * dNs[k];
+        }
+        S.gw[g] = fabs(detJ);
+        // membrane internal modes, patch-consistent Jacobian (src/FEMKernels.jl:1847-1896)
+        double ratio = (iJ11 * iJ22 - iJ12 * iJ21) / (iJ11c * iJ22c - iJ12c * iJ21c);
+        double m11 = ratio * iJ11c, m12 = ratio * iJ12c, m21 = ratio * iJ21c, m22 = ratio * iJ22c;
+        double p1x = m11 * (-2.0 * r), p1y = m21 * (-2.0 * r), p2x = m12 * (-2.0 * s), p2y = m22 * (-2.0 * s);
+        double *Bi = S.Bi[g];
+        for (int q = 0; q < 12; ++q) Bi[q] = 0.0;
+        Bi[0 + 3 * 0] = p1x; Bi[2 + 3 * 0] = p1y; Bi[1 + 3 * 1] = p1y; Bi[2 + 3 * 1] = p1x;
+        Bi[0 + 3 * 2] = p2x; Bi[2 + 3 * 2] = p2y; Bi[1 + 3 * 3] = p2y; Bi[2 + 3 * 3] = p2x;
+        // bending internal modes, qm6 Jacobian (src/FEMKernels.jl:3478-3488)
+        double gq = detJc / detJ;
+        double b1x = (iJ11c * (-2.0 * r)) * gq, b1y = (iJ21c * (-2.0 * r)) * gq;
+        double b2x = (iJ12c * (-2.0 * s)) * gq, b2y = (iJ22c * (-2.0 * s)) * gq;
+        double *Bb = S.Bib[g];
+        for (int q = 0; q < 12; ++q) Bb[q] = 0.0;
+        Bb[1 + 3 * 0] = -b1y; Bb[1 + 3 * 1] = -b2y;
+        Bb[0 + 3 * 2] = b1x;  Bb[0 + 3 * 3] = b2x;
+        Bb[2 + 3 * 0] = -b1x; Bb[2 + 3 * 1] = -b2x;
+        Bb[2 + 3 * 2] = b1y;  Bb[2 + 3 * 3] = b2y;
+    }
+    if (valid && lane == 63 && S.do_shear) {
+        if (!q4_macneal(S)) S.status = JFEM_ERR_DEGENERATE;
+    }
+    if (valid && lane >= 59 && lane <= 62) {  // lumped drilling rows, src/FEMKernels.jl:3772-3817
+        int i = lane - 59;
+        const double cr[4] = {-1, 1, 1, -1}, cs_[4] = {-1, -1, 1, 1};
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(cr[i], cs_[i], dNr, dNs);
+        jac4(dNr, dNs, S.lc, J11, J12, J21, J22);
+        double detJ = J11 * J22 - J12 * J21;
+        if (!jac_ok(detJ, J11, J12, J21, J22)) S.status = JFEM_ERR_DEGENERATE;
+        double inv_det = 1.0 / detJ;
+        double iJ11 = J22 * inv_det, iJ12 = -J12 * inv_det, iJ21 = -J21 * inv_det, iJ22 = J11 * inv_det;
+        for (int q = 0; q < 24; ++q) S.Bd[i][q] = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            double dx = iJ11 * dNr[k] + iJ12 * dNs[k];
+            double dy = iJ21 * dNr[k] + iJ22 * dNs[k];
+            S.Bd[i][6 * k + 0] = 0.5 * dy;
+            S.Bd[i][6 * k + 1] = -0.5 * dx;
+        }
+        S.Bd[i][6 * i + 5] = 1.0;
+        double A_drill = 4.0 * abs_detJc;
+        double alpha_L = ((S.ds * (S.k6 * 1e-6)) * S.Cm[8]) * A_drill;
+        S.dcoef[i] = alpha_L * (fabs(detJ) / fmax(abs_detJc, 1e-30));
+    }
+    __syncthreads();
+    // Phase 2: internal-mode couplings K_ab, K_bb (membrane), K_ab/K_bb with the
+    // shape metric and with Cb (bending), src/FEMKernels.jl:3565-3610
+    if (valid) {
+        for (int idx = lane; idx < 336; idx += kWave) {
+            int which, i, m;
+            double *dst;
+            if (idx < 96) { which = 0; i = idx % 24; m = idx / 24; dst = &S.Kab[idx]; }
+            else if (idx < 112) { which = 1; i = (idx - 96) % 4; m = (idx - 96) / 4; dst = &S.Kbb[idx - 96]; }
+            else if (idx < 208) { which = 2; i = (idx - 112) % 24; m = (idx - 112) / 24; dst = &S.Kabs[idx - 112]; }
+            else if (idx < 224) { which = 3; i = (idx - 208) % 4; m = (idx - 208) / 4; dst = &S.Kbbs[idx - 208]; }
+            else if (idx < 320) { which = 4; i = (idx - 224) % 24; m = (idx - 224) / 24; dst = &S.Kabb[idx - 224]; }
+            else { which = 5; i = (idx - 320) % 4; m = (idx - 320) / 4; dst = &S.Kbbb[idx - 320]; }
+            const double *C = (which <= 1) ? S.Cm : ((which <= 3) ? S.Cm : S.Cb);
+            double acc = 0.0;
+            for (int g = 0; g < 4; ++g) {
+                const double *Bx = (which <= 1) ? S.Bi[g] : S.Bib[g];
+                double t[3];
+                for (int l = 0; l < 3; ++l) {
+                    double v = 0.0;
+                    for (int lp = 0; lp < 3; ++lp) v += C[l + 3 * lp] * Bx[lp + 3 * m];
+                    t[l] = v;
+                }
+                double w = S.gw[g];
+                for (int l = 0; l < 3; ++l) {
+                    double a;
+                    if (which == 0) a = bm_entry(S, g, l, i);
+                    else if (which == 2 || which == 4) a = bb_entry(S, g, l, i);
+                    else if (which == 1) a = S.Bi[g][l + 3 * i];
+                    else a = S.Bib[g][l + 3 * i];
+                    acc += a * (w * t[l]);
+                }
+            }
+            *dst = acc;
+        }
+    }
+    __syncthreads();
+    // Phase 3: internal-mode compliances (lanes 0, 1), MacNeal plate block (others)
+    if (valid) {
+        if (lane == 0) {
+            double mx = 0.0;
+            for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(S.Kbb[q]));
+            if (mx > 0) {
+                if (!active_inverse_dev(S.Kbb, S.modes, S.Cim)) S.status = JFEM_ERR_DEGENERATE;
+            } else {
+                for (int q = 0; q < 16; ++q) S.Cim[q] = 0.0;
+            }
+        } else if (lane == 1) {
+            double mx = 0.0;
+            for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(S.Kbbs[q]));
+            if (mx > 0) {
+                if (!inv4_dev(S.Kbbs, S.Cib)) S.status = JFEM_ERR_DEGENERATE;
+            } else {
+                for (int q = 0; q < 16; ++q) S.Cib[q] = 0.0;
+            }
+        } else if (S.do_shear) {
+            for (int idx = lane - 2; idx < 144; idx += kWave - 2) {
+                int i = idx % 12, j = idx / 12;
+                double rij = 0.0, rji = 0.0;
+                for (int k = 0; k < 4; ++k) rij += S.Dm[k + 4 * i] * S.Xm[k + 4 * j];
+                for (int k = 0; k < 4; ++k) rji += S.Dm[k + 4 * j] * S.Xm[k + 4 * i];
+                S.Kp[idx] = 0.5 * (rij + rji);
+            }
+        }
+    }
+    __syncthreads();
+    // Phase 4: enrichment maps Pm = -Cm_inv K_ab', Pb = -Cb_inv K_abs' (src/FEMKernels.jl:1989-2003)
+    if (valid) {
+        for (int idx = lane; idx < 192; idx += kWave) {
+            bool bend = idx >= 96;
+            int q = bend ? idx - 96 : idx;
+            int i = q % 4, j = q / 4;
+            const double *C = bend ? S.Cib : S.Cim;
+            const double *Ka = bend ? S.Kabs : S.Kab;
+            double v = 0.0;
+            for (int k = 0; k < 4; ++k) v -= C[i + 4 * k] * Ka[j + 24 * k];
+            (bend ? S.Pb : S.Pm)[q] = v;
+        }
+    }
+    __syncthreads();
+    // Phase 5: every lane forms its 3x3 block (A, B), then rotates it
+    if (valid) {
+        const int A = lane / 8, B = lane % 8;
+        double blk[9];
+        for (int jj = 0; jj < 3; ++jj)
+            for (int ii = 0; ii < 3; ++ii) {
+                const int i = 3 * A + ii, j = 3 * B + jj;
+                double v = 0.0;
+                // Gauss-point membrane and bending terms, src/FEMKernels.jl:3528-3532
+                for (int g = 0; g < 4; ++g) {
+                    double w = S.gw[g];
+                    double tm[3], tb[3];
+                    for (int l = 0; l < 3; ++l) {
+                        double a = 0.0, b = 0.0;
+                        for (int lp = 0; lp < 3; ++lp) {
+                            a += S.Cm[l + 3 * lp] * bm_entry(S, g, lp, j);
+                            b += S.Cb[l + 3 * lp] * bb_entry(S, g, lp, j);
+                        }
+                        tm[l] = a; tb[l] = b;
+                    }
+                    for (int l = 0; l < 3; ++l) v += bm_entry(S, g, l, i) * (w * tm[l]);
+                    for (int l = 0; l < 3; ++l) v += bb_entry(S, g, l, i) * (w * tb[l]);
+                }
+                // MacNeal plate block, src/FEMKernels.jl:6279-6285
+                int pi = plate_index(i), pj = plate_index(j);
+                if (S.do_shear && pi >= 0 && pj >= 0) v += S.Kp[pi + 12 * pj];
+                // lumped drilling
+                for (int c = 0; c < 4; ++c) v += S.Bd[c][i] * (S.dcoef[c] * S.Bd[c][j]);
+                // section enrichment, src/FEMKernels.jl:2005-2021
+                double e = 0.0;
+                for (int k = 0; k < 4; ++k) {
+                    double qim = S.Kab[i + 24 * k], qjm = S.Kab[j + 24 * k];
+                    double qib = S.Kabb[i + 24 * k], qjb = S.Kabb[j + 24 * k];
+                    e += qim * S.Pm[k + 4 * j] + S.Pm[k + 4 * i] * qjm + qib * S.Pb[k + 4 * j] + S.Pb[k + 4 * i] * qjb;
+                    for (int l = 0; l < 4; ++l)
+                        e += S.Pm[k + 4 * i] * S.Kbb[k + 4 * l] * S.Pm[l + 4 * j] +
+                             S.Pb[k + 4 * i] * S.Kbbb[k + 4 * l] * S.Pb[l + 4 * j];
+                }
+                blk[ii + 3 * jj] = v + e;
+            }
+        // R' * blk * R, R rows = (v1, v2, v3) (src/solver/assembly.jl:392-424)
+        const double *R = S.R;
+        double t[9];
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                double s = 0;
+                for (int l = 0; l < 3; ++l) s += blk[i + 3 * l] * R[l * 3 + j];
+                t[i + 3 * j] = s;
+            }
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                double s = 0;
+                for (int l = 0; l < 3; ++l) s += R[l * 3 + i] * t[l + 3 * j];
+                S.out[(3 * A + i) + 24 * (3 * B + j)] = s;
+            }
+    }
+    __syncthreads();
+    if (valid) {
+        if (S.status != 0) {
+            if (lane == 0) report(status, elem, S.status);
+        }
+        double *dst = Ke_out + elem * 576;
+        for (int q = lane; q < 576; q += kWave) dst[q] = S.out[q];
+    }
+}
+
+// frame + local coordinates + section for a CQUAD4 in global coordinates
+// (shell_element_frame_quad4 diag mode, src/solver/helpers.jl:1422-1481;
+//  src/solver/assembly.jl:3686-3691, 4392-4426)
+__device__ void q4_setup(Q4Shared &S, const double *p, const double *prop, double k6rot) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6, *p4 = p + 9;
+    double gxi[3], geta[3], vr[3], d13[3], d24[3], v1[3], v2[3], v3[3];
+    for (int c = 0; c < 3; ++c) {
+        gxi[c] = 0.25 * (((-p1[c]) + p2[c]) + p3[c] - p4[c]);
+        geta[c] = 0.25 * (((-p1[c]) - p2[c]) + p3[c] + p4[c]);
+        d13[c] = p3[c] - p1[c];
+        d24[c] = p4[c] - p2[c];
+    }
+    cross3(gxi, geta, vr);
+    if (dot3(vr, vr) <= 1e-24) cross3(d13, d24, vr);
+    v3[0] = vr[0]; v3[1] = vr[1]; v3[2] = vr[2];
+    normalize3(v3);
+    double a13[3] = {d13[0], d13[1], d13[2]}, a24[3] = {d24[0], d24[1], d24[2]};
+    normalize3(a13); normalize3(a24);
+    double xr[3] = {a13[0] - a24[0], a13[1] - a24[1], a13[2] - a24[2]};
+    if (dot3(xr, xr) < 1e-20) { xr[0] = a13[0] + a24[0]; xr[1] = a13[1] + a24[1]; xr[2] = a13[2] + a24[2]; }
+    double dd = dot3(xr, v3);
+    double xp[3] = {xr[0] - dd * v3[0], xr[1] - dd * v3[1], xr[2] - dd * v3[2]};
+    if (dot3(xp, xp) <= 1e-24) {
+        double de = dot3(geta, v3);
+        double xa[3] = {geta[0] - de * v3[0], geta[1] - de * v3[1], geta[2] - de * v3[2]};
+        if (dot3(xa, xa) <= 1e-24) {
+            double e[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+            double d2 = dot3(e, v3);
+            xa[0] = e[0] - d2 * v3[0]; xa[1] = e[1] - d2 * v3[1]; xa[2] = e[2] - d2 * v3[2];
+        }
+        xp[0] = xa[0]; xp[1] = xa[1]; xp[2] = xa[2];
+    }
+    v1[0] = xp[0]; v1[1] = xp[1]; v1[2] = xp[2];
+    normalize3(v1);
+    cross3(v3, v1, v2);
+    double c[3];
+    for (int k = 0; k < 3; ++k) c[k] = (((p1[k] + p2[k]) + p3[k]) + p4[k]) / 4.0;
+    double lc[8];
+    for (int k = 0; k < 4; ++k) {
+        double d[3] = {p[3 * k] - c[0], p[3 * k + 1] - c[1], p[3 * k + 2] - c[2]};
+        lc[2 * k] = dot3(d, v1);
+        lc[2 * k + 1] = dot3(d, v2);
+    }
+    // warp gate (quad4_local_z_from_coords3d, src/FEMKernels.jl:853-865, 1100-1102)
+    {
+        double a[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+        double b[3] = {p4[0] - p2[0], p4[1] - p2[1], p4[2] - p2[2]};
+        double n[3];
+        cross3(a, b, n);
+        double nr = sqrt(dot3(n, n));
+        double w3[3] = {0, 0, 1};
+        if (nr > 1e-30) { w3[0] = n[0] / nr; w3[1] = n[1] / nr; w3[2] = n[2] / nr; }
+        double zmax = 0, xmax = 0;
+        for (int k = 0; k < 4; ++k) {
+            double d[3] = {p[3 * k] - c[0], p[3 * k + 1] - c[1], p[3 * k + 2] - c[2]};
+            zmax = fmax(zmax, fabs(dot3(d, w3)));
+            xmax = fmax(xmax, fabs(lc[2 * k]));
+        }
+        if (xmax < 1e-30) xmax = 1e-30;
+        if (zmax > 1e-12 * xmax) S.status = JFEM_ERR_WARPED;
+    }
+    double Cm[9], Cb[9], Cs[4], G;
+    bool mo;
+    shell_section(prop, Cm, Cb, Cs, G, mo);
+    {
+        double e[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+        double de = dot3(e, v3);
+        double ep[3] = {e[0] - de * v3[0], e[1] - de * v3[1], e[2] - de * v3[2]};
+        double l2 = dot3(ep, ep);
+        if (l2 > 1e-24) {
+            double l = sqrt(l2);
+            double xm[3] = {ep[0] / l, ep[1] / l, ep[2] / l};
+            double beta = 0.0 + atan2(dot3(xm, v2), dot3(xm, v1));
+            if (fabs(beta) > 1e-10) {
+                double cb = cos(beta), sb = sin(beta);
+                double c2 = cb * cb, s2 = sb * sb, cs = cb * sb;
+                rot3(Cm, c2, s2, cs, s2, c2, -cs, -2 * cs, 2 * cs, c2 - s2);
+                rot3(Cb, c2, s2, cs, s2, c2, -cs, -2 * cs, 2 * cs, c2 - s2);
+                double a11 = Cs[0], a12 = Cs[2], a22 = Cs[3];
+                Cs[0] = cb * cb * a11 - 2 * cb * sb * a12 + sb * sb * a22;
+                Cs[2] = cb * sb * a11 + (cb * cb - sb * sb) * a12 - cb * sb * a22;
+                Cs[1] = Cs[2];
+                Cs[3] = sb * sb * a11 + 2 * cb * sb * a12 + cb * cb * a22;
+            }
+        }
+    }
+    for (int q = 0; q < 8; ++q) S.lc[q] = lc[q];
+    for (int q = 0; q < 9; ++q) { S.Cm[q] = Cm[q]; S.Cb[q] = Cb[q]; }
+    for (int q = 0; q < 4; ++q) S.Cs[q] = Cs[q];
+    S.R[0] = v1[0]; S.R[1] = v1[1]; S.R[2] = v1[2];
+    S.R[3] = v2[0]; S.R[4] = v2[1]; S.R[5] = v2[2];
+    S.R[6] = v3[0]; S.R[7] = v3[1]; S.R[8] = v3[2];
+    S.h = prop[JFEM_SP_H];
+    S.rigid = (prop[JFEM_SP_RIGID_SHEAR] != 0.0) && !mo;
+    S.k6 = mo ? 0.0 : k6rot;
+    S.ds = mo ? 0.0 : 1.0;
+    int modes = (int)prop[JFEM_SP_MEMBRANE_MODES];
+    S.modes = (modes <= 0 || modes > 15) ? 15 : modes;
+    double mcs = fmax(fmax(fabs(Cs[0]), fabs(Cs[1])), fmax(fabs(Cs[2]), fabs(Cs[3])));
+    S.do_shear = !(S.h < 1e-30 || (!S.rigid && mcs < 1e-30));
+}
+
+__global__ void __launch_bounds__(kWave * kElemsPerBlock)
+q4_global_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
+                 const double *__restrict__ z, const int32_t *__restrict__ conn,
+                 const double *__restrict__ prop, double k6rot, double *__restrict__ Ke,
+                 int64_t *status) {
+    __shared__ Q4Shared sh[kElemsPerBlock];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t e = (int64_t)blockIdx.x * kElemsPerBlock + wave;
+    const bool valid = e < n;
+    Q4Shared &S = sh[wave];
+    if (valid) {
+        // every lane derives the same frame/section; lane 0 publishes it
+        double p[12], pr[8];
+        for (int k = 0; k < 4; ++k) {
+            int nd = conn[k * n + e];
+            p[3 * k] = x[nd]; p[3 * k + 1] = y[nd]; p[3 * k + 2] = z[nd];
+        }
+        for (int c = 0; c < 8; ++c) pr[c] = prop[c * n + e];
+        if (lane == 0) {
+            S.status = 0;
+            q4_setup(S, p, pr, k6rot);
+        }
+    }
+    __syncthreads();
+    q4_element(S, lane, valid, e, status, Ke);
+}
+
+__global__ void __launch_bounds__(kWave * kElemsPerBlock)
+q4_local_kernel(int64_t n, const double *__restrict__ lc, const double *__restrict__ Cm,
+                const double *__restrict__ Cb, const double *__restrict__ Cs,
+                const double *__restrict__ h, const double *__restrict__ k6,
+                const double *__restrict__ ds, const int32_t *__restrict__ flags,
+                double *__restrict__ Ke, int64_t *status) {
+    __shared__ Q4Shared sh[kElemsPerBlock];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t e = (int64_t)blockIdx.x * kElemsPerBlock + wave;
+    const bool valid = e < n;
+    Q4Shared &S = sh[wave];
+    if (valid && lane == 0) {
+        S.status = 0;
+        for (int q = 0; q < 8; ++q) S.lc[q] = lc[8 * e + q];
+        for (int q = 0; q < 9; ++q) { S.Cm[q] = Cm[9 * e + q]; S.Cb[q] = Cb[9 * e + q]; }
+        for (int q = 0; q < 4; ++q) S.Cs[q] = Cs[4 * e + q];
+        for (int q = 0; q < 9; ++q) S.R[q] = (q % 4 == 0) ? 1.0 : 0.0;
+        S.h = h[e]; S.k6 = k6[e]; S.ds = ds[e];
+        int f = flags[e];
+        S.rigid = f & 1;
+        int modes = (f >> 4) & 15;
+        S.modes = modes == 0 ? 15 : modes;
+        double mcs = fmax(fmax(fabs(S.Cs[0]), fabs(S.Cs[1])), fmax(fabs(S.Cs[2]), fabs(S.Cs[3])));
+        S.do_shear = !(S.h < 1e-30 || (!S.rigid && mcs < 1e-30));
+    }
+    __syncthreads();
+    q4_element(S, lane, valid, e, status, Ke);
+}
+
+// ------------------------------------------------------------------------------
+// CTRIA3 (stiffness_tria3_matrices_generic defaults, src/FEMKernels.jl:9375-9809)
+// ------------------------------------------------------------------------------
+struct T3Shared {
+    double R[9], Dm[9], Db[9];
+    double A, bv[3], cv[3], alpha;
+    double W[9], D3[27], Kb[81];
+    double out[324];
+    int status, zero;
+};
+
+__device__ void tria3_guard(double *Zb, double La, double Lb, double c, double s, double Dn) {
+    double scale = 6 * (La / Lb) * s * Dn;
+    if (!(scale > 0 && Zb[8] > 0)) return;
+    double z33 = Zb[8];
+    double h11 = scale * (Zb[0] - Zb[6] * Zb[6] / z33) / (La * La);
+    double h22 = scale * (Zb[4] - Zb[7] * Zb[7] / z33) / (Lb * Lb);
+    double h12 = scale * (Zb[3] - Zb[6] * Zb[7] / z33) / (La * Lb);
+    double p11 = h11, p12 = (h12 - c * h11) / s, p22 = (h22 - 2 * c * h12 + c * c * h11) / (s * s);
+    double a = fmax(p11, 1.0 / 10000), q = fmax(p22, a / 10000);
+    double limit = sqrt((85.0 / 100) * a * q);
+    double b = p12 < -limit ? -limit : (p12 > limit ? limit : p12);
+    if (a == p11 && b == p12 && q == p22) return;
+    double n11 = a, n12 = c * a + s * b, n22 = c * c * a + 2 * c * s * b + s * s * q;
+    Zb[0] += (n11 - h11) * La * La / scale;
+    Zb[3] += (n12 - h12) * La * Lb / scale;
+    Zb[1] = Zb[3];
+    Zb[4] += (n22 - h22) * Lb * Lb / scale;
+}
+
+// serial part of the CTRIA3 kernel (one lane): geometry, MITC3 tying rows,
+// tying rigidity with residual bending flexibility (src/FEMKernels.jl:9379-9733)
+__device__ void t3_setup(T3Shared &S, const double *lc, const double *Ds, double k6) {
+    double x1 = lc[0], y1 = lc[1], x2 = lc[2], y2 = lc[3], x3 = lc[4], y3 = lc[5];
+    double A2 = x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2);
+    double A = 0.5 * fabs(A2);
+    S.zero = A < 1e-12;
+    S.A = A;
+    if (S.zero) return;
+    S.bv[0] = (y2 - y3) / (2 * A); S.bv[1] = (y3 - y1) / (2 * A); S.bv[2] = (y1 - y2) / (2 * A);
+    S.cv[0] = (x3 - x2) / (2 * A); S.cv[1] = (x1 - x3) / (2 * A); S.cv[2] = (x2 - x1) / (2 * A);
+    // constant-curvature bending Kb = Bb' Db Bb A
+    double Bb[27];
+    for (int q = 0; q < 27; ++q) Bb[q] = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        Bb[0 + 3 * (3 * i + 2)] = S.bv[i]; Bb[1 + 3 * (3 * i + 1)] = -S.cv[i];
+        Bb[2 + 3 * (3 * i + 1)] = -S.bv[i]; Bb[2 + 3 * (3 * i + 2)] = S.cv[i];
+    }
+    for (int j = 0; j < 9; ++j)
+        for (int i = 0; i < 9; ++i) {
+            double s = 0;
+            for (int a = 0; a < 3; ++a) {
+                double t = 0;
+                for (int b = 0; b < 3; ++b) t += Bb[b + 3 * i] * S.Db[b + 3 * a];
+                s += t * Bb[a + 3 * j];
+            }
+            S.Kb[i + 9 * j] = s * A;
+        }
+    double Jm11 = x2 - x1, Jm12 = y2 - y1, Jm21 = x3 - x1, Jm22 = y3 - y1;
+    double dj = Jm11 * Jm22 - Jm12 * Jm21;
+    double Ji11 = Jm22 / dj, Ji12 = -Jm12 / dj, Ji21 = -Jm21 / dj, Ji22 = Jm11 / dj;
+    double grA[9], gsB[9], grC[9], gsC[9];
+    const double rr[3] = {0.5, 0.0, 0.5}, ss[3] = {0.0, 0.5, 0.5};
+    const double dwdr[3] = {-1, 1, 0}, dwds[3] = {-1, 0, 1};
+    for (int pnt = 0; pnt < 3; ++pnt) {
+        double N[3] = {1 - rr[pnt] - ss[pnt], rr[pnt], ss[pnt]};
+        double Br[9], Bs[9];
+        for (int i = 0; i < 3; ++i) {
+            Br[3 * i] = dwdr[i]; Bs[3 * i] = dwds[i];
+            Br[3 * i + 2] = N[i] * Jm11; Bs[3 * i + 2] = N[i] * Jm21;
+            Br[3 * i + 1] = -N[i] * Jm12; Bs[3 * i + 1] = -N[i] * Jm22;
+        }
+        for (int q = 0; q < 9; ++q) {
+            if (pnt == 0) grA[q] = Br[q];
+            else if (pnt == 1) gsB[q] = Bs[q];
+            else { grC[q] = Br[q]; gsC[q] = Bs[q]; }
+        }
+    }
+    for (int q = 0; q < 9; ++q) {
+        S.D3[0 + 3 * q] = grA[q];
+        S.D3[1 + 3 * q] = gsB[q];
+        S.D3[2 + 3 * q] = (grC[q] - grA[q]) - (gsC[q] - gsB[q]);
+    }
+    double La = hypot(x2 - x1, y2 - y1), Lb = hypot(x3 - x1, y3 - y1), Lc = hypot(x3 - x2, y3 - y2);
+    double P = La + Lb + Lc;
+    double cth = ((x2 - x1) * (x3 - x1) + (y2 - y1) * (y3 - y1)) / fmax(La * Lb, 1e-30);
+    cth = cth < -1 ? -1 : (cth > 1 ? 1 : cth);
+    double sth = sqrt(fmax(1 - cth * cth, 0.0));
+    double t11 = Ds[0] * Ji11 + Ds[2] * Ji21, t12 = Ds[0] * Ji12 + Ds[2] * Ji22;
+    double t21 = Ds[1] * Ji11 + Ds[3] * Ji21, t22 = Ds[1] * Ji12 + Ds[3] * Ji22;
+    double G11 = Ji11 * t11 + Ji21 * t21, G12 = Ji11 * t12 + Ji21 * t22, G22 = Ji12 * t12 + Ji22 * t22;
+    double uu = 3 * La / P, vv = 3 * Lb / P;
+    double *W = S.W;
+    W[0] = G11 * A; W[3] = G12 * A; W[1] = W[3]; W[4] = G22 * A;
+    W[6] = (A / 3) * (uu * G11 - vv * G12); W[2] = W[6];
+    W[7] = (A / 3) * (uu * G12 - vv * G22); W[5] = W[7];
+    W[8] = uu * uu * G11 * (A / 6) - 2 * uu * vv * G12 * (A * (2 - cth) / 18) + vv * vv * G22 * (A / 6);
+    double Dbend = fabs(S.Db[0]);
+    if (sth > 1e-9 && Dbend > 1e-30 && La > 1e-30 && Lb > 1e-30) {
+        double S1 = La, S2 = Lb, S3 = P, s6 = 1.0 / 6;
+        double D11e = Dbend, D22e = fabs(S.Db[4]);
+        bool dir_pair = D11e > 1e-30 && D22e > 1e-30;
+        double rho = La / Lb, Rr2 = rho + 1 / rho - 2, Tt2 = rho - 1 / rho;
+        double fr = ((1 - cth) * (1 - cth) - (rho - cth)) / (6 * sth) + Rr2 * (1 + cth * cth) / (12 * sth) + Tt2 * sth / 12;
+        double fri = ((1 - cth) * (1 - cth) - (1 / rho - cth)) / (6 * sth) + Rr2 * (1 + cth * cth) / (12 * sth) - Tt2 * sth / 12;
+        double w12 = (1 / rho - cth) / (6 * sth), w13 = (rho - cth) / (6 * sth), w23 = cth / (6 * sth);
+        double o12 = (Rr2 * cth - (1 - cth) * (1 - cth) - Tt2 * cth) / (12 * sth);
+        double o13 = (Rr2 * cth - (1 - cth) * (1 - cth) + Tt2 * cth) / (12 * sth);
+        double o23 = (1 + cth * cth) / (12 * sth);
+        double Rr = (La - Lb) * (La - Lb) / (La * Lb), Tt = (La * La - Lb * Lb) / (La * Lb);
+        double sum_d = 2 * ((1 - cth + cth * cth) / (6 * sth) + 1.0 / 6) + Rr * (1 + cth * cth) / (6 * sth);
+        double dif_d = Tt * sth / 6;
+        double a11 = (sum_d + dif_d) / 2, a22 = (sum_d - dif_d) / 2;
+        double a12 = (-1 + 4 * cth - cth * cth) / (12 * sth) - 1.0 / 6 + Rr * cth / (6 * sth);
+        double ev[3][2] = {{x2 - x1, y2 - y1}, {x3 - x1, y3 - y1}, {x3 - x2, y3 - y2}};
+        double Dn[3];
+        for (int e = 0; e < 3; ++e) {
+            double vx = ev[e][0], vy = ev[e][1], n = hypot(vx, vy);
+            if (!dir_pair || n < 1e-30) { Dn[e] = Dbend; continue; }
+            double nx = -vy / n, ny = vx / n;
+            double wv[3] = {nx * nx, ny * ny, 2 * nx * ny};
+            double d = 0;
+            for (int ii = 0; ii < 3; ++ii)
+                for (int jj = 0; jj < 3; ++jj) d += wv[ii] * S.Db[ii + 3 * jj] * wv[jj];
+            Dn[e] = fabs(d) > 1e-30 ? d : Dbend;
+        }
+        double Gd = dir_pair ? sqrt(D11e * D22e) : Dbend;
+        double c11 = dir_pair ? s6 / Gd + fr / Dn[0] + w13 / Dn[1] + w23 / Dn[2] : a11 / Dbend;
+        double c22 = dir_pair ? s6 / Gd + w12 / Dn[0] + fri / Dn[1] + w23 / Dn[2] : a22 / Dbend;
+        double c12 = dir_pair ? -s6 / Gd + o12 / Dn[0] + o13 / Dn[1] + o23 / Dn[2] : a12 / Dbend;
+        double Zb[9];
+        Zb[0] = c11 * S1 * S1; Zb[3] = c12 * S1 * S2; Zb[1] = Zb[3]; Zb[4] = c22 * S2 * S2;
+        Zb[6] = -s6 / Gd * S1 * S3; Zb[2] = Zb[6];
+        Zb[7] = s6 / Gd * S2 * S3; Zb[5] = Zb[7];
+        Zb[8] = s6 / Gd * S3 * S3;
+        tria3_guard(Zb, La, Lb, cth, sth, dir_pair ? Dn[0] : Dbend);
+        double Wi[9], Zt[9], Zs[9];
+        if (!inv3_dev(W, Wi)) { S.status = JFEM_ERR_DEGENERATE; return; }
+        for (int q = 0; q < 9; ++q) Zt[q] = Wi[q] + Zb[q];
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) Zs[i + 3 * j] = 0.5 * (Zt[i + 3 * j] + Zt[j + 3 * i]);
+        if (!inv3_dev(Zs, W)) { S.status = JFEM_ERR_DEGENERATE; return; }
+    }
+    S.alpha = (k6 * 1e-6) * S.Dm[8] * (2 * A);
+}
+
+__device__ inline int t3_mem(int d) {  // local DOF -> CST membrane index or -1
+    int k = d / 6, c = d % 6;
+    return c < 2 ? 2 * k + c : -1;
+}
+__device__ inline int t3_plate(int d) {
+    int k = d / 6, c = d % 6;
+    return (c >= 2 && c <= 4) ? 3 * k + (c - 2) : -1;
+}
+__device__ inline double t3_bm(const T3Shared &S, int l, int m) {  // Bm (3x6) entry
+    int i = m / 2, c = m % 2;
+    if (c == 0) return l == 0 ? S.bv[i] : (l == 1 ? 0.0 : S.cv[i]);
+    return l == 0 ? 0.0 : (l == 1 ? S.cv[i] : S.bv[i]);
+}
+
+__global__ void __launch_bounds__(kWave * kElemsPerBlock)
+t3_global_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
+                 const double *__restrict__ z, const int32_t *__restrict__ conn,
+                 const double *__restrict__ prop, double k6rot, double *__restrict__ Ke,
+                 int64_t *status) {
+    __shared__ T3Shared sh[kElemsPerBlock];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t e = (int64_t)blockIdx.x * kElemsPerBlock + wave;
+    const bool valid = e < n;
+    T3Shared &S = sh[wave];
+    if (valid && lane == 0) {
+        double p[9], pr[8];
+        for (int k = 0; k < 3; ++k) {
+            int nd = conn[k * n + e];
+            p[3 * k] = x[nd]; p[3 * k + 1] = y[nd]; p[3 * k + 2] = z[nd];
+        }
+        for (int c = 0; c < 8; ++c) pr[c] = prop[c * n + e];
+        // shell_element_frame_fast n=3, src/solver/helpers.jl:1486-1490
+        double v1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]};
+        normalize3(v1);
+        double e13[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]}, v3[3], v2[3];
+        cross3(v1, e13, v3);
+        normalize3(v3);
+        cross3(v3, v1, v2);
+        double c[3], lc[6];
+        for (int k = 0; k < 3; ++k) c[k] = ((p[k] + p[3 + k]) + p[6 + k]) / 3.0;
+        for (int k = 0; k < 3; ++k) {
+            double d[3] = {p[3 * k] - c[0], p[3 * k + 1] - c[1], p[3 * k + 2] - c[2]};
+            lc[2 * k] = dot3(d, v1);
+            lc[2 * k + 1] = dot3(d, v2);
+        }
+        double Cs[4], G;
+        bool mo;
+        shell_section(pr, S.Dm, S.Db, Cs, G, mo);
+        S.R[0] = v1[0]; S.R[1] = v1[1]; S.R[2] = v1[2];
+        S.R[3] = v2[0]; S.R[4] = v2[1]; S.R[5] = v2[2];
+        S.R[6] = v3[0]; S.R[7] = v3[1]; S.R[8] = v3[2];
+        S.status = 0;
+        t3_setup(S, lc, Cs, mo ? 0.0 : k6rot);
+    }
+    __syncthreads();
+    if (valid && lane < 36) {
+        const int A = lane / 6, B = lane % 6;
+        double blk[9];
+        for (int jj = 0; jj < 3; ++jj)
+            for (int ii = 0; ii < 3; ++ii) {
+                const int i = 3 * A + ii, j = 3 * B + jj;
+                double v = 0.0;
+                if (!S.zero) {
+                    int mi = t3_mem(i), mj = t3_mem(j);
+                    if (mi >= 0 && mj >= 0) {  // CST membrane, src/FEMKernels.jl:9390-9396
+                        double s = 0;
+                        for (int a = 0; a < 3; ++a) {
+                            double t = 0;
+                            for (int b = 0; b < 3; ++b) t += t3_bm(S, b, mi) * S.Dm[b + 3 * a];
+                            s += t * t3_bm(S, a, mj);
+                        }
+                        v = s * S.A;
+                    }
+                    int pi = t3_plate(i), pj = t3_plate(j);
+                    if (pi >= 0 && pj >= 0) {  // Kb + D3' W D3, :9731-9733
+                        double s = 0;
+                        for (int a = 0; a < 3; ++a) {
+                            double t = 0;
+                            for (int b = 0; b < 3; ++b) t += S.D3[b + 3 * pi] * S.W[b + 3 * a];
+                            s += t * S.D3[a + 3 * pj];
+                        }
+                        v += S.Kb[pi + 9 * pj] + s;
+                    }
+                    // lumped drilling, :9780-9795
+                    for (int c = 0; c < 3; ++c) {
+                        int ki = i / 6, ci = i % 6, kj = j / 6, cj = j % 6;
+                        double bi = ci == 0 ? 0.5 * S.cv[ki] : (ci == 1 ? -0.5 * S.bv[ki] : (ci == 5 && ki == c ? 1.0 : 0.0));
+                        double bj = cj == 0 ? 0.5 * S.cv[kj] : (cj == 1 ? -0.5 * S.bv[kj] : (cj == 5 && kj == c ? 1.0 : 0.0));
+                        v += S.alpha * (bi * bj);
+                    }
+                }
+                blk[ii + 3 * jj] = v;
+            }
+        const double *R = S.R;
+        double t[9];
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                double s = 0;
+                for (int l = 0; l < 3; ++l) s += blk[i + 3 * l] * R[l * 3 + j];
+                t[i + 3 * j] = s;
+            }
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                double s = 0;
+                for (int l = 0; l < 3; ++l) s += R[l * 3 + i] * t[l + 3 * j];
+                S.out[(3 * A + i) + 18 * (3 * B + j)] = s;
+            }
+    }
+    __syncthreads();
+    if (valid) {
+        if (lane == 0 && S.status != 0) report(status, e, S.status);
+        double *dst = Ke + e * 324;
+        for (int q = lane; q < 324; q += kWave) dst[q] = S.out[q];
+    }
+}
+
+// ------------------------------------------------------------------------------
+// CTETRA10 (stiffness_tetra10, src/tetra10.jl:6-79; iso_3d_constitutive,
+// src/FEMKernels.jl:12056-12069)
+// ------------------------------------------------------------------------------
+struct TetShared {
+    double X[30];
+    double g[4][3][10];
+    double w[4];
+    double D[36];
+    double out[900];
+    int status;
+};
+
+__device__ inline double tet_b(const TetShared &S, int q, int l, int d) {  // B (6x30) entry
+    int n = d / 3, c = d % 3;
+    double dx = S.g[q][0][n], dy = S.g[q][1][n], dz = S.g[q][2][n];
+    switch (l) {
+        case 0: return c == 0 ? dx : 0.0;
+        case 1: return c == 1 ? dy : 0.0;
+        case 2: return c == 2 ? dz : 0.0;
+        case 3: return c == 0 ? dy : (c == 1 ? dx : 0.0);
+        case 4: return c == 1 ? dz : (c == 2 ? dy : 0.0);
+        default: return c == 0 ? dz : (c == 2 ? dx : 0.0);
+    }
+}
+
+__global__ void __launch_bounds__(kWave * kElemsPerBlock)
+tet10_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
+             const double *__restrict__ z, const int32_t *__restrict__ conn,
+             const double *__restrict__ prop, double *__restrict__ Ke, int64_t *status) {
+    __shared__ TetShared sh[kElemsPerBlock];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t e = (int64_t)blockIdx.x * kElemsPerBlock + wave;
+    const bool valid = e < n;
+    TetShared &S = sh[wave];
+    if (valid) {
+        if (lane < 10) {
+            int nd = conn[lane * n + e];
+            S.X[lane] = x[nd]; S.X[lane + 10] = y[nd]; S.X[lane + 20] = z[nd];
+        }
+        if (lane == 0) {
+            S.status = 0;
+            double E = prop[e], nu = prop[n + e];
+            if (!(isfinite(E) && E > 0 && isfinite(nu) && nu > -1 && nu < 0.5)) S.status = JFEM_ERR_BAD_ARG;
+            double cc = E / ((1.0 + nu) * (1.0 - 2.0 * nu)), dd = 1.0 - nu, ss = (1.0 - 2.0 * nu) / 2.0;
+            for (int q = 0; q < 36; ++q) S.D[q] = 0.0;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) S.D[i + 6 * j] = (i == j) ? cc * dd : cc * nu;
+            for (int i = 3; i < 6; ++i) S.D[i + 6 * i] = cc * ss;
+        }
+    }
+    __syncthreads();
+    if (valid && lane < 4) {
+        const double a = (5.0 + 3.0 * sqrt(5.0)) / 20.0, b = (5.0 - sqrt(5.0)) / 20.0;
+        const double pts[4][3] = {{b, b, b}, {a, b, b}, {b, a, b}, {b, b, a}};
+        const int edges[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+        const double dL[3][4] = {{-1, 1, 0, 0}, {-1, 0, 1, 0}, {-1, 0, 0, 1}};
+        int q = lane;
+        double L[4] = {1.0 - pts[q][0] - pts[q][1] - pts[q][2], pts[q][0], pts[q][1], pts[q][2]};
+        double dN[3][10];
+        for (int i = 0; i < 4; ++i)
+            for (int k = 0; k < 3; ++k) dN[k][i] = (4.0 * L[i] - 1.0) * dL[k][i];
+        for (int ed = 0; ed < 6; ++ed) {
+            int i = edges[ed][0], j = edges[ed][1];
+            for (int k = 0; k < 3; ++k) dN[k][ed + 4] = 4.0 * (L[j] * dL[k][i] + L[i] * dL[k][j]);
+        }
+        double J[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double s = 0;
+                for (int nn = 0; nn < 10; ++nn) s += dN[r][nn] * S.X[nn + 10 * c];
+                J[r][c] = s;
+            }
+        double c1[3] = {J[0][0], J[1][0], J[2][0]}, c2[3] = {J[0][1], J[1][1], J[2][1]}, c3[3] = {J[0][2], J[1][2], J[2][2]};
+        double cr[3];
+        cross3(c2, c3, cr);
+        double det = dot3(c1, cr);
+        if (!(isfinite(det) && det > 0.0)) S.status = JFEM_ERR_INVERTED;
+        S.w[q] = det / 24.0;
+        double Jc[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) Jc[r + 3 * c] = J[r][c];
+        double Ji[9];
+        Ji[0] = (Jc[4] * Jc[8] - Jc[7] * Jc[5]) / det; Ji[3] = (Jc[6] * Jc[5] - Jc[3] * Jc[8]) / det; Ji[6] = (Jc[3] * Jc[7] - Jc[6] * Jc[4]) / det;
+        Ji[1] = (Jc[7] * Jc[2] - Jc[1] * Jc[8]) / det; Ji[4] = (Jc[0] * Jc[8] - Jc[6] * Jc[2]) / det; Ji[7] = (Jc[6] * Jc[1] - Jc[0] * Jc[7]) / det;
+        Ji[2] = (Jc[1] * Jc[5] - Jc[4] * Jc[2]) / det; Ji[5] = (Jc[3] * Jc[2] - Jc[0] * Jc[5]) / det; Ji[8] = (Jc[0] * Jc[4] - Jc[3] * Jc[1]) / det;
+        for (int r = 0; r < 3; ++r)
+            for (int nn = 0; nn < 10; ++nn) S.g[q][r][nn] = Ji[r] * dN[0][nn] + Ji[r + 3] * dN[1][nn] + Ji[r + 6] * dN[2][nn];
+    }
+    __syncthreads();
+    if (valid) {
+        for (int blk = lane; blk < 100; blk += kWave) {
+            int a = blk / 10, b = blk % 10;
+            for (int jj = 0; jj < 3; ++jj)
+                for (int ii = 0; ii < 3; ++ii) {
+                    int I = 3 * a + ii, J = 3 * b + jj;
+                    int r = I <= J ? I : J, c = I <= J ? J : I;  // Matrix(Symmetric(K)): upper mirrored
+                    double kv = 0.0;
+                    for (int q = 0; q < 4; ++q) {
+                        double s = 0;
+                        for (int l = 0; l < 6; ++l) {
+                            double db = 0;
+                            for (int lp = 0; lp < 6; ++lp) db += S.D[l + 6 * lp] * tet_b(S, q, lp, c);
+                            s += tet_b(S, q, l, r) * db;
+                        }
+                        kv += S.w[q] * s;
+                    }
+                    S.out[I + 30 * J] = kv;
+                }
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        if (lane == 0 && S.status != 0) report(status, e, S.status);
+        double *dst = Ke + e * 900;
+        for (int q = lane; q < 900; q += kWave) dst[q] = S.out[q];
+    }
+}
+
+// ------------------------------------------------------------------------------
+// Assembly: one thread per CSR slot sums its contributions in triplet order.
+// ------------------------------------------------------------------------------
+__global__ void assemble_kernel(int64_t nnz, const int64_t *__restrict__ seg_ptr,
+                                const int32_t *__restrict__ seg_src, const double *__restrict__ Ke,
+                                double *__restrict__ val) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nnz) return;
+    int64_t b = seg_ptr[s], e = seg_ptr[s + 1];
+    double v = 0.0;
+    for (int64_t p = b; p < e; ++p) v += Ke[(uint32_t)seg_src[p]];
+    val[s] = v;
+}
+
+__global__ void gather_nodeblk_kernel(int64_t n, const int64_t *__restrict__ pos, const double *__restrict__ val,
+                                      double *__restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t p = pos[i];
+    out[i] = p >= 0 ? val[p] : 0.0;
+}
+
+__global__ void reset_status_kernel(int64_t *status) {
+    status[0] = INT64_MAX;
+    status[1] = 0;
+}
+
+}  // namespace
+
+namespace jfem {
+
+hipError_t launch_reset_status(hipStream_t s, int64_t *status) {
+    hipLaunchKernelGGL(reset_status_kernel, dim3(1), dim3(1), 0, s, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_q4_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status) {
+    if (m->nq == 0) return hipSuccess;
+    dim3 grid((unsigned)((m->nq + kElemsPerBlock - 1) / kElemsPerBlock));
+    hipLaunchKernelGGL(q4_global_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, m->nq, m->d_x, m->d_y, m->d_z,
+                       m->d_q, m->d_qp, m->k6rot, Ke, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_t3_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status) {
+    if (m->nt == 0) return hipSuccess;
+    dim3 grid((unsigned)((m->nt + kElemsPerBlock - 1) / kElemsPerBlock));
+    hipLaunchKernelGGL(t3_global_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, m->nt, m->d_x, m->d_y, m->d_z,
+                       m->d_t, m->d_tp, m->k6rot, Ke, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_tet10(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status) {
+    if (m->ntet == 0) return hipSuccess;
+    dim3 grid((unsigned)((m->ntet + kElemsPerBlock - 1) / kElemsPerBlock));
+    hipLaunchKernelGGL(tet10_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, m->ntet, m->d_x, m->d_y, m->d_z,
+                       m->d_tet, m->d_tetp, Ke, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_q4_local(hipStream_t s, int64_t n, const double *lc, const double *Cm, const double *Cb,
+                           const double *Cs, const double *h, const double *k6, const double *ds,
+                           const int32_t *flags, double *Ke, int64_t *status) {
+    if (n == 0) return hipSuccess;
+    dim3 grid((unsigned)((n + kElemsPerBlock - 1) / kElemsPerBlock));
+    hipLaunchKernelGGL(q4_local_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, n, lc, Cm, Cb, Cs, h, k6, ds,
+                       flags, Ke, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *Ke, double *val) {
+    if (P->nnz == 0) return hipSuccess;
+    const int bs = 256;
+    dim3 grid((unsigned)((P->nnz + bs - 1) / bs));
+    hipLaunchKernelGGL(assemble_kernel, grid, dim3(bs), 0, s, P->nnz, P->d_seg_ptr, P->d_seg_src, Ke, val);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_nodeblk(hipStream_t s, const jfem_pattern *P, const double *val, double *out) {
+    int64_t n = P->n_nodes * 36;
+    if (n == 0) return hipSuccess;
+    const int bs = 256;
+    hipLaunchKernelGGL(gather_nodeblk_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, n, P->d_nodeblk,
+                       val, out);
+    return hipGetLastError();
+}
+
+}  // namespace jfem

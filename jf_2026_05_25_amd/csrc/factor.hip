@@ -1,0 +1,838 @@
+// Supernodal multifrontal Cholesky of K_ff on gfx950, its level-scheduled
+// triangular solves, the FP64 residual SpMV and the iterative-refinement driver.
+//
+// Storage per supernode s (ncol n, front order m): L_s is m x n column-major
+// (rows = the front's row list), the update matrix U_s is (m-n) x (m-n).
+// Factorization, level by level from the leaves:
+//   scatter K entries into L (once, all supernodes), extend-add children's U into
+//   the parent's L/U (deterministic: children in order), then for each 64-column
+//   panel: factor the diagonal block + triangular solve of the panel rows
+//   (panel kernel), and a rank-64 update of the trailing front (update kernel,
+//   MFMA tiles).  All supernodes of a level are batched into the same launches.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "jfem_internal.h"
+#include "symbolic.h"
+
+using namespace jfem;
+
+struct jfem_symbolic {
+    int device = 0;
+    const jfem_pattern *P = nullptr;
+    SymbolicHost H;
+    int64_t *d_col0 = nullptr, *d_ncol = nullptr, *d_m = nullptr, *d_lofs = nullptr, *d_uofs = nullptr,
+            *d_rofs = nullptr, *d_relofs = nullptr;
+    int32_t *d_rows = nullptr, *d_relmap = nullptr, *d_child_ptr = nullptr, *d_child = nullptr, *d_perm = nullptr,
+            *d_level_sn = nullptr;
+    int32_t *d_amap_src = nullptr;
+    int64_t *d_amap_dst = nullptr;
+    PanelTask *d_ptasks = nullptr, *d_fwd = nullptr, *d_bwd = nullptr;
+    UpdateTask *d_utasks = nullptr;
+    EaTask *d_etasks = nullptr;
+    std::vector<void *> allocs;
+};
+
+struct jfem_factor {
+    int device = 0;
+    const jfem_symbolic *S = nullptr;
+    int precision = 64;
+    void *d_L = nullptr;
+    void *d_U = nullptr;
+    int64_t bytes = 0;
+};
+
+namespace {
+
+constexpr int kBS = 256;
+
+template <class T>
+__device__ inline T tsqrt(T x);
+template <>
+__device__ inline double tsqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ inline float tsqrt<float>(float x) { return sqrtf(x); }
+
+__device__ inline void report_min(int64_t *status, int64_t idx, int code) {
+    unsigned long long old = atomicMin((unsigned long long *)&status[0], (unsigned long long)idx);
+    if ((long long)old > idx) atomicExch((unsigned long long *)&status[1], (unsigned long long)code);
+}
+
+// ----------------------------------------------------------------- fill / scatter
+template <class T>
+__global__ void fill_kernel(T *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = T(0);
+}
+
+template <class T>
+__global__ void scatter_kernel(int64_t n, const int32_t *__restrict__ src, const int64_t *__restrict__ dst,
+                               const double *__restrict__ val, T *__restrict__ L) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        L[dst[i]] = (T)val[src[i]];
+}
+
+// ----------------------------------------------------------------- extend-add
+// One workgroup per (parent, front column range); children added in order.
+template <class T>
+__global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+                                                         const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
+                                                         const int64_t *__restrict__ uofs, const int64_t *__restrict__ relofs,
+                                                         const int32_t *__restrict__ relmap, const int32_t *__restrict__ child_ptr,
+                                                         const int32_t *__restrict__ child, T *__restrict__ L, T *__restrict__ U) {
+    const EaTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s], mu = ms - ns;
+    T *Ls = L + lofs[s];
+    T *Us = U + uofs[s];
+    for (int q = child_ptr[s]; q < child_ptr[s + 1]; ++q) {
+        const int c = child[q];
+        const int64_t nu = mrow[c] - ncol[c];
+        const int32_t *rel = relmap + relofs[c];
+        const T *Uc = U + uofs[c];
+        // child columns whose parent column lies in [c0, c1)
+        int64_t lo = 0, hi = nu;
+        while (lo < hi) { int64_t md = (lo + hi) / 2; if (rel[md] < t.c0) lo = md + 1; else hi = md; }
+        const int64_t j0 = lo;
+        hi = nu;
+        while (lo < hi) { int64_t md = (lo + hi) / 2; if (rel[md] < t.c1) lo = md + 1; else hi = md; }
+        const int64_t j1 = lo;
+        for (int64_t j = j0; j < j1; ++j) {
+            const int64_t pc = rel[j];
+            for (int64_t i = j + threadIdx.x; i < nu; i += blockDim.x) {
+                const int64_t pr = rel[i];
+                const T v = Uc[i + nu * j];
+                if (pc < ns) Ls[pr + ms * pc] += v;
+                else Us[(pr - ns) + mu * (pc - ns)] += v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------- panel kernel
+// first == 1: the task's tile starts at the diagonal: factor the b x b diagonal
+// block of panel k in LDS, store it, and solve the tile rows below it.
+// first == 0: load the stored factored block and solve the tile rows:
+// X <- X L_kk^{-T}.  (Two launches so no workgroup reads a block another one is
+// overwriting.)
+template <class T>
+__global__ void __launch_bounds__(kBS) panel_kernel(const PanelTask *__restrict__ tasks, int first,
+                                                    const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
+                                                    const int64_t *__restrict__ lofs, const int64_t *__restrict__ col0,
+                                                    T *__restrict__ L, int64_t *__restrict__ status) {
+    __shared__ T D[kPanel][kPanel + 1];
+    __shared__ T X[kTile][kPanel + 1];
+    const PanelTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
+    const int b = (int)min<int64_t>(kPanel, ns - k);
+    T *Ls = L + lofs[s];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < b * b; q += blockDim.x) {
+        int i = q % b, j = q / b;
+        D[i][j] = (i >= j) ? Ls[(k + i) + ms * (k + j)] : T(0);
+    }
+    __syncthreads();
+    if (first) {
+        // right-looking Cholesky of the diagonal block
+        for (int j = 0; j < b; ++j) {
+            if (tid == 0) {
+                T d = D[j][j];
+                if (!(d > T(0))) {
+                    report_min(status, col0[s] + k + j, JFEM_ERR_NOT_SPD);
+                    d = T(1);
+                }
+                D[j][j] = tsqrt(d);
+            }
+            __syncthreads();
+            const T djj = D[j][j];
+            for (int i = j + 1 + tid; i < b; i += blockDim.x) D[i][j] /= djj;
+            __syncthreads();
+            for (int q = tid; q < (b - j - 1) * (b - j - 1); q += blockDim.x) {
+                int i = j + 1 + q % (b - j - 1), c = j + 1 + q / (b - j - 1);
+                if (i >= c) D[i][c] -= D[i][j] * D[c][j];
+            }
+            __syncthreads();
+        }
+        for (int q = tid; q < b * b; q += blockDim.x) {
+            int i = q % b, j = q / b;
+            if (i >= j) Ls[(k + i) + ms * (k + j)] = D[i][j];
+        }
+    }
+    const int64_t r0 = t.r0;
+    const int64_t r1 = min<int64_t>(r0 + kTile, ms);
+    const int64_t rs = max<int64_t>(r0, k + b);
+    const int nr = (int)max<int64_t>(0, r1 - rs);
+    if (nr == 0) return;
+    for (int q = tid; q < nr * b; q += blockDim.x) {
+        int i = q % nr, j = q / nr;
+        X[i][j] = Ls[(rs + i) + ms * (k + j)];
+    }
+    __syncthreads();
+    // X <- X L^{-T}: column by column, rows in parallel
+    if (tid < nr) {
+        const int i = tid;
+        for (int j = 0; j < b; ++j) {
+            T v = X[i][j];
+            for (int p = 0; p < j; ++p) v -= X[i][p] * D[j][p];
+            X[i][j] = v / D[j][j];
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < nr * b; q += blockDim.x) {
+        int i = q % nr, j = q / nr;
+        Ls[(rs + i) + ms * (k + j)] = X[i][j];
+    }
+}
+
+// ----------------------------------------------------------------- update kernel
+// C[r, c] -= sum_p A[r, p] A[c, p] over the panel columns p in [k, k+b) for the
+// 64 x 64 tile at (r0, c0) of the trailing front (lower triangle only).
+template <class T>
+__global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+                                                     const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
+                                                     const int64_t *__restrict__ uofs, T *__restrict__ L, T *__restrict__ U) {
+    __shared__ T A[kPanel][kTile + 1];   // A[p][i]  (transposed tile of rows r0..)
+    __shared__ T B[kPanel][kTile + 1];   // B[p][j]  (rows c0..)
+    const UpdateTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
+    const int b = (int)min<int64_t>(kPanel, ns - k);
+    const int64_t r0 = t.r0, c0 = t.c0;
+    const int64_t rb = r0 < ns ? ns : ms, cb = c0 < ns ? ns : ms;
+    const int nr = (int)(min<int64_t>(r0 + kTile, rb) - r0);
+    const int nc = (int)(min<int64_t>(c0 + kTile, cb) - c0);
+    const T *Ls = L + lofs[s];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < kTile * b; q += blockDim.x) {
+        int i = q % kTile, p = q / kTile;
+        A[p][i] = i < nr ? Ls[(r0 + i) + ms * (k + p)] : T(0);
+        B[p][i] = i < nc ? Ls[(c0 + i) + ms * (k + p)] : T(0);
+    }
+    __syncthreads();
+    // 256 threads, each a 4x4 micro-tile of the 64x64 tile
+    const int ti = (tid % 16) * 4, tj = (tid / 16) * 4;
+    T acc[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int c = 0; c < 4; ++c) acc[a][c] = T(0);
+    for (int p = 0; p < b; ++p) {
+        T av[4], bv[4];
+        for (int a = 0; a < 4; ++a) { av[a] = A[p][ti + a]; bv[a] = B[p][tj + a]; }
+        for (int a = 0; a < 4; ++a)
+            for (int c = 0; c < 4; ++c) acc[a][c] += av[a] * bv[c];
+    }
+    const bool inL = c0 < ns;
+    T *Cs = inL ? (L + lofs[s]) : (U + uofs[s]);
+    const int64_t ld = inL ? ms : ms - ns;
+    const int64_t roff = inL ? 0 : ns, coff = inL ? 0 : ns;
+    for (int a = 0; a < 4; ++a)
+        for (int c = 0; c < 4; ++c) {
+            const int i = ti + a, j = tj + c;
+            if (i < nr && j < nc && r0 + i >= c0 + j) Cs[(r0 + i - roff) + ld * (c0 + j - coff)] -= acc[a][c];
+        }
+}
+
+// ----------------------------------------------------------------- solves
+// vectors in elimination order, right-hand sides interleaved: Y[i * nrhs + r]
+template <class T>
+__global__ void __launch_bounds__(kBS) fwd_diag_kernel(const int32_t *__restrict__ level_sn, const int64_t *__restrict__ col0,
+                                                       const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
+                                                       const int64_t *__restrict__ lofs, const int64_t *__restrict__ relofs,
+                                                       const int32_t *__restrict__ relmap, const int32_t *__restrict__ child_ptr,
+                                                       const int32_t *__restrict__ child, const T *__restrict__ L,
+                                                       T *__restrict__ Y, T *__restrict__ W, int nrhs) {
+    const int s = level_sn[blockIdx.x];
+    const int64_t ns = ncol[s], ms = mrow[s], c0 = col0[s];
+    const T *Ls = L + lofs[s];
+    T *Ws = W + relofs[s] * nrhs;
+    T *Ys = Y + c0 * nrhs;
+    const int tid = threadIdx.x;
+    for (int64_t q = tid; q < (ms - ns) * nrhs; q += blockDim.x) Ws[q] = T(0);
+    __syncthreads();
+    for (int qc = child_ptr[s]; qc < child_ptr[s + 1]; ++qc) {
+        const int c = child[qc];
+        const int64_t nu = mrow[c] - ncol[c];
+        const int32_t *rel = relmap + relofs[c];
+        const T *Wc = W + relofs[c] * nrhs;
+        for (int64_t q = tid; q < nu * nrhs; q += blockDim.x) {
+            int64_t i = q / nrhs;
+            int r = (int)(q % nrhs);
+            int64_t pr = rel[i];
+            if (pr < ns) Ys[pr * nrhs + r] += Wc[q];
+            else Ws[(pr - ns) * nrhs + r] += Wc[q];
+        }
+        __syncthreads();
+    }
+    // forward substitution with L11, blocked by 64 columns
+    for (int64_t jb = 0; jb < ns; jb += kPanel) {
+        const int bsz = (int)min<int64_t>(kPanel, ns - jb);
+        for (int j = 0; j < bsz; ++j) {
+            const int64_t jj = jb + j;
+            const T d = Ls[jj + ms * jj];
+            for (int r = tid; r < nrhs; r += blockDim.x) Ys[jj * nrhs + r] /= d;
+            __syncthreads();
+            for (int64_t q = tid; q < (int64_t)(bsz - j - 1) * nrhs; q += blockDim.x) {
+                int64_t i = jj + 1 + q / nrhs;
+                int r = (int)(q % nrhs);
+                Ys[i * nrhs + r] -= Ls[i + ms * jj] * Ys[jj * nrhs + r];
+            }
+            __syncthreads();
+        }
+        for (int64_t q = tid; q < (ns - jb - bsz) * nrhs; q += blockDim.x) {
+            int64_t i = jb + bsz + q / nrhs;
+            int r = (int)(q % nrhs);
+            T v = T(0);
+            for (int j = 0; j < bsz; ++j) v += Ls[i + ms * (jb + j)] * Ys[(jb + j) * nrhs + r];
+            Ys[i * nrhs + r] -= v;
+        }
+        __syncthreads();
+    }
+}
+
+// W_s[rows] -= L21[rows, :] y_s, 64-row tiles
+template <class T>
+__global__ void __launch_bounds__(kBS) fwd_offdiag_kernel(const PanelTask *__restrict__ tiles, const int64_t *__restrict__ col0,
+                                                          const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
+                                                          const int64_t *__restrict__ lofs, const int64_t *__restrict__ relofs,
+                                                          const T *__restrict__ L, const T *__restrict__ Y, T *__restrict__ W,
+                                                          int nrhs) {
+    const PanelTask t = tiles[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s];
+    const int64_t r1 = min<int64_t>(t.r0 + kTile, ms);
+    const T *Ls = L + lofs[s];
+    const T *Ys = Y + col0[s] * nrhs;
+    T *Ws = W + relofs[s] * nrhs;
+    for (int64_t q = threadIdx.x; q < (r1 - t.r0) * nrhs; q += blockDim.x) {
+        int64_t i = t.r0 + q % (r1 - t.r0);
+        int r = (int)(q / (r1 - t.r0));
+        T v = T(0);
+        for (int64_t j = 0; j < ns; ++j) v += Ls[i + ms * j] * Ys[j * nrhs + r];
+        Ws[(i - ns) * nrhs + r] -= v;
+    }
+}
+
+// y_s[cols] -= L21[:, cols]' x[rows], 64-column tiles
+template <class T>
+__global__ void __launch_bounds__(kBS) bwd_offdiag_kernel(const PanelTask *__restrict__ tiles, const int64_t *__restrict__ col0,
+                                                          const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
+                                                          const int64_t *__restrict__ lofs, const int64_t *__restrict__ rofs,
+                                                          const int32_t *__restrict__ rows, const T *__restrict__ L,
+                                                          T *__restrict__ Y, int nrhs) {
+    __shared__ T part[4][kTile][4];
+    const PanelTask t = tiles[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s];
+    const int64_t c1 = min<int64_t>(t.r0 + kTile, ns);
+    const T *Ls = L + lofs[s];
+    const int32_t *rw = rows + rofs[s];
+    T *Ys = Y + col0[s] * nrhs;
+    const int tid = threadIdx.x;
+    const int lane = tid % 64, ph = tid / 64;    // 4 row phases
+    for (int rb = 0; rb < nrhs; rb += 4) {
+        const int nrb = min(4, nrhs - rb);
+        T acc[4] = {T(0), T(0), T(0), T(0)};
+        const int64_t j = t.r0 + lane;
+        if (j < c1) {
+            for (int64_t i = ns + ph; i < ms; i += 4) {
+                const T l = Ls[i + ms * j];
+                const int64_t g = rw[i];
+                for (int r = 0; r < nrb; ++r) acc[r] += l * Y[g * nrhs + rb + r];
+            }
+        }
+        for (int r = 0; r < 4; ++r) part[ph][lane][r] = acc[r];
+        __syncthreads();
+        if (ph == 0 && j < c1)
+            for (int r = 0; r < nrb; ++r)
+                Ys[j * nrhs + rb + r] -= ((part[0][lane][r] + part[1][lane][r]) + part[2][lane][r]) + part[3][lane][r];
+        __syncthreads();
+    }
+}
+
+// back substitution with L11'
+template <class T>
+__global__ void __launch_bounds__(kBS) bwd_diag_kernel(const int32_t *__restrict__ level_sn, const int64_t *__restrict__ col0,
+                                                       const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
+                                                       const int64_t *__restrict__ lofs, const T *__restrict__ L,
+                                                       T *__restrict__ Y, int nrhs) {
+    const int s = level_sn[blockIdx.x];
+    const int64_t ns = ncol[s], ms = mrow[s], c0 = col0[s];
+    const T *Ls = L + lofs[s];
+    T *Ys = Y + c0 * nrhs;
+    const int tid = threadIdx.x;
+    const int64_t nblk = (ns + kPanel - 1) / kPanel;
+    for (int64_t bi = nblk - 1; bi >= 0; --bi) {
+        const int64_t jb = bi * kPanel;
+        const int bsz = (int)min<int64_t>(kPanel, ns - jb);
+        // contributions of already-solved later columns within L11
+        for (int64_t q = tid; q < (int64_t)bsz * nrhs; q += blockDim.x) {
+            int64_t j = jb + q / nrhs;
+            int r = (int)(q % nrhs);
+            T v = T(0);
+            for (int64_t i = jb + bsz; i < ns; ++i) v += Ls[i + ms * j] * Ys[i * nrhs + r];
+            Ys[j * nrhs + r] -= v;
+        }
+        __syncthreads();
+        for (int j = bsz - 1; j >= 0; --j) {
+            const int64_t jj = jb + j;
+            const T d = Ls[jj + ms * jj];
+            for (int r = tid; r < nrhs; r += blockDim.x) Ys[jj * nrhs + r] /= d;
+            __syncthreads();
+            for (int64_t q = tid; q < (int64_t)j * nrhs; q += blockDim.x) {
+                int64_t i = jb + q / nrhs;
+                int r = (int)(q % nrhs);
+                Ys[i * nrhs + r] -= Ls[jj + ms * i] * Ys[jj * nrhs + r];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ----------------------------------------------------------------- refinement helpers
+// Bp[i*nrhs + r] = b[perm[i] + ndof*r]
+__global__ void gather_rhs_kernel(int64_t n, int nrhs, int64_t ndof, const int32_t *__restrict__ perm,
+                                  const double *__restrict__ b, double *__restrict__ Bp) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n * nrhs; q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = q / nrhs;
+        int r = (int)(q % nrhs);
+        Bp[q] = b[perm[i] + ndof * r];
+    }
+}
+
+template <class T>
+__global__ void to_t_kernel(int64_t n, const double *__restrict__ a, T *__restrict__ o) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) o[q] = (T)a[q];
+}
+
+// X += (double) Y, and scatter X to the global interleaved vector xg (ndof x nrhs)
+template <class T>
+__global__ void accumulate_kernel(int64_t n, int nrhs, const int32_t *__restrict__ perm, const T *__restrict__ Y,
+                                  double *__restrict__ X, double *__restrict__ xg, const int32_t *__restrict__ active) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n * nrhs; q += (int64_t)gridDim.x * blockDim.x) {
+        int r = (int)(q % nrhs);
+        if (!active[r]) continue;
+        double v = X[q] + (double)Y[q];
+        X[q] = v;
+        xg[(int64_t)perm[q / nrhs] * nrhs + r] = v;
+    }
+}
+
+// R = Bp - K_ff x (FP64); xg is zero on fixed DOFs so the full CSR row can be used
+// after masking fixed columns through xg.
+__global__ void residual_kernel(int64_t n, int nrhs, const int32_t *__restrict__ perm, const int64_t *__restrict__ rowptr,
+                                const int32_t *__restrict__ col, const double *__restrict__ val,
+                                const double *__restrict__ xg, const double *__restrict__ Bp, double *__restrict__ R) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = perm[i];
+        double acc[16];
+        for (int r = 0; r < nrhs; ++r) acc[r] = 0.0;
+        for (int64_t p = rowptr[g]; p < rowptr[g + 1]; ++p) {
+            const double kv = val[p];
+            const int64_t c = col[p];
+            for (int r = 0; r < nrhs; ++r) acc[r] += kv * xg[c * nrhs + r];
+        }
+        for (int r = 0; r < nrhs; ++r) R[i * nrhs + r] = Bp[i * nrhs + r] - acc[r];
+    }
+}
+
+// per-block partial sums of squares, per RHS; then a fixed-order final reduction
+__global__ void sumsq_partial_kernel(int64_t n, int nrhs, const double *__restrict__ v, double *__restrict__ part) {
+    __shared__ double sh[kBS];
+    for (int r = 0; r < nrhs; ++r) {
+        double a = 0.0;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+            double x = v[i * nrhs + r];
+            a += x * x;
+        }
+        sh[threadIdx.x] = a;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(int64_t)r * gridDim.x + blockIdx.x] = sh[0];
+        __syncthreads();
+    }
+}
+
+__global__ void sumsq_final_kernel(int nblk, int nrhs, const double *__restrict__ part, double *__restrict__ out) {
+    int r = threadIdx.x;
+    if (r >= nrhs) return;
+    double a = 0.0;
+    for (int b = 0; b < nblk; ++b) a += part[(int64_t)r * nblk + b];
+    out[r] = a;
+}
+
+// global column-major x (ndof x nrhs) from the interleaved xg
+__global__ void export_kernel(int64_t ndof, int nrhs, const double *__restrict__ xg, double *__restrict__ x) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ndof * nrhs; q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t g = q % ndof;
+        int r = (int)(q / ndof);
+        x[q] = xg[g * nrhs + r];
+    }
+}
+
+__global__ void diag_ratio_kernel(int64_t n, const int32_t *__restrict__ perm, const int32_t *__restrict__ pos2sn,
+                                  const int64_t *__restrict__ col0, const int64_t *__restrict__ mrow,
+                                  const int64_t *__restrict__ lofs, const int32_t *__restrict__ diagpos,
+                                  const double *__restrict__ val, const void *__restrict__ L, int is64,
+                                  double *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int s = pos2sn[i];
+        const int64_t j = i - col0[s];
+        const int64_t off = lofs[s] + j + mrow[s] * j;
+        const double l = is64 ? ((const double *)L)[off] : (double)((const float *)L)[off];
+        const int32_t dp = diagpos[perm[i]];
+        const double kd = dp >= 0 ? fabs(val[dp]) : 0.0;
+        out[i] = kd > 1e-30 ? (l * l) / kd : 1.0;
+    }
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBS - 1) / kBS, 65536)); }
+
+template <class T>
+hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, T *L, T *U) {
+    hipStream_t st = ctx->stream;
+    const SymbolicHost &H = S->H;
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(H.L_entries)), dim3(kBS), 0, st, L, H.L_entries);
+    if (H.U_entries) hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(H.U_entries)), dim3(kBS), 0, st, U, H.U_entries);
+    int64_t na = (int64_t)H.amap_src.size();
+    if (na)
+        hipLaunchKernelGGL(scatter_kernel<T>, dim3(grid_for(na)), dim3(kBS), 0, st, na, S->d_amap_src, S->d_amap_dst,
+                           K->d_val, L);
+    const int64_t nlev = (int64_t)H.level_ptr.size() - 1;
+    for (int64_t l = 0; l < nlev; ++l) {
+        int64_t ne = H.ea_off[l + 1] - H.ea_off[l];
+        if (ne)
+            hipLaunchKernelGGL(extend_add_kernel<T>, dim3((unsigned)ne), dim3(kBS), 0, st, S->d_etasks + H.ea_off[l],
+                               S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, S->d_relofs, S->d_relmap, S->d_child_ptr,
+                               S->d_child, L, U);
+        for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
+            int64_t nd = H.pstep_mid[stp] - H.pstep_off[stp];
+            int64_t nr = H.pstep_off[stp + 1] - H.pstep_mid[stp];
+            int64_t nu = H.ustep_off[stp + 1] - H.ustep_off[stp];
+            if (nd)
+                hipLaunchKernelGGL(panel_kernel<T>, dim3((unsigned)nd), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp], 1,
+                                   S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status);
+            if (nr)
+                hipLaunchKernelGGL(panel_kernel<T>, dim3((unsigned)nr), dim3(kBS), 0, st, S->d_ptasks + H.pstep_mid[stp], 0,
+                                   S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status);
+            if (nu)
+                hipLaunchKernelGGL(update_kernel<T>, dim3((unsigned)nu), dim3(kBS), 0, st, S->d_utasks + H.ustep_off[stp],
+                                   S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, L, U);
+        }
+    }
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t solve_t(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, int nrhs) {
+    hipStream_t st = ctx->stream;
+    const SymbolicHost &H = S->H;
+    const int64_t nlev = (int64_t)H.level_ptr.size() - 1;
+    for (int64_t l = 0; l < nlev; ++l) {
+        int64_t nsn = H.level_ptr[l + 1] - H.level_ptr[l];
+        hipLaunchKernelGGL(fwd_diag_kernel<T>, dim3((unsigned)nsn), dim3(kBS), 0, st, S->d_level_sn + H.level_ptr[l],
+                           S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, S->d_relmap, S->d_child_ptr, S->d_child,
+                           L, Y, W, nrhs);
+        int64_t nt = H.fwd_off[l + 1] - H.fwd_off[l];
+        if (nt)
+            hipLaunchKernelGGL(fwd_offdiag_kernel<T>, dim3((unsigned)nt), dim3(kBS), 0, st, S->d_fwd + H.fwd_off[l],
+                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L, Y, W, nrhs);
+    }
+    for (int64_t l = nlev - 1; l >= 0; --l) {
+        int64_t nt = H.bwd_off[l + 1] - H.bwd_off[l];
+        if (nt)
+            hipLaunchKernelGGL(bwd_offdiag_kernel<T>, dim3((unsigned)nt), dim3(kBS), 0, st, S->d_bwd + H.bwd_off[l],
+                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_rofs, S->d_rows, L, Y, nrhs);
+        int64_t nsn = H.level_ptr[l + 1] - H.level_ptr[l];
+        hipLaunchKernelGGL(bwd_diag_kernel<T>, dim3((unsigned)nsn), dim3(kBS), 0, st, S->d_level_sn + H.level_ptr[l],
+                           S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, Y, nrhs);
+    }
+    return hipGetLastError();
+}
+
+template <class V>
+hipError_t up(jfem_symbolic *S, V **dst, const std::vector<V> &src, hipStream_t st) {
+    *dst = nullptr;
+    if (src.empty()) return hipSuccess;
+    hipError_t e = hipMalloc((void **)dst, src.size() * sizeof(V));
+    if (e != hipSuccess) return e;
+    S->allocs.push_back(*dst);
+    return hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice, st);
+}
+
+hipError_t ensure_scratch(jfem_ctx *ctx, size_t bytes) {
+    if (ctx->scratch_bytes >= bytes) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return e;
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    e = hipMalloc(&ctx->scratch, bytes);
+    if (e != hipSuccess) return e;
+    ctx->scratch_bytes = bytes;
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *mesh, const uint8_t *fixed,
+                         jfem_symbolic **out, jfem_symbolic_info *info) {
+    *out = nullptr;
+    if (!mesh || mesh->n_nodes * 6 != P->ndof) return fail(ctx, JFEM_ERR_BAD_ARG, "mesh/pattern mismatch");
+    jfem_symbolic *S = new jfem_symbolic();
+    S->device = ctx->device;
+    S->P = P;
+    std::string err;
+    if (build_symbolic(P->ndof, P->rowptr.data(), P->col.data(), mesh->xyz.data(), fixed, S->H, err)) {
+        delete S;
+        return fail(ctx, JFEM_ERR_BAD_ARG, "symbolic: " + err);
+    }
+    const SymbolicHost &H = S->H;
+    if (H.L_entries >= (int64_t)1 << 40) { delete S; return fail(ctx, JFEM_ERR_OOM, "factor too large"); }
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipSuccess;
+    std::vector<int32_t> level_sn32(H.level_sn.begin(), H.level_sn.end());
+    if (e == hipSuccess) e = up(S, &S->d_col0, H.col0, st);
+    if (e == hipSuccess) e = up(S, &S->d_ncol, H.ncol, st);
+    if (e == hipSuccess) e = up(S, &S->d_m, H.m, st);
+    if (e == hipSuccess) e = up(S, &S->d_lofs, H.lofs, st);
+    if (e == hipSuccess) e = up(S, &S->d_uofs, H.uofs, st);
+    if (e == hipSuccess) e = up(S, &S->d_rofs, H.rofs, st);
+    if (e == hipSuccess) e = up(S, &S->d_relofs, H.relofs, st);
+    if (e == hipSuccess) e = up(S, &S->d_rows, H.rows, st);
+    if (e == hipSuccess) e = up(S, &S->d_relmap, H.relmap, st);
+    if (e == hipSuccess) e = up(S, &S->d_child_ptr, H.child_ptr, st);
+    if (e == hipSuccess) e = up(S, &S->d_child, H.child, st);
+    if (e == hipSuccess) e = up(S, &S->d_perm, H.perm, st);
+    if (e == hipSuccess) e = up(S, &S->d_level_sn, level_sn32, st);
+    if (e == hipSuccess) e = up(S, &S->d_amap_src, H.amap_src, st);
+    if (e == hipSuccess) e = up(S, &S->d_amap_dst, H.amap_dst, st);
+    if (e == hipSuccess) e = up(S, &S->d_ptasks, H.ptasks, st);
+    if (e == hipSuccess) e = up(S, &S->d_utasks, H.utasks, st);
+    if (e == hipSuccess) e = up(S, &S->d_etasks, H.etasks, st);
+    if (e == hipSuccess) e = up(S, &S->d_fwd, H.fwd_tiles, st);
+    if (e == hipSuccess) e = up(S, &S->d_bwd, H.bwd_tiles, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        jfem_symbolic_free(S);
+        return fail(ctx, e == hipErrorOutOfMemory ? JFEM_ERR_OOM : JFEM_ERR_HIP, hipGetErrorString(e));
+    }
+    if (info) {
+        info->n_free = H.n;
+        info->n_supernodes = H.ns;
+        info->n_levels = (int64_t)H.level_ptr.size() - 1;
+        info->nnz_L = H.L_entries;
+        info->flops = H.flops;
+        info->max_front = H.max_front;
+        info->update_entries = H.U_entries;
+    }
+    *out = S;
+    return JFEM_OK;
+}
+
+int jfem_symbolic_free(jfem_symbolic *S) {
+    if (!S) return JFEM_OK;
+    (void)hipSetDevice(S->device);
+    for (void *p : S->allocs) (void)hipFree(p);
+    delete S;
+    return JFEM_OK;
+}
+
+static int do_numeric(jfem_ctx *ctx, jfem_factor *F, const jfem_matrix *K, int64_t *bad_column) {
+    const jfem_symbolic *S = F->S;
+    if (K->P != S->P) return fail(ctx, JFEM_ERR_BAD_ARG, "matrix and symbolic use different patterns");
+    if (bad_column) *bad_column = -1;
+    JFEM_CHECK_HIP(ctx, launch_reset_status(ctx->stream, ctx->d_status));
+    timer_start(ctx);
+    hipError_t e = F->precision == 64
+                       ? numeric<double>(ctx, S, K, (double *)F->d_L, (double *)F->d_U)
+                       : numeric<float>(ctx, S, K, (float *)F->d_L, (float *)F->d_U);
+    JFEM_CHECK_HIP(ctx, e);
+    timer_stop(ctx, FAM_FACTOR, 1);
+    int64_t st[2];
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (st[1] != 0) {
+        int64_t g = (st[0] >= 0 && st[0] < S->H.n) ? S->H.perm[st[0]] : -1;
+        if (bad_column) *bad_column = g;
+        return fail(ctx, JFEM_ERR_NOT_SPD, "non-positive pivot at DOF " + std::to_string(g));
+    }
+    return JFEM_OK;
+}
+
+int jfem_factor_create(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, int precision, jfem_factor **out,
+                       int64_t *bad_column) {
+    *out = nullptr;
+    if (precision != 64 && precision != 32) return fail(ctx, JFEM_ERR_BAD_ARG, "precision must be 32 or 64");
+    jfem_factor *F = new jfem_factor();
+    F->device = ctx->device;
+    F->S = S;
+    F->precision = precision;
+    const size_t es = precision == 64 ? 8 : 4;
+    hipError_t e = hipMalloc(&F->d_L, std::max<int64_t>(S->H.L_entries, 1) * es);
+    if (e == hipSuccess) e = hipMalloc(&F->d_U, std::max<int64_t>(S->H.U_entries, 1) * es);
+    if (e != hipSuccess) {
+        jfem_factor_free(F);
+        return fail(ctx, JFEM_ERR_OOM, "factor storage");
+    }
+    F->bytes = (S->H.L_entries + S->H.U_entries) * (int64_t)es;
+    int st = do_numeric(ctx, F, K, bad_column);
+    if (st != JFEM_OK) {
+        jfem_factor_free(F);
+        return st;
+    }
+    *out = F;
+    return JFEM_OK;
+}
+
+int jfem_refactor(jfem_ctx *ctx, jfem_factor *F, const jfem_matrix *K, int64_t *bad_column) {
+    return do_numeric(ctx, F, K, bad_column);
+}
+
+int jfem_factor_free(jfem_factor *F) {
+    if (!F) return JFEM_OK;
+    (void)hipSetDevice(F->device);
+    if (F->d_L) (void)hipFree(F->d_L);
+    if (F->d_U) (void)hipFree(F->d_U);
+    delete F;
+    return JFEM_OK;
+}
+
+int64_t jfem_factor_device_bytes(const jfem_factor *F) { return F ? F->bytes : 0; }
+
+int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, double threshold,
+                             int64_t *dofs_out, int64_t cap, int64_t *count) {
+    const jfem_symbolic *S = F->S;
+    const SymbolicHost &H = S->H;
+    *count = 0;
+    if (H.n == 0) return JFEM_OK;
+    std::vector<int32_t> pos2sn(H.n), diagpos(K->P->diagpos);
+    for (int64_t s = 0; s < H.ns; ++s)
+        for (int64_t j = 0; j < H.ncol[s]; ++j) pos2sn[H.col0[s] + j] = (int32_t)s;
+    int32_t *d_p2s = nullptr, *d_dp = nullptr;
+    double *d_out = nullptr;
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_p2s, H.n * sizeof(int32_t)));
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_dp, diagpos.size() * sizeof(int32_t)));
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_out, H.n * sizeof(double)));
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(d_p2s, pos2sn.data(), H.n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(d_dp, diagpos.data(), diagpos.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(diag_ratio_kernel, dim3(grid_for(H.n)), dim3(kBS), 0, ctx->stream, H.n, S->d_perm, d_p2s, S->d_col0,
+                       S->d_m, S->d_lofs, d_dp, K->d_val, F->d_L, F->precision == 64 ? 1 : 0, d_out);
+    std::vector<double> ratio(H.n);
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(ratio.data(), d_out, H.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(d_p2s); (void)hipFree(d_dp); (void)hipFree(d_out);
+    int64_t c = 0;
+    for (int64_t i = 0; i < H.n; ++i)
+        if (ratio[i] < threshold) {
+            if (c < cap) dofs_out[c] = H.perm[i];
+            ++c;
+        }
+    *count = c;
+    return JFEM_OK;
+}
+
+int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_dev,
+                   double *x_dev, double tol, int max_refine, int32_t *iters, double *relres) {
+    const jfem_symbolic *S = F->S;
+    const SymbolicHost &H = S->H;
+    const jfem_pattern *P = K->P;
+    if (K->P != S->P) return fail(ctx, JFEM_ERR_BAD_ARG, "matrix and factor use different patterns");
+    if (nrhs < 1 || nrhs > 16) return fail(ctx, JFEM_ERR_BAD_ARG, "nrhs must be in 1..16");
+    const int64_t n = H.n, ndof = P->ndof;
+    const size_t es = F->precision == 64 ? 8 : 4;
+    const int nblk = 256;
+    // scratch layout
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    size_t oY = take(n * nrhs * es), oW = take(std::max<int64_t>(H.upd_rows, 1) * nrhs * es);
+    size_t oX = take(n * nrhs * 8), oR = take(n * nrhs * 8), oB = take(n * nrhs * 8), oXg = take(ndof * nrhs * 8);
+    size_t oPart = take((size_t)nblk * nrhs * 8), oNrm = take(16 * 8), oAct = take(16 * 4);
+    JFEM_CHECK_HIP(ctx, ensure_scratch(ctx, off));
+    char *base = (char *)ctx->scratch;
+    void *Y = base + oY, *W = base + oW;
+    double *X = (double *)(base + oX), *R = (double *)(base + oR), *Bp = (double *)(base + oB), *xg = (double *)(base + oXg);
+    double *part = (double *)(base + oPart), *nrm = (double *)(base + oNrm);
+    int32_t *act = (int32_t *)(base + oAct);
+    hipStream_t st = ctx->stream;
+    if (n > 0) hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, X, n * nrhs);
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, xg, ndof * nrhs);
+    if (n > 0) {
+        hipLaunchKernelGGL(gather_rhs_kernel, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n, nrhs, ndof, S->d_perm, b_dev, Bp);
+        JFEM_CHECK_HIP(ctx, hipMemcpyAsync(R, Bp, n * nrhs * 8, hipMemcpyDeviceToDevice, st));
+    }
+    double bn[16], rn[16];
+    int32_t active[16];
+    for (int r = 0; r < nrhs; ++r) { active[r] = 1; iters[r] = 0; relres[r] = 0.0; }
+    auto norms = [&](const double *v, double *outn) -> hipError_t {
+        hipLaunchKernelGGL(sumsq_partial_kernel, dim3(nblk), dim3(kBS), 0, st, n, nrhs, v, part);
+        hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), dim3(64), 0, st, nblk, nrhs, part, nrm);
+        hipError_t e = hipMemcpyAsync(outn, nrm, nrhs * 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        return e;
+    };
+    if (n > 0) {
+        JFEM_CHECK_HIP(ctx, norms(Bp, bn));
+        for (int r = 0; r < nrhs; ++r) bn[r] = std::sqrt(bn[r]);
+    }
+    bool converged = n == 0;
+    for (int it = 0; n > 0 && it <= max_refine; ++it) {
+        JFEM_CHECK_HIP(ctx, hipMemcpyAsync(act, active, nrhs * 4, hipMemcpyHostToDevice, st));
+        timer_start(ctx);
+        if (F->precision == 64) {
+            JFEM_CHECK_HIP(ctx, hipMemcpyAsync(Y, R, n * nrhs * 8, hipMemcpyDeviceToDevice, st));
+            JFEM_CHECK_HIP(ctx, solve_t<double>(ctx, S, (const double *)F->d_L, (double *)Y, (double *)W, nrhs));
+            hipLaunchKernelGGL(accumulate_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n, nrhs, S->d_perm,
+                               (const double *)Y, X, xg, act);
+        } else {
+            hipLaunchKernelGGL(to_t_kernel<float>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n * nrhs, R, (float *)Y);
+            JFEM_CHECK_HIP(ctx, solve_t<float>(ctx, S, (const float *)F->d_L, (float *)Y, (float *)W, nrhs));
+            hipLaunchKernelGGL(accumulate_kernel<float>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n, nrhs, S->d_perm,
+                               (const float *)Y, X, xg, act);
+        }
+        timer_stop(ctx, FAM_SOLVE, 1);
+        timer_start(ctx);
+        hipLaunchKernelGGL(residual_kernel, dim3(grid_for(n)), dim3(kBS), 0, st, n, nrhs, S->d_perm, P->d_rowptr, P->d_col,
+                           K->d_val, xg, Bp, R);
+        timer_stop(ctx, FAM_SPMV, 1);
+        JFEM_CHECK_HIP(ctx, norms(R, rn));
+        converged = true;
+        for (int r = 0; r < nrhs; ++r) {
+            double rr = std::sqrt(rn[r]) / std::max(bn[r], 1e-30);
+            if (active[r]) { relres[r] = rr; iters[r] = it; }
+            if (active[r] && !(rr <= tol)) converged = false;
+            else active[r] = 0;
+        }
+        if (converged) break;
+    }
+    hipLaunchKernelGGL(export_kernel, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, ndof, nrhs, xg, x_dev);
+    JFEM_CHECK_HIP(ctx, hipGetLastError());
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+    if (!converged) return fail(ctx, JFEM_ERR_NOT_CONVERGED, "refinement did not reach the requested residual");
+    return JFEM_OK;
+}
+
+int jfem_solve(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_host, double *x_host,
+               double tol, int max_refine, int32_t *iters, double *relres) {
+    const int64_t ndof = K->P->ndof;
+    double *d_b = nullptr, *d_x = nullptr;
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_b, std::max<int64_t>(ndof * nrhs, 1) * 8));
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_x, std::max<int64_t>(ndof * nrhs, 1) * 8));
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(d_b, b_host, ndof * nrhs * 8, hipMemcpyHostToDevice, ctx->stream));
+    int st = jfem_solve_dev(ctx, F, K, nrhs, d_b, d_x, tol, max_refine, iters, relres);
+    hipError_t e = hipMemcpyAsync(x_host, d_x, ndof * nrhs * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_b);
+    (void)hipFree(d_x);
+    if (e != hipSuccess) return fail(ctx, JFEM_ERR_HIP, hipGetErrorString(e));
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Internal types shared by the HIP translation units of libjfem_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/jfem_hip.h"
+
+#define JFEM_CHECK_HIP(ctx, call)                                                   \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            return jfem::fail(ctx, e_ == hipErrorOutOfMemory ? JFEM_ERR_OOM : JFEM_ERR_HIP, \
+                              std::string(#call) + ": " + hipGetErrorString(e_));   \
+        }                                                                           \
+    } while (0)
+
+namespace jfem {
+
+enum Family { FAM_ELEM = 0, FAM_ASSEMBLY = 1, FAM_FACTOR = 2, FAM_SOLVE = 3, FAM_SPMV = 4, FAM_COUNT = 8 };
+
+}  // namespace jfem
+
+struct jfem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool timing = false;
+    double ms[jfem::FAM_COUNT] = {0};
+    int64_t launches[jfem::FAM_COUNT] = {0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device status word shared by kernels that report element / pivot failures
+    int64_t *d_status = nullptr;   // [0] = first bad index (INT64_MAX if none), [1] = code
+    // scratch for solves
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+struct jfem_mesh {
+    int device = 0;
+    int64_t n_nodes = 0, nq = 0, nt = 0, ntet = 0;
+    double k6rot = 100.0;
+    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
+    int32_t *d_q = nullptr, *d_t = nullptr, *d_tet = nullptr;   // column-major conn
+    double *d_qp = nullptr, *d_tp = nullptr, *d_tetp = nullptr; // column-major props
+    // host copies needed by the symbolic phases
+    std::vector<double> xyz;        // n_nodes x 3, row-major (x,y,z per node)
+    std::vector<int32_t> q, t, tet; // column-major conn
+};
+
+struct jfem_pattern {
+    int device = 0;
+    int64_t ndof = 0, nnz = 0, ntrip = 0, n_nodes = 0;
+    int64_t nq = 0, nt = 0, ntet = 0;
+    std::vector<int64_t> rowptr;
+    std::vector<int32_t> col;
+    int64_t *d_rowptr = nullptr;
+    int32_t *d_col = nullptr;
+    int64_t *d_seg_ptr = nullptr;   // nnz + 1
+    int32_t *d_seg_src = nullptr;   // ntrip: index into the element-matrix buffer
+    int64_t *d_nodeblk = nullptr;   // n_nodes * 36: CSR position of node diagonal 6x6 block (-1 = absent)
+    std::vector<int64_t> nodeblk;
+    std::vector<int32_t> diagpos;   // ndof: CSR position of (i,i) or -1
+};
+
+struct jfem_matrix {
+    int device = 0;
+    const jfem_pattern *P = nullptr;
+    double *d_val = nullptr;
+    double *d_Ke = nullptr;         // element matrices (ntrip doubles), element-major
+};
+
+namespace jfem {
+int fail(jfem_ctx *ctx, int code, const std::string &msg);
+void timer_start(jfem_ctx *ctx);
+void timer_stop(jfem_ctx *ctx, int family, int64_t launches);
+
+// element generation (elements.hip)
+hipError_t launch_q4_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status);
+hipError_t launch_t3_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status);
+hipError_t launch_tet10(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status);
+hipError_t launch_q4_local(hipStream_t s, int64_t n, const double *lc, const double *Cm,
+                           const double *Cb, const double *Cs, const double *h,
+                           const double *k6, const double *ds, const int32_t *flags,
+                           double *Ke, int64_t *status);
+hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *Ke, double *val);
+hipError_t launch_gather_nodeblk(hipStream_t s, const jfem_pattern *P, const double *val, double *out);
+hipError_t launch_reset_status(hipStream_t s, int64_t *status);
+}  // namespace jfem

@@ -1,0 +1,508 @@
+// Symbolic analysis of K_ff for the supernodal multifrontal Cholesky.
+//
+//  1. grid points with free DOFs are the supervariables (their DOFs share one
+//     sparsity pattern); the grid-point graph comes from the CSR pattern of K;
+//  2. fill-reducing ordering: geometric nested dissection on the grid coordinates
+//     (median plane along the longest extent, one-sided vertex separator);
+//  3. elimination tree, grid-point column structures, fundamental supernodes and
+//     relaxed amalgamation (merge a child into its parent when the merged front
+//     adds few explicit zeros);
+//  4. DOF-level fronts: column ranges, row lists, storage offsets, the K -> front
+//     scatter map (upper triangle of K in original numbering, as
+//     cholesky(Symmetric(K_ff)) reads it, src/solver/boundary_conditions.jl:1660),
+//     child -> parent relative maps, level schedule and the task lists of the
+//     batched panel/update/extend-add/solve kernels.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "symbolic.h"
+
+namespace jfem {
+
+namespace {
+
+struct Graph {
+    std::vector<int64_t> ptr;
+    std::vector<int32_t> adj;
+};
+
+// geometric nested dissection over the active grid points
+struct ND {
+    const double *xyz;
+    const Graph &g;
+    std::vector<int32_t> mark;   // 0 = outside, 1 = side A, 2 = side B
+    std::vector<int32_t> out;
+    int leaf;
+    ND(const double *x, const Graph &gr, int64_t nn, int lf) : xyz(x), g(gr), mark(nn, 0), leaf(lf) {}
+
+    void run(std::vector<int32_t> set) {
+        if ((int64_t)set.size() <= leaf) {
+            out.insert(out.end(), set.begin(), set.end());
+            return;
+        }
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int32_t v : set)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], xyz[3 * v + a]);
+                hi[a] = std::max(hi[a], xyz[3 * v + a]);
+            }
+        int ax = 0;
+        for (int a = 1; a < 3; ++a)
+            if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
+        std::stable_sort(set.begin(), set.end(), [&](int32_t a, int32_t b) { return xyz[3 * a + ax] < xyz[3 * b + ax]; });
+        size_t h = set.size() / 2;
+        // keep points with equal coordinate on one side
+        while (h < set.size() && h > 0 && xyz[3 * set[h] + ax] == xyz[3 * set[h - 1] + ax]) ++h;
+        if (h == set.size()) {
+            h = set.size() / 2;
+            while (h > 0 && xyz[3 * set[h] + ax] == xyz[3 * set[h - 1] + ax]) --h;
+        }
+        if (h == 0 || h == set.size()) {
+            out.insert(out.end(), set.begin(), set.end());
+            return;
+        }
+        for (size_t k = 0; k < set.size(); ++k) mark[set[k]] = k < h ? 1 : 2;
+        std::vector<int32_t> A, B, SA, SB;
+        for (size_t k = 0; k < set.size(); ++k) {
+            int32_t v = set[k];
+            int other = k < h ? 2 : 1;
+            bool sep = false;
+            for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p)
+                if (mark[g.adj[p]] == other) { sep = true; break; }
+            if (k < h) (sep ? SA : A).push_back(v);
+            else (sep ? SB : B).push_back(v);
+        }
+        for (int32_t v : set) mark[v] = 0;
+        std::vector<int32_t> S;
+        if (SA.size() <= SB.size()) {
+            S = SA;
+            B.insert(B.end(), SB.begin(), SB.end());
+        } else {
+            S = SB;
+            A.insert(A.end(), SA.begin(), SA.end());
+        }
+        if (A.empty() || B.empty()) {
+            out.insert(out.end(), set.begin(), set.end());
+            return;
+        }
+        std::stable_sort(B.begin(), B.end(), [&](int32_t a, int32_t b) { return xyz[3 * a + ax] < xyz[3 * b + ax]; });
+        run(std::move(A));
+        run(std::move(B));
+        out.insert(out.end(), S.begin(), S.end());
+    }
+};
+
+}  // namespace
+
+int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, const double *xyz, const uint8_t *fixed,
+                   SymbolicHost &S, std::string &err) {
+    const int64_t nn = ndof / 6;
+    S = SymbolicHost();
+    S.ndof = ndof;
+    // ---- supervariables: grid points with free DOFs
+    std::vector<int32_t> wdof(nn, 0);
+    for (int64_t g = 0; g < ndof; ++g)
+        if (!fixed[g]) wdof[g / 6]++;
+    std::vector<int32_t> act;  // active grid points
+    for (int64_t a = 0; a < nn; ++a)
+        if (wdof[a]) act.push_back((int32_t)a);
+    const int64_t na = (int64_t)act.size();
+    S.n = 0;
+    for (int64_t a = 0; a < nn; ++a) S.n += wdof[a];
+    if (na == 0) {
+        S.level_ptr.assign(1, 0);
+        S.step_ptr.assign(1, 0);
+        S.pstep_off.assign(1, 0);
+        S.ustep_off.assign(1, 0);
+        S.ea_off.assign(1, 0);
+        S.fwd_off.assign(1, 0);
+        S.bwd_off.assign(1, 0);
+        S.iperm.assign(ndof, -1);
+        return 0;
+    }
+    std::vector<int32_t> aidx(nn, -1);
+    for (int64_t i = 0; i < na; ++i) aidx[act[i]] = (int32_t)i;
+    // ---- grid-point graph among active points (free-free couplings)
+    Graph G;
+    G.ptr.assign(na + 1, 0);
+    {
+        std::vector<int32_t> stamp(na, -1);
+        std::vector<int32_t> tmp;
+        for (int64_t i = 0; i < na; ++i) {
+            int64_t a = act[i];
+            for (int d = 0; d < 6; ++d) {
+                int64_t gr = 6 * a + d;
+                if (fixed[gr]) continue;
+                for (int64_t p = rowptr[gr]; p < rowptr[gr + 1]; ++p) {
+                    int32_t c = col[p];
+                    if (fixed[c]) continue;
+                    int32_t b = aidx[c / 6];
+                    if (b == i || stamp[b] == i) continue;
+                    stamp[b] = (int32_t)i;
+                    G.adj.push_back(b);
+                }
+            }
+            G.ptr[i + 1] = (int64_t)G.adj.size();
+        }
+    }
+    // ---- nested dissection ordering
+    std::vector<double> axyz(3 * na);
+    for (int64_t i = 0; i < na; ++i)
+        for (int c = 0; c < 3; ++c) axyz[3 * i + c] = xyz[3 * act[i] + c];
+    std::vector<int32_t> order;
+    {
+        ND nd(axyz.data(), G, na, 24);
+        std::vector<int32_t> all(na);
+        std::iota(all.begin(), all.end(), 0);
+        nd.run(std::move(all));
+        order = std::move(nd.out);
+    }
+    if ((int64_t)order.size() != na) {
+        err = "ordering lost grid points";
+        return 1;
+    }
+    // ---- elimination tree (Liu) on the permuted grid-point graph
+    std::vector<int32_t> pos(na);
+    for (int64_t k = 0; k < na; ++k) pos[order[k]] = (int32_t)k;
+    std::vector<int32_t> par(na, -1), anc(na, -1);
+    for (int64_t k = 0; k < na; ++k) {
+        int32_t v = order[k];
+        for (int64_t p = G.ptr[v]; p < G.ptr[v + 1]; ++p) {
+            int32_t i = pos[G.adj[p]];
+            while (i != -1 && i < k) {
+                int32_t nx = anc[i];
+                anc[i] = (int32_t)k;
+                if (nx == -1) { par[i] = (int32_t)k; break; }
+                i = nx;
+            }
+        }
+    }
+    // postorder
+    {
+        std::vector<int32_t> head(na, -1), next(na, -1), post;
+        post.reserve(na);
+        for (int64_t j = na - 1; j >= 0; --j)
+            if (par[j] != -1) { next[j] = head[par[j]]; head[par[j]] = (int32_t)j; }
+        std::vector<int32_t> stack;
+        for (int64_t r = 0; r < na; ++r) {
+            if (par[r] != -1) continue;
+            stack.push_back((int32_t)r);
+            while (!stack.empty()) {
+                int32_t p = stack.back();
+                int32_t c = head[p];
+                if (c == -1) { post.push_back(p); stack.pop_back(); }
+                else { head[p] = next[c]; stack.push_back(c); }
+            }
+        }
+        std::vector<int32_t> norder(na), npar(na);
+        std::vector<int32_t> newpos(na);
+        for (int64_t k = 0; k < na; ++k) newpos[post[k]] = (int32_t)k;
+        for (int64_t k = 0; k < na; ++k) {
+            norder[k] = order[post[k]];
+            npar[k] = par[post[k]] == -1 ? -1 : newpos[par[post[k]]];
+        }
+        order.swap(norder);
+        par.swap(npar);
+        for (int64_t k = 0; k < na; ++k) pos[order[k]] = (int32_t)k;
+    }
+    // ---- grid-point column structures (strictly below), bottom-up
+    std::vector<int32_t> nchild(na, 0);
+    for (int64_t j = 0; j < na; ++j)
+        if (par[j] != -1) nchild[par[j]]++;
+    std::vector<std::vector<int32_t>> st(na);
+    {
+        std::vector<int32_t> stamp(na, -1);
+        std::vector<std::vector<int32_t>> kids(na);
+        for (int64_t j = 0; j < na; ++j)
+            if (par[j] != -1) kids[par[j]].push_back((int32_t)j);
+        for (int64_t j = 0; j < na; ++j) {
+            std::vector<int32_t> s;
+            int32_t v = order[j];
+            for (int64_t p = G.ptr[v]; p < G.ptr[v + 1]; ++p) {
+                int32_t i = pos[G.adj[p]];
+                if (i > j && stamp[i] != j) { stamp[i] = (int32_t)j; s.push_back(i); }
+            }
+            for (int32_t c : kids[j]) {
+                for (int32_t i : st[c])
+                    if (i > j && stamp[i] != j) { stamp[i] = (int32_t)j; s.push_back(i); }
+                // a child's structure is no longer needed unless it heads a supernode
+            }
+            std::sort(s.begin(), s.end());
+            st[j] = std::move(s);
+        }
+    }
+    // ---- fundamental supernodes (grid-point level)
+    std::vector<int32_t> snode_of(na);
+    std::vector<int32_t> sfirst, slast;
+    for (int64_t j = 0; j < na; ++j) {
+        bool join = j > 0 && par[j - 1] == j && nchild[j] == 1 && st[j - 1].size() == st[j].size() + 1;
+        if (!join) { sfirst.push_back((int32_t)j); slast.push_back((int32_t)j); }
+        else slast.back() = (int32_t)j;
+        snode_of[j] = (int32_t)sfirst.size() - 1;
+    }
+    const int64_t nf = (int64_t)sfirst.size();
+    std::vector<int32_t> fpar(nf, -1);
+    for (int64_t s = 0; s < nf; ++s) {
+        int32_t p = par[slast[s]];
+        fpar[s] = p == -1 ? -1 : snode_of[p];
+    }
+    // DOF weights
+    std::vector<int64_t> fcol(nf, 0), frow(nf, 0);
+    for (int64_t s = 0; s < nf; ++s) {
+        for (int32_t j = sfirst[s]; j <= slast[s]; ++j) fcol[s] += wdof[act[order[j]]];
+        for (int32_t i : st[slast[s]]) frow[s] += wdof[act[order[i]]];
+    }
+    // ---- relaxed amalgamation
+    std::vector<std::vector<int32_t>> members(nf);   // fundamental supernodes merged into s (in column order)
+    std::vector<double> actual(nf);
+    std::vector<int64_t> mcol(fcol);
+    std::vector<int32_t> merged_into(nf, -1);
+    std::vector<std::vector<int32_t>> fkids(nf);
+    for (int64_t s = 0; s < nf; ++s) {
+        members[s].push_back((int32_t)s);
+        double c = (double)fcol[s];
+        actual[s] = c * (c + 1) / 2 + c * (double)frow[s];
+        if (fpar[s] != -1) fkids[fpar[s]].push_back((int32_t)s);
+    }
+    auto dense = [](double c, double r) { return c * (c + 1) / 2 + c * r; };
+    for (int64_t s = 0; s < nf; ++s) {
+        // children in reverse column order: the last child is adjacent to s
+        std::vector<int32_t> kids = fkids[s];
+        std::vector<int32_t> keep;
+        for (auto it = kids.rbegin(); it != kids.rend(); ++it) {
+            int32_t c = *it;
+            double nc = (double)(mcol[c] + mcol[s]);
+            double dz = dense(nc, (double)frow[s]);
+            double z = 1.0 - (actual[c] + actual[s]) / dz;
+            bool ok = nc <= 4 || (nc <= 16 && z < 0.8) || (nc <= 48 && z < 0.1) || (nc <= 256 && z < 0.05);
+            if (ok) {
+                merged_into[c] = (int32_t)s;
+                std::vector<int32_t> mem = members[c];
+                mem.insert(mem.end(), members[s].begin(), members[s].end());
+                members[s].swap(mem);
+                actual[s] += actual[c];
+                mcol[s] += mcol[c];
+            }
+        }
+    }
+    // final supernodes = fundamental supernodes not merged; tree over them
+    auto top = [&](int32_t s) {
+        while (merged_into[s] != -1) s = merged_into[s];
+        return s;
+    };
+    std::vector<int32_t> fin;                 // final supernode -> representative fundamental id
+    std::vector<int32_t> fin_of(nf, -1);
+    for (int64_t s = 0; s < nf; ++s)
+        if (merged_into[s] == -1) { fin_of[s] = (int32_t)fin.size(); fin.push_back((int32_t)s); }
+    const int64_t nsn = (int64_t)fin.size();
+    std::vector<int32_t> spar(nsn, -1);
+    for (int64_t k = 0; k < nsn; ++k) {
+        int32_t p = fpar[fin[k]];
+        spar[k] = p == -1 ? -1 : fin_of[top(p)];
+    }
+    // children of final supernodes (every fundamental child of any member that is not itself a member)
+    std::vector<std::vector<int32_t>> skids(nsn);
+    for (int64_t k = 0; k < nsn; ++k)
+        if (spar[k] != -1) skids[spar[k]].push_back((int32_t)k);
+    // postorder of the final tree; emit grid points of each supernode (members in order)
+    std::vector<int32_t> spost;
+    spost.reserve(nsn);
+    {
+        std::vector<int32_t> stack, it(nsn, 0);
+        for (int64_t r = 0; r < nsn; ++r) {
+            if (spar[r] != -1) continue;
+            stack.push_back((int32_t)r);
+            while (!stack.empty()) {
+                int32_t p = stack.back();
+                if (it[p] < (int32_t)skids[p].size()) {
+                    stack.push_back(skids[p][it[p]++]);
+                } else {
+                    spost.push_back(p);
+                    stack.pop_back();
+                }
+            }
+        }
+    }
+    std::vector<int32_t> snew(nsn);
+    for (int64_t k = 0; k < nsn; ++k) snew[spost[k]] = (int32_t)k;
+    std::vector<int32_t> final_order;         // new grid-point order (old position indices j)
+    final_order.reserve(na);
+    std::vector<int32_t> sn_gfirst(nsn + 1);
+    for (int64_t k = 0; k < nsn; ++k) {
+        int32_t rep = fin[spost[k]];
+        sn_gfirst[k] = (int32_t)final_order.size();
+        for (int32_t f : members[rep])
+            for (int32_t j = sfirst[f]; j <= slast[f]; ++j) final_order.push_back(j);
+    }
+    sn_gfirst[nsn] = (int32_t)final_order.size();
+    std::vector<int32_t> gpos(na);            // old position j -> new position
+    for (int64_t k = 0; k < na; ++k) gpos[final_order[k]] = (int32_t)k;
+    // DOF positions
+    std::vector<int64_t> dofpos(na + 1, 0);   // by new grid position
+    for (int64_t k = 0; k < na; ++k) dofpos[k + 1] = dofpos[k] + wdof[act[order[final_order[k]]]];
+    S.perm.assign(S.n, 0);
+    S.iperm.assign(ndof, -1);
+    for (int64_t k = 0; k < na; ++k) {
+        int32_t a = act[order[final_order[k]]];
+        int64_t p = dofpos[k];
+        for (int d = 0; d < 6; ++d)
+            if (!fixed[6 * a + d]) { S.perm[p] = (int32_t)(6 * a + d); S.iperm[6 * a + d] = (int32_t)p; ++p; }
+    }
+    // ---- DOF-level fronts
+    S.ns = nsn;
+    S.col0.resize(nsn); S.ncol.resize(nsn); S.m.resize(nsn); S.lofs.resize(nsn + 1); S.uofs.resize(nsn + 1);
+    S.rofs.resize(nsn + 1); S.parent.resize(nsn);
+    int64_t lo = 0, uo = 0, ro = 0;
+    std::vector<int32_t> snode_of_pos(S.n);
+    for (int64_t k = 0; k < nsn; ++k) {
+        int32_t rep = fin[spost[k]];
+        int32_t topf = members[rep].back();       // highest fundamental supernode of the merged set
+        S.col0[k] = dofpos[sn_gfirst[k]];
+        S.ncol[k] = dofpos[sn_gfirst[k + 1]] - dofpos[sn_gfirst[k]];
+        std::vector<int32_t> sg;                  // structure grid points in new positions
+        for (int32_t i : st[slast[topf]]) sg.push_back(gpos[i]);
+        std::sort(sg.begin(), sg.end());
+        S.rofs[k] = ro;
+        for (int64_t c = 0; c < S.ncol[k]; ++c) S.rows.push_back((int32_t)(S.col0[k] + c));
+        for (int32_t gp : sg)
+            for (int64_t p = dofpos[gp]; p < dofpos[gp + 1]; ++p) S.rows.push_back((int32_t)p);
+        S.m[k] = (int64_t)S.rows.size() - ro;
+        ro = (int64_t)S.rows.size();
+        S.lofs[k] = lo;
+        lo += S.m[k] * S.ncol[k];
+        S.uofs[k] = uo;
+        uo += (S.m[k] - S.ncol[k]) * (S.m[k] - S.ncol[k]);
+        S.parent[k] = spar[spost[k]] == -1 ? -1 : snew[spar[spost[k]]];
+        for (int64_t c = 0; c < S.ncol[k]; ++c) snode_of_pos[S.col0[k] + c] = (int32_t)k;
+        S.max_front = std::max<int64_t>(S.max_front, S.m[k]);
+        for (int64_t j = 0; j < S.ncol[k]; ++j) S.flops += (double)(S.m[k] - j) * (double)(S.m[k] - j);
+        S.upd_rows += S.m[k] - S.ncol[k];
+    }
+    S.lofs[nsn] = lo; S.uofs[nsn] = uo; S.rofs[nsn] = ro;
+    S.L_entries = lo; S.U_entries = uo;
+    // children lists and relative maps
+    S.child_ptr.assign(nsn + 1, 0);
+    for (int64_t k = 0; k < nsn; ++k)
+        if (S.parent[k] != -1) S.child_ptr[S.parent[k] + 1]++;
+    for (int64_t k = 0; k < nsn; ++k) S.child_ptr[k + 1] += S.child_ptr[k];
+    S.child.resize(S.child_ptr[nsn]);
+    {
+        std::vector<int32_t> fill(S.child_ptr.begin(), S.child_ptr.end() - 1);
+        for (int64_t k = 0; k < nsn; ++k)
+            if (S.parent[k] != -1) S.child[fill[S.parent[k]]++] = (int32_t)k;
+    }
+    S.relofs.resize(nsn + 1);
+    int64_t rel = 0;
+    for (int64_t k = 0; k < nsn; ++k) {
+        S.relofs[k] = rel;
+        int32_t p = S.parent[k];
+        int64_t nu = S.m[k] - S.ncol[k];
+        if (p == -1) {
+            if (nu != 0) { err = "root supernode with a non-empty update"; return 1; }
+            continue;
+        }
+        const int32_t *pr = S.rows.data() + S.rofs[p];
+        for (int64_t i = 0; i < nu; ++i) {
+            int32_t r = S.rows[S.rofs[k] + S.ncol[k] + i];
+            const int32_t *it = std::lower_bound(pr, pr + S.m[p], r);
+            if (it == pr + S.m[p] || *it != r) { err = "child row missing from parent front"; return 1; }
+            S.relmap.push_back((int32_t)(it - pr));
+        }
+        rel += nu;
+    }
+    S.relofs[nsn] = rel;
+    // K -> front scatter map: entries (g, c) with c >= g (upper triangle, original numbering)
+    for (int64_t g = 0; g < ndof; ++g) {
+        if (fixed[g]) continue;
+        for (int64_t p = rowptr[g]; p < rowptr[g + 1]; ++p) {
+            int64_t c = col[p];
+            if (c < g || fixed[c]) continue;
+            int32_t a = S.iperm[g], b = S.iperm[c];
+            int32_t hi = std::max(a, b), lo2 = std::min(a, b);
+            int32_t s = snode_of_pos[lo2];
+            const int32_t *rr = S.rows.data() + S.rofs[s];
+            const int32_t *it = std::lower_bound(rr, rr + S.m[s], hi);
+            if (it == rr + S.m[s] || *it != hi) { err = "matrix entry outside the symbolic structure"; return 1; }
+            int64_t lr = it - rr, lc = lo2 - S.col0[s];
+            S.amap_src.push_back((int32_t)p);
+            S.amap_dst.push_back(S.lofs[s] + lr + S.m[s] * lc);
+        }
+    }
+    // levels (height in the supernode tree)
+    std::vector<int32_t> lev(nsn, 0);
+    int32_t nlev = 0;
+    for (int64_t k = 0; k < nsn; ++k) {
+        int32_t l = 0;
+        for (int32_t q = S.child_ptr[k]; q < S.child_ptr[k + 1]; ++q) l = std::max(l, lev[S.child[q]] + 1);
+        lev[k] = l;
+        nlev = std::max(nlev, l + 1);
+    }
+    S.level_ptr.assign(nlev + 1, 0);
+    for (int64_t k = 0; k < nsn; ++k) S.level_ptr[lev[k] + 1]++;
+    for (int32_t l = 0; l < nlev; ++l) S.level_ptr[l + 1] += S.level_ptr[l];
+    S.level_sn.resize(nsn);
+    {
+        std::vector<int32_t> fill(S.level_ptr.begin(), S.level_ptr.end() - 1);
+        for (int64_t k = 0; k < nsn; ++k) S.level_sn[fill[lev[k]]++] = (int32_t)k;
+    }
+    // task lists
+    S.step_ptr.assign(nlev + 1, 0);
+    S.pstep_off.clear(); S.ustep_off.clear(); S.pstep_mid.clear();
+    S.ea_off.assign(nlev + 1, 0);
+    S.fwd_off.assign(nlev + 1, 0);
+    S.bwd_off.assign(nlev + 1, 0);
+    for (int32_t l = 0; l < nlev; ++l) {
+        S.ea_off[l] = (int64_t)S.etasks.size();
+        S.fwd_off[l] = (int64_t)S.fwd_tiles.size();
+        S.bwd_off[l] = (int64_t)S.bwd_tiles.size();
+        int64_t maxsteps = 0;
+        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+            int32_t s = S.level_sn[q];
+            maxsteps = std::max<int64_t>(maxsteps, (S.ncol[s] + kPanel - 1) / kPanel);
+            if (S.child_ptr[s + 1] > S.child_ptr[s])
+                for (int64_t c0 = 0; c0 < S.m[s]; c0 += kTile)
+                    S.etasks.push_back({s, (int32_t)c0, (int32_t)std::min<int64_t>(c0 + kTile, S.m[s]), 0});
+            for (int64_t r0 = S.ncol[s]; r0 < S.m[s]; r0 += kTile) S.fwd_tiles.push_back({s, 0, (int32_t)r0, 0});
+            if (S.m[s] > S.ncol[s])
+                for (int64_t c0 = 0; c0 < S.ncol[s]; c0 += kTile) S.bwd_tiles.push_back({s, 0, (int32_t)c0, 0});
+        }
+        S.step_ptr[l] = (int64_t)S.pstep_off.size();
+        for (int64_t t = 0; t < maxsteps; ++t) {
+            S.pstep_off.push_back((int64_t)S.ptasks.size());
+            S.ustep_off.push_back((int64_t)S.utasks.size());
+            // diagonal tiles first (they factor and store the diagonal block), then the rest
+            for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+                int32_t s = S.level_sn[q];
+                int64_t k = t * kPanel;
+                if (k < S.ncol[s]) S.ptasks.push_back({s, (int32_t)k, (int32_t)k, 0});
+            }
+            S.pstep_mid.push_back((int64_t)S.ptasks.size());
+            for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+                int32_t s = S.level_sn[q];
+                int64_t k = t * kPanel;
+                if (k >= S.ncol[s]) continue;
+                int64_t b = std::min<int64_t>(kPanel, S.ncol[s] - k);
+                for (int64_t r0 = k + kTile; r0 < S.m[s]; r0 += kTile) S.ptasks.push_back({s, (int32_t)k, (int32_t)r0, 0});
+                std::vector<int32_t> bnd;
+                for (int64_t c = k + b; c < S.ncol[s]; c += kTile) bnd.push_back((int32_t)c);
+                for (int64_t c = std::max<int64_t>(k + b, S.ncol[s]); c < S.m[s]; c += kTile) bnd.push_back((int32_t)c);
+                for (size_t ci = 0; ci < bnd.size(); ++ci)
+                    for (size_t ri = ci; ri < bnd.size(); ++ri) S.utasks.push_back({s, (int32_t)k, bnd[ri], bnd[ci]});
+            }
+        }
+    }
+    S.step_ptr[nlev] = (int64_t)S.pstep_off.size();
+    S.pstep_off.push_back((int64_t)S.ptasks.size());
+    S.ustep_off.push_back((int64_t)S.utasks.size());
+    S.ea_off[nlev] = (int64_t)S.etasks.size();
+    S.fwd_off[nlev] = (int64_t)S.fwd_tiles.size();
+    S.bwd_off[nlev] = (int64_t)S.bwd_tiles.size();
+    return 0;
+}
+
+}  // namespace jfem

@@ -1,0 +1,51 @@
+// Symbolic analysis for the supernodal multifrontal Cholesky of K_ff.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace jfem {
+
+constexpr int kPanel = 64;   // panel width of the blocked partial factorization
+constexpr int kTile = 64;    // tile edge of panel/update tasks
+
+struct PanelTask { int32_t s, k, r0, pad; };          // supernode, panel column, first row
+struct UpdateTask { int32_t s, k, r0, c0; };          // supernode, panel column, tile origin
+struct EaTask { int32_t s, c0, c1, pad; };            // parent supernode, front column range
+
+struct SymbolicHost {
+    int64_t ndof = 0, n = 0;        // all DOFs, free DOFs
+    int64_t ns = 0;                 // supernodes
+    std::vector<int32_t> perm;      // elimination position -> global DOF
+    std::vector<int32_t> iperm;     // global DOF -> elimination position (-1 fixed)
+    std::vector<int64_t> col0, ncol, m, lofs, uofs, rofs, relofs;
+    std::vector<int32_t> parent;    // -1 for roots
+    std::vector<int32_t> rows;      // concatenated row lists (elimination positions)
+    std::vector<int32_t> relmap;    // per child: update row -> parent front row
+    std::vector<int32_t> child_ptr, child;   // children lists (ascending)
+    std::vector<int32_t> level_ptr, level_sn;
+    // A scatter (upper triangle of K in original numbering -> lower front)
+    std::vector<int32_t> amap_src;
+    std::vector<int64_t> amap_dst;
+    // per-level task lists
+    std::vector<PanelTask> ptasks;
+    std::vector<UpdateTask> utasks;
+    std::vector<EaTask> etasks;
+    // ptask/utask ranges per (level, step): offsets into the arrays
+    std::vector<int64_t> step_ptr;          // per level: first step index (size levels+1)
+    std::vector<int64_t> pstep_off, ustep_off;   // per step: offset (size steps+1)
+    std::vector<int64_t> pstep_mid;              // per step: first non-diagonal panel tile
+    std::vector<int64_t> ea_off;            // per level: offset into etasks (size levels+1)
+    // solve task lists: row tiles of L21 per level (forward), column tiles (backward)
+    std::vector<PanelTask> fwd_tiles, bwd_tiles;
+    std::vector<int64_t> fwd_off, bwd_off;  // per level
+    int64_t L_entries = 0, U_entries = 0, upd_rows = 0;
+    double flops = 0;
+    int64_t max_front = 0;
+};
+
+// Build the symbolic structure.  rowptr/col: CSR pattern of the full K;
+// xyz: grid coordinates (3 per node); fixed: ndof mask.
+int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, const double *xyz,
+                   const uint8_t *fixed, SymbolicHost &S, std::string &err);
+
+}  // namespace jfem

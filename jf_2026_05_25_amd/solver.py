@@ -1,0 +1,398 @@
+"""Host side of the drop-in: the reference's assemble / factor / solve calls,
+re-expressed over the device C ABI (include/jfem_hip.h).
+
+Mirrors, for this path, the reference's
+  * FEM.stiffness_quad4_matrices / stiffness_tria3_matrices / stiffness_tetra10
+    (src/FEMKernels.jl:2839, 9811; src/tetra10.jl:68),
+  * Solver.assemble_stiffness (src/solver/assembly.jl:2630),
+  * Solver.apply_bc_and_solve (src/solver/boundary_conditions.jl:1324) with its
+    linear-solve cache (:62-125) and post-factor pivot check (:1662-1710).
+Everything runs on the GPU through libjfem_hip.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, cm, ptr
+
+FP64 = 64
+FP32 = 32
+KIND = {"CQUAD4": 0, "CTRIA3": 1, "CTETRA10": 2}
+
+
+class Context:
+    """One HIP stream on one device (one per host thread / serving slot)."""
+
+    def __init__(self, device: int = 0):
+        self.L = _lib.load()
+        h = C.c_void_p()
+        code = self.L.jfem_ctx_create(device, C.byref(h))
+        if code != 0:
+            raise _lib.JfemError(code, "jfem_ctx_create: no usable HIP device")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.L.jfem_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(self.h, self.L.jfem_ctx_synchronize(self.h))
+
+    def memory(self):
+        f = C.c_int64(); t = C.c_int64()
+        check(self.h, self.L.jfem_device_memory(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def enable_timing(self, on=True):
+        check(self.h, self.L.jfem_ctx_enable_timing(self.h, int(on)))
+
+    def reset_timing(self):
+        check(self.h, self.L.jfem_ctx_reset_kernel_time(self.h))
+
+    def kernel_time(self, family: int):
+        ms = C.c_double(); n = C.c_int64()
+        check(self.h, self.L.jfem_ctx_kernel_time(self.h, family, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+_default = None
+_default_lock = threading.Lock()
+
+
+def default_context() -> Context:
+    global _default
+    with _default_lock:
+        if _default is None:
+            _default = Context(0)
+        return _default
+
+
+# --------------------------------------------------------------------------- elements
+def stiffness_quad4_matrices(coords, Cm, Cb, Cs, h, E_ref=None, *, k6rot=100.0, drill_scale=1.0,
+                             rigid_shear=False, membrane_modes=15, ctx: Context | None = None):
+    """Local-frame CQUAD4 stiffness (src/FEMKernels.jl:2839), batched.
+
+    ``coords``: (4,2) or (n,4,2); ``Cm``/``Cb``: (3,3) or (n,3,3); ``Cs``: (2,2) or
+    (n,2,2); ``h``, ``k6rot``, ``drill_scale``: scalar or (n,).  Returns (24,24) or
+    (n,24,24).  Raises JfemError(DEGENERATE) like the reference's ArgumentError."""
+    ctx = ctx or default_context()
+    coords = np.asarray(coords, float)
+    single = coords.ndim == 2
+    coords = coords.reshape(-1, 4, 2)
+    n = coords.shape[0]
+
+    def per(a, shape):
+        a = np.asarray(a, float)
+        a = np.broadcast_to(a, (n,) + shape) if a.shape == shape else a.reshape((n,) + shape)
+        return np.ascontiguousarray(np.transpose(a, (0,) + tuple(range(len(shape), 0, -1))).reshape(n, -1))
+
+    lc = np.ascontiguousarray(coords.reshape(n, 8))
+    Cm_ = per(Cm, (3, 3)); Cb_ = per(Cb, (3, 3)); Cs_ = per(Cs, (2, 2))
+    h_ = np.ascontiguousarray(np.broadcast_to(np.asarray(h, float), (n,)))
+    k6 = np.ascontiguousarray(np.broadcast_to(np.asarray(k6rot, float), (n,)))
+    ds = np.ascontiguousarray(np.broadcast_to(np.asarray(drill_scale, float), (n,)))
+    flags = (np.broadcast_to(np.asarray(rigid_shear, np.int32), (n,)).astype(np.int32) |
+             ((np.broadcast_to(np.asarray(membrane_modes, np.int32), (n,)) & 15) << 4)).astype(np.int32)
+    flags = np.ascontiguousarray(flags)
+    Ke = np.zeros(n * 576)
+    bad = C.c_int64(-1)
+    check(ctx.h, ctx.L.jfem_q4_local_batch(ctx.h, n, ptr(lc), ptr(Cm_), ptr(Cb_), ptr(Cs_), ptr(h_), ptr(k6),
+                                           ptr(ds), ptr(flags), ptr(Ke), C.byref(bad)))
+    Ke = Ke.reshape(n, 24, 24).transpose(0, 2, 1)   # column-major blocks -> (n, row, col)
+    return Ke[0] if single else Ke
+
+
+class DeviceMesh:
+    """A model's grid points, connectivity and sections resident on the device."""
+
+    def __init__(self, model, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.model = model
+        L = self.ctx.L
+        x = np.ascontiguousarray(model.xyz[:, 0]); y = np.ascontiguousarray(model.xyz[:, 1])
+        z = np.ascontiguousarray(model.xyz[:, 2])
+        self._keep = [cm(model.quads, np.int32), cm(model.quad_props, float), cm(model.trias, np.int32),
+                      cm(model.tria_props, float), cm(model.tets, np.int32), cm(model.tet_props, float)]
+        qc, qp, tc, tp, ec, ep = self._keep
+        h = C.c_void_p()
+        check(self.ctx.h, L.jfem_mesh_create(self.ctx.h, model.n_nodes, ptr(x), ptr(y), ptr(z),
+                                             len(model.quads), ptr(qc), ptr(qp), len(model.trias), ptr(tc), ptr(tp),
+                                             len(model.tets), ptr(ec), ptr(ep), float(model.k6rot), C.byref(h)))
+        self.h = h
+
+    def update_props(self, model):
+        qp, tp, ep = cm(model.quad_props, float), cm(model.tria_props, float), cm(model.tet_props, float)
+        check(self.ctx.h, self.ctx.L.jfem_mesh_update_props(self.ctx.h, self.h, ptr(qp), ptr(tp), ptr(ep)))
+        self.model = model
+
+    def element_matrices(self, kind: str):
+        """Global-frame element matrices exactly as assembled, (n, nd, nd)."""
+        m = self.model
+        n, nd = {"CQUAD4": (len(m.quads), 24), "CTRIA3": (len(m.trias), 18), "CTETRA10": (len(m.tets), 30)}[kind]
+        out = np.zeros(n * nd * nd)
+        bad = C.c_int64(-1)
+        check(self.ctx.h, self.ctx.L.jfem_element_matrices(self.ctx.h, self.h, KIND[kind], ptr(out), C.byref(bad)))
+        return out.reshape(n, nd, nd).transpose(0, 2, 1)
+
+    def free(self):
+        if self.h:
+            self.ctx.L.jfem_mesh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Pattern:
+    """CSR pattern of K plus the element-slot gather lists (topology only)."""
+
+    def __init__(self, mesh: DeviceMesh):
+        self.ctx = mesh.ctx
+        h = C.c_void_p(); nnz = C.c_int64()
+        check(self.ctx.h, self.ctx.L.jfem_pattern_create(self.ctx.h, mesh.h, C.byref(h), C.byref(nnz)))
+        self.h = h
+        self.nnz = nnz.value
+        self.ndof = mesh.model.ndof
+
+    def free(self):
+        if self.h:
+            self.ctx.L.jfem_pattern_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceMatrix:
+    """Assembled global K (CSR values, FP64) resident on the device."""
+
+    def __init__(self, mesh: DeviceMesh, pattern: Pattern, ctx: Context | None = None):
+        self.ctx = ctx or mesh.ctx
+        self.pattern = pattern
+        h = C.c_void_p(); bad = C.c_int64(-1)
+        code = self.ctx.L.jfem_assemble(self.ctx.h, mesh.h, pattern.h, C.byref(h), C.byref(bad))
+        self.bad_element = bad.value
+        check(self.ctx.h, code)
+        self.h = h
+
+    def reassemble(self, mesh: DeviceMesh):
+        bad = C.c_int64(-1)
+        check(self.ctx.h, self.ctx.L.jfem_reassemble(self.ctx.h, mesh.h, self.h, C.byref(bad)))
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        n = self.pattern.ndof
+        rp = np.zeros(n + 1, np.int64); col = np.zeros(self.pattern.nnz, np.int32); val = np.zeros(self.pattern.nnz)
+        check(self.ctx.h, self.ctx.L.jfem_matrix_download(self.ctx.h, self.h, ptr(rp), ptr(col), ptr(val)))
+        return sp.csr_matrix((val, col, rp), shape=(n, n))
+
+    def free(self):
+        if self.h:
+            self.ctx.L.jfem_matrix_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class AssembledStiffness:
+    """assemble_stiffness result (src/solver/assembly.jl:5949), device-resident K."""
+    K: DeviceMatrix
+    mesh: DeviceMesh
+    pattern: Pattern
+    ndof: int
+    node_coords: np.ndarray
+    timings: dict = field(default_factory=dict)
+
+
+def assemble_stiffness(model, ctx: Context | None = None, mesh: DeviceMesh | None = None,
+                       pattern: Pattern | None = None) -> AssembledStiffness:
+    """Element generation + deterministic assembly of the global K on the GPU."""
+    ctx = ctx or default_context()
+    t0 = time.perf_counter()
+    mesh = mesh or DeviceMesh(model, ctx)
+    t1 = time.perf_counter()
+    pattern = pattern or Pattern(mesh)
+    t2 = time.perf_counter()
+    K = DeviceMatrix(mesh, pattern, ctx)
+    t3 = time.perf_counter()
+    return AssembledStiffness(K, mesh, pattern, model.ndof, model.xyz,
+                              {"upload_s": t1 - t0, "pattern_s": t2 - t1, "assemble_s": t3 - t2})
+
+
+def bc_partition(K: DeviceMatrix, spc: np.ndarray, eps_zero: float = 1e-8):
+    """Fixed-DOF mask: SPC + AUTOSPC + grid-point singularity (src/solver/boundary_conditions.jl:1480-1607)."""
+    ctx = K.ctx
+    spc = np.ascontiguousarray(np.asarray(spc, np.uint8))
+    fixed = np.zeros(K.pattern.ndof, np.uint8)
+    nf = C.c_int64()
+    check(ctx.h, ctx.L.jfem_bc_partition(ctx.h, K.h, ptr(spc), eps_zero, ptr(fixed), C.byref(nf)))
+    return fixed
+
+
+# --------------------------------------------------------------------- factor / solve
+class Symbolic:
+    def __init__(self, K: DeviceMatrix, mesh: DeviceMesh, fixed: np.ndarray, ctx: Context | None = None):
+        self.ctx = ctx or K.ctx
+        self.fixed = np.ascontiguousarray(np.asarray(fixed, np.uint8))
+        h = C.c_void_p()
+        info = _lib.SymbolicInfo()
+        check(self.ctx.h, self.ctx.L.jfem_symbolic_create(self.ctx.h, K.pattern.h, mesh.h, ptr(self.fixed),
+                                                          C.byref(h), C.byref(info)))
+        self.h = h
+        self.info = {k: getattr(info, k) for k, _ in _lib.SymbolicInfo._fields_}
+
+    def free(self):
+        if self.h:
+            self.ctx.L.jfem_symbolic_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Factor:
+    def __init__(self, S: Symbolic, K: DeviceMatrix, precision: int = FP64, ctx: Context | None = None):
+        self.ctx = ctx or K.ctx
+        self.symbolic = S
+        self.precision = precision
+        h = C.c_void_p(); bad = C.c_int64(-1)
+        code = self.ctx.L.jfem_factor_create(self.ctx.h, S.h, K.h, int(precision), C.byref(h), C.byref(bad))
+        self.bad_column = bad.value
+        check(self.ctx.h, code)
+        self.h = h
+
+    def refactor(self, K: DeviceMatrix):
+        bad = C.c_int64(-1)
+        code = self.ctx.L.jfem_refactor(self.ctx.h, self.h, K.h, C.byref(bad))
+        self.bad_column = bad.value
+        check(self.ctx.h, code)
+
+    def small_pivots(self, K: DeviceMatrix, threshold: float = 1e-7):
+        cap = self.symbolic.info["n_free"]
+        out = np.zeros(max(cap, 1), np.int64); cnt = C.c_int64()
+        check(self.ctx.h, self.ctx.L.jfem_factor_small_pivots(self.ctx.h, self.h, K.h, threshold, ptr(out), cap,
+                                                              C.byref(cnt)))
+        return out[:cnt.value].copy()
+
+    def device_bytes(self):
+        return int(self.ctx.L.jfem_factor_device_bytes(self.h))
+
+    def solve(self, K: DeviceMatrix, B: np.ndarray, tol: float = 1e-10, max_refine: int = 10):
+        B = np.asarray(B, float)
+        one = B.ndim == 1
+        if one:
+            B = B[:, None]
+        nrhs = B.shape[1]
+        Bc = np.ascontiguousarray(B.T)        # column-major ndof x nrhs
+        X = np.zeros_like(Bc)
+        iters = np.zeros(nrhs, np.int32); rr = np.zeros(nrhs)
+        code = self.ctx.L.jfem_solve(self.ctx.h, self.h, K.h, nrhs, ptr(Bc), ptr(X), tol, max_refine, ptr(iters),
+                                     ptr(rr))
+        if code not in (0, 8):
+            check(self.ctx.h, code)
+        X = X.T.copy()
+        return (X[:, 0] if one else X), rr, iters, code == 0
+
+    def free(self):
+        if self.h:
+            self.ctx.L.jfem_factor_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class LinearSolveCache:
+    """Factor reuse across right-hand sides sharing one K and one free set
+    (LinearSolveCacheEntry, src/solver/boundary_conditions.jl:62-125)."""
+
+    def __init__(self):
+        self.entries = {}
+
+    def get(self, key):
+        return self.entries.get(key)
+
+    def put(self, key, value):
+        self.entries[key] = value
+
+
+def apply_bc_and_solve(asm: AssembledStiffness, model, F_applied=None, *, precision: int = FP64,
+                       tol: float = 1e-10, max_refine: int = 10, pivot_threshold: float = 1e-7,
+                       linear_cache: LinearSolveCache | None = None, symbolic: Symbolic | None = None):
+    """BC partition, factor and solve of K_ff u = F_ff on the GPU.
+
+    Returns ``(u_global, fixed_dofs, spc_dofs, diagnostics)`` like the reference
+    (src/solver/boundary_conditions.jl:1324, 1974).  ``precision`` = 64 (plain FP64
+    factor) or 32 (FP32 factor + FP64 iterative refinement to ``tol``)."""
+    ctx = asm.K.ctx
+    F = model.loads if F_applied is None else np.asarray(F_applied, float)
+    diag = {"bc_partition": {}, "linear_solver": {}}
+    t0 = time.perf_counter()
+    key = (id(asm.K), precision)
+    entry = linear_cache.get(key) if linear_cache is not None else None
+    if entry is None:
+        fixed = bc_partition(asm.K, model.spc)
+        S = symbolic if (symbolic is not None and np.array_equal(symbolic.fixed, fixed)) else Symbolic(asm.K, asm.mesh, fixed, ctx)
+        t1 = time.perf_counter()
+        Fct = Factor(S, asm.K, precision, ctx)
+        small = Fct.small_pivots(asm.K, pivot_threshold)
+        n_post = 0
+        if len(small):   # post-factor pivot check: extra SPC and refactor (:1684-1710)
+            n_post = len(small)
+            fixed = fixed.copy()
+            fixed[small] = 1
+            Fct.free(); S.free()
+            S = Symbolic(asm.K, asm.mesh, fixed, ctx)
+            Fct = Factor(S, asm.K, precision, ctx)
+        t2 = time.perf_counter()
+        entry = (fixed, S, Fct, t1 - t0, t2 - t1, n_post)
+        if linear_cache is not None:
+            linear_cache.put(key, entry)
+        diag["linear_solver"]["cache_hit"] = False
+    else:
+        diag["linear_solver"]["cache_hit"] = True
+    fixed, S, Fct, t_sym, t_num, n_post = entry
+    t3 = time.perf_counter()
+    u, rr, iters, ok = Fct.solve(asm.K, F, tol=tol, max_refine=max_refine)
+    t4 = time.perf_counter()
+    spc = np.asarray(model.spc, np.uint8)
+    diag["bc_partition"].update({"free_dofs": int((fixed == 0).sum()), "fixed_dofs": int(fixed.sum()),
+                                 "explicit_spc_dofs": int(spc.sum()),
+                                 "autospc_dofs": int(((fixed == 1) & (spc == 0)).sum()),
+                                 "post_factorization_singular_dofs": n_post})
+    diag["linear_solver"].update({"backend": f"jfem_hip_supernodal_fp{precision}", "relative_residual": rr.tolist(),
+                                  "refinement_iterations": iters.tolist(), "converged": bool(ok),
+                                  "symbolic": S.info, "time_symbolic_s": t_sym, "time_factor_s": t_num,
+                                  "time_solve_s": t4 - t3})
+    return u, np.where(fixed)[0], np.where(spc)[0], diag
