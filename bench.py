@@ -1,0 +1,196 @@
+"""Benchmark: solves/s + p50/p99 latency at <= 1e-10 relative residual,
+FP32 factor + FP64 refinement vs plain FP64, on the wing-box shell
+(box 4 x 1 x 0.2 m, 256 x 64 CQUAD4 on each of its four faces, per-element
+thickness U[1,4] mm, root clamped, 8 right-hand sides of random nodal forces).
+
+One step = one pass of the hot path over one request: element generation +
+deterministic assembly of K, BC partition, numeric factorization, and the 8
+solves refined to ||K_ff u - F_ff|| / ||F_ff|| <= 1e-10.  The symbolic analysis
+(ordering, supernodes; topology only) is computed once before timing and reused,
+as the serving loop does; its time is reported as ``symbolic_s``.
+
+Prints ONE JSON line.  ``--gpus N`` with N > 1 is rejected: this build measures
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_HBM_GBS = 8000.0
+MI355X_FP32_TFLOPS = 157.3
+MI355X_FP64_TFLOPS = 78.6
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def run_cpu_baseline(nx_sample: int):
+    """The FP64 CPU oracle (single thread) on a bounded sample of the workload:
+    the same wing box cut to nx_sample spanwise cells (same cross-section, same
+    8 right-hand sides).  Returns solves/s on that sample."""
+    from jf_2026_05_25_amd import meshes
+    from oracle import oracle as O
+    m = meshes.wing_box(nx_sample, 64, nrhs=8)
+    t0 = time.perf_counter()
+    K = O.assemble(m)
+    fixed = O.bc_partition(K, m.spc)
+    u, rr, info = O.solve(K, fixed, m.loads, order="nd", xyz=m.xyz)
+    dt = time.perf_counter() - t0
+    return {"value": 8.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP64 assemble+BC+Cholesky+8 solves on the wing box cut to {nx_sample} spanwise "
+                      f"cells ({m.ndof} DOFs, {int((fixed == 0).sum())} free); {dt:.2f} s; "
+                      f"max relative residual {rr.max():.2e}; CPU {cpu_model()}",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=256, help="spanwise cells (256 = the benchmark workload)")
+    ap.add_argument("--nw", type=int, default=64, help="cells across each face")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--cpu-sample-nx", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["both", "fp32", "fp64"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures the single-GPU path; N > 1 is not run "
+                                   "(see DESIGN.md: how the path would shard)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from jf_2026_05_25_amd import meshes, solver
+
+    model = meshes.wing_box(args.nx, args.nw, nrhs=8)
+    ctx = solver.Context(0)
+    t0 = time.perf_counter()
+    mesh = solver.DeviceMesh(model, ctx)
+    pattern = solver.Pattern(mesh)
+    K = solver.DeviceMatrix(mesh, pattern, ctx)
+    t_setup = time.perf_counter() - t0
+    fixed = solver.bc_partition(K, model.spc)
+    t0 = time.perf_counter()
+    S = solver.Symbolic(K, mesh, fixed, ctx)
+    t_sym = time.perf_counter() - t0
+    info = S.info
+    B = model.loads
+
+    def step(precision, factor):
+        """One request: reassemble, BC, factor, 8 refined solves. Returns (factor, u, relres, iters)."""
+        K.reassemble(mesh)
+        fx = solver.bc_partition(K, model.spc)
+        if not np.array_equal(fx, fixed):
+            raise RuntimeError("free set changed between steps")
+        if factor is None:
+            factor = solver.Factor(S, K, precision, ctx)
+        else:
+            factor.refactor(K)
+        u, rr, it, ok = factor.solve(K, B, tol=args.tol, max_refine=20)
+        if not ok:
+            raise RuntimeError(f"refinement did not reach {args.tol}: {rr}")
+        return factor, u, rr, it
+
+    results = {}
+    for prec in ([32, 64] if args.only == "both" else ([32] if args.only == "fp32" else [64])):
+        F = None
+        for _ in range(args.warmup):
+            F, u, rr, it = step(prec, F)
+        ctx.sync()
+        lat = []
+        t_all = time.perf_counter()
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            F, u, rr, it = step(prec, F)
+            lat.append(time.perf_counter() - ts)
+        ctx.sync()
+        total = time.perf_counter() - t_all
+        # per-family kernel times (HIP events) on one extra instrumented step
+        ctx.enable_timing(2)
+        ctx.reset_timing()
+        F, u_t, rr_t, it_t = step(prec, F)
+        fam = {name: ctx.kernel_time(i) for i, name in enumerate(
+            ["element", "assembly", "factor", "solve", "spmv", "factor_update", "factor_panel", "extend_add"])}
+        ctx.enable_timing(0)
+        results[prec] = {"total": total, "lat": np.array(lat), "u": u, "relres": rr, "iters": it, "fam": fam,
+                         "factor_bytes": F.device_bytes()}
+        F.free()
+
+    main_prec = 32 if 32 in results else 64
+    r = results[main_prec]
+    nrhs = B.shape[1]
+    solves_per_s = nrhs * args.steps / r["total"]
+    out = {
+        "metric": "solves/s + p50/p99 latency @1e-10 residual: FP32-factor+refine vs FP64",
+        "value": solves_per_s,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * r["total"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 factor + fp64 refinement" if main_prec == 32 else "fp64",
+        "data": "synthetic (generated wing-box mesh, random per-element thickness and nodal forces)",
+        "config": {"workload": f"wing box 4x1x0.2 m, {args.nx}x{args.nw} CQUAD4 per face, 8 RHS",
+                   "dofs": model.ndof, "free_dofs": int(info["n_free"]), "cquad4": int(len(model.quads)),
+                   "rhs_per_request": nrhs, "tol": args.tol},
+        "p50_latency_ms": float(np.percentile(r["lat"], 50) * 1000),
+        "p99_latency_ms": float(np.percentile(r["lat"], 99) * 1000),
+        "max_relative_residual": float(np.max(r["relres"])),
+        "refinement_iterations": [int(x) for x in r["iters"]],
+        "dof_per_s": info["n_free"] * nrhs * args.steps / r["total"],
+        "symbolic_s": t_sym,
+        "setup_s": t_setup,
+        "symbolic": {k: (float(v) if isinstance(v, float) else int(v)) for k, v in info.items()},
+    }
+    if 64 in results and 32 in results:
+        r64 = results[64]
+        out["fp64_solves_per_s"] = nrhs * args.steps / r64["total"]
+        out["fp32_vs_fp64_speedup"] = solves_per_s / out["fp64_solves_per_s"]
+        out["fp64_p50_latency_ms"] = float(np.percentile(r64["lat"], 50) * 1000)
+        out["fp64_p99_latency_ms"] = float(np.percentile(r64["lat"], 99) * 1000)
+        out["fp64_max_relative_residual"] = float(np.max(r64["relres"]))
+        u32, u64 = r["u"], r64["u"]
+        out["fp32_refined_error_vs_fp64"] = float(np.max(np.linalg.norm(u32 - u64, axis=0) /
+                                                         np.linalg.norm(u64, axis=0)))
+    # kernel-family breakdown and roofline of the dominant kernel (factor update tiles)
+    fam = r["fam"]
+    out["kernel_ms"] = {k: round(v[0], 4) for k, v in fam.items()}
+    upd_ms, upd_n = fam["factor_update"]
+    peak = MI355X_FP32_TFLOPS if main_prec == 32 else MI355X_FP64_TFLOPS
+    if upd_n:
+        per_launch_flops = info["update_flops"] / upd_n
+        avg_ms = upd_ms / upd_n
+        achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
+        out["roofline"] = {"bound": "mfma", "kernel": f"update_kernel<{'float' if main_prec == 32 else 'double'}>",
+                           "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                           "traffic": None, "launches": upd_n, "avg_launch_ms": avg_ms,
+                           "flops_per_launch": per_launch_flops}
+    if not args.no_cpu_baseline:
+        cb = run_cpu_baseline(args.cpu_sample_nx)
+        cb.pop("seconds", None)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

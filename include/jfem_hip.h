@@ -62,7 +62,9 @@ const char *jfem_last_error(jfem_ctx *ctx);
 int jfem_device_memory(jfem_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
 /* milliseconds accumulated by the named kernel family since the last reset
  * (timed with HIP events on the context's stream); family: 0 element generation,
- * 1 assembly, 2 numeric factor, 3 triangular solves, 4 residual SpMV. */
+ * 1 assembly, 2 numeric factor, 3 triangular solves, 4 residual SpMV; with timing
+ * level 2 also per launch: 5 factor update kernel, 6 factor panel kernels,
+ * 7 extend-add kernel.  jfem_ctx_enable_timing(ctx, 0 | 1 | 2). */
 int jfem_ctx_kernel_time(jfem_ctx *ctx, int family, double *ms, int64_t *launches);
 int jfem_ctx_reset_kernel_time(jfem_ctx *ctx);
 int jfem_ctx_enable_timing(jfem_ctx *ctx, int on);
@@ -139,6 +141,8 @@ typedef struct {
     double flops;            /* sum_j (eta_j + 1)^2 over columns of L (counted per column) */
     int64_t max_front;       /* largest front order */
     int64_t update_entries;  /* stored update-matrix entries */
+    double update_flops;     /* multiply-adds x 2 of the trailing-update tiles (lower triangles) */
+    int64_t update_launches; /* update-kernel launches per factorization */
 } jfem_symbolic_info;
 
 /* Fill-reducing ordering (geometric nested dissection over grid points),
@@ -184,6 +188,11 @@ int jfem_dev_alloc(jfem_ctx *ctx, int64_t bytes, void **dev_ptr);
 int jfem_dev_free(jfem_ctx *ctx, void *dev_ptr);
 int jfem_memcpy_h2d(jfem_ctx *ctx, void *dev_dst, const void *host_src, int64_t bytes);
 int jfem_memcpy_d2h(jfem_ctx *ctx, void *host_dst, const void *dev_src, int64_t bytes);
+
+/* Diagnostics: device self-test of the matrix-core trailing-update tile kernel
+ * against its scalar form (precision 64 | 32); max_rel_diff receives the largest
+ * difference relative to max |entry|.  (No reference counterpart.) */
+int jfem_selftest_update(jfem_ctx *ctx, int precision, double *max_rel_diff);
 
 /* memory held by a handle (device bytes) */
 int64_t jfem_factor_device_bytes(const jfem_factor *F);

@@ -29,7 +29,7 @@ EXPORTS = [
     "jfem_symbolic_create", "jfem_symbolic_free", "jfem_factor_create", "jfem_refactor",
     "jfem_factor_small_pivots", "jfem_factor_free", "jfem_solve", "jfem_solve_dev",
     "jfem_dev_alloc", "jfem_dev_free", "jfem_memcpy_h2d", "jfem_memcpy_d2h",
-    "jfem_factor_device_bytes", "jfem_matrix_device_bytes",
+    "jfem_factor_device_bytes", "jfem_matrix_device_bytes", "jfem_selftest_update",
 ]
 
 
@@ -42,7 +42,7 @@ class JfemError(RuntimeError):
 class SymbolicInfo(C.Structure):
     _fields_ = [("n_free", C.c_int64), ("n_supernodes", C.c_int64), ("n_levels", C.c_int64),
                 ("nnz_L", C.c_int64), ("flops", C.c_double), ("max_front", C.c_int64),
-                ("update_entries", C.c_int64)]
+                ("update_entries", C.c_int64), ("update_flops", C.c_double), ("update_launches", C.c_int64)]
 
 
 _lib = None
@@ -96,6 +96,7 @@ def load():
         "jfem_memcpy_d2h": ([vp, vp, vp, i64], C.c_int),
         "jfem_factor_device_bytes": ([vp], i64),
         "jfem_matrix_device_bytes": ([vp], i64),
+        "jfem_selftest_update": ([vp, C.c_int, P(dbl)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -82,6 +82,7 @@ int jfem_ctx_create(int device, jfem_ctx **out) {
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
         hipMalloc((void **)&c->d_status, 2 * sizeof(int64_t)) != hipSuccess) {
         delete c;
         return JFEM_ERR_HIP;
@@ -98,6 +99,8 @@ int jfem_ctx_destroy(jfem_ctx *ctx) {
     (void)hipFree(ctx->d_status);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    (void)hipEventDestroy(ctx->ev2);
+    (void)hipEventDestroy(ctx->ev3);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return JFEM_OK;
@@ -133,6 +136,7 @@ int jfem_ctx_reset_kernel_time(jfem_ctx *ctx) {
 
 int jfem_ctx_enable_timing(jfem_ctx *ctx, int on) {
     ctx->timing = on != 0;
+    ctx->timing_level = on;
     return JFEM_OK;
 }
 

@@ -196,7 +196,10 @@ struct Q4Shared {
     double Dm[48], Xm[48];             // MacNeal D (4x12) and Z \ D
     double Kp[144];
     double Bd[4][24], dcoef[4];
+    double zl[4];                      // corner heights of a warped cell (finite-warp map)
+    int warped;
     double out[576];
+    double Tg[576], W[576];            // finite-warp map and K*Tg (warped cells only)
 };
 
 // B-matrix entries as functions of the local DOF d = 6k + c
@@ -494,9 +497,9 @@ __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int6
     }
     __syncthreads();
     // Phase 5: every lane forms its 3x3 block (A, B), then rotates it
+    double blk[9];
     if (valid) {
         const int A = lane / 8, B = lane % 8;
-        double blk[9];
         for (int jj = 0; jj < 3; ++jj)
             for (int ii = 0; ii < 3; ++ii) {
                 const int i = 3 * A + ii, j = 3 * B + jj;
@@ -533,6 +536,85 @@ __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int6
                 }
                 blk[ii + 3 * jj] = v + e;
             }
+        if (S.warped)
+            for (int jj = 0; jj < 3; ++jj)
+                for (int ii = 0; ii < 3; ++ii) S.out[(3 * A + ii) + 24 * (3 * B + jj)] = blk[ii + 3 * jj];
+    }
+    // finite-warp work-dual congruence Ke <- sym(Tg' Ke Tg) for warped cells
+    // (apply_quad4_finite_warp_equilibrium!, src/FEMKernels.jl:1082-1116; map
+    // _quad4_finite_warp_fill!, :880-945).  Every lane passes the barriers.
+    if (valid && S.warped)
+        for (int q = lane; q < 576; q += kWave) S.Tg[q] = (q % 25 == 0) ? 1.0 : 0.0;
+    __syncthreads();
+    if (valid && S.warped && lane < 4) {
+        const int i = lane;
+        const double cr[4] = {-1, 1, 1, -1}, cs_[4] = {-1, -1, 1, 1};
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(cr[i], cs_[i], dNr, dNs);
+        jac4(dNr, dNs, S.lc, J11, J12, J21, J22);
+        double dj = J11 * J22 - J12 * J21;
+        if (fabs(dj) >= 1e-30) {
+            double i11 = J22 / dj, i12 = -J12 / dj, i21 = -J21 / dj, i22 = J11 / dj;
+            double zr = dNr[0] * S.zl[0] + dNr[1] * S.zl[1] + dNr[2] * S.zl[2] + dNr[3] * S.zl[3];
+            double zs = dNs[0] * S.zl[0] + dNs[1] * S.zl[1] + dNs[2] * S.zl[2] + dNs[3] * S.zl[3];
+            double gx = i11 * zr + i12 * zs, gy = i21 * zr + i22 * zs;
+            int rxi = 6 * i + 3, ryi = 6 * i + 4;
+            S.Tg[rxi + 24 * (6 * i + 5)] += gx;
+            S.Tg[ryi + 24 * (6 * i + 5)] += gy;
+            int a = (i % 2 == 0) ? 0 : 1, b = (i % 2 == 0) ? 2 : 3;
+            double dx = S.lc[2 * b] - S.lc[2 * a], dy = S.lc[2 * b + 1] - S.lc[2 * a + 1];
+            double dl2 = dx * dx + dy * dy;
+            if (dl2 > 1e-30) {
+                int js[2] = {a, b};
+                double cu[2] = {dy / dl2, -dy / dl2}, cv[2] = {-dx / dl2, dx / dl2};
+                for (int t = 0; t < 2; ++t) {
+                    int j = js[t];
+                    S.Tg[rxi + 24 * (6 * j + 0)] -= gx * cu[t];
+                    S.Tg[rxi + 24 * (6 * j + 1)] -= gx * cv[t];
+                    S.Tg[ryi + 24 * (6 * j + 0)] -= gy * cu[t];
+                    S.Tg[ryi + 24 * (6 * j + 1)] -= gy * cv[t];
+                }
+            }
+            for (int j = 0; j < 4; ++j) {
+                double dNdx = i11 * dNr[j] + i12 * dNs[j];
+                double dNdy = i21 * dNr[j] + i22 * dNs[j];
+                S.Tg[(6 * i + 0) + 24 * (6 * j + 2)] += S.zl[i] * dNdx;
+                S.Tg[(6 * i + 1) + 24 * (6 * j + 2)] += S.zl[i] * dNdy;
+            }
+        }
+    }
+    __syncthreads();
+    if (valid && S.warped) {   // W = Ke * Tg
+        const int A = lane / 8, B = lane % 8;
+        for (int jj = 0; jj < 3; ++jj)
+            for (int ii = 0; ii < 3; ++ii) {
+                const int i = 3 * A + ii, j = 3 * B + jj;
+                double v = 0;
+                for (int m = 0; m < 24; ++m) v += S.out[i + 24 * m] * S.Tg[m + 24 * j];
+                S.W[i + 24 * j] = v;
+            }
+    }
+    __syncthreads();
+    if (valid && S.warped) {   // Kt = Tg' * W  (written over the local Ke)
+        const int A = lane / 8, B = lane % 8;
+        for (int jj = 0; jj < 3; ++jj)
+            for (int ii = 0; ii < 3; ++ii) {
+                const int i = 3 * A + ii, j = 3 * B + jj;
+                double v = 0;
+                for (int l = 0; l < 24; ++l) v += S.Tg[l + 24 * i] * S.W[l + 24 * j];
+                S.out[i + 24 * j] = v;
+            }
+    }
+    __syncthreads();
+    if (valid) {
+        const int A = lane / 8, B = lane % 8;
+        if (S.warped) {
+            for (int jj = 0; jj < 3; ++jj)
+                for (int ii = 0; ii < 3; ++ii) {
+                    const int i = 3 * A + ii, j = 3 * B + jj;
+                    blk[ii + 3 * jj] = 0.5 * (S.out[i + 24 * j] + S.out[j + 24 * i]);
+                }
+        }
         // R' * blk * R, R rows = (v1, v2, v3) (src/solver/assembly.jl:392-424)
         const double *R = S.R;
         double t[9];
@@ -618,7 +700,11 @@ __device__ void q4_setup(Q4Shared &S, const double *p, const double *prop, doubl
             xmax = fmax(xmax, fabs(lc[2 * k]));
         }
         if (xmax < 1e-30) xmax = 1e-30;
-        if (zmax > 1e-12 * xmax) S.status = JFEM_ERR_WARPED;
+        S.warped = zmax > 1e-12 * xmax;
+        for (int k = 0; k < 4; ++k) {
+            double d[3] = {p[3 * k] - c[0], p[3 * k + 1] - c[1], p[3 * k + 2] - c[2]};
+            S.zl[k] = dot3(d, w3);
+        }
     }
     double Cm[9], Cb[9], Cs[4], G;
     bool mo;
@@ -706,6 +792,7 @@ q4_local_kernel(int64_t n, const double *__restrict__ lc, const double *__restri
         for (int q = 0; q < 4; ++q) S.Cs[q] = Cs[4 * e + q];
         for (int q = 0; q < 9; ++q) S.R[q] = (q % 4 == 0) ? 1.0 : 0.0;
         S.h = h[e]; S.k6 = k6[e]; S.ds = ds[e];
+        S.warped = 0;
         int f = flags[e];
         S.rigid = f & 1;
         int modes = (f >> 4) & 15;

@@ -188,11 +188,13 @@ __global__ void __launch_bounds__(kBS) panel_kernel(const PanelTask *__restrict_
     }
 }
 
-// ----------------------------------------------------------------- update kernel
+// ----------------------------------------------------------------- update kernel (reference)
+// Scalar version of the trailing update, used only by jfem_selftest to check the
+// MFMA kernel's operand/accumulator lane layouts on the device.
 // C[r, c] -= sum_p A[r, p] A[c, p] over the panel columns p in [k, k+b) for the
 // 64 x 64 tile at (r0, c0) of the trailing front (lower triangle only).
 template <class T>
-__global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+__global__ void __launch_bounds__(kBS) update_ref_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                      const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
                                                      const int64_t *__restrict__ uofs, T *__restrict__ L, T *__restrict__ U) {
     __shared__ T A[kPanel][kTile + 1];   // A[p][i]  (transposed tile of rows r0..)
@@ -233,6 +235,81 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
             const int i = ti + a, j = tj + c;
             if (i < nr && j < nc && r0 + i >= c0 + j) Cs[(r0 + i - roff) + ld * (c0 + j - coff)] -= acc[a][c];
         }
+}
+
+// ----------------------------------------------------------------- update kernel
+// C[r, c] -= sum_p A[r, p] A[c, p] over the panel columns p in [k, k+b) for the
+// 64 x 64 tile at (r0, c0) of the trailing front (lower triangle only), on the
+// matrix cores: 4 wavefronts, each a 32 x 32 quarter as 2 x 2 blocks of
+// v_mfma_{f64,f32}_16x16x4.  The product is formed transposed, D[j][i] =
+// sum_p A[c0+j][p] A[r0+i][p], so the accumulator's lane index runs along the
+// column-major rows of C and the read-modify-write of C is coalesced.
+// 16x16x4 operand layout: lane l holds A[l%16][l/16] and B[l/16][l%16];
+// accumulator element v of lane l is D[4*(l/16)+v][l%16].
+template <class T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mac(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mac(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+
+template <class T>
+__global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+                                                     const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
+                                                     const int64_t *__restrict__ uofs, T *__restrict__ L, T *__restrict__ U) {
+    typedef typename Mfma<T>::v4 v4;
+    __shared__ T A[kPanel][kTile + 1];   // A[p][i] = L[r0 + i, k + p]
+    __shared__ T B[kPanel][kTile + 1];   // B[p][j] = L[c0 + j, k + p]
+    const UpdateTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
+    const int b = (int)min<int64_t>(kPanel, ns - k);
+    const int b4 = (b + 3) & ~3;
+    const int64_t r0 = t.r0, c0 = t.c0;
+    const int64_t rb = r0 < ns ? ns : ms, cb = c0 < ns ? ns : ms;
+    const int nr = (int)(min<int64_t>(r0 + kTile, rb) - r0);
+    const int nc = (int)(min<int64_t>(c0 + kTile, cb) - c0);
+    const T *Ls = L + lofs[s];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < kTile * b4; q += blockDim.x) {
+        int i = q % kTile, p = q / kTile;
+        bool inp = p < b;
+        A[p][i] = (inp && i < nr) ? Ls[(r0 + i) + ms * (k + p)] : T(0);
+        B[p][i] = (inp && i < nc) ? Ls[(c0 + i) + ms * (k + p)] : T(0);
+    }
+    __syncthreads();
+    const int wave = tid / 64, lane = tid % 64;
+    const int wi = (wave & 1) * 32;      // rows of C (i) handled by this wave
+    const int wj = (wave >> 1) * 32;     // columns of C (j)
+    const int l16 = lane & 15, l4 = lane >> 4;
+    v4 acc[2][2];
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) acc[x][y] = v4{T(0), T(0), T(0), T(0)};
+    for (int p = 0; p < b4; p += 4) {
+        // transposed product: MFMA "A" operand = B (columns j), "B" operand = A (rows i)
+        T aj[2], bi[2];
+        for (int y = 0; y < 2; ++y) aj[y] = B[p + l4][wj + 16 * y + l16];
+        for (int x = 0; x < 2; ++x) bi[x] = A[p + l4][wi + 16 * x + l16];
+        for (int x = 0; x < 2; ++x)
+            for (int y = 0; y < 2; ++y) acc[x][y] = Mfma<T>::mac(aj[y], bi[x], acc[x][y]);
+    }
+    const bool inL = c0 < ns;
+    T *Cs = inL ? (L + lofs[s]) : (U + uofs[s]);
+    const int64_t ld = inL ? ms : ms - ns;
+    const int64_t off = inL ? 0 : ns;
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+            for (int v = 0; v < 4; ++v) {
+                const int i = wi + 16 * x + l16;           // row of C within the tile
+                const int j = wj + 16 * y + 4 * l4 + v;    // column of C within the tile
+                if (i < nr && j < nc && r0 + i >= c0 + j) Cs[(r0 + i - off) + ld * (c0 + j - off)] -= acc[x][y][v];
+            }
 }
 
 // ----------------------------------------------------------------- solves
@@ -497,6 +574,14 @@ template <class T>
 hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, T *L, T *U) {
     hipStream_t st = ctx->stream;
     const SymbolicHost &H = S->H;
+    const bool detail = ctx->timing_level >= 2;
+    auto lap = [&](int fam, int64_t launches) {
+        (void)hipEventSynchronize(ctx->ev3);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ctx->ev2, ctx->ev3);
+        ctx->ms[fam] += ms;
+        ctx->launches[fam] += launches;
+    };
     hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(H.L_entries)), dim3(kBS), 0, st, L, H.L_entries);
     if (H.U_entries) hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(H.U_entries)), dim3(kBS), 0, st, U, H.U_entries);
     int64_t na = (int64_t)H.amap_src.size();
@@ -506,23 +591,31 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
     const int64_t nlev = (int64_t)H.level_ptr.size() - 1;
     for (int64_t l = 0; l < nlev; ++l) {
         int64_t ne = H.ea_off[l + 1] - H.ea_off[l];
-        if (ne)
+        if (ne) {
+            if (detail) (void)hipEventRecord(ctx->ev2, st);
             hipLaunchKernelGGL(extend_add_kernel<T>, dim3((unsigned)ne), dim3(kBS), 0, st, S->d_etasks + H.ea_off[l],
                                S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, S->d_relofs, S->d_relmap, S->d_child_ptr,
                                S->d_child, L, U);
+            if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_EA, 1); }
+        }
         for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
             int64_t nd = H.pstep_mid[stp] - H.pstep_off[stp];
             int64_t nr = H.pstep_off[stp + 1] - H.pstep_mid[stp];
             int64_t nu = H.ustep_off[stp + 1] - H.ustep_off[stp];
+            if (detail) (void)hipEventRecord(ctx->ev2, st);
             if (nd)
                 hipLaunchKernelGGL(panel_kernel<T>, dim3((unsigned)nd), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp], 1,
                                    S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status);
             if (nr)
                 hipLaunchKernelGGL(panel_kernel<T>, dim3((unsigned)nr), dim3(kBS), 0, st, S->d_ptasks + H.pstep_mid[stp], 0,
                                    S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status);
-            if (nu)
+            if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, (nd ? 1 : 0) + (nr ? 1 : 0)); }
+            if (nu) {
+                if (detail) (void)hipEventRecord(ctx->ev2, st);
                 hipLaunchKernelGGL(update_kernel<T>, dim3((unsigned)nu), dim3(kBS), 0, st, S->d_utasks + H.ustep_off[stp],
                                    S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, L, U);
+                if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_UPDATE, 1); }
+            }
         }
     }
     return hipGetLastError();
@@ -632,6 +725,8 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
         info->flops = H.flops;
         info->max_front = H.max_front;
         info->update_entries = H.U_entries;
+        info->update_flops = H.update_flops;
+        info->update_launches = (int64_t)H.ustep_off.size() - 1;
     }
     *out = S;
     return JFEM_OK;
@@ -816,6 +911,79 @@ int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, in
     JFEM_CHECK_HIP(ctx, hipGetLastError());
     JFEM_CHECK_HIP(ctx, hipStreamSynchronize(st));
     if (!converged) return fail(ctx, JFEM_ERR_NOT_CONVERGED, "refinement did not reach the requested residual");
+    return JFEM_OK;
+}
+
+// Device self-test of the matrix-core trailing update against its scalar form on
+// one random 200 x 70 front (three panels, L and U parts): returns the largest
+// relative difference.  precision = 64 | 32.
+int jfem_selftest_update(jfem_ctx *ctx, int precision, double *max_rel_diff) {
+    const int64_t m = 200, n = 70;
+    std::vector<UpdateTask> tasks;
+    for (int64_t c0 = 64; c0 < m; c0 += 64)
+        for (int64_t r0 = c0; r0 < m; r0 += 64) tasks.push_back({0, 0, (int32_t)r0, (int32_t)c0});
+    // boundaries split at n: use the same rule as the symbolic task builder
+    tasks.clear();
+    std::vector<int32_t> bnd;
+    for (int64_t c = 64; c < n; c += 64) bnd.push_back((int32_t)c);
+    for (int64_t c = n; c < m; c += 64) bnd.push_back((int32_t)c);
+    for (size_t ci = 0; ci < bnd.size(); ++ci)
+        for (size_t ri = ci; ri < bnd.size(); ++ri) tasks.push_back({0, 0, bnd[ri], bnd[ci]});
+    const int64_t nL = m * n, nU = (m - n) * (m - n);
+    std::vector<double> hL(nL), hU(nU);
+    uint64_t st = 0x9e3779b97f4a7c15ull;
+    auto rnd = [&]() { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return (double)(st % 2000001) / 1e6 - 1.0; };
+    for (auto &v : hL) v = rnd();
+    for (auto &v : hU) v = rnd();
+    const size_t es = precision == 64 ? 8 : 4;
+    int64_t h_ncol = n, h_m = m, h_lofs = 0, h_uofs = 0;
+    void *dL[2], *dU[2];
+    int64_t *d_meta;
+    UpdateTask *d_tasks;
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_meta, 4 * sizeof(int64_t)));
+    int64_t meta[4] = {h_ncol, h_m, h_lofs, h_uofs};
+    JFEM_CHECK_HIP(ctx, hipMemcpy(d_meta, meta, sizeof(meta), hipMemcpyHostToDevice));
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_tasks, tasks.size() * sizeof(UpdateTask)));
+    JFEM_CHECK_HIP(ctx, hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(UpdateTask), hipMemcpyHostToDevice));
+    std::vector<float> fL(hL.begin(), hL.end()), fU(hU.begin(), hU.end());
+    for (int v = 0; v < 2; ++v) {
+        JFEM_CHECK_HIP(ctx, hipMalloc(&dL[v], nL * es));
+        JFEM_CHECK_HIP(ctx, hipMalloc(&dU[v], nU * es));
+        JFEM_CHECK_HIP(ctx, hipMemcpy(dL[v], precision == 64 ? (void *)hL.data() : (void *)fL.data(), nL * es, hipMemcpyHostToDevice));
+        JFEM_CHECK_HIP(ctx, hipMemcpy(dU[v], precision == 64 ? (void *)hU.data() : (void *)fU.data(), nU * es, hipMemcpyHostToDevice));
+    }
+    dim3 g((unsigned)tasks.size());
+    if (precision == 64) {
+        hipLaunchKernelGGL(update_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[0], (double *)dU[0]);
+        hipLaunchKernelGGL(update_ref_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[1], (double *)dU[1]);
+    } else {
+        hipLaunchKernelGGL(update_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[0], (float *)dU[0]);
+        hipLaunchKernelGGL(update_ref_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[1], (float *)dU[1]);
+    }
+    JFEM_CHECK_HIP(ctx, hipGetLastError());
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double worst = 0, scale = 0;
+    for (int part = 0; part < 2; ++part) {
+        int64_t cnt = part == 0 ? nL : nU;
+        std::vector<double> a(cnt), r(cnt);
+        if (precision == 64) {
+            JFEM_CHECK_HIP(ctx, hipMemcpy(a.data(), part ? dU[0] : dL[0], cnt * 8, hipMemcpyDeviceToHost));
+            JFEM_CHECK_HIP(ctx, hipMemcpy(r.data(), part ? dU[1] : dL[1], cnt * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> fa(cnt), fr(cnt);
+            JFEM_CHECK_HIP(ctx, hipMemcpy(fa.data(), part ? dU[0] : dL[0], cnt * 4, hipMemcpyDeviceToHost));
+            JFEM_CHECK_HIP(ctx, hipMemcpy(fr.data(), part ? dU[1] : dL[1], cnt * 4, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < cnt; ++i) { a[i] = fa[i]; r[i] = fr[i]; }
+        }
+        for (int64_t i = 0; i < cnt; ++i) {
+            worst = std::max(worst, std::fabs(a[i] - r[i]));
+            scale = std::max(scale, std::fabs(r[i]));
+        }
+    }
+    for (int v = 0; v < 2; ++v) { (void)hipFree(dL[v]); (void)hipFree(dU[v]); }
+    (void)hipFree(d_meta);
+    (void)hipFree(d_tasks);
+    *max_rel_diff = worst / std::max(scale, 1e-300);
     return JFEM_OK;
 }
 

@@ -18,7 +18,8 @@
 
 namespace jfem {
 
-enum Family { FAM_ELEM = 0, FAM_ASSEMBLY = 1, FAM_FACTOR = 2, FAM_SOLVE = 3, FAM_SPMV = 4, FAM_COUNT = 8 };
+enum Family { FAM_ELEM = 0, FAM_ASSEMBLY = 1, FAM_FACTOR = 2, FAM_SOLVE = 3, FAM_SPMV = 4, FAM_UPDATE = 5,
+              FAM_PANEL = 6, FAM_EA = 7, FAM_COUNT = 8 };
 
 }  // namespace jfem
 
@@ -27,6 +28,8 @@ struct jfem_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     bool timing = false;
+    int timing_level = 0;          // 2 = per-launch timing inside the factorization
+    hipEvent_t ev2 = nullptr, ev3 = nullptr;
     double ms[jfem::FAM_COUNT] = {0};
     int64_t launches[jfem::FAM_COUNT] = {0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -3,7 +3,8 @@
 //  1. grid points with free DOFs are the supervariables (their DOFs share one
 //     sparsity pattern); the grid-point graph comes from the CSR pattern of K;
 //  2. fill-reducing ordering: geometric nested dissection on the grid coordinates
-//     (median plane along the longest extent, one-sided vertex separator);
+//     (median plane along the axis that gives the smallest one-sided vertex
+//     separator);
 //  3. elimination tree, grid-point column structures, fundamental supernodes and
 //     relaxed amalgamation (merge a child into its parent when the merged front
 //     adds few explicit zeros);
@@ -14,8 +15,10 @@
 //     batched panel/update/extend-add/solve kernels.
 #include <algorithm>
 #include <cstdint>
+#include <cstdint>
 #include <cstring>
 #include <functional>
+#include <climits>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -40,6 +43,36 @@ struct ND {
     int leaf;
     ND(const double *x, const Graph &gr, int64_t nn, int lf) : xyz(x), g(gr), mark(nn, 0), leaf(lf) {}
 
+    // median split of `set` along axis ax: returns the split index h (0 = none)
+    size_t split(std::vector<int32_t> &set, int ax) {
+        std::stable_sort(set.begin(), set.end(), [&](int32_t a, int32_t b) { return xyz[3 * a + ax] < xyz[3 * b + ax]; });
+        size_t h = set.size() / 2;
+        // keep points with equal coordinate on one side
+        while (h < set.size() && h > 0 && xyz[3 * set[h] + ax] == xyz[3 * set[h - 1] + ax]) ++h;
+        if (h == set.size()) {
+            h = set.size() / 2;
+            while (h > 0 && xyz[3 * set[h] + ax] == xyz[3 * set[h - 1] + ax]) --h;
+        }
+        return (h == set.size()) ? 0 : h;
+    }
+
+    // one-sided separators of a split: SA = A-side points adjacent to B, SB likewise
+    void separators(const std::vector<int32_t> &set, size_t h, std::vector<int32_t> &A, std::vector<int32_t> &B,
+                    std::vector<int32_t> &SA, std::vector<int32_t> &SB) {
+        A.clear(); B.clear(); SA.clear(); SB.clear();
+        for (size_t k = 0; k < set.size(); ++k) mark[set[k]] = k < h ? 1 : 2;
+        for (size_t k = 0; k < set.size(); ++k) {
+            int32_t v = set[k];
+            int other = k < h ? 2 : 1;
+            bool sep = false;
+            for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p)
+                if (mark[g.adj[p]] == other) { sep = true; break; }
+            if (k < h) (sep ? SA : A).push_back(v);
+            else (sep ? SB : B).push_back(v);
+        }
+        for (int32_t v : set) mark[v] = 0;
+    }
+
     void run(std::vector<int32_t> set) {
         if ((int64_t)set.size() <= leaf) {
             out.insert(out.end(), set.begin(), set.end());
@@ -51,33 +84,30 @@ struct ND {
                 lo[a] = std::min(lo[a], xyz[3 * v + a]);
                 hi[a] = std::max(hi[a], xyz[3 * v + a]);
             }
-        int ax = 0;
-        for (int a = 1; a < 3; ++a)
-            if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
-        std::stable_sort(set.begin(), set.end(), [&](int32_t a, int32_t b) { return xyz[3 * a + ax] < xyz[3 * b + ax]; });
-        size_t h = set.size() / 2;
-        // keep points with equal coordinate on one side
-        while (h < set.size() && h > 0 && xyz[3 * set[h] + ax] == xyz[3 * set[h - 1] + ax]) ++h;
-        if (h == set.size()) {
-            h = set.size() / 2;
-            while (h > 0 && xyz[3 * set[h] + ax] == xyz[3 * set[h - 1] + ax]) --h;
+        // choose the axis whose median plane gives the smallest separator
+        // (ties: larger extent); meshes with stretched cells otherwise cut along
+        // the coarse direction and produce large separators
+        int best_ax = -1;
+        size_t best_sep = SIZE_MAX;
+        double best_ext = -1;
+        std::vector<int32_t> A, B, SA, SB;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (!(hi[ax] - lo[ax] > 0)) continue;
+            size_t h = split(set, ax);
+            if (h == 0) continue;
+            separators(set, h, A, B, SA, SB);
+            size_t sz = std::min(SA.size(), SB.size());
+            if (A.empty() && SA.size() <= SB.size()) continue;
+            if (B.empty() && SB.size() < SA.size()) continue;
+            double ext = hi[ax] - lo[ax];
+            if (sz < best_sep || (sz == best_sep && ext > best_ext)) { best_sep = sz; best_ax = ax; best_ext = ext; }
         }
-        if (h == 0 || h == set.size()) {
+        if (best_ax < 0) {
             out.insert(out.end(), set.begin(), set.end());
             return;
         }
-        for (size_t k = 0; k < set.size(); ++k) mark[set[k]] = k < h ? 1 : 2;
-        std::vector<int32_t> A, B, SA, SB;
-        for (size_t k = 0; k < set.size(); ++k) {
-            int32_t v = set[k];
-            int other = k < h ? 2 : 1;
-            bool sep = false;
-            for (int64_t p = g.ptr[v]; p < g.ptr[v + 1]; ++p)
-                if (mark[g.adj[p]] == other) { sep = true; break; }
-            if (k < h) (sep ? SA : A).push_back(v);
-            else (sep ? SB : B).push_back(v);
-        }
-        for (int32_t v : set) mark[v] = 0;
+        size_t h = split(set, best_ax);
+        separators(set, h, A, B, SA, SB);
         std::vector<int32_t> S;
         if (SA.size() <= SB.size()) {
             S = SA;
@@ -90,6 +120,7 @@ struct ND {
             out.insert(out.end(), set.begin(), set.end());
             return;
         }
+        const int ax = best_ax;
         std::stable_sort(B.begin(), B.end(), [&](int32_t a, int32_t b) { return xyz[3 * a + ax] < xyz[3 * b + ax]; });
         run(std::move(A));
         run(std::move(B));
@@ -492,7 +523,15 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 for (int64_t c = k + b; c < S.ncol[s]; c += kTile) bnd.push_back((int32_t)c);
                 for (int64_t c = std::max<int64_t>(k + b, S.ncol[s]); c < S.m[s]; c += kTile) bnd.push_back((int32_t)c);
                 for (size_t ci = 0; ci < bnd.size(); ++ci)
-                    for (size_t ri = ci; ri < bnd.size(); ++ri) S.utasks.push_back({s, (int32_t)k, bnd[ri], bnd[ci]});
+                    for (size_t ri = ci; ri < bnd.size(); ++ri) {
+                        S.utasks.push_back({s, (int32_t)k, bnd[ri], bnd[ci]});
+                        int64_t r0 = bnd[ri], c0 = bnd[ci];
+                        int64_t r1 = std::min<int64_t>(r0 + kTile, r0 < S.ncol[s] ? S.ncol[s] : S.m[s]);
+                        int64_t c1 = std::min<int64_t>(c0 + kTile, c0 < S.ncol[s] ? S.ncol[s] : S.m[s]);
+                        double cnt = 0;
+                        for (int64_t c = c0; c < c1; ++c) cnt += (double)std::max<int64_t>(0, r1 - std::max(r0, c));
+                        S.update_flops += 2.0 * (double)b * cnt;
+                    }
             }
         }
     }

@@ -40,6 +40,7 @@ struct SymbolicHost {
     std::vector<int64_t> fwd_off, bwd_off;  // per level
     int64_t L_entries = 0, U_entries = 0, upd_rows = 0;
     double flops = 0;
+    double update_flops = 0;   // 2 x multiply-adds of the update tiles (lower triangles)
     int64_t max_front = 0;
 };
 

@@ -538,6 +538,69 @@ static int quad4_is_warped(const double *p, const double lc[8]) {
     return zmax > 1e-12 * xmax;
 }
 
+/* local corner heights about the centroid along normalize(d13 x d24)
+ * (quad4_local_z_from_coords3d, src/FEMKernels.jl:853-865) */
+static void quad4_local_z(const double *p, double zl[4]) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6, *p4 = p + 9;
+    double c[3], a[3], b[3], n[3];
+    for (int k = 0; k < 3; ++k) c[k] = (((p1[k] + p2[k]) + p3[k]) + p4[k]) / 4;
+    v3sub(p3, p1, a); v3sub(p4, p2, b); v3cross(a, b, n);
+    double nr = sqrt(v3dot(n, n));
+    double v3[3] = {0, 0, 1};
+    if (nr > 1e-30) { v3[0] = n[0] / nr; v3[1] = n[1] / nr; v3[2] = n[2] / nr; }
+    for (int k = 0; k < 4; ++k) { double d[3]; v3sub(p + 3 * k, c, d); zl[k] = v3dot(d, v3); }
+}
+
+/* finite-warp work-dual congruence Ke <- sym(Tg' Ke Tg)
+ * (apply_quad4_finite_warp_equilibrium!, src/FEMKernels.jl:1082-1116;
+ *  _quad4_finite_warp_fill!, :880-945) */
+static void quad4_finite_warp(double *Ke, const double lc[8], const double *p) {
+    double zl[4], Tg[576];
+    quad4_local_z(p, zl);
+    for (int q = 0; q < 576; ++q) Tg[q] = (q % 25 == 0) ? 1.0 : 0.0;
+    const double cr[4] = {-1, 1, 1, -1}, cs_[4] = {-1, -1, 1, 1};
+    for (int i = 0; i < 4; ++i) {
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs_quad(cr[i], cs_[i], dNr, dNs);
+        jac4(dNr, dNs, lc, &J11, &J12, &J21, &J22);
+        double dj = J11 * J22 - J12 * J21;
+        if (fabs(dj) < 1e-30) continue;
+        double i11 = J22 / dj, i12 = -J12 / dj, i21 = -J21 / dj, i22 = J11 / dj;
+        double zr = dNr[0] * zl[0] + dNr[1] * zl[1] + dNr[2] * zl[2] + dNr[3] * zl[3];
+        double zs = dNs[0] * zl[0] + dNs[1] * zl[1] + dNs[2] * zl[2] + dNs[3] * zl[3];
+        double gx = i11 * zr + i12 * zs, gy = i21 * zr + i22 * zs;
+        int rxi = 6 * i + 3, ryi = 6 * i + 4;
+        M(Tg, rxi, 6 * i + 5, 24) += gx;
+        M(Tg, ryi, 6 * i + 5, 24) += gy;
+        int a = (i % 2 == 0) ? 0 : 1, b = (i % 2 == 0) ? 2 : 3;
+        double dx = lc[2 * b] - lc[2 * a], dy = lc[2 * b + 1] - lc[2 * a + 1];
+        double dl2 = dx * dx + dy * dy;
+        if (dl2 > 1e-30) {
+            int js[2] = {a, b};
+            double cu[2] = {dy / dl2, -dy / dl2}, cv[2] = {-dx / dl2, dx / dl2};
+            for (int t = 0; t < 2; ++t) {
+                int j = js[t];
+                M(Tg, rxi, 6 * j + 0, 24) -= gx * cu[t];
+                M(Tg, rxi, 6 * j + 1, 24) -= gx * cv[t];
+                M(Tg, ryi, 6 * j + 0, 24) -= gy * cu[t];
+                M(Tg, ryi, 6 * j + 1, 24) -= gy * cv[t];
+            }
+        }
+        for (int j = 0; j < 4; ++j) {
+            double dNdx = i11 * dNr[j] + i12 * dNs[j];
+            double dNdy = i21 * dNr[j] + i22 * dNs[j];
+            M(Tg, 6 * i + 0, 6 * j + 2, 24) += zl[i] * dNdx;
+            M(Tg, 6 * i + 1, 6 * j + 2, 24) += zl[i] * dNdy;
+        }
+    }
+    double W[576], Kt[576];
+    for (int j = 0; j < 24; ++j) for (int i = 0; i < 24; ++i) {
+        double v = 0; for (int m = 0; m < 24; ++m) v += M(Ke, i, m, 24) * M(Tg, m, j, 24); M(W, i, j, 24) = v; }
+    for (int j = 0; j < 24; ++j) for (int i = 0; i < 24; ++i) {
+        double v = 0; for (int l = 0; l < 24; ++l) v += M(Tg, l, i, 24) * M(W, l, j, 24); M(Kt, i, j, 24) = v; }
+    for (int j = 0; j < 24; ++j) for (int i = 0; i < 24; ++i) M(Ke, i, j, 24) = 0.5 * (M(Kt, i, j, 24) + M(Kt, j, i, 24));
+}
+
 int or_q4_global(const double p[12], const double prop[8], double k6rot, double Ke[576]) {
     const double *p1 = p, *p2 = p + 3, *p3 = p + 6, *p4 = p + 9;
     double v1[3], v2[3], v3[3];
@@ -549,7 +612,7 @@ int or_q4_global(const double p[12], const double prop[8], double k6rot, double 
         double d[3]; v3sub(p + 3 * k, c, d);
         lc[2 * k] = v3dot(d, v1); lc[2 * k + 1] = v3dot(d, v2);
     }
-    if (quad4_is_warped(p, lc)) return OR_ERR_WARPED;
+    int warped = quad4_is_warped(p, lc);
     double Cm[9], Cb[9], Cs[4], G; int mo;
     shell_section(prop, Cm, Cb, Cs, &G, &mo);
     /* PSHELL material axis from edge 1-2, helpers.jl:1749-1759, assembly.jl:4400-4425 */
@@ -581,6 +644,7 @@ int or_q4_global(const double p[12], const double prop[8], double k6rot, double 
     if (modes <= 0 || modes > 15) modes = 15;
     int st = or_q4_local_modes(lc, Cm, Cb, Cs, prop[OR_SP_H], prop[OR_SP_E], k6, ds, rigid, modes, Kl);
     if (st != OR_OK) return st;
+    if (warped) quad4_finite_warp(Kl, lc, p);
     double R[9] = {v1[0], v1[1], v1[2], v2[0], v2[1], v2[2], v3[0], v3[1], v3[2]};
     transform_blocks(Kl, 24, R, Ke);
     return OR_OK;
@@ -1057,17 +1121,40 @@ static int nd_cmp(const void *a, const void *b) {
     double u = nd_xyz[3 * *(const int32_t *)a + nd_cmp_axis], w = nd_xyz[3 * *(const int32_t *)b + nd_cmp_axis];
     return u < w ? -1 : (u > w);
 }
+/* split set at the median of axis ax; count the one-sided separator (A-side points
+ * adjacent to B); returns the separator size, or -1 if the split is empty */
+static int64_t nd_try(nd_ctx *c, int32_t *set, int64_t n, int ax) {
+    nd_cmp_axis = ax; nd_xyz = c->xyz;
+    qsort(set, (size_t)n, sizeof(int32_t), nd_cmp);
+    int64_t h = n / 2;
+    for (int64_t k = 0; k < n; ++k) c->mark[set[k]] = k < h ? 1 : 2;
+    int64_t ns = 0;
+    for (int64_t k = 0; k < h; ++k) {
+        int32_t v = set[k];
+        for (int64_t p = c->ap[v]; p < c->ap[v + 1]; ++p) if (c->mark[c->ai[p]] == 2) { ++ns; break; }
+    }
+    for (int64_t k = 0; k < n; ++k) c->mark[set[k]] = 0;
+    return (ns == h) ? -1 : ns;
+}
+
 static void nd_rec(nd_ctx *c, int32_t *set, int64_t n, int32_t tag) {
     if (n <= 64) { for (int64_t k = 0; k < n; ++k) c->out[c->nout++] = set[k]; return; }
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int64_t k = 0; k < n; ++k) for (int a = 0; a < 3; ++a) {
         double v = c->xyz[3 * set[k] + a]; if (v < lo[a]) lo[a] = v; if (v > hi[a]) hi[a] = v;
     }
-    int ax = 0; for (int a = 1; a < 3; ++a) if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
+    /* axis with the smallest median-plane separator */
+    int ax = -1; int64_t best = -1;
+    for (int a = 0; a < 3; ++a) {
+        if (!(hi[a] - lo[a] > 0)) continue;
+        int64_t sz = nd_try(c, set, n, a);
+        if (sz < 0) continue;
+        if (ax < 0 || sz < best) { ax = a; best = sz; }
+    }
+    if (ax < 0) { for (int64_t k = 0; k < n; ++k) c->out[c->nout++] = set[k]; return; }
     nd_cmp_axis = ax; nd_xyz = c->xyz;
     qsort(set, (size_t)n, sizeof(int32_t), nd_cmp);
     int64_t h = n / 2;
-    /* tag: side A = tag*2+1, side B = tag*2+2 (within this call only) */
     for (int64_t k = 0; k < n; ++k) c->mark[set[k]] = k < h ? 1 : 2;
     int32_t *A = malloc(sizeof(int32_t) * n), *B = malloc(sizeof(int32_t) * n), *S = malloc(sizeof(int32_t) * n);
     int64_t na = 0, nb = 0, ns = 0;

@@ -6,9 +6,12 @@ and their recorded displacement vectors ``results/*.JU.JSON``:
 * validation/cases/MYSTRAN_sol101_simple  (1 CQUAD4, FORCE and PLOAD4 subcases)
 * validation/cases/MH_curved_beam_in_plane (6 tapered CQUAD4, in-plane FORCE)
 * validation/cases/MH_curved_beam_refined  (96 CQUAD4, in-plane FORCE)
+* validation/cases/MH_twisted_beam_{in_plane,out_of_plane} (warped CQUAD4, FORCE)
+* validation/cases/MH_pinched_cylinder{,_refined}, MH_hemispherical_shell
+  (curved shells of warped CQUAD4, FORCE, symmetry SPC1)
 
-Curved and twisted decks (warped quads) and the TACS-backend uCRM-9 vector are
-not on this path and are not extracted.  Output: tests/golden/reference_ju.json.
+Scordelis-Lo (GRAV load) and the TACS-backend uCRM-9 vector are not on this
+path and are not extracted.  Output: tests/golden/reference_ju.json.
 Run from the repository root: ``python tests/golden/make_fixtures.py``.
 """
 import json
@@ -22,6 +25,11 @@ CASES = [
     ("MYSTRAN_sol101_simple", "results/sol101_simple.JU.JSON", {1: 1992, 3: 1994}, 101),
     ("MH_curved_beam_in_plane", "results/curved_beam.JU.JSON", {1: 1}, 1),
     ("MH_curved_beam_refined", "results/curved_beam_refined.JU.JSON", {1: 1}, 1),
+    ("MH_twisted_beam_in_plane", "results/twisted_beam.JU.JSON", {1: 1}, 1),
+    ("MH_twisted_beam_out_of_plane", "results/twisted_beam_out_of_plane.JU.JSON", {1: 1}, 1),
+    ("MH_pinched_cylinder", "results/pinched_cylinder.JU.JSON", {1: 1}, 1),
+    ("MH_pinched_cylinder_refined", "results/pinched_cylinder_refined.JU.JSON", {1: 1}, 1),
+    ("MH_hemispherical_shell", "results/hemispherical_shell.JU.JSON", {1: 1}, 1),
 ]
 
 
@@ -90,9 +98,16 @@ def parse(case, loads_by_subcase, spc_sid):
         elif name == "SPC1":
             sid = int(g(1))
             comps = [int(c) for c in g(2)]
-            for k in range(3, len(f)):
-                if f[k]:
-                    spc.append((sid, int(f[k]), comps))
+            ids = [x for x in f[3:] if x]
+            k = 0
+            while k < len(ids):
+                if k + 2 < len(ids) and ids[k + 1].upper() == "THRU":
+                    for n in range(int(ids[k]), int(ids[k + 2]) + 1):
+                        spc.append((sid, n, comps))
+                    k += 3
+                else:
+                    spc.append((sid, int(ids[k]), comps))
+                    k += 1
         elif name == "FORCE":
             sid = int(g(1))
             forces.setdefault(sid, []).append({"grid": int(g(2)), "mag": num(g(4)),

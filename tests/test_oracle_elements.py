@@ -294,15 +294,6 @@ def test_shell_rigid_motion_and_rotation_covariance(oracle, kind):
     np.testing.assert_allclose(K1, T @ K0 @ T.T, rtol=3e-12, atol=3e-12 * np.abs(K0).max())
 
 
-def test_q4_warped_rejected(oracle):
-    from jf_2026_05_25_amd import meshes
-    prop = meshes.pshell_props(1, 0.002, 70e9, 0.33)[0]
-    P = np.array([[0., 0., 0.], [1., 0., 0.], [1., 1., 0.05], [0., 1., 0.]])
-    with pytest.raises(oracle.OracleError) as e:
-        oracle.q4_global(P, prop)
-    assert e.value.code == 4
-
-
 def test_q4_degenerate_rejected(oracle):
     x = np.array([[0., 0.], [1., 0.], [1., 0.], [0., 1.]])
     A = iso(E7, .3, H3)
