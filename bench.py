@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--cpu-sample-nx", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["both", "fp32", "fp64"], default="both")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="before timing, run one device copy of this many MiB (torch) so that hardware "
+                         "counters (FETCH_SIZE / WRITE_SIZE) can be scaled by a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures the single-GPU path; N > 1 is not run "
@@ -78,6 +81,15 @@ def main():
         sys.exit(2)
 
     from jf_2026_05_25_amd import meshes, solver
+
+    if args.calibrate_copy_mib > 0:
+        import torch
+        nel = args.calibrate_copy_mib * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     model = meshes.wing_box(args.nx, args.nw, nrhs=8)
     ctx = solver.Context(0)

@@ -169,19 +169,27 @@ int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_mat
                              double threshold, int64_t *dofs_out, int64_t cap, int64_t *count);
 int jfem_factor_free(jfem_factor *F);
 
-/* Solve K_ff x = b for nrhs right-hand sides given in FULL DOF numbering
+/* Solve K_ff x = b for nrhs <= 16 right-hand sides given in FULL DOF numbering
  * (b: ndof x nrhs column-major, fixed rows ignored; x: ndof x nrhs, fixed rows 0).
  * Replaces F \ F_ff (src/solver/boundary_conditions.jl:1442, 1714) and the
- * residual (:1859-1868).  Runs triangular solves in the factor's precision and, while
- * ||K_ff x - b|| / max(||b||, 1e-30) > tol and iterations < max_refine, refines in FP64:
- * r = b - K_ff x (FP64 SpMV), x += F \ r.  relres[k] and iters[k] per RHS.
- * b / x on the host: *_host; device-resident variants take device pointers. */
+ * residual (:1859-1868).  Triangular solves run in the factor's precision; the
+ * iterate is kept as a double-double x = x_hi + x_lo and refined,
+ *   r = b - K_ff x  (residual accumulated exactly: fma-split products, two-sum),
+ *   x += F \ r,
+ * until ||K_ff x - b|| / max(||b||, 1e-30) <= tol, the residual stops decreasing
+ * (by 10 % per step), or max_refine steps.  relres[k] / iters[k]: residual of the
+ * double-double iterate and the steps taken, per RHS.  x receives x_hi; x_lo
+ * (optional, may be NULL) receives the low parts; relres_out (optional) receives
+ * the residual of x_hi alone, i.e. of the double vector handed back, whose floor
+ * is about 2^-53 || |K_ff| |x| || / ||b||.  Returns JFEM_ERR_NOT_CONVERGED (x still
+ * written) when a right-hand side misses tol.
+ * b / x on the host: *_host; the _dev variant takes device pointers. */
 int jfem_solve(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs,
                const double *b_host, double *x_host, double tol, int max_refine,
-               int32_t *iters, double *relres);
+               int32_t *iters, double *relres, double *x_lo_host, double *relres_out);
 int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs,
                    const double *b_dev, double *x_dev, double tol, int max_refine,
-                   int32_t *iters, double *relres);
+                   int32_t *iters, double *relres, double *x_lo_dev, double *relres_out);
 
 /* device buffers for the _dev entry points (owned by the caller) */
 int jfem_dev_alloc(jfem_ctx *ctx, int64_t bytes, void **dev_ptr);

@@ -88,8 +88,8 @@ def load():
         "jfem_refactor": ([vp, vp, vp, P(i64)], C.c_int),
         "jfem_factor_small_pivots": ([vp, vp, vp, dbl, vp, i64, P(i64)], C.c_int),
         "jfem_factor_free": ([vp], C.c_int),
-        "jfem_solve": ([vp, vp, vp, C.c_int, vp, vp, dbl, C.c_int, vp, vp], C.c_int),
-        "jfem_solve_dev": ([vp, vp, vp, C.c_int, vp, vp, dbl, C.c_int, vp, vp], C.c_int),
+        "jfem_solve": ([vp, vp, vp, C.c_int, vp, vp, dbl, C.c_int, vp, vp, vp, vp], C.c_int),
+        "jfem_solve_dev": ([vp, vp, vp, C.c_int, vp, vp, dbl, C.c_int, vp, vp, vp, vp], C.c_int),
         "jfem_dev_alloc": ([vp, i64, P(vp)], C.c_int),
         "jfem_dev_free": ([vp, vp], C.c_int),
         "jfem_memcpy_h2d": ([vp, vp, vp, i64], C.c_int),

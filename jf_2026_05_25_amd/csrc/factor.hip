@@ -29,7 +29,7 @@ struct jfem_symbolic {
             *d_level_sn = nullptr;
     int32_t *d_amap_src = nullptr;
     int64_t *d_amap_dst = nullptr;
-    PanelTask *d_ptasks = nullptr, *d_fwd = nullptr, *d_bwd = nullptr;
+    PanelTask *d_ptasks = nullptr, *d_btasks = nullptr, *d_bwd = nullptr;
     UpdateTask *d_utasks = nullptr;
     EaTask *d_etasks = nullptr;
     std::vector<void *> allocs;
@@ -244,19 +244,24 @@ __global__ void __launch_bounds__(kBS) update_ref_kernel(const UpdateTask *__res
 // v_mfma_{f64,f32}_16x16x4.  The product is formed transposed, D[j][i] =
 // sum_p A[c0+j][p] A[r0+i][p], so the accumulator's lane index runs along the
 // column-major rows of C and the read-modify-write of C is coalesced.
-// 16x16x4 operand layout: lane l holds A[l%16][l/16] and B[l/16][l%16];
-// accumulator element v of lane l is D[4*(l/16)+v][l%16].
+// 16x16x4 operand layout (both types): lane l holds A[l%16][l/16] and
+// B[l/16][l%16].  Accumulator layouts differ: the f32 instruction returns one
+// group of 4 consecutive rows per 16-lane block, element v of lane l being
+// D[4*(l/16)+v][l%16]; the f64 instruction returns 4 groups of one row,
+// element v of lane l being D[4*v+(l/16)][l%16].  drow() gives that row.
 template <class T>
 struct Mfma;
 template <>
 struct Mfma<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
     static __device__ inline v4 mac(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ inline int drow(int blk, int v) { return 4 * v + blk; }
 };
 template <>
 struct Mfma<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
     static __device__ inline v4 mac(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ inline int drow(int blk, int v) { return 4 * blk + v; }
 };
 
 template <class T>
@@ -307,25 +312,47 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
         for (int y = 0; y < 2; ++y)
             for (int v = 0; v < 4; ++v) {
                 const int i = wi + 16 * x + l16;           // row of C within the tile
-                const int j = wj + 16 * y + 4 * l4 + v;    // column of C within the tile
+                const int j = wj + 16 * y + Mfma<T>::drow(l4, v);   // column of C within the tile
                 if (i < nr && j < nc && r0 + i >= c0 + j) Cs[(r0 + i - off) + ld * (c0 + j - off)] -= acc[x][y][v];
             }
 }
 
 // ----------------------------------------------------------------- solves
-// vectors in elimination order, right-hand sides interleaved: Y[i * nrhs + r]
+// Vectors in elimination order, right-hand sides interleaved: v[i * nrhs + r].
+// Forward: Y (pending) -> Z (solved), W holds each supernode's update vector.
+// Backward: Z (pending) -> X (solved).  Each step launch solves one 64-column
+// diagonal block per supernode of the level; every workgroup of the step redoes
+// that small solve from the pending values (no workgroup of the launch writes
+// them) and then applies its own tile, so a step is one launch.  Wave w of a
+// 256-thread workgroup handles right-hand sides r = w, w+4, w+8, w+12.
+
+// value of v in lane `lane` (uniform lane index), for all lanes of the wave
 template <class T>
-__global__ void __launch_bounds__(kBS) fwd_diag_kernel(const int32_t *__restrict__ level_sn, const int64_t *__restrict__ col0,
-                                                       const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
-                                                       const int64_t *__restrict__ lofs, const int64_t *__restrict__ relofs,
-                                                       const int32_t *__restrict__ relmap, const int32_t *__restrict__ child_ptr,
-                                                       const int32_t *__restrict__ child, const T *__restrict__ L,
-                                                       T *__restrict__ Y, T *__restrict__ W, int nrhs) {
+__device__ inline T bcast(T v, int lane);
+template <>
+__device__ inline float bcast<float>(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+template <>
+__device__ inline double bcast<double>(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// per supernode of the level: W_s = 0, then add the children's update vectors
+// (in child order) into Y_s (pivot rows) and W_s (update rows)
+template <class T>
+__global__ void __launch_bounds__(kBS) fwd_gather_kernel(const int32_t *__restrict__ level_sn, const int64_t *__restrict__ col0,
+                                                         const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
+                                                         const int64_t *__restrict__ relofs, const int32_t *__restrict__ relmap,
+                                                         const int32_t *__restrict__ child_ptr, const int32_t *__restrict__ child,
+                                                         T *__restrict__ Y, T *__restrict__ W, int nrhs) {
     const int s = level_sn[blockIdx.x];
-    const int64_t ns = ncol[s], ms = mrow[s], c0 = col0[s];
-    const T *Ls = L + lofs[s];
+    const int64_t ns = ncol[s], ms = mrow[s];
     T *Ws = W + relofs[s] * nrhs;
-    T *Ys = Y + c0 * nrhs;
+    T *Ys = Y + col0[s] * nrhs;
     const int tid = threadIdx.x;
     for (int64_t q = tid; q < (ms - ns) * nrhs; q += blockDim.x) Ws[q] = T(0);
     __syncthreads();
@@ -335,136 +362,205 @@ __global__ void __launch_bounds__(kBS) fwd_diag_kernel(const int32_t *__restrict
         const int32_t *rel = relmap + relofs[c];
         const T *Wc = W + relofs[c] * nrhs;
         for (int64_t q = tid; q < nu * nrhs; q += blockDim.x) {
-            int64_t i = q / nrhs;
-            int r = (int)(q % nrhs);
-            int64_t pr = rel[i];
+            const int64_t i = q / nrhs;
+            const int r = (int)(q % nrhs);
+            const int64_t pr = rel[i];
             if (pr < ns) Ys[pr * nrhs + r] += Wc[q];
             else Ws[(pr - ns) * nrhs + r] += Wc[q];
         }
         __syncthreads();
     }
-    // forward substitution with L11, blocked by 64 columns
-    for (int64_t jb = 0; jb < ns; jb += kPanel) {
-        const int bsz = (int)min<int64_t>(kPanel, ns - jb);
-        for (int j = 0; j < bsz; ++j) {
-            const int64_t jj = jb + j;
-            const T d = Ls[jj + ms * jj];
-            for (int r = tid; r < nrhs; r += blockDim.x) Ys[jj * nrhs + r] /= d;
-            __syncthreads();
-            for (int64_t q = tid; q < (int64_t)(bsz - j - 1) * nrhs; q += blockDim.x) {
-                int64_t i = jj + 1 + q / nrhs;
-                int r = (int)(q % nrhs);
-                Ys[i * nrhs + r] -= Ls[i + ms * jj] * Ys[jj * nrhs + r];
-            }
-            __syncthreads();
-        }
-        for (int64_t q = tid; q < (ns - jb - bsz) * nrhs; q += blockDim.x) {
-            int64_t i = jb + bsz + q / nrhs;
-            int r = (int)(q % nrhs);
-            T v = T(0);
-            for (int j = 0; j < bsz; ++j) v += Ls[i + ms * (jb + j)] * Ys[(jb + j) * nrhs + r];
-            Ys[i * nrhs + r] -= v;
-        }
-        __syncthreads();
-    }
 }
 
-// W_s[rows] -= L21[rows, :] y_s, 64-row tiles
+// forward step task (s, k, r0): y_k = L_kk^{-1} (pending rows k..k+b), written to Z
+// by the diagonal task (r0 == k); rows [max(r0, k+b), r0+kTile) of the front get
+// -= L[rows, k..k+b) y_k (pivot rows into Y, update rows into W).
 template <class T>
-__global__ void __launch_bounds__(kBS) fwd_offdiag_kernel(const PanelTask *__restrict__ tiles, const int64_t *__restrict__ col0,
-                                                          const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
-                                                          const int64_t *__restrict__ lofs, const int64_t *__restrict__ relofs,
-                                                          const T *__restrict__ L, const T *__restrict__ Y, T *__restrict__ W,
-                                                          int nrhs) {
-    const PanelTask t = tiles[blockIdx.x];
+__global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ col0,
+                                                       const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
+                                                       const int64_t *__restrict__ lofs, const int64_t *__restrict__ relofs,
+                                                       const T *__restrict__ L, T *__restrict__ Y, T *__restrict__ W,
+                                                       T *__restrict__ Z, int nrhs) {
+    const PanelTask t = tasks[blockIdx.x];
     const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s];
-    const int64_t r1 = min<int64_t>(t.r0 + kTile, ms);
+    const int64_t ns = ncol[s], ms = mrow[s], k = t.k, c0 = col0[s];
+    const int b = (int)min<int64_t>(kPanel, ns - k);
     const T *Ls = L + lofs[s];
-    const T *Ys = Y + col0[s] * nrhs;
-    T *Ws = W + relofs[s] * nrhs;
-    for (int64_t q = threadIdx.x; q < (r1 - t.r0) * nrhs; q += blockDim.x) {
-        int64_t i = t.r0 + q % (r1 - t.r0);
-        int r = (int)(q / (r1 - t.r0));
-        T v = T(0);
-        for (int64_t j = 0; j < ns; ++j) v += Ls[i + ms * j] * Ys[j * nrhs + r];
-        Ws[(i - ns) * nrhs + r] -= v;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (w >= nrhs) return;                    // whole wave idle (uniform)
+    T lr[kPanel];                             // row `lane` of L_kk
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) lr[j] = (j < b && j <= lane && lane < b) ? Ls[(k + lane) + ms * (k + j)] : T(0);
+    T y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = w + 4 * q;
+        y[q] = (r < nrhs && lane < b) ? Y[(c0 + k + lane) * nrhs + r] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        if (j < b) {
+            const T inv = T(1) / bcast(lr[j], j);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (w + 4 * q < nrhs) {
+                    const T yj = bcast(y[q], j) * inv;
+                    if (lane == j) y[q] = yj;
+                    else if (lane > j) y[q] -= lr[j] * yj;
+                }
+            }
+        }
+    }
+    if (t.r0 == k && lane < b) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = w + 4 * q;
+            if (r < nrhs) Z[(c0 + k + lane) * nrhs + r] = y[q];
+        }
+    }
+    const int64_t rs = max<int64_t>(t.r0, k + b), re = min<int64_t>(t.r0 + kTile, ms);
+    if (rs >= re) return;
+    const int64_t row = t.r0 + lane;
+    const bool act = row >= rs && row < re;
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        if (j < b) {
+            const T l = act ? Ls[row + ms * (k + j)] : T(0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (w + 4 * q < nrhs) acc[q] += l * bcast(y[q], j);
+        }
+    }
+    if (act) {
+        T *dst = row < ns ? (Y + (c0 + row) * nrhs) : (W + (relofs[s] + row - ns) * nrhs);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = w + 4 * q;
+            if (r < nrhs) dst[r] -= acc[q];
+        }
     }
 }
 
-// y_s[cols] -= L21[:, cols]' x[rows], 64-column tiles
+// backward, per column tile (s, 0, c0) of a supernode with update rows:
+// Z[cols] -= L[ns..ms, cols]' X[rows] (X at the ancestors' rows is final)
 template <class T>
 __global__ void __launch_bounds__(kBS) bwd_offdiag_kernel(const PanelTask *__restrict__ tiles, const int64_t *__restrict__ col0,
                                                           const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
                                                           const int64_t *__restrict__ lofs, const int64_t *__restrict__ rofs,
                                                           const int32_t *__restrict__ rows, const T *__restrict__ L,
-                                                          T *__restrict__ Y, int nrhs) {
-    __shared__ T part[4][kTile][4];
+                                                          const T *__restrict__ X, T *__restrict__ Z, int nrhs) {
+    __shared__ T Lt[kTile][kTile + 1];
+    __shared__ T xs[kTile][17];
+    __shared__ T red[4][kTile][17];
     const PanelTask t = tiles[blockIdx.x];
     const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s];
-    const int64_t c1 = min<int64_t>(t.r0 + kTile, ns);
+    const int64_t ns = ncol[s], ms = mrow[s], ct = t.r0;
+    const int nc = (int)min<int64_t>(kTile, ns - ct);
     const T *Ls = L + lofs[s];
     const int32_t *rw = rows + rofs[s];
-    T *Ys = Y + col0[s] * nrhs;
-    const int tid = threadIdx.x;
-    const int lane = tid % 64, ph = tid / 64;    // 4 row phases
-    for (int rb = 0; rb < nrhs; rb += 4) {
-        const int nrb = min(4, nrhs - rb);
-        T acc[4] = {T(0), T(0), T(0), T(0)};
-        const int64_t j = t.r0 + lane;
-        if (j < c1) {
-            for (int64_t i = ns + ph; i < ms; i += 4) {
-                const T l = Ls[i + ms * j];
-                const int64_t g = rw[i];
-                for (int r = 0; r < nrb; ++r) acc[r] += l * Y[g * nrhs + rb + r];
-            }
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    T acc[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = T(0);
+    for (int64_t i0 = ns; i0 < ms; i0 += kTile) {
+        const int nr = (int)min<int64_t>(kTile, ms - i0);
+        __syncthreads();
+        for (int q = tid; q < kTile * kTile; q += blockDim.x) {
+            const int i = q % kTile, c = q / kTile;
+            Lt[i][c] = (i < nr && c < nc) ? Ls[(i0 + i) + ms * (ct + c)] : T(0);
         }
-        for (int r = 0; r < 4; ++r) part[ph][lane][r] = acc[r];
+        for (int q = tid; q < kTile * nrhs; q += blockDim.x) {
+            const int i = q / nrhs, r = q % nrhs;
+            xs[i][r] = i < nr ? X[(int64_t)rw[i0 + i] * nrhs + r] : T(0);
+        }
         __syncthreads();
-        if (ph == 0 && j < c1)
-            for (int r = 0; r < nrb; ++r)
-                Ys[j * nrhs + rb + r] -= ((part[0][lane][r] + part[1][lane][r]) + part[2][lane][r]) + part[3][lane][r];
-        __syncthreads();
+        for (int i = w; i < nr; i += 4) {
+            const T l = Lt[i][lane];
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                if (r < nrhs) acc[r] += l * xs[i][r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) red[w][lane][r] = acc[r];
+    __syncthreads();
+    if (w == 0 && lane < nc) {
+        T *dst = Z + (col0[s] + ct + lane) * nrhs;
+        for (int r = 0; r < nrhs; ++r) dst[r] -= ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
     }
 }
 
-// back substitution with L11'
+// backward step task (s, k, j0): x_k = L_kk^{-T} (pending rows k..k+b of Z), written
+// to X by the diagonal task (j0 == k); for j0 < k, Z[j0..j0+kTile) -= L[k..k+b, j0..]' x_k.
 template <class T>
-__global__ void __launch_bounds__(kBS) bwd_diag_kernel(const int32_t *__restrict__ level_sn, const int64_t *__restrict__ col0,
+__global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ col0,
                                                        const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
                                                        const int64_t *__restrict__ lofs, const T *__restrict__ L,
-                                                       T *__restrict__ Y, int nrhs) {
-    const int s = level_sn[blockIdx.x];
-    const int64_t ns = ncol[s], ms = mrow[s], c0 = col0[s];
+                                                       T *__restrict__ Z, T *__restrict__ X, int nrhs) {
+    __shared__ T D[kPanel][kPanel + 1];      // D[i][c] = L[k+i][k+c]
+    __shared__ T E[kPanel][kTile + 1];       // E[i][c] = L[k+i][j0+c]
+    const PanelTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s], k = t.k, c0 = col0[s], j0 = t.r0;
+    const int b = (int)min<int64_t>(kPanel, ns - k);
     const T *Ls = L + lofs[s];
-    T *Ys = Y + c0 * nrhs;
-    const int tid = threadIdx.x;
-    const int64_t nblk = (ns + kPanel - 1) / kPanel;
-    for (int64_t bi = nblk - 1; bi >= 0; --bi) {
-        const int64_t jb = bi * kPanel;
-        const int bsz = (int)min<int64_t>(kPanel, ns - jb);
-        // contributions of already-solved later columns within L11
-        for (int64_t q = tid; q < (int64_t)bsz * nrhs; q += blockDim.x) {
-            int64_t j = jb + q / nrhs;
-            int r = (int)(q % nrhs);
-            T v = T(0);
-            for (int64_t i = jb + bsz; i < ns; ++i) v += Ls[i + ms * j] * Ys[i * nrhs + r];
-            Ys[j * nrhs + r] -= v;
-        }
-        __syncthreads();
-        for (int j = bsz - 1; j >= 0; --j) {
-            const int64_t jj = jb + j;
-            const T d = Ls[jj + ms * jj];
-            for (int r = tid; r < nrhs; r += blockDim.x) Ys[jj * nrhs + r] /= d;
-            __syncthreads();
-            for (int64_t q = tid; q < (int64_t)j * nrhs; q += blockDim.x) {
-                int64_t i = jb + q / nrhs;
-                int r = (int)(q % nrhs);
-                Ys[i * nrhs + r] -= Ls[jj + ms * i] * Ys[jj * nrhs + r];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const bool diag = j0 == k;
+    for (int q = tid; q < kPanel * kPanel; q += blockDim.x) {
+        const int i = q % kPanel, c = q / kPanel;
+        D[i][c] = (i < b && c <= i) ? Ls[(k + i) + ms * (k + c)] : T(0);
+        if (!diag) E[i][c] = i < b ? Ls[(k + i) + ms * (j0 + c)] : T(0);
+    }
+    __syncthreads();
+    if (w >= nrhs) return;
+    T lc[kPanel];                             // column `lane` of L_kk
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) lc[i] = D[i][lane];
+    T z[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = w + 4 * q;
+        z[q] = (r < nrhs && lane < b) ? Z[(c0 + k + lane) * nrhs + r] : T(0);
+    }
+#pragma unroll
+    for (int j = kPanel - 1; j >= 0; --j) {
+        if (j < b) {
+            const T inv = T(1) / bcast(lc[j], j);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (w + 4 * q < nrhs) {
+                    const T xj = bcast(z[q], j) * inv;
+                    if (lane == j) z[q] = xj;
+                    else if (lane < j) z[q] -= lc[j] * xj;
+                }
             }
-            __syncthreads();
         }
+    }
+    if (diag) {
+        if (lane < b) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = w + 4 * q;
+                if (r < nrhs) X[(c0 + k + lane) * nrhs + r] = z[q];
+            }
+        }
+        return;
+    }
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) {
+        if (i < b) {
+            const T e = E[i][lane];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (w + 4 * q < nrhs) acc[q] += e * bcast(z[q], i);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = w + 4 * q;
+        if (r < nrhs) Z[(c0 + j0 + lane) * nrhs + r] -= acc[q];
     }
 }
 
@@ -484,34 +580,72 @@ __global__ void to_t_kernel(int64_t n, const double *__restrict__ a, T *__restri
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) o[q] = (T)a[q];
 }
 
-// X += (double) Y, and scatter X to the global interleaved vector xg (ndof x nrhs)
+// The refined iterate is a double-double (hi + lo, |lo| <= ulp(hi)/2) so that its
+// residual can fall below the ~2^-53 |K||x| floor of any vector stored in double.
+// X += (double) Y in double-double, and scatter to the global interleaved
+// vectors xg_hi / xg_lo (ndof x nrhs, zero on fixed DOFs).
+// Error-free transformations need every product and sum rounded on its own:
+// contraction into fma is switched off where they are used.
+__device__ inline void two_sum(double a, double b, double &s, double &e) {
+#pragma clang fp contract(off)
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
 template <class T>
 __global__ void accumulate_kernel(int64_t n, int nrhs, const int32_t *__restrict__ perm, const T *__restrict__ Y,
-                                  double *__restrict__ X, double *__restrict__ xg, const int32_t *__restrict__ active) {
+                                  double *__restrict__ Xh, double *__restrict__ Xl, double *__restrict__ xgh,
+                                  double *__restrict__ xgl, const int32_t *__restrict__ active) {
+#pragma clang fp contract(off)
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n * nrhs; q += (int64_t)gridDim.x * blockDim.x) {
         int r = (int)(q % nrhs);
         if (!active[r]) continue;
-        double v = X[q] + (double)Y[q];
-        X[q] = v;
-        xg[(int64_t)perm[q / nrhs] * nrhs + r] = v;
+        double s, e;
+        two_sum(Xh[q], (double)Y[q], s, e);
+        e += Xl[q];
+        const double h = s + e;
+        const double l = e - (h - s);
+        Xh[q] = h;
+        Xl[q] = l;
+        const int64_t g = (int64_t)perm[q / nrhs] * nrhs + r;
+        xgh[g] = h;
+        xgl[g] = l;
     }
 }
 
-// R = Bp - K_ff x (FP64); xg is zero on fixed DOFs so the full CSR row can be used
-// after masking fixed columns through xg.
+// R = Bp - K_ff x for x = xgh + xgl (xgl may be null: x = xgh), rows of K_ff in
+// elimination order.  Products are split exactly with fma and summed with
+// two_sum, so R is the exact residual rounded once to double.  xg is zero on
+// fixed DOFs, so the full CSR row can be used.
 __global__ void residual_kernel(int64_t n, int nrhs, const int32_t *__restrict__ perm, const int64_t *__restrict__ rowptr,
                                 const int32_t *__restrict__ col, const double *__restrict__ val,
-                                const double *__restrict__ xg, const double *__restrict__ Bp, double *__restrict__ R) {
+                                const double *__restrict__ xgh, const double *__restrict__ xgl,
+                                const double *__restrict__ Bp, double *__restrict__ R) {
+#pragma clang fp contract(off)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t g = perm[i];
-        double acc[16];
-        for (int r = 0; r < nrhs; ++r) acc[r] = 0.0;
+        double hi[16], lo[16];
+        for (int r = 0; r < nrhs; ++r) { hi[r] = 0.0; lo[r] = 0.0; }
         for (int64_t p = rowptr[g]; p < rowptr[g + 1]; ++p) {
             const double kv = val[p];
             const int64_t c = col[p];
-            for (int r = 0; r < nrhs; ++r) acc[r] += kv * xg[c * nrhs + r];
+            for (int r = 0; r < nrhs; ++r) {
+                const double xv = xgh[c * nrhs + r];
+                const double pr = kv * xv;
+                const double pe = fma(kv, xv, -pr);
+                double s, e;
+                two_sum(hi[r], pr, s, e);
+                hi[r] = s;
+                lo[r] += e + pe;
+                if (xgl) lo[r] = fma(kv, xgl[c * nrhs + r], lo[r]);
+            }
         }
-        for (int r = 0; r < nrhs; ++r) R[i * nrhs + r] = Bp[i * nrhs + r] - acc[r];
+        for (int r = 0; r < nrhs; ++r) {
+            double t, te;
+            two_sum(Bp[i * nrhs + r], -hi[r], t, te);
+            R[i * nrhs + r] = t + (te - lo[r]);
+        }
     }
 }
 
@@ -622,28 +756,32 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
 }
 
 template <class T>
-hipError_t solve_t(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, int nrhs) {
+hipError_t solve_t(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, int nrhs) {
     hipStream_t st = ctx->stream;
     const SymbolicHost &H = S->H;
     const int64_t nlev = (int64_t)H.level_ptr.size() - 1;
     for (int64_t l = 0; l < nlev; ++l) {
-        int64_t nsn = H.level_ptr[l + 1] - H.level_ptr[l];
-        hipLaunchKernelGGL(fwd_diag_kernel<T>, dim3((unsigned)nsn), dim3(kBS), 0, st, S->d_level_sn + H.level_ptr[l],
-                           S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, S->d_relmap, S->d_child_ptr, S->d_child,
-                           L, Y, W, nrhs);
-        int64_t nt = H.fwd_off[l + 1] - H.fwd_off[l];
-        if (nt)
-            hipLaunchKernelGGL(fwd_offdiag_kernel<T>, dim3((unsigned)nt), dim3(kBS), 0, st, S->d_fwd + H.fwd_off[l],
-                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L, Y, W, nrhs);
+        const int64_t nsn = H.level_ptr[l + 1] - H.level_ptr[l];
+        hipLaunchKernelGGL(fwd_gather_kernel<T>, dim3((unsigned)nsn), dim3(kBS), 0, st, S->d_level_sn + H.level_ptr[l],
+                           S->d_col0, S->d_ncol, S->d_m, S->d_relofs, S->d_relmap, S->d_child_ptr, S->d_child, Y, W, nrhs);
+        for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
+            const int64_t nt = H.pstep_off[stp + 1] - H.pstep_off[stp];
+            if (nt)
+                hipLaunchKernelGGL(fwd_step_kernel<T>, dim3((unsigned)nt), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
+                                   S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L, Y, W, Z, nrhs);
+        }
     }
     for (int64_t l = nlev - 1; l >= 0; --l) {
-        int64_t nt = H.bwd_off[l + 1] - H.bwd_off[l];
+        const int64_t nt = H.bwd_off[l + 1] - H.bwd_off[l];
         if (nt)
             hipLaunchKernelGGL(bwd_offdiag_kernel<T>, dim3((unsigned)nt), dim3(kBS), 0, st, S->d_bwd + H.bwd_off[l],
-                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_rofs, S->d_rows, L, Y, nrhs);
-        int64_t nsn = H.level_ptr[l + 1] - H.level_ptr[l];
-        hipLaunchKernelGGL(bwd_diag_kernel<T>, dim3((unsigned)nsn), dim3(kBS), 0, st, S->d_level_sn + H.level_ptr[l],
-                           S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, Y, nrhs);
+                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_rofs, S->d_rows, L, (const T *)X, Z, nrhs);
+        for (int64_t stp = H.bstep_ptr[l]; stp < H.bstep_ptr[l + 1]; ++stp) {
+            const int64_t nb = H.bstep_off[stp + 1] - H.bstep_off[stp];
+            if (nb)
+                hipLaunchKernelGGL(bwd_step_kernel<T>, dim3((unsigned)nb), dim3(kBS), 0, st, S->d_btasks + H.bstep_off[stp],
+                                   S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, Z, X, nrhs);
+        }
     }
     return hipGetLastError();
 }
@@ -710,7 +848,7 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_ptasks, H.ptasks, st);
     if (e == hipSuccess) e = up(S, &S->d_utasks, H.utasks, st);
     if (e == hipSuccess) e = up(S, &S->d_etasks, H.etasks, st);
-    if (e == hipSuccess) e = up(S, &S->d_fwd, H.fwd_tiles, st);
+    if (e == hipSuccess) e = up(S, &S->d_btasks, H.btasks, st);
     if (e == hipSuccess) e = up(S, &S->d_bwd, H.bwd_tiles, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -835,12 +973,14 @@ int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_mat
 }
 
 int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_dev,
-                   double *x_dev, double tol, int max_refine, int32_t *iters, double *relres) {
+                   double *x_dev, double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev,
+                   double *relres_out) {
     const jfem_symbolic *S = F->S;
     const SymbolicHost &H = S->H;
     const jfem_pattern *P = K->P;
     if (K->P != S->P) return fail(ctx, JFEM_ERR_BAD_ARG, "matrix and factor use different patterns");
     if (nrhs < 1 || nrhs > 16) return fail(ctx, JFEM_ERR_BAD_ARG, "nrhs must be in 1..16");
+    if (max_refine < 0) return fail(ctx, JFEM_ERR_BAD_ARG, "max_refine must be >= 0");
     const int64_t n = H.n, ndof = P->ndof;
     const size_t es = F->precision == 64 ? 8 : 4;
     const int nblk = 256;
@@ -848,24 +988,31 @@ int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, in
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     size_t oY = take(n * nrhs * es), oW = take(std::max<int64_t>(H.upd_rows, 1) * nrhs * es);
-    size_t oX = take(n * nrhs * 8), oR = take(n * nrhs * 8), oB = take(n * nrhs * 8), oXg = take(ndof * nrhs * 8);
+    size_t oZ = take(n * nrhs * es), oXs = take(n * nrhs * es);
+    size_t oXh = take(n * nrhs * 8), oXl = take(n * nrhs * 8), oR = take(n * nrhs * 8), oB = take(n * nrhs * 8);
+    size_t oXgh = take(ndof * nrhs * 8), oXgl = take(ndof * nrhs * 8);
     size_t oPart = take((size_t)nblk * nrhs * 8), oNrm = take(16 * 8), oAct = take(16 * 4);
     JFEM_CHECK_HIP(ctx, ensure_scratch(ctx, off));
     char *base = (char *)ctx->scratch;
-    void *Y = base + oY, *W = base + oW;
-    double *X = (double *)(base + oX), *R = (double *)(base + oR), *Bp = (double *)(base + oB), *xg = (double *)(base + oXg);
+    void *Y = base + oY, *W = base + oW, *Zs = base + oZ, *Xs = base + oXs;
+    double *Xh = (double *)(base + oXh), *Xl = (double *)(base + oXl), *R = (double *)(base + oR);
+    double *Bp = (double *)(base + oB), *xgh = (double *)(base + oXgh), *xgl = (double *)(base + oXgl);
     double *part = (double *)(base + oPart), *nrm = (double *)(base + oNrm);
     int32_t *act = (int32_t *)(base + oAct);
     hipStream_t st = ctx->stream;
-    if (n > 0) hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, X, n * nrhs);
-    hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, xg, ndof * nrhs);
+    if (n > 0) {
+        hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, Xh, n * nrhs);
+        hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, Xl, n * nrhs);
+    }
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, xgh, ndof * nrhs);
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, xgl, ndof * nrhs);
     if (n > 0) {
         hipLaunchKernelGGL(gather_rhs_kernel, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n, nrhs, ndof, S->d_perm, b_dev, Bp);
         JFEM_CHECK_HIP(ctx, hipMemcpyAsync(R, Bp, n * nrhs * 8, hipMemcpyDeviceToDevice, st));
     }
-    double bn[16], rn[16];
+    double bn[16], rn[16], prev[16];
     int32_t active[16];
-    for (int r = 0; r < nrhs; ++r) { active[r] = 1; iters[r] = 0; relres[r] = 0.0; }
+    for (int r = 0; r < nrhs; ++r) { active[r] = 1; iters[r] = 0; relres[r] = 0.0; prev[r] = HUGE_VAL; }
     auto norms = [&](const double *v, double *outn) -> hipError_t {
         hipLaunchKernelGGL(sumsq_partial_kernel, dim3(nblk), dim3(kBS), 0, st, n, nrhs, v, part);
         hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), dim3(64), 0, st, nblk, nrhs, part, nrm);
@@ -878,39 +1025,60 @@ int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, in
         for (int r = 0; r < nrhs; ++r) bn[r] = std::sqrt(bn[r]);
     }
     bool converged = n == 0;
+    bool all_ok = true;
     for (int it = 0; n > 0 && it <= max_refine; ++it) {
         JFEM_CHECK_HIP(ctx, hipMemcpyAsync(act, active, nrhs * 4, hipMemcpyHostToDevice, st));
         timer_start(ctx);
         if (F->precision == 64) {
             JFEM_CHECK_HIP(ctx, hipMemcpyAsync(Y, R, n * nrhs * 8, hipMemcpyDeviceToDevice, st));
-            JFEM_CHECK_HIP(ctx, solve_t<double>(ctx, S, (const double *)F->d_L, (double *)Y, (double *)W, nrhs));
+            JFEM_CHECK_HIP(ctx, solve_t<double>(ctx, S, (const double *)F->d_L, (double *)Y, (double *)W, (double *)Zs,
+                                                (double *)Xs, nrhs));
             hipLaunchKernelGGL(accumulate_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n, nrhs, S->d_perm,
-                               (const double *)Y, X, xg, act);
+                               (const double *)Xs, Xh, Xl, xgh, xgl, act);
         } else {
             hipLaunchKernelGGL(to_t_kernel<float>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n * nrhs, R, (float *)Y);
-            JFEM_CHECK_HIP(ctx, solve_t<float>(ctx, S, (const float *)F->d_L, (float *)Y, (float *)W, nrhs));
+            JFEM_CHECK_HIP(ctx, solve_t<float>(ctx, S, (const float *)F->d_L, (float *)Y, (float *)W, (float *)Zs,
+                                               (float *)Xs, nrhs));
             hipLaunchKernelGGL(accumulate_kernel<float>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n, nrhs, S->d_perm,
-                               (const float *)Y, X, xg, act);
+                               (const float *)Xs, Xh, Xl, xgh, xgl, act);
         }
         timer_stop(ctx, FAM_SOLVE, 1);
         timer_start(ctx);
         hipLaunchKernelGGL(residual_kernel, dim3(grid_for(n)), dim3(kBS), 0, st, n, nrhs, S->d_perm, P->d_rowptr, P->d_col,
-                           K->d_val, xg, Bp, R);
+                           K->d_val, xgh, xgl, Bp, R);
         timer_stop(ctx, FAM_SPMV, 1);
         JFEM_CHECK_HIP(ctx, norms(R, rn));
         converged = true;
+        all_ok = true;
         for (int r = 0; r < nrhs; ++r) {
-            double rr = std::sqrt(rn[r]) / std::max(bn[r], 1e-30);
-            if (active[r]) { relres[r] = rr; iters[r] = it; }
-            if (active[r] && !(rr <= tol)) converged = false;
-            else active[r] = 0;
+            if (!active[r]) { if (!(relres[r] <= tol)) all_ok = false; continue; }
+            const double rr = std::sqrt(rn[r]) / std::max(bn[r], 1e-30);
+            relres[r] = rr;
+            iters[r] = it;
+            if (rr <= tol) { active[r] = 0; continue; }
+            // stop a right-hand side whose residual no longer decreases
+            if (!(rr < 0.9 * prev[r])) { active[r] = 0; all_ok = false; continue; }
+            prev[r] = rr;
+            converged = false;
         }
         if (converged) break;
     }
-    hipLaunchKernelGGL(export_kernel, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, ndof, nrhs, xg, x_dev);
+    if (!converged) all_ok = false;
+    hipLaunchKernelGGL(export_kernel, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, ndof, nrhs, xgh, x_dev);
+    if (xlo_dev) hipLaunchKernelGGL(export_kernel, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, ndof, nrhs, xgl, xlo_dev);
+    if (relres_out) {
+        // residual of the double vector handed back (x_hi alone)
+        for (int r = 0; r < nrhs; ++r) relres_out[r] = 0.0;
+        if (n > 0) {
+            hipLaunchKernelGGL(residual_kernel, dim3(grid_for(n)), dim3(kBS), 0, st, n, nrhs, S->d_perm, P->d_rowptr,
+                               P->d_col, K->d_val, xgh, (const double *)nullptr, Bp, R);
+            JFEM_CHECK_HIP(ctx, norms(R, rn));
+            for (int r = 0; r < nrhs; ++r) relres_out[r] = std::sqrt(rn[r]) / std::max(bn[r], 1e-30);
+        }
+    }
     JFEM_CHECK_HIP(ctx, hipGetLastError());
     JFEM_CHECK_HIP(ctx, hipStreamSynchronize(st));
-    if (!converged) return fail(ctx, JFEM_ERR_NOT_CONVERGED, "refinement did not reach the requested residual");
+    if (!all_ok) return fail(ctx, JFEM_ERR_NOT_CONVERGED, "refinement did not reach the requested residual");
     return JFEM_OK;
 }
 
@@ -988,17 +1156,21 @@ int jfem_selftest_update(jfem_ctx *ctx, int precision, double *max_rel_diff) {
 }
 
 int jfem_solve(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_host, double *x_host,
-               double tol, int max_refine, int32_t *iters, double *relres) {
+               double tol, int max_refine, int32_t *iters, double *relres, double *xlo_host, double *relres_out) {
     const int64_t ndof = K->P->ndof;
-    double *d_b = nullptr, *d_x = nullptr;
-    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_b, std::max<int64_t>(ndof * nrhs, 1) * 8));
-    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_x, std::max<int64_t>(ndof * nrhs, 1) * 8));
+    const int64_t cnt = std::max<int64_t>(ndof * nrhs, 1);
+    double *d_b = nullptr, *d_x = nullptr, *d_xl = nullptr;
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_b, cnt * 8));
+    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_x, cnt * 8));
+    if (xlo_host) JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_xl, cnt * 8));
     JFEM_CHECK_HIP(ctx, hipMemcpyAsync(d_b, b_host, ndof * nrhs * 8, hipMemcpyHostToDevice, ctx->stream));
-    int st = jfem_solve_dev(ctx, F, K, nrhs, d_b, d_x, tol, max_refine, iters, relres);
+    int st = jfem_solve_dev(ctx, F, K, nrhs, d_b, d_x, tol, max_refine, iters, relres, d_xl, relres_out);
     hipError_t e = hipMemcpyAsync(x_host, d_x, ndof * nrhs * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && xlo_host) e = hipMemcpyAsync(xlo_host, d_xl, ndof * nrhs * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d_b);
     (void)hipFree(d_x);
+    if (d_xl) (void)hipFree(d_xl);
     if (e != hipSuccess) return fail(ctx, JFEM_ERR_HIP, hipGetErrorString(e));
     return st;
 }

@@ -151,8 +151,9 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         S.pstep_off.assign(1, 0);
         S.ustep_off.assign(1, 0);
         S.ea_off.assign(1, 0);
-        S.fwd_off.assign(1, 0);
         S.bwd_off.assign(1, 0);
+        S.bstep_ptr.assign(1, 0);
+        S.bstep_off.assign(1, 0);
         S.iperm.assign(ndof, -1);
         return 0;
     }
@@ -485,11 +486,11 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
     S.step_ptr.assign(nlev + 1, 0);
     S.pstep_off.clear(); S.ustep_off.clear(); S.pstep_mid.clear();
     S.ea_off.assign(nlev + 1, 0);
-    S.fwd_off.assign(nlev + 1, 0);
     S.bwd_off.assign(nlev + 1, 0);
+    S.bstep_ptr.assign(nlev + 1, 0);
+    S.bstep_off.clear();
     for (int32_t l = 0; l < nlev; ++l) {
         S.ea_off[l] = (int64_t)S.etasks.size();
-        S.fwd_off[l] = (int64_t)S.fwd_tiles.size();
         S.bwd_off[l] = (int64_t)S.bwd_tiles.size();
         int64_t maxsteps = 0;
         for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
@@ -498,9 +499,21 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
             if (S.child_ptr[s + 1] > S.child_ptr[s])
                 for (int64_t c0 = 0; c0 < S.m[s]; c0 += kTile)
                     S.etasks.push_back({s, (int32_t)c0, (int32_t)std::min<int64_t>(c0 + kTile, S.m[s]), 0});
-            for (int64_t r0 = S.ncol[s]; r0 < S.m[s]; r0 += kTile) S.fwd_tiles.push_back({s, 0, (int32_t)r0, 0});
             if (S.m[s] > S.ncol[s])
                 for (int64_t c0 = 0; c0 < S.ncol[s]; c0 += kTile) S.bwd_tiles.push_back({s, 0, (int32_t)c0, 0});
+        }
+        // backward steps: step t handles block nb_s - 1 - t of every supernode s
+        S.bstep_ptr[l] = (int64_t)S.bstep_off.size();
+        for (int64_t t = 0; t < maxsteps; ++t) {
+            S.bstep_off.push_back((int64_t)S.btasks.size());
+            for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+                int32_t s = S.level_sn[q];
+                int64_t nb = (S.ncol[s] + kPanel - 1) / kPanel;
+                int64_t kb = nb - 1 - t;
+                if (kb < 0) continue;
+                int64_t k = kb * kPanel;
+                for (int64_t j0 = 0; j0 <= k; j0 += kTile) S.btasks.push_back({s, (int32_t)k, (int32_t)j0, 0});
+            }
         }
         S.step_ptr[l] = (int64_t)S.pstep_off.size();
         for (int64_t t = 0; t < maxsteps; ++t) {
@@ -539,8 +552,9 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
     S.pstep_off.push_back((int64_t)S.ptasks.size());
     S.ustep_off.push_back((int64_t)S.utasks.size());
     S.ea_off[nlev] = (int64_t)S.etasks.size();
-    S.fwd_off[nlev] = (int64_t)S.fwd_tiles.size();
     S.bwd_off[nlev] = (int64_t)S.bwd_tiles.size();
+    S.bstep_ptr[nlev] = (int64_t)S.bstep_off.size();
+    S.bstep_off.push_back((int64_t)S.btasks.size());
     return 0;
 }
 

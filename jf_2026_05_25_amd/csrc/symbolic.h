@@ -35,9 +35,16 @@ struct SymbolicHost {
     std::vector<int64_t> pstep_off, ustep_off;   // per step: offset (size steps+1)
     std::vector<int64_t> pstep_mid;              // per step: first non-diagonal panel tile
     std::vector<int64_t> ea_off;            // per level: offset into etasks (size levels+1)
-    // solve task lists: row tiles of L21 per level (forward), column tiles (backward)
-    std::vector<PanelTask> fwd_tiles, bwd_tiles;
-    std::vector<int64_t> fwd_off, bwd_off;  // per level
+    // Solve task lists.  Forward: per level one gather (children's update
+    // vectors), then the panel steps reuse ptasks: task (s, k, r0) solves the
+    // diagonal block at column k and updates rows [max(r0, k+b), r0+kTile).
+    // Backward: per level the column tiles of L21' (bwd_tiles, (s, 0, c0)), then
+    // steps over the diagonal blocks from the last: task (s, k, j0) solves block k
+    // (j0 == k) or updates columns [j0, j0+kTile) of the pending vector (j0 < k).
+    std::vector<PanelTask> bwd_tiles, btasks;
+    std::vector<int64_t> bwd_off;            // per level: offset into bwd_tiles
+    std::vector<int64_t> bstep_ptr;          // per level: first backward step
+    std::vector<int64_t> bstep_off;          // per backward step: offset into btasks
     int64_t L_entries = 0, U_entries = 0, upd_rows = 0;
     double flops = 0;
     double update_flops = 0;   // 2 x multiply-adds of the update tiles (lower triangles)

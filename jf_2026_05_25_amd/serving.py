@@ -74,6 +74,7 @@ def matrix_key(model) -> str:
     h = hashlib.sha1(topology_key(model).encode())
     for a in (model.quad_props, model.tria_props, model.tet_props):
         h.update(np.ascontiguousarray(a).view(np.uint8))
+    h.update(np.ascontiguousarray(model.spc, np.uint8).tobytes())
     h.update(repr(model.k6rot).encode())
     return h.hexdigest()
 
@@ -90,6 +91,8 @@ class SolveServer:
         self.lock = threading.Lock()
         self.mem_cv = threading.Condition(self.lock)
         self.committed = 0            # bytes promised to admitted requests
+        self.active = 0               # requests holding an admission
+        self.peak_active = 0
         self.topo = {}                # topology key -> (DeviceMesh, Pattern, {fixed hash: Symbolic})
         self.factors = OrderedDict()  # matrix key -> (DeviceMesh, DeviceMatrix, Factor, fixed)
         self.max_cached_factors = max_cached_factors
@@ -121,14 +124,17 @@ class SolveServer:
         with self.mem_cv:
             while True:
                 free, total = self.ctxs[0].memory()
-                if free - self.committed - self.reserve >= nbytes or self.committed == 0:
+                if free - self.committed - self.reserve >= nbytes or self.active == 0:
                     self.committed += nbytes
+                    self.active += 1
+                    self.peak_active = max(self.peak_active, self.active)
                     return
                 self.mem_cv.wait(timeout=0.05)
 
     def _release_bytes(self, nbytes: int):
         with self.mem_cv:
             self.committed -= nbytes
+            self.active -= 1
             self.mem_cv.notify_all()
 
     def process(self, ctx, req: Request, t0: float) -> Result:
@@ -144,7 +150,7 @@ class SolveServer:
                     self.factors.move_to_end(mkey)
             if cached is not None:
                 mesh_p, K, F, fixed = cached
-                u, rr, it, ok = F.solve(K, m.loads, tol=self.tol, max_refine=20)
+                u, rr, it, ok = F.solve(K, m.loads, tol=self.tol, max_refine=20, ctx=ctx)
                 tim["solve_s"] = time.perf_counter() - ta
                 return Result(req.rid, u, rr, it, req.arrival, t_admit, time.perf_counter() - t0, m.ndof,
                               int((fixed == 0).sum()), m.nrhs, bool(ok), "" if ok else "not converged", True, tim)
@@ -178,7 +184,7 @@ class SolveServer:
                     F = solver.Factor(S, K, req.precision, ctx)
                 tim["factor_s"] = time.perf_counter() - tc
                 ts = time.perf_counter()
-                u, rr, it, ok = F.solve(K, m.loads, tol=self.tol, max_refine=20)
+                u, rr, it, ok = F.solve(K, m.loads, tol=self.tol, max_refine=20, ctx=ctx)
                 tim["solve_s"] = time.perf_counter() - ts
                 with self.lock:
                     self.factors[mkey] = (mesh_r, K, F, fixed)

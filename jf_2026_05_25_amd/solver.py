@@ -305,7 +305,16 @@ class Factor:
     def device_bytes(self):
         return int(self.ctx.L.jfem_factor_device_bytes(self.h))
 
-    def solve(self, K: DeviceMatrix, B: np.ndarray, tol: float = 1e-10, max_refine: int = 10):
+    def solve(self, K: DeviceMatrix, B: np.ndarray, tol: float = 1e-10, max_refine: int = 10,
+              ctx: Context | None = None, details: bool = False):
+        """Solve K_ff X = B_ff with refinement on ``ctx``'s stream (default: the
+        factor's own context).  Solve scratch lives in the context, so concurrent
+        solves with one factor must use different contexts.
+
+        Returns ``(X, relres, iters, ok)``; with ``details`` also a dict holding
+        ``x_lo`` (low parts of the double-double solution) and ``relres_fp64``
+        (residual of the returned double X itself)."""
+        ctx = ctx or self.ctx
         B = np.asarray(B, float)
         one = B.ndim == 1
         if one:
@@ -313,13 +322,18 @@ class Factor:
         nrhs = B.shape[1]
         Bc = np.ascontiguousarray(B.T)        # column-major ndof x nrhs
         X = np.zeros_like(Bc)
-        iters = np.zeros(nrhs, np.int32); rr = np.zeros(nrhs)
-        code = self.ctx.L.jfem_solve(self.ctx.h, self.h, K.h, nrhs, ptr(Bc), ptr(X), tol, max_refine, ptr(iters),
-                                     ptr(rr))
+        Xl = np.zeros_like(Bc) if details else None
+        iters = np.zeros(nrhs, np.int32); rr = np.zeros(nrhs); rr64 = np.zeros(nrhs)
+        code = ctx.L.jfem_solve(ctx.h, self.h, K.h, nrhs, ptr(Bc), ptr(X), tol, max_refine, ptr(iters), ptr(rr),
+                                ptr(Xl) if details else None, ptr(rr64))
         if code not in (0, 8):
-            check(self.ctx.h, code)
+            check(ctx.h, code)
         X = X.T.copy()
-        return (X[:, 0] if one else X), rr, iters, code == 0
+        out = ((X[:, 0] if one else X), rr, iters, code == 0)
+        if details:
+            Xl = Xl.T.copy()
+            return out + ({"x_lo": Xl[:, 0] if one else Xl, "relres_fp64": rr64},)
+        return out
 
     def free(self):
         if self.h:
@@ -384,7 +398,7 @@ def apply_bc_and_solve(asm: AssembledStiffness, model, F_applied=None, *, precis
         diag["linear_solver"]["cache_hit"] = True
     fixed, S, Fct, t_sym, t_num, n_post = entry
     t3 = time.perf_counter()
-    u, rr, iters, ok = Fct.solve(asm.K, F, tol=tol, max_refine=max_refine)
+    u, rr, iters, ok, extra = Fct.solve(asm.K, F, tol=tol, max_refine=max_refine, details=True)
     t4 = time.perf_counter()
     spc = np.asarray(model.spc, np.uint8)
     diag["bc_partition"].update({"free_dofs": int((fixed == 0).sum()), "fixed_dofs": int(fixed.sum()),
@@ -392,6 +406,7 @@ def apply_bc_and_solve(asm: AssembledStiffness, model, F_applied=None, *, precis
                                  "autospc_dofs": int(((fixed == 1) & (spc == 0)).sum()),
                                  "post_factorization_singular_dofs": n_post})
     diag["linear_solver"].update({"backend": f"jfem_hip_supernodal_fp{precision}", "relative_residual": rr.tolist(),
+                                  "relative_residual_fp64_output": extra["relres_fp64"].tolist(),
                                   "refinement_iterations": iters.tolist(), "converged": bool(ok),
                                   "symbolic": S.info, "time_symbolic_s": t_sym, "time_factor_s": t_num,
                                   "time_solve_s": t4 - t3})

@@ -1171,10 +1171,16 @@ static void nd_rec(nd_ctx *c, int32_t *set, int64_t n, int32_t tag) {
     free(A); free(B); free(S);
 }
 
+/* y <- (L L')^{-1} y for the column-compressed Cholesky factor (diagonal first) */
+static void ltsolve(int64_t n, const int64_t *Lp, const int32_t *Li, const double *Lx, double *y) {
+    for (int64_t j = 0; j < n; ++j) { y[j] /= Lx[Lp[j]]; for (int64_t p = Lp[j] + 1; p < Lp[j + 1]; ++p) y[Li[p]] -= Lx[p] * y[j]; }
+    for (int64_t j = n - 1; j >= 0; --j) { for (int64_t p = Lp[j] + 1; p < Lp[j + 1]; ++p) y[j] -= Lx[p] * y[Li[p]]; y[j] /= Lx[Lp[j]]; }
+}
+
 int or_solve(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *val,
              const uint8_t *fixed, int order_kind, const double *xyz_nodes,
              int nrhs, const double *b, double *u, double *relres,
-             int64_t *nnz_L_out, double *flops_out, int64_t *n_small) {
+             int64_t *nnz_L_out, double *flops_out, int64_t *n_small, int refine) {
     /* free DOF numbering (ascending global DOF) */
     int64_t *loc = malloc(sizeof(int64_t) * ndof); int64_t n = 0;
     for (int64_t i = 0; i < ndof; ++i) loc[i] = fixed[i] ? -1 : n++;
@@ -1275,21 +1281,45 @@ int or_solve(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *v
             double kd = fabs(csr_get(rp, ci, val, g, g));
             double ratio = kd > 1e-30 ? lkk * lkk / kd : 1.0; if (ratio < 1e-7) ++smallp; }
         double *y = malloc(sizeof(double) * (n ? n : 1));
+        long double *xl = refine > 0 ? malloc(sizeof(long double) * (n ? n : 1)) : NULL;
         for (int r = 0; r < nrhs; ++r) {
             const double *br = b + (size_t)ndof * r; double *ur = u + (size_t)ndof * r;
             for (int64_t k = 0; k < n; ++k) y[k] = br[glob[perm[k]]];
-            for (int64_t j = 0; j < n; ++j) { y[j] /= Lx[Lp[j]]; for (int64_t p = Lp[j] + 1; p < Lp[j + 1]; ++p) y[Li[p]] -= Lx[p] * y[j]; }
-            for (int64_t j = n - 1; j >= 0; --j) { for (int64_t p = Lp[j] + 1; p < Lp[j + 1]; ++p) y[j] -= Lx[p] * y[Li[p]]; y[j] /= Lx[Lp[j]]; }
+            ltsolve(n, Lp, Li, Lx, y);
             for (int64_t i = 0; i < ndof; ++i) ur[i] = 0.0;
             for (int64_t k = 0; k < n; ++k) ur[glob[perm[k]]] = y[k];
-            /* relative residual ||K_ff u - F_ff|| / max(||F_ff||, 1e-30), :1859-1868 */
+            if (refine > 0) {
+                /* iterative refinement with the residual and the iterate in long
+                 * double (64-bit significand): converges to the solution correctly
+                 * rounded to double up to a few units, independent of cond(K_ff)
+                 * while cond(K_ff) * 2^-53 << 1 (Moler 1967; Higham, Accuracy and
+                 * Stability of Numerical Algorithms, 2nd ed., section 12.1). */
+                for (int64_t k = 0; k < n; ++k) xl[k] = y[k];
+                for (int it = 0; it < refine; ++it) {
+                    for (int64_t k = 0; k < n; ++k) { int64_t g = glob[perm[k]]; long double s = br[g];
+                        for (int64_t p = rp[g]; p < rp[g + 1]; ++p) { int64_t c = loc[ci[p]]; if (c >= 0) s -= (long double)val[p] * xl[pinv[c]]; }
+                        y[k] = (double)s; }
+                    ltsolve(n, Lp, Li, Lx, y);
+                    long double dmax = 0, xmax = 0;
+                    for (int64_t k = 0; k < n; ++k) { xl[k] += y[k]; long double a = fabsl(y[k]), bx = fabsl(xl[k]);
+                        if (a > dmax) dmax = a; if (bx > xmax) xmax = bx; }
+                    if (dmax <= 1e-19L * xmax) break;
+                }
+                for (int64_t k = 0; k < n; ++k) ur[glob[perm[k]]] = (double)xl[k];
+            }
+            /* relative residual ||K_ff u - F_ff|| / max(||F_ff||, 1e-30), :1859-1868
+             * (accumulated in long double when refining, so the figure is the
+             * residual of the returned double vector itself) */
             double rn = 0, fn = 0;
-            for (int64_t r2 = 0; r2 < n; ++r2) { int64_t g = glob[r2]; double s = 0;
-                for (int64_t p = rp[g]; p < rp[g + 1]; ++p) if (loc[ci[p]] >= 0) s += val[p] * ur[ci[p]];
-                double e = s - br[g]; rn += e * e; fn += br[g] * br[g]; }
+            for (int64_t r2 = 0; r2 < n; ++r2) { int64_t g = glob[r2]; double e;
+                if (refine > 0) { long double s = 0; for (int64_t p = rp[g]; p < rp[g + 1]; ++p) if (loc[ci[p]] >= 0) s += (long double)val[p] * ur[ci[p]];
+                                  e = (double)(s - br[g]); }
+                else { double s = 0; for (int64_t p = rp[g]; p < rp[g + 1]; ++p) if (loc[ci[p]] >= 0) s += val[p] * ur[ci[p]];
+                       e = s - br[g]; }
+                rn += e * e; fn += br[g] * br[g]; }
             relres[r] = sqrt(rn) / fmax(sqrt(fn), 1e-30);
         }
-        free(y);
+        free(y); free(xl);
     }
     if (nnz_L_out) *nnz_L_out = nzL;
     if (flops_out) *flops_out = fl;

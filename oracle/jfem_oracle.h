@@ -74,11 +74,15 @@ int64_t or_bc_partition(int64_t ndof, const int64_t *row_ptr, const int32_t *col
  * solve nrhs right-hand sides (b: ndof x nrhs column-major, full DOF numbering,
  * fixed rows ignored), write u (ndof x nrhs; fixed rows = 0) and the relative
  * residual ||K_ff u - F_ff|| / max(||F_ff||, 1e-30) per RHS.
- * Also returns the post-factor pivot check count (L_ii^2/K_ii < 1e-7). */
+ * Also returns the post-factor pivot check count (L_ii^2/K_ii < 1e-7).
+ * refine = 0: the plain factor-and-solve of the reference (residual in double).
+ * refine > 0: up to that many refinement steps with residual and iterate in
+ * long double; u is then the solution rounded to double (to a few units while
+ * cond(K_ff) * 2^-53 << 1) and relres is that double vector's own residual. */
 int or_solve(int64_t ndof, const int64_t *row_ptr, const int32_t *col, const double *val,
              const uint8_t *fixed, int order_kind, const double *xyz_nodes,
              int nrhs, const double *b, double *u, double *relres,
-             int64_t *nnz_L, double *flops, int64_t *n_small_pivots);
+             int64_t *nnz_L, double *flops, int64_t *n_small_pivots, int refine);
 
 /* dense helper exposed for tests: symmetric 3x3 Jacobi eigen (ascending) */
 void or_eig3(const double A[9], double w[3], double V[9]);

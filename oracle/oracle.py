@@ -48,7 +48,7 @@ def lib():
         L.or_bc_partition.argtypes = [C.c_int64, ip64, ip32, dp, u8p, C.c_double]
         L.or_bc_partition.restype = C.c_int64
         L.or_solve.argtypes = [C.c_int64, ip64, ip32, dp, u8p, C.c_int, C.c_void_p, C.c_int, dp, dp, dp,
-                               C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+                               C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]
         L.or_eig3.argtypes = [dp, dp, dp]
         _LIB = L
     return _LIB
@@ -164,8 +164,12 @@ def bc_partition(K, spc, eps_zero=1e-8):
     return fixed
 
 
-def solve(K, fixed, B, order="nd", xyz=None):
+def solve(K, fixed, B, order="nd", xyz=None, refine=0):
     """Reference sparse Cholesky solve of K_ff u = F_ff for each column of B.
+    ``refine`` = 0 is the plain solve; ``refine`` > 0 allows that many steps of
+    iterative refinement with long-double residual and iterate, returning the
+    solution correctly rounded to double up to a few units (for
+    cond(K_ff)·2^-53 << 1) and the residual of that double vector.
     Returns (u (ndof, nrhs), relres (nrhs,), info dict)."""
     K = sp.csr_matrix(K)
     K.sort_indices()
@@ -183,7 +187,7 @@ def solve(K, fixed, B, order="nd", xyz=None):
                         np.ascontiguousarray(np.asarray(fixed, np.uint8)),
                         1 if (order == "nd" and xyz is not None) else 0,
                         xyzc.ctypes.data if xyzc is not None else None, nrhs, Bc, U, rr,
-                        C.byref(nzl), C.byref(fl), C.byref(ns))
+                        C.byref(nzl), C.byref(fl), C.byref(ns), int(refine))
     if st:
         raise OracleError(st)
     return U.reshape(nrhs, n).T, rr, {"nnz_L": nzl.value, "flops_colcount2": fl.value,
@@ -196,10 +200,11 @@ def eig3(A):
     return w, V.reshape(3, 3, order="F")
 
 
-def sol101(model, order="nd"):
-    """assemble -> BC partition -> factor -> solve, all on the CPU oracle."""
+def sol101(model, order="nd", refine=0):
+    """assemble -> BC partition -> factor -> solve, all on the CPU oracle.
+    ``refine`` > 0 adds long-double iterative refinement (see ``solve``)."""
     K = assemble(model)
     fixed = bc_partition(K, model.spc)
-    u, rr, info = solve(K, fixed, model.loads, order=order, xyz=model.xyz)
+    u, rr, info = solve(K, fixed, model.loads, order=order, xyz=model.xyz, refine=refine)
     info["free_dofs"] = int((fixed == 0).sum())
     return K, fixed, u, rr, info

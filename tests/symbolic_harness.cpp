@@ -95,82 +95,140 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             }
         }
     }
-    // solves, interleaved right-hand sides
+    // solves, interleaved right-hand sides.  Vectors: Y (forward pending), W (update
+    // vectors), Z (forward result = backward pending), X (backward result).  Every
+    // launch is checked for read/write conflicts between its tasks: an element
+    // written by one task must not be read or written by another task of the same
+    // launch (tasks of one launch run concurrently on the device).
     const int64_t n = S.n;
-    std::vector<double> Y(n * nrhs), W(std::max<int64_t>(S.upd_rows, 1) * nrhs);
+    const int64_t nw = std::max<int64_t>(S.upd_rows, 1);
+    std::vector<double> Y(n * nrhs), W(nw * nrhs), Z(n * nrhs, 0.0), X(n * nrhs, 0.0);
     for (int64_t i = 0; i < n; ++i)
         for (int r = 0; r < nrhs; ++r) Y[i * nrhs + r] = b[S.perm[i] + ndof * r];
+    // conflict tracking: per vector element, the task that wrote it / last read it in this launch
+    std::vector<int64_t> wY(n, -1), wW(nw, -1), wZ(n, -1), wX(n, -1);
+    std::vector<int64_t> rY(n, -1), rW(nw, -1), rZ(n, -1), rX(n, -1);
+    int64_t launch = 0, conflict = 0;
+    // task ids are launch * 2^32 + task index
+    auto rd = [&](std::vector<int64_t> &w, std::vector<int64_t> &r, int64_t i, int64_t id) {
+        if (w[i] >= 0 && (w[i] >> 32) == launch && w[i] != id) ++conflict;
+        r[i] = id;
+    };
+    auto wr = [&](std::vector<int64_t> &w, std::vector<int64_t> &r, int64_t i, int64_t id) {
+        if (w[i] >= 0 && (w[i] >> 32) == launch && w[i] != id) ++conflict;
+        if (r[i] >= 0 && (r[i] >> 32) == launch && r[i] != id) ++conflict;
+        w[i] = id;
+    };
+    // the forward step of task t: solve the diagonal block at column k from the
+    // pending rows, then update the tile's rows
+    auto fwd_task = [&](PanelTask t, int64_t id) {
+        int s = t.s;
+        int64_t ns = S.ncol[s], ms = S.m[s], k = t.k, c0 = S.col0[s];
+        int b = (int)std::min<int64_t>(kPanel, ns - k);
+        const double *Ls = L.data() + S.lofs[s];
+        std::vector<double> y(b * nrhs);
+        for (int i = 0; i < b; ++i)
+            for (int r = 0; r < nrhs; ++r) { rd(wY, rY, c0 + k + i, id); y[i * nrhs + r] = Y[(c0 + k + i) * nrhs + r]; }
+        for (int j = 0; j < b; ++j)
+            for (int r = 0; r < nrhs; ++r) {
+                y[j * nrhs + r] /= Ls[(k + j) + ms * (k + j)];
+                for (int i = j + 1; i < b; ++i) y[i * nrhs + r] -= Ls[(k + i) + ms * (k + j)] * y[j * nrhs + r];
+            }
+        if (t.r0 == k)
+            for (int i = 0; i < b; ++i)
+                for (int r = 0; r < nrhs; ++r) { wr(wZ, rZ, c0 + k + i, id); Z[(c0 + k + i) * nrhs + r] = y[i * nrhs + r]; }
+        int64_t r1 = std::min<int64_t>(t.r0 + kTile, ms);
+        for (int64_t i = std::max<int64_t>(t.r0, k + b); i < r1; ++i)
+            for (int r = 0; r < nrhs; ++r) {
+                double v = 0;
+                for (int j = 0; j < b; ++j) v += Ls[i + ms * (k + j)] * y[j * nrhs + r];
+                if (i < ns) { wr(wY, rY, c0 + i, id); Y[(c0 + i) * nrhs + r] -= v; }
+                else { wr(wW, rW, S.relofs[s] + i - ns, id); W[(S.relofs[s] + i - ns) * nrhs + r] -= v; }
+            }
+    };
+    auto bwd_task = [&](PanelTask t, int64_t id) {
+        int s = t.s;
+        int64_t ns = S.ncol[s], ms = S.m[s], k = t.k, c0 = S.col0[s];
+        int b = (int)std::min<int64_t>(kPanel, ns - k);
+        const double *Ls = L.data() + S.lofs[s];
+        std::vector<double> x(b * nrhs);
+        for (int i = 0; i < b; ++i)
+            for (int r = 0; r < nrhs; ++r) { rd(wZ, rZ, c0 + k + i, id); x[i * nrhs + r] = Z[(c0 + k + i) * nrhs + r]; }
+        for (int j = b - 1; j >= 0; --j)
+            for (int r = 0; r < nrhs; ++r) {
+                double v = x[j * nrhs + r];
+                for (int i = j + 1; i < b; ++i) v -= Ls[(k + i) + ms * (k + j)] * x[i * nrhs + r];
+                x[j * nrhs + r] = v / Ls[(k + j) + ms * (k + j)];
+            }
+        if (t.r0 == k) {
+            for (int i = 0; i < b; ++i)
+                for (int r = 0; r < nrhs; ++r) { wr(wX, rX, c0 + k + i, id); X[(c0 + k + i) * nrhs + r] = x[i * nrhs + r]; }
+        } else {
+            for (int64_t j = t.r0; j < t.r0 + kTile; ++j)
+                for (int r = 0; r < nrhs; ++r) {
+                    double v = 0;
+                    for (int i = 0; i < b; ++i) v += Ls[(k + i) + ms * j] * x[i * nrhs + r];
+                    wr(wZ, rZ, c0 + j, id);
+                    Z[(c0 + j) * nrhs + r] -= v;
+                }
+        }
+    };
     for (int64_t l = 0; l < nlev; ++l) {
+        ++launch;   // gather: one task per supernode of the level
         for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+            int64_t id = (launch << 32) + q;
             int s = S.level_sn[q];
-            int64_t ns = S.ncol[s], ms = S.m[s];
-            double *Ws = W.data() + S.relofs[s] * nrhs, *Ys = Y.data() + S.col0[s] * nrhs;
-            const double *Ls = L.data() + S.lofs[s];
-            for (int64_t i = 0; i < (ms - ns) * nrhs; ++i) Ws[i] = 0;
+            int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
+            for (int64_t i = 0; i < ms - ns; ++i) {
+                wr(wW, rW, S.relofs[s] + i, id);
+                for (int r = 0; r < nrhs; ++r) W[(S.relofs[s] + i) * nrhs + r] = 0;
+            }
             for (int32_t qc = S.child_ptr[s]; qc < S.child_ptr[s + 1]; ++qc) {
                 int c = S.child[qc];
                 int64_t nu = S.m[c] - S.ncol[c];
                 const int32_t *rel = S.relmap.data() + S.relofs[c];
-                const double *Wc = W.data() + S.relofs[c] * nrhs;
-                for (int64_t i = 0; i < nu; ++i)
+                for (int64_t i = 0; i < nu; ++i) {
+                    rd(wW, rW, S.relofs[c] + i, id);
+                    int64_t pr = rel[i];
                     for (int r = 0; r < nrhs; ++r) {
-                        int64_t pr = rel[i];
-                        if (pr < ns) Ys[pr * nrhs + r] += Wc[i * nrhs + r];
-                        else Ws[(pr - ns) * nrhs + r] += Wc[i * nrhs + r];
+                        double v = W[(S.relofs[c] + i) * nrhs + r];
+                        if (pr < ns) { wr(wY, rY, c0 + pr, id); Y[(c0 + pr) * nrhs + r] += v; }
+                        else { wr(wW, rW, S.relofs[s] + pr - ns, id); W[(S.relofs[s] + pr - ns) * nrhs + r] += v; }
                     }
+                }
             }
-            for (int64_t j = 0; j < ns; ++j)
-                for (int r = 0; r < nrhs; ++r) {
-                    Ys[j * nrhs + r] /= Ls[j + ms * j];
-                    for (int64_t i = j + 1; i < ns; ++i) Ys[i * nrhs + r] -= Ls[i + ms * j] * Ys[j * nrhs + r];
-                }
         }
-        for (int64_t q = S.fwd_off[l]; q < S.fwd_off[l + 1]; ++q) {
-            PanelTask t = S.fwd_tiles[q];
-            int s = t.s;
-            int64_t ns = S.ncol[s], ms = S.m[s];
-            const double *Ls = L.data() + S.lofs[s];
-            double *Ws = W.data() + S.relofs[s] * nrhs;
-            const double *Ys = Y.data() + S.col0[s] * nrhs;
-            for (int64_t i = t.r0; i < std::min<int64_t>(t.r0 + kTile, ms); ++i)
-                for (int r = 0; r < nrhs; ++r) {
-                    double v = 0;
-                    for (int64_t j = 0; j < ns; ++j) v += Ls[i + ms * j] * Ys[j * nrhs + r];
-                    Ws[(i - ns) * nrhs + r] -= v;
-                }
+        for (int64_t stp = S.step_ptr[l]; stp < S.step_ptr[l + 1]; ++stp) {
+            ++launch;
+            for (int64_t q = S.pstep_off[stp]; q < S.pstep_off[stp + 1]; ++q) fwd_task(S.ptasks[q], (launch << 32) + q);
         }
     }
     for (int64_t l = nlev - 1; l >= 0; --l) {
+        ++launch;
         for (int64_t q = S.bwd_off[l]; q < S.bwd_off[l + 1]; ++q) {
+            int64_t id = (launch << 32) + q;
             PanelTask t = S.bwd_tiles[q];
             int s = t.s;
-            int64_t ns = S.ncol[s], ms = S.m[s];
+            int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
             const double *Ls = L.data() + S.lofs[s];
             const int32_t *rw = S.rows.data() + S.rofs[s];
-            double *Ys = Y.data() + S.col0[s] * nrhs;
             for (int64_t j = t.r0; j < std::min<int64_t>(t.r0 + kTile, ns); ++j)
                 for (int r = 0; r < nrhs; ++r) {
                     double v = 0;
-                    for (int64_t i = ns; i < ms; ++i) v += Ls[i + ms * j] * Y[(int64_t)rw[i] * nrhs + r];
-                    Ys[j * nrhs + r] -= v;
+                    for (int64_t i = ns; i < ms; ++i) { rd(wX, rX, rw[i], id); v += Ls[i + ms * j] * X[(int64_t)rw[i] * nrhs + r]; }
+                    wr(wZ, rZ, c0 + j, id);
+                    Z[(c0 + j) * nrhs + r] -= v;
                 }
         }
-        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-            int s = S.level_sn[q];
-            int64_t ns = S.ncol[s], ms = S.m[s];
-            const double *Ls = L.data() + S.lofs[s];
-            double *Ys = Y.data() + S.col0[s] * nrhs;
-            for (int64_t j = ns - 1; j >= 0; --j)
-                for (int r = 0; r < nrhs; ++r) {
-                    double v = Ys[j * nrhs + r];
-                    for (int64_t i = j + 1; i < ns; ++i) v -= Ls[i + ms * j] * Ys[i * nrhs + r];
-                    Ys[j * nrhs + r] = v / Ls[j + ms * j];
-                }
+        for (int64_t stp = S.bstep_ptr[l]; stp < S.bstep_ptr[l + 1]; ++stp) {
+            ++launch;
+            for (int64_t q = S.bstep_off[stp]; q < S.bstep_off[stp + 1]; ++q) bwd_task(S.btasks[q], (launch << 32) + q);
         }
     }
+    if (conflict) return 5;
     for (int64_t i = 0; i < ndof * nrhs; ++i) x[i] = 0;
     for (int64_t i = 0; i < n; ++i)
-        for (int r = 0; r < nrhs; ++r) x[S.perm[i] + ndof * r] = Y[i * nrhs + r];
+        for (int r = 0; r < nrhs; ++r) x[S.perm[i] + ndof * r] = X[i * nrhs + r];
     info[0] = S.ns;
     info[1] = nlev;
     info[2] = S.L_entries;

@@ -171,3 +171,58 @@ def test_golden_default_membrane_within_historical_tolerance(oracle):
     K, fixed, u, rr, info = oracle.sol101(m)
     exp = m.meta["expected"]
     assert np.linalg.norm(u[:, 0] - exp) / np.linalg.norm(exp) < 5e-3
+
+
+def _exact_solve(Kff, b):
+    """Exact rational Gaussian elimination (the doubles in Kff and b are exact rationals)."""
+    from fractions import Fraction
+    n = Kff.shape[0]
+    A = [[Fraction(float(Kff[i, j])) for j in range(n)] + [Fraction(float(b[i]))] for i in range(n)]
+    for k in range(n):
+        piv = max(range(k, n), key=lambda i: abs(A[i][k]))
+        A[k], A[piv] = A[piv], A[k]
+        for i in range(k + 1, n):
+            if A[i][k] != 0:
+                f = A[i][k] / A[k][k]
+                A[i] = [a - f * c for a, c in zip(A[i], A[k])]
+    x = [Fraction(0)] * n
+    for i in range(n - 1, -1, -1):
+        s = A[i][n] - sum(A[i][j] * x[j] for j in range(i + 1, n))
+        x[i] = s / A[i][i]
+    return np.array([float(v) for v in x])           # correctly rounded to double
+
+
+def test_refined_solve_is_correctly_rounded(oracle):
+    """Long-double refinement returns the exact solution to within one unit in
+    the last place of max |x| (normwise), checked against exact rational
+    arithmetic; the plain solve is off by more."""
+    m = meshes.flat_plate(2, 2)
+    m.loads = np.random.default_rng(3).normal(0.0, 1e3, (m.ndof, 1))
+    K = oracle.assemble(m)
+    fixed = oracle.bc_partition(K, m.spc)
+    free = np.where(fixed == 0)[0]
+    Kd = K.toarray()[np.ix_(free, free)]
+    xe = _exact_solve(Kd, m.loads[free, 0])
+    ur, _, _ = oracle.solve(K, fixed, m.loads, xyz=m.xyz, refine=10)
+    up, _, _ = oracle.solve(K, fixed, m.loads, xyz=m.xyz, refine=0)
+    unit = np.spacing(np.abs(xe).max())             # one unit in the last place of max |x|
+    err_ref = np.abs(ur[free, 0] - xe).max()
+    err_plain = np.abs(up[free, 0] - xe).max()
+    assert err_ref <= 1.0 * unit, err_ref / unit
+    assert err_plain > 8 * unit                      # the plain solve is not already exact
+
+
+def test_refined_solve_is_ordering_independent(oracle):
+    """Plain solves under two orderings differ by ~cond*eps; refined ones agree
+    to a few units of the last place."""
+    m = meshes.wing_box(12, 3, nrhs=2)
+    K = oracle.assemble(m)
+    fixed = oracle.bc_partition(K, m.spc)
+    a, _, _ = oracle.solve(K, fixed, m.loads, order="nd", xyz=m.xyz, refine=10)
+    b, _, _ = oracle.solve(K, fixed, m.loads, order="rcm", refine=10)
+    pa, _, _ = oracle.solve(K, fixed, m.loads, order="nd", xyz=m.xyz)
+    pb, _, _ = oracle.solve(K, fixed, m.loads, order="rcm")
+    d_ref = np.abs(a - b).max() / np.abs(a).max()
+    d_plain = np.abs(pa - pb).max() / np.abs(pa).max()
+    assert d_ref <= 1e-14, d_ref
+    assert d_plain > d_ref
