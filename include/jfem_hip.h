@@ -172,16 +172,18 @@ int jfem_factor_free(jfem_factor *F);
 /* Solve K_ff x = b for nrhs <= 16 right-hand sides given in FULL DOF numbering
  * (b: ndof x nrhs column-major, fixed rows ignored; x: ndof x nrhs, fixed rows 0).
  * Replaces F \ F_ff (src/solver/boundary_conditions.jl:1442, 1714) and the
- * residual (:1859-1868).  Triangular solves run in the factor's precision; the
- * iterate is kept as a double-double x = x_hi + x_lo and refined,
- *   r = b - K_ff x  (residual accumulated exactly: fma-split products, two-sum),
- *   x += F \ r,
- * until ||K_ff x - b|| / max(||b||, 1e-30) <= tol, the residual stops decreasing
- * (by 10 % per step), or max_refine steps.  relres[k] / iters[k]: residual of the
- * double-double iterate and the steps taken, per RHS.  x receives x_hi; x_lo
- * (optional, may be NULL) receives the low parts; relres_out (optional) receives
- * the residual of x_hi alone, i.e. of the double vector handed back, whose floor
- * is about 2^-53 || |K_ff| |x| || / ||b||.  Returns JFEM_ERR_NOT_CONVERGED (x still
+ * residual (:1859-1868).  The iterate is a double-double x = x_hi + x_lo refined
+ * in an outer loop on the exact residual
+ *   r = b - K_ff x  (accumulated exactly: fma-split products, two-sum),
+ * each step solving K_ff d = r by conjugate gradients preconditioned with the
+ * factor (triangular solves in its precision) to 1e-5 relative, x += d, until
+ * ||K_ff x - b|| / max(||b||, 1e-30) <= tol or the residual no longer halves.
+ * max_refine bounds the triangular solves to max_refine + 1 per call;
+ * max_refine = 0 is the plain solve x = F^{-1} b.  relres[k] / iters[k]: residual
+ * of the double-double iterate and triangular solves used, per RHS.  x receives
+ * x_hi; x_lo (optional, may be NULL) the low parts; relres_out (optional) the
+ * residual of x_hi alone, i.e. of the double vector handed back, whose floor is
+ * about 2^-53 || |K_ff| |x| || / ||b||.  Returns JFEM_ERR_NOT_CONVERGED (x still
  * written) when a right-hand side misses tol.
  * b / x on the host: *_host; the _dev variant takes device pointers. */
 int jfem_solve(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs,

@@ -32,6 +32,8 @@ struct jfem_symbolic {
     PanelTask *d_ptasks = nullptr, *d_btasks = nullptr, *d_bwd = nullptr;
     UpdateTask *d_utasks = nullptr;
     EaTask *d_etasks = nullptr;
+    int32_t *d_ea_bounds = nullptr;
+    int64_t *d_ea_boff = nullptr;
     std::vector<void *> allocs;
 };
 
@@ -75,37 +77,62 @@ __global__ void scatter_kernel(int64_t n, const int32_t *__restrict__ src, const
 }
 
 // ----------------------------------------------------------------- extend-add
-// One workgroup per (parent, front column range); children added in order.
+// One workgroup per (parent, 64 front columns, 512 front rows); the children are
+// added in child order.  The child entries that land in the task's block are the
+// update columns [j0, j1) x update rows [max(j, i0), i1), precomputed per
+// (task, child).  Each thread handles 4 independent entries per pass so their
+// loads overlap.
 template <class T>
-__global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+__global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restrict__ tasks, const int64_t *__restrict__ boff,
+                                                         const int32_t *__restrict__ bounds, const int64_t *__restrict__ ncol,
                                                          const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
                                                          const int64_t *__restrict__ uofs, const int64_t *__restrict__ relofs,
                                                          const int32_t *__restrict__ relmap, const int32_t *__restrict__ child_ptr,
                                                          const int32_t *__restrict__ child, T *__restrict__ L, T *__restrict__ U) {
+    constexpr int kU = 4;
     const EaTask t = tasks[blockIdx.x];
     const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s], mu = ms - ns;
     T *Ls = L + lofs[s];
     T *Us = U + uofs[s];
-    for (int q = child_ptr[s]; q < child_ptr[s + 1]; ++q) {
+    const int32_t *bd = bounds + 4 * boff[blockIdx.x];
+    for (int q = child_ptr[s]; q < child_ptr[s + 1]; ++q, bd += 4) {
         const int c = child[q];
         const int64_t nu = mrow[c] - ncol[c];
         const int32_t *rel = relmap + relofs[c];
         const T *Uc = U + uofs[c];
-        // child columns whose parent column lies in [c0, c1)
-        int64_t lo = 0, hi = nu;
-        while (lo < hi) { int64_t md = (lo + hi) / 2; if (rel[md] < t.c0) lo = md + 1; else hi = md; }
-        const int64_t j0 = lo;
-        hi = nu;
-        while (lo < hi) { int64_t md = (lo + hi) / 2; if (rel[md] < t.c1) lo = md + 1; else hi = md; }
-        const int64_t j1 = lo;
-        for (int64_t j = j0; j < j1; ++j) {
-            const int64_t pc = rel[j];
-            for (int64_t i = j + threadIdx.x; i < nu; i += blockDim.x) {
-                const int64_t pr = rel[i];
-                const T v = Uc[i + nu * j];
-                if (pc < ns) Ls[pr + ms * pc] += v;
-                else Us[(pr - ns) + mu * (pc - ns)] += v;
+        const int64_t j0 = bd[0], j1 = bd[1], i1 = bd[3];
+        const int64_t ib = max<int64_t>(bd[2], j0);
+        const int64_t nrow = i1 - ib, nj = j1 - j0;
+        if (nrow > 0 && nj > 0) {
+            const int64_t total = nj * nrow;
+            for (int64_t base = threadIdx.x; base < total; base += kU * kBS) {
+                int64_t jj[kU], ii[kU];
+                bool ok[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int64_t qq = base + (int64_t)u * kBS;
+                    const int64_t qc = qq < total ? qq : 0;
+                    jj[u] = j0 + qc / nrow;
+                    ii[u] = ib + qc % nrow;
+                    ok[u] = qq < total && ii[u] >= jj[u];
+                }
+                int32_t pr[kU], pc[kU];
+                T v[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    pr[u] = rel[ii[u]];
+                    pc[u] = rel[jj[u]];
+                    v[u] = Uc[ii[u] + nu * jj[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    if (ok[u]) {
+                        T *dst = pc[u] < ns ? (Ls + pr[u] + ms * (int64_t)pc[u])
+                                            : (Us + (pr[u] - ns) + mu * (int64_t)(pc[u] - ns));
+                        *dst += v[u];
+                    }
+                }
             }
         }
         __syncthreads();
@@ -372,10 +399,16 @@ __global__ void __launch_bounds__(kBS) fwd_gather_kernel(const int32_t *__restri
     }
 }
 
+// Loads in the step kernels use clamped, always-valid addresses and masks applied
+// afterwards (no data-dependent branches around loads), so that every load of a
+// tile can be in flight at once.  Beyond the block edge b the diagonal block is
+// padded with the identity and the vectors with zeros, which leaves the
+// arithmetic branch-free.  NQ = right-hand sides per wave (1, 2 or 4).
+
 // forward step task (s, k, r0): y_k = L_kk^{-1} (pending rows k..k+b), written to Z
 // by the diagonal task (r0 == k); rows [max(r0, k+b), r0+kTile) of the front get
 // -= L[rows, k..k+b) y_k (pivot rows into Y, update rows into W).
-template <class T>
+template <class T, int NQ>
 __global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ col0,
                                                        const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
                                                        const int64_t *__restrict__ lofs, const int64_t *__restrict__ relofs,
@@ -387,72 +420,73 @@ __global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restri
     const int b = (int)min<int64_t>(kPanel, ns - k);
     const T *Ls = L + lofs[s];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (w >= nrhs) return;                    // whole wave idle (uniform)
-    T lr[kPanel];                             // row `lane` of L_kk
+    if (w * NQ >= nrhs) return;               // whole wave idle (uniform)
+    const int li = min(lane, b - 1);
+    T lr[kPanel];                             // row `lane` of L_kk (identity row beyond b)
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) lr[j] = (j < b && j <= lane && lane < b) ? Ls[(k + lane) + ms * (k + j)] : T(0);
-    T y[4];
+    for (int j = 0; j < kPanel; ++j) {
+        const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
+        lr[j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
+    }
+    T y[NQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int r = w + 4 * q;
-        y[q] = (r < nrhs && lane < b) ? Y[(c0 + k + lane) * nrhs + r] : T(0);
+    for (int q = 0; q < NQ; ++q) {
+        const int r = w * NQ + q;
+        const T v = Y[(c0 + k + li) * nrhs + min(r, nrhs - 1)];
+        y[q] = (lane < b && r < nrhs) ? v : T(0);
     }
 #pragma unroll
     for (int j = 0; j < kPanel; ++j) {
-        if (j < b) {
-            const T inv = T(1) / bcast(lr[j], j);
+        const T inv = T(1) / bcast(lr[j], j);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (w + 4 * q < nrhs) {
-                    const T yj = bcast(y[q], j) * inv;
-                    if (lane == j) y[q] = yj;
-                    else if (lane > j) y[q] -= lr[j] * yj;
-                }
-            }
+        for (int q = 0; q < NQ; ++q) {
+            const T yj = bcast(y[q], j) * inv;
+            const T nv = y[q] - lr[j] * yj;
+            y[q] = lane == j ? yj : nv;
         }
     }
     if (t.r0 == k && lane < b) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int r = w + 4 * q;
+        for (int q = 0; q < NQ; ++q) {
+            const int r = w * NQ + q;
             if (r < nrhs) Z[(c0 + k + lane) * nrhs + r] = y[q];
         }
     }
     const int64_t rs = max<int64_t>(t.r0, k + b), re = min<int64_t>(t.r0 + kTile, ms);
     if (rs >= re) return;
     const int64_t row = t.r0 + lane;
-    const bool act = row >= rs && row < re;
-    T acc[4] = {T(0), T(0), T(0), T(0)};
+    const int64_t rowc = min<int64_t>(max<int64_t>(row, rs), re - 1);
+    T acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = T(0);
 #pragma unroll
     for (int j = 0; j < kPanel; ++j) {
-        if (j < b) {
-            const T l = act ? Ls[row + ms * (k + j)] : T(0);
+        const T l = Ls[rowc + ms * (k + min(j, b - 1))];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (w + 4 * q < nrhs) acc[q] += l * bcast(y[q], j);
-        }
+        for (int q = 0; q < NQ; ++q) acc[q] += l * bcast(y[q], j);
     }
-    if (act) {
+    if (row >= rs && row < re) {
         T *dst = row < ns ? (Y + (c0 + row) * nrhs) : (W + (relofs[s] + row - ns) * nrhs);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int r = w + 4 * q;
+        for (int q = 0; q < NQ; ++q) {
+            const int r = w * NQ + q;
             if (r < nrhs) dst[r] -= acc[q];
         }
     }
 }
 
 // backward, per column tile (s, 0, c0) of a supernode with update rows:
-// Z[cols] -= L[ns..ms, cols]' X[rows] (X at the ancestors' rows is final)
-template <class T>
+// Z[cols] -= L[ns..ms, cols]' X[rows] (X at the ancestors' rows is final).
+// Waves split the rows of each 64-row chunk; partial sums are added in a fixed order.
+template <class T, int NR>
 __global__ void __launch_bounds__(kBS) bwd_offdiag_kernel(const PanelTask *__restrict__ tiles, const int64_t *__restrict__ col0,
                                                           const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
                                                           const int64_t *__restrict__ lofs, const int64_t *__restrict__ rofs,
                                                           const int32_t *__restrict__ rows, const T *__restrict__ L,
                                                           const T *__restrict__ X, T *__restrict__ Z, int nrhs) {
     __shared__ T Lt[kTile][kTile + 1];
-    __shared__ T xs[kTile][17];
-    __shared__ T red[4][kTile][17];
+    __shared__ T xs[kTile][NR + 1];
+    __shared__ T red[4][kTile][NR + 1];
     const PanelTask t = tiles[blockIdx.x];
     const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s], ct = t.r0;
@@ -460,46 +494,56 @@ __global__ void __launch_bounds__(kBS) bwd_offdiag_kernel(const PanelTask *__res
     const T *Ls = L + lofs[s];
     const int32_t *rw = rows + rofs[s];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    T acc[16];
+    T acc[NR];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = T(0);
+    for (int r = 0; r < NR; ++r) acc[r] = T(0);
     for (int64_t i0 = ns; i0 < ms; i0 += kTile) {
         const int nr = (int)min<int64_t>(kTile, ms - i0);
         __syncthreads();
-        for (int q = tid; q < kTile * kTile; q += blockDim.x) {
+#pragma unroll
+        for (int p = 0; p < kTile * kTile / kBS; ++p) {
+            const int q = tid + kBS * p;
             const int i = q % kTile, c = q / kTile;
-            Lt[i][c] = (i < nr && c < nc) ? Ls[(i0 + i) + ms * (ct + c)] : T(0);
+            const T v = Ls[(i0 + min(i, nr - 1)) + ms * (ct + min(c, nc - 1))];
+            Lt[i][c] = (i < nr && c < nc) ? v : T(0);
         }
-        for (int q = tid; q < kTile * nrhs; q += blockDim.x) {
-            const int i = q / nrhs, r = q % nrhs;
-            xs[i][r] = i < nr ? X[(int64_t)rw[i0 + i] * nrhs + r] : T(0);
+#pragma unroll
+        for (int p = 0; p < (kTile * NR + kBS - 1) / kBS; ++p) {
+            const int q = tid + kBS * p;
+            const int i = q / NR, r = q % NR;
+            if (i < kTile) {
+                const T v = X[(int64_t)rw[i0 + min(i, nr - 1)] * nrhs + min(r, nrhs - 1)];
+                xs[i][r] = (i < nr && r < nrhs) ? v : T(0);
+            }
         }
         __syncthreads();
-        for (int i = w; i < nr; i += 4) {
+#pragma unroll 4
+        for (int i = w; i < kTile; i += 4) {
             const T l = Lt[i][lane];
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                if (r < nrhs) acc[r] += l * xs[i][r];
+            for (int r = 0; r < NR; ++r) acc[r] += l * xs[i][r];
         }
     }
 #pragma unroll
-    for (int r = 0; r < 16; ++r) red[w][lane][r] = acc[r];
+    for (int r = 0; r < NR; ++r) red[w][lane][r] = acc[r];
     __syncthreads();
     if (w == 0 && lane < nc) {
         T *dst = Z + (col0[s] + ct + lane) * nrhs;
-        for (int r = 0; r < nrhs; ++r) dst[r] -= ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (r < nrhs) dst[r] -= ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
     }
 }
 
 // backward step task (s, k, j0): x_k = L_kk^{-T} (pending rows k..k+b of Z), written
 // to X by the diagonal task (j0 == k); for j0 < k, Z[j0..j0+kTile) -= L[k..k+b, j0..]' x_k.
-template <class T>
+template <class T, int NQ>
 __global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ col0,
                                                        const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
                                                        const int64_t *__restrict__ lofs, const T *__restrict__ L,
                                                        T *__restrict__ Z, T *__restrict__ X, int nrhs) {
-    __shared__ T D[kPanel][kPanel + 1];      // D[i][c] = L[k+i][k+c]
-    __shared__ T E[kPanel][kTile + 1];       // E[i][c] = L[k+i][j0+c]
+    __shared__ T D[kPanel][kPanel + 1];      // D[i][c] = L[k+i][k+c] (identity beyond b)
+    __shared__ T E[kPanel][kTile + 1];       // E[i][c] = L[k+i][j0+c] (zero rows beyond b)
     const PanelTask t = tasks[blockIdx.x];
     const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s], k = t.k, c0 = col0[s], j0 = t.r0;
@@ -507,59 +551,61 @@ __global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restri
     const T *Ls = L + lofs[s];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const bool diag = j0 == k;
-    for (int q = tid; q < kPanel * kPanel; q += blockDim.x) {
+#pragma unroll
+    for (int p = 0; p < kPanel * kPanel / kBS; ++p) {
+        const int q = tid + kBS * p;
         const int i = q % kPanel, c = q / kPanel;
-        D[i][c] = (i < b && c <= i) ? Ls[(k + i) + ms * (k + c)] : T(0);
-        if (!diag) E[i][c] = i < b ? Ls[(k + i) + ms * (j0 + c)] : T(0);
+        const int ic = min(i, b - 1);
+        const T v = Ls[(k + ic) + ms * (k + min(c, b - 1))];
+        D[i][c] = (i < b && c <= i) ? v : (i == c ? T(1) : T(0));
+        const T e = Ls[(k + ic) + ms * (diag ? k + min(c, b - 1) : j0 + c)];
+        E[i][c] = i < b ? e : T(0);
     }
     __syncthreads();
-    if (w >= nrhs) return;
+    if (w * NQ >= nrhs) return;
     T lc[kPanel];                             // column `lane` of L_kk
 #pragma unroll
     for (int i = 0; i < kPanel; ++i) lc[i] = D[i][lane];
-    T z[4];
+    T z[NQ];
+    const int li = min(lane, b - 1);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int r = w + 4 * q;
-        z[q] = (r < nrhs && lane < b) ? Z[(c0 + k + lane) * nrhs + r] : T(0);
+    for (int q = 0; q < NQ; ++q) {
+        const int r = w * NQ + q;
+        const T v = Z[(c0 + k + li) * nrhs + min(r, nrhs - 1)];
+        z[q] = (r < nrhs && lane < b) ? v : T(0);
     }
 #pragma unroll
     for (int j = kPanel - 1; j >= 0; --j) {
-        if (j < b) {
-            const T inv = T(1) / bcast(lc[j], j);
+        const T inv = T(1) / bcast(lc[j], j);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (w + 4 * q < nrhs) {
-                    const T xj = bcast(z[q], j) * inv;
-                    if (lane == j) z[q] = xj;
-                    else if (lane < j) z[q] -= lc[j] * xj;
-                }
-            }
+        for (int q = 0; q < NQ; ++q) {
+            const T xj = bcast(z[q], j) * inv;
+            const T nv = z[q] - lc[j] * xj;
+            z[q] = lane == j ? xj : nv;
         }
     }
     if (diag) {
         if (lane < b) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int r = w + 4 * q;
+            for (int q = 0; q < NQ; ++q) {
+                const int r = w * NQ + q;
                 if (r < nrhs) X[(c0 + k + lane) * nrhs + r] = z[q];
             }
         }
         return;
     }
-    T acc[4] = {T(0), T(0), T(0), T(0)};
+    T acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = T(0);
 #pragma unroll
     for (int i = 0; i < kPanel; ++i) {
-        if (i < b) {
-            const T e = E[i][lane];
+        const T e = E[i][lane];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (w + 4 * q < nrhs) acc[q] += e * bcast(z[q], i);
-        }
+        for (int q = 0; q < NQ; ++q) acc[q] += e * bcast(z[q], i);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int r = w + 4 * q;
+    for (int q = 0; q < NQ; ++q) {
+        const int r = w * NQ + q;
         if (r < nrhs) Z[(c0 + j0 + lane) * nrhs + r] -= acc[q];
     }
 }
@@ -614,68 +660,235 @@ __global__ void accumulate_kernel(int64_t n, int nrhs, const int32_t *__restrict
     }
 }
 
-// R = Bp - K_ff x for x = xgh + xgl (xgl may be null: x = xgh), rows of K_ff in
-// elimination order.  Products are split exactly with fma and summed with
-// two_sum, so R is the exact residual rounded once to double.  xg is zero on
-// fixed DOFs, so the full CSR row can be used.
-__global__ void residual_kernel(int64_t n, int nrhs, const int32_t *__restrict__ perm, const int64_t *__restrict__ rowptr,
-                                const int32_t *__restrict__ col, const double *__restrict__ val,
-                                const double *__restrict__ xgh, const double *__restrict__ xgl,
-                                const double *__restrict__ Bp, double *__restrict__ R) {
+// (h, l) += (h2, l2) in double-double
+__device__ inline void dd_add(double &h, double &l, double h2, double l2) {
 #pragma clang fp contract(off)
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t g = perm[i];
-        double hi[16], lo[16];
-        for (int r = 0; r < nrhs; ++r) { hi[r] = 0.0; lo[r] = 0.0; }
-        for (int64_t p = rowptr[g]; p < rowptr[g + 1]; ++p) {
-            const double kv = val[p];
-            const int64_t c = col[p];
-            for (int r = 0; r < nrhs; ++r) {
-                const double xv = xgh[c * nrhs + r];
-                const double pr = kv * xv;
-                const double pe = fma(kv, xv, -pr);
-                double s, e;
-                two_sum(hi[r], pr, s, e);
-                hi[r] = s;
-                lo[r] += e + pe;
-                if (xgl) lo[r] = fma(kv, xgl[c * nrhs + r], lo[r]);
-            }
+    double s, e;
+    two_sum(h, h2, s, e);
+    e += l + l2;
+    h = s + e;
+    l = e - (h - s);
+}
+
+// R = Bp - K_ff x for x = xgh + xgl (xgl may be null: x = xgh), rows of K_ff in
+// elimination order, NR right-hand sides.  Eight lanes share a row (coalesced
+// column/value reads); each accumulates its products exactly split with fma into
+// a double-double, and the eight partial sums are combined in a fixed butterfly,
+// so R is the exact residual to ~2^-104 relative, rounded once to double.  xg is
+// zero on fixed DOFs, so the full CSR row can be used.
+template <int NR>
+__global__ void __launch_bounds__(kBS) residual_kernel(int64_t n, const int32_t *__restrict__ perm,
+                                                       const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                                       const double *__restrict__ val, const double *__restrict__ xgh,
+                                                       const double *__restrict__ xgl, const double *__restrict__ Bp,
+                                                       double *__restrict__ R) {
+#pragma clang fp contract(off)
+    const int sub = threadIdx.x & 7;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const bool valid = i < n;
+    const int64_t g = valid ? perm[i] : 0;
+    const int64_t p0 = valid ? rowptr[g] : 0, p1 = valid ? rowptr[g + 1] : 0;
+    double hi[NR], lo[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) { hi[r] = 0.0; lo[r] = 0.0; }
+    for (int64_t p = p0 + sub; p < p1; p += 8) {
+        const double kv = val[p];
+        const int64_t c = col[p];
+        const double *xh = xgh + c * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double xv = xh[r];
+            const double pr = kv * xv;
+            const double pe = fma(kv, xv, -pr);
+            double t, e;
+            two_sum(hi[r], pr, t, e);
+            hi[r] = t;
+            lo[r] += e + pe;
         }
-        for (int r = 0; r < nrhs; ++r) {
+        if (xgl) {
+            const double *xl = xgl + c * NR;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) lo[r] = fma(kv, xl[r], lo[r]);
+        }
+    }
+#pragma unroll
+    for (int o = 4; o >= 1; o >>= 1) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double h2 = __shfl_xor(hi[r], o, 8), l2 = __shfl_xor(lo[r], o, 8);
+            dd_add(hi[r], lo[r], h2, l2);
+        }
+    }
+    if (valid && sub == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
             double t, te;
-            two_sum(Bp[i * nrhs + r], -hi[r], t, te);
-            R[i * nrhs + r] = t + (te - lo[r]);
+            two_sum(Bp[i * NR + r], -hi[r], t, te);
+            R[i * NR + r] = t + (te - lo[r]);
         }
     }
 }
 
-// per-block partial sums of squares, per RHS; then a fixed-order final reduction
-__global__ void sumsq_partial_kernel(int64_t n, int nrhs, const double *__restrict__ v, double *__restrict__ part) {
+// per-block partial sums of squares, per RHS; then a fixed-order tree reduction
+template <int NR>
+__global__ void __launch_bounds__(kBS) sumsq_partial_kernel(int64_t n, const double *__restrict__ v, double *__restrict__ part) {
+    __shared__ double sh[NR][kBS];
+    double a[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) a[r] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double x = v[i * NR + r];
+            a[r] += x * x;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] = a[r];
+    __syncthreads();
+    for (int w = kBS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < NR) part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+// out[r] = sum of the nblk (= kBS) partials of RHS r, one block, tree order
+__global__ void __launch_bounds__(kBS) sumsq_final_kernel(int nrhs, const double *__restrict__ part, double *__restrict__ out) {
     __shared__ double sh[kBS];
     for (int r = 0; r < nrhs; ++r) {
-        double a = 0.0;
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-            double x = v[i * nrhs + r];
-            a += x * x;
-        }
-        sh[threadIdx.x] = a;
+        sh[threadIdx.x] = part[(int64_t)r * kBS + threadIdx.x];
         __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        for (int w = kBS / 2; w > 0; w >>= 1) {
             if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) part[(int64_t)r * gridDim.x + blockIdx.x] = sh[0];
+        if (threadIdx.x == 0) out[r] = sh[0];
         __syncthreads();
     }
 }
 
-__global__ void sumsq_final_kernel(int nblk, int nrhs, const double *__restrict__ part, double *__restrict__ out) {
-    int r = threadIdx.x;
-    if (r >= nrhs) return;
-    double a = 0.0;
-    for (int b = 0; b < nblk; ++b) a += part[(int64_t)r * nblk + b];
-    out[r] = a;
+// q = K_ff p (plain FP64), p given in the global interleaved layout (zero on fixed
+// DOFs), q in elimination order; eight lanes per row, fixed-order lane reduction
+template <int NR>
+__global__ void __launch_bounds__(kBS) spmv_kernel(int64_t n, const int32_t *__restrict__ perm,
+                                                   const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                                   const double *__restrict__ val, const double *__restrict__ pg,
+                                                   double *__restrict__ q) {
+    const int sub = threadIdx.x & 7;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const bool valid = i < n;
+    const int64_t g = valid ? perm[i] : 0;
+    const int64_t p0 = valid ? rowptr[g] : 0, p1 = valid ? rowptr[g + 1] : 0;
+    double a[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) a[r] = 0.0;
+    for (int64_t p = p0 + sub; p < p1; p += 8) {
+        const double kv = val[p];
+        const double *x = pg + (int64_t)col[p] * NR;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) a[r] = fma(kv, x[r], a[r]);
+    }
+#pragma unroll
+    for (int o = 4; o >= 1; o >>= 1) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) a[r] += __shfl_xor(a[r], o, 8);
+    }
+    if (valid && sub == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) q[i * NR + r] = a[r];
+    }
 }
+
+// per-block partial dot products a.b per RHS (tree order), for reduce_final_kernel
+template <int NR>
+__global__ void __launch_bounds__(kBS) dot_partial_kernel(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                                                          double *__restrict__ part) {
+    __shared__ double sh[NR][kBS];
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] = fma(a[i * NR + r], b[i * NR + r], acc[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] = acc[r];
+    __syncthreads();
+    for (int w = kBS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < NR) part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+// conjugate-gradient scalars per RHS (one thread per RHS):
+// mode 0: alpha = rho / pq;  mode 1: beta = rho_new / rho, rho = rho_new
+__global__ void pcg_scalars_kernel(int nrhs, int mode, const int32_t *__restrict__ act, const double *__restrict__ num,
+                                   double *__restrict__ rho, double *__restrict__ coef) {
+    const int r = threadIdx.x;
+    if (r >= nrhs) return;
+    if (mode == 0) {
+        const double pq = num[r];
+        coef[r] = (act[r] && pq > 0.0) ? rho[r] / pq : 0.0;
+    } else {
+        const double rn = num[r];
+        coef[r] = (act[r] && rho[r] != 0.0) ? rn / rho[r] : 0.0;
+        rho[r] = rn;
+    }
+}
+
+// d += alpha p;  s -= alpha q
+template <int NR>
+__global__ void pcg_step_kernel(int64_t n, const double *__restrict__ alpha, const double *__restrict__ p,
+                                const double *__restrict__ q, double *__restrict__ d, double *__restrict__ sres) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * NR; e += (int64_t)gridDim.x * blockDim.x) {
+        const double a = alpha[e % NR];
+        d[e] = fma(a, p[e], d[e]);
+        sres[e] = fma(-a, q[e], sres[e]);
+    }
+}
+
+// p = z + beta p  (first: p = z, d = 0)
+template <int NR>
+__global__ void pcg_dir_kernel(int64_t n, int first, const double *__restrict__ beta, const double *__restrict__ z,
+                               double *__restrict__ p, double *__restrict__ d) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * NR; e += (int64_t)gridDim.x * blockDim.x) {
+        if (first) {
+            p[e] = z[e];
+            d[e] = 0.0;
+        } else {
+            p[e] = fma(beta[e % NR], p[e], z[e]);
+        }
+    }
+}
+
+// global interleaved pg[perm[i]] = p[i]
+template <int NR>
+__global__ void scatter_global_kernel(int64_t n, const int32_t *__restrict__ perm, const double *__restrict__ p,
+                                      double *__restrict__ pg) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * NR; e += (int64_t)gridDim.x * blockDim.x)
+        pg[(int64_t)perm[e / NR] * NR + e % NR] = p[e];
+}
+
+template <class T>
+__global__ void from_t_kernel(int64_t n, const T *__restrict__ a, double *__restrict__ o) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) o[q] = (double)a[q];
+}
+
+#define JFEM_NR_SWITCH(nrhs, CALL)                                                                            \
+    switch (nrhs) {                                                                                           \
+    case 1: CALL(1); break;   case 2: CALL(2); break;   case 3: CALL(3); break;   case 4: CALL(4); break;    \
+    case 5: CALL(5); break;   case 6: CALL(6); break;   case 7: CALL(7); break;   case 8: CALL(8); break;    \
+    case 9: CALL(9); break;   case 10: CALL(10); break; case 11: CALL(11); break; case 12: CALL(12); break;  \
+    case 13: CALL(13); break; case 14: CALL(14); break; case 15: CALL(15); break; default: CALL(16); break; \
+    }
 
 // global column-major x (ndof x nrhs) from the interleaved xg
 __global__ void export_kernel(int64_t ndof, int nrhs, const double *__restrict__ xg, double *__restrict__ x) {
@@ -728,8 +941,8 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
         if (ne) {
             if (detail) (void)hipEventRecord(ctx->ev2, st);
             hipLaunchKernelGGL(extend_add_kernel<T>, dim3((unsigned)ne), dim3(kBS), 0, st, S->d_etasks + H.ea_off[l],
-                               S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, S->d_relofs, S->d_relmap, S->d_child_ptr,
-                               S->d_child, L, U);
+                               S->d_ea_boff + H.ea_off[l], S->d_ea_bounds, S->d_ncol, S->d_m, S->d_lofs, S->d_uofs,
+                               S->d_relofs, S->d_relmap, S->d_child_ptr, S->d_child, L, U);
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_EA, 1); }
         }
         for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
@@ -755,8 +968,8 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
     return hipGetLastError();
 }
 
-template <class T>
-hipError_t solve_t(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, int nrhs) {
+template <class T, int NQ>
+hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, int nrhs) {
     hipStream_t st = ctx->stream;
     const SymbolicHost &H = S->H;
     const int64_t nlev = (int64_t)H.level_ptr.size() - 1;
@@ -767,23 +980,32 @@ hipError_t solve_t(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W
         for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
             const int64_t nt = H.pstep_off[stp + 1] - H.pstep_off[stp];
             if (nt)
-                hipLaunchKernelGGL(fwd_step_kernel<T>, dim3((unsigned)nt), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
-                                   S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L, Y, W, Z, nrhs);
+                hipLaunchKernelGGL((fwd_step_kernel<T, NQ>), dim3((unsigned)nt), dim3(kBS), 0, st,
+                                   S->d_ptasks + H.pstep_off[stp], S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L, Y,
+                                   W, Z, nrhs);
         }
     }
     for (int64_t l = nlev - 1; l >= 0; --l) {
         const int64_t nt = H.bwd_off[l + 1] - H.bwd_off[l];
         if (nt)
-            hipLaunchKernelGGL(bwd_offdiag_kernel<T>, dim3((unsigned)nt), dim3(kBS), 0, st, S->d_bwd + H.bwd_off[l],
+            hipLaunchKernelGGL((bwd_offdiag_kernel<T, 4 * NQ>), dim3((unsigned)nt), dim3(kBS), 0, st, S->d_bwd + H.bwd_off[l],
                                S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_rofs, S->d_rows, L, (const T *)X, Z, nrhs);
         for (int64_t stp = H.bstep_ptr[l]; stp < H.bstep_ptr[l + 1]; ++stp) {
             const int64_t nb = H.bstep_off[stp + 1] - H.bstep_off[stp];
             if (nb)
-                hipLaunchKernelGGL(bwd_step_kernel<T>, dim3((unsigned)nb), dim3(kBS), 0, st, S->d_btasks + H.bstep_off[stp],
-                                   S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, Z, X, nrhs);
+                hipLaunchKernelGGL((bwd_step_kernel<T, NQ>), dim3((unsigned)nb), dim3(kBS), 0, st,
+                                   S->d_btasks + H.bstep_off[stp], S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, Z, X, nrhs);
         }
     }
     return hipGetLastError();
+}
+
+// y = (L L')^{-1} y for nrhs <= 16 interleaved right-hand sides (Y in, X out)
+template <class T>
+hipError_t solve_t(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, int nrhs) {
+    if (nrhs <= 4) return solve_nq<T, 1>(ctx, S, L, Y, W, Z, X, nrhs);
+    if (nrhs <= 8) return solve_nq<T, 2>(ctx, S, L, Y, W, Z, X, nrhs);
+    return solve_nq<T, 4>(ctx, S, L, Y, W, Z, X, nrhs);
 }
 
 template <class V>
@@ -849,6 +1071,8 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_utasks, H.utasks, st);
     if (e == hipSuccess) e = up(S, &S->d_etasks, H.etasks, st);
     if (e == hipSuccess) e = up(S, &S->d_btasks, H.btasks, st);
+    if (e == hipSuccess) e = up(S, &S->d_ea_bounds, H.ea_bounds, st);
+    if (e == hipSuccess) e = up(S, &S->d_ea_boff, H.ea_boff, st);
     if (e == hipSuccess) e = up(S, &S->d_bwd, H.bwd_tiles, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -972,114 +1196,193 @@ int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_mat
     return JFEM_OK;
 }
 
-int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_dev,
-                   double *x_dev, double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev,
-                   double *relres_out) {
+}  // extern "C"
+
+namespace {
+
+// Refinement of K_ff x = b (nrhs = NR) on a double-double iterate x = xh + xl:
+//   outer: r = b - K_ff x exactly (residual_kernel); stop a right-hand side at
+//          ||r|| <= tol ||b|| or when ||r|| no longer halves;
+//   inner: d ~= K_ff^{-1} r by conjugate gradients preconditioned with the
+//          factor (triangular solves in its precision, FP64 vectors, plain FP64
+//          SpMV), until ||r - K_ff d|| <= eta ||r||; then x += d in double-double.
+// With an FP64 factor the inner loop ends after one preconditioner application
+// (classical refinement); with an FP32 factor the Krylov acceleration keeps the
+// outer loop converging where plain refinement would stall.  max_refine bounds
+// the preconditioner applications to max_refine + 1; max_refine = 0 is the plain
+// solve x = F^{-1} b.
+template <int NR, class T>
+int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, const double *b_dev, double *x_dev,
+                 double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev, double *relres_out) {
     const jfem_symbolic *S = F->S;
     const SymbolicHost &H = S->H;
     const jfem_pattern *P = K->P;
-    if (K->P != S->P) return fail(ctx, JFEM_ERR_BAD_ARG, "matrix and factor use different patterns");
-    if (nrhs < 1 || nrhs > 16) return fail(ctx, JFEM_ERR_BAD_ARG, "nrhs must be in 1..16");
-    if (max_refine < 0) return fail(ctx, JFEM_ERR_BAD_ARG, "max_refine must be >= 0");
     const int64_t n = H.n, ndof = P->ndof;
-    const size_t es = F->precision == 64 ? 8 : 4;
-    const int nblk = 256;
-    // scratch layout
+    const int64_t nv = n * NR, ng = ndof * NR;
+    const int nblk = kBS;
+    const double eta = 1e-5;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
-    size_t oY = take(n * nrhs * es), oW = take(std::max<int64_t>(H.upd_rows, 1) * nrhs * es);
-    size_t oZ = take(n * nrhs * es), oXs = take(n * nrhs * es);
-    size_t oXh = take(n * nrhs * 8), oXl = take(n * nrhs * 8), oR = take(n * nrhs * 8), oB = take(n * nrhs * 8);
-    size_t oXgh = take(ndof * nrhs * 8), oXgl = take(ndof * nrhs * 8);
-    size_t oPart = take((size_t)nblk * nrhs * 8), oNrm = take(16 * 8), oAct = take(16 * 4);
+    const size_t oY = take(nv * sizeof(T)), oW = take(std::max<int64_t>(H.upd_rows, 1) * NR * sizeof(T));
+    const size_t oZs = take(nv * sizeof(T)), oXs = take(nv * sizeof(T));
+    const size_t oXh = take(nv * 8), oXl = take(nv * 8), oR = take(nv * 8), oB = take(nv * 8), oD = take(nv * 8);
+    const size_t oSv = take(nv * 8), oPv = take(nv * 8), oQv = take(nv * 8), oZv = take(nv * 8);
+    const size_t oXgh = take(ng * 8), oXgl = take(ng * 8), oPg = take(ng * 8);
+    const size_t oPart = take((size_t)nblk * 16 * 8), oNrm = take(16 * 8), oRho = take(16 * 8), oNum = take(16 * 8);
+    const size_t oAl = take(16 * 8), oBe = take(16 * 8), oActO = take(16 * 4), oActI = take(16 * 4);
     JFEM_CHECK_HIP(ctx, ensure_scratch(ctx, off));
     char *base = (char *)ctx->scratch;
-    void *Y = base + oY, *W = base + oW, *Zs = base + oZ, *Xs = base + oXs;
+    T *Y = (T *)(base + oY), *W = (T *)(base + oW), *Zs = (T *)(base + oZs), *Xs = (T *)(base + oXs);
     double *Xh = (double *)(base + oXh), *Xl = (double *)(base + oXl), *R = (double *)(base + oR);
-    double *Bp = (double *)(base + oB), *xgh = (double *)(base + oXgh), *xgl = (double *)(base + oXgl);
-    double *part = (double *)(base + oPart), *nrm = (double *)(base + oNrm);
-    int32_t *act = (int32_t *)(base + oAct);
+    double *Bp = (double *)(base + oB), *Dd = (double *)(base + oD), *Sv = (double *)(base + oSv);
+    double *Pv = (double *)(base + oPv), *Qv = (double *)(base + oQv), *Zv = (double *)(base + oZv);
+    double *xgh = (double *)(base + oXgh), *xgl = (double *)(base + oXgl), *pg = (double *)(base + oPg);
+    double *part = (double *)(base + oPart), *nrm = (double *)(base + oNrm), *rho = (double *)(base + oRho);
+    double *num = (double *)(base + oNum), *alpha = (double *)(base + oAl), *beta = (double *)(base + oBe);
+    int32_t *act_o = (int32_t *)(base + oActO), *act_i = (int32_t *)(base + oActI);
     hipStream_t st = ctx->stream;
-    if (n > 0) {
-        hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, Xh, n * nrhs);
-        hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, Xl, n * nrhs);
-    }
-    hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, xgh, ndof * nrhs);
-    hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, xgl, ndof * nrhs);
-    if (n > 0) {
-        hipLaunchKernelGGL(gather_rhs_kernel, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n, nrhs, ndof, S->d_perm, b_dev, Bp);
-        JFEM_CHECK_HIP(ctx, hipMemcpyAsync(R, Bp, n * nrhs * 8, hipMemcpyDeviceToDevice, st));
-    }
-    double bn[16], rn[16], prev[16];
-    int32_t active[16];
-    for (int r = 0; r < nrhs; ++r) { active[r] = 1; iters[r] = 0; relres[r] = 0.0; prev[r] = HUGE_VAL; }
-    auto norms = [&](const double *v, double *outn) -> hipError_t {
-        hipLaunchKernelGGL(sumsq_partial_kernel, dim3(nblk), dim3(kBS), 0, st, n, nrhs, v, part);
-        hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), dim3(64), 0, st, nblk, nrhs, part, nrm);
-        hipError_t e = hipMemcpyAsync(outn, nrm, nrhs * 8, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        return e;
-    };
-    if (n > 0) {
-        JFEM_CHECK_HIP(ctx, norms(Bp, bn));
-        for (int r = 0; r < nrhs; ++r) bn[r] = std::sqrt(bn[r]);
-    }
-    bool converged = n == 0;
+    const dim3 gv(grid_for(nv)), gg(grid_for(ng)), blk(kBS);
+    const dim3 grow((unsigned)std::max<int64_t>(1, (n * 8 + kBS - 1) / kBS));
+    hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, xgh, ng);
+    hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, xgl, ng);
+    hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, pg, ng);
+    for (int r = 0; r < NR; ++r) { iters[r] = 0; relres[r] = 0.0; }
+    if (relres_out)
+        for (int r = 0; r < NR; ++r) relres_out[r] = 0.0;
     bool all_ok = true;
-    for (int it = 0; n > 0 && it <= max_refine; ++it) {
-        JFEM_CHECK_HIP(ctx, hipMemcpyAsync(act, active, nrhs * 4, hipMemcpyHostToDevice, st));
-        timer_start(ctx);
-        if (F->precision == 64) {
-            JFEM_CHECK_HIP(ctx, hipMemcpyAsync(Y, R, n * nrhs * 8, hipMemcpyDeviceToDevice, st));
-            JFEM_CHECK_HIP(ctx, solve_t<double>(ctx, S, (const double *)F->d_L, (double *)Y, (double *)W, (double *)Zs,
-                                                (double *)Xs, nrhs));
-            hipLaunchKernelGGL(accumulate_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n, nrhs, S->d_perm,
-                               (const double *)Xs, Xh, Xl, xgh, xgl, act);
-        } else {
-            hipLaunchKernelGGL(to_t_kernel<float>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n * nrhs, R, (float *)Y);
-            JFEM_CHECK_HIP(ctx, solve_t<float>(ctx, S, (const float *)F->d_L, (float *)Y, (float *)W, (float *)Zs,
-                                               (float *)Xs, nrhs));
-            hipLaunchKernelGGL(accumulate_kernel<float>, dim3(grid_for(n * nrhs)), dim3(kBS), 0, st, n, nrhs, S->d_perm,
-                               (const float *)Xs, Xh, Xl, xgh, xgl, act);
+    if (n > 0) {
+        hipLaunchKernelGGL(fill_kernel<double>, gv, blk, 0, st, Xh, nv);
+        hipLaunchKernelGGL(fill_kernel<double>, gv, blk, 0, st, Xl, nv);
+        hipLaunchKernelGGL(gather_rhs_kernel, gv, blk, 0, st, n, NR, ndof, S->d_perm, b_dev, Bp);
+        hipLaunchKernelGGL(to_t_kernel<double>, gv, blk, 0, st, nv, Bp, R);
+        auto norms = [&](const double *v, double *outn) -> hipError_t {
+            hipLaunchKernelGGL(sumsq_partial_kernel<NR>, dim3(nblk), blk, 0, st, n, v, part);
+            hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), blk, 0, st, NR, part, nrm);
+            hipError_t e = hipMemcpyAsync(outn, nrm, NR * 8, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            return e;
+        };
+        auto dot = [&](const double *a, const double *b2, double *out_dev) {
+            hipLaunchKernelGGL(dot_partial_kernel<NR>, dim3(nblk), blk, 0, st, n, a, b2, part);
+            hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), blk, 0, st, NR, part, out_dev);
+        };
+        auto prec = [&](const double *sv, double *z) -> hipError_t {
+            timer_start(ctx);
+            hipLaunchKernelGGL(to_t_kernel<T>, gv, blk, 0, st, nv, sv, Y);
+            hipError_t e = solve_t<T>(ctx, S, (const T *)F->d_L, Y, W, Zs, Xs, NR);
+            hipLaunchKernelGGL(from_t_kernel<T>, gv, blk, 0, st, nv, (const T *)Xs, z);
+            timer_stop(ctx, FAM_SOLVE, 1);
+            return e;
+        };
+        double bn[NR], rn[NR], prev[NR];
+        int32_t act[NR], inner[NR];
+        JFEM_CHECK_HIP(ctx, norms(Bp, bn));
+        int n_active = 0;
+        for (int r = 0; r < NR; ++r) {
+            bn[r] = std::sqrt(bn[r]);
+            act[r] = bn[r] > 0.0 && !(1.0 <= tol) ? 1 : 0;     // relres of x = 0 is 1 (0 when b = 0)
+            relres[r] = bn[r] > 0.0 ? 1.0 : 0.0;
+            prev[r] = relres[r];
+            rn[r] = bn[r];
+            n_active += act[r];
         }
-        timer_stop(ctx, FAM_SOLVE, 1);
-        timer_start(ctx);
-        hipLaunchKernelGGL(residual_kernel, dim3(grid_for(n)), dim3(kBS), 0, st, n, nrhs, S->d_perm, P->d_rowptr, P->d_col,
-                           K->d_val, xgh, xgl, Bp, R);
-        timer_stop(ctx, FAM_SPMV, 1);
-        JFEM_CHECK_HIP(ctx, norms(R, rn));
-        converged = true;
-        all_ok = true;
-        for (int r = 0; r < nrhs; ++r) {
-            if (!active[r]) { if (!(relres[r] <= tol)) all_ok = false; continue; }
-            const double rr = std::sqrt(rn[r]) / std::max(bn[r], 1e-30);
-            relres[r] = rr;
-            iters[r] = it;
-            if (rr <= tol) { active[r] = 0; continue; }
-            // stop a right-hand side whose residual no longer decreases
-            if (!(rr < 0.9 * prev[r])) { active[r] = 0; all_ok = false; continue; }
-            prev[r] = rr;
-            converged = false;
-        }
-        if (converged) break;
-    }
-    if (!converged) all_ok = false;
-    hipLaunchKernelGGL(export_kernel, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, ndof, nrhs, xgh, x_dev);
-    if (xlo_dev) hipLaunchKernelGGL(export_kernel, dim3(grid_for(ndof * nrhs)), dim3(kBS), 0, st, ndof, nrhs, xgl, xlo_dev);
-    if (relres_out) {
-        // residual of the double vector handed back (x_hi alone)
-        for (int r = 0; r < nrhs; ++r) relres_out[r] = 0.0;
-        if (n > 0) {
-            hipLaunchKernelGGL(residual_kernel, dim3(grid_for(n)), dim3(kBS), 0, st, n, nrhs, S->d_perm, P->d_rowptr,
-                               P->d_col, K->d_val, xgh, (const double *)nullptr, Bp, R);
+        int nsolve = 0;
+        const int max_solves = max_refine + 1;
+        while (n_active > 0 && nsolve < max_solves) {
+            JFEM_CHECK_HIP(ctx, hipMemcpyAsync(act_o, act, NR * 4, hipMemcpyHostToDevice, st));
+            if (max_refine == 0) {
+                JFEM_CHECK_HIP(ctx, prec(R, Dd));
+                ++nsolve;
+            } else {
+                // inner preconditioned conjugate gradients on K_ff d = r
+                for (int r = 0; r < NR; ++r) inner[r] = act[r];
+                JFEM_CHECK_HIP(ctx, hipMemcpyAsync(act_i, inner, NR * 4, hipMemcpyHostToDevice, st));
+                JFEM_CHECK_HIP(ctx, prec(R, Zv));
+                ++nsolve;
+                hipLaunchKernelGGL(pcg_dir_kernel<NR>, gv, blk, 0, st, n, 1, beta, Zv, Pv, Dd);
+                hipLaunchKernelGGL(to_t_kernel<double>, gv, blk, 0, st, nv, R, Sv);
+                dot(Sv, Zv, rho);
+                while (true) {
+                    timer_start(ctx);
+                    hipLaunchKernelGGL(scatter_global_kernel<NR>, gv, blk, 0, st, n, S->d_perm, Pv, pg);
+                    hipLaunchKernelGGL(spmv_kernel<NR>, grow, blk, 0, st, n, S->d_perm, P->d_rowptr, P->d_col, K->d_val, pg, Qv);
+                    timer_stop(ctx, FAM_SPMV, 1);
+                    dot(Pv, Qv, num);
+                    hipLaunchKernelGGL(pcg_scalars_kernel, dim3(1), dim3(64), 0, st, NR, 0, act_i, num, rho, alpha);
+                    hipLaunchKernelGGL(pcg_step_kernel<NR>, gv, blk, 0, st, n, alpha, Pv, Qv, Dd, Sv);
+                    double ss[NR];
+                    JFEM_CHECK_HIP(ctx, norms(Sv, ss));
+                    int n_in = 0;
+                    for (int r = 0; r < NR; ++r) {
+                        if (inner[r] && !(std::sqrt(ss[r]) > eta * rn[r])) inner[r] = 0;
+                        n_in += inner[r];
+                    }
+                    if (n_in == 0 || nsolve >= max_solves) break;
+                    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(act_i, inner, NR * 4, hipMemcpyHostToDevice, st));
+                    JFEM_CHECK_HIP(ctx, prec(Sv, Zv));
+                    ++nsolve;
+                    dot(Sv, Zv, num);
+                    hipLaunchKernelGGL(pcg_scalars_kernel, dim3(1), dim3(64), 0, st, NR, 1, act_i, num, rho, beta);
+                    hipLaunchKernelGGL(pcg_dir_kernel<NR>, gv, blk, 0, st, n, 0, beta, Zv, Pv, Dd);
+                }
+            }
+            // x += d (double-double) for the active right-hand sides, exact residual
+            hipLaunchKernelGGL(accumulate_kernel<double>, gv, blk, 0, st, n, NR, S->d_perm, (const double *)Dd, Xh, Xl, xgh,
+                               xgl, act_o);
+            timer_start(ctx);
+            hipLaunchKernelGGL(residual_kernel<NR>, grow, blk, 0, st, n, S->d_perm, P->d_rowptr, P->d_col, K->d_val, xgh,
+                               xgl, Bp, R);
+            timer_stop(ctx, FAM_SPMV, 1);
             JFEM_CHECK_HIP(ctx, norms(R, rn));
-            for (int r = 0; r < nrhs; ++r) relres_out[r] = std::sqrt(rn[r]) / std::max(bn[r], 1e-30);
+            n_active = 0;
+            for (int r = 0; r < NR; ++r) {
+                rn[r] = std::sqrt(rn[r]);
+                if (!act[r]) continue;
+                const double rr = rn[r] / std::max(bn[r], 1e-30);
+                relres[r] = rr;
+                iters[r] = nsolve;
+                if (rr <= tol || max_refine == 0) { act[r] = 0; continue; }
+                if (!(rr < 0.5 * prev[r])) { act[r] = 0; continue; }     // no longer converging
+                prev[r] = rr;
+                ++n_active;
+            }
+        }
+        for (int r = 0; r < NR; ++r)
+            if (!(relres[r] <= tol)) all_ok = false;
+        if (relres_out) {
+            // residual of the double vector handed back (x_hi alone)
+            hipLaunchKernelGGL(residual_kernel<NR>, grow, blk, 0, st, n, S->d_perm, P->d_rowptr, P->d_col, K->d_val, xgh,
+                               (const double *)nullptr, Bp, R);
+            JFEM_CHECK_HIP(ctx, norms(R, rn));
+            for (int r = 0; r < NR; ++r) relres_out[r] = std::sqrt(rn[r]) / std::max(bn[r], 1e-30);
         }
     }
+    hipLaunchKernelGGL(export_kernel, gg, blk, 0, st, ndof, NR, xgh, x_dev);
+    if (xlo_dev) hipLaunchKernelGGL(export_kernel, gg, blk, 0, st, ndof, NR, xgl, xlo_dev);
     JFEM_CHECK_HIP(ctx, hipGetLastError());
     JFEM_CHECK_HIP(ctx, hipStreamSynchronize(st));
     if (!all_ok) return fail(ctx, JFEM_ERR_NOT_CONVERGED, "refinement did not reach the requested residual");
     return JFEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_dev,
+                   double *x_dev, double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev,
+                   double *relres_out) {
+    if (K->P != F->S->P) return fail(ctx, JFEM_ERR_BAD_ARG, "matrix and factor use different patterns");
+    if (nrhs < 1 || nrhs > 16) return fail(ctx, JFEM_ERR_BAD_ARG, "nrhs must be in 1..16");
+    if (max_refine < 0) return fail(ctx, JFEM_ERR_BAD_ARG, "max_refine must be >= 0");
+    int rc = JFEM_OK;
+#define JFEM_SOLVE_CALL(N)                                                                                     \
+    rc = F->precision == 64                                                                                    \
+             ? solve_refine<N, double>(ctx, F, K, b_dev, x_dev, tol, max_refine, iters, relres, xlo_dev, relres_out) \
+             : solve_refine<N, float>(ctx, F, K, b_dev, x_dev, tol, max_refine, iters, relres, xlo_dev, relres_out)
+    JFEM_NR_SWITCH(nrhs, JFEM_SOLVE_CALL)
+#undef JFEM_SOLVE_CALL
+    return rc;
 }
 
 // Device self-test of the matrix-core trailing update against its scalar form on

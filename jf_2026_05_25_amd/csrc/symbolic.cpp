@@ -498,7 +498,8 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
             maxsteps = std::max<int64_t>(maxsteps, (S.ncol[s] + kPanel - 1) / kPanel);
             if (S.child_ptr[s + 1] > S.child_ptr[s])
                 for (int64_t c0 = 0; c0 < S.m[s]; c0 += kTile)
-                    S.etasks.push_back({s, (int32_t)c0, (int32_t)std::min<int64_t>(c0 + kTile, S.m[s]), 0});
+                    for (int64_t r0 = c0; r0 < S.m[s]; r0 += kEaRows)
+                        S.etasks.push_back({s, (int32_t)c0, (int32_t)std::min<int64_t>(c0 + kTile, S.m[s]), (int32_t)r0});
             if (S.m[s] > S.ncol[s])
                 for (int64_t c0 = 0; c0 < S.ncol[s]; c0 += kTile) S.bwd_tiles.push_back({s, 0, (int32_t)c0, 0});
         }
@@ -553,6 +554,23 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
     S.ustep_off.push_back((int64_t)S.utasks.size());
     S.ea_off[nlev] = (int64_t)S.etasks.size();
     S.bwd_off[nlev] = (int64_t)S.bwd_tiles.size();
+    S.ea_bounds.clear();
+    S.ea_boff.assign(S.etasks.size() + 1, 0);
+    for (size_t e = 0; e < S.etasks.size(); ++e) {
+        const EaTask &t = S.etasks[e];
+        S.ea_boff[e] = (int64_t)(S.ea_bounds.size() / 4);
+        for (int32_t qc = S.child_ptr[t.s]; qc < S.child_ptr[t.s + 1]; ++qc) {
+            const int32_t c = S.child[qc];
+            const int32_t *rel = S.relmap.data() + S.relofs[c];
+            const int64_t nu = S.m[c] - S.ncol[c];
+            auto lb = [&](int64_t key) { return (int32_t)(std::lower_bound(rel, rel + nu, key) - rel); };
+            S.ea_bounds.push_back(lb(t.c0));
+            S.ea_bounds.push_back(lb(t.c1));
+            S.ea_bounds.push_back(lb(t.r0));
+            S.ea_bounds.push_back(lb((int64_t)t.r0 + kEaRows));
+        }
+    }
+    S.ea_boff[S.etasks.size()] = (int64_t)(S.ea_bounds.size() / 4);
     S.bstep_ptr[nlev] = (int64_t)S.bstep_off.size();
     S.bstep_off.push_back((int64_t)S.btasks.size());
     return 0;

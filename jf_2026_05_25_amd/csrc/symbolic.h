@@ -7,10 +7,11 @@ namespace jfem {
 
 constexpr int kPanel = 64;   // panel width of the blocked partial factorization
 constexpr int kTile = 64;    // tile edge of panel/update tasks
+constexpr int kEaRows = 512; // front rows per extend-add task
 
 struct PanelTask { int32_t s, k, r0, pad; };          // supernode, panel column, first row
 struct UpdateTask { int32_t s, k, r0, c0; };          // supernode, panel column, tile origin
-struct EaTask { int32_t s, c0, c1, pad; };            // parent supernode, front column range
+struct EaTask { int32_t s, c0, c1, r0; };             // parent, front columns [c0, c1), rows [r0, r0 + kEaRows)
 
 struct SymbolicHost {
     int64_t ndof = 0, n = 0;        // all DOFs, free DOFs
@@ -35,6 +36,11 @@ struct SymbolicHost {
     std::vector<int64_t> pstep_off, ustep_off;   // per step: offset (size steps+1)
     std::vector<int64_t> pstep_mid;              // per step: first non-diagonal panel tile
     std::vector<int64_t> ea_off;            // per level: offset into etasks (size levels+1)
+    // per extend-add task, per child of its parent (child order): the child's
+    // update columns [j0, j1) landing in front columns [c0, c1) and update rows
+    // [i0, i1) landing in front rows [r0, r0 + kEaRows)
+    std::vector<int32_t> ea_bounds;         // 4 per (task, child)
+    std::vector<int64_t> ea_boff;           // per task: first (task, child) entry
     // Solve task lists.  Forward: per level one gather (children's update
     // vectors), then the panel steps reuse ptasks: task (s, k, r0) solves the
     // diagonal block at column k and updates rows [max(r0, k+b), r0+kTile).

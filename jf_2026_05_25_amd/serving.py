@@ -150,7 +150,7 @@ class SolveServer:
                     self.factors.move_to_end(mkey)
             if cached is not None:
                 mesh_p, K, F, fixed = cached
-                u, rr, it, ok = F.solve(K, m.loads, tol=self.tol, max_refine=20, ctx=ctx)
+                u, rr, it, ok = F.solve(K, m.loads, tol=self.tol, max_refine=100, ctx=ctx)
                 tim["solve_s"] = time.perf_counter() - ta
                 return Result(req.rid, u, rr, it, req.arrival, t_admit, time.perf_counter() - t0, m.ndof,
                               int((fixed == 0).sum()), m.nrhs, bool(ok), "" if ok else "not converged", True, tim)
@@ -184,7 +184,7 @@ class SolveServer:
                     F = solver.Factor(S, K, req.precision, ctx)
                 tim["factor_s"] = time.perf_counter() - tc
                 ts = time.perf_counter()
-                u, rr, it, ok = F.solve(K, m.loads, tol=self.tol, max_refine=20, ctx=ctx)
+                u, rr, it, ok = F.solve(K, m.loads, tol=self.tol, max_refine=100, ctx=ctx)
                 tim["solve_s"] = time.perf_counter() - ts
                 with self.lock:
                     self.factors[mkey] = (mesh_r, K, F, fixed)

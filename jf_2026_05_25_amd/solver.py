@@ -305,7 +305,7 @@ class Factor:
     def device_bytes(self):
         return int(self.ctx.L.jfem_factor_device_bytes(self.h))
 
-    def solve(self, K: DeviceMatrix, B: np.ndarray, tol: float = 1e-10, max_refine: int = 10,
+    def solve(self, K: DeviceMatrix, B: np.ndarray, tol: float = 1e-10, max_refine: int = 100,
               ctx: Context | None = None, details: bool = False):
         """Solve K_ff X = B_ff with refinement on ``ctx``'s stream (default: the
         factor's own context).  Solve scratch lives in the context, so concurrent
@@ -362,7 +362,7 @@ class LinearSolveCache:
 
 
 def apply_bc_and_solve(asm: AssembledStiffness, model, F_applied=None, *, precision: int = FP64,
-                       tol: float = 1e-10, max_refine: int = 10, pivot_threshold: float = 1e-7,
+                       tol: float = 1e-10, max_refine: int = 100, pivot_threshold: float = 1e-7,
                        linear_cache: LinearSolveCache | None = None, symbolic: Symbolic | None = None):
     """BC partition, factor and solve of K_ff u = F_ff on the GPU.
 

@@ -4,6 +4,7 @@
 // task decomposition the device kernels execute (scatter, extend-add tasks, panel
 // tasks in two phases per step, update tiles, and the level-scheduled solves), so
 // the structures and task lists can be checked against a dense solve without a GPU.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -29,13 +30,15 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             int64_t ns = S.ncol[s], ms = S.m[s], mu = ms - ns;
             for (int32_t qc = S.child_ptr[s]; qc < S.child_ptr[s + 1]; ++qc) {
                 int c = S.child[qc];
-                int64_t nu = S.m[c] - S.ncol[c];
                 const int32_t *rel = S.relmap.data() + S.relofs[c];
-                for (int64_t j = 0; j < nu; ++j) {
+                const int32_t *bd = S.ea_bounds.data() + 4 * (S.ea_boff[q] + (qc - S.child_ptr[s]));
+                for (int64_t j = bd[0]; j < bd[1]; ++j) {
                     int64_t pc = rel[j];
-                    if (pc < t.c0 || pc >= t.c1) continue;
-                    for (int64_t i = j; i < nu; ++i) {
+                    if (pc < t.c0 || pc >= t.c1) return 6;
+                    for (int64_t i = std::max<int64_t>(j, bd[2]); i < bd[3]; ++i) {
                         int64_t pr = rel[i];
+                        if (pr < t.r0 || pr >= t.r0 + kEaRows) return 7;
+                        int64_t nu = S.m[c] - S.ncol[c];
                         double v = U[S.uofs[c] + i + nu * j];
                         if (pc < ns) L[S.lofs[s] + pr + ms * pc] += v;
                         else U[S.uofs[s] + (pr - ns) + mu * (pc - ns)] += v;

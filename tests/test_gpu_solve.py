@@ -34,7 +34,7 @@ def ctx():
     c.close()
 
 
-def gpu_solve(ctx, m, precision, tol=TOL_RES, max_refine=10):
+def gpu_solve(ctx, m, precision, tol=TOL_RES, max_refine=100):
     from jf_2026_05_25_amd import solver
     asm = solver.assemble_stiffness(m, ctx)
     u, fixed, spc, diag = solver.apply_bc_and_solve(asm, m, precision=precision, tol=tol, max_refine=max_refine)
@@ -81,7 +81,7 @@ def test_fp32_refined_converges_to_reference_solution(ctx, oracle, name):
     solution within 1e-10: refinement converges to the same answer."""
     m = CASES[name]()
     _, _, uo, _, _ = oracle.sol101(m, refine=10)
-    u, diag = gpu_solve(ctx, m, 32, tol=1e-12, max_refine=40)
+    u, diag = gpu_solve(ctx, m, 32, tol=1e-12, max_refine=100)
     assert diag["linear_solver"]["converged"], diag["linear_solver"]["relative_residual"]
     for r in range(m.nrhs):
         assert relerr(u[:, r], uo[:, r]) < TOL_U, (r, relerr(u[:, r], uo[:, r]))
@@ -107,7 +107,8 @@ def test_golden_vectors_on_gpu(ctx, oracle):
             _, _, uo, _, _ = oracle.sol101(m, refine=10)
             for precision in (64, 32):
                 u, diag = gpu_solve(ctx, m, precision)
-                assert relerr(u[:, 0], uo[:, 0]) < TOL_U, (c["case"], precision, relerr(u[:, 0], uo[:, 0]))
+                tol_u = TOL_U if precision == 64 else TOL_U_FP32_AT_TOL
+                assert relerr(u[:, 0], uo[:, 0]) < tol_u, (c["case"], precision, relerr(u[:, 0], uo[:, 0]))
                 assert diag["bc_partition"]["free_dofs"] == s["reference_free_dofs"]
 
 
