@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libjfem_hip.so")
 
 JFEM_OK = 0
+JFEM_ERR_NOT_SPD = 6
+JFEM_ERR_NOT_CONVERGED = 8
 STATUS = {1: "HIP error", 2: "degenerate element", 3: "inverted element", 4: "warped element",
           5: "bad argument", 6: "matrix not positive definite", 7: "out of device memory",
           8: "refinement did not converge"}

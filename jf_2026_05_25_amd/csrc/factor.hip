@@ -139,79 +139,108 @@ __global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restric
     }
 }
 
-// ----------------------------------------------------------------- panel kernel
-// first == 1: the task's tile starts at the diagonal: factor the b x b diagonal
-// block of panel k in LDS, store it, and solve the tile rows below it.
-// first == 0: load the stored factored block and solve the tile rows:
-// X <- X L_kk^{-T}.  (Two launches so no workgroup reads a block another one is
-// overwriting.)
+// ----------------------------------------------------------------- panel factorization
+// The b x b diagonal block of panel k is factored by one wavefront holding one
+// row per lane in registers (identity beyond b); pivots and multipliers are
+// broadcast with readlane, so there is no barrier in the loop.  On return lane i
+// holds row i of the factor L_kk in r[0..i].
+// value of v in lane `lane` (uniform lane index), for all lanes of the wave
 template <class T>
-__global__ void __launch_bounds__(kBS) panel_kernel(const PanelTask *__restrict__ tasks, int first,
-                                                    const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
-                                                    const int64_t *__restrict__ lofs, const int64_t *__restrict__ col0,
-                                                    T *__restrict__ L, int64_t *__restrict__ status) {
-    __shared__ T D[kPanel][kPanel + 1];
-    __shared__ T X[kTile][kPanel + 1];
+__device__ inline T bcast(T v, int lane);
+template <>
+__device__ inline float bcast<float>(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+template <>
+__device__ inline double bcast<double>(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+template <class T>
+__device__ __forceinline__ void wave_potrf(T (&r)[kPanel], int lane, int b, int64_t col_base, int64_t *status) {
+    int bad = kPanel;                         // first non-positive pivot (uniform)
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        const T d0 = bcast(r[j], j);
+        const bool ok = d0 > T(0);
+        bad = (!ok && j < b && bad == kPanel) ? j : bad;
+        const T sq = tsqrt(ok ? d0 : T(1));
+        const T inv = T(1) / sq;
+        r[j] = lane == j ? sq : (lane > j ? r[j] * inv : r[j]);
+#pragma unroll
+        for (int c = j + 1; c < kPanel; ++c) r[c] -= r[j] * bcast(r[j], c);
+    }
+    if (bad < kPanel && lane == 0) report_min(status, col_base + bad, JFEM_ERR_NOT_SPD);
+}
+
+// factor the diagonal blocks of the first panels of a level's supernodes (one
+// wavefront per task; tasks are the level's step-0 diagonal panel tasks)
+template <class T>
+__global__ void __launch_bounds__(64) diag_factor_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+                                                         const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
+                                                         const int64_t *__restrict__ col0, T *__restrict__ L,
+                                                         int64_t *__restrict__ status) {
     const PanelTask t = tasks[blockIdx.x];
     const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
     const int b = (int)min<int64_t>(kPanel, ns - k);
     T *Ls = L + lofs[s];
-    const int tid = threadIdx.x;
-    for (int q = tid; q < b * b; q += blockDim.x) {
-        int i = q % b, j = q / b;
-        D[i][j] = (i >= j) ? Ls[(k + i) + ms * (k + j)] : T(0);
+    const int lane = threadIdx.x;
+    const int li = min(lane, b - 1);
+    T r[kPanel];
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
+        r[j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
     }
-    __syncthreads();
-    if (first) {
-        // right-looking Cholesky of the diagonal block
-        for (int j = 0; j < b; ++j) {
-            if (tid == 0) {
-                T d = D[j][j];
-                if (!(d > T(0))) {
-                    report_min(status, col0[s] + k + j, JFEM_ERR_NOT_SPD);
-                    d = T(1);
-                }
-                D[j][j] = tsqrt(d);
-            }
-            __syncthreads();
-            const T djj = D[j][j];
-            for (int i = j + 1 + tid; i < b; i += blockDim.x) D[i][j] /= djj;
-            __syncthreads();
-            for (int q = tid; q < (b - j - 1) * (b - j - 1); q += blockDim.x) {
-                int i = j + 1 + q % (b - j - 1), c = j + 1 + q / (b - j - 1);
-                if (i >= c) D[i][c] -= D[i][j] * D[c][j];
-            }
-            __syncthreads();
-        }
-        for (int q = tid; q < b * b; q += blockDim.x) {
-            int i = q % b, j = q / b;
-            if (i >= j) Ls[(k + i) + ms * (k + j)] = D[i][j];
-        }
+    wave_potrf(r, lane, b, col0[s] + k, status);
+    if (lane < b) {
+#pragma unroll
+        for (int j = 0; j < kPanel; ++j)
+            if (j <= lane) Ls[(k + lane) + ms * (k + j)] = r[j];
     }
-    const int64_t r0 = t.r0;
-    const int64_t r1 = min<int64_t>(r0 + kTile, ms);
-    const int64_t rs = max<int64_t>(r0, k + b);
-    const int nr = (int)max<int64_t>(0, r1 - rs);
-    if (nr == 0) return;
-    for (int q = tid; q < nr * b; q += blockDim.x) {
-        int i = q % nr, j = q / nr;
-        X[i][j] = Ls[(rs + i) + ms * (k + j)];
+}
+
+// tile solve of panel task (s, k, r0): rows [max(r0, k+b), r0+kTile) of the panel
+// get X <- X L_kk^{-T} (L_kk already factored).  One wavefront per task, lane =
+// row: lane i holds its tile row x and row i of L_kk; L_kk entries are taken by
+// readlane, so the solve touches no memory.
+template <class T>
+__global__ void __launch_bounds__(64) tile_solve_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+                                                        const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
+                                                        T *__restrict__ L) {
+    const PanelTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
+    const int b = (int)min<int64_t>(kPanel, ns - k);
+    const int64_t rs = max<int64_t>(t.r0, k + b), re = min<int64_t>(t.r0 + kTile, ms);
+    if (rs >= re) return;
+    T *Ls = L + lofs[s];
+    const int lane = threadIdx.x;
+    const int li = min(lane, b - 1);
+    const int64_t row = t.r0 + lane;
+    const int64_t rowc = min<int64_t>(max<int64_t>(row, rs), re - 1);
+    T d[kPanel], x[kPanel];
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
+        d[j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
+        const T w = Ls[rowc + ms * (k + min(j, b - 1))];
+        x[j] = j < b ? w : T(0);
     }
-    __syncthreads();
-    // X <- X L^{-T}: column by column, rows in parallel
-    if (tid < nr) {
-        const int i = tid;
-        for (int j = 0; j < b; ++j) {
-            T v = X[i][j];
-            for (int p = 0; p < j; ++p) v -= X[i][p] * D[j][p];
-            X[i][j] = v / D[j][j];
-        }
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        x[j] = x[j] / bcast(d[j], j);
+#pragma unroll
+        for (int c = j + 1; c < kPanel; ++c) x[c] -= x[j] * bcast(d[j], c);
     }
-    __syncthreads();
-    for (int q = tid; q < nr * b; q += blockDim.x) {
-        int i = q % nr, j = q / nr;
-        Ls[(rs + i) + ms * (k + j)] = X[i][j];
+    if (row >= rs && row < re) {
+#pragma unroll
+        for (int j = 0; j < kPanel; ++j)
+            if (j < b) Ls[row + ms * (k + j)] = x[j];
     }
 }
 
@@ -291,10 +320,15 @@ struct Mfma<float> {
     static __device__ inline int drow(int blk, int v) { return 4 * blk + v; }
 };
 
+// factor_next != 0: the task whose tile holds the next panel's diagonal block
+// (r0 == c0 == k + b < ns) also factors that block after its update (through
+// LDS, by wavefront 0), so the next step's tile solves can start at once.
 template <class T>
 __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                      const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
-                                                     const int64_t *__restrict__ uofs, T *__restrict__ L, T *__restrict__ U) {
+                                                     const int64_t *__restrict__ uofs, T *__restrict__ L, T *__restrict__ U,
+                                                     int factor_next, const int64_t *__restrict__ col0,
+                                                     int64_t *__restrict__ status) {
     typedef typename Mfma<T>::v4 v4;
     __shared__ T A[kPanel][kTile + 1];   // A[p][i] = L[r0 + i, k + p]
     __shared__ T B[kPanel][kTile + 1];   // B[p][j] = L[c0 + j, k + p]
@@ -335,13 +369,33 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
     T *Cs = inL ? (L + lofs[s]) : (U + uofs[s]);
     const int64_t ld = inL ? ms : ms - ns;
     const int64_t off = inL ? 0 : ns;
+    const bool next = factor_next && r0 == c0 && c0 == k + b && c0 < ns;   // uniform
+    if (next) __syncthreads();                 // all waves done reading A/B: A becomes the staging tile
     for (int x = 0; x < 2; ++x)
         for (int y = 0; y < 2; ++y)
             for (int v = 0; v < 4; ++v) {
                 const int i = wi + 16 * x + l16;           // row of C within the tile
                 const int j = wj + 16 * y + Mfma<T>::drow(l4, v);   // column of C within the tile
-                if (i < nr && j < nc && r0 + i >= c0 + j) Cs[(r0 + i - off) + ld * (c0 + j - off)] -= acc[x][y][v];
+                if (i < nr && j < nc && r0 + i >= c0 + j) {
+                    T *dst = Cs + (r0 + i - off) + ld * (c0 + j - off);
+                    const T nv = *dst - acc[x][y][v];
+                    *dst = nv;
+                    if (next) A[j][i] = nv;            // A[col][row] of the next diagonal block
+                }
             }
+    if (!next) return;
+    __syncthreads();
+    if (wave != 0) return;
+    const int bn = nr;                         // = min(kPanel, ns - c0)
+    T r[kPanel];
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) r[j] = (j <= lane && lane < bn) ? A[j][lane] : (j == lane ? T(1) : T(0));
+    wave_potrf(r, lane, bn, col0[s] + c0, status);
+    if (lane < bn) {
+#pragma unroll
+        for (int j = 0; j < kPanel; ++j)
+            if (j <= lane) L[lofs[s] + (c0 + lane) + ms * (c0 + j)] = r[j];
+    }
 }
 
 // ----------------------------------------------------------------- solves
@@ -352,21 +406,6 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
 // that small solve from the pending values (no workgroup of the launch writes
 // them) and then applies its own tile, so a step is one launch.  Wave w of a
 // 256-thread workgroup handles right-hand sides r = w, w+4, w+8, w+12.
-
-// value of v in lane `lane` (uniform lane index), for all lanes of the wave
-template <class T>
-__device__ inline T bcast(T v, int lane);
-template <>
-__device__ inline float bcast<float>(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-template <>
-__device__ inline double bcast<double>(double v, int lane) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
-}
 
 // per supernode of the level: W_s = 0, then add the children's update vectors
 // (in child order) into Y_s (pivot rows) and W_s (update rows)
@@ -945,22 +984,28 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
                                S->d_relofs, S->d_relmap, S->d_child_ptr, S->d_child, L, U);
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_EA, 1); }
         }
-        for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
-            int64_t nd = H.pstep_mid[stp] - H.pstep_off[stp];
-            int64_t nr = H.pstep_off[stp + 1] - H.pstep_mid[stp];
-            int64_t nu = H.ustep_off[stp + 1] - H.ustep_off[stp];
+        if (H.step_ptr[l + 1] > H.step_ptr[l]) {
+            // diagonal blocks of the level's first panels
+            const int64_t stp = H.step_ptr[l];
+            const int64_t nd = H.pstep_mid[stp] - H.pstep_off[stp];
             if (detail) (void)hipEventRecord(ctx->ev2, st);
             if (nd)
-                hipLaunchKernelGGL(panel_kernel<T>, dim3((unsigned)nd), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp], 1,
+                hipLaunchKernelGGL(diag_factor_kernel<T>, dim3((unsigned)nd), dim3(64), 0, st, S->d_ptasks + H.pstep_off[stp],
                                    S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status);
-            if (nr)
-                hipLaunchKernelGGL(panel_kernel<T>, dim3((unsigned)nr), dim3(kBS), 0, st, S->d_ptasks + H.pstep_mid[stp], 0,
-                                   S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status);
-            if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, (nd ? 1 : 0) + (nr ? 1 : 0)); }
+            if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, nd ? 1 : 0); }
+        }
+        for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
+            const int64_t np = H.pstep_off[stp + 1] - H.pstep_off[stp];
+            const int64_t nu = H.ustep_off[stp + 1] - H.ustep_off[stp];
+            if (detail) (void)hipEventRecord(ctx->ev2, st);
+            if (np)
+                hipLaunchKernelGGL(tile_solve_kernel<T>, dim3((unsigned)np), dim3(64), 0, st, S->d_ptasks + H.pstep_off[stp],
+                                   S->d_ncol, S->d_m, S->d_lofs, L);
+            if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, np ? 1 : 0); }
             if (nu) {
                 if (detail) (void)hipEventRecord(ctx->ev2, st);
                 hipLaunchKernelGGL(update_kernel<T>, dim3((unsigned)nu), dim3(kBS), 0, st, S->d_utasks + H.ustep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, L, U);
+                                   S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, L, U, 1, S->d_col0, ctx->d_status);
                 if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_UPDATE, 1); }
             }
         }
@@ -1425,10 +1470,10 @@ int jfem_selftest_update(jfem_ctx *ctx, int precision, double *max_rel_diff) {
     }
     dim3 g((unsigned)tasks.size());
     if (precision == 64) {
-        hipLaunchKernelGGL(update_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[0], (double *)dU[0]);
+        hipLaunchKernelGGL(update_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[0], (double *)dU[0], 0, d_meta, ctx->d_status);
         hipLaunchKernelGGL(update_ref_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[1], (double *)dU[1]);
     } else {
-        hipLaunchKernelGGL(update_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[0], (float *)dU[0]);
+        hipLaunchKernelGGL(update_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[0], (float *)dU[0], 0, d_meta, ctx->d_status);
         hipLaunchKernelGGL(update_ref_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[1], (float *)dU[1]);
     }
     JFEM_CHECK_HIP(ctx, hipGetLastError());

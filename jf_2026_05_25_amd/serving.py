@@ -174,14 +174,15 @@ class SolveServer:
             self._admit_bytes(need)
             try:
                 tc = time.perf_counter()
-                F = solver.Factor(S, K, req.precision, ctx)
+                F, used = solver.factor_with_fallback(S, K, req.precision, ctx)
                 small = F.small_pivots(K)
                 if len(small):
                     F.free()
                     fixed = fixed.copy()
                     fixed[small] = 1
                     S = solver.Symbolic(K, mesh, fixed, ctx)
-                    F = solver.Factor(S, K, req.precision, ctx)
+                    F, used = solver.factor_with_fallback(S, K, req.precision, ctx)
+                tim["factor_precision"] = used
                 tim["factor_s"] = time.perf_counter() - tc
                 ts = time.perf_counter()
                 u, rr, it, ok = F.solve(K, m.loads, tol=self.tol, max_refine=100, ctx=ctx)

@@ -347,6 +347,19 @@ class Factor:
             pass
 
 
+def factor_with_fallback(S: Symbolic, K: DeviceMatrix, precision: int, ctx: Context | None = None):
+    """Numeric factor in ``precision``; an FP32 factor that meets a non-positive
+    pivot (cond(K_ff)·2^-24 too large) is redone in FP64, the way the reference
+    falls back from a failed Cholesky (src/solver/boundary_conditions.jl:1660-1710).
+    Returns ``(factor, precision_used)``; an FP64 breakdown raises."""
+    try:
+        return Factor(S, K, precision, ctx), precision
+    except _lib.JfemError as e:
+        if e.code != _lib.JFEM_ERR_NOT_SPD or precision == FP64:
+            raise
+    return Factor(S, K, FP64, ctx), FP64
+
+
 class LinearSolveCache:
     """Factor reuse across right-hand sides sharing one K and one free set
     (LinearSolveCacheEntry, src/solver/boundary_conditions.jl:62-125)."""
@@ -379,7 +392,7 @@ def apply_bc_and_solve(asm: AssembledStiffness, model, F_applied=None, *, precis
         fixed = bc_partition(asm.K, model.spc)
         S = symbolic if (symbolic is not None and np.array_equal(symbolic.fixed, fixed)) else Symbolic(asm.K, asm.mesh, fixed, ctx)
         t1 = time.perf_counter()
-        Fct = Factor(S, asm.K, precision, ctx)
+        Fct, used = factor_with_fallback(S, asm.K, precision, ctx)
         small = Fct.small_pivots(asm.K, pivot_threshold)
         n_post = 0
         if len(small):   # post-factor pivot check: extra SPC and refactor (:1684-1710)
@@ -388,15 +401,15 @@ def apply_bc_and_solve(asm: AssembledStiffness, model, F_applied=None, *, precis
             fixed[small] = 1
             Fct.free(); S.free()
             S = Symbolic(asm.K, asm.mesh, fixed, ctx)
-            Fct = Factor(S, asm.K, precision, ctx)
+            Fct, used = factor_with_fallback(S, asm.K, precision, ctx)
         t2 = time.perf_counter()
-        entry = (fixed, S, Fct, t1 - t0, t2 - t1, n_post)
+        entry = (fixed, S, Fct, t1 - t0, t2 - t1, n_post, used)
         if linear_cache is not None:
             linear_cache.put(key, entry)
         diag["linear_solver"]["cache_hit"] = False
     else:
         diag["linear_solver"]["cache_hit"] = True
-    fixed, S, Fct, t_sym, t_num, n_post = entry
+    fixed, S, Fct, t_sym, t_num, n_post, used = entry
     t3 = time.perf_counter()
     u, rr, iters, ok, extra = Fct.solve(asm.K, F, tol=tol, max_refine=max_refine, details=True)
     t4 = time.perf_counter()
@@ -405,7 +418,8 @@ def apply_bc_and_solve(asm: AssembledStiffness, model, F_applied=None, *, precis
                                  "explicit_spc_dofs": int(spc.sum()),
                                  "autospc_dofs": int(((fixed == 1) & (spc == 0)).sum()),
                                  "post_factorization_singular_dofs": n_post})
-    diag["linear_solver"].update({"backend": f"jfem_hip_supernodal_fp{precision}", "relative_residual": rr.tolist(),
+    diag["linear_solver"].update({"backend": f"jfem_hip_supernodal_fp{used}", "factor_precision": used,
+                                  "fp32_breakdown_fallback": used != precision, "relative_residual": rr.tolist(),
                                   "relative_residual_fp64_output": extra["relres_fp64"].tolist(),
                                   "refinement_iterations": iters.tolist(), "converged": bool(ok),
                                   "symbolic": S.info, "time_symbolic_s": t_sym, "time_factor_s": t_num,

@@ -46,40 +46,50 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 }
             }
         }
-        for (int64_t stp = S.step_ptr[l]; stp < S.step_ptr[l + 1]; ++stp) {
-            for (int phase = 0; phase < 2; ++phase) {
-                int64_t a = phase == 0 ? S.pstep_off[stp] : S.pstep_mid[stp];
-                int64_t e = phase == 0 ? S.pstep_mid[stp] : S.pstep_off[stp + 1];
-                for (int64_t q = a; q < e; ++q) {
-                    PanelTask t = S.ptasks[q];
-                    int s = t.s;
-                    int64_t ns = S.ncol[s], ms = S.m[s], k = t.k;
-                    int b = (int)std::min<int64_t>(kPanel, ns - k);
-                    double *Ls = L.data() + S.lofs[s];
-                    if (phase == 0) {
-                        if (t.r0 != k) return 2;
-                        for (int j = 0; j < b; ++j) {
-                            double d = Ls[(k + j) + ms * (k + j)];
-                            if (!(d > 0)) return 3;
-                            d = std::sqrt(d);
-                            Ls[(k + j) + ms * (k + j)] = d;
-                            for (int i = j + 1; i < b; ++i) Ls[(k + i) + ms * (k + j)] /= d;
-                            for (int c2 = j + 1; c2 < b; ++c2)
-                                for (int i = c2; i < b; ++i)
-                                    Ls[(k + i) + ms * (k + c2)] -= Ls[(k + i) + ms * (k + j)] * Ls[(k + c2) + ms * (k + j)];
-                        }
-                    } else if (t.r0 == k) {
-                        return 4;
-                    }
-                    int64_t r1 = std::min<int64_t>(t.r0 + kTile, ms);
-                    for (int64_t i = std::max<int64_t>(t.r0, k + b); i < r1; ++i)
-                        for (int j = 0; j < b; ++j) {
-                            double v = Ls[i + ms * (k + j)];
-                            for (int p = 0; p < j; ++p) v -= Ls[i + ms * (k + p)] * Ls[(k + j) + ms * (k + p)];
-                            Ls[i + ms * (k + j)] = v / Ls[(k + j) + ms * (k + j)];
-                        }
-                }
+        // factor the b x b diagonal block at column k of supernode s in place
+        auto potrf = [&](int s, int64_t k) -> int {
+            int64_t ns = S.ncol[s], ms = S.m[s];
+            int b = (int)std::min<int64_t>(kPanel, ns - k);
+            double *Ls = L.data() + S.lofs[s];
+            for (int j = 0; j < b; ++j) {
+                double d = Ls[(k + j) + ms * (k + j)];
+                if (!(d > 0)) return 3;
+                d = std::sqrt(d);
+                Ls[(k + j) + ms * (k + j)] = d;
+                for (int i = j + 1; i < b; ++i) Ls[(k + i) + ms * (k + j)] /= d;
+                for (int c2 = j + 1; c2 < b; ++c2)
+                    for (int i = c2; i < b; ++i)
+                        Ls[(k + i) + ms * (k + c2)] -= Ls[(k + i) + ms * (k + j)] * Ls[(k + c2) + ms * (k + j)];
             }
+            return 0;
+        };
+        // step 0 of the level: diagonal blocks of the first panels
+        if (S.step_ptr[l + 1] > S.step_ptr[l]) {
+            int64_t stp = S.step_ptr[l];
+            for (int64_t q = S.pstep_off[stp]; q < S.pstep_mid[stp]; ++q) {
+                PanelTask t = S.ptasks[q];
+                if (t.r0 != t.k || t.k != 0) return 2;
+                if (int e = potrf(t.s, t.k)) return e;
+            }
+        }
+        for (int64_t stp = S.step_ptr[l]; stp < S.step_ptr[l + 1]; ++stp) {
+            // tile solves: rows [max(r0, k+b), r0+kTile) of every panel task, X <- X L_kk^{-T}
+            for (int64_t q = S.pstep_off[stp]; q < S.pstep_off[stp + 1]; ++q) {
+                PanelTask t = S.ptasks[q];
+                int s = t.s;
+                int64_t ns = S.ncol[s], ms = S.m[s], k = t.k;
+                int b = (int)std::min<int64_t>(kPanel, ns - k);
+                double *Ls = L.data() + S.lofs[s];
+                int64_t r1 = std::min<int64_t>(t.r0 + kTile, ms);
+                for (int64_t i = std::max<int64_t>(t.r0, k + b); i < r1; ++i)
+                    for (int j = 0; j < b; ++j) {
+                        double v = Ls[i + ms * (k + j)];
+                        for (int p = 0; p < j; ++p) v -= Ls[i + ms * (k + p)] * Ls[(k + j) + ms * (k + p)];
+                        Ls[i + ms * (k + j)] = v / Ls[(k + j) + ms * (k + j)];
+                    }
+            }
+            // trailing update tiles; the tile holding the next panel's diagonal block
+            // (r0 == c0 == k + b < ns) then factors that block
             for (int64_t q = S.ustep_off[stp]; q < S.ustep_off[stp + 1]; ++q) {
                 UpdateTask t = S.utasks[q];
                 int s = t.s;
@@ -95,6 +105,8 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                         if (c < ns) L[S.lofs[s] + r + ms * c] -= acc;
                         else U[S.uofs[s] + (r - ns) + (ms - ns) * (c - ns)] -= acc;
                     }
+                if (t.r0 == t.c0 && t.c0 == k + b && t.c0 < ns)
+                    if (int e = potrf(s, t.c0)) return e;
             }
         }
     }

@@ -101,15 +101,41 @@ def test_plain_fp64_as_accurate_as_oracle(ctx, oracle, name):
 
 def test_golden_vectors_on_gpu(ctx, oracle):
     from _helpers import golden_cases, golden_model
+    from jf_2026_05_25_amd import _lib
+    bad = []
     for c in golden_cases():
         for s in c["subcases"]:
             m = golden_model(c, s, membrane_modes=6)
             _, _, uo, _, _ = oracle.sol101(m, refine=10)
             for precision in (64, 32):
-                u, diag = gpu_solve(ctx, m, precision)
+                try:
+                    u, diag = gpu_solve(ctx, m, precision)
+                except _lib.JfemError as e:
+                    bad.append((c["case"], precision, str(e)))
+                    continue
                 tol_u = TOL_U if precision == 64 else TOL_U_FP32_AT_TOL
-                assert relerr(u[:, 0], uo[:, 0]) < tol_u, (c["case"], precision, relerr(u[:, 0], uo[:, 0]))
-                assert diag["bc_partition"]["free_dofs"] == s["reference_free_dofs"]
+                err = relerr(u[:, 0], uo[:, 0])
+                if not err < tol_u:
+                    bad.append((c["case"], precision, err))
+                if diag["bc_partition"]["free_dofs"] != s["reference_free_dofs"]:
+                    bad.append((c["case"], precision, "free dofs"))
+    assert not bad, bad
+
+
+def test_fp32_breakdown_falls_back_to_fp64(ctx, oracle):
+    """A model with cond(K_ff)·2^-24 >> 1 (refined curved beam, cond 1.7e9): the
+    FP32 factor may meet a non-positive pivot; the solve then refactors in FP64
+    and still reaches the reference solution."""
+    from _helpers import golden_cases, golden_model
+    c = [g for g in golden_cases() if g["case"] == "MH_curved_beam_refined"][0]
+    m = golden_model(c, c["subcases"][0], membrane_modes=6)
+    _, _, uo, _, _ = oracle.sol101(m, refine=10)
+    u, diag = gpu_solve(ctx, m, 32)
+    ls = diag["linear_solver"]
+    assert ls["converged"]
+    tol_u = TOL_U if ls["factor_precision"] == 64 else TOL_U_FP32_AT_TOL
+    assert relerr(u[:, 0], uo[:, 0]) < tol_u
+    assert ls["fp32_breakdown_fallback"] == (ls["factor_precision"] == 64)
 
 
 def test_edge_cases(ctx, oracle):
