@@ -189,6 +189,7 @@ def main():
         u32, u64 = r["u"], r64["u"]
         out["fp32_refined_error_vs_fp64"] = float(np.max(np.linalg.norm(u32 - u64, axis=0) /
                                                          np.linalg.norm(u64, axis=0)))
+        out["fp64_kernel_ms"] = {k: round(v[0], 4) for k, v in r64["fam"].items()}
     # kernel-family breakdown and roofline of the dominant kernel (factor update tiles)
     fam = r["fam"]
     out["kernel_ms"] = {k: round(v[0], 4) for k, v in fam.items()}

@@ -96,6 +96,8 @@ int jfem_ctx_destroy(jfem_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->bc_dev) (void)hipFree(ctx->bc_dev);
+    if (ctx->bc_host) (void)hipHostFree(ctx->bc_host);
     (void)hipFree(ctx->d_status);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
@@ -474,24 +476,83 @@ int64_t jfem_matrix_device_bytes(const jfem_matrix *K) {
 // ------------------------------------------------------------------ BC partition
 // src/solver/boundary_conditions.jl:1480-1607 (SPC), 1028-1138 (AUTOSPC with grid
 // point processing on: only exact-zero / negative diagonals), 745-845 (GPST).
+}  // extern "C"
+
+namespace {
+
+hipError_t grow_bc_dev(jfem_ctx *ctx, size_t bytes) {
+    if (ctx->bc_dev_bytes >= bytes) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return e;
+    if (ctx->bc_dev) (void)hipFree(ctx->bc_dev);
+    ctx->bc_dev = nullptr;
+    ctx->bc_dev_bytes = 0;
+    e = hipMalloc(&ctx->bc_dev, bytes);
+    if (e == hipSuccess) ctx->bc_dev_bytes = bytes;
+    return e;
+}
+
+hipError_t grow_bc_host(jfem_ctx *ctx, size_t bytes) {
+    if (ctx->bc_host_bytes >= bytes) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return e;
+    if (ctx->bc_host) (void)hipHostFree(ctx->bc_host);
+    ctx->bc_host = nullptr;
+    ctx->bc_host_bytes = 0;
+    e = hipHostMalloc(&ctx->bc_host, bytes, hipHostMallocDefault);
+    if (e == hipSuccess) ctx->bc_host_bytes = bytes;
+    return e;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+// SPC + AUTOSPC (zero / negative diagonal) + grid-point singularity, with the
+// decision rules of src/solver/boundary_conditions.jl:1480-1607.  The device
+// applies the diagonal rule to every DOF and screens every grid point; the
+// eigen test (and closest-DOF claim) runs on the host for the flagged grid
+// points only, from their 6 x 6 blocks.
 int jfem_bc_partition(jfem_ctx *ctx, const jfem_matrix *K, const uint8_t *spc, double eps_zero, uint8_t *fixed,
                       int64_t *n_free) {
     const jfem_pattern *P = K->P;
     const int64_t nn = P->n_nodes, ndof = P->ndof;
-    std::vector<double> blk(36 * nn);
-    double *d_blk = nullptr;
-    JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_blk, std::max<int64_t>(36 * nn, 1) * sizeof(double)));
-    JFEM_CHECK_HIP(ctx, launch_gather_nodeblk(ctx->stream, P, K->d_val, d_blk));
-    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(blk.data(), d_blk, 36 * nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (n_free) *n_free = 0;
+    if (ndof == 0) return JFEM_OK;
+    const size_t o_spc = 0, o_fix = align256(ndof), o_flag = o_fix + align256(ndof), o_nodes = o_flag + align256(nn);
+    const size_t o_blk = o_nodes + align256(4 * nn);
+    JFEM_CHECK_HIP(ctx, grow_bc_dev(ctx, o_blk));
+    JFEM_CHECK_HIP(ctx, grow_bc_host(ctx, align256(ndof) + align256(nn)));
+    char *dv = (char *)ctx->bc_dev;
+    uint8_t *hfix = (uint8_t *)ctx->bc_host, *hflag = hfix + align256(ndof);
+    if (spc) JFEM_CHECK_HIP(ctx, hipMemcpyAsync(dv + o_spc, spc, ndof, hipMemcpyHostToDevice, ctx->stream));
+    else JFEM_CHECK_HIP(ctx, hipMemsetAsync(dv + o_spc, 0, ndof, ctx->stream));
+    const double thresh = 1e3 * eps_zero;
+    JFEM_CHECK_HIP(ctx, launch_bc_screen(ctx->stream, P, K->d_val, (const uint8_t *)(dv + o_spc), thresh,
+                                         (uint8_t *)(dv + o_fix), (uint8_t *)(dv + o_flag)));
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(hfix, dv + o_fix, ndof, hipMemcpyDeviceToHost, ctx->stream));
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(hflag, dv + o_flag, nn, hipMemcpyDeviceToHost, ctx->stream));
     JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    (void)hipFree(d_blk);
-    for (int64_t i = 0; i < ndof; ++i) fixed[i] = spc ? (spc[i] ? 1 : 0) : 0;
-    auto kv = [&](int64_t node, int a, int b) { return blk[36 * node + 6 * a + b]; };
-    for (int64_t i = 0; i < ndof; ++i) {
-        double d = kv(i / 6, (int)(i % 6), (int)(i % 6));
-        if (!fixed[i] && (d == 0.0 || d < 0)) fixed[i] = 1;
+    std::memcpy(fixed, hfix, ndof);
+    std::vector<int32_t> nodes;
+    for (int64_t n = 0; n < nn; ++n)
+        if (hflag[n]) nodes.push_back((int32_t)n);
+    const int64_t nf = (int64_t)nodes.size();
+    std::vector<double> blk(36 * (size_t)nf);
+    if (nf > 0) {
+        JFEM_CHECK_HIP(ctx, grow_bc_dev(ctx, o_blk + 36 * 8 * (size_t)nf));
+        dv = (char *)ctx->bc_dev;
+        JFEM_CHECK_HIP(ctx, hipMemcpyAsync(dv + o_nodes, nodes.data(), 4 * nf, hipMemcpyHostToDevice, ctx->stream));
+        JFEM_CHECK_HIP(ctx, launch_gather_nodes(ctx->stream, P, K->d_val, (const int32_t *)(dv + o_nodes), nf,
+                                                (double *)(dv + o_blk)));
+        JFEM_CHECK_HIP(ctx, hipMemcpyAsync(blk.data(), dv + o_blk, 36 * 8 * nf, hipMemcpyDeviceToHost, ctx->stream));
+        JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
-    for (int64_t n = 0; n < nn; ++n) {
+    for (int64_t q = 0; q < nf; ++q) {
+        const int64_t n = nodes[q];
+        auto kv = [&](int a, int b) { return blk[36 * q + 6 * a + b]; };
         for (int off = 0; off <= 3; off += 3) {
             int fl[3], m = 0;
             for (int a = 0; a < 3; ++a)
@@ -500,7 +561,7 @@ int jfem_bc_partition(jfem_ctx *ctx, const jfem_matrix *K, const uint8_t *spc, d
             double S[9] = {0}, vals[3] = {0}, vecs[9] = {0};
             for (int ii = 0; ii < m; ++ii)
                 for (int jj = 0; jj < m; ++jj)
-                    S[ii + 3 * jj] = (kv(n, off + fl[ii], off + fl[jj]) + kv(n, off + fl[jj], off + fl[ii])) / 2;
+                    S[ii + 3 * jj] = (kv(off + fl[ii], off + fl[jj]) + kv(off + fl[jj], off + fl[ii])) / 2;
             if (m == 3) {
                 eig3(S, vals, vecs);
             } else if (m == 2) {
@@ -540,9 +601,9 @@ int jfem_bc_partition(jfem_ctx *ctx, const jfem_matrix *K, const uint8_t *spc, d
             }
         }
     }
-    int64_t nf = 0;
-    for (int64_t i = 0; i < ndof; ++i) nf += fixed[i] ? 0 : 1;
-    if (n_free) *n_free = nf;
+    int64_t cnt = 0;
+    for (int64_t i = 0; i < ndof; ++i) cnt += fixed[i] ? 0 : 1;
+    if (n_free) *n_free = cnt;
     return JFEM_OK;
 }
 

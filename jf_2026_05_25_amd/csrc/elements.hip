@@ -1207,6 +1207,62 @@ __global__ void gather_nodeblk_kernel(int64_t n, const int64_t *__restrict__ pos
     out[i] = p >= 0 ? val[p] : 0.0;
 }
 
+// Per grid point: SPC + AUTOSPC of zero / negative diagonal entries (fixed0), and
+// a flag set when a translational or rotational group of still-free DOFs could
+// hold an eigenvalue below eps_zero * lambda_max.  For a symmetric positive
+// definite m x m block lambda_min / lambda_max >= det / trace^m (lambda_max <=
+// trace), so a group with det >= thresh * trace^m (thresh >> eps_zero) can make
+// no claim and is not flagged; indefinite or NaN groups are flagged.
+__global__ void bc_screen_kernel(int64_t nn, const int64_t *__restrict__ pos, const double *__restrict__ val,
+                                 const uint8_t *__restrict__ spc, double thresh, uint8_t *__restrict__ fixed0,
+                                 uint8_t *__restrict__ flag) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    double kv[36];
+#pragma unroll
+    for (int q = 0; q < 36; ++q) {
+        const int64_t p = pos[36 * n + q];
+        kv[q] = p >= 0 ? val[p] : 0.0;
+    }
+    uint8_t fx[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        const double d = kv[6 * a + a];
+        fx[a] = spc[6 * n + a] ? 1 : ((d == 0.0 || d < 0) ? 1 : 0);
+        fixed0[6 * n + a] = fx[a];
+    }
+    uint8_t fl = 0;
+#pragma unroll
+    for (int off = 0; off <= 3; off += 3) {
+        int idx[3], m = 0;
+        for (int a = 0; a < 3; ++a)
+            if (!fx[off + a]) idx[m++] = off + a;
+        if (m < 2) continue;
+        auto S = [&](int i, int j) { return 0.5 * (kv[6 * idx[i] + idx[j]] + kv[6 * idx[j] + idx[i]]); };
+        double det, tr;
+        if (m == 2) {
+            det = S(0, 0) * S(1, 1) - S(0, 1) * S(0, 1);
+            tr = S(0, 0) + S(1, 1);
+            if (!(det >= thresh * tr * tr)) fl = 1;
+        } else {
+            det = S(0, 0) * (S(1, 1) * S(2, 2) - S(1, 2) * S(1, 2)) - S(0, 1) * (S(0, 1) * S(2, 2) - S(1, 2) * S(0, 2)) +
+                  S(0, 2) * (S(0, 1) * S(1, 2) - S(1, 1) * S(0, 2));
+            tr = S(0, 0) + S(1, 1) + S(2, 2);
+            if (!(det >= thresh * tr * tr * tr)) fl = 1;
+        }
+    }
+    flag[n] = fl;
+}
+
+// out[36 * q + e] = 6 x 6 diagonal block entries of grid point nodes[q]
+__global__ void gather_nodes_kernel(int64_t count, const int64_t *__restrict__ pos, const double *__restrict__ val,
+                                    const int32_t *__restrict__ nodes, double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count * 36) return;
+    const int64_t p = pos[36 * (int64_t)nodes[i / 36] + i % 36];
+    out[i] = p >= 0 ? val[p] : 0.0;
+}
+
 __global__ void reset_status_kernel(int64_t *status) {
     status[0] = INT64_MAX;
     status[1] = 0;
@@ -1260,6 +1316,25 @@ hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *K
     const int bs = 256;
     dim3 grid((unsigned)((P->nnz + bs - 1) / bs));
     hipLaunchKernelGGL(assemble_kernel, grid, dim3(bs), 0, s, P->nnz, P->d_seg_ptr, P->d_seg_src, Ke, val);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_screen(hipStream_t s, const jfem_pattern *P, const double *val, const uint8_t *spc, double thresh,
+                            uint8_t *fixed0, uint8_t *flag) {
+    const int64_t nn = P->n_nodes;
+    if (nn == 0) return hipSuccess;
+    const int bs = 128;
+    hipLaunchKernelGGL(bc_screen_kernel, dim3((unsigned)((nn + bs - 1) / bs)), dim3(bs), 0, s, nn, P->d_nodeblk, val, spc,
+                       thresh, fixed0, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_nodes(hipStream_t s, const jfem_pattern *P, const double *val, const int32_t *nodes,
+                               int64_t count, double *out) {
+    if (count == 0) return hipSuccess;
+    const int bs = 256;
+    hipLaunchKernelGGL(gather_nodes_kernel, dim3((unsigned)((count * 36 + bs - 1) / bs)), dim3(bs), 0, s, count,
+                       P->d_nodeblk, val, nodes, out);
     return hipGetLastError();
 }
 

@@ -29,7 +29,9 @@ struct jfem_symbolic {
             *d_level_sn = nullptr;
     int32_t *d_amap_src = nullptr;
     int64_t *d_amap_dst = nullptr;
-    PanelTask *d_ptasks = nullptr, *d_btasks = nullptr, *d_bwd = nullptr;
+    PanelTask *d_ptasks = nullptr, *d_btasks = nullptr, *d_bwd = nullptr, *d_bred = nullptr, *d_gtasks = nullptr;
+    int32_t *d_g_bounds = nullptr;
+    int64_t *d_g_boff = nullptr;
     UpdateTask *d_utasks = nullptr;
     EaTask *d_etasks = nullptr;
     int32_t *d_ea_bounds = nullptr;
@@ -407,32 +409,58 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
 // them) and then applies its own tile, so a step is one launch.  Wave w of a
 // 256-thread workgroup handles right-hand sides r = w, w+4, w+8, w+12.
 
-// per supernode of the level: W_s = 0, then add the children's update vectors
-// (in child order) into Y_s (pivot rows) and W_s (update rows)
+// forward gather task (s, 0, r0): front rows [r0, r0+kGatherRows) of supernode s:
+// update rows there start at zero, then the children's update vectors entries
+// landing there (child update rows [i0, i1), precomputed) are added in child
+// order -- pivot rows into Y_s, update rows into W_s.
 template <class T>
-__global__ void __launch_bounds__(kBS) fwd_gather_kernel(const int32_t *__restrict__ level_sn, const int64_t *__restrict__ col0,
+__global__ void __launch_bounds__(kBS) fwd_gather_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ boff,
+                                                         const int32_t *__restrict__ bounds, const int64_t *__restrict__ col0,
                                                          const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
                                                          const int64_t *__restrict__ relofs, const int32_t *__restrict__ relmap,
                                                          const int32_t *__restrict__ child_ptr, const int32_t *__restrict__ child,
                                                          T *__restrict__ Y, T *__restrict__ W, int nrhs) {
-    const int s = level_sn[blockIdx.x];
+    constexpr int kU = 4;
+    const PanelTask t = tasks[blockIdx.x];
+    const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s];
+    const int64_t rhi = min<int64_t>((int64_t)t.r0 + kGatherRows, ms), z0 = max<int64_t>(t.r0, ns);
     T *Ws = W + relofs[s] * nrhs;
     T *Ys = Y + col0[s] * nrhs;
     const int tid = threadIdx.x;
-    for (int64_t q = tid; q < (ms - ns) * nrhs; q += blockDim.x) Ws[q] = T(0);
+    for (int64_t q = tid; q < (rhi - z0) * nrhs; q += blockDim.x) Ws[(z0 - ns) * nrhs + q] = T(0);
     __syncthreads();
-    for (int qc = child_ptr[s]; qc < child_ptr[s + 1]; ++qc) {
+    const int32_t *gb = bounds + 2 * boff[blockIdx.x];
+    for (int qc = child_ptr[s]; qc < child_ptr[s + 1]; ++qc, gb += 2) {
         const int c = child[qc];
-        const int64_t nu = mrow[c] - ncol[c];
         const int32_t *rel = relmap + relofs[c];
         const T *Wc = W + relofs[c] * nrhs;
-        for (int64_t q = tid; q < nu * nrhs; q += blockDim.x) {
-            const int64_t i = q / nrhs;
-            const int r = (int)(q % nrhs);
-            const int64_t pr = rel[i];
-            if (pr < ns) Ys[pr * nrhs + r] += Wc[q];
-            else Ws[(pr - ns) * nrhs + r] += Wc[q];
+        const int64_t i0 = gb[0];
+        const int64_t cnt = ((int64_t)gb[1] - i0) * nrhs;
+        for (int64_t base = tid; base < cnt; base += kU * kBS) {
+            int64_t pr[kU], src[kU];
+            int rr[kU];
+            bool ok[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int64_t q = base + (int64_t)u * kBS;
+                ok[u] = q < cnt;
+                const int64_t qq = ok[u] ? q : cnt - 1;
+                const int64_t i = i0 + qq / nrhs;
+                rr[u] = (int)(qq % nrhs);
+                src[u] = i * nrhs + rr[u];
+                pr[u] = rel[i];
+            }
+            T v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) v[u] = Wc[src[u]];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (ok[u]) {
+                    if (pr[u] < ns) Ys[pr[u] * nrhs + rr[u]] += v[u];
+                    else Ws[(pr[u] - ns) * nrhs + rr[u]] += v[u];
+                }
+            }
         }
         __syncthreads();
     }
@@ -514,21 +542,25 @@ __global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restri
     }
 }
 
-// backward, per column tile (s, 0, c0) of a supernode with update rows:
-// Z[cols] -= L[ns..ms, cols]' X[rows] (X at the ancestors' rows is final).
-// Waves split the rows of each 64-row chunk; partial sums are added in a fixed order.
+// backward off-diagonal task (s, c0, i0, slot): sum over update rows
+// [i0, i0+kBwdRows) of L[rows, c0..c0+kTile)' X[rows] (X at the ancestors' rows is
+// final); slot < 0: subtract from Z directly, else store the partial sums in slot
+// `slot` of P for bwd_reduce_kernel.  Waves split each 64-row chunk's rows;
+// partial sums are added in a fixed order.
 template <class T, int NR>
 __global__ void __launch_bounds__(kBS) bwd_offdiag_kernel(const PanelTask *__restrict__ tiles, const int64_t *__restrict__ col0,
                                                           const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
                                                           const int64_t *__restrict__ lofs, const int64_t *__restrict__ rofs,
                                                           const int32_t *__restrict__ rows, const T *__restrict__ L,
-                                                          const T *__restrict__ X, T *__restrict__ Z, int nrhs) {
+                                                          const T *__restrict__ X, T *__restrict__ Z, T *__restrict__ Pp,
+                                                          int nrhs) {
     __shared__ T Lt[kTile][kTile + 1];
     __shared__ T xs[kTile][NR + 1];
     __shared__ T red[4][kTile][NR + 1];
     const PanelTask t = tiles[blockIdx.x];
     const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s], ct = t.r0;
+    const int64_t ns = ncol[s], ms = mrow[s], ct = t.k;
+    const int64_t iend = min<int64_t>((int64_t)t.r0 + kBwdRows, ms);
     const int nc = (int)min<int64_t>(kTile, ns - ct);
     const T *Ls = L + lofs[s];
     const int32_t *rw = rows + rofs[s];
@@ -536,8 +568,8 @@ __global__ void __launch_bounds__(kBS) bwd_offdiag_kernel(const PanelTask *__res
     T acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
-    for (int64_t i0 = ns; i0 < ms; i0 += kTile) {
-        const int nr = (int)min<int64_t>(kTile, ms - i0);
+    for (int64_t i0 = t.r0; i0 < iend; i0 += kTile) {
+        const int nr = (int)min<int64_t>(kTile, iend - i0);
         __syncthreads();
 #pragma unroll
         for (int p = 0; p < kTile * kTile / kBS; ++p) {
@@ -567,10 +599,34 @@ __global__ void __launch_bounds__(kBS) bwd_offdiag_kernel(const PanelTask *__res
     for (int r = 0; r < NR; ++r) red[w][lane][r] = acc[r];
     __syncthreads();
     if (w == 0 && lane < nc) {
-        T *dst = Z + (col0[s] + ct + lane) * nrhs;
+        if (t.pad < 0) {
+            T *dst = Z + (col0[s] + ct + lane) * nrhs;
 #pragma unroll
-        for (int r = 0; r < NR; ++r)
-            if (r < nrhs) dst[r] -= ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
+            for (int r = 0; r < NR; ++r)
+                if (r < nrhs) dst[r] -= ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
+        } else {
+            T *dst = Pp + ((int64_t)t.pad * kTile + lane) * nrhs;
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (r < nrhs) dst[r] = ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
+        }
+    }
+}
+
+// reduce task (s, c0, slot0, count): Z[c0..c0+kTile) -= sum of the partial slots
+// slot0 .. slot0+count-1, in slot order
+template <class T>
+__global__ void __launch_bounds__(kBS) bwd_reduce_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ col0,
+                                                         const int64_t *__restrict__ ncol, const T *__restrict__ Pp,
+                                                         T *__restrict__ Z, int nrhs) {
+    const PanelTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int nc = (int)min<int64_t>(kTile, ncol[s] - t.k);
+    for (int q = threadIdx.x; q < nc * nrhs; q += blockDim.x) {
+        const int j = q / nrhs, r = q % nrhs;
+        T v = T(0);
+        for (int k = 0; k < t.pad; ++k) v += Pp[((int64_t)(t.r0 + k) * kTile + j) * nrhs + r];
+        Z[(col0[s] + t.k + j) * nrhs + r] -= v;
     }
 }
 
@@ -1014,14 +1070,16 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
 }
 
 template <class T, int NQ>
-hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, int nrhs) {
+hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, T *Pp, int nrhs) {
     hipStream_t st = ctx->stream;
     const SymbolicHost &H = S->H;
     const int64_t nlev = (int64_t)H.level_ptr.size() - 1;
     for (int64_t l = 0; l < nlev; ++l) {
-        const int64_t nsn = H.level_ptr[l + 1] - H.level_ptr[l];
-        hipLaunchKernelGGL(fwd_gather_kernel<T>, dim3((unsigned)nsn), dim3(kBS), 0, st, S->d_level_sn + H.level_ptr[l],
-                           S->d_col0, S->d_ncol, S->d_m, S->d_relofs, S->d_relmap, S->d_child_ptr, S->d_child, Y, W, nrhs);
+        const int64_t ng = H.g_off[l + 1] - H.g_off[l];
+        if (ng)
+            hipLaunchKernelGGL(fwd_gather_kernel<T>, dim3((unsigned)ng), dim3(kBS), 0, st, S->d_gtasks + H.g_off[l],
+                               S->d_g_boff + H.g_off[l], S->d_g_bounds, S->d_col0, S->d_ncol, S->d_m, S->d_relofs,
+                               S->d_relmap, S->d_child_ptr, S->d_child, Y, W, nrhs);
         for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
             const int64_t nt = H.pstep_off[stp + 1] - H.pstep_off[stp];
             if (nt)
@@ -1034,7 +1092,11 @@ hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *
         const int64_t nt = H.bwd_off[l + 1] - H.bwd_off[l];
         if (nt)
             hipLaunchKernelGGL((bwd_offdiag_kernel<T, 4 * NQ>), dim3((unsigned)nt), dim3(kBS), 0, st, S->d_bwd + H.bwd_off[l],
-                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_rofs, S->d_rows, L, (const T *)X, Z, nrhs);
+                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_rofs, S->d_rows, L, (const T *)X, Z, Pp, nrhs);
+        const int64_t nr = H.bred_off[l + 1] - H.bred_off[l];
+        if (nr)
+            hipLaunchKernelGGL(bwd_reduce_kernel<T>, dim3((unsigned)nr), dim3(kBS), 0, st, S->d_bred + H.bred_off[l],
+                               S->d_col0, S->d_ncol, (const T *)Pp, Z, nrhs);
         for (int64_t stp = H.bstep_ptr[l]; stp < H.bstep_ptr[l + 1]; ++stp) {
             const int64_t nb = H.bstep_off[stp + 1] - H.bstep_off[stp];
             if (nb)
@@ -1047,10 +1109,10 @@ hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *
 
 // y = (L L')^{-1} y for nrhs <= 16 interleaved right-hand sides (Y in, X out)
 template <class T>
-hipError_t solve_t(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, int nrhs) {
-    if (nrhs <= 4) return solve_nq<T, 1>(ctx, S, L, Y, W, Z, X, nrhs);
-    if (nrhs <= 8) return solve_nq<T, 2>(ctx, S, L, Y, W, Z, X, nrhs);
-    return solve_nq<T, 4>(ctx, S, L, Y, W, Z, X, nrhs);
+hipError_t solve_t(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, T *Pp, int nrhs) {
+    if (nrhs <= 4) return solve_nq<T, 1>(ctx, S, L, Y, W, Z, X, Pp, nrhs);
+    if (nrhs <= 8) return solve_nq<T, 2>(ctx, S, L, Y, W, Z, X, Pp, nrhs);
+    return solve_nq<T, 4>(ctx, S, L, Y, W, Z, X, Pp, nrhs);
 }
 
 template <class V>
@@ -1117,6 +1179,10 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_etasks, H.etasks, st);
     if (e == hipSuccess) e = up(S, &S->d_btasks, H.btasks, st);
     if (e == hipSuccess) e = up(S, &S->d_ea_bounds, H.ea_bounds, st);
+    if (e == hipSuccess) e = up(S, &S->d_bred, H.bred, st);
+    if (e == hipSuccess) e = up(S, &S->d_gtasks, H.gtasks, st);
+    if (e == hipSuccess) e = up(S, &S->d_g_bounds, H.g_bounds, st);
+    if (e == hipSuccess) e = up(S, &S->d_g_boff, H.g_boff, st);
     if (e == hipSuccess) e = up(S, &S->d_ea_boff, H.ea_boff, st);
     if (e == hipSuccess) e = up(S, &S->d_bwd, H.bwd_tiles, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1270,6 +1336,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     const size_t oY = take(nv * sizeof(T)), oW = take(std::max<int64_t>(H.upd_rows, 1) * NR * sizeof(T));
     const size_t oZs = take(nv * sizeof(T)), oXs = take(nv * sizeof(T));
+    const size_t oPp = take(std::max<int64_t>(H.bslots_max, 1) * kTile * NR * sizeof(T));
     const size_t oXh = take(nv * 8), oXl = take(nv * 8), oR = take(nv * 8), oB = take(nv * 8), oD = take(nv * 8);
     const size_t oSv = take(nv * 8), oPv = take(nv * 8), oQv = take(nv * 8), oZv = take(nv * 8);
     const size_t oXgh = take(ng * 8), oXgl = take(ng * 8), oPg = take(ng * 8);
@@ -1278,6 +1345,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     JFEM_CHECK_HIP(ctx, ensure_scratch(ctx, off));
     char *base = (char *)ctx->scratch;
     T *Y = (T *)(base + oY), *W = (T *)(base + oW), *Zs = (T *)(base + oZs), *Xs = (T *)(base + oXs);
+    T *Pp = (T *)(base + oPp);
     double *Xh = (double *)(base + oXh), *Xl = (double *)(base + oXl), *R = (double *)(base + oR);
     double *Bp = (double *)(base + oB), *Dd = (double *)(base + oD), *Sv = (double *)(base + oSv);
     double *Pv = (double *)(base + oPv), *Qv = (double *)(base + oQv), *Zv = (double *)(base + oZv);
@@ -1314,7 +1382,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
         auto prec = [&](const double *sv, double *z) -> hipError_t {
             timer_start(ctx);
             hipLaunchKernelGGL(to_t_kernel<T>, gv, blk, 0, st, nv, sv, Y);
-            hipError_t e = solve_t<T>(ctx, S, (const T *)F->d_L, Y, W, Zs, Xs, NR);
+            hipError_t e = solve_t<T>(ctx, S, (const T *)F->d_L, Y, W, Zs, Xs, Pp, NR);
             hipLaunchKernelGGL(from_t_kernel<T>, gv, blk, 0, st, nv, (const T *)Xs, z);
             timer_stop(ctx, FAM_SOLVE, 1);
             return e;

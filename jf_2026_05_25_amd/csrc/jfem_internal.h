@@ -38,6 +38,11 @@ struct jfem_ctx {
     // scratch for solves
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // boundary-condition screening buffers (device + pinned host staging)
+    void *bc_dev = nullptr;
+    size_t bc_dev_bytes = 0;
+    void *bc_host = nullptr;
+    size_t bc_host_bytes = 0;
 };
 
 struct jfem_mesh {
@@ -88,6 +93,10 @@ hipError_t launch_q4_local(hipStream_t s, int64_t n, const double *lc, const dou
                            const double *k6, const double *ds, const int32_t *flags,
                            double *Ke, int64_t *status);
 hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *Ke, double *val);
+hipError_t launch_bc_screen(hipStream_t s, const jfem_pattern *P, const double *val, const uint8_t *spc, double thresh,
+                            uint8_t *fixed0, uint8_t *flag);
+hipError_t launch_gather_nodes(hipStream_t s, const jfem_pattern *P, const double *val, const int32_t *nodes,
+                               int64_t count, double *out);
 hipError_t launch_gather_nodeblk(hipStream_t s, const jfem_pattern *P, const double *val, double *out);
 hipError_t launch_reset_status(hipStream_t s, int64_t *status);
 }  // namespace jfem

@@ -8,6 +8,8 @@ namespace jfem {
 constexpr int kPanel = 64;   // panel width of the blocked partial factorization
 constexpr int kTile = 64;    // tile edge of panel/update tasks
 constexpr int kEaRows = 512; // front rows per extend-add task
+constexpr int kBwdRows = 256; // update rows per backward off-diagonal task
+constexpr int kGatherRows = 512; // front rows per forward gather task
 
 struct PanelTask { int32_t s, k, r0, pad; };          // supernode, panel column, first row
 struct UpdateTask { int32_t s, k, r0, c0; };          // supernode, panel column, tile origin
@@ -47,8 +49,18 @@ struct SymbolicHost {
     // Backward: per level the column tiles of L21' (bwd_tiles, (s, 0, c0)), then
     // steps over the diagonal blocks from the last: task (s, k, j0) solves block k
     // (j0 == k) or updates columns [j0, j0+kTile) of the pending vector (j0 < k).
-    std::vector<PanelTask> bwd_tiles, btasks;
+    // Backward off-diagonal tasks (s, c0, i0, slot): columns [c0, c0+kTile) x
+    // update rows [i0, i0+kBwdRows); slot < 0: the only row chunk of the tile,
+    // applied to Z directly; else a partial-sum slot (level-relative), summed in
+    // slot order by the reduce task (s, c0, first slot, count) of the tile.
+    // Forward gather tasks (s, 0, r0, 0): front rows [r0, r0+kGatherRows), with
+    // per (task, child) the child's update rows [i0, i1) landing there.
+    std::vector<PanelTask> bwd_tiles, btasks, bred, gtasks;
     std::vector<int64_t> bwd_off;            // per level: offset into bwd_tiles
+    std::vector<int64_t> bred_off, g_off;    // per level: offsets into bred / gtasks
+    std::vector<int32_t> g_bounds;           // 2 per (gather task, child)
+    std::vector<int64_t> g_boff;             // per gather task: first (task, child) entry
+    int64_t bslots_max = 0;                  // partial slots of the busiest level
     std::vector<int64_t> bstep_ptr;          // per level: first backward step
     std::vector<int64_t> bstep_off;          // per backward step: offset into btasks
     int64_t L_entries = 0, U_entries = 0, upd_rows = 0;

@@ -189,24 +189,27 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
         }
     };
     for (int64_t l = 0; l < nlev; ++l) {
-        ++launch;   // gather: one task per supernode of the level
-        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-            int64_t id = (launch << 32) + q;
-            int s = S.level_sn[q];
-            int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
-            for (int64_t i = 0; i < ms - ns; ++i) {
-                wr(wW, rW, S.relofs[s] + i, id);
-                for (int r = 0; r < nrhs; ++r) W[(S.relofs[s] + i) * nrhs + r] = 0;
+        ++launch;   // gather: one task per (supernode, front row range)
+        for (int64_t q = S.g_off[l]; q < S.g_off[l + 1]; ++q) {
+            const int64_t id = (launch << 32) + q;
+            const PanelTask t = S.gtasks[q];
+            const int s = t.s;
+            const int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
+            const int64_t rlo = t.r0, rhi = std::min<int64_t>(t.r0 + kGatherRows, ms);
+            for (int64_t i = std::max<int64_t>(rlo, ns); i < rhi; ++i) {
+                wr(wW, rW, S.relofs[s] + i - ns, id);
+                for (int r = 0; r < nrhs; ++r) W[(S.relofs[s] + i - ns) * nrhs + r] = 0;
             }
-            for (int32_t qc = S.child_ptr[s]; qc < S.child_ptr[s + 1]; ++qc) {
-                int c = S.child[qc];
-                int64_t nu = S.m[c] - S.ncol[c];
+            const int32_t *gb = S.g_bounds.data() + 2 * S.g_boff[q];
+            for (int32_t qc = S.child_ptr[s]; qc < S.child_ptr[s + 1]; ++qc, gb += 2) {
+                const int c = S.child[qc];
                 const int32_t *rel = S.relmap.data() + S.relofs[c];
-                for (int64_t i = 0; i < nu; ++i) {
+                for (int64_t i = gb[0]; i < gb[1]; ++i) {
                     rd(wW, rW, S.relofs[c] + i, id);
-                    int64_t pr = rel[i];
+                    const int64_t pr = rel[i];
+                    if (pr < rlo || pr >= rhi) return 8;
                     for (int r = 0; r < nrhs; ++r) {
-                        double v = W[(S.relofs[c] + i) * nrhs + r];
+                        const double v = W[(S.relofs[c] + i) * nrhs + r];
                         if (pr < ns) { wr(wY, rY, c0 + pr, id); Y[(c0 + pr) * nrhs + r] += v; }
                         else { wr(wW, rW, S.relofs[s] + pr - ns, id); W[(S.relofs[s] + pr - ns) * nrhs + r] += v; }
                     }
@@ -219,18 +222,36 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
         }
     }
     for (int64_t l = nlev - 1; l >= 0; --l) {
-        ++launch;
+        ++launch;   // off-diagonal row chunks: direct, or into partial slots
+        std::vector<double> Pb(std::max<int64_t>(S.bslots_max, 1) * kTile * nrhs, 0.0);
         for (int64_t q = S.bwd_off[l]; q < S.bwd_off[l + 1]; ++q) {
-            int64_t id = (launch << 32) + q;
-            PanelTask t = S.bwd_tiles[q];
-            int s = t.s;
-            int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
+            const int64_t id = (launch << 32) + q;
+            const PanelTask t = S.bwd_tiles[q];
+            const int s = t.s;
+            const int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s], ct = t.k;
             const double *Ls = L.data() + S.lofs[s];
             const int32_t *rw = S.rows.data() + S.rofs[s];
-            for (int64_t j = t.r0; j < std::min<int64_t>(t.r0 + kTile, ns); ++j)
+            for (int64_t j = ct; j < std::min<int64_t>(ct + kTile, ns); ++j)
                 for (int r = 0; r < nrhs; ++r) {
                     double v = 0;
-                    for (int64_t i = ns; i < ms; ++i) { rd(wX, rX, rw[i], id); v += Ls[i + ms * j] * X[(int64_t)rw[i] * nrhs + r]; }
+                    for (int64_t i = t.r0; i < std::min<int64_t>(t.r0 + kBwdRows, ms); ++i) {
+                        rd(wX, rX, rw[i], id);
+                        v += Ls[i + ms * j] * X[(int64_t)rw[i] * nrhs + r];
+                    }
+                    if (t.pad < 0) { wr(wZ, rZ, c0 + j, id); Z[(c0 + j) * nrhs + r] -= v; }
+                    else Pb[((int64_t)t.pad * kTile + (j - ct)) * nrhs + r] = v;
+                }
+        }
+        ++launch;   // reduce the partial slots of multi-chunk tiles, in slot order
+        for (int64_t q = S.bred_off[l]; q < S.bred_off[l + 1]; ++q) {
+            const int64_t id = (launch << 32) + q;
+            const PanelTask t = S.bred[q];
+            const int s = t.s;
+            const int64_t ns = S.ncol[s], c0 = S.col0[s], ct = t.k;
+            for (int64_t j = ct; j < std::min<int64_t>(ct + kTile, ns); ++j)
+                for (int r = 0; r < nrhs; ++r) {
+                    double v = 0;
+                    for (int qq = 0; qq < t.pad; ++qq) v += Pb[((int64_t)(t.r0 + qq) * kTile + (j - ct)) * nrhs + r];
                     wr(wZ, rZ, c0 + j, id);
                     Z[(c0 + j) * nrhs + r] -= v;
                 }
