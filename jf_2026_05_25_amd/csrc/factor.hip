@@ -30,6 +30,7 @@ struct jfem_symbolic {
     int32_t *d_amap_src = nullptr;
     int64_t *d_amap_dst = nullptr;
     PanelTask *d_ptasks = nullptr, *d_btasks = nullptr, *d_bwd = nullptr, *d_bred = nullptr, *d_gtasks = nullptr;
+    PanelTask *d_ftiles = nullptr;
     int32_t *d_g_bounds = nullptr;
     int64_t *d_g_boff = nullptr;
     UpdateTask *d_utasks = nullptr;
@@ -161,26 +162,26 @@ __device__ inline double bcast<double>(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 
-template <class T>
-__device__ __forceinline__ void wave_potrf(T (&r)[kPanel], int lane, int b, int64_t col_base, int64_t *status) {
-    int bad = kPanel;                         // first non-positive pivot (uniform)
+template <class T, int BW>
+__device__ __forceinline__ void wave_potrf(T (&r)[BW], int lane, int b, int64_t col_base, int64_t *status) {
+    int bad = BW;                         // first non-positive pivot (uniform)
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
+    for (int j = 0; j < BW; ++j) {
         const T d0 = bcast(r[j], j);
         const bool ok = d0 > T(0);
-        bad = (!ok && j < b && bad == kPanel) ? j : bad;
+        bad = (!ok && j < b && bad == BW) ? j : bad;
         const T sq = tsqrt(ok ? d0 : T(1));
         const T inv = T(1) / sq;
         r[j] = lane == j ? sq : (lane > j ? r[j] * inv : r[j]);
 #pragma unroll
-        for (int c = j + 1; c < kPanel; ++c) r[c] -= r[j] * bcast(r[j], c);
+        for (int c = j + 1; c < BW; ++c) r[c] -= r[j] * bcast(r[j], c);
     }
-    if (bad < kPanel && lane == 0) report_min(status, col_base + bad, JFEM_ERR_NOT_SPD);
+    if (bad < BW && lane == 0) report_min(status, col_base + bad, JFEM_ERR_NOT_SPD);
 }
 
 // factor the diagonal blocks of the first panels of a level's supernodes (one
 // wavefront per task; tasks are the level's step-0 diagonal panel tasks)
-template <class T>
+template <class T, int BW>
 __global__ void __launch_bounds__(64) diag_factor_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                          const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
                                                          const int64_t *__restrict__ col0, T *__restrict__ L,
@@ -192,16 +193,16 @@ __global__ void __launch_bounds__(64) diag_factor_kernel(const PanelTask *__rest
     T *Ls = L + lofs[s];
     const int lane = threadIdx.x;
     const int li = min(lane, b - 1);
-    T r[kPanel];
+    T r[BW];
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
+    for (int j = 0; j < BW; ++j) {
         const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
         r[j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
     }
-    wave_potrf(r, lane, b, col0[s] + k, status);
+    wave_potrf<T, BW>(r, lane, b, col0[s] + k, status);
     if (lane < b) {
 #pragma unroll
-        for (int j = 0; j < kPanel; ++j)
+        for (int j = 0; j < BW; ++j)
             if (j <= lane) Ls[(k + lane) + ms * (k + j)] = r[j];
     }
 }
@@ -210,7 +211,7 @@ __global__ void __launch_bounds__(64) diag_factor_kernel(const PanelTask *__rest
 // get X <- X L_kk^{-T} (L_kk already factored).  One wavefront per task, lane =
 // row: lane i holds its tile row x and row i of L_kk; L_kk entries are taken by
 // readlane, so the solve touches no memory.
-template <class T>
+template <class T, int BW>
 __global__ void __launch_bounds__(64) tile_solve_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                         const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
                                                         T *__restrict__ L) {
@@ -225,23 +226,23 @@ __global__ void __launch_bounds__(64) tile_solve_kernel(const PanelTask *__restr
     const int li = min(lane, b - 1);
     const int64_t row = t.r0 + lane;
     const int64_t rowc = min<int64_t>(max<int64_t>(row, rs), re - 1);
-    T d[kPanel], x[kPanel];
+    T d[BW], x[BW];
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
+    for (int j = 0; j < BW; ++j) {
         const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
         d[j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
         const T w = Ls[rowc + ms * (k + min(j, b - 1))];
         x[j] = j < b ? w : T(0);
     }
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
+    for (int j = 0; j < BW; ++j) {
         x[j] = x[j] / bcast(d[j], j);
 #pragma unroll
-        for (int c = j + 1; c < kPanel; ++c) x[c] -= x[j] * bcast(d[j], c);
+        for (int c = j + 1; c < BW; ++c) x[c] -= x[j] * bcast(d[j], c);
     }
     if (row >= rs && row < re) {
 #pragma unroll
-        for (int j = 0; j < kPanel; ++j)
+        for (int j = 0; j < BW; ++j)
             if (j < b) Ls[row + ms * (k + j)] = x[j];
     }
 }
@@ -392,7 +393,7 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
     T r[kPanel];
 #pragma unroll
     for (int j = 0; j < kPanel; ++j) r[j] = (j <= lane && lane < bn) ? A[j][lane] : (j == lane ? T(1) : T(0));
-    wave_potrf(r, lane, bn, col0[s] + c0, status);
+    wave_potrf<T, kPanel>(r, lane, bn, col0[s] + c0, status);
     if (lane < bn) {
 #pragma unroll
         for (int j = 0; j < kPanel; ++j)
@@ -475,7 +476,7 @@ __global__ void __launch_bounds__(kBS) fwd_gather_kernel(const PanelTask *__rest
 // forward step task (s, k, r0): y_k = L_kk^{-1} (pending rows k..k+b), written to Z
 // by the diagonal task (r0 == k); rows [max(r0, k+b), r0+kTile) of the front get
 // -= L[rows, k..k+b) y_k (pivot rows into Y, update rows into W).
-template <class T, int NQ>
+template <class T, int NQ, int BW>
 __global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ col0,
                                                        const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
                                                        const int64_t *__restrict__ lofs, const int64_t *__restrict__ relofs,
@@ -489,9 +490,9 @@ __global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restri
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (w * NQ >= nrhs) return;               // whole wave idle (uniform)
     const int li = min(lane, b - 1);
-    T lr[kPanel];                             // row `lane` of L_kk (identity row beyond b)
+    T lr[BW];                             // row `lane` of L_kk (identity row beyond b)
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
+    for (int j = 0; j < BW; ++j) {
         const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
         lr[j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
     }
@@ -503,7 +504,7 @@ __global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restri
         y[q] = (lane < b && r < nrhs) ? v : T(0);
     }
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
+    for (int j = 0; j < BW; ++j) {
         const T inv = T(1) / bcast(lr[j], j);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
@@ -527,7 +528,7 @@ __global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restri
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = T(0);
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
+    for (int j = 0; j < BW; ++j) {
         const T l = Ls[rowc + ms * (k + min(j, b - 1))];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) acc[q] += l * bcast(y[q], j);
@@ -632,13 +633,13 @@ __global__ void __launch_bounds__(kBS) bwd_reduce_kernel(const PanelTask *__rest
 
 // backward step task (s, k, j0): x_k = L_kk^{-T} (pending rows k..k+b of Z), written
 // to X by the diagonal task (j0 == k); for j0 < k, Z[j0..j0+kTile) -= L[k..k+b, j0..]' x_k.
-template <class T, int NQ>
+template <class T, int NQ, int BW>
 __global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ col0,
                                                        const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
                                                        const int64_t *__restrict__ lofs, const T *__restrict__ L,
                                                        T *__restrict__ Z, T *__restrict__ X, int nrhs) {
-    __shared__ T D[kPanel][kPanel + 1];      // D[i][c] = L[k+i][k+c] (identity beyond b)
-    __shared__ T E[kPanel][kTile + 1];       // E[i][c] = L[k+i][j0+c] (zero rows beyond b)
+    __shared__ T D[BW][BW + 1];              // D[i][c] = L[k+i][k+c] (identity beyond b)
+    __shared__ T E[BW][kTile + 1];           // E[i][c] = L[k+i][j0+c] (zero rows beyond b)
     const PanelTask t = tasks[blockIdx.x];
     const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s], k = t.k, c0 = col0[s], j0 = t.r0;
@@ -647,20 +648,26 @@ __global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restri
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const bool diag = j0 == k;
 #pragma unroll
-    for (int p = 0; p < kPanel * kPanel / kBS; ++p) {
+    for (int p = 0; p < (BW * BW + kBS - 1) / kBS; ++p) {
         const int q = tid + kBS * p;
-        const int i = q % kPanel, c = q / kPanel;
-        const int ic = min(i, b - 1);
-        const T v = Ls[(k + ic) + ms * (k + min(c, b - 1))];
-        D[i][c] = (i < b && c <= i) ? v : (i == c ? T(1) : T(0));
-        const T e = Ls[(k + ic) + ms * (diag ? k + min(c, b - 1) : j0 + c)];
+        const int i = q % BW, c = q / BW;
+        if (c < BW) {
+            const T v = Ls[(k + min(i, b - 1)) + ms * (k + min(c, b - 1))];
+            D[i][c] = (i < b && c <= i) ? v : (i == c ? T(1) : T(0));
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < BW * kTile / kBS; ++p) {
+        const int q = tid + kBS * p;
+        const int i = q % BW, c = q / BW;
+        const T e = Ls[(k + min(i, b - 1)) + ms * (diag ? k + min(c, b - 1) : j0 + c)];
         E[i][c] = i < b ? e : T(0);
     }
     __syncthreads();
     if (w * NQ >= nrhs) return;
-    T lc[kPanel];                             // column `lane` of L_kk
+    T lc[BW];                                 // column `lane` of L_kk
 #pragma unroll
-    for (int i = 0; i < kPanel; ++i) lc[i] = D[i][lane];
+    for (int i = 0; i < BW; ++i) lc[i] = lane < BW ? D[i][min(lane, BW - 1)] : T(0);
     T z[NQ];
     const int li = min(lane, b - 1);
 #pragma unroll
@@ -670,7 +677,7 @@ __global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restri
         z[q] = (r < nrhs && lane < b) ? v : T(0);
     }
 #pragma unroll
-    for (int j = kPanel - 1; j >= 0; --j) {
+    for (int j = BW - 1; j >= 0; --j) {
         const T inv = T(1) / bcast(lc[j], j);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
@@ -693,7 +700,7 @@ __global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restri
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = T(0);
 #pragma unroll
-    for (int i = 0; i < kPanel; ++i) {
+    for (int i = 0; i < BW; ++i) {
         const T e = E[i][lane];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) acc[q] += e * bcast(z[q], i);
@@ -702,6 +709,275 @@ __global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restri
     for (int q = 0; q < NQ; ++q) {
         const int r = w * NQ + q;
         if (r < nrhs) Z[(c0 + j0 + lane) * nrhs + r] -= acc[q];
+    }
+}
+
+// ---------------------------------------------------------- whole-pivot-block solves
+// One workgroup of WG threads per supernode solves its entire pivot block L11
+// (ns <= R * WG): thread t owns rows (forward) or columns (backward) t + WG * rho,
+// rho < R, whose pending values stay in registers for the whole solve.  Per
+// 64-column block the workgroup stages the diagonal block in LDS; the wavefront
+// owning the block's rows solves it (lane = row, values broadcast with readlane)
+// and publishes the result in LDS; then every thread applies it to its own rows.
+// Two workgroup barriers per block replace a kernel launch per block.
+// NR >= nrhs right-hand sides per thread (4, 8 or 16).
+
+template <class T, int NT>
+__device__ __forceinline__ void stage_diag_block(T (*Dk)[kPanel + 1], const T *Ls, int64_t ms, int64_t k, int b, int t) {
+    T v[kPanel * kPanel / NT];
+#pragma unroll
+    for (int p = 0; p < kPanel * kPanel / NT; ++p) {
+        const int q = t + NT * p;
+        const int i = q % kPanel, c = q / kPanel;
+        v[p] = Ls[(k + min(i, b - 1)) + ms * (k + min(c, b - 1))];
+    }
+#pragma unroll
+    for (int p = 0; p < kPanel * kPanel / NT; ++p) {
+        const int q = t + NT * p;
+        const int i = q % kPanel, c = q / kPanel;
+        Dk[i][c] = (i < b && c <= i) ? v[p] : (i == c ? T(1) : T(0));
+    }
+}
+
+// forward: Z[pivot rows of s] = L11^{-1} Y[pivot rows of s]
+template <class T, int NR, int R, int WG>
+__global__ void __launch_bounds__(WG) fwd_diag_kernel(const int32_t *__restrict__ level_sn, const int64_t *__restrict__ col0,
+                                                      const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
+                                                      const int64_t *__restrict__ lofs, const T *__restrict__ L,
+                                                      const T *__restrict__ Y, T *__restrict__ Z, int nrhs) {
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    __shared__ T Dk[kPanel][kPanel + 1];
+    __shared__ v4 yb[kPanel][NR / 4];         // published block, read as 16-byte broadcasts
+    const int s = level_sn[blockIdx.x];
+    const int64_t ns = ncol[s], ms = mrow[s], c0 = col0[s];
+    const T *Ls = L + lofs[s];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    T y[R][NR];
+#pragma unroll
+    for (int rho = 0; rho < R; ++rho) {
+        const int64_t i = t + (int64_t)WG * rho;
+        const int64_t ic = min<int64_t>(i, ns - 1);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const T v = Y[(c0 + ic) * nrhs + min(r, nrhs - 1)];
+            y[rho][r] = (i < ns && r < nrhs) ? v : T(0);
+        }
+    }
+    const int nb = (int)((ns + kPanel - 1) / kPanel);
+    for (int kb = 0; kb < nb; ++kb) {
+        const int64_t k = (int64_t)kb * kPanel;
+        const int b = (int)min<int64_t>(kPanel, ns - k);
+        stage_diag_block<T, WG>(Dk, Ls, ms, k, b, t);
+        __syncthreads();
+        const int ow = (int)((k % WG) / kPanel), orho = (int)(k / WG);
+        if (w == ow) {
+            T yy[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) yy[r] = T(0);
+#pragma unroll
+            for (int rho = 0; rho < R; ++rho)
+                if (rho == orho) {
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) yy[r] = y[rho][r];
+                }
+#pragma unroll
+            for (int j = 0; j < kPanel; ++j) {
+                const T lj = Dk[lane][j];
+                const T inv = T(1) / Dk[j][j];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const T yj = bcast(yy[r], j) * inv;
+                    const T nv = yy[r] - lj * yj;
+                    yy[r] = lane == j ? yj : nv;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NR / 4; ++q) yb[lane][q] = v4{yy[4 * q], yy[4 * q + 1], yy[4 * q + 2], yy[4 * q + 3]};
+            if (lane < b) {
+#pragma unroll
+                for (int r = 0; r < NR; ++r)
+                    if (r < nrhs) Z[(c0 + k + lane) * nrhs + r] = yy[r];
+            }
+        }
+        __syncthreads();
+        if (kb + 1 < nb) {                     // rows of later blocks (b == kPanel here)
+            bool act[R];
+            int64_t ic[R];
+#pragma unroll
+            for (int rho = 0; rho < R; ++rho) {
+                const int64_t i = t + (int64_t)WG * rho;
+                act[rho] = i >= k + kPanel && i < ns;
+                ic[rho] = min<int64_t>(max<int64_t>(i, k + kPanel), ns - 1);
+            }
+            constexpr int kUnroll = (R * NR >= 32) ? 1 : (R * NR >= 16 ? 4 : 8);
+#pragma unroll kUnroll
+            for (int j = 0; j < kPanel; ++j) {
+                v4 yv[NR / 4];
+#pragma unroll
+                for (int q = 0; q < NR / 4; ++q) yv[q] = yb[j][q];
+#pragma unroll
+                for (int rho = 0; rho < R; ++rho) {
+                    const T lv = Ls[ic[rho] + ms * (k + j)];
+                    const T l = act[rho] ? lv : T(0);
+#pragma unroll
+                    for (int q = 0; q < NR / 4; ++q) {
+                        y[rho][4 * q] -= l * yv[q].x;
+                        y[rho][4 * q + 1] -= l * yv[q].y;
+                        y[rho][4 * q + 2] -= l * yv[q].z;
+                        y[rho][4 * q + 3] -= l * yv[q].w;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// forward L21 row tile (s, 0, r0): W_s[rows r0..r0+kTile) -= L21[rows, :] y1, y1
+// staged from Z in 256-column chunks; the four waves split each chunk's columns
+// and their partial sums are added in a fixed order.
+template <class T, int NR>
+__global__ void __launch_bounds__(kBS) fwd_offdiag_kernel(const PanelTask *__restrict__ tiles, const int64_t *__restrict__ col0,
+                                                          const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
+                                                          const int64_t *__restrict__ lofs, const int64_t *__restrict__ relofs,
+                                                          const T *__restrict__ L, const T *__restrict__ Z, T *__restrict__ W,
+                                                          int nrhs) {
+    constexpr int kC = 256;
+    __shared__ T ys[kC][NR];
+    __shared__ T red[4][kTile][NR + 1];
+    const PanelTask t = tiles[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s], c0 = col0[s];
+    const T *Ls = L + lofs[s];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t row = t.r0 + lane;
+    const int64_t rowc = min<int64_t>(row, ms - 1);
+    T acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = T(0);
+    for (int64_t cc0 = 0; cc0 < ns; cc0 += kC) {
+        const int nc = (int)min<int64_t>(kC, ns - cc0);
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < kC * NR / kBS; ++p) {
+            const int q = tid + kBS * p;
+            const int cc = q / NR, r = q % NR;
+            const T v = Z[(c0 + cc0 + min(cc, nc - 1)) * nrhs + min(r, nrhs - 1)];
+            ys[cc][r] = (cc < nc && r < nrhs) ? v : T(0);
+        }
+        __syncthreads();
+        const int cb = w * (kC / 4);
+#pragma unroll 8
+        for (int jj = 0; jj < kC / 4; ++jj) {
+            const int cc = cb + jj;
+            const T l = Ls[rowc + ms * (cc0 + min(cc, nc - 1))];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] += l * ys[cc][r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) red[w][lane][r] = acc[r];
+    __syncthreads();
+    if (w == 0 && row < ms) {
+        T *dst = W + (relofs[s] + row - ns) * nrhs;
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (r < nrhs) dst[r] -= ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
+    }
+}
+
+// backward: X[pivot rows of s] = L11^{-T} Z[pivot rows of s] (Z already holds the
+// L21' contributions); thread t owns columns t + WG * rho
+template <class T, int NR, int R, int WG>
+__global__ void __launch_bounds__(WG) bwd_diag_kernel(const int32_t *__restrict__ level_sn, const int64_t *__restrict__ col0,
+                                                      const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
+                                                      const int64_t *__restrict__ lofs, const T *__restrict__ L,
+                                                      const T *__restrict__ Z, T *__restrict__ X, int nrhs) {
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    __shared__ T Dk[kPanel][kPanel + 1];
+    __shared__ v4 xb[kPanel][NR / 4];         // published block, read as 16-byte broadcasts
+    const int s = level_sn[blockIdx.x];
+    const int64_t ns = ncol[s], ms = mrow[s], c0 = col0[s];
+    const T *Ls = L + lofs[s];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    T z[R][NR];
+#pragma unroll
+    for (int rho = 0; rho < R; ++rho) {
+        const int64_t c = t + (int64_t)WG * rho;
+        const int64_t cc = min<int64_t>(c, ns - 1);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const T v = Z[(c0 + cc) * nrhs + min(r, nrhs - 1)];
+            z[rho][r] = (c < ns && r < nrhs) ? v : T(0);
+        }
+    }
+    const int nb = (int)((ns + kPanel - 1) / kPanel);
+    for (int kb = nb - 1; kb >= 0; --kb) {
+        const int64_t k = (int64_t)kb * kPanel;
+        const int b = (int)min<int64_t>(kPanel, ns - k);
+        stage_diag_block<T, WG>(Dk, Ls, ms, k, b, t);
+        __syncthreads();
+        const int ow = (int)((k % WG) / kPanel), orho = (int)(k / WG);
+        if (w == ow) {
+            T zz[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) zz[r] = T(0);
+#pragma unroll
+            for (int rho = 0; rho < R; ++rho)
+                if (rho == orho) {
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) zz[r] = z[rho][r];
+                }
+#pragma unroll
+            for (int j = kPanel - 1; j >= 0; --j) {
+                const T lj = Dk[j][lane];            // L[k+j][k+lane]
+                const T inv = T(1) / Dk[j][j];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const T xj = bcast(zz[r], j) * inv;
+                    const T nv = zz[r] - lj * xj;
+                    zz[r] = lane == j ? xj : nv;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NR / 4; ++q) xb[lane][q] = v4{zz[4 * q], zz[4 * q + 1], zz[4 * q + 2], zz[4 * q + 3]};
+            if (lane < b) {
+#pragma unroll
+                for (int r = 0; r < NR; ++r)
+                    if (r < nrhs) X[(c0 + k + lane) * nrhs + r] = zz[r];
+            }
+        }
+        __syncthreads();
+        if (kb > 0) {                          // columns of earlier blocks
+            bool act[R];
+            int64_t cc[R];
+#pragma unroll
+            for (int rho = 0; rho < R; ++rho) {
+                const int64_t c = t + (int64_t)WG * rho;
+                act[rho] = c < k;
+                cc[rho] = min<int64_t>(c, k - 1);
+            }
+            constexpr int kUnroll = (R * NR >= 32) ? 1 : (R * NR >= 16 ? 4 : 8);
+#pragma unroll kUnroll
+            for (int i = 0; i < kPanel; ++i) {
+                v4 xv[NR / 4];
+#pragma unroll
+                for (int q = 0; q < NR / 4; ++q) xv[q] = xb[i][q];
+#pragma unroll
+                for (int rho = 0; rho < R; ++rho) {
+                    const T lv = Ls[(k + min(i, b - 1)) + ms * cc[rho]];
+                    const T l = act[rho] ? lv : T(0);
+#pragma unroll
+                    for (int q = 0; q < NR / 4; ++q) {
+                        z[rho][4 * q] -= l * xv[q].x;
+                        z[rho][4 * q + 1] -= l * xv[q].y;
+                        z[rho][4 * q + 2] -= l * xv[q].z;
+                        z[rho][4 * q + 3] -= l * xv[q].w;
+                    }
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1045,18 +1321,32 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
             const int64_t stp = H.step_ptr[l];
             const int64_t nd = H.pstep_mid[stp] - H.pstep_off[stp];
             if (detail) (void)hipEventRecord(ctx->ev2, st);
-            if (nd)
-                hipLaunchKernelGGL(diag_factor_kernel<T>, dim3((unsigned)nd), dim3(64), 0, st, S->d_ptasks + H.pstep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status);
+            if (nd) {
+#define JFEM_DIAG(BWV)                                                                                        \
+    hipLaunchKernelGGL((diag_factor_kernel<T, BWV>), dim3((unsigned)nd), dim3(64), 0, st, S->d_ptasks + H.pstep_off[stp], \
+                       S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status)
+                const int bw = H.pstep_bw[stp];
+                if (bw <= 16) JFEM_DIAG(16);
+                else if (bw <= 32) JFEM_DIAG(32);
+                else JFEM_DIAG(64);
+#undef JFEM_DIAG
+            }
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, nd ? 1 : 0); }
         }
         for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
             const int64_t np = H.pstep_off[stp + 1] - H.pstep_off[stp];
             const int64_t nu = H.ustep_off[stp + 1] - H.ustep_off[stp];
             if (detail) (void)hipEventRecord(ctx->ev2, st);
-            if (np)
-                hipLaunchKernelGGL(tile_solve_kernel<T>, dim3((unsigned)np), dim3(64), 0, st, S->d_ptasks + H.pstep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, L);
+            if (np) {
+#define JFEM_TILE(BWV)                                                                                         \
+    hipLaunchKernelGGL((tile_solve_kernel<T, BWV>), dim3((unsigned)np), dim3(64), 0, st, S->d_ptasks + H.pstep_off[stp], \
+                       S->d_ncol, S->d_m, S->d_lofs, L)
+                const int bw = H.pstep_bw[stp];
+                if (bw <= 16) JFEM_TILE(16);
+                else if (bw <= 32) JFEM_TILE(32);
+                else JFEM_TILE(64);
+#undef JFEM_TILE
+            }
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, np ? 1 : 0); }
             if (nu) {
                 if (detail) (void)hipEventRecord(ctx->ev2, st);
@@ -1067,6 +1357,34 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
         }
     }
     return hipGetLastError();
+}
+
+// whole-pivot-block solve of one level: workgroup size / rows per thread chosen
+// from the level's widest pivot block; false if no variant fits (R * NR <= 64)
+template <class T, int NR>
+bool level_diag_solve(bool forward, hipStream_t st, const jfem_symbolic *S, int64_t l, const T *L, const T *in, T *out,
+                      int nrhs) {
+    const SymbolicHost &H = S->H;
+    const int64_t maxns = H.level_maxns[l];
+    const int64_t nsn = H.level_ptr[l + 1] - H.level_ptr[l];
+    const int32_t *lsn = S->d_level_sn + H.level_ptr[l];
+#define JFEM_DIAGSOLVE(RV, WGV)                                                                                   \
+    do {                                                                                                          \
+        if (forward)                                                                                              \
+            hipLaunchKernelGGL((fwd_diag_kernel<T, NR, RV, WGV>), dim3((unsigned)nsn), dim3(WGV), 0, st, lsn,     \
+                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, in, out, nrhs);                        \
+        else                                                                                                      \
+            hipLaunchKernelGGL((bwd_diag_kernel<T, NR, RV, WGV>), dim3((unsigned)nsn), dim3(WGV), 0, st, lsn,     \
+                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, in, out, nrhs);                        \
+        return true;                                                                                              \
+    } while (0)
+    // Only levels whose pivot blocks fit one 64-column block: there a single
+    // wavefront per supernode needs no in-kernel update phase.  Wider blocks go
+    // through the step launches, whose update tiles spread over the whole GPU
+    // (one workgroup cannot keep enough loads in flight to stream a wide block).
+    if (maxns <= 64) JFEM_DIAGSOLVE(1, 64);
+#undef JFEM_DIAGSOLVE
+    return false;
 }
 
 template <class T, int NQ>
@@ -1080,12 +1398,26 @@ hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *
             hipLaunchKernelGGL(fwd_gather_kernel<T>, dim3((unsigned)ng), dim3(kBS), 0, st, S->d_gtasks + H.g_off[l],
                                S->d_g_boff + H.g_off[l], S->d_g_bounds, S->d_col0, S->d_ncol, S->d_m, S->d_relofs,
                                S->d_relmap, S->d_child_ptr, S->d_child, Y, W, nrhs);
+        if (level_diag_solve<T, 4 * NQ>(true, st, S, l, L, Y, Z, nrhs)) {
+            const int64_t nf = H.ftile_off[l + 1] - H.ftile_off[l];
+            if (nf)
+                hipLaunchKernelGGL((fwd_offdiag_kernel<T, 4 * NQ>), dim3((unsigned)nf), dim3(kBS), 0, st,
+                                   S->d_ftiles + H.ftile_off[l], S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L,
+                                   (const T *)Z, W, nrhs);
+            continue;
+        }
         for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
             const int64_t nt = H.pstep_off[stp + 1] - H.pstep_off[stp];
-            if (nt)
-                hipLaunchKernelGGL((fwd_step_kernel<T, NQ>), dim3((unsigned)nt), dim3(kBS), 0, st,
-                                   S->d_ptasks + H.pstep_off[stp], S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L, Y,
-                                   W, Z, nrhs);
+            if (nt) {
+#define JFEM_FWD(BWV)                                                                                           \
+    hipLaunchKernelGGL((fwd_step_kernel<T, NQ, BWV>), dim3((unsigned)nt), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp], \
+                       S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L, Y, W, Z, nrhs)
+                const int bw = H.pstep_bw[stp];
+                if (bw <= 16) JFEM_FWD(16);
+                else if (bw <= 32) JFEM_FWD(32);
+                else JFEM_FWD(64);
+#undef JFEM_FWD
+            }
         }
     }
     for (int64_t l = nlev - 1; l >= 0; --l) {
@@ -1097,11 +1429,19 @@ hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *
         if (nr)
             hipLaunchKernelGGL(bwd_reduce_kernel<T>, dim3((unsigned)nr), dim3(kBS), 0, st, S->d_bred + H.bred_off[l],
                                S->d_col0, S->d_ncol, (const T *)Pp, Z, nrhs);
+        if (level_diag_solve<T, 4 * NQ>(false, st, S, l, L, Z, X, nrhs)) continue;
         for (int64_t stp = H.bstep_ptr[l]; stp < H.bstep_ptr[l + 1]; ++stp) {
             const int64_t nb = H.bstep_off[stp + 1] - H.bstep_off[stp];
-            if (nb)
-                hipLaunchKernelGGL((bwd_step_kernel<T, NQ>), dim3((unsigned)nb), dim3(kBS), 0, st,
-                                   S->d_btasks + H.bstep_off[stp], S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, Z, X, nrhs);
+            if (nb) {
+#define JFEM_BWD(BWV)                                                                                           \
+    hipLaunchKernelGGL((bwd_step_kernel<T, NQ, BWV>), dim3((unsigned)nb), dim3(kBS), 0, st, S->d_btasks + H.bstep_off[stp], \
+                       S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, Z, X, nrhs)
+                const int bw = H.bstep_bw[stp];
+                if (bw <= 16) JFEM_BWD(16);
+                else if (bw <= 32) JFEM_BWD(32);
+                else JFEM_BWD(64);
+#undef JFEM_BWD
+            }
         }
     }
     return hipGetLastError();
@@ -1180,6 +1520,7 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_btasks, H.btasks, st);
     if (e == hipSuccess) e = up(S, &S->d_ea_bounds, H.ea_bounds, st);
     if (e == hipSuccess) e = up(S, &S->d_bred, H.bred, st);
+    if (e == hipSuccess) e = up(S, &S->d_ftiles, H.ftiles, st);
     if (e == hipSuccess) e = up(S, &S->d_gtasks, H.gtasks, st);
     if (e == hipSuccess) e = up(S, &S->d_g_bounds, H.g_bounds, st);
     if (e == hipSuccess) e = up(S, &S->d_g_boff, H.g_boff, st);

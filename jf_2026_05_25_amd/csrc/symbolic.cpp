@@ -154,6 +154,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         S.bwd_off.assign(1, 0);
         S.bred_off.assign(1, 0);
         S.g_off.assign(1, 0);
+        S.ftile_off.assign(1, 0);
         S.g_boff.assign(1, 0);
         S.bstep_ptr.assign(1, 0);
         S.bstep_off.assign(1, 0);
@@ -487,11 +488,13 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
     }
     // task lists
     S.step_ptr.assign(nlev + 1, 0);
-    S.pstep_off.clear(); S.ustep_off.clear(); S.pstep_mid.clear();
+    S.pstep_off.clear(); S.ustep_off.clear(); S.pstep_mid.clear(); S.pstep_bw.clear(); S.bstep_bw.clear();
     S.ea_off.assign(nlev + 1, 0);
     S.bwd_off.assign(nlev + 1, 0);
     S.bred_off.assign(nlev + 1, 0);
     S.g_off.assign(nlev + 1, 0);
+    S.ftile_off.assign(nlev + 1, 0);
+    S.level_maxns.assign(nlev, 0);
     S.bstep_ptr.assign(nlev + 1, 0);
     S.bstep_off.clear();
     S.bslots_max = 0;
@@ -500,9 +503,12 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         S.bwd_off[l] = (int64_t)S.bwd_tiles.size();
         S.bred_off[l] = (int64_t)S.bred.size();
         S.g_off[l] = (int64_t)S.gtasks.size();
+        S.ftile_off[l] = (int64_t)S.ftiles.size();
         int64_t slots = 0;
         for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
             const int32_t s = S.level_sn[q];
+            S.level_maxns[l] = std::max<int64_t>(S.level_maxns[l], S.ncol[s]);
+            for (int64_t r0 = S.ncol[s]; r0 < S.m[s]; r0 += kTile) S.ftiles.push_back({s, 0, (int32_t)r0, 0});
             for (int64_t r0 = 0; r0 < S.m[s]; r0 += kGatherRows) S.gtasks.push_back({s, 0, (int32_t)r0, 0});
             const int64_t nu = S.m[s] - S.ncol[s];
             if (nu == 0) continue;
@@ -531,25 +537,33 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         S.bstep_ptr[l] = (int64_t)S.bstep_off.size();
         for (int64_t t = 0; t < maxsteps; ++t) {
             S.bstep_off.push_back((int64_t)S.btasks.size());
+            int32_t bw = 1;
             for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
                 int32_t s = S.level_sn[q];
                 int64_t nb = (S.ncol[s] + kPanel - 1) / kPanel;
                 int64_t kb = nb - 1 - t;
                 if (kb < 0) continue;
                 int64_t k = kb * kPanel;
+                bw = std::max<int32_t>(bw, (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - k));
                 for (int64_t j0 = 0; j0 <= k; j0 += kTile) S.btasks.push_back({s, (int32_t)k, (int32_t)j0, 0});
             }
+            S.bstep_bw.push_back(bw);
         }
         S.step_ptr[l] = (int64_t)S.pstep_off.size();
         for (int64_t t = 0; t < maxsteps; ++t) {
             S.pstep_off.push_back((int64_t)S.ptasks.size());
             S.ustep_off.push_back((int64_t)S.utasks.size());
             // diagonal tiles first (they factor and store the diagonal block), then the rest
+            int32_t bw = 1;
             for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
                 int32_t s = S.level_sn[q];
                 int64_t k = t * kPanel;
-                if (k < S.ncol[s]) S.ptasks.push_back({s, (int32_t)k, (int32_t)k, 0});
+                if (k < S.ncol[s]) {
+                    S.ptasks.push_back({s, (int32_t)k, (int32_t)k, 0});
+                    bw = std::max<int32_t>(bw, (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - k));
+                }
             }
+            S.pstep_bw.push_back(bw);
             S.pstep_mid.push_back((int64_t)S.ptasks.size());
             for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
                 int32_t s = S.level_sn[q];
@@ -580,6 +594,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
     S.bwd_off[nlev] = (int64_t)S.bwd_tiles.size();
     S.bred_off[nlev] = (int64_t)S.bred.size();
     S.g_off[nlev] = (int64_t)S.gtasks.size();
+    S.ftile_off[nlev] = (int64_t)S.ftiles.size();
     S.g_bounds.clear();
     S.g_boff.assign(S.gtasks.size() + 1, 0);
     for (size_t e = 0; e < S.gtasks.size(); ++e) {

@@ -37,6 +37,7 @@ struct SymbolicHost {
     std::vector<int64_t> step_ptr;          // per level: first step index (size levels+1)
     std::vector<int64_t> pstep_off, ustep_off;   // per step: offset (size steps+1)
     std::vector<int64_t> pstep_mid;              // per step: first non-diagonal panel tile
+    std::vector<int32_t> pstep_bw, bstep_bw;     // per (panel / backward) step: widest diagonal block
     std::vector<int64_t> ea_off;            // per level: offset into etasks (size levels+1)
     // per extend-add task, per child of its parent (child order): the child's
     // update columns [j0, j1) landing in front columns [c0, c1) and update rows
@@ -61,6 +62,11 @@ struct SymbolicHost {
     std::vector<int32_t> g_bounds;           // 2 per (gather task, child)
     std::vector<int64_t> g_boff;             // per gather task: first (task, child) entry
     int64_t bslots_max = 0;                  // partial slots of the busiest level
+    // Forward L21 row tiles (s, 0, r0, 0), r0 in [ns, ms) step kTile, per level;
+    // widest pivot block per level
+    std::vector<PanelTask> ftiles;
+    std::vector<int64_t> ftile_off;
+    std::vector<int64_t> level_maxns;
     std::vector<int64_t> bstep_ptr;          // per level: first backward step
     std::vector<int64_t> bstep_off;          // per backward step: offset into btasks
     int64_t L_entries = 0, U_entries = 0, upd_rows = 0;

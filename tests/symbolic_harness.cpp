@@ -216,9 +216,37 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 }
             }
         }
-        for (int64_t stp = S.step_ptr[l]; stp < S.step_ptr[l + 1]; ++stp) {
-            ++launch;
-            for (int64_t q = S.pstep_off[stp]; q < S.pstep_off[stp + 1]; ++q) fwd_task(S.ptasks[q], (launch << 32) + q);
+        ++launch;   // pivot-block solves: one task per supernode (y1 = L11^{-1} y1 into Z)
+        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+            const int64_t id = (launch << 32) + q;
+            const int s = S.level_sn[q];
+            const int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
+            const double *Ls = L.data() + S.lofs[s];
+            std::vector<double> y(ns * nrhs);
+            for (int64_t i = 0; i < ns; ++i)
+                for (int r = 0; r < nrhs; ++r) { rd(wY, rY, c0 + i, id); y[i * nrhs + r] = Y[(c0 + i) * nrhs + r]; }
+            for (int64_t j = 0; j < ns; ++j)
+                for (int r = 0; r < nrhs; ++r) {
+                    y[j * nrhs + r] /= Ls[j + ms * j];
+                    for (int64_t i = j + 1; i < ns; ++i) y[i * nrhs + r] -= Ls[i + ms * j] * y[j * nrhs + r];
+                }
+            for (int64_t i = 0; i < ns; ++i)
+                for (int r = 0; r < nrhs; ++r) { wr(wZ, rZ, c0 + i, id); Z[(c0 + i) * nrhs + r] = y[i * nrhs + r]; }
+        }
+        ++launch;   // L21 row tiles: W_s[rows] -= L21[rows, :] y1
+        for (int64_t q = S.ftile_off[l]; q < S.ftile_off[l + 1]; ++q) {
+            const int64_t id = (launch << 32) + q;
+            const PanelTask t = S.ftiles[q];
+            const int s = t.s;
+            const int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
+            const double *Ls = L.data() + S.lofs[s];
+            for (int64_t i = t.r0; i < std::min<int64_t>(t.r0 + kTile, ms); ++i)
+                for (int r = 0; r < nrhs; ++r) {
+                    double v = 0;
+                    for (int64_t j = 0; j < ns; ++j) { rd(wZ, rZ, c0 + j, id); v += Ls[i + ms * j] * Z[(c0 + j) * nrhs + r]; }
+                    wr(wW, rW, S.relofs[s] + i - ns, id);
+                    W[(S.relofs[s] + i - ns) * nrhs + r] -= v;
+                }
         }
     }
     for (int64_t l = nlev - 1; l >= 0; --l) {
@@ -256,9 +284,23 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                     Z[(c0 + j) * nrhs + r] -= v;
                 }
         }
-        for (int64_t stp = S.bstep_ptr[l]; stp < S.bstep_ptr[l + 1]; ++stp) {
-            ++launch;
-            for (int64_t q = S.bstep_off[stp]; q < S.bstep_off[stp + 1]; ++q) bwd_task(S.btasks[q], (launch << 32) + q);
+        ++launch;   // pivot-block backward solves: one task per supernode (x1 = L11^{-T} z1 into X)
+        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+            const int64_t id = (launch << 32) + q;
+            const int s = S.level_sn[q];
+            const int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
+            const double *Ls = L.data() + S.lofs[s];
+            std::vector<double> x(ns * nrhs);
+            for (int64_t i = 0; i < ns; ++i)
+                for (int r = 0; r < nrhs; ++r) { rd(wZ, rZ, c0 + i, id); x[i * nrhs + r] = Z[(c0 + i) * nrhs + r]; }
+            for (int64_t j = ns - 1; j >= 0; --j)
+                for (int r = 0; r < nrhs; ++r) {
+                    double v = x[j * nrhs + r];
+                    for (int64_t i = j + 1; i < ns; ++i) v -= Ls[i + ms * j] * x[i * nrhs + r];
+                    x[j * nrhs + r] = v / Ls[j + ms * j];
+                }
+            for (int64_t i = 0; i < ns; ++i)
+                for (int r = 0; r < nrhs; ++r) { wr(wX, rX, c0 + i, id); X[(c0 + i) * nrhs + r] = x[i * nrhs + r]; }
         }
     }
     if (conflict) return 5;
