@@ -41,6 +41,19 @@ def cpu_model():
     return platform.processor()
 
 
+def read_traffic(path: str, precision: int):
+    """HBM bytes per launch of the forward step kernel measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known size), from
+    the committed summary; None when absent."""
+    try:
+        for line in open(path):
+            if line.startswith(f"hbm_bytes_per_launch_fp{precision}:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def run_cpu_baseline(nx_sample: int):
     """The FP64 CPU oracle (single thread) on a bounded sample of the workload:
     the same wing box cut to nx_sample spanwise cells (same cross-section, same
@@ -74,8 +87,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="before timing, run one device copy of this many MiB (torch) so that hardware "
-                         "counters (FETCH_SIZE / WRITE_SIZE) can be scaled by a kernel of known traffic")
+                    help="before timing, run a fill and an elementwise multiply over this many MiB (torch) so "
+                         "that hardware counters (FETCH_SIZE / WRITE_SIZE) can be scaled by kernels of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures the single-GPU path; N > 1 is not run "
@@ -85,11 +98,13 @@ def main():
     from jf_2026_05_25_amd import meshes, solver
 
     if args.calibrate_copy_mib > 0:
+        # two elementwise kernels of known traffic: a fill (writes N bytes) and
+        # dst = 2 * src (reads N, writes N)
         import torch
         nel = args.calibrate_copy_mib * (1 << 20) // 4
         src = torch.ones(nel, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.cuda.synchronize()
+        dst = src * 2.0
         torch.cuda.synchronize()
         del src, dst
 
@@ -141,7 +156,8 @@ def main():
         ctx.reset_timing()
         F, u_t, rr_t, it_t, ok_t = step(prec, F)
         fam = {name: ctx.kernel_time(i) for i, name in enumerate(
-            ["element", "assembly", "factor", "solve", "spmv", "factor_update", "factor_panel", "extend_add"])}
+            ["element", "assembly", "factor", "solve", "spmv", "factor_update", "factor_panel", "extend_add",
+             "fwd_step"])}
         ctx.enable_timing(0)
         results[prec] = {"total": total, "lat": np.array(lat), "u": u, "relres": rr, "iters": it, "fam": fam,
                          "factor_bytes": F.device_bytes(), "converged": conv}
@@ -190,19 +206,33 @@ def main():
         out["fp32_refined_error_vs_fp64"] = float(np.max(np.linalg.norm(u32 - u64, axis=0) /
                                                          np.linalg.norm(u64, axis=0)))
         out["fp64_kernel_ms"] = {k: round(v[0], 4) for k, v in r64["fam"].items()}
-    # kernel-family breakdown and roofline of the dominant kernel (factor update tiles)
+    # kernel-family breakdown; roofline of the dominant kernel of the headline path:
+    # the forward-solve step kernel (HBM-bound: streams the L panel of each
+    # 64-column step plus the pending right-hand-side rows)
     fam = r["fam"]
     out["kernel_ms"] = {k: round(v[0], 4) for k, v in fam.items()}
+    es = 4 if main_prec == 32 else 8
+    f_ms, f_n = fam["fwd_step"]
+    if f_n and info["fwd_step_launches"]:
+        sweeps = f_n / info["fwd_step_launches"]
+        bytes_sweep = es * (info["fwd_step_L_entries"] + nrhs * (2 * info["fwd_step_rows"] + info["fwd_step_bsum"]))
+        per_launch = bytes_sweep / info["fwd_step_launches"]
+        avg_ms = f_ms / f_n
+        achieved = per_launch / (avg_ms * 1e-3) / 1e9
+        traffic = read_traffic(os.path.join(ROOT, "profiles", "fwd_step_kernel_wingbox_traffic.txt"), main_prec)
+        out["roofline"] = {"bound": "hbm", "kernel": f"fwd_step_kernel<{'float' if main_prec == 32 else 'double'}>",
+                           "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS,
+                           "traffic": traffic, "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms,
+                           "launches_per_sweep": info["fwd_step_launches"], "sweeps_timed": sweeps}
     upd_ms, upd_n = fam["factor_update"]
     peak = MI355X_FP32_TFLOPS if main_prec == 32 else MI355X_FP64_TFLOPS
     if upd_n:
         per_launch_flops = info["update_flops"] / upd_n
         avg_ms = upd_ms / upd_n
         achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
-        out["roofline"] = {"bound": "mfma", "kernel": f"update_kernel<{'float' if main_prec == 32 else 'double'}>",
-                           "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                           "traffic": None, "launches": upd_n, "avg_launch_ms": avg_ms,
-                           "flops_per_launch": per_launch_flops}
+        out["update_kernel_roofline"] = {"bound": "mfma", "kernel": f"update_kernel<{'float' if main_prec == 32 else 'double'}>",
+                                         "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                                         "launches": upd_n, "avg_launch_ms": avg_ms, "flops_per_launch": per_launch_flops}
     if not args.no_cpu_baseline:
         cb = run_cpu_baseline(args.cpu_sample_nx)
         cb.pop("seconds", None)

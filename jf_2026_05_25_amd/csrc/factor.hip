@@ -1412,11 +1412,21 @@ hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *
 #define JFEM_FWD(BWV)                                                                                           \
     hipLaunchKernelGGL((fwd_step_kernel<T, NQ, BWV>), dim3((unsigned)nt), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp], \
                        S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L, Y, W, Z, nrhs)
+                const bool detail = ctx->timing_level >= 2;
+                if (detail) (void)hipEventRecord(ctx->ev2, st);
                 const int bw = H.pstep_bw[stp];
                 if (bw <= 16) JFEM_FWD(16);
                 else if (bw <= 32) JFEM_FWD(32);
                 else JFEM_FWD(64);
 #undef JFEM_FWD
+                if (detail) {
+                    (void)hipEventRecord(ctx->ev3, st);
+                    (void)hipEventSynchronize(ctx->ev3);
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, ctx->ev2, ctx->ev3);
+                    ctx->ms[FAM_FWDSTEP] += ms;
+                    ctx->launches[FAM_FWDSTEP] += 1;
+                }
             }
         }
     }
@@ -1541,6 +1551,25 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
         info->update_entries = H.U_entries;
         info->update_flops = H.update_flops;
         info->update_launches = (int64_t)H.ustep_off.size() - 1;
+        // forward step kernel: levels wider than one block use it (see level_diag_solve)
+        int64_t fl = 0, fL = 0, fr = 0, fb = 0;
+        for (int64_t l = 0; l + 1 < (int64_t)H.level_ptr.size(); ++l) {
+            if (H.level_maxns[l] <= kPanel) continue;
+            for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
+                if (H.pstep_off[stp + 1] > H.pstep_off[stp]) ++fl;
+                for (int64_t q = H.pstep_off[stp]; q < H.pstep_mid[stp]; ++q) {
+                    const PanelTask &t = H.ptasks[q];
+                    const int64_t b = std::min<int64_t>(kPanel, H.ncol[t.s] - t.k), rows = H.m[t.s] - t.k;
+                    fL += rows * b - b * (b - 1) / 2;
+                    fr += rows;
+                    fb += b;
+                }
+            }
+        }
+        info->fwd_step_launches = fl;
+        info->fwd_step_L_entries = fL;
+        info->fwd_step_rows = fr;
+        info->fwd_step_bsum = fb;
     }
     *out = S;
     return JFEM_OK;
