@@ -31,6 +31,7 @@ struct jfem_symbolic {
     int64_t *d_amap_dst = nullptr;
     PanelTask *d_ptasks = nullptr, *d_btasks = nullptr, *d_bwd = nullptr, *d_bred = nullptr, *d_gtasks = nullptr;
     PanelTask *d_ftiles = nullptr;
+    StepTask *d_fsteps = nullptr, *d_bsteps = nullptr;
     int32_t *d_g_bounds = nullptr;
     int64_t *d_g_boff = nullptr;
     UpdateTask *d_utasks = nullptr;
@@ -477,20 +478,24 @@ __global__ void __launch_bounds__(kBS) fwd_gather_kernel(const PanelTask *__rest
 // by the diagonal task (r0 == k); rows [max(r0, k+b), r0+kTile) of the front get
 // -= L[rows, k..k+b) y_k (pivot rows into Y, update rows into W).
 template <class T, int NQ, int BW>
-__global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ col0,
-                                                       const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
-                                                       const int64_t *__restrict__ lofs, const int64_t *__restrict__ relofs,
-                                                       const T *__restrict__ L, T *__restrict__ Y, T *__restrict__ W,
-                                                       T *__restrict__ Z, int nrhs) {
-    const PanelTask t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s], k = t.k, c0 = col0[s];
-    const int b = (int)min<int64_t>(kPanel, ns - k);
-    const T *Ls = L + lofs[s];
+__global__ void __launch_bounds__(kBS) fwd_step_kernel(const StepTask *__restrict__ tasks, const T *__restrict__ L,
+                                                       T *__restrict__ Y, T *__restrict__ W, T *__restrict__ Z, int nrhs) {
+    const StepTask t = tasks[blockIdx.x];
+    const int64_t ns = t.ns, ms = t.ms, k = t.k, c0 = t.c0;
+    const int b = t.b;
+    const T *Ls = L + t.lofs;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (w * NQ >= nrhs) return;               // whole wave idle (uniform)
     const int li = min(lane, b - 1);
-    T lr[BW];                             // row `lane` of L_kk (identity row beyond b)
+    // the tile's rows do not depend on the diagonal solve: their loads go first
+    const int64_t rs = max<int64_t>(t.r0, k + b), re = min<int64_t>((int64_t)t.r0 + kTile, ms);
+    const bool has_tile = rs < re;
+    const int64_t row = t.r0 + lane;
+    const int64_t rowc = has_tile ? min<int64_t>(max<int64_t>(row, rs), re - 1) : k + li;
+    T tl[BW];
+#pragma unroll
+    for (int j = 0; j < BW; ++j) tl[j] = Ls[rowc + ms * (k + min(j, b - 1))];
+    T lr[BW];                                 // row `lane` of L_kk (identity row beyond b)
 #pragma unroll
     for (int j = 0; j < BW; ++j) {
         const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
@@ -503,9 +508,13 @@ __global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restri
         const T v = Y[(c0 + k + li) * nrhs + min(r, nrhs - 1)];
         y[q] = (lane < b && r < nrhs) ? v : T(0);
     }
+    T dg = T(1);                              // own diagonal entry (unrolled select, no indexing)
+#pragma unroll
+    for (int j = 0; j < BW; ++j) dg = j == lane ? lr[j] : dg;
+    const T inv_own = T(1) / dg;
 #pragma unroll
     for (int j = 0; j < BW; ++j) {
-        const T inv = T(1) / bcast(lr[j], j);
+        const T inv = bcast(inv_own, j);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const T yj = bcast(y[q], j) * inv;
@@ -520,21 +529,17 @@ __global__ void __launch_bounds__(kBS) fwd_step_kernel(const PanelTask *__restri
             if (r < nrhs) Z[(c0 + k + lane) * nrhs + r] = y[q];
         }
     }
-    const int64_t rs = max<int64_t>(t.r0, k + b), re = min<int64_t>(t.r0 + kTile, ms);
-    if (rs >= re) return;
-    const int64_t row = t.r0 + lane;
-    const int64_t rowc = min<int64_t>(max<int64_t>(row, rs), re - 1);
+    if (!has_tile) return;
     T acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) acc[q] = T(0);
 #pragma unroll
     for (int j = 0; j < BW; ++j) {
-        const T l = Ls[rowc + ms * (k + min(j, b - 1))];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] += l * bcast(y[q], j);
+        for (int q = 0; q < NQ; ++q) acc[q] += tl[j] * bcast(y[q], j);
     }
     if (row >= rs && row < re) {
-        T *dst = row < ns ? (Y + (c0 + row) * nrhs) : (W + (relofs[s] + row - ns) * nrhs);
+        T *dst = row < ns ? (Y + (c0 + row) * nrhs) : (W + (t.relofs + row - ns) * nrhs);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int r = w * NQ + q;
@@ -634,17 +639,14 @@ __global__ void __launch_bounds__(kBS) bwd_reduce_kernel(const PanelTask *__rest
 // backward step task (s, k, j0): x_k = L_kk^{-T} (pending rows k..k+b of Z), written
 // to X by the diagonal task (j0 == k); for j0 < k, Z[j0..j0+kTile) -= L[k..k+b, j0..]' x_k.
 template <class T, int NQ, int BW>
-__global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ col0,
-                                                       const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
-                                                       const int64_t *__restrict__ lofs, const T *__restrict__ L,
+__global__ void __launch_bounds__(kBS) bwd_step_kernel(const StepTask *__restrict__ tasks, const T *__restrict__ L,
                                                        T *__restrict__ Z, T *__restrict__ X, int nrhs) {
     __shared__ T D[BW][BW + 1];              // D[i][c] = L[k+i][k+c] (identity beyond b)
     __shared__ T E[BW][kTile + 1];           // E[i][c] = L[k+i][j0+c] (zero rows beyond b)
-    const PanelTask t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s], k = t.k, c0 = col0[s], j0 = t.r0;
-    const int b = (int)min<int64_t>(kPanel, ns - k);
-    const T *Ls = L + lofs[s];
+    const StepTask t = tasks[blockIdx.x];
+    const int64_t ms = t.ms, k = t.k, c0 = t.c0, j0 = t.r0;
+    const int b = t.b;
+    const T *Ls = L + t.lofs;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const bool diag = j0 == k;
 #pragma unroll
@@ -663,11 +665,6 @@ __global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restri
         const T e = Ls[(k + min(i, b - 1)) + ms * (diag ? k + min(c, b - 1) : j0 + c)];
         E[i][c] = i < b ? e : T(0);
     }
-    __syncthreads();
-    if (w * NQ >= nrhs) return;
-    T lc[BW];                                 // column `lane` of L_kk
-#pragma unroll
-    for (int i = 0; i < BW; ++i) lc[i] = lane < BW ? D[i][min(lane, BW - 1)] : T(0);
     T z[NQ];
     const int li = min(lane, b - 1);
 #pragma unroll
@@ -676,6 +673,11 @@ __global__ void __launch_bounds__(kBS) bwd_step_kernel(const PanelTask *__restri
         const T v = Z[(c0 + k + li) * nrhs + min(r, nrhs - 1)];
         z[q] = (r < nrhs && lane < b) ? v : T(0);
     }
+    __syncthreads();
+    if (w * NQ >= nrhs) return;
+    T lc[BW];                                 // column `lane` of L_kk
+#pragma unroll
+    for (int i = 0; i < BW; ++i) lc[i] = lane < BW ? D[i][min(lane, BW - 1)] : T(0);
 #pragma unroll
     for (int j = BW - 1; j >= 0; --j) {
         const T inv = T(1) / bcast(lc[j], j);
@@ -1410,8 +1412,8 @@ hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *
             const int64_t nt = H.pstep_off[stp + 1] - H.pstep_off[stp];
             if (nt) {
 #define JFEM_FWD(BWV)                                                                                           \
-    hipLaunchKernelGGL((fwd_step_kernel<T, NQ, BWV>), dim3((unsigned)nt), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp], \
-                       S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L, Y, W, Z, nrhs)
+    hipLaunchKernelGGL((fwd_step_kernel<T, NQ, BWV>), dim3((unsigned)nt), dim3(kBS), 0, st, S->d_fsteps + H.pstep_off[stp], \
+                       L, Y, W, Z, nrhs)
                 const bool detail = ctx->timing_level >= 2;
                 if (detail) (void)hipEventRecord(ctx->ev2, st);
                 const int bw = H.pstep_bw[stp];
@@ -1444,8 +1446,8 @@ hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *
             const int64_t nb = H.bstep_off[stp + 1] - H.bstep_off[stp];
             if (nb) {
 #define JFEM_BWD(BWV)                                                                                           \
-    hipLaunchKernelGGL((bwd_step_kernel<T, NQ, BWV>), dim3((unsigned)nb), dim3(kBS), 0, st, S->d_btasks + H.bstep_off[stp], \
-                       S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, Z, X, nrhs)
+    hipLaunchKernelGGL((bwd_step_kernel<T, NQ, BWV>), dim3((unsigned)nb), dim3(kBS), 0, st, S->d_bsteps + H.bstep_off[stp], \
+                       L, Z, X, nrhs)
                 const int bw = H.bstep_bw[stp];
                 if (bw <= 16) JFEM_BWD(16);
                 else if (bw <= 32) JFEM_BWD(32);
@@ -1531,6 +1533,23 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_ea_bounds, H.ea_bounds, st);
     if (e == hipSuccess) e = up(S, &S->d_bred, H.bred, st);
     if (e == hipSuccess) e = up(S, &S->d_ftiles, H.ftiles, st);
+    // (host copies kept alive until the stream synchronisation below)
+    std::vector<StepTask> fsteps_h, bsteps_h;
+    {
+        auto mk = [&](const std::vector<PanelTask> &src) {
+            std::vector<StepTask> out(src.size());
+            for (size_t i = 0; i < src.size(); ++i) {
+                const PanelTask &p = src[i];
+                out[i] = StepTask{p.s, p.k, p.r0, (int32_t)std::min<int64_t>(kPanel, H.ncol[p.s] - p.k), H.ncol[p.s],
+                                  H.m[p.s], H.lofs[p.s], H.col0[p.s], H.relofs[p.s], 0};
+            }
+            return out;
+        };
+        fsteps_h = mk(H.ptasks);
+        bsteps_h = mk(H.btasks);
+        if (e == hipSuccess) e = up(S, &S->d_fsteps, fsteps_h, st);
+        if (e == hipSuccess) e = up(S, &S->d_bsteps, bsteps_h, st);
+    }
     if (e == hipSuccess) e = up(S, &S->d_gtasks, H.gtasks, st);
     if (e == hipSuccess) e = up(S, &S->d_g_bounds, H.g_bounds, st);
     if (e == hipSuccess) e = up(S, &S->d_g_boff, H.g_boff, st);

@@ -13,7 +13,13 @@ constexpr int kGatherRows = 512; // front rows per forward gather task
 
 struct PanelTask { int32_t s, k, r0, pad; };          // supernode, panel column, first row
 struct UpdateTask { int32_t s, k, r0, c0; };          // supernode, panel column, tile origin
-struct EaTask { int32_t s, c0, c1, r0; };             // parent, front columns [c0, c1), rows [r0, r0 + kEaRows)
+struct EaTask { int32_t s, c0, c1, r0; };
+// a panel/solve task with its supernode's metadata inlined (one 64-byte record,
+// so that a step kernel starts without dependent metadata loads)
+struct StepTask {
+    int32_t s, k, r0, b;            // supernode, panel column, first row/column of the tile, block width
+    int64_t ns, ms, lofs, c0, relofs, pad;
+};             // parent, front columns [c0, c1), rows [r0, r0 + kEaRows)
 
 struct SymbolicHost {
     int64_t ndof = 0, n = 0;        // all DOFs, free DOFs

@@ -335,6 +335,19 @@ class Factor:
             return out + ({"x_lo": Xl[:, 0] if one else Xl, "relres_fp64": rr64},)
         return out
 
+    def solve_device(self, K: DeviceMatrix, b_dev: "DeviceBuffer", x_dev: "DeviceBuffer", nrhs: int,
+                     tol: float = 1e-10, max_refine: int = 100, ctx: Context | None = None):
+        """As ``solve`` with right-hand sides and solution in device memory
+        (ndof x nrhs column-major FP64, ``DeviceBuffer``): nothing crosses PCIe but
+        the per-RHS residuals.  Returns ``(relres, iters, ok)``."""
+        ctx = ctx or self.ctx
+        iters = np.zeros(nrhs, np.int32); rr = np.zeros(nrhs)
+        code = ctx.L.jfem_solve_dev(ctx.h, self.h, K.h, nrhs, b_dev.ptr, x_dev.ptr, tol, max_refine, ptr(iters),
+                                    ptr(rr), None, None)
+        if code not in (0, 8):
+            check(ctx.h, code)
+        return rr, iters, code == 0
+
     def free(self):
         if self.h:
             self.ctx.L.jfem_factor_free(self.h)
@@ -358,6 +371,38 @@ def factor_with_fallback(S: Symbolic, K: DeviceMatrix, precision: int, ctx: Cont
         if e.code != _lib.JFEM_ERR_NOT_SPD or precision == FP64:
             raise
     return Factor(S, K, FP64, ctx), FP64
+
+
+class DeviceBuffer:
+    """Device memory owned by the caller of the ``_dev`` entry points."""
+
+    def __init__(self, nbytes: int, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(self.ctx.h, self.ctx.L.jfem_dev_alloc(self.ctx.h, max(self.nbytes, 8), C.byref(p)))
+        self.ptr = p
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(self.ctx.h, self.ctx.L.jfem_memcpy_h2d(self.ctx.h, self.ptr, ptr(a), a.nbytes))
+
+    def download(self, out: np.ndarray):
+        assert out.flags["C_CONTIGUOUS"] and out.nbytes <= self.nbytes
+        check(self.ctx.h, self.ctx.L.jfem_memcpy_d2h(self.ctx.h, ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.L.jfem_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class LinearSolveCache:
