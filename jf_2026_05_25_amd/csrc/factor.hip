@@ -32,6 +32,8 @@ struct jfem_symbolic {
     PanelTask *d_ptasks = nullptr, *d_btasks = nullptr, *d_bwd = nullptr, *d_bred = nullptr, *d_gtasks = nullptr;
     PanelTask *d_ftiles = nullptr;
     StepTask *d_fsteps = nullptr, *d_bsteps = nullptr;
+    FlowTask *d_flow_f = nullptr, *d_flow_b = nullptr;
+    FlowChild *d_flow_gb = nullptr;
     int32_t *d_g_bounds = nullptr;
     int64_t *d_g_boff = nullptr;
     UpdateTask *d_utasks = nullptr;
@@ -47,6 +49,7 @@ struct jfem_factor {
     int precision = 64;
     void *d_L = nullptr;
     void *d_U = nullptr;
+    void *d_Linv = nullptr;         // inverse diagonal blocks of the panels (solves)
     int64_t bytes = 0;
 };
 
@@ -712,6 +715,390 @@ __global__ void __launch_bounds__(kBS) bwd_step_kernel(const StepTask *__restric
         const int r = w * NQ + q;
         if (r < nrhs) Z[(c0 + j0 + lane) * nrhs + r] -= acc[q];
     }
+}
+
+// ---------------------------------------------------------------- dataflow solves
+// One launch per direction.  The workgroup of index e takes task e of a list in
+// dependency order (children before parents, panels in solve order), and a task
+// waits only on tasks listed before it, through per-supernode counters in device
+// memory.  Each XCD dispatches its workgroups in index order, so the task a
+// workgroup waits on is already running or finished.
+// Vectors exchanged between tasks of one launch (W, Z in the forward solve, X in
+// the backward solve) are stored and loaded with device-scope accesses (sc1),
+// coherent across the XCDs' L2 caches; a producer waits for its stores to
+// complete (s_waitcnt) before it bumps the counter, and a consumer polls the
+// counter with device-scope loads.  This is the agent-scope release/acquire
+// sequence without the L2 write-back and invalidate that plain stores and loads
+// would need (each of those walks the whole L2 of its XCD, and they serialise).
+// Waits are bounded (kWaitTicks of the 100 MHz wall clock): on expiry the abort
+// flag is raised, every waiting workgroup leaves, and the host reports an error.
+// Counters (int32, nsn each): ready[s] = finished update blocks of s's children,
+// zdone[s] = forward panels of s finished (they finish in order), xdone[s] =
+// backward panels of s finished (from the last); then the abort flag.
+// Every sum has a fixed order -- children in child order, panels in solve order,
+// the four waves' partial sums in wave order -- whatever the timing, so results
+// are bitwise reproducible.
+// Diagonal blocks are applied through their inverses (computed once per factor by
+// diag_inverse_kernel), so a step is a matrix-vector product and not a
+// 64-step recurrence.
+constexpr long long kWaitTicks = 100000000;
+
+template <class T>
+__device__ __forceinline__ T ld_dev(const T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <class T>
+__device__ __forceinline__ void st_dev(T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// every memory operation of the wavefront issued so far has completed
+__device__ __forceinline__ void complete_stores() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0);
+    asm volatile("" ::: "memory");
+}
+
+__device__ bool wait_at_least(const int32_t *ctr, int32_t target, int32_t *abort_flag) {
+    if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        const long long t0 = wall_clock64();
+        int spins = 0;
+        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if ((++spins & 15) == 0) {
+                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+                if (wall_clock64() - t0 > kWaitTicks) {
+                    __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    return false;
+                }
+            }
+        }
+    }
+    asm volatile("" ::: "memory");
+    return true;
+}
+
+// thread (lane = row i, wave w) loads columns [16w, 16w+16) of its row of a
+// rows x cols block (column-major, leading dimension ld); indices are clamped
+// into the block, the caller masks what lies outside
+template <class T>
+__device__ __forceinline__ void load_cols16(T (&v)[16], const T *base, int64_t ld, int rows, int cols, int lane, int w) {
+    const int64_t i = min(lane, rows - 1);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[c] = base[i + ld * min(16 * w + c, cols - 1)];
+}
+
+// Vectors hold nrhs interleaved right-hand sides (stride nrhs); the kernels work
+// on NR >= nrhs of them (4, 8 or 16), the padding being zero.
+// zs[c][r] = src[c * nrhs + r] for c < cnt, r < nrhs, zero beyond (whole workgroup;
+// src was published by another task of the launch)
+template <class T, int NR>
+__device__ __forceinline__ void load_vec(T (*zs)[NR], const T *src, int cnt, int nrhs, int tid) {
+#pragma unroll
+    for (int j = 0; j < kTile * NR / kBS; ++j) {
+        const int q = tid + kBS * j, c = q / NR, r = q % NR;
+        const T v = ld_dev(src + (int64_t)min(c, cnt - 1) * nrhs + min(r, nrhs - 1));
+        zs[c][r] = (c < cnt && r < nrhs) ? v : T(0);
+    }
+}
+
+// acc[r] += sum over the wave's 16 columns c of v[c] * zs[16w + c][r]
+template <class T, int NR>
+__device__ __forceinline__ void fwd_mul(T (&acc)[NR], const T (&v)[16], const T (*zs)[NR], int w) {
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const T *z = zs[16 * w + c];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += v[c] * z[r];
+    }
+}
+
+// forward: pivot task (s, p): z_p = L_pp^{-1} (y_p + children's contributions
+// - sum_{k<p} L[p rows, k] z_k), written to Z; update task: w = children's
+// contributions - sum_k L[rows, k] z_k, written to s's update vector in W.
+template <class T, int NR>
+__global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restrict__ tasks,
+                                                       const FlowChild *__restrict__ gch,
+                                                       const int32_t *__restrict__ relmap, const T *__restrict__ L,
+                                                       const T *__restrict__ Linv, const T *__restrict__ Y, T *W, T *Z,
+                                                       int nrhs, int32_t *cnt, int64_t nsn) {
+    __shared__ __attribute__((aligned(16))) T vs[kTile][NR];
+    __shared__ __attribute__((aligned(16))) T zs[kTile][NR];
+    __shared__ __attribute__((aligned(16))) T red[3][kTile][NR];
+    __shared__ int ok;
+    const FlowTask t = tasks[blockIdx.x];
+    int32_t *ready = cnt, *zdone = cnt + nsn, *abort_flag = cnt + 3 * nsn;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const bool pivot = t.p >= 0;
+    const int kend = pivot ? t.p : t.np;
+    const T *Ls = L + t.lofs + t.r0;                   // front row r0, column 0
+    T lv[16], inv[16];
+    if (kend > 0) load_cols16(lv, Ls, t.ms, t.nr, (int)min<int64_t>(kPanel, t.ns), lane, w);
+    if (pivot) load_cols16(inv, Linv + t.iofs, t.nr, t.nr, t.nr, lane, w);
+#pragma unroll
+    for (int j = 0; j < kTile * NR / kBS; ++j) {
+        const int q = tid + kBS * j, i = q / NR, r = q % NR;
+        const int64_t yi = pivot ? t.c0 + t.r0 + min(i, t.nr - 1) : 0;
+        const T y = Y[yi * nrhs + min(r, nrhs - 1)];
+        vs[i][r] = (pivot && i < t.nr && r < nrhs) ? y : T(0);
+    }
+    if (t.need > 0) {
+        if (tid == 0) ok = wait_at_least(&ready[t.s], t.need, abort_flag);
+        __syncthreads();
+        if (!ok) return;
+        for (int j = 0; j < t.nch; ++j) {
+            const FlowChild ch = gch[t.gb + j];
+            const int64_t nq = (int64_t)(ch.i1 - ch.i0) * nrhs;
+            for (int64_t q = tid; q < nq; q += kBS) {
+                const int64_t i = ch.i0 + q / nrhs;
+                const int r = (int)(q % nrhs);
+                const int row = relmap[ch.relofs + i] - t.r0;
+                vs[row][r] += ld_dev(W + (ch.relofs + i) * nrhs + r);
+            }
+            __syncthreads();
+        }
+    }
+    T acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = T(0);
+    for (int k = 0; k < kend; ++k) {
+        if (tid == 0) ok = wait_at_least(&zdone[t.s], k + 1, abort_flag);
+        __syncthreads();
+        if (!ok) return;
+        const int64_t col = (int64_t)kPanel * k;
+        load_vec<T, NR>(zs, Z + (t.c0 + col) * nrhs, (int)min<int64_t>(kPanel, t.ns - col), nrhs, tid);
+        __syncthreads();
+        T cur[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) cur[c] = lv[c];
+        if (k + 1 < kend)
+            load_cols16(lv, Ls + t.ms * (col + kPanel), t.ms, t.nr, (int)min<int64_t>(kPanel, t.ns - col - kPanel), lane, w);
+        fwd_mul<T, NR>(acc, cur, zs, w);
+    }
+    if (w > 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
+    }
+    __syncthreads();
+    T v[NR];
+    if (w == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) v[r] = vs[lane][r] - (((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
+    }
+    if (!pivot) {
+        if (w == 0) {
+            if (lane < t.nr) {
+                T *dst = W + (t.relofs + t.r0 - t.ns + lane) * nrhs;
+#pragma unroll
+                for (int r = 0; r < NR; ++r)
+                    if (r < nrhs) st_dev(dst + r, v[r]);
+            }
+            complete_stores();
+            if (lane == 0) __hip_atomic_fetch_add(&ready[t.parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+    }
+    if (w == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) zs[lane][r] = lane < t.nr ? v[r] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = T(0);
+    fwd_mul<T, NR>(acc, inv, zs, w);
+    if (w > 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
+    }
+    __syncthreads();
+    if (w == 0) {
+        if (lane < t.nr) {
+            T *dst = Z + (t.c0 + t.r0 + lane) * nrhs;
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (r < nrhs) st_dev(dst + r, ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
+        }
+        complete_stores();
+        if (lane == 0) __hip_atomic_store(&zdone[t.s], t.p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Lt[i][c] = block entry (i, c) for i < rows, c < cols, zero elsewhere, from the
+// registers of load_cols16
+template <class T>
+__device__ __forceinline__ void stage_tile(T (*Lt)[kTile + 1], const T (&v)[16], int rows, int cols, int lane, int w) {
+#pragma unroll
+    for (int c = 0; c < 16; ++c) Lt[lane][16 * w + c] = (lane < rows && 16 * w + c < cols) ? v[c] : T(0);
+}
+
+// acc[r] += sum over the wave's 16 rows i of Lt[i][lane] * xs[i][r]
+template <class T, int NR>
+__device__ __forceinline__ void bwd_mul(T (&acc)[NR], const T (*Lt)[kTile + 1], const T (*xs)[NR], int lane, int w) {
+#pragma unroll
+    for (int i = 16 * w; i < 16 * w + 16; ++i) {
+        const T l = Lt[i][lane];
+        const T *x = xs[i];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] += l * x[r];
+    }
+}
+
+// backward: task (s, p): x_p = L_pp^{-T} (z_p - sum over the rows i below the
+// panel of L[i, p]' x_i): first the update rows (their x final once the parent
+// is done), then the later panels of s from the last, each once it is done.
+template <class T, int NR>
+__global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restrict__ tasks, const int32_t *__restrict__ rows,
+                                                       const T *__restrict__ L, const T *__restrict__ Linv,
+                                                       const T *__restrict__ Z, T *X, int nrhs, int32_t *cnt, int64_t nsn) {
+    constexpr int XQ = kTile * NR / kBS;               // x entries per thread and tile
+    __shared__ __attribute__((aligned(16))) T Lt[kTile][kTile + 1];
+    __shared__ __attribute__((aligned(16))) T xs[kTile][NR];
+    __shared__ __attribute__((aligned(16))) T red[3][kTile][NR];
+    __shared__ int ok;
+    const FlowTask t = tasks[blockIdx.x];
+    int32_t *xdone = cnt + 2 * nsn, *abort_flag = cnt + 3 * nsn;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nc = t.nr;
+    const int64_t nu = t.ms - t.ns;
+    const int nchunk = (int)((nu + kTile - 1) / kTile);
+    const int ntile = nchunk + (t.np - 1 - t.p);
+    const T *Lc = L + t.lofs + t.ms * t.r0;            // front row 0, column r0
+    const int32_t *rw = rows + t.rofs;
+    const int xr = tid % NR, xi0 = tid / NR;           // this thread's x entries: rows xi0 + (kBS/NR) j, rhs xr
+    // tile e: update rows [ns + 64e, ...) for e < nchunk, else panel q = np-1-(e-nchunk)
+    auto tile_row0 = [&](int e) -> int64_t {
+        return e < nchunk ? t.ns + (int64_t)kTile * e : (int64_t)kPanel * (t.np - 1 - (e - nchunk));
+    };
+    auto tile_rows = [&](int e) -> int {
+        const int64_t r0 = tile_row0(e);
+        return (int)min<int64_t>(kTile, (e < nchunk ? t.ms : t.ns) - r0);
+    };
+    T lv[16], inv[16], zv[NR];
+    T xv[XQ];
+    int32_t ri[XQ];
+    load_cols16(inv, Linv + t.iofs, nc, nc, nc, lane, w);
+    {
+        const T *zp = Z + (t.c0 + t.r0 + min(lane, nc - 1)) * nrhs;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) zv[r] = zp[min(r, nrhs - 1)];
+    }
+    if (t.parent >= 0) {
+        if (tid == 0) ok = wait_at_least(&xdone[t.parent], t.pnp, abort_flag);
+        __syncthreads();
+        if (!ok) return;
+    }
+    if (ntile > 0) load_cols16(lv, Lc + tile_row0(0), t.ms, tile_rows(0), nc, lane, w);
+    if (nchunk > 0) {
+        const int nr0 = tile_rows(0);
+#pragma unroll
+        for (int j = 0; j < XQ; ++j) {
+            const int i = xi0 + (kBS / NR) * j;
+            const int32_t g = rw[t.ns + min(i, nr0 - 1)];
+            xv[j] = ld_dev(X + (int64_t)g * nrhs + min(xr, nrhs - 1));
+        }
+        if (nchunk > 1) {
+            const int nr1 = tile_rows(1);
+#pragma unroll
+            for (int j = 0; j < XQ; ++j) ri[j] = rw[t.ns + kTile + min(xi0 + (kBS / NR) * j, nr1 - 1)];
+        }
+    }
+    T acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = T(0);
+    for (int e = 0; e < ntile; ++e) {
+        const int nre = tile_rows(e);
+        if (e >= nchunk) {
+            const int q = t.np - 1 - (e - nchunk);
+            if (tid == 0) ok = wait_at_least(&xdone[t.s], t.np - q, abort_flag);
+            __syncthreads();
+            if (!ok) return;
+            load_vec<T, NR>(xs, X + (t.c0 + (int64_t)kPanel * q) * nrhs, nre, nrhs, tid);
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < XQ; ++j) {
+                const int i = xi0 + (kBS / NR) * j;
+                xs[i][xr] = (i < nre && xr < nrhs) ? xv[j] : T(0);
+            }
+        }
+        stage_tile(Lt, lv, nre, nc, lane, w);
+        __syncthreads();
+        if (e + 1 < ntile) load_cols16(lv, Lc + tile_row0(e + 1), t.ms, tile_rows(e + 1), nc, lane, w);
+        if (e + 1 < nchunk) {
+#pragma unroll
+            for (int j = 0; j < XQ; ++j) xv[j] = ld_dev(X + (int64_t)ri[j] * nrhs + min(xr, nrhs - 1));
+            if (e + 2 < nchunk) {
+                const int nr2 = tile_rows(e + 2);
+#pragma unroll
+                for (int j = 0; j < XQ; ++j)
+                    ri[j] = rw[t.ns + (int64_t)kTile * (e + 2) + min(xi0 + (kBS / NR) * j, nr2 - 1)];
+            }
+        }
+        bwd_mul<T, NR>(acc, Lt, xs, lane, w);
+    }
+    if (w > 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
+    }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const T v = zv[r] - (((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
+            xs[lane][r] = (lane < nc && r < nrhs) ? v : T(0);
+        }
+    }
+    stage_tile(Lt, inv, nc, nc, lane, w);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = T(0);
+    bwd_mul<T, NR>(acc, Lt, xs, lane, w);
+    __syncthreads();
+    if (w > 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
+    }
+    __syncthreads();
+    if (w == 0) {
+        if (lane < nc) {
+            T *dst = X + (t.c0 + t.r0 + lane) * nrhs;
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (r < nrhs) st_dev(dst + r, ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
+        }
+        complete_stores();
+        if (lane == 0) __hip_atomic_fetch_add(&xdone[t.s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// inverse of each panel's diagonal block (one wavefront per backward task):
+// lane c computes column c of L_pp^{-1} by forward substitution, L_pp read as
+// broadcasts from LDS; stored column-major with leading dimension b
+template <class T>
+__global__ void __launch_bounds__(64) diag_inverse_kernel(const FlowTask *__restrict__ tasks, const T *__restrict__ L,
+                                                          T *__restrict__ Linv) {
+    __shared__ __attribute__((aligned(16))) T D[kPanel][kPanel + 4];
+    const FlowTask t = tasks[blockIdx.x];
+    const int b = t.nr, lane = threadIdx.x;
+    const int64_t k = t.r0, ms = t.ms;
+    const T *Ls = L + t.lofs;
+    const int li = min(lane, b - 1);
+#pragma unroll 8
+    for (int j = 0; j < kPanel; ++j) {
+        const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
+        D[lane][j] = (lane < b && j <= lane) ? v : (lane == j ? T(1) : T(0));
+    }
+    __syncthreads();
+    T x[kPanel];
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) {
+        T s = i == lane ? T(1) : T(0);
+#pragma unroll
+        for (int j = 0; j < i; ++j) s -= D[i][j] * x[j];
+        x[i] = s / D[i][i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) D[i][lane] = x[i];
+    __syncthreads();
+    T *dst = Linv + t.iofs;
+    if (lane < b)
+        for (int c = 0; c < b; ++c) dst[lane + (int64_t)b * c] = D[lane][c];
 }
 
 // ---------------------------------------------------------- whole-pivot-block solves
@@ -1459,6 +1846,33 @@ hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *
     return hipGetLastError();
 }
 
+// y = (L L')^{-1} y by the dataflow kernels; cnt: 3 * ns + 1 counters (the abort
+// flag, last, is left alone: it is cleared once per refinement call)
+template <class T, int NR>
+hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_factor *F, const T *Y, T *W, T *Z, T *X,
+                         int nrhs, int32_t *cnt) {
+    const SymbolicHost &H = S->H;
+    hipStream_t st = ctx->stream;
+    const int64_t nsn = H.ns;
+    const T *L = (const T *)F->d_L, *Linv = (const T *)F->d_Linv;
+    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(3 * nsn)), dim3(kBS), 0, st, cnt, 3 * nsn);
+    if (!H.flow_f.empty())
+        hipLaunchKernelGGL((fwd_flow_kernel<T, NR>), dim3((unsigned)H.flow_f.size()), dim3(kBS), 0, st, S->d_flow_f,
+                           S->d_flow_gb, S->d_relmap, L, Linv, Y, W, Z, nrhs, cnt, nsn);
+    if (!H.flow_b.empty())
+        hipLaunchKernelGGL((bwd_flow_kernel<T, NR>), dim3((unsigned)H.flow_b.size()), dim3(kBS), 0, st, S->d_flow_b,
+                           S->d_rows, L, Linv, (const T *)Z, X, nrhs, cnt, nsn);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t flow_solve(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_factor *F, const T *Y, T *W, T *Z, T *X,
+                      int nrhs, int32_t *cnt) {
+    if (nrhs <= 4) return flow_solve_nr<T, 4>(ctx, S, F, Y, W, Z, X, nrhs, cnt);
+    if (nrhs <= 8) return flow_solve_nr<T, 8>(ctx, S, F, Y, W, Z, X, nrhs, cnt);
+    return flow_solve_nr<T, 16>(ctx, S, F, Y, W, Z, X, nrhs, cnt);
+}
+
 // y = (L L')^{-1} y for nrhs <= 16 interleaved right-hand sides (Y in, X out)
 template <class T>
 hipError_t solve_t(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, T *Pp, int nrhs) {
@@ -1555,6 +1969,9 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_g_boff, H.g_boff, st);
     if (e == hipSuccess) e = up(S, &S->d_ea_boff, H.ea_boff, st);
     if (e == hipSuccess) e = up(S, &S->d_bwd, H.bwd_tiles, st);
+    if (e == hipSuccess) e = up(S, &S->d_flow_f, H.flow_f, st);
+    if (e == hipSuccess) e = up(S, &S->d_flow_b, H.flow_b, st);
+    if (e == hipSuccess) e = up(S, &S->d_flow_gb, H.flow_gb, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         jfem_symbolic_free(S);
@@ -1612,6 +2029,16 @@ static int do_numeric(jfem_ctx *ctx, jfem_factor *F, const jfem_matrix *K, int64
                        ? numeric<double>(ctx, S, K, (double *)F->d_L, (double *)F->d_U)
                        : numeric<float>(ctx, S, K, (float *)F->d_L, (float *)F->d_U);
     JFEM_CHECK_HIP(ctx, e);
+    if (!S->H.flow_b.empty()) {
+        const dim3 g((unsigned)S->H.flow_b.size());
+        if (F->precision == 64)
+            hipLaunchKernelGGL(diag_inverse_kernel<double>, g, dim3(64), 0, ctx->stream, S->d_flow_b,
+                               (const double *)F->d_L, (double *)F->d_Linv);
+        else
+            hipLaunchKernelGGL(diag_inverse_kernel<float>, g, dim3(64), 0, ctx->stream, S->d_flow_b,
+                               (const float *)F->d_L, (float *)F->d_Linv);
+        JFEM_CHECK_HIP(ctx, hipGetLastError());
+    }
     timer_stop(ctx, FAM_FACTOR, 1);
     int64_t st[2];
     JFEM_CHECK_HIP(ctx, hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
@@ -1635,11 +2062,12 @@ int jfem_factor_create(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix 
     const size_t es = precision == 64 ? 8 : 4;
     hipError_t e = hipMalloc(&F->d_L, std::max<int64_t>(S->H.L_entries, 1) * es);
     if (e == hipSuccess) e = hipMalloc(&F->d_U, std::max<int64_t>(S->H.U_entries, 1) * es);
+    if (e == hipSuccess) e = hipMalloc(&F->d_Linv, std::max<int64_t>(S->H.inv_entries, 1) * es);
     if (e != hipSuccess) {
         jfem_factor_free(F);
         return fail(ctx, JFEM_ERR_OOM, "factor storage");
     }
-    F->bytes = (S->H.L_entries + S->H.U_entries) * (int64_t)es;
+    F->bytes = (S->H.L_entries + S->H.U_entries + S->H.inv_entries) * (int64_t)es;
     int st = do_numeric(ctx, F, K, bad_column);
     if (st != JFEM_OK) {
         jfem_factor_free(F);
@@ -1658,6 +2086,7 @@ int jfem_factor_free(jfem_factor *F) {
     (void)hipSetDevice(F->device);
     if (F->d_L) (void)hipFree(F->d_L);
     if (F->d_U) (void)hipFree(F->d_U);
+    if (F->d_Linv) (void)hipFree(F->d_Linv);
     delete F;
     return JFEM_OK;
 }
@@ -1725,7 +2154,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     const size_t oY = take(nv * sizeof(T)), oW = take(std::max<int64_t>(H.upd_rows, 1) * NR * sizeof(T));
     const size_t oZs = take(nv * sizeof(T)), oXs = take(nv * sizeof(T));
-    const size_t oPp = take(std::max<int64_t>(H.bslots_max, 1) * kTile * NR * sizeof(T));
+    const size_t oCnt = take((3 * H.ns + 1) * sizeof(int32_t));
     const size_t oXh = take(nv * 8), oXl = take(nv * 8), oR = take(nv * 8), oB = take(nv * 8), oD = take(nv * 8);
     const size_t oSv = take(nv * 8), oPv = take(nv * 8), oQv = take(nv * 8), oZv = take(nv * 8);
     const size_t oXgh = take(ng * 8), oXgl = take(ng * 8), oPg = take(ng * 8);
@@ -1734,7 +2163,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     JFEM_CHECK_HIP(ctx, ensure_scratch(ctx, off));
     char *base = (char *)ctx->scratch;
     T *Y = (T *)(base + oY), *W = (T *)(base + oW), *Zs = (T *)(base + oZs), *Xs = (T *)(base + oXs);
-    T *Pp = (T *)(base + oPp);
+    int32_t *cnt = (int32_t *)(base + oCnt);
     double *Xh = (double *)(base + oXh), *Xl = (double *)(base + oXl), *R = (double *)(base + oR);
     double *Bp = (double *)(base + oB), *Dd = (double *)(base + oD), *Sv = (double *)(base + oSv);
     double *Pv = (double *)(base + oPv), *Qv = (double *)(base + oQv), *Zv = (double *)(base + oZv);
@@ -1748,6 +2177,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, xgh, ng);
     hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, xgl, ng);
     hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, pg, ng);
+    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(1), dim3(64), 0, st, cnt + 3 * H.ns, (int64_t)1);
     for (int r = 0; r < NR; ++r) { iters[r] = 0; relres[r] = 0.0; }
     if (relres_out)
         for (int r = 0; r < NR; ++r) relres_out[r] = 0.0;
@@ -1771,7 +2201,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
         auto prec = [&](const double *sv, double *z) -> hipError_t {
             timer_start(ctx);
             hipLaunchKernelGGL(to_t_kernel<T>, gv, blk, 0, st, nv, sv, Y);
-            hipError_t e = solve_t<T>(ctx, S, (const T *)F->d_L, Y, W, Zs, Xs, Pp, NR);
+            hipError_t e = flow_solve<T>(ctx, S, F, Y, W, Zs, Xs, NR, cnt);
             hipLaunchKernelGGL(from_t_kernel<T>, gv, blk, 0, st, nv, (const T *)Xs, z);
             timer_stop(ctx, FAM_SOLVE, 1);
             return e;
@@ -1861,8 +2291,11 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     }
     hipLaunchKernelGGL(export_kernel, gg, blk, 0, st, ndof, NR, xgh, x_dev);
     if (xlo_dev) hipLaunchKernelGGL(export_kernel, gg, blk, 0, st, ndof, NR, xgl, xlo_dev);
+    int32_t aborted = 0;
     JFEM_CHECK_HIP(ctx, hipGetLastError());
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(&aborted, cnt + 3 * H.ns, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     JFEM_CHECK_HIP(ctx, hipStreamSynchronize(st));
+    if (aborted) return fail(ctx, JFEM_ERR_HIP, "triangular solve: a dependency wait timed out");
     if (!all_ok) return fail(ctx, JFEM_ERR_NOT_CONVERGED, "refinement did not reach the requested residual");
     return JFEM_OK;
 }

@@ -628,6 +628,80 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
     S.ea_boff[S.etasks.size()] = (int64_t)(S.ea_bounds.size() / 4);
     S.bstep_ptr[nlev] = (int64_t)S.bstep_off.size();
     S.bstep_off.push_back((int64_t)S.btasks.size());
+    // dataflow solve lists
+    auto npan = [&](int64_t s) { return (int32_t)((S.ncol[s] + kPanel - 1) / kPanel); };
+    auto nupd = [&](int64_t s) { return (int32_t)((S.m[s] - S.ncol[s] + kTile - 1) / kTile); };
+    S.iofs.assign(nsn + 1, 0);
+    for (int64_t k = 0; k < nsn; ++k) {
+        int64_t e = 0;
+        for (int64_t c = 0; c < S.ncol[k]; c += kPanel) {
+            const int64_t b = std::min<int64_t>(kPanel, S.ncol[k] - c);
+            e += b * b;
+        }
+        S.iofs[k + 1] = S.iofs[k] + e;
+    }
+    S.inv_entries = S.iofs[nsn];
+    std::vector<int32_t> need(nsn, 0);
+    for (int64_t k = 0; k < nsn; ++k)
+        for (int32_t q = S.child_ptr[k]; q < S.child_ptr[k + 1]; ++q) need[k] += nupd(S.child[q]);
+    auto base = [&](int32_t s) {
+        FlowTask t{};
+        t.s = s;
+        t.np = npan(s);
+        t.need = need[s];
+        t.parent = S.parent[s];
+        t.pnp = t.parent >= 0 ? npan(t.parent) : 0;
+        t.nch = S.child_ptr[s + 1] - S.child_ptr[s];
+        t.ns = S.ncol[s]; t.ms = S.m[s]; t.lofs = S.lofs[s]; t.c0 = S.col0[s];
+        t.relofs = S.relofs[s]; t.rofs = S.rofs[s];
+        return t;
+    };
+    S.flow_f.clear(); S.flow_b.clear(); S.flow_gb.clear();
+    auto add_fwd = [&](FlowTask t) {
+        t.gb = (int64_t)S.flow_gb.size();
+        for (int32_t q = S.child_ptr[t.s]; q < S.child_ptr[t.s + 1]; ++q) {
+            const int32_t c = S.child[q];
+            const int32_t *rel = S.relmap.data() + S.relofs[c];
+            const int64_t nu = S.m[c] - S.ncol[c];
+            FlowChild f;
+            f.relofs = S.relofs[c];
+            f.i0 = (int32_t)(std::lower_bound(rel, rel + nu, (int64_t)t.r0) - rel);
+            f.i1 = (int32_t)(std::lower_bound(rel, rel + nu, (int64_t)t.r0 + t.nr) - rel);
+            S.flow_gb.push_back(f);
+        }
+        S.flow_f.push_back(t);
+    };
+    for (int32_t l = 0; l < nlev; ++l)
+        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+            const int32_t s = S.level_sn[q];
+            FlowTask t = base(s);
+            for (int32_t p = 0; p < t.np; ++p) {
+                t.p = p;
+                t.r0 = p * kPanel;
+                t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
+                t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
+                add_fwd(t);
+            }
+            t.iofs = 0;
+            for (int32_t u = 0; u < nupd(s); ++u) {
+                t.p = -1 - u;
+                t.r0 = (int32_t)(S.ncol[s] + (int64_t)u * kTile);
+                t.nr = (int32_t)std::min<int64_t>(kTile, S.m[s] - t.r0);
+                add_fwd(t);
+            }
+        }
+    for (int32_t l = nlev - 1; l >= 0; --l)
+        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+            const int32_t s = S.level_sn[q];
+            FlowTask t = base(s);
+            for (int32_t p = t.np - 1; p >= 0; --p) {
+                t.p = p;
+                t.r0 = p * kPanel;
+                t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
+                t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
+                S.flow_b.push_back(t);
+            }
+        }
     return 0;
 }
 

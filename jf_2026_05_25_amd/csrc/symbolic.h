@@ -21,6 +21,27 @@ struct StepTask {
     int64_t ns, ms, lofs, c0, relofs, pad;
 };             // parent, front columns [c0, c1), rows [r0, r0 + kEaRows)
 
+// Dataflow solve task: one workgroup each; a list is in dependency order, so a
+// task waits only on tasks listed before it.  Forward, p >= 0: pivot rows
+// [r0, r0+nr) (panel p) of the front of s; p < 0: update rows [r0, r0+nr).
+// Backward: columns [r0, r0+nr) (panel p) of s.  The supernode's metadata is
+// inlined so that a task starts without dependent loads.
+struct FlowTask {
+    int32_t s, p, r0, nr;
+    int32_t np, need;               // panels of s; update blocks of s's children
+    int32_t parent, pnp;            // parent supernode (-1 for a root) and its panels
+    int32_t nch, pad;               // children of s
+    int64_t ns, ms, lofs, c0, relofs, rofs;
+    int64_t iofs;                   // inverse of this task's diagonal block (pivot tasks)
+    int64_t gb;                     // first FlowChild entry of this task (forward)
+};
+// per (forward task, child of its supernode), in child order: the child's update
+// rows [i0, i1) that land in the task's rows, and the child's update-vector offset
+struct FlowChild {
+    int64_t relofs;
+    int32_t i0, i1;
+};
+
 struct SymbolicHost {
     int64_t ndof = 0, n = 0;        // all DOFs, free DOFs
     int64_t ns = 0;                 // supernodes
@@ -75,6 +96,14 @@ struct SymbolicHost {
     std::vector<int64_t> level_maxns;
     std::vector<int64_t> bstep_ptr;          // per level: first backward step
     std::vector<int64_t> bstep_off;          // per backward step: offset into btasks
+    // dataflow solves: forward list (levels ascending; per supernode its panels,
+    // then its update blocks of kTile rows), backward list (levels descending; per
+    // supernode its panels from the last); per supernode the offset of its
+    // inverse diagonal blocks (panel p: b x b entries at iofs[s] + kPanel^2 p)
+    std::vector<FlowTask> flow_f, flow_b;
+    std::vector<FlowChild> flow_gb;
+    std::vector<int64_t> iofs;
+    int64_t inv_entries = 0;
     int64_t L_entries = 0, U_entries = 0, upd_rows = 0;
     double flops = 0;
     double update_flops = 0;   // 2 x multiply-adds of the update tiles (lower triangles)

@@ -304,6 +304,188 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
         }
     }
     if (conflict) return 5;
+    // Dataflow solves: the tasks of each list run sequentially in list order, as
+    // the kernels compute them (four partial sums per row in wave order, inverse
+    // diagonal blocks).  Each wait must already be satisfied (a task may wait only
+    // on tasks listed before it: returns 9), every index a kernel forms -- clamped
+    // loads included -- must lie inside its array or block (10), every vector entry
+    // is written once and read only after it is written (11), children's
+    // rows land inside the task's rows (8), and the result must equal the level
+    // solve's (12).
+    {
+        std::vector<double> Linv(std::max<int64_t>(S.inv_entries, 1), 0.0);
+        for (const FlowTask &t : S.flow_b) {
+            const int b = t.nr;
+            const int64_t k = t.r0, ms = t.ms;
+            if (b < 1 || b > kPanel || t.iofs < 0 || t.iofs + (int64_t)b * b > S.inv_entries) return 10;
+            if (k + b > t.ns || t.ns > ms) return 10;
+            for (int c = 0; c < b; ++c) {
+                std::vector<double> xc(b, 0.0);
+                for (int i = 0; i < b; ++i) {
+                    double v = i == c ? 1.0 : 0.0;
+                    for (int j = 0; j < i; ++j) v -= L[t.lofs + (k + i) + ms * (k + j)] * xc[j];
+                    xc[i] = v / L[t.lofs + (k + i) + ms * (k + i)];
+                }
+                for (int i = 0; i < b; ++i) Linv[t.iofs + i + (int64_t)b * c] = xc[i];
+            }
+        }
+        const int64_t nsn = S.ns;
+        std::vector<int32_t> ready(nsn, 0), zdone(nsn, 0), xdone(nsn, 0);
+        std::vector<double> Y2(n * nrhs), W2(nw * nrhs, 0.0), Z2(n * nrhs, 0.0), X2(n * nrhs, 0.0);
+        std::vector<char> wW2(nw, 0), wZ2(n, 0), wX2(n, 0);
+        for (int64_t i = 0; i < n; ++i)
+            for (int r = 0; r < nrhs; ++r) Y2[i * nrhs + r] = b[S.perm[i] + ndof * r];
+        // an L index formed for block (s, rows [r0, r0+nr), cols [c, c+nc)), clamped as load_cols16 does
+        auto Lblock_ok = [&](const FlowTask &t, int64_t r0, int nr, int64_t c, int nc) {
+            if (nr < 1 || nc < 1) return false;
+            const int64_t lo = t.lofs + r0 + t.ms * c, hi = lo + (nr - 1) + t.ms * (nc - 1);
+            return r0 >= 0 && r0 + nr <= t.ms && c >= 0 && c + nc <= t.ns && lo >= t.lofs && hi < t.lofs + t.ms * t.ns &&
+                   hi < S.L_entries;
+        };
+        for (size_t e = 0; e < S.flow_f.size(); ++e) {
+            const FlowTask &t = S.flow_f[e];
+            const bool pivot = t.p >= 0;
+            const int kend = pivot ? t.p : t.np;
+            const int nr = t.nr;
+            if (t.s < 0 || t.s >= nsn || nr < 1 || nr > kTile) return 10;
+            if (pivot && (t.r0 != kPanel * t.p || t.r0 + nr > t.ns)) return 10;
+            if (!pivot && (t.r0 < t.ns || t.r0 + nr > t.ms || t.parent < 0)) return 10;
+            std::vector<double> vs(kTile * nrhs, 0.0);
+            if (pivot)
+                for (int i = 0; i < nr; ++i)
+                    for (int r = 0; r < nrhs; ++r) vs[i * nrhs + r] = Y2[(t.c0 + t.r0 + i) * nrhs + r];
+            if (t.need > 0) {
+                if (ready[t.s] < t.need) return 9;
+                if (t.gb < 0 || t.gb + t.nch > (int64_t)S.flow_gb.size()) return 10;
+                for (int j = 0; j < t.nch; ++j) {
+                    const FlowChild ch = S.flow_gb[t.gb + j];
+                    for (int64_t i = ch.i0; i < ch.i1; ++i) {
+                        if (ch.relofs + i < 0 || ch.relofs + i >= (int64_t)S.relmap.size() || ch.relofs + i >= nw) return 10;
+                        const int64_t row = S.relmap[ch.relofs + i] - t.r0;
+                        if (row < 0 || row >= nr) return 8;
+                        if (!wW2[ch.relofs + i]) return 11;
+                        for (int r = 0; r < nrhs; ++r) vs[row * nrhs + r] += W2[(ch.relofs + i) * nrhs + r];
+                    }
+                }
+            }
+            std::vector<double> acc(4 * kTile * nrhs, 0.0);
+            for (int k = 0; k < kend; ++k) {
+                if (zdone[t.s] < k + 1) return 9;
+                const int64_t col = (int64_t)kPanel * k;
+                const int bk = (int)std::min<int64_t>(kPanel, t.ns - col);
+                if (!Lblock_ok(t, t.r0, nr, col, bk)) return 10;
+                for (int c = 0; c < bk; ++c)
+                    if (!wZ2[t.c0 + col + c]) return 11;
+                for (int w = 0; w < 4; ++w)
+                    for (int i = 0; i < nr; ++i)
+                        for (int c = 16 * w; c < std::min(16 * w + 16, bk); ++c)
+                            for (int r = 0; r < nrhs; ++r)
+                                acc[(w * kTile + i) * nrhs + r] +=
+                                    L[t.lofs + (t.r0 + i) + t.ms * (col + c)] * Z2[(t.c0 + col + c) * nrhs + r];
+            }
+            std::vector<double> v(kTile * nrhs, 0.0);
+            for (int i = 0; i < nr; ++i)
+                for (int r = 0; r < nrhs; ++r) {
+                    const double *a = &acc[i * nrhs + r];
+                    const int64_t st = (int64_t)kTile * nrhs;
+                    v[i * nrhs + r] = vs[i * nrhs + r] - (((a[0] + a[st]) + a[2 * st]) + a[3 * st]);
+                }
+            if (!pivot) {
+                for (int i = 0; i < nr; ++i) {
+                    const int64_t wi = t.relofs + t.r0 - t.ns + i;
+                    if (wi < 0 || wi >= nw || wW2[wi]) return 11;
+                    wW2[wi] = 1;
+                    for (int r = 0; r < nrhs; ++r) W2[wi * nrhs + r] = v[i * nrhs + r];
+                }
+                ready[t.parent] += 1;
+                continue;
+            }
+            if (t.iofs < 0 || t.iofs + (int64_t)nr * nr > S.inv_entries) return 10;
+            if (zdone[t.s] != t.p) return 9;
+            std::fill(acc.begin(), acc.end(), 0.0);
+            for (int w = 0; w < 4; ++w)
+                for (int i = 0; i < nr; ++i)
+                    for (int c = 16 * w; c < std::min(16 * w + 16, nr); ++c)
+                        for (int r = 0; r < nrhs; ++r)
+                            acc[(w * kTile + i) * nrhs + r] += Linv[t.iofs + i + (int64_t)nr * c] * v[c * nrhs + r];
+            for (int i = 0; i < nr; ++i) {
+                const int64_t zi = t.c0 + t.r0 + i;
+                if (wZ2[zi]) return 11;
+                wZ2[zi] = 1;
+                for (int r = 0; r < nrhs; ++r) {
+                    const double *a = &acc[i * nrhs + r];
+                    const int64_t st = (int64_t)kTile * nrhs;
+                    Z2[zi * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                }
+            }
+            zdone[t.s] = t.p + 1;
+        }
+        for (int64_t i = 0; i < n; ++i)
+            if (!wZ2[i]) return 11;
+        for (size_t e = 0; e < S.flow_b.size(); ++e) {
+            const FlowTask &t = S.flow_b[e];
+            const int nc = t.nr;
+            if (t.parent >= 0 && xdone[t.parent] < t.pnp) return 9;
+            if (t.parent >= 0 && t.pnp != (int32_t)((S.ncol[t.parent] + kPanel - 1) / kPanel)) return 10;
+            std::vector<double> acc(4 * kTile * nrhs, 0.0);
+            const int64_t nu = t.ms - t.ns;
+            const int nchunk = (int)((nu + kTile - 1) / kTile);
+            const int ntile = nchunk + (t.np - 1 - t.p);
+            for (int e2 = 0; e2 < ntile; ++e2) {
+                const bool upd = e2 < nchunk;
+                const int q = upd ? -1 : t.np - 1 - (e2 - nchunk);
+                const int64_t row0 = upd ? t.ns + (int64_t)kTile * e2 : (int64_t)kPanel * q;
+                const int nre = (int)std::min<int64_t>(kTile, (upd ? t.ms : t.ns) - row0);
+                if (!upd && xdone[t.s] < t.np - q) return 9;
+                if (!Lblock_ok(t, row0, nre, t.r0, nc)) {
+                    // update rows lie below the pivot columns: check against the whole front
+                    const int64_t lo = t.lofs + row0 + t.ms * t.r0, hi = lo + (nre - 1) + t.ms * (nc - 1);
+                    if (!(row0 >= 0 && row0 + nre <= t.ms && hi < t.lofs + t.ms * t.ns)) return 10;
+                }
+                for (int i = 0; i < nre; ++i) {
+                    if (upd && (t.rofs + row0 + i >= (int64_t)S.rows.size())) return 10;
+                    const int64_t g = upd ? S.rows[t.rofs + row0 + i] : t.c0 + row0 + i;
+                    if (g < 0 || g >= n || !wX2[g]) return 11;
+                    for (int c = 0; c < nc; ++c) {
+                        const int w = i / 16;
+                        for (int r = 0; r < nrhs; ++r)
+                            acc[(w * kTile + c) * nrhs + r] += L[t.lofs + (row0 + i) + t.ms * (t.r0 + c)] * X2[g * nrhs + r];
+                    }
+                }
+            }
+            std::vector<double> v(kTile * nrhs, 0.0);
+            for (int c = 0; c < nc; ++c)
+                for (int r = 0; r < nrhs; ++r) {
+                    const double *a = &acc[c * nrhs + r];
+                    const int64_t st = (int64_t)kTile * nrhs;
+                    v[c * nrhs + r] = Z2[(t.c0 + t.r0 + c) * nrhs + r] - (((a[0] + a[st]) + a[2 * st]) + a[3 * st]);
+                }
+            std::fill(acc.begin(), acc.end(), 0.0);
+            for (int i = 0; i < nc; ++i)
+                for (int c = 0; c < nc; ++c)
+                    for (int r = 0; r < nrhs; ++r)
+                        acc[((i / 16) * kTile + c) * nrhs + r] += Linv[t.iofs + i + (int64_t)nc * c] * v[i * nrhs + r];
+            if (xdone[t.s] != t.np - 1 - t.p) return 9;
+            for (int c = 0; c < nc; ++c) {
+                const int64_t xi = t.c0 + t.r0 + c;
+                if (wX2[xi]) return 11;
+                wX2[xi] = 1;
+                for (int r = 0; r < nrhs; ++r) {
+                    const double *a = &acc[c * nrhs + r];
+                    const int64_t st = (int64_t)kTile * nrhs;
+                    X2[xi * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                }
+            }
+            xdone[t.s] += 1;
+        }
+        double dmax = 0, xmax = 0;
+        for (int64_t i = 0; i < n * nrhs; ++i) {
+            dmax = std::max(dmax, std::abs(X2[i] - X[i]));
+            xmax = std::max(xmax, std::abs(X[i]));
+        }
+        if (!(dmax <= 1e-8 * xmax)) return 12;
+        X.swap(X2);
+    }
     for (int64_t i = 0; i < ndof * nrhs; ++i) x[i] = 0;
     for (int64_t i = 0; i < n; ++i)
         for (int r = 0; r < nrhs; ++r) x[S.perm[i] + ndof * r] = X[i * nrhs + r];
