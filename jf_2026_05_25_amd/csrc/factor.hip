@@ -32,7 +32,7 @@ struct jfem_symbolic {
     PanelTask *d_ptasks = nullptr, *d_btasks = nullptr, *d_bwd = nullptr, *d_bred = nullptr, *d_gtasks = nullptr;
     PanelTask *d_ftiles = nullptr;
     StepTask *d_fsteps = nullptr, *d_bsteps = nullptr;
-    FlowTask *d_flow_f = nullptr, *d_flow_b = nullptr;
+    FlowTask *d_flow_f = nullptr, *d_flow_b = nullptr, *d_flow_p = nullptr;
     FlowChild *d_flow_gb = nullptr;
     int32_t *d_g_bounds = nullptr;
     int64_t *d_g_boff = nullptr;
@@ -70,6 +70,12 @@ __device__ inline void report_min(int64_t *status, int64_t idx, int code) {
 }
 
 // ----------------------------------------------------------------- fill / scatter
+template <class T>
+__global__ void fill_value_kernel(T *__restrict__ p, int64_t n, T v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 template <class T>
 __global__ void fill_kernel(T *__restrict__ p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -732,9 +738,13 @@ __global__ void __launch_bounds__(kBS) bwd_step_kernel(const StepTask *__restric
 // would need (each of those walks the whole L2 of its XCD, and they serialise).
 // Waits are bounded (kWaitTicks of the 100 MHz wall clock): on expiry the abort
 // flag is raised, every waiting workgroup leaves, and the host reports an error.
-// Counters (int32, nsn each): ready[s] = finished update blocks of s's children,
-// zdone[s] = forward panels of s finished (they finish in order), xdone[s] =
-// backward panels of s finished (from the last); then the abort flag.
+// Solved values need no counter: Z and X are filled with a signalling-NaN mark
+// before the solve (arithmetic never produces one), each entry is written once,
+// and a consumer polls the entries it needs until none holds the mark -- one
+// memory round trip per dependency instead of a counter and then the data.
+// Update vectors (W) are gathered by the parent from many child tasks and are
+// counted instead: ready[s] = finished update blocks of s's children (int32,
+// nsn of them), followed by the abort flag.
 // Every sum has a fixed order -- children in child order, panels in solve order,
 // the four waves' partial sums in wave order -- whatever the timing, so results
 // are bitwise reproducible.
@@ -747,6 +757,15 @@ template <class T>
 __device__ __forceinline__ T ld_dev(const T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 template <class T>
 __device__ __forceinline__ void st_dev(T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <class T>
+__host__ __device__ inline T pending_mark();
+template <>
+__host__ __device__ inline float pending_mark<float>() { return __builtin_bit_cast(float, 0x7FBADBADu); }
+template <>
+__host__ __device__ inline double pending_mark<double>() { return __builtin_bit_cast(double, 0x7FF4DEADBEEFDEADull); }
+__device__ __forceinline__ bool is_pending(float v) { return __builtin_bit_cast(uint32_t, v) == 0x7FBADBADu; }
+__device__ __forceinline__ bool is_pending(double v) { return __builtin_bit_cast(uint64_t, v) == 0x7FF4DEADBEEFDEADull; }
+
 // every memory operation of the wavefront issued so far has completed
 __device__ __forceinline__ void complete_stores() {
     asm volatile("" ::: "memory");
@@ -773,6 +792,26 @@ __device__ bool wait_at_least(const int32_t *ctr, int32_t target, int32_t *abort
     return true;
 }
 
+// re-load *p until it holds a published value; false on abort or time-out
+template <class T>
+__device__ bool await_value(T &v, const T *p, int32_t *abort_flag) {
+    if (!is_pending(v)) return true;
+    const long long t0 = wall_clock64();
+    int spins = 0;
+    do {
+        __builtin_amdgcn_s_sleep(2);
+        v = ld_dev(p);
+        if ((++spins & 15) == 0) {
+            if (ld_dev(abort_flag)) return false;
+            if (wall_clock64() - t0 > kWaitTicks) {
+                st_dev(abort_flag, 1);
+                return false;
+            }
+        }
+    } while (is_pending(v));
+    return true;
+}
+
 // thread (lane = row i, wave w) loads columns [16w, 16w+16) of its row of a
 // rows x cols block (column-major, leading dimension ld); indices are clamped
 // into the block, the caller masks what lies outside
@@ -785,15 +824,26 @@ __device__ __forceinline__ void load_cols16(T (&v)[16], const T *base, int64_t l
 
 // Vectors hold nrhs interleaved right-hand sides (stride nrhs); the kernels work
 // on NR >= nrhs of them (4, 8 or 16), the padding being zero.
-// zs[c][r] = src[c * nrhs + r] for c < cnt, r < nrhs, zero beyond (whole workgroup;
-// src was published by another task of the launch)
+// zs[c][r] = src[c * nrhs + r] for c < cnt, r < nrhs, zero beyond (whole workgroup),
+// once published by another task of the launch; sets *failed on abort/time-out
 template <class T, int NR>
-__device__ __forceinline__ void load_vec(T (*zs)[NR], const T *src, int cnt, int nrhs, int tid) {
+__device__ __forceinline__ void load_vec_wait(T (*zs)[NR], const T *src, int cnt, int nrhs, int tid, int32_t *abort_flag,
+                                              int *failed) {
+    constexpr int Q = kTile * NR / kBS;
+    T v[Q];
+    const T *p[Q];
 #pragma unroll
-    for (int j = 0; j < kTile * NR / kBS; ++j) {
+    for (int j = 0; j < Q; ++j) {
         const int q = tid + kBS * j, c = q / NR, r = q % NR;
-        const T v = ld_dev(src + (int64_t)min(c, cnt - 1) * nrhs + min(r, nrhs - 1));
-        zs[c][r] = (c < cnt && r < nrhs) ? v : T(0);
+        p[j] = src + (int64_t)min(c, cnt - 1) * nrhs + min(r, nrhs - 1);
+        v[j] = ld_dev(p[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+        const int q = tid + kBS * j, c = q / NR, r = q % NR;
+        const bool live = c < cnt && r < nrhs;
+        if (live && !await_value(v[j], p[j], abort_flag)) *failed = 1;
+        zs[c][r] = live ? v[j] : T(0);
     }
 }
 
@@ -810,37 +860,39 @@ __device__ __forceinline__ void fwd_mul(T (&acc)[NR], const T (&v)[16], const T 
 
 // forward: pivot task (s, p): z_p = L_pp^{-1} (y_p + children's contributions
 // - sum_{k<p} L[p rows, k] z_k), written to Z; update task: w = children's
-// contributions - sum_k L[rows, k] z_k, written to s's update vector in W.
+// contributions - sum_k L[rows, k] z_k, written to s's update vector in W;
+// whole-front task: both for a supernode with one panel (rows of the front
+// < kWholeRows), the update rows one per thread.
 template <class T, int NR>
 __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restrict__ tasks,
                                                        const FlowChild *__restrict__ gch,
                                                        const int32_t *__restrict__ relmap, const T *__restrict__ L,
                                                        const T *__restrict__ Linv, const T *__restrict__ Y, T *W, T *Z,
                                                        int nrhs, int32_t *cnt, int64_t nsn) {
-    __shared__ __attribute__((aligned(16))) T vs[kTile][NR];
-    __shared__ __attribute__((aligned(16))) T zs[kTile][NR];
+    __shared__ __attribute__((aligned(16))) T vs[kWholeRows][NR];
+    __shared__ __attribute__((aligned(16))) T zs[2][kTile][NR];   // double-buffered
     __shared__ __attribute__((aligned(16))) T red[3][kTile][NR];
-    __shared__ int ok;
+    __shared__ int failed;
     const FlowTask t = tasks[blockIdx.x];
-    int32_t *ready = cnt, *zdone = cnt + nsn, *abort_flag = cnt + 3 * nsn;
+    int32_t *ready = cnt, *abort_flag = cnt + nsn;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const bool pivot = t.p >= 0;
+    const bool pivot = t.p >= 0, whole = t.kind == 1;
     const int kend = pivot ? t.p : t.np;
     const T *Ls = L + t.lofs + t.r0;                   // front row r0, column 0
     T lv[16], inv[16];
     if (kend > 0) load_cols16(lv, Ls, t.ms, t.nr, (int)min<int64_t>(kPanel, t.ns), lane, w);
     if (pivot) load_cols16(inv, Linv + t.iofs, t.nr, t.nr, t.nr, lane, w);
 #pragma unroll
-    for (int j = 0; j < kTile * NR / kBS; ++j) {
+    for (int j = 0; j < kWholeRows * NR / kBS; ++j) {
         const int q = tid + kBS * j, i = q / NR, r = q % NR;
         const int64_t yi = pivot ? t.c0 + t.r0 + min(i, t.nr - 1) : 0;
         const T y = Y[yi * nrhs + min(r, nrhs - 1)];
         vs[i][r] = (pivot && i < t.nr && r < nrhs) ? y : T(0);
     }
+    if (tid == 0) failed = t.need > 0 && !wait_at_least(&ready[t.s], t.need, abort_flag);
+    __syncthreads();
+    if (failed) return;
     if (t.need > 0) {
-        if (tid == 0) ok = wait_at_least(&ready[t.s], t.need, abort_flag);
-        __syncthreads();
-        if (!ok) return;
         for (int j = 0; j < t.nch; ++j) {
             const FlowChild ch = gch[t.gb + j];
             const int64_t nq = (int64_t)(ch.i1 - ch.i0) * nrhs;
@@ -856,19 +908,70 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
     T acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
+    if (whole) {
+        // z = L_11^{-1} v[0:ns] (4 waves, 16 columns each), then w_i = v_i - L[i, :] z
+        // for the update rows, one row per thread
+#pragma unroll
+        for (int j = 0; j < kTile * NR / kBS; ++j) {
+            const int q = tid + kBS * j, i = q / NR, r = q % NR;
+            zs[0][i][r] = i < t.nr ? vs[i][r] : T(0);       // update rows of the front are not part of v
+        }
+        __syncthreads();
+        fwd_mul<T, NR>(acc, inv, zs[0], w);
+        if (w > 0) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
+        }
+        __syncthreads();
+        if (w == 0) {
+            T *dst = Z + (t.c0 + lane) * nrhs;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const T z = ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
+                zs[1][lane][r] = lane < t.nr ? z : T(0);
+                if (lane < t.nr && r < nrhs) st_dev(dst + r, z);
+            }
+        }
+        __syncthreads();
+        const int64_t i = t.ns + tid;
+        if (i < t.ms) {
+            const T *Li = L + t.lofs + i;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] = T(0);
+            for (int c0 = 0; c0 < t.nr; c0 += 4) {
+                T l4[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) l4[u] = Li[t.ms * min(c0 + u, t.nr - 1)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const T *z = zs[1][min(c0 + u, kTile - 1)];
+                    const T l = c0 + u < t.nr ? l4[u] : T(0);
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) acc[r] += l * z[r];
+                }
+            }
+            T *dst = W + (t.relofs + i - t.ns) * nrhs;
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (r < nrhs) st_dev(dst + r, vs[i][r] - acc[r]);
+        }
+        complete_stores();
+        __syncthreads();
+        if (tid == 0 && t.parent >= 0) __hip_atomic_fetch_add(&ready[t.parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
     for (int k = 0; k < kend; ++k) {
-        if (tid == 0) ok = wait_at_least(&zdone[t.s], k + 1, abort_flag);
-        __syncthreads();
-        if (!ok) return;
         const int64_t col = (int64_t)kPanel * k;
-        load_vec<T, NR>(zs, Z + (t.c0 + col) * nrhs, (int)min<int64_t>(kPanel, t.ns - col), nrhs, tid);
+        load_vec_wait<T, NR>(zs[k & 1], Z + (t.c0 + col) * nrhs, (int)min<int64_t>(kPanel, t.ns - col), nrhs, tid,
+                             abort_flag, &failed);
         __syncthreads();
+        if (failed) return;
         T cur[16];
 #pragma unroll
         for (int c = 0; c < 16; ++c) cur[c] = lv[c];
         if (k + 1 < kend)
             load_cols16(lv, Ls + t.ms * (col + kPanel), t.ms, t.nr, (int)min<int64_t>(kPanel, t.ns - col - kPanel), lane, w);
-        fwd_mul<T, NR>(acc, cur, zs, w);
+        fwd_mul<T, NR>(acc, cur, zs[k & 1], w);
     }
     if (w > 0) {
 #pragma unroll
@@ -895,12 +998,12 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
     }
     if (w == 0) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) zs[lane][r] = lane < t.nr ? v[r] : T(0);
+        for (int r = 0; r < NR; ++r) zs[kend & 1][lane][r] = lane < t.nr ? v[r] : T(0);
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
-    fwd_mul<T, NR>(acc, inv, zs, w);
+    fwd_mul<T, NR>(acc, inv, zs[kend & 1], w);
     if (w > 0) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
@@ -913,8 +1016,6 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
             for (int r = 0; r < NR; ++r)
                 if (r < nrhs) st_dev(dst + r, ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
         }
-        complete_stores();
-        if (lane == 0) __hip_atomic_store(&zdone[t.s], t.p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -939,8 +1040,9 @@ __device__ __forceinline__ void bwd_mul(T (&acc)[NR], const T (*Lt)[kTile + 1], 
 }
 
 // backward: task (s, p): x_p = L_pp^{-T} (z_p - sum over the rows i below the
-// panel of L[i, p]' x_i): first the update rows (their x final once the parent
-// is done), then the later panels of s from the last, each once it is done.
+// panel of L[i, p]' x_i): first the update rows (ancestors' values), tile by tile
+// from the last -- the order in which the ancestors' panels publish them -- then
+// the later panels of s from the last, each value awaited as it is published.
 template <class T, int NR>
 __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restrict__ tasks, const int32_t *__restrict__ rows,
                                                        const T *__restrict__ L, const T *__restrict__ Linv,
@@ -949,52 +1051,47 @@ __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restric
     __shared__ __attribute__((aligned(16))) T Lt[kTile][kTile + 1];
     __shared__ __attribute__((aligned(16))) T xs[kTile][NR];
     __shared__ __attribute__((aligned(16))) T red[3][kTile][NR];
-    __shared__ int ok;
+    __shared__ int failed;
     const FlowTask t = tasks[blockIdx.x];
-    int32_t *xdone = cnt + 2 * nsn, *abort_flag = cnt + 3 * nsn;
+    int32_t *abort_flag = cnt + nsn;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int nc = t.nr;
-    const int64_t nu = t.ms - t.ns;
-    const int nchunk = (int)((nu + kTile - 1) / kTile);
+    const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;
     const int ntile = nchunk + (t.np - 1 - t.p);
     const T *Lc = L + t.lofs + t.ms * t.r0;            // front row 0, column r0
     const int32_t *rw = rows + t.rofs;
     const int xr = tid % NR, xi0 = tid / NR;           // this thread's x entries: rows xi0 + (kBS/NR) j, rhs xr
-    // tile e: update rows [ns + 64e, ...) for e < nchunk, else panel q = np-1-(e-nchunk)
+    // tile e: update rows [u0 + 64(nchunk-1-e), ...) for e < nchunk, else panel q = np-1-(e-nchunk)
     auto tile_row0 = [&](int e) -> int64_t {
-        return e < nchunk ? t.ns + (int64_t)kTile * e : (int64_t)kPanel * (t.np - 1 - (e - nchunk));
+        return e < nchunk ? t.u0 + (int64_t)kTile * (nchunk - 1 - e) : (int64_t)kPanel * (t.np - 1 - (e - nchunk));
     };
     auto tile_rows = [&](int e) -> int {
         const int64_t r0 = tile_row0(e);
-        return (int)min<int64_t>(kTile, (e < nchunk ? t.ms : t.ns) - r0);
+        return (int)min<int64_t>(kTile, (e < nchunk ? (int64_t)t.u1 : t.ns) - r0);
     };
     T lv[16], inv[16], zv[NR];
     T xv[XQ];
-    int32_t ri[XQ];
+    int32_t ri[XQ], rc[XQ];                            // row indices of the next / current chunk
     load_cols16(inv, Linv + t.iofs, nc, nc, nc, lane, w);
     {
         const T *zp = Z + (t.c0 + t.r0 + min(lane, nc - 1)) * nrhs;
 #pragma unroll
         for (int r = 0; r < NR; ++r) zv[r] = zp[min(r, nrhs - 1)];
     }
-    if (t.parent >= 0) {
-        if (tid == 0) ok = wait_at_least(&xdone[t.parent], t.pnp, abort_flag);
-        __syncthreads();
-        if (!ok) return;
-    }
+    if (tid == 0) failed = 0;
     if (ntile > 0) load_cols16(lv, Lc + tile_row0(0), t.ms, tile_rows(0), nc, lane, w);
     if (nchunk > 0) {
         const int nr0 = tile_rows(0);
 #pragma unroll
         for (int j = 0; j < XQ; ++j) {
             const int i = xi0 + (kBS / NR) * j;
-            const int32_t g = rw[t.ns + min(i, nr0 - 1)];
-            xv[j] = ld_dev(X + (int64_t)g * nrhs + min(xr, nrhs - 1));
+            rc[j] = rw[tile_row0(0) + min(i, nr0 - 1)];
+            xv[j] = ld_dev(X + (int64_t)rc[j] * nrhs + min(xr, nrhs - 1));
         }
         if (nchunk > 1) {
             const int nr1 = tile_rows(1);
 #pragma unroll
-            for (int j = 0; j < XQ; ++j) ri[j] = rw[t.ns + kTile + min(xi0 + (kBS / NR) * j, nr1 - 1)];
+            for (int j = 0; j < XQ; ++j) ri[j] = rw[tile_row0(1) + min(xi0 + (kBS / NR) * j, nr1 - 1)];
         }
     }
     T acc[NR];
@@ -1002,31 +1099,34 @@ __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restric
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
     for (int e = 0; e < ntile; ++e) {
         const int nre = tile_rows(e);
+        __syncthreads();
         if (e >= nchunk) {
             const int q = t.np - 1 - (e - nchunk);
-            if (tid == 0) ok = wait_at_least(&xdone[t.s], t.np - q, abort_flag);
-            __syncthreads();
-            if (!ok) return;
-            load_vec<T, NR>(xs, X + (t.c0 + (int64_t)kPanel * q) * nrhs, nre, nrhs, tid);
+            load_vec_wait<T, NR>(xs, X + (t.c0 + (int64_t)kPanel * q) * nrhs, nre, nrhs, tid, abort_flag, &failed);
         } else {
-            __syncthreads();
 #pragma unroll
             for (int j = 0; j < XQ; ++j) {
                 const int i = xi0 + (kBS / NR) * j;
-                xs[i][xr] = (i < nre && xr < nrhs) ? xv[j] : T(0);
+                const bool live = i < nre && xr < nrhs;
+                if (live && !await_value(xv[j], X + (int64_t)rc[j] * nrhs + min(xr, nrhs - 1), abort_flag)) failed = 1;
+                xs[i][xr] = live ? xv[j] : T(0);
             }
         }
         stage_tile(Lt, lv, nre, nc, lane, w);
         __syncthreads();
+        if (failed) return;
         if (e + 1 < ntile) load_cols16(lv, Lc + tile_row0(e + 1), t.ms, tile_rows(e + 1), nc, lane, w);
         if (e + 1 < nchunk) {
 #pragma unroll
-            for (int j = 0; j < XQ; ++j) xv[j] = ld_dev(X + (int64_t)ri[j] * nrhs + min(xr, nrhs - 1));
+            for (int j = 0; j < XQ; ++j) {
+                rc[j] = ri[j];
+                xv[j] = ld_dev(X + (int64_t)rc[j] * nrhs + min(xr, nrhs - 1));
+            }
             if (e + 2 < nchunk) {
                 const int nr2 = tile_rows(e + 2);
 #pragma unroll
                 for (int j = 0; j < XQ; ++j)
-                    ri[j] = rw[t.ns + (int64_t)kTile * (e + 2) + min(xi0 + (kBS / NR) * j, nr2 - 1)];
+                    ri[j] = rw[tile_row0(e + 2) + min(xi0 + (kBS / NR) * j, nr2 - 1)];
             }
         }
         bwd_mul<T, NR>(acc, Lt, xs, lane, w);
@@ -1061,8 +1161,6 @@ __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restric
             for (int r = 0; r < NR; ++r)
                 if (r < nrhs) st_dev(dst + r, ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
         }
-        complete_stores();
-        if (lane == 0) __hip_atomic_fetch_add(&xdone[t.s], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1846,8 +1944,8 @@ hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *
     return hipGetLastError();
 }
 
-// y = (L L')^{-1} y by the dataflow kernels; cnt: 3 * ns + 1 counters (the abort
-// flag, last, is left alone: it is cleared once per refinement call)
+// y = (L L')^{-1} y by the dataflow kernels; cnt: ns + 1 counters (the abort flag,
+// last, is left alone: it is cleared once per refinement call)
 template <class T, int NR>
 hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_factor *F, const T *Y, T *W, T *Z, T *X,
                          int nrhs, int32_t *cnt) {
@@ -1855,7 +1953,10 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
     hipStream_t st = ctx->stream;
     const int64_t nsn = H.ns;
     const T *L = (const T *)F->d_L, *Linv = (const T *)F->d_Linv;
-    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(3 * nsn)), dim3(kBS), 0, st, cnt, 3 * nsn);
+    const int64_t nv = H.n * nrhs;
+    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(nsn)), dim3(kBS), 0, st, cnt, nsn);
+    hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nv)), dim3(kBS), 0, st, Z, nv, pending_mark<T>());
+    hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nv)), dim3(kBS), 0, st, X, nv, pending_mark<T>());
     if (!H.flow_f.empty())
         hipLaunchKernelGGL((fwd_flow_kernel<T, NR>), dim3((unsigned)H.flow_f.size()), dim3(kBS), 0, st, S->d_flow_f,
                            S->d_flow_gb, S->d_relmap, L, Linv, Y, W, Z, nrhs, cnt, nsn);
@@ -1971,6 +2072,7 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_bwd, H.bwd_tiles, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_f, H.flow_f, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_b, H.flow_b, st);
+    if (e == hipSuccess) e = up(S, &S->d_flow_p, H.flow_p, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_gb, H.flow_gb, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -2029,13 +2131,13 @@ static int do_numeric(jfem_ctx *ctx, jfem_factor *F, const jfem_matrix *K, int64
                        ? numeric<double>(ctx, S, K, (double *)F->d_L, (double *)F->d_U)
                        : numeric<float>(ctx, S, K, (float *)F->d_L, (float *)F->d_U);
     JFEM_CHECK_HIP(ctx, e);
-    if (!S->H.flow_b.empty()) {
-        const dim3 g((unsigned)S->H.flow_b.size());
+    if (!S->H.flow_p.empty()) {
+        const dim3 g((unsigned)S->H.flow_p.size());
         if (F->precision == 64)
-            hipLaunchKernelGGL(diag_inverse_kernel<double>, g, dim3(64), 0, ctx->stream, S->d_flow_b,
+            hipLaunchKernelGGL(diag_inverse_kernel<double>, g, dim3(64), 0, ctx->stream, S->d_flow_p,
                                (const double *)F->d_L, (double *)F->d_Linv);
         else
-            hipLaunchKernelGGL(diag_inverse_kernel<float>, g, dim3(64), 0, ctx->stream, S->d_flow_b,
+            hipLaunchKernelGGL(diag_inverse_kernel<float>, g, dim3(64), 0, ctx->stream, S->d_flow_p,
                                (const float *)F->d_L, (float *)F->d_Linv);
         JFEM_CHECK_HIP(ctx, hipGetLastError());
     }
@@ -2154,7 +2256,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     const size_t oY = take(nv * sizeof(T)), oW = take(std::max<int64_t>(H.upd_rows, 1) * NR * sizeof(T));
     const size_t oZs = take(nv * sizeof(T)), oXs = take(nv * sizeof(T));
-    const size_t oCnt = take((3 * H.ns + 1) * sizeof(int32_t));
+    const size_t oCnt = take((H.ns + 1) * sizeof(int32_t));
     const size_t oXh = take(nv * 8), oXl = take(nv * 8), oR = take(nv * 8), oB = take(nv * 8), oD = take(nv * 8);
     const size_t oSv = take(nv * 8), oPv = take(nv * 8), oQv = take(nv * 8), oZv = take(nv * 8);
     const size_t oXgh = take(ng * 8), oXgl = take(ng * 8), oPg = take(ng * 8);
@@ -2177,7 +2279,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, xgh, ng);
     hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, xgl, ng);
     hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, pg, ng);
-    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(1), dim3(64), 0, st, cnt + 3 * H.ns, (int64_t)1);
+    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(1), dim3(64), 0, st, cnt + H.ns, (int64_t)1);
     for (int r = 0; r < NR; ++r) { iters[r] = 0; relres[r] = 0.0; }
     if (relres_out)
         for (int r = 0; r < NR; ++r) relres_out[r] = 0.0;
@@ -2293,7 +2395,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     if (xlo_dev) hipLaunchKernelGGL(export_kernel, gg, blk, 0, st, ndof, NR, xgl, xlo_dev);
     int32_t aborted = 0;
     JFEM_CHECK_HIP(ctx, hipGetLastError());
-    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(&aborted, cnt + 3 * H.ns, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(&aborted, cnt + H.ns, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     JFEM_CHECK_HIP(ctx, hipStreamSynchronize(st));
     if (aborted) return fail(ctx, JFEM_ERR_HIP, "triangular solve: a dependency wait timed out");
     if (!all_ok) return fail(ctx, JFEM_ERR_NOT_CONVERGED, "refinement did not reach the requested residual");

@@ -130,6 +130,69 @@ struct ND {
 
 }  // namespace
 
+// Order the dataflow lists by estimated start time.  A workgroup that waits holds
+// its slot on the GPU, so a task should be dispatched about when its inputs are
+// ready.  List scheduling over the task graph with unlimited workers (a fixed
+// latency per dependency, a duration growing with the tiles a task streams)
+// gives each supernode's panel chain a start time.  The panel tasks of a supernode are
+// keyed at the start of its chain (so the whole chain is resident when it runs),
+// in chain order, and forward update tasks right after them; sorting by key
+// (ties by the construction order, itself topological) keeps every task after
+// the tasks it waits on (a backward chain is keyed after its parent's panels,
+// whose keys lie below the parent's first finish).
+static void order_flow_lists(SymbolicHost &S, const std::vector<int32_t> &snode_of_pos) {
+    const double lat = 1.0, tile = 1.5, base = 3.0, eps = 1e-6;
+    const int64_t nsn = S.ns;
+    auto stable_by = [](std::vector<FlowTask> &v, const std::vector<double> &key) {
+        std::vector<size_t> idx(v.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return key[a] < key[b]; });
+        std::vector<FlowTask> out(v.size());
+        for (size_t i = 0; i < idx.size(); ++i) out[i] = v[idx[i]];
+        v.swap(out);
+    };
+    // forward: children's outputs (whole-front or update tasks) -> the panel chain;
+    // update tasks trail the chain of their supernode
+    {
+        std::vector<double> out_fin(nsn, 0.0), chain(nsn, 0.0), last_fin(nsn, 0.0), key(S.flow_f.size());
+        for (size_t i = 0; i < S.flow_f.size(); ++i) {
+            const FlowTask &t = S.flow_f[i];
+            double ready = 0.0;
+            for (int32_t q = S.child_ptr[t.s]; q < S.child_ptr[t.s + 1]; ++q) ready = std::max(ready, out_fin[S.child[q]]);
+            if (t.nch) ready += lat;
+            if (t.kind == 1) {
+                key[i] = ready;
+                out_fin[t.s] = ready + base + tile * (double)(t.ms - t.ns) / kTile;
+            } else if (t.p >= 0) {
+                if (t.p == 0) chain[t.s] = ready;
+                key[i] = chain[t.s] + eps * t.p;
+                last_fin[t.s] = (t.p == 0 ? ready : last_fin[t.s] + lat) + base;
+            } else {
+                key[i] = chain[t.s] + eps * (t.np + 1);
+                out_fin[t.s] = std::max({out_fin[t.s], ready + base + tile * t.np, last_fin[t.s] + lat + tile});
+            }
+        }
+        stable_by(S.flow_f, key);
+    }
+    // backward: a supernode's panel tasks read its update rows' values, which its
+    // ancestors publish from their last panel to their first; they are keyed at
+    // the time the parent's first-finishing panel is done, and then trail the
+    // parent's chain (their tiles go through the update rows from the last)
+    {
+        std::vector<double> first_fin(nsn, 0.0), key(S.flow_b.size());
+        std::vector<double> chain_key(nsn, 0.0);
+        for (size_t i = 0; i < S.flow_b.size(); ++i) {
+            const FlowTask &t = S.flow_b[i];
+            if (t.p == t.np - 1) {
+                chain_key[t.s] = t.parent >= 0 ? first_fin[t.parent] : 0.0;
+                first_fin[t.s] = chain_key[t.s] + base + tile * (double)(t.u1 - t.u0) / kTile;
+            }
+            key[i] = chain_key[t.s] + eps * (t.np - 1 - t.p);
+        }
+        stable_by(S.flow_b, key);
+    }
+}
+
 int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, const double *xyz, const uint8_t *fixed,
                    SymbolicHost &S, std::string &err) {
     const int64_t nn = ndof / 6;
@@ -641,9 +704,13 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         S.iofs[k + 1] = S.iofs[k] + e;
     }
     S.inv_entries = S.iofs[nsn];
+    auto whole = [&](int64_t s) { return S.ncol[s] <= kPanel && S.m[s] <= kWholeRows; };
     std::vector<int32_t> need(nsn, 0);
     for (int64_t k = 0; k < nsn; ++k)
-        for (int32_t q = S.child_ptr[k]; q < S.child_ptr[k + 1]; ++q) need[k] += nupd(S.child[q]);
+        for (int32_t q = S.child_ptr[k]; q < S.child_ptr[k + 1]; ++q) {
+            const int32_t c = S.child[q];
+            need[k] += whole(c) ? 1 : nupd(c);
+        }
     auto base = [&](int32_t s) {
         FlowTask t{};
         t.s = s;
@@ -656,8 +723,8 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         t.relofs = S.relofs[s]; t.rofs = S.rofs[s];
         return t;
     };
-    S.flow_f.clear(); S.flow_b.clear(); S.flow_gb.clear();
-    auto add_fwd = [&](FlowTask t) {
+    S.flow_f.clear(); S.flow_b.clear(); S.flow_p.clear(); S.flow_gb.clear();
+    auto add_fwd = [&](FlowTask t, int64_t rows_end) {
         t.gb = (int64_t)S.flow_gb.size();
         for (int32_t q = S.child_ptr[t.s]; q < S.child_ptr[t.s + 1]; ++q) {
             const int32_t c = S.child[q];
@@ -666,7 +733,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
             FlowChild f;
             f.relofs = S.relofs[c];
             f.i0 = (int32_t)(std::lower_bound(rel, rel + nu, (int64_t)t.r0) - rel);
-            f.i1 = (int32_t)(std::lower_bound(rel, rel + nu, (int64_t)t.r0 + t.nr) - rel);
+            f.i1 = (int32_t)(std::lower_bound(rel, rel + nu, rows_end) - rel);
             S.flow_gb.push_back(f);
         }
         S.flow_f.push_back(t);
@@ -675,33 +742,44 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
             const int32_t s = S.level_sn[q];
             FlowTask t = base(s);
+            if (whole(s)) {
+                t.kind = 1;
+                t.p = 0; t.r0 = 0; t.nr = (int32_t)S.ncol[s];
+                t.iofs = S.iofs[s];
+                add_fwd(t, S.m[s]);
+                continue;
+            }
             for (int32_t p = 0; p < t.np; ++p) {
                 t.p = p;
                 t.r0 = p * kPanel;
                 t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
                 t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
-                add_fwd(t);
+                add_fwd(t, (int64_t)t.r0 + t.nr);
             }
             t.iofs = 0;
             for (int32_t u = 0; u < nupd(s); ++u) {
                 t.p = -1 - u;
                 t.r0 = (int32_t)(S.ncol[s] + (int64_t)u * kTile);
                 t.nr = (int32_t)std::min<int64_t>(kTile, S.m[s] - t.r0);
-                add_fwd(t);
+                add_fwd(t, (int64_t)t.r0 + t.nr);
             }
         }
     for (int32_t l = nlev - 1; l >= 0; --l)
         for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
             const int32_t s = S.level_sn[q];
             FlowTask t = base(s);
+            t.u0 = (int32_t)S.ncol[s];
+            t.u1 = (int32_t)S.m[s];
             for (int32_t p = t.np - 1; p >= 0; --p) {
                 t.p = p;
                 t.r0 = p * kPanel;
                 t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
                 t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
                 S.flow_b.push_back(t);
+                S.flow_p.push_back(t);
             }
         }
+    order_flow_lists(S, snode_of_pos);
     return 0;
 }
 

@@ -22,18 +22,22 @@ struct StepTask {
 };             // parent, front columns [c0, c1), rows [r0, r0 + kEaRows)
 
 // Dataflow solve task: one workgroup each; a list is in dependency order, so a
-// task waits only on tasks listed before it.  Forward, p >= 0: pivot rows
-// [r0, r0+nr) (panel p) of the front of s; p < 0: update rows [r0, r0+nr).
-// Backward: columns [r0, r0+nr) (panel p) of s.  The supernode's metadata is
+// task waits only on tasks listed before it.  The supernode's metadata is
 // inlined so that a task starts without dependent loads.
+// Forward, kind 0: p >= 0: pivot rows [r0, r0+nr) (panel p) of the front of s;
+//   p < 0: update rows [r0, r0+nr).  Kind 1: the whole front of a supernode with
+//   one panel and at most kWholeRows rows (pivot and update rows in one task).
+// Backward: columns [r0, r0+nr) (panel p) of s; update rows [u0, u1) = [ns, ms).
+constexpr int kWholeRows = 256;  // front rows of a whole-front forward task
 struct FlowTask {
     int32_t s, p, r0, nr;
-    int32_t np, need;               // panels of s; update blocks of s's children
+    int32_t np, need;               // panels of s; update tasks of s's children (forward)
     int32_t parent, pnp;            // parent supernode (-1 for a root) and its panels
-    int32_t nch, pad;               // children of s
+    int32_t nch, kind;              // children of s; task kind
+    int32_t u0, u1;                 // backward: update rows [u0, u1) of the front
     int64_t ns, ms, lofs, c0, relofs, rofs;
-    int64_t iofs;                   // inverse of this task's diagonal block (pivot tasks)
-    int64_t gb;                     // first FlowChild entry of this task (forward)
+    int64_t iofs;                   // inverse of the diagonal block of panel p
+    int64_t gb;                     // forward: first FlowChild entry
 };
 // per (forward task, child of its supernode), in child order: the child's update
 // rows [i0, i1) that land in the task's rows, and the child's update-vector offset
@@ -96,11 +100,12 @@ struct SymbolicHost {
     std::vector<int64_t> level_maxns;
     std::vector<int64_t> bstep_ptr;          // per level: first backward step
     std::vector<int64_t> bstep_off;          // per backward step: offset into btasks
-    // dataflow solves: forward list (levels ascending; per supernode its panels,
-    // then its update blocks of kTile rows), backward list (levels descending; per
-    // supernode its panels from the last); per supernode the offset of its
+    // dataflow solves: forward list (per supernode its whole front, or its panels
+    // and then its update blocks of kTile rows), backward list (per supernode its
+    // panels from the last), both ordered by estimated start; the panels alone
+    // (flow_p, for the inverse diagonal blocks); per supernode the offset of its
     // inverse diagonal blocks (panel p: b x b entries at iofs[s] + kPanel^2 p)
-    std::vector<FlowTask> flow_f, flow_b;
+    std::vector<FlowTask> flow_f, flow_b, flow_p;
     std::vector<FlowChild> flow_gb;
     std::vector<int64_t> iofs;
     int64_t inv_entries = 0;

@@ -314,7 +314,7 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
     // solve's (12).
     {
         std::vector<double> Linv(std::max<int64_t>(S.inv_entries, 1), 0.0);
-        for (const FlowTask &t : S.flow_b) {
+        for (const FlowTask &t : S.flow_p) {
             const int b = t.nr;
             const int64_t k = t.r0, ms = t.ms;
             if (b < 1 || b > kPanel || t.iofs < 0 || t.iofs + (int64_t)b * b > S.inv_entries) return 10;
@@ -350,7 +350,9 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             if (t.s < 0 || t.s >= nsn || nr < 1 || nr > kTile) return 10;
             if (pivot && (t.r0 != kPanel * t.p || t.r0 + nr > t.ns)) return 10;
             if (!pivot && (t.r0 < t.ns || t.r0 + nr > t.ms || t.parent < 0)) return 10;
-            std::vector<double> vs(kTile * nrhs, 0.0);
+            const bool whole = t.kind == 1;
+            if (whole && (t.p != 0 || t.np != 1 || t.ms > kWholeRows || nr != t.ns)) return 10;
+            std::vector<double> vs(kWholeRows * nrhs, 0.0);
             if (pivot)
                 for (int i = 0; i < nr; ++i)
                     for (int r = 0; r < nrhs; ++r) vs[i * nrhs + r] = Y2[(t.c0 + t.r0 + i) * nrhs + r];
@@ -362,11 +364,44 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                     for (int64_t i = ch.i0; i < ch.i1; ++i) {
                         if (ch.relofs + i < 0 || ch.relofs + i >= (int64_t)S.relmap.size() || ch.relofs + i >= nw) return 10;
                         const int64_t row = S.relmap[ch.relofs + i] - t.r0;
-                        if (row < 0 || row >= nr) return 8;
+                        if (row < 0 || row >= (whole ? t.ms : nr)) return 8;
                         if (!wW2[ch.relofs + i]) return 11;
                         for (int r = 0; r < nrhs; ++r) vs[row * nrhs + r] += W2[(ch.relofs + i) * nrhs + r];
                     }
                 }
+            }
+            if (whole) {
+                // z = L11^{-1} v[0:ns] (four partial sums), then the update rows
+                if (t.iofs < 0 || t.iofs + (int64_t)nr * nr > S.inv_entries) return 10;
+                std::vector<double> acc(4 * kTile * nrhs, 0.0), z(kTile * nrhs, 0.0);
+                for (int w = 0; w < 4; ++w)
+                    for (int i = 0; i < nr; ++i)
+                        for (int c = 16 * w; c < std::min(16 * w + 16, nr); ++c)
+                            for (int r = 0; r < nrhs; ++r)
+                                acc[(w * kTile + i) * nrhs + r] += Linv[t.iofs + i + (int64_t)nr * c] * vs[c * nrhs + r];
+                for (int i = 0; i < nr; ++i) {
+                    const int64_t zi = t.c0 + i;
+                    if (wZ2[zi]) return 11;
+                    wZ2[zi] = 1;
+                    for (int r = 0; r < nrhs; ++r) {
+                        const double *a = &acc[i * nrhs + r];
+                        const int64_t st = (int64_t)kTile * nrhs;
+                        z[i * nrhs + r] = Z2[zi * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                    }
+                }
+                for (int64_t i = t.ns; i < t.ms; ++i) {
+                    const int64_t wi = t.relofs + i - t.ns;
+                    if (wi < 0 || wi >= nw || wW2[wi]) return 11;
+                    wW2[wi] = 1;
+                    for (int r = 0; r < nrhs; ++r) {
+                        double a = 0;
+                        for (int c = 0; c < nr; ++c) a += L[t.lofs + i + t.ms * c] * z[c * nrhs + r];
+                        W2[wi * nrhs + r] = vs[i * nrhs + r] - a;
+                    }
+                }
+                zdone[t.s] = 1;
+                if (t.parent >= 0) ready[t.parent] += 1;
+                continue;
             }
             std::vector<double> acc(4 * kTile * nrhs, 0.0);
             for (int k = 0; k < kend; ++k) {
@@ -425,17 +460,16 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
         for (size_t e = 0; e < S.flow_b.size(); ++e) {
             const FlowTask &t = S.flow_b[e];
             const int nc = t.nr;
-            if (t.parent >= 0 && xdone[t.parent] < t.pnp) return 9;
             if (t.parent >= 0 && t.pnp != (int32_t)((S.ncol[t.parent] + kPanel - 1) / kPanel)) return 10;
+            if (t.u0 != t.ns || t.u1 != t.ms) return 10;
             std::vector<double> acc(4 * kTile * nrhs, 0.0);
-            const int64_t nu = t.ms - t.ns;
-            const int nchunk = (int)((nu + kTile - 1) / kTile);
+            const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;
             const int ntile = nchunk + (t.np - 1 - t.p);
             for (int e2 = 0; e2 < ntile; ++e2) {
                 const bool upd = e2 < nchunk;
                 const int q = upd ? -1 : t.np - 1 - (e2 - nchunk);
-                const int64_t row0 = upd ? t.ns + (int64_t)kTile * e2 : (int64_t)kPanel * q;
-                const int nre = (int)std::min<int64_t>(kTile, (upd ? t.ms : t.ns) - row0);
+                const int64_t row0 = upd ? t.u0 + (int64_t)kTile * (nchunk - 1 - e2) : (int64_t)kPanel * q;
+                const int nre = (int)std::min<int64_t>(kTile, (upd ? (int64_t)t.u1 : t.ns) - row0);
                 if (!upd && xdone[t.s] < t.np - q) return 9;
                 if (!Lblock_ok(t, row0, nre, t.r0, nc)) {
                     // update rows lie below the pivot columns: check against the whole front
