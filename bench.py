@@ -42,9 +42,9 @@ def cpu_model():
 
 
 def read_traffic(path: str, precision: int):
-    """HBM bytes per launch of the forward step kernel measured with rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known size), from
-    the committed summary; None when absent."""
+    """HBM bytes per launch of a kernel measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (scaled by a copy kernel of known size), from the committed
+    summary; None when absent."""
     try:
         for line in open(path):
             if line.startswith(f"hbm_bytes_per_launch_fp{precision}:"):
@@ -200,7 +200,7 @@ def main():
         F, rr_t, it_t, ok_t = step(prec, F)
         fam = {name: ctx.kernel_time(i) for i, name in enumerate(
             ["element", "assembly", "factor", "solve", "spmv", "factor_update", "factor_panel", "extend_add",
-             "fwd_step"])}
+             "fwd_flow", "bwd_flow"])}
         ctx.enable_timing(0)
         results[prec] = {"total": total, "lat": np.array(lat), "u": u, "relres": rr, "iters": it, "fam": fam,
                          "factor_bytes": F.device_bytes(), "converged": conv, "host_rate": host_rate}
@@ -251,35 +251,46 @@ def main():
         out["fp32_refined_error_vs_fp64"] = float(np.max(np.linalg.norm(u32 - u64, axis=0) /
                                                          np.linalg.norm(u64, axis=0)))
         out["fp64_kernel_ms"] = {k: round(v[0], 4) for k, v in r64["fam"].items()}
-    # kernel-family breakdown; roofline of the dominant kernel of the headline path:
-    # the forward-solve step kernel (HBM-bound: streams the L panel of each
-    # 64-column step plus the pending right-hand-side rows)
+    # kernel-family breakdown; roofline of the dominant kernel of the headline path
+    # (largest total time in the instrumented step) among the factor's trailing
+    # update (matrix cores) and the two dataflow triangular-solve kernels (HBM:
+    # per launch they stream the factor entries one solve needs plus the vectors)
     fam = r["fam"]
     out["kernel_ms"] = {k: round(v[0], 4) for k, v in fam.items()}
     es = 4 if main_prec == 32 else 8
-    f_ms, f_n = fam["fwd_step"]
-    if f_n and info["fwd_step_launches"]:
-        sweeps = f_n / info["fwd_step_launches"]
-        bytes_sweep = es * (info["fwd_step_L_entries"] + nrhs * (2 * info["fwd_step_rows"] + info["fwd_step_bsum"]))
-        per_launch = bytes_sweep / info["fwd_step_launches"]
+    tname = "float" if main_prec == 32 else "double"
+    nr_pad = 4 if nrhs <= 4 else (8 if nrhs <= 8 else 16)
+    peak = MI355X_FP32_TFLOPS if main_prec == 32 else MI355X_FP64_TFLOPS
+    lines = {}
+    n_free = info["n_free"]
+    vec = {"fwd_flow": 2 * n_free + 2 * info["upd_rows"], "bwd_flow": 2 * n_free + info["upd_rows"]}
+    for fname, kname in (("fwd_flow", f"fwd_flow_kernel<{tname}, {nr_pad}>"), ("bwd_flow", f"bwd_flow_kernel<{tname}, {nr_pad}>")):
+        f_ms, f_n = fam[fname]
+        if not f_n:
+            continue
+        per_launch = es * (info["solve_entries"] + nrhs * vec[fname])
         avg_ms = f_ms / f_n
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        traffic = None
-        if args.workload == "wingbox" and args.nx == 256 and args.nw == 64:   # the workload the counters were read on
-            traffic = read_traffic(os.path.join(ROOT, "profiles", "fwd_step_kernel_wingbox_traffic.txt"), main_prec)
-        out["roofline"] = {"bound": "hbm", "kernel": f"fwd_step_kernel<{'float' if main_prec == 32 else 'double'}>",
-                           "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS,
-                           "traffic": traffic, "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms,
-                           "launches_per_sweep": info["fwd_step_launches"], "sweeps_timed": sweeps}
+        lines[fname] = (f_ms, {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": MI355X_HBM_GBS,
+                               "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS, "traffic": None,
+                               "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": f_n})
     upd_ms, upd_n = fam["factor_update"]
-    peak = MI355X_FP32_TFLOPS if main_prec == 32 else MI355X_FP64_TFLOPS
     if upd_n:
         per_launch_flops = info["update_flops"] / upd_n
         avg_ms = upd_ms / upd_n
         achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
-        out["update_kernel_roofline"] = {"bound": "mfma", "kernel": f"update_kernel<{'float' if main_prec == 32 else 'double'}>",
-                                         "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                                         "launches": upd_n, "avg_launch_ms": avg_ms, "flops_per_launch": per_launch_flops}
+        lines["factor_update"] = (upd_ms, {"bound": "mfma", "kernel": f"update_kernel<{tname}>", "achieved": achieved,
+                                           "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                                           "flops_per_launch": per_launch_flops, "avg_launch_ms": avg_ms,
+                                           "launches": upd_n})
+    if lines:
+        dom = max(lines, key=lambda k: lines[k][0])
+        roof = lines[dom][1]
+        if args.workload == "wingbox" and args.nx == 256 and args.nw == 64:   # the workload the counters were read on
+            kfile = roof["kernel"].split("<")[0]
+            roof["traffic"] = read_traffic(os.path.join(ROOT, "profiles", f"{kfile}_wingbox_traffic.txt"), main_prec)
+        out["roofline"] = roof
+        out["other_rooflines"] = {k: v[1] for k, v in lines.items() if k != dom}
     if not args.no_cpu_baseline and args.workload == "wingbox":
         cb = run_cpu_baseline(args.cpu_sample_nx)
         cb.pop("seconds", None)

@@ -64,7 +64,8 @@ int jfem_device_memory(jfem_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes)
  * (timed with HIP events on the context's stream); family: 0 element generation,
  * 1 assembly, 2 numeric factor, 3 triangular solves, 4 residual SpMV; with timing
  * level 2 also per launch: 5 factor update kernel, 6 factor panel kernels,
- * 7 extend-add kernel, 8 forward-solve step kernel.
+ * 7 extend-add kernel, 8 forward dataflow solve kernel, 9 backward dataflow
+ * solve kernel.
  * jfem_ctx_enable_timing(ctx, 0 | 1 | 2). */
 int jfem_ctx_kernel_time(jfem_ctx *ctx, int family, double *ms, int64_t *launches);
 int jfem_ctx_reset_kernel_time(jfem_ctx *ctx);
@@ -144,9 +145,11 @@ typedef struct {
     int64_t update_entries;  /* stored update-matrix entries */
     double update_flops;     /* multiply-adds x 2 of the trailing-update tiles (lower triangles) */
     int64_t update_launches; /* update-kernel launches per factorization */
-    /* forward-solve step kernel, per forward sweep: launches, L entries read
-     * (lower panel entries), rows of pending vector touched, sum of block widths */
-    int64_t fwd_step_launches, fwd_step_L_entries, fwd_step_rows, fwd_step_bsum;
+    int64_t inv_entries;      /* entries of the panels' inverse diagonal blocks (solves) */
+    int64_t upd_rows;         /* update-vector rows over all supernodes */
+    int64_t flow_fwd_tasks, flow_bwd_tasks;   /* workgroups of the two dataflow solve kernels */
+    int64_t solve_entries;    /* factor entries one triangular solve reads: per panel its
+                                 inverse diagonal block and the rows below that block */
 } jfem_symbolic_info;
 
 /* Fill-reducing ordering (geometric nested dissection over grid points),

@@ -45,8 +45,8 @@ class SymbolicInfo(C.Structure):
     _fields_ = [("n_free", C.c_int64), ("n_supernodes", C.c_int64), ("n_levels", C.c_int64),
                 ("nnz_L", C.c_int64), ("flops", C.c_double), ("max_front", C.c_int64),
                 ("update_entries", C.c_int64), ("update_flops", C.c_double), ("update_launches", C.c_int64),
-                ("fwd_step_launches", C.c_int64), ("fwd_step_L_entries", C.c_int64), ("fwd_step_rows", C.c_int64),
-                ("fwd_step_bsum", C.c_int64)]
+                ("inv_entries", C.c_int64), ("upd_rows", C.c_int64), ("flow_fwd_tasks", C.c_int64),
+                ("flow_bwd_tasks", C.c_int64), ("solve_entries", C.c_int64)]
 
 
 _lib = None

@@ -29,13 +29,9 @@ struct jfem_symbolic {
             *d_level_sn = nullptr;
     int32_t *d_amap_src = nullptr;
     int64_t *d_amap_dst = nullptr;
-    PanelTask *d_ptasks = nullptr, *d_btasks = nullptr, *d_bwd = nullptr, *d_bred = nullptr, *d_gtasks = nullptr;
-    PanelTask *d_ftiles = nullptr;
-    StepTask *d_fsteps = nullptr, *d_bsteps = nullptr;
+    PanelTask *d_ptasks = nullptr;
     FlowTask *d_flow_f = nullptr, *d_flow_b = nullptr, *d_flow_p = nullptr;
     FlowChild *d_flow_gb = nullptr;
-    int32_t *d_g_bounds = nullptr;
-    int64_t *d_g_boff = nullptr;
     UpdateTask *d_utasks = nullptr;
     EaTask *d_etasks = nullptr;
     int32_t *d_ea_bounds = nullptr;
@@ -408,318 +404,6 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
 #pragma unroll
         for (int j = 0; j < kPanel; ++j)
             if (j <= lane) L[lofs[s] + (c0 + lane) + ms * (c0 + j)] = r[j];
-    }
-}
-
-// ----------------------------------------------------------------- solves
-// Vectors in elimination order, right-hand sides interleaved: v[i * nrhs + r].
-// Forward: Y (pending) -> Z (solved), W holds each supernode's update vector.
-// Backward: Z (pending) -> X (solved).  Each step launch solves one 64-column
-// diagonal block per supernode of the level; every workgroup of the step redoes
-// that small solve from the pending values (no workgroup of the launch writes
-// them) and then applies its own tile, so a step is one launch.  Wave w of a
-// 256-thread workgroup handles right-hand sides r = w, w+4, w+8, w+12.
-
-// forward gather task (s, 0, r0): front rows [r0, r0+kGatherRows) of supernode s:
-// update rows there start at zero, then the children's update vectors entries
-// landing there (child update rows [i0, i1), precomputed) are added in child
-// order -- pivot rows into Y_s, update rows into W_s.
-template <class T>
-__global__ void __launch_bounds__(kBS) fwd_gather_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ boff,
-                                                         const int32_t *__restrict__ bounds, const int64_t *__restrict__ col0,
-                                                         const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
-                                                         const int64_t *__restrict__ relofs, const int32_t *__restrict__ relmap,
-                                                         const int32_t *__restrict__ child_ptr, const int32_t *__restrict__ child,
-                                                         T *__restrict__ Y, T *__restrict__ W, int nrhs) {
-    constexpr int kU = 4;
-    const PanelTask t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s];
-    const int64_t rhi = min<int64_t>((int64_t)t.r0 + kGatherRows, ms), z0 = max<int64_t>(t.r0, ns);
-    T *Ws = W + relofs[s] * nrhs;
-    T *Ys = Y + col0[s] * nrhs;
-    const int tid = threadIdx.x;
-    for (int64_t q = tid; q < (rhi - z0) * nrhs; q += blockDim.x) Ws[(z0 - ns) * nrhs + q] = T(0);
-    __syncthreads();
-    const int32_t *gb = bounds + 2 * boff[blockIdx.x];
-    for (int qc = child_ptr[s]; qc < child_ptr[s + 1]; ++qc, gb += 2) {
-        const int c = child[qc];
-        const int32_t *rel = relmap + relofs[c];
-        const T *Wc = W + relofs[c] * nrhs;
-        const int64_t i0 = gb[0];
-        const int64_t cnt = ((int64_t)gb[1] - i0) * nrhs;
-        for (int64_t base = tid; base < cnt; base += kU * kBS) {
-            int64_t pr[kU], src[kU];
-            int rr[kU];
-            bool ok[kU];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const int64_t q = base + (int64_t)u * kBS;
-                ok[u] = q < cnt;
-                const int64_t qq = ok[u] ? q : cnt - 1;
-                const int64_t i = i0 + qq / nrhs;
-                rr[u] = (int)(qq % nrhs);
-                src[u] = i * nrhs + rr[u];
-                pr[u] = rel[i];
-            }
-            T v[kU];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) v[u] = Wc[src[u]];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                if (ok[u]) {
-                    if (pr[u] < ns) Ys[pr[u] * nrhs + rr[u]] += v[u];
-                    else Ws[(pr[u] - ns) * nrhs + rr[u]] += v[u];
-                }
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// Loads in the step kernels use clamped, always-valid addresses and masks applied
-// afterwards (no data-dependent branches around loads), so that every load of a
-// tile can be in flight at once.  Beyond the block edge b the diagonal block is
-// padded with the identity and the vectors with zeros, which leaves the
-// arithmetic branch-free.  NQ = right-hand sides per wave (1, 2 or 4).
-
-// forward step task (s, k, r0): y_k = L_kk^{-1} (pending rows k..k+b), written to Z
-// by the diagonal task (r0 == k); rows [max(r0, k+b), r0+kTile) of the front get
-// -= L[rows, k..k+b) y_k (pivot rows into Y, update rows into W).
-template <class T, int NQ, int BW>
-__global__ void __launch_bounds__(kBS) fwd_step_kernel(const StepTask *__restrict__ tasks, const T *__restrict__ L,
-                                                       T *__restrict__ Y, T *__restrict__ W, T *__restrict__ Z, int nrhs) {
-    const StepTask t = tasks[blockIdx.x];
-    const int64_t ns = t.ns, ms = t.ms, k = t.k, c0 = t.c0;
-    const int b = t.b;
-    const T *Ls = L + t.lofs;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (w * NQ >= nrhs) return;               // whole wave idle (uniform)
-    const int li = min(lane, b - 1);
-    // the tile's rows do not depend on the diagonal solve: their loads go first
-    const int64_t rs = max<int64_t>(t.r0, k + b), re = min<int64_t>((int64_t)t.r0 + kTile, ms);
-    const bool has_tile = rs < re;
-    const int64_t row = t.r0 + lane;
-    const int64_t rowc = has_tile ? min<int64_t>(max<int64_t>(row, rs), re - 1) : k + li;
-    T tl[BW];
-#pragma unroll
-    for (int j = 0; j < BW; ++j) tl[j] = Ls[rowc + ms * (k + min(j, b - 1))];
-    T lr[BW];                                 // row `lane` of L_kk (identity row beyond b)
-#pragma unroll
-    for (int j = 0; j < BW; ++j) {
-        const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
-        lr[j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
-    }
-    T y[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int r = w * NQ + q;
-        const T v = Y[(c0 + k + li) * nrhs + min(r, nrhs - 1)];
-        y[q] = (lane < b && r < nrhs) ? v : T(0);
-    }
-    T dg = T(1);                              // own diagonal entry (unrolled select, no indexing)
-#pragma unroll
-    for (int j = 0; j < BW; ++j) dg = j == lane ? lr[j] : dg;
-    const T inv_own = T(1) / dg;
-#pragma unroll
-    for (int j = 0; j < BW; ++j) {
-        const T inv = bcast(inv_own, j);
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const T yj = bcast(y[q], j) * inv;
-            const T nv = y[q] - lr[j] * yj;
-            y[q] = lane == j ? yj : nv;
-        }
-    }
-    if (t.r0 == k && lane < b) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int r = w * NQ + q;
-            if (r < nrhs) Z[(c0 + k + lane) * nrhs + r] = y[q];
-        }
-    }
-    if (!has_tile) return;
-    T acc[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = T(0);
-#pragma unroll
-    for (int j = 0; j < BW; ++j) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] += tl[j] * bcast(y[q], j);
-    }
-    if (row >= rs && row < re) {
-        T *dst = row < ns ? (Y + (c0 + row) * nrhs) : (W + (t.relofs + row - ns) * nrhs);
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int r = w * NQ + q;
-            if (r < nrhs) dst[r] -= acc[q];
-        }
-    }
-}
-
-// backward off-diagonal task (s, c0, i0, slot): sum over update rows
-// [i0, i0+kBwdRows) of L[rows, c0..c0+kTile)' X[rows] (X at the ancestors' rows is
-// final); slot < 0: subtract from Z directly, else store the partial sums in slot
-// `slot` of P for bwd_reduce_kernel.  Waves split each 64-row chunk's rows;
-// partial sums are added in a fixed order.
-template <class T, int NR>
-__global__ void __launch_bounds__(kBS) bwd_offdiag_kernel(const PanelTask *__restrict__ tiles, const int64_t *__restrict__ col0,
-                                                          const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
-                                                          const int64_t *__restrict__ lofs, const int64_t *__restrict__ rofs,
-                                                          const int32_t *__restrict__ rows, const T *__restrict__ L,
-                                                          const T *__restrict__ X, T *__restrict__ Z, T *__restrict__ Pp,
-                                                          int nrhs) {
-    __shared__ T Lt[kTile][kTile + 1];
-    __shared__ T xs[kTile][NR + 1];
-    __shared__ T red[4][kTile][NR + 1];
-    const PanelTask t = tiles[blockIdx.x];
-    const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s], ct = t.k;
-    const int64_t iend = min<int64_t>((int64_t)t.r0 + kBwdRows, ms);
-    const int nc = (int)min<int64_t>(kTile, ns - ct);
-    const T *Ls = L + lofs[s];
-    const int32_t *rw = rows + rofs[s];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    T acc[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = T(0);
-    for (int64_t i0 = t.r0; i0 < iend; i0 += kTile) {
-        const int nr = (int)min<int64_t>(kTile, iend - i0);
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < kTile * kTile / kBS; ++p) {
-            const int q = tid + kBS * p;
-            const int i = q % kTile, c = q / kTile;
-            const T v = Ls[(i0 + min(i, nr - 1)) + ms * (ct + min(c, nc - 1))];
-            Lt[i][c] = (i < nr && c < nc) ? v : T(0);
-        }
-#pragma unroll
-        for (int p = 0; p < (kTile * NR + kBS - 1) / kBS; ++p) {
-            const int q = tid + kBS * p;
-            const int i = q / NR, r = q % NR;
-            if (i < kTile) {
-                const T v = X[(int64_t)rw[i0 + min(i, nr - 1)] * nrhs + min(r, nrhs - 1)];
-                xs[i][r] = (i < nr && r < nrhs) ? v : T(0);
-            }
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int i = w; i < kTile; i += 4) {
-            const T l = Lt[i][lane];
-#pragma unroll
-            for (int r = 0; r < NR; ++r) acc[r] += l * xs[i][r];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < NR; ++r) red[w][lane][r] = acc[r];
-    __syncthreads();
-    if (w == 0 && lane < nc) {
-        if (t.pad < 0) {
-            T *dst = Z + (col0[s] + ct + lane) * nrhs;
-#pragma unroll
-            for (int r = 0; r < NR; ++r)
-                if (r < nrhs) dst[r] -= ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
-        } else {
-            T *dst = Pp + ((int64_t)t.pad * kTile + lane) * nrhs;
-#pragma unroll
-            for (int r = 0; r < NR; ++r)
-                if (r < nrhs) dst[r] = ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
-        }
-    }
-}
-
-// reduce task (s, c0, slot0, count): Z[c0..c0+kTile) -= sum of the partial slots
-// slot0 .. slot0+count-1, in slot order
-template <class T>
-__global__ void __launch_bounds__(kBS) bwd_reduce_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ col0,
-                                                         const int64_t *__restrict__ ncol, const T *__restrict__ Pp,
-                                                         T *__restrict__ Z, int nrhs) {
-    const PanelTask t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int nc = (int)min<int64_t>(kTile, ncol[s] - t.k);
-    for (int q = threadIdx.x; q < nc * nrhs; q += blockDim.x) {
-        const int j = q / nrhs, r = q % nrhs;
-        T v = T(0);
-        for (int k = 0; k < t.pad; ++k) v += Pp[((int64_t)(t.r0 + k) * kTile + j) * nrhs + r];
-        Z[(col0[s] + t.k + j) * nrhs + r] -= v;
-    }
-}
-
-// backward step task (s, k, j0): x_k = L_kk^{-T} (pending rows k..k+b of Z), written
-// to X by the diagonal task (j0 == k); for j0 < k, Z[j0..j0+kTile) -= L[k..k+b, j0..]' x_k.
-template <class T, int NQ, int BW>
-__global__ void __launch_bounds__(kBS) bwd_step_kernel(const StepTask *__restrict__ tasks, const T *__restrict__ L,
-                                                       T *__restrict__ Z, T *__restrict__ X, int nrhs) {
-    __shared__ T D[BW][BW + 1];              // D[i][c] = L[k+i][k+c] (identity beyond b)
-    __shared__ T E[BW][kTile + 1];           // E[i][c] = L[k+i][j0+c] (zero rows beyond b)
-    const StepTask t = tasks[blockIdx.x];
-    const int64_t ms = t.ms, k = t.k, c0 = t.c0, j0 = t.r0;
-    const int b = t.b;
-    const T *Ls = L + t.lofs;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const bool diag = j0 == k;
-#pragma unroll
-    for (int p = 0; p < (BW * BW + kBS - 1) / kBS; ++p) {
-        const int q = tid + kBS * p;
-        const int i = q % BW, c = q / BW;
-        if (c < BW) {
-            const T v = Ls[(k + min(i, b - 1)) + ms * (k + min(c, b - 1))];
-            D[i][c] = (i < b && c <= i) ? v : (i == c ? T(1) : T(0));
-        }
-    }
-#pragma unroll
-    for (int p = 0; p < BW * kTile / kBS; ++p) {
-        const int q = tid + kBS * p;
-        const int i = q % BW, c = q / BW;
-        const T e = Ls[(k + min(i, b - 1)) + ms * (diag ? k + min(c, b - 1) : j0 + c)];
-        E[i][c] = i < b ? e : T(0);
-    }
-    T z[NQ];
-    const int li = min(lane, b - 1);
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int r = w * NQ + q;
-        const T v = Z[(c0 + k + li) * nrhs + min(r, nrhs - 1)];
-        z[q] = (r < nrhs && lane < b) ? v : T(0);
-    }
-    __syncthreads();
-    if (w * NQ >= nrhs) return;
-    T lc[BW];                                 // column `lane` of L_kk
-#pragma unroll
-    for (int i = 0; i < BW; ++i) lc[i] = lane < BW ? D[i][min(lane, BW - 1)] : T(0);
-#pragma unroll
-    for (int j = BW - 1; j >= 0; --j) {
-        const T inv = T(1) / bcast(lc[j], j);
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const T xj = bcast(z[q], j) * inv;
-            const T nv = z[q] - lc[j] * xj;
-            z[q] = lane == j ? xj : nv;
-        }
-    }
-    if (diag) {
-        if (lane < b) {
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const int r = w * NQ + q;
-                if (r < nrhs) X[(c0 + k + lane) * nrhs + r] = z[q];
-            }
-        }
-        return;
-    }
-    T acc[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = T(0);
-#pragma unroll
-    for (int i = 0; i < BW; ++i) {
-        const T e = E[i][lane];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] += e * bcast(z[q], i);
-    }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int r = w * NQ + q;
-        if (r < nrhs) Z[(c0 + j0 + lane) * nrhs + r] -= acc[q];
     }
 }
 
@@ -1199,275 +883,6 @@ __global__ void __launch_bounds__(64) diag_inverse_kernel(const FlowTask *__rest
         for (int c = 0; c < b; ++c) dst[lane + (int64_t)b * c] = D[lane][c];
 }
 
-// ---------------------------------------------------------- whole-pivot-block solves
-// One workgroup of WG threads per supernode solves its entire pivot block L11
-// (ns <= R * WG): thread t owns rows (forward) or columns (backward) t + WG * rho,
-// rho < R, whose pending values stay in registers for the whole solve.  Per
-// 64-column block the workgroup stages the diagonal block in LDS; the wavefront
-// owning the block's rows solves it (lane = row, values broadcast with readlane)
-// and publishes the result in LDS; then every thread applies it to its own rows.
-// Two workgroup barriers per block replace a kernel launch per block.
-// NR >= nrhs right-hand sides per thread (4, 8 or 16).
-
-template <class T, int NT>
-__device__ __forceinline__ void stage_diag_block(T (*Dk)[kPanel + 1], const T *Ls, int64_t ms, int64_t k, int b, int t) {
-    T v[kPanel * kPanel / NT];
-#pragma unroll
-    for (int p = 0; p < kPanel * kPanel / NT; ++p) {
-        const int q = t + NT * p;
-        const int i = q % kPanel, c = q / kPanel;
-        v[p] = Ls[(k + min(i, b - 1)) + ms * (k + min(c, b - 1))];
-    }
-#pragma unroll
-    for (int p = 0; p < kPanel * kPanel / NT; ++p) {
-        const int q = t + NT * p;
-        const int i = q % kPanel, c = q / kPanel;
-        Dk[i][c] = (i < b && c <= i) ? v[p] : (i == c ? T(1) : T(0));
-    }
-}
-
-// forward: Z[pivot rows of s] = L11^{-1} Y[pivot rows of s]
-template <class T, int NR, int R, int WG>
-__global__ void __launch_bounds__(WG) fwd_diag_kernel(const int32_t *__restrict__ level_sn, const int64_t *__restrict__ col0,
-                                                      const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
-                                                      const int64_t *__restrict__ lofs, const T *__restrict__ L,
-                                                      const T *__restrict__ Y, T *__restrict__ Z, int nrhs) {
-    typedef T v4 __attribute__((ext_vector_type(4)));
-    __shared__ T Dk[kPanel][kPanel + 1];
-    __shared__ v4 yb[kPanel][NR / 4];         // published block, read as 16-byte broadcasts
-    const int s = level_sn[blockIdx.x];
-    const int64_t ns = ncol[s], ms = mrow[s], c0 = col0[s];
-    const T *Ls = L + lofs[s];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    T y[R][NR];
-#pragma unroll
-    for (int rho = 0; rho < R; ++rho) {
-        const int64_t i = t + (int64_t)WG * rho;
-        const int64_t ic = min<int64_t>(i, ns - 1);
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const T v = Y[(c0 + ic) * nrhs + min(r, nrhs - 1)];
-            y[rho][r] = (i < ns && r < nrhs) ? v : T(0);
-        }
-    }
-    const int nb = (int)((ns + kPanel - 1) / kPanel);
-    for (int kb = 0; kb < nb; ++kb) {
-        const int64_t k = (int64_t)kb * kPanel;
-        const int b = (int)min<int64_t>(kPanel, ns - k);
-        stage_diag_block<T, WG>(Dk, Ls, ms, k, b, t);
-        __syncthreads();
-        const int ow = (int)((k % WG) / kPanel), orho = (int)(k / WG);
-        if (w == ow) {
-            T yy[NR];
-#pragma unroll
-            for (int r = 0; r < NR; ++r) yy[r] = T(0);
-#pragma unroll
-            for (int rho = 0; rho < R; ++rho)
-                if (rho == orho) {
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) yy[r] = y[rho][r];
-                }
-#pragma unroll
-            for (int j = 0; j < kPanel; ++j) {
-                const T lj = Dk[lane][j];
-                const T inv = T(1) / Dk[j][j];
-#pragma unroll
-                for (int r = 0; r < NR; ++r) {
-                    const T yj = bcast(yy[r], j) * inv;
-                    const T nv = yy[r] - lj * yj;
-                    yy[r] = lane == j ? yj : nv;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < NR / 4; ++q) yb[lane][q] = v4{yy[4 * q], yy[4 * q + 1], yy[4 * q + 2], yy[4 * q + 3]};
-            if (lane < b) {
-#pragma unroll
-                for (int r = 0; r < NR; ++r)
-                    if (r < nrhs) Z[(c0 + k + lane) * nrhs + r] = yy[r];
-            }
-        }
-        __syncthreads();
-        if (kb + 1 < nb) {                     // rows of later blocks (b == kPanel here)
-            bool act[R];
-            int64_t ic[R];
-#pragma unroll
-            for (int rho = 0; rho < R; ++rho) {
-                const int64_t i = t + (int64_t)WG * rho;
-                act[rho] = i >= k + kPanel && i < ns;
-                ic[rho] = min<int64_t>(max<int64_t>(i, k + kPanel), ns - 1);
-            }
-            constexpr int kUnroll = (R * NR >= 32) ? 1 : (R * NR >= 16 ? 4 : 8);
-#pragma unroll kUnroll
-            for (int j = 0; j < kPanel; ++j) {
-                v4 yv[NR / 4];
-#pragma unroll
-                for (int q = 0; q < NR / 4; ++q) yv[q] = yb[j][q];
-#pragma unroll
-                for (int rho = 0; rho < R; ++rho) {
-                    const T lv = Ls[ic[rho] + ms * (k + j)];
-                    const T l = act[rho] ? lv : T(0);
-#pragma unroll
-                    for (int q = 0; q < NR / 4; ++q) {
-                        y[rho][4 * q] -= l * yv[q].x;
-                        y[rho][4 * q + 1] -= l * yv[q].y;
-                        y[rho][4 * q + 2] -= l * yv[q].z;
-                        y[rho][4 * q + 3] -= l * yv[q].w;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// forward L21 row tile (s, 0, r0): W_s[rows r0..r0+kTile) -= L21[rows, :] y1, y1
-// staged from Z in 256-column chunks; the four waves split each chunk's columns
-// and their partial sums are added in a fixed order.
-template <class T, int NR>
-__global__ void __launch_bounds__(kBS) fwd_offdiag_kernel(const PanelTask *__restrict__ tiles, const int64_t *__restrict__ col0,
-                                                          const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
-                                                          const int64_t *__restrict__ lofs, const int64_t *__restrict__ relofs,
-                                                          const T *__restrict__ L, const T *__restrict__ Z, T *__restrict__ W,
-                                                          int nrhs) {
-    constexpr int kC = 256;
-    __shared__ T ys[kC][NR];
-    __shared__ T red[4][kTile][NR + 1];
-    const PanelTask t = tiles[blockIdx.x];
-    const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s], c0 = col0[s];
-    const T *Ls = L + lofs[s];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int64_t row = t.r0 + lane;
-    const int64_t rowc = min<int64_t>(row, ms - 1);
-    T acc[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = T(0);
-    for (int64_t cc0 = 0; cc0 < ns; cc0 += kC) {
-        const int nc = (int)min<int64_t>(kC, ns - cc0);
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < kC * NR / kBS; ++p) {
-            const int q = tid + kBS * p;
-            const int cc = q / NR, r = q % NR;
-            const T v = Z[(c0 + cc0 + min(cc, nc - 1)) * nrhs + min(r, nrhs - 1)];
-            ys[cc][r] = (cc < nc && r < nrhs) ? v : T(0);
-        }
-        __syncthreads();
-        const int cb = w * (kC / 4);
-#pragma unroll 8
-        for (int jj = 0; jj < kC / 4; ++jj) {
-            const int cc = cb + jj;
-            const T l = Ls[rowc + ms * (cc0 + min(cc, nc - 1))];
-#pragma unroll
-            for (int r = 0; r < NR; ++r) acc[r] += l * ys[cc][r];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < NR; ++r) red[w][lane][r] = acc[r];
-    __syncthreads();
-    if (w == 0 && row < ms) {
-        T *dst = W + (relofs[s] + row - ns) * nrhs;
-#pragma unroll
-        for (int r = 0; r < NR; ++r)
-            if (r < nrhs) dst[r] -= ((red[0][lane][r] + red[1][lane][r]) + red[2][lane][r]) + red[3][lane][r];
-    }
-}
-
-// backward: X[pivot rows of s] = L11^{-T} Z[pivot rows of s] (Z already holds the
-// L21' contributions); thread t owns columns t + WG * rho
-template <class T, int NR, int R, int WG>
-__global__ void __launch_bounds__(WG) bwd_diag_kernel(const int32_t *__restrict__ level_sn, const int64_t *__restrict__ col0,
-                                                      const int64_t *__restrict__ ncol, const int64_t *__restrict__ mrow,
-                                                      const int64_t *__restrict__ lofs, const T *__restrict__ L,
-                                                      const T *__restrict__ Z, T *__restrict__ X, int nrhs) {
-    typedef T v4 __attribute__((ext_vector_type(4)));
-    __shared__ T Dk[kPanel][kPanel + 1];
-    __shared__ v4 xb[kPanel][NR / 4];         // published block, read as 16-byte broadcasts
-    const int s = level_sn[blockIdx.x];
-    const int64_t ns = ncol[s], ms = mrow[s], c0 = col0[s];
-    const T *Ls = L + lofs[s];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    T z[R][NR];
-#pragma unroll
-    for (int rho = 0; rho < R; ++rho) {
-        const int64_t c = t + (int64_t)WG * rho;
-        const int64_t cc = min<int64_t>(c, ns - 1);
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const T v = Z[(c0 + cc) * nrhs + min(r, nrhs - 1)];
-            z[rho][r] = (c < ns && r < nrhs) ? v : T(0);
-        }
-    }
-    const int nb = (int)((ns + kPanel - 1) / kPanel);
-    for (int kb = nb - 1; kb >= 0; --kb) {
-        const int64_t k = (int64_t)kb * kPanel;
-        const int b = (int)min<int64_t>(kPanel, ns - k);
-        stage_diag_block<T, WG>(Dk, Ls, ms, k, b, t);
-        __syncthreads();
-        const int ow = (int)((k % WG) / kPanel), orho = (int)(k / WG);
-        if (w == ow) {
-            T zz[NR];
-#pragma unroll
-            for (int r = 0; r < NR; ++r) zz[r] = T(0);
-#pragma unroll
-            for (int rho = 0; rho < R; ++rho)
-                if (rho == orho) {
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) zz[r] = z[rho][r];
-                }
-#pragma unroll
-            for (int j = kPanel - 1; j >= 0; --j) {
-                const T lj = Dk[j][lane];            // L[k+j][k+lane]
-                const T inv = T(1) / Dk[j][j];
-#pragma unroll
-                for (int r = 0; r < NR; ++r) {
-                    const T xj = bcast(zz[r], j) * inv;
-                    const T nv = zz[r] - lj * xj;
-                    zz[r] = lane == j ? xj : nv;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < NR / 4; ++q) xb[lane][q] = v4{zz[4 * q], zz[4 * q + 1], zz[4 * q + 2], zz[4 * q + 3]};
-            if (lane < b) {
-#pragma unroll
-                for (int r = 0; r < NR; ++r)
-                    if (r < nrhs) X[(c0 + k + lane) * nrhs + r] = zz[r];
-            }
-        }
-        __syncthreads();
-        if (kb > 0) {                          // columns of earlier blocks
-            bool act[R];
-            int64_t cc[R];
-#pragma unroll
-            for (int rho = 0; rho < R; ++rho) {
-                const int64_t c = t + (int64_t)WG * rho;
-                act[rho] = c < k;
-                cc[rho] = min<int64_t>(c, k - 1);
-            }
-            constexpr int kUnroll = (R * NR >= 32) ? 1 : (R * NR >= 16 ? 4 : 8);
-#pragma unroll kUnroll
-            for (int i = 0; i < kPanel; ++i) {
-                v4 xv[NR / 4];
-#pragma unroll
-                for (int q = 0; q < NR / 4; ++q) xv[q] = xb[i][q];
-#pragma unroll
-                for (int rho = 0; rho < R; ++rho) {
-                    const T lv = Ls[(k + min(i, b - 1)) + ms * cc[rho]];
-                    const T l = act[rho] ? lv : T(0);
-#pragma unroll
-                    for (int q = 0; q < NR / 4; ++q) {
-                        z[rho][4 * q] -= l * xv[q].x;
-                        z[rho][4 * q + 1] -= l * xv[q].y;
-                        z[rho][4 * q + 2] -= l * xv[q].z;
-                        z[rho][4 * q + 3] -= l * xv[q].w;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // ----------------------------------------------------------------- refinement helpers
 // Bp[i*nrhs + r] = b[perm[i] + ndof*r]
 __global__ void gather_rhs_kernel(int64_t n, int nrhs, int64_t ndof, const int32_t *__restrict__ perm,
@@ -1846,104 +1261,6 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
     return hipGetLastError();
 }
 
-// whole-pivot-block solve of one level: workgroup size / rows per thread chosen
-// from the level's widest pivot block; false if no variant fits (R * NR <= 64)
-template <class T, int NR>
-bool level_diag_solve(bool forward, hipStream_t st, const jfem_symbolic *S, int64_t l, const T *L, const T *in, T *out,
-                      int nrhs) {
-    const SymbolicHost &H = S->H;
-    const int64_t maxns = H.level_maxns[l];
-    const int64_t nsn = H.level_ptr[l + 1] - H.level_ptr[l];
-    const int32_t *lsn = S->d_level_sn + H.level_ptr[l];
-#define JFEM_DIAGSOLVE(RV, WGV)                                                                                   \
-    do {                                                                                                          \
-        if (forward)                                                                                              \
-            hipLaunchKernelGGL((fwd_diag_kernel<T, NR, RV, WGV>), dim3((unsigned)nsn), dim3(WGV), 0, st, lsn,     \
-                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, in, out, nrhs);                        \
-        else                                                                                                      \
-            hipLaunchKernelGGL((bwd_diag_kernel<T, NR, RV, WGV>), dim3((unsigned)nsn), dim3(WGV), 0, st, lsn,     \
-                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, L, in, out, nrhs);                        \
-        return true;                                                                                              \
-    } while (0)
-    // Only levels whose pivot blocks fit one 64-column block: there a single
-    // wavefront per supernode needs no in-kernel update phase.  Wider blocks go
-    // through the step launches, whose update tiles spread over the whole GPU
-    // (one workgroup cannot keep enough loads in flight to stream a wide block).
-    if (maxns <= 64) JFEM_DIAGSOLVE(1, 64);
-#undef JFEM_DIAGSOLVE
-    return false;
-}
-
-template <class T, int NQ>
-hipError_t solve_nq(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, T *Pp, int nrhs) {
-    hipStream_t st = ctx->stream;
-    const SymbolicHost &H = S->H;
-    const int64_t nlev = (int64_t)H.level_ptr.size() - 1;
-    for (int64_t l = 0; l < nlev; ++l) {
-        const int64_t ng = H.g_off[l + 1] - H.g_off[l];
-        if (ng)
-            hipLaunchKernelGGL(fwd_gather_kernel<T>, dim3((unsigned)ng), dim3(kBS), 0, st, S->d_gtasks + H.g_off[l],
-                               S->d_g_boff + H.g_off[l], S->d_g_bounds, S->d_col0, S->d_ncol, S->d_m, S->d_relofs,
-                               S->d_relmap, S->d_child_ptr, S->d_child, Y, W, nrhs);
-        if (level_diag_solve<T, 4 * NQ>(true, st, S, l, L, Y, Z, nrhs)) {
-            const int64_t nf = H.ftile_off[l + 1] - H.ftile_off[l];
-            if (nf)
-                hipLaunchKernelGGL((fwd_offdiag_kernel<T, 4 * NQ>), dim3((unsigned)nf), dim3(kBS), 0, st,
-                                   S->d_ftiles + H.ftile_off[l], S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_relofs, L,
-                                   (const T *)Z, W, nrhs);
-            continue;
-        }
-        for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
-            const int64_t nt = H.pstep_off[stp + 1] - H.pstep_off[stp];
-            if (nt) {
-#define JFEM_FWD(BWV)                                                                                           \
-    hipLaunchKernelGGL((fwd_step_kernel<T, NQ, BWV>), dim3((unsigned)nt), dim3(kBS), 0, st, S->d_fsteps + H.pstep_off[stp], \
-                       L, Y, W, Z, nrhs)
-                const bool detail = ctx->timing_level >= 2;
-                if (detail) (void)hipEventRecord(ctx->ev2, st);
-                const int bw = H.pstep_bw[stp];
-                if (bw <= 16) JFEM_FWD(16);
-                else if (bw <= 32) JFEM_FWD(32);
-                else JFEM_FWD(64);
-#undef JFEM_FWD
-                if (detail) {
-                    (void)hipEventRecord(ctx->ev3, st);
-                    (void)hipEventSynchronize(ctx->ev3);
-                    float ms = 0.f;
-                    (void)hipEventElapsedTime(&ms, ctx->ev2, ctx->ev3);
-                    ctx->ms[FAM_FWDSTEP] += ms;
-                    ctx->launches[FAM_FWDSTEP] += 1;
-                }
-            }
-        }
-    }
-    for (int64_t l = nlev - 1; l >= 0; --l) {
-        const int64_t nt = H.bwd_off[l + 1] - H.bwd_off[l];
-        if (nt)
-            hipLaunchKernelGGL((bwd_offdiag_kernel<T, 4 * NQ>), dim3((unsigned)nt), dim3(kBS), 0, st, S->d_bwd + H.bwd_off[l],
-                               S->d_col0, S->d_ncol, S->d_m, S->d_lofs, S->d_rofs, S->d_rows, L, (const T *)X, Z, Pp, nrhs);
-        const int64_t nr = H.bred_off[l + 1] - H.bred_off[l];
-        if (nr)
-            hipLaunchKernelGGL(bwd_reduce_kernel<T>, dim3((unsigned)nr), dim3(kBS), 0, st, S->d_bred + H.bred_off[l],
-                               S->d_col0, S->d_ncol, (const T *)Pp, Z, nrhs);
-        if (level_diag_solve<T, 4 * NQ>(false, st, S, l, L, Z, X, nrhs)) continue;
-        for (int64_t stp = H.bstep_ptr[l]; stp < H.bstep_ptr[l + 1]; ++stp) {
-            const int64_t nb = H.bstep_off[stp + 1] - H.bstep_off[stp];
-            if (nb) {
-#define JFEM_BWD(BWV)                                                                                           \
-    hipLaunchKernelGGL((bwd_step_kernel<T, NQ, BWV>), dim3((unsigned)nb), dim3(kBS), 0, st, S->d_bsteps + H.bstep_off[stp], \
-                       L, Z, X, nrhs)
-                const int bw = H.bstep_bw[stp];
-                if (bw <= 16) JFEM_BWD(16);
-                else if (bw <= 32) JFEM_BWD(32);
-                else JFEM_BWD(64);
-#undef JFEM_BWD
-            }
-        }
-    }
-    return hipGetLastError();
-}
-
 // y = (L L')^{-1} y by the dataflow kernels; cnt: ns + 1 counters (the abort flag,
 // last, is left alone: it is cleared once per refinement call)
 template <class T, int NR>
@@ -1957,12 +1274,28 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
     hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(nsn)), dim3(kBS), 0, st, cnt, nsn);
     hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nv)), dim3(kBS), 0, st, Z, nv, pending_mark<T>());
     hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nv)), dim3(kBS), 0, st, X, nv, pending_mark<T>());
-    if (!H.flow_f.empty())
+    // timing level 2: each kernel between its own pair of events on this stream
+    const bool detail = ctx->timing_level >= 2;
+    auto lap = [&](int fam) {
+        (void)hipEventRecord(ctx->ev3, st);
+        (void)hipEventSynchronize(ctx->ev3);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ctx->ev2, ctx->ev3);
+        ctx->ms[fam] += ms;
+        ctx->launches[fam] += 1;
+    };
+    if (!H.flow_f.empty()) {
+        if (detail) (void)hipEventRecord(ctx->ev2, st);
         hipLaunchKernelGGL((fwd_flow_kernel<T, NR>), dim3((unsigned)H.flow_f.size()), dim3(kBS), 0, st, S->d_flow_f,
                            S->d_flow_gb, S->d_relmap, L, Linv, Y, W, Z, nrhs, cnt, nsn);
-    if (!H.flow_b.empty())
+        if (detail) lap(FAM_FLOW_FWD);
+    }
+    if (!H.flow_b.empty()) {
+        if (detail) (void)hipEventRecord(ctx->ev2, st);
         hipLaunchKernelGGL((bwd_flow_kernel<T, NR>), dim3((unsigned)H.flow_b.size()), dim3(kBS), 0, st, S->d_flow_b,
                            S->d_rows, L, Linv, (const T *)Z, X, nrhs, cnt, nsn);
+        if (detail) lap(FAM_FLOW_BWD);
+    }
     return hipGetLastError();
 }
 
@@ -1972,14 +1305,6 @@ hipError_t flow_solve(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_factor *
     if (nrhs <= 4) return flow_solve_nr<T, 4>(ctx, S, F, Y, W, Z, X, nrhs, cnt);
     if (nrhs <= 8) return flow_solve_nr<T, 8>(ctx, S, F, Y, W, Z, X, nrhs, cnt);
     return flow_solve_nr<T, 16>(ctx, S, F, Y, W, Z, X, nrhs, cnt);
-}
-
-// y = (L L')^{-1} y for nrhs <= 16 interleaved right-hand sides (Y in, X out)
-template <class T>
-hipError_t solve_t(jfem_ctx *ctx, const jfem_symbolic *S, const T *L, T *Y, T *W, T *Z, T *X, T *Pp, int nrhs) {
-    if (nrhs <= 4) return solve_nq<T, 1>(ctx, S, L, Y, W, Z, X, Pp, nrhs);
-    if (nrhs <= 8) return solve_nq<T, 2>(ctx, S, L, Y, W, Z, X, Pp, nrhs);
-    return solve_nq<T, 4>(ctx, S, L, Y, W, Z, X, Pp, nrhs);
 }
 
 template <class V>
@@ -2044,32 +1369,8 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_ptasks, H.ptasks, st);
     if (e == hipSuccess) e = up(S, &S->d_utasks, H.utasks, st);
     if (e == hipSuccess) e = up(S, &S->d_etasks, H.etasks, st);
-    if (e == hipSuccess) e = up(S, &S->d_btasks, H.btasks, st);
     if (e == hipSuccess) e = up(S, &S->d_ea_bounds, H.ea_bounds, st);
-    if (e == hipSuccess) e = up(S, &S->d_bred, H.bred, st);
-    if (e == hipSuccess) e = up(S, &S->d_ftiles, H.ftiles, st);
-    // (host copies kept alive until the stream synchronisation below)
-    std::vector<StepTask> fsteps_h, bsteps_h;
-    {
-        auto mk = [&](const std::vector<PanelTask> &src) {
-            std::vector<StepTask> out(src.size());
-            for (size_t i = 0; i < src.size(); ++i) {
-                const PanelTask &p = src[i];
-                out[i] = StepTask{p.s, p.k, p.r0, (int32_t)std::min<int64_t>(kPanel, H.ncol[p.s] - p.k), H.ncol[p.s],
-                                  H.m[p.s], H.lofs[p.s], H.col0[p.s], H.relofs[p.s], 0};
-            }
-            return out;
-        };
-        fsteps_h = mk(H.ptasks);
-        bsteps_h = mk(H.btasks);
-        if (e == hipSuccess) e = up(S, &S->d_fsteps, fsteps_h, st);
-        if (e == hipSuccess) e = up(S, &S->d_bsteps, bsteps_h, st);
-    }
-    if (e == hipSuccess) e = up(S, &S->d_gtasks, H.gtasks, st);
-    if (e == hipSuccess) e = up(S, &S->d_g_bounds, H.g_bounds, st);
-    if (e == hipSuccess) e = up(S, &S->d_g_boff, H.g_boff, st);
     if (e == hipSuccess) e = up(S, &S->d_ea_boff, H.ea_boff, st);
-    if (e == hipSuccess) e = up(S, &S->d_bwd, H.bwd_tiles, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_f, H.flow_f, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_b, H.flow_b, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_p, H.flow_p, st);
@@ -2089,25 +1390,14 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
         info->update_entries = H.U_entries;
         info->update_flops = H.update_flops;
         info->update_launches = (int64_t)H.ustep_off.size() - 1;
-        // forward step kernel: levels wider than one block use it (see level_diag_solve)
-        int64_t fl = 0, fL = 0, fr = 0, fb = 0;
-        for (int64_t l = 0; l + 1 < (int64_t)H.level_ptr.size(); ++l) {
-            if (H.level_maxns[l] <= kPanel) continue;
-            for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
-                if (H.pstep_off[stp + 1] > H.pstep_off[stp]) ++fl;
-                for (int64_t q = H.pstep_off[stp]; q < H.pstep_mid[stp]; ++q) {
-                    const PanelTask &t = H.ptasks[q];
-                    const int64_t b = std::min<int64_t>(kPanel, H.ncol[t.s] - t.k), rows = H.m[t.s] - t.k;
-                    fL += rows * b - b * (b - 1) / 2;
-                    fr += rows;
-                    fb += b;
-                }
-            }
-        }
-        info->fwd_step_launches = fl;
-        info->fwd_step_L_entries = fL;
-        info->fwd_step_rows = fr;
-        info->fwd_step_bsum = fb;
+        info->inv_entries = H.inv_entries;
+        info->upd_rows = H.upd_rows;
+        info->flow_fwd_tasks = (int64_t)H.flow_f.size();
+        info->flow_bwd_tasks = (int64_t)H.flow_b.size();
+        int64_t se = 0;
+        for (int64_t k = 0; k < H.ns; ++k)
+            for (int64_t c = 0; c < H.ncol[k]; c += kPanel) se += (H.m[k] - c) * std::min<int64_t>(kPanel, H.ncol[k] - c);
+        info->solve_entries = se;
     }
     *out = S;
     return JFEM_OK;

@@ -214,13 +214,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         S.pstep_off.assign(1, 0);
         S.ustep_off.assign(1, 0);
         S.ea_off.assign(1, 0);
-        S.bwd_off.assign(1, 0);
-        S.bred_off.assign(1, 0);
-        S.g_off.assign(1, 0);
-        S.ftile_off.assign(1, 0);
-        S.g_boff.assign(1, 0);
-        S.bstep_ptr.assign(1, 0);
-        S.bstep_off.assign(1, 0);
+        S.iofs.assign(1, 0);
         S.iperm.assign(ndof, -1);
         return 0;
     }
@@ -551,42 +545,10 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
     }
     // task lists
     S.step_ptr.assign(nlev + 1, 0);
-    S.pstep_off.clear(); S.ustep_off.clear(); S.pstep_mid.clear(); S.pstep_bw.clear(); S.bstep_bw.clear();
+    S.pstep_off.clear(); S.ustep_off.clear(); S.pstep_mid.clear(); S.pstep_bw.clear();
     S.ea_off.assign(nlev + 1, 0);
-    S.bwd_off.assign(nlev + 1, 0);
-    S.bred_off.assign(nlev + 1, 0);
-    S.g_off.assign(nlev + 1, 0);
-    S.ftile_off.assign(nlev + 1, 0);
-    S.level_maxns.assign(nlev, 0);
-    S.bstep_ptr.assign(nlev + 1, 0);
-    S.bstep_off.clear();
-    S.bslots_max = 0;
     for (int32_t l = 0; l < nlev; ++l) {
         S.ea_off[l] = (int64_t)S.etasks.size();
-        S.bwd_off[l] = (int64_t)S.bwd_tiles.size();
-        S.bred_off[l] = (int64_t)S.bred.size();
-        S.g_off[l] = (int64_t)S.gtasks.size();
-        S.ftile_off[l] = (int64_t)S.ftiles.size();
-        int64_t slots = 0;
-        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-            const int32_t s = S.level_sn[q];
-            S.level_maxns[l] = std::max<int64_t>(S.level_maxns[l], S.ncol[s]);
-            for (int64_t r0 = S.ncol[s]; r0 < S.m[s]; r0 += kTile) S.ftiles.push_back({s, 0, (int32_t)r0, 0});
-            for (int64_t r0 = 0; r0 < S.m[s]; r0 += kGatherRows) S.gtasks.push_back({s, 0, (int32_t)r0, 0});
-            const int64_t nu = S.m[s] - S.ncol[s];
-            if (nu == 0) continue;
-            const int64_t nch = (nu + kBwdRows - 1) / kBwdRows;
-            for (int64_t c0 = 0; c0 < S.ncol[s]; c0 += kTile) {
-                if (nch == 1) {
-                    S.bwd_tiles.push_back({s, (int32_t)c0, (int32_t)S.ncol[s], -1});
-                } else {
-                    S.bred.push_back({s, (int32_t)c0, (int32_t)slots, (int32_t)nch});
-                    for (int64_t qq = 0; qq < nch; ++qq)
-                        S.bwd_tiles.push_back({s, (int32_t)c0, (int32_t)(S.ncol[s] + qq * kBwdRows), (int32_t)(slots++)});
-                }
-            }
-        }
-        S.bslots_max = std::max(S.bslots_max, slots);
         int64_t maxsteps = 0;
         for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
             int32_t s = S.level_sn[q];
@@ -595,22 +557,6 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 for (int64_t c0 = 0; c0 < S.m[s]; c0 += kTile)
                     for (int64_t r0 = c0; r0 < S.m[s]; r0 += kEaRows)
                         S.etasks.push_back({s, (int32_t)c0, (int32_t)std::min<int64_t>(c0 + kTile, S.m[s]), (int32_t)r0});
-        }
-        // backward steps: step t handles block nb_s - 1 - t of every supernode s
-        S.bstep_ptr[l] = (int64_t)S.bstep_off.size();
-        for (int64_t t = 0; t < maxsteps; ++t) {
-            S.bstep_off.push_back((int64_t)S.btasks.size());
-            int32_t bw = 1;
-            for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-                int32_t s = S.level_sn[q];
-                int64_t nb = (S.ncol[s] + kPanel - 1) / kPanel;
-                int64_t kb = nb - 1 - t;
-                if (kb < 0) continue;
-                int64_t k = kb * kPanel;
-                bw = std::max<int32_t>(bw, (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - k));
-                for (int64_t j0 = 0; j0 <= k; j0 += kTile) S.btasks.push_back({s, (int32_t)k, (int32_t)j0, 0});
-            }
-            S.bstep_bw.push_back(bw);
         }
         S.step_ptr[l] = (int64_t)S.pstep_off.size();
         for (int64_t t = 0; t < maxsteps; ++t) {
@@ -654,24 +600,6 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
     S.pstep_off.push_back((int64_t)S.ptasks.size());
     S.ustep_off.push_back((int64_t)S.utasks.size());
     S.ea_off[nlev] = (int64_t)S.etasks.size();
-    S.bwd_off[nlev] = (int64_t)S.bwd_tiles.size();
-    S.bred_off[nlev] = (int64_t)S.bred.size();
-    S.g_off[nlev] = (int64_t)S.gtasks.size();
-    S.ftile_off[nlev] = (int64_t)S.ftiles.size();
-    S.g_bounds.clear();
-    S.g_boff.assign(S.gtasks.size() + 1, 0);
-    for (size_t e = 0; e < S.gtasks.size(); ++e) {
-        const PanelTask &t = S.gtasks[e];
-        S.g_boff[e] = (int64_t)(S.g_bounds.size() / 2);
-        for (int32_t qc = S.child_ptr[t.s]; qc < S.child_ptr[t.s + 1]; ++qc) {
-            const int32_t c = S.child[qc];
-            const int32_t *rel = S.relmap.data() + S.relofs[c];
-            const int64_t nu = S.m[c] - S.ncol[c];
-            S.g_bounds.push_back((int32_t)(std::lower_bound(rel, rel + nu, (int64_t)t.r0) - rel));
-            S.g_bounds.push_back((int32_t)(std::lower_bound(rel, rel + nu, (int64_t)t.r0 + kGatherRows) - rel));
-        }
-    }
-    S.g_boff[S.gtasks.size()] = (int64_t)(S.g_bounds.size() / 2);
     S.ea_bounds.clear();
     S.ea_boff.assign(S.etasks.size() + 1, 0);
     for (size_t e = 0; e < S.etasks.size(); ++e) {
@@ -689,8 +617,6 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         }
     }
     S.ea_boff[S.etasks.size()] = (int64_t)(S.ea_bounds.size() / 4);
-    S.bstep_ptr[nlev] = (int64_t)S.bstep_off.size();
-    S.bstep_off.push_back((int64_t)S.btasks.size());
     // dataflow solve lists
     auto npan = [&](int64_t s) { return (int32_t)((S.ncol[s] + kPanel - 1) / kPanel); };
     auto nupd = [&](int64_t s) { return (int32_t)((S.m[s] - S.ncol[s] + kTile - 1) / kTile); };

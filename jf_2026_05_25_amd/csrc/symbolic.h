@@ -8,19 +8,10 @@ namespace jfem {
 constexpr int kPanel = 64;   // panel width of the blocked partial factorization
 constexpr int kTile = 64;    // tile edge of panel/update tasks
 constexpr int kEaRows = 512; // front rows per extend-add task
-constexpr int kBwdRows = 256; // update rows per backward off-diagonal task
-constexpr int kGatherRows = 512; // front rows per forward gather task
 
 struct PanelTask { int32_t s, k, r0, pad; };          // supernode, panel column, first row
 struct UpdateTask { int32_t s, k, r0, c0; };          // supernode, panel column, tile origin
 struct EaTask { int32_t s, c0, c1, r0; };
-// a panel/solve task with its supernode's metadata inlined (one 64-byte record,
-// so that a step kernel starts without dependent metadata loads)
-struct StepTask {
-    int32_t s, k, r0, b;            // supernode, panel column, first row/column of the tile, block width
-    int64_t ns, ms, lofs, c0, relofs, pad;
-};             // parent, front columns [c0, c1), rows [r0, r0 + kEaRows)
-
 // Dataflow solve task: one workgroup each; a list is in dependency order, so a
 // task waits only on tasks listed before it.  The supernode's metadata is
 // inlined so that a task starts without dependent loads.
@@ -68,38 +59,13 @@ struct SymbolicHost {
     std::vector<int64_t> step_ptr;          // per level: first step index (size levels+1)
     std::vector<int64_t> pstep_off, ustep_off;   // per step: offset (size steps+1)
     std::vector<int64_t> pstep_mid;              // per step: first non-diagonal panel tile
-    std::vector<int32_t> pstep_bw, bstep_bw;     // per (panel / backward) step: widest diagonal block
+    std::vector<int32_t> pstep_bw;               // per panel step: widest diagonal block
     std::vector<int64_t> ea_off;            // per level: offset into etasks (size levels+1)
     // per extend-add task, per child of its parent (child order): the child's
     // update columns [j0, j1) landing in front columns [c0, c1) and update rows
     // [i0, i1) landing in front rows [r0, r0 + kEaRows)
     std::vector<int32_t> ea_bounds;         // 4 per (task, child)
     std::vector<int64_t> ea_boff;           // per task: first (task, child) entry
-    // Solve task lists.  Forward: per level one gather (children's update
-    // vectors), then the panel steps reuse ptasks: task (s, k, r0) solves the
-    // diagonal block at column k and updates rows [max(r0, k+b), r0+kTile).
-    // Backward: per level the column tiles of L21' (bwd_tiles, (s, 0, c0)), then
-    // steps over the diagonal blocks from the last: task (s, k, j0) solves block k
-    // (j0 == k) or updates columns [j0, j0+kTile) of the pending vector (j0 < k).
-    // Backward off-diagonal tasks (s, c0, i0, slot): columns [c0, c0+kTile) x
-    // update rows [i0, i0+kBwdRows); slot < 0: the only row chunk of the tile,
-    // applied to Z directly; else a partial-sum slot (level-relative), summed in
-    // slot order by the reduce task (s, c0, first slot, count) of the tile.
-    // Forward gather tasks (s, 0, r0, 0): front rows [r0, r0+kGatherRows), with
-    // per (task, child) the child's update rows [i0, i1) landing there.
-    std::vector<PanelTask> bwd_tiles, btasks, bred, gtasks;
-    std::vector<int64_t> bwd_off;            // per level: offset into bwd_tiles
-    std::vector<int64_t> bred_off, g_off;    // per level: offsets into bred / gtasks
-    std::vector<int32_t> g_bounds;           // 2 per (gather task, child)
-    std::vector<int64_t> g_boff;             // per gather task: first (task, child) entry
-    int64_t bslots_max = 0;                  // partial slots of the busiest level
-    // Forward L21 row tiles (s, 0, r0, 0), r0 in [ns, ms) step kTile, per level;
-    // widest pivot block per level
-    std::vector<PanelTask> ftiles;
-    std::vector<int64_t> ftile_off;
-    std::vector<int64_t> level_maxns;
-    std::vector<int64_t> bstep_ptr;          // per level: first backward step
-    std::vector<int64_t> bstep_off;          // per backward step: offset into btasks
     // dataflow solves: forward list (per supernode its whole front, or its panels
     // and then its update blocks of kTile rows), backward list (per supernode its
     // panels from the last), both ordered by estimated start; the panels alone

@@ -2,8 +2,9 @@
 //
 // Runs jfem::build_symbolic and then executes, sequentially on the CPU, the same
 // task decomposition the device kernels execute (scatter, extend-add tasks, panel
-// tasks in two phases per step, update tiles, and the level-scheduled solves), so
-// the structures and task lists can be checked against a dense solve without a GPU.
+// tasks in two phases per step, update tiles, and the dataflow solve lists in
+// their order), so the structures and task lists can be checked against a dense
+// solve without a GPU.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -110,208 +111,17 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             }
         }
     }
-    // solves, interleaved right-hand sides.  Vectors: Y (forward pending), W (update
-    // vectors), Z (forward result = backward pending), X (backward result).  Every
-    // launch is checked for read/write conflicts between its tasks: an element
-    // written by one task must not be read or written by another task of the same
-    // launch (tasks of one launch run concurrently on the device).
+    // solves, interleaved right-hand sides (v[i * nrhs + r])
     const int64_t n = S.n;
     const int64_t nw = std::max<int64_t>(S.upd_rows, 1);
-    std::vector<double> Y(n * nrhs), W(nw * nrhs), Z(n * nrhs, 0.0), X(n * nrhs, 0.0);
-    for (int64_t i = 0; i < n; ++i)
-        for (int r = 0; r < nrhs; ++r) Y[i * nrhs + r] = b[S.perm[i] + ndof * r];
-    // conflict tracking: per vector element, the task that wrote it / last read it in this launch
-    std::vector<int64_t> wY(n, -1), wW(nw, -1), wZ(n, -1), wX(n, -1);
-    std::vector<int64_t> rY(n, -1), rW(nw, -1), rZ(n, -1), rX(n, -1);
-    int64_t launch = 0, conflict = 0;
-    // task ids are launch * 2^32 + task index
-    auto rd = [&](std::vector<int64_t> &w, std::vector<int64_t> &r, int64_t i, int64_t id) {
-        if (w[i] >= 0 && (w[i] >> 32) == launch && w[i] != id) ++conflict;
-        r[i] = id;
-    };
-    auto wr = [&](std::vector<int64_t> &w, std::vector<int64_t> &r, int64_t i, int64_t id) {
-        if (w[i] >= 0 && (w[i] >> 32) == launch && w[i] != id) ++conflict;
-        if (r[i] >= 0 && (r[i] >> 32) == launch && r[i] != id) ++conflict;
-        w[i] = id;
-    };
-    // the forward step of task t: solve the diagonal block at column k from the
-    // pending rows, then update the tile's rows
-    auto fwd_task = [&](PanelTask t, int64_t id) {
-        int s = t.s;
-        int64_t ns = S.ncol[s], ms = S.m[s], k = t.k, c0 = S.col0[s];
-        int b = (int)std::min<int64_t>(kPanel, ns - k);
-        const double *Ls = L.data() + S.lofs[s];
-        std::vector<double> y(b * nrhs);
-        for (int i = 0; i < b; ++i)
-            for (int r = 0; r < nrhs; ++r) { rd(wY, rY, c0 + k + i, id); y[i * nrhs + r] = Y[(c0 + k + i) * nrhs + r]; }
-        for (int j = 0; j < b; ++j)
-            for (int r = 0; r < nrhs; ++r) {
-                y[j * nrhs + r] /= Ls[(k + j) + ms * (k + j)];
-                for (int i = j + 1; i < b; ++i) y[i * nrhs + r] -= Ls[(k + i) + ms * (k + j)] * y[j * nrhs + r];
-            }
-        if (t.r0 == k)
-            for (int i = 0; i < b; ++i)
-                for (int r = 0; r < nrhs; ++r) { wr(wZ, rZ, c0 + k + i, id); Z[(c0 + k + i) * nrhs + r] = y[i * nrhs + r]; }
-        int64_t r1 = std::min<int64_t>(t.r0 + kTile, ms);
-        for (int64_t i = std::max<int64_t>(t.r0, k + b); i < r1; ++i)
-            for (int r = 0; r < nrhs; ++r) {
-                double v = 0;
-                for (int j = 0; j < b; ++j) v += Ls[i + ms * (k + j)] * y[j * nrhs + r];
-                if (i < ns) { wr(wY, rY, c0 + i, id); Y[(c0 + i) * nrhs + r] -= v; }
-                else { wr(wW, rW, S.relofs[s] + i - ns, id); W[(S.relofs[s] + i - ns) * nrhs + r] -= v; }
-            }
-    };
-    auto bwd_task = [&](PanelTask t, int64_t id) {
-        int s = t.s;
-        int64_t ns = S.ncol[s], ms = S.m[s], k = t.k, c0 = S.col0[s];
-        int b = (int)std::min<int64_t>(kPanel, ns - k);
-        const double *Ls = L.data() + S.lofs[s];
-        std::vector<double> x(b * nrhs);
-        for (int i = 0; i < b; ++i)
-            for (int r = 0; r < nrhs; ++r) { rd(wZ, rZ, c0 + k + i, id); x[i * nrhs + r] = Z[(c0 + k + i) * nrhs + r]; }
-        for (int j = b - 1; j >= 0; --j)
-            for (int r = 0; r < nrhs; ++r) {
-                double v = x[j * nrhs + r];
-                for (int i = j + 1; i < b; ++i) v -= Ls[(k + i) + ms * (k + j)] * x[i * nrhs + r];
-                x[j * nrhs + r] = v / Ls[(k + j) + ms * (k + j)];
-            }
-        if (t.r0 == k) {
-            for (int i = 0; i < b; ++i)
-                for (int r = 0; r < nrhs; ++r) { wr(wX, rX, c0 + k + i, id); X[(c0 + k + i) * nrhs + r] = x[i * nrhs + r]; }
-        } else {
-            for (int64_t j = t.r0; j < t.r0 + kTile; ++j)
-                for (int r = 0; r < nrhs; ++r) {
-                    double v = 0;
-                    for (int i = 0; i < b; ++i) v += Ls[(k + i) + ms * j] * x[i * nrhs + r];
-                    wr(wZ, rZ, c0 + j, id);
-                    Z[(c0 + j) * nrhs + r] -= v;
-                }
-        }
-    };
-    for (int64_t l = 0; l < nlev; ++l) {
-        ++launch;   // gather: one task per (supernode, front row range)
-        for (int64_t q = S.g_off[l]; q < S.g_off[l + 1]; ++q) {
-            const int64_t id = (launch << 32) + q;
-            const PanelTask t = S.gtasks[q];
-            const int s = t.s;
-            const int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
-            const int64_t rlo = t.r0, rhi = std::min<int64_t>(t.r0 + kGatherRows, ms);
-            for (int64_t i = std::max<int64_t>(rlo, ns); i < rhi; ++i) {
-                wr(wW, rW, S.relofs[s] + i - ns, id);
-                for (int r = 0; r < nrhs; ++r) W[(S.relofs[s] + i - ns) * nrhs + r] = 0;
-            }
-            const int32_t *gb = S.g_bounds.data() + 2 * S.g_boff[q];
-            for (int32_t qc = S.child_ptr[s]; qc < S.child_ptr[s + 1]; ++qc, gb += 2) {
-                const int c = S.child[qc];
-                const int32_t *rel = S.relmap.data() + S.relofs[c];
-                for (int64_t i = gb[0]; i < gb[1]; ++i) {
-                    rd(wW, rW, S.relofs[c] + i, id);
-                    const int64_t pr = rel[i];
-                    if (pr < rlo || pr >= rhi) return 8;
-                    for (int r = 0; r < nrhs; ++r) {
-                        const double v = W[(S.relofs[c] + i) * nrhs + r];
-                        if (pr < ns) { wr(wY, rY, c0 + pr, id); Y[(c0 + pr) * nrhs + r] += v; }
-                        else { wr(wW, rW, S.relofs[s] + pr - ns, id); W[(S.relofs[s] + pr - ns) * nrhs + r] += v; }
-                    }
-                }
-            }
-        }
-        ++launch;   // pivot-block solves: one task per supernode (y1 = L11^{-1} y1 into Z)
-        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-            const int64_t id = (launch << 32) + q;
-            const int s = S.level_sn[q];
-            const int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
-            const double *Ls = L.data() + S.lofs[s];
-            std::vector<double> y(ns * nrhs);
-            for (int64_t i = 0; i < ns; ++i)
-                for (int r = 0; r < nrhs; ++r) { rd(wY, rY, c0 + i, id); y[i * nrhs + r] = Y[(c0 + i) * nrhs + r]; }
-            for (int64_t j = 0; j < ns; ++j)
-                for (int r = 0; r < nrhs; ++r) {
-                    y[j * nrhs + r] /= Ls[j + ms * j];
-                    for (int64_t i = j + 1; i < ns; ++i) y[i * nrhs + r] -= Ls[i + ms * j] * y[j * nrhs + r];
-                }
-            for (int64_t i = 0; i < ns; ++i)
-                for (int r = 0; r < nrhs; ++r) { wr(wZ, rZ, c0 + i, id); Z[(c0 + i) * nrhs + r] = y[i * nrhs + r]; }
-        }
-        ++launch;   // L21 row tiles: W_s[rows] -= L21[rows, :] y1
-        for (int64_t q = S.ftile_off[l]; q < S.ftile_off[l + 1]; ++q) {
-            const int64_t id = (launch << 32) + q;
-            const PanelTask t = S.ftiles[q];
-            const int s = t.s;
-            const int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
-            const double *Ls = L.data() + S.lofs[s];
-            for (int64_t i = t.r0; i < std::min<int64_t>(t.r0 + kTile, ms); ++i)
-                for (int r = 0; r < nrhs; ++r) {
-                    double v = 0;
-                    for (int64_t j = 0; j < ns; ++j) { rd(wZ, rZ, c0 + j, id); v += Ls[i + ms * j] * Z[(c0 + j) * nrhs + r]; }
-                    wr(wW, rW, S.relofs[s] + i - ns, id);
-                    W[(S.relofs[s] + i - ns) * nrhs + r] -= v;
-                }
-        }
-    }
-    for (int64_t l = nlev - 1; l >= 0; --l) {
-        ++launch;   // off-diagonal row chunks: direct, or into partial slots
-        std::vector<double> Pb(std::max<int64_t>(S.bslots_max, 1) * kTile * nrhs, 0.0);
-        for (int64_t q = S.bwd_off[l]; q < S.bwd_off[l + 1]; ++q) {
-            const int64_t id = (launch << 32) + q;
-            const PanelTask t = S.bwd_tiles[q];
-            const int s = t.s;
-            const int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s], ct = t.k;
-            const double *Ls = L.data() + S.lofs[s];
-            const int32_t *rw = S.rows.data() + S.rofs[s];
-            for (int64_t j = ct; j < std::min<int64_t>(ct + kTile, ns); ++j)
-                for (int r = 0; r < nrhs; ++r) {
-                    double v = 0;
-                    for (int64_t i = t.r0; i < std::min<int64_t>(t.r0 + kBwdRows, ms); ++i) {
-                        rd(wX, rX, rw[i], id);
-                        v += Ls[i + ms * j] * X[(int64_t)rw[i] * nrhs + r];
-                    }
-                    if (t.pad < 0) { wr(wZ, rZ, c0 + j, id); Z[(c0 + j) * nrhs + r] -= v; }
-                    else Pb[((int64_t)t.pad * kTile + (j - ct)) * nrhs + r] = v;
-                }
-        }
-        ++launch;   // reduce the partial slots of multi-chunk tiles, in slot order
-        for (int64_t q = S.bred_off[l]; q < S.bred_off[l + 1]; ++q) {
-            const int64_t id = (launch << 32) + q;
-            const PanelTask t = S.bred[q];
-            const int s = t.s;
-            const int64_t ns = S.ncol[s], c0 = S.col0[s], ct = t.k;
-            for (int64_t j = ct; j < std::min<int64_t>(ct + kTile, ns); ++j)
-                for (int r = 0; r < nrhs; ++r) {
-                    double v = 0;
-                    for (int qq = 0; qq < t.pad; ++qq) v += Pb[((int64_t)(t.r0 + qq) * kTile + (j - ct)) * nrhs + r];
-                    wr(wZ, rZ, c0 + j, id);
-                    Z[(c0 + j) * nrhs + r] -= v;
-                }
-        }
-        ++launch;   // pivot-block backward solves: one task per supernode (x1 = L11^{-T} z1 into X)
-        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-            const int64_t id = (launch << 32) + q;
-            const int s = S.level_sn[q];
-            const int64_t ns = S.ncol[s], ms = S.m[s], c0 = S.col0[s];
-            const double *Ls = L.data() + S.lofs[s];
-            std::vector<double> x(ns * nrhs);
-            for (int64_t i = 0; i < ns; ++i)
-                for (int r = 0; r < nrhs; ++r) { rd(wZ, rZ, c0 + i, id); x[i * nrhs + r] = Z[(c0 + i) * nrhs + r]; }
-            for (int64_t j = ns - 1; j >= 0; --j)
-                for (int r = 0; r < nrhs; ++r) {
-                    double v = x[j * nrhs + r];
-                    for (int64_t i = j + 1; i < ns; ++i) v -= Ls[i + ms * j] * x[i * nrhs + r];
-                    x[j * nrhs + r] = v / Ls[j + ms * j];
-                }
-            for (int64_t i = 0; i < ns; ++i)
-                for (int r = 0; r < nrhs; ++r) { wr(wX, rX, c0 + i, id); X[(c0 + i) * nrhs + r] = x[i * nrhs + r]; }
-        }
-    }
-    if (conflict) return 5;
+    std::vector<double> X;
     // Dataflow solves: the tasks of each list run sequentially in list order, as
     // the kernels compute them (four partial sums per row in wave order, inverse
     // diagonal blocks).  Each wait must already be satisfied (a task may wait only
     // on tasks listed before it: returns 9), every index a kernel forms -- clamped
     // loads included -- must lie inside its array or block (10), every vector entry
-    // is written once and read only after it is written (11), children's
-    // rows land inside the task's rows (8), and the result must equal the level
-    // solve's (12).
+    // is written once and read only after it is written (11), and children's rows
+    // land inside the task's rows (8).
     {
         std::vector<double> Linv(std::max<int64_t>(S.inv_entries, 1), 0.0);
         for (const FlowTask &t : S.flow_p) {
@@ -512,12 +322,8 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             }
             xdone[t.s] += 1;
         }
-        double dmax = 0, xmax = 0;
-        for (int64_t i = 0; i < n * nrhs; ++i) {
-            dmax = std::max(dmax, std::abs(X2[i] - X[i]));
-            xmax = std::max(xmax, std::abs(X[i]));
-        }
-        if (!(dmax <= 1e-8 * xmax)) return 12;
+        for (int64_t i = 0; i < n; ++i)
+            if (!wX2[i]) return 11;
         X.swap(X2);
     }
     for (int64_t i = 0; i < ndof * nrhs; ++i) x[i] = 0;
