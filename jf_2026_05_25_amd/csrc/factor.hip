@@ -30,7 +30,7 @@ struct jfem_symbolic {
     int32_t *d_amap_src = nullptr;
     int64_t *d_amap_dst = nullptr;
     PanelTask *d_ptasks = nullptr;
-    FlowTask *d_flow_f = nullptr, *d_flow_b = nullptr, *d_flow_p = nullptr;
+    FlowTask *d_flow_f = nullptr, *d_flow_b = nullptr;
     FlowChild *d_flow_gb = nullptr;
     UpdateTask *d_utasks = nullptr;
     EaTask *d_etasks = nullptr;
@@ -168,59 +168,136 @@ __device__ inline double bcast<double>(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 
-template <class T, int BW>
-__device__ __forceinline__ void wave_potrf(T (&r)[BW], int lane, int b, int64_t col_base, int64_t *status) {
-    int bad = BW;                         // first non-positive pivot (uniform)
+// Blocked 64 x 64 Cholesky and triangular solve on a 256-thread workgroup.
+// Thread (lane = row i, wave w) keeps entries [16w, 16w+16) of row i of the
+// block in a[16].  Block step jb: wavefront jb alone eliminates its 16 columns
+// over all 64 rows (lane broadcasts, no workgroup barrier: the sequential part
+// stays inside one wavefront), publishes them in LDS, and the later wavefronts
+// apply the 16-wide update to their own columns.  Four block steps, two barriers
+// each.  (A column-at-a-time sweep needs a workgroup barrier per column, ~0.4 us
+// per column; fully unrolled on one wavefront the code overflows the
+// instruction cache and spills scalar registers, ~50 us per block.)
+
+// A = L L' in place (lower triangle; rows and columns >= b are the identity).
+// P: LDS panel of 64 x 17.  Returns the first column with a non-positive pivot,
+// b if none (uniform).
+template <class T>
+__device__ __forceinline__ int wg_potrf(T (&a)[16], T (*P)[17], int lane, int w, int b) {
+    int bad = b;
+    const int nb = (b + 15) / 16;
+    for (int jb = 0; jb < nb; ++jb) {
+        if (w == jb) {
 #pragma unroll
-    for (int j = 0; j < BW; ++j) {
-        const T d0 = bcast(r[j], j);
-        const bool ok = d0 > T(0);
-        bad = (!ok && j < b && bad == BW) ? j : bad;
-        const T sq = tsqrt(ok ? d0 : T(1));
-        const T inv = T(1) / sq;
-        r[j] = lane == j ? sq : (lane > j ? r[j] * inv : r[j]);
+            for (int jj = 0; jj < 16; ++jj) {
+                const int j = 16 * jb + jj;
+                const T d = bcast(a[jj], j);
+                const bool ok = d > T(0);
+                bad = (!ok && j < b && bad == b) ? j : bad;
+                const T sq = tsqrt(ok ? d : T(1));
+                const T inv = T(1) / sq;
+                a[jj] = lane > j ? a[jj] * inv : (lane == j ? sq : a[jj]);
 #pragma unroll
-        for (int c = j + 1; c < BW; ++c) r[c] -= r[j] * bcast(r[j], c);
+                for (int cc = jj + 1; cc < 16; ++cc) a[cc] -= a[jj] * bcast(a[jj], 16 * jb + cc);
+            }
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) P[lane][jj] = a[jj];
+        }
+        __syncthreads();
+        if (w > jb) {
+            T pl[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) pl[jj] = P[lane][jj];
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc) {
+                const T *pc = P[16 * w + cc];
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj) a[cc] -= pl[jj] * pc[jj];
+            }
+        }
+        __syncthreads();
     }
-    if (bad < BW && lane == 0) report_min(status, col_base + bad, JFEM_ERR_NOT_SPD);
+    // the pivot check ran in the owning wavefront: make the result uniform
+    if (lane == 0 && w < nb) P[w][16] = (T)bad;
+    __syncthreads();
+    int first = b;
+    for (int q = 0; q < nb; ++q) first = min(first, (int)P[q][16]);
+    return first;
+}
+
+// X L' = A in place for the tile rows held in a, L = L_kk in LDS as
+// Ld[c][j] = L[c][j] (identity beyond the block width b)
+template <class T>
+__device__ __forceinline__ void wg_trsm(T (&a)[16], const T (*Ld)[kPanel + 4], T (*P)[17], int lane, int w, int b) {
+    const int nb = (b + 15) / 16;
+    for (int jb = 0; jb < nb; ++jb) {
+        if (w == jb) {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int j = 16 * jb + jj;
+                a[jj] = a[jj] / Ld[j][j];
+#pragma unroll
+                for (int cc = jj + 1; cc < 16; ++cc) a[cc] -= a[jj] * Ld[16 * jb + cc][j];
+            }
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) P[lane][jj] = a[jj];
+        }
+        __syncthreads();
+        if (w > jb) {
+            T pl[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) pl[jj] = P[lane][jj];
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc) {
+                const T *lc = &Ld[16 * w + cc][16 * jb];
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj) a[cc] -= pl[jj] * lc[jj];
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // factor the diagonal blocks of the first panels of a level's supernodes (one
-// wavefront per task; tasks are the level's step-0 diagonal panel tasks)
-template <class T, int BW>
-__global__ void __launch_bounds__(64) diag_factor_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
-                                                         const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
-                                                         const int64_t *__restrict__ col0, T *__restrict__ L,
-                                                         int64_t *__restrict__ status) {
+// workgroup per task; tasks are the level's step-0 diagonal panel tasks)
+template <class T>
+__global__ void __launch_bounds__(kBS) diag_factor_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+                                                          const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
+                                                          const int64_t *__restrict__ col0, T *__restrict__ L,
+                                                          int64_t *__restrict__ status) {
+    __shared__ T P[kPanel][17];
     const PanelTask t = tasks[blockIdx.x];
     const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
     const int b = (int)min<int64_t>(kPanel, ns - k);
     T *Ls = L + lofs[s];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = min(lane, b - 1);
-    T r[BW];
+    T a[16];
 #pragma unroll
-    for (int j = 0; j < BW; ++j) {
-        const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
-        r[j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
+    for (int cc = 0; cc < 16; ++cc) {
+        const int c = 16 * w + cc;
+        const T v = Ls[(k + li) + ms * (k + min(c, b - 1))];
+        a[cc] = (c <= lane && lane < b) ? v : (c == lane ? T(1) : T(0));
     }
-    wave_potrf<T, BW>(r, lane, b, col0[s] + k, status);
+    const int bad = wg_potrf(a, P, lane, w, b);
+    if (bad < b && tid == 0) report_min(status, col0[s] + k + bad, JFEM_ERR_NOT_SPD);
     if (lane < b) {
 #pragma unroll
-        for (int j = 0; j < BW; ++j)
-            if (j <= lane) Ls[(k + lane) + ms * (k + j)] = r[j];
+        for (int cc = 0; cc < 16; ++cc) {
+            const int c = 16 * w + cc;
+            if (c <= lane) Ls[(k + lane) + ms * (k + c)] = a[cc];
+        }
     }
 }
 
 // tile solve of panel task (s, k, r0): rows [max(r0, k+b), r0+kTile) of the panel
-// get X <- X L_kk^{-T} (L_kk already factored).  One wavefront per task, lane =
-// row: lane i holds its tile row x and row i of L_kk; L_kk entries are taken by
-// readlane, so the solve touches no memory.
-template <class T, int BW>
-__global__ void __launch_bounds__(64) tile_solve_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
-                                                        const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
-                                                        T *__restrict__ L) {
+// get X <- X L_kk^{-T} (L_kk already factored).  One workgroup per task.
+template <class T>
+__global__ void __launch_bounds__(kBS) tile_solve_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+                                                         const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
+                                                         T *__restrict__ L) {
+    __shared__ __attribute__((aligned(16))) T Ld[kPanel][kPanel + 4];
+    __shared__ T P[kPanel][17];
     const PanelTask t = tasks[blockIdx.x];
     const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
@@ -228,29 +305,64 @@ __global__ void __launch_bounds__(64) tile_solve_kernel(const PanelTask *__restr
     const int64_t rs = max<int64_t>(t.r0, k + b), re = min<int64_t>(t.r0 + kTile, ms);
     if (rs >= re) return;
     T *Ls = L + lofs[s];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = min(lane, b - 1);
     const int64_t row = t.r0 + lane;
     const int64_t rowc = min<int64_t>(max<int64_t>(row, rs), re - 1);
-    T d[BW], x[BW];
+    T a[16];
 #pragma unroll
-    for (int j = 0; j < BW; ++j) {
+    for (int cc = 0; cc < 16; ++cc) {
+        const int j = 16 * w + cc;                 // L_kk column j, row `lane`
         const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
-        d[j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
-        const T w = Ls[rowc + ms * (k + min(j, b - 1))];
-        x[j] = j < b ? w : T(0);
+        Ld[lane][j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
+        const T x = Ls[rowc + ms * (k + min(j, b - 1))];
+        a[cc] = j < b ? x : T(0);
     }
-#pragma unroll
-    for (int j = 0; j < BW; ++j) {
-        x[j] = x[j] / bcast(d[j], j);
-#pragma unroll
-        for (int c = j + 1; c < BW; ++c) x[c] -= x[j] * bcast(d[j], c);
-    }
+    __syncthreads();
+    wg_trsm(a, Ld, P, lane, w, b);
     if (row >= rs && row < re) {
 #pragma unroll
-        for (int j = 0; j < BW; ++j)
-            if (j < b) Ls[row + ms * (k + j)] = x[j];
+        for (int cc = 0; cc < 16; ++cc) {
+            const int j = 16 * w + cc;
+            if (j < b) Ls[row + ms * (k + j)] = a[cc];
+        }
     }
+}
+
+// inverse of each panel's diagonal block, for the triangular solves (one
+// wavefront per panel, after the factorization): lane c computes column c of
+// L_pp^{-1} by forward substitution, L_pp read as broadcasts from LDS; stored
+// column-major with leading dimension b
+template <class T>
+__global__ void __launch_bounds__(64) diag_inverse_kernel(const FlowTask *__restrict__ tasks, const T *__restrict__ L,
+                                                          T *__restrict__ Linv) {
+    __shared__ __attribute__((aligned(16))) T D[kPanel][kPanel + 4];
+    const FlowTask t = tasks[blockIdx.x];
+    const int b = t.nr, lane = threadIdx.x;
+    const int64_t k = t.r0, ms = t.ms;
+    const T *Ls = L + t.lofs;
+    const int li = min(lane, b - 1);
+#pragma unroll 8
+    for (int j = 0; j < kPanel; ++j) {
+        const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
+        D[lane][j] = (lane < b && j <= lane) ? v : (lane == j ? T(1) : T(0));
+    }
+    __syncthreads();
+    T x[kPanel];
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) {
+        T s = i == lane ? T(1) : T(0);
+#pragma unroll
+        for (int j = 0; j < i; ++j) s -= D[i][j] * x[j];
+        x[i] = s / D[i][i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPanel; ++i) D[i][lane] = x[i];
+    __syncthreads();
+    T *dst = Linv + t.iofs;
+    if (lane < b)
+        for (int c = 0; c < b; ++c) dst[lane + (int64_t)b * c] = D[lane][c];
 }
 
 // ----------------------------------------------------------------- update kernel (reference)
@@ -330,8 +442,9 @@ struct Mfma<float> {
 };
 
 // factor_next != 0: the task whose tile holds the next panel's diagonal block
-// (r0 == c0 == k + b < ns) also factors that block after its update (through
-// LDS, by wavefront 0), so the next step's tile solves can start at once.
+// (r0 == c0 == k + b < ns) also factors that block after its update (staged
+// through LDS, blocked over the four wavefronts), so the next step's tile solves
+// can start at once.
 template <class T>
 __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                      const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
@@ -394,16 +507,21 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
             }
     if (!next) return;
     __syncthreads();
-    if (wave != 0) return;
     const int bn = nr;                         // = min(kPanel, ns - c0)
-    T r[kPanel];
+    T a[16];
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) r[j] = (j <= lane && lane < bn) ? A[j][lane] : (j == lane ? T(1) : T(0));
-    wave_potrf<T, kPanel>(r, lane, bn, col0[s] + c0, status);
+    for (int cc = 0; cc < 16; ++cc) {
+        const int c = 16 * wave + cc;
+        a[cc] = (c <= lane && lane < bn) ? A[c][lane] : (c == lane ? T(1) : T(0));
+    }
+    const int bad = wg_potrf(a, reinterpret_cast<T (*)[17]>(&B[0][0]), lane, wave, bn);
+    if (bad < bn && tid == 0) report_min(status, col0[s] + c0 + bad, JFEM_ERR_NOT_SPD);
     if (lane < bn) {
 #pragma unroll
-        for (int j = 0; j < kPanel; ++j)
-            if (j <= lane) L[lofs[s] + (c0 + lane) + ms * (c0 + j)] = r[j];
+        for (int cc = 0; cc < 16; ++cc) {
+            const int c = 16 * wave + cc;
+            if (c <= lane) L[lofs[s] + (c0 + lane) + ms * (c0 + c)] = a[cc];
+        }
     }
 }
 
@@ -848,41 +966,6 @@ __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restric
     }
 }
 
-// inverse of each panel's diagonal block (one wavefront per backward task):
-// lane c computes column c of L_pp^{-1} by forward substitution, L_pp read as
-// broadcasts from LDS; stored column-major with leading dimension b
-template <class T>
-__global__ void __launch_bounds__(64) diag_inverse_kernel(const FlowTask *__restrict__ tasks, const T *__restrict__ L,
-                                                          T *__restrict__ Linv) {
-    __shared__ __attribute__((aligned(16))) T D[kPanel][kPanel + 4];
-    const FlowTask t = tasks[blockIdx.x];
-    const int b = t.nr, lane = threadIdx.x;
-    const int64_t k = t.r0, ms = t.ms;
-    const T *Ls = L + t.lofs;
-    const int li = min(lane, b - 1);
-#pragma unroll 8
-    for (int j = 0; j < kPanel; ++j) {
-        const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
-        D[lane][j] = (lane < b && j <= lane) ? v : (lane == j ? T(1) : T(0));
-    }
-    __syncthreads();
-    T x[kPanel];
-#pragma unroll
-    for (int i = 0; i < kPanel; ++i) {
-        T s = i == lane ? T(1) : T(0);
-#pragma unroll
-        for (int j = 0; j < i; ++j) s -= D[i][j] * x[j];
-        x[i] = s / D[i][i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < kPanel; ++i) D[i][lane] = x[i];
-    __syncthreads();
-    T *dst = Linv + t.iofs;
-    if (lane < b)
-        for (int c = 0; c < b; ++c) dst[lane + (int64_t)b * c] = D[lane][c];
-}
-
 // ----------------------------------------------------------------- refinement helpers
 // Bp[i*nrhs + r] = b[perm[i] + ndof*r]
 __global__ void gather_rhs_kernel(int64_t n, int nrhs, int64_t ndof, const int32_t *__restrict__ perm,
@@ -1191,7 +1274,7 @@ __global__ void diag_ratio_kernel(int64_t n, const int32_t *__restrict__ perm, c
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBS - 1) / kBS, 65536)); }
 
 template <class T>
-hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, T *L, T *U) {
+hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, T *L, T *U, T *Linv) {
     hipStream_t st = ctx->stream;
     const SymbolicHost &H = S->H;
     const bool detail = ctx->timing_level >= 2;
@@ -1223,32 +1306,18 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
             const int64_t stp = H.step_ptr[l];
             const int64_t nd = H.pstep_mid[stp] - H.pstep_off[stp];
             if (detail) (void)hipEventRecord(ctx->ev2, st);
-            if (nd) {
-#define JFEM_DIAG(BWV)                                                                                        \
-    hipLaunchKernelGGL((diag_factor_kernel<T, BWV>), dim3((unsigned)nd), dim3(64), 0, st, S->d_ptasks + H.pstep_off[stp], \
-                       S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status)
-                const int bw = H.pstep_bw[stp];
-                if (bw <= 16) JFEM_DIAG(16);
-                else if (bw <= 32) JFEM_DIAG(32);
-                else JFEM_DIAG(64);
-#undef JFEM_DIAG
-            }
+            if (nd)
+                hipLaunchKernelGGL(diag_factor_kernel<T>, dim3((unsigned)nd), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
+                                   S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status);
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, nd ? 1 : 0); }
         }
         for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
             const int64_t np = H.pstep_off[stp + 1] - H.pstep_off[stp];
             const int64_t nu = H.ustep_off[stp + 1] - H.ustep_off[stp];
             if (detail) (void)hipEventRecord(ctx->ev2, st);
-            if (np) {
-#define JFEM_TILE(BWV)                                                                                         \
-    hipLaunchKernelGGL((tile_solve_kernel<T, BWV>), dim3((unsigned)np), dim3(64), 0, st, S->d_ptasks + H.pstep_off[stp], \
-                       S->d_ncol, S->d_m, S->d_lofs, L)
-                const int bw = H.pstep_bw[stp];
-                if (bw <= 16) JFEM_TILE(16);
-                else if (bw <= 32) JFEM_TILE(32);
-                else JFEM_TILE(64);
-#undef JFEM_TILE
-            }
+            if (np)
+                hipLaunchKernelGGL(tile_solve_kernel<T>, dim3((unsigned)np), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
+                                   S->d_ncol, S->d_m, S->d_lofs, L);
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, np ? 1 : 0); }
             if (nu) {
                 if (detail) (void)hipEventRecord(ctx->ev2, st);
@@ -1258,6 +1327,10 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
             }
         }
     }
+    // inverse diagonal blocks for the triangular solves
+    if (!H.flow_b.empty())
+        hipLaunchKernelGGL(diag_inverse_kernel<T>, dim3((unsigned)H.flow_b.size()), dim3(64), 0, st, S->d_flow_b, (const T *)L,
+                           Linv);
     return hipGetLastError();
 }
 
@@ -1373,7 +1446,6 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_ea_boff, H.ea_boff, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_f, H.flow_f, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_b, H.flow_b, st);
-    if (e == hipSuccess) e = up(S, &S->d_flow_p, H.flow_p, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_gb, H.flow_gb, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -1418,19 +1490,9 @@ static int do_numeric(jfem_ctx *ctx, jfem_factor *F, const jfem_matrix *K, int64
     JFEM_CHECK_HIP(ctx, launch_reset_status(ctx->stream, ctx->d_status));
     timer_start(ctx);
     hipError_t e = F->precision == 64
-                       ? numeric<double>(ctx, S, K, (double *)F->d_L, (double *)F->d_U)
-                       : numeric<float>(ctx, S, K, (float *)F->d_L, (float *)F->d_U);
+                       ? numeric<double>(ctx, S, K, (double *)F->d_L, (double *)F->d_U, (double *)F->d_Linv)
+                       : numeric<float>(ctx, S, K, (float *)F->d_L, (float *)F->d_U, (float *)F->d_Linv);
     JFEM_CHECK_HIP(ctx, e);
-    if (!S->H.flow_p.empty()) {
-        const dim3 g((unsigned)S->H.flow_p.size());
-        if (F->precision == 64)
-            hipLaunchKernelGGL(diag_inverse_kernel<double>, g, dim3(64), 0, ctx->stream, S->d_flow_p,
-                               (const double *)F->d_L, (double *)F->d_Linv);
-        else
-            hipLaunchKernelGGL(diag_inverse_kernel<float>, g, dim3(64), 0, ctx->stream, S->d_flow_p,
-                               (const float *)F->d_L, (float *)F->d_Linv);
-        JFEM_CHECK_HIP(ctx, hipGetLastError());
-    }
     timer_stop(ctx, FAM_FACTOR, 1);
     int64_t st[2];
     JFEM_CHECK_HIP(ctx, hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));

@@ -649,7 +649,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         t.relofs = S.relofs[s]; t.rofs = S.rofs[s];
         return t;
     };
-    S.flow_f.clear(); S.flow_b.clear(); S.flow_p.clear(); S.flow_gb.clear();
+    S.flow_f.clear(); S.flow_b.clear(); S.flow_gb.clear();
     auto add_fwd = [&](FlowTask t, int64_t rows_end) {
         t.gb = (int64_t)S.flow_gb.size();
         for (int32_t q = S.child_ptr[t.s]; q < S.child_ptr[t.s + 1]; ++q) {
@@ -702,7 +702,6 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
                 t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
                 S.flow_b.push_back(t);
-                S.flow_p.push_back(t);
             }
         }
     order_flow_lists(S, snode_of_pos);

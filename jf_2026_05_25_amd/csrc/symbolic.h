@@ -68,10 +68,10 @@ struct SymbolicHost {
     std::vector<int64_t> ea_boff;           // per task: first (task, child) entry
     // dataflow solves: forward list (per supernode its whole front, or its panels
     // and then its update blocks of kTile rows), backward list (per supernode its
-    // panels from the last), both ordered by estimated start; the panels alone
-    // (flow_p, for the inverse diagonal blocks); per supernode the offset of its
-    // inverse diagonal blocks (panel p: b x b entries at iofs[s] + kPanel^2 p)
-    std::vector<FlowTask> flow_f, flow_b, flow_p;
+    // panels from the last), both ordered by estimated start; per supernode the
+    // offset of its inverse diagonal blocks, written by the factorization (panel
+    // p: b x b entries at iofs[s] + kPanel^2 p)
+    std::vector<FlowTask> flow_f, flow_b;
     std::vector<FlowChild> flow_gb;
     std::vector<int64_t> iofs;
     int64_t inv_entries = 0;

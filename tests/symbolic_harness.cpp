@@ -124,7 +124,7 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
     // land inside the task's rows (8).
     {
         std::vector<double> Linv(std::max<int64_t>(S.inv_entries, 1), 0.0);
-        for (const FlowTask &t : S.flow_p) {
+        for (const FlowTask &t : S.flow_b) {
             const int b = t.nr;
             const int64_t k = t.r0, ms = t.ms;
             if (b < 1 || b > kPanel || t.iofs < 0 || t.iofs + (int64_t)b * b > S.inv_entries) return 10;
