@@ -660,6 +660,37 @@ __device__ __forceinline__ void fwd_mul(T (&acc)[NR], const T (&v)[16], const T 
     }
 }
 
+// Gather of the children's update-vector entries landing in rows [r0, r0+nrows)
+// of a front into vs (LDS), nt threads: the (index, value) loads of 8 entries per
+// thread go out together (a loop of single dependent loads would wait a memory
+// latency per entry); children in child order.
+template <class T, int NR>
+__device__ __forceinline__ void gather_children(T (*vs)[NR], const FlowChild *__restrict__ gch, int64_t gb, int nch,
+                                                const int32_t *__restrict__ relmap, const T *W, int r0, int nrhs,
+                                                int tid, int nt) {
+    for (int j = 0; j < nch; ++j) {
+        const FlowChild ch = gch[gb + j];
+        const int64_t nq = (int64_t)(ch.i1 - ch.i0) * nrhs;
+        for (int64_t q0 = 0; q0 < nq; q0 += 8 * nt) {
+            int32_t row[8];
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int64_t q = min<int64_t>(q0 + (int64_t)nt * u + tid, nq - 1);
+                const int64_t i = ch.i0 + q / nrhs;
+                row[u] = relmap[ch.relofs + i];
+                v[u] = ld_dev(W + (ch.relofs + i) * nrhs + (int)(q % nrhs));
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int64_t q = q0 + (int64_t)nt * u + tid;
+                if (q < nq) vs[row[u] - r0][(int)(q % nrhs)] += v[u];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // forward: pivot task (s, p): z_p = L_pp^{-1} (y_p + children's contributions
 // - sum_{k<p} L[p rows, k] z_k), written to Z; update task: w = children's
 // contributions - sum_k L[rows, k] z_k, written to s's update vector in W;
@@ -694,19 +725,7 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
     if (tid == 0) failed = t.need > 0 && !wait_at_least(&ready[t.s], t.need, abort_flag);
     __syncthreads();
     if (failed) return;
-    if (t.need > 0) {
-        for (int j = 0; j < t.nch; ++j) {
-            const FlowChild ch = gch[t.gb + j];
-            const int64_t nq = (int64_t)(ch.i1 - ch.i0) * nrhs;
-            for (int64_t q = tid; q < nq; q += kBS) {
-                const int64_t i = ch.i0 + q / nrhs;
-                const int r = (int)(q % nrhs);
-                const int row = relmap[ch.relofs + i] - t.r0;
-                vs[row][r] += ld_dev(W + (ch.relofs + i) * nrhs + r);
-            }
-            __syncthreads();
-        }
-    }
+    if (t.need > 0) gather_children<T, NR>(vs, gch, t.gb, t.nch, relmap, W, t.r0, nrhs, tid, kBS);
     T acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
@@ -740,14 +759,14 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
             const T *Li = L + t.lofs + i;
 #pragma unroll
             for (int r = 0; r < NR; ++r) acc[r] = T(0);
-            for (int c0 = 0; c0 < t.nr; c0 += 4) {
-                T l4[4];
+            for (int c0 = 0; c0 < t.nr; c0 += 16) {
+                T l16[16];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) l4[u] = Li[t.ms * min(c0 + u, t.nr - 1)];
+                for (int u = 0; u < 16; ++u) l16[u] = Li[t.ms * min(c0 + u, t.nr - 1)];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < 16; ++u) {
                     const T *z = zs[1][min(c0 + u, kTile - 1)];
-                    const T l = c0 + u < t.nr ? l4[u] : T(0);
+                    const T l = c0 + u < t.nr ? l16[u] : T(0);
 #pragma unroll
                     for (int r = 0; r < NR; ++r) acc[r] += l * z[r];
                 }
