@@ -182,8 +182,11 @@ int jfem_factor_free(jfem_factor *F);
  * residual (:1859-1868).  The iterate is a double-double x = x_hi + x_lo refined
  * in an outer loop on the exact residual
  *   r = b - K_ff x  (accumulated exactly: fma-split products, two-sum),
- * each step solving K_ff d = r by conjugate gradients preconditioned with the
- * factor (triangular solves in its precision) to 1e-5 relative, x += d, until
+ * each step advancing one conjugate-gradient iteration on K_ff d = r,
+ * preconditioned with the factor (triangular solves in its precision), until its
+ * residual is below 1e-4 x r for every right-hand side still running (or
+ * tol/2 x ||b||); then x += d and the iteration continues from the new exact
+ * residual, keeping its search directions; until
  * ||K_ff x - b|| / max(||b||, 1e-30) <= tol or the residual no longer halves.
  * max_refine bounds the triangular solves to max_refine + 1 per call;
  * max_refine = 0 is the plain solve x = F^{-1} b.  relres[k] / iters[k]: residual
