@@ -13,6 +13,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <utility>
 
 #include "jfem_internal.h"
 #include "symbolic.h"
@@ -614,6 +615,36 @@ __device__ bool await_value(T &v, const T *p, int32_t *abort_flag) {
     return true;
 }
 
+// the same for N values of a lane at once: every poll re-loads all entries
+// still pending together, so the wait costs one memory round trip per poll and
+// not one per entry
+template <class T, int N>
+__device__ bool await_values(T (&v)[N], const T *const (&p)[N], const bool (&live)[N], int32_t *abort_flag) {
+    bool pend = false;
+#pragma unroll
+    for (int g = 0; g < N; ++g) pend = pend || (live[g] && is_pending(v[g]));
+    if (!pend) return true;
+    const long long t0 = wall_clock64();
+    int spins = 0;
+    do {
+        __builtin_amdgcn_s_sleep(2);
+#pragma unroll
+        for (int g = 0; g < N; ++g)
+            if (live[g] && is_pending(v[g])) v[g] = ld_dev(p[g]);
+        if ((++spins & 15) == 0) {
+            if (ld_dev(abort_flag)) return false;
+            if (wall_clock64() - t0 > kWaitTicks) {
+                st_dev(abort_flag, 1);
+                return false;
+            }
+        }
+        pend = false;
+#pragma unroll
+        for (int g = 0; g < N; ++g) pend = pend || (live[g] && is_pending(v[g]));
+    } while (pend);
+    return true;
+}
+
 // thread (lane = row i, wave w) loads columns [16w, 16w+16) of its row of a
 // rows x cols block (column-major, leading dimension ld); indices are clamped
 // into the block, the caller masks what lies outside
@@ -624,41 +655,120 @@ __device__ __forceinline__ void load_cols16(T (&v)[16], const T *base, int64_t l
     for (int c = 0; c < 16; ++c) v[c] = base[i + ld * min(16 * w + c, cols - 1)];
 }
 
+// ---- 64 x 64 block times 64 x NR vector block, spread over a workgroup ("g16")
+// Wave w takes rows [16w, 16w+16) of the block: lane l has row i = 16w + (l & 15)
+// and column group cg = l >> 4, i.e. columns [16cg, 16cg+16), whose 16 entries
+// of row i it holds in m[16].  The vector block x (64 columns x NR right-hand
+// sides) is held as xr[NR]: in the 16 lanes of column group cg, lane j of xr[g]
+// has value q = 16g + j of the group's 16 x NR values, x[16cg + q / NR][q % NR].
+// A product term m[c] * x[16cg + c][r] reads its x value by a DPP row broadcast
+// (row_newbcast: lane k of each 16-lane row to the whole row), so the values a
+// lane needs come from registers of its own row group and not from LDS.  The
+// four column groups' partial sums are then added across lanes (two swaps), and
+// every lane of a row ends with the same sums.
 // Vectors hold nrhs interleaved right-hand sides (stride nrhs); the kernels work
 // on NR >= nrhs of them (4, 8 or 16), the padding being zero.
-// zs[c][r] = src[c * nrhs + r] for c < cnt, r < nrhs, zero beyond (whole workgroup),
-// once published by another task of the launch; sets *failed on abort/time-out
+//
+// acc += m * x{lane K of the row}.  The DPP source must not have been written by
+// a VALU instruction in the two preceding cycles, nor EXEC by a VALU instruction
+// in the five preceding: the first product of a call carries s_nop 4 (NOP = 2),
+// the first product of every other xr register s_nop 1 (NOP = 1).
+template <int K, int NOP>
+__device__ __forceinline__ void fmac_bcast(float &acc, float x, float m) {
+    if constexpr (NOP == 2)
+        asm volatile("s_nop 4\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                     : "+v"(acc) : "v"(x), "v"(m), "i"(K));
+    else if constexpr (NOP == 1)
+        asm volatile("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                     : "+v"(acc) : "v"(x), "v"(m), "i"(K));
+    else
+        asm volatile("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                     : "+v"(acc) : "v"(x), "v"(m), "i"(K));
+}
+template <int K, int NOP>
+__device__ __forceinline__ void fmac_bcast(double &acc, double x, double m) {
+    if constexpr (NOP == 2)
+        asm volatile("s_nop 4\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                     : "+v"(acc) : "v"(x), "v"(m), "i"(K));
+    else if constexpr (NOP == 1)
+        asm volatile("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                     : "+v"(acc) : "v"(x), "v"(m), "i"(K));
+    else
+        asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                     : "+v"(acc) : "v"(x), "v"(m), "i"(K));
+}
+// term Q = NR c + r: acc[r] += m[c] * xr[Q / 16]{lane Q % 16}; per acc[r] the
+// columns are added in increasing order
+template <class T, int NR, int... Q>
+__device__ __forceinline__ void g16_mac_terms(T (&acc)[NR], const T (&m)[16], const T (&xr)[NR],
+                                              std::integer_sequence<int, Q...>) {
+    (fmac_bcast<Q % 16, Q == 0 ? 2 : (Q % 16 == 0 ? 1 : 0)>(acc[Q % NR], xr[Q / 16], m[Q / NR]), ...);
+}
 template <class T, int NR>
-__device__ __forceinline__ void load_vec_wait(T (*zs)[NR], const T *src, int cnt, int nrhs, int tid, int32_t *abort_flag,
-                                              int *failed) {
-    constexpr int Q = kTile * NR / kBS;
-    T v[Q];
-    const T *p[Q];
+__device__ __forceinline__ void g16_mac(T (&acc)[NR], const T (&m)[16], const T (&xr)[NR]) {
+    g16_mac_terms<T, NR>(acc, m, xr, std::make_integer_sequence<int, 16 * NR>{});
+}
+// sum over the four column groups: (p0 + p2) + (p1 + p3), the same bits in all four
+template <class T, int NR>
+__device__ __forceinline__ void g16_reduce(T (&acc)[NR]) {
 #pragma unroll
-    for (int j = 0; j < Q; ++j) {
-        const int q = tid + kBS * j, c = q / NR, r = q % NR;
-        p[j] = src + (int64_t)min(c, cnt - 1) * nrhs + min(r, nrhs - 1);
-        v[j] = ld_dev(p[j]);
-    }
+    for (int r = 0; r < NR; ++r) acc[r] += __shfl_xor(acc[r], 32);
 #pragma unroll
-    for (int j = 0; j < Q; ++j) {
-        const int q = tid + kBS * j, c = q / NR, r = q % NR;
-        const bool live = c < cnt && r < nrhs;
-        if (live && !await_value(v[j], p[j], abort_flag)) *failed = 1;
-        zs[c][r] = live ? v[j] : T(0);
+    for (int r = 0; r < NR; ++r) acc[r] += __shfl_xor(acc[r], 16);
+}
+// m[c] = B[i + ld (16cg + c)] of a rows x cols block, indices clamped into it
+// (the vector block is zero beyond cols, so clamped columns add nothing)
+template <class T>
+__device__ __forceinline__ void g16_load(T (&m)[16], const T *B, int64_t ld, int rows, int cols, int i, int cg) {
+    const int64_t ic = min(i, rows - 1);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) m[c] = B[ic + ld * min(16 * cg + c, cols - 1)];
+}
+// transposed: m[k] = B[(16cg + k) + ld i], i.e. row i of B^T
+template <class T>
+__device__ __forceinline__ void g16_load_t(T (&m)[16], const T *B, int64_t ld, int rows, int cols, int i, int cg) {
+    const int64_t ic = min(i, cols - 1);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = B[min(16 * cg + k, rows - 1) + ld * ic];
+}
+// (column, rhs) of the value lane (cg, j) holds in xr[g]
+template <int NR>
+__device__ __forceinline__ int g16_col(int g, int cg, int j) { return 16 * cg + (16 * g + j) / NR; }
+template <int NR>
+__device__ __forceinline__ int g16_rhs(int g, int j) { return (16 * g + j) % NR; }
+// xr from a vector block in LDS (V[c][r]), zero beyond cnt columns
+template <class T, int NR>
+__device__ __forceinline__ void g16_from_lds(T (&xr)[NR], const T (*V)[NR], int cnt, int cg, int j) {
+#pragma unroll
+    for (int g = 0; g < NR; ++g) {
+        const int c = g16_col<NR>(g, cg, j), r = g16_rhs<NR>(g, j);
+        xr[g] = c < cnt ? V[c][r] : T(0);
     }
 }
-
-// acc[r] += sum over the wave's 16 columns c of v[c] * zs[16w + c][r]
-template <class T, int NR>
-__device__ __forceinline__ void fwd_mul(T (&acc)[NR], const T (&v)[16], const T (*zs)[NR], int w) {
+// xr from device memory once published by another task: column c at src_c(c)
+// (stride-1 right-hand sides); zero beyond cnt columns or nrhs; false on
+// abort / time-out
+template <class T, int NR, class Addr>
+__device__ __forceinline__ bool g16_load_wait(T (&xr)[NR], Addr src_c, int cnt, int nrhs, int cg, int j,
+                                              int32_t *abort_flag) {
+    const T *p[NR];
+    bool live[NR];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        const T *z = zs[16 * w + c];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) acc[r] += v[c] * z[r];
+    for (int g = 0; g < NR; ++g) {
+        const int c = g16_col<NR>(g, cg, j), r = g16_rhs<NR>(g, j);
+        p[g] = src_c(min(c, cnt - 1)) + min(r, nrhs - 1);
+        live[g] = c < cnt && r < nrhs;
+        xr[g] = ld_dev(p[g]);
     }
+    const bool ok = await_values<T, NR>(xr, p, live, abort_flag);
+#pragma unroll
+    for (int g = 0; g < NR; ++g) xr[g] = live[g] ? xr[g] : T(0);
+    return ok;
 }
+// this lane's share of a row's NR sums when storing: right-hand sides
+// [cg NR/4, (cg+1) NR/4)
+template <int NR>
+__device__ __forceinline__ int g16_r0(int cg) { return cg * (NR / 4); }
 
 // Gather of the children's update-vector entries landing in rows [r0, r0+nrows)
 // of a front into vs (LDS), nt threads, NR slots per row (right-hand sides
@@ -697,7 +807,9 @@ __device__ __forceinline__ void gather_children(T (*vs)[NR], const FlowChild *__
 // - sum_{k<p} L[p rows, k] z_k), written to Z; update task: w = children's
 // contributions - sum_k L[rows, k] z_k, written to s's update vector in W;
 // whole-front task: both for a supernode with one panel (rows of the front
-// < kWholeRows), the update rows one per thread.
+// <= kWholeRows), the update rows in blocks of 64.
+// The k loop runs without LDS or barriers: every lane polls the z_k values it
+// needs straight into its registers.
 template <class T, int NR>
 __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restrict__ tasks,
                                                        const FlowChild *__restrict__ gch,
@@ -705,160 +817,117 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
                                                        const T *__restrict__ Linv, const T *__restrict__ Y, T *W, T *Z,
                                                        int nrhs, int32_t *cnt, int64_t nsn) {
     __shared__ __attribute__((aligned(16))) T vs[kWholeRows][NR];
-    __shared__ __attribute__((aligned(16))) T zs[2][kTile][NR];   // double-buffered
-    __shared__ __attribute__((aligned(16))) T red[3][kTile][NR];
+    __shared__ __attribute__((aligned(16))) T vx[kTile][NR];
     __shared__ int failed;
     const FlowTask t = tasks[blockIdx.x];
     int32_t *ready = cnt, *abort_flag = cnt + nsn;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, j = lane & 15, cg = lane >> 4;
+    const int i = 16 * w + j;                          // row of the 64-row block
+    const int rq = g16_r0<NR>(cg);
     const bool pivot = t.p >= 0, whole = t.kind == 1;
     const int kend = pivot ? t.p : t.np;
     const T *Ls = L + t.lofs + t.r0;                   // front row r0, column 0
-    T lv[16], inv[16];
-    if (kend > 0) load_cols16(lv, Ls, t.ms, t.nr, (int)min<int64_t>(kPanel, t.ns), lane, w);
-    if (pivot) load_cols16(inv, Linv + t.iofs, t.nr, t.nr, t.nr, lane, w);
+    T lm[16], im[16];
+    if (kend > 0) g16_load(lm, Ls, t.ms, t.nr, (int)min<int64_t>(kPanel, t.ns), i, cg);
+    else if (whole && t.ms > t.ns) g16_load(lm, L + t.lofs + t.ns, t.ms, (int)(t.ms - t.ns), t.nr, i, cg);
+    if (pivot) g16_load(im, Linv + t.iofs, t.nr, t.nr, t.nr, i, cg);
 #pragma unroll
-    for (int j = 0; j < kWholeRows * NR / kBS; ++j) {
-        const int q = tid + kBS * j, i = q / NR, r = q % NR;
-        const int64_t yi = pivot ? t.c0 + t.r0 + min(i, t.nr - 1) : 0;
+    for (int q2 = 0; q2 < kWholeRows * NR / kBS; ++q2) {
+        const int q = tid + kBS * q2, row = q / NR, r = q % NR;
+        const int64_t yi = pivot ? t.c0 + t.r0 + min(row, t.nr - 1) : 0;
         const T y = Y[yi * nrhs + min(r, nrhs - 1)];
-        vs[i][r] = (pivot && i < t.nr && r < nrhs) ? y : T(0);
+        vs[row][r] = (pivot && row < t.nr && r < nrhs) ? y : T(0);
     }
     if (tid == 0) failed = t.need > 0 && !wait_at_least(&ready[t.s], t.need, abort_flag);
     __syncthreads();
     if (failed) return;
     if (t.need > 0) gather_children<T, NR>(vs, gch, t.gb, t.nch, relmap, W, t.r0, nrhs, tid, kBS);
-    T acc[NR];
+    T acc[NR], xr[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
     if (whole) {
-        // z = L_11^{-1} v[0:ns] (4 waves, 16 columns each), then w_i = v_i - L[i, :] z
-        // for the update rows, one row per thread
+        // z = L_11^{-1} v[0:ns], then w = v - L[update rows, :] z in blocks of 64 rows
+        g16_from_lds<T, NR>(xr, vs, t.nr, cg, j);      // update rows of the front are not part of v
+        g16_mac<T, NR>(acc, im, xr);
+        g16_reduce<T, NR>(acc);
 #pragma unroll
-        for (int j = 0; j < kTile * NR / kBS; ++j) {
-            const int q = tid + kBS * j, i = q / NR, r = q % NR;
-            zs[0][i][r] = i < t.nr ? vs[i][r] : T(0);       // update rows of the front are not part of v
+        for (int rr = 0; rr < NR / 4; ++rr) {
+            const int r = rq + rr;
+            vx[i][r] = i < t.nr ? acc[r] : T(0);
+            if (i < t.nr && r < nrhs) st_dev(Z + (t.c0 + i) * nrhs + r, acc[r]);
         }
         __syncthreads();
-        fwd_mul<T, NR>(acc, inv, zs[0], w);
-        if (w > 0) {
+        g16_from_lds<T, NR>(xr, vx, t.nr, cg, j);
+        const int64_t nb = t.ms - t.ns;                 // block 0 was loaded at the start
+        for (int64_t b0 = 0; b0 < nb; b0 += kTile) {
+            T cur[16];
 #pragma unroll
-            for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
-        }
-        __syncthreads();
-        if (w == 0) {
-            T *dst = Z + (t.c0 + lane) * nrhs;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const T z = ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r];
-                zs[1][lane][r] = lane < t.nr ? z : T(0);
-                if (lane < t.nr && r < nrhs) st_dev(dst + r, z);
-            }
-        }
-        __syncthreads();
-        const int64_t i = t.ns + tid;
-        if (i < t.ms) {
-            const T *Li = L + t.lofs + i;
+            for (int c = 0; c < 16; ++c) cur[c] = lm[c];
+            if (b0 + kTile < nb) g16_load(lm, L + t.lofs + t.ns + b0 + kTile, t.ms, (int)(nb - b0 - kTile), t.nr, i, cg);
 #pragma unroll
             for (int r = 0; r < NR; ++r) acc[r] = T(0);
-            for (int c0 = 0; c0 < t.nr; c0 += 16) {
-                T l16[16];
+            g16_mac<T, NR>(acc, cur, xr);
+            g16_reduce<T, NR>(acc);
+            const int64_t row = t.ns + b0 + i;         // front row
+            if (row < t.ms) {
 #pragma unroll
-                for (int u = 0; u < 16; ++u) l16[u] = Li[t.ms * min(c0 + u, t.nr - 1)];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const T *z = zs[1][min(c0 + u, kTile - 1)];
-                    const T l = c0 + u < t.nr ? l16[u] : T(0);
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) acc[r] += l * z[r];
+                for (int rr = 0; rr < NR / 4; ++rr) {
+                    const int r = rq + rr;
+                    if (r < nrhs) st_dev(W + (t.relofs + row - t.ns) * nrhs + r, vs[row][r] - acc[r]);
                 }
             }
-            T *dst = W + (t.relofs + i - t.ns) * nrhs;
-#pragma unroll
-            for (int r = 0; r < NR; ++r)
-                if (r < nrhs) st_dev(dst + r, vs[i][r] - acc[r]);
         }
         complete_stores();
         __syncthreads();
         if (tid == 0 && t.parent >= 0) __hip_atomic_fetch_add(&ready[t.parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
+    bool ok = true;
     for (int k = 0; k < kend; ++k) {
         const int64_t col = (int64_t)kPanel * k;
-        load_vec_wait<T, NR>(zs[k & 1], Z + (t.c0 + col) * nrhs, (int)min<int64_t>(kPanel, t.ns - col), nrhs, tid,
-                             abort_flag, &failed);
-        __syncthreads();
-        if (failed) return;
+        const T *zk = Z + (t.c0 + col) * nrhs;
+        ok = g16_load_wait<T, NR>(xr, [&](int c) { return zk + (int64_t)c * nrhs; },
+                                  (int)min<int64_t>(kPanel, t.ns - col), nrhs, cg, j, abort_flag) && ok;
         T cur[16];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) cur[c] = lv[c];
+        for (int c = 0; c < 16; ++c) cur[c] = lm[c];
         if (k + 1 < kend)
-            load_cols16(lv, Ls + t.ms * (col + kPanel), t.ms, t.nr, (int)min<int64_t>(kPanel, t.ns - col - kPanel), lane, w);
-        fwd_mul<T, NR>(acc, cur, zs[k & 1], w);
+            g16_load(lm, Ls + t.ms * (col + kPanel), t.ms, t.nr, (int)min<int64_t>(kPanel, t.ns - col - kPanel), i, cg);
+        g16_mac<T, NR>(acc, cur, xr);
     }
-    if (w > 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
-    }
-    __syncthreads();
-    T v[NR];
-    if (w == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) v[r] = vs[lane][r] - (((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
-    }
+    if (!ok) failed = 1;
+    g16_reduce<T, NR>(acc);
     if (!pivot) {
-        if (w == 0) {
-            if (lane < t.nr) {
-                T *dst = W + (t.relofs + t.r0 - t.ns + lane) * nrhs;
+        if (i < t.nr) {
 #pragma unroll
-                for (int r = 0; r < NR; ++r)
-                    if (r < nrhs) st_dev(dst + r, v[r]);
+            for (int rr = 0; rr < NR / 4; ++rr) {
+                const int r = rq + rr;
+                if (r < nrhs) st_dev(W + (t.relofs + t.r0 - t.ns + i) * nrhs + r, vs[i][r] - acc[r]);
             }
-            complete_stores();
-            if (lane == 0) __hip_atomic_fetch_add(&ready[t.parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        complete_stores();
+        __syncthreads();
+        if (tid == 0 && !failed) __hip_atomic_fetch_add(&ready[t.parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    if (w == 0) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) zs[kend & 1][lane][r] = lane < t.nr ? v[r] : T(0);
+    for (int rr = 0; rr < NR / 4; ++rr) {
+        const int r = rq + rr;
+        vx[i][r] = i < t.nr ? vs[i][r] - acc[r] : T(0);
     }
     __syncthreads();
+    if (failed) return;
+    g16_from_lds<T, NR>(xr, vx, t.nr, cg, j);
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
-    fwd_mul<T, NR>(acc, inv, zs[kend & 1], w);
-    if (w > 0) {
+    g16_mac<T, NR>(acc, im, xr);
+    g16_reduce<T, NR>(acc);
+    if (i < t.nr) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
-    }
-    __syncthreads();
-    if (w == 0) {
-        if (lane < t.nr) {
-            T *dst = Z + (t.c0 + t.r0 + lane) * nrhs;
-#pragma unroll
-            for (int r = 0; r < NR; ++r)
-                if (r < nrhs) st_dev(dst + r, ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
+        for (int rr = 0; rr < NR / 4; ++rr) {
+            const int r = rq + rr;
+            if (r < nrhs) st_dev(Z + (t.c0 + t.r0 + i) * nrhs + r, acc[r]);
         }
-    }
-}
-
-// Lt[i][c] = block entry (i, c) for i < rows, c < cols, zero elsewhere, from the
-// registers of load_cols16
-template <class T>
-__device__ __forceinline__ void stage_tile(T (*Lt)[kTile + 1], const T (&v)[16], int rows, int cols, int lane, int w) {
-#pragma unroll
-    for (int c = 0; c < 16; ++c) Lt[lane][16 * w + c] = (lane < rows && 16 * w + c < cols) ? v[c] : T(0);
-}
-
-// acc[r] += sum over the wave's 16 rows i of Lt[i][lane] * xs[i][r]
-template <class T, int NR>
-__device__ __forceinline__ void bwd_mul(T (&acc)[NR], const T (*Lt)[kTile + 1], const T (*xs)[NR], int lane, int w) {
-#pragma unroll
-    for (int i = 16 * w; i < 16 * w + 16; ++i) {
-        const T l = Lt[i][lane];
-        const T *x = xs[i];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) acc[r] += l * x[r];
     }
 }
 
@@ -866,24 +935,27 @@ __device__ __forceinline__ void bwd_mul(T (&acc)[NR], const T (*Lt)[kTile + 1], 
 // panel of L[i, p]' x_i): first the update rows (ancestors' values), tile by tile
 // from the last -- the order in which the ancestors' panels publish them -- then
 // the later panels of s from the last, each value awaited as it is published.
+// The products are transposed (lane row = panel column): each tile is loaded
+// coalesced, staged through LDS (double-buffered in FP32, so one barrier per
+// tile), and read back transposed before its x values are awaited.
 template <class T, int NR>
 __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restrict__ tasks, const int32_t *__restrict__ rows,
                                                        const T *__restrict__ L, const T *__restrict__ Linv,
                                                        const T *__restrict__ Z, T *X, int nrhs, int32_t *cnt, int64_t nsn) {
-    constexpr int XQ = kTile * NR / kBS;               // x entries per thread and tile
-    __shared__ __attribute__((aligned(16))) T Lt[kTile][kTile + 1];
-    __shared__ __attribute__((aligned(16))) T xs[kTile][NR];
-    __shared__ __attribute__((aligned(16))) T red[3][kTile][NR];
+    constexpr int NB = sizeof(T) == 4 ? 2 : 1;         // staging buffers (LDS budget)
+    __shared__ __attribute__((aligned(16))) T Lt[NB][kTile][kTile + 1];
+    __shared__ __attribute__((aligned(16))) T vx[kTile][NR];
     __shared__ int failed;
     const FlowTask t = tasks[blockIdx.x];
     int32_t *abort_flag = cnt + nsn;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, j = lane & 15, cg = lane >> 4;
+    const int c = 16 * w + j;                          // panel column of this lane's output
+    const int rq = g16_r0<NR>(cg);
     const int nc = t.nr;
     const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;
     const int ntile = nchunk + (t.np - 1 - t.p);
     const T *Lc = L + t.lofs + t.ms * t.r0;            // front row 0, column r0
     const int32_t *rw = rows + t.rofs;
-    const int xr = tid % NR, xi0 = tid / NR;           // this thread's x entries: rows xi0 + (kBS/NR) j, rhs xr
     // tile e: update rows [u0 + 64(nchunk-1-e), ...) for e < nchunk, else panel q = np-1-(e-nchunk)
     auto tile_row0 = [&](int e) -> int64_t {
         return e < nchunk ? t.u0 + (int64_t)kTile * (nchunk - 1 - e) : (int64_t)kPanel * (t.np - 1 - (e - nchunk));
@@ -892,97 +964,59 @@ __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restric
         const int64_t r0 = tile_row0(e);
         return (int)min<int64_t>(kTile, (e < nchunk ? (int64_t)t.u1 : t.ns) - r0);
     };
-    T lv[16], inv[16], zv[NR];
-    T xv[XQ];
-    int32_t ri[XQ], rc[XQ];                            // row indices of the next / current chunk
-    load_cols16(inv, Linv + t.iofs, nc, nc, nc, lane, w);
+    T lv[16], im[16], zv[NR / 4], acc[NR], xr[NR];
+    g16_load_t(im, Linv + t.iofs, nc, nc, nc, c, cg); // row c of L_pp^{-T}
     {
-        const T *zp = Z + (t.c0 + t.r0 + min(lane, nc - 1)) * nrhs;
+        const T *zp = Z + (t.c0 + t.r0 + min(c, nc - 1)) * nrhs;
 #pragma unroll
-        for (int r = 0; r < NR; ++r) zv[r] = zp[min(r, nrhs - 1)];
+        for (int rr = 0; rr < NR / 4; ++rr) zv[rr] = zp[min(rq + rr, nrhs - 1)];
     }
     if (tid == 0) failed = 0;
     if (ntile > 0) load_cols16(lv, Lc + tile_row0(0), t.ms, tile_rows(0), nc, lane, w);
-    if (nchunk > 0) {
-        const int nr0 = tile_rows(0);
-#pragma unroll
-        for (int j = 0; j < XQ; ++j) {
-            const int i = xi0 + (kBS / NR) * j;
-            rc[j] = rw[tile_row0(0) + min(i, nr0 - 1)];
-            xv[j] = ld_dev(X + (int64_t)rc[j] * nrhs + min(xr, nrhs - 1));
-        }
-        if (nchunk > 1) {
-            const int nr1 = tile_rows(1);
-#pragma unroll
-            for (int j = 0; j < XQ; ++j) ri[j] = rw[tile_row0(1) + min(xi0 + (kBS / NR) * j, nr1 - 1)];
-        }
-    }
-    T acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
+    bool ok = true;
     for (int e = 0; e < ntile; ++e) {
         const int nre = tile_rows(e);
-        __syncthreads();
-        if (e >= nchunk) {
-            const int q = t.np - 1 - (e - nchunk);
-            load_vec_wait<T, NR>(xs, X + (t.c0 + (int64_t)kPanel * q) * nrhs, nre, nrhs, tid, abort_flag, &failed);
-        } else {
+        T (*S)[kTile + 1] = Lt[e % NB];
 #pragma unroll
-            for (int j = 0; j < XQ; ++j) {
-                const int i = xi0 + (kBS / NR) * j;
-                const bool live = i < nre && xr < nrhs;
-                if (live && !await_value(xv[j], X + (int64_t)rc[j] * nrhs + min(xr, nrhs - 1), abort_flag)) failed = 1;
-                xs[i][xr] = live ? xv[j] : T(0);
-            }
-        }
-        stage_tile(Lt, lv, nre, nc, lane, w);
+        for (int k = 0; k < 16; ++k) S[lane][16 * w + k] = (lane < nre && 16 * w + k < nc) ? lv[k] : T(0);
         __syncthreads();
-        if (failed) return;
         if (e + 1 < ntile) load_cols16(lv, Lc + tile_row0(e + 1), t.ms, tile_rows(e + 1), nc, lane, w);
-        if (e + 1 < nchunk) {
+        T m[16];
 #pragma unroll
-            for (int j = 0; j < XQ; ++j) {
-                rc[j] = ri[j];
-                xv[j] = ld_dev(X + (int64_t)rc[j] * nrhs + min(xr, nrhs - 1));
-            }
-            if (e + 2 < nchunk) {
-                const int nr2 = tile_rows(e + 2);
-#pragma unroll
-                for (int j = 0; j < XQ; ++j)
-                    ri[j] = rw[tile_row0(e + 2) + min(xi0 + (kBS / NR) * j, nr2 - 1)];
-            }
+        for (int k = 0; k < 16; ++k) m[k] = S[16 * cg + k][c];
+        if (NB == 1) __syncthreads();                  // the next tile overwrites the only buffer
+        if (e < nchunk) {
+            const int64_t r0 = tile_row0(e);
+            ok = g16_load_wait<T, NR>(xr, [&](int k) { return X + (int64_t)rw[r0 + k] * nrhs; }, nre, nrhs, cg, j,
+                                      abort_flag) && ok;
+        } else {
+            const T *xq = X + (t.c0 + tile_row0(e)) * nrhs;
+            ok = g16_load_wait<T, NR>(xr, [&](int k) { return xq + (int64_t)k * nrhs; }, nre, nrhs, cg, j,
+                                      abort_flag) && ok;
         }
-        bwd_mul<T, NR>(acc, Lt, xs, lane, w);
+        g16_mac<T, NR>(acc, m, xr);
     }
-    if (w > 0) {
+    if (!ok) failed = 1;
+    g16_reduce<T, NR>(acc);
 #pragma unroll
-        for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
+    for (int rr = 0; rr < NR / 4; ++rr) {
+        const int r = rq + rr;
+        vx[c][r] = (c < nc && r < nrhs) ? zv[rr] - acc[r] : T(0);
     }
     __syncthreads();
-    if (w == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const T v = zv[r] - (((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
-            xs[lane][r] = (lane < nc && r < nrhs) ? v : T(0);
-        }
-    }
-    stage_tile(Lt, inv, nc, nc, lane, w);
-    __syncthreads();
+    if (failed) return;
+    g16_from_lds<T, NR>(xr, vx, nc, cg, j);
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
-    bwd_mul<T, NR>(acc, Lt, xs, lane, w);
-    __syncthreads();
-    if (w > 0) {
+    g16_mac<T, NR>(acc, im, xr);
+    g16_reduce<T, NR>(acc);
+    if (c < nc) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) red[w - 1][lane][r] = acc[r];
-    }
-    __syncthreads();
-    if (w == 0) {
-        if (lane < nc) {
-            T *dst = X + (t.c0 + t.r0 + lane) * nrhs;
-#pragma unroll
-            for (int r = 0; r < NR; ++r)
-                if (r < nrhs) st_dev(dst + r, ((acc[r] + red[0][lane][r]) + red[1][lane][r]) + red[2][lane][r]);
+        for (int rr = 0; rr < NR / 4; ++rr) {
+            const int r = rq + rr;
+            if (r < nrhs) st_dev(X + (t.c0 + t.r0 + c) * nrhs + r, acc[r]);
         }
     }
 }

@@ -372,7 +372,12 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
             double nc = (double)(mcol[c] + mcol[s]);
             double dz = dense(nc, (double)frow[s]);
             double z = 1.0 - (actual[c] + actual[s]) / dz;
-            bool ok = nc <= 4 || (nc <= 16 && z < 0.8) || (nc <= 48 && z < 0.1) || (nc <= 256 && z < 0.05);
+            // merged width nc in DOF columns, z the share of explicit zeros in the
+            // merged front.  Looser than the usual CPU settings (4/16/48 columns):
+            // on the GPU every supernode level costs launches in the factor and a
+            // dependency step in each triangular solve, while the extra
+            // arithmetic on small fronts is nearly free.
+            bool ok = nc <= 12 || (nc <= 24 && z < 0.8) || (nc <= 72 && z < 0.2) || (nc <= 256 && z < 0.05);
             if (ok) {
                 merged_into[c] = (int32_t)s;
                 std::vector<int32_t> mem = members[c];
