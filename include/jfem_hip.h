@@ -185,7 +185,7 @@ int jfem_factor_free(jfem_factor *F);
  * each step advancing one conjugate-gradient iteration on K_ff d = r,
  * preconditioned with the factor (triangular solves in its precision), until its
  * residual is below 1e-4 x r for every right-hand side still running (or
- * tol/2 x ||b||); then x += d and the iteration continues from the new exact
+ * 0.9 tol x ||b||); then x += d and the iteration continues from the new exact
  * residual, keeping its search directions; until
  * ||K_ff x - b|| / max(||b||, 1e-30) <= tol or the residual no longer halves.
  * max_refine bounds the triangular solves to max_refine + 1 per call;

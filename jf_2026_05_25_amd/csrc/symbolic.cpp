@@ -634,7 +634,17 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         }
         S.iofs[k + 1] = S.iofs[k] + e;
     }
-    S.inv_entries = S.iofs[nsn];
+    // chain products after the inverses: per supernode, boundary q between
+    // panels q and q+1 holds [M_{q+1}, G_q], 2 x 64 x 64 entries
+    std::vector<int64_t> bofs(nsn, -1);
+    int64_t nbnd = S.iofs[nsn];
+    for (int64_t k = 0; k < nsn; ++k) {
+        const int64_t np = (S.ncol[k] + kPanel - 1) / kPanel;
+        if (np < 2) continue;
+        bofs[k] = nbnd;
+        nbnd += (np - 1) * 2 * kPanel * kPanel;
+    }
+    S.inv_entries = nbnd;
     auto whole = [&](int64_t s) { return S.ncol[s] <= kPanel && S.m[s] <= kWholeRows; };
     std::vector<int32_t> need(nsn, 0);
     for (int64_t k = 0; k < nsn; ++k)
@@ -673,6 +683,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
             const int32_t s = S.level_sn[q];
             FlowTask t = base(s);
+            t.bofs = -1;
             if (whole(s)) {
                 t.kind = 1;
                 t.p = 0; t.r0 = 0; t.nr = (int32_t)S.ncol[s];
@@ -685,9 +696,11 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 t.r0 = p * kPanel;
                 t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
                 t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
+                t.bofs = p >= 1 ? bofs[s] + 2 * (int64_t)kPanel * kPanel * (p - 1) : -1;     // M_p
                 add_fwd(t, (int64_t)t.r0 + t.nr);
             }
             t.iofs = 0;
+            t.bofs = -1;
             for (int32_t u = 0; u < nupd(s); ++u) {
                 t.p = -1 - u;
                 t.r0 = (int32_t)(S.ncol[s] + (int64_t)u * kTile);
@@ -706,6 +719,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 t.r0 = p * kPanel;
                 t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
                 t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
+                t.bofs = p + 1 < t.np ? bofs[s] + 2 * (int64_t)kPanel * kPanel * p + kPanel * kPanel : -1;   // G_p
                 S.flow_b.push_back(t);
             }
         }

@@ -28,6 +28,12 @@ struct FlowTask {
     int32_t u0, u1;                 // backward: update rows [u0, u1) of the front
     int64_t ns, ms, lofs, c0, relofs, rofs;
     int64_t iofs;                   // inverse of the diagonal block of panel p
+    // chain product of the task's last awaited panel (-1 if none): forward pivot
+    // task, p >= 1: M_p = L_pp^{-1} L[p rows, p-1 cols] (64 x 64, column-major);
+    // backward task, p < np-1: G_p = (L[p+1 rows, p cols] L_pp^{-1})^T (64 x 64,
+    // column-major).  Both live in the inverse array after the diagonal inverses:
+    // per supernode and panel boundary q, [M_{q+1}, G_q].
+    int64_t bofs;
     int64_t gb;                     // forward: first FlowChild entry
 };
 // per (forward task, child of its supernode), in child order: the child's update
@@ -74,7 +80,7 @@ struct SymbolicHost {
     std::vector<FlowTask> flow_f, flow_b;
     std::vector<FlowChild> flow_gb;
     std::vector<int64_t> iofs;
-    int64_t inv_entries = 0;
+    int64_t inv_entries = 0;        // diagonal-block inverses, then the chain products (bofs)
     int64_t L_entries = 0, U_entries = 0, upd_rows = 0;
     double flops = 0;
     double update_flops = 0;   // 2 x multiply-adds of the update tiles (lower triangles)

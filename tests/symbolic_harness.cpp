@@ -139,6 +139,28 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 for (int i = 0; i < b; ++i) Linv[t.iofs + i + (int64_t)b * c] = xc[i];
             }
         }
+        // chain products: boundary q of s (backward task (s, q), q < np-1):
+        // M_{q+1} = L_{q+1,q+1}^{-1} A at bofs - 4096, G_q = (A L_qq^{-1})^T at bofs,
+        // A = L[q+1 rows, q cols], both 64 x 64 column-major
+        const int64_t PP = (int64_t)kPanel * kPanel;
+        for (const FlowTask &t : S.flow_b) {
+            if ((t.bofs >= 0) != (t.p + 1 < t.np)) return 10;
+            if (t.bofs < 0) continue;
+            if (t.bofs - PP < 0 || t.bofs + PP > S.inv_entries || t.nr != kPanel) return 10;
+            const int b1 = (int)std::min<int64_t>(kPanel, t.ns - (int64_t)kPanel * (t.p + 1));
+            if (t.iofs + PP + (int64_t)b1 * b1 > S.inv_entries) return 10;
+            auto A = [&](int k, int m) { return k < b1 ? L[t.lofs + (kPanel * (t.p + 1) + k) + t.ms * (kPanel * t.p + m)] : 0.0; };
+            for (int c = 0; c < kPanel; ++c)
+                for (int i = 0; i < kPanel; ++i) {
+                    double sm = 0, sg = 0;
+                    if (i < b1)
+                        for (int m = 0; m <= i; ++m) sm += Linv[t.iofs + PP + i + (int64_t)b1 * m] * A(m, c);
+                    if (c < b1)
+                        for (int m = i; m < kPanel; ++m) sg += A(c, m) * Linv[t.iofs + m + (int64_t)kPanel * i];
+                    Linv[t.bofs - PP + i + kPanel * c] = sm;
+                    Linv[t.bofs + i + kPanel * c] = sg;
+                }
+        }
         const int64_t nsn = S.ns;
         std::vector<int32_t> ready(nsn, 0), zdone(nsn, 0), xdone(nsn, 0);
         std::vector<double> Y2(n * nrhs), W2(nw * nrhs, 0.0), Z2(n * nrhs, 0.0), X2(n * nrhs, 0.0);
@@ -155,7 +177,10 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
         for (size_t e = 0; e < S.flow_f.size(); ++e) {
             const FlowTask &t = S.flow_f[e];
             const bool pivot = t.p >= 0;
-            const int kend = pivot ? t.p : t.np;
+            const bool chain = pivot && t.bofs >= 0;
+            if (chain != (pivot && t.kind != 1 && t.p >= 1)) return 10;
+            if (chain && (t.bofs < 0 || t.bofs + PP > S.inv_entries)) return 10;
+            const int kend = pivot ? (chain ? t.p - 1 : t.p) : t.np;
             const int nr = t.nr;
             if (t.s < 0 || t.s >= nsn || nr < 1 || nr > kTile) return 10;
             if (pivot && (t.r0 != kPanel * t.p || t.r0 + nr > t.ns)) return 10;
@@ -260,7 +285,14 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 for (int r = 0; r < nrhs; ++r) {
                     const double *a = &acc[i * nrhs + r];
                     const int64_t st = (int64_t)kTile * nrhs;
-                    Z2[zi * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                    double z = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                    if (chain)      // z_p = u - M_p z_{p-1}
+                        for (int c = 0; c < kPanel; ++c) {
+                            const int64_t zc = t.c0 + t.r0 - kPanel + c;
+                            if (!wZ2[zc]) return 11;
+                            z -= Linv[t.bofs + i + kPanel * c] * Z2[zc * nrhs + r];
+                        }
+                    Z2[zi * nrhs + r] = z;
                 }
             }
             zdone[t.s] = t.p + 1;
@@ -275,7 +307,8 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             std::vector<double> acc(4 * kTile * nrhs, 0.0);
             const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;
             const int ntile = nchunk + (t.np - 1 - t.p);
-            for (int e2 = 0; e2 < ntile; ++e2) {
+            const bool chain = t.bofs >= 0;              // panel p+1 (the last tile) through G_p
+            for (int e2 = 0; e2 < (chain ? ntile - 1 : ntile); ++e2) {
                 const bool upd = e2 < nchunk;
                 const int q = upd ? -1 : t.np - 1 - (e2 - nchunk);
                 const int64_t row0 = upd ? t.u0 + (int64_t)kTile * (nchunk - 1 - e2) : (int64_t)kPanel * q;
@@ -310,6 +343,7 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                     for (int r = 0; r < nrhs; ++r)
                         acc[((i / 16) * kTile + c) * nrhs + r] += Linv[t.iofs + i + (int64_t)nc * c] * v[i * nrhs + r];
             if (xdone[t.s] != t.np - 1 - t.p) return 9;
+            const int b1 = chain ? (int)std::min<int64_t>(kPanel, t.ns - t.r0 - kPanel) : 0;
             for (int c = 0; c < nc; ++c) {
                 const int64_t xi = t.c0 + t.r0 + c;
                 if (wX2[xi]) return 11;
@@ -317,7 +351,13 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 for (int r = 0; r < nrhs; ++r) {
                     const double *a = &acc[c * nrhs + r];
                     const int64_t st = (int64_t)kTile * nrhs;
-                    X2[xi * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                    double x = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                    for (int k = 0; k < b1; ++k) {     // x_p = u - G_p x_{p+1}
+                        const int64_t xk = t.c0 + t.r0 + kPanel + k;
+                        if (!wX2[xk]) return 11;
+                        x -= Linv[t.bofs + c + kPanel * k] * X2[xk * nrhs + r];
+                    }
+                    X2[xi * nrhs + r] = x;
                 }
             }
             xdone[t.s] += 1;
