@@ -188,19 +188,42 @@ struct Q4Shared {
     double lc[8], Cm[9], Cb[9], Cs[4], R[9];
     double h, k6, ds;
     int rigid, modes, do_shear, status;
-    double gdx[4][4], gdy[4][4], gw[4];
-    double Bi[4][12], Bib[4][12];      // per Gauss point, 3x4 column-major
-    double Kab[96], Kbb[16], Kabs[96], Kbbs[16], Kabb[96], Kbbb[16];
-    double Cim[16], Cib[16];
-    double Pm[96], Pb[96];
-    double Dm[48], Xm[48];             // MacNeal D (4x12) and Z \ D
-    double Kp[144];
-    double Bd[4][24], dcoef[4];
     double zl[4];                      // corner heights of a warped cell (finite-warp map)
     int warped;
+    // the element's working arrays are dead once its blocks are formed, when the
+    // finite-warp map and K*Tg (warped cells only) take their place: one wavefront
+    // per element runs these phases in program order
+    union {
+        struct {
+            double gdx[4][4], gdy[4][4], gw[4];
+            double Bi[4][12], Bib[4][12];      // per Gauss point, 3x4 column-major
+            double Kab[96], Kbb[16], Kabs[96], Kbbs[16], Kabb[96], Kbbb[16];
+            double Cim[16], Cib[16];
+            double Pm[96], Pb[96];
+            double Dm[48], Xm[48];             // MacNeal D (4x12) and Z \ D
+            double Kp[144];
+            double Bd[4][24], dcoef[4];
+        };
+        struct {
+            double Tg[576], W[576];
+        };
+    };
     double out[576];
-    double Tg[576], W[576];            // finite-warp map and K*Tg (warped cells only)
 };
+
+// Per-element quantities of the serial set-up (frame, section, MacNeal tying
+// operators), computed one element per thread by the prep kernels and handed to
+// the wavefront kernel through the first kQ4Prep doubles of the element's own
+// output slot (overwritten by the element matrix at the end).
+struct Q4Prep {
+    double lc[8], Cm[9], Cb[9], Cs[4], R[9], zl[4];
+    double h, k6, ds;
+    int rigid, modes, do_shear, warped;
+    double Dm[48], Xm[48];
+};
+// record layout: lc 0, Cm 8, Cb 17, Cs 26, R 30, zl 39, h 43, k6 44, ds 45,
+// rigid 46, modes 47, do_shear 48, warped 49, status 50, Dm 51, Xm 99
+constexpr int kQ4Prep = 147;
 
 // B-matrix entries as functions of the local DOF d = 6k + c
 __device__ inline double bm_entry(const Q4Shared &S, int g, int l, int d) {
@@ -220,7 +243,8 @@ __device__ inline double bb_entry(const Q4Shared &S, int g, int l, int d) {
 
 // MacNeal transverse-shear block (one lane): add_quad4_macneal_shear_rbf! default slice,
 // src/FEMKernels.jl:4495-4834, 6188-6200; quad4_natural_shear_flexibility! :4223-4352
-__device__ bool q4_macneal(Q4Shared &S) {
+template <class St>
+__device__ bool q4_macneal(St &S) {
     const double *lc = S.lc;
     const double pt = 1.0 / sqrt(3.0);
     const double sxi[4] = {0.0, 0.0, -pt, pt}, seta[4] = {-pt, pt, 0.0, 0.0};
@@ -348,7 +372,8 @@ __device__ inline int plate_index(int d) {  // local DOF -> MacNeal plate index 
 // Per-wavefront CQUAD4 kernel body.  p: 4 node xyz (p[3k+c]); prop: section row.
 // global_frame: compute frame + section from p/prop; otherwise use lc/C* given.
 __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int64_t *status, double *Ke_out) {
-    // Phase 1: Gauss-point rows (lanes 0..3), MacNeal (lane 63), drilling rows (lanes 59..62)
+    // Phase 1: Gauss-point rows (lanes 0..3), drilling rows (lanes 59..62); the
+    // MacNeal operators Dm, Xm come with the prep record
     const double pt = 1.0 / sqrt(3.0);
     const double gr[4] = {-pt, pt, pt, -pt}, gs[4] = {-pt, -pt, pt, pt};
     const double dNr_c[4] = {-0.25, 0.25, 0.25, -0.25}, dNs_c[4] = {-0.25, -0.25, 0.25, 0.25};
@@ -390,9 +415,6 @@ __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int6
         Bb[0 + 3 * 2] = b1x;  Bb[0 + 3 * 3] = b2x;
         Bb[2 + 3 * 0] = -b1x; Bb[2 + 3 * 1] = -b2x;
         Bb[2 + 3 * 2] = b1y;  Bb[2 + 3 * 3] = b2y;
-    }
-    if (valid && lane == 63 && S.do_shear) {
-        if (!q4_macneal(S)) S.status = JFEM_ERR_DEGENERATE;
     }
     if (valid && lane >= 59 && lane <= 62) {  // lumped drilling rows, src/FEMKernels.jl:3772-3817
         int i = lane - 59;
@@ -496,11 +518,14 @@ __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int6
         }
     }
     __syncthreads();
-    // Phase 5: every lane forms its 3x3 block (A, B), then rotates it
-    double blk[9];
+    // Phase 5: every lane forms its 3x3 block (A, B) in S.out, one entry at a time
+    // (the entry loops stay rolled: unrolled, they pushed the kernel past the
+    // register file), then rotates it
     if (valid) {
         const int A = lane / 8, B = lane % 8;
+#pragma unroll 1
         for (int jj = 0; jj < 3; ++jj)
+#pragma unroll 1
             for (int ii = 0; ii < 3; ++ii) {
                 const int i = 3 * A + ii, j = 3 * B + jj;
                 double v = 0.0;
@@ -534,11 +559,8 @@ __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int6
                         e += S.Pm[k + 4 * i] * S.Kbb[k + 4 * l] * S.Pm[l + 4 * j] +
                              S.Pb[k + 4 * i] * S.Kbbb[k + 4 * l] * S.Pb[l + 4 * j];
                 }
-                blk[ii + 3 * jj] = v + e;
+                S.out[i + 24 * j] = v + e;
             }
-        if (S.warped)
-            for (int jj = 0; jj < 3; ++jj)
-                for (int ii = 0; ii < 3; ++ii) S.out[(3 * A + ii) + 24 * (3 * B + jj)] = blk[ii + 3 * jj];
     }
     // finite-warp work-dual congruence Ke <- sym(Tg' Ke Tg) for warped cells
     // (apply_quad4_finite_warp_equilibrium!, src/FEMKernels.jl:1082-1116; map
@@ -608,13 +630,12 @@ __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int6
     __syncthreads();
     if (valid) {
         const int A = lane / 8, B = lane % 8;
-        if (S.warped) {
-            for (int jj = 0; jj < 3; ++jj)
-                for (int ii = 0; ii < 3; ++ii) {
-                    const int i = 3 * A + ii, j = 3 * B + jj;
-                    blk[ii + 3 * jj] = 0.5 * (S.out[i + 24 * j] + S.out[j + 24 * i]);
-                }
-        }
+        double blk[9];
+        for (int jj = 0; jj < 3; ++jj)
+            for (int ii = 0; ii < 3; ++ii) {
+                const int i = 3 * A + ii, j = 3 * B + jj;
+                blk[ii + 3 * jj] = S.warped ? 0.5 * (S.out[i + 24 * j] + S.out[j + 24 * i]) : S.out[i + 24 * j];
+            }
         // R' * blk * R, R rows = (v1, v2, v3) (src/solver/assembly.jl:392-424)
         const double *R = S.R;
         double t[9];
@@ -644,7 +665,8 @@ __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int6
 // frame + local coordinates + section for a CQUAD4 in global coordinates
 // (shell_element_frame_quad4 diag mode, src/solver/helpers.jl:1422-1481;
 //  src/solver/assembly.jl:3686-3691, 4392-4426)
-__device__ void q4_setup(Q4Shared &S, const double *p, const double *prop, double k6rot) {
+template <class St>
+__device__ void q4_setup(St &S, const double *p, const double *prop, double k6rot) {
     const double *p1 = p, *p2 = p + 3, *p3 = p + 6, *p4 = p + 9;
     double gxi[3], geta[3], vr[3], d13[3], d24[3], v1[3], v2[3], v3[3];
     for (int c = 0; c < 3; ++c) {
@@ -747,59 +769,101 @@ __device__ void q4_setup(Q4Shared &S, const double *p, const double *prop, doubl
     S.do_shear = !(S.h < 1e-30 || (!S.rigid && mcs < 1e-30));
 }
 
-__global__ void __launch_bounds__(kWave * kElemsPerBlock)
-q4_global_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
-                 const double *__restrict__ z, const int32_t *__restrict__ conn,
-                 const double *__restrict__ prop, double k6rot, double *__restrict__ Ke,
-                 int64_t *status) {
-    __shared__ Q4Shared sh[kElemsPerBlock];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int64_t e = (int64_t)blockIdx.x * kElemsPerBlock + wave;
-    const bool valid = e < n;
-    Q4Shared &S = sh[wave];
-    if (valid) {
-        // every lane derives the same frame/section; lane 0 publishes it
-        double p[12], pr[8];
-        for (int k = 0; k < 4; ++k) {
-            int nd = conn[k * n + e];
-            p[3 * k] = x[nd]; p[3 * k + 1] = y[nd]; p[3 * k + 2] = z[nd];
-        }
-        for (int c = 0; c < 8; ++c) pr[c] = prop[c * n + e];
-        if (lane == 0) {
-            S.status = 0;
-            q4_setup(S, p, pr, k6rot);
-        }
-    }
-    __syncthreads();
-    q4_element(S, lane, valid, e, status, Ke);
+__device__ inline void q4_prep_store(const Q4Prep &P, int st, double *d) {
+    for (int q = 0; q < 8; ++q) d[q] = P.lc[q];
+    for (int q = 0; q < 9; ++q) { d[8 + q] = P.Cm[q]; d[17 + q] = P.Cb[q]; }
+    for (int q = 0; q < 4; ++q) d[26 + q] = P.Cs[q];
+    for (int q = 0; q < 9; ++q) d[30 + q] = P.R[q];
+    for (int q = 0; q < 4; ++q) d[39 + q] = P.zl[q];
+    d[43] = P.h; d[44] = P.k6; d[45] = P.ds;
+    d[46] = P.rigid; d[47] = P.modes; d[48] = P.do_shear; d[49] = P.warped; d[50] = st;
+    for (int q = 0; q < 48; ++q) { d[51 + q] = P.Dm[q]; d[99 + q] = P.Xm[q]; }
 }
 
-__global__ void __launch_bounds__(kWave * kElemsPerBlock)
-q4_local_kernel(int64_t n, const double *__restrict__ lc, const double *__restrict__ Cm,
-                const double *__restrict__ Cb, const double *__restrict__ Cs,
-                const double *__restrict__ h, const double *__restrict__ k6,
-                const double *__restrict__ ds, const int32_t *__restrict__ flags,
-                double *__restrict__ Ke, int64_t *status) {
+// the wavefront's lanes copy an element's prep record into its LDS state
+__device__ inline void q4_prep_load(Q4Shared &S, const double *d, int lane) {
+    for (int q = lane; q < kQ4Prep; q += kWave) {
+        const double v = d[q];
+        if (q < 8) S.lc[q] = v;
+        else if (q < 17) S.Cm[q - 8] = v;
+        else if (q < 26) S.Cb[q - 17] = v;
+        else if (q < 30) S.Cs[q - 26] = v;
+        else if (q < 39) S.R[q - 30] = v;
+        else if (q < 43) S.zl[q - 39] = v;
+        else if (q == 43) S.h = v;
+        else if (q == 44) S.k6 = v;
+        else if (q == 45) S.ds = v;
+        else if (q == 46) S.rigid = (int)v;
+        else if (q == 47) S.modes = (int)v;
+        else if (q == 48) S.do_shear = (int)v;
+        else if (q == 49) S.warped = (int)v;
+        else if (q == 50) S.status = (int)v;
+        else if (q < 99) S.Dm[q - 51] = v;
+        else S.Xm[q - 99] = v;
+    }
+}
+
+// MacNeal tying operators of a prepared element (zero when it has no shear block)
+__device__ inline int q4_prep_shear(Q4Prep &P) {
+    for (int q = 0; q < 48; ++q) { P.Dm[q] = 0.0; P.Xm[q] = 0.0; }
+    return (P.do_shear && !q4_macneal(P)) ? JFEM_ERR_DEGENERATE : 0;
+}
+
+// serial set-up of CQUAD4s in global coordinates, one element per thread
+__global__ void __launch_bounds__(64)
+q4_prep_global_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
+                      const double *__restrict__ z, const int32_t *__restrict__ conn,
+                      const double *__restrict__ prop, double k6rot, double *__restrict__ Ke) {
+    const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (e >= n) return;
+    double p[12], pr[8];
+    for (int k = 0; k < 4; ++k) {
+        int nd = conn[k * n + e];
+        p[3 * k] = x[nd]; p[3 * k + 1] = y[nd]; p[3 * k + 2] = z[nd];
+    }
+    for (int c = 0; c < 8; ++c) pr[c] = prop[c * n + e];
+    Q4Prep P;
+    q4_setup(P, p, pr, k6rot);
+    const int st = q4_prep_shear(P);
+    q4_prep_store(P, st, Ke + e * 576);
+}
+
+// serial set-up of CQUAD4s given in local coordinates with their sections
+__global__ void __launch_bounds__(64)
+q4_prep_local_kernel(int64_t n, const double *__restrict__ lc, const double *__restrict__ Cm,
+                     const double *__restrict__ Cb, const double *__restrict__ Cs,
+                     const double *__restrict__ h, const double *__restrict__ k6,
+                     const double *__restrict__ ds, const int32_t *__restrict__ flags,
+                     double *__restrict__ Ke) {
+    const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (e >= n) return;
+    Q4Prep P;
+    for (int q = 0; q < 8; ++q) P.lc[q] = lc[8 * e + q];
+    for (int q = 0; q < 9; ++q) { P.Cm[q] = Cm[9 * e + q]; P.Cb[q] = Cb[9 * e + q]; }
+    for (int q = 0; q < 4; ++q) P.Cs[q] = Cs[4 * e + q];
+    for (int q = 0; q < 9; ++q) P.R[q] = (q % 4 == 0) ? 1.0 : 0.0;
+    for (int q = 0; q < 4; ++q) P.zl[q] = 0.0;
+    P.h = h[e]; P.k6 = k6[e]; P.ds = ds[e];
+    P.warped = 0;
+    const int f = flags[e];
+    P.rigid = f & 1;
+    const int modes = (f >> 4) & 15;
+    P.modes = modes == 0 ? 15 : modes;
+    const double mcs = fmax(fmax(fabs(P.Cs[0]), fabs(P.Cs[1])), fmax(fabs(P.Cs[2]), fabs(P.Cs[3])));
+    P.do_shear = !(P.h < 1e-30 || (!P.rigid && mcs < 1e-30));
+    const int st = q4_prep_shear(P);
+    q4_prep_store(P, st, Ke + e * 576);
+}
+
+// the lane-parallel part: one wavefront per prepared element, result over the record
+__global__ void __launch_bounds__(kWave * kElemsPerBlock, 2)
+q4_elem_kernel(int64_t n, double *__restrict__ Ke, int64_t *status) {
     __shared__ Q4Shared sh[kElemsPerBlock];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t e = (int64_t)blockIdx.x * kElemsPerBlock + wave;
     const bool valid = e < n;
     Q4Shared &S = sh[wave];
-    if (valid && lane == 0) {
-        S.status = 0;
-        for (int q = 0; q < 8; ++q) S.lc[q] = lc[8 * e + q];
-        for (int q = 0; q < 9; ++q) { S.Cm[q] = Cm[9 * e + q]; S.Cb[q] = Cb[9 * e + q]; }
-        for (int q = 0; q < 4; ++q) S.Cs[q] = Cs[4 * e + q];
-        for (int q = 0; q < 9; ++q) S.R[q] = (q % 4 == 0) ? 1.0 : 0.0;
-        S.h = h[e]; S.k6 = k6[e]; S.ds = ds[e];
-        S.warped = 0;
-        int f = flags[e];
-        S.rigid = f & 1;
-        int modes = (f >> 4) & 15;
-        S.modes = modes == 0 ? 15 : modes;
-        double mcs = fmax(fmax(fabs(S.Cs[0]), fabs(S.Cs[1])), fmax(fabs(S.Cs[2]), fabs(S.Cs[3])));
-        S.do_shear = !(S.h < 1e-30 || (!S.rigid && mcs < 1e-30));
-    }
+    if (valid) q4_prep_load(S, Ke + e * 576, lane);
     __syncthreads();
     q4_element(S, lane, valid, e, status, Ke);
 }
@@ -1279,9 +1343,10 @@ hipError_t launch_reset_status(hipStream_t s, int64_t *status) {
 
 hipError_t launch_q4_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status) {
     if (m->nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(q4_prep_global_kernel, dim3((unsigned)((m->nq + 63) / 64)), dim3(64), 0, s, m->nq, m->d_x, m->d_y,
+                       m->d_z, m->d_q, m->d_qp, m->k6rot, Ke);
     dim3 grid((unsigned)((m->nq + kElemsPerBlock - 1) / kElemsPerBlock));
-    hipLaunchKernelGGL(q4_global_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, m->nq, m->d_x, m->d_y, m->d_z,
-                       m->d_q, m->d_qp, m->k6rot, Ke, status);
+    hipLaunchKernelGGL(q4_elem_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, m->nq, Ke, status);
     return hipGetLastError();
 }
 
@@ -1305,9 +1370,10 @@ hipError_t launch_q4_local(hipStream_t s, int64_t n, const double *lc, const dou
                            const double *Cs, const double *h, const double *k6, const double *ds,
                            const int32_t *flags, double *Ke, int64_t *status) {
     if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(q4_prep_local_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, lc, Cm, Cb, Cs, h, k6,
+                       ds, flags, Ke);
     dim3 grid((unsigned)((n + kElemsPerBlock - 1) / kElemsPerBlock));
-    hipLaunchKernelGGL(q4_local_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, n, lc, Cm, Cb, Cs, h, k6, ds,
-                       flags, Ke, status);
+    hipLaunchKernelGGL(q4_elem_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, n, Ke, status);
     return hipGetLastError();
 }
 
