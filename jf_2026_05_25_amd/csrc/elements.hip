@@ -520,29 +520,55 @@ __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int6
     __syncthreads();
     // Phase 5: every lane forms its 3x3 block (A, B) in S.out, one entry at a time
     // (the entry loops stay rolled: unrolled, they pushed the kernel past the
-    // register file), then rotates it
+    // register file).  Rows 3A..3A+2 are the translations (A even) or rotations
+    // (A odd) of node A/2: membrane B-rows and the membrane enrichment are nonzero
+    // only for translations u, v, the bending ones only for rotations rx, ry, so a
+    // lane adds the membrane terms when A and B are both even, the bending terms
+    // when both are odd, in the order of the full sums (the skipped terms are
+    // exact zeros).  The Gauss-point derivatives of its two nodes sit in registers.
     if (valid) {
         const int A = lane / 8, B = lane % 8;
+        const int kA = A >> 1, kB = B >> 1;
+        const bool mem = !(A & 1) && !(B & 1), bend = (A & 1) && (B & 1);
+        double ax[4], ay[4], bx[4], by[4], gw[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            ax[g] = S.gdx[g][kA]; ay[g] = S.gdy[g][kA];
+            bx[g] = S.gdx[g][kB]; by[g] = S.gdy[g][kB];
+            gw[g] = S.gw[g];
+        }
+        const double *C = mem ? S.Cm : S.Cb;
+        const double *Ka = mem ? S.Kab : S.Kabb, *Pa = mem ? S.Pm : S.Pb, *Kb = mem ? S.Kbb : S.Kbbb;
 #pragma unroll 1
         for (int jj = 0; jj < 3; ++jj)
 #pragma unroll 1
             for (int ii = 0; ii < 3; ++ii) {
                 const int i = 3 * A + ii, j = 3 * B + jj;
                 double v = 0.0;
-                // Gauss-point membrane and bending terms, src/FEMKernels.jl:3528-3532
-                for (int g = 0; g < 4; ++g) {
-                    double w = S.gw[g];
-                    double tm[3], tb[3];
-                    for (int l = 0; l < 3; ++l) {
-                        double a = 0.0, b = 0.0;
-                        for (int lp = 0; lp < 3; ++lp) {
-                            a += S.Cm[l + 3 * lp] * bm_entry(S, g, lp, j);
-                            b += S.Cb[l + 3 * lp] * bb_entry(S, g, lp, j);
+                if (mem || bend) {
+                    // B-row entries (l = 0..2) of DOF i / j at Gauss point g: membrane
+                    // u: (dx, 0, dy), v: (0, dy, dx); bending rx: (0, -dy, -dx), ry: (dx, 0, dy)
+                    for (int g = 0; g < 4; ++g) {
+                        double bi[3], bj[3];
+                        const int ci = mem ? ii : ii + 3, cj = mem ? jj : jj + 3;
+                        const double dxi = ax[g], dyi = ay[g], dxj = bx[g], dyj = by[g];
+                        if (ci == 0 || ci == 4) { bi[0] = dxi; bi[1] = 0.0; bi[2] = dyi; }
+                        else if (ci == 1) { bi[0] = 0.0; bi[1] = dyi; bi[2] = dxi; }
+                        else if (ci == 3) { bi[0] = 0.0; bi[1] = -dyi; bi[2] = -dxi; }
+                        else { bi[0] = 0.0; bi[1] = 0.0; bi[2] = 0.0; }
+                        if (cj == 0 || cj == 4) { bj[0] = dxj; bj[1] = 0.0; bj[2] = dyj; }
+                        else if (cj == 1) { bj[0] = 0.0; bj[1] = dyj; bj[2] = dxj; }
+                        else if (cj == 3) { bj[0] = 0.0; bj[1] = -dyj; bj[2] = -dxj; }
+                        else { bj[0] = 0.0; bj[1] = 0.0; bj[2] = 0.0; }
+                        const double w = gw[g];
+                        double t[3];
+                        for (int l = 0; l < 3; ++l) {
+                            double a = 0.0;
+                            for (int lp = 0; lp < 3; ++lp) a += C[l + 3 * lp] * bj[lp];
+                            t[l] = a;
                         }
-                        tm[l] = a; tb[l] = b;
+                        for (int l = 0; l < 3; ++l) v += bi[l] * (w * t[l]);
                     }
-                    for (int l = 0; l < 3; ++l) v += bm_entry(S, g, l, i) * (w * tm[l]);
-                    for (int l = 0; l < 3; ++l) v += bb_entry(S, g, l, i) * (w * tb[l]);
                 }
                 // MacNeal plate block, src/FEMKernels.jl:6279-6285
                 int pi = plate_index(i), pj = plate_index(j);
@@ -551,13 +577,13 @@ __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int6
                 for (int c = 0; c < 4; ++c) v += S.Bd[c][i] * (S.dcoef[c] * S.Bd[c][j]);
                 // section enrichment, src/FEMKernels.jl:2005-2021
                 double e = 0.0;
-                for (int k = 0; k < 4; ++k) {
-                    double qim = S.Kab[i + 24 * k], qjm = S.Kab[j + 24 * k];
-                    double qib = S.Kabb[i + 24 * k], qjb = S.Kabb[j + 24 * k];
-                    e += qim * S.Pm[k + 4 * j] + S.Pm[k + 4 * i] * qjm + qib * S.Pb[k + 4 * j] + S.Pb[k + 4 * i] * qjb;
-                    for (int l = 0; l < 4; ++l)
-                        e += S.Pm[k + 4 * i] * S.Kbb[k + 4 * l] * S.Pm[l + 4 * j] +
-                             S.Pb[k + 4 * i] * S.Kbbb[k + 4 * l] * S.Pb[l + 4 * j];
+                if (mem || bend) {
+                    for (int k = 0; k < 4; ++k) {
+                        const double qi = Ka[i + 24 * k], qj = Ka[j + 24 * k];
+                        const double pik = Pa[k + 4 * i];
+                        e += qi * Pa[k + 4 * j] + pik * qj;
+                        for (int l = 0; l < 4; ++l) e += pik * Kb[k + 4 * l] * Pa[l + 4 * j];
+                    }
                 }
                 S.out[i + 24 * j] = v + e;
             }
