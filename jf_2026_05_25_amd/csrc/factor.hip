@@ -90,7 +90,7 @@ __global__ void scatter_kernel(int64_t n, const int32_t *__restrict__ src, const
 // One workgroup per (parent, 64 front columns, 512 front rows); the children are
 // added in child order.  The child entries that land in the task's block are the
 // update columns [j0, j1) x update rows [max(j, i0), i1), precomputed per
-// (task, child).  Each thread handles 4 independent entries per pass so their
+// (task, child).  Each thread handles 8 independent entries per pass so their
 // loads overlap.
 template <class T>
 __global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restrict__ tasks, const int64_t *__restrict__ boff,
@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restric
                                                          const int64_t *__restrict__ uofs, const int64_t *__restrict__ relofs,
                                                          const int32_t *__restrict__ relmap, const int32_t *__restrict__ child_ptr,
                                                          const int32_t *__restrict__ child, T *__restrict__ L, T *__restrict__ U) {
-    constexpr int kU = 4;
+    constexpr int kU = 8;
     const EaTask t = tasks[blockIdx.x];
     const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s], mu = ms - ns;
@@ -115,18 +115,23 @@ __global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restric
         const int64_t ib = max<int64_t>(bd[2], j0);
         const int64_t nrow = i1 - ib, nj = j1 - j0;
         if (nrow > 0 && nj > 0) {
-            const int64_t total = nj * nrow;
-            for (int64_t base = threadIdx.x; base < total; base += kU * kBS) {
+            // block of at most 64 columns x kEaRows rows: 32-bit indices
+            const int total = (int)(nj * nrow), nr32 = (int)nrow;
+            for (int base = threadIdx.x; base < total; base += kU * kBS) {
                 int64_t jj[kU], ii[kU];
                 bool ok[kU];
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
-                    const int64_t qq = base + (int64_t)u * kBS;
-                    const int64_t qc = qq < total ? qq : 0;
-                    jj[u] = j0 + qc / nrow;
-                    ii[u] = ib + qc % nrow;
+                    const int qq = base + u * kBS;
+                    const int qc = qq < total ? qq : total - 1;
+                    const int jq = qc / nr32;
+                    jj[u] = j0 + jq;
+                    ii[u] = ib + (qc - jq * nr32);
                     ok[u] = qq < total && ii[u] >= jj[u];
                 }
+                // all loads of the pass before the first wait: indices and child
+                // entries, then the parent entries (valid addresses also for the
+                // entries that are skipped)
                 int32_t pr[kU], pc[kU];
                 T v[kU];
 #pragma unroll
@@ -135,14 +140,16 @@ __global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restric
                     pc[u] = rel[jj[u]];
                     v[u] = Uc[ii[u] + nu * jj[u]];
                 }
+                T *dst[kU];
+                T dv[kU];
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
-                    if (ok[u]) {
-                        T *dst = pc[u] < ns ? (Ls + pr[u] + ms * (int64_t)pc[u])
-                                            : (Us + (pr[u] - ns) + mu * (int64_t)(pc[u] - ns));
-                        *dst += v[u];
-                    }
+                    dst[u] = pc[u] < ns ? (Ls + pr[u] + ms * (int64_t)pc[u]) : (Us + (pr[u] - ns) + mu * (int64_t)(pc[u] - ns));
+                    dv[u] = *dst[u];
                 }
+#pragma unroll
+                for (int u = 0; u < kU; ++u)
+                    if (ok[u]) *dst[u] = dv[u] + v[u];
             }
         }
         __syncthreads();
@@ -467,17 +474,41 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
     const int nc = (int)(min<int64_t>(c0 + kTile, cb) - c0);
     const T *Ls = L + lofs[s];
     const int tid = threadIdx.x;
-    for (int q = tid; q < kTile * b4; q += blockDim.x) {
-        int i = q % kTile, p = q / kTile;
-        bool inp = p < b;
-        A[p][i] = (inp && i < nr) ? Ls[(r0 + i) + ms * (k + p)] : T(0);
-        B[p][i] = (inp && i < nc) ? Ls[(c0 + i) + ms * (k + p)] : T(0);
-    }
-    __syncthreads();
     const int wave = tid / 64, lane = tid % 64;
     const int wi = (wave & 1) * 32;      // rows of C (i) handled by this wave
     const int wj = (wave >> 1) * 32;     // columns of C (j)
     const int l16 = lane & 15, l4 = lane >> 4;
+    const bool inL = c0 < ns;
+    T *Cs = inL ? (L + lofs[s]) : (U + uofs[s]);
+    const int64_t ld = inL ? ms : ms - ns;
+    const int64_t off = inL ? 0 : ns;
+    // Every load of the workgroup goes out before the first wait: the two panel
+    // blocks (16 + 16 entries per thread) and this lane's 16 entries of C, at
+    // addresses clamped into the blocks (out-of-block values are replaced by
+    // zero afterwards), so one memory round trip covers them all.
+    T av[16], bv[16], cpre[2][2][4];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int q = tid + kBS * u, i = q % kTile, p = min(q / kTile, b - 1);
+        av[u] = Ls[(r0 + min(i, nr - 1)) + ms * (k + p)];
+        bv[u] = Ls[(c0 + min(i, nc - 1)) + ms * (k + p)];
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = min(wi + 16 * x + l16, nr - 1), j = min(wj + 16 * y + Mfma<T>::drow(l4, v), nc - 1);
+                cpre[x][y][v] = Cs[(r0 + i - off) + ld * (c0 + j - off)];
+            }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int q = tid + kBS * u, i = q % kTile, p = q / kTile;
+        A[p][i] = (p < b && i < nr) ? av[u] : T(0);
+        B[p][i] = (p < b && i < nc) ? bv[u] : T(0);
+    }
+    __syncthreads();
     v4 acc[2][2];
     for (int x = 0; x < 2; ++x)
         for (int y = 0; y < 2; ++y) acc[x][y] = v4{T(0), T(0), T(0), T(0)};
@@ -489,10 +520,6 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
         for (int x = 0; x < 2; ++x)
             for (int y = 0; y < 2; ++y) acc[x][y] = Mfma<T>::mac(aj[y], bi[x], acc[x][y]);
     }
-    const bool inL = c0 < ns;
-    T *Cs = inL ? (L + lofs[s]) : (U + uofs[s]);
-    const int64_t ld = inL ? ms : ms - ns;
-    const int64_t off = inL ? 0 : ns;
     const bool next = factor_next && r0 == c0 && c0 == k + b && c0 < ns;   // uniform
     if (next) __syncthreads();                 // all waves done reading A/B: A becomes the staging tile
     for (int x = 0; x < 2; ++x)
@@ -501,9 +528,8 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
                 const int i = wi + 16 * x + l16;           // row of C within the tile
                 const int j = wj + 16 * y + Mfma<T>::drow(l4, v);   // column of C within the tile
                 if (i < nr && j < nc && r0 + i >= c0 + j) {
-                    T *dst = Cs + (r0 + i - off) + ld * (c0 + j - off);
-                    const T nv = *dst - acc[x][y][v];
-                    *dst = nv;
+                    const T nv = cpre[x][y][v] - acc[x][y][v];
+                    Cs[(r0 + i - off) + ld * (c0 + j - off)] = nv;
                     if (next) A[j][i] = nv;            // A[col][row] of the next diagonal block
                 }
             }
