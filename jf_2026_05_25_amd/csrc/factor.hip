@@ -874,15 +874,11 @@ __device__ __forceinline__ void gather_children(T (*vs)[NR], const FlowChild *__
 // The k loop runs without LDS or barriers: every lane polls the z_k values it
 // needs straight into its registers.
 template <class T, int NR>
-__global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restrict__ tasks,
-                                                       const FlowChild *__restrict__ gch,
-                                                       const int32_t *__restrict__ relmap, const T *__restrict__ L,
-                                                       const T *__restrict__ Linv, const T *__restrict__ Y, T *W, T *Z,
-                                                       int nrhs, int32_t *cnt, int64_t nsn) {
-    __shared__ __attribute__((aligned(16))) T vs[kWholeRows][NR];
-    __shared__ __attribute__((aligned(16))) T vx[kTile][NR];
-    __shared__ int failed;
-    const FlowTask t = tasks[blockIdx.x];
+__device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T (*vx)[NR], int &failed,
+                                              const FlowChild *__restrict__ gch, const int32_t *__restrict__ relmap,
+                                              const T *__restrict__ L, const T *__restrict__ Linv,
+                                              const T *__restrict__ Y, T *W, T *Z, int nrhs, int32_t *cnt,
+                                              int64_t nsn) {
     int32_t *ready = cnt, *abort_flag = cnt + nsn;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, j = lane & 15, cg = lane >> 4;
     const int i = 16 * w + j;                          // row of the 64-row block
@@ -1015,15 +1011,14 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
 // The products are transposed (lane row = panel column): each tile is loaded
 // coalesced, staged through LDS (double-buffered in FP32, so one barrier per
 // tile), and read back transposed before its x values are awaited.
+template <class T>
+constexpr int kBwdStage = sizeof(T) == 4 ? 2 : 1;      // staging buffers of the four-wave task (LDS budget)
 template <class T, int NR>
-__global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restrict__ tasks, const int32_t *__restrict__ rows,
-                                                       const T *__restrict__ L, const T *__restrict__ Linv,
-                                                       const T *__restrict__ Z, T *X, int nrhs, int32_t *cnt, int64_t nsn) {
-    constexpr int NB = sizeof(T) == 4 ? 2 : 1;         // staging buffers (LDS budget)
-    __shared__ __attribute__((aligned(16))) T Lt[NB][kTile][kTile + 1];
-    __shared__ __attribute__((aligned(16))) T vx[kTile][NR];
-    __shared__ int failed;
-    const FlowTask t = tasks[blockIdx.x];
+__device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][kTile + 1], T (*vx)[NR], int &failed,
+                                              const int32_t *__restrict__ rows, const T *__restrict__ L,
+                                              const T *__restrict__ Linv, const T *__restrict__ Z, T *X, int nrhs,
+                                              int32_t *cnt, int64_t nsn) {
+    constexpr int NB = kBwdStage<T>;
     int32_t *abort_flag = cnt + nsn;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, j = lane & 15, cg = lane >> 4;
     const int c = 16 * w + j;                          // panel column of this lane's output
@@ -1112,6 +1107,370 @@ __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restric
             if (r < nrhs) st_dev(X + (t.c0 + t.r0 + c) * nrhs + r, acc[r]);
         }
     }
+}
+
+// ---- bottom subtrees: one wavefront per task
+// The supernodes of the bottom subtrees (one panel, at most kWholeRows front rows,
+// all children such) are most of the tasks and little work each; the four-wave
+// kernels above hold two or three of them per CU (their register files), and the
+// bottom levels wait on memory.  Here a task is one wavefront that works through
+// its rows in passes of 16 -- lane (cg, j) holding row j of the pass and columns
+// [16cg, 16cg+16), the same products and the same summation order as the
+// four-wave kernels, so the results are bitwise the same -- with the loads of
+// the next kLeafRing passes in flight while a pass is computed.  The forward
+// leaf launch runs before the main one, the backward after it.
+// Ordering within the wavefront: LDS instructions of one wavefront execute in
+// issue order, so a wave fence suffices between writing LDS and reading what
+// other lanes wrote (a workgroup barrier would also wait for the loads in flight).
+constexpr int kLeafRing = 4;
+
+// loads at 32-bit element offsets from a wave-uniform base (a supernode's block
+// or a diagonal inverse, < 2^16 entries): one VGPR per address and the SGPR-base
+// form of the global load, where 64-bit per-lane addresses would take two VGPRs
+// and a 64-bit add each
+// a wave-uniform pointer or index, held in SGPRs
+template <class T>
+__device__ __forceinline__ const T *uniform_ptr(const T *p) {
+    const uint64_t a = (uint64_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+    return (const T *)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <class T>
+__device__ __forceinline__ T ld_off(const T *base, uint32_t idx) {
+    typedef const __attribute__((address_space(1))) char *global_bytes;
+    typedef const __attribute__((address_space(1))) T *global_ptr;
+    return *(global_ptr)((global_bytes)base + idx * (uint32_t)sizeof(T));
+}
+// g16_load, g16_load_t and load_cols16 in that form (ld, rows, cols < 2^16)
+template <class T>
+__device__ __forceinline__ void g16_load32(T (&m)[16], const T *B, int ld, int rows, int cols, int i, int cg) {
+    const uint32_t ic = (uint32_t)min(i, rows - 1);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) m[c] = ld_off(B, ic + (uint32_t)ld * (uint32_t)min(16 * cg + c, cols - 1));
+}
+template <class T>
+__device__ __forceinline__ void g16_load_t32(T (&m)[16], const T *B, int ld, int rows, int cols, int i, int cg) {
+    const uint32_t ic = (uint32_t)min(i, cols - 1) * (uint32_t)ld;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = ld_off(B, (uint32_t)min(16 * cg + k, rows - 1) + ic);
+}
+template <class T>
+__device__ __forceinline__ void load_cols16_32(T (&v)[16], const T *base, int ld, int rows, int cols, int lane, int w) {
+    const uint32_t i = (uint32_t)min(lane, rows - 1);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[c] = ld_off(base, i + (uint32_t)ld * (uint32_t)min(16 * w + c, cols - 1));
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Children's update-vector entries landing in a whole front, one wavefront, U
+// entries per lane per round with every load of the round issued before the
+// first addition; children in child order, so each row receives y and then its
+// children's values in child order, as in gather_children.
+template <class T, int NR, int U>
+__device__ __forceinline__ void gather_children_wave(T (*vs)[NR], const FlowChild *__restrict__ gch, int64_t gb, int nch,
+                                                     const int32_t *__restrict__ relmap, const T *W, int nrhs,
+                                                     int lane) {
+    for (int jc = 0; jc < nch; ++jc) {
+        const FlowChild ch = gch[gb + jc];
+        const int nq = (ch.i1 - ch.i0) * NR;
+        const int32_t *rel = relmap + ch.relofs + ch.i0;
+        const T *wc = W + (ch.relofs + ch.i0) * nrhs;
+        for (int q0 = 0; q0 < nq; q0 += 64 * U) {
+            int32_t row[U];
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = min(q0 + 64 * u + lane, nq - 1), i = q / NR, r = min(q % NR, nrhs - 1);
+                row[u] = rel[i];
+                v[u] = ld_dev(wc + i * nrhs + r);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = q0 + 64 * u + lane, r = q % NR;
+                if (q < nq && r < nrhs) vs[row[u]][r] += v[u];
+            }
+            wave_sync();
+        }
+    }
+}
+
+// forward whole-front task (s one panel, front rows <= kWholeRows): pass q < npz
+// is rows [16q, 16q+16) of z = L_11^{-1} v, pass npz + u is update rows
+// [ns + 16u, ns + 16u + 16) of w = v - L[update rows, :] z
+template <class T, int NR>
+__device__ __forceinline__ void fwd_leaf_task(const FlowTask &t, T (*vs)[NR], T (*vx)[NR], int lane,
+                                              const FlowChild *__restrict__ gch, const int32_t *__restrict__ relmap,
+                                              const T *__restrict__ L, const T *__restrict__ Linv,
+                                              const T *__restrict__ Y, T *W, T *Z, int nrhs, int32_t *cnt,
+                                              int64_t nsn) {
+    int32_t *ready = cnt, *abort_flag = cnt + nsn;
+    const int j = lane & 15, cg = lane >> 4;
+    const int rq = g16_r0<NR>(cg);
+    const int nr = t.nr, nu = (int)(t.ms - t.ns);
+    const int npz = (nr + 15) >> 4, npass = npz + ((nu + 15) >> 4);
+    const T *Li = Linv + t.iofs, *Lu = L + t.lofs + t.ns;
+    // one load path for both kinds of pass (uniform parameters selected first),
+    // so that every load keeps the SGPR-base form
+    auto load_pass = [&](T (&m)[16], int q) {
+        const bool zp = q < npz;
+        const int r0 = zp ? 16 * q : 16 * (q - npz);
+        const T *base = uniform_ptr(zp ? Li + r0 : Lu + r0);
+        g16_load32(m, base, uniform_int(zp ? nr : (int)t.ms), uniform_int((zp ? nr : nu) - r0), nr, j, cg);
+    };
+    constexpr int QY = kTile * NR / 64;
+    T yv[QY];
+#pragma unroll
+    for (int q2 = 0; q2 < QY; ++q2) {
+        const int q = lane + 64 * q2, row = q / NR, r = q % NR;
+        yv[q2] = Y[(t.c0 + min(row, nr - 1)) * nrhs + min(r, nrhs - 1)];
+    }
+    // the ring loads are issued unconditionally (clamped to the last pass), so the
+    // compiler's wait counts stay exact: no global access sits under a branch in
+    // the pass loop, and the results go out after it
+    T buf[kLeafRing][16];
+#pragma unroll
+    for (int u = 0; u < kLeafRing; ++u) load_pass(buf[u], min(u, npass - 1));
+#pragma unroll
+    for (int q2 = 0; q2 < QY; ++q2) {
+        const int q = lane + 64 * q2, row = q / NR, r = q % NR;
+        vs[row][r] = (row < nr && r < nrhs) ? yv[q2] : T(0);
+    }
+    for (int q = kTile * NR + lane; q < t.ms * NR; q += 64) vs[q / NR][q % NR] = T(0);
+    if (t.need > 0 && !wait_at_least(&ready[t.s], t.need, abort_flag)) return;
+    wave_sync();
+    if (t.need > 0) gather_children_wave<T, NR, 16>(vs, gch, t.gb, t.nch, relmap, W, nrhs, lane);
+    T acc[NR], xr[NR];
+    g16_from_lds<T, NR>(xr, vs, nr, cg, j);            // update rows of the front are not part of v
+    for (int q0 = 0; q0 < npass; q0 += kLeafRing) {
+#pragma unroll
+        for (int u = 0; u < kLeafRing; ++u) {
+            const int q = q0 + u;
+            if (q == npz) {                            // z complete
+                wave_sync();
+                g16_from_lds<T, NR>(xr, vx, nr, cg, j);
+            }
+            if (q < npass) {
+#pragma unroll
+                for (int r = 0; r < NR; ++r) acc[r] = T(0);
+                g16_mac<T, NR>(acc, buf[u], xr);
+                g16_reduce<T, NR>(acc);
+                if (q < npz) {                         // z rows into vx
+                    const int row = 16 * q + j;
+#pragma unroll
+                    for (int rr = 0; rr < NR / 4; ++rr) vx[row][rq + rr] = row < nr ? acc[rq + rr] : T(0);
+                } else {                               // w rows in place of v's update rows
+                    const int row = (int)t.ns + 16 * (q - npz) + j;
+                    if (row < t.ms) {
+#pragma unroll
+                        for (int rr = 0; rr < NR / 4; ++rr) vs[row][rq + rr] -= acc[rq + rr];
+                    }
+                }
+            }
+            load_pass(buf[u], min(q + kLeafRing, npass - 1));
+        }
+    }
+    wave_sync();
+    for (int q = lane; q < nr * NR; q += 64) {
+        const int row = q / NR, r = q % NR;
+        if (r < nrhs) st_dev(Z + (t.c0 + row) * nrhs + r, vx[row][r]);
+    }
+    for (int q = lane; q < nu * NR; q += 64) {
+        const int row = q / NR, r = q % NR;
+        if (r < nrhs) st_dev(W + (t.relofs + row) * nrhs + r, vs[t.ns + row][r]);
+    }
+    complete_stores();
+    if (lane == 0 && t.parent >= 0) __hip_atomic_fetch_add(&ready[t.parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The forward launch: workgroups [0, nq) take the bottom subtrees' tasks four at
+// a time (task 4b + w on wave w, each wave on its own, on its own quarter of the
+// LDS), the rest take one task each of the remaining list.  The bottom tasks
+// come first in the list and wait only on earlier ones (or on a wave of their
+// own workgroup, which is resident), so in-order dispatch always progresses, and
+// the upper tasks start as soon as their own subtrees are done.
+template <class T, int NR>
+constexpr bool kLeafQuads = NR <= 8;
+template <class T, int NR>
+__global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restrict__ tasks,
+                                                       const FlowChild *__restrict__ gch,
+                                                       const int32_t *__restrict__ relmap, const T *__restrict__ L,
+                                                       const T *__restrict__ Linv, const T *__restrict__ Y, T *W, T *Z,
+                                                       int nrhs, int32_t *cnt, int64_t nsn, int64_t nleaf) {
+    constexpr int NQ = kLeafQuads<T, NR> ? 4 : 1;
+    __shared__ __attribute__((aligned(16))) T lds[NQ][kWholeRows + kTile][NR];
+    __shared__ int failed_one;                         // (with quads: in the main path's unused quarter)
+    const int64_t nq = (nleaf + 3) / 4;
+    if constexpr (kLeafQuads<T, NR>) {
+        if (blockIdx.x < nq) {
+            const int w = uniform_int((int)(threadIdx.x >> 6));
+            const int64_t k = 4 * (int64_t)blockIdx.x + w;
+            if (k < nleaf)
+                fwd_leaf_task<T, NR>(tasks[k], lds[w], lds[w] + kWholeRows, (int)(threadIdx.x & 63), gch, relmap, L,
+                                     Linv, Y, W, Z, nrhs, cnt, nsn);
+            return;
+        }
+    }
+    const FlowTask t = tasks[nleaf + (int64_t)blockIdx.x - nq];
+    int *failed;
+    if constexpr (NQ > 1) failed = (int *)&lds[1][0][0];
+    else failed = &failed_one;
+    fwd_main_task<T, NR>(t, lds[0], lds[0] + kWholeRows, *failed, gch, relmap, L, Linv, Y, W, Z, nrhs, cnt, nsn);
+}
+
+// backward task of a one-panel supernode: x = L_11^{-T} (z - L[update rows, :]' x_u),
+// two wavefronts: wave w takes the output column groups q = w and w + 2
+// (columns [16q, 16q+16), acc[h] for q = w + 2h).  The update rows go by 64-row
+// tiles (from the last, as the four-wave kernel does); a tile's 64 x 16 column
+// chunk q is read coalesced (lane = row) and transposed through the wave's LDS
+// buffer; chunk q of the next tile is in flight while the current one is used,
+// and after the last tile the same slot receives row block q of L_11^{-T}.
+// pair barrier: both waves of a pair count in on *ctr (zeroed before any wait of
+// the launch) and wait for two; the other pair of the workgroup is not involved
+__device__ __forceinline__ void pair_barrier(int *ctr, int lane) {
+    wave_sync();
+    if (lane == 0) __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < 2) __builtin_amdgcn_s_sleep(1);
+}
+template <class T, int NR>
+__device__ __forceinline__ void bwd_leaf_task(const FlowTask &t, T (*Sb)[kTile][17], T (*vx)[NR], int *sync, int w,
+                                              int lane, const int32_t *__restrict__ rows, const T *__restrict__ L,
+                                              const T *__restrict__ Linv, const T *__restrict__ Z, T *X, int nrhs,
+                                              int32_t *cnt, int64_t nsn) {
+    int32_t *abort_flag = cnt + nsn;
+    const int j = lane & 15, cg = lane >> 4;
+    const int rq = g16_r0<NR>(cg);
+    const int nc = t.nr, npz = (nc + 15) >> 4;
+    const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;
+    const T *Lc = L + t.lofs;                          // front row 0, column 0
+    const int32_t *rw = rows + t.rofs;
+    T (*S)[17] = Sb[w];                                // this wave's transposition buffer
+    auto tile_row0 = [&](int e) -> int { return t.u0 + kTile * (nchunk - 1 - e); };
+    auto tile_rows = [&](int e) -> int { return min(kTile, t.u1 - tile_row0(e)); };
+    // slot h holds chunk q = w + 2h of the current tile, after the last tile row
+    // block q of L_11^{-T} (g16 lane layout); loaded unconditionally (indices
+    // clamped), so the compiler's wait counts stay exact
+    // entry c: base[a + b min(c0 + c, nc - 1)] -- chunk: a = row (lane), b = ld,
+    // c0 = 16q; inverse: a = nc (column 16q + j), b = 1, c0 = 16cg; one load path
+    // for both, so that every load keeps the SGPR-base form
+    auto load_next = [&](T (&v)[16], int e, int q) {
+        const bool tile = e < nchunk;
+        const T *base = uniform_ptr(tile ? Lc + tile_row0(e) : Linv + t.iofs);
+        const uint32_t a = tile ? (uint32_t)min(lane, tile_rows(e) - 1) : (uint32_t)(nc * min(16 * q + j, nc - 1));
+        const uint32_t b = (uint32_t)uniform_int(tile ? (int)t.ms : 1);
+        const int c0 = tile ? 16 * q : 16 * cg;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[c] = ld_off(base, a + b * (uint32_t)min(c0 + c, nc - 1));
+    };
+    T buf[2][16], acc[2][NR], zv[2][NR / 4], xr[NR];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const T *zp = Z + (t.c0 + min(16 * (w + 2 * h) + j, nc - 1)) * nrhs;
+#pragma unroll
+        for (int rr = 0; rr < NR / 4; ++rr) zv[h][rr] = zp[min(rq + rr, nrhs - 1)];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) load_next(buf[h], 0, w + 2 * h);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[h][r] = T(0);
+    bool ok = true;
+    for (int e = 0; e < nchunk; ++e) {
+        const int r0 = tile_row0(e), nre = tile_rows(e);
+        ok = g16_load_wait<T, NR>(xr, [&](int k) { return X + (int64_t)rw[r0 + k] * nrhs; }, nre, nrhs, cg, j,
+                                  abort_flag) && ok;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int q = w + 2 * h;
+            if (q < npz) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) S[lane][k] = (lane < nre && 16 * q + k < nc) ? buf[h][k] : T(0);
+                wave_sync();
+                T m[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) m[k] = S[16 * cg + k][j];
+                wave_sync();
+                g16_mac<T, NR>(acc[h], m, xr);
+            }
+            load_next(buf[h], e + 1, q);
+        }
+    }
+    if (!ok) sync[1] = 1;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = w + 2 * h;
+        if (q < npz) {
+            g16_reduce<T, NR>(acc[h]);
+            const int c = 16 * q + j;
+#pragma unroll
+            for (int rr = 0; rr < NR / 4; ++rr) {
+                const int r = rq + rr;
+                vx[c][r] = (c < nc && r < nrhs) ? zv[h][rr] - acc[h][r] : T(0);
+            }
+        }
+    }
+    pair_barrier(&sync[0], lane);
+    if (sync[1]) return;
+    g16_from_lds<T, NR>(xr, vx, nc, cg, j);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = w + 2 * h;
+        if (q < npz) {
+            const int c = 16 * q + j;
+            T a[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) a[r] = T(0);
+            g16_mac<T, NR>(a, buf[h], xr);
+            g16_reduce<T, NR>(a);
+            if (c < nc) {
+#pragma unroll
+                for (int rr = 0; rr < NR / 4; ++rr) {
+                    const int r = rq + rr;
+                    if (r < nrhs) st_dev(X + (t.c0 + c) * nrhs + r, a[r]);
+                }
+            }
+        }
+    }
+}
+
+// The backward launch: workgroups [0, nmain) take one task each of the list's
+// head, the rest the bottom subtrees' tasks (the tail) two at a time, task
+// nmain + 2(b - nmain) + p on waves 2p, 2p+1.  A bottom task waits only on earlier
+// tasks (the other pair of its workgroup, if at all, is the earlier one and
+// resident), so in-order dispatch always progresses.  The pairs' buffers overlay
+// the four-wave task's staging tiles.
+template <class T, int NR>
+constexpr bool kLeafPairs = sizeof(T) * NR <= 64;     // (FP64 with 16 right-hand sides would spill)
+template <class T, int NR>
+__global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restrict__ tasks, const int32_t *__restrict__ rows,
+                                                       const T *__restrict__ L, const T *__restrict__ Linv,
+                                                       const T *__restrict__ Z, T *X, int nrhs, int32_t *cnt, int64_t nsn,
+                                                       int64_t nmain, int64_t nleaf) {
+    constexpr int NB = kBwdStage<T>;
+    constexpr int kMain = NB * kTile * (kTile + 1) + kTile * NR;
+    constexpr int kPair = 2 * kTile * 17 + kTile * NR;
+    __shared__ __attribute__((aligned(16))) T lds[kLeafPairs<T, NR> && 2 * kPair > kMain ? 2 * kPair : kMain];
+    __shared__ int flags[5];                           // main: failed; pairs: arrivals and failure, per pair
+    if (kLeafPairs<T, NR> && (int64_t)blockIdx.x >= nmain) {
+        const int tid = threadIdx.x, pw = uniform_int(tid >> 6), p = pw >> 1;
+        if (tid < 4) flags[1 + tid] = 0;
+        __syncthreads();                               // before any wait
+        const int64_t k = nmain + 2 * ((int64_t)blockIdx.x - nmain) + p;
+        if (k >= nmain + nleaf) return;
+        T *base = lds + p * kPair;
+        bwd_leaf_task<T, NR>(tasks[k], (T (*)[kTile][17])base, (T (*)[NR])(base + 2 * kTile * 17), &flags[1 + 2 * p],
+                             pw & 1, tid & 63, rows, L, Linv, Z, X, nrhs, cnt, nsn);
+        return;
+    }
+    bwd_main_task<T, NR>(tasks[blockIdx.x], (T (*)[kTile][kTile + 1])lds, (T (*)[NR])(lds + NB * kTile * (kTile + 1)),
+                         flags[0], rows, L, Linv, Z, X, nrhs, cnt, nsn);
 }
 
 // ----------------------------------------------------------------- refinement helpers
@@ -1290,11 +1649,30 @@ __global__ void __launch_bounds__(kBS) spmv_kernel(int64_t n, const int32_t *__r
     double a[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) a[r] = 0.0;
-    for (int64_t p = p0 + sub; p < p1; p += 8) {
-        const double kv = val[p];
-        const double *x = pg + (int64_t)col[p] * NR;
+    // U entries per lane at a time: their (value, column) loads, then their x
+    // loads, all in flight before the first product (a loop of single entries
+    // waits two memory latencies per entry); products in entry order
+    constexpr int U = NR >= 16 ? 2 : 4;
+    for (int64_t p = p0 + sub; p < p1; p += 8 * U) {
+        double kv[U];
+        int64_t c[U];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) a[r] = fma(kv, x[r], a[r]);
+        for (int u = 0; u < U; ++u) {
+            const int64_t pu = min(p + 8 * u, p1 - 1);
+            kv[u] = val[pu];
+            c[u] = col[pu];
+        }
+        double xv[U][NR];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) xv[u][r] = pg[c[u] * NR + r];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool in = p + 8 * u < p1;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) a[r] = in ? fma(kv[u], xv[u][r], a[r]) : a[r];
+        }
     }
 #pragma unroll
     for (int o = 4; o >= 1; o >>= 1) {
@@ -1513,16 +1891,24 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
         ctx->ms[fam] += ms;
         ctx->launches[fam] += 1;
     };
+    // forward: the bottom subtrees' tasks (the head of the list) one wavefront
+    // each, then the rest; backward: the rest, then the bottom subtrees (the tail)
+    const int64_t nfl = H.flow_f_leaf;
+    const int64_t nbl = H.flow_b_leaf, nbm = (int64_t)H.flow_b.size() - nbl;
     if (!H.flow_f.empty()) {
         if (detail) (void)hipEventRecord(ctx->ev2, st);
-        hipLaunchKernelGGL((fwd_flow_kernel<T, NR>), dim3((unsigned)H.flow_f.size()), dim3(kBS), 0, st, S->d_flow_f,
-                           S->d_flow_gb, S->d_relmap, L, Linv, Y, W, Z, nrhs, cnt, nsn);
+        const int64_t nleaf = kLeafQuads<T, NR> ? nfl : 0;
+        const int64_t grid = (nleaf + 3) / 4 + (int64_t)H.flow_f.size() - nleaf;
+        hipLaunchKernelGGL((fwd_flow_kernel<T, NR>), dim3((unsigned)grid), dim3(kBS), 0, st, S->d_flow_f, S->d_flow_gb,
+                           S->d_relmap, L, Linv, Y, W, Z, nrhs, cnt, nsn, nleaf);
         if (detail) lap(FAM_FLOW_FWD);
     }
     if (!H.flow_b.empty()) {
         if (detail) (void)hipEventRecord(ctx->ev2, st);
-        hipLaunchKernelGGL((bwd_flow_kernel<T, NR>), dim3((unsigned)H.flow_b.size()), dim3(kBS), 0, st, S->d_flow_b,
-                           S->d_rows, L, Linv, (const T *)Z, X, nrhs, cnt, nsn);
+        const int64_t nmain = kLeafPairs<T, NR> ? nbm : (int64_t)H.flow_b.size(), nleaf = (int64_t)H.flow_b.size() - nmain;
+        const int64_t grid = nmain + (nleaf + 1) / 2;
+        hipLaunchKernelGGL((bwd_flow_kernel<T, NR>), dim3((unsigned)grid), dim3(kBS), 0, st, S->d_flow_b, S->d_rows, L,
+                           Linv, (const T *)Z, X, nrhs, cnt, nsn, nmain, nleaf);
         if (detail) lap(FAM_FLOW_BWD);
     }
     return hipGetLastError();

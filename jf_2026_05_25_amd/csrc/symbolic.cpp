@@ -140,6 +140,28 @@ struct ND {
 // (ties by the construction order, itself topological) keeps every task after
 // the tasks it waits on (a backward chain is keyed after its parent's panels,
 // whose keys lie below the parent's first finish).
+// Move the tasks of the bottom subtrees to the front of the forward list and to
+// the back of the backward list (stable, so both lists stay in dependency order:
+// a bottom supernode's children are bottom supernodes, its ancestors come first
+// backward).
+static void split_flow_leaves(SymbolicHost &S) {
+    const int64_t nsn = S.ns;
+    std::vector<char> leaf(nsn, 0);
+    for (int64_t s = 0; s < nsn; ++s) {          // children precede parents
+        bool ok = S.ncol[s] <= kPanel && S.m[s] <= kWholeRows;
+        for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1] && ok; ++q) ok = leaf[S.child[q]] != 0;
+        leaf[s] = ok;
+    }
+    auto part = [&](std::vector<FlowTask> &v, bool leaves_first) {
+        std::stable_partition(v.begin(), v.end(), [&](const FlowTask &t) { return (leaf[t.s] != 0) == leaves_first; });
+        int64_t n = 0;
+        for (const FlowTask &t : v) n += leaf[t.s] ? 1 : 0;
+        return n;
+    };
+    S.flow_f_leaf = part(S.flow_f, true);
+    S.flow_b_leaf = part(S.flow_b, false);
+}
+
 static void order_flow_lists(SymbolicHost &S, const std::vector<int32_t> &snode_of_pos) {
     const double lat = 1.0, tile = 1.5, base = 3.0, eps = 1e-6;
     const int64_t nsn = S.ns;
@@ -724,6 +746,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
             }
         }
     order_flow_lists(S, snode_of_pos);
+    split_flow_leaves(S);
     return 0;
 }
 

@@ -78,6 +78,11 @@ struct SymbolicHost {
     // offset of its inverse diagonal blocks, written by the factorization (panel
     // p: b x b entries at iofs[s] + kPanel^2 p)
     std::vector<FlowTask> flow_f, flow_b;
+    // tasks of the bottom subtrees (supernodes with one panel and at most kWholeRows
+    // front rows whose children are all such): the first flow_f_leaf entries of
+    // flow_f and the last flow_b_leaf entries of flow_b, run by the one-wavefront
+    // leaf kernels (forward before, backward after the main launch)
+    int64_t flow_f_leaf = 0, flow_b_leaf = 0;
     std::vector<FlowChild> flow_gb;
     std::vector<int64_t> iofs;
     int64_t inv_entries = 0;        // diagonal-block inverses, then the chain products (bofs)

@@ -174,6 +174,31 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             return r0 >= 0 && r0 + nr <= t.ms && c >= 0 && c + nc <= t.ns && lo >= t.lofs && hi < t.lofs + t.ms * t.ns &&
                    hi < S.L_entries;
         };
+        // bottom subtrees, run one or two wavefronts per task: the head of the
+        // forward list and the tail of the backward list, one task per supernode
+        // (one panel, a whole front of at most kWholeRows rows), whose children
+        // are all bottom supernodes (12)
+        {
+            if (S.flow_f_leaf < 0 || S.flow_f_leaf > (int64_t)S.flow_f.size() || S.flow_b_leaf != S.flow_f_leaf) return 12;
+            std::vector<char> leaf(nsn, 0);
+            for (int64_t e = 0; e < S.flow_f_leaf; ++e) {
+                const FlowTask &t = S.flow_f[e];
+                if (t.kind != 1 || t.p != 0 || t.np != 1 || t.ms > kWholeRows || leaf[t.s]) return 12;
+                leaf[t.s] = 1;
+            }
+            for (int64_t e = S.flow_f_leaf; e < (int64_t)S.flow_f.size(); ++e)
+                if (leaf[S.flow_f[e].s]) return 12;
+            const int64_t b0 = (int64_t)S.flow_b.size() - S.flow_b_leaf;
+            for (int64_t e = 0; e < (int64_t)S.flow_b.size(); ++e) {
+                const FlowTask &t = S.flow_b[e];
+                if ((e >= b0) != (leaf[t.s] != 0)) return 12;
+                if (e >= b0 && (t.np != 1 || t.p != 0 || t.bofs >= 0 || t.u0 != t.ns || t.u1 != t.ms)) return 12;
+            }
+            for (int64_t s = 0; s < nsn; ++s)
+                if (leaf[s])
+                    for (int32_t q = S.child_ptr[s]; q < S.child_ptr[s + 1]; ++q)
+                        if (!leaf[S.child[q]]) return 12;
+        }
         for (size_t e = 0; e < S.flow_f.size(); ++e) {
             const FlowTask &t = S.flow_f[e];
             const bool pivot = t.p >= 0;
@@ -376,5 +401,6 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
     info[4] = (int64_t)S.ptasks.size();
     info[5] = (int64_t)S.utasks.size();
     info[6] = S.max_front;
+    info[7] = S.flow_f_leaf;
     return 0;
 }

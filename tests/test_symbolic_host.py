@@ -63,8 +63,9 @@ def test_task_decomposition_solves(harness, oracle, name):
     xe = np.linalg.solve(Ks, m.loads[free])
     np.testing.assert_allclose(x[free], xe, rtol=1e-8, atol=1e-9 * np.abs(xe).max())
     assert np.all(x[fixed == 1] == 0)
-    ns, nlev, nL, nU, npt, nut, maxf = info[:7]
+    ns, nlev, nL, nU, npt, nut, maxf, nleaf = info[:8]
     assert ns > 1 and nlev > 1 and nL >= np.count_nonzero(np.tril(Ks))
+    assert 0 < nleaf < ns            # bottom subtrees (one wavefront per task) and the rest
 
 
 def test_fill_is_nested_dissection_quality(harness, oracle):
