@@ -291,11 +291,47 @@ def main():
             roof["traffic"] = read_traffic(os.path.join(ROOT, "profiles", f"{kfile}_wingbox_traffic.txt"), main_prec)
         out["roofline"] = roof
         out["other_rooflines"] = {k: v[1] for k, v in lines.items() if k != dom}
+    if args.workload == "wingbox":
+        out["load_vector"] = measure_load_vector(ctx, mesh, model)
     if not args.no_cpu_baseline and args.workload == "wingbox":
         cb = run_cpu_baseline(args.cpu_sample_nx)
         cb.pop("seconds", None)
         out["cpu_baseline"] = cb
     print(json.dumps(out))
+
+
+def measure_load_vector(ctx, mesh, model, calls: int = 200):
+    """The load vector of one load set on the device (jfem_load_vector_dev): a
+    uniform 5 kPa PLOAD4 on every shell element plus the model's first right-hand
+    side as FORCE cards, into HBM.  Per call: two launches; the algorithmic bytes
+    are the set's card data, the coordinates and the vector written once."""
+    from jf_2026_05_25_amd import solver
+    ne = len(model.quads) + len(model.trias)
+    f = model.loads[:, 0].reshape(-1, 6)[:, :3]
+    nodes = np.nonzero(np.any(f != 0.0, axis=1))[0]
+    mag = np.linalg.norm(f[nodes], axis=1)
+    ls = solver.LoadSet(mesh, pload4=(np.arange(ne), np.full(ne, 5000.0)),
+                        force=(nodes, mag, f[nodes] / mag[:, None]))
+    F = solver.DeviceBuffer(8 * model.ndof, ctx)
+    ls.vector_dev(F.ptr)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        ls.vector_dev(F.ptr)
+    ctx.sync()
+    us = (time.perf_counter() - t0) / calls * 1e6
+    n = model.n_nodes
+    npl_slots = 4 * len(model.quads) + 3 * len(model.trias)
+    algo = (24 * n + 8 * (n + 1) + 48 * n                 # coordinates, per-grid offsets, F
+            + 4 * ne + 8 * ne + 4 * npl_slots + 32 * ne    # cards, connectivity, nodal shares written
+            + 4 * npl_slots + 32 * npl_slots               # per-grid card lists and shares read
+            + 8 * (n + 1) + 32 * len(nodes))               # FORCE offsets and records
+    ls.free()
+    F.free()
+    return {"us_per_call": us, "pload4_cards": int(ne), "force_cards": int(len(nodes)),
+            "algorithmic_bytes": int(algo), "achieved_GBps": algo / (us * 1e-6) / 1e9,
+            "note": "wall clock per call over 200 back-to-back calls (two launches each); "
+                    "launch-latency bound at this size"}
 
 
 def run_variants(args):

@@ -50,6 +50,7 @@ typedef struct jfem_pattern jfem_pattern;
 typedef struct jfem_matrix jfem_matrix;
 typedef struct jfem_symbolic jfem_symbolic;
 typedef struct jfem_factor jfem_factor;
+typedef struct jfem_loadset jfem_loadset;
 
 /* ---- contexts ----------------------------------------------------------- */
 /* One per host thread or request slot: a HIP stream on `device` plus scratch.
@@ -202,6 +203,35 @@ int jfem_solve(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nr
 int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs,
                    const double *b_dev, double *x_dev, double tol, int max_refine,
                    int32_t *iters, double *relres, double *x_lo_dev, double *relres_out);
+
+/* ---- load vectors ------------------------------------------------------- */
+/* A load set: the FORCE / MOMENT cards on grid points and the uniform PLOAD4 cards
+ * on shell elements of one load SID, in card order, basic coordinates (CID 0).
+ * Replaces the FORCE, MOMENT and shell-PLOAD4 branches of _resolve_loads!
+ * (src/solver/loads.jl:517-587) and the point-force application (:1338-1340).
+ *   PLOAD4 card i: shell element pload4_elem[i] (CQUAD4 e for e < n_quad of the
+ *     mesh, else CTRIA3 e - n_quad), pressure pload4_p[i], direction
+ *     pload4_dir3[3i..3i+2] (NULL, or a vector of norm <= 1e-30: the element
+ *     normal); elements of area <= 1e-30 take no load, as in the reference;
+ *   FORCE / MOMENT card i: grid point *_node[i], magnitude *_mag[i], direction
+ *     *_dir3[3i..3i+2], taken as given (the reference's reader rounds both to
+ *     single precision: src/parsing/extract_loads.jl:15-29).
+ * GRAV/ACCEL, PLOAD1/2, coordinate systems and LOAD combinations stay with the
+ * caller (a combination is a sum of load vectors with their factors). */
+int jfem_loadset_create(jfem_ctx *ctx, const jfem_mesh *mesh, int64_t n_pload4,
+                        const int32_t *pload4_elem, const double *pload4_p,
+                        const double *pload4_dir3, int64_t n_force, const int32_t *force_node,
+                        const double *force_mag, const double *force_dir3, int64_t n_moment,
+                        const int32_t *moment_node, const double *moment_mag,
+                        const double *moment_dir3, jfem_loadset **out);
+/* The set's load vector with scale factor `scale` (applied where the reference
+ * applies it: area*P*scale per PLOAD4, (Dir*Mag)*scale per FORCE/MOMENT), all
+ * 6*n_nodes DOFs, to the host (F_host) or to device memory (F_dev, on this
+ * context's stream).  Sums run in the reference's order: bitwise reproducible.
+ * A load set keeps per-call buffers: calls on one load set must not overlap. */
+int jfem_load_vector(jfem_ctx *ctx, const jfem_loadset *ls, double scale, double *F_host);
+int jfem_load_vector_dev(jfem_ctx *ctx, const jfem_loadset *ls, double scale, double *F_dev);
+int jfem_loadset_free(jfem_loadset *ls);
 
 /* device buffers for the _dev entry points (owned by the caller) */
 int jfem_dev_alloc(jfem_ctx *ctx, int64_t bytes, void **dev_ptr);

@@ -254,6 +254,78 @@ def bc_partition(K: DeviceMatrix, spc: np.ndarray, eps_zero: float = 1e-8):
     return fixed
 
 
+# --------------------------------------------------------------------- load vectors
+def single_precision(a):
+    """FORCE / MOMENT magnitudes and directions as the reference's reader keeps
+    them: rounded to single precision when finite and in range
+    (load_single_precision, src/parsing/extract_loads.jl:15-22)."""
+    a = np.asarray(a, dtype=float)
+    ok = np.isfinite(a) & (np.abs(a) <= np.finfo(np.float32).max)
+    with np.errstate(over="ignore"):
+        return np.where(ok, a.astype(np.float32).astype(float), a)
+
+
+class LoadSet:
+    """One load SID on the device: FORCE / MOMENT cards on grid points and uniform
+    PLOAD4 cards on shell elements, in card order (the FORCE, MOMENT and shell
+    PLOAD4 branches of _resolve_loads!, src/solver/loads.jl:517-587, 1338-1340).
+
+    pload4 = (elem, p[, dir3]): shell element e is CQUAD4 e (e < n_quad) or CTRIA3
+    e - n_quad; a dir3 row of norm <= 1e-30 (or no dir3) takes the element normal.
+    force / moment = (node, mag, dir3), values as given (see single_precision)."""
+
+    def __init__(self, mesh: DeviceMesh, pload4=None, force=None, moment=None, ctx: Context | None = None):
+        self.ctx = ctx or mesh.ctx
+        self.mesh = mesh
+        L = self.ctx.L
+
+        def cards(c, nd):
+            if c is None or len(c[0]) == 0:
+                return 0, None, None, None
+            idx = np.ascontiguousarray(np.asarray(c[0], np.int32))
+            val = np.ascontiguousarray(np.asarray(c[1], float).reshape(len(idx)))
+            d = None
+            if len(c) > 2 and c[2] is not None:
+                d = np.ascontiguousarray(np.asarray(c[2], float).reshape(len(idx), 3))
+            elif nd:
+                raise ValueError("FORCE / MOMENT cards need directions")
+            return len(idx), idx, val, d
+
+        npl, pe, pp, pd = cards(pload4, False)
+        nf, fn, fm, fd = cards(force, True)
+        nm, mn, mm, md = cards(moment, True)
+        self._keep = [pe, pp, pd, fn, fm, fd, mn, mm, md]
+        h = C.c_void_p()
+        check(self.ctx.h, L.jfem_loadset_create(self.ctx.h, mesh.h, npl, ptr(pe), ptr(pp), ptr(pd),
+                                                nf, ptr(fn), ptr(fm), ptr(fd), nm, ptr(mn), ptr(mm), ptr(md),
+                                                C.byref(h)))
+        self.h = h
+        self.ndof = 6 * mesh.model.n_nodes
+
+    def vector(self, scale: float = 1.0) -> np.ndarray:
+        """The load vector (all 6 n DOFs) with the SID's scale factor."""
+        F = np.zeros(self.ndof)
+        check(self.ctx.h, self.ctx.L.jfem_load_vector(self.ctx.h, self.h, float(scale), ptr(F)))
+        return F
+
+    def vector_dev(self, F_dev, scale: float = 1.0):
+        """Write the load vector into device memory F_dev (6 n doubles), on the
+        context's stream."""
+        dst = F_dev if isinstance(F_dev, C.c_void_p) else C.c_void_p(F_dev)
+        check(self.ctx.h, self.ctx.L.jfem_load_vector_dev(self.ctx.h, self.h, float(scale), dst))
+
+    def free(self):
+        if self.h:
+            self.ctx.L.jfem_loadset_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 # --------------------------------------------------------------------- factor / solve
 class Symbolic:
     def __init__(self, K: DeviceMatrix, mesh: DeviceMesh, fixed: np.ndarray, ctx: Context | None = None):

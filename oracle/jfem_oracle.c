@@ -1328,3 +1328,115 @@ int or_solve(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *v
     free(parent); free(anc); free(cc); free(flag); free(stk); free(Lp); free(Li); free(Lx); free(xw); free(cnext);
     return st;
 }
+
+
+/* ======================================================================== */
+/* Load vector of one load set: FORCE / MOMENT on grid points, uniform PLOAD4 on
+ * CQUAD4 / CTRIA3, basic coordinates.  _resolve_loads!, src/solver/loads.jl. */
+
+/* LinearAlgebra.generic_norm2 for a 3-vector (below the BLAS nrm2 cutoff):
+ * maxabs = normInf; plain sum of squares unless length*maxabs^2 is not finite
+ * or maxabs^2 is zero, then the sum of (|v_i|/maxabs)^2 times maxabs. */
+double or_jl_norm3(const double v[3]) {
+    double mx = fabs(v[0]);
+    for (int i = 1; i < 3; ++i) {
+        const double a = fabs(v[i]);
+        if (!(isnan(mx) || mx > a)) mx = a;        /* ifelse(isnan(m) | (m > a), m, a) */
+    }
+    if (mx == 0.0 || isinf(mx)) return mx;
+    if (isfinite(3.0 * mx * mx) && mx * mx != 0.0) {
+        double sum = v[0] * v[0];
+        sum += v[1] * v[1];
+        sum += v[2] * v[2];
+        return sqrt(sum);
+    }
+    double sum = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        const double q = fabs(v[i]) / mx;
+        sum += q * q;
+    }
+    return mx * sqrt(sum);
+}
+
+/* LinearAlgebra.normalize -> __normalize!(a, nrm): delta = inv(prevfloat(Inf));
+ * nrm >= delta: a * inv(nrm); else a * (eps/delta), then * inv(nrm * (eps/delta)). */
+void or_jl_normalize3(double v[3]) {
+    const double nrm = or_jl_norm3(v);
+    const double delta = 1.0 / DBL_MAX;
+    if (nrm >= delta) {
+        const double inv = 1.0 / nrm;
+        for (int i = 0; i < 3; ++i) v[i] *= inv;
+    } else {
+        const double ed = DBL_EPSILON / delta;
+        for (int i = 0; i < 3; ++i) v[i] *= ed;
+        const double inv = 1.0 / (nrm * ed);
+        for (int i = 0; i < 3; ++i) v[i] *= inv;
+    }
+}
+
+int or_load_vector(int64_t n_nodes, const double *x, const double *y, const double *z, int64_t nq,
+                   const int32_t *quad_conn, int64_t nt, const int32_t *tria_conn, int64_t n_pload4,
+                   const int32_t *pl_elem, const double *pl_p, const double *pl_dir3, int64_t n_force,
+                   const int32_t *f_node, const double *f_mag, const double *f_dir3, int64_t n_moment,
+                   const int32_t *m_node, const double *m_mag, const double *m_dir3, double scale,
+                   double *F) {
+    if (n_nodes <= 0) return OR_ERR_BAD_ARG;
+    for (int64_t i = 0; i < 6 * n_nodes; ++i) F[i] = 0.0;
+    /* PLOAD4 on shells, loads.jl:541-587 (the CSHELL branch) */
+    for (int64_t i = 0; i < n_pload4; ++i) {
+        const int64_t e = pl_elem[i];
+        if (e < 0 || e >= nq + nt) return OR_ERR_BAD_ARG;
+        int nids[4], nn;
+        if (e < nq) { nn = 4; for (int k = 0; k < 4; ++k) nids[k] = quad_conn[e + nq * k]; }
+        else { nn = 3; for (int k = 0; k < 3; ++k) nids[k] = tria_conn[(e - nq) + nt * k]; }
+        double Xc[4][3];
+        for (int k = 0; k < nn; ++k) { Xc[k][0] = x[nids[k]]; Xc[k][1] = y[nids[k]]; Xc[k][2] = z[nids[k]]; }
+        double a[3], b[3], normal_vec[3];
+        if (nn == 4) { v3sub(Xc[2], Xc[0], a); v3sub(Xc[3], Xc[1], b); }   /* cross(X3-X1, X4-X2) */
+        else { v3sub(Xc[1], Xc[0], a); v3sub(Xc[2], Xc[0], b); }          /* cross(v1, v2) */
+        v3cross(a, b, normal_vec);
+        const double area = 0.5 * or_jl_norm3(normal_vec);
+        if (!(area > 1e-30)) continue;                                   /* area > 1e-30 || continue */
+        double load_dir[3];
+        int have_dir = 0;
+        if (pl_dir3) {                                                   /* haskey(pload, "N") */
+            const double *dir_vec = pl_dir3 + 3 * i;
+            const double n_dir = or_jl_norm3(dir_vec);
+            if (n_dir > 1e-30) {
+                for (int c = 0; c < 3; ++c) load_dir[c] = dir_vec[c] / n_dir;   /* dir_vec ./ n_dir */
+                have_dir = 1;
+            }
+        }
+        if (!have_dir) {
+            for (int c = 0; c < 3; ++c) load_dir[c] = normal_vec[c];
+            or_jl_normalize3(load_dir);
+        }
+        const double tf = area * pl_p[i] * scale;
+        double f_node[3];
+        for (int c = 0; c < 3; ++c) f_node[c] = load_dir[c] * (tf / (double)nn);
+        for (int k = 0; k < nn; ++k)
+            for (int c = 0; c < 3; ++c) F[6 * (int64_t)nids[k] + c] += f_node[c];
+    }
+    /* FORCE / MOMENT into raw_forces[gid] (zeros(6) on first use), loads.jl:519-539;
+     * applied after the distributed loads, :1338-1340 */
+    double *raw = (double *)calloc(6 * (size_t)n_nodes, sizeof(double));
+    unsigned char *has = (unsigned char *)calloc((size_t)n_nodes, 1);
+    if (!raw || !has) { free(raw); free(has); return OR_ERR_BAD_ARG; }
+    for (int64_t i = 0; i < n_force; ++i) {
+        const int64_t g = f_node[i];
+        if (g < 0 || g >= n_nodes) { free(raw); free(has); return OR_ERR_BAD_ARG; }
+        has[g] = 1;
+        for (int c = 0; c < 3; ++c) raw[6 * g + c] += f_dir3[3 * i + c] * f_mag[i] * scale;
+    }
+    for (int64_t i = 0; i < n_moment; ++i) {
+        const int64_t g = m_node[i];
+        if (g < 0 || g >= n_nodes) { free(raw); free(has); return OR_ERR_BAD_ARG; }
+        has[g] = 1;
+        for (int c = 0; c < 3; ++c) raw[6 * g + 3 + c] += m_dir3[3 * i + c] * m_mag[i] * scale;
+    }
+    for (int64_t g = 0; g < n_nodes; ++g)
+        if (has[g])
+            for (int c = 0; c < 6; ++c) F[6 * g + c] += raw[6 * g + c];
+    free(raw); free(has);
+    return 0;
+}

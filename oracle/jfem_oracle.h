@@ -84,6 +84,22 @@ int or_solve(int64_t ndof, const int64_t *row_ptr, const int32_t *col, const dou
              int nrhs, const double *b, double *u, double *relres,
              int64_t *nnz_L, double *flops, int64_t *n_small_pivots, int refine);
 
+/* ---------- load vector of one load set (FORCE / MOMENT / shell PLOAD4) ---------- */
+/* Follows _resolve_loads! (src/solver/loads.jl:510-587, 1338-1340) for one SID in
+ * basic coordinates: PLOAD4 cards in order straight into F, then the per-grid sums
+ * of the FORCE and MOMENT cards (card order) added to F.  Shell element e: CQUAD4 e
+ * (e < nq) or CTRIA3 e - nq; pl_dir3 NULL (or a direction of norm <= 1e-30) takes
+ * the element normal.  F: 6 n_nodes doubles (overwritten). */
+int or_load_vector(int64_t n_nodes, const double *x, const double *y, const double *z, int64_t nq,
+                   const int32_t *quad_conn, int64_t nt, const int32_t *tria_conn, int64_t n_pload4,
+                   const int32_t *pl_elem, const double *pl_p, const double *pl_dir3, int64_t n_force,
+                   const int32_t *f_node, const double *f_mag, const double *f_dir3, int64_t n_moment,
+                   const int32_t *m_node, const double *m_mag, const double *m_dir3, double scale,
+                   double *F);
+/* Julia's norm / normalize of a 3-vector, exposed for their pins */
+double or_jl_norm3(const double v[3]);
+void or_jl_normalize3(double v[3]);
+
 /* dense helper exposed for tests: symmetric 3x3 Jacobi eigen (ascending) */
 void or_eig3(const double A[9], double w[3], double V[9]);
 

@@ -50,6 +50,12 @@ def lib():
         L.or_solve.argtypes = [C.c_int64, ip64, ip32, dp, u8p, C.c_int, C.c_void_p, C.c_int, dp, dp, dp,
                                C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]
         L.or_eig3.argtypes = [dp, dp, dp]
+        vpp = C.c_void_p
+        L.or_load_vector.argtypes = [C.c_int64, dp, dp, dp, C.c_int64, vpp, C.c_int64, vpp, C.c_int64, vpp, vpp, vpp,
+                                     C.c_int64, vpp, vpp, vpp, C.c_int64, vpp, vpp, vpp, C.c_double, dp]
+        L.or_jl_norm3.argtypes = [dp]
+        L.or_jl_norm3.restype = C.c_double
+        L.or_jl_normalize3.argtypes = [dp]
         _LIB = L
     return _LIB
 
@@ -208,3 +214,53 @@ def sol101(model, order="nd", refine=0):
     u, rr, info = solve(K, fixed, model.loads, order=order, xyz=model.xyz, refine=refine)
     info["free_dofs"] = int((fixed == 0).sum())
     return K, fixed, u, rr, info
+
+
+def jl_norm3(v):
+    """Julia's norm of a 3-vector, as restated in the oracle."""
+    return lib().or_jl_norm3(np.ascontiguousarray(v, dtype=float))
+
+
+def jl_normalize3(v):
+    a = np.array(v, dtype=float)
+    lib().or_jl_normalize3(a)
+    return a
+
+
+def load_vector(model, pload4=None, force=None, moment=None, scale=1.0):
+    """Load vector of one load set (FORCE / MOMENT / shell PLOAD4, card order), all
+    6 n DOFs: src/solver/loads.jl:510-587, 1338-1340.  Cards as in
+    jf_2026_05_25_amd.solver.LoadSet."""
+    keep = []
+
+    def arr(a, dt, shape=None):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(np.asarray(a, dt))
+        if shape is not None:
+            a = a.reshape(shape)
+        keep.append(a)
+        return a.ctypes.data if a.size else None
+
+    def cards(c):
+        if c is None or len(c[0]) == 0:
+            return 0, None, None, None
+        n = len(c[0])
+        d = c[2] if len(c) > 2 else None
+        return n, arr(c[0], np.int32), arr(c[1], float, (n,)), arr(d, float, (n, 3))
+
+    npl, pe, pp, pd = cards(pload4)
+    nf, fn, fm, fd = cards(force)
+    nm, mn, mm, md = cards(moment)
+    xyz = np.asarray(model.xyz, float)
+    x, y, z = (np.ascontiguousarray(xyz[:, k]) for k in range(3))
+    qc = _cm_i32(model.quads)
+    tc = _cm_i32(model.trias)
+    keep += [qc, tc]
+    F = np.zeros(6 * len(xyz))
+    st = lib().or_load_vector(len(xyz), x, y, z, len(model.quads), qc.ctypes.data if qc.size else None,
+                              len(model.trias), tc.ctypes.data if tc.size else None, npl, pe, pp, pd,
+                              nf, fn, fm, fd, nm, mn, mm, md, float(scale), F)
+    if st != 0:
+        raise OracleError(st, "load_vector")
+    return F

@@ -1,0 +1,167 @@
+"""Load-vector restatement of the oracle (FORCE / MOMENT / shell PLOAD4 of one
+load set, src/solver/loads.jl:510-587, 1338-1340), pinned on the CPU by closed
+forms, by Julia's documented norm / normalize arithmetic, by the reference's
+summation order, and by the golden decks' own loads."""
+import math
+
+import numpy as np
+import pytest
+
+from jf_2026_05_25_amd import meshes
+from jf_2026_05_25_amd.solver import single_precision
+
+from _helpers import golden_cases, golden_model
+
+
+def one_quad(X, tria=False):
+    """A model with one CQUAD4 (or CTRIA3) on the given corners."""
+    X = np.asarray(X, float)
+    n = len(X)
+    conn = np.arange(n, dtype=np.int32)[None, :]
+    empty4 = np.zeros((0, 4), np.int32)
+    empty3 = np.zeros((0, 3), np.int32)
+    return meshes.Model("one", X, empty4 if tria else conn, np.zeros((0 if tria else 1, 8)),
+                        conn if tria else empty3, np.zeros((1 if tria else 0, 8)), np.zeros((0, 10), np.int32),
+                        np.zeros((0, 2)), np.zeros(6 * n, np.uint8), np.zeros((6 * n, 1)), 100.0, {})
+
+
+# ---- Julia's norm / normalize (LinearAlgebra.generic_norm2, __normalize!) -------
+def test_jl_norm3_paths(oracle):
+    assert oracle.jl_norm3([3.0, 4.0, 0.0]) == 5.0
+    assert oracle.jl_norm3([0.0, 0.0, 0.0]) == 0.0
+    assert oracle.jl_norm3([np.inf, 1.0, 0.0]) == np.inf
+    assert math.isnan(oracle.jl_norm3([np.nan, 1.0, 0.0]))
+    # squares overflow / underflow: the scaled branch keeps the exact magnitude
+    big = oracle.jl_norm3([3e200, 4e200, 0.0])
+    assert big == pytest.approx(5e200, rel=1e-15)
+    tiny = oracle.jl_norm3([3e-200, 4e-200, 0.0])
+    assert tiny == pytest.approx(5e-200, rel=1e-15)
+    # plain branch: the left-to-right sum of squares, rounded once by sqrt
+    v = np.array([0.1, 0.2, 0.3])
+    assert oracle.jl_norm3(v) == math.sqrt((0.1 * 0.1 + 0.2 * 0.2) + 0.3 * 0.3)
+
+
+def test_jl_normalize3_multiplies_by_the_reciprocal(oracle):
+    # Julia: normalize([3.0, 4.0, 0.0]) == [0.6000000000000001, 0.8, 0.0]
+    # (3 * inv(5), not 3 / 5)
+    d = oracle.jl_normalize3([3.0, 4.0, 0.0])
+    assert d[0] == 3.0 * (1.0 / 5.0) and d[0] != 3.0 / 5.0
+    assert d[1] == 0.8 and d[2] == 0.0
+    # a norm below 1/floatmax goes through the two-step scaling and stays a unit vector
+    d = oracle.jl_normalize3([1e-310, 0.0, 0.0])
+    assert d[0] == pytest.approx(1.0, rel=1e-15) and d[1] == 0.0
+
+
+# ---- PLOAD4 closed forms -----------------------------------------------------------
+def test_plate_pressure_totals_and_shares(oracle):
+    p = 1250.0
+    m = meshes.flat_plate(8, 6, Lx=0.4, Ly=0.3, pressure=p, tria_fraction=0.3)
+    ne = len(m.quads) + len(m.trias)
+    F = oracle.load_vector(m, pload4=(np.arange(ne), np.full(ne, p)))
+    Fn = F.reshape(-1, 6)
+    assert np.all(Fn[:, [0, 1, 3, 4, 5]] == 0.0)               # z = 0 plane: normals are exactly +z
+    assert Fn[:, 2].sum() == pytest.approx(p * 0.4 * 0.3, rel=1e-13)
+    # the mesh's own (numpy) builder agrees to rounding
+    np.testing.assert_allclose(F, m.loads[:, 0], rtol=1e-13, atol=1e-13 * abs(F).max())
+
+
+def test_quad_plate_nodal_shares(oracle):
+    # uniform quads: an interior grid point collects four quarter shares p dx dy / 4,
+    # a plate corner one
+    p, dx, dy = 1250.0, 0.05, 0.05
+    m = meshes.flat_plate(8, 6, Lx=0.4, Ly=0.3, pressure=p)
+    F = oracle.load_vector(m, pload4=(np.arange(len(m.quads)), np.full(len(m.quads), p))).reshape(9 * 7 // 9, 9, 6)
+    np.testing.assert_allclose(F[1:-1, 1:-1, 2], p * dx * dy, rtol=1e-13)
+    np.testing.assert_allclose(F[[0, 0, -1, -1], [0, -1, 0, -1], 2], p * dx * dy / 4, rtol=1e-13)
+    np.testing.assert_allclose(F[0, 1:-1, 2], p * dx * dy / 2, rtol=1e-13)
+
+
+def test_tilted_rectangle_area_and_direction(oracle):
+    # a 0.3 x 0.2 rectangle rotated about x by 30 deg, then about z by 40 deg
+    a, b = np.radians(30.0), np.radians(40.0)
+    Rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    Rz = np.array([[np.cos(b), -np.sin(b), 0], [np.sin(b), np.cos(b), 0], [0, 0, 1]])
+    R = Rz @ Rx
+    X = np.array([[0, 0, 0], [0.3, 0, 0], [0.3, 0.2, 0], [0, 0.2, 0]]) @ R.T + [1.0, -2.0, 0.5]
+    m = one_quad(X)
+    F = oracle.load_vector(m, pload4=([0], [100.0])).reshape(-1, 6)
+    n_hat = R @ np.array([0.0, 0.0, 1.0])                      # right-hand rule on 1-2-3-4
+    for k in range(4):
+        np.testing.assert_allclose(F[k, :3], n_hat * 100.0 * 0.06 / 4, rtol=1e-13, atol=1e-15)
+    # reversed node order flips the load
+    F2 = oracle.load_vector(one_quad(X[::-1]), pload4=([0], [100.0])).reshape(-1, 6)
+    np.testing.assert_allclose(F2[::-1, :3], -F[:, :3], rtol=1e-13, atol=1e-15)
+
+
+def test_tria_share_and_explicit_direction(oracle):
+    X = np.array([[0, 0, 0], [2.0, 0, 0], [0, 1.0, 1.0]])
+    m = one_quad(X, tria=True)
+    area = 0.5 * np.linalg.norm(np.cross(X[1] - X[0], X[2] - X[0]))
+    F = oracle.load_vector(m, pload4=([0], [3.0])).reshape(-1, 6)
+    assert np.linalg.norm(F[0, :3]) == pytest.approx(3.0 * area / 3.0, rel=1e-14)
+    # an explicit direction N: the share along N / |N| whatever the normal
+    N = np.array([[0.0, 2.0, 0.0]])
+    F = oracle.load_vector(m, pload4=([0], [3.0], N)).reshape(-1, 6)
+    for k in range(3):
+        assert F[k, 1] == (2.0 / 2.0) * (area * 3.0 * 1.0 / 3.0) and F[k, 0] == 0.0 and F[k, 2] == 0.0
+    # a direction of norm <= 1e-30 falls back to the normal
+    F0 = oracle.load_vector(m, pload4=([0], [3.0], np.zeros((1, 3)))).reshape(-1, 6)
+    F1 = oracle.load_vector(m, pload4=([0], [3.0])).reshape(-1, 6)
+    assert np.array_equal(F0, F1)
+
+
+def test_degenerate_element_takes_no_load(oracle):
+    X = np.array([[0, 0, 0], [1.0, 0, 0], [1.0, 0, 0], [0, 0, 0]])   # zero area
+    F = oracle.load_vector(one_quad(X), pload4=([0], [1e6]))
+    assert np.all(F == 0.0)
+
+
+# ---- point loads and the reference's order ---------------------------------------------
+def test_force_and_moment_cards(oracle):
+    m = meshes.flat_plate(2, 2)
+    F = oracle.load_vector(m, force=([4, 4, 0], [2.0, 3.0, -1.0], [[1, 0, 0], [0, 1, 1], [0, 0, 1]]),
+                           moment=([8], [5.0], [[0, 0, 2]]), scale=0.5)
+    Fn = F.reshape(-1, 6)
+    assert list(Fn[4]) == [1.0, 1.5, 1.5, 0, 0, 0]
+    assert list(Fn[0]) == [0, 0, -0.5, 0, 0, 0]
+    assert list(Fn[8]) == [0, 0, 0, 0, 0, 5.0]
+    assert np.count_nonzero(Fn) == 5
+
+
+def test_point_loads_are_summed_per_grid_before_they_are_added(oracle):
+    # unit-square CQUAD4, p = 4: each corner gets exactly 1.0 along +z.  Two FORCE
+    # cards on corner 0 of +2^53 and -2^53 along z: the reference sums them into
+    # raw_forces first (0 + 2^53 - 2^53 = 0) and then adds, so F_z = 1.0; adding
+    # each card to F in turn would give (1 + 2^53) - 2^53 = 0.
+    X = np.array([[0, 0, 0], [1.0, 0, 0], [1.0, 1.0, 0], [0, 1.0, 0]])
+    m = one_quad(X)
+    big = 2.0 ** 53
+    F = oracle.load_vector(m, pload4=([0], [4.0]), force=([0, 0], [big, -big], [[0, 0, 1], [0, 0, 1]]))
+    assert F[2] == 1.0 and F[6 + 2] == 1.0
+
+
+def test_single_precision_of_point_loads():
+    # load_single_precision (src/parsing/extract_loads.jl:15-22): finite values in
+    # range are rounded to Float32, others kept
+    v = single_precision([0.1, 1e300, np.inf, 0.5])
+    assert v[0] == float(np.float32(0.1)) and v[0] != 0.1
+    assert v[1] == 1e300 and v[2] == np.inf and v[3] == 0.5
+
+
+# ---- golden decks: the recorded load cases rebuilt by the oracle ----------------
+@pytest.mark.parametrize("ci", range(len(golden_cases())))
+def test_golden_deck_loads(oracle, ci):
+    case = golden_cases()[ci]
+    for sub in case["subcases"]:
+        m = golden_model(case, sub)
+        ids = sorted(int(k) for k in case["grids"])
+        idx = {g: i for i, g in enumerate(ids)}
+        eid_row = {q["eid"]: e for e, q in enumerate(case["cquad4"])}
+        fc = sub["force"]
+        pl = sub["pload4"]
+        F = oracle.load_vector(
+            m,
+            pload4=([eid_row[p["eid"]] for p in pl], [p["p"] for p in pl]) if pl else None,
+            force=([idx[f["grid"]] for f in fc], single_precision([f["mag"] for f in fc]),
+                   single_precision([f["dir"] for f in fc])) if fc else None)
+        np.testing.assert_allclose(F, m.loads[:, 0], rtol=1e-14, atol=1e-14 * abs(F).max())
