@@ -1,0 +1,471 @@
+"""Bulk-data decks in, displacement results out, for the linear-static path.
+
+Reads the cards this path covers -- GRID, CQUAD4, CTRIA3, PSHELL, MAT1, SPC1,
+FORCE, MOMENT, PLOAD4, PARAM K6ROT -- and the case control that selects them
+(SUBCASE, SPC =, LOAD =), with the reference's field rules
+(src/parsing/card_processor.jl:73-139, src/parsing/utilities.jl:18-170):
+small-field (8-column), large-field (name*, 16-column) and free-field (comma)
+lines, continuations that start with '+' or '*', have a blank name field or
+start with a comma, inline '$' comments, and NASTRAN numbers with an embedded
+exponent ("1.0+7", "2.5-3", D exponents).  The card fields follow the
+reference's extractors: GRID (extract_geometry.jl), CQUAD4/CTRIA3
+(extract_elements.jl), PSHELL (extract_properties.jl:3-40), MAT1
+(extract_materials.jl:13-45), SPC1 with THRU (extract_constraints.jl:3-14,
+utilities.jl expand_reference_list), FORCE/MOMENT with single-precision
+magnitudes and directions (extract_loads.jl:15-40), PLOAD4 with THRU
+(extract_loads.jl:75-125).
+
+Anything else a selected subcase would need -- coordinate systems (CP/CD/CID
+not 0), other element or load cards (PLOAD2, GRAV, LOAD combinations, ...),
+corner pressures on CQUAD4/CTRIA3, different materials per PSHELL slot -- is
+rejected with NotImplementedError rather than dropped.
+
+``solve_deck`` runs every subcase through the device path (element kernels,
+assembly, BC partition, factor, refined solve, device load vectors), and
+``ju_results`` / ``write_ju`` give the JU.JSON layout of the reference's
+recorded results (``analysis_type``, ``displacements`` per subcase in grid
+order, ``solver_diagnostics`` with the free-DOF count and the residual).
+"""
+from __future__ import annotations
+
+import json
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import meshes
+
+SUPPORTED = {"GRID", "CQUAD4", "CTRIA3", "PSHELL", "MAT1", "SPC1", "FORCE", "MOMENT", "PLOAD4", "PARAM"}
+LOAD_CARDS = {"FORCE", "MOMENT", "PLOAD4", "PLOAD", "PLOAD1", "PLOAD2", "GRAV", "ACCEL", "ACCEL1", "LOAD",
+              "FORCE1", "FORCE2", "MOMENT1", "MOMENT2", "TEMP", "TEMPD", "SLOAD", "RFORCE"}
+_EXP = re.compile(r"([\d.])([+-])(\d)")
+
+
+def parse_number(field, default=None):
+    """A bulk-data number (parse_reference_number, src/parsing/utilities.jl:23-47)."""
+    if field is None:
+        return default
+    s = str(field).strip()
+    if not s:
+        return default
+    try:
+        return float(s)
+    except ValueError:
+        pass
+    t = s.replace("D", "e").replace("d", "e")
+    t = _EXP.sub(r"\1e\2\3", t).replace("ee", "e")
+    try:
+        return float(t)
+    except ValueError:
+        return default
+
+
+def to_id(v) -> int:
+    return int(round(v)) if isinstance(v, float) else (v if isinstance(v, int) else 0)
+
+
+def _strip_comment(line: str) -> str:
+    k = line.find("$")
+    return line if k < 0 else line[:k]
+
+
+def _card_name(line: str) -> str:
+    k = line.find(",")
+    if k >= 0:
+        return line[:k].strip().upper()
+    if "\t" in line:
+        parts = line.split()
+        if parts:
+            return parts[0].strip().upper()
+    return line[:8].strip().upper()
+
+
+def _fixed_fields(line: str, large: bool):
+    padded = line.ljust(80)
+    if large:
+        return [padded[9 + 16 * i - 1:9 + 16 * i + 15].strip() for i in range(4)]
+    return [padded[9 + 8 * i - 1:9 + 8 * i + 7].strip() for i in range(8)]
+
+
+def _fields(line: str, large: bool):
+    """Data fields of one line (get_reference_fields_from_line)."""
+    if "," in line:
+        parts = line.split(",")
+        head = parts[0]
+        out = []
+        hs = head.strip()
+        if (hs.startswith("+") or hs.startswith("*")) and len(head) > 8:
+            payload = head[8:].strip()
+            if payload:
+                out.append(payload)
+        out += [p.strip() for p in parts[1:]]
+        while len(out) < 8:
+            out.append("")
+        return out
+    if "\t" in line:
+        raise NotImplementedError("tab-separated bulk data is not read here")
+    return _fixed_fields(line, large)
+
+
+def bulk_cards(lines):
+    """Cards as [name, field1, field2, ...] (process_cards, src/parsing/card_processor.jl:73-139)."""
+    out = []
+    i = 0
+    n = len(lines)
+    while i < n:
+        line = _strip_comment(lines[i])
+        clean = line.strip()
+        if not clean or clean.startswith("$"):
+            i += 1
+            continue
+        name = _card_name(line)
+        if name.startswith("+") or name == "*":
+            i += 1
+            continue
+        large = name.endswith("*")
+        base = name[:-1].strip() if large else name
+        if not base:
+            i += 1
+            continue
+        f = [base] + _fields(line, large)
+        steps = 1
+        while i + steps < n:
+            nxt = _strip_comment(lines[i + steps])
+            nc = nxt.strip()
+            if not nc:
+                steps += 1
+                continue
+            cont_large = nc.startswith("*")
+            cont = cont_large or not _card_name(nxt) or nc.startswith("+") or ("," in nxt and nc.startswith(","))
+            if not cont:
+                break
+            f += _fields(nxt, large or cont_large)
+            steps += 1
+        out.append(f)
+        i += steps
+    return out
+
+
+@dataclass
+class Subcase:
+    id: int
+    spc: int = 0
+    load: int = 0
+    label: str = ""
+
+
+@dataclass
+class Deck:
+    grid_ids: list
+    xyz: np.ndarray
+    ps: dict                                   # grid id -> permanent SPC component digits
+    quads: list                                # (eid, pid, [g1..g4])
+    trias: list
+    pshell: dict
+    mat1: dict
+    spc1: dict                                 # sid -> [(grid, comps)]
+    force: dict                                # sid -> [(grid, mag, dir3)]
+    moment: dict
+    pload4: dict                               # sid -> [(eid, p, dir3 or None)]
+    other_loads: dict                          # sid -> set of unsupported load card names
+    subcases: list
+    k6rot: float = 100.0
+    unsupported: set = field(default_factory=set)
+
+    # ---------------------------------------------------------------- model
+    def grid_index(self):
+        return {g: i for i, g in enumerate(self.grid_ids)}
+
+    def shell_order(self):
+        """Element id -> shell index (CQUAD4 in card order, then CTRIA3)."""
+        m = {e: k for k, (e, _, _) in enumerate(self.quads)}
+        m.update({e: len(self.quads) + k for k, (e, _, _) in enumerate(self.trias)})
+        return m
+
+    def _section(self, pid, membrane_modes):
+        if pid not in self.pshell:
+            raise NotImplementedError(f"PSHELL {pid} is not in the deck")
+        ps = self.pshell[pid]
+        mids = [m for m in (ps["mid1"], ps["mid2"], ps["mid3"]) if m]
+        if ps["mid4"]:
+            raise NotImplementedError(f"PSHELL {pid}: MID4 coupling is not on this path")
+        mats = {m: self.mat1.get(m) for m in mids}
+        if any(v is None for v in mats.values()):
+            raise NotImplementedError(f"PSHELL {pid}: a material that is not a MAT1 of the deck")
+        ref = mats[ps["mid1"]]
+        for m, v in mats.items():
+            if (v["E"], v["G"], v["nu"]) != (ref["E"], ref["G"], ref["nu"]):
+                raise NotImplementedError(f"PSHELL {pid}: different MAT1 laws per slot are not on this path")
+        return meshes.pshell_props(1, ps["t"], ref["E"], ref["nu"], ref["G"], ps["bend_ratio"], ps["ts_t"],
+                                   mid3_blank=(ps["mid3"] == 0), membrane_modes=membrane_modes)[0]
+
+    def model(self, subcase: int | None = None, membrane_modes: int = 15) -> meshes.Model:
+        """The grid, elements, sections and the SPC set of a subcase (loads zero:
+        the load vector is built on the device, see load_cards)."""
+        gi = self.grid_index()
+        quads = np.array([[gi[g] for g in n] for (_, _, n) in self.quads], np.int32).reshape(-1, 4)
+        trias = np.array([[gi[g] for g in n] for (_, _, n) in self.trias], np.int32).reshape(-1, 3)
+        qp = np.array([self._section(p, membrane_modes) for (_, p, _) in self.quads]).reshape(-1, meshes.SP_NCOL)
+        tp = np.array([self._section(p, membrane_modes) for (_, p, _) in self.trias]).reshape(-1, meshes.SP_NCOL)
+        ndof = 6 * len(self.grid_ids)
+        spc = np.zeros(ndof, np.uint8)
+        for g, comps in self.ps.items():
+            for c in comps:
+                spc[6 * gi[g] + c - 1] = 1
+        sc = self._subcase(subcase)
+        if sc.spc:
+            if sc.spc not in self.spc1:
+                raise NotImplementedError(f"SPC set {sc.spc} has no SPC1 card (SPCADD/SPC are not read here)")
+            for g, comps in self.spc1[sc.spc]:
+                if g not in gi:
+                    continue
+                for c in comps:
+                    spc[6 * gi[g] + c - 1] = 1
+        return meshes.Model(f"deck subcase {sc.id}", self.xyz, quads, qp, trias, tp, np.zeros((0, 10), np.int32),
+                            np.zeros((0, 2)), spc, np.zeros((ndof, 1)), self.k6rot,
+                            {"subcase": sc.id, "grid_ids": list(self.grid_ids)})
+
+    def _subcase(self, subcase):
+        if subcase is None:
+            return self.subcases[0]
+        for s in self.subcases:
+            if s.id == subcase:
+                return s
+        raise KeyError(f"no subcase {subcase}")
+
+    def load_cards(self, subcase: int | None = None):
+        """The selected load set as jf_2026_05_25_amd.solver.LoadSet arguments."""
+        sc = self._subcase(subcase)
+        sid = sc.load
+        if sid in self.other_loads:
+            raise NotImplementedError(f"load set {sid} uses {sorted(self.other_loads[sid])}, not on this path")
+        gi = self.grid_index()
+        so = self.shell_order()
+
+        def pts(rows):
+            rows = [r for r in rows if r[0] in gi]        # grids missing from the model are skipped (loads.jl:526)
+            if not rows:
+                return None
+            return (np.array([gi[g] for g, _, _ in rows], np.int32), np.array([m for _, m, _ in rows]),
+                    np.array([d for _, _, d in rows]))
+
+        force = pts(self.force.get(sid, []))
+        moment = pts(self.moment.get(sid, []))
+        pl = self.pload4.get(sid, [])
+        pload4 = None
+        if pl:
+            missing = [e for e, _, _ in pl if e not in so]
+            if missing:
+                raise NotImplementedError(f"PLOAD4 on elements {missing[:5]} that are not CQUAD4/CTRIA3")
+            dirs = np.array([d if d is not None else (0.0, 0.0, 0.0) for _, _, d in pl])
+            pload4 = (np.array([so[e] for e, _, _ in pl], np.int32), np.array([p for _, p, _ in pl]),
+                      dirs if any(d is not None for _, _, d in pl) else None)
+        return {"pload4": pload4, "force": force, "moment": moment}
+
+
+def _single(x):
+    """load_single_precision (src/parsing/extract_loads.jl:15-22)."""
+    if x is None or not np.isfinite(x) or abs(x) > np.finfo(np.float32).max:
+        return x
+    return float(np.float32(x))
+
+
+def _ids(fields):
+    """Grid lists with THRU (expand_reference_list, src/parsing/utilities.jl)."""
+    toks = []
+    for f in fields:
+        s = str(f).strip()
+        if s:
+            toks += s.split()
+    out = []
+    k = 0
+    while k < len(toks):
+        if toks[k].upper() == "THRU":
+            if out and k + 1 < len(toks):
+                end = to_id(parse_number(toks[k + 1]))
+                out += list(range(out[-1] + 1, end + 1))
+            k += 2
+            continue
+        v = to_id(parse_number(toks[k], 0))
+        if v > 0:
+            out.append(v)
+        k += 1
+    return out
+
+
+def read_deck(source: str) -> Deck:
+    """Parse a deck from a path or from its text."""
+    text = source
+    if "\n" not in source:
+        with open(source) as fh:
+            text = fh.read()
+    lines = text.splitlines()
+    bulk_at = next((i for i, l in enumerate(lines) if l.strip().upper().startswith("BEGIN BULK")), None)
+    case_lines = lines[:bulk_at] if bulk_at is not None else []
+    bulk = lines[bulk_at + 1:] if bulk_at is not None else lines
+    end = next((i for i, l in enumerate(bulk) if l.strip().upper().startswith("ENDDATA")), None)
+    if end is not None:
+        bulk = bulk[:end]
+
+    # case control
+    glob = Subcase(0)
+    subs = []
+    cur = glob
+    for raw in case_lines:
+        l = _strip_comment(raw).strip()
+        if not l:
+            continue
+        u = l.upper()
+        m = re.match(r"SUBCASE\s+(\d+)", u)
+        if m:
+            cur = Subcase(int(m.group(1)), glob.spc, glob.load)
+            subs.append(cur)
+            continue
+        m = re.match(r"(SPC|LOAD)\s*=\s*(\d+)", u)
+        if m:
+            setattr(cur, m.group(1).lower(), int(m.group(2)))
+            continue
+        m = re.match(r"LABEL\s*=\s*(.*)", l, re.I)
+        if m:
+            cur.label = m.group(1).strip()
+    if not subs:
+        subs = [Subcase(1, glob.spc, glob.load, glob.label)]
+
+    grids, ps = {}, {}
+    quads, trias, pshell, mat1 = [], [], {}, {}
+    spc1, force, moment, pload4, other = {}, {}, {}, {}, {}
+    k6rot = 100.0
+    unsupported = set()
+    for c in bulk_cards(bulk):
+        name = c[0]
+        g = lambda k: c[k] if k < len(c) else ""
+        num = lambda k, d=None: parse_number(g(k), d)
+        if name == "PARAM":
+            if g(1).strip().upper() == "K6ROT":
+                k6rot = num(2, 100.0)
+        elif name == "GRID":
+            gid = to_id(num(1))
+            cp, cd = to_id(num(2, 0.0)), to_id(num(6, 0.0))
+            if cp or cd:
+                raise NotImplementedError(f"GRID {gid}: coordinate systems (CP/CD) are not on this path")
+            grids[gid] = (num(3, 0.0), num(4, 0.0), num(5, 0.0))
+            if g(7).strip():
+                ps[gid] = [int(ch) for ch in g(7).strip() if ch.isdigit() and ch != "0"]
+        elif name in ("CQUAD4", "CTRIA3"):
+            nn = 4 if name == "CQUAD4" else 3
+            eid, pid = to_id(num(1)), to_id(num(2))
+            nodes = [to_id(num(3 + k)) for k in range(nn)]
+            theta, zoffs = num(3 + nn, 0.0), num(4 + nn, 0.0)
+            if (theta or 0.0) != 0.0 or (zoffs or 0.0) != 0.0:
+                raise NotImplementedError(f"{name} {eid}: THETA/MCID and ZOFFS are not on this path")
+            if any(str(g(k)).strip() for k in range(10, 15)):
+                raise NotImplementedError(f"{name} {eid}: corner thicknesses (TFLAG, Ti) are not on this path")
+            (quads if nn == 4 else trias).append((eid, pid, nodes))
+        elif name == "PSHELL":
+            pid = to_id(num(1))
+            mid2v = num(4)
+            mid2 = 0 if mid2v in (None, 0.0) else to_id(mid2v)
+            mid3v = num(6)
+            mid4v = num(11)
+            pshell[pid] = {"mid1": to_id(num(2)), "t": num(3, 0.0), "mid2": mid2,
+                           "bend_ratio": 0.0 if mid2 == 0 else num(5, 1.0),
+                           "mid3": 0 if mid3v in (None, 0.0) else to_id(mid3v), "ts_t": num(7, 5.0 / 6.0),
+                           "mid4": 0 if mid4v in (None, 0.0) else to_id(mid4v)}
+        elif name == "MAT1":
+            mid = to_id(num(1))
+            E_raw, G_raw, nu_raw = num(2), num(3), num(4)
+            E = 0.0 if E_raw is None else E_raw
+            G = 0.0 if G_raw is None else G_raw
+            nu = -1.0 if nu_raw is None else nu_raw
+            if E > 0 and G > 0 and nu < 0:
+                nu = E / (2 * G) - 1.0
+            elif E > 0 and nu >= 0 and G <= 0:
+                G = E / (2 * (1 + nu))
+            elif G > 0 and nu >= 0 and E <= 0:
+                E = 2 * G * (1 + nu)
+            if nu < 0:
+                nu = 0.3
+            mat1[mid] = {"E": E, "G": G, "nu": nu}
+        elif name == "SPC1":
+            sid = to_id(num(1))
+            comps = [int(ch) for ch in str(g(2)).strip() if ch.isdigit() and ch != "0"]
+            spc1.setdefault(sid, []).extend((gid, comps) for gid in _ids(c[3:]))
+        elif name in ("FORCE", "MOMENT"):
+            sid, gid, cid = to_id(num(1)), to_id(num(2)), to_id(num(3, 0.0))
+            if cid:
+                raise NotImplementedError(f"{name} SID={sid}: CID {cid} is not on this path")
+            mag = _single(num(4, 0.0))
+            d = tuple(_single(num(5 + k, 0.0)) for k in range(3))
+            (force if name == "FORCE" else moment).setdefault(sid, []).append((gid, mag, d))
+        elif name == "PLOAD4":
+            sid, eid = to_id(num(1)), to_id(num(2))
+            p = num(3, 0.0)
+            corner = [num(4 + k, p) for k in range(3)]
+            if any(x != p for x in corner):
+                raise NotImplementedError(f"PLOAD4 SID={sid} EID={eid}: corner pressures on CQUAD4/CTRIA3")
+            thru = str(g(7)).strip().upper() == "THRU"
+            eid2 = to_id(num(8, 0.0)) if thru else 0
+            cid = to_id(num(9, 0.0))
+            if cid:
+                raise NotImplementedError(f"PLOAD4 SID={sid}: CID {cid} is not on this path")
+            nvals = [str(g(10 + k)).strip() for k in range(3)]
+            d = tuple(parse_number(v, 0.0) for v in nvals) if any(nvals) else None
+            sorl, ldir = str(g(13)).strip().upper(), str(g(14)).strip().upper()
+            if sorl not in ("", "SURF") or ldir not in ("", "NORM"):
+                raise NotImplementedError(f"PLOAD4 SID={sid}: SORL/LDIR other than SURF/NORM")
+            for e in (range(eid, eid2 + 1) if thru and eid2 > eid else [eid]):
+                pload4.setdefault(sid, []).append((e, p, d))
+        elif name in LOAD_CARDS:
+            other.setdefault(to_id(num(1)), set()).add(name)
+        else:
+            unsupported.add(name)
+    ids = sorted(grids)                                    # GRID rank order (assembly.jl:2653-2657)
+    xyz = np.array([grids[i] for i in ids], float).reshape(-1, 3)
+    return Deck(ids, xyz, ps, quads, trias, pshell, mat1, spc1, force, moment, pload4, other, subs, k6rot,
+                unsupported)
+
+
+def solve_deck(deck: Deck, precision: int = 64, tol: float = 1e-10, membrane_modes: int = 15, ctx=None,
+               subcases=None):
+    """Every (selected) subcase through the device path.  Returns
+    {subcase id: (u (6n,), diagnostics)}."""
+    from . import solver
+    ctx = ctx or solver.default_context()
+    out = {}
+    asm = None
+    for sc in deck.subcases:
+        if subcases is not None and sc.id not in subcases:
+            continue
+        m = deck.model(sc.id, membrane_modes)
+        if asm is None:
+            asm = solver.assemble_stiffness(m, ctx)
+        ls = solver.LoadSet(asm.mesh, ctx=ctx, **deck.load_cards(sc.id))
+        F = ls.vector()
+        ls.free()
+        u, fixed, spc, diag = solver.apply_bc_and_solve(asm, m, F_applied=F[:, None], precision=precision, tol=tol)
+        out[sc.id] = (np.asarray(u).reshape(m.ndof, -1)[:, 0], diag)
+    return out
+
+
+def ju_results(deck: Deck, results: dict) -> dict:
+    """JU.JSON layout of the recorded results: displacements per subcase (grid
+    order), solver diagnostics per subcase."""
+    disp, diags = [], []
+    for sid, (u, diag) in results.items():
+        U = u.reshape(-1, 6)
+        for g, row in zip(deck.grid_ids, U):
+            disp.append({"grid_id": int(g), "t1": float(row[0]), "t2": float(row[1]), "t3": float(row[2]),
+                         "r1": float(row[3]), "r2": float(row[4]), "r3": float(row[5])})
+        lin = diag.get("linear_solver", {})
+        diags.append({"sid": int(sid), "details": {
+            "bc_partition": {"free_dofs": int(diag.get("bc_partition", {}).get("free_dofs", -1))},
+            "linear_solver": {"relative_residual": float(np.max(lin.get("relative_residual", [np.nan])))}}})
+    return {"analysis_type": "SOL101_STATIC", "displacements": disp, "solver_diagnostics": diags}
+
+
+def write_ju(path: str, deck: Deck, results: dict) -> dict:
+    d = ju_results(deck, results)
+    with open(path, "w") as fh:
+        json.dump(d, fh)
+    return d

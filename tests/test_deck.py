@@ -1,0 +1,118 @@
+"""Deck reader (GRID/CQUAD4/CTRIA3/PSHELL/MAT1/SPC1/FORCE/MOMENT/PLOAD4/PARAM and
+case control) on the CPU: the reference's field rules, the golden decks written
+in small-, large- and free-field form read back into the golden models, and
+those models solved by the oracle reproducing the reference's recorded
+displacements.  Where the reference tree is present, its own eight decks are
+read as text and compared with the fixtures made from them."""
+import os
+
+import numpy as np
+import pytest
+
+from jf_2026_05_25_amd import deck as D
+
+from _decks import fixture_deck
+from _helpers import golden_cases, golden_model
+
+REF_CASES = "/root/reference/validation/cases"
+
+
+def test_numbers():
+    assert D.parse_number("1.0+7") == 1.0e7
+    assert D.parse_number("2.5-3") == 2.5e-3
+    assert D.parse_number("-.5") == -0.5
+    assert D.parse_number("1.D3") == 1000.0
+    assert D.parse_number("6.825+7") == 6.825e7
+    assert D.parse_number("7") == 7.0
+    assert D.parse_number("", 3.0) == 3.0
+    assert D.parse_number("THRU") is None
+
+
+def test_field_rules():
+    text = "\n".join([
+        "GRID    1               1.      2.      3.",
+        "GRID,2,,4.,5.,6.",
+        "GRID*   3                               7.0             8.0",
+        "*       9.0",
+        "CQUAD4  10      1       1       2       3       4      ",
+        "+C1                     0.       0.",
+        "PSHELL  1       1       .1      1               1       $ comment",
+        "        .8",
+        "MAT1,1,2.+5,,.3",
+        "SPC1,1,123,1,THRU,3",
+        ",4",
+    ])
+    cards = D.bulk_cards(text.splitlines())
+    names = [c[0] for c in cards]
+    assert names == ["GRID", "GRID", "GRID", "CQUAD4", "PSHELL", "MAT1", "SPC1"]
+    assert cards[0][1:6] == ["1", "", "1.", "2.", "3."]
+    assert cards[1][1:6] == ["2", "", "4.", "5.", "6."]
+    assert [D.parse_number(x) for x in cards[2][3:6]] == [7.0, 8.0, 9.0]
+    assert cards[3][9:13] == ["", "", "0.", "0."]                      # continuation: fields 9.. of the card
+    assert cards[4][6] == "1" and D.parse_number(cards[4][9]) == 0.8    # blank-name continuation
+    assert D._ids(cards[6][3:]) == [1, 2, 3, 4]                         # THRU, comma continuation
+
+
+@pytest.mark.parametrize("style", ["small", "large", "free"])
+@pytest.mark.parametrize("ci", range(len(golden_cases())))
+def test_golden_decks_read_back(oracle, ci, style):
+    case = golden_cases()[ci]
+    dk = D.read_deck(fixture_deck(case, style))
+    assert [s.id for s in dk.subcases] == [s["subcase"] for s in case["subcases"]]
+    for s in case["subcases"]:
+        ref = golden_model(case, s, membrane_modes=6)
+        m = dk.model(s["subcase"], membrane_modes=6)
+        assert np.array_equal(m.xyz, ref.xyz) and np.array_equal(m.quads, ref.quads)
+        assert np.array_equal(m.spc, ref.spc) and m.k6rot == ref.k6rot
+        # sections: G derived from E, nu where MAT1 leaves it blank (the kernels do the same)
+        mp, rp = m.quad_props.copy(), ref.quad_props.copy()
+        blank = rp[:, 3] == 0.0
+        rp[blank, 3] = rp[blank, 1] / (2 * (1 + rp[blank, 2]))
+        assert np.array_equal(mp, rp)
+        F = oracle.load_vector(m, **dk.load_cards(s["subcase"]))
+        np.testing.assert_allclose(F, ref.loads[:, 0], rtol=1e-14, atol=1e-14 * abs(F).max())
+
+
+@pytest.mark.parametrize("ci", [0, 1, 3, 5])
+def test_golden_decks_solved_by_the_oracle(oracle, ci):
+    """Deck text -> model + load vector -> oracle solve -> the recorded displacements."""
+    from test_oracle_system import GOLDEN_TOL
+    case = golden_cases()[ci]
+    dk = D.read_deck(fixture_deck(case, "small"))
+    for s in case["subcases"]:
+        m = dk.model(s["subcase"], membrane_modes=6)
+        m.loads = oracle.load_vector(m, **dk.load_cards(s["subcase"]))[:, None]
+        K, fixed, u, rr, info = oracle.sol101(m)
+        assert info["free_dofs"] == s["reference_free_dofs"]
+        exp = golden_model(case, s).meta["expected"]
+        assert np.linalg.norm(u[:, 0] - exp) / np.linalg.norm(exp) < GOLDEN_TOL[(case["case"], s["subcase"])]
+        ju = D.ju_results(dk, {s["subcase"]: (u[:, 0], {"bc_partition": {"free_dofs": info["free_dofs"]},
+                                                        "linear_solver": {"relative_residual": [rr[0]]}})})
+        assert ju["analysis_type"] == "SOL101_STATIC" and len(ju["displacements"]) == len(dk.grid_ids)
+        assert ju["solver_diagnostics"][0]["details"]["bc_partition"]["free_dofs"] == s["reference_free_dofs"]
+
+
+def test_rejections():
+    with pytest.raises(NotImplementedError):
+        D.read_deck("BEGIN BULK\nGRID,1,,0.,0.,0.,2\nENDDATA\n")          # CD = 2
+    with pytest.raises(NotImplementedError):
+        D.read_deck("BEGIN BULK\nPLOAD4,1,7,1.,2.\nENDDATA\n")             # corner pressures
+    dk = D.read_deck("SUBCASE 1\nLOAD = 5\nBEGIN BULK\nGRID,1,,0.,0.,0.\nPLOAD2,5,1.,7\nENDDATA\n")
+    with pytest.raises(NotImplementedError):
+        dk.load_cards(1)                                                      # PLOAD2 is not on this path
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_CASES), reason="reference tree not present")
+@pytest.mark.parametrize("ci", range(len(golden_cases())))
+def test_reference_decks_match_their_fixtures(oracle, ci):
+    """The reference's own deck (read as text) gives the model and loads the
+    fixtures were made from (tests/golden/make_fixtures.py)."""
+    case = golden_cases()[ci]
+    dk = D.read_deck(os.path.join(REF_CASES, case["case"], "model.bdf"))
+    for s in case["subcases"]:
+        ref = golden_model(case, s, membrane_modes=6)
+        m = dk.model(s["subcase"], membrane_modes=6)
+        assert np.array_equal(m.xyz, ref.xyz) and np.array_equal(m.quads, ref.quads)
+        assert np.array_equal(m.spc, ref.spc)
+        F = oracle.load_vector(m, **dk.load_cards(s["subcase"]))
+        np.testing.assert_allclose(F, ref.loads[:, 0], rtol=1e-14, atol=1e-14 * abs(F).max())
