@@ -1592,9 +1592,45 @@ __global__ void __launch_bounds__(kBS) residual_kernel(int64_t n, const int32_t 
     }
 }
 
-// per-block partial sums of squares, per RHS; then a fixed-order tree reduction
+// The block that finishes last (a device-scope ticket, reset by that block for
+// the next launch) adds the kBS blocks' partials of each RHS in a fixed tree
+// order: one launch per reduction, the same sums whichever block is last.  The
+// partials go out as device-coherent stores, completed before the ticket, and
+// are read back with device-coherent loads (the protocol of the solve kernels:
+// no fence, which would write back and invalidate the whole L2 per block).
+// sh: the block's reduction buffer (free once its partials are stored).
 template <int NR>
-__global__ void __launch_bounds__(kBS) sumsq_partial_kernel(int64_t n, const double *__restrict__ v, double *__restrict__ part) {
+__device__ __forceinline__ void reduce_last_block(double (*sh)[kBS], double *part, double *out,
+                                                  unsigned *ticket) {
+    __shared__ bool last;
+    if (threadIdx.x < NR) {
+        st_dev(part + (int64_t)threadIdx.x * gridDim.x + blockIdx.x, sh[threadIdx.x][0]);
+        complete_stores();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] = ld_dev(part + (int64_t)r * kBS + threadIdx.x);
+    __syncthreads();
+    for (int w = kBS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < NR) out[threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x == 0) st_dev(ticket, 0u);
+}
+
+// per-block partial sums of squares, per RHS, in tree order; the last block adds
+// the partials (reduce_last_block) into out.  Launched with kBS blocks.
+template <int NR>
+__global__ void __launch_bounds__(kBS) sumsq_partial_kernel(int64_t n, const double *__restrict__ v, double *__restrict__ part,
+                                                            double *__restrict__ out, unsigned *ticket) {
     __shared__ double sh[NR][kBS];
     double a[NR];
 #pragma unroll
@@ -1616,22 +1652,7 @@ __global__ void __launch_bounds__(kBS) sumsq_partial_kernel(int64_t n, const dou
         }
         __syncthreads();
     }
-    if (threadIdx.x < NR) part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
-}
-
-// out[r] = sum of the nblk (= kBS) partials of RHS r, one block, tree order
-__global__ void __launch_bounds__(kBS) sumsq_final_kernel(int nrhs, const double *__restrict__ part, double *__restrict__ out) {
-    __shared__ double sh[kBS];
-    for (int r = 0; r < nrhs; ++r) {
-        sh[threadIdx.x] = part[(int64_t)r * kBS + threadIdx.x];
-        __syncthreads();
-        for (int w = kBS / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[r] = sh[0];
-        __syncthreads();
-    }
+    reduce_last_block<NR>(sh, part, out, ticket);
 }
 
 // q = K_ff p (plain FP64), p given in the global interleaved layout (zero on fixed
@@ -1685,10 +1706,12 @@ __global__ void __launch_bounds__(kBS) spmv_kernel(int64_t n, const int32_t *__r
     }
 }
 
-// per-block partial dot products a.b per RHS (tree order), for reduce_final_kernel
+// per-block partial dot products a.b per RHS (tree order); the last block adds
+// the partials into out.  Launched with kBS blocks.
 template <int NR>
 __global__ void __launch_bounds__(kBS) dot_partial_kernel(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
-                                                          double *__restrict__ part) {
+                                                          double *__restrict__ part, double *__restrict__ out,
+                                                          unsigned *ticket) {
     __shared__ double sh[NR][kBS];
     double acc[NR];
 #pragma unroll
@@ -1707,7 +1730,7 @@ __global__ void __launch_bounds__(kBS) dot_partial_kernel(int64_t n, const doubl
         }
         __syncthreads();
     }
-    if (threadIdx.x < NR) part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
+    reduce_last_block<NR>(sh, part, out, ticket);
 }
 
 // conjugate-gradient scalars per RHS (one thread per RHS):
@@ -2168,6 +2191,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     const size_t oXgh = take(ng * 8), oXgl = take(ng * 8), oPg = take(ng * 8);
     const size_t oPart = take((size_t)nblk * 16 * 8), oNrm = take(16 * 8), oRho = take(16 * 8), oNum = take(16 * 8);
     const size_t oAl = take(16 * 8), oBe = take(16 * 8), oActO = take(16 * 4), oActI = take(16 * 4);
+    const size_t oTicket = take(sizeof(unsigned));
     JFEM_CHECK_HIP(ctx, ensure_scratch(ctx, off));
     char *base = (char *)ctx->scratch;
     T *Y = (T *)(base + oY), *W = (T *)(base + oW), *Zs = (T *)(base + oZs), *Xs = (T *)(base + oXs);
@@ -2179,6 +2203,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     double *part = (double *)(base + oPart), *nrm = (double *)(base + oNrm), *rho = (double *)(base + oRho);
     double *num = (double *)(base + oNum), *alpha = (double *)(base + oAl), *beta = (double *)(base + oBe);
     int32_t *act_o = (int32_t *)(base + oActO), *act_i = (int32_t *)(base + oActI);
+    unsigned *ticket = (unsigned *)(base + oTicket);
     hipStream_t st = ctx->stream;
     const dim3 gv(grid_for(nv)), gg(grid_for(ng)), blk(kBS);
     const dim3 grow((unsigned)std::max<int64_t>(1, (n * 8 + kBS - 1) / kBS));
@@ -2186,6 +2211,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, xgl, ng);
     hipLaunchKernelGGL(fill_kernel<double>, gg, blk, 0, st, pg, ng);
     hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(1), dim3(64), 0, st, cnt + H.ns, (int64_t)1);
+    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(1), dim3(64), 0, st, (int32_t *)ticket, (int64_t)1);
     for (int r = 0; r < NR; ++r) { iters[r] = 0; relres[r] = 0.0; }
     if (relres_out)
         for (int r = 0; r < NR; ++r) relres_out[r] = 0.0;
@@ -2196,15 +2222,13 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
         hipLaunchKernelGGL(gather_rhs_kernel, gv, blk, 0, st, n, NR, ndof, S->d_perm, b_dev, Bp);
         hipLaunchKernelGGL(to_t_kernel<double>, gv, blk, 0, st, nv, Bp, R);
         auto norms = [&](const double *v, double *outn) -> hipError_t {
-            hipLaunchKernelGGL(sumsq_partial_kernel<NR>, dim3(nblk), blk, 0, st, n, v, part);
-            hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), blk, 0, st, NR, part, nrm);
+            hipLaunchKernelGGL(sumsq_partial_kernel<NR>, dim3(nblk), blk, 0, st, n, v, part, nrm, ticket);
             hipError_t e = hipMemcpyAsync(outn, nrm, NR * 8, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);
             return e;
         };
         auto dot = [&](const double *a, const double *b2, double *out_dev) {
-            hipLaunchKernelGGL(dot_partial_kernel<NR>, dim3(nblk), blk, 0, st, n, a, b2, part);
-            hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), blk, 0, st, NR, part, out_dev);
+            hipLaunchKernelGGL(dot_partial_kernel<NR>, dim3(nblk), blk, 0, st, n, a, b2, part, out_dev, ticket);
         };
         auto prec = [&](const double *sv, double *z) -> hipError_t {
             timer_start(ctx);
