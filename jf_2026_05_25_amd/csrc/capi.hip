@@ -37,36 +37,6 @@ static hipError_t upload(T **dst, const T *src, size_t n, hipStream_t s) {
     return hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s);
 }
 
-// symmetric 3x3 eigen-decomposition by cyclic Jacobi (ascending eigenvalues)
-static void eig3(const double A[9], double w[3], double V[9]) {
-    double a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) a[i][j] = A[i + 3 * j];
-    for (int sweep = 0; sweep < 50; ++sweep) {
-        double off = std::fabs(a[0][1]) + std::fabs(a[0][2]) + std::fabs(a[1][2]);
-        double sc = std::fabs(a[0][0]) + std::fabs(a[1][1]) + std::fabs(a[2][2]);
-        if (off <= 1e-300 || off <= 1e-18 * sc) break;
-        for (int p = 0; p < 2; ++p)
-            for (int q = p + 1; q < 3; ++q) {
-                if (a[p][q] == 0.0) continue;
-                double th = (a[q][q] - a[p][p]) / (2 * a[p][q]);
-                double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
-                double c = 1 / std::sqrt(t * t + 1), s = t * c;
-                for (int k = 0; k < 3; ++k) { double akp = a[k][p], akq = a[k][q]; a[k][p] = c * akp - s * akq; a[k][q] = s * akp + c * akq; }
-                for (int k = 0; k < 3; ++k) { double apk = a[p][k], aqk = a[q][k]; a[p][k] = c * apk - s * aqk; a[q][k] = s * apk + c * aqk; }
-                for (int k = 0; k < 3; ++k) { double vkp = v[k][p], vkq = v[k][q]; v[k][p] = c * vkp - s * vkq; v[k][q] = s * vkp + c * vkq; }
-            }
-    }
-    int idx[3] = {0, 1, 2};
-    for (int i = 0; i < 3; ++i)
-        for (int j = i + 1; j < 3; ++j)
-            if (a[idx[j]][idx[j]] < a[idx[i]][idx[i]]) std::swap(idx[i], idx[j]);
-    for (int k = 0; k < 3; ++k) {
-        w[k] = a[idx[k]][idx[k]];
-        for (int r = 0; r < 3; ++r) V[r + 3 * k] = v[r][idx[k]];
-    }
-}
-
 }  // namespace jfem
 
 using namespace jfem;
@@ -521,86 +491,22 @@ int jfem_bc_partition(jfem_ctx *ctx, const jfem_matrix *K, const uint8_t *spc, d
     const int64_t nn = P->n_nodes, ndof = P->ndof;
     if (n_free) *n_free = 0;
     if (ndof == 0) return JFEM_OK;
-    const size_t o_spc = 0, o_fix = align256(ndof), o_flag = o_fix + align256(ndof), o_nodes = o_flag + align256(nn);
-    const size_t o_blk = o_nodes + align256(4 * nn);
-    JFEM_CHECK_HIP(ctx, grow_bc_dev(ctx, o_blk));
-    JFEM_CHECK_HIP(ctx, grow_bc_host(ctx, align256(ndof) + align256(nn)));
+    const size_t o_spc = 0, o_fix = align256(ndof), o_flag = o_fix + align256(ndof);
+    JFEM_CHECK_HIP(ctx, grow_bc_dev(ctx, o_flag + align256(nn)));
+    JFEM_CHECK_HIP(ctx, grow_bc_host(ctx, align256(ndof)));
     char *dv = (char *)ctx->bc_dev;
-    uint8_t *hfix = (uint8_t *)ctx->bc_host, *hflag = hfix + align256(ndof);
+    uint8_t *hfix = (uint8_t *)ctx->bc_host;
     if (spc) JFEM_CHECK_HIP(ctx, hipMemcpyAsync(dv + o_spc, spc, ndof, hipMemcpyHostToDevice, ctx->stream));
     else JFEM_CHECK_HIP(ctx, hipMemsetAsync(dv + o_spc, 0, ndof, ctx->stream));
     const double thresh = 1e3 * eps_zero;
     JFEM_CHECK_HIP(ctx, launch_bc_screen(ctx->stream, P, K->d_val, (const uint8_t *)(dv + o_spc), thresh,
                                          (uint8_t *)(dv + o_fix), (uint8_t *)(dv + o_flag)));
+    // grid-point singularity of the flagged groups, decided on the device
+    JFEM_CHECK_HIP(ctx, launch_bc_decide(ctx->stream, P, K->d_val, (const uint8_t *)(dv + o_flag), eps_zero,
+                                         (uint8_t *)(dv + o_fix)));
     JFEM_CHECK_HIP(ctx, hipMemcpyAsync(hfix, dv + o_fix, ndof, hipMemcpyDeviceToHost, ctx->stream));
-    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(hflag, dv + o_flag, nn, hipMemcpyDeviceToHost, ctx->stream));
     JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::memcpy(fixed, hfix, ndof);
-    std::vector<int32_t> nodes;
-    for (int64_t n = 0; n < nn; ++n)
-        if (hflag[n]) nodes.push_back((int32_t)n);
-    const int64_t nf = (int64_t)nodes.size();
-    std::vector<double> blk(36 * (size_t)nf);
-    if (nf > 0) {
-        JFEM_CHECK_HIP(ctx, grow_bc_dev(ctx, o_blk + 36 * 8 * (size_t)nf));
-        dv = (char *)ctx->bc_dev;
-        JFEM_CHECK_HIP(ctx, hipMemcpyAsync(dv + o_nodes, nodes.data(), 4 * nf, hipMemcpyHostToDevice, ctx->stream));
-        JFEM_CHECK_HIP(ctx, launch_gather_nodes(ctx->stream, P, K->d_val, (const int32_t *)(dv + o_nodes), nf,
-                                                (double *)(dv + o_blk)));
-        JFEM_CHECK_HIP(ctx, hipMemcpyAsync(blk.data(), dv + o_blk, 36 * 8 * nf, hipMemcpyDeviceToHost, ctx->stream));
-        JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    }
-    for (int64_t q = 0; q < nf; ++q) {
-        const int64_t n = nodes[q];
-        auto kv = [&](int a, int b) { return blk[36 * q + 6 * a + b]; };
-        for (int off = 0; off <= 3; off += 3) {
-            int fl[3], m = 0;
-            for (int a = 0; a < 3; ++a)
-                if (!fixed[6 * n + off + a]) fl[m++] = a;
-            if (m < 1) continue;
-            double S[9] = {0}, vals[3] = {0}, vecs[9] = {0};
-            for (int ii = 0; ii < m; ++ii)
-                for (int jj = 0; jj < m; ++jj)
-                    S[ii + 3 * jj] = (kv(off + fl[ii], off + fl[jj]) + kv(off + fl[jj], off + fl[ii])) / 2;
-            if (m == 3) {
-                eig3(S, vals, vecs);
-            } else if (m == 2) {
-                double a = S[0], b = S[3], d = S[4];
-                double tr = 0.5 * (a + d), df = 0.5 * (a - d), r = std::sqrt(df * df + b * b);
-                vals[0] = tr - r; vals[1] = tr + r;
-                for (int k = 0; k < 2; ++k) {
-                    double vx, vy;
-                    if (b != 0.0) { vx = b; vy = vals[k] - a; }
-                    else { vx = (k == 0) == (a <= d) ? 1.0 : 0.0; vy = 1.0 - vx; }
-                    double nr = std::hypot(vx, vy);
-                    vecs[0 + 3 * k] = vx / nr; vecs[1 + 3 * k] = vy / nr;
-                }
-            } else {
-                vals[0] = S[0]; vecs[0] = 1.0;
-            }
-            double lmax = 0;
-            for (int k = 0; k < m; ++k) lmax = std::max(lmax, std::fabs(vals[k]));
-            if (!(lmax > 0.0)) continue;
-            int claimed[3] = {0, 0, 0};
-            for (int k = 0; k < m; ++k) {
-                double ratio = std::fabs(vals[k]) / lmax;
-                if (!(ratio < eps_zero)) continue;
-                int best_a = -1;
-                double best_w = -1.0;
-                for (int ii = 0; ii < m; ++ii) {
-                    double wv = std::fabs(vecs[ii + 3 * k]);
-                    int a = fl[ii];
-                    if (claimed[a]) continue;
-                    if (wv > best_w + 1e-12) { best_w = wv; best_a = a; }
-                }
-                if (best_a < 0) continue;
-                int64_t g = 6 * n + off + best_a;
-                if (fixed[g]) continue;
-                claimed[best_a] = 1;
-                fixed[g] = 1;
-            }
-        }
-    }
     int64_t cnt = 0;
     for (int64_t i = 0; i < ndof; ++i) cnt += fixed[i] ? 0 : 1;
     if (n_free) *n_free = cnt;

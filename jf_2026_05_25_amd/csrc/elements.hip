@@ -1344,13 +1344,98 @@ __global__ void bc_screen_kernel(int64_t nn, const int64_t *__restrict__ pos, co
     flag[n] = fl;
 }
 
-// out[36 * q + e] = 6 x 6 diagonal block entries of grid point nodes[q]
-__global__ void gather_nodes_kernel(int64_t count, const int64_t *__restrict__ pos, const double *__restrict__ val,
-                                    const int32_t *__restrict__ nodes, double *__restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count * 36) return;
-    const int64_t p = pos[36 * (int64_t)nodes[i / 36] + i % 36];
-    out[i] = p >= 0 ? val[p] : 0.0;
+// symmetric 3x3 eigen-decomposition by cyclic Jacobi, ascending eigenvalues,
+// eigenvectors in the columns of V
+__device__ void bc_eig3(const double A[9], double w[3], double V[9]) {
+#pragma clang fp contract(off)
+    double a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) a[i][j] = A[i + 3 * j];
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        const double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+        const double sc = fabs(a[0][0]) + fabs(a[1][1]) + fabs(a[2][2]);
+        if (off <= 1e-300 || off <= 1e-18 * sc) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (a[p][q] == 0.0) continue;
+                const double th = (a[q][q] - a[p][p]) / (2 * a[p][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                const double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 3; ++k) { const double x = a[k][p], y = a[k][q]; a[k][p] = c * x - s * y; a[k][q] = s * x + c * y; }
+                for (int k = 0; k < 3; ++k) { const double x = a[p][k], y = a[q][k]; a[p][k] = c * x - s * y; a[q][k] = s * x + c * y; }
+                for (int k = 0; k < 3; ++k) { const double x = v[k][p], y = v[k][q]; v[k][p] = c * x - s * y; v[k][q] = s * x + c * y; }
+            }
+    }
+    int idx[3] = {0, 1, 2};
+    for (int i = 0; i < 3; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (a[idx[j]][idx[j]] < a[idx[i]][idx[i]]) { const int t = idx[i]; idx[i] = idx[j]; idx[j] = t; }
+    for (int k = 0; k < 3; ++k) {
+        w[k] = a[idx[k]][idx[k]];
+        for (int r = 0; r < 3; ++r) V[r + 3 * k] = v[r][idx[k]];
+    }
+}
+
+// Grid-point singularity test of the flagged grid points (src/solver/
+// boundary_conditions.jl:1480-1607): per translational and rotational group of
+// still-free DOFs, the symmetric part of its block; each eigenvalue below
+// eps_zero * lambda_max claims the unclaimed DOF with the largest eigenvector
+// component (ties within 1e-12 to the first), which becomes fixed.
+__global__ void bc_decide_kernel(int64_t nn, const int64_t *__restrict__ pos, const double *__restrict__ val,
+                                 const uint8_t *__restrict__ flag, double eps_zero, uint8_t *__restrict__ fixed) {
+#pragma clang fp contract(off)
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn || !flag[n]) return;
+    auto kv = [&](int a, int b) {
+        const int64_t p = pos[36 * n + 6 * a + b];
+        return p >= 0 ? val[p] : 0.0;
+    };
+    for (int off = 0; off <= 3; off += 3) {
+        int fl[3], m = 0;
+        for (int a = 0; a < 3; ++a)
+            if (!fixed[6 * n + off + a]) fl[m++] = a;
+        if (m < 1) continue;
+        double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, vals[3] = {0, 0, 0}, vecs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int ii = 0; ii < m; ++ii)
+            for (int jj = 0; jj < m; ++jj)
+                S[ii + 3 * jj] = (kv(off + fl[ii], off + fl[jj]) + kv(off + fl[jj], off + fl[ii])) / 2;
+        if (m == 3) {
+            bc_eig3(S, vals, vecs);
+        } else if (m == 2) {
+            const double a = S[0], b = S[3], d = S[4];
+            const double tr = 0.5 * (a + d), df = 0.5 * (a - d), r = sqrt(df * df + b * b);
+            vals[0] = tr - r; vals[1] = tr + r;
+            for (int k = 0; k < 2; ++k) {
+                double vx, vy;
+                if (b != 0.0) { vx = b; vy = vals[k] - a; }
+                else { vx = (k == 0) == (a <= d) ? 1.0 : 0.0; vy = 1.0 - vx; }
+                const double nr = hypot(vx, vy);
+                vecs[0 + 3 * k] = vx / nr; vecs[1 + 3 * k] = vy / nr;
+            }
+        } else {
+            vals[0] = S[0]; vecs[0] = 1.0;
+        }
+        double lmax = 0;
+        for (int k = 0; k < m; ++k) lmax = fmax(lmax, fabs(vals[k]));
+        if (!(lmax > 0.0)) continue;
+        int claimed[3] = {0, 0, 0};
+        for (int k = 0; k < m; ++k) {
+            if (!(fabs(vals[k]) / lmax < eps_zero)) continue;
+            int best_a = -1;
+            double best_w = -1.0;
+            for (int ii = 0; ii < m; ++ii) {
+                const double wv = fabs(vecs[ii + 3 * k]);
+                const int a = fl[ii];
+                if (claimed[a]) continue;
+                if (wv > best_w + 1e-12) { best_w = wv; best_a = a; }
+            }
+            if (best_a < 0) continue;
+            const int64_t g = 6 * n + off + best_a;
+            if (fixed[g]) continue;
+            claimed[best_a] = 1;
+            fixed[g] = 1;
+        }
+    }
 }
 
 __global__ void reset_status_kernel(int64_t *status) {
@@ -1421,12 +1506,13 @@ hipError_t launch_bc_screen(hipStream_t s, const jfem_pattern *P, const double *
     return hipGetLastError();
 }
 
-hipError_t launch_gather_nodes(hipStream_t s, const jfem_pattern *P, const double *val, const int32_t *nodes,
-                               int64_t count, double *out) {
-    if (count == 0) return hipSuccess;
-    const int bs = 256;
-    hipLaunchKernelGGL(gather_nodes_kernel, dim3((unsigned)((count * 36 + bs - 1) / bs)), dim3(bs), 0, s, count,
-                       P->d_nodeblk, val, nodes, out);
+hipError_t launch_bc_decide(hipStream_t s, const jfem_pattern *P, const double *val, const uint8_t *flag,
+                            double eps_zero, uint8_t *fixed) {
+    const int64_t nn = P->n_nodes;
+    if (nn == 0) return hipSuccess;
+    const int bs = 128;
+    hipLaunchKernelGGL(bc_decide_kernel, dim3((unsigned)((nn + bs - 1) / bs)), dim3(bs), 0, s, nn, P->d_nodeblk, val,
+                       flag, eps_zero, fixed);
     return hipGetLastError();
 }
 

@@ -95,8 +95,8 @@ hipError_t launch_q4_local(hipStream_t s, int64_t n, const double *lc, const dou
 hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *Ke, double *val);
 hipError_t launch_bc_screen(hipStream_t s, const jfem_pattern *P, const double *val, const uint8_t *spc, double thresh,
                             uint8_t *fixed0, uint8_t *flag);
-hipError_t launch_gather_nodes(hipStream_t s, const jfem_pattern *P, const double *val, const int32_t *nodes,
-                               int64_t count, double *out);
+hipError_t launch_bc_decide(hipStream_t s, const jfem_pattern *P, const double *val, const uint8_t *flag,
+                            double eps_zero, uint8_t *fixed);
 hipError_t launch_gather_nodeblk(hipStream_t s, const jfem_pattern *P, const double *val, double *out);
 hipError_t launch_reset_status(hipStream_t s, int64_t *status);
 }  // namespace jfem
