@@ -36,7 +36,12 @@ import numpy as np
 
 from . import meshes
 
-SUPPORTED = {"GRID", "CQUAD4", "CTRIA3", "PSHELL", "MAT1", "SPC1", "FORCE", "MOMENT", "PLOAD4", "PARAM"}
+# cards that do not change a linear static solution of this path (eigenvalue
+# requests; coordinate systems, whose uses -- CP, CD, CID -- are checked where
+# they occur) and PARAMs without effect on it (output, mass, solver library,
+# the default AUTOSPC)
+NEUTRAL_CARDS = {"EIGRL", "EIGR", "CORD1R", "CORD2R", "CORD1C", "CORD2C", "CORD1S", "CORD2S"}
+NEUTRAL_PARAMS = {"POST", "GRDPNT", "COUPMASS", "WTMASS", "SOLLIB", "PRTMAXIM", "OGEOM"}
 LOAD_CARDS = {"FORCE", "MOMENT", "PLOAD4", "PLOAD", "PLOAD1", "PLOAD2", "GRAV", "ACCEL", "ACCEL1", "LOAD",
               "FORCE1", "FORCE2", "MOMENT1", "MOMENT2", "TEMP", "TEMPD", "SLOAD", "RFORCE"}
 _EXP = re.compile(r"([\d.])([+-])(\d)")
@@ -202,7 +207,11 @@ class Deck:
 
     def model(self, subcase: int | None = None, membrane_modes: int = 15) -> meshes.Model:
         """The grid, elements, sections and the SPC set of a subcase (loads zero:
-        the load vector is built on the device, see load_cards)."""
+        the load vector is built on the device, see load_cards).  Raises
+        NotImplementedError when the deck holds cards or PARAMs this path does
+        not model (they would change K or the constraints)."""
+        if self.unsupported:
+            raise NotImplementedError(f"cards not on this path: {sorted(self.unsupported)}")
         gi = self.grid_index()
         quads = np.array([[gi[g] for g in n] for (_, _, n) in self.quads], np.int32).reshape(-1, 4)
         trias = np.array([[gi[g] for g in n] for (_, _, n) in self.trias], np.int32).reshape(-1, 3)
@@ -342,8 +351,11 @@ def read_deck(source: str) -> Deck:
         g = lambda k: c[k] if k < len(c) else ""
         num = lambda k, d=None: parse_number(g(k), d)
         if name == "PARAM":
-            if g(1).strip().upper() == "K6ROT":
+            pname, pval = g(1).strip().upper(), str(g(2)).strip().upper()
+            if pname == "K6ROT":
                 k6rot = num(2, 100.0)
+            elif not (pname in NEUTRAL_PARAMS or (pname == "AUTOSPC" and pval in ("", "YES"))):
+                unsupported.add(f"PARAM {pname}")
         elif name == "GRID":
             gid = to_id(num(1))
             cp, cd = to_id(num(2, 0.0)), to_id(num(6, 0.0))
@@ -418,7 +430,7 @@ def read_deck(source: str) -> Deck:
                 pload4.setdefault(sid, []).append((e, p, d))
         elif name in LOAD_CARDS:
             other.setdefault(to_id(num(1)), set()).add(name)
-        else:
+        elif name not in NEUTRAL_CARDS:
             unsupported.add(name)
     ids = sorted(grids)                                    # GRID rank order (assembly.jl:2653-2657)
     xyz = np.array([grids[i] for i in ids], float).reshape(-1, 3)

@@ -100,6 +100,12 @@ def test_rejections():
     dk = D.read_deck("SUBCASE 1\nLOAD = 5\nBEGIN BULK\nGRID,1,,0.,0.,0.\nPLOAD2,5,1.,7\nENDDATA\n")
     with pytest.raises(NotImplementedError):
         dk.load_cards(1)                                                      # PLOAD2 is not on this path
+    for extra in ("CBAR,9,1,1,2,0.,0.,1.", "PARAM,AUTOSPC,NO", "PARAM,LGDISP,1", "RBE2,9,1,123456,2"):
+        dk = D.read_deck("BEGIN BULK\nGRID,1,,0.,0.,0.\nPARAM,POST,-1\nPARAM,AUTOSPC,YES\n" + extra + "\nENDDATA\n")
+        with pytest.raises(NotImplementedError):
+            dk.model()                                                        # would change K or the constraints
+    dk = D.read_deck("BEGIN BULK\nGRID,1,,0.,0.,0.\nPARAM,POST,-1\nEIGRL,1,,,5\nENDDATA\n")
+    assert dk.model().n_nodes == 1                                            # neutral cards
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_CASES), reason="reference tree not present")
