@@ -605,25 +605,20 @@ __global__ void __launch_bounds__(kBS) chain_product_kernel(const FlowTask *__re
 // ---------------------------------------------------------------- dataflow solves
 // One launch per direction.  The workgroup of index e takes task e of a list in
 // dependency order (children before parents, panels in solve order), and a task
-// waits only on tasks listed before it, through per-supernode counters in device
-// memory.  Each XCD dispatches its workgroups in index order, so the task a
-// workgroup waits on is already running or finished.
+// waits only on tasks listed before it.  Each XCD dispatches its workgroups in
+// index order, so the task a workgroup waits on is already running or finished.
 // Vectors exchanged between tasks of one launch (W, Z in the forward solve, X in
 // the backward solve) are stored and loaded with device-scope accesses (sc1),
-// coherent across the XCDs' L2 caches; a producer waits for its stores to
-// complete (s_waitcnt) before it bumps the counter, and a consumer polls the
-// counter with device-scope loads.  This is the agent-scope release/acquire
-// sequence without the L2 write-back and invalidate that plain stores and loads
-// would need (each of those walks the whole L2 of its XCD, and they serialise).
+// coherent across the XCDs' L2 caches, without the L2 write-back and invalidate
+// that plain stores and loads behind a fence would need (each of those walks the
+// whole L2 of its XCD, and they serialise).  No counters: W, Z and X are filled
+// with a signalling-NaN mark before the solve (arithmetic never produces one),
+// each entry is written once, and a consumer polls the entries it needs until
+// none holds the mark -- one memory round trip per dependency, and a producer
+// never waits for its stores.
 // Waits are bounded (kWaitTicks of the 100 MHz wall clock): on expiry the abort
 // flag is raised, every waiting workgroup leaves, and the host reports an error.
-// Solved values need no counter: Z and X are filled with a signalling-NaN mark
-// before the solve (arithmetic never produces one), each entry is written once,
-// and a consumer polls the entries it needs until none holds the mark -- one
-// memory round trip per dependency instead of a counter and then the data.
-// Update vectors (W) are gathered by the parent from many child tasks and are
-// counted instead: ready[s] = finished update blocks of s's children (int32,
-// nsn of them), followed by the abort flag.
+// cnt: nsn per-supernode words (unused by the polls), then the abort flag.
 // Every sum has a fixed order -- children in child order, panels in solve order,
 // the four waves' partial sums in wave order -- whatever the timing, so results
 // are bitwise reproducible.
@@ -851,10 +846,14 @@ __device__ __forceinline__ int g16_r0(int cg) { return cg * (NR / 4); }
 // beyond nrhs are skipped): the (index, value) loads of 8 entries per thread go
 // out together (a loop of single dependent loads would wait a memory latency per
 // entry); children in child order.
+// The update vectors hold the pending mark until their task publishes them, so
+// the loads poll (all pending values of a thread per round trip) instead of a
+// counter wait followed by the loads.  False on abort / time-out.
 template <class T, int NR>
-__device__ __forceinline__ void gather_children(T (*vs)[NR], const FlowChild *__restrict__ gch, int64_t gb, int nch,
+__device__ __forceinline__ bool gather_children(T (*vs)[NR], const FlowChild *__restrict__ gch, int64_t gb, int nch,
                                                 const int32_t *__restrict__ relmap, const T *W, int r0, int nrhs,
-                                                int tid, int nt) {
+                                                int tid, int nt, int32_t *abort_flag) {
+    bool ok = true;
     for (int j = 0; j < nch; ++j) {
         const FlowChild ch = gch[gb + j];
         const int nq = (ch.i1 - ch.i0) * NR;
@@ -863,12 +862,17 @@ __device__ __forceinline__ void gather_children(T (*vs)[NR], const FlowChild *__
         for (int q0 = 0; q0 < nq; q0 += 8 * nt) {
             int32_t row[8];
             T v[8];
+            const T *pp[8];
+            bool live[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int q = min(q0 + nt * u + tid, nq - 1), i = q / NR, r = min(q % NR, nrhs - 1);
                 row[u] = rel[i];
-                v[u] = ld_dev(wc + i * nrhs + r);
+                pp[u] = wc + i * nrhs + r;
+                live[u] = q0 + nt * u + tid < nq && (q0 + nt * u + tid) % NR < nrhs;
+                v[u] = ld_dev(pp[u]);
             }
+            ok = await_values<T, 8>(v, pp, live, abort_flag) && ok;
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int q = q0 + nt * u + tid, r = q % NR;
@@ -877,6 +881,7 @@ __device__ __forceinline__ void gather_children(T (*vs)[NR], const FlowChild *__
         }
         __syncthreads();
     }
+    return ok;
 }
 
 // forward: pivot task (s, p): z_p = L_pp^{-1} (y_p + children's contributions
@@ -892,7 +897,7 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
                                               const T *__restrict__ L, const T *__restrict__ Linv,
                                               const T *__restrict__ Y, T *W, T *Z, int nrhs, int32_t *cnt,
                                               int64_t nsn) {
-    int32_t *ready = cnt, *abort_flag = cnt + nsn;
+    int32_t *abort_flag = cnt + nsn;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, j = lane & 15, cg = lane >> 4;
     const int i = 16 * w + j;                          // row of the 64-row block
     const int rq = g16_r0<NR>(cg);
@@ -912,10 +917,12 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
         const T y = Y[yi * nrhs + min(r, nrhs - 1)];
         vs[row][r] = (pivot && row < t.nr && r < nrhs) ? y : T(0);
     }
-    if (tid == 0) failed = t.need > 0 && !wait_at_least(&ready[t.s], t.need, abort_flag);
+    if (tid == 0) failed = 0;
+    __syncthreads();
+    if (t.need > 0 && !gather_children<T, NR>(vs, gch, t.gb, t.nch, relmap, W, t.r0, nrhs, tid, kBS, abort_flag))
+        failed = 1;
     __syncthreads();
     if (failed) return;
-    if (t.need > 0) gather_children<T, NR>(vs, gch, t.gb, t.nch, relmap, W, t.r0, nrhs, tid, kBS);
     T acc[NR], xr[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
@@ -951,9 +958,6 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
                 }
             }
         }
-        complete_stores();
-        __syncthreads();
-        if (tid == 0 && t.parent >= 0) __hip_atomic_fetch_add(&ready[t.parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
     bool ok = true;
@@ -976,12 +980,9 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
 #pragma unroll
             for (int rr = 0; rr < NR / 4; ++rr) {
                 const int r = rq + rr;
-                if (r < nrhs) st_dev(W + (t.relofs + t.r0 - t.ns + i) * nrhs + r, vs[i][r] - acc[r]);
+                if (r < nrhs && !failed) st_dev(W + (t.relofs + t.r0 - t.ns + i) * nrhs + r, vs[i][r] - acc[r]);
             }
         }
-        complete_stores();
-        __syncthreads();
-        if (tid == 0 && !failed) __hip_atomic_fetch_add(&ready[t.parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
 #pragma unroll
@@ -1187,9 +1188,10 @@ __device__ __forceinline__ void wave_sync() {
 // first addition; children in child order, so each row receives y and then its
 // children's values in child order, as in gather_children.
 template <class T, int NR, int U>
-__device__ __forceinline__ void gather_children_wave(T (*vs)[NR], const FlowChild *__restrict__ gch, int64_t gb, int nch,
+__device__ __forceinline__ bool gather_children_wave(T (*vs)[NR], const FlowChild *__restrict__ gch, int64_t gb, int nch,
                                                      const int32_t *__restrict__ relmap, const T *W, int nrhs,
-                                                     int lane) {
+                                                     int lane, int32_t *abort_flag) {
+    bool ok = true;
     for (int jc = 0; jc < nch; ++jc) {
         const FlowChild ch = gch[gb + jc];
         const int nq = (ch.i1 - ch.i0) * NR;
@@ -1198,12 +1200,17 @@ __device__ __forceinline__ void gather_children_wave(T (*vs)[NR], const FlowChil
         for (int q0 = 0; q0 < nq; q0 += 64 * U) {
             int32_t row[U];
             T v[U];
+            const T *pp[U];
+            bool live[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int q = min(q0 + 64 * u + lane, nq - 1), i = q / NR, r = min(q % NR, nrhs - 1);
                 row[u] = rel[i];
-                v[u] = ld_dev(wc + i * nrhs + r);
+                pp[u] = wc + i * nrhs + r;
+                live[u] = q0 + 64 * u + lane < nq && (q0 + 64 * u + lane) % NR < nrhs;
+                v[u] = ld_dev(pp[u]);
             }
+            ok = await_values<T, U>(v, pp, live, abort_flag) && ok;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int q = q0 + 64 * u + lane, r = q % NR;
@@ -1212,6 +1219,7 @@ __device__ __forceinline__ void gather_children_wave(T (*vs)[NR], const FlowChil
             wave_sync();
         }
     }
+    return ok;
 }
 
 // forward whole-front task (s one panel, front rows <= kWholeRows): pass q < npz
@@ -1223,7 +1231,7 @@ __device__ __forceinline__ void fwd_leaf_task(const FlowTask &t, T (*vs)[NR], T 
                                               const T *__restrict__ L, const T *__restrict__ Linv,
                                               const T *__restrict__ Y, T *W, T *Z, int nrhs, int32_t *cnt,
                                               int64_t nsn) {
-    int32_t *ready = cnt, *abort_flag = cnt + nsn;
+    int32_t *abort_flag = cnt + nsn;
     const int j = lane & 15, cg = lane >> 4;
     const int rq = g16_r0<NR>(cg);
     const int nr = t.nr, nu = (int)(t.ms - t.ns);
@@ -1256,9 +1264,9 @@ __device__ __forceinline__ void fwd_leaf_task(const FlowTask &t, T (*vs)[NR], T 
         vs[row][r] = (row < nr && r < nrhs) ? yv[q2] : T(0);
     }
     for (int q = kTile * NR + lane; q < t.ms * NR; q += 64) vs[q / NR][q % NR] = T(0);
-    if (t.need > 0 && !wait_at_least(&ready[t.s], t.need, abort_flag)) return;
     wave_sync();
-    if (t.need > 0) gather_children_wave<T, NR, 16>(vs, gch, t.gb, t.nch, relmap, W, nrhs, lane);
+    if (t.need > 0 && !gather_children_wave<T, NR, 16>(vs, gch, t.gb, t.nch, relmap, W, nrhs, lane, abort_flag))
+        return;
     T acc[NR], xr[NR];
     g16_from_lds<T, NR>(xr, vs, nr, cg, j);            // update rows of the front are not part of v
     for (int q0 = 0; q0 < npass; q0 += kLeafRing) {
@@ -1298,8 +1306,6 @@ __device__ __forceinline__ void fwd_leaf_task(const FlowTask &t, T (*vs)[NR], T 
         const int row = q / NR, r = q % NR;
         if (r < nrhs) st_dev(W + (t.relofs + row) * nrhs + r, vs[t.ns + row][r]);
     }
-    complete_stores();
-    if (lane == 0 && t.parent >= 0) __hip_atomic_fetch_add(&ready[t.parent], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The forward launch: workgroups [0, nq) take the bottom subtrees' tasks four at
@@ -1920,6 +1926,8 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
     hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(nsn)), dim3(kBS), 0, st, cnt, nsn);
     hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nv)), dim3(kBS), 0, st, Z, nv, pending_mark<T>());
     hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nv)), dim3(kBS), 0, st, X, nv, pending_mark<T>());
+    const int64_t nwv = H.upd_rows * nrhs;     // update vectors: polled by the parents
+    if (nwv > 0) hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nwv)), dim3(kBS), 0, st, W, nwv, pending_mark<T>());
     // timing level 2: each kernel between its own pair of events on this stream
     const bool detail = ctx->timing_level >= 2;
     auto lap = [&](int fam) {
