@@ -836,6 +836,33 @@ __device__ __forceinline__ bool g16_load_wait(T (&xr)[NR], Addr src_c, int cnt, 
     for (int g = 0; g < NR; ++g) xr[g] = live[g] ? xr[g] : T(0);
     return ok;
 }
+// the front row indices behind this lane's xr entries for a tile of cnt update
+// rows starting at rows[0] (indices clamped into the tile); loaded ahead so that
+// the awaited X loads do not wait for them
+template <int NR>
+__device__ __forceinline__ void g16_rows(int32_t (&ridx)[NR], const int32_t *rows, int cnt, int cg, int j) {
+#pragma unroll
+    for (int g = 0; g < NR; ++g) ridx[g] = rows[min(g16_col<NR>(g, cg, j), cnt - 1)];
+}
+// g16_load_wait for values X[ridx[g]] (stride-1 right-hand sides)
+template <class T, int NR>
+__device__ __forceinline__ bool g16_load_wait_rows(T (&xr)[NR], const T *X, const int32_t (&ridx)[NR], int cnt, int nrhs,
+                                                   int cg, int j, int32_t *abort_flag) {
+    const T *p[NR];
+    bool live[NR];
+#pragma unroll
+    for (int g = 0; g < NR; ++g) {
+        const int c = g16_col<NR>(g, cg, j), r = g16_rhs<NR>(g, j);
+        p[g] = X + (int64_t)ridx[g] * nrhs + min(r, nrhs - 1);
+        live[g] = c < cnt && r < nrhs;
+        xr[g] = ld_dev(p[g]);
+    }
+    const bool ok = await_values<T, NR>(xr, p, live, abort_flag);
+#pragma unroll
+    for (int g = 0; g < NR; ++g) xr[g] = live[g] ? xr[g] : T(0);
+    return ok;
+}
+
 // this lane's share of a row's NR sums when storing: right-hand sides
 // [cg NR/4, (cg+1) NR/4)
 template <int NR>
@@ -1062,6 +1089,8 @@ __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][
     }
     if (tid == 0) failed = 0;
     if (nloop > 0) load_cols16(lv, Lc + tile_row0(0), t.ms, tile_rows(0), nc, lane, w);
+    int32_t ridx[NR];                                  // row indices of the next update-row tile
+    if (nchunk > 0) g16_rows<NR>(ridx, rw + tile_row0(0), tile_rows(0), cg, j);
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
     bool ok = true;
@@ -1077,9 +1106,9 @@ __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][
         for (int k = 0; k < 16; ++k) m[k] = S[16 * cg + k][c];
         if (NB == 1) __syncthreads();                  // the next tile overwrites the only buffer
         if (e < nchunk) {
-            const int64_t r0 = tile_row0(e);
-            ok = g16_load_wait<T, NR>(xr, [&](int k) { return X + (int64_t)rw[r0 + k] * nrhs; }, nre, nrhs, cg, j,
-                                      abort_flag) && ok;
+            ok = g16_load_wait_rows<T, NR>(xr, X, ridx, nre, nrhs, cg, j, abort_flag) && ok;
+            const int en = min(e + 1, nchunk - 1);
+            g16_rows<NR>(ridx, rw + tile_row0(en), tile_rows(en), cg, j);
         } else {
             const T *xq = X + (t.c0 + tile_row0(e)) * nrhs;
             ok = g16_load_wait<T, NR>(xr, [&](int k) { return xq + (int64_t)k * nrhs; }, nre, nrhs, cg, j,
@@ -1400,11 +1429,14 @@ __device__ __forceinline__ void bwd_leaf_task(const FlowTask &t, T (*Sb)[kTile][
     for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int r = 0; r < NR; ++r) acc[h][r] = T(0);
+    int32_t ridx[NR];                                  // row indices of the next tile
+    if (nchunk > 0) g16_rows<NR>(ridx, rw + tile_row0(0), tile_rows(0), cg, j);
     bool ok = true;
     for (int e = 0; e < nchunk; ++e) {
-        const int r0 = tile_row0(e), nre = tile_rows(e);
-        ok = g16_load_wait<T, NR>(xr, [&](int k) { return X + (int64_t)rw[r0 + k] * nrhs; }, nre, nrhs, cg, j,
-                                  abort_flag) && ok;
+        const int nre = tile_rows(e);
+        ok = g16_load_wait_rows<T, NR>(xr, X, ridx, nre, nrhs, cg, j, abort_flag) && ok;
+        const int en = min(e + 1, nchunk - 1);
+        g16_rows<NR>(ridx, rw + tile_row0(en), tile_rows(en), cg, j);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int q = w + 2 * h;
