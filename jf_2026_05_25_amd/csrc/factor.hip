@@ -876,13 +876,16 @@ __device__ __forceinline__ int g16_r0(int cg) { return cg * (NR / 4); }
 // The update vectors hold the pending mark until their task publishes them, so
 // the loads poll (all pending values of a thread per round trip) instead of a
 // counter wait followed by the loads.  False on abort / time-out.
+// first: the task's first child record, loaded by the caller at the start of the
+// task; each next record is loaded while the current child is gathered.
 template <class T, int NR>
 __device__ __forceinline__ bool gather_children(T (*vs)[NR], const FlowChild *__restrict__ gch, int64_t gb, int nch,
-                                                const int32_t *__restrict__ relmap, const T *W, int r0, int nrhs,
-                                                int tid, int nt, int32_t *abort_flag) {
+                                                FlowChild first, const int32_t *__restrict__ relmap, const T *W,
+                                                int r0, int nrhs, int tid, int nt, int32_t *abort_flag) {
     bool ok = true;
+    FlowChild ch = first;
     for (int j = 0; j < nch; ++j) {
-        const FlowChild ch = gch[gb + j];
+        const FlowChild nxt = gch[gb + min(j + 1, nch - 1)];
         const int nq = (ch.i1 - ch.i0) * NR;
         const int32_t *rel = relmap + ch.relofs + ch.i0;
         const T *wc = W + (ch.relofs + ch.i0) * nrhs;
@@ -907,6 +910,7 @@ __device__ __forceinline__ bool gather_children(T (*vs)[NR], const FlowChild *__
             }
         }
         __syncthreads();
+        ch = nxt;
     }
     return ok;
 }
@@ -925,6 +929,7 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
                                               const T *__restrict__ Y, T *W, T *Z, int nrhs, int32_t *cnt,
                                               int64_t nsn) {
     int32_t *abort_flag = cnt + nsn;
+    const FlowChild first = t.nch > 0 ? gch[t.gb] : FlowChild{};   // (in flight with the task's first loads)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, j = lane & 15, cg = lane >> 4;
     const int i = 16 * w + j;                          // row of the 64-row block
     const int rq = g16_r0<NR>(cg);
@@ -946,7 +951,7 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
     }
     if (tid == 0) failed = 0;
     __syncthreads();
-    if (t.need > 0 && !gather_children<T, NR>(vs, gch, t.gb, t.nch, relmap, W, t.r0, nrhs, tid, kBS, abort_flag))
+    if (t.need > 0 && !gather_children<T, NR>(vs, gch, t.gb, t.nch, first, relmap, W, t.r0, nrhs, tid, kBS, abort_flag))
         failed = 1;
     __syncthreads();
     if (failed) return;
@@ -1218,11 +1223,12 @@ __device__ __forceinline__ void wave_sync() {
 // children's values in child order, as in gather_children.
 template <class T, int NR, int U>
 __device__ __forceinline__ bool gather_children_wave(T (*vs)[NR], const FlowChild *__restrict__ gch, int64_t gb, int nch,
-                                                     const int32_t *__restrict__ relmap, const T *W, int nrhs,
-                                                     int lane, int32_t *abort_flag) {
+                                                     FlowChild first, const int32_t *__restrict__ relmap, const T *W,
+                                                     int nrhs, int lane, int32_t *abort_flag) {
     bool ok = true;
+    FlowChild ch = first;
     for (int jc = 0; jc < nch; ++jc) {
-        const FlowChild ch = gch[gb + jc];
+        const FlowChild nxt = gch[gb + min(jc + 1, nch - 1)];
         const int nq = (ch.i1 - ch.i0) * NR;
         const int32_t *rel = relmap + ch.relofs + ch.i0;
         const T *wc = W + (ch.relofs + ch.i0) * nrhs;
@@ -1247,6 +1253,7 @@ __device__ __forceinline__ bool gather_children_wave(T (*vs)[NR], const FlowChil
             }
             wave_sync();
         }
+        ch = nxt;
     }
     return ok;
 }
@@ -1261,6 +1268,7 @@ __device__ __forceinline__ void fwd_leaf_task(const FlowTask &t, T (*vs)[NR], T 
                                               const T *__restrict__ Y, T *W, T *Z, int nrhs, int32_t *cnt,
                                               int64_t nsn) {
     int32_t *abort_flag = cnt + nsn;
+    const FlowChild first = t.nch > 0 ? gch[t.gb] : FlowChild{};   // (in flight with the task's first loads)
     const int j = lane & 15, cg = lane >> 4;
     const int rq = g16_r0<NR>(cg);
     const int nr = t.nr, nu = (int)(t.ms - t.ns);
@@ -1294,7 +1302,7 @@ __device__ __forceinline__ void fwd_leaf_task(const FlowTask &t, T (*vs)[NR], T 
     }
     for (int q = kTile * NR + lane; q < t.ms * NR; q += 64) vs[q / NR][q % NR] = T(0);
     wave_sync();
-    if (t.need > 0 && !gather_children_wave<T, NR, 16>(vs, gch, t.gb, t.nch, relmap, W, nrhs, lane, abort_flag))
+    if (t.need > 0 && !gather_children_wave<T, NR, 16>(vs, gch, t.gb, t.nch, first, relmap, W, nrhs, lane, abort_flag))
         return;
     T acc[NR], xr[NR];
     g16_from_lds<T, NR>(xr, vs, nr, cg, j);            // update rows of the front are not part of v
