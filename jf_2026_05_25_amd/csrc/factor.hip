@@ -1953,8 +1953,8 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
     return hipGetLastError();
 }
 
-// y = (L L')^{-1} y by the dataflow kernels; cnt: ns + 1 counters (the abort flag,
-// last, is left alone: it is cleared once per refinement call)
+// y = (L L')^{-1} y by the dataflow kernels; cnt: ns + 1 words, the last one the
+// abort flag (cleared once per refinement call)
 template <class T, int NR>
 hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_factor *F, const T *Y, T *W, T *Z, T *X,
                          int nrhs, int32_t *cnt) {
@@ -1963,7 +1963,6 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
     const int64_t nsn = H.ns;
     const T *L = (const T *)F->d_L, *Linv = (const T *)F->d_Linv;
     const int64_t nv = H.n * nrhs;
-    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(nsn)), dim3(kBS), 0, st, cnt, nsn);
     hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nv)), dim3(kBS), 0, st, Z, nv, pending_mark<T>());
     hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nv)), dim3(kBS), 0, st, X, nv, pending_mark<T>());
     const int64_t nwv = H.upd_rows * nrhs;     // update vectors: polled by the parents
