@@ -647,48 +647,8 @@ __device__ __forceinline__ void complete_stores() {
     asm volatile("" ::: "memory");
 }
 
-__device__ bool wait_at_least(const int32_t *ctr, int32_t target, int32_t *abort_flag) {
-    if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        const long long t0 = wall_clock64();
-        int spins = 0;
-        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(2);
-            if ((++spins & 15) == 0) {
-                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-                if (wall_clock64() - t0 > kWaitTicks) {
-                    __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    return false;
-                }
-            }
-        }
-    }
-    asm volatile("" ::: "memory");
-    return true;
-}
-
-// re-load *p until it holds a published value; false on abort or time-out
-template <class T>
-__device__ bool await_value(T &v, const T *p, int32_t *abort_flag) {
-    if (!is_pending(v)) return true;
-    const long long t0 = wall_clock64();
-    int spins = 0;
-    do {
-        __builtin_amdgcn_s_sleep(2);
-        v = ld_dev(p);
-        if ((++spins & 15) == 0) {
-            if (ld_dev(abort_flag)) return false;
-            if (wall_clock64() - t0 > kWaitTicks) {
-                st_dev(abort_flag, 1);
-                return false;
-            }
-        }
-    } while (is_pending(v));
-    return true;
-}
-
-// the same for N values of a lane at once: every poll re-loads all entries
-// still pending together, so the wait costs one memory round trip per poll and
-// not one per entry
+// re-load the entries of a lane until each holds a published value, all
+// pending entries per poll: one memory round trip per poll, not one per entry
 template <class T, int N>
 __device__ bool await_values(T (&v)[N], const T *const (&p)[N], const bool (&live)[N], int32_t *abort_flag) {
     bool pend = false;
