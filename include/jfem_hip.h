@@ -30,7 +30,7 @@ typedef enum {
     JFEM_ERR_HIP = 1,             /* HIP runtime error */
     JFEM_ERR_DEGENERATE = 2,      /* degenerate/folded shell: ArgumentError, src/FEMKernels.jl:2862-2867, 4244-4251 */
     JFEM_ERR_INVERTED = 3,        /* non-positive solid Jacobian: DomainError, src/tetra10.jl:31-35 */
-    JFEM_ERR_WARPED = 4,          /* warped CQUAD4: finite-warp congruence not built here */
+    JFEM_ERR_WARPED = 4,          /* reserved (warped CQUAD4s are handled by the finite-warp congruence) */
     JFEM_ERR_BAD_ARG = 5,         /* bad sizes / material: ArgumentError, src/tetra10.jl:55-58 */
     JFEM_ERR_NOT_SPD = 6,         /* factorization hit a non-positive pivot: PosDefException */
     JFEM_ERR_OOM = 7,             /* device allocation failed */
@@ -244,9 +244,18 @@ int jfem_memcpy_d2h(jfem_ctx *ctx, void *host_dst, const void *dev_src, int64_t 
  * difference relative to max |entry|.  (No reference counterpart.) */
 int jfem_selftest_update(jfem_ctx *ctx, int precision, double *max_rel_diff);
 
-/* memory held by a handle (device bytes) */
+/* memory held by a handle (device bytes); admission control of the serving loop
+ * (no reference counterpart) */
 int64_t jfem_factor_device_bytes(const jfem_factor *F);
 int64_t jfem_matrix_device_bytes(const jfem_matrix *K);
+int64_t jfem_mesh_device_bytes(const jfem_mesh *mesh);
+int64_t jfem_pattern_device_bytes(const jfem_pattern *pattern);
+int64_t jfem_symbolic_device_bytes(const jfem_symbolic *S);
+/* solve scratch and BC buffers a context currently holds */
+int64_t jfem_ctx_device_bytes(const jfem_ctx *ctx);
+/* solve scratch a jfem_solve call with nrhs right-hand sides on a factor of the
+ * given precision over S needs in its context (ndof = all DOFs of the pattern) */
+int64_t jfem_solve_scratch_bytes(const jfem_symbolic *S, int64_t ndof, int nrhs, int precision);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,8 @@ EXPORTS = [
     "jfem_symbolic_create", "jfem_symbolic_free", "jfem_factor_create", "jfem_refactor",
     "jfem_factor_small_pivots", "jfem_factor_free", "jfem_solve", "jfem_solve_dev",
     "jfem_dev_alloc", "jfem_dev_free", "jfem_memcpy_h2d", "jfem_memcpy_d2h",
-    "jfem_factor_device_bytes", "jfem_matrix_device_bytes", "jfem_selftest_update",
+    "jfem_factor_device_bytes", "jfem_matrix_device_bytes", "jfem_mesh_device_bytes", "jfem_pattern_device_bytes",
+    "jfem_symbolic_device_bytes", "jfem_ctx_device_bytes", "jfem_solve_scratch_bytes", "jfem_selftest_update",
     "jfem_loadset_create", "jfem_load_vector", "jfem_load_vector_dev", "jfem_loadset_free",
 ]
 
@@ -101,6 +102,11 @@ def load():
         "jfem_memcpy_d2h": ([vp, vp, vp, i64], C.c_int),
         "jfem_factor_device_bytes": ([vp], i64),
         "jfem_matrix_device_bytes": ([vp], i64),
+        "jfem_mesh_device_bytes": ([vp], i64),
+        "jfem_pattern_device_bytes": ([vp], i64),
+        "jfem_symbolic_device_bytes": ([vp], i64),
+        "jfem_ctx_device_bytes": ([vp], i64),
+        "jfem_solve_scratch_bytes": ([vp, i64, C.c_int, C.c_int], i64),
         "jfem_selftest_update": ([vp, C.c_int, P(dbl)], C.c_int),
         "jfem_loadset_create": ([vp, vp, i64, vp, vp, vp, i64, vp, vp, vp, i64, vp, vp, vp, P(vp)], C.c_int),
         "jfem_load_vector": ([vp, vp, dbl, vp], C.c_int),

@@ -443,6 +443,24 @@ int64_t jfem_matrix_device_bytes(const jfem_matrix *K) {
     return K ? (K->P->nnz + K->P->ntrip) * (int64_t)sizeof(double) : 0;
 }
 
+int64_t jfem_mesh_device_bytes(const jfem_mesh *m) {
+    if (!m) return 0;
+    return 3 * m->n_nodes * (int64_t)sizeof(double) + (4 * m->nq + 3 * m->nt + 10 * m->ntet) * (int64_t)sizeof(int32_t) +
+           (8 * m->nq + 8 * m->nt + 2 * m->ntet) * (int64_t)sizeof(double);
+}
+
+int64_t jfem_pattern_device_bytes(const jfem_pattern *P) {
+    if (!P) return 0;
+    return (P->ndof + 1) * (int64_t)sizeof(int64_t) + P->nnz * (int64_t)sizeof(int32_t) +
+           (P->nnz + 1) * (int64_t)sizeof(int64_t) + P->ntrip * (int64_t)sizeof(int32_t) +
+           36 * P->n_nodes * (int64_t)sizeof(int64_t);
+}
+
+int64_t jfem_ctx_device_bytes(const jfem_ctx *ctx) {
+    if (!ctx) return 0;
+    return (int64_t)(ctx->scratch_bytes + ctx->bc_dev_bytes) + 2 * (int64_t)sizeof(int64_t);
+}
+
 // ------------------------------------------------------------------ BC partition
 // src/solver/boundary_conditions.jl:1480-1607 (SPC), 1028-1138 (AUTOSPC with grid
 // point processing on: only exact-zero / negative diagonals), 745-845 (GPST).

@@ -38,6 +38,7 @@ struct jfem_symbolic {
     int32_t *d_ea_bounds = nullptr;
     int64_t *d_ea_boff = nullptr;
     std::vector<void *> allocs;
+    int64_t bytes = 0;              // device bytes behind allocs
 };
 
 struct jfem_factor {
@@ -67,10 +68,19 @@ __device__ inline void report_min(int64_t *status, int64_t idx, int code) {
 }
 
 // ----------------------------------------------------------------- fill / scatter
+// before a dataflow solve: Z and X (nv entries each) and W (nw) to the pending
+// mark, the two launch tickets to zero
 template <class T>
-__global__ void fill_value_kernel(T *__restrict__ p, int64_t n, T v) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
+__global__ void solve_marks_kernel(T *__restrict__ Z, T *__restrict__ X, int64_t nv, T *__restrict__ W, int64_t nw,
+                                   T mark, int32_t *__restrict__ tickets) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 < 2) tickets[i0] = 0;
+    for (int64_t i = i0; i < nv; i += stride) {
+        Z[i] = mark;
+        X[i] = mark;
+    }
+    for (int64_t i = i0; i < nw; i += stride) W[i] = mark;
 }
 
 template <class T>
@@ -603,10 +613,15 @@ __global__ void __launch_bounds__(kBS) chain_product_kernel(const FlowTask *__re
 }
 
 // ---------------------------------------------------------------- dataflow solves
-// One launch per direction.  The workgroup of index e takes task e of a list in
-// dependency order (children before parents, panels in solve order), and a task
-// waits only on tasks listed before it.  Each XCD dispatches its workgroups in
-// index order, so the task a workgroup waits on is already running or finished.
+// One launch per direction over a task list in dependency order (children before
+// parents, panels in solve order); a task waits only on tasks listed before it.
+// A workgroup does not take the task of its own index: its first lane draws a
+// ticket from a per-launch counter (zeroed before the launch) and the workgroup
+// takes the task of that ticket.  Tickets are drawn by running workgroups only,
+// so every task listed before a running one is running or finished, whatever
+// order the hardware dispatches in and whatever else is resident on the device
+// (other contexts' solve launches included): the oldest unfinished task's
+// predecessors are all finished, so it always progresses.
 // Vectors exchanged between tasks of one launch (W, Z in the forward solve, X in
 // the backward solve) are stored and loaded with device-scope accesses (sc1),
 // coherent across the XCDs' L2 caches, without the L2 write-back and invalidate
@@ -618,7 +633,8 @@ __global__ void __launch_bounds__(kBS) chain_product_kernel(const FlowTask *__re
 // never waits for its stores.
 // Waits are bounded (kWaitTicks of the 100 MHz wall clock): on expiry the abort
 // flag is raised, every waiting workgroup leaves, and the host reports an error.
-// cnt: nsn per-supernode words (unused by the polls), then the abort flag.
+// cnt: nsn per-supernode words (unused by the polls), then the abort flag, then
+// the forward and backward launch tickets.
 // Every sum has a fixed order -- children in child order, panels in solve order,
 // the four waves' partial sums in wave order -- whatever the timing, so results
 // are bitwise reproducible.
@@ -645,6 +661,14 @@ __device__ __forceinline__ void complete_stores() {
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_waitcnt(0);
     asm volatile("" ::: "memory");
+}
+
+// the workgroup's task slot: one ticket from *ticket per workgroup, broadcast
+// through LDS (the caller's first barrier)
+__device__ __forceinline__ int64_t draw_ticket(int32_t *ticket, int32_t *slot) {
+    if (threadIdx.x == 0) *slot = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return (int64_t)__builtin_amdgcn_readfirstlane(*slot);
 }
 
 // re-load the entries of a lane until each holds a published value, all
@@ -1322,18 +1346,20 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
     constexpr int NQ = kLeafQuads<T, NR> ? 4 : 1;
     __shared__ __attribute__((aligned(16))) T lds[NQ][kWholeRows + kTile][NR];
     __shared__ int failed_one;                         // (with quads: in the main path's unused quarter)
+    __shared__ int32_t ticket_slot;
+    const int64_t b = draw_ticket(cnt + nsn + 1, &ticket_slot);
     const int64_t nq = (nleaf + 3) / 4;
     if constexpr (kLeafQuads<T, NR>) {
-        if (blockIdx.x < nq) {
+        if (b < nq) {
             const int w = uniform_int((int)(threadIdx.x >> 6));
-            const int64_t k = 4 * (int64_t)blockIdx.x + w;
+            const int64_t k = 4 * b + w;
             if (k < nleaf)
                 fwd_leaf_task<T, NR>(tasks[k], lds[w], lds[w] + kWholeRows, (int)(threadIdx.x & 63), gch, relmap, L,
                                      Linv, Y, W, Z, nrhs, cnt, nsn);
             return;
         }
     }
-    const FlowTask t = tasks[nleaf + (int64_t)blockIdx.x - nq];
+    const FlowTask t = tasks[nleaf + b - nq];
     int *failed;
     if constexpr (NQ > 1) failed = (int *)&lds[1][0][0];
     else failed = &failed_one;
@@ -1477,18 +1503,20 @@ __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restric
     constexpr int kPair = 2 * kTile * 17 + kTile * NR;
     __shared__ __attribute__((aligned(16))) T lds[kLeafPairs<T, NR> && 2 * kPair > kMain ? 2 * kPair : kMain];
     __shared__ int flags[5];                           // main: failed; pairs: arrivals and failure, per pair
-    if (kLeafPairs<T, NR> && (int64_t)blockIdx.x >= nmain) {
+    __shared__ int32_t ticket_slot;
+    const int64_t b = draw_ticket(cnt + nsn + 2, &ticket_slot);
+    if (kLeafPairs<T, NR> && b >= nmain) {
         const int tid = threadIdx.x, pw = uniform_int(tid >> 6), p = pw >> 1;
         if (tid < 4) flags[1 + tid] = 0;
         __syncthreads();                               // before any wait
-        const int64_t k = nmain + 2 * ((int64_t)blockIdx.x - nmain) + p;
+        const int64_t k = nmain + 2 * (b - nmain) + p;
         if (k >= nmain + nleaf) return;
         T *base = lds + p * kPair;
         bwd_leaf_task<T, NR>(tasks[k], (T (*)[kTile][17])base, (T (*)[NR])(base + 2 * kTile * 17), &flags[1 + 2 * p],
                              pw & 1, tid & 63, rows, L, Linv, Z, X, nrhs, cnt, nsn);
         return;
     }
-    bwd_main_task<T, NR>(tasks[blockIdx.x], (T (*)[kTile][kTile + 1])lds, (T (*)[NR])(lds + NB * kTile * (kTile + 1)),
+    bwd_main_task<T, NR>(tasks[b], (T (*)[kTile][kTile + 1])lds, (T (*)[NR])(lds + NB * kTile * (kTile + 1)),
                          flags[0], rows, L, Linv, Z, X, nrhs, cnt, nsn);
 }
 
@@ -1913,8 +1941,9 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
     return hipGetLastError();
 }
 
-// y = (L L')^{-1} y by the dataflow kernels; cnt: ns + 1 words, the last one the
-// abort flag (cleared once per refinement call)
+// y = (L L')^{-1} y by the dataflow kernels; cnt: ns + 3 words: ns unused, the
+// abort flag (cleared once per refinement call), the two launch tickets (cleared
+// before every solve)
 template <class T, int NR>
 hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_factor *F, const T *Y, T *W, T *Z, T *X,
                          int nrhs, int32_t *cnt) {
@@ -1923,10 +1952,9 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
     const int64_t nsn = H.ns;
     const T *L = (const T *)F->d_L, *Linv = (const T *)F->d_Linv;
     const int64_t nv = H.n * nrhs;
-    hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nv)), dim3(kBS), 0, st, Z, nv, pending_mark<T>());
-    hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nv)), dim3(kBS), 0, st, X, nv, pending_mark<T>());
     const int64_t nwv = H.upd_rows * nrhs;     // update vectors: polled by the parents
-    if (nwv > 0) hipLaunchKernelGGL(fill_value_kernel<T>, dim3(grid_for(nwv)), dim3(kBS), 0, st, W, nwv, pending_mark<T>());
+    hipLaunchKernelGGL(solve_marks_kernel<T>, dim3(grid_for(std::max(nv, nwv))), dim3(kBS), 0, st, Z, X, nv, W, nwv,
+                       pending_mark<T>(), cnt + nsn + 1);
     // timing level 2: each kernel between its own pair of events on this stream
     const bool detail = ctx->timing_level >= 2;
     auto lap = [&](int fam) {
@@ -1975,6 +2003,7 @@ hipError_t up(jfem_symbolic *S, V **dst, const std::vector<V> &src, hipStream_t 
     hipError_t e = hipMalloc((void **)dst, src.size() * sizeof(V));
     if (e != hipSuccess) return e;
     S->allocs.push_back(*dst);
+    S->bytes += (int64_t)(src.size() * sizeof(V));
     return hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice, st);
 }
 
@@ -2171,6 +2200,28 @@ int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_mat
 
 namespace {
 
+// Solve scratch of one refinement call (in the context's scratch buffer): the
+// factor-precision vectors of the dataflow solves (Y, W, Z, X), their wait words,
+// the FP64 iterate, residual and CG vectors (interleaved, n x NR), the global
+// iterate for the SpMV (ndof x NR) and the reduction partials.
+struct SolveLayout {
+    size_t oY, oW, oZs, oXs, oCnt, oXh, oXl, oR, oB, oD, oSv, oPv, oQv, oZv, oXgh, oXgl, oPg;
+    size_t oPart, oNrm, oRho, oNum, oAl, oBe, oActO, oActI, oTicket, bytes = 0;
+    SolveLayout(const SymbolicHost &H, int64_t ndof, int NR, size_t es) {
+        const size_t nv = (size_t)H.n * NR, ng = (size_t)ndof * NR;
+        auto take = [&](size_t b) { size_t o = bytes; bytes += (b + 255) & ~(size_t)255; return o; };
+        oY = take(nv * es); oW = take((size_t)std::max<int64_t>(H.upd_rows, 1) * NR * es);
+        oZs = take(nv * es); oXs = take(nv * es);
+        oCnt = take((H.ns + 3) * sizeof(int32_t));
+        oXh = take(nv * 8); oXl = take(nv * 8); oR = take(nv * 8); oB = take(nv * 8); oD = take(nv * 8);
+        oSv = take(nv * 8); oPv = take(nv * 8); oQv = take(nv * 8); oZv = take(nv * 8);
+        oXgh = take(ng * 8); oXgl = take(ng * 8); oPg = take(ng * 8);
+        oPart = take((size_t)kBS * 16 * 8); oNrm = take(16 * 8); oRho = take(16 * 8); oNum = take(16 * 8);
+        oAl = take(16 * 8); oBe = take(16 * 8); oActO = take(16 * 4); oActI = take(16 * 4);
+        oTicket = take(sizeof(unsigned));
+    }
+};
+
 // Refinement of K_ff x = b (nrhs = NR) on a double-double iterate x = xh + xl:
 //   outer: r = b - K_ff x exactly (residual_kernel); stop a right-hand side at
 //          ||r|| <= tol ||b|| or when ||r|| no longer halves;
@@ -2204,29 +2255,19 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     // past the checkpoint, so a right-hand side can stop at 0.9 tol ||b||: if its
     // exact residual then misses tol, it simply runs on.
     const double eta = 1e-4, stop = 0.9;
-    size_t off = 0;
-    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
-    const size_t oY = take(nv * sizeof(T)), oW = take(std::max<int64_t>(H.upd_rows, 1) * NR * sizeof(T));
-    const size_t oZs = take(nv * sizeof(T)), oXs = take(nv * sizeof(T));
-    const size_t oCnt = take((H.ns + 1) * sizeof(int32_t));
-    const size_t oXh = take(nv * 8), oXl = take(nv * 8), oR = take(nv * 8), oB = take(nv * 8), oD = take(nv * 8);
-    const size_t oSv = take(nv * 8), oPv = take(nv * 8), oQv = take(nv * 8), oZv = take(nv * 8);
-    const size_t oXgh = take(ng * 8), oXgl = take(ng * 8), oPg = take(ng * 8);
-    const size_t oPart = take((size_t)nblk * 16 * 8), oNrm = take(16 * 8), oRho = take(16 * 8), oNum = take(16 * 8);
-    const size_t oAl = take(16 * 8), oBe = take(16 * 8), oActO = take(16 * 4), oActI = take(16 * 4);
-    const size_t oTicket = take(sizeof(unsigned));
-    JFEM_CHECK_HIP(ctx, ensure_scratch(ctx, off));
+    const SolveLayout lay(H, ndof, NR, sizeof(T));
+    JFEM_CHECK_HIP(ctx, ensure_scratch(ctx, lay.bytes));
     char *base = (char *)ctx->scratch;
-    T *Y = (T *)(base + oY), *W = (T *)(base + oW), *Zs = (T *)(base + oZs), *Xs = (T *)(base + oXs);
-    int32_t *cnt = (int32_t *)(base + oCnt);
-    double *Xh = (double *)(base + oXh), *Xl = (double *)(base + oXl), *R = (double *)(base + oR);
-    double *Bp = (double *)(base + oB), *Dd = (double *)(base + oD), *Sv = (double *)(base + oSv);
-    double *Pv = (double *)(base + oPv), *Qv = (double *)(base + oQv), *Zv = (double *)(base + oZv);
-    double *xgh = (double *)(base + oXgh), *xgl = (double *)(base + oXgl), *pg = (double *)(base + oPg);
-    double *part = (double *)(base + oPart), *nrm = (double *)(base + oNrm), *rho = (double *)(base + oRho);
-    double *num = (double *)(base + oNum), *alpha = (double *)(base + oAl), *beta = (double *)(base + oBe);
-    int32_t *act_o = (int32_t *)(base + oActO), *act_i = (int32_t *)(base + oActI);
-    unsigned *ticket = (unsigned *)(base + oTicket);
+    T *Y = (T *)(base + lay.oY), *W = (T *)(base + lay.oW), *Zs = (T *)(base + lay.oZs), *Xs = (T *)(base + lay.oXs);
+    int32_t *cnt = (int32_t *)(base + lay.oCnt);
+    double *Xh = (double *)(base + lay.oXh), *Xl = (double *)(base + lay.oXl), *R = (double *)(base + lay.oR);
+    double *Bp = (double *)(base + lay.oB), *Dd = (double *)(base + lay.oD), *Sv = (double *)(base + lay.oSv);
+    double *Pv = (double *)(base + lay.oPv), *Qv = (double *)(base + lay.oQv), *Zv = (double *)(base + lay.oZv);
+    double *xgh = (double *)(base + lay.oXgh), *xgl = (double *)(base + lay.oXgl), *pg = (double *)(base + lay.oPg);
+    double *part = (double *)(base + lay.oPart), *nrm = (double *)(base + lay.oNrm), *rho = (double *)(base + lay.oRho);
+    double *num = (double *)(base + lay.oNum), *alpha = (double *)(base + lay.oAl), *beta = (double *)(base + lay.oBe);
+    int32_t *act_o = (int32_t *)(base + lay.oActO), *act_i = (int32_t *)(base + lay.oActI);
+    unsigned *ticket = (unsigned *)(base + lay.oTicket);
     hipStream_t st = ctx->stream;
     const dim3 gv(grid_for(nv)), gg(grid_for(ng)), blk(kBS);
     const dim3 grow((unsigned)std::max<int64_t>(1, (n * 8 + kBS - 1) / kBS));
@@ -2390,6 +2431,13 @@ int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, in
 // Device self-test of the matrix-core trailing update against its scalar form on
 // one random 200 x 70 front (three panels, L and U parts): returns the largest
 // relative difference.  precision = 64 | 32.
+int64_t jfem_symbolic_device_bytes(const jfem_symbolic *S) { return S ? S->bytes : 0; }
+
+int64_t jfem_solve_scratch_bytes(const jfem_symbolic *S, int64_t ndof, int nrhs, int precision) {
+    if (!S || nrhs < 1 || nrhs > 16) return 0;
+    return (int64_t)SolveLayout(S->H, ndof, nrhs, precision == 64 ? 8 : 4).bytes;
+}
+
 int jfem_selftest_update(jfem_ctx *ctx, int precision, double *max_rel_diff) {
     const int64_t m = 200, n = 70;
     std::vector<UpdateTask> tasks;

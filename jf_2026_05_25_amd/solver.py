@@ -57,6 +57,10 @@ class Context:
         check(self.h, self.L.jfem_device_memory(self.h, C.byref(f), C.byref(t)))
         return f.value, t.value
 
+    def device_bytes(self) -> int:
+        """Solve scratch and BC buffers this context holds on the device."""
+        return int(self.L.jfem_ctx_device_bytes(self.h))
+
     def enable_timing(self, on=True):
         check(self.h, self.L.jfem_ctx_enable_timing(self.h, int(on)))
 
@@ -139,6 +143,9 @@ class DeviceMesh:
         check(self.ctx.h, self.ctx.L.jfem_mesh_update_props(self.ctx.h, self.h, ptr(qp), ptr(tp), ptr(ep)))
         self.model = model
 
+    def device_bytes(self) -> int:
+        return int(self.ctx.L.jfem_mesh_device_bytes(self.h))
+
     def element_matrices(self, kind: str):
         """Global-frame element matrices exactly as assembled, (n, nd, nd)."""
         m = self.model
@@ -171,6 +178,9 @@ class Pattern:
         self.nnz = nnz.value
         self.ndof = mesh.model.ndof
 
+    def device_bytes(self) -> int:
+        return int(self.ctx.L.jfem_pattern_device_bytes(self.h))
+
     def free(self):
         if self.h:
             self.ctx.L.jfem_pattern_free(self.h)
@@ -194,6 +204,9 @@ class DeviceMatrix:
         self.bad_element = bad.value
         check(self.ctx.h, code)
         self.h = h
+
+    def device_bytes(self) -> int:
+        return int(self.ctx.L.jfem_matrix_device_bytes(self.h))
 
     def reassemble(self, mesh: DeviceMesh):
         bad = C.c_int64(-1)
@@ -337,6 +350,20 @@ class Symbolic:
                                                           C.byref(h), C.byref(info)))
         self.h = h
         self.info = {k: getattr(info, k) for k, _ in _lib.SymbolicInfo._fields_}
+        self.ndof = K.pattern.ndof
+
+    def device_bytes(self) -> int:
+        return int(self.ctx.L.jfem_symbolic_device_bytes(self.h))
+
+    def factor_bytes(self, precision: int) -> int:
+        """Device bytes of a numeric factor of this symbolic (L, update matrices,
+        inverse diagonal blocks)."""
+        es = 8 if precision == FP64 else 4
+        return (self.info["nnz_L"] + self.info["update_entries"] + self.info["inv_entries"]) * es
+
+    def solve_scratch_bytes(self, nrhs: int, precision: int) -> int:
+        """Context scratch one solve with ``nrhs`` right-hand sides needs."""
+        return int(self.ctx.L.jfem_solve_scratch_bytes(self.h, self.ndof, int(nrhs), int(precision)))
 
     def free(self):
         if self.h:
@@ -445,6 +472,27 @@ def factor_with_fallback(S: Symbolic, K: DeviceMatrix, precision: int, ctx: Cont
     return Factor(S, K, FP64, ctx), FP64
 
 
+def small_pivot_dofs(S: Symbolic, K: DeviceMatrix, Fct: Factor, threshold: float = 1e-7, guard: float = 1e-4,
+                     ctx: Context | None = None):
+    """Post-factor pivot check: global DOFs whose pivot ratio L_jj^2 / |K_jj| is
+    below ``threshold`` (src/solver/boundary_conditions.jl:1662-1684; the ratio is
+    taken under this factor's own elimination order, as the reference takes it
+    under CHOLMOD's).  The reference decides on an FP64 factor.  An FP32 factor's
+    ratios carry errors of order 2^-24 relative to |K_jj| times the pivot growth,
+    so for one the ratios below ``guard`` are undecided and the whole set is
+    decided on an FP64 factor of the same symbolic; with none below ``guard`` the
+    FP32 factor decides (nothing is small).  Returns ``(dofs, decided_in)``."""
+    if Fct.precision == FP64:
+        return Fct.small_pivots(K, threshold), FP64
+    if len(Fct.small_pivots(K, guard)) == 0:
+        return np.zeros(0, np.int64), FP32
+    F64 = Factor(S, K, FP64, ctx or Fct.ctx)
+    try:
+        return F64.small_pivots(K, threshold), FP64
+    finally:
+        F64.free()
+
+
 class DeviceBuffer:
     """Device memory owned by the caller of the ``_dev`` entry points."""
 
@@ -493,34 +541,46 @@ class LinearSolveCache:
 
 def apply_bc_and_solve(asm: AssembledStiffness, model, F_applied=None, *, precision: int = FP64,
                        tol: float = 1e-10, max_refine: int = 100, pivot_threshold: float = 1e-7,
-                       linear_cache: LinearSolveCache | None = None, symbolic: Symbolic | None = None):
+                       linear_cache: LinearSolveCache | None = None, symbolic: Symbolic | None = None,
+                       autospc: bool | None = None):
     """BC partition, factor and solve of K_ff u = F_ff on the GPU.
 
     Returns ``(u_global, fixed_dofs, spc_dofs, diagnostics)`` like the reference
     (src/solver/boundary_conditions.jl:1324, 1974).  ``precision`` = 64 (plain FP64
-    factor) or 32 (FP32 factor + FP64 iterative refinement to ``tol``)."""
+    factor) or 32 (FP32 factor + FP64 iterative refinement to ``tol``).
+    ``autospc`` (default: ``model.meta["autospc"]``, else on, as PARAM,AUTOSPC
+    defaults to YES: boundary_conditions.jl:3-18) gates the post-factor pivot
+    check the way the reference gates it (:1684).  A ``symbolic`` passed in is
+    used when its free set matches and is never freed here."""
     ctx = asm.K.ctx
     F = model.loads if F_applied is None else np.asarray(F_applied, float)
+    if autospc is None:
+        autospc = bool(getattr(model, "meta", {}).get("autospc", True))
     diag = {"bc_partition": {}, "linear_solver": {}}
     t0 = time.perf_counter()
     key = (id(asm.K), precision)
     entry = linear_cache.get(key) if linear_cache is not None else None
     if entry is None:
         fixed = bc_partition(asm.K, model.spc)
-        S = symbolic if (symbolic is not None and np.array_equal(symbolic.fixed, fixed)) else Symbolic(asm.K, asm.mesh, fixed, ctx)
+        own_S = not (symbolic is not None and np.array_equal(symbolic.fixed, fixed))
+        S = Symbolic(asm.K, asm.mesh, fixed, ctx) if own_S else symbolic
         t1 = time.perf_counter()
         Fct, used = factor_with_fallback(S, asm.K, precision, ctx)
-        small = Fct.small_pivots(asm.K, pivot_threshold)
-        n_post = 0
-        if len(small):   # post-factor pivot check: extra SPC and refactor (:1684-1710)
-            n_post = len(small)
-            fixed = fixed.copy()
-            fixed[small] = 1
-            Fct.free(); S.free()
-            S = Symbolic(asm.K, asm.mesh, fixed, ctx)
-            Fct, used = factor_with_fallback(S, asm.K, precision, ctx)
+        n_post, decided = 0, used
+        if autospc:
+            small, decided = small_pivot_dofs(S, asm.K, Fct, pivot_threshold, ctx=ctx)
+            if len(small):   # post-factor pivot check: extra SPC and refactor (:1684-1710)
+                n_post = len(small)
+                fixed = fixed.copy()
+                fixed[small] = 1
+                Fct.free()
+                if own_S:
+                    S.free()
+                S = Symbolic(asm.K, asm.mesh, fixed, ctx)
+                Fct, used = factor_with_fallback(S, asm.K, precision, ctx)
         t2 = time.perf_counter()
         entry = (fixed, S, Fct, t1 - t0, t2 - t1, n_post, used)
+        diag["bc_partition"]["post_factorization_check_precision"] = decided
         if linear_cache is not None:
             linear_cache.put(key, entry)
         diag["linear_solver"]["cache_hit"] = False
