@@ -175,6 +175,15 @@ int jfem_refactor(jfem_ctx *ctx, jfem_factor *F, const jfem_matrix *K, int64_t *
  * into dofs_out (capacity cap) and their count. */
 int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K,
                              double threshold, int64_t *dofs_out, int64_t cap, int64_t *count);
+/* the ratios behind that check, per elimination position k: L_kk^2 / |K_gg| for
+ * the global DOF g eliminated at k (1.0 where |K_gg| <= 1e-30); ratios_out holds
+ * n_free doubles.  The pivot_ratios loop of src/solver/boundary_conditions.jl:1664-1675. */
+int jfem_factor_pivot_ratios(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K,
+                             double *ratios_out);
+/* the elimination order of a symbolic factorization: dofs_out[k] (n_free entries)
+ * is the global DOF eliminated at position k (the reference's F_chol.p,
+ * src/solver/boundary_conditions.jl:1666, with the free-set indexing resolved) */
+int jfem_symbolic_order(const jfem_symbolic *S, int64_t *dofs_out);
 int jfem_factor_free(jfem_factor *F);
 
 /* Solve K_ff x = b for nrhs <= 16 right-hand sides given in FULL DOF numbering

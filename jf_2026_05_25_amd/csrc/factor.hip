@@ -2164,11 +2164,16 @@ int jfem_factor_free(jfem_factor *F) {
 
 int64_t jfem_factor_device_bytes(const jfem_factor *F) { return F ? F->bytes : 0; }
 
-int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, double threshold,
-                             int64_t *dofs_out, int64_t cap, int64_t *count) {
+int jfem_symbolic_order(const jfem_symbolic *S, int64_t *dofs_out) {
+    if (!S) return JFEM_ERR_BAD_ARG;
+    for (int64_t k = 0; k < S->H.n; ++k) dofs_out[k] = S->H.perm[k];
+    return JFEM_OK;
+}
+
+int jfem_factor_pivot_ratios(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, double *ratios_out) {
     const jfem_symbolic *S = F->S;
     const SymbolicHost &H = S->H;
-    *count = 0;
+    if (K->P != S->P) return fail(ctx, JFEM_ERR_BAD_ARG, "matrix and factor use different patterns");
     if (H.n == 0) return JFEM_OK;
     std::vector<int32_t> pos2sn(H.n), diagpos(K->P->diagpos);
     for (int64_t s = 0; s < H.ns; ++s)
@@ -2182,10 +2187,20 @@ int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_mat
     JFEM_CHECK_HIP(ctx, hipMemcpyAsync(d_dp, diagpos.data(), diagpos.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(diag_ratio_kernel, dim3(grid_for(H.n)), dim3(kBS), 0, ctx->stream, H.n, S->d_perm, d_p2s, S->d_col0,
                        S->d_m, S->d_lofs, d_dp, K->d_val, F->d_L, F->precision == 64 ? 1 : 0, d_out);
-    std::vector<double> ratio(H.n);
-    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(ratio.data(), d_out, H.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(ratios_out, d_out, H.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     (void)hipFree(d_p2s); (void)hipFree(d_dp); (void)hipFree(d_out);
+    return JFEM_OK;
+}
+
+int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, double threshold,
+                             int64_t *dofs_out, int64_t cap, int64_t *count) {
+    const SymbolicHost &H = F->S->H;
+    *count = 0;
+    if (H.n == 0) return JFEM_OK;
+    std::vector<double> ratio(H.n);
+    const int rc = jfem_factor_pivot_ratios(ctx, F, K, ratio.data());
+    if (rc != JFEM_OK) return rc;
     int64_t c = 0;
     for (int64_t i = 0; i < H.n; ++i)
         if (ratio[i] < threshold) {

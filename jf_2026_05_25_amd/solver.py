@@ -355,6 +355,12 @@ class Symbolic:
     def device_bytes(self) -> int:
         return int(self.ctx.L.jfem_symbolic_device_bytes(self.h))
 
+    def order(self) -> np.ndarray:
+        """Elimination order: the global DOF eliminated at each position."""
+        out = np.zeros(max(self.info["n_free"], 1), np.int64)
+        check(self.ctx.h, self.ctx.L.jfem_symbolic_order(self.h, ptr(out)))
+        return out[:self.info["n_free"]]
+
     def factor_bytes(self, precision: int) -> int:
         """Device bytes of a numeric factor of this symbolic (L, update matrices,
         inverse diagonal blocks)."""
@@ -400,6 +406,12 @@ class Factor:
         check(self.ctx.h, self.ctx.L.jfem_factor_small_pivots(self.ctx.h, self.h, K.h, threshold, ptr(out), cap,
                                                               C.byref(cnt)))
         return out[:cnt.value].copy()
+
+    def pivot_ratios(self, K: DeviceMatrix) -> np.ndarray:
+        """L_kk^2 / |K_gg| per elimination position (see Symbolic.order)."""
+        out = np.zeros(max(self.symbolic.info["n_free"], 1))
+        check(self.ctx.h, self.ctx.L.jfem_factor_pivot_ratios(self.ctx.h, self.h, K.h, ptr(out)))
+        return out[:self.symbolic.info["n_free"]]
 
     def device_bytes(self):
         return int(self.ctx.L.jfem_factor_device_bytes(self.h))

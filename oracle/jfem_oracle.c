@@ -1177,10 +1177,29 @@ static void ltsolve(int64_t n, const int64_t *Lp, const int32_t *Li, const doubl
     for (int64_t j = n - 1; j >= 0; --j) { for (int64_t p = Lp[j] + 1; p < Lp[j + 1]; ++p) y[j] -= Lx[p] * y[Li[p]]; y[j] /= Lx[Lp[j]]; }
 }
 
+static int solve_impl(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *val,
+                      const uint8_t *fixed, int order_kind, const double *xyz_nodes, const int64_t *given_order,
+                      int nrhs, const double *b, double *u, double *relres,
+                      int64_t *nnz_L_out, double *flops_out, int64_t *n_small, int refine, double *ratios_out);
+
 int or_solve(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *val,
              const uint8_t *fixed, int order_kind, const double *xyz_nodes,
              int nrhs, const double *b, double *u, double *relres,
              int64_t *nnz_L_out, double *flops_out, int64_t *n_small, int refine) {
+    return solve_impl(ndof, rp, ci, val, fixed, order_kind, xyz_nodes, NULL, nrhs, b, u, relres, nnz_L_out, flops_out,
+                      n_small, refine, NULL);
+}
+
+int or_pivot_ratios(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *val, const uint8_t *fixed,
+                    const int64_t *order, double *ratios, int64_t *n_small) {
+    int64_t nz = 0; double fl = 0;
+    return solve_impl(ndof, rp, ci, val, fixed, 2, NULL, order, 0, NULL, NULL, NULL, &nz, &fl, n_small, 0, ratios);
+}
+
+static int solve_impl(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *val,
+                      const uint8_t *fixed, int order_kind, const double *xyz_nodes, const int64_t *given_order,
+                      int nrhs, const double *b, double *u, double *relres,
+                      int64_t *nnz_L_out, double *flops_out, int64_t *n_small, int refine, double *ratios_out) {
     /* free DOF numbering (ascending global DOF) */
     int64_t *loc = malloc(sizeof(int64_t) * ndof); int64_t n = 0;
     for (int64_t i = 0; i < ndof; ++i) loc[i] = fixed[i] ? -1 : n++;
@@ -1196,7 +1215,14 @@ int or_solve(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *v
       free(fill); }
     /* ordering */
     int32_t *perm = malloc(sizeof(int32_t) * (n ? n : 1));
-    if (order_kind == 1 && xyz_nodes) {
+    if (order_kind == 2) {
+        /* the caller's elimination order, as global DOFs (position k eliminates
+         * DOF given_order[k]); must be a permutation of the free DOFs */
+        for (int64_t k = 0; k < n; ++k) {
+            int64_t g = given_order[k];
+            perm[k] = (g >= 0 && g < ndof && loc[g] >= 0) ? (int32_t)loc[g] : 0;
+        }
+    } else if (order_kind == 1 && xyz_nodes) {
         /* supervariables = grid points: order grid points by ND, expand to their free DOFs */
         int64_t nn = ndof / 6;
         int64_t *np_ = calloc(nn + 1, sizeof(int64_t));
@@ -1279,7 +1305,8 @@ int or_solve(int64_t ndof, const int64_t *rp, const int32_t *ci, const double *v
     if (st == OR_OK) {
         for (int64_t k = 0; k < n; ++k) { double lkk = Lx[Lp[k]]; int64_t g = glob[perm[k]];
             double kd = fabs(csr_get(rp, ci, val, g, g));
-            double ratio = kd > 1e-30 ? lkk * lkk / kd : 1.0; if (ratio < 1e-7) ++smallp; }
+            double ratio = kd > 1e-30 ? lkk * lkk / kd : 1.0; if (ratio < 1e-7) ++smallp;
+            if (ratios_out) ratios_out[k] = ratio; }
         double *y = malloc(sizeof(double) * (n ? n : 1));
         long double *xl = refine > 0 ? malloc(sizeof(long double) * (n ? n : 1)) : NULL;
         for (int r = 0; r < nrhs; ++r) {

@@ -84,6 +84,15 @@ int or_solve(int64_t ndof, const int64_t *row_ptr, const int32_t *col, const dou
              int nrhs, const double *b, double *u, double *relres,
              int64_t *nnz_L, double *flops, int64_t *n_small_pivots, int refine);
 
+/* Post-factor pivot ratios L_kk^2 / |K_gg| (1.0 where |K_gg| <= 1e-30) of the
+ * Cholesky factor of K_ff under the caller's elimination order: order[k] is the
+ * global DOF eliminated at position k (a permutation of the free DOFs);
+ * ratios[k] is that position's ratio and n_small counts ratios below 1e-7
+ * (src/solver/boundary_conditions.jl:1662-1684).  OR_ERR_NOT_SPD on a
+ * non-positive pivot. */
+int or_pivot_ratios(int64_t ndof, const int64_t *row_ptr, const int32_t *col, const double *val,
+                    const uint8_t *fixed, const int64_t *order, double *ratios, int64_t *n_small);
+
 /* ---------- load vector of one load set (FORCE / MOMENT / shell PLOAD4) ---------- */
 /* Follows _resolve_loads! (src/solver/loads.jl:510-587, 1338-1340) for one SID in
  * basic coordinates: PLOAD4 cards in order straight into F, then the per-grid sums

@@ -49,6 +49,7 @@ def lib():
         L.or_bc_partition.restype = C.c_int64
         L.or_solve.argtypes = [C.c_int64, ip64, ip32, dp, u8p, C.c_int, C.c_void_p, C.c_int, dp, dp, dp,
                                C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]
+        L.or_pivot_ratios.argtypes = [C.c_int64, ip64, ip32, dp, u8p, ip64, dp, C.POINTER(C.c_int64)]
         L.or_eig3.argtypes = [dp, dp, dp]
         vpp = C.c_void_p
         L.or_load_vector.argtypes = [C.c_int64, dp, dp, dp, C.c_int64, vpp, C.c_int64, vpp, C.c_int64, vpp, vpp, vpp,
@@ -198,6 +199,23 @@ def solve(K, fixed, B, order="nd", xyz=None, refine=0):
         raise OracleError(st)
     return U.reshape(nrhs, n).T, rr, {"nnz_L": nzl.value, "flops_colcount2": fl.value,
                                       "small_pivots": ns.value}
+
+
+def pivot_ratios(K, fixed, order):
+    """Post-factor pivot ratios L_kk^2/|K_gg| of K_ff's Cholesky factor under the
+    elimination order ``order`` (global DOFs, a permutation of the free ones);
+    position k's ratio belongs to DOF order[k].  Raises OracleError on a
+    non-positive pivot."""
+    K = sp.csr_matrix(K)
+    K.sort_indices()
+    order = np.ascontiguousarray(np.asarray(order, np.int64))
+    ratios = np.zeros(max(len(order), 1))
+    ns = C.c_int64()
+    st = lib().or_pivot_ratios(K.shape[0], K.indptr.astype(np.int64), K.indices.astype(np.int32), _f(K.data),
+                               np.ascontiguousarray(np.asarray(fixed, np.uint8)), order, ratios, C.byref(ns))
+    if st:
+        raise OracleError(st)
+    return ratios[:len(order)]
 
 
 def eig3(A):

@@ -48,3 +48,38 @@ def golden_model(case: dict, sub: dict, membrane_modes: int = 15) -> meshes.Mode
                      np.zeros((0, 8)), np.zeros((0, 10), np.int32), np.zeros((0, 2)),
                      spc, F[:, None], case["k6rot"], {"expected": expected})
     return m
+
+
+def flap_model(n: int = 4, k6rot: float = 0.01) -> meshes.Model:
+    """A near-mechanism for the post-factor pivot check: plate A on [0,1]^2
+    (n x n CQUAD4, edge x = 0 clamped) and plate B on [1,2] x [1,2] joined to A at
+    the single grid point (1,1).  B's in-plane rotation about that point is
+    resisted only by drilling stiffness (proportional to K6ROT), so with a small
+    K6ROT one pivot ratio is tiny (about 8e-8 K6ROT under some orders) and the next
+    smallest is about 2e-4: the check flags one DOF, which one depending on the
+    elimination order."""
+    xs = np.linspace(0.0, 1.0, n + 1)
+    pts = [(x, y, 0.0) for y in xs for x in xs]
+    ia = {(i, j): j * (n + 1) + i for j in range(n + 1) for i in range(n + 1)}
+    ib = {}
+    for j in range(n + 1):
+        for i in range(n + 1):
+            if i == 0 and j == 0:
+                ib[(i, j)] = ia[(n, n)]
+            else:
+                ib[(i, j)] = len(pts)
+                pts.append((1.0 + xs[i], 1.0 + xs[j], 0.0))
+    quads = [[idx[(i, j)], idx[(i + 1, j)], idx[(i + 1, j + 1)], idx[(i, j + 1)]]
+             for idx in (ia, ib) for j in range(n) for i in range(n)]
+    xyz = np.array(pts)
+    quads = np.array(quads, np.int32)
+    props = meshes.pshell_props(len(quads), 0.002, 70e9, 0.33)
+    ndof = 6 * len(xyz)
+    spc = np.zeros(ndof, np.uint8)
+    for j in range(n + 1):
+        spc[6 * ia[(0, j)]:6 * ia[(0, j)] + 6] = 1
+    F = np.zeros((ndof, 2))
+    F[2::6, 0] = 1.0
+    F[0::6, 1] = np.linspace(-1.0, 1.0, len(xyz))
+    return meshes.Model("flap", xyz, quads, props, np.zeros((0, 3), np.int32), np.zeros((0, 8)),
+                        np.zeros((0, 10), np.int32), np.zeros((0, 2)), spc, F, k6rot)

@@ -226,3 +226,44 @@ def test_refined_solve_is_ordering_independent(oracle):
     d_plain = np.abs(pa - pb).max() / np.abs(pa).max()
     assert d_ref <= 1e-14, d_ref
     assert d_plain > d_ref
+
+
+def test_pivot_ratios_closed_forms(oracle):
+    """or_pivot_ratios under a given order: 2x2 Schur complements in either order,
+    a diagonal matrix (all ratios 1), agreement with or_solve's count."""
+    import scipy.sparse as sp
+    a, b, c = 4.0, 1.5, 2.0
+    K = sp.csr_matrix(np.array([[a, b], [b, c]]))
+    fixed = np.zeros(2, np.uint8)
+    r01 = oracle.pivot_ratios(K, fixed, [0, 1])
+    r10 = oracle.pivot_ratios(K, fixed, [1, 0])
+    assert np.allclose(r01, [1.0, (c - b * b / a) / c], rtol=1e-15)
+    assert np.allclose(r10, [1.0, (a - b * b / c) / a], rtol=1e-15)
+    # fixed DOFs are skipped; the order names global DOFs
+    K3 = sp.csr_matrix(np.array([[a, b, 0.0], [b, c, 0.0], [0.0, 0.0, 7.0]]))
+    r = oracle.pivot_ratios(K3, np.array([0, 0, 1], np.uint8), [1, 0])
+    assert np.allclose(r, r10, rtol=1e-15)
+    assert np.allclose(oracle.pivot_ratios(sp.diags([1.0, 2.0, 3.0]).tocsr(), np.zeros(3, np.uint8), [2, 0, 1]), 1.0)
+    # a non-positive pivot is an error
+    with pytest.raises(oracle.OracleError):
+        oracle.pivot_ratios(sp.csr_matrix(np.array([[1.0, 2.0], [2.0, 1.0]])), fixed, [0, 1])
+
+
+def test_pivot_ratios_flap_and_count(oracle):
+    """The near-mechanism of the pivot-check tests: one ratio proportional to
+    K6ROT (drilling is the only stiffness of the flap's in-plane rotation), the
+    rest O(1e-4) or larger; or_solve's small-pivot count under ND agrees with the
+    ratios under the same order."""
+    from _helpers import flap_model
+    r1 = []
+    for k6 in (0.01, 0.001):
+        m = flap_model(4, k6)
+        K = oracle.assemble(m)
+        fixed = oracle.bc_partition(K, m.spc)
+        free = np.where(fixed == 0)[0]
+        r = np.sort(oracle.pivot_ratios(K, fixed, free))
+        assert r[1] > 1e-5
+        r1.append(r[0])
+        _, _, info = oracle.solve(K, fixed, m.loads[:, :1])
+        assert info["small_pivots"] in (0, 1)
+    assert abs(r1[0] / r1[1] - 10.0) < 1e-3            # linear in K6ROT
