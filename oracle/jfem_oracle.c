@@ -654,9 +654,9 @@ int or_q4_global(const double p[12], const double prop[8], double k6rot, double 
 /* CTRIA3, stiffness_tria3_matrices_generic default branches,                 */
 /* src/FEMKernels.jl:9375-9809, guard :9281-9312                               */
 /* ------------------------------------------------------------------------- */
-static void tria3_guard(double *Zb, double La, double Lb, double c, double s, double Dn) {
+static int tria3_guard(double *Zb, double La, double Lb, double c, double s, double Dn) {
     double scale = 6 * (La / Lb) * s * Dn;
-    if (!(scale > 0 && M(Zb, 2, 2, 3) > 0)) return;
+    if (!(scale > 0 && M(Zb, 2, 2, 3) > 0)) return 0;
     double z33 = M(Zb, 2, 2, 3);
     double h11 = scale * (M(Zb, 0, 0, 3) - M(Zb, 0, 2, 3) * M(Zb, 0, 2, 3) / z33) / (La * La);
     double h22 = scale * (M(Zb, 1, 1, 3) - M(Zb, 1, 2, 3) * M(Zb, 1, 2, 3) / z33) / (Lb * Lb);
@@ -665,55 +665,43 @@ static void tria3_guard(double *Zb, double La, double Lb, double c, double s, do
     double a = fmax(p11, 1.0 / 10000), q = fmax(p22, a / 10000);
     double limit = sqrt((85.0 / 100) * a * q);
     double b = p12 < -limit ? -limit : (p12 > limit ? limit : p12);
-    if (a == p11 && b == p12 && q == p22) return;
+    if (a == p11 && b == p12 && q == p22) return 0;
     double n11 = a, n12 = c * a + s * b, n22 = c * c * a + 2 * c * s * b + s * s * q;
     M(Zb, 0, 0, 3) += (n11 - h11) * La * La / scale;
     M(Zb, 0, 1, 3) += (n12 - h12) * La * Lb / scale;
     M(Zb, 1, 0, 3) = M(Zb, 0, 1, 3);
     M(Zb, 1, 1, 3) += (n22 - h22) * Lb * Lb / scale;
+    return 1;
 }
 
-int or_t3_local(const double lc[6], const double Dm[9], const double Db[9],
-                const double Ds[4], double h, double G_ref, double k6rot, double Ke[324]) {
-    (void)h; (void)G_ref;
-    for (int q = 0; q < 324; ++q) Ke[q] = 0;
+int or_t3_guard(double Zb[9], double La, double Lb, double c, double s, double Dn) {
+    double scale = 6 * (La / Lb) * s * Dn;
+    if (scale > 0 && Zb[8] > 0) {
+        double z33 = Zb[8];
+        double h11 = scale * (Zb[0] - Zb[6] * Zb[6] / z33) / (La * La);
+        double h22 = scale * (Zb[4] - Zb[7] * Zb[7] / z33) / (Lb * Lb);
+        double h12 = scale * (Zb[3] - Zb[6] * Zb[7] / z33) / (La * Lb);
+        double p22 = (h22 - 2 * c * h12 + c * c * h11) / (s * s);
+        if (!(isfinite(p22) && p22 > 0)) return -1;   /* ArgumentError, :9297-9298 */
+    }
+    return tria3_guard(Zb, La, Lb, c, s, Dn);
+}
+
+/* MITC3 shear of the CTRIA3 plate part: tying rows D3 (grA, gsB, cvec; 9 entries
+ * each, w / rx / ry per node), tying rigidity W (:9541-9550), residual bending
+ * flexibility Zb3 with directional pairing before (Zb) and after (Zg) the guard
+ * (:9578-9724); W_out receives the rigidity that enters Ks = D3' W D3, i.e.
+ * inv(sym(inv(W) + Zg)) when Zb3 applies.  Returns 1 when Zb3 applies, 0 when not,
+ * a negative status on a degenerate inverse. */
+static int t3_shear(const double lc[6], const double Db[9], const double Ds[4], double A,
+                    double grA[9], double gsB[9], double cvec[9], double Wrig[9], double Zb_raw[9],
+                    double Zb_guarded[9], double W_out[9]) {
     double x1 = lc[0], y1 = lc[1], x2 = lc[2], y2 = lc[3], x3 = lc[4], y3 = lc[5];
-    double A2 = x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2);
-    double A = 0.5 * fabs(A2);
-    if (A < 1e-12) return OR_OK; /* zero matrix, :9381 */
-    double bv[3] = {(y2 - y3) / (2 * A), (y3 - y1) / (2 * A), (y1 - y2) / (2 * A)};
-    double cv[3] = {(x3 - x2) / (2 * A), (x1 - x3) / (2 * A), (x2 - x1) / (2 * A)};
-    /* CST membrane :9390-9396 */
-    double Bm[18] = {0};
-    for (int i = 0; i < 3; ++i) {
-        M(Bm, 0, 2 * i, 3) = bv[i]; M(Bm, 1, 2 * i + 1, 3) = cv[i];
-        M(Bm, 2, 2 * i, 3) = cv[i]; M(Bm, 2, 2 * i + 1, 3) = bv[i];
-    }
-    static const int mi[6] = {0, 1, 6, 7, 12, 13};
-    for (int j = 0; j < 6; ++j)
-        for (int i = 0; i < 6; ++i) {
-            double s = 0;
-            for (int a = 0; a < 3; ++a) { double t = 0; for (int b = 0; b < 3; ++b) t += M(Bm, b, i, 3) * M(Dm, b, a, 3); s += t * M(Bm, a, j, 3); }
-            M(Ke, mi[i], mi[j], 18) = s * A;
-        }
-    /* constant-curvature bending :9399-9408 */
-    double Bb[27] = {0};
-    for (int i = 0; i < 3; ++i) {
-        M(Bb, 0, 3 * i + 2, 3) = bv[i]; M(Bb, 1, 3 * i + 1, 3) = -cv[i];
-        M(Bb, 2, 3 * i + 1, 3) = -bv[i]; M(Bb, 2, 3 * i + 2, 3) = cv[i];
-    }
-    double Kb[81];
-    for (int j = 0; j < 9; ++j)
-        for (int i = 0; i < 9; ++i) {
-            double s = 0;
-            for (int a = 0; a < 3; ++a) { double t = 0; for (int b = 0; b < 3; ++b) t += M(Bb, b, i, 3) * M(Db, b, a, 3); s += t * M(Bb, a, j, 3); }
-            M(Kb, i, j, 9) = s * A;
-        }
     /* MITC3 tying rows :9484-9508 */
     double Jm11 = x2 - x1, Jm12 = y2 - y1, Jm21 = x3 - x1, Jm22 = y3 - y1;
     double dj = Jm11 * Jm22 - Jm12 * Jm21;
     double Ji11 = Jm22 / dj, Ji12 = -Jm12 / dj, Ji21 = -Jm21 / dj, Ji22 = Jm11 / dj;
-    double grA[9], gsB[9], grC[9], gsC[9], tmp9[9];
+    double grC[9], gsC[9];
     {
         const double rr[3] = {0.5, 0.0, 0.5}, ss[3] = {0.0, 0.5, 0.5};
         const double dwdr[3] = {-1, 1, 0}, dwds[3] = {-1, 0, 1};
@@ -725,13 +713,11 @@ int or_t3_local(const double lc[6], const double Dm[9], const double Db[9],
                 Br[3 * i + 2] = N[i] * Jm11; Bs[3 * i + 2] = N[i] * Jm21;
                 Br[3 * i + 1] = -N[i] * Jm12; Bs[3 * i + 1] = -N[i] * Jm22;
             }
-            if (pnt == 0) memcpy(grA, Br, sizeof(grA));
-            else if (pnt == 1) memcpy(gsB, Bs, sizeof(gsB));
-            else { memcpy(grC, Br, sizeof(grC)); memcpy(gsC, Bs, sizeof(gsC)); }
+            if (pnt == 0) memcpy(grA, Br, sizeof(Br));
+            else if (pnt == 1) memcpy(gsB, Bs, sizeof(Bs));
+            else { memcpy(grC, Br, sizeof(Br)); memcpy(gsC, Bs, sizeof(Bs)); }
         }
-        (void)tmp9;
     }
-    double cvec[9];
     for (int i = 0; i < 9; ++i) cvec[i] = (grC[i] - grA[i]) - (gsC[i] - gsB[i]);
     double La = hypot(x2 - x1, y2 - y1), Lb = hypot(x3 - x1, y3 - y1), Lc = hypot(x3 - x2, y3 - y2);
     double P = La + Lb + Lc;
@@ -747,11 +733,13 @@ int or_t3_local(const double lc[6], const double Dm[9], const double Db[9],
         G11 = Ji11 * t11 + Ji21 * t21; G12 = Ji11 * t12 + Ji21 * t22; G22 = Ji12 * t12 + Ji22 * t22;
     }
     double uu = 3 * La / P, vv = 3 * Lb / P;
-    double W[9];
+    double *W = W_out;
     M(W, 0, 0, 3) = G11 * A; M(W, 0, 1, 3) = G12 * A; M(W, 1, 0, 3) = M(W, 0, 1, 3); M(W, 1, 1, 3) = G22 * A;
     M(W, 0, 2, 3) = (A / 3) * (uu * G11 - vv * G12); M(W, 2, 0, 3) = M(W, 0, 2, 3);
     M(W, 1, 2, 3) = (A / 3) * (uu * G12 - vv * G22); M(W, 2, 1, 3) = M(W, 1, 2, 3);
     M(W, 2, 2, 3) = uu * uu * G11 * (A / 6) - 2 * uu * vv * G12 * (A * (2 - cth) / 18) + vv * vv * G22 * (A / 6);
+    memcpy(Wrig, W, 9 * sizeof(double));
+    for (int q = 0; q < 9; ++q) { Zb_raw[q] = 0; Zb_guarded[q] = 0; }
     /* residual bending flexibility Zb3 with directional pairing :9578-9728 */
     double Dbend = fabs(M(Db, 0, 0, 3));
     if (sth > 1e-9 && Dbend > 1e-30 && La > 1e-30 && Lb > 1e-30) {
@@ -792,13 +780,65 @@ int or_t3_local(const double lc[6], const double Dm[9], const double Db[9],
         M(Zb, 0, 2, 3) = -s6 / Gd * S1 * S3; M(Zb, 2, 0, 3) = M(Zb, 0, 2, 3);
         M(Zb, 1, 2, 3) = s6 / Gd * S2 * S3; M(Zb, 2, 1, 3) = M(Zb, 1, 2, 3);
         M(Zb, 2, 2, 3) = s6 / Gd * S3 * S3;
+        memcpy(Zb_raw, Zb, sizeof(Zb));
         tria3_guard(Zb, La, Lb, cth, sth, dir_pair ? Dn[0] : Dbend);
+        memcpy(Zb_guarded, Zb, sizeof(Zb));
         double Wi[9], Zt[9], Zs[9];
-        if (inv3(W, Wi) != OR_OK) return OR_ERR_DEGENERATE;
+        if (inv3(W, Wi) != OR_OK) return -OR_ERR_DEGENERATE;
         for (int q = 0; q < 9; ++q) Zt[q] = Wi[q] + Zb[q];
         for (int j = 0; j < 3; ++j) for (int i = 0; i < 3; ++i) Zs[i + 3 * j] = 0.5 * (Zt[i + 3 * j] + Zt[j + 3 * i]);
-        if (inv3(Zs, W) != OR_OK) return OR_ERR_DEGENERATE;
+        if (inv3(Zs, W) != OR_OK) return -OR_ERR_DEGENERATE;
+        return 1;
     }
+    return 0;
+}
+
+int or_t3_shear_parts(const double lc[6], const double Db[9], const double Ds[4], double D3[27], double Wrig[9],
+                      double Zb[9], double Zg[9], double W[9]) {
+    double x1 = lc[0], y1 = lc[1], x2 = lc[2], y2 = lc[3], x3 = lc[4], y3 = lc[5];
+    double A = 0.5 * fabs(x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2));
+    return t3_shear(lc, Db, Ds, A, D3, D3 + 9, D3 + 18, Wrig, Zb, Zg, W);
+}
+
+int or_t3_local(const double lc[6], const double Dm[9], const double Db[9],
+                const double Ds[4], double h, double G_ref, double k6rot, double Ke[324]) {
+    (void)h; (void)G_ref;
+    for (int q = 0; q < 324; ++q) Ke[q] = 0;
+    double x1 = lc[0], y1 = lc[1], x2 = lc[2], y2 = lc[3], x3 = lc[4], y3 = lc[5];
+    double A2 = x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2);
+    double A = 0.5 * fabs(A2);
+    if (A < 1e-12) return OR_OK; /* zero matrix, :9381 */
+    double bv[3] = {(y2 - y3) / (2 * A), (y3 - y1) / (2 * A), (y1 - y2) / (2 * A)};
+    double cv[3] = {(x3 - x2) / (2 * A), (x1 - x3) / (2 * A), (x2 - x1) / (2 * A)};
+    /* CST membrane :9390-9396 */
+    double Bm[18] = {0};
+    for (int i = 0; i < 3; ++i) {
+        M(Bm, 0, 2 * i, 3) = bv[i]; M(Bm, 1, 2 * i + 1, 3) = cv[i];
+        M(Bm, 2, 2 * i, 3) = cv[i]; M(Bm, 2, 2 * i + 1, 3) = bv[i];
+    }
+    static const int mi[6] = {0, 1, 6, 7, 12, 13};
+    for (int j = 0; j < 6; ++j)
+        for (int i = 0; i < 6; ++i) {
+            double s = 0;
+            for (int a = 0; a < 3; ++a) { double t = 0; for (int b = 0; b < 3; ++b) t += M(Bm, b, i, 3) * M(Dm, b, a, 3); s += t * M(Bm, a, j, 3); }
+            M(Ke, mi[i], mi[j], 18) = s * A;
+        }
+    /* constant-curvature bending :9399-9408 */
+    double Bb[27] = {0};
+    for (int i = 0; i < 3; ++i) {
+        M(Bb, 0, 3 * i + 2, 3) = bv[i]; M(Bb, 1, 3 * i + 1, 3) = -cv[i];
+        M(Bb, 2, 3 * i + 1, 3) = -bv[i]; M(Bb, 2, 3 * i + 2, 3) = cv[i];
+    }
+    double Kb[81];
+    for (int j = 0; j < 9; ++j)
+        for (int i = 0; i < 9; ++i) {
+            double s = 0;
+            for (int a = 0; a < 3; ++a) { double t = 0; for (int b = 0; b < 3; ++b) t += M(Bb, b, i, 3) * M(Db, b, a, 3); s += t * M(Bb, a, j, 3); }
+            M(Kb, i, j, 9) = s * A;
+        }
+    double grA[9], gsB[9], cvec[9], Wrig[9], Zb_raw[9], Zb_g[9], W[9];
+    int sh = t3_shear(lc, Db, Ds, A, grA, gsB, cvec, Wrig, Zb_raw, Zb_g, W);
+    if (sh < 0) return -sh;
     /* Ks = D3' W D3, D3 rows grA, gsB, cvec :9731-9733 */
     const double *D3[3] = {grA, gsB, cvec};
     static const int bi[9] = {2, 3, 4, 8, 9, 10, 14, 15, 16};

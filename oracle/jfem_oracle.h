@@ -43,6 +43,16 @@ int or_q4_macneal_parts(const double lc[8], const double Cb[9], const double Cs[
                         double Z[16], double Kp[144]);
 int or_t3_local(const double lc[6], const double Dm[9], const double Db[9],
                 const double Ds[4], double h, double G_ref, double k6rot, double Ke[324]);
+/* CTRIA3 MITC3 shear pieces for the closed-form pins: tying rows D3 (3 rows of 9,
+ * row-major), tying rigidity Wrig, residual bending flexibility before (Zb) and
+ * after (Zg) the guard, and the rigidity W entering D3' W D3 (3x3 column-major).
+ * Returns 1 when Zb3 applies, 0 when not, -status on error. */
+int or_t3_shear_parts(const double lc[6], const double Db[9], const double Ds[4], double D3[27],
+                      double Wrig[9], double Zb[9], double Zg[9], double W[9]);
+/* tria3_guard_bending_flexibility! (src/FEMKernels.jl:9281-9312) on Zb (3x3
+ * column-major) in place: 1 when a bound changed it, 0 when not, -1 on a
+ * non-positive normal flexibility (ArgumentError). */
+int or_t3_guard(double Zb[9], double La, double Lb, double c, double s, double Dn);
 int or_tet10(const double X[30] /* 10x3 column-major */, double E, double nu, double Ke[900]);
 
 /* ---------- element kernels (global frame, as assembled) ---------- */

@@ -37,6 +37,8 @@ def lib():
         L.or_q4_macneal_parts.argtypes = [dp, dp, dp, C.c_int, dp, dp, dp, dp, dp]
         L.or_t3_local.argtypes = [dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, dp]
         L.or_tet10.argtypes = [dp, C.c_double, C.c_double, dp]
+        L.or_t3_shear_parts.argtypes = [dp, dp, dp, dp, dp, dp, dp, dp]
+        L.or_t3_guard.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
         L.or_q4_global.argtypes = [dp, dp, C.c_double, dp]
         L.or_t3_global.argtypes = [dp, dp, C.c_double, dp]
         L.or_assemble.argtypes = [C.c_int64, dp, dp, dp, C.c_int64, C.c_void_p, C.c_void_p,
@@ -93,6 +95,28 @@ def q4_macneal_parts(lc, Cb, Cs, rigid=False):
     return {"D": D.reshape(4, 12, order="F"), "Zb": Zb.reshape(4, 4, order="F"),
             "Zs": Zs.reshape(4, 4, order="F"), "Z": Z.reshape(4, 4, order="F"),
             "Kp": Kp.reshape(12, 12, order="F")}
+
+
+def t3_shear_parts(lc, Db, Ds):
+    """CTRIA3 MITC3 shear pieces: tying rows D3 (3x9), tying rigidity W (3x3),
+    residual bending flexibility Zb3 before and after the guard, and the rigidity
+    that enters Ks = D3' W D3.  ``applies`` is False when Zb3 is skipped."""
+    D3 = np.zeros(27); W = np.zeros(9); Zb = np.zeros(9); Zg = np.zeros(9); Wf = np.zeros(9)
+    st = lib().or_t3_shear_parts(_f(np.asarray(lc, float).reshape(3, 2).ravel()), _colmajor(Db), _colmajor(Ds),
+                                 D3, W, Zb, Zg, Wf)
+    if st < 0:
+        raise OracleError(-st)
+    return {"D3": D3.reshape(3, 9), "W": W.reshape(3, 3, order="F"), "Zb": Zb.reshape(3, 3, order="F"),
+            "Zb_guarded": Zg.reshape(3, 3, order="F"), "W_total": Wf.reshape(3, 3, order="F"), "applies": st == 1}
+
+
+def t3_guard(Zb, La, Lb, c, s, Dn):
+    """tria3_guard_bending_flexibility! on a copy of Zb: returns (Zb', changed)."""
+    Z = _colmajor(Zb)
+    st = lib().or_t3_guard(Z, La, Lb, c, s, Dn)
+    if st < 0:
+        raise OracleError(5)
+    return Z.reshape(3, 3, order="F"), bool(st)
 
 
 def t3_local(lc, Dm, Db, Ds, h=1.0, G_ref=1.0, k6rot=100.0):

@@ -9,6 +9,8 @@ and their recorded displacement vectors ``results/*.JU.JSON``:
 * validation/cases/MH_twisted_beam_{in_plane,out_of_plane} (warped CQUAD4, FORCE)
 * validation/cases/MH_pinched_cylinder{,_refined}, MH_hemispherical_shell
   (curved shells of warped CQUAD4, FORCE, symmetry SPC1)
+* validation/cases/MH_hemispherical_shell/convergence/{N08,N16} (the same
+  hemisphere on 8x8 and 16x16 CQUAD4 meshes)
 
 Scordelis-Lo (GRAV load) and the TACS-backend uCRM-9 vector are not on this
 path and are not extracted.  Output: tests/golden/reference_ju.json.
@@ -30,7 +32,12 @@ CASES = [
     ("MH_pinched_cylinder", "results/pinched_cylinder.JU.JSON", {1: 1}, 1),
     ("MH_pinched_cylinder_refined", "results/pinched_cylinder_refined.JU.JSON", {1: 1}, 1),
     ("MH_hemispherical_shell", "results/hemispherical_shell.JU.JSON", {1: 1}, 1),
+    ("MH_hemispherical_shell_N08", "results/MH_hemispherical_shell_N08.JU.JSON", {1: 1}, 1),
+    ("MH_hemispherical_shell_N16", "results/MH_hemispherical_shell_N16.JU.JSON", {1: 1}, 1),
 ]
+# deck directory under validation/cases when it is not the case name
+CASE_DIR = {"MH_hemispherical_shell_N08": "MH_hemispherical_shell/convergence/N08",
+            "MH_hemispherical_shell_N16": "MH_hemispherical_shell/convergence/N16"}
 
 
 def num(s):
@@ -75,7 +82,8 @@ def cards(path):
 
 
 def parse(case, loads_by_subcase, spc_sid):
-    deck = os.path.join(REF, case, "model.bdf")
+    cdir = CASE_DIR.get(case, case)
+    deck = os.path.join(REF, cdir, "model.bdf")
     grids, quads, pshell, mat1, spc, forces, pload4 = {}, [], {}, {}, [], {}, {}
     k6rot = 100.0
     for f in cards(deck):
@@ -114,7 +122,7 @@ def parse(case, loads_by_subcase, spc_sid):
                                                "dir": [num(g(5)) or 0.0, num(g(6)) or 0.0, num(g(7)) or 0.0]})
         elif name == "PLOAD4":
             pload4.setdefault(int(g(1)), []).append({"eid": int(g(2)), "p": num(g(3))})
-    ju = json.load(open(os.path.join(REF, case, CASE_JU[case])))
+    ju = json.load(open(os.path.join(REF, cdir, CASE_JU[case])))
     disp = ju["displacements"]
     n_sub = len(ju["solver_diagnostics"])
     per = len(disp) // n_sub
@@ -132,12 +140,12 @@ def parse(case, loads_by_subcase, spc_sid):
             "reference_free_dofs": diag["details"]["bc_partition"]["free_dofs"],
             "reference_relative_residual": diag["details"]["linear_solver"]["relative_residual"],
         })
-    return {"case": case, "k6rot": k6rot, "grids": {str(k): v for k, v in grids.items()},
+    return {"case": case, "dir": cdir, "k6rot": k6rot, "grids": {str(k): v for k, v in grids.items()},
             "cquad4": quads, "pshell": {str(k): v for k, v in pshell.items()},
             "mat1": {str(k): v for k, v in mat1.items()},
             "spc1": [{"grid": n, "comps": c} for (s, n, c) in spc if s == spc_sid],
             "subcases": subcases,
-            "provenance": f"validation/cases/{case}/model.bdf + {CASE_JU[case]}"}
+            "provenance": f"validation/cases/{cdir}/model.bdf + {CASE_JU[case]}"}
 
 
 CASE_JU = {c: j for c, j, _, _ in CASES}

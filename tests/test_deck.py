@@ -114,7 +114,7 @@ def test_reference_decks_match_their_fixtures(oracle, ci):
     """The reference's own deck (read as text) gives the model and loads the
     fixtures were made from (tests/golden/make_fixtures.py)."""
     case = golden_cases()[ci]
-    dk = D.read_deck(os.path.join(REF_CASES, case["case"], "model.bdf"))
+    dk = D.read_deck(os.path.join(REF_CASES, case.get("dir", case["case"]), "model.bdf"))
     for s in case["subcases"]:
         ref = golden_model(case, s, membrane_modes=6)
         m = dk.model(s["subcase"], membrane_modes=6)
