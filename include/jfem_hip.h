@@ -95,6 +95,29 @@ int jfem_q4_local_batch(jfem_ctx *ctx, int64_t n, const double *lc, const double
                         const double *k6rot, const double *drill_scale, const int32_t *flags,
                         double *Ke_out, int64_t *bad_element);
 
+/* Element-frame CTRIA3 kernel over n elements, the counterpart of
+ * stiffness_tria3_matrices(coords, Dm, Db, Ds, h, G_ref; bend_ratio, k6rot)
+ * (src/FEMKernels.jl:9811 -> stiffness_tria3_matrices_generic :9375-9809, default
+ * branches: CST membrane, constant-curvature bending, MITC3 shear with residual
+ * bending flexibility and its guard, lumped drilling).  lc: n x 6 (x1,y1,x2,y2,x3,y3
+ * per element); Dm, Db: n x 9, Ds: n x 4 (each element's matrix column-major);
+ * h, G_ref, bend_ratio, k6rot: n (h, G_ref and bend_ratio are accepted for the
+ * reference's signature; its default branch reads none of them).  Ke_out: n x 324
+ * (18 x 18 column-major per element; DOFs u,v,w,rx,ry,rz per node). */
+int jfem_t3_local_batch(jfem_ctx *ctx, int64_t n, const double *lc, const double *Dm,
+                        const double *Db, const double *Ds, const double *h, const double *G_ref,
+                        const double *bend_ratio, const double *k6rot, double *Ke_out,
+                        int64_t *bad_element);
+
+/* CTETRA10 kernel over n elements, the counterpart of stiffness_tetra10(coords, E, nu)
+ * (src/tetra10.jl:68-79): X: n x 30 (each element's 10 x 3 coordinates column-major,
+ * corners then edges (1,2),(2,3),(1,3),(1,4),(2,4),(3,4)), E, nu: n.  Ke_out: n x 900
+ * (30 x 30 column-major, Matrix(Symmetric(K))).  A non-positive Jacobian at a
+ * sample point returns JFEM_ERR_INVERTED (DomainError, src/tetra10.jl:31-35), bad E
+ * or nu JFEM_ERR_BAD_ARG (ArgumentError, :55-58); bad_element is the first such. */
+int jfem_tet10_batch(jfem_ctx *ctx, int64_t n, const double *X, const double *E, const double *nu,
+                     double *Ke_out, int64_t *bad_element);
+
 /* ---- mesh --------------------------------------------------------------- */
 /* Upload a model (coordinates SoA, connectivity, sections) to the device.
  * Replaces the per-call model Dict walk of assemble_stiffness

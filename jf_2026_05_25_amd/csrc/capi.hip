@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <numeric>
 
 #include "jfem_internal.h"
@@ -266,6 +267,64 @@ int jfem_q4_local_batch(jfem_ctx *ctx, int64_t n, const double *lc, const double
         return fail(ctx, (int)st[1], "element " + std::to_string(st[0]) + " rejected");
     }
     return JFEM_OK;
+}
+
+// element kernels on caller-given per-element inputs: upload, launch, download,
+// status check (the inputs are n x per_k doubles each)
+static int run_direct(jfem_ctx *ctx, int64_t n, int nd, std::initializer_list<std::pair<const double *, int>> in,
+                      double *Ke_out, int64_t *bad,
+                      const std::function<hipError_t(const std::vector<double *> &, double *)> &launch) {
+    if (bad) *bad = -1;
+    if (n <= 0) return JFEM_OK;
+    hipStream_t s = ctx->stream;
+    std::vector<double *> d;
+    double *d_Ke = nullptr;
+    auto release = [&]() {
+        for (double *p : d) (void)hipFree(p);
+        if (d_Ke) (void)hipFree(d_Ke);
+    };
+    hipError_t e = hipSuccess;
+    for (const auto &a : in) {
+        double *p = nullptr;
+        if (e == hipSuccess) e = upload(&p, a.first, (size_t)a.second * n, s);
+        d.push_back(p);
+    }
+    if (e == hipSuccess) e = hipMalloc((void **)&d_Ke, (size_t)nd * nd * n * sizeof(double));
+    if (e == hipSuccess) e = launch_reset_status(s, ctx->d_status);
+    if (e == hipSuccess) e = launch(d, d_Ke);
+    int64_t st[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(Ke_out, d_Ke, (size_t)nd * nd * n * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    release();
+    if (e != hipSuccess) return fail(ctx, e == hipErrorOutOfMemory ? JFEM_ERR_OOM : JFEM_ERR_HIP, hipGetErrorString(e));
+    if (st[1] != 0) {
+        if (bad) *bad = st[0];
+        return fail(ctx, (int)st[1], "element " + std::to_string(st[0]) + " rejected");
+    }
+    return JFEM_OK;
+}
+
+int jfem_t3_local_batch(jfem_ctx *ctx, int64_t n, const double *lc, const double *Dm, const double *Db,
+                        const double *Ds, const double *h, const double *G_ref, const double *bend_ratio,
+                        const double *k6rot, double *Ke_out, int64_t *bad) {
+    (void)h; (void)G_ref; (void)bend_ratio;      // not read by the default branch
+    return run_direct(ctx, n, 18, {{lc, 6}, {Dm, 9}, {Db, 9}, {Ds, 4}, {k6rot, 1}}, Ke_out, bad,
+                      [&](const std::vector<double *> &d, double *Ke) {
+                          return launch_t3_local(ctx->stream, n, d[0], d[1], d[2], d[3], d[4], Ke, ctx->d_status);
+                      });
+}
+
+int jfem_tet10_batch(jfem_ctx *ctx, int64_t n, const double *X, const double *E, const double *nu, double *Ke_out,
+                     int64_t *bad) {
+    if (bad) *bad = -1;
+    if (n <= 0) return JFEM_OK;
+    std::vector<double> prop(2 * (size_t)n);
+    for (int64_t e = 0; e < n; ++e) { prop[e] = E[e]; prop[n + e] = nu[e]; }
+    return run_direct(ctx, n, 30, {{X, 30}, {prop.data(), 2}}, Ke_out, bad,
+                      [&](const std::vector<double *> &d, double *Ke) {
+                          return launch_tet10_direct(ctx->stream, n, d[0], d[1], Ke, ctx->d_status);
+                      });
 }
 
 // ------------------------------------------------------------------------- pattern

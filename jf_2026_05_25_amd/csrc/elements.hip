@@ -1054,17 +1054,35 @@ __device__ inline double t3_bm(const T3Shared &S, int l, int m) {  // Bm (3x6) e
     return l == 0 ? 0.0 : (l == 1 ? S.cv[i] : S.bv[i]);
 }
 
+// Local = false: element e of a mesh (frame, section, T'KT), as assembled;
+// Local = true: the element-frame kernel on caller-given local coordinates and
+// section matrices (lc n x 6, Dm / Db n x 9, Ds n x 4 column-major per element,
+// k6 per element), frame = identity.
+struct T3LocalIn {
+    const double *lc, *Dm, *Db, *Ds, *k6;
+};
+
+template <bool Local>
 __global__ void __launch_bounds__(kWave * kElemsPerBlock)
-t3_global_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
-                 const double *__restrict__ z, const int32_t *__restrict__ conn,
-                 const double *__restrict__ prop, double k6rot, double *__restrict__ Ke,
-                 int64_t *status) {
+t3_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
+          const double *__restrict__ z, const int32_t *__restrict__ conn,
+          const double *__restrict__ prop, double k6rot, T3LocalIn loc, double *__restrict__ Ke,
+          int64_t *status) {
     __shared__ T3Shared sh[kElemsPerBlock];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t e = (int64_t)blockIdx.x * kElemsPerBlock + wave;
     const bool valid = e < n;
     T3Shared &S = sh[wave];
-    if (valid && lane == 0) {
+    if (Local && valid && lane == 0) {
+        for (int q = 0; q < 9; ++q) {
+            S.Dm[q] = loc.Dm[9 * e + q];
+            S.Db[q] = loc.Db[9 * e + q];
+            S.R[q] = (q % 4 == 0) ? 1.0 : 0.0;
+        }
+        S.status = 0;
+        t3_setup(S, loc.lc + 6 * e, loc.Ds + 4 * e, loc.k6[e]);
+    }
+    if (!Local && valid && lane == 0) {
         double p[9], pr[8];
         for (int k = 0; k < 3; ++k) {
             int nd = conn[k * n + e];
@@ -1182,9 +1200,14 @@ __device__ inline double tet_b(const TetShared &S, int q, int l, int d) {  // B 
     }
 }
 
+// Direct = false: element e of a mesh (connectivity into the grid-point
+// coordinates, section record (E, nu) column-major n x 2); Direct = true: the
+// caller's coordinates Xd (n x 30: per element the 10 x 3 matrix, column-major)
+// and E, nu given as prop[e], prop[n + e].
+template <bool Direct>
 __global__ void __launch_bounds__(kWave * kElemsPerBlock)
 tet10_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
-             const double *__restrict__ z, const int32_t *__restrict__ conn,
+             const double *__restrict__ z, const int32_t *__restrict__ conn, const double *__restrict__ Xd,
              const double *__restrict__ prop, double *__restrict__ Ke, int64_t *status) {
     __shared__ TetShared sh[kElemsPerBlock];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -1193,8 +1216,13 @@ tet10_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__
     TetShared &S = sh[wave];
     if (valid) {
         if (lane < 10) {
-            int nd = conn[lane * n + e];
-            S.X[lane] = x[nd]; S.X[lane + 10] = y[nd]; S.X[lane + 20] = z[nd];
+            if (Direct) {
+                const double *Xe = Xd + 30 * e;
+                S.X[lane] = Xe[lane]; S.X[lane + 10] = Xe[lane + 10]; S.X[lane + 20] = Xe[lane + 20];
+            } else {
+                int nd = conn[lane * n + e];
+                S.X[lane] = x[nd]; S.X[lane + 10] = y[nd]; S.X[lane + 20] = z[nd];
+            }
         }
         if (lane == 0) {
             S.status = 0;
@@ -1464,16 +1492,34 @@ hipError_t launch_q4_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64
 hipError_t launch_t3_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status) {
     if (m->nt == 0) return hipSuccess;
     dim3 grid((unsigned)((m->nt + kElemsPerBlock - 1) / kElemsPerBlock));
-    hipLaunchKernelGGL(t3_global_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, m->nt, m->d_x, m->d_y, m->d_z,
-                       m->d_t, m->d_tp, m->k6rot, Ke, status);
+    hipLaunchKernelGGL(t3_kernel<false>, grid, dim3(kWave * kElemsPerBlock), 0, s, m->nt, m->d_x, m->d_y, m->d_z,
+                       m->d_t, m->d_tp, m->k6rot, T3LocalIn{}, Ke, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_t3_local(hipStream_t s, int64_t n, const double *lc, const double *Dm, const double *Db,
+                           const double *Ds, const double *k6, double *Ke, int64_t *status) {
+    if (n == 0) return hipSuccess;
+    dim3 grid((unsigned)((n + kElemsPerBlock - 1) / kElemsPerBlock));
+    hipLaunchKernelGGL(t3_kernel<true>, grid, dim3(kWave * kElemsPerBlock), 0, s, n, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, 0.0, T3LocalIn{lc, Dm, Db, Ds, k6}, Ke, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_tet10_direct(hipStream_t s, int64_t n, const double *X, const double *prop, double *Ke,
+                               int64_t *status) {
+    if (n == 0) return hipSuccess;
+    dim3 grid((unsigned)((n + kElemsPerBlock - 1) / kElemsPerBlock));
+    hipLaunchKernelGGL(tet10_kernel<true>, grid, dim3(kWave * kElemsPerBlock), 0, s, n, nullptr, nullptr, nullptr,
+                       nullptr, X, prop, Ke, status);
     return hipGetLastError();
 }
 
 hipError_t launch_tet10(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status) {
     if (m->ntet == 0) return hipSuccess;
     dim3 grid((unsigned)((m->ntet + kElemsPerBlock - 1) / kElemsPerBlock));
-    hipLaunchKernelGGL(tet10_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, m->ntet, m->d_x, m->d_y, m->d_z,
-                       m->d_tet, m->d_tetp, Ke, status);
+    hipLaunchKernelGGL(tet10_kernel<false>, grid, dim3(kWave * kElemsPerBlock), 0, s, m->ntet, m->d_x, m->d_y, m->d_z,
+                       m->d_tet, nullptr, m->d_tetp, Ke, status);
     return hipGetLastError();
 }
 

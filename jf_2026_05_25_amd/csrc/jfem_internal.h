@@ -92,6 +92,10 @@ hipError_t launch_q4_local(hipStream_t s, int64_t n, const double *lc, const dou
                            const double *Cb, const double *Cs, const double *h,
                            const double *k6, const double *ds, const int32_t *flags,
                            double *Ke, int64_t *status);
+hipError_t launch_t3_local(hipStream_t s, int64_t n, const double *lc, const double *Dm, const double *Db,
+                           const double *Ds, const double *k6, double *Ke, int64_t *status);
+hipError_t launch_tet10_direct(hipStream_t s, int64_t n, const double *X, const double *prop, double *Ke,
+                               int64_t *status);
 hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *Ke, double *val);
 hipError_t launch_bc_screen(hipStream_t s, const jfem_pattern *P, const double *val, const uint8_t *spc, double thresh,
                             uint8_t *fixed0, uint8_t *flag);

@@ -120,6 +120,56 @@ def stiffness_quad4_matrices(coords, Cm, Cb, Cs, h, E_ref=None, *, k6rot=100.0, 
     return Ke[0] if single else Ke
 
 
+def _per_element(a, n, shape):
+    """(n,) + shape array (or one shape broadcast to n) as n column-major blocks."""
+    a = np.asarray(a, float)
+    a = np.broadcast_to(a, (n,) + shape) if a.shape == shape else a.reshape((n,) + shape)
+    return np.ascontiguousarray(np.transpose(a, (0,) + tuple(range(len(shape), 0, -1))).reshape(n, -1))
+
+
+def stiffness_tria3_matrices(coords, Dm, Db, Ds, h=1.0, G_ref=1.0, *, bend_ratio=1.0, k6rot=100.0,
+                             ctx: Context | None = None):
+    """Element-frame CTRIA3 stiffness (src/FEMKernels.jl:9811), batched.
+
+    ``coords``: (3,2) or (n,3,2); ``Dm``/``Db``: (3,3) or (n,3,3); ``Ds``: (2,2) or
+    (n,2,2); ``h``, ``G_ref``, ``bend_ratio``, ``k6rot``: scalar or (n,).  Returns
+    (18,18) or (n,18,18)."""
+    ctx = ctx or default_context()
+    coords = np.asarray(coords, float)
+    single = coords.ndim == 2
+    coords = coords.reshape(-1, 3, 2)
+    n = coords.shape[0]
+    lc = np.ascontiguousarray(coords.reshape(n, 6))
+    Dm_, Db_, Ds_ = _per_element(Dm, n, (3, 3)), _per_element(Db, n, (3, 3)), _per_element(Ds, n, (2, 2))
+    vec = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, float), (n,))) for v in (h, G_ref, bend_ratio, k6rot)]
+    Ke = np.zeros(n * 324)
+    bad = C.c_int64(-1)
+    check(ctx.h, ctx.L.jfem_t3_local_batch(ctx.h, n, ptr(lc), ptr(Dm_), ptr(Db_), ptr(Ds_), *[ptr(v) for v in vec],
+                                           ptr(Ke), C.byref(bad)))
+    Ke = Ke.reshape(n, 18, 18).transpose(0, 2, 1)
+    return Ke[0] if single else Ke
+
+
+def stiffness_tetra10(coords, E, nu, ctx: Context | None = None):
+    """CTETRA10 stiffness (src/tetra10.jl:68), batched: ``coords`` (10,3) or
+    (n,10,3), ``E``/``nu`` scalar or (n,).  Returns (30,30) or (n,30,30).  Raises
+    JfemError(INVERTED) on a non-positive Jacobian like the reference's
+    DomainError, JfemError(BAD_ARG) on bad E / nu."""
+    ctx = ctx or default_context()
+    X = np.asarray(coords, float)
+    single = X.ndim == 2
+    X = X.reshape(-1, 10, 3)
+    n = X.shape[0]
+    Xc = _per_element(X, n, (10, 3))
+    E_ = np.ascontiguousarray(np.broadcast_to(np.asarray(E, float), (n,)))
+    nu_ = np.ascontiguousarray(np.broadcast_to(np.asarray(nu, float), (n,)))
+    Ke = np.zeros(n * 900)
+    bad = C.c_int64(-1)
+    check(ctx.h, ctx.L.jfem_tet10_batch(ctx.h, n, ptr(Xc), ptr(E_), ptr(nu_), ptr(Ke), C.byref(bad)))
+    Ke = Ke.reshape(n, 30, 30).transpose(0, 2, 1)
+    return Ke[0] if single else Ke
+
+
 class DeviceMesh:
     """A model's grid points, connectivity and sections resident on the device."""
 
