@@ -111,7 +111,9 @@ def test_tet10_batch_matches_oracle(ctx, oracle):
     worst = max(_rel(K[e], oracle.tet10(X[e], E[e], nu[e])) for e in range(n))
     assert worst < TOL_K, worst
     bad = X.copy()
-    bad[7, [1, 2]] = bad[7, [2, 1]]                    # swapped corners: negative Jacobian
+    bad[7, :, 0] *= -1.0                               # mirrored element: negative Jacobian everywhere
+    with pytest.raises(oracle.OracleError):
+        oracle.tet10(bad[7], E[7], nu[7])
     with pytest.raises(_lib.JfemError) as ei:
         solver.stiffness_tetra10(bad, E, nu, ctx=ctx)
     assert ei.value.code == 3 and "element 7" in str(ei.value)
