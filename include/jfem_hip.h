@@ -236,6 +236,19 @@ int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, in
                    const double *b_dev, double *x_dev, double tol, int max_refine,
                    int32_t *iters, double *relres, double *x_lo_dev, double *relres_out);
 
+/* jfem_solve_dev that allocates nothing: the same solve (same kernels, same
+ * results bit for bit) with its scratch in a static device array of the library
+ * (128 MiB per GPU, held until the process ends) instead of the context's
+ * scratch, which jfem_solve_dev grows with hipMalloc.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  A solve that needs more scratch than
+ * the array holds (jfem_solve_scratch_bytes) returns JFEM_ERR_BAD_ARG.  Every
+ * call on one GPU uses the same array: calls that overlap on one device, from any
+ * context, race. */
+int jfem_solve_dev_static(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs,
+                          const double *b_dev, double *x_dev, double tol, int max_refine,
+                          int32_t *iters, double *relres, double *x_lo_dev, double *relres_out);
+
 /* ---- load vectors ------------------------------------------------------- */
 /* A load set: the FORCE / MOMENT cards on grid points and the uniform PLOAD4 cards
  * on shell elements of one load SID, in card order, basic coordinates (CID 0).

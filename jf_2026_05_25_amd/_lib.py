@@ -30,6 +30,7 @@ EXPORTS = [
     "jfem_matrix_free", "jfem_bc_partition",
     "jfem_symbolic_create", "jfem_symbolic_free", "jfem_factor_create", "jfem_refactor",
     "jfem_factor_small_pivots", "jfem_factor_pivot_ratios", "jfem_symbolic_order", "jfem_factor_free", "jfem_solve", "jfem_solve_dev",
+    "jfem_solve_dev_static",
     "jfem_dev_alloc", "jfem_dev_free", "jfem_memcpy_h2d", "jfem_memcpy_d2h",
     "jfem_factor_device_bytes", "jfem_matrix_device_bytes", "jfem_mesh_device_bytes", "jfem_pattern_device_bytes",
     "jfem_symbolic_device_bytes", "jfem_ctx_device_bytes", "jfem_solve_scratch_bytes", "jfem_selftest_update",
@@ -100,6 +101,7 @@ def load():
         "jfem_factor_free": ([vp], C.c_int),
         "jfem_solve": ([vp, vp, vp, C.c_int, vp, vp, dbl, C.c_int, vp, vp, vp, vp], C.c_int),
         "jfem_solve_dev": ([vp, vp, vp, C.c_int, vp, vp, dbl, C.c_int, vp, vp, vp, vp], C.c_int),
+        "jfem_solve_dev_static": ([vp, vp, vp, C.c_int, vp, vp, dbl, C.c_int, vp, vp, vp, vp], C.c_int),
         "jfem_dev_alloc": ([vp, i64, P(vp)], C.c_int),
         "jfem_dev_free": ([vp, vp], C.c_int),
         "jfem_memcpy_h2d": ([vp, vp, vp, i64], C.c_int),

@@ -13,6 +13,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <utility>
 
 #include "jfem_internal.h"
@@ -2250,7 +2251,8 @@ struct SolveLayout {
 // solve x = F^{-1} b.
 template <int NR, class T>
 int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, const double *b_dev, double *x_dev,
-                 double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev, double *relres_out) {
+                 double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev, double *relres_out,
+                 char *static_base) {
     const jfem_symbolic *S = F->S;
     const SymbolicHost &H = S->H;
     const jfem_pattern *P = K->P;
@@ -2271,8 +2273,11 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
     // exact residual then misses tol, it simply runs on.
     const double eta = 1e-4, stop = 0.9;
     const SolveLayout lay(H, ndof, NR, sizeof(T));
-    JFEM_CHECK_HIP(ctx, ensure_scratch(ctx, lay.bytes));
-    char *base = (char *)ctx->scratch;
+    char *base = static_base;
+    if (!base) {
+        JFEM_CHECK_HIP(ctx, ensure_scratch(ctx, lay.bytes));
+        base = (char *)ctx->scratch;
+    }
     T *Y = (T *)(base + lay.oY), *W = (T *)(base + lay.oW), *Zs = (T *)(base + lay.oZs), *Xs = (T *)(base + lay.oXs);
     int32_t *cnt = (int32_t *)(base + lay.oCnt);
     double *Xh = (double *)(base + lay.oXh), *Xl = (double *)(base + lay.oXl), *R = (double *)(base + lay.oR);
@@ -2425,22 +2430,87 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
 
 }  // namespace
 
+// ---------------------------------------------------------------- allocation-free solve
+// Solve scratch of jfem_solve_dev_static: one array per GPU in device memory of
+// the code object, held until the process ends.  In host code `scratch` names a
+// host-side stand-in, never device memory: its device address is looked up once
+// per device (hipGetSymbolAddress with that device current) and only that address
+// reaches kernels.  Every call on a device uses the same array and leaves its
+// contents behind; each solve clears what it reads first (pending marks, tickets,
+// the abort flag, the iterate), and calls that overlap on one device race.
+constexpr size_t kStaticScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch_ptr(jfem_ctx *ctx, char **out) {
+    const int dev = ctx->device;
+    if (dev < 0 || dev >= kMaxDevices) return fail(ctx, JFEM_ERR_BAD_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        int prev = 0;
+        JFEM_CHECK_HIP(ctx, hipGetDevice(&prev));
+        JFEM_CHECK_HIP(ctx, hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(prev);
+        if (e != hipSuccess || !p)
+            return fail(ctx, JFEM_ERR_HIP, std::string("static solve scratch: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return JFEM_OK;
+}
+}  // namespace
+
 extern "C" {
+
+static int solve_dev_impl(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_dev,
+                          double *x_dev, double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev,
+                          double *relres_out, char *static_base) {
+    int rc = JFEM_OK;
+#define JFEM_SOLVE_CALL(N)                                                                                      \
+    rc = F->precision == 64 ? solve_refine<N, double>(ctx, F, K, b_dev, x_dev, tol, max_refine, iters, relres,   \
+                                                      xlo_dev, relres_out, static_base)                         \
+                            : solve_refine<N, float>(ctx, F, K, b_dev, x_dev, tol, max_refine, iters, relres,    \
+                                                     xlo_dev, relres_out, static_base)
+    JFEM_NR_SWITCH(nrhs, JFEM_SOLVE_CALL)
+#undef JFEM_SOLVE_CALL
+    return rc;
+}
+
+static int check_solve_args(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, int max_refine) {
+    if (K->P != F->S->P) return fail(ctx, JFEM_ERR_BAD_ARG, "matrix and factor use different patterns");
+    if (nrhs < 1 || nrhs > 16) return fail(ctx, JFEM_ERR_BAD_ARG, "nrhs must be in 1..16");
+    if (max_refine < 0) return fail(ctx, JFEM_ERR_BAD_ARG, "max_refine must be >= 0");
+    return JFEM_OK;
+}
 
 int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_dev,
                    double *x_dev, double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev,
                    double *relres_out) {
-    if (K->P != F->S->P) return fail(ctx, JFEM_ERR_BAD_ARG, "matrix and factor use different patterns");
-    if (nrhs < 1 || nrhs > 16) return fail(ctx, JFEM_ERR_BAD_ARG, "nrhs must be in 1..16");
-    if (max_refine < 0) return fail(ctx, JFEM_ERR_BAD_ARG, "max_refine must be >= 0");
-    int rc = JFEM_OK;
-#define JFEM_SOLVE_CALL(N)                                                                                     \
-    rc = F->precision == 64                                                                                    \
-             ? solve_refine<N, double>(ctx, F, K, b_dev, x_dev, tol, max_refine, iters, relres, xlo_dev, relres_out) \
-             : solve_refine<N, float>(ctx, F, K, b_dev, x_dev, tol, max_refine, iters, relres, xlo_dev, relres_out)
-    JFEM_NR_SWITCH(nrhs, JFEM_SOLVE_CALL)
-#undef JFEM_SOLVE_CALL
-    return rc;
+    const int rc = check_solve_args(ctx, F, K, nrhs, max_refine);
+    if (rc != JFEM_OK) return rc;
+    return solve_dev_impl(ctx, F, K, nrhs, b_dev, x_dev, tol, max_refine, iters, relres, xlo_dev, relres_out, nullptr);
+}
+
+int jfem_solve_dev_static(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_dev,
+                          double *x_dev, double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev,
+                          double *relres_out) {
+    int rc = check_solve_args(ctx, F, K, nrhs, max_refine);
+    if (rc != JFEM_OK) return rc;
+    const int64_t need = jfem_solve_scratch_bytes(F->S, K->P->ndof, nrhs, F->precision);
+    if (need > (int64_t)kStaticScratchBytes)
+        return fail(ctx, JFEM_ERR_BAD_ARG,
+                    "solve needs " + std::to_string(need) + " bytes of scratch; the static scratch holds " +
+                        std::to_string((long long)kStaticScratchBytes));
+    char *base = nullptr;
+    rc = static_scratch_ptr(ctx, &base);
+    if (rc != JFEM_OK) return rc;
+    return solve_dev_impl(ctx, F, K, nrhs, b_dev, x_dev, tol, max_refine, iters, relres, xlo_dev, relres_out, base);
 }
 
 // Device self-test of the matrix-core trailing update against its scalar form on

@@ -497,14 +497,17 @@ class Factor:
         return out
 
     def solve_device(self, K: DeviceMatrix, b_dev: "DeviceBuffer", x_dev: "DeviceBuffer", nrhs: int,
-                     tol: float = 1e-10, max_refine: int = 100, ctx: Context | None = None):
+                     tol: float = 1e-10, max_refine: int = 100, ctx: Context | None = None,
+                     static_scratch: bool = False):
         """As ``solve`` with right-hand sides and solution in device memory
         (ndof x nrhs column-major FP64, ``DeviceBuffer``): nothing crosses PCIe but
-        the per-RHS residuals.  Returns ``(relres, iters, ok)``."""
+        the per-RHS residuals.  ``static_scratch`` selects jfem_solve_dev_static,
+        which allocates nothing (its scratch is the library's static device array;
+        calls that overlap on one device race).  Returns ``(relres, iters, ok)``."""
         ctx = ctx or self.ctx
         iters = np.zeros(nrhs, np.int32); rr = np.zeros(nrhs)
-        code = ctx.L.jfem_solve_dev(ctx.h, self.h, K.h, nrhs, b_dev.ptr, x_dev.ptr, tol, max_refine, ptr(iters),
-                                    ptr(rr), None, None)
+        fn = ctx.L.jfem_solve_dev_static if static_scratch else ctx.L.jfem_solve_dev
+        code = fn(ctx.h, self.h, K.h, nrhs, b_dev.ptr, x_dev.ptr, tol, max_refine, ptr(iters), ptr(rr), None, None)
         if code not in (0, 8):
             check(ctx.h, code)
         return rr, iters, code == 0
