@@ -390,16 +390,39 @@ int jfem_pattern_create(jfem_ctx *ctx, const jfem_mesh *m, jfem_pattern **out, i
         const int32_t *it = std::lower_bound(b, e, (int32_t)c);
         return (it != e && *it == c) ? (int64_t)(it - P->col.data()) : -1;
     };
-    // slot of every triplet, in triplet order
+    // The columns of row 6a+i are the node blocks of a's neighbours in order, each
+    // 6 wide (shell coupling) or 3 wide (solid coupling, translation rows only):
+    // the entry (6a+i, 6b+j) sits at rowptr[6a+i] + (width of the blocks before
+    // b's) + j.  Those prefix widths per adjacency entry, for translation and for
+    // rotation rows:
+    std::vector<int32_t> off_t(nadj.size()), off_r(nadj.size());
+    for (int64_t a = 0; a < nn; ++a) {
+        int32_t wt = 0, wr = 0;
+        for (int64_t p = nptr[a]; p < nptr[a + 1]; ++p) {
+            off_t[p] = wt; off_r[p] = wr;
+            wt += ntype[p] == 2 ? 6 : 3;
+            wr += ntype[p] == 2 ? 6 : 0;
+        }
+    }
+    auto adj_pos = [&](int64_t a, int32_t b) -> int64_t {   // b's entry in a's sorted adjacency
+        const int32_t *lo = nadj.data() + nptr[a], *hi = nadj.data() + nptr[a + 1];
+        return (int64_t)(std::lower_bound(lo, hi, b) - nadj.data());
+    };
+    // slot of every triplet, in triplet order: one adjacency search per node pair
+    // of an element, then the DOF offsets within the node blocks
     std::vector<int32_t> slot(ntrip);
     int64_t t = 0;
     auto emit = [&](const std::vector<int32_t> &conn, int64_t n, int k, int dpn, int64_t e) {
-        int ndl = k * dpn;
+        int64_t pos[10][10];
+        for (int bb = 0; bb < k; ++bb)
+            for (int aa = 0; aa < k; ++aa) pos[aa][bb] = adj_pos(conn[aa * n + e], conn[bb * n + e]);
+        const int ndl = k * dpn;
         for (int cc = 0; cc < ndl; ++cc)
             for (int rr = 0; rr < ndl; ++rr) {
-                int64_t r = 6 * (int64_t)conn[(rr / dpn) * n + e] + rr % dpn;
-                int64_t c = 6 * (int64_t)conn[(cc / dpn) * n + e] + cc % dpn;
-                slot[t++] = (int32_t)find(r, c);
+                const int aa = rr / dpn, i = rr % dpn, bb = cc / dpn, j = cc % dpn;
+                const int64_t p = pos[aa][bb];
+                const int64_t r = 6 * (int64_t)conn[aa * n + e] + i;
+                slot[t++] = (int32_t)(P->rowptr[r] + (i < 3 ? off_t[p] : off_r[p]) + j);
             }
     };
     for (int64_t e = 0; e < m->nq; ++e) emit(m->q, m->nq, 4, 6, e);

@@ -536,21 +536,59 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         rel += nu;
     }
     S.relofs[nsn] = rel;
-    // K -> front scatter map: entries (g, c) with c >= g (upper triangle, original numbering)
-    for (int64_t g = 0; g < ndof; ++g) {
-        if (fixed[g]) continue;
-        for (int64_t p = rowptr[g]; p < rowptr[g + 1]; ++p) {
-            int64_t c = col[p];
-            if (c < g || fixed[c]) continue;
-            int32_t a = S.iperm[g], b = S.iperm[c];
-            int32_t hi = std::max(a, b), lo2 = std::min(a, b);
-            int32_t s = snode_of_pos[lo2];
-            const int32_t *rr = S.rows.data() + S.rofs[s];
-            const int32_t *it = std::lower_bound(rr, rr + S.m[s], hi);
-            if (it == rr + S.m[s] || *it != hi) { err = "matrix entry outside the symbolic structure"; return 1; }
-            int64_t lr = it - rr, lc = lo2 - S.col0[s];
-            S.amap_src.push_back((int32_t)p);
-            S.amap_dst.push_back(S.lofs[s] + lr + S.m[s] * lc);
+    // K -> front scatter map: entries (g, c) with c >= g (upper triangle, original
+    // numbering), in K order.  The entries are grouped by the supernode owning
+    // their column (a stable counting sort); per supernode a position map
+    // row -> local row (stamped with the supernode, never cleared) gives each
+    // entry's front row in O(1).
+    {
+        // pass 1: kept entries per supernode (and in total)
+        std::vector<int64_t> sptr(nsn + 1, 0);
+        int64_t ne = 0;
+        for (int64_t g = 0; g < ndof; ++g) {
+            if (fixed[g]) continue;
+            const int32_t a = S.iperm[g];
+            for (int64_t p = rowptr[g]; p < rowptr[g + 1]; ++p) {
+                const int64_t c = col[p];
+                if (c < g || fixed[c]) continue;
+                sptr[snode_of_pos[std::min(a, S.iperm[c])] + 1]++;
+                ++ne;
+            }
+        }
+        for (int64_t q = 0; q < nsn; ++q) sptr[q + 1] += sptr[q];
+        // pass 2: output slot k (K order) of each entry, grouped by supernode
+        S.amap_src.resize(ne);
+        S.amap_dst.resize(ne);
+        std::vector<int64_t> order(ne);
+        std::vector<int32_t> e_lo(ne), e_hi(ne);
+        {
+            std::vector<int64_t> fill(sptr.begin(), sptr.end() - 1);
+            int64_t k = 0;
+            for (int64_t g = 0; g < ndof; ++g) {
+                if (fixed[g]) continue;
+                const int32_t a = S.iperm[g];
+                for (int64_t p = rowptr[g]; p < rowptr[g + 1]; ++p) {
+                    const int64_t c = col[p];
+                    if (c < g || fixed[c]) continue;
+                    const int32_t b = S.iperm[c], lo = std::min(a, b);
+                    S.amap_src[k] = (int32_t)p;
+                    e_lo[k] = lo;
+                    e_hi[k] = std::max(a, b);
+                    order[fill[snode_of_pos[lo]]++] = k;
+                    ++k;
+                }
+            }
+        }
+        std::vector<int32_t> stamp(S.n, -1), local(S.n, 0);
+        for (int64_t q = 0; q < nsn; ++q) {
+            const int32_t *rr = S.rows.data() + S.rofs[q];
+            for (int64_t i = 0; i < S.m[q]; ++i) { stamp[rr[i]] = (int32_t)q; local[rr[i]] = (int32_t)i; }
+            for (int64_t t = sptr[q]; t < sptr[q + 1]; ++t) {
+                const int64_t k = order[t];
+                const int32_t hi = e_hi[k];
+                if (stamp[hi] != q) { err = "matrix entry outside the symbolic structure"; return 1; }
+                S.amap_dst[k] = S.lofs[q] + local[hi] + S.m[q] * (int64_t)(e_lo[k] - S.col0[q]);
+            }
         }
     }
     // levels (height in the supernode tree)

@@ -154,6 +154,14 @@ class SolveServer:
         self.topo_locks = {}
         self.evictions = 0
 
+    def clear_caches(self):
+        """Evict every cached factor and topology (entries in use are freed by
+        their last holder)."""
+        with self.lock:
+            for d in (self.factors, self.topo):
+                for k in list(d):
+                    self._evict(d, k)
+
     def close(self):
         with self.lock:
             for d in (self.factors, self.topo):
@@ -330,14 +338,20 @@ class SolveServer:
 
     def _process_new(self, ctx, req, t0, t_admit, fkey, tim) -> Result:
         m = req.model
+        t0_ = time.perf_counter()
         topo = self._topology(ctx, m)
+        tim["topology_s"] = time.perf_counter() - t0_
         keep = False
         mesh_r = K = F = S_own = None
         try:
             # per-request mesh with this request's sections (same topology)
+            ta = time.perf_counter()
             mesh_r = self.B.DeviceMesh(m, ctx)
             K = self.B.DeviceMatrix(mesh_r, topo.objs["pattern"], ctx)
+            tim["assemble_s"] = time.perf_counter() - ta
+            ta = time.perf_counter()
             fixed = self.B.bc_partition(K, m.spc)
+            tim["bc_s"] = time.perf_counter() - ta
             tb = time.perf_counter()
             S = self._symbolic(topo, K, fixed, ctx)
             tim["symbolic_s"] = time.perf_counter() - tb
@@ -426,6 +440,17 @@ class SolveServer:
         for th in threads:
             th.join()
         return sorted(results, key=lambda r: r.rid)
+
+
+def stage_times(results: list[Result]) -> dict:
+    """Mean seconds per request of each timed stage (requests that ran it)."""
+    keys = ("topology_s", "assemble_s", "bc_s", "symbolic_s", "factor_s", "solve_s")
+    out = {}
+    for k in keys:
+        v = [r.timings[k] for r in results if k in r.timings]
+        if v:
+            out[k] = float(np.mean(v))
+    return out
 
 
 def summarize(results: list[Result]) -> dict:

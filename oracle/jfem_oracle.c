@@ -14,6 +14,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <stdint.h>
 
 #define M(A, i, j, ld) (A)[(i) + (ld) * (j)]
 static const double EPS = 2.220446049250313e-16; /* eps(Float64) */
@@ -960,11 +961,16 @@ int or_assemble(int64_t n_nodes, const double *x, const double *y, const double 
                 int64_t ntet, const int32_t *ec, const double *ep,
                 double k6rot, int64_t *out_nnz, int64_t **row_ptr, int32_t **col,
                 double **val, int64_t *bad_elem) {
-    int64_t ntrip = nq * 576 + nt * 324 + ntet * 900;
+    /* triplets in element order (CQUAD4s, CTRIA3s, CTETRA10s), column-major per
+     * element: element e owns a fixed slot range, so the element loops run in
+     * parallel (OpenMP) and the triplet order does not depend on the schedule */
+    const int64_t ntrip = nq * 576 + nt * 324 + ntet * 900;
+    const int64_t t3_base = nq * 576, tet_base = t3_base + nt * 324;
     trip_t *T = (trip_t *)malloc(sizeof(trip_t) * (ntrip > 0 ? ntrip : 1));
     if (!T) return OR_ERR_OOM;
-    int64_t seq = 0;
     *bad_elem = -1;
+    int64_t bad = INT64_MAX; int bad_st = OR_OK;
+#pragma omp parallel for schedule(static) reduction(min : bad)
     for (int64_t e = 0; e < nq; ++e) {
         double p[12], prop[8], Ke[576]; int nd[4];
         for (int k = 0; k < 4; ++k) {
@@ -972,14 +978,22 @@ int or_assemble(int64_t n_nodes, const double *x, const double *y, const double 
             p[3 * k] = x[nd[k]]; p[3 * k + 1] = y[nd[k]]; p[3 * k + 2] = z[nd[k]];
         }
         for (int c = 0; c < 8; ++c) prop[c] = qp[c * nq + e];
-        int st = or_q4_global(p, prop, k6rot, Ke);
-        if (st != OR_OK) { free(T); *bad_elem = e; return st; }
+        if (or_q4_global(p, prop, k6rot, Ke) != OR_OK) { if (e < bad) bad = e; continue; }
+        int64_t seq = e * 576;
         for (int cc = 0; cc < 24; ++cc)
-            for (int rr = 0; rr < 24; ++rr) {
+            for (int rr = 0; rr < 24; ++rr, ++seq) {
                 T[seq].row = 6 * (int64_t)nd[rr / 6] + rr % 6; T[seq].col = 6 * nd[cc / 6] + cc % 6;
-                T[seq].v = Ke[rr + 24 * cc]; T[seq].seq = seq; ++seq;
+                T[seq].v = Ke[rr + 24 * cc]; T[seq].seq = seq;
             }
     }
+    if (bad != INT64_MAX) {   /* the first failing element, with its status (serial re-run) */
+        double p[12], prop[8], Ke[576];
+        for (int k = 0; k < 4; ++k) { int nd = qc[k * nq + bad]; p[3 * k] = x[nd]; p[3 * k + 1] = y[nd]; p[3 * k + 2] = z[nd]; }
+        for (int c = 0; c < 8; ++c) prop[c] = qp[c * nq + bad];
+        bad_st = or_q4_global(p, prop, k6rot, Ke);
+        free(T); *bad_elem = bad; return bad_st;
+    }
+#pragma omp parallel for schedule(static) reduction(min : bad)
     for (int64_t e = 0; e < nt; ++e) {
         double p[9], prop[8], Ke[324]; int nd[3];
         for (int k = 0; k < 3; ++k) {
@@ -987,47 +1001,85 @@ int or_assemble(int64_t n_nodes, const double *x, const double *y, const double 
             p[3 * k] = x[nd[k]]; p[3 * k + 1] = y[nd[k]]; p[3 * k + 2] = z[nd[k]];
         }
         for (int c = 0; c < 8; ++c) prop[c] = tp[c * nt + e];
-        int st = or_t3_global(p, prop, k6rot, Ke);
-        if (st != OR_OK) { free(T); *bad_elem = nq + e; return st; }
+        if (or_t3_global(p, prop, k6rot, Ke) != OR_OK) { if (e < bad) bad = e; continue; }
+        int64_t seq = t3_base + e * 324;
         for (int cc = 0; cc < 18; ++cc)
-            for (int rr = 0; rr < 18; ++rr) {
+            for (int rr = 0; rr < 18; ++rr, ++seq) {
                 T[seq].row = 6 * (int64_t)nd[rr / 6] + rr % 6; T[seq].col = 6 * nd[cc / 6] + cc % 6;
-                T[seq].v = Ke[rr + 18 * cc]; T[seq].seq = seq; ++seq;
+                T[seq].v = Ke[rr + 18 * cc]; T[seq].seq = seq;
             }
     }
+    if (bad != INT64_MAX) {
+        double p[9], prop[8], Ke[324];
+        for (int k = 0; k < 3; ++k) { int nd = tc[k * nt + bad]; p[3 * k] = x[nd]; p[3 * k + 1] = y[nd]; p[3 * k + 2] = z[nd]; }
+        for (int c = 0; c < 8; ++c) prop[c] = tp[c * nt + bad];
+        bad_st = or_t3_global(p, prop, k6rot, Ke);
+        free(T); *bad_elem = nq + bad; return bad_st;
+    }
+#pragma omp parallel for schedule(static) reduction(min : bad)
     for (int64_t e = 0; e < ntet; ++e) {
         double X[30], Ke[900]; int nd[10];
         for (int k = 0; k < 10; ++k) {
             nd[k] = ec[k * ntet + e];
             X[k] = x[nd[k]]; X[k + 10] = y[nd[k]]; X[k + 20] = z[nd[k]];
         }
-        int st = or_tet10(X, ep[e], ep[ntet + e], Ke);
-        if (st != OR_OK) { free(T); *bad_elem = nq + nt + e; return st; }
+        if (or_tet10(X, ep[e], ep[ntet + e], Ke) != OR_OK) { if (e < bad) bad = e; continue; }
+        int64_t seq = tet_base + e * 900;
         for (int cc = 0; cc < 30; ++cc)
-            for (int rr = 0; rr < 30; ++rr) {
+            for (int rr = 0; rr < 30; ++rr, ++seq) {
                 T[seq].row = 6 * (int64_t)nd[rr / 3] + rr % 3; T[seq].col = 6 * nd[cc / 3] + cc % 3;
-                T[seq].v = Ke[rr + 30 * cc]; T[seq].seq = seq; ++seq;
+                T[seq].v = Ke[rr + 30 * cc]; T[seq].seq = seq;
             }
     }
-    qsort(T, (size_t)ntrip, sizeof(trip_t), trip_cmp);
-    int64_t ndof = 6 * n_nodes, nnz = 0;
-    for (int64_t i = 0; i < ntrip; ++i)
-        if (i == 0 || T[i].row != T[i - 1].row || T[i].col != T[i - 1].col) ++nnz;
-    int64_t *rp = (int64_t *)calloc((size_t)ndof + 1, sizeof(int64_t));
+    if (bad != INT64_MAX) {
+        double X[30], Ke[900];
+        for (int k = 0; k < 10; ++k) { int nd = ec[k * ntet + bad]; X[k] = x[nd]; X[k + 10] = y[nd]; X[k + 20] = z[nd]; }
+        bad_st = or_tet10(X, ep[bad], ep[ntet + bad], Ke);
+        free(T); *bad_elem = nq + nt + bad; return bad_st;
+    }
+    /* sparse(I,J,V): order by (row, col, triplet order) -- a stable counting sort
+     * by row, then per row a sort by (col, seq) -- and sum duplicates in that order */
+    const int64_t ndof = 6 * n_nodes;
+    int64_t *rstart = (int64_t *)calloc((size_t)ndof + 1, sizeof(int64_t));
+    trip_t *S = (trip_t *)malloc(sizeof(trip_t) * (ntrip > 0 ? ntrip : 1));
+    int64_t *rnnz = (int64_t *)calloc((size_t)ndof + 1, sizeof(int64_t));
+    if (!rstart || !S || !rnnz) { free(T); free(rstart); free(S); free(rnnz); return OR_ERR_OOM; }
+    for (int64_t i = 0; i < ntrip; ++i) rstart[T[i].row + 1]++;
+    for (int64_t r = 0; r < ndof; ++r) rstart[r + 1] += rstart[r];
+    {
+        int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (ndof > 0 ? ndof : 1));
+        memcpy(fill, rstart, sizeof(int64_t) * ndof);
+        for (int64_t i = 0; i < ntrip; ++i) S[fill[T[i].row]++] = T[i];
+        free(fill);
+    }
+    free(T);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t r = 0; r < ndof; ++r) {
+        const int64_t a = rstart[r], b = rstart[r + 1];
+        if (b > a) qsort(S + a, (size_t)(b - a), sizeof(trip_t), trip_cmp);
+        int64_t c = 0;
+        for (int64_t i = a; i < b; ++i)
+            if (i == a || S[i].col != S[i - 1].col) ++c;
+        rnnz[r + 1] = c;
+    }
+    for (int64_t r = 0; r < ndof; ++r) rnnz[r + 1] += rnnz[r];
+    const int64_t nnz = rnnz[ndof];
     int32_t *ci = (int32_t *)malloc(sizeof(int32_t) * (nnz ? nnz : 1));
     double *cv = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
-    if (!rp || !ci || !cv) { free(T); free(rp); free(ci); free(cv); return OR_ERR_OOM; }
-    int64_t k = -1;
-    for (int64_t i = 0; i < ntrip; ++i) {
-        if (i == 0 || T[i].row != T[i - 1].row || T[i].col != T[i - 1].col) {
-            ++k; ci[k] = T[i].col; cv[k] = T[i].v; rp[T[i].row + 1]++;
-        } else {
-            cv[k] += T[i].v; /* duplicates summed in triplet order */
+    if (!ci || !cv) { free(S); free(rstart); free(rnnz); free(ci); free(cv); return OR_ERR_OOM; }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t r = 0; r < ndof; ++r) {
+        int64_t k = rnnz[r] - 1;
+        for (int64_t i = rstart[r]; i < rstart[r + 1]; ++i) {
+            if (i == rstart[r] || S[i].col != S[i - 1].col) {
+                ++k; ci[k] = S[i].col; cv[k] = S[i].v;
+            } else {
+                cv[k] += S[i].v; /* duplicates summed in triplet order */
+            }
         }
     }
-    for (int64_t r = 0; r < ndof; ++r) rp[r + 1] += rp[r];
-    free(T);
-    *out_nnz = nnz; *row_ptr = rp; *col = ci; *val = cv;
+    free(S); free(rstart);
+    *out_nnz = nnz; *row_ptr = rnnz; *col = ci; *val = cv;
     return OR_OK;
 }
 
