@@ -51,6 +51,7 @@ typedef struct jfem_matrix jfem_matrix;
 typedef struct jfem_symbolic jfem_symbolic;
 typedef struct jfem_factor jfem_factor;
 typedef struct jfem_loadset jfem_loadset;
+typedef struct jfem_batch jfem_batch;
 
 /* ---- contexts ----------------------------------------------------------- */
 /* One per host thread or request slot: a HIP stream on `device` plus scratch.
@@ -150,6 +151,37 @@ int jfem_reassemble(jfem_ctx *ctx, const jfem_mesh *mesh, jfem_matrix *K, int64_
 int jfem_matrix_download(jfem_ctx *ctx, const jfem_matrix *K, int64_t *row_ptr, int32_t *col,
                          double *val);
 int jfem_matrix_free(jfem_matrix *K);
+
+/* ---- design variants ------------------------------------------------------ */
+/* nvar design variants of one topology: the mesh's grid points and connectivity,
+ * each variant with its own section records (quad_props: nvar x (n_quad x 8),
+ * variant after variant, each in the layout of jfem_mesh_create; likewise
+ * tria_props and tet_props).  The reference runs each variant as its own model
+ * through assemble_stiffness and cholesky (src/solver/assembly.jl:2630,
+ * src/solver/boundary_conditions.jl:1660); here all variants share the pattern
+ * and one symbolic factorization, and their element generation, assembly and
+ * numeric factorization each run as one launch sequence over all variants. */
+int jfem_batch_create(jfem_ctx *ctx, const jfem_mesh *mesh, const jfem_pattern *pattern, int64_t nvar,
+                      const double *quad_props, const double *tria_props, const double *tet_props,
+                      jfem_batch **out);
+int jfem_batch_update_props(jfem_ctx *ctx, jfem_batch *batch, const double *quad_props,
+                            const double *tria_props, const double *tet_props);
+/* element generation + assembly of every variant (bitwise the same K values as
+ * jfem_assemble of that variant alone); on a rejected element bad_variant and
+ * bad_element (index in triplet order) name the first one */
+int jfem_batch_assemble(jfem_ctx *ctx, jfem_batch *batch, int64_t *bad_variant, int64_t *bad_element);
+/* variant v's matrix (owned by the batch; jfem_matrix_free ignores it), usable
+ * wherever a jfem_matrix is: BC partition, symbolic, factor, solves */
+int jfem_batch_matrix(const jfem_batch *batch, int64_t v, jfem_matrix **view);
+/* numeric factors of all variants over S (they must share its free set), in one
+ * launch sequence, each bitwise the factor jfem_factor_create gives for that
+ * variant alone; a non-positive pivot names its variant and DOF */
+int jfem_batch_factor(jfem_ctx *ctx, jfem_batch *batch, const jfem_symbolic *S, int precision,
+                      int64_t *bad_variant, int64_t *bad_column);
+/* variant v's factor (owned by the batch; jfem_factor_free ignores it), for the
+ * solve entry points */
+int jfem_batch_factor_view(const jfem_batch *batch, int64_t v, jfem_factor **view);
+int jfem_batch_free(jfem_batch *batch);
 
 /* ---- boundary conditions -------------------------------------------------- */
 /* SPC + AUTOSPC (zero/negative diagonal) + grid-point singularity processing:

@@ -35,6 +35,8 @@ EXPORTS = [
     "jfem_factor_device_bytes", "jfem_matrix_device_bytes", "jfem_mesh_device_bytes", "jfem_pattern_device_bytes",
     "jfem_symbolic_device_bytes", "jfem_ctx_device_bytes", "jfem_solve_scratch_bytes", "jfem_selftest_update",
     "jfem_loadset_create", "jfem_load_vector", "jfem_load_vector_dev", "jfem_loadset_free",
+    "jfem_batch_create", "jfem_batch_update_props", "jfem_batch_assemble", "jfem_batch_matrix", "jfem_batch_factor",
+    "jfem_batch_factor_view", "jfem_batch_free",
 ]
 
 
@@ -118,6 +120,13 @@ def load():
         "jfem_load_vector": ([vp, vp, dbl, vp], C.c_int),
         "jfem_load_vector_dev": ([vp, vp, dbl, vp], C.c_int),
         "jfem_loadset_free": ([vp], C.c_int),
+        "jfem_batch_create": ([vp, vp, vp, i64, vp, vp, vp, P(vp)], C.c_int),
+        "jfem_batch_update_props": ([vp, vp, vp, vp, vp], C.c_int),
+        "jfem_batch_assemble": ([vp, vp, P(i64), P(i64)], C.c_int),
+        "jfem_batch_matrix": ([vp, i64, P(vp)], C.c_int),
+        "jfem_batch_factor": ([vp, vp, vp, C.c_int, P(i64), P(i64)], C.c_int),
+        "jfem_batch_factor_view": ([vp, i64, P(vp)], C.c_int),
+        "jfem_batch_free": ([vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -201,12 +201,8 @@ static int run_elements(jfem_ctx *ctx, const jfem_mesh *m, double *d_Ke, int64_t
     hipStream_t s = ctx->stream;
     JFEM_CHECK_HIP(ctx, launch_reset_status(s, ctx->d_status));
     timer_start(ctx);
-    int64_t off_t = m->nq * 576, off_e = off_t + m->nt * 324;
-    int64_t n = 0;
-    if (kinds_mask & 1) { JFEM_CHECK_HIP(ctx, launch_q4_global(s, m, d_Ke, ctx->d_status)); ++n; }
-    if (kinds_mask & 2) { JFEM_CHECK_HIP(ctx, launch_t3_global(s, m, d_Ke + ((kinds_mask & 1) ? off_t : 0), ctx->d_status)); ++n; }
-    if (kinds_mask & 4) { JFEM_CHECK_HIP(ctx, launch_tet10(s, m, d_Ke + ((kinds_mask & 1) ? off_e : 0), ctx->d_status)); ++n; }
-    timer_stop(ctx, FAM_ELEM, n);
+    JFEM_CHECK_HIP(ctx, launch_elements(s, m, m->d_qp, m->d_tp, m->d_tetp, d_Ke, 0, 1, kinds_mask, ctx->d_status));
+    timer_stop(ctx, FAM_ELEM, 1);
     int64_t st[2];
     JFEM_CHECK_HIP(ctx, hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, s));
     JFEM_CHECK_HIP(ctx, hipStreamSynchronize(s));
@@ -475,7 +471,7 @@ int jfem_reassemble(jfem_ctx *ctx, const jfem_mesh *m, jfem_matrix *K, int64_t *
     int st = run_elements(ctx, m, K->d_Ke, bad, 7);
     if (st != JFEM_OK) return st;
     timer_start(ctx);
-    JFEM_CHECK_HIP(ctx, launch_assemble(ctx->stream, P, K->d_Ke, K->d_val));
+    JFEM_CHECK_HIP(ctx, launch_assemble(ctx->stream, P, K->d_Ke, K->d_val, 1, 0, 0));
     timer_stop(ctx, FAM_ASSEMBLY, 1);
     JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return JFEM_OK;
@@ -513,7 +509,7 @@ int jfem_matrix_download(jfem_ctx *ctx, const jfem_matrix *K, int64_t *row_ptr, 
 }
 
 int jfem_matrix_free(jfem_matrix *K) {
-    if (!K) return JFEM_OK;
+    if (!K || K->view) return JFEM_OK;     // variants are freed with their batch
     (void)hipSetDevice(K->device);
     if (K->d_val) (void)hipFree(K->d_val);
     if (K->d_Ke) (void)hipFree(K->d_Ke);
@@ -541,6 +537,101 @@ int64_t jfem_pattern_device_bytes(const jfem_pattern *P) {
 int64_t jfem_ctx_device_bytes(const jfem_ctx *ctx) {
     if (!ctx) return 0;
     return (int64_t)(ctx->scratch_bytes + ctx->bc_dev_bytes) + 2 * (int64_t)sizeof(int64_t);
+}
+
+// ------------------------------------------------------------------ design variants
+// V section sets of one topology, generated and assembled in one launch sequence:
+// one element launch per kind and one assembly launch cover every variant
+int jfem_batch_create(jfem_ctx *ctx, const jfem_mesh *m, const jfem_pattern *P, int64_t nvar,
+                      const double *quad_props, const double *tria_props, const double *tet_props, jfem_batch **out) {
+    *out = nullptr;
+    if (nvar < 1 || nvar > 65535) return fail(ctx, JFEM_ERR_BAD_ARG, "variant count must be in 1..65535");
+    if (P->nq != m->nq || P->nt != m->nt || P->ntet != m->ntet) return fail(ctx, JFEM_ERR_BAD_ARG, "mesh/pattern mismatch");
+    JFEM_CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    jfem_batch *B = new jfem_batch();
+    B->device = ctx->device;
+    B->mesh = m;
+    B->P = P;
+    B->V = nvar;
+    B->Ke_stride = jfem_stride(std::max<int64_t>(P->ntrip, 1), 8);
+    B->val_stride = jfem_stride(std::max<int64_t>(P->nnz, 1), 8);
+    hipStream_t s = ctx->stream;
+    hipError_t e = upload(&B->d_qp, quad_props, (size_t)(8 * m->nq * nvar), s);
+    if (e == hipSuccess) e = upload(&B->d_tp, tria_props, (size_t)(8 * m->nt * nvar), s);
+    if (e == hipSuccess) e = upload(&B->d_tetp, tet_props, (size_t)(2 * m->ntet * nvar), s);
+    if (e == hipSuccess) e = hipMalloc((void **)&B->d_Ke, (size_t)(B->Ke_stride * nvar) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&B->d_val, (size_t)(B->val_stride * nvar) * sizeof(double));
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        jfem_batch_free(B);
+        return fail(ctx, e == hipErrorOutOfMemory ? JFEM_ERR_OOM : JFEM_ERR_HIP, hipGetErrorString(e));
+    }
+    for (int64_t v = 0; v < nvar; ++v) {
+        jfem_matrix *K = new jfem_matrix();
+        K->device = ctx->device;
+        K->P = P;
+        K->d_val = B->d_val + v * B->val_stride;
+        K->d_Ke = B->d_Ke + v * B->Ke_stride;
+        K->view = true;
+        B->mviews.push_back(K);
+    }
+    *out = B;
+    return JFEM_OK;
+}
+
+int jfem_batch_update_props(jfem_ctx *ctx, jfem_batch *B, const double *quad_props, const double *tria_props,
+                            const double *tet_props) {
+    const jfem_mesh *m = B->mesh;
+    hipStream_t s = ctx->stream;
+    if (m->nq) JFEM_CHECK_HIP(ctx, hipMemcpyAsync(B->d_qp, quad_props, 8 * m->nq * B->V * sizeof(double), hipMemcpyHostToDevice, s));
+    if (m->nt) JFEM_CHECK_HIP(ctx, hipMemcpyAsync(B->d_tp, tria_props, 8 * m->nt * B->V * sizeof(double), hipMemcpyHostToDevice, s));
+    if (m->ntet) JFEM_CHECK_HIP(ctx, hipMemcpyAsync(B->d_tetp, tet_props, 2 * m->ntet * B->V * sizeof(double), hipMemcpyHostToDevice, s));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    return JFEM_OK;
+}
+
+int jfem_batch_assemble(jfem_ctx *ctx, jfem_batch *B, int64_t *bad_variant, int64_t *bad_element) {
+    if (bad_variant) *bad_variant = -1;
+    if (bad_element) *bad_element = -1;
+    const jfem_mesh *m = B->mesh;
+    hipStream_t s = ctx->stream;
+    JFEM_CHECK_HIP(ctx, launch_reset_status(s, ctx->d_status));
+    timer_start(ctx);
+    JFEM_CHECK_HIP(ctx, launch_elements(s, m, B->d_qp, B->d_tp, B->d_tetp, B->d_Ke, B->Ke_stride, (int)B->V, 7,
+                                        ctx->d_status));
+    timer_stop(ctx, FAM_ELEM, 1);
+    timer_start(ctx);
+    JFEM_CHECK_HIP(ctx, launch_assemble(s, B->P, B->d_Ke, B->d_val, (int)B->V, B->Ke_stride, B->val_stride));
+    timer_stop(ctx, FAM_ASSEMBLY, 1);
+    int64_t st[2];
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, s));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(s));
+    if (st[1] != 0) {
+        const int64_t total = std::max<int64_t>(1, m->nq + m->nt + m->ntet);
+        if (bad_variant) *bad_variant = st[0] / total;
+        if (bad_element) *bad_element = st[0] % total;
+        return fail(ctx, (int)st[1], "variant " + std::to_string(st[0] / total) + ": element " +
+                                         std::to_string(st[0] % total) + " rejected");
+    }
+    return JFEM_OK;
+}
+
+int jfem_batch_matrix(const jfem_batch *B, int64_t v, jfem_matrix **view) {
+    *view = nullptr;
+    if (!B || v < 0 || v >= B->V) return JFEM_ERR_BAD_ARG;
+    *view = B->mviews[(size_t)v];
+    return JFEM_OK;
+}
+
+int jfem_batch_free(jfem_batch *B) {
+    if (!B) return JFEM_OK;
+    (void)hipSetDevice(B->device);
+    batch_free_factor(B);
+    for (jfem_matrix *K : B->mviews) delete K;
+    for (void *p : {(void *)B->d_qp, (void *)B->d_tp, (void *)B->d_tetp, (void *)B->d_Ke, (void *)B->d_val})
+        if (p) (void)hipFree(p);
+    delete B;
+    return JFEM_OK;
 }
 
 // ------------------------------------------------------------------ BC partition

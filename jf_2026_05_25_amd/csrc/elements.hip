@@ -22,6 +22,15 @@ constexpr double kEps = 2.220446049250313e-16;
 constexpr int kWave = 64;
 constexpr int kElemsPerBlock = 2;  // wavefronts (= elements) per workgroup
 
+// Variants of one mesh (design variants): a launch over V section sets runs
+// variant blockIdx.y with its section records at blockIdx.y * prop, its element
+// matrices at blockIdx.y * Ke (doubles); a failing element is reported at index
+// base + blockIdx.y * elem + e, base being the kind's first element in triplet
+// order (CQUAD4s, CTRIA3s, CTETRA10s).  Zero strides for a single model.
+struct ElemVar {
+    int64_t prop = 0, Ke = 0, elem = 0, base = 0;
+};
+
 __device__ inline void report(int64_t *status, int64_t elem, int code) {
     // first failing element wins (smallest index); code stored by that element
     unsigned long long old = atomicMin((unsigned long long *)&status[0], (unsigned long long)elem);
@@ -371,7 +380,8 @@ __device__ inline int plate_index(int d) {  // local DOF -> MacNeal plate index 
 
 // Per-wavefront CQUAD4 kernel body.  p: 4 node xyz (p[3k+c]); prop: section row.
 // global_frame: compute frame + section from p/prop; otherwise use lc/C* given.
-__device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int64_t *status, double *Ke_out) {
+__device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int64_t *status, double *Ke_out,
+                           int64_t report_index) {
     // Phase 1: Gauss-point rows (lanes 0..3), drilling rows (lanes 59..62); the
     // MacNeal operators Dm, Xm come with the prep record
     const double pt = 1.0 / sqrt(3.0);
@@ -681,7 +691,7 @@ __device__ void q4_element(Q4Shared &S, int lane, bool valid, int64_t elem, int6
     __syncthreads();
     if (valid) {
         if (S.status != 0) {
-            if (lane == 0) report(status, elem, S.status);
+            if (lane == 0) report(status, report_index, S.status);
         }
         double *dst = Ke_out + elem * 576;
         for (int q = lane; q < 576; q += kWave) dst[q] = S.out[q];
@@ -839,9 +849,11 @@ __device__ inline int q4_prep_shear(Q4Prep &P) {
 __global__ void __launch_bounds__(64)
 q4_prep_global_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
                       const double *__restrict__ z, const int32_t *__restrict__ conn,
-                      const double *__restrict__ prop, double k6rot, double *__restrict__ Ke) {
+                      const double *__restrict__ prop, double k6rot, double *__restrict__ Ke, ElemVar ev) {
     const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (e >= n) return;
+    prop += blockIdx.y * ev.prop;
+    Ke += blockIdx.y * ev.Ke;
     double p[12], pr[8];
     for (int k = 0; k < 4; ++k) {
         int nd = conn[k * n + e];
@@ -883,15 +895,16 @@ q4_prep_local_kernel(int64_t n, const double *__restrict__ lc, const double *__r
 
 // the lane-parallel part: one wavefront per prepared element, result over the record
 __global__ void __launch_bounds__(kWave * kElemsPerBlock, 2)
-q4_elem_kernel(int64_t n, double *__restrict__ Ke, int64_t *status) {
+q4_elem_kernel(int64_t n, double *__restrict__ Ke, int64_t *status, ElemVar ev) {
     __shared__ Q4Shared sh[kElemsPerBlock];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t e = (int64_t)blockIdx.x * kElemsPerBlock + wave;
     const bool valid = e < n;
+    Ke += blockIdx.y * ev.Ke;
     Q4Shared &S = sh[wave];
     if (valid) q4_prep_load(S, Ke + e * 576, lane);
     __syncthreads();
-    q4_element(S, lane, valid, e, status, Ke);
+    q4_element(S, lane, valid, e, status, Ke, ev.base + blockIdx.y * ev.elem + e);
 }
 
 // ------------------------------------------------------------------------------
@@ -1067,11 +1080,13 @@ __global__ void __launch_bounds__(kWave * kElemsPerBlock)
 t3_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
           const double *__restrict__ z, const int32_t *__restrict__ conn,
           const double *__restrict__ prop, double k6rot, T3LocalIn loc, double *__restrict__ Ke,
-          int64_t *status) {
+          int64_t *status, ElemVar ev) {
     __shared__ T3Shared sh[kElemsPerBlock];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t e = (int64_t)blockIdx.x * kElemsPerBlock + wave;
     const bool valid = e < n;
+    if (!Local) prop += blockIdx.y * ev.prop;
+    Ke += blockIdx.y * ev.Ke;
     T3Shared &S = sh[wave];
     if (Local && valid && lane == 0) {
         for (int q = 0; q < 9; ++q) {
@@ -1168,7 +1183,7 @@ t3_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
     }
     __syncthreads();
     if (valid) {
-        if (lane == 0 && S.status != 0) report(status, e, S.status);
+        if (lane == 0 && S.status != 0) report(status, ev.base + blockIdx.y * ev.elem + e, S.status);
         double *dst = Ke + e * 324;
         for (int q = lane; q < 324; q += kWave) dst[q] = S.out[q];
     }
@@ -1208,11 +1223,13 @@ template <bool Direct>
 __global__ void __launch_bounds__(kWave * kElemsPerBlock)
 tet10_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
              const double *__restrict__ z, const int32_t *__restrict__ conn, const double *__restrict__ Xd,
-             const double *__restrict__ prop, double *__restrict__ Ke, int64_t *status) {
+             const double *__restrict__ prop, double *__restrict__ Ke, int64_t *status, ElemVar ev) {
     __shared__ TetShared sh[kElemsPerBlock];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t e = (int64_t)blockIdx.x * kElemsPerBlock + wave;
     const bool valid = e < n;
+    prop += blockIdx.y * ev.prop;
+    Ke += blockIdx.y * ev.Ke;
     TetShared &S = sh[wave];
     if (valid) {
         if (lane < 10) {
@@ -1297,7 +1314,7 @@ tet10_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__
     }
     __syncthreads();
     if (valid) {
-        if (lane == 0 && S.status != 0) report(status, e, S.status);
+        if (lane == 0 && S.status != 0) report(status, ev.base + blockIdx.y * ev.elem + e, S.status);
         double *dst = Ke + e * 900;
         for (int q = lane; q < 900; q += kWave) dst[q] = S.out[q];
     }
@@ -1308,9 +1325,11 @@ tet10_kernel(int64_t n, const double *__restrict__ x, const double *__restrict__
 // ------------------------------------------------------------------------------
 __global__ void assemble_kernel(int64_t nnz, const int64_t *__restrict__ seg_ptr,
                                 const int32_t *__restrict__ seg_src, const double *__restrict__ Ke,
-                                double *__restrict__ val) {
+                                double *__restrict__ val, int64_t Ke_stride, int64_t val_stride) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nnz) return;
+    Ke += blockIdx.y * Ke_stride;
+    val += blockIdx.y * val_stride;
     int64_t b = seg_ptr[s], e = seg_ptr[s + 1];
     double v = 0.0;
     for (int64_t p = b; p < e; ++p) v += Ke[(uint32_t)seg_src[p]];
@@ -1480,20 +1499,35 @@ hipError_t launch_reset_status(hipStream_t s, int64_t *status) {
     return hipGetLastError();
 }
 
-hipError_t launch_q4_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status) {
-    if (m->nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(q4_prep_global_kernel, dim3((unsigned)((m->nq + 63) / 64)), dim3(64), 0, s, m->nq, m->d_x, m->d_y,
-                       m->d_z, m->d_q, m->d_qp, m->k6rot, Ke);
-    dim3 grid((unsigned)((m->nq + kElemsPerBlock - 1) / kElemsPerBlock));
-    hipLaunchKernelGGL(q4_elem_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, m->nq, Ke, status);
-    return hipGetLastError();
-}
-
-hipError_t launch_t3_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status) {
-    if (m->nt == 0) return hipSuccess;
-    dim3 grid((unsigned)((m->nt + kElemsPerBlock - 1) / kElemsPerBlock));
-    hipLaunchKernelGGL(t3_kernel<false>, grid, dim3(kWave * kElemsPerBlock), 0, s, m->nt, m->d_x, m->d_y, m->d_z,
-                       m->d_t, m->d_tp, m->k6rot, T3LocalIn{}, Ke, status);
+// element matrices of the kinds in kinds_mask (1 CQUAD4, 2 CTRIA3, 4 CTETRA10) for V
+// section sets (variant v's records at qp / tp / tetp + v * (records of a set),
+// its matrices at Ke + v * Ke_stride); within one set the kinds follow each other
+// when several are generated (triplet order)
+hipError_t launch_elements(hipStream_t s, const jfem_mesh *m, const double *qp, const double *tp, const double *tetp,
+                           double *Ke, int64_t Ke_stride, int V, int kinds_mask, int64_t *status) {
+    const int64_t total = m->nq + m->nt + m->ntet;
+    int64_t off = 0;
+    if ((kinds_mask & 1) && m->nq) {
+        const ElemVar ev{8 * m->nq, Ke_stride, total, 0};
+        hipLaunchKernelGGL(q4_prep_global_kernel, dim3((unsigned)((m->nq + 63) / 64), V), dim3(64), 0, s, m->nq, m->d_x,
+                           m->d_y, m->d_z, m->d_q, qp, m->k6rot, Ke, ev);
+        dim3 grid((unsigned)((m->nq + kElemsPerBlock - 1) / kElemsPerBlock), V);
+        hipLaunchKernelGGL(q4_elem_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, m->nq, Ke, status, ev);
+        off += m->nq * 576;
+    }
+    if ((kinds_mask & 2) && m->nt) {
+        const ElemVar ev{8 * m->nt, Ke_stride, total, m->nq};
+        dim3 grid((unsigned)((m->nt + kElemsPerBlock - 1) / kElemsPerBlock), V);
+        hipLaunchKernelGGL(t3_kernel<false>, grid, dim3(kWave * kElemsPerBlock), 0, s, m->nt, m->d_x, m->d_y, m->d_z,
+                           m->d_t, tp, m->k6rot, T3LocalIn{}, Ke + off, status, ev);
+        off += m->nt * 324;
+    }
+    if ((kinds_mask & 4) && m->ntet) {
+        const ElemVar ev{2 * m->ntet, Ke_stride, total, m->nq + m->nt};
+        dim3 grid((unsigned)((m->ntet + kElemsPerBlock - 1) / kElemsPerBlock), V);
+        hipLaunchKernelGGL(tet10_kernel<false>, grid, dim3(kWave * kElemsPerBlock), 0, s, m->ntet, m->d_x, m->d_y,
+                           m->d_z, m->d_tet, nullptr, tetp, Ke + off, status, ev);
+    }
     return hipGetLastError();
 }
 
@@ -1502,7 +1536,7 @@ hipError_t launch_t3_local(hipStream_t s, int64_t n, const double *lc, const dou
     if (n == 0) return hipSuccess;
     dim3 grid((unsigned)((n + kElemsPerBlock - 1) / kElemsPerBlock));
     hipLaunchKernelGGL(t3_kernel<true>, grid, dim3(kWave * kElemsPerBlock), 0, s, n, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, 0.0, T3LocalIn{lc, Dm, Db, Ds, k6}, Ke, status);
+                       nullptr, 0.0, T3LocalIn{lc, Dm, Db, Ds, k6}, Ke, status, ElemVar{});
     return hipGetLastError();
 }
 
@@ -1511,15 +1545,7 @@ hipError_t launch_tet10_direct(hipStream_t s, int64_t n, const double *X, const 
     if (n == 0) return hipSuccess;
     dim3 grid((unsigned)((n + kElemsPerBlock - 1) / kElemsPerBlock));
     hipLaunchKernelGGL(tet10_kernel<true>, grid, dim3(kWave * kElemsPerBlock), 0, s, n, nullptr, nullptr, nullptr,
-                       nullptr, X, prop, Ke, status);
-    return hipGetLastError();
-}
-
-hipError_t launch_tet10(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status) {
-    if (m->ntet == 0) return hipSuccess;
-    dim3 grid((unsigned)((m->ntet + kElemsPerBlock - 1) / kElemsPerBlock));
-    hipLaunchKernelGGL(tet10_kernel<false>, grid, dim3(kWave * kElemsPerBlock), 0, s, m->ntet, m->d_x, m->d_y, m->d_z,
-                       m->d_tet, nullptr, m->d_tetp, Ke, status);
+                       nullptr, X, prop, Ke, status, ElemVar{});
     return hipGetLastError();
 }
 
@@ -1530,15 +1556,17 @@ hipError_t launch_q4_local(hipStream_t s, int64_t n, const double *lc, const dou
     hipLaunchKernelGGL(q4_prep_local_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, lc, Cm, Cb, Cs, h, k6,
                        ds, flags, Ke);
     dim3 grid((unsigned)((n + kElemsPerBlock - 1) / kElemsPerBlock));
-    hipLaunchKernelGGL(q4_elem_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, n, Ke, status);
+    hipLaunchKernelGGL(q4_elem_kernel, grid, dim3(kWave * kElemsPerBlock), 0, s, n, Ke, status, ElemVar{});
     return hipGetLastError();
 }
 
-hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *Ke, double *val) {
+hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *Ke, double *val, int V,
+                           int64_t Ke_stride, int64_t val_stride) {
     if (P->nnz == 0) return hipSuccess;
     const int bs = 256;
-    dim3 grid((unsigned)((P->nnz + bs - 1) / bs));
-    hipLaunchKernelGGL(assemble_kernel, grid, dim3(bs), 0, s, P->nnz, P->d_seg_ptr, P->d_seg_src, Ke, val);
+    dim3 grid((unsigned)((P->nnz + bs - 1) / bs), V);
+    hipLaunchKernelGGL(assemble_kernel, grid, dim3(bs), 0, s, P->nnz, P->d_seg_ptr, P->d_seg_src, Ke, val, Ke_stride,
+                       val_stride);
     return hipGetLastError();
 }
 

@@ -50,11 +50,20 @@ struct jfem_factor {
     void *d_U = nullptr;
     void *d_Linv = nullptr;         // inverse diagonal blocks of the panels (solves)
     int64_t bytes = 0;
+    bool view = false;              // a variant of a jfem_batch: storage owned by the batch
 };
 
 namespace {
 
 constexpr int kBS = 256;
+
+// Variants: a launch over V matrices that share one symbolic factorization runs
+// variant blockIdx.y on storage offset by blockIdx.y times these strides
+// (elements): K values, factor L, update matrices U, inverse blocks; failures are
+// reported at index blockIdx.y * col + (column).  All zero for a single matrix.
+struct VarStride {
+    int64_t K = 0, L = 0, U = 0, inv = 0, col = 0;
+};
 
 template <class T>
 __device__ inline T tsqrt(T x);
@@ -105,7 +114,9 @@ __global__ void zero16_kernel(T *__restrict__ p, int64_t n) {
 
 template <class T>
 __global__ void scatter_kernel(int64_t n, const int32_t *__restrict__ src, const int64_t *__restrict__ dst,
-                               const double *__restrict__ val, T *__restrict__ L) {
+                               const double *__restrict__ val, T *__restrict__ L, VarStride vs) {
+    val += blockIdx.y * vs.K;
+    L += blockIdx.y * vs.L;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         L[dst[i]] = (T)val[src[i]];
 }
@@ -122,7 +133,10 @@ __global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restric
                                                          const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
                                                          const int64_t *__restrict__ uofs, const int64_t *__restrict__ relofs,
                                                          const int32_t *__restrict__ relmap, const int32_t *__restrict__ child_ptr,
-                                                         const int32_t *__restrict__ child, T *__restrict__ L, T *__restrict__ U) {
+                                                         const int32_t *__restrict__ child, T *__restrict__ L, T *__restrict__ U,
+                                                         VarStride vs) {
+    L += blockIdx.y * vs.L;
+    U += blockIdx.y * vs.U;
     constexpr int kU = 8;
     const EaTask t = tasks[blockIdx.x];
     const int s = t.s;
@@ -295,8 +309,9 @@ template <class T>
 __global__ void __launch_bounds__(kBS) diag_factor_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                           const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
                                                           const int64_t *__restrict__ col0, T *__restrict__ L,
-                                                          int64_t *__restrict__ status) {
+                                                          int64_t *__restrict__ status, VarStride vs) {
     __shared__ T P[kPanel][17];
+    L += blockIdx.y * vs.L;
     const PanelTask t = tasks[blockIdx.x];
     const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
@@ -312,7 +327,7 @@ __global__ void __launch_bounds__(kBS) diag_factor_kernel(const PanelTask *__res
         a[cc] = (c <= lane && lane < b) ? v : (c == lane ? T(1) : T(0));
     }
     const int bad = wg_potrf(a, P, lane, w, b);
-    if (bad < b && tid == 0) report_min(status, col0[s] + k + bad, JFEM_ERR_NOT_SPD);
+    if (bad < b && tid == 0) report_min(status, blockIdx.y * vs.col + col0[s] + k + bad, JFEM_ERR_NOT_SPD);
     if (lane < b) {
 #pragma unroll
         for (int cc = 0; cc < 16; ++cc) {
@@ -327,8 +342,9 @@ __global__ void __launch_bounds__(kBS) diag_factor_kernel(const PanelTask *__res
 template <class T>
 __global__ void __launch_bounds__(kBS) tile_solve_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                          const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
-                                                         T *__restrict__ L) {
+                                                         T *__restrict__ L, VarStride vs) {
     __shared__ __attribute__((aligned(16))) T Ld[kPanel][kPanel + 4];
+    L += blockIdx.y * vs.L;
     __shared__ T P[kPanel][17];
     const PanelTask t = tasks[blockIdx.x];
     const int s = t.s;
@@ -367,8 +383,10 @@ __global__ void __launch_bounds__(kBS) tile_solve_kernel(const PanelTask *__rest
 // stored column-major with leading dimension b
 template <class T>
 __global__ void __launch_bounds__(64) diag_inverse_kernel(const FlowTask *__restrict__ tasks, const T *__restrict__ L,
-                                                          T *__restrict__ Linv) {
+                                                          T *__restrict__ Linv, VarStride vs) {
     __shared__ __attribute__((aligned(16))) T D[kPanel][kPanel + 4];
+    L += blockIdx.y * vs.L;
+    Linv += blockIdx.y * vs.inv;
     const FlowTask t = tasks[blockIdx.x];
     const int b = t.nr, lane = threadIdx.x;
     const int64_t k = t.r0, ms = t.ms;
@@ -483,8 +501,10 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
                                                      const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
                                                      const int64_t *__restrict__ uofs, T *__restrict__ L, T *__restrict__ U,
                                                      int factor_next, const int64_t *__restrict__ col0,
-                                                     int64_t *__restrict__ status) {
+                                                     int64_t *__restrict__ status, VarStride vs) {
     typedef typename Mfma<T>::v4 v4;
+    L += blockIdx.y * vs.L;
+    U += blockIdx.y * vs.U;
     __shared__ T A[kPanel][kTile + 1];   // A[p][i] = L[r0 + i, k + p]
     __shared__ T B[kPanel][kTile + 1];   // B[p][j] = L[c0 + j, k + p]
     const UpdateTask t = tasks[blockIdx.x];
@@ -567,7 +587,7 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
         a[cc] = (c <= lane && lane < bn) ? A[c][lane] : (c == lane ? T(1) : T(0));
     }
     const int bad = wg_potrf(a, reinterpret_cast<T (*)[17]>(&B[0][0]), lane, wave, bn);
-    if (bad < bn && tid == 0) report_min(status, col0[s] + c0 + bad, JFEM_ERR_NOT_SPD);
+    if (bad < bn && tid == 0) report_min(status, blockIdx.y * vs.col + col0[s] + c0 + bad, JFEM_ERR_NOT_SPD);
     if (lane < bn) {
 #pragma unroll
         for (int cc = 0; cc < 16; ++cc) {
@@ -585,8 +605,10 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
 // x_p = u - G_p x_{p+1}, u formed before the awaited value arrives.
 template <class T>
 __global__ void __launch_bounds__(kBS) chain_product_kernel(const FlowTask *__restrict__ tasks, const T *__restrict__ L,
-                                                            T *__restrict__ Linv) {
+                                                            T *__restrict__ Linv, VarStride vs) {
     __shared__ T A[kPanel][kPanel + 1];
+    L += blockIdx.y * vs.L;
+    Linv += blockIdx.y * vs.inv;
     const FlowTask t = tasks[blockIdx.x];
     if (t.bofs < 0) return;                            // last panel: no boundary below it
     const int b1 = (int)min<int64_t>(kPanel, t.ns - (int64_t)kPanel * (t.p + 1));
@@ -1875,8 +1897,11 @@ __global__ void diag_ratio_kernel(int64_t n, const int32_t *__restrict__ perm, c
 
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBS - 1) / kBS, 65536)); }
 
+// numeric factorization of V matrices sharing S (K values, factors and update
+// matrices of variant v at v times the strides in vs), in one launch sequence
 template <class T>
-hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, T *L, T *U, T *Linv) {
+hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T *L, T *U, T *Linv, int V,
+                   VarStride vs) {
     hipStream_t st = ctx->stream;
     const SymbolicHost &H = S->H;
     const bool detail = ctx->timing_level >= 2;
@@ -1887,23 +1912,23 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
         ctx->ms[fam] += ms;
         ctx->launches[fam] += launches;
     };
-    hipLaunchKernelGGL(zero16_kernel<T>, dim3(grid_for(H.L_entries * (int64_t)sizeof(T) / 16)), dim3(kBS), 0, st, L,
-                       H.L_entries);
+    const int64_t nLv = V > 1 ? vs.L * (V - 1) + H.L_entries : H.L_entries;
+    const int64_t nUv = V > 1 ? vs.U * (V - 1) + H.U_entries : H.U_entries;
+    hipLaunchKernelGGL(zero16_kernel<T>, dim3(grid_for(nLv * (int64_t)sizeof(T) / 16)), dim3(kBS), 0, st, L, nLv);
     if (H.U_entries)
-        hipLaunchKernelGGL(zero16_kernel<T>, dim3(grid_for(H.U_entries * (int64_t)sizeof(T) / 16)), dim3(kBS), 0, st, U,
-                           H.U_entries);
+        hipLaunchKernelGGL(zero16_kernel<T>, dim3(grid_for(nUv * (int64_t)sizeof(T) / 16)), dim3(kBS), 0, st, U, nUv);
     int64_t na = (int64_t)H.amap_src.size();
     if (na)
-        hipLaunchKernelGGL(scatter_kernel<T>, dim3(grid_for(na)), dim3(kBS), 0, st, na, S->d_amap_src, S->d_amap_dst,
-                           K->d_val, L);
+        hipLaunchKernelGGL(scatter_kernel<T>, dim3(grid_for(na), V), dim3(kBS), 0, st, na, S->d_amap_src, S->d_amap_dst,
+                           Kval, L, vs);
     const int64_t nlev = (int64_t)H.level_ptr.size() - 1;
     for (int64_t l = 0; l < nlev; ++l) {
         int64_t ne = H.ea_off[l + 1] - H.ea_off[l];
         if (ne) {
             if (detail) (void)hipEventRecord(ctx->ev2, st);
-            hipLaunchKernelGGL(extend_add_kernel<T>, dim3((unsigned)ne), dim3(kBS), 0, st, S->d_etasks + H.ea_off[l],
+            hipLaunchKernelGGL(extend_add_kernel<T>, dim3((unsigned)ne, V), dim3(kBS), 0, st, S->d_etasks + H.ea_off[l],
                                S->d_ea_boff + H.ea_off[l], S->d_ea_bounds, S->d_ncol, S->d_m, S->d_lofs, S->d_uofs,
-                               S->d_relofs, S->d_relmap, S->d_child_ptr, S->d_child, L, U);
+                               S->d_relofs, S->d_relmap, S->d_child_ptr, S->d_child, L, U, vs);
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_EA, 1); }
         }
         if (H.step_ptr[l + 1] > H.step_ptr[l]) {
@@ -1912,8 +1937,8 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
             const int64_t nd = H.pstep_mid[stp] - H.pstep_off[stp];
             if (detail) (void)hipEventRecord(ctx->ev2, st);
             if (nd)
-                hipLaunchKernelGGL(diag_factor_kernel<T>, dim3((unsigned)nd), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status);
+                hipLaunchKernelGGL(diag_factor_kernel<T>, dim3((unsigned)nd, V), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
+                                   S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status, vs);
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, nd ? 1 : 0); }
         }
         for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
@@ -1921,23 +1946,23 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, 
             const int64_t nu = H.ustep_off[stp + 1] - H.ustep_off[stp];
             if (detail) (void)hipEventRecord(ctx->ev2, st);
             if (np)
-                hipLaunchKernelGGL(tile_solve_kernel<T>, dim3((unsigned)np), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, L);
+                hipLaunchKernelGGL(tile_solve_kernel<T>, dim3((unsigned)np, V), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
+                                   S->d_ncol, S->d_m, S->d_lofs, L, vs);
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, np ? 1 : 0); }
             if (nu) {
                 if (detail) (void)hipEventRecord(ctx->ev2, st);
-                hipLaunchKernelGGL(update_kernel<T>, dim3((unsigned)nu), dim3(kBS), 0, st, S->d_utasks + H.ustep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, L, U, 1, S->d_col0, ctx->d_status);
+                hipLaunchKernelGGL(update_kernel<T>, dim3((unsigned)nu, V), dim3(kBS), 0, st, S->d_utasks + H.ustep_off[stp],
+                                   S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, L, U, 1, S->d_col0, ctx->d_status, vs);
                 if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_UPDATE, 1); }
             }
         }
     }
     // inverse diagonal blocks and chain products for the triangular solves
     if (!H.flow_b.empty()) {
-        hipLaunchKernelGGL(diag_inverse_kernel<T>, dim3((unsigned)H.flow_b.size()), dim3(64), 0, st, S->d_flow_b, (const T *)L,
-                           Linv);
-        hipLaunchKernelGGL(chain_product_kernel<T>, dim3((unsigned)H.flow_b.size()), dim3(kBS), 0, st, S->d_flow_b,
-                           (const T *)L, Linv);
+        hipLaunchKernelGGL(diag_inverse_kernel<T>, dim3((unsigned)H.flow_b.size(), V), dim3(64), 0, st, S->d_flow_b,
+                           (const T *)L, Linv, vs);
+        hipLaunchKernelGGL(chain_product_kernel<T>, dim3((unsigned)H.flow_b.size(), V), dim3(kBS), 0, st, S->d_flow_b,
+                           (const T *)L, Linv, vs);
     }
     return hipGetLastError();
 }
@@ -2108,8 +2133,10 @@ static int do_numeric(jfem_ctx *ctx, jfem_factor *F, const jfem_matrix *K, int64
     JFEM_CHECK_HIP(ctx, launch_reset_status(ctx->stream, ctx->d_status));
     timer_start(ctx);
     hipError_t e = F->precision == 64
-                       ? numeric<double>(ctx, S, K, (double *)F->d_L, (double *)F->d_U, (double *)F->d_Linv)
-                       : numeric<float>(ctx, S, K, (float *)F->d_L, (float *)F->d_U, (float *)F->d_Linv);
+                       ? numeric<double>(ctx, S, K->d_val, (double *)F->d_L, (double *)F->d_U, (double *)F->d_Linv, 1,
+                                         VarStride{})
+                       : numeric<float>(ctx, S, K->d_val, (float *)F->d_L, (float *)F->d_U, (float *)F->d_Linv, 1,
+                                        VarStride{});
     JFEM_CHECK_HIP(ctx, e);
     timer_stop(ctx, FAM_FACTOR, 1);
     int64_t st[2];
@@ -2154,7 +2181,7 @@ int jfem_refactor(jfem_ctx *ctx, jfem_factor *F, const jfem_matrix *K, int64_t *
 }
 
 int jfem_factor_free(jfem_factor *F) {
-    if (!F) return JFEM_OK;
+    if (!F || F->view) return JFEM_OK;     // variants are freed with their batch
     (void)hipSetDevice(F->device);
     if (F->d_L) (void)hipFree(F->d_L);
     if (F->d_U) (void)hipFree(F->d_U);
@@ -2164,6 +2191,79 @@ int jfem_factor_free(jfem_factor *F) {
 }
 
 int64_t jfem_factor_device_bytes(const jfem_factor *F) { return F ? F->bytes : 0; }
+
+// numeric factors of every variant of a batch over one symbolic, in one launch
+// sequence (each launch of the factorization covers all variants)
+int jfem_batch_factor(jfem_ctx *ctx, jfem_batch *B, const jfem_symbolic *S, int precision, int64_t *bad_variant,
+                      int64_t *bad_column) {
+    if (bad_variant) *bad_variant = -1;
+    if (bad_column) *bad_column = -1;
+    if (precision != 64 && precision != 32) return fail(ctx, JFEM_ERR_BAD_ARG, "precision must be 32 or 64");
+    if (S->P != B->P) return fail(ctx, JFEM_ERR_BAD_ARG, "batch and symbolic use different patterns");
+    const SymbolicHost &H = S->H;
+    const size_t es = precision == 64 ? 8 : 4;
+    if (B->S != S || B->precision != precision) {
+        batch_free_factor(B);
+        B->L_stride = jfem_stride(std::max<int64_t>(H.L_entries, 1), es);
+        B->U_stride = jfem_stride(std::max<int64_t>(H.U_entries, 1), es);
+        B->inv_stride = jfem_stride(std::max<int64_t>(H.inv_entries, 1), es);
+        hipError_t e = hipMalloc(&B->d_L, (size_t)(B->L_stride * B->V) * es);
+        if (e == hipSuccess) e = hipMalloc(&B->d_U, (size_t)(B->U_stride * B->V) * es);
+        if (e == hipSuccess) e = hipMalloc(&B->d_Linv, (size_t)(B->inv_stride * B->V) * es);
+        if (e != hipSuccess) {
+            batch_free_factor(B);
+            return fail(ctx, JFEM_ERR_OOM, "batch factor storage");
+        }
+        B->S = S;
+        B->precision = precision;
+        for (int64_t v = 0; v < B->V; ++v) {
+            jfem_factor *F = new jfem_factor();
+            F->device = B->device;
+            F->S = S;
+            F->precision = precision;
+            F->d_L = (char *)B->d_L + v * B->L_stride * es;
+            F->d_U = (char *)B->d_U + v * B->U_stride * es;
+            F->d_Linv = (char *)B->d_Linv + v * B->inv_stride * es;
+            F->bytes = (B->L_stride + B->U_stride + B->inv_stride) * (int64_t)es;
+            F->view = true;
+            B->fviews.push_back(F);
+        }
+    }
+    VarStride vs;
+    vs.K = B->val_stride;
+    vs.L = B->L_stride;
+    vs.U = B->U_stride;
+    vs.inv = B->inv_stride;
+    vs.col = H.n;
+    JFEM_CHECK_HIP(ctx, launch_reset_status(ctx->stream, ctx->d_status));
+    timer_start(ctx);
+    hipError_t e = precision == 64
+                       ? numeric<double>(ctx, S, B->d_val, (double *)B->d_L, (double *)B->d_U, (double *)B->d_Linv,
+                                         (int)B->V, vs)
+                       : numeric<float>(ctx, S, B->d_val, (float *)B->d_L, (float *)B->d_U, (float *)B->d_Linv,
+                                        (int)B->V, vs);
+    JFEM_CHECK_HIP(ctx, e);
+    timer_stop(ctx, FAM_FACTOR, 1);
+    int64_t st[2];
+    JFEM_CHECK_HIP(ctx, hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    JFEM_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (st[1] != 0) {
+        const int64_t n = std::max<int64_t>(H.n, 1), v = st[0] / n, c = st[0] % n;
+        const int64_t g = (c >= 0 && c < H.n) ? H.perm[c] : -1;
+        if (bad_variant) *bad_variant = v;
+        if (bad_column) *bad_column = g;
+        return fail(ctx, JFEM_ERR_NOT_SPD, "variant " + std::to_string(v) + ": non-positive pivot at DOF " +
+                                               std::to_string(g));
+    }
+    return JFEM_OK;
+}
+
+int jfem_batch_factor_view(const jfem_batch *B, int64_t v, jfem_factor **view) {
+    *view = nullptr;
+    if (!B || v < 0 || v >= (int64_t)B->fviews.size()) return JFEM_ERR_BAD_ARG;
+    *view = B->fviews[(size_t)v];
+    return JFEM_OK;
+}
 
 int jfem_symbolic_order(const jfem_symbolic *S, int64_t *dofs_out) {
     if (!S) return JFEM_ERR_BAD_ARG;
@@ -2560,10 +2660,10 @@ int jfem_selftest_update(jfem_ctx *ctx, int precision, double *max_rel_diff) {
     }
     dim3 g((unsigned)tasks.size());
     if (precision == 64) {
-        hipLaunchKernelGGL(update_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[0], (double *)dU[0], 0, d_meta, ctx->d_status);
+        hipLaunchKernelGGL(update_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[0], (double *)dU[0], 0, d_meta, ctx->d_status, VarStride{});
         hipLaunchKernelGGL(update_ref_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[1], (double *)dU[1]);
     } else {
-        hipLaunchKernelGGL(update_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[0], (float *)dU[0], 0, d_meta, ctx->d_status);
+        hipLaunchKernelGGL(update_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[0], (float *)dU[0], 0, d_meta, ctx->d_status, VarStride{});
         hipLaunchKernelGGL(update_ref_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[1], (float *)dU[1]);
     }
     JFEM_CHECK_HIP(ctx, hipGetLastError());
@@ -2614,3 +2714,15 @@ int jfem_solve(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nr
 }
 
 }  // extern "C"
+
+namespace jfem {
+void batch_free_factor(jfem_batch *B) {
+    for (jfem_factor *F : B->fviews) delete F;
+    B->fviews.clear();
+    for (void *p : {B->d_L, B->d_U, B->d_Linv})
+        if (p) (void)hipFree(p);
+    B->d_L = B->d_U = B->d_Linv = nullptr;
+    B->S = nullptr;
+    B->precision = 0;
+}
+}  // namespace jfem

@@ -77,7 +77,37 @@ struct jfem_matrix {
     const jfem_pattern *P = nullptr;
     double *d_val = nullptr;
     double *d_Ke = nullptr;         // element matrices (ntrip doubles), element-major
+    bool view = false;              // a variant of a jfem_batch: storage owned by the batch
 };
+
+struct jfem_factor;
+struct jfem_symbolic;
+
+// V design variants of one topology (same mesh and pattern, own section records):
+// sections, element matrices, K values and, after jfem_batch_factor, the numeric
+// factors of all variants, each stored variant after variant at the strides below
+struct jfem_batch {
+    int device = 0;
+    const jfem_mesh *mesh = nullptr;
+    const jfem_pattern *P = nullptr;
+    int64_t V = 0;
+    double *d_qp = nullptr, *d_tp = nullptr, *d_tetp = nullptr;   // V section sets
+    double *d_Ke = nullptr, *d_val = nullptr;
+    int64_t Ke_stride = 0, val_stride = 0;                         // doubles
+    std::vector<jfem_matrix *> mviews;
+    const jfem_symbolic *S = nullptr;                              // of the factors below
+    int precision = 0;
+    void *d_L = nullptr, *d_U = nullptr, *d_Linv = nullptr;
+    int64_t L_stride = 0, U_stride = 0, inv_stride = 0;            // entries
+    std::vector<jfem_factor *> fviews;
+};
+
+// per-variant strides rounded up to 256 bytes, so every variant's storage starts
+// as aligned as a single allocation
+inline int64_t jfem_stride(int64_t n, int64_t es) {
+    const int64_t q = 256 / es;
+    return ((n + q - 1) / q) * q;
+}
 
 namespace jfem {
 int fail(jfem_ctx *ctx, int code, const std::string &msg);
@@ -85,9 +115,8 @@ void timer_start(jfem_ctx *ctx);
 void timer_stop(jfem_ctx *ctx, int family, int64_t launches);
 
 // element generation (elements.hip)
-hipError_t launch_q4_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status);
-hipError_t launch_t3_global(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status);
-hipError_t launch_tet10(hipStream_t s, const jfem_mesh *m, double *Ke, int64_t *status);
+hipError_t launch_elements(hipStream_t s, const jfem_mesh *m, const double *qp, const double *tp, const double *tetp,
+                           double *Ke, int64_t Ke_stride, int V, int kinds_mask, int64_t *status);
 hipError_t launch_q4_local(hipStream_t s, int64_t n, const double *lc, const double *Cm,
                            const double *Cb, const double *Cs, const double *h,
                            const double *k6, const double *ds, const int32_t *flags,
@@ -96,7 +125,9 @@ hipError_t launch_t3_local(hipStream_t s, int64_t n, const double *lc, const dou
                            const double *Ds, const double *k6, double *Ke, int64_t *status);
 hipError_t launch_tet10_direct(hipStream_t s, int64_t n, const double *X, const double *prop, double *Ke,
                                int64_t *status);
-hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *Ke, double *val);
+hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *Ke, double *val, int V,
+                           int64_t Ke_stride, int64_t val_stride);
+void batch_free_factor(jfem_batch *B);   // factor.hip
 hipError_t launch_bc_screen(hipStream_t s, const jfem_pattern *P, const double *val, const uint8_t *spc, double thresh,
                             uint8_t *fixed0, uint8_t *flag);
 hipError_t launch_bc_decide(hipStream_t s, const jfem_pattern *P, const double *val, const uint8_t *flag,

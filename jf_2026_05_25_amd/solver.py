@@ -524,6 +524,89 @@ class Factor:
             pass
 
 
+class DeviceBatch:
+    """Design variants of one topology on the device: the variants' section
+    records (each model must share the mesh's grid points and connectivity), their
+    element matrices and K values, and after ``factor`` the numeric factors of all
+    of them over one symbolic.  Element generation + assembly and the numeric
+    factorization each run as one launch sequence over every variant.
+
+    ``matrix(v)`` / ``factor_view(v)`` give variant v's DeviceMatrix / Factor,
+    owned by the batch (valid until ``free``)."""
+
+    def __init__(self, mesh: DeviceMesh, pattern: Pattern, models, ctx: Context | None = None):
+        self.ctx = ctx or mesh.ctx
+        self.mesh, self.pattern = mesh, pattern
+        self.V = len(models)
+        qp, tp, ep = self._props(models)
+        h = C.c_void_p()
+        check(self.ctx.h, self.ctx.L.jfem_batch_create(self.ctx.h, mesh.h, pattern.h, self.V, ptr(qp), ptr(tp),
+                                                       ptr(ep), C.byref(h)))
+        self.h = h
+        self._mviews = []
+        for v in range(self.V):
+            vh = C.c_void_p()
+            check(self.ctx.h, self.ctx.L.jfem_batch_matrix(self.h, v, C.byref(vh)))
+            K = object.__new__(DeviceMatrix)
+            K.ctx, K.pattern, K.h, K.bad_element = self.ctx, pattern, vh, -1
+            self._mviews.append(K)
+        self._fviews = []
+        self.symbolic = None
+
+    @staticmethod
+    def _props(models):
+        qp = np.concatenate([cm(m.quad_props, float) for m in models])
+        tp = np.concatenate([cm(m.tria_props, float) for m in models])
+        ep = np.concatenate([cm(m.tet_props, float) for m in models])
+        return qp, tp, ep
+
+    def update(self, models):
+        """New section records for the same number of variants."""
+        assert len(models) == self.V
+        qp, tp, ep = self._props(models)
+        check(self.ctx.h, self.ctx.L.jfem_batch_update_props(self.ctx.h, self.h, ptr(qp), ptr(tp), ptr(ep)))
+
+    def assemble(self):
+        bv = C.c_int64(-1); be = C.c_int64(-1)
+        code = self.ctx.L.jfem_batch_assemble(self.ctx.h, self.h, C.byref(bv), C.byref(be))
+        self.bad_variant, self.bad_element = bv.value, be.value
+        check(self.ctx.h, code)
+
+    def matrix(self, v: int) -> DeviceMatrix:
+        return self._mviews[v]
+
+    def factor(self, S: Symbolic, precision: int):
+        bv = C.c_int64(-1); bc = C.c_int64(-1)
+        code = self.ctx.L.jfem_batch_factor(self.ctx.h, self.h, S.h, int(precision), C.byref(bv), C.byref(bc))
+        self.bad_variant, self.bad_column = bv.value, bc.value
+        check(self.ctx.h, code)
+        if self.symbolic is not S or not self._fviews or self._fviews[0].precision != precision:
+            self._fviews = []
+            for v in range(self.V):
+                vh = C.c_void_p()
+                check(self.ctx.h, self.ctx.L.jfem_batch_factor_view(self.h, v, C.byref(vh)))
+                F = object.__new__(Factor)
+                F.ctx, F.symbolic, F.precision, F.h, F.bad_column = self.ctx, S, precision, vh, -1
+                self._fviews.append(F)
+            self.symbolic = S
+
+    def factor_view(self, v: int) -> "Factor":
+        return self._fviews[v]
+
+    def free(self):
+        if self.h:
+            for o in self._mviews + self._fviews:
+                o.h = None                    # views die with the batch
+            self.ctx.L.jfem_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def factor_with_fallback(S: Symbolic, K: DeviceMatrix, precision: int, ctx: Context | None = None):
     """Numeric factor in ``precision``; an FP32 factor that meets a non-positive
     pivot (cond(K_ff)·2^-24 too large) is redone in FP64, the way the reference
