@@ -25,7 +25,7 @@ struct jfem_symbolic {
     int device = 0;
     const jfem_pattern *P = nullptr;
     SymbolicHost H;
-    int64_t *d_col0 = nullptr, *d_ncol = nullptr, *d_m = nullptr, *d_lofs = nullptr, *d_uofs = nullptr,
+    int64_t *d_col0 = nullptr, *d_ncol = nullptr, *d_m = nullptr, *d_lofs = nullptr, *d_uofs = nullptr, *d_iofs = nullptr,
             *d_rofs = nullptr, *d_relofs = nullptr;
     int32_t *d_rows = nullptr, *d_relmap = nullptr, *d_child_ptr = nullptr, *d_child = nullptr, *d_perm = nullptr,
             *d_level_sn = nullptr;
@@ -194,11 +194,7 @@ __global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restric
     }
 }
 
-// ----------------------------------------------------------------- panel factorization
-// The b x b diagonal block of panel k is factored by one wavefront holding one
-// row per lane in registers (identity beyond b); pivots and multipliers are
-// broadcast with readlane, so there is no barrier in the loop.  On return lane i
-// holds row i of the factor L_kk in r[0..i].
+// ----------------------------------------------------------------- broadcasts
 // value of v in lane `lane` (uniform lane index), for all lanes of the wave
 template <class T>
 __device__ inline T bcast(T v, int lane);
@@ -212,169 +208,6 @@ __device__ inline double bcast<double>(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
-}
-
-// Blocked 64 x 64 Cholesky and triangular solve on a 256-thread workgroup.
-// Thread (lane = row i, wave w) keeps entries [16w, 16w+16) of row i of the
-// block in a[16].  Block step jb: wavefront jb alone eliminates its 16 columns
-// over all 64 rows (lane broadcasts, no workgroup barrier: the sequential part
-// stays inside one wavefront), publishes them in LDS, and the later wavefronts
-// apply the 16-wide update to their own columns.  Four block steps, two barriers
-// each.  (A column-at-a-time sweep needs a workgroup barrier per column, ~0.4 us
-// per column; fully unrolled on one wavefront the code overflows the
-// instruction cache and spills scalar registers, ~50 us per block.)
-
-// A = L L' in place (lower triangle; rows and columns >= b are the identity).
-// P: LDS panel of 64 x 17.  Returns the first column with a non-positive pivot,
-// b if none (uniform).
-template <class T>
-__device__ __forceinline__ int wg_potrf(T (&a)[16], T (*P)[17], int lane, int w, int b) {
-    int bad = b;
-    const int nb = (b + 15) / 16;
-    for (int jb = 0; jb < nb; ++jb) {
-        if (w == jb) {
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                const int j = 16 * jb + jj;
-                const T d = bcast(a[jj], j);
-                const bool ok = d > T(0);
-                bad = (!ok && j < b && bad == b) ? j : bad;
-                const T sq = tsqrt(ok ? d : T(1));
-                const T inv = T(1) / sq;
-                a[jj] = lane > j ? a[jj] * inv : (lane == j ? sq : a[jj]);
-#pragma unroll
-                for (int cc = jj + 1; cc < 16; ++cc) a[cc] -= a[jj] * bcast(a[jj], 16 * jb + cc);
-            }
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) P[lane][jj] = a[jj];
-        }
-        __syncthreads();
-        if (w > jb) {
-            T pl[16];
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) pl[jj] = P[lane][jj];
-#pragma unroll
-            for (int cc = 0; cc < 16; ++cc) {
-                const T *pc = P[16 * w + cc];
-#pragma unroll
-                for (int jj = 0; jj < 16; ++jj) a[cc] -= pl[jj] * pc[jj];
-            }
-        }
-        __syncthreads();
-    }
-    // the pivot check ran in the owning wavefront: make the result uniform
-    if (lane == 0 && w < nb) P[w][16] = (T)bad;
-    __syncthreads();
-    int first = b;
-    for (int q = 0; q < nb; ++q) first = min(first, (int)P[q][16]);
-    return first;
-}
-
-// X L' = A in place for the tile rows held in a, L = L_kk in LDS as
-// Ld[c][j] = L[c][j] (identity beyond the block width b)
-template <class T>
-__device__ __forceinline__ void wg_trsm(T (&a)[16], const T (*Ld)[kPanel + 4], T (*P)[17], int lane, int w, int b) {
-    const int nb = (b + 15) / 16;
-    for (int jb = 0; jb < nb; ++jb) {
-        if (w == jb) {
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) {
-                const int j = 16 * jb + jj;
-                a[jj] = a[jj] / Ld[j][j];
-#pragma unroll
-                for (int cc = jj + 1; cc < 16; ++cc) a[cc] -= a[jj] * Ld[16 * jb + cc][j];
-            }
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) P[lane][jj] = a[jj];
-        }
-        __syncthreads();
-        if (w > jb) {
-            T pl[16];
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj) pl[jj] = P[lane][jj];
-#pragma unroll
-            for (int cc = 0; cc < 16; ++cc) {
-                const T *lc = &Ld[16 * w + cc][16 * jb];
-#pragma unroll
-                for (int jj = 0; jj < 16; ++jj) a[cc] -= pl[jj] * lc[jj];
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// factor the diagonal blocks of the first panels of a level's supernodes (one
-// workgroup per task; tasks are the level's step-0 diagonal panel tasks)
-template <class T>
-__global__ void __launch_bounds__(kBS) diag_factor_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
-                                                          const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
-                                                          const int64_t *__restrict__ col0, T *__restrict__ L,
-                                                          int64_t *__restrict__ status, VarStride vs) {
-    __shared__ T P[kPanel][17];
-    L += blockIdx.y * vs.L;
-    const PanelTask t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
-    const int b = (int)min<int64_t>(kPanel, ns - k);
-    T *Ls = L + lofs[s];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int li = min(lane, b - 1);
-    T a[16];
-#pragma unroll
-    for (int cc = 0; cc < 16; ++cc) {
-        const int c = 16 * w + cc;
-        const T v = Ls[(k + li) + ms * (k + min(c, b - 1))];
-        a[cc] = (c <= lane && lane < b) ? v : (c == lane ? T(1) : T(0));
-    }
-    const int bad = wg_potrf(a, P, lane, w, b);
-    if (bad < b && tid == 0) report_min(status, blockIdx.y * vs.col + col0[s] + k + bad, JFEM_ERR_NOT_SPD);
-    if (lane < b) {
-#pragma unroll
-        for (int cc = 0; cc < 16; ++cc) {
-            const int c = 16 * w + cc;
-            if (c <= lane) Ls[(k + lane) + ms * (k + c)] = a[cc];
-        }
-    }
-}
-
-// tile solve of panel task (s, k, r0): rows [max(r0, k+b), r0+kTile) of the panel
-// get X <- X L_kk^{-T} (L_kk already factored).  One workgroup per task.
-template <class T>
-__global__ void __launch_bounds__(kBS) tile_solve_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
-                                                         const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
-                                                         T *__restrict__ L, VarStride vs) {
-    __shared__ __attribute__((aligned(16))) T Ld[kPanel][kPanel + 4];
-    L += blockIdx.y * vs.L;
-    __shared__ T P[kPanel][17];
-    const PanelTask t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
-    const int b = (int)min<int64_t>(kPanel, ns - k);
-    const int64_t rs = max<int64_t>(t.r0, k + b), re = min<int64_t>(t.r0 + kTile, ms);
-    if (rs >= re) return;
-    T *Ls = L + lofs[s];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int li = min(lane, b - 1);
-    const int64_t row = t.r0 + lane;
-    const int64_t rowc = min<int64_t>(max<int64_t>(row, rs), re - 1);
-    T a[16];
-#pragma unroll
-    for (int cc = 0; cc < 16; ++cc) {
-        const int j = 16 * w + cc;                 // L_kk column j, row `lane`
-        const T v = Ls[(k + li) + ms * (k + min(j, b - 1))];
-        Ld[lane][j] = (j <= lane && lane < b) ? v : (j == lane ? T(1) : T(0));
-        const T x = Ls[rowc + ms * (k + min(j, b - 1))];
-        a[cc] = j < b ? x : T(0);
-    }
-    __syncthreads();
-    wg_trsm(a, Ld, P, lane, w, b);
-    if (row >= rs && row < re) {
-#pragma unroll
-        for (int cc = 0; cc < 16; ++cc) {
-            const int j = 16 * w + cc;
-            if (j < b) Ls[row + ms * (k + j)] = a[cc];
-        }
-    }
 }
 
 // inverse of each panel's diagonal block, for the triangular solves (one
@@ -492,6 +325,221 @@ struct Mfma<float> {
     static __device__ inline int drow(int blk, int v) { return 4 * blk + v; }
 };
 
+// ----------------------------------------------------------------- panel factorization
+// 64 x 64 Cholesky of a panel's diagonal block on a 256-thread workgroup, with
+// the inverses of its four diagonal 16 x 16 blocks.  Thread (lane = row i, wave
+// w) keeps entries [16w, 16w+16) of row i in a[16] (identity beyond the block
+// width b).  Block step jb: wavefront jb alone eliminates its 16 columns over all
+// 64 rows (pivots and multipliers broadcast with readlane: the sequential part
+// stays inside one wavefront, with no barrier), publishes them in an LDS panel
+// (double-buffered, rows padded to 16 bytes for 128-bit broadcast reads), and the
+// later wavefronts apply the 16-wide update to their own columns: one workgroup
+// barrier per block step.  Pivots use the hardware reciprocal square root
+// (L_jj = d * rsq(d), column scaled by rsq(d)).
+// Then wavefront w forms D_w^{-1}, the inverse of diagonal block w, by forward
+// substitution in FP64 (lane c: column c; entries of D_w broadcast from the
+// registers), rounded once to T: the tile solves of the panel rows use them.
+template <class T>
+__device__ inline T rsq_hw(T x);
+template <>
+__device__ inline float rsq_hw<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <>
+__device__ inline double rsq_hw<double>(double x) { return rsqrt(x); }
+
+template <class T>
+struct PanelLds {
+    static constexpr int kRow = 16 + 16 / (int)sizeof(T);          // padded panel row (elements)
+    static constexpr int bytes = 2 * 64 * kRow * (int)sizeof(T) + 16;
+};
+
+// A = L L' in place for the block (rows/columns >= b: identity).  lds: PanelLds<T>::bytes
+// of 16-byte aligned shared memory.  Stores L's lower triangle at Lblk (leading
+// dimension ld, rows/columns < b) and the lower triangles of D_w^{-1} at their
+// positions in the b x b column-major block inverse at inv.  Returns the first
+// column with a non-positive pivot, b if none (uniform).
+template <class T>
+__device__ __forceinline__ int potrf_dinv(T (&a)[16], char *lds, int lane, int w, int b, T *Lblk, int64_t ld, T *inv) {
+    constexpr int PS = PanelLds<T>::kRow;
+    T (*P)[64][PS] = reinterpret_cast<T (*)[64][PS]>(lds);
+    int *badw = reinterpret_cast<int *>(lds + 2 * 64 * PS * sizeof(T));
+    int bad = b;
+    const int nb = (b + 15) / 16;
+    for (int jb = 0; jb < nb; ++jb) {
+        if (w == jb) {
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int j = 16 * jb + jj;
+                const T d = bcast(a[jj], j);
+                const bool ok = d > T(0);
+                bad = (!ok && j < b && bad == b) ? j : bad;
+                const T dd = ok ? d : T(1);
+                const T r = rsq_hw(dd);
+                a[jj] = lane > j ? a[jj] * r : (lane == j ? dd * r : a[jj]);
+#pragma unroll
+                for (int cc = jj + 1; cc < 16; ++cc) a[cc] -= a[jj] * bcast(a[jj], 16 * jb + cc);
+            }
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) P[jb & 1][lane][jj] = a[jj];
+        }
+        __syncthreads();
+        if (w > jb) {
+            T pl[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) pl[jj] = P[jb & 1][lane][jj];
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc) {
+                const T *pc = P[jb & 1][16 * w + cc];
+                T s = T(0);
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj) s += pl[jj] * pc[jj];
+                a[cc] -= s;
+            }
+        }
+        // (the next block's owner writes the other buffer; this one is rewritten
+        // two block steps later, after every wave has passed the barrier between)
+    }
+    if (lane == 0) badw[w] = w < nb ? bad : b;
+    if (lane < b) {
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) {
+            const int c = 16 * w + cc;
+            if (c <= lane) Lblk[lane + ld * c] = a[cc];
+        }
+    }
+    if (w < nb) {
+        const int o = 16 * w, c = lane & 15;
+        T dl = a[0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) dl = (lane - o == r) ? a[r] : dl;
+        const double rdl = 1.0 / (double)dl;
+        double y[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) y[r] = r == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            y[r] *= bcast(rdl, o + r);
+#pragma unroll
+            for (int q = r + 1; q < 16; ++q) y[q] = fma(-(double)bcast(a[r], o + q), y[r], y[q]);
+        }
+        if (lane < 16 && o + c < b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                if (r >= c && o + r < b) inv[(o + r) + (int64_t)b * (o + c)] = (T)y[r];
+    }
+    __syncthreads();
+    int first = b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) first = min(first, badw[q]);
+    return first;
+}
+
+// factor the diagonal blocks of the first panels of a level's supernodes (one
+// workgroup per task; tasks are the level's step-0 diagonal panel tasks)
+template <class T>
+__global__ void __launch_bounds__(kBS) diag_factor_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+                                                          const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
+                                                          const int64_t *__restrict__ iofs, const int64_t *__restrict__ col0,
+                                                          T *__restrict__ L, T *__restrict__ Linv,
+                                                          int64_t *__restrict__ status, VarStride vs) {
+    __shared__ __attribute__((aligned(16))) char lds[PanelLds<T>::bytes];
+    L += blockIdx.y * vs.L;
+    Linv += blockIdx.y * vs.inv;
+    const PanelTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
+    const int b = (int)min<int64_t>(kPanel, ns - k);
+    T *Ls = L + lofs[s];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = min(lane, b - 1);
+    T a[16];
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) {
+        const int c = 16 * w + cc;
+        const T v = Ls[(k + li) + ms * (k + min(c, b - 1))];
+        a[cc] = (c <= lane && lane < b) ? v : (c == lane ? T(1) : T(0));
+    }
+    const int bad = potrf_dinv(a, lds, lane, w, b, Ls + k + ms * k, ms, Linv + iofs[s] + (int64_t)kPanel * kPanel * (k / kPanel));
+    if (bad < b && tid == 0) report_min(status, blockIdx.y * vs.col + col0[s] + k + bad, JFEM_ERR_NOT_SPD);
+}
+
+// tile solve of panel task (s, k, r0): rows [max(r0, k+b), r0+kTile) of the panel
+// get X <- X L_kk^{-T}, on the matrix cores, blocked by 16 columns.  Wave w takes
+// tile rows [16w, 16w+16) and needs no other wave: with Y_j = X_j' (columns
+// [16j, 16j+16) of its rows, transposed), Y_j = D_j^{-1} (A_j' - sum_{q<j} L_jq Y_q)
+// where L_jq are 16-blocks of L_kk and D_j^{-1} the inverses of its diagonal
+// blocks (written by the factorization of L_kk).  The transposed form puts the
+// accumulator's lane index along the column-major rows, so loads and stores of
+// the tile are coalesced.  Operand layout of the 16x16x4 MFMA as for the update.
+template <class T>
+__global__ void __launch_bounds__(kBS) tile_solve_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+                                                         const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
+                                                         const int64_t *__restrict__ iofs, T *__restrict__ L,
+                                                         const T *__restrict__ Linv, VarStride vs) {
+    typedef typename Mfma<T>::v4 v4;
+    __shared__ T Lb[6][16][17];          // L_jq (j > q) at index j(j-1)/2 + q
+    __shared__ T Di[4][16][17];          // D_j^{-1} (identity beyond b)
+    __shared__ T Ys[4][4][16][16];       // per wave: Y_j, and the staging of the right-hand factor
+    L += blockIdx.y * vs.L;
+    Linv += blockIdx.y * vs.inv;
+    const PanelTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
+    const int b = (int)min<int64_t>(kPanel, ns - k);
+    const int64_t rs = max<int64_t>(t.r0, k + b), re = min<int64_t>(t.r0 + kTile, ms);
+    if (rs >= re) return;
+    T *Ls = L + lofs[s];
+    const T *Dg = Linv + iofs[s] + (int64_t)kPanel * kPanel * (k / kPanel);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int nbj = (b + 15) / 16;
+    // this lane's row of the tile (accumulator column l16 of wave w)
+    const int64_t row = t.r0 + 16 * w + l16;
+    const bool rowok = row >= rs && row < re;
+    const int64_t rowc = min<int64_t>(max<int64_t>(row, rs), re - 1);
+    // the tile's blocks A_j' in accumulator layout: element v = A[row][16j + drow(l4, v)]
+    T pre[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int col = 16 * j + Mfma<T>::drow(l4, v);
+            pre[j][v] = Ls[rowc + ms * (k + min(col, b - 1))];
+            if (!(rowok && col < b)) pre[j][v] = T(0);
+        }
+    for (int e = tid; e < 6 * 256; e += kBS) {
+        const int blk = e >> 8, r = e & 15, c = (e >> 4) & 15;
+        const int j = blk < 1 ? 1 : (blk < 3 ? 2 : 3), q = blk - j * (j - 1) / 2;
+        const int rr = 16 * j + r, cc = 16 * q + c;
+        Lb[blk][r][c] = (rr < b && cc < b) ? Ls[(k + rr) + ms * (k + cc)] : T(0);
+    }
+    for (int e = tid; e < 4 * 256; e += kBS) {
+        const int j = e >> 8, r = e & 15, c = (e >> 4) & 15;
+        const int rr = 16 * j + r, cc = 16 * j + c;
+        Di[j][r][c] = (rr < b && cc < b) ? (r >= c ? Dg[rr + (int64_t)b * cc] : T(0)) : (r == c ? T(1) : T(0));
+    }
+    __syncthreads();
+    for (int j = 0; j < nbj; ++j) {
+        v4 acc = v4{pre[j][0], pre[j][1], pre[j][2], pre[j][3]};
+        for (int q = 0; q < j; ++q) {
+            const int blk = j * (j - 1) / 2 + q;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                acc = Mfma<T>::mac(-Lb[blk][l16][4 * kk + l4], Ys[w][q][4 * kk + l4][l16], acc);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ys[w][j][Mfma<T>::drow(l4, v)][l16] = acc[v];
+        v4 y = v4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) y = Mfma<T>::mac(Di[j][l16][4 * kk + l4], Ys[w][j][4 * kk + l4][l16], y);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            Ys[w][j][Mfma<T>::drow(l4, v)][l16] = y[v];
+            const int col = 16 * j + Mfma<T>::drow(l4, v);
+            if (rowok && col < b) Ls[row + ms * (k + col)] = y[v];
+        }
+    }
+}
+
 // factor_next != 0: the task whose tile holds the next panel's diagonal block
 // (r0 == c0 == k + b < ns) also factors that block after its update (staged
 // through LDS, blocked over the four wavefronts), so the next step's tile solves
@@ -499,14 +547,17 @@ struct Mfma<float> {
 template <class T>
 __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                      const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
-                                                     const int64_t *__restrict__ uofs, T *__restrict__ L, T *__restrict__ U,
+                                                     const int64_t *__restrict__ uofs, const int64_t *__restrict__ iofs,
+                                                     T *__restrict__ L, T *__restrict__ U, T *__restrict__ Linv,
                                                      int factor_next, const int64_t *__restrict__ col0,
                                                      int64_t *__restrict__ status, VarStride vs) {
     typedef typename Mfma<T>::v4 v4;
     L += blockIdx.y * vs.L;
     U += blockIdx.y * vs.U;
+    Linv += blockIdx.y * vs.inv;
     __shared__ T A[kPanel][kTile + 1];   // A[p][i] = L[r0 + i, k + p]
-    __shared__ T B[kPanel][kTile + 1];   // B[p][j] = L[c0 + j, k + p]
+    __shared__ __attribute__((aligned(16))) T B[kPanel][kTile + 1];   // B[p][j] = L[c0 + j, k + p]
+    static_assert(PanelLds<T>::bytes <= (int)sizeof(T) * kPanel * (kTile + 1), "panel LDS fits in B");
     const UpdateTask t = tasks[blockIdx.x];
     const int s = t.s;
     const int64_t ns = ncol[s], ms = mrow[s], k = t.k;
@@ -573,8 +624,10 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
                 const int j = wj + 16 * y + Mfma<T>::drow(l4, v);   // column of C within the tile
                 if (i < nr && j < nc && r0 + i >= c0 + j) {
                     const T nv = cpre[x][y][v] - acc[x][y][v];
-                    Cs[(r0 + i - off) + ld * (c0 + j - off)] = nv;
-                    if (next) A[j][i] = nv;            // A[col][row] of the next diagonal block
+                    if (next)
+                        A[j][i] = nv;                  // A[col][row] of the next diagonal block (factored below)
+                    else
+                        Cs[(r0 + i - off) + ld * (c0 + j - off)] = nv;
                 }
             }
     if (!next) return;
@@ -586,15 +639,9 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
         const int c = 16 * wave + cc;
         a[cc] = (c <= lane && lane < bn) ? A[c][lane] : (c == lane ? T(1) : T(0));
     }
-    const int bad = wg_potrf(a, reinterpret_cast<T (*)[17]>(&B[0][0]), lane, wave, bn);
+    const int bad = potrf_dinv(a, reinterpret_cast<char *>(&B[0][0]), lane, wave, bn, L + lofs[s] + c0 + ms * c0, ms,
+                               Linv + iofs[s] + (int64_t)kPanel * c0);
     if (bad < bn && tid == 0) report_min(status, blockIdx.y * vs.col + col0[s] + c0 + bad, JFEM_ERR_NOT_SPD);
-    if (lane < bn) {
-#pragma unroll
-        for (int cc = 0; cc < 16; ++cc) {
-            const int c = 16 * wave + cc;
-            if (c <= lane) L[lofs[s] + (c0 + lane) + ms * (c0 + c)] = a[cc];
-        }
-    }
 }
 
 // chain products of the triangular solves, one workgroup per panel boundary q
@@ -1938,7 +1985,7 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
             if (detail) (void)hipEventRecord(ctx->ev2, st);
             if (nd)
                 hipLaunchKernelGGL(diag_factor_kernel<T>, dim3((unsigned)nd, V), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, S->d_col0, L, ctx->d_status, vs);
+                                   S->d_ncol, S->d_m, S->d_lofs, S->d_iofs, S->d_col0, L, Linv, ctx->d_status, vs);
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, nd ? 1 : 0); }
         }
         for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
@@ -1947,12 +1994,13 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
             if (detail) (void)hipEventRecord(ctx->ev2, st);
             if (np)
                 hipLaunchKernelGGL(tile_solve_kernel<T>, dim3((unsigned)np, V), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, L, vs);
+                                   S->d_ncol, S->d_m, S->d_lofs, S->d_iofs, L, (const T *)Linv, vs);
             if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, np ? 1 : 0); }
             if (nu) {
                 if (detail) (void)hipEventRecord(ctx->ev2, st);
                 hipLaunchKernelGGL(update_kernel<T>, dim3((unsigned)nu, V), dim3(kBS), 0, st, S->d_utasks + H.ustep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, L, U, 1, S->d_col0, ctx->d_status, vs);
+                                   S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, S->d_iofs, L, U, Linv, 1, S->d_col0,
+                                   ctx->d_status, vs);
                 if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_UPDATE, 1); }
             }
         }
@@ -2072,6 +2120,7 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_m, H.m, st);
     if (e == hipSuccess) e = up(S, &S->d_lofs, H.lofs, st);
     if (e == hipSuccess) e = up(S, &S->d_uofs, H.uofs, st);
+    if (e == hipSuccess) e = up(S, &S->d_iofs, H.iofs, st);
     if (e == hipSuccess) e = up(S, &S->d_rofs, H.rofs, st);
     if (e == hipSuccess) e = up(S, &S->d_relofs, H.relofs, st);
     if (e == hipSuccess) e = up(S, &S->d_rows, H.rows, st);
@@ -2660,10 +2709,10 @@ int jfem_selftest_update(jfem_ctx *ctx, int precision, double *max_rel_diff) {
     }
     dim3 g((unsigned)tasks.size());
     if (precision == 64) {
-        hipLaunchKernelGGL(update_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[0], (double *)dU[0], 0, d_meta, ctx->d_status, VarStride{});
+        hipLaunchKernelGGL(update_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, d_meta + 2, (double *)dL[0], (double *)dU[0], (double *)nullptr, 0, d_meta, ctx->d_status, VarStride{});
         hipLaunchKernelGGL(update_ref_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[1], (double *)dU[1]);
     } else {
-        hipLaunchKernelGGL(update_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[0], (float *)dU[0], 0, d_meta, ctx->d_status, VarStride{});
+        hipLaunchKernelGGL(update_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, d_meta + 2, (float *)dL[0], (float *)dU[0], (float *)nullptr, 0, d_meta, ctx->d_status, VarStride{});
         hipLaunchKernelGGL(update_ref_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[1], (float *)dU[1]);
     }
     JFEM_CHECK_HIP(ctx, hipGetLastError());

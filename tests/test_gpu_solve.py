@@ -8,9 +8,9 @@ The FP64 path then matches it within 1e-10 relative (L2) on displacements; the
 FP32 path, stopped at residual 1e-10, within 1e-9 (its error is the residual
 amplified by the conditioning), and within 1e-10 when refined to residual
 1e-12.  The plain FP64 solve
-(no refinement) is as accurate as the oracle's plain solve, i.e. it differs
-from the reference solution by no more than 10x the oracle's own plain-solve
-error (both are cond(K)·eps-limited; orderings differ).  Edge cases: single
+(no refinement) is backward stable like the oracle's plain solve: its normwise
+backward error is within 10x of the oracle's, and its forward error within
+100x (both are cond(K)·eps-limited noise; orderings differ).  Edge cases: single
 element, fully clamped (no free DOF), zero load, a mechanism (not positive
 definite), many right-hand sides.
 """
@@ -89,14 +89,30 @@ def test_fp32_refined_converges_to_reference_solution(ctx, oracle, name):
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_plain_fp64_as_accurate_as_oracle(ctx, oracle, name):
+    """The unrefined FP64 solve is backward stable like the oracle's: normwise
+    backward error ||K x - b|| / (||K||_F ||x|| + ||b||) within 10x of the
+    oracle's plain solve.  The forward error of either is rounding noise of size
+    up to cond(K) eps (these matrices reach cond ~1e9, where reordering the
+    roundings of one Cholesky moves it tenfold), so it gets a looser 100x bound."""
+    import scipy.sparse.linalg as spla
     m = CASES[name]()
-    _, _, ue, _, _ = oracle.sol101(m, refine=10)        # reference solution
+    K, fixed, ue, _, _ = oracle.sol101(m, refine=10)    # reference solution
     _, _, up, _, _ = oracle.sol101(m)                   # oracle plain solve
     u, diag = gpu_solve(ctx, m, 64, max_refine=0)       # GPU plain solve
+    free = np.where(fixed == 0)[0]
+    Kff = K[free][:, free]
+    nK = spla.norm(Kff)
+
+    def backward(x, b):
+        return np.linalg.norm(Kff @ x - b) / (nK * np.linalg.norm(x) + np.linalg.norm(b))
+
     for r in range(m.nrhs):
+        b = m.loads[free, r]
+        eta_gpu, eta_orc = backward(u[free, r], b), backward(up[free, r], b)
+        assert eta_gpu <= max(10 * eta_orc, 1e-18), (r, eta_gpu, eta_orc)
         e_gpu = relerr(u[:, r], ue[:, r])
         e_orc = relerr(up[:, r], ue[:, r])
-        assert e_gpu <= max(10 * e_orc, 1e-13), (r, e_gpu, e_orc)
+        assert e_gpu <= max(100 * e_orc, 1e-13), (r, e_gpu, e_orc)
 
 
 def test_golden_vectors_on_gpu(ctx, oracle):
