@@ -190,6 +190,25 @@ int jfem_batch_free(jfem_batch *batch);
 int jfem_bc_partition(jfem_ctx *ctx, const jfem_matrix *K, const uint8_t *spc,
                       double eps_zero, uint8_t *fixed_out, int64_t *n_free);
 
+/* ---- shell recovery -------------------------------------------------------- */
+/* Force / stress / strain recovery of a mesh's isotropic PSHELL CQUAD4 and CTRIA3
+ * elements from one displacement vector, the counterpart of recover_shell_stresses!
+ * (src/solver/stress_recovery.jl:505; called from src/solver/solve_case.jl:1204)
+ * for the default (no offset, no SNORM, no thermal) branches.  u: ndof doubles in
+ * global DOF order (6 per grid point, basic frame).  Per CQUAD4, q4_out holds 44
+ * doubles (element frame): N[3] M[3] Q[2], stress at z = -h/2 [3] and +h/2 [3],
+ * strain at -h/2 [3] and +h/2 [3], corner N[4][3], corner M[4][3] (corners in
+ * connectivity order; M the printed moment).  Per CTRIA3, t3_out holds the first 20.
+ * mean_shear != 0 selects the branch of solution sequences 101/105/106 (MacNeal
+ * area-mean shear, corner moments from the section fields,
+ * stress_recovery.jl:873-946); 0 that of other sequences (MITC4 centre shear
+ * blended with the equilibrium shear of the corner moments, :948-952).
+ * Either output may be NULL when the mesh has no element of its kind; a
+ * degenerate element returns JFEM_ERR_DEGENERATE with bad_element its index
+ * (CQUAD4s first, then CTRIA3s). */
+int jfem_shell_recover(jfem_ctx *ctx, const jfem_mesh *mesh, const double *u, int mean_shear,
+                       double *q4_out, double *t3_out, int64_t *bad_element);
+
 /* ---- factor / solve -------------------------------------------------------- */
 typedef struct {
     int64_t n_free;          /* order of K_ff */

@@ -24,7 +24,7 @@ STATUS = {1: "HIP error", 2: "degenerate element", 3: "inverted element", 4: "wa
 EXPORTS = [
     "jfem_ctx_create", "jfem_ctx_destroy", "jfem_ctx_synchronize", "jfem_last_error", "jfem_device_memory",
     "jfem_ctx_kernel_time", "jfem_ctx_reset_kernel_time", "jfem_ctx_enable_timing",
-    "jfem_element_matrices", "jfem_q4_local_batch", "jfem_t3_local_batch", "jfem_tet10_batch",
+    "jfem_element_matrices", "jfem_q4_local_batch", "jfem_t3_local_batch", "jfem_tet10_batch", "jfem_shell_recover",
     "jfem_mesh_create", "jfem_mesh_update_props", "jfem_mesh_free",
     "jfem_pattern_create", "jfem_pattern_free", "jfem_assemble", "jfem_reassemble", "jfem_matrix_download",
     "jfem_matrix_free", "jfem_bc_partition",
@@ -83,6 +83,7 @@ def load():
         "jfem_q4_local_batch": ([vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, P(i64)], C.c_int),
         "jfem_t3_local_batch": ([vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, P(i64)], C.c_int),
         "jfem_tet10_batch": ([vp, i64, vp, vp, vp, vp, P(i64)], C.c_int),
+        "jfem_shell_recover": ([vp, vp, vp, C.c_int, vp, vp, P(i64)], C.c_int),
         "jfem_mesh_create": ([vp, i64, vp, vp, vp, i64, vp, vp, i64, vp, vp, i64, vp, vp, dbl, P(vp)], C.c_int),
         "jfem_mesh_update_props": ([vp, vp, vp, vp, vp], C.c_int),
         "jfem_mesh_free": ([vp], C.c_int),

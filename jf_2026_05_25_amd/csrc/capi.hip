@@ -323,6 +323,38 @@ int jfem_tet10_batch(jfem_ctx *ctx, int64_t n, const double *X, const double *E,
                       });
 }
 
+// ------------------------------------------------------------------------- shell recovery
+int jfem_shell_recover(jfem_ctx *ctx, const jfem_mesh *m, const double *u, int mean_shear, double *q4_out,
+                       double *t3_out, int64_t *bad) {
+    if (bad) *bad = -1;
+    if (!ctx || !m || !u) return fail(ctx, JFEM_ERR_BAD_ARG, "null argument");
+    if ((m->nq > 0 && !q4_out) || (m->nt > 0 && !t3_out)) return fail(ctx, JFEM_ERR_BAD_ARG, "null output");
+    if (m->nq + m->nt == 0) return JFEM_OK;
+    hipStream_t s = ctx->stream;
+    const int64_t ndof = 6 * m->n_nodes;
+    double *d_u = nullptr, *d_q = nullptr, *d_t = nullptr;
+    hipError_t e = upload(&d_u, u, (size_t)ndof, s);
+    if (e == hipSuccess && m->nq) e = hipMalloc((void **)&d_q, (size_t)m->nq * 44 * sizeof(double));
+    if (e == hipSuccess && m->nt) e = hipMalloc((void **)&d_t, (size_t)m->nt * 20 * sizeof(double));
+    if (e == hipSuccess) e = launch_reset_status(s, ctx->d_status);
+    if (e == hipSuccess) e = launch_shell_recover(s, m, d_u, mean_shear, d_q, d_t, ctx->d_status);
+    int64_t st[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && m->nq)
+        e = hipMemcpyAsync(q4_out, d_q, (size_t)m->nq * 44 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && m->nt)
+        e = hipMemcpyAsync(t3_out, d_t, (size_t)m->nt * 20 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (double *p : {d_u, d_q, d_t})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return fail(ctx, e == hipErrorOutOfMemory ? JFEM_ERR_OOM : JFEM_ERR_HIP, hipGetErrorString(e));
+    if (st[1] != 0) {
+        if (bad) *bad = st[0];
+        return fail(ctx, (int)st[1], "element " + std::to_string(st[0]) + " rejected");
+    }
+    return JFEM_OK;
+}
+
 // ------------------------------------------------------------------------- pattern
 int jfem_pattern_create(jfem_ctx *ctx, const jfem_mesh *m, jfem_pattern **out, int64_t *nnz_out) {
     *out = nullptr;

@@ -1492,6 +1492,384 @@ __global__ void reset_status_kernel(int64_t *status) {
 
 }  // namespace
 
+// ----------------------------------------------------------------- shell recovery
+// Force / stress / strain recovery of isotropic PSHELL shells from a displacement
+// vector (recover_shell_stresses!, src/solver/stress_recovery.jl:505-1268), one
+// element per thread: a post-processing pass, so the element's operators are
+// rebuilt in registers and the 24 x 4 / 12 x 12 couplings are never stored (only
+// their products with the element's displacements).
+// CQUAD4 output (44 doubles per element): N[3] M[3] Q[2] stress(-h/2)[3]
+// stress(+h/2)[3] strain(-h/2)[3] strain(+h/2)[3] corner N[4][3] corner M[4][3];
+// CTRIA3 output: the first 20.
+
+// bilinear extrapolation of 2 x 2 Gauss rows G[3g + c] to (xi, eta)
+// (interp_2x2_gauss_sigma, src/experimental/plate_kernels.jl:53-70)
+__device__ inline void rec_interp(const double *G, double xi, double eta, double *o) {
+    const double pt = 1.0 / sqrt(3.0);
+    const double lx1 = (pt - xi) / (2.0 * pt), lx2 = (xi + pt) / (2.0 * pt);
+    const double ly1 = (pt - eta) / (2.0 * pt), ly2 = (eta + pt) / (2.0 * pt);
+    const double w1 = lx1 * ly1, w2 = lx2 * ly1, w3 = lx2 * ly2, w4 = lx1 * ly2;
+    for (int c = 0; c < 3; ++c) o[c] = w1 * G[c] + w2 * G[3 + c] + w3 * G[6 + c] + w4 * G[9 + c];
+}
+
+// u <- Tg u for a warped cell, Tg the finite-warp displacement map (identity plus
+// the entries of _quad4_finite_warp_fill!, src/FEMKernels.jl:880-945), applied
+// entry by entry from a copy of the physical displacements
+__device__ void rec_warp_map(const double *lc, const double *zl, double *u) {
+    double u0[24];
+    for (int q = 0; q < 24; ++q) u0[q] = u[q];
+    const double cr[4] = {-1, 1, 1, -1}, cs_[4] = {-1, -1, 1, 1};
+    for (int i = 0; i < 4; ++i) {
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(cr[i], cs_[i], dNr, dNs);
+        jac4(dNr, dNs, lc, J11, J12, J21, J22);
+        const double dj = J11 * J22 - J12 * J21;
+        if (fabs(dj) < 1e-30) continue;
+        const double i11 = J22 / dj, i12 = -J12 / dj, i21 = -J21 / dj, i22 = J11 / dj;
+        const double zr = dNr[0] * zl[0] + dNr[1] * zl[1] + dNr[2] * zl[2] + dNr[3] * zl[3];
+        const double zs = dNs[0] * zl[0] + dNs[1] * zl[1] + dNs[2] * zl[2] + dNs[3] * zl[3];
+        const double gx = i11 * zr + i12 * zs, gy = i21 * zr + i22 * zs;
+        double *urx = u + 6 * i + 3, *ury = u + 6 * i + 4;
+        *urx += gx * u0[6 * i + 5];
+        *ury += gy * u0[6 * i + 5];
+        const int a = (i % 2 == 0) ? 0 : 1, b = (i % 2 == 0) ? 2 : 3;
+        const double dx = lc[2 * b] - lc[2 * a], dy = lc[2 * b + 1] - lc[2 * a + 1];
+        const double dl2 = dx * dx + dy * dy;
+        if (dl2 > 1e-30) {
+            const int js[2] = {a, b};
+            const double cu[2] = {dy / dl2, -dy / dl2}, cv[2] = {-dx / dl2, dx / dl2};
+            for (int t = 0; t < 2; ++t) {
+                const int j = js[t];
+                *urx -= (gx * cu[t]) * u0[6 * j] + (gx * cv[t]) * u0[6 * j + 1];
+                *ury -= (gy * cu[t]) * u0[6 * j] + (gy * cv[t]) * u0[6 * j + 1];
+            }
+        }
+        for (int j = 0; j < 4; ++j) {
+            const double dNdx = i11 * dNr[j] + i12 * dNs[j], dNdy = i21 * dNr[j] + i22 * dNs[j];
+            u[6 * i + 0] += (zl[i] * dNdx) * u0[6 * j + 2];
+            u[6 * i + 1] += (zl[i] * dNdy) * u0[6 * j + 2];
+        }
+    }
+}
+
+// 3-vector C x (C column-major 3 x 3)
+__device__ inline void rec_mv3(const double *C, const double *x, double *y) {
+    for (int l = 0; l < 3; ++l) y[l] = C[l] * x[0] + C[l + 3] * x[1] + C[l + 6] * x[2];
+}
+
+// CQUAD4: stress_strain_quad4 default path (src/FEMKernels.jl:6759-6975) with the
+// SOL 101/105/106 section corner moments and MacNeal area-mean shear
+// (stress_recovery.jl:873-946, quad4_macneal_mean_shear_resultant :6623-6645,
+// quad4_condensed_section_fields :7140-7209) when mean_shear, else the MITC4
+// centre shear (:6515-6599) blended with the equilibrium shear of the compatible
+// corner moments (stress_recovery.jl:126-168, 948-952).  Returns a status code.
+__device__ int q4_recover(const double *p, const double *pr, const double *ug, int mean_shear, double *out) {
+    Q4Prep P;
+    q4_setup(P, p, pr, 0.0);
+    const double *lc = P.lc;
+    double u[24];
+    for (int k = 0; k < 4; ++k)           // element frame (stress_recovery.jl:609-617)
+        for (int hf = 0; hf < 2; ++hf)
+            for (int i = 0; i < 3; ++i) {
+                const double *s = ug + 6 * k + 3 * hf;
+                u[6 * k + 3 * hf + i] = P.R[3 * i] * s[0] + P.R[3 * i + 1] * s[1] + P.R[3 * i + 2] * s[2];
+            }
+    if (P.warped) rec_warp_map(lc, P.zl, u);
+    const double h = pr[JFEM_SP_H], E = pr[JFEM_SP_E], nu = pr[JFEM_SP_NU], br = pr[JFEM_SP_BEND_RATIO];
+    const double tst = pr[JFEM_SP_TS_T];
+    // stress_strain_quad4's plane-stress law D and membrane matrix Cm = D h;
+    // section matrices A = h C, D = br h^3/12 C of the mean-shear data
+    const double cmem = E / (1 - nu * nu);
+    const double Dm3[9] = {cmem * 1.0, cmem * nu, cmem * 0.0, cmem * nu, cmem * 1.0, cmem * 0.0,
+                           cmem * 0.0, cmem * 0.0, cmem * ((1 - nu) / 2)};
+    const double pc = E / fmax(1.0 - nu * nu, 1e-30);
+    const double Csec[9] = {pc * 1.0, pc * nu, pc * 0.0, pc * nu, pc * 1.0, pc * 0.0, pc * 0.0, pc * 0.0,
+                            pc * ((1.0 - nu) / 2.0)};
+    double Cm[9], Asec[9], Dsec[9], Cbc[9];
+    for (int q = 0; q < 9; ++q) {
+        Cm[q] = Dm3[q] * h;
+        Asec[q] = h * Csec[q];
+        Dsec[q] = ((br * (h * h * h)) / 12) * Csec[q];
+        Cbc[q] = (br * Dm3[q]) * ((h * h * h) / 12.0);
+    }
+    const double dNr_c[4] = {-0.25, 0.25, 0.25, -0.25}, dNs_c[4] = {-0.25, -0.25, 0.25, 0.25};
+    double J11c, J12c, J21c, J22c;
+    jac4(dNr_c, dNs_c, lc, J11c, J12c, J21c, J22c);
+    const double detJc = J11c * J22c - J12c * J21c;
+    const double iJ11c = J22c / detJc, iJ12c = -J12c / detJc, iJ21c = -J21c / detJc, iJ22c = J11c / detJc;
+    const double pt = 1.0 / sqrt(3.0);
+    const double gr[4] = {-pt, pt, pt, -pt}, gs[4] = {-pt, -pt, pt, pt};
+    // per Gauss point: Bm u, Bb u, the internal-mode B generators, the weight
+    double em[4][3], kb[4][3], bi[4][4], bb[4][4], wg[4];
+    double Kabu[4] = {0, 0, 0, 0}, Kbb[16], Kabsu[4] = {0, 0, 0, 0}, Kbbs[16];
+    for (int q = 0; q < 16; ++q) { Kbb[q] = 0.0; Kbbs[q] = 0.0; }
+    for (int g = 0; g < 4; ++g) {
+        const double r = gr[g], s = gs[g];
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(r, s, dNr, dNs);
+        jac4(dNr, dNs, lc, J11, J12, J21, J22);
+        const double detJ = J11 * J22 - J12 * J21;
+        if (!jac_ok(detJ, J11, J12, J21, J22)) return JFEM_ERR_DEGENERATE;
+        const double iJ11 = J22 / detJ, iJ12 = -J12 / detJ, iJ21 = -J21 / detJ, iJ22 = J11 / detJ;
+        double e0 = 0, e1 = 0, e2 = 0, k0 = 0, k1 = 0, k2 = 0;
+        for (int k = 0; k < 4; ++k) {
+            const double dx = iJ11 * dNr[k] + iJ12 * dNs[k], dy = iJ21 * dNr[k] + iJ22 * dNs[k];
+            const double *uk = u + 6 * k;
+            e0 += dx * uk[0]; e1 += dy * uk[1]; e2 += dy * uk[0] + dx * uk[1];
+            k0 += dx * uk[4]; k1 += -dy * uk[3]; k2 += dy * uk[4] - dx * uk[3];
+        }
+        em[g][0] = e0; em[g][1] = e1; em[g][2] = e2;
+        kb[g][0] = k0; kb[g][1] = k1; kb[g][2] = k2;
+        wg[g] = fabs(detJ);
+        // membrane modes (patch-consistent Jacobian), bending modes (qm6 Jacobian)
+        const double ratio = (iJ11 * iJ22 - iJ12 * iJ21) / (iJ11c * iJ22c - iJ12c * iJ21c);
+        bi[g][0] = (ratio * iJ11c) * (-2.0 * r); bi[g][1] = (ratio * iJ21c) * (-2.0 * r);
+        bi[g][2] = (ratio * iJ12c) * (-2.0 * s); bi[g][3] = (ratio * iJ22c) * (-2.0 * s);
+        const double gq = detJc / detJ;
+        bb[g][0] = (iJ11c * (-2.0 * r)) * gq; bb[g][1] = (iJ21c * (-2.0 * r)) * gq;
+        bb[g][2] = (iJ12c * (-2.0 * s)) * gq; bb[g][3] = (iJ22c * (-2.0 * s)) * gq;
+        // columns m of Bi (membrane) and Bib (bending) as 3-vectors
+        double Bm_[4][3] = {{bi[g][0], 0.0, bi[g][1]}, {0.0, bi[g][1], bi[g][0]},
+                            {bi[g][2], 0.0, bi[g][3]}, {0.0, bi[g][3], bi[g][2]}};
+        double Bb_[4][3] = {{0.0, -bb[g][1], -bb[g][0]}, {0.0, -bb[g][3], -bb[g][2]},
+                            {bb[g][0], 0.0, bb[g][1]}, {bb[g][2], 0.0, bb[g][3]}};
+        double tm[4][3], tb[4][3];
+        for (int m = 0; m < 4; ++m) { rec_mv3(Cm, Bm_[m], tm[m]); rec_mv3(Asec, Bb_[m], tb[m]); }
+        const double w = wg[g];
+        for (int m = 0; m < 4; ++m) {
+            Kabu[m] += (e0 * tm[m][0] + e1 * tm[m][1] + e2 * tm[m][2]) * w;
+            Kabsu[m] += (k0 * tb[m][0] + k1 * tb[m][1] + k2 * tb[m][2]) * w;
+            for (int i = 0; i < 4; ++i) {
+                Kbb[i + 4 * m] += (Bm_[i][0] * tm[m][0] + Bm_[i][1] * tm[m][1] + Bm_[i][2] * tm[m][2]) * w;
+                Kbbs[i + 4 * m] += (Bb_[i][0] * tb[m][0] + Bb_[i][1] * tb[m][1] + Bb_[i][2] * tb[m][2]) * w;
+            }
+        }
+    }
+    // alpha = -K_bb^{-1} K_ab' u; bending enrichment ab = -K_bbs^{-1} K_abs' u
+    double Ci[16], alpha[4] = {0, 0, 0, 0}, ab[4] = {0, 0, 0, 0};
+    if (!inv4_dev(Kbb, Ci)) return JFEM_ERR_DEGENERATE;
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k) alpha[i] -= Ci[i + 4 * k] * Kabu[k];
+    double mk = 0.0;
+    for (int q = 0; q < 16; ++q) mk = fmax(mk, fabs(Kbbs[q]));
+    if (mk > 0) {
+        if (!inv4_dev(Kbbs, Ci)) return JFEM_ERR_DEGENERATE;
+        for (int i = 0; i < 4; ++i)
+            for (int k = 0; k < 4; ++k) ab[i] -= Ci[i + 4 * k] * Kabsu[k];
+    }
+    double Ngp[12], Mgp[12], eavg[3] = {0, 0, 0}, kavg[3] = {0, 0, 0}, area = 0.0;
+    for (int g = 0; g < 4; ++g) {
+        const double eps[3] = {em[g][0] + (bi[g][0] * alpha[0] + bi[g][2] * alpha[2]),
+                               em[g][1] + (bi[g][1] * alpha[1] + bi[g][3] * alpha[3]),
+                               em[g][2] + (((bi[g][1] * alpha[0] + bi[g][0] * alpha[1]) + bi[g][3] * alpha[2]) +
+                                           bi[g][2] * alpha[3])};
+        rec_mv3(Cm, eps, Ngp + 3 * g);
+        if (mean_shear) {
+            const double kap[3] = {kb[g][0] + (bb[g][0] * ab[2] + bb[g][2] * ab[3]),
+                                   kb[g][1] + (-bb[g][1] * ab[0] - bb[g][3] * ab[1]),
+                                   kb[g][2] + (((-bb[g][0] * ab[0] - bb[g][2] * ab[1]) + bb[g][1] * ab[2]) +
+                                               bb[g][3] * ab[3])};
+            rec_mv3(Dsec, kap, Mgp + 3 * g);
+        } else {
+            double t[3];
+            rec_mv3(Cbc, kb[g], t);
+            for (int l = 0; l < 3; ++l) Mgp[3 * g + l] = -t[l];
+        }
+        for (int l = 0; l < 3; ++l) { eavg[l] += eps[l] * wg[g]; kavg[l] += kb[g][l] * wg[g]; }
+        area += wg[g];
+    }
+    for (int l = 0; l < 3; ++l) { eavg[l] /= area; kavg[l] /= area; }
+    double *N = out, *Mo = out + 3, *Q = out + 6, *sz1 = out + 8, *sz2 = out + 11, *ez1 = out + 14, *ez2 = out + 17;
+    double *Nc = out + 20, *Mc = out + 32;
+    rec_mv3(Cm, eavg, N);
+    {
+        double dk[3];
+        rec_mv3(Dm3, kavg, dk);
+        for (int l = 0; l < 3; ++l) Mo[l] = ((-br) * dk[l]) * ((h * h * h) / 12.0);
+    }
+    const double z1 = -h / 2.0, z2 = h / 2.0;
+    for (int l = 0; l < 3; ++l) { ez1[l] = eavg[l] + z1 * kavg[l]; ez2[l] = eavg[l] + z2 * kavg[l]; }
+    rec_mv3(Dm3, ez1, sz1);
+    rec_mv3(Dm3, ez2, sz2);
+    const double cr[4] = {-1.0, 1.0, 1.0, -1.0}, cs_[4] = {-1.0, -1.0, 1.0, 1.0};
+    const bool bending = br > 1e-12;
+    for (int k = 0; k < 4; ++k) {
+        double t[3];
+        rec_interp(Ngp, cr[k], cs_[k], Nc + 3 * k);
+        rec_interp(Mgp, cr[k], cs_[k], t);
+        for (int l = 0; l < 3; ++l) Mc[3 * k + l] = mean_shear ? (bending ? -t[l] : 0.0) : t[l];
+    }
+    Q[0] = 0.0; Q[1] = 0.0;
+    if (!mean_shear) {
+        if (bending) {
+            // MITC4 centre shear: covariant shear at the edge tying points, phi2 = min(1, 10 h^2 / (4 |J_c|))
+            const double tr[4] = {0.0, 0.0, -1.0, 1.0}, tsy[4] = {-1.0, 1.0, 0.0, 0.0};
+            double g1 = 0.0, g2 = 0.0;
+            for (int t = 0; t < 4; ++t) {
+                double dNr[4], dNs[4], J11, J12, J21, J22;
+                shape_derivs(tr[t], tsy[t], dNr, dNs);
+                jac4(dNr, dNs, lc, J11, J12, J21, J22);
+                const double Nt[4] = {0.25 * (1.0 - tr[t]) * (1.0 - tsy[t]), 0.25 * (1.0 + tr[t]) * (1.0 - tsy[t]),
+                                      0.25 * (1.0 + tr[t]) * (1.0 + tsy[t]), 0.25 * (1.0 - tr[t]) * (1.0 + tsy[t])};
+                double gam = 0.0;
+                for (int k = 0; k < 4; ++k) {
+                    const double *uk = u + 6 * k;
+                    gam += t < 2 ? (dNr[k] * uk[2] + (-J12 * Nt[k]) * uk[3] + (J11 * Nt[k]) * uk[4])
+                                 : (dNs[k] * uk[2] + (-J22 * Nt[k]) * uk[3] + (J21 * Nt[k]) * uk[4]);
+                }
+                if (t < 2) g1 += 0.5 * gam; else g2 += 0.5 * gam;
+            }
+            const double Gs = E / (2 * (1 + nu));
+            const double phi2 = fmin(1.0, 10.0 * (h * h) / fmax(4.0 * fmax(fabs(detJc), 1e-30), 1e-30));
+            const double kk = ((5.0 / 6.0) * Gs) * h;
+            const double t11 = kk * iJ11c, t12 = kk * iJ12c, t21 = kk * iJ21c, t22 = kk * iJ22c;
+            const double q1 = (phi2 * (iJ11c * t11 + iJ21c * t21)) * g1 + (phi2 * (iJ11c * t12 + iJ21c * t22)) * g2;
+            const double q2 = (phi2 * (iJ12c * t11 + iJ22c * t21)) * g1 + (phi2 * (iJ12c * t12 + iJ22c * t22)) * g2;
+            Q[0] = J11c * q1 + J21c * q2;
+            Q[1] = J12c * q1 + J22c * q2;
+            // equilibrium shear (dMx/dx, dMy/dy) of the corner moments, smaller magnitude kept
+            if (fabs(detJc) >= 1e-12) {
+                const double i11 = J22c / detJc, i12 = -J12c / detJc, i21 = -J21c / detJc, i22 = J11c / detJc;
+                double eq[2];
+                for (int comp = 0; comp < 2; ++comp) {
+                    double dr = 0.0, ds = 0.0;
+                    for (int k = 0; k < 4; ++k) { dr += dNr_c[k] * Mc[3 * k + comp]; ds += dNs_c[k] * Mc[3 * k + comp]; }
+                    eq[comp] = comp == 0 ? i11 * dr + i12 * ds : i21 * dr + i22 * ds;
+                }
+                for (int i = 0; i < 2; ++i)
+                    if (isfinite(eq[i]) && fabs(eq[i]) < fabs(Q[i])) Q[i] = eq[i];
+            }
+        }
+        for (int k = 0; k < 4; ++k) Mc[3 * k + 2] = Mo[2];
+        return 0;
+    }
+    for (int k = 0; k < 4; ++k) Mc[3 * k + 2] = Mo[2];
+    if (bending && tst > 0 && h > 0 && P.do_shear) {
+        // MacNeal plate operator of the section (Cb = D, Cs), rows for rx / ry of
+        // each corner against the plate DOFs (w, rx, ry)
+        if (q4_prep_shear(P) != 0) return JFEM_ERR_DEGENERATE;
+        double qx = 0.0, qy = 0.0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                for (int c = 0; c < 3; ++c) {
+                    auto kp = [&](int a, int b) {
+                        double rab = 0.0, rba = 0.0;
+                        for (int k = 0; k < 4; ++k) rab += P.Dm[k + 4 * a] * P.Xm[k + 4 * b];
+                        for (int k = 0; k < 4; ++k) rba += P.Dm[k + 4 * b] * P.Xm[k + 4 * a];
+                        return 0.5 * (rab + rba);
+                    };
+                    qx += kp(3 * i + 2, 3 * j + c) * u[6 * j + c + 2];
+                    qy -= kp(3 * i + 1, 3 * j + c) * u[6 * j + c + 2];
+                }
+        const double ar = 4 * fabs(detJc);
+        Q[0] = qx / ar; Q[1] = qy / ar;
+    }
+    return 0;
+}
+
+// CTRIA3: stress_strain_tria3 with the MITC3 centroid shear (src/FEMKernels.jl:9859-9937),
+// frame shell_element_frame_fast n = 3 (src/solver/helpers.jl:1486-1490)
+__device__ int t3_recover(const double *p, const double *pr, const double *ug, double *out) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6;
+    double v1[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]}, e13[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+    double v2[3], v3[3];
+    normalize3(v1);
+    cross3(v1, e13, v3);
+    normalize3(v3);
+    cross3(v3, v1, v2);
+    double c[3], lc[6];
+    for (int k = 0; k < 3; ++k) c[k] = ((p1[k] + p2[k]) + p3[k]) / 3.0;
+    for (int k = 0; k < 3; ++k) {
+        const double d[3] = {p[3 * k] - c[0], p[3 * k + 1] - c[1], p[3 * k + 2] - c[2]};
+        lc[2 * k] = dot3(d, v1); lc[2 * k + 1] = dot3(d, v2);
+    }
+    const double *R[3] = {v1, v2, v3};
+    double u[18];
+    for (int k = 0; k < 3; ++k)
+        for (int hf = 0; hf < 2; ++hf)
+            for (int i = 0; i < 3; ++i) {
+                const double *s = ug + 6 * k + 3 * hf;
+                u[6 * k + 3 * hf + i] = R[i][0] * s[0] + R[i][1] * s[1] + R[i][2] * s[2];
+            }
+    for (int q = 0; q < 20; ++q) out[q] = 0.0;
+    const double h = pr[JFEM_SP_H], E = pr[JFEM_SP_E], nu = pr[JFEM_SP_NU], br = pr[JFEM_SP_BEND_RATIO];
+    const double x1 = lc[0], y1 = lc[1], x2 = lc[2], y2 = lc[3], x3 = lc[4], y3 = lc[5];
+    const double A = 0.5 * fabs(x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2));
+    if (A < 1e-12) return 0;
+    const double b[3] = {(y2 - y3) / (2 * A), (y3 - y1) / (2 * A), (y1 - y2) / (2 * A)};
+    const double cc[3] = {(x3 - x2) / (2 * A), (x1 - x3) / (2 * A), (x2 - x1) / (2 * A)};
+    const double cm = E / (1 - nu * nu);
+    const double D[9] = {cm * 1.0, cm * nu, cm * 0.0, cm * nu, cm * 1.0, cm * 0.0, cm * 0.0, cm * 0.0, cm * ((1 - nu) / 2)};
+    double eps[3] = {0, 0, 0}, kap[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+        const double *ui = u + 6 * i;
+        eps[0] += b[i] * ui[0]; eps[1] += cc[i] * ui[1]; eps[2] += cc[i] * ui[0] + b[i] * ui[1];
+        kap[0] += b[i] * ui[4]; kap[1] += -cc[i] * ui[3]; kap[2] += -b[i] * ui[3] + cc[i] * ui[4];
+    }
+    double de[3], dk[3];
+    rec_mv3(D, eps, de);
+    rec_mv3(D, kap, dk);
+    for (int l = 0; l < 3; ++l) { out[l] = de[l] * h; out[3 + l] = ((-br) * dk[l]) * ((h * h * h) / 12.0); }
+    if (br > 1e-12) {
+        const double Jm11 = x2 - x1, Jm12 = y2 - y1, Jm21 = x3 - x1, Jm22 = y3 - y1;
+        const double detJ = Jm11 * Jm22 - Jm12 * Jm21;
+        if (fabs(detJ) >= 1e-30) {
+            // covariant shear at A (1/2, 0) along r, B (0, 1/2) along s, and C (1/2, 1/2)
+            auto gcov = [&](double r, double s, bool along_r) {
+                const double Nn[3] = {1.0 - r - s, r, s};
+                const double dwdr[3] = {-1.0, 1.0, 0.0}, dwds[3] = {-1.0, 0.0, 1.0};
+                double g = 0.0;
+                for (int i = 0; i < 3; ++i) {
+                    const double *ui = u + 6 * i;
+                    g += along_r ? (dwdr[i] * ui[2] + (-Nn[i] * Jm12) * ui[3] + (Nn[i] * Jm11) * ui[4])
+                                 : (dwds[i] * ui[2] + (-Nn[i] * Jm22) * ui[3] + (Nn[i] * Jm21) * ui[4]);
+                }
+                return g;
+            };
+            const double a = gcov(0.5, 0.0, true), bb = gcov(0.0, 0.5, false);
+            const double cv = (gcov(0.5, 0.5, true) - a) - (gcov(0.5, 0.5, false) - bb);
+            const double g1 = a + cv / 3, g2 = bb - cv / 3;
+            const double i11 = Jm22 / detJ, i12 = -Jm12 / detJ, i21 = -Jm21 / detJ, i22 = Jm11 / detJ;
+            const double f = ((5.0 / 6.0) * (E / (2 * (1 + nu)))) * h;
+            out[6] = f * (i11 * g1 + i12 * g2);
+            out[7] = f * (i21 * g1 + i22 * g2);
+        }
+    }
+    const double z1 = -h / 2.0, z2 = h / 2.0;
+    double e1[3], e2[3];
+    for (int l = 0; l < 3; ++l) { e1[l] = eps[l] + z1 * kap[l]; e2[l] = eps[l] + z2 * kap[l]; }
+    rec_mv3(D, e1, out + 8);
+    rec_mv3(D, e2, out + 11);
+    for (int l = 0; l < 3; ++l) { out[14 + l] = e1[l]; out[17 + l] = e2[l]; }
+    return 0;
+}
+
+// one thread per shell: CQUAD4s [0, nq), then CTRIA3s; the failing shell's index in
+// that order is reported
+__global__ void __launch_bounds__(64)
+shell_recover_kernel(int64_t nq, int64_t nt, const double *__restrict__ x, const double *__restrict__ y,
+                     const double *__restrict__ z, const int32_t *__restrict__ qconn, const double *__restrict__ qprop,
+                     const int32_t *__restrict__ tconn, const double *__restrict__ tprop, const double *__restrict__ u,
+                     int mean_shear, double *__restrict__ qout, double *__restrict__ tout, int64_t *status) {
+    const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (e >= nq + nt) return;
+    const bool quad = e < nq;
+    const int64_t el = quad ? e : e - nq, n = quad ? nq : nt;
+    const int nn = quad ? 4 : 3;
+    const int32_t *conn = quad ? qconn : tconn;
+    const double *prop = quad ? qprop : tprop;
+    double p[12], pr[8], ug[24];
+    for (int k = 0; k < nn; ++k) {
+        const int64_t nd = conn[k * n + el];
+        p[3 * k] = x[nd]; p[3 * k + 1] = y[nd]; p[3 * k + 2] = z[nd];
+        for (int c = 0; c < 6; ++c) ug[6 * k + c] = u[6 * nd + c];
+    }
+    for (int c = 0; c < 8; ++c) pr[c] = prop[c * n + el];
+    const int st = quad ? q4_recover(p, pr, ug, mean_shear, qout + 44 * el) : t3_recover(p, pr, ug, tout + 20 * el);
+    if (st != 0) report(status, e, st);
+}
+
+
 namespace jfem {
 
 hipError_t launch_reset_status(hipStream_t s, int64_t *status) {
@@ -1537,6 +1915,15 @@ hipError_t launch_t3_local(hipStream_t s, int64_t n, const double *lc, const dou
     dim3 grid((unsigned)((n + kElemsPerBlock - 1) / kElemsPerBlock));
     hipLaunchKernelGGL(t3_kernel<true>, grid, dim3(kWave * kElemsPerBlock), 0, s, n, nullptr, nullptr, nullptr, nullptr,
                        nullptr, 0.0, T3LocalIn{lc, Dm, Db, Ds, k6}, Ke, status, ElemVar{});
+    return hipGetLastError();
+}
+
+hipError_t launch_shell_recover(hipStream_t s, const jfem_mesh *m, const double *u, int mean_shear, double *qout,
+                                double *tout, int64_t *status) {
+    const int64_t n = m->nq + m->nt;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(shell_recover_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, m->nq, m->nt, m->d_x,
+                       m->d_y, m->d_z, m->d_q, m->d_qp, m->d_t, m->d_tp, u, mean_shear, qout, tout, status);
     return hipGetLastError();
 }
 

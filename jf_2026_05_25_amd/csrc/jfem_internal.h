@@ -123,6 +123,10 @@ hipError_t launch_q4_local(hipStream_t s, int64_t n, const double *lc, const dou
                            double *Ke, int64_t *status);
 hipError_t launch_t3_local(hipStream_t s, int64_t n, const double *lc, const double *Dm, const double *Db,
                            const double *Ds, const double *k6, double *Ke, int64_t *status);
+// shell force / stress / strain recovery of a mesh's CQUAD4s (44 doubles each) and
+// CTRIA3s (20 each) from device displacements u (6 per node)
+hipError_t launch_shell_recover(hipStream_t s, const jfem_mesh *m, const double *u, int mean_shear, double *qout,
+                                double *tout, int64_t *status);
 hipError_t launch_tet10_direct(hipStream_t s, int64_t n, const double *X, const double *prop, double *Ke,
                                int64_t *status);
 hipError_t launch_assemble(hipStream_t s, const jfem_pattern *P, const double *Ke, double *val, int V,

@@ -177,6 +177,12 @@ class Deck:
     subcases: list
     k6rot: float = 100.0
     unsupported: set = field(default_factory=set)
+    sol: int = 101                             # executive SOL number (src/parsing/card_processor.jl:227-233)
+
+    def recovery_mean_shear(self) -> bool:
+        """Shell recovery branch of this deck's solution sequence: the MacNeal
+        area-mean shear for SOL 101/105/106 (src/solver/stress_recovery.jl:258)."""
+        return self.sol in (101, 105, 106)
 
     # ---------------------------------------------------------------- model
     def grid_index(self):
@@ -321,11 +327,16 @@ def read_deck(source: str) -> Deck:
     glob = Subcase(0)
     subs = []
     cur = glob
+    sol = 101
     for raw in case_lines:
         l = _strip_comment(raw).strip()
         if not l:
             continue
         u = l.upper()
+        m = re.match(r"SOL\s+(\d+)", u)
+        if m:
+            sol = int(m.group(1))
+            continue
         m = re.match(r"SUBCASE\s+(\d+)", u)
         if m:
             cur = Subcase(int(m.group(1)), glob.spc, glob.load)
@@ -435,7 +446,7 @@ def read_deck(source: str) -> Deck:
     ids = sorted(grids)                                    # GRID rank order (assembly.jl:2653-2657)
     xyz = np.array([grids[i] for i in ids], float).reshape(-1, 3)
     return Deck(ids, xyz, ps, quads, trias, pshell, mat1, spc1, force, moment, pload4, other, subs, k6rot,
-                unsupported)
+                unsupported, sol)
 
 
 def solve_deck(deck: Deck, precision: int = 64, tol: float = 1e-10, membrane_modes: int = 15, ctx=None,

@@ -205,6 +205,23 @@ class DeviceMesh:
         check(self.ctx.h, self.ctx.L.jfem_element_matrices(self.ctx.h, self.h, KIND[kind], ptr(out), C.byref(bad)))
         return out.reshape(n, nd, nd).transpose(0, 2, 1)
 
+    def shell_recover(self, u, mean_shear: bool = True):
+        """Shell force / stress / strain recovery from one displacement vector u
+        (ndof, global DOF order) on the device: raw CQUAD4 rows (n_quad4, 44) and
+        CTRIA3 rows (n_tria3, 20), layout of jfem_shell_recover; mean_shear selects
+        the branch of solution sequences 101/105/106 (recover_shell_stresses!,
+        src/solver/stress_recovery.jl:505)."""
+        m = self.model
+        u = np.ascontiguousarray(np.asarray(u, dtype=float).reshape(-1))
+        if u.size != m.ndof:
+            raise ValueError(f"u has {u.size} entries, the model {m.ndof} DOFs")
+        q = np.zeros((len(m.quads), 44))
+        t = np.zeros((len(m.trias), 20))
+        bad = C.c_int64(-1)
+        check(self.ctx.h, self.ctx.L.jfem_shell_recover(self.ctx.h, self.h, ptr(u), int(bool(mean_shear)),
+                                                        ptr(q), ptr(t), C.byref(bad)))
+        return q, t
+
     def free(self):
         if self.h:
             self.ctx.L.jfem_mesh_free(self.h)

@@ -552,11 +552,10 @@ static void quad4_local_z(const double *p, double zl[4]) {
     for (int k = 0; k < 4; ++k) { double d[3]; v3sub(p + 3 * k, c, d); zl[k] = v3dot(d, v3); }
 }
 
-/* finite-warp work-dual congruence Ke <- sym(Tg' Ke Tg)
- * (apply_quad4_finite_warp_equilibrium!, src/FEMKernels.jl:1082-1116;
- *  _quad4_finite_warp_fill!, :880-945) */
-static void quad4_finite_warp(double *Ke, const double lc[8], const double *p) {
-    double zl[4], Tg[576];
+/* the finite-warp displacement map Tg (24 x 24, column-major)
+ * (quad4_finite_warp_displacement_map / _quad4_finite_warp_fill!, src/FEMKernels.jl:866-945) */
+static void quad4_warp_map(const double lc[8], const double *p, double Tg[576]) {
+    double zl[4];
     quad4_local_z(p, zl);
     for (int q = 0; q < 576; ++q) Tg[q] = (q % 25 == 0) ? 1.0 : 0.0;
     const double cr[4] = {-1, 1, 1, -1}, cs_[4] = {-1, -1, 1, 1};
@@ -594,6 +593,13 @@ static void quad4_finite_warp(double *Ke, const double lc[8], const double *p) {
             M(Tg, 6 * i + 1, 6 * j + 2, 24) += zl[i] * dNdy;
         }
     }
+}
+
+/* finite-warp work-dual congruence Ke <- sym(Tg' Ke Tg)
+ * (apply_quad4_finite_warp_equilibrium!, src/FEMKernels.jl:1082-1116) */
+static void quad4_finite_warp(double *Ke, const double lc[8], const double *p) {
+    double Tg[576];
+    quad4_warp_map(lc, p, Tg);
     double W[576], Kt[576];
     for (int j = 0; j < 24; ++j) for (int i = 0; i < 24; ++i) {
         double v = 0; for (int m = 0; m < 24; ++m) v += M(Ke, i, m, 24) * M(Tg, m, j, 24); M(W, i, j, 24) = v; }
@@ -877,6 +883,373 @@ int or_t3_global(const double p[9], const double prop[8], double k6rot, double K
     if (st != OR_OK) return st;
     double R[9] = {v1[0], v1[1], v1[2], v2[0], v2[1], v2[2], v3[0], v3[1], v3[2]};
     transform_blocks(Kl, 18, R, Ke);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* shell force / stress / strain recovery, isotropic PSHELL default branches  */
+/* (recover_shell_stresses!, src/solver/stress_recovery.jl:505-1268)           */
+/* ------------------------------------------------------------------------- */
+/* bilinear extrapolation of 2 x 2 Gauss-point rows G[g*3+c] to (xi, eta):
+ * interp_2x2_gauss_sigma, src/experimental/plate_kernels.jl:53-70 */
+static void interp_gauss(const double G[12], double xi, double eta, double out[3]) {
+    const double pt = 1.0 / sqrt(3.0);
+    double lx1 = (pt - xi) / (2.0 * pt), lx2 = (xi + pt) / (2.0 * pt);
+    double ly1 = (pt - eta) / (2.0 * pt), ly2 = (eta + pt) / (2.0 * pt);
+    double w1 = lx1 * ly1, w2 = lx2 * ly1, w3 = lx2 * ly2, w4 = lx1 * ly2;
+    for (int c = 0; c < 3; ++c) out[c] = w1 * G[c] + w2 * G[3 + c] + w3 * G[6 + c] + w4 * G[9 + c];
+}
+
+/* element-frame displacements: per node R u (translations) and R r (rotations),
+ * R rows (v1, v2, v3), src/solver/stress_recovery.jl:609-617 / :1003-1009 */
+static void element_displacements(const double *ug, int nn, const double *v1, const double *v2,
+                                  const double *v3, double *u) {
+    const double *R[3] = {v1, v2, v3};
+    for (int k = 0; k < nn; ++k)
+        for (int hf = 0; hf < 2; ++hf)
+            for (int i = 0; i < 3; ++i) {
+                const double *s = ug + 6 * k + 3 * hf;
+                u[6 * k + 3 * hf + i] = R[i][0] * s[0] + R[i][1] * s[1] + R[i][2] * s[2];
+            }
+}
+
+/* MITC4 centre transverse resultants with the phi2 factor (PHI2_ALPHA = 10):
+ * quad4_mitc4_center_shear_resultant, src/FEMKernels.jl:6515-6599 (no SNORM) */
+static void mitc4_center_shear(const double lc[8], const double u[24], double G, double h, double Q[2]) {
+    const double tr[4] = {0.0, 0.0, -1.0, 1.0}, ts[4] = {-1.0, 1.0, 0.0, 0.0};
+    double Bs[4][24];
+    memset(Bs, 0, sizeof(Bs));
+    for (int t = 0; t < 4; ++t) {
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs_quad(tr[t], ts[t], dNr, dNs);
+        jac4(dNr, dNs, lc, &J11, &J12, &J21, &J22);
+        double Nt[4] = {0.25 * (1.0 - tr[t]) * (1.0 - ts[t]), 0.25 * (1.0 + tr[t]) * (1.0 - ts[t]),
+                        0.25 * (1.0 + tr[t]) * (1.0 + ts[t]), 0.25 * (1.0 - tr[t]) * (1.0 + ts[t])};
+        for (int k = 0; k < 4; ++k) {
+            if (t < 2) { Bs[t][6 * k + 2] = dNr[k]; Bs[t][6 * k + 3] = -J12 * Nt[k]; Bs[t][6 * k + 4] = J11 * Nt[k]; }
+            else       { Bs[t][6 * k + 2] = dNs[k]; Bs[t][6 * k + 3] = -J22 * Nt[k]; Bs[t][6 * k + 4] = J21 * Nt[k]; }
+        }
+    }
+    const double dNr_c[4] = {-0.25, 0.25, 0.25, -0.25}, dNs_c[4] = {-0.25, -0.25, 0.25, 0.25};
+    double J11c, J12c, J21c, J22c;
+    jac4(dNr_c, dNs_c, lc, &J11c, &J12c, &J21c, &J22c);
+    double detJc = J11c * J22c - J12c * J21c, abs_detJc = fmax(fabs(detJc), 1e-30);
+    double inv_detc = 1.0 / detJc;
+    double iJ11c = J22c * inv_detc, iJ12c = -J12c * inv_detc, iJ21c = -J21c * inv_detc, iJ22c = J11c * inv_detc;
+    double g1 = 0.0, g2 = 0.0;
+    for (int j = 0; j < 24; ++j) {
+        g1 += (0.5 * (Bs[0][j] + Bs[1][j])) * u[j];
+        g2 += (0.5 * (Bs[2][j] + Bs[3][j])) * u[j];
+    }
+    double phi2 = fmin(1.0, 10.0 * (h * h) / fmax(4.0 * abs_detJc, 1e-30));
+    double k = ((5.0 / 6.0) * G) * h;
+    double t11 = k * iJ11c, t12 = k * iJ12c, t21 = k * iJ21c, t22 = k * iJ22c;
+    double q1 = (phi2 * (iJ11c * t11 + iJ21c * t21)) * g1 + (phi2 * (iJ11c * t12 + iJ21c * t22)) * g2;
+    double q2 = (phi2 * (iJ12c * t11 + iJ22c * t21)) * g1 + (phi2 * (iJ12c * t12 + iJ22c * t22)) * g2;
+    Q[0] = J11c * q1 + J21c * q2;
+    Q[1] = J12c * q1 + J22c * q2;
+}
+
+/* equilibrium shear (dMx/dx, dMy/dy) of bilinear corner moments and the
+ * smaller-magnitude blend: _quad4_equilibrium_shear_from_bending /
+ * _quad4_blend_recovered_shear, src/solver/stress_recovery.jl:126-168 */
+static void blend_equilibrium_shear(const double lc[8], const double Mc[12], double Q[2]) {
+    const double dNr[4] = {-0.25, 0.25, 0.25, -0.25}, dNs[4] = {-0.25, -0.25, 0.25, 0.25};
+    double J11, J12, J21, J22;
+    jac4(dNr, dNs, lc, &J11, &J12, &J21, &J22);
+    double detJ = J11 * J22 - J12 * J21;
+    if (fabs(detJ) < 1e-12) return;
+    double i11 = J22 / detJ, i12 = -J12 / detJ, i21 = -J21 / detJ, i22 = J11 / detJ;
+    double eq[2];
+    for (int comp = 0; comp < 2; ++comp) {
+        double dr = 0.0, ds = 0.0;
+        for (int k = 0; k < 4; ++k) { dr += dNr[k] * Mc[3 * k + comp]; ds += dNs[k] * Mc[3 * k + comp]; }
+        eq[comp] = comp == 0 ? i11 * dr + i12 * ds : i21 * dr + i22 * ds;
+    }
+    for (int i = 0; i < 2; ++i)
+        if (isfinite(eq[i]) && fabs(eq[i]) < fabs(Q[i])) Q[i] = eq[i];
+}
+
+int or_q4_recover(const double p[12], const double prop[8], const double ug[24], int mean_shear,
+                  double out[OR_Q4R_N]) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6, *p4 = p + 9;
+    double v1[3], v2[3], v3[3];
+    frame_quad4(p1, p2, p3, p4, v1, v2, v3);
+    double c[3], lc[8];
+    for (int k = 0; k < 3; ++k) c[k] = (((p1[k] + p2[k]) + p3[k]) + p4[k]) / 4.0;
+    for (int k = 0; k < 4; ++k) {
+        double d[3]; v3sub(p + 3 * k, c, d);
+        lc[2 * k] = v3dot(d, v1); lc[2 * k + 1] = v3dot(d, v2);
+    }
+    double u0[24], u[24];
+    element_displacements(ug, 4, v1, v2, v3, u0);
+    /* physical -> projected displacement (finite-warp map of a warped cell):
+     * quad4_physical_displacement_map, src/FEMKernels.jl:1062-1079, applied by
+     * stress_strain_quad4 :6768-6771 and by the section fields :7160-7161 */
+    if (quad4_is_warped(p, lc)) {
+        double Tg[576];
+        quad4_warp_map(lc, p, Tg);
+        for (int i = 0; i < 24; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < 24; ++j) s += M(Tg, i, j, 24) * u0[j];
+            u[i] = s;
+        }
+    } else {
+        memcpy(u, u0, sizeof(u));
+    }
+    const double h = prop[OR_SP_H], E = prop[OR_SP_E], nu = prop[OR_SP_NU], br = prop[OR_SP_BR];
+    const double tst = prop[OR_SP_TST];
+    double G = prop[OR_SP_G];
+    if (!(G > 0.0)) G = E > 0.0 ? E / (2 * (1 + nu)) : 0.0;
+    /* stress_strain_quad4's plane-stress law and membrane matrix, :6759-6763 */
+    const double cmem = E / (1 - nu * nu);
+    const double Dm3[9] = {cmem * 1.0, cmem * nu, cmem * 0.0, cmem * nu, cmem * 1.0, cmem * 0.0,
+                           cmem * 0.0, cmem * 0.0, cmem * ((1 - nu) / 2)};
+    double Cm[9];
+    for (int q = 0; q < 9; ++q) Cm[q] = Dm3[q] * h;
+    /* section law of the MacNeal mean-shear data (A = h C, D = br h^3/12 C,
+     * Cs = ts h G): _pshell_macneal_mean_shear_data, src/solver/stress_recovery.jl:248-292,
+     * pshell_plane_stress_matrix src/solver/assembly.jl:2183-2233 */
+    const double pc = E / fmax(1.0 - nu * nu, 1e-30);
+    double Csec[9] = {pc * 1.0, pc * nu, pc * 0.0, pc * nu, pc * 1.0, pc * 0.0, pc * 0.0, pc * 0.0, pc * ((1.0 - nu) / 2.0)};
+    double Asec[9], Dsec[9];
+    for (int q = 0; q < 9; ++q) { Asec[q] = h * Csec[q]; Dsec[q] = ((br * (h * h * h)) / 12) * Csec[q]; }
+    double Cs[4] = {(tst * h) * G, 0.0, 0.0, (tst * h) * G};
+    /* centre Jacobian, :6774-6778 */
+    const double dNr_c[4] = {-0.25, 0.25, 0.25, -0.25}, dNs_c[4] = {-0.25, -0.25, 0.25, 0.25};
+    double J11c, J12c, J21c, J22c;
+    jac4(dNr_c, dNs_c, lc, &J11c, &J12c, &J21c, &J22c);
+    double detJc = J11c * J22c - J12c * J21c;
+    double iJ11c = J22c / detJc, iJ12c = -J12c / detJc, iJ21c = -J21c / detJc, iJ22c = J11c / detJc;
+    const double pt = 1.0 / sqrt(3.0);
+    const double gr[4] = {-pt, pt, pt, -pt}, gs[4] = {-pt, -pt, pt, pt};
+    /* per Gauss point: compatible membrane strain Bm u, curvature Bb u, the
+     * membrane / bending internal-mode B matrices, weight */
+    double em[4][3], kb[4][3], Bi[4][12], Bib[4][12], wg[4];
+    double Kabu[4] = {0}, Kbb[16] = {0}, Kabsu[4] = {0}, Kbbs[16] = {0};
+    for (int g = 0; g < 4; ++g) {
+        double r = gr[g], s = gs[g], dNr[4], dNs[4], J11, J12, J21, J22, w;
+        shape_derivs_quad(r, s, dNr, dNs);
+        jac4(dNr, dNs, lc, &J11, &J12, &J21, &J22);
+        double detJ = J11 * J22 - J12 * J21;
+        if (jac_measure(detJ, J11, J12, J21, J22, &w) != OR_OK) return OR_ERR_DEGENERATE;
+        double iJ11 = J22 / detJ, iJ12 = -J12 / detJ, iJ21 = -J21 / detJ, iJ22 = J11 / detJ;
+        double e0 = 0, e1 = 0, e2 = 0, k0 = 0, k1 = 0, k2 = 0;
+        for (int k = 0; k < 4; ++k) {
+            double dx = iJ11 * dNr[k] + iJ12 * dNs[k], dy = iJ21 * dNr[k] + iJ22 * dNs[k];
+            const double *uk = u + 6 * k;
+            e0 += dx * uk[0]; e1 += dy * uk[1]; e2 += dy * uk[0] + dx * uk[1];
+            k0 += dx * uk[4]; k1 += -dy * uk[3]; k2 += dy * uk[4] - dx * uk[3];
+        }
+        em[g][0] = e0; em[g][1] = e1; em[g][2] = e2;
+        kb[g][0] = k0; kb[g][1] = k1; kb[g][2] = k2;
+        wg[g] = w;
+        /* membrane internal modes (fill_quad4_membrane_incompatible_B!, :1868-1896) */
+        double ratio = (iJ11 * iJ22 - iJ12 * iJ21) / (iJ11c * iJ22c - iJ12c * iJ21c);
+        double m11 = ratio * iJ11c, m12 = ratio * iJ12c, m21 = ratio * iJ21c, m22 = ratio * iJ22c;
+        double p1x = m11 * (-2.0 * r), p1y = m21 * (-2.0 * r), p2x = m12 * (-2.0 * s), p2y = m22 * (-2.0 * s);
+        double *B = Bi[g];
+        memset(B, 0, 12 * sizeof(double));
+        M(B, 0, 0, 3) = p1x; M(B, 2, 0, 3) = p1y; M(B, 1, 1, 3) = p1y; M(B, 2, 1, 3) = p1x;
+        M(B, 0, 2, 3) = p2x; M(B, 2, 2, 3) = p2y; M(B, 1, 3, 3) = p2y; M(B, 2, 3, 3) = p2x;
+        /* bending internal modes of the section fields' stiffness kernel (:3478-3488) */
+        double gq = detJc / detJ;
+        double b1x = (iJ11c * (-2.0 * r)) * gq, b1y = (iJ21c * (-2.0 * r)) * gq;
+        double b2x = (iJ12c * (-2.0 * s)) * gq, b2y = (iJ22c * (-2.0 * s)) * gq;
+        double *Bb = Bib[g];
+        memset(Bb, 0, 12 * sizeof(double));
+        M(Bb, 1, 0, 3) = -b1y; M(Bb, 1, 1, 3) = -b2y;
+        M(Bb, 0, 2, 3) = b1x;  M(Bb, 0, 3, 3) = b2x;
+        M(Bb, 2, 0, 3) = -b1x; M(Bb, 2, 1, 3) = -b2x;
+        M(Bb, 2, 2, 3) = b1y;  M(Bb, 2, 3, 3) = b2y;
+        /* K_ab' u, K_bb (membrane, Cm; :6824-6871) and K_ab' u, K_bb of the bending
+         * modes with the shape metric A (the enrichment map of the section fields) */
+        double tm[12], tb[12];
+        ts_mul(tm, Cm, B, 3, 3, 4);
+        ts_mul(tb, Asec, Bb, 3, 3, 4);
+        for (int m = 0; m < 4; ++m) {
+            Kabu[m] += (e0 * M(tm, 0, m, 3) + e1 * M(tm, 1, m, 3) + e2 * M(tm, 2, m, 3)) * w;
+            Kabsu[m] += (k0 * M(tb, 0, m, 3) + k1 * M(tb, 1, m, 3) + k2 * M(tb, 2, m, 3)) * w;
+        }
+        ts_mul_At_add(Kbb, B, tm, w, 3, 4, 4);
+        ts_mul_At_add(Kbbs, Bb, tb, w, 3, 4, 4);
+    }
+    /* alpha = -(K_bb \ K_ab' u) (:6873); bending enrichment ab = -K_bbs^{-1} K_abs' u
+     * (section_kinematic enrichment of quad4_condensed_section_fields :7173-7175) */
+    double Ci[16], alpha[4] = {0}, ab[4] = {0};
+    if (inv4(Kbb, Ci) != OR_OK) return OR_ERR_DEGENERATE;
+    for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) alpha[i] -= M(Ci, i, k, 4) * Kabu[k];
+    if (maxabs(Kbbs, 16) > 0) {
+        if (inv4(Kbbs, Ci) != OR_OK) return OR_ERR_DEGENERATE;
+        for (int i = 0; i < 4; ++i) for (int k = 0; k < 4; ++k) ab[i] -= M(Ci, i, k, 4) * Kabsu[k];
+    }
+    /* Gauss-point fields and area averages (:6896-6955); the section moments
+     * M = D kappa with the bending enrichment (mean-shear branch) or the
+     * compatible -Cb kappa of stress_strain_quad4 (:6898-6900, :6944-6947) */
+    double Cbc[9];
+    for (int q = 0; q < 9; ++q) Cbc[q] = (br * Dm3[q]) * ((h * h * h) / 12.0);
+    double Ngp[12], Msec[12], eavg[3] = {0}, kavg[3] = {0}, area = 0.0;
+    for (int g = 0; g < 4; ++g) {
+        double eps[3], kap[3];
+        for (int l = 0; l < 3; ++l) {
+            double v = 0.0;
+            for (int m = 0; m < 4; ++m) v += M(Bi[g], l, m, 3) * alpha[m];
+            eps[l] = em[g][l] + v;
+            double vb = 0.0;
+            for (int m = 0; m < 4; ++m) vb += M(Bib[g], l, m, 3) * ab[m];
+            kap[l] = kb[g][l] + vb;
+        }
+        for (int l = 0; l < 3; ++l) {
+            Ngp[3 * g + l] = M(Cm, l, 0, 3) * eps[0] + M(Cm, l, 1, 3) * eps[1] + M(Cm, l, 2, 3) * eps[2];
+            if (mean_shear)
+                Msec[3 * g + l] = M(Dsec, l, 0, 3) * kap[0] + M(Dsec, l, 1, 3) * kap[1] + M(Dsec, l, 2, 3) * kap[2];
+            else
+                Msec[3 * g + l] = -(M(Cbc, l, 0, 3) * kb[g][0] + M(Cbc, l, 1, 3) * kb[g][1] + M(Cbc, l, 2, 3) * kb[g][2]);
+            eavg[l] += eps[l] * wg[g];
+            kavg[l] += kb[g][l] * wg[g];
+        }
+        area += wg[g];
+    }
+    for (int l = 0; l < 3; ++l) { eavg[l] /= area; kavg[l] /= area; }
+    double *N = out + OR_Q4R_N0, *Mo = out + OR_Q4R_M0, *Q = out + OR_Q4R_Q0;
+    double *sz1 = out + OR_Q4R_S1, *sz2 = out + OR_Q4R_S2, *ez1 = out + OR_Q4R_E1, *ez2 = out + OR_Q4R_E2;
+    double *Nc = out + OR_Q4R_NC, *Mc = out + OR_Q4R_MC;
+    /* resultants and fibre fields at -h/2, +h/2 (:6956-6966) */
+    for (int l = 0; l < 3; ++l) {
+        N[l] = M(Cm, l, 0, 3) * eavg[0] + M(Cm, l, 1, 3) * eavg[1] + M(Cm, l, 2, 3) * eavg[2];
+        double dk = M(Dm3, l, 0, 3) * kavg[0] + M(Dm3, l, 1, 3) * kavg[1] + M(Dm3, l, 2, 3) * kavg[2];
+        Mo[l] = ((-br) * dk) * ((h * h * h) / 12.0);
+    }
+    const double z1 = -h / 2.0, z2 = h / 2.0;
+    for (int l = 0; l < 3; ++l) { ez1[l] = eavg[l] + z1 * kavg[l]; ez2[l] = eavg[l] + z2 * kavg[l]; }
+    for (int l = 0; l < 3; ++l) {
+        sz1[l] = M(Dm3, l, 0, 3) * ez1[0] + M(Dm3, l, 1, 3) * ez1[1] + M(Dm3, l, 2, 3) * ez1[2];
+        sz2[l] = M(Dm3, l, 0, 3) * ez2[0] + M(Dm3, l, 1, 3) * ez2[1] + M(Dm3, l, 2, 3) * ez2[2];
+    }
+    /* corner resultants: N from the condensed Gauss-point membrane field, M
+     * (printed sign) from the section fields (stress_recovery.jl:933-946) or from
+     * stress_strain_quad4's Gauss points (:6967-6974) */
+    const double cr[4] = {-1.0, 1.0, 1.0, -1.0}, cs[4] = {-1.0, -1.0, 1.0, 1.0};
+    const int bending = br > 1e-12;
+    for (int k = 0; k < 4; ++k) {
+        double t[3];
+        interp_gauss(Ngp, cr[k], cs[k], Nc + 3 * k);
+        interp_gauss(Msec, cr[k], cs[k], t);
+        for (int l = 0; l < 3; ++l) Mc[3 * k + l] = mean_shear ? (bending ? -t[l] : 0.0) : t[l];
+    }
+    Q[0] = Q[1] = 0.0;
+    if (!mean_shear) {
+        /* other solution sequences: MITC4 centre shear (:6953-6954) blended with the
+         * equilibrium shear of the corner moments (stress_recovery.jl:948-952) */
+        if (bending) {
+            mitc4_center_shear(lc, u, E / (2 * (1 + nu)), h, Q);
+            blend_equilibrium_shear(lc, Mc, Q);
+        }
+        for (int k = 0; k < 4; ++k) Mc[3 * k + 2] = Mo[2];
+        return OR_OK;
+    }
+    /* twist held at the centre value (:960) */
+    for (int k = 0; k < 4; ++k) Mc[3 * k + 2] = Mo[2];
+    /* area-mean MacNeal transverse resultants (quad4_macneal_mean_shear_resultant
+     * :6623-6645): work of the plate operator against unit shear states */
+    const int rigid = prop[OR_SP_RIGID] != 0.0;
+    if (bending && tst > 0 && h > 0 && !(!rigid && maxabs(Cs, 4) < 1e-30)) {
+        double D[48], Zb[16], Zs[16], Z[16], Kp[144];
+        int st = or_q4_macneal_parts(lc, Dsec, Cs, rigid, D, Zb, Zs, Z, Kp);
+        if (st != OR_OK) return st;
+        double qx = 0.0, qy = 0.0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                for (int cc = 0; cc < 3; ++cc) {
+                    qx += M(Kp, 3 * i + 2, 3 * j + cc, 12) * u[6 * j + cc + 2];
+                    qy -= M(Kp, 3 * i + 1, 3 * j + cc, 12) * u[6 * j + cc + 2];
+                }
+        double ar = 4 * fabs(detJc);
+        Q[0] = qx / ar; Q[1] = qy / ar;
+    }
+    return OR_OK;
+}
+
+/* CTRIA3: stress_strain_tria3, src/FEMKernels.jl:9895-9937, with the MITC3
+ * centroid shear tria3_mitc3_shear_resultant :9859-9893; frame and element
+ * displacements src/solver/stress_recovery.jl:985-1009 */
+int or_t3_recover(const double p[9], const double prop[8], const double ug[18], double out[OR_T3R_N]) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6;
+    double v1[3], v2[3], v3[3], e13[3];
+    v3sub(p2, p1, v1); v3normalize(v1);
+    v3sub(p3, p1, e13); v3cross(v1, e13, v3); v3normalize(v3);
+    v3cross(v3, v1, v2);
+    double c[3], lc[6];
+    for (int k = 0; k < 3; ++k) c[k] = ((p1[k] + p2[k]) + p3[k]) / 3.0;
+    for (int k = 0; k < 3; ++k) { double d[3]; v3sub(p + 3 * k, c, d); lc[2 * k] = v3dot(d, v1); lc[2 * k + 1] = v3dot(d, v2); }
+    double u[18];
+    element_displacements(ug, 3, v1, v2, v3, u);
+    for (int q = 0; q < OR_T3R_N; ++q) out[q] = 0.0;
+    const double h = prop[OR_SP_H], E = prop[OR_SP_E], nu = prop[OR_SP_NU], br = prop[OR_SP_BR];
+    const double x1 = lc[0], y1 = lc[1], x2 = lc[2], y2 = lc[3], x3 = lc[4], y3 = lc[5];
+    double A = 0.5 * fabs(x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2));
+    if (A < 1e-12) return OR_OK;
+    const double b[3] = {(y2 - y3) / (2 * A), (y3 - y1) / (2 * A), (y1 - y2) / (2 * A)};
+    const double cc[3] = {(x3 - x2) / (2 * A), (x1 - x3) / (2 * A), (x2 - x1) / (2 * A)};
+    const double cm = E / (1 - nu * nu);
+    const double D[9] = {cm * 1.0, cm * nu, cm * 0.0, cm * nu, cm * 1.0, cm * 0.0, cm * 0.0, cm * 0.0, cm * ((1 - nu) / 2)};
+    double eps[3] = {0, 0, 0}, kap[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i) {   /* Bm u_mem, columns (u_i, v_i); Bb u_rot, columns (rx_i, ry_i) */
+        const double *ui = u + 6 * i;
+        eps[0] += b[i] * ui[0]; eps[1] += cc[i] * ui[1]; eps[2] += cc[i] * ui[0] + b[i] * ui[1];
+        kap[0] += b[i] * ui[4]; kap[1] += -cc[i] * ui[3]; kap[2] += -b[i] * ui[3] + cc[i] * ui[4];
+    }
+    double *N = out + OR_T3R_N0, *Mo = out + OR_T3R_M0, *Q = out + OR_T3R_Q0;
+    double *sz1 = out + OR_T3R_S1, *sz2 = out + OR_T3R_S2, *ez1 = out + OR_T3R_E1, *ez2 = out + OR_T3R_E2;
+    for (int l = 0; l < 3; ++l) {
+        double de = M(D, l, 0, 3) * eps[0] + M(D, l, 1, 3) * eps[1] + M(D, l, 2, 3) * eps[2];
+        double dk = M(D, l, 0, 3) * kap[0] + M(D, l, 1, 3) * kap[1] + M(D, l, 2, 3) * kap[2];
+        N[l] = de * h;
+        Mo[l] = ((-br) * dk) * ((h * h * h) / 12.0);
+    }
+    if (br > 1e-12) {
+        double Jm11 = x2 - x1, Jm12 = y2 - y1, Jm21 = x3 - x1, Jm22 = y3 - y1;
+        double detJ = Jm11 * Jm22 - Jm12 * Jm21;
+        if (fabs(detJ) >= 1e-30) {
+            /* covariant shear rows at the tying points A (1/2, 0), B (0, 1/2), C (1/2, 1/2) */
+            double grA[9], gsB[9], grC[9], gsC[9], tmp[9];
+            const double rr[3] = {0.5, 0.0, 0.5}, ss[3] = {0.0, 0.5, 0.5};
+            for (int t = 0; t < 3; ++t) {
+                double Nn[3] = {1.0 - rr[t] - ss[t], rr[t], ss[t]};
+                const double dwdr[3] = {-1.0, 1.0, 0.0}, dwds[3] = {-1.0, 0.0, 1.0};
+                double Br[9], Bs[9];
+                for (int i = 0; i < 3; ++i) {
+                    Br[3 * i] = dwdr[i]; Bs[3 * i] = dwds[i];
+                    Br[3 * i + 2] = Nn[i] * Jm11; Bs[3 * i + 2] = Nn[i] * Jm21;
+                    Br[3 * i + 1] = -Nn[i] * Jm12; Bs[3 * i + 1] = -Nn[i] * Jm22;
+                }
+                double *dr_ = t == 0 ? grA : (t == 2 ? grC : tmp);
+                double *ds_ = t == 1 ? gsB : (t == 2 ? gsC : tmp);
+                memcpy(dr_, Br, sizeof(Br));
+                if (t != 0) memcpy(ds_, Bs, sizeof(Bs));
+            }
+            double up[9];
+            for (int i = 0; i < 3; ++i) { up[3 * i] = u[6 * i + 2]; up[3 * i + 1] = u[6 * i + 3]; up[3 * i + 2] = u[6 * i + 4]; }
+            double a = 0, bb = 0, cv = 0;
+            for (int q = 0; q < 9; ++q) {
+                a += grA[q] * up[q];
+                bb += gsB[q] * up[q];
+                cv += ((grC[q] - grA[q]) - (gsC[q] - gsB[q])) * up[q];
+            }
+            double g1 = a + cv / 3, g2 = bb - cv / 3;
+            double i11 = Jm22 / detJ, i12 = -Jm12 / detJ, i21 = -Jm21 / detJ, i22 = Jm11 / detJ;
+            double Gs = E / (2 * (1 + nu));
+            double f = ((5.0 / 6.0) * Gs) * h;
+            Q[0] = f * (i11 * g1 + i12 * g2);
+            Q[1] = f * (i21 * g1 + i22 * g2);
+        }
+    }
+    const double z1 = -h / 2.0, z2 = h / 2.0;
+    for (int l = 0; l < 3; ++l) { ez1[l] = eps[l] + z1 * kap[l]; ez2[l] = eps[l] + z2 * kap[l]; }
+    for (int l = 0; l < 3; ++l) {
+        sz1[l] = M(D, l, 0, 3) * ez1[0] + M(D, l, 1, 3) * ez1[1] + M(D, l, 2, 3) * ez1[2];
+        sz2[l] = M(D, l, 0, 3) * ez2[0] + M(D, l, 1, 3) * ez2[1] + M(D, l, 2, 3) * ez2[2];
+    }
     return OR_OK;
 }
 

@@ -60,6 +60,24 @@ int or_tet10(const double X[30] /* 10x3 column-major */, double E, double nu, do
 int or_q4_global(const double p[12], const double prop[8], double k6rot, double Ke[576]);
 int or_t3_global(const double p[9], const double prop[8], double k6rot, double Ke[324]);
 
+/* Shell force / stress / strain recovery from global displacements ug (6 per
+ * node, basic frame) for an isotropic PSHELL (prop as for the global element
+ * matrices), in the element frame:
+ *   CQUAD4 (recover_shell_stresses! default branch, src/solver/stress_recovery.jl:582-985):
+ *     N[3] M[3] Q[2] stress(z1)[3] stress(z2)[3] strain(z1)[3] strain(z2)[3]
+ *     corner N[4][3], corner M[4][3]  (z1 = -h/2, z2 = +h/2; M the printed moment)
+ *     mean_shear != 0: the SOL 101/105/106 branch (MacNeal area-mean Q, corner M of
+ *     the enriched section fields); 0: other solution sequences (MITC4 centre Q
+ *     blended with the equilibrium shear, corner M of the compatible curvature)
+ *   CTRIA3 (:985-1110): the same first 20 entries. */
+enum { OR_Q4R_N0 = 0, OR_Q4R_M0 = 3, OR_Q4R_Q0 = 6, OR_Q4R_S1 = 8, OR_Q4R_S2 = 11, OR_Q4R_E1 = 14,
+       OR_Q4R_E2 = 17, OR_Q4R_NC = 20, OR_Q4R_MC = 32, OR_Q4R_N = 44 };
+enum { OR_T3R_N0 = 0, OR_T3R_M0 = 3, OR_T3R_Q0 = 6, OR_T3R_S1 = 8, OR_T3R_S2 = 11, OR_T3R_E1 = 14,
+       OR_T3R_E2 = 17, OR_T3R_N = 20 };
+int or_q4_recover(const double p[12], const double prop[8], const double ug[24], int mean_shear,
+                  double out[OR_Q4R_N]);
+int or_t3_recover(const double p[9], const double prop[8], const double ug[18], double out[OR_T3R_N]);
+
 /* ---------- assembly with sparse(I,J,V) semantics ---------- */
 /* Triplet order: all CQUAD4 (element order, column-major 24x24), then CTRIA3, then
  * CTETRA10.  Duplicates summed in triplet order.  Output CSR (row_ptr n+1, col, val)

@@ -41,6 +41,8 @@ def lib():
         L.or_t3_guard.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
         L.or_q4_global.argtypes = [dp, dp, C.c_double, dp]
         L.or_t3_global.argtypes = [dp, dp, C.c_double, dp]
+        L.or_q4_recover.argtypes = [dp, dp, dp, C.c_int, dp]
+        L.or_t3_recover.argtypes = [dp, dp, dp, dp]
         L.or_assemble.argtypes = [C.c_int64, dp, dp, dp, C.c_int64, C.c_void_p, C.c_void_p,
                                   C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                   C.c_double, C.POINTER(C.c_int64), C.POINTER(C.POINTER(C.c_int64)),
@@ -150,6 +152,46 @@ def t3_global(p, prop, k6rot=100.0):
     if st:
         raise OracleError(st)
     return Ke.reshape(18, 18, order="F")
+
+
+Q4R_N, T3R_N = 44, 20
+
+
+def q4_recover(p, prop, ug, mean_shear=True):
+    """Raw CQUAD4 recovery vector (44 doubles, layout of or_q4_recover) for the
+    4 x 3 corner coordinates p, section prop and global displacements ug (24);
+    mean_shear selects the SOL 101/105/106 branch."""
+    out = np.zeros(Q4R_N)
+    st = lib().or_q4_recover(_f(np.asarray(p, float).reshape(4, 3).ravel()), _f(prop),
+                             _f(np.asarray(ug, float).ravel()), int(bool(mean_shear)), out)
+    if st:
+        raise OracleError(st)
+    return out
+
+
+def t3_recover(p, prop, ug):
+    """Raw CTRIA3 recovery vector (20 doubles, layout of or_t3_recover)."""
+    out = np.zeros(T3R_N)
+    st = lib().or_t3_recover(_f(np.asarray(p, float).reshape(3, 3).ravel()), _f(prop),
+                             _f(np.asarray(ug, float).ravel()), out)
+    if st:
+        raise OracleError(st)
+    return out
+
+
+def shell_recover(model, u, mean_shear=True):
+    """Per-element raw recovery of a model for one displacement vector u (ndof):
+    (CQUAD4 rows (n_quad4, 44), CTRIA3 rows (n_tria3, 20))."""
+    u = np.asarray(u, float).reshape(-1)
+    q = np.zeros((len(model.quads), Q4R_N))
+    for e, nodes in enumerate(np.asarray(model.quads)):
+        ug = np.concatenate([u[6 * n:6 * n + 6] for n in nodes])
+        q[e] = q4_recover(model.xyz[nodes], model.quad_props[e], ug, mean_shear)
+    t = np.zeros((len(model.trias), T3R_N))
+    for e, nodes in enumerate(np.asarray(model.trias)):
+        ug = np.concatenate([u[6 * n:6 * n + 6] for n in nodes])
+        t[e] = t3_recover(model.xyz[nodes], model.tria_props[e], ug)
+    return q, t
 
 
 def _cm_i32(a):

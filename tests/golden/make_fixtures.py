@@ -12,6 +12,11 @@ and their recorded displacement vectors ``results/*.JU.JSON``:
 * validation/cases/MH_hemispherical_shell/convergence/{N08,N16} (the same
   hemisphere on 8x8 and 16x16 CQUAD4 meshes)
 
+Where a result file also records CQUAD4 shell recovery (the MYSTRAN case:
+forces, forces_bilin, stresses, strains of its element), each subcase carries
+those rows under "shell_recovery", and the case its solution sequence ("sol",
+the executive SOL number, which selects the recovery branch).
+
 Scordelis-Lo (GRAV load) and the TACS-backend uCRM-9 vector are not on this
 path and are not extracted.  Output: tests/golden/reference_ju.json.
 Run from the repository root: ``python tests/golden/make_fixtures.py``.
@@ -133,14 +138,30 @@ def parse(case, loads_by_subcase, spc_sid):
             continue
         lsid = loads_by_subcase[sid]
         rows = disp[k * per:(k + 1) * per]
-        subcases.append({
+        sub = {
             "subcase": sid, "load_sid": lsid,
             "force": forces.get(lsid, []), "pload4": pload4.get(lsid, []),
             "expected": {str(r["grid_id"]): [r["t1"], r["t2"], r["t3"], r["r1"], r["r2"], r["r3"]] for r in rows},
             "reference_free_dofs": diag["details"]["bc_partition"]["free_dofs"],
             "reference_relative_residual": diag["details"]["linear_solver"]["relative_residual"],
-        })
-    return {"case": case, "dir": cdir, "k6rot": k6rot, "grids": {str(k): v for k, v in grids.items()},
+        }
+        nq = len(quads)
+        if nq and len(ju.get("stresses", {}).get("quad4", [])) == n_sub * nq:
+            sub["shell_recovery"] = {
+                "forces": ju["forces"]["quad4"][k * nq:(k + 1) * nq],
+                "forces_bilin": ju["forces_bilin"]["quad4"][5 * k * nq:5 * (k + 1) * nq],
+                "stresses": ju["stresses"]["quad4"][k * nq:(k + 1) * nq],
+                "strains": ju["strains"]["quad4"][k * nq:(k + 1) * nq],
+            }
+        subcases.append(sub)
+    sol = 101
+    for line in open(os.path.join(REF, cdir, "model.bdf")).read().splitlines():
+        t = line.split("$")[0].strip().upper()
+        if t.startswith("CEND") or t.startswith("BEGIN"):
+            break
+        if t.startswith("SOL ") and t.split()[1].isdigit():
+            sol = int(t.split()[1])
+    return {"case": case, "dir": cdir, "sol": sol, "k6rot": k6rot, "grids": {str(k): v for k, v in grids.items()},
             "cquad4": quads, "pshell": {str(k): v for k, v in pshell.items()},
             "mat1": {str(k): v for k, v in mat1.items()},
             "spc1": [{"grid": n, "comps": c} for (s, n, c) in spc if s == spc_sid],
