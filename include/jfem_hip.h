@@ -312,16 +312,28 @@ int jfem_solve_dev_static(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix
  *   FORCE / MOMENT card i: grid point *_node[i], magnitude *_mag[i], direction
  *     *_dir3[3i..3i+2], taken as given (the reference's reader rounds both to
  *     single precision: src/parsing/extract_loads.jl:15-29).
- * GRAV/ACCEL, PLOAD1/2, coordinate systems and LOAD combinations stay with the
- * caller (a combination is a sum of load vectors with their factors). */
+ * GRAV on shells is added by jfem_loadset_set_gravity; ACCEL, PLOAD1/2, GRAV on
+ * solids, coordinate systems and LOAD combinations stay with the caller (a
+ * combination is a sum of load vectors with their factors). */
 int jfem_loadset_create(jfem_ctx *ctx, const jfem_mesh *mesh, int64_t n_pload4,
                         const int32_t *pload4_elem, const double *pload4_p,
                         const double *pload4_dir3, int64_t n_force, const int32_t *force_node,
                         const double *force_mag, const double *force_dir3, int64_t n_moment,
                         const int32_t *moment_node, const double *moment_mag,
                         const double *moment_dir3, jfem_loadset **out);
+/* GRAV cards of the set on its shells, replacing any set before: card i has
+ * acceleration grav_a[i] along grav_n3[3i..3i+2] (basic frame, CID 0); shell_areal
+ * (n_quad + n_tria, CQUAD4s then CTRIA3s) is each shell's mass per area rho T + NSM,
+ * 0 for a shell without mass.  Each shell adds (areal * area / nn) .* (A scale) N
+ * to its nodes' translations, area half the norm of the diagonal (CQUAD4) or edge
+ * (CTRIA3) cross product, cards in order and shells in element order, after the
+ * PLOAD4 and before the FORCE / MOMENT contributions: the GRAV shell branch of
+ * _resolve_loads! (src/solver/loads.jl:680-684, 730-772).  n_grav = 0 removes them. */
+int jfem_loadset_set_gravity(jfem_ctx *ctx, jfem_loadset *ls, int64_t n_grav, const double *grav_a,
+                             const double *grav_n3, const double *shell_areal);
 /* The set's load vector with scale factor `scale` (applied where the reference
- * applies it: area*P*scale per PLOAD4, (Dir*Mag)*scale per FORCE/MOMENT), all
+ * applies it: area*P*scale per PLOAD4, (Dir*Mag)*scale per FORCE/MOMENT, A*scale
+ * per GRAV), all
  * 6*n_nodes DOFs, to the host (F_host) or to device memory (F_dev, on this
  * context's stream).  Sums run in the reference's order: bitwise reproducible.
  * A load set keeps per-call buffers: calls on one load set must not overlap. */

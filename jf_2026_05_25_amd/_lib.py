@@ -34,7 +34,7 @@ EXPORTS = [
     "jfem_dev_alloc", "jfem_dev_free", "jfem_memcpy_h2d", "jfem_memcpy_d2h",
     "jfem_factor_device_bytes", "jfem_matrix_device_bytes", "jfem_mesh_device_bytes", "jfem_pattern_device_bytes",
     "jfem_symbolic_device_bytes", "jfem_ctx_device_bytes", "jfem_solve_scratch_bytes", "jfem_selftest_update",
-    "jfem_loadset_create", "jfem_load_vector", "jfem_load_vector_dev", "jfem_loadset_free",
+    "jfem_loadset_create", "jfem_loadset_set_gravity", "jfem_load_vector", "jfem_load_vector_dev", "jfem_loadset_free",
     "jfem_batch_create", "jfem_batch_update_props", "jfem_batch_assemble", "jfem_batch_matrix", "jfem_batch_factor",
     "jfem_batch_factor_view", "jfem_batch_free",
 ]
@@ -118,6 +118,7 @@ def load():
         "jfem_solve_scratch_bytes": ([vp, i64, C.c_int, C.c_int], i64),
         "jfem_selftest_update": ([vp, C.c_int, P(dbl)], C.c_int),
         "jfem_loadset_create": ([vp, vp, i64, vp, vp, vp, i64, vp, vp, vp, i64, vp, vp, vp, P(vp)], C.c_int),
+        "jfem_loadset_set_gravity": ([vp, vp, i64, vp, vp, vp], C.c_int),
         "jfem_load_vector": ([vp, vp, dbl, vp], C.c_int),
         "jfem_load_vector_dev": ([vp, vp, dbl, vp], C.c_int),
         "jfem_loadset_free": ([vp], C.c_int),

@@ -1,5 +1,6 @@
-// Load vectors on the device: FORCE / MOMENT cards on grid points and uniform
-// PLOAD4 cards on shell elements, one load set (SID) in card order.
+// Load vectors on the device: FORCE / MOMENT cards on grid points, uniform
+// PLOAD4 cards on shell elements and GRAV cards (lumped shell mass), one load
+// set (SID) in card order.
 //
 // Reference: _resolve_loads! (src/solver/loads.jl:510-587) accumulates, per
 // subcase, every PLOAD4 on a CQUAD4/CTRIA3 straight into F (the element's
@@ -47,6 +48,14 @@ struct jfem_loadset {
     int64_t *d_m_ptr = nullptr;
     double *d_m_rec = nullptr;
     double *d_F = nullptr;          // 6 n_nodes: staging of the host variant
+    // GRAV on shells: per card (A, N1, N2, N3), per shell its areal mass
+    // (rho T + NSM; 0: no mass) and lumped nodal share, per grid point its shells
+    int64_t ng = 0, nsh = 0;
+    double *d_grav = nullptr;       // 4 ng
+    double *d_areal = nullptr;      // nsh
+    double *d_gsh = nullptr;        // nsh
+    int64_t *d_sh_ptr = nullptr;    // n_nodes + 1
+    int32_t *d_sh_elem = nullptr;   // shells per grid point, element order
 };
 
 namespace {
@@ -146,6 +155,37 @@ __global__ void __launch_bounds__(kLoadBS) pload4_kernel(int64_t npl, const int3
     out[3] = 1.0;
 }
 
+// lumped gravity share of each shell with mass: (areal * area) / nn, area half
+// the norm of the diagonal (CQUAD4) or edge (CTRIA3) cross product
+// (src/solver/loads.jl:758-766)
+__global__ void __launch_bounds__(kLoadBS) grav_share_kernel(int64_t nsh, const double *__restrict__ areal,
+                                                             const double *__restrict__ x, const double *__restrict__ y,
+                                                             const double *__restrict__ z, const int32_t *__restrict__ qc,
+                                                             int64_t nq, const int32_t *__restrict__ tc, int64_t nt,
+                                                             double *__restrict__ gsh) {
+#pragma clang fp contract(off)
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nsh) return;
+    const bool quad = e < nq;
+    const int nn = quad ? 4 : 3;
+    double X[4][3];
+    for (int k = 0; k < nn; ++k) {
+        const int32_t g = quad ? qc[e + nq * k] : tc[(e - nq) + nt * k];
+        X[k][0] = x[g]; X[k][1] = y[g]; X[k][2] = z[g];
+    }
+    double a[3], b[3], n[3];
+    for (int c = 0; c < 3; ++c) {
+        a[c] = quad ? X[2][c] - X[0][c] : X[1][c] - X[0][c];
+        b[c] = quad ? X[3][c] - X[1][c] : X[2][c] - X[0][c];
+    }
+    n[0] = a[1] * b[2] - a[2] * b[1];
+    n[1] = a[2] * b[0] - a[0] * b[2];
+    n[2] = a[0] * b[1] - a[1] * b[0];
+    const double area = 0.5 * jl_norm3(n);
+    const double total_mass = areal[e] * area;
+    gsh[e] = total_mass / (double)nn;
+}
+
 __global__ void __launch_bounds__(kLoadBS) load_node_kernel(int64_t n_nodes, const int64_t *__restrict__ pl_ptr,
                                                             const int32_t *__restrict__ pl_card,
                                                             const double *__restrict__ plf,
@@ -153,6 +193,11 @@ __global__ void __launch_bounds__(kLoadBS) load_node_kernel(int64_t n_nodes, con
                                                             const double *__restrict__ f_rec,
                                                             const int64_t *__restrict__ m_ptr,
                                                             const double *__restrict__ m_rec, double scale,
+                                                            int64_t ng, const double *__restrict__ grav,
+                                                            const int64_t *__restrict__ sh_ptr,
+                                                            const int32_t *__restrict__ sh_elem,
+                                                            const double *__restrict__ areal,
+                                                            const double *__restrict__ gsh,
                                                             double *__restrict__ F) {
 #pragma clang fp contract(off)
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -164,6 +209,19 @@ __global__ void __launch_bounds__(kLoadBS) load_node_kernel(int64_t n_nodes, con
             if (s[3] == 0.0) continue;
             for (int c = 0; c < 3; ++c) v[c] = v[c] + s[c];
         }
+    // GRAV cards in order, each over the grid point's shells in element order
+    // (the shell loop of src/solver/loads.jl:730-772)
+    for (int64_t i = 0; i < ng; ++i) {
+        const double *gr = grav + 4 * i;
+        double gv[3];
+        for (int c = 0; c < 3; ++c) gv[c] = (gr[0] * scale) * gr[1 + c];
+        for (int64_t q = sh_ptr[g]; q < sh_ptr[g + 1]; ++q) {
+            const int32_t e = sh_elem[q];
+            if (areal[e] == 0.0) continue;
+            const double sh = gsh[e];
+            for (int c = 0; c < 3; ++c) v[c] = v[c] + sh * gv[c];
+        }
+    }
     const int64_t f0 = f_ptr ? f_ptr[g] : 0, f1 = f_ptr ? f_ptr[g + 1] : 0;
     const int64_t m0 = m_ptr ? m_ptr[g] : 0, m1 = m_ptr ? m_ptr[g + 1] : 0;
     if (f1 > f0 || m1 > m0) {
@@ -208,7 +266,8 @@ extern "C" int jfem_loadset_free(jfem_loadset *ls) {
     (void)hipSetDevice(ls->device);
     for (void *p : {(void *)ls->d_pl_elem, (void *)ls->d_pl_p, (void *)ls->d_pl_dir, (void *)ls->d_plf,
                     (void *)ls->d_pl_ptr, (void *)ls->d_pl_card, (void *)ls->d_f_ptr, (void *)ls->d_f_rec,
-                    (void *)ls->d_m_ptr, (void *)ls->d_m_rec, (void *)ls->d_F})
+                    (void *)ls->d_m_ptr, (void *)ls->d_m_rec, (void *)ls->d_F, (void *)ls->d_grav,
+                    (void *)ls->d_areal, (void *)ls->d_gsh, (void *)ls->d_sh_ptr, (void *)ls->d_sh_elem})
         if (p) (void)hipFree(p);
     delete ls;
     return JFEM_OK;
@@ -298,6 +357,59 @@ extern "C" int jfem_loadset_create(jfem_ctx *ctx, const jfem_mesh *mesh, int64_t
     return JFEM_OK;
 }
 
+extern "C" int jfem_loadset_set_gravity(jfem_ctx *ctx, jfem_loadset *ls, int64_t n_grav, const double *grav_a,
+                                        const double *grav_n3, const double *shell_areal) {
+    using jfem::fail;
+    if (!ls) return fail(ctx, JFEM_ERR_BAD_ARG, "null load set");
+    if (ls->device != ctx->device) return fail(ctx, JFEM_ERR_BAD_ARG, "load set lives on another device");
+    if (n_grav < 0) return fail(ctx, JFEM_ERR_BAD_ARG, "negative card count");
+    const jfem_mesh *m = ls->mesh;
+    const int64_t nsh = m->nq + m->nt, nn = m->n_nodes;
+    if (n_grav > 0 && (!grav_a || !grav_n3 || (nsh > 0 && !shell_areal)))
+        return fail(ctx, JFEM_ERR_BAD_ARG, "missing GRAV arrays");
+    for (int64_t i = 0; i < nsh && n_grav > 0; ++i)
+        if (!std::isfinite(shell_areal[i])) return fail(ctx, JFEM_ERR_BAD_ARG, "non-finite shell mass");
+    JFEM_CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    for (void *p : {(void *)ls->d_grav, (void *)ls->d_areal, (void *)ls->d_gsh, (void *)ls->d_sh_ptr,
+                    (void *)ls->d_sh_elem})
+        if (p) (void)hipFree(p);
+    ls->d_grav = ls->d_areal = ls->d_gsh = nullptr;
+    ls->d_sh_ptr = nullptr;
+    ls->d_sh_elem = nullptr;
+    ls->ng = 0;
+    ls->nsh = 0;
+    if (n_grav == 0 || nsh == 0) return JFEM_OK;
+    std::vector<double> gr(4 * (size_t)n_grav);
+    for (int64_t i = 0; i < n_grav; ++i) {
+        gr[4 * i] = grav_a[i];
+        for (int c = 0; c < 3; ++c) gr[4 * i + 1 + c] = grav_n3[3 * i + c];
+    }
+    // shells per grid point in element order (CQUAD4s, then CTRIA3s)
+    std::vector<int32_t> node_of, elem_of;
+    for (int64_t e = 0; e < nsh; ++e) {
+        const bool quad = e < m->nq;
+        for (int k = 0; k < (quad ? 4 : 3); ++k) {
+            node_of.push_back(quad ? m->q[e + m->nq * k] : m->t[(e - m->nq) + m->nt * k]);
+            elem_of.push_back((int32_t)e);
+        }
+    }
+    std::vector<int32_t> ord;
+    std::vector<int64_t> sh_ptr = node_ptr(nn, node_of, ord);
+    std::vector<int32_t> sh_elem(ord.size());
+    for (size_t q = 0; q < ord.size(); ++q) sh_elem[q] = elem_of[ord[q]];
+    hipStream_t s = ctx->stream;
+    hipError_t e = upload_n(&ls->d_grav, gr.data(), gr.size(), s);
+    if (e == hipSuccess) e = upload_n(&ls->d_areal, shell_areal, (size_t)nsh, s);
+    if (e == hipSuccess) e = hipMalloc((void **)&ls->d_gsh, (size_t)nsh * sizeof(double));
+    if (e == hipSuccess) e = upload_n(&ls->d_sh_ptr, sh_ptr.data(), sh_ptr.size(), s);
+    if (e == hipSuccess) e = upload_n(&ls->d_sh_elem, sh_elem.data(), sh_elem.size(), s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(ctx, e == hipErrorOutOfMemory ? JFEM_ERR_OOM : JFEM_ERR_HIP, hipGetErrorString(e));
+    ls->ng = n_grav;
+    ls->nsh = nsh;
+    return JFEM_OK;
+}
+
 extern "C" int jfem_load_vector_dev(jfem_ctx *ctx, const jfem_loadset *ls, double scale, double *F_dev) {
     using jfem::fail;
     if (!ls || !F_dev) return fail(ctx, JFEM_ERR_BAD_ARG, "null load set or output");
@@ -309,10 +421,14 @@ extern "C" int jfem_load_vector_dev(jfem_ctx *ctx, const jfem_loadset *ls, doubl
         hipLaunchKernelGGL(pload4_kernel, dim3((unsigned)((ls->npl + kLoadBS - 1) / kLoadBS)), dim3(kLoadBS), 0, s,
                            ls->npl, ls->d_pl_elem, ls->d_pl_p, ls->d_pl_dir, scale, m->d_x, m->d_y, m->d_z, m->d_q,
                            m->nq, m->d_t, m->nt, ls->d_plf);
+    if (ls->ng > 0 && ls->nsh > 0)
+        hipLaunchKernelGGL(grav_share_kernel, dim3((unsigned)((ls->nsh + kLoadBS - 1) / kLoadBS)), dim3(kLoadBS), 0, s,
+                           ls->nsh, ls->d_areal, m->d_x, m->d_y, m->d_z, m->d_q, m->nq, m->d_t, m->nt, ls->d_gsh);
     hipLaunchKernelGGL(load_node_kernel, dim3((unsigned)((ls->n_nodes + kLoadBS - 1) / kLoadBS)), dim3(kLoadBS), 0, s,
                        ls->n_nodes, ls->npl > 0 ? ls->d_pl_ptr : nullptr, ls->d_pl_card, ls->d_plf,
                        ls->nf > 0 ? ls->d_f_ptr : nullptr, ls->d_f_rec, ls->nm > 0 ? ls->d_m_ptr : nullptr,
-                       ls->d_m_rec, scale, F_dev);
+                       ls->d_m_rec, scale, ls->nsh > 0 ? ls->ng : 0, ls->d_grav, ls->d_sh_ptr, ls->d_sh_elem,
+                       ls->d_areal, ls->d_gsh, F_dev);
     JFEM_CHECK_HIP(ctx, hipGetLastError());
     return JFEM_OK;
 }

@@ -1,8 +1,8 @@
 """Bulk-data decks in, displacement results out, for the linear-static path.
 
 Reads the cards this path covers -- GRID, CQUAD4, CTRIA3, PSHELL, MAT1, SPC1,
-FORCE, MOMENT, PLOAD4, PARAM K6ROT -- and the case control that selects them
-(SUBCASE, SPC =, LOAD =), with the reference's field rules
+FORCE, MOMENT, PLOAD4, GRAV, PARAM K6ROT -- and the executive / case control
+that selects them (SOL, SUBCASE, SPC =, LOAD =), with the reference's field rules
 (src/parsing/card_processor.jl:73-139, src/parsing/utilities.jl:18-170):
 small-field (8-column), large-field (name*, 16-column) and free-field (comma)
 lines, continuations that start with '+' or '*', have a blank name field or
@@ -13,10 +13,11 @@ reference's extractors: GRID (extract_geometry.jl), CQUAD4/CTRIA3
 (extract_materials.jl:13-45), SPC1 with THRU (extract_constraints.jl:3-14,
 utilities.jl expand_reference_list), FORCE/MOMENT with single-precision
 magnitudes and directions (extract_loads.jl:15-40), PLOAD4 with THRU
-(extract_loads.jl:75-125).
+(extract_loads.jl:75-125), GRAV (extract_loads.jl:282-297) on the shells'
+mass (MAT1 RHO, PSHELL NSM).
 
 Anything else a selected subcase would need -- coordinate systems (CP/CD/CID
-not 0), other element or load cards (PLOAD2, GRAV, LOAD combinations, ...),
+not 0), other element or load cards (PLOAD2, ACCEL, LOAD combinations, ...),
 corner pressures on CQUAD4/CTRIA3, different materials per PSHELL slot -- is
 rejected with NotImplementedError rather than dropped.
 
@@ -178,6 +179,7 @@ class Deck:
     k6rot: float = 100.0
     unsupported: set = field(default_factory=set)
     sol: int = 101                             # executive SOL number (src/parsing/card_processor.jl:227-233)
+    grav: dict = field(default_factory=dict)   # sid -> [(A, N3)] (CID 0)
 
     def recovery_mean_shear(self) -> bool:
         """Shell recovery branch of this deck's solution sequence: the MacNeal
@@ -276,7 +278,24 @@ class Deck:
             dirs = np.array([d if d is not None else (0.0, 0.0, 0.0) for _, _, d in pl])
             pload4 = (np.array([so[e] for e, _, _ in pl], np.int32), np.array([p for _, p, _ in pl]),
                       dirs if any(d is not None for _, _, d in pl) else None)
-        return {"pload4": pload4, "force": force, "moment": moment}
+        cards = {"pload4": pload4, "force": force, "moment": moment}
+        gr = self.grav.get(sid, [])
+        if gr:
+            cards["grav"] = (np.array([a for a, _ in gr]), np.array([n for _, n in gr]).reshape(-1, 3))
+            cards["shell_areal"] = self.shell_areal()
+        return cards
+
+    def shell_areal(self) -> np.ndarray:
+        """Mass per area rho T + NSM of each shell (CQUAD4s, then CTRIA3s), 0 for a
+        shell with neither density nor NSM (skipped by the GRAV loop,
+        src/solver/loads.jl:735-740; rho from the PSHELL's MID1)."""
+        out = []
+        for (_, pid, _) in self.quads + self.trias:
+            ps = self.pshell[pid]
+            rho = self.mat1.get(ps["mid1"], {}).get("rho", 0.0) or 0.0
+            nsm = ps.get("nsm", 0.0) or 0.0
+            out.append(0.0 if (rho <= 0 and nsm <= 0) else rho * ps["t"] + nsm)
+        return np.array(out, float)
 
 
 def _single(x):
@@ -354,7 +373,7 @@ def read_deck(source: str) -> Deck:
 
     grids, ps = {}, {}
     quads, trias, pshell, mat1 = [], [], {}, {}
-    spc1, force, moment, pload4, other = {}, {}, {}, {}, {}
+    spc1, force, moment, pload4, other, grav = {}, {}, {}, {}, {}, {}
     k6rot = 100.0
     unsupported = set()
     for c in bulk_cards(bulk):
@@ -394,7 +413,7 @@ def read_deck(source: str) -> Deck:
             pshell[pid] = {"mid1": to_id(num(2)), "t": num(3, 0.0), "mid2": mid2,
                            "bend_ratio": 0.0 if mid2 == 0 else num(5, 1.0),
                            "mid3": 0 if mid3v in (None, 0.0) else to_id(mid3v), "ts_t": num(7, 5.0 / 6.0),
-                           "mid4": 0 if mid4v in (None, 0.0) else to_id(mid4v)}
+                           "mid4": 0 if mid4v in (None, 0.0) else to_id(mid4v), "nsm": num(8, 0.0)}
         elif name == "MAT1":
             mid = to_id(num(1))
             E_raw, G_raw, nu_raw = num(2), num(3), num(4)
@@ -409,7 +428,7 @@ def read_deck(source: str) -> Deck:
                 E = 2 * G * (1 + nu)
             if nu < 0:
                 nu = 0.3
-            mat1[mid] = {"E": E, "G": G, "nu": nu}
+            mat1[mid] = {"E": E, "G": G, "nu": nu, "rho": num(5, 0.0)}
         elif name == "SPC1":
             sid = to_id(num(1))
             comps = [int(ch) for ch in str(g(2)).strip() if ch.isdigit() and ch != "0"]
@@ -439,6 +458,12 @@ def read_deck(source: str) -> Deck:
                 raise NotImplementedError(f"PLOAD4 SID={sid}: SORL/LDIR other than SURF/NORM")
             for e in (range(eid, eid2 + 1) if thru and eid2 > eid else [eid]):
                 pload4.setdefault(sid, []).append((e, p, d))
+        elif name == "GRAV":                       # extract_grav, src/parsing/extract_loads.jl:282-297
+            sid, cid = to_id(num(1)), to_id(num(2, 0.0))
+            if sid > 0:
+                if cid:
+                    raise NotImplementedError(f"GRAV SID={sid}: CID {cid} is not on this path")
+                grav.setdefault(sid, []).append((num(3, 0.0), tuple(num(4 + k, 0.0) for k in range(3))))
         elif name in LOAD_CARDS:
             other.setdefault(to_id(num(1)), set()).add(name)
         elif name not in NEUTRAL_CARDS:
@@ -446,7 +471,7 @@ def read_deck(source: str) -> Deck:
     ids = sorted(grids)                                    # GRID rank order (assembly.jl:2653-2657)
     xyz = np.array([grids[i] for i in ids], float).reshape(-1, 3)
     return Deck(ids, xyz, ps, quads, trias, pshell, mat1, spc1, force, moment, pload4, other, subs, k6rot,
-                unsupported, sol)
+                unsupported, sol, grav)
 
 
 def solve_deck(deck: Deck, precision: int = 64, tol: float = 1e-10, membrane_modes: int = 15, ctx=None,

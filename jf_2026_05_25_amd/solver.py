@@ -346,15 +346,17 @@ def single_precision(a):
 
 
 class LoadSet:
-    """One load SID on the device: FORCE / MOMENT cards on grid points and uniform
-    PLOAD4 cards on shell elements, in card order (the FORCE, MOMENT and shell
-    PLOAD4 branches of _resolve_loads!, src/solver/loads.jl:517-587, 1338-1340).
+    """One load SID on the device: FORCE / MOMENT cards on grid points, uniform
+    PLOAD4 cards on shell elements and GRAV cards on shell mass, in card order
+    (the FORCE, MOMENT, shell PLOAD4 and shell GRAV branches of _resolve_loads!,
+    src/solver/loads.jl:517-587, 680-772, 1338-1340).
 
     pload4 = (elem, p[, dir3]): shell element e is CQUAD4 e (e < n_quad) or CTRIA3
     e - n_quad; a dir3 row of norm <= 1e-30 (or no dir3) takes the element normal.
     force / moment = (node, mag, dir3), values as given (see single_precision)."""
 
-    def __init__(self, mesh: DeviceMesh, pload4=None, force=None, moment=None, ctx: Context | None = None):
+    def __init__(self, mesh: DeviceMesh, pload4=None, force=None, moment=None, ctx: Context | None = None,
+                 grav=None, shell_areal=None):
         self.ctx = ctx or mesh.ctx
         self.mesh = mesh
         L = self.ctx.L
@@ -381,6 +383,19 @@ class LoadSet:
                                                 C.byref(h)))
         self.h = h
         self.ndof = 6 * mesh.model.n_nodes
+        if grav is not None and len(grav[0]):
+            self.set_gravity(grav, shell_areal)
+
+    def set_gravity(self, grav, shell_areal):
+        """GRAV cards on the shells: grav = (A (n,), N (n, 3)) in the basic frame,
+        shell_areal = rho T + NSM per shell (CQUAD4s then CTRIA3s, 0 without mass):
+        the lumped shell branch of the GRAV loads (src/solver/loads.jl:680-772)."""
+        m = self.mesh.model
+        a = np.ascontiguousarray(np.asarray(grav[0], float).reshape(-1))
+        n3 = np.ascontiguousarray(np.asarray(grav[1], float).reshape(len(a), 3))
+        ar = np.ascontiguousarray(np.asarray(shell_areal, float).reshape(len(m.quads) + len(m.trias)))
+        self._gkeep = [a, n3, ar]
+        check(self.ctx.h, self.ctx.L.jfem_loadset_set_gravity(self.ctx.h, self.h, len(a), ptr(a), ptr(n3), ptr(ar)))
 
     def vector(self, scale: float = 1.0) -> np.ndarray:
         """The load vector (all 6 n DOFs) with the SID's scale factor."""

@@ -1871,6 +1871,7 @@ int or_load_vector(int64_t n_nodes, const double *x, const double *y, const doub
                    const int32_t *pl_elem, const double *pl_p, const double *pl_dir3, int64_t n_force,
                    const int32_t *f_node, const double *f_mag, const double *f_dir3, int64_t n_moment,
                    const int32_t *m_node, const double *m_mag, const double *m_dir3, double scale,
+                   const double *shell_areal, int64_t n_grav, const double *grav_a, const double *grav_n3,
                    double *F) {
     if (n_nodes <= 0) return OR_ERR_BAD_ARG;
     for (int64_t i = 0; i < 6 * n_nodes; ++i) F[i] = 0.0;
@@ -1908,6 +1909,31 @@ int or_load_vector(int64_t n_nodes, const double *x, const double *y, const doub
         for (int c = 0; c < 3; ++c) f_node[c] = load_dir[c] * (tf / (double)nn);
         for (int k = 0; k < nn; ++k)
             for (int c = 0; c < 3; ++c) F[6 * (int64_t)nids[k] + c] += f_node[c];
+    }
+    /* GRAV on shells, loads.jl:680-684, 730-772: per card the acceleration
+     * (A scale) N, per shell (element order: CQUAD4s, then CTRIA3s) with a mass
+     * (rho T + NSM, given per shell; 0 skips it) the lumped share
+     * (areal * area / nn) .* g on each node's translations, area = norm of the
+     * diagonal (quad) / edge (tria) cross product over 2 */
+    for (int64_t gi = 0; gi < n_grav; ++gi) {
+        double gv[3];
+        for (int c = 0; c < 3; ++c) gv[c] = (grav_a[gi] * scale) * grav_n3[3 * gi + c];
+        for (int64_t e = 0; e < nq + nt; ++e) {
+            if (!shell_areal || shell_areal[e] == 0.0) continue;
+            int nids[4], nn;
+            if (e < nq) { nn = 4; for (int k = 0; k < 4; ++k) nids[k] = quad_conn[e + nq * k]; }
+            else { nn = 3; for (int k = 0; k < 3; ++k) nids[k] = tria_conn[(e - nq) + nt * k]; }
+            double Xc[4][3];
+            for (int k = 0; k < nn; ++k) { Xc[k][0] = x[nids[k]]; Xc[k][1] = y[nids[k]]; Xc[k][2] = z[nids[k]]; }
+            double a[3], b[3], cr[3];
+            if (nn == 4) { v3sub(Xc[2], Xc[0], a); v3sub(Xc[3], Xc[1], b); }
+            else { v3sub(Xc[1], Xc[0], a); v3sub(Xc[2], Xc[0], b); }
+            v3cross(a, b, cr);
+            const double area = 0.5 * or_jl_norm3(cr);
+            const double total_mass = shell_areal[e] * area;
+            for (int k = 0; k < nn; ++k)
+                for (int c = 0; c < 3; ++c) F[6 * (int64_t)nids[k] + c] += (total_mass / nn) * gv[c];
+        }
     }
     /* FORCE / MOMENT into raw_forces[gid] (zeros(6) on first use), loads.jl:519-539;
      * applied after the distributed loads, :1338-1340 */

@@ -121,17 +121,21 @@ int or_solve(int64_t ndof, const int64_t *row_ptr, const int32_t *col, const dou
 int or_pivot_ratios(int64_t ndof, const int64_t *row_ptr, const int32_t *col, const double *val,
                     const uint8_t *fixed, const int64_t *order, double *ratios, int64_t *n_small);
 
-/* ---------- load vector of one load set (FORCE / MOMENT / shell PLOAD4) ---------- */
-/* Follows _resolve_loads! (src/solver/loads.jl:510-587, 1338-1340) for one SID in
- * basic coordinates: PLOAD4 cards in order straight into F, then the per-grid sums
- * of the FORCE and MOMENT cards (card order) added to F.  Shell element e: CQUAD4 e
- * (e < nq) or CTRIA3 e - nq; pl_dir3 NULL (or a direction of norm <= 1e-30) takes
- * the element normal.  F: 6 n_nodes doubles (overwritten). */
+/* ---------- load vector of one load set (FORCE / MOMENT / shell PLOAD4 / GRAV) ---------- */
+/* Follows _resolve_loads! (src/solver/loads.jl:510-587, 680-772, 1338-1340) for one
+ * SID in basic coordinates: PLOAD4 cards in order straight into F, then the GRAV
+ * cards' lumped shell body forces (per card, shells in element order; shell_areal
+ * = rho T + NSM per shell, 0 where the shell carries no mass), then the per-grid
+ * sums of the FORCE and MOMENT cards (card order) added to F.  Shell element e:
+ * CQUAD4 e (e < nq) or CTRIA3 e - nq; pl_dir3 NULL (or a direction of norm
+ * <= 1e-30) takes the element normal; GRAV card i: acceleration A = grav_a[i]
+ * along grav_n3[3i..3i+2] (basic frame).  F: 6 n_nodes doubles (overwritten). */
 int or_load_vector(int64_t n_nodes, const double *x, const double *y, const double *z, int64_t nq,
                    const int32_t *quad_conn, int64_t nt, const int32_t *tria_conn, int64_t n_pload4,
                    const int32_t *pl_elem, const double *pl_p, const double *pl_dir3, int64_t n_force,
                    const int32_t *f_node, const double *f_mag, const double *f_dir3, int64_t n_moment,
                    const int32_t *m_node, const double *m_mag, const double *m_dir3, double scale,
+                   const double *shell_areal, int64_t n_grav, const double *grav_a, const double *grav_n3,
                    double *F);
 /* Julia's norm / normalize of a 3-vector, exposed for their pins */
 double or_jl_norm3(const double v[3]);

@@ -57,7 +57,8 @@ def lib():
         L.or_eig3.argtypes = [dp, dp, dp]
         vpp = C.c_void_p
         L.or_load_vector.argtypes = [C.c_int64, dp, dp, dp, C.c_int64, vpp, C.c_int64, vpp, C.c_int64, vpp, vpp, vpp,
-                                     C.c_int64, vpp, vpp, vpp, C.c_int64, vpp, vpp, vpp, C.c_double, dp]
+                                     C.c_int64, vpp, vpp, vpp, C.c_int64, vpp, vpp, vpp, C.c_double,
+                                     vpp, C.c_int64, vpp, vpp, dp]
         L.or_jl_norm3.argtypes = [dp]
         L.or_jl_norm3.restype = C.c_double
         L.or_jl_normalize3.argtypes = [dp]
@@ -311,10 +312,11 @@ def jl_normalize3(v):
     return a
 
 
-def load_vector(model, pload4=None, force=None, moment=None, scale=1.0):
-    """Load vector of one load set (FORCE / MOMENT / shell PLOAD4, card order), all
-    6 n DOFs: src/solver/loads.jl:510-587, 1338-1340.  Cards as in
-    jf_2026_05_25_amd.solver.LoadSet."""
+def load_vector(model, pload4=None, force=None, moment=None, scale=1.0, grav=None, shell_areal=None):
+    """Load vector of one load set (FORCE / MOMENT / shell PLOAD4 / GRAV, card
+    order), all 6 n DOFs: src/solver/loads.jl:510-587, 680-772, 1338-1340.  Cards
+    as in jf_2026_05_25_amd.solver.LoadSet; grav: (A (n,), N (n, 3)) with the
+    per-shell areal mass shell_areal (rho T + NSM, CQUAD4s then CTRIA3s)."""
     keep = []
 
     def arr(a, dt, shape=None):
@@ -342,9 +344,14 @@ def load_vector(model, pload4=None, force=None, moment=None, scale=1.0):
     tc = _cm_i32(model.trias)
     keep += [qc, tc]
     F = np.zeros(6 * len(xyz))
+    ng, ga, gn, sa = 0, None, None, None
+    if grav is not None and len(grav[0]):
+        ng = len(grav[0])
+        ga, gn = arr(grav[0], float, (ng,)), arr(grav[1], float, (ng, 3))
+        sa = arr(shell_areal, float, (len(model.quads) + len(model.trias),))
     st = lib().or_load_vector(len(xyz), x, y, z, len(model.quads), qc.ctypes.data if qc.size else None,
                               len(model.trias), tc.ctypes.data if tc.size else None, npl, pe, pp, pd,
-                              nf, fn, fm, fd, nm, mn, mm, md, float(scale), F)
+                              nf, fn, fm, fd, nm, mn, mm, md, float(scale), sa, ng, ga, gn, F)
     if st != 0:
         raise OracleError(st, "load_vector")
     return F

@@ -91,9 +91,10 @@ def fixture_deck(case, style):
     for pid, ps in case["pshell"].items():
         out.append(write("PSHELL", [pid, str(ps["mid1"]), ps["t"], str(ps["mid2"]) if ps["mid2"] else "",
                                     ps["bend_ratio"] if ps["mid2"] else "", str(ps["mid3"]) if ps["mid3"] else "",
-                                    ps["ts_t"]]))
+                                    ps["ts_t"]] + ([ps["nsm"]] if ps.get("nsm") else [])))
     for mid, mt in case["mat1"].items():
-        out.append(write("MAT1", [mid, mt["E"], mt["G"] if mt["G"] else "", mt["nu"]]))
+        out.append(write("MAT1", [mid, mt["E"], mt["G"] if mt["G"] else "", mt["nu"]] +
+                         ([mt["rho"]] if mt.get("rho") else [])))
         out.append("$ inline comments are stripped $ like this one")
     comps = {}
     for s in case["spc1"]:
@@ -110,5 +111,7 @@ def fixture_deck(case, style):
             out.append(write("FORCE", [str(s["load_sid"]), str(f["grid"]), "", f["mag"]] + list(f["dir"])))
         for p in s["pload4"]:
             out.append(write("PLOAD4", [str(s["load_sid"]), str(p["eid"]), p["p"]]))
+        for gr in s.get("grav", []):
+            out.append(write("GRAV", [str(s["load_sid"]), "0", gr["a"]] + list(gr["n"])))
     out.append("ENDDATA")
     return "\n".join(out) + "\n"

@@ -16,7 +16,8 @@ def golden_cases():
 
 def golden_model(case: dict, sub: dict, membrane_modes: int = 15) -> meshes.Model:
     """Model for one subcase of a golden deck: GRIDs in ascending ID order
-    (src/solver/assembly.jl:2653-2657), PSHELL/MAT1 sections, SPC1, FORCE/PLOAD4."""
+    (src/solver/assembly.jl:2653-2657), PSHELL/MAT1 sections, SPC1, FORCE/PLOAD4
+    and GRAV (lumped shell self-weight)."""
     ids = sorted(int(k) for k in case["grids"])
     idx = {g: i for i, g in enumerate(ids)}
     xyz = np.array([case["grids"][str(g)] for g in ids], dtype=float)
@@ -41,6 +42,21 @@ def golden_model(case: dict, sub: dict, membrane_modes: int = 15) -> meshes.Mode
     for p in sub["pload4"]:
         e = eid_row[p["eid"]]
         F += meshes.pload4_forces(xyz, quads[e:e + 1], p["p"])
+    for gcard in sub.get("grav", []):
+        # lumped shell self-weight: (rho T + NSM) * area / 4 per corner, area half the
+        # norm of the diagonal cross product (src/solver/loads.jl:758-772)
+        g = gcard["a"] * np.asarray(gcard["n"], float)
+        for e, q in enumerate(case["cquad4"]):
+            ps = case["pshell"][str(q["pid"])]
+            rho = case["mat1"][str(ps["mid1"])].get("rho", 0.0)
+            nsm = ps.get("nsm", 0.0)
+            if rho <= 0 and nsm <= 0:
+                continue
+            X = xyz[quads[e]]
+            area = 0.5 * np.linalg.norm(np.cross(X[2] - X[0], X[3] - X[1]))
+            share = (rho * ps["t"] + nsm) * area / 4
+            for n in quads[e]:
+                F[6 * n:6 * n + 3] += share * g
     expected = np.zeros(ndof)
     for g, v in sub["expected"].items():
         expected[6 * idx[int(g)]:6 * idx[int(g)] + 6] = v
@@ -48,6 +64,22 @@ def golden_model(case: dict, sub: dict, membrane_modes: int = 15) -> meshes.Mode
                      np.zeros((0, 8)), np.zeros((0, 10), np.int32), np.zeros((0, 2)),
                      spc, F[:, None], case["k6rot"], {"expected": expected})
     return m
+
+
+def golden_grav(case: dict, sub: dict):
+    """The subcase's GRAV cards as LoadSet / oracle.load_vector arguments:
+    {"grav": (A, N), "shell_areal": rho T + NSM per CQUAD4} (empty without GRAV)."""
+    gr = sub.get("grav", [])
+    if not gr:
+        return {}
+    areal = []
+    for q in case["cquad4"]:
+        ps = case["pshell"][str(q["pid"])]
+        rho = case["mat1"][str(ps["mid1"])].get("rho", 0.0)
+        nsm = ps.get("nsm", 0.0)
+        areal.append(0.0 if (rho <= 0 and nsm <= 0) else rho * ps["t"] + nsm)
+    return {"grav": (np.array([g["a"] for g in gr]), np.array([g["n"] for g in gr], float)),
+            "shell_areal": np.array(areal)}
 
 
 def flap_model(n: int = 4, k6rot: float = 0.01) -> meshes.Model:

@@ -11,14 +11,15 @@ and their recorded displacement vectors ``results/*.JU.JSON``:
   (curved shells of warped CQUAD4, FORCE, symmetry SPC1)
 * validation/cases/MH_hemispherical_shell/convergence/{N08,N16} (the same
   hemisphere on 8x8 and 16x16 CQUAD4 meshes)
+* validation/cases/MH_scordelis_lo{,/convergence/N08,/convergence/N16}
+  (the Scordelis-Lo roof under self-weight: GRAV on shell mass)
 
 Where a result file also records CQUAD4 shell recovery (the MYSTRAN case:
 forces, forces_bilin, stresses, strains of its element), each subcase carries
 those rows under "shell_recovery", and the case its solution sequence ("sol",
 the executive SOL number, which selects the recovery branch).
 
-Scordelis-Lo (GRAV load) and the TACS-backend uCRM-9 vector are not on this
-path and are not extracted.  Output: tests/golden/reference_ju.json.
+The TACS-backend uCRM-9 vector is not on this path and is not extracted.  Output: tests/golden/reference_ju.json.
 Run from the repository root: ``python tests/golden/make_fixtures.py``.
 """
 import json
@@ -39,10 +40,15 @@ CASES = [
     ("MH_hemispherical_shell", "results/hemispherical_shell.JU.JSON", {1: 1}, 1),
     ("MH_hemispherical_shell_N08", "results/MH_hemispherical_shell_N08.JU.JSON", {1: 1}, 1),
     ("MH_hemispherical_shell_N16", "results/MH_hemispherical_shell_N16.JU.JSON", {1: 1}, 1),
+    ("MH_scordelis_lo", "results/scordelis_lo.JU.JSON", {1: 1}, 1),
+    ("MH_scordelis_lo_N08", "results/MH_scordelis_lo_N08.JU.JSON", {1: 1}, 1),
+    ("MH_scordelis_lo_N16", "results/MH_scordelis_lo_N16.JU.JSON", {1: 1}, 1),
 ]
 # deck directory under validation/cases when it is not the case name
 CASE_DIR = {"MH_hemispherical_shell_N08": "MH_hemispherical_shell/convergence/N08",
-            "MH_hemispherical_shell_N16": "MH_hemispherical_shell/convergence/N16"}
+            "MH_hemispherical_shell_N16": "MH_hemispherical_shell/convergence/N16",
+            "MH_scordelis_lo_N08": "MH_scordelis_lo/convergence/N08",
+            "MH_scordelis_lo_N16": "MH_scordelis_lo/convergence/N16"}
 
 
 def num(s):
@@ -89,7 +95,7 @@ def cards(path):
 def parse(case, loads_by_subcase, spc_sid):
     cdir = CASE_DIR.get(case, case)
     deck = os.path.join(REF, cdir, "model.bdf")
-    grids, quads, pshell, mat1, spc, forces, pload4 = {}, [], {}, {}, [], {}, {}
+    grids, quads, pshell, mat1, spc, forces, pload4, grav = {}, [], {}, {}, [], {}, {}, {}
     k6rot = 100.0
     for f in cards(deck):
         name = f[0].upper()
@@ -105,9 +111,16 @@ def parse(case, loads_by_subcase, spc_sid):
                                  "mid2": int(g(4)) if g(4) else 0,
                                  "bend_ratio": num(g(5)) if g(5) else 1.0,
                                  "mid3": int(g(6)) if g(6) else 0,
-                                 "ts_t": num(g(7)) if g(7) else 5.0 / 6.0}
+                                 "ts_t": num(g(7)) if g(7) else 5.0 / 6.0,
+                                 "nsm": num(g(8)) or 0.0}
         elif name == "MAT1":
-            mat1[int(g(1))] = {"E": num(g(2)), "G": num(g(3)) or 0.0, "nu": num(g(4)) or 0.0}
+            mat1[int(g(1))] = {"E": num(g(2)), "G": num(g(3)) or 0.0, "nu": num(g(4)) or 0.0,
+                               "rho": num(g(5)) or 0.0}
+        elif name == "GRAV":
+            if int(g(2) or 0) != 0:
+                raise ValueError("GRAV with a coordinate system")
+            grav.setdefault(int(g(1)), []).append({"a": num(g(3)) or 0.0,
+                                                   "n": [num(g(4)) or 0.0, num(g(5)) or 0.0, num(g(6)) or 0.0]})
         elif name == "SPC1":
             sid = int(g(1))
             comps = [int(c) for c in g(2)]
@@ -140,7 +153,7 @@ def parse(case, loads_by_subcase, spc_sid):
         rows = disp[k * per:(k + 1) * per]
         sub = {
             "subcase": sid, "load_sid": lsid,
-            "force": forces.get(lsid, []), "pload4": pload4.get(lsid, []),
+            "force": forces.get(lsid, []), "pload4": pload4.get(lsid, []), "grav": grav.get(lsid, []),
             "expected": {str(r["grid_id"]): [r["t1"], r["t2"], r["t3"], r["r1"], r["r2"], r["r3"]] for r in rows},
             "reference_free_dofs": diag["details"]["bc_partition"]["free_dofs"],
             "reference_relative_residual": diag["details"]["linear_solver"]["relative_residual"],

@@ -1,6 +1,6 @@
-"""GPU parity of the device load-vector builder (FORCE / MOMENT / shell PLOAD4 of
-one load set) with the oracle: bit-exact (the sums run in the reference's order
-on both sides), on random load sets over the wing box and a mixed plate, with
+"""GPU parity of the device load-vector builder (FORCE / MOMENT / shell PLOAD4 /
+GRAV on shell mass of one load set) with the oracle: bit-exact (the sums run in
+the reference's order on both sides), on random load sets over the wing box and a mixed plate, with
 repeated cards, explicit directions, a zero-area element, an empty set, the
 device-output entry point, and end to end on the golden decks (device-built
 loads -> solve -> the reference's recorded displacements)."""
@@ -45,11 +45,20 @@ def test_load_vector_bitwise_equals_oracle(ctx, oracle, name, scale):
     rng = np.random.default_rng(17)
     pl, fc, mc = random_cards(m, rng, 3 * (len(m.quads) + len(m.trias)), 40, 25)
     print(f"load set: {name}, scale {scale}, {len(pl[0])} PLOAD4, {len(fc[0])} FORCE, {len(mc[0])} MOMENT", flush=True)
+    # two GRAV cards on shell mass, a fifth of the shells massless
+    ne = len(m.quads) + len(m.trias)
+    areal = rng.uniform(1.0, 20.0, ne)
+    areal[rng.random(ne) < 0.2] = 0.0
+    gr = {"grav": (rng.normal(0.0, 10.0, 2), rng.normal(size=(2, 3))), "shell_areal": areal}
     mesh = solver.DeviceMesh(m, ctx)
-    ls = solver.LoadSet(mesh, pload4=pl, force=fc, moment=mc)
+    ls = solver.LoadSet(mesh, pload4=pl, force=fc, moment=mc, **gr)
     F = ls.vector(scale)
-    Fo = oracle.load_vector(m, pload4=pl, force=fc, moment=mc, scale=scale)
+    Fo = oracle.load_vector(m, pload4=pl, force=fc, moment=mc, scale=scale, **gr)
     assert np.array_equal(F, Fo), np.abs(F - Fo).max()
+    # without the GRAV cards
+    ls.set_gravity(([], np.zeros((0, 3))), areal)
+    assert np.array_equal(ls.vector(scale), oracle.load_vector(m, pload4=pl, force=fc, moment=mc, scale=scale))
+    ls.set_gravity(gr["grav"], areal)
     # device output on the context's stream
     nb = 8 * m.ndof
     dptr = C.c_void_p()
@@ -90,7 +99,7 @@ def test_load_vector_edge_cases(ctx, oracle):
 def test_golden_decks_with_device_loads(ctx, oracle):
     """Device-built loads of the golden decks' subcases, solved on the GPU (FP64),
     reproduce the reference's recorded displacements as the oracle does."""
-    from _helpers import golden_cases, golden_model
+    from _helpers import golden_cases, golden_grav, golden_model
     from test_oracle_system import GOLDEN_TOL
     from jf_2026_05_25_amd import solver
     bad = []
@@ -106,8 +115,9 @@ def test_golden_decks_with_device_loads(ctx, oracle):
             pload4 = ([eid_row[p["eid"]] for p in pl], [p["p"] for p in pl]) if pl else None
             print(f"golden deck {c['case']} subcase {s['subcase']}", flush=True)
             asm = solver.assemble_stiffness(m, ctx)
-            F = solver.LoadSet(asm.mesh, pload4=pload4, force=force).vector()
-            if not np.array_equal(F, oracle.load_vector(m, pload4=pload4, force=force)):
+            gr = golden_grav(c, s)
+            F = solver.LoadSet(asm.mesh, pload4=pload4, force=force, **gr).vector()
+            if not np.array_equal(F, oracle.load_vector(m, pload4=pload4, force=force, **gr)):
                 bad.append((c["case"], s["subcase"], "load vector"))
                 continue
             u, fixed, spc, diag = solver.apply_bc_and_solve(asm, m, F_applied=F[:, None], precision=64)

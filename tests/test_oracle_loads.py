@@ -10,7 +10,7 @@ import pytest
 from jf_2026_05_25_amd import meshes
 from jf_2026_05_25_amd.solver import single_precision
 
-from _helpers import golden_cases, golden_model
+from _helpers import golden_cases, golden_grav, golden_model
 
 
 def one_quad(X, tria=False):
@@ -163,5 +163,51 @@ def test_golden_deck_loads(oracle, ci):
             m,
             pload4=([eid_row[p["eid"]] for p in pl], [p["p"] for p in pl]) if pl else None,
             force=([idx[f["grid"]] for f in fc], single_precision([f["mag"] for f in fc]),
-                   single_precision([f["dir"] for f in fc])) if fc else None)
+                   single_precision([f["dir"] for f in fc])) if fc else None,
+            **golden_grav(case, sub))
         np.testing.assert_allclose(F, m.loads[:, 0], rtol=1e-14, atol=1e-14 * abs(F).max())
+
+
+# ---- GRAV on shell mass (src/solver/loads.jl:680-684, 730-772) -------------------------
+def test_grav_totals_and_shares(oracle):
+    """Lumped self-weight: the total is A (rho T + NSM) times the mesh area along N;
+    on a uniform CQUAD4 mesh an interior grid point collects four quarter shares,
+    a corner one; a CTRIA3 gives each node a third of its mass."""
+    A, N = 9.81, np.array([0.0, -0.6, -0.8])
+    m = meshes.flat_plate(8, 6, Lx=0.4, Ly=0.3)
+    areal = np.full(len(m.quads), 7800.0 * 0.002 + 0.5)
+    F = oracle.load_vector(m, grav=([A], [N]), shell_areal=areal).reshape(9 * 7 // 9, 9, 6)
+    assert np.all(F[:, :, [0, 3, 4, 5]] == 0.0)
+    tot = F[:, :, :3].sum(axis=(0, 1))
+    np.testing.assert_allclose(tot, A * areal[0] * 0.4 * 0.3 * N, rtol=1e-13)
+    q = A * areal[0] * 0.05 * 0.05 / 4 * N
+    np.testing.assert_allclose(F[1:-1, 1:-1, :3], np.broadcast_to(4 * q, F[1:-1, 1:-1, :3].shape), rtol=1e-13)
+    np.testing.assert_allclose(F[0, 0, :3], q, rtol=1e-13)
+    X = np.array([[0, 0, 0], [2.0, 0, 0], [0, 1.0, 1.0]])
+    Ft = oracle.load_vector(one_quad(X, tria=True), grav=([2.0], [[0, 0, 1.0]]), shell_areal=[3.0]).reshape(-1, 6)
+    area = 0.5 * np.linalg.norm(np.cross(X[1] - X[0], X[2] - X[0]))
+    for k in range(3):
+        assert Ft[k, 2] == pytest.approx(3.0 * area / 3 * 2.0, rel=1e-14) and Ft[k, 0] == 0.0
+
+
+def test_grav_skips_massless_shells_and_scales(oracle):
+    m = meshes.flat_plate(2, 1)
+    F = oracle.load_vector(m, grav=([1.0], [[0, 0, -1.0]]), shell_areal=[0.0, 2.0], scale=3.0).reshape(-1, 6)
+    # quad 0 (nodes 0, 1, 4, 3) has no mass: grid 0 and 3 carry nothing
+    assert F[0, 2] == 0.0 and F[3, 2] == 0.0
+    assert F[2, 2] == pytest.approx(-3.0 * 2.0 * 0.25 * 0.5 / 4, rel=1e-14)  # 0.25 x 0.5 cells
+    # two cards add in turn
+    F2 = oracle.load_vector(m, grav=([1.0, 1.0], [[0, 0, -1.0], [0, 0, -1.0]]), shell_areal=[0.0, 2.0]).reshape(-1, 6)
+    assert F2[2, 2] == 2 * F[2, 2] / 3.0
+
+
+def test_grav_enters_after_pressure_and_before_point_loads(oracle):
+    """Unit-square CQUAD4: p = 4 gives each corner 1.0 along z, a GRAV share of
+    2^53 along z follows it (1 + 2^53 rounds to 2^53), then the FORCE card of
+    -2^53: F_z = 0.  Adding the point load before the GRAV share would give 1."""
+    X = np.array([[0, 0, 0], [1.0, 0, 0], [1.0, 1.0, 0], [0, 1.0, 0]])
+    m = one_quad(X)
+    big = 2.0 ** 53
+    F = oracle.load_vector(m, pload4=([0], [4.0]), force=([0], [-big], [[0, 0, 1.0]]),
+                           grav=([1.0], [[0, 0, 1.0]]), shell_areal=[4 * big])
+    assert F[2] == 0.0 and F[6 + 2] == big
