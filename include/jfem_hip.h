@@ -209,6 +209,44 @@ int jfem_bc_partition(jfem_ctx *ctx, const jfem_matrix *K, const uint8_t *spc,
 int jfem_shell_recover(jfem_ctx *ctx, const jfem_mesh *mesh, const double *u, int mean_shear,
                        double *q4_out, double *t3_out, int64_t *bad_element);
 
+/* ---- SOL 105: differential stiffness, shifted matrices, block products ------ */
+/* Differential (geometric) stiffness of the CQUAD4s of a mesh under the membrane
+ * state of the preload displacements u (host, ndof doubles, global DOF order):
+ * per element the default operator of a flat isotropic CQUAD4,
+ * geometric_stiffness_quad4_reference_kdjj_iso (src/FEMKernels.jl:8810-8942),
+ * with the assembly's preload map, warp congruence and element-frame transform
+ * (src/solver/assembly.jl:6985-6998, 8063-8075, 8264-8320).  A cell whose corners
+ * leave the plane of its diagonals by 1e-2 of the longer diagonal or more is not
+ * on this operator (assembly.jl:6575-6594): JFEM_ERR_WARPED, bad_element its
+ * index.  Meshes with CTRIA3 or CTETRA10 elements return JFEM_ERR_BAD_ARG.
+ * jfem_q4_kg_batch: per element the global 24x24 matrix (column-major, nq x 576
+ * doubles, host).  jfem_assemble_kg: the assembled Kg on the pattern of K
+ * (assemble_geometric_stiffness, src/solver/assembly.jl:6106; duplicates summed
+ * in element order), a jfem_matrix like K's. */
+int jfem_q4_kg_batch(jfem_ctx *ctx, const jfem_mesh *mesh, const double *u, double *kg_out,
+                     int64_t *bad_element);
+int jfem_assemble_kg(jfem_ctx *ctx, const jfem_mesh *mesh, const jfem_pattern *pattern,
+                     const double *u, jfem_matrix **out, int64_t *bad_element);
+/* out = K + sigma * Kg (values of two matrices on one pattern): the shifted
+ * matrix of the buckling pencil, factored with jfem_factor_create / jfem_refactor
+ * on the symbolic analysis of K (the reference's shifted factorizations of
+ * K + sigma Kg, src/solver/helpers.jl:105-143, 280-420).  *out == NULL allocates
+ * a new matrix; otherwise *out (same pattern) is overwritten. */
+int jfem_matrix_shift(jfem_ctx *ctx, const jfem_matrix *K, double sigma, const jfem_matrix *Kg,
+                      jfem_matrix **out);
+/* Block products of the eigen-extraction, device blocks in full DOF numbering
+ * (ndof x ncol column-major, at most 32 columns), sums in a fixed order:
+ *   jfem_matrix_spmv:   y = A_ff x (fixed: host mask of ndof bytes, 1 = fixed, or
+ *                       NULL; fixed rows of y are zero, fixed columns skipped);
+ *   jfem_block_gram:    c_host (p x q column-major) = x' y over n rows;
+ *   jfem_block_combine: x = y Q for a host p x q matrix Q (column-major). */
+int jfem_matrix_spmv(jfem_ctx *ctx, const jfem_matrix *A, const uint8_t *fixed, int ncol,
+                     const double *x_dev, double *y_dev);
+int jfem_block_gram(jfem_ctx *ctx, int64_t n, int p, const double *x_dev, int q,
+                    const double *y_dev, double *c_host);
+int jfem_block_combine(jfem_ctx *ctx, int64_t n, int p, const double *y_dev, int q,
+                       const double *Q_host, double *x_dev);
+
 /* ---- factor / solve -------------------------------------------------------- */
 typedef struct {
     int64_t n_free;          /* order of K_ff */

@@ -37,6 +37,8 @@ EXPORTS = [
     "jfem_loadset_create", "jfem_loadset_set_gravity", "jfem_load_vector", "jfem_load_vector_dev", "jfem_loadset_free",
     "jfem_batch_create", "jfem_batch_update_props", "jfem_batch_assemble", "jfem_batch_matrix", "jfem_batch_factor",
     "jfem_batch_factor_view", "jfem_batch_free",
+    "jfem_q4_kg_batch", "jfem_assemble_kg", "jfem_matrix_shift", "jfem_matrix_spmv", "jfem_block_gram",
+    "jfem_block_combine",
 ]
 
 
@@ -84,6 +86,12 @@ def load():
         "jfem_t3_local_batch": ([vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, P(i64)], C.c_int),
         "jfem_tet10_batch": ([vp, i64, vp, vp, vp, vp, P(i64)], C.c_int),
         "jfem_shell_recover": ([vp, vp, vp, C.c_int, vp, vp, P(i64)], C.c_int),
+        "jfem_q4_kg_batch": ([vp, vp, vp, vp, P(i64)], C.c_int),
+        "jfem_assemble_kg": ([vp, vp, vp, vp, P(vp), P(i64)], C.c_int),
+        "jfem_matrix_shift": ([vp, vp, dbl, vp, P(vp)], C.c_int),
+        "jfem_matrix_spmv": ([vp, vp, vp, C.c_int, vp, vp], C.c_int),
+        "jfem_block_gram": ([vp, i64, C.c_int, vp, C.c_int, vp, vp], C.c_int),
+        "jfem_block_combine": ([vp, i64, C.c_int, vp, C.c_int, vp, vp], C.c_int),
         "jfem_mesh_create": ([vp, i64, vp, vp, vp, i64, vp, vp, i64, vp, vp, i64, vp, vp, dbl, P(vp)], C.c_int),
         "jfem_mesh_update_props": ([vp, vp, vp, vp, vp], C.c_int),
         "jfem_mesh_free": ([vp], C.c_int),

@@ -1,0 +1,312 @@
+"""SOL 105 linear buckling on the device: the differential stiffness of a static
+preload, shifted factorizations on the symbolic analysis of K, and EIGB root
+extraction by subspace inverse iteration.
+
+Reference flow (_solve_sol105, src/JFEMSolver.jl:3156-3600): the static preload
+subcase is solved with the elastic K; assemble_geometric_stiffness
+(src/solver/assembly.jl:6106) builds Kg from the preload displacements; the
+buckling subcase's EIGB request (_resolve_eigb_request, src/solver/helpers.jl:
+56-97) selects the positive roots lambda of K_ff phi = lambda (-Kg_ff) phi in
+(L1, L2], at most NDP of them (3 NEP when NDP is blank), and
+_solve_eigb_buckling (:280-420) returns them ascending with each mode scaled to
+unit largest free component (NORM = MAX).
+
+Here Kg is generated and assembled on the device on K's pattern
+(jfem_assemble_kg), K + sigma Kg is factored on K's symbolic analysis
+(jfem_matrix_shift + jfem_factor_create), and the roots come from subspace
+inverse iteration: X <- (K + sigma Kg)^{-1} (Kg X) with the device factor and
+refined solves, Rayleigh-Ritz on the projected pencil (X'KX, -X'KgX) whose
+O(ndof) products run on the device (jfem_matrix_spmv, jfem_block_gram,
+jfem_block_combine); only the p x p projected eigenproblem is solved on the
+host.  The iteration stops when the wanted Ritz values change by less than
+``tol`` relative and their modes' residuals ||K x + lambda Kg x|| / ||K x|| are
+below ``res_tol``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg as sl
+
+from . import solver
+from ._lib import JfemError, check, ptr
+
+MAX_BLOCK = 32          # columns of the device block kernels
+SOLVE_CHUNK = 16        # right-hand sides per refined solve
+
+
+@dataclass
+class EigbRequest:
+    """An EIGB entry (SID METHOD L1 L2 NEP NDP NDN; NORM = MAX) resolved as
+    _resolve_eigb_request does (src/solver/helpers.jl:56-97)."""
+    sid: int = 1
+    method: str = "INV"
+    lower: float = -math.inf
+    upper: float = math.inf
+    nep: int | None = None
+    ndp: int | None = None
+    ndn: int | None = None
+
+    def __post_init__(self):
+        if self.method not in ("INV", "SINV"):
+            raise ValueError(f"EIGB SID={self.sid} requires METHOD=INV or SINV; received {self.method!r}")
+        for k in ("nep", "ndp", "ndn"):
+            v = getattr(self, k)
+            if v is not None and not (isinstance(v, (int, np.integer)) and v > 0):
+                raise ValueError(f"EIGB SID={self.sid} {k.upper()} must be a positive integer; received {v!r}")
+        if self.nep is not None:
+            if self.ndp is None:
+                self.ndp = 3 * self.nep
+            if self.ndn is None:
+                self.ndn = 3 * self.nep
+        if self.method == "INV" and self.nep is None:
+            raise ValueError(f"EIGB SID={self.sid} METHOD=INV requires NEP")
+        if not self.lower < self.upper:
+            raise ValueError(f"EIGB SID={self.sid} requires finite specified L1 < L2")
+        if self.upper > 0 and self.ndp is None and not math.isfinite(self.upper):
+            raise ValueError(f"EIGB SID={self.sid} requires L2 or a positive-root count NDP")
+
+    @property
+    def positive_window(self):
+        """(lower, upper, quota) of the positive branch, or None when empty."""
+        lo, hi = max(0.0, self.lower), self.upper
+        return (lo, hi, self.ndp) if hi > lo else None
+
+
+class GeometricStiffness(solver.DeviceMatrix):
+    """Kg of the CQUAD4s for a preload u (device CSR values on K's pattern)."""
+
+    def __init__(self, asm: solver.AssembledStiffness, u, ctx: solver.Context | None = None):
+        self.ctx = ctx or asm.K.ctx
+        self.pattern = asm.pattern
+        u = np.ascontiguousarray(np.asarray(u, float).reshape(-1))
+        if u.size != asm.ndof:
+            raise ValueError("u must hold one value per DOF")
+        h = C.c_void_p(); bad = C.c_int64(-1)
+        code = self.ctx.L.jfem_assemble_kg(self.ctx.h, asm.mesh.h, asm.pattern.h, ptr(u), C.byref(h), C.byref(bad))
+        self.bad_element = bad.value
+        check(self.ctx.h, code)
+        self.h = h
+
+
+class ShiftedMatrix(solver.DeviceMatrix):
+    """K + sigma Kg on K's pattern (jfem_matrix_shift); ``shift`` overwrites it."""
+
+    def __init__(self, K: solver.DeviceMatrix, sigma: float, Kg: solver.DeviceMatrix):
+        self.ctx = K.ctx
+        self.pattern = K.pattern
+        self.h = None
+        self.shift(K, sigma, Kg)
+
+    def shift(self, K, sigma, Kg):
+        h = C.c_void_p(self.h.value if self.h is not None else None)
+        check(self.ctx.h, self.ctx.L.jfem_matrix_shift(self.ctx.h, K.h, float(sigma), Kg.h, C.byref(h)))
+        self.h = h
+        self.sigma = float(sigma)
+
+
+def element_kg(mesh: solver.DeviceMesh, u) -> np.ndarray:
+    """Per-CQUAD4 global differential stiffness (n_quad4, 24, 24) for displacements u."""
+    ctx = mesh.ctx
+    nq = mesh.model.quads.shape[0]
+    out = np.zeros((max(nq, 1), 576))
+    u = np.ascontiguousarray(np.asarray(u, float).reshape(-1))
+    bad = C.c_int64(-1)
+    code = ctx.L.jfem_q4_kg_batch(ctx.h, mesh.h, ptr(u), ptr(out), C.byref(bad))
+    if code:
+        check(ctx.h, code)
+    return out[:nq].reshape(nq, 24, 24).transpose(0, 2, 1).copy()
+
+
+class _Blocks:
+    """Device blocks of ndof x p doubles (column-major) and the block products."""
+
+    def __init__(self, ctx: solver.Context, n: int, p: int, fixed: np.ndarray):
+        self.ctx, self.n, self.p = ctx, n, p
+        self.fixed = np.ascontiguousarray(np.asarray(fixed, np.uint8))
+        self.bufs = {k: solver.DeviceBuffer(8 * n * p, ctx) for k in ("X", "W", "Y", "KY", "GY")}
+
+    def col(self, name, c):
+        return C.c_void_p(self.bufs[name].ptr.value + 8 * self.n * c)
+
+    def spmv(self, A, src, dst, ncol):
+        check(self.ctx.h, self.ctx.L.jfem_matrix_spmv(self.ctx.h, A.h, ptr(self.fixed), ncol, self.bufs[src].ptr,
+                                                      self.bufs[dst].ptr))
+
+    def gram(self, a, b, p, q):
+        out = np.zeros(p * q)
+        check(self.ctx.h, self.ctx.L.jfem_block_gram(self.ctx.h, self.n, p, self.bufs[a].ptr, q, self.bufs[b].ptr,
+                                                     ptr(out)))
+        return out.reshape(q, p).T
+
+    def combine(self, src, Q, dst):
+        p, q = Q.shape
+        Qc = np.ascontiguousarray(Q.T).ravel()       # column-major p x q
+        check(self.ctx.h, self.ctx.L.jfem_block_combine(self.ctx.h, self.n, p, self.bufs[src].ptr, q, ptr(Qc),
+                                                        self.bufs[dst].ptr))
+
+    def solve(self, F: solver.Factor, A, src, dst, ncol, tol, max_refine):
+        iters = 0
+        for c0 in range(0, ncol, SOLVE_CHUNK):
+            nr = min(SOLVE_CHUNK, ncol - c0)
+            it = np.zeros(nr, np.int32); rr = np.zeros(nr)
+            code = self.ctx.L.jfem_solve_dev(self.ctx.h, F.h, A.h, nr, self.col(src, c0), self.col(dst, c0), tol,
+                                             max_refine, ptr(it), ptr(rr), None, None)
+            if code not in (0, 8):
+                check(self.ctx.h, code)
+            iters += int(it.sum())
+        return iters
+
+    def download(self, name, ncol):
+        out = np.zeros((ncol, self.n))
+        self.bufs[name].download(out)
+        return out.T.copy()
+
+    def free(self):
+        for b in self.bufs.values():
+            b.free()
+
+
+@dataclass
+class BucklingResult:
+    eigenvalues: np.ndarray                  # ascending positive roots in the window
+    modes: np.ndarray                        # ndof x m, NORM = MAX
+    diagnostics: dict = field(default_factory=dict)
+
+
+def _ritz(Kr, Br):
+    """Projected pencil Br q = mu Kr q (Kr SPD): eigenvalues lambda = 1/mu of the
+    positive mu, ascending, then the rest; Kr-orthonormal vectors."""
+    Kr = 0.5 * (Kr + Kr.T)
+    Br = 0.5 * (Br + Br.T)
+    mu, Q = sl.eigh(Br, Kr)
+    order = np.argsort(-mu)                  # largest mu: smallest positive lambda
+    mu, Q = mu[order], Q[:, order]
+    with np.errstate(divide="ignore"):
+        lam = np.where(mu > 0, 1.0 / mu, np.inf)
+    return lam, mu, Q
+
+
+def buckling_roots(K: solver.DeviceMatrix, Kg: solver.DeviceMatrix, fixed: np.ndarray, factor: solver.Factor,
+                   window: tuple, *, factor_matrix: solver.DeviceMatrix | None = None, block: int | None = None,
+                   tol: float = 1e-12, res_tol: float = 1e-8, max_iter: int = 500, solve_tol: float = 1e-13,
+                   seed: int = 20260525) -> BucklingResult:
+    """Positive roots of K_ff phi = lambda (-Kg_ff) phi in (lower, upper], at most
+    ``quota`` (window = (lower, upper, quota)).  ``factor`` factors
+    ``factor_matrix`` (default K), i.e. K + sigma Kg for a shift sigma below the
+    first root, on the free set ``fixed == 0``; refined solves run against
+    ``factor_matrix``."""
+    t0 = time.perf_counter()
+    ctx = K.ctx
+    A = factor_matrix or K
+    lower, upper, quota = window
+    n = K.pattern.ndof
+    free = np.asarray(fixed) == 0
+    n_free = int(free.sum())
+    want = min(quota if quota is not None else n_free, n_free)
+    p = block or min(MAX_BLOCK, n_free, max(want + 8, 2 * want))
+    if p < 1:
+        return BucklingResult(np.zeros(0), np.zeros((n, 0)), {"free_dofs": n_free})
+    rng = np.random.default_rng(seed)
+    X0 = rng.standard_normal((n, p))
+    X0[~free] = 0.0
+    B = _Blocks(ctx, n, p, fixed)
+    try:
+        B.bufs["X"].upload(np.ascontiguousarray(X0.T))
+        lam_prev = None
+        solves = 0
+        converged = False
+        for it in range(1, max_iter + 1):
+            B.spmv(Kg, "X", "W", p)                              # W = Kg X
+            solves += B.solve(factor, A, "W", "Y", p, solve_tol, 100)   # Y = A^{-1} Kg X
+            B.spmv(K, "Y", "KY", p)
+            B.spmv(Kg, "Y", "GY", p)
+            Kr = B.gram("Y", "KY", p, p)
+            Br = -B.gram("Y", "GY", p, p)
+            lam, mu, Q = _ritz(Kr, Br)
+            B.combine("Y", Q, "X")                               # X'KX = I
+            inwin = np.where((lam > lower) & (lam <= upper))[0]
+            m = min(len(inwin), want)
+            # converged when the wanted values settle and the first Ritz value above
+            # the window (or beyond the quota) has settled as well
+            k_chk = min(m + 1, p - 1) if m < p else m
+            if lam_prev is not None and k_chk > 0:
+                rel = np.abs(lam[:k_chk] - lam_prev[:k_chk]) / np.maximum(np.abs(lam[:k_chk]), 1e-300)
+                finite = np.isfinite(rel)
+                if np.all(rel[finite] <= tol):
+                    converged = True
+                    break
+            lam_prev = lam
+        # residuals of the returned modes (device products of the final block)
+        m = min(int(np.sum((lam > lower) & (lam <= upper))), want)
+        B.spmv(K, "X", "KY", p)
+        B.spmv(Kg, "X", "GY", p)
+        KX = B.download("KY", p)[:, :m]
+        GX = B.download("GY", p)[:, :m]
+        X = B.download("X", p)[:, :m]
+    finally:
+        B.free()
+    vals = lam[:m].copy()
+    res = np.array([np.linalg.norm(KX[:, k] + vals[k] * GX[:, k]) / max(np.linalg.norm(KX[:, k]), 1e-300)
+                    for k in range(m)])
+    # NORM = MAX over the analysis (free) set, largest component positive
+    modes = X.copy()
+    for k in range(m):
+        col = modes[:, k]
+        i = int(np.argmax(np.abs(np.where(free, col, 0.0))))
+        a = col[i]
+        if a != 0.0:
+            modes[:, k] = col / a
+    diag = {"free_dofs": n_free, "block": p, "iterations": it, "converged": bool(converged),
+            "triangular_solves": solves, "mode_residual": res.tolist(),
+            "window": {"lower": lower, "upper": upper, "quota": quota},
+            "in_window_count": int(np.sum((lam > lower) & (lam <= upper))),
+            "ritz_values": [float(v) for v in lam],
+            "complete": bool(np.any(lam[m:] > upper)) or m == want,
+            "time_s": time.perf_counter() - t0}
+    if not converged or np.any(res > res_tol):
+        raise JfemError(8, f"buckling roots did not converge (residuals {res.tolist()})")
+    return BucklingResult(vals, modes, diag)
+
+
+def solve_buckling(model, request: EigbRequest, F_static=None, *, ctx: solver.Context | None = None,
+                   shift: float = 0.0, asm: solver.AssembledStiffness | None = None, **kw):
+    """SOL 105 on the device for one static preload and one EIGB request:
+    assemble K, solve the preload (FP64 factor, refined), assemble Kg on K's
+    pattern, factor K + shift Kg on K's symbolic analysis (the static factor
+    itself when shift == 0) and extract the roots.  Returns
+    (BucklingResult, u_static, diagnostics)."""
+    ctx = ctx or solver.default_context()
+    t0 = time.perf_counter()
+    asm = asm or solver.assemble_stiffness(model, ctx)
+    cache = solver.LinearSolveCache()
+    u, fixed, spc, sdiag = solver.apply_bc_and_solve(asm, model, F_static, precision=64, linear_cache=cache)
+    u = u.reshape(model.ndof, -1)[:, 0]
+    fixed_e, S, Fct = next(iter(cache.entries.values()))[:3]
+    t1 = time.perf_counter()
+    Kg = GeometricStiffness(asm, u, ctx)
+    t2 = time.perf_counter()
+    win = request.positive_window
+    if win is None:
+        raise NotImplementedError("EIGB windows without a positive branch (L2 <= 0) are not on this path")
+    Ash, Fsh = None, Fct
+    try:
+        if shift != 0.0:
+            Ash = ShiftedMatrix(asm.K, shift, Kg)
+            Fsh = solver.Factor(S, Ash, solver.FP64, ctx)
+        res = buckling_roots(asm.K, Kg, fixed_e, Fsh, win, factor_matrix=Ash, **kw)
+    finally:
+        if Fsh is not Fct:
+            Fsh.free()
+        if Ash is not None:
+            Ash.free()
+        Kg.free()
+    t3 = time.perf_counter()
+    diag = {"static": sdiag, "phase_timings": {"static_reference_solve": t1 - t0, "kg_assembly": t2 - t1,
+                                               "buckling_eigensolve": t3 - t2},
+            "eigenvalue_extraction_method": request.method, "shift": shift}
+    return res, u, diag

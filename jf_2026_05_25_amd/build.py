@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libjfem_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["elements.hip", "capi.hip", "symbolic.cpp", "factor.hip", "loads.hip"]
+SOURCES = ["elements.hip", "capi.hip", "symbolic.cpp", "factor.hip", "loads.hip", "buckling.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -355,6 +355,103 @@ int jfem_shell_recover(jfem_ctx *ctx, const jfem_mesh *m, const double *u, int m
     return JFEM_OK;
 }
 
+// ------------------------------------------------------------------ geometric stiffness
+namespace {
+// Kg of every CQUAD4 into Ke (device, nq x 576) for host displacements u
+int run_kg(jfem_ctx *ctx, const jfem_mesh *m, const double *u, double *Ke, int64_t *bad) {
+    if (bad) *bad = -1;
+    if (m->nt || m->ntet) return fail(ctx, JFEM_ERR_BAD_ARG, "geometric stiffness covers CQUAD4 meshes only");
+    hipStream_t s = ctx->stream;
+    double *d_u = nullptr;
+    hipError_t e = upload(&d_u, u, (size_t)(6 * m->n_nodes), s);
+    if (e == hipSuccess) e = launch_reset_status(s, ctx->d_status);
+    if (e == hipSuccess) e = launch_q4_kg(s, m, d_u, Ke, ctx->d_status);
+    int64_t st[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (d_u) (void)hipFree(d_u);
+    if (e != hipSuccess) return fail(ctx, e == hipErrorOutOfMemory ? JFEM_ERR_OOM : JFEM_ERR_HIP, hipGetErrorString(e));
+    if (st[1] != 0) {
+        if (bad) *bad = st[0];
+        return fail(ctx, (int)st[1], "element " + std::to_string(st[0]) + " rejected");
+    }
+    return JFEM_OK;
+}
+}  // namespace
+
+int jfem_q4_kg_batch(jfem_ctx *ctx, const jfem_mesh *m, const double *u, double *kg_out, int64_t *bad) {
+    if (bad) *bad = -1;
+    if (!ctx || !m || !u || (m->nq && !kg_out)) return fail(ctx, JFEM_ERR_BAD_ARG, "null argument");
+    if (m->nq == 0) return m->nt || m->ntet ? fail(ctx, JFEM_ERR_BAD_ARG, "geometric stiffness covers CQUAD4 meshes only")
+                                            : JFEM_OK;
+    double *d_kg = nullptr;
+    if (hipMalloc((void **)&d_kg, (size_t)m->nq * 576 * sizeof(double)) != hipSuccess)
+        return fail(ctx, JFEM_ERR_OOM, "element matrices");
+    int st = run_kg(ctx, m, u, d_kg, bad);
+    if (st == JFEM_OK) {
+        hipError_t e = hipMemcpyAsync(kg_out, d_kg, (size_t)m->nq * 576 * sizeof(double), hipMemcpyDeviceToHost,
+                                      ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = fail(ctx, JFEM_ERR_HIP, hipGetErrorString(e));
+    }
+    (void)hipFree(d_kg);
+    return st;
+}
+
+int jfem_assemble_kg(jfem_ctx *ctx, const jfem_mesh *m, const jfem_pattern *P, const double *u, jfem_matrix **out,
+                     int64_t *bad) {
+    if (bad) *bad = -1;
+    if (!out) return fail(ctx, JFEM_ERR_BAD_ARG, "null output");
+    *out = nullptr;
+    if (!ctx || !m || !P || !u) return fail(ctx, JFEM_ERR_BAD_ARG, "null argument");
+    if (P->nq != m->nq || P->nt != m->nt || P->ntet != m->ntet) return fail(ctx, JFEM_ERR_BAD_ARG, "mesh/pattern mismatch");
+    jfem_matrix *G = new jfem_matrix();
+    G->device = ctx->device;
+    G->P = P;
+    hipError_t e = hipMalloc((void **)&G->d_val, std::max<int64_t>(P->nnz, 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&G->d_Ke, std::max<int64_t>(P->ntrip, 1) * sizeof(double));
+    if (e != hipSuccess) {
+        jfem_matrix_free(G);
+        return fail(ctx, JFEM_ERR_OOM, "geometric stiffness buffers");
+    }
+    int st = run_kg(ctx, m, u, G->d_Ke, bad);
+    if (st == JFEM_OK) {
+        hipError_t e2 = launch_assemble(ctx->stream, P, G->d_Ke, G->d_val, 1, 0, 0);
+        if (e2 == hipSuccess) e2 = hipStreamSynchronize(ctx->stream);
+        if (e2 != hipSuccess) st = fail(ctx, JFEM_ERR_HIP, hipGetErrorString(e2));
+    }
+    if (st != JFEM_OK) {
+        jfem_matrix_free(G);
+        return st;
+    }
+    *out = G;
+    return JFEM_OK;
+}
+
+int jfem_matrix_shift(jfem_ctx *ctx, const jfem_matrix *K, double sigma, const jfem_matrix *G, jfem_matrix **out) {
+    if (!ctx || !K || !G || !out) return fail(ctx, JFEM_ERR_BAD_ARG, "null argument");
+    if (K->P != G->P || (*out && (*out)->P != K->P)) return fail(ctx, JFEM_ERR_BAD_ARG, "matrices of different patterns");
+    if (!std::isfinite(sigma)) return fail(ctx, JFEM_ERR_BAD_ARG, "shift is not finite");
+    jfem_matrix *A = *out;
+    if (!A) {
+        A = new jfem_matrix();
+        A->device = ctx->device;
+        A->P = K->P;
+        if (hipMalloc((void **)&A->d_val, std::max<int64_t>(K->P->nnz, 1) * sizeof(double)) != hipSuccess) {
+            jfem_matrix_free(A);
+            return fail(ctx, JFEM_ERR_OOM, "shifted matrix");
+        }
+    }
+    hipError_t e = launch_csr_shift(ctx->stream, K->P->nnz, K->d_val, sigma, G->d_val, A->d_val);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        if (!*out) jfem_matrix_free(A);
+        return fail(ctx, JFEM_ERR_HIP, hipGetErrorString(e));
+    }
+    *out = A;
+    return JFEM_OK;
+}
+
 // ------------------------------------------------------------------------- pattern
 int jfem_pattern_create(jfem_ctx *ctx, const jfem_mesh *m, jfem_pattern **out, int64_t *nnz_out) {
     *out = nullptr;

@@ -1870,6 +1870,246 @@ shell_recover_kernel(int64_t nq, int64_t nt, const double *__restrict__ x, const
 }
 
 
+// ------------------------------------------------------------------------------
+// SOL 105 differential stiffness of a flat isotropic CQUAD4 under the membrane
+// state of the preload displacements: geometric_stiffness_quad4_reference_kdjj_iso
+// (src/FEMKernels.jl:8810-8942) with the assembly's flatness gate, preload map,
+// warp congruence and element-frame transform (src/solver/assembly.jl:6575-6594,
+// 8063-8075, 8264-8320).  One thread per element.
+// ------------------------------------------------------------------------------
+
+// bilinear gradients at (r, s) in the diagonal-bisector frame (ce, se) and |detJ|;
+// false when |detJ| < 1e-14 (grad_p, src/FEMKernels.jl:8857-8876)
+__device__ bool kg_grad(const double *X, const double *Y, double ce, double se, double r, double s, double *dNx,
+                        double *dNy, double &adet) {
+    const double dNr[4] = {-(1.0 - s) * 0.25, (1.0 - s) * 0.25, (1.0 + s) * 0.25, -(1.0 + s) * 0.25};
+    const double dNs[4] = {-(1.0 - r) * 0.25, -(1.0 + r) * 0.25, (1.0 + r) * 0.25, (1.0 - r) * 0.25};
+    const double J11 = dNr[0] * X[0] + dNr[1] * X[1] + dNr[2] * X[2] + dNr[3] * X[3];
+    const double J12 = dNr[0] * Y[0] + dNr[1] * Y[1] + dNr[2] * Y[2] + dNr[3] * Y[3];
+    const double J21 = dNs[0] * X[0] + dNs[1] * X[1] + dNs[2] * X[2] + dNs[3] * X[3];
+    const double J22 = dNs[0] * Y[0] + dNs[1] * Y[1] + dNs[2] * Y[2] + dNs[3] * Y[3];
+    const double detJ = J11 * J22 - J12 * J21;
+    if (fabs(detJ) < 1e-14) return false;
+    const double i11 = J22 / detJ, i12 = -J12 / detJ, i21 = -J21 / detJ, i22 = J11 / detJ;
+    for (int k = 0; k < 4; ++k) {
+        const double gx = i11 * dNr[k] + i12 * dNs[k], gy = i21 * dNr[k] + i22 * dNs[k];
+        dNx[k] = ce * gx + se * gy;
+        dNy[k] = -se * gx + ce * gy;
+    }
+    adet = fabs(detJ);
+    return true;
+}
+
+// element-frame operator on local coordinates lc and element-frame displacements
+// ue; Kg (24 x 24 column-major) is overwritten.  Only u, v, w rows are written.
+__device__ void q4_kg_local(const double *lc, const double *ue, double E, double nu, double h, bool assumed,
+                            double *Kg) {
+    for (int q = 0; q < 576; ++q) Kg[q] = 0.0;
+    if (h < 1e-30) return;
+    const double X[4] = {lc[0], lc[2], lc[4], lc[6]}, Y[4] = {lc[1], lc[3], lc[5], lc[7]};
+    // an exact rectangle keeps the centre-gradient transverse form
+    const double e12x = X[1] - X[0], e12y = Y[1] - Y[0], e14x = X[3] - X[0], e14y = Y[3] - Y[0];
+    const bool exact_rectangle = (e12x != 0.0 || e12y != 0.0) && (e14x != 0.0 || e14y != 0.0) &&
+                                 X[2] - X[3] == e12x && Y[2] - Y[3] == e12y && X[2] - X[1] == e14x &&
+                                 Y[2] - Y[1] == e14y && e12x * e14x + e12y * e14y == 0.0;
+    assumed = assumed && !exact_rectangle;
+    const double d13x = X[2] - X[0], d13y = Y[2] - Y[0], l13 = hypot(d13x, d13y);
+    const double d42x = X[1] - X[3], d42y = Y[1] - Y[3], l42 = hypot(d42x, d42y);
+    if (l13 < 1e-12 || l42 < 1e-12) return;
+    const double bx = d13x / l13 + d42x / l42, by = d13y / l13 + d42y / l42, lb = hypot(bx, by);
+    if (lb < 1e-12) return;
+    const double ce = bx / lb, se = by / lb;
+    double up[4], vp[4];
+    for (int k = 0; k < 4; ++k) {
+        up[k] = ce * ue[6 * k] + se * ue[6 * k + 1];
+        vp[k] = -se * ue[6 * k] + ce * ue[6 * k + 1];
+    }
+    const double d11 = E / (1.0 - nu * nu), d12 = nu * d11, d33 = E / (2.0 * (1.0 + nu));
+    double dNx0[4], dNy0[4], a0;
+    if (!kg_grad(X, Y, ce, se, 0.0, 0.0, dNx0, dNy0, a0)) return;
+    double gam0 = 0.0;                      // centre-sampled membrane shear
+    for (int k = 0; k < 4; ++k) gam0 += dNy0[k] * up[k] + dNx0[k] * vp[k];
+    const double gp = 1.0 / sqrt(3.0);
+    // tying gradients: w,x at (0, -+gp), w,y at (-+gp, 0)
+    double tx[4][4], ty[4][4], ta;
+    if (assumed) {
+        const double tr[4] = {0.0, 0.0, -gp, gp}, ts[4] = {-gp, gp, 0.0, 0.0};
+        for (int t = 0; t < 4; ++t)
+            if (!kg_grad(X, Y, ce, se, tr[t], ts[t], tx[t], ty[t], ta)) return;
+    }
+    const double gr[4] = {-gp, gp, gp, -gp}, gs[4] = {-gp, -gp, gp, gp};
+    for (int q = 0; q < 4; ++q) {
+        double dNx[4], dNy[4], adet;
+        if (!kg_grad(X, Y, ce, se, gr[q], gs[q], dNx, dNy, adet)) continue;
+        const double w = h * adet;
+        double epx = 0.0, epy = 0.0;
+        for (int k = 0; k < 4; ++k) { epx += dNx[k] * up[k]; epy += dNy[k] * vp[k]; }
+        const double sxx = d11 * epx + d12 * epy, syy = d12 * epx + d11 * epy, sxy = d33 * gam0;
+        const double *dWx = assumed ? tx[q <= 1 ? 0 : 1] : dNx;
+        const double *dWy = assumed ? ty[(q == 0 || q == 3) ? 2 : 3] : dNy;
+        for (int j = 0; j < 4; ++j)
+            for (int i = 0; i < 4; ++i) {
+                const int r0 = 6 * i, c0 = 6 * j;
+                const double sym_xy = 0.5 * (dNx[i] * dNy[j] + dNy[i] * dNx[j]);
+                const double kuu = w * (syy * dNy[i] * dNy[j] + sxy * sym_xy);
+                const double kvv = w * (sxx * dNx[i] * dNx[j] + sxy * sym_xy);
+                const double kuv = -w * sxy * 0.5 * (dNx[i] * dNx[j] + dNy[i] * dNy[j]);
+                Kg[r0 + 24 * c0] += ce * ce * kuu - 2.0 * ce * se * kuv + se * se * kvv;
+                Kg[r0 + 24 * (c0 + 1)] += ce * se * kuu + (ce * ce - se * se) * kuv - se * ce * kvv;
+                Kg[(r0 + 1) + 24 * c0] += se * ce * kuu + (ce * ce - se * se) * kuv - ce * se * kvv;
+                Kg[(r0 + 1) + 24 * (c0 + 1)] += se * se * kuu + 2.0 * se * ce * kuv + ce * ce * kvv;
+                Kg[(r0 + 2) + 24 * (c0 + 2)] +=
+                    assumed ? w * (sxx * dWx[i] * dWx[j] + syy * dWy[i] * dWy[j] + sxy * (dWx[i] * dWy[j] + dWy[i] * dWx[j]))
+                            : w * (sxx * dNx[i] * dNx[j] + syy * dNy[i] * dNy[j] +
+                                   sxy * (dNx0[i] * dNy0[j] + dNy0[i] * dNx0[j]));
+            }
+    }
+}
+
+// the finite-warp displacement map Tg (24 x 24 column-major; identity for a flat
+// cell), _quad4_finite_warp_fill! (src/FEMKernels.jl:880-945)
+__device__ void kg_warp_map(const double *lc, const double *zl, double *Tg) {
+    for (int q = 0; q < 576; ++q) Tg[q] = (q % 25 == 0) ? 1.0 : 0.0;
+    const double cr[4] = {-1, 1, 1, -1}, cs_[4] = {-1, -1, 1, 1};
+    for (int i = 0; i < 4; ++i) {
+        double dNr[4], dNs[4], J11, J12, J21, J22;
+        shape_derivs(cr[i], cs_[i], dNr, dNs);
+        jac4(dNr, dNs, lc, J11, J12, J21, J22);
+        const double dj = J11 * J22 - J12 * J21;
+        if (fabs(dj) < 1e-30) continue;
+        const double i11 = J22 / dj, i12 = -J12 / dj, i21 = -J21 / dj, i22 = J11 / dj;
+        const double zr = dNr[0] * zl[0] + dNr[1] * zl[1] + dNr[2] * zl[2] + dNr[3] * zl[3];
+        const double zs = dNs[0] * zl[0] + dNs[1] * zl[1] + dNs[2] * zl[2] + dNs[3] * zl[3];
+        const double gx = i11 * zr + i12 * zs, gy = i21 * zr + i22 * zs;
+        const int rxi = 6 * i + 3, ryi = 6 * i + 4;
+        Tg[rxi + 24 * (6 * i + 5)] += gx;
+        Tg[ryi + 24 * (6 * i + 5)] += gy;
+        const int a = (i % 2 == 0) ? 0 : 1, b = (i % 2 == 0) ? 2 : 3;
+        const double dx = lc[2 * b] - lc[2 * a], dy = lc[2 * b + 1] - lc[2 * a + 1];
+        const double dl2 = dx * dx + dy * dy;
+        if (dl2 > 1e-30) {
+            const int js[2] = {a, b};
+            const double cu[2] = {dy / dl2, -dy / dl2}, cv[2] = {-dx / dl2, dx / dl2};
+            for (int t = 0; t < 2; ++t) {
+                const int j = js[t];
+                Tg[rxi + 24 * (6 * j + 0)] -= gx * cu[t];
+                Tg[rxi + 24 * (6 * j + 1)] -= gx * cv[t];
+                Tg[ryi + 24 * (6 * j + 0)] -= gy * cu[t];
+                Tg[ryi + 24 * (6 * j + 1)] -= gy * cv[t];
+            }
+        }
+        for (int j = 0; j < 4; ++j) {
+            const double dNdx = i11 * dNr[j] + i12 * dNs[j], dNdy = i21 * dNr[j] + i22 * dNs[j];
+            Tg[(6 * i + 0) + 24 * (6 * j + 2)] += zl[i] * dNdx;
+            Tg[(6 * i + 1) + 24 * (6 * j + 2)] += zl[i] * dNdy;
+        }
+    }
+}
+
+// global 24 x 24 Kg of one CQUAD4 (column-major into out); returns a status code
+__device__ int q4_kg(const double *p, const double *pr, const double *ug, double *out) {
+    {   // flat-element gate: corner deviation below 1e-2 of the longer diagonal
+        const double d13[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
+        const double d24[3] = {p[9] - p[3], p[10] - p[4], p[11] - p[5]};
+        double n[3];
+        cross3(d13, d24, n);
+        const double nl = sqrt(dot3(n, n));
+        if (nl > 1e-12) {
+            const double v3g[3] = {n[0] / nl, n[1] / nl, n[2] / nl};
+            double cg[3], dev = 0.0;
+            for (int k = 0; k < 3; ++k) cg[k] = (((p[k] + p[3 + k]) + p[6 + k]) + p[9 + k]) / 4.0;
+            for (int k = 0; k < 4; ++k) {
+                const double d[3] = {p[3 * k] - cg[0], p[3 * k + 1] - cg[1], p[3 * k + 2] - cg[2]};
+                dev = fmax(dev, fabs(dot3(d, v3g)));
+            }
+            const double ld = fmax(sqrt(dot3(d13, d13)), sqrt(dot3(d24, d24)));
+            if (!(dev < 1e-2 * fmax(ld, 1e-12))) return JFEM_ERR_WARPED;
+        }
+    }
+    Q4Prep P;
+    q4_setup(P, p, pr, 0.0);
+    double ul[24], uk[24], Tg[576], Kl[576];
+    for (int k = 0; k < 4; ++k)
+        for (int hf = 0; hf < 2; ++hf)
+            for (int i = 0; i < 3; ++i) {
+                const double *s = ug + 6 * k + 3 * hf;
+                ul[6 * k + 3 * hf + i] = P.R[3 * i] * s[0] + P.R[3 * i + 1] * s[1] + P.R[3 * i + 2] * s[2];
+            }
+    kg_warp_map(P.lc, P.zl, Tg);            // preload map (identity on a flat cell)
+    for (int i = 0; i < 24; ++i) {
+        double s = 0.0;
+        for (int m = 0; m < 24; ++m) s += Tg[i + 24 * m] * ul[m];
+        uk[i] = s;
+    }
+    q4_kg_local(P.lc, uk, pr[JFEM_SP_E], pr[JFEM_SP_NU], pr[JFEM_SP_H], true, Kl);
+    if (P.warped) {                         // sym(W' Kg W); Tg reused for W' (Kg W)
+        double W[576];
+        for (int j = 0; j < 24; ++j)
+            for (int i = 0; i < 24; ++i) {
+                double v = 0.0;
+                for (int m = 0; m < 24; ++m) v += Kl[i + 24 * m] * Tg[m + 24 * j];
+                W[i + 24 * j] = v;
+            }
+        for (int j = 0; j < 24; ++j)
+            for (int i = 0; i < 24; ++i) {
+                double v = 0.0;
+                for (int l = 0; l < 24; ++l) v += Tg[l + 24 * i] * W[l + 24 * j];
+                Kl[i + 24 * j] = v;
+            }
+        for (int j = 0; j < 24; ++j)
+            for (int i = 0; i < j; ++i) {
+                const double s = 0.5 * (Kl[i + 24 * j] + Kl[j + 24 * i]);
+                Kl[i + 24 * j] = s;
+                Kl[j + 24 * i] = s;
+            }
+    }
+    // R' * blk * R per 3 x 3 block, R rows = (v1, v2, v3)
+    const double *R = P.R;
+    for (int B = 0; B < 8; ++B)
+        for (int A = 0; A < 8; ++A) {
+            double t[9];
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    double s = 0;
+                    for (int l = 0; l < 3; ++l) s += Kl[(3 * A + i) + 24 * (3 * B + l)] * R[l * 3 + j];
+                    t[i + 3 * j] = s;
+                }
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    double s = 0;
+                    for (int l = 0; l < 3; ++l) s += R[l * 3 + i] * t[l + 3 * j];
+                    out[(3 * A + i) + 24 * (3 * B + j)] = s;
+                }
+        }
+    return 0;
+}
+
+// c = a + sigma b over the values of matrices sharing one pattern (K + sigma Kg)
+__global__ void csr_shift_kernel(int64_t n, const double *__restrict__ a, double sigma, const double *__restrict__ b,
+                                 double *__restrict__ c) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = a[i] + sigma * b[i];
+}
+
+// one thread per CQUAD4: its global Kg at Ke + 576 e (the layout the assembly sums)
+__global__ void __launch_bounds__(64)
+q4_kg_kernel(int64_t nq, const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+             const int32_t *__restrict__ qconn, const double *__restrict__ qprop, const double *__restrict__ u,
+             double *__restrict__ Ke, int64_t *status) {
+    const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (e >= nq) return;
+    double p[12], pr[8], ug[24];
+    for (int k = 0; k < 4; ++k) {
+        const int64_t nd = qconn[k * nq + e];
+        p[3 * k] = x[nd]; p[3 * k + 1] = y[nd]; p[3 * k + 2] = z[nd];
+        for (int c = 0; c < 6; ++c) ug[6 * k + c] = u[6 * nd + c];
+    }
+    for (int c = 0; c < 8; ++c) pr[c] = qprop[c * nq + e];
+    const int st = q4_kg(p, pr, ug, Ke + 576 * e);
+    if (st != 0) report(status, e, st);
+}
+
+
 namespace jfem {
 
 hipError_t launch_reset_status(hipStream_t s, int64_t *status) {
@@ -1924,6 +2164,20 @@ hipError_t launch_shell_recover(hipStream_t s, const jfem_mesh *m, const double 
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(shell_recover_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, m->nq, m->nt, m->d_x,
                        m->d_y, m->d_z, m->d_q, m->d_qp, m->d_t, m->d_tp, u, mean_shear, qout, tout, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_csr_shift(hipStream_t s, int64_t n, const double *a, double sigma, const double *b, double *c) {
+    if (n == 0) return hipSuccess;
+    const int bs = 256;
+    hipLaunchKernelGGL(csr_shift_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, n, a, sigma, b, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_q4_kg(hipStream_t s, const jfem_mesh *m, const double *u, double *Ke, int64_t *status) {
+    if (m->nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(q4_kg_kernel, dim3((unsigned)((m->nq + 63) / 64)), dim3(64), 0, s, m->nq, m->d_x, m->d_y,
+                       m->d_z, m->d_q, m->d_qp, u, Ke, status);
     return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // Symbolic analysis for the supernodal multifrontal Cholesky of K_ff.
 #pragma once
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace jfem {

@@ -26,6 +26,12 @@ assembly, BC partition, factor, refined solve, device load vectors), and
 ``ju_results`` / ``write_ju`` give the JU.JSON layout of the reference's
 recorded results (``analysis_type``, ``displacements`` per subcase in grid
 order, ``solver_diagnostics`` with the free-DOF count and the residual).
+
+SOL 105 decks: the case control's STATSUB and METHOD select, per buckling
+subcase, its static preload subcase and its EIGB entry (SID METHOD L1 L2 NEP
+NDP NDN, continuation NORM; extract_eigb, src/parsing/extract_properties.jl:
+851-905); ``solve_buckling_deck`` runs each through the device path
+(buckling.solve_buckling).  EIGRL requests are not on this path.
 """
 from __future__ import annotations
 
@@ -159,6 +165,8 @@ class Subcase:
     spc: int = 0
     load: int = 0
     label: str = ""
+    statsub: int = 0                           # SOL 105: the static preload subcase
+    method: int = 0                            # SOL 105: the EIGB / EIGRL set
 
 
 @dataclass
@@ -180,6 +188,7 @@ class Deck:
     unsupported: set = field(default_factory=set)
     sol: int = 101                             # executive SOL number (src/parsing/card_processor.jl:227-233)
     grav: dict = field(default_factory=dict)   # sid -> [(A, N3)] (CID 0)
+    eigb: dict = field(default_factory=dict)   # sid -> EIGB fields (METHOD, L1, L2, NEP, NDP, NDN, NORM)
 
     def recovery_mean_shear(self) -> bool:
         """Shell recovery branch of this deck's solution sequence: the MacNeal
@@ -361,7 +370,7 @@ def read_deck(source: str) -> Deck:
             cur = Subcase(int(m.group(1)), glob.spc, glob.load)
             subs.append(cur)
             continue
-        m = re.match(r"(SPC|LOAD)\s*=\s*(\d+)", u)
+        m = re.match(r"(SPC|LOAD|STATSUB|METHOD)\s*=\s*(\d+)", u)
         if m:
             setattr(cur, m.group(1).lower(), int(m.group(2)))
             continue
@@ -373,7 +382,7 @@ def read_deck(source: str) -> Deck:
 
     grids, ps = {}, {}
     quads, trias, pshell, mat1 = [], [], {}, {}
-    spc1, force, moment, pload4, other, grav = {}, {}, {}, {}, {}, {}
+    spc1, force, moment, pload4, other, grav, eigb = {}, {}, {}, {}, {}, {}, {}
     k6rot = 100.0
     unsupported = set()
     for c in bulk_cards(bulk):
@@ -464,6 +473,12 @@ def read_deck(source: str) -> Deck:
                 if cid:
                     raise NotImplementedError(f"GRAV SID={sid}: CID {cid} is not on this path")
                 grav.setdefault(sid, []).append((num(3, 0.0), tuple(num(4 + k, 0.0) for k in range(3))))
+        elif name == "EIGB":                       # extract_eigb, src/parsing/extract_properties.jl:851-905
+            sid = to_id(num(1))
+            if sid > 0:
+                cnt = lambda k: None if not str(g(k)).strip() else num(k)
+                eigb[sid] = {"method": str(g(2)).strip().upper(), "l1": num(3), "l2": num(4), "nep": cnt(5),
+                             "ndp": cnt(6), "ndn": cnt(7), "norm": str(g(9)).strip().upper() or "MAX"}
         elif name in LOAD_CARDS:
             other.setdefault(to_id(num(1)), set()).add(name)
         elif name not in NEUTRAL_CARDS:
@@ -471,7 +486,7 @@ def read_deck(source: str) -> Deck:
     ids = sorted(grids)                                    # GRID rank order (assembly.jl:2653-2657)
     xyz = np.array([grids[i] for i in ids], float).reshape(-1, 3)
     return Deck(ids, xyz, ps, quads, trias, pshell, mat1, spc1, force, moment, pload4, other, subs, k6rot,
-                unsupported, sol, grav)
+                unsupported, sol, grav, eigb)
 
 
 def solve_deck(deck: Deck, precision: int = 64, tol: float = 1e-10, membrane_modes: int = 15, ctx=None,
@@ -493,6 +508,46 @@ def solve_deck(deck: Deck, precision: int = 64, tol: float = 1e-10, membrane_mod
         ls.free()
         u, fixed, spc, diag = solver.apply_bc_and_solve(asm, m, F_applied=F[:, None], precision=precision, tol=tol)
         out[sc.id] = (np.asarray(u).reshape(m.ndof, -1)[:, 0], diag)
+    return out
+
+
+def eigb_request(deck: Deck, method: int):
+    """The EIGB request a buckling subcase's METHOD selects
+    (_resolve_sol105_eigen_request, src/JFEMSolver.jl:3100-3150)."""
+    from .buckling import EigbRequest
+    e = deck.eigb.get(method)
+    if e is None:
+        raise NotImplementedError(f"METHOD={method} does not reference an EIGB entry (EIGRL is not on this path)")
+    if e["norm"] != "MAX":
+        raise NotImplementedError(f"EIGB SID={method}: NORM={e['norm']} is not on this path")
+    cnt = lambda v: None if v is None else int(v)
+    return EigbRequest(method, e["method"], -np.inf if e["l1"] is None else float(e["l1"]),
+                       np.inf if e["l2"] is None else float(e["l2"]), cnt(e["nep"]), cnt(e["ndp"]), cnt(e["ndn"]))
+
+
+def solve_buckling_deck(deck: Deck, ctx=None, membrane_modes: int = 15, **kw):
+    """Every SOL 105 buckling subcase (one with STATSUB and METHOD) through the
+    device path: its static preload subcase solved, Kg assembled, the EIGB roots
+    extracted (buckling.solve_buckling).  Returns {buckling subcase id:
+    (BucklingResult, u_static, diagnostics)}."""
+    from . import buckling, solver
+    ctx = ctx or solver.default_context()
+    subs = {sc.id: sc for sc in deck.subcases}
+    out = {}
+    for sc in deck.subcases:
+        if not sc.method:
+            continue
+        if sc.statsub not in subs:
+            raise NotImplementedError(f"buckling subcase {sc.id}: STATSUB={sc.statsub} is not a subcase of the deck")
+        st = subs[sc.statsub]
+        if sc.spc != st.spc:
+            raise NotImplementedError(f"buckling subcase {sc.id}: an SPC set other than its preload's")
+        m = deck.model(st.id, membrane_modes)
+        asm = solver.assemble_stiffness(m, ctx)
+        ls = solver.LoadSet(asm.mesh, ctx=ctx, **deck.load_cards(st.id))
+        F = ls.vector()
+        ls.free()
+        out[sc.id] = buckling.solve_buckling(m, eigb_request(deck, sc.method), F[:, None], ctx=ctx, asm=asm, **kw)
     return out
 
 

@@ -1254,6 +1254,149 @@ int or_t3_recover(const double p[9], const double prop[8], const double ug[18], 
 }
 
 /* ------------------------------------------------------------------------- */
+/* CQUAD4 geometric (differential) stiffness for SOL 105: the default operator */
+/* of a flat isotropic CQUAD4, geometric_stiffness_quad4_reference_kdjj_iso    */
+/* (src/FEMKernels.jl:8810-8942), as assembled by assemble_geometric_stiffness */
+/* (src/solver/assembly.jl:6575-6594 flatness, 6985-6998 element displacements,*/
+/* 8063-8075 preload map, 8264-8320 drilling, warp congruence and transform).  */
+/* ------------------------------------------------------------------------- */
+
+/* grad_p of the kernel: bilinear gradients at (r, s) in the diagonal-bisector
+ * frame (ce, se) and |detJ|; 0 when |detJ| < 1e-14 (src/FEMKernels.jl:8857-8876) */
+static int kg_grad(const double X[4], const double Y[4], double ce, double se, double r, double s,
+                   double dNx[4], double dNy[4], double *adet) {
+    const double dNr[4] = {-(1.0 - s) * 0.25, (1.0 - s) * 0.25, (1.0 + s) * 0.25, -(1.0 + s) * 0.25};
+    const double dNs[4] = {-(1.0 - r) * 0.25, -(1.0 + r) * 0.25, (1.0 + r) * 0.25, (1.0 - r) * 0.25};
+    double J11 = dNr[0] * X[0] + dNr[1] * X[1] + dNr[2] * X[2] + dNr[3] * X[3];
+    double J12 = dNr[0] * Y[0] + dNr[1] * Y[1] + dNr[2] * Y[2] + dNr[3] * Y[3];
+    double J21 = dNs[0] * X[0] + dNs[1] * X[1] + dNs[2] * X[2] + dNs[3] * X[3];
+    double J22 = dNs[0] * Y[0] + dNs[1] * Y[1] + dNs[2] * Y[2] + dNs[3] * Y[3];
+    double detJ = J11 * J22 - J12 * J21;
+    if (fabs(detJ) < 1e-14) return 0;
+    double i11 = J22 / detJ, i12 = -J12 / detJ, i21 = -J21 / detJ, i22 = J11 / detJ;
+    for (int k = 0; k < 4; ++k) {
+        double gx = i11 * dNr[k] + i12 * dNs[k];
+        double gy = i21 * dNr[k] + i22 * dNs[k];
+        dNx[k] = ce * gx + se * gy;
+        dNy[k] = -se * gx + ce * gy;
+    }
+    *adet = fabs(detJ);
+    return 1;
+}
+
+int or_q4_kg_local(const double lc[8], const double ue[24], double E, double nu, double h, int assumed_transverse,
+                   double Kg[576]) {
+    for (int q = 0; q < 576; ++q) Kg[q] = 0.0;
+    if (h < 1e-30) return OR_OK;
+    const double X[4] = {lc[0], lc[2], lc[4], lc[6]}, Y[4] = {lc[1], lc[3], lc[5], lc[7]};
+    /* an exact rectangle keeps the centre-gradient transverse form (:8824-8837) */
+    double e12x = X[1] - X[0], e12y = Y[1] - Y[0], e14x = X[3] - X[0], e14y = Y[3] - Y[0];
+    int exact_rectangle = (e12x != 0.0 || e12y != 0.0) && (e14x != 0.0 || e14y != 0.0) &&
+                          X[2] - X[3] == e12x && Y[2] - Y[3] == e12y && X[2] - X[1] == e14x &&
+                          Y[2] - Y[1] == e14y && e12x * e14x + e12y * e14y == 0.0;
+    assumed_transverse = assumed_transverse && !exact_rectangle;
+    /* element frame: bisector of the diagonals G1->G3 and G4->G2 (:8838-8846) */
+    double d13x = X[2] - X[0], d13y = Y[2] - Y[0], l13 = hypot(d13x, d13y);
+    double d42x = X[1] - X[3], d42y = Y[1] - Y[3], l42 = hypot(d42x, d42y);
+    if (l13 < 1e-12 || l42 < 1e-12) return OR_OK;
+    double bx = d13x / l13 + d42x / l42, by = d13y / l13 + d42y / l42, lb = hypot(bx, by);
+    if (lb < 1e-12) return OR_OK;
+    double ce = bx / lb, se = by / lb;
+    double up[4], vp[4];
+    for (int k = 0; k < 4; ++k) {
+        double ux = ue[6 * k], uy = ue[6 * k + 1];
+        up[k] = ce * ux + se * uy;
+        vp[k] = -se * ux + ce * uy;
+    }
+    double d11 = E / (1.0 - nu * nu), d12 = nu * d11, d33 = E / (2.0 * (1.0 + nu));
+    double dNx0[4], dNy0[4], a0;
+    if (!kg_grad(X, Y, ce, se, 0.0, 0.0, dNx0, dNy0, &a0)) return OR_OK;
+    /* centre-sampled membrane shear (:8881-8885) */
+    double gam0 = 0.0;
+    for (int k = 0; k < 4; ++k) gam0 += dNy0[k] * up[k] + dNx0[k] * vp[k];
+    const double gp = 1.0 / sqrt(3.0);
+    /* tying gradients: w,x at (0, -+gp), w,y at (-+gp, 0) (:8887-8899) */
+    double tx[4][4], ty[4][4], ta;
+    if (assumed_transverse) {
+        const double tr[4] = {0.0, 0.0, -gp, gp}, ts[4] = {-gp, gp, 0.0, 0.0};
+        for (int t = 0; t < 4; ++t)
+            if (!kg_grad(X, Y, ce, se, tr[t], ts[t], tx[t], ty[t], &ta)) return OR_OK;
+    }
+    const double gr[4] = {-gp, gp, gp, -gp}, gs[4] = {-gp, -gp, gp, gp};
+    for (int q = 0; q < 4; ++q) {
+        double dNx[4], dNy[4], adet;
+        if (!kg_grad(X, Y, ce, se, gr[q], gs[q], dNx, dNy, &adet)) continue;
+        double w = h * adet, epx = 0.0, epy = 0.0;
+        for (int k = 0; k < 4; ++k) { epx += dNx[k] * up[k]; epy += dNy[k] * vp[k]; }
+        double sxx = d11 * epx + d12 * epy, syy = d12 * epx + d11 * epy, sxy = d33 * gam0;
+        const double *dWx = assumed_transverse ? tx[q <= 1 ? 0 : 1] : dNx;
+        const double *dWy = assumed_transverse ? ty[(q == 0 || q == 3) ? 2 : 3] : dNy;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) {
+                int r0 = 6 * i, c0 = 6 * j;
+                /* in-plane component string rule in the primed frame (:8912-8920) */
+                double sym_xy = 0.5 * (dNx[i] * dNy[j] + dNy[i] * dNx[j]);
+                double kuu = w * (syy * dNy[i] * dNy[j] + sxy * sym_xy);
+                double kvv = w * (sxx * dNx[i] * dNx[j] + sxy * sym_xy);
+                double kuv = -w * sxy * 0.5 * (dNx[i] * dNx[j] + dNy[i] * dNy[j]);
+                M(Kg, r0, c0, 24) += ce * ce * kuu - 2.0 * ce * se * kuv + se * se * kvv;
+                M(Kg, r0, c0 + 1, 24) += ce * se * kuu + (ce * ce - se * se) * kuv - se * ce * kvv;
+                M(Kg, r0 + 1, c0, 24) += se * ce * kuu + (ce * ce - se * se) * kuv - ce * se * kvv;
+                M(Kg, r0 + 1, c0 + 1, 24) += se * se * kuu + 2.0 * se * ce * kuv + ce * ce * kvv;
+                if (assumed_transverse)
+                    M(Kg, r0 + 2, c0 + 2, 24) += w * (sxx * dWx[i] * dWx[j] + syy * dWy[i] * dWy[j] +
+                                                      sxy * (dWx[i] * dWy[j] + dWy[i] * dWx[j]));
+                else
+                    M(Kg, r0 + 2, c0 + 2, 24) += w * (sxx * dNx[i] * dNx[j] + syy * dNy[i] * dNy[j] +
+                                                      sxy * (dNx0[i] * dNy0[j] + dNy0[i] * dNx0[j]));
+            }
+    }
+    return OR_OK;
+}
+
+int or_q4_kg(const double p[12], const double prop[8], const double ug[24], double Kg[576]) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6, *p4 = p + 9;
+    /* geometric flatness gate of the flat-element operator (assembly.jl:6575-6594) */
+    {
+        double d13[3], d24[3], n[3];
+        v3sub(p3, p1, d13); v3sub(p4, p2, d24); v3cross(d13, d24, n);
+        double nl = sqrt(v3dot(n, n));
+        if (nl > 1e-12) {
+            double cg[3], v3g[3] = {n[0] / nl, n[1] / nl, n[2] / nl}, dev = 0.0;
+            for (int k = 0; k < 3; ++k) cg[k] = (((p1[k] + p2[k]) + p3[k]) + p4[k]) / 4.0;
+            for (int k = 0; k < 4; ++k) { double d[3]; v3sub(p + 3 * k, cg, d); dev = fmax(dev, fabs(v3dot(d, v3g))); }
+            double ld = fmax(sqrt(v3dot(d13, d13)), sqrt(v3dot(d24, d24)));
+            if (!(dev < 1e-2 * fmax(ld, 1e-12))) return OR_ERR_WARPED;
+        }
+    }
+    double v1[3], v2[3], v3[3];
+    frame_quad4(p1, p2, p3, p4, v1, v2, v3);
+    double c[3], lc[8];
+    for (int k = 0; k < 3; ++k) c[k] = (((p1[k] + p2[k]) + p3[k]) + p4[k]) / 4.0;
+    for (int k = 0; k < 4; ++k) {
+        double d[3]; v3sub(p + 3 * k, c, d);
+        lc[2 * k] = v3dot(d, v1); lc[2 * k + 1] = v3dot(d, v2);
+    }
+    /* element-frame displacements, then the finite-warp preload map W u */
+    double ul[24], uk[24], Tg[576], Kl[576];
+    element_displacements(ug, 4, v1, v2, v3, ul);
+    quad4_warp_map(lc, p, Tg);
+    for (int i = 0; i < 24; ++i) {
+        double s = 0.0;
+        for (int m = 0; m < 24; ++m) s += M(Tg, i, m, 24) * ul[m];
+        uk[i] = s;
+    }
+    int st = or_q4_kg_local(lc, uk, prop[OR_SP_E], prop[OR_SP_NU], prop[OR_SP_H], 1, Kl);
+    if (st != OR_OK) return st;
+    /* the operator has no drilling rows or columns (JFEM_KG_SHELL_DRILL_ZERO);
+     * a warped cell takes sym(W' Kg W), then the element frame */
+    if (quad4_is_warped(p, lc)) quad4_finite_warp(Kl, lc, p);
+    double R[9] = {v1[0], v1[1], v1[2], v2[0], v2[1], v2[2], v3[0], v3[1], v3[2]};
+    transform_blocks(Kl, 24, R, Kg);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------- */
 /* CTETRA10, src/tetra10.jl:6-79; iso_3d_constitutive src/FEMKernels.jl:12056 */
 /* ------------------------------------------------------------------------- */
 int or_tet10(const double X[30], double E, double nu, double Ke[900]) {

@@ -78,6 +78,17 @@ int or_q4_recover(const double p[12], const double prop[8], const double ug[24],
                   double out[OR_Q4R_N]);
 int or_t3_recover(const double p[9], const double prop[8], const double ug[18], double out[OR_T3R_N]);
 
+/* SOL 105 differential stiffness of a flat isotropic CQUAD4 under the membrane
+ * state of global displacements ug (geometric_stiffness_quad4_reference_kdjj_iso,
+ * src/FEMKernels.jl:8810-8942, as assembled by src/solver/assembly.jl:6575-6594,
+ * 8063-8075, 8264-8320): global 24x24 Kg (column-major).  OR_ERR_WARPED for a
+ * cell outside the flat-element gate (corner deviation >= 1e-2 of the longer
+ * diagonal).  or_q4_kg_local is the element-frame kernel on local coordinates
+ * lc and element-frame displacements ue (assumed_transverse as in assembly). */
+int or_q4_kg(const double p[12], const double prop[8], const double ug[24], double Kg[576]);
+int or_q4_kg_local(const double lc[8], const double ue[24], double E, double nu, double h, int assumed_transverse,
+                   double Kg[576]);
+
 /* ---------- assembly with sparse(I,J,V) semantics ---------- */
 /* Triplet order: all CQUAD4 (element order, column-major 24x24), then CTRIA3, then
  * CTETRA10.  Duplicates summed in triplet order.  Output CSR (row_ptr n+1, col, val)

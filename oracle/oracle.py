@@ -43,6 +43,8 @@ def lib():
         L.or_t3_global.argtypes = [dp, dp, C.c_double, dp]
         L.or_q4_recover.argtypes = [dp, dp, dp, C.c_int, dp]
         L.or_t3_recover.argtypes = [dp, dp, dp, dp]
+        L.or_q4_kg.argtypes = [dp, dp, dp, dp]
+        L.or_q4_kg_local.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp]
         L.or_assemble.argtypes = [C.c_int64, dp, dp, dp, C.c_int64, C.c_void_p, C.c_void_p,
                                   C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                   C.c_double, C.POINTER(C.c_int64), C.POINTER(C.POINTER(C.c_int64)),
@@ -193,6 +195,69 @@ def shell_recover(model, u, mean_shear=True):
         ug = np.concatenate([u[6 * n:6 * n + 6] for n in nodes])
         t[e] = t3_recover(model.xyz[nodes], model.tria_props[e], ug)
     return q, t
+
+
+def q4_kg(p, prop, ug):
+    """Global 24 x 24 differential stiffness of one flat isotropic CQUAD4 (or_q4_kg)
+    for corner coordinates p (4 x 3), section prop and global displacements ug."""
+    out = np.zeros(576)
+    st = lib().or_q4_kg(_f(np.asarray(p, float).reshape(4, 3).ravel()), _f(prop),
+                        _f(np.asarray(ug, float).ravel()), out)
+    if st:
+        raise OracleError(st)
+    return out.reshape(24, 24, order="F")
+
+
+def q4_kg_local(lc, ue, E, nu, h, assumed_transverse=True):
+    """The element-frame kernel (or_q4_kg_local) on local coordinates lc (4 x 2)
+    and element-frame displacements ue (24)."""
+    out = np.zeros(576)
+    st = lib().or_q4_kg_local(_f(np.asarray(lc, float).reshape(4, 2).ravel()), _f(np.asarray(ue, float).ravel()),
+                              float(E), float(nu), float(h), int(bool(assumed_transverse)), out)
+    if st:
+        raise OracleError(st)
+    return out.reshape(24, 24, order="F")
+
+
+def kg_assemble(model, u):
+    """Global differential stiffness (scipy CSR, ndof x ndof) of a CQUAD4 model for
+    the displacement vector u: element matrices summed in element order."""
+    u = np.asarray(u, float).reshape(-1)
+    nq = len(model.quads)
+    rows, cols, vals = [], [], []
+    for e, nodes in enumerate(np.asarray(model.quads)):
+        ug = np.concatenate([u[6 * n:6 * n + 6] for n in nodes])
+        kg = q4_kg(model.xyz[nodes], model.quad_props[e], ug)
+        dofs = np.concatenate([np.arange(6 * n, 6 * n + 6) for n in nodes])
+        rows.append(np.repeat(dofs, 24)); cols.append(np.tile(dofs, 24)); vals.append(kg.ravel())
+    if len(model.trias) or len(model.tets):
+        raise OracleError(5)
+    n = model.ndof
+    if nq == 0:
+        return sp.csr_matrix((n, n))
+    return sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n)).tocsr()
+
+
+def buckling_dense(K, Kg, fixed):
+    """All buckling factors of the pencil K_ff phi = lambda (-Kg_ff) phi on the free
+    DOFs (dense, for small models): (sorted finite eigenvalues, modes (ndof x m))
+    with each mode scaled to unit largest component, K_ff positive definite."""
+    import scipy.linalg as sl
+    free = np.where(np.asarray(fixed) == 0)[0]
+    Kf = K[free][:, free].toarray()
+    Gf = -Kg[free][:, free].toarray()
+    Gf = 0.5 * (Gf + Gf.T)
+    # K_ff SPD: eigen-pairs of (G, K): G phi = mu K phi, lambda = 1 / mu
+    mu, V = sl.eigh(Gf, Kf)
+    keep = np.abs(mu) > 1e-14 * np.abs(mu).max()
+    lam = 1.0 / mu[keep]
+    V = V[:, keep]
+    order = np.argsort(lam)
+    lam, V = lam[order], V[:, order]
+    modes = np.zeros((K.shape[0], len(lam)))
+    modes[free] = V
+    modes /= np.abs(modes).max(axis=0)
+    return lam, modes
 
 
 def _cm_i32(a):

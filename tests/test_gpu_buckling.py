@@ -1,0 +1,163 @@
+"""GPU parity of the SOL 105 path against the oracle and the reference's recorded
+buckling results.
+
+* Per-element differential stiffness (jfem_q4_kg_batch) equals the oracle's
+  or_q4_kg to 1e-12 of the largest entry, on skewed, tilted and slightly warped
+  cells under random preloads; the assembled Kg (jfem_assemble_kg) equals the
+  oracle's element-order sum to 1e-12; a cell beyond the flat-element gate is
+  rejected with JFEM_ERR_WARPED and its index.
+* K + sigma Kg (jfem_matrix_shift) and the block products (jfem_matrix_spmv,
+  jfem_block_gram, jfem_block_combine) equal their numpy counterparts.
+* The recorded plate (validation/cases/TG_plate_uniaxial_buckling, 5 x 5 CQUAD4,
+  EIGB 1 INV 0.0 1.0E9 4) read from its deck and solved on the device: 189 free
+  DOFs, the preload within 1e-6 of the recorded one, the seven roots below L2 =
+  1e9 within 1e-9 of the oracle's dense eigensolution, within 1.2 % of the
+  recorded eigenvalues, and modes of MAC >= 0.9995 with the recorded ones; the
+  same roots from a factorization of K + sigma Kg shifted toward them.
+* A 16 x 16 plate: the first six roots within 1e-9 of the dense eigensolution.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from jf_2026_05_25_amd import deck as D
+from jf_2026_05_25_amd import meshes
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from jf_2026_05_25_amd import solver
+    c = solver.Context(0)
+    yield c
+    c.close()
+
+
+def golden():
+    return json.load(open(os.path.join(HERE, "golden", "reference_buckling.json")))
+
+
+def kg_models():
+    rng = np.random.default_rng(11)
+    skew = meshes.flat_plate(7, 5, Lx=0.7, Ly=0.5)
+    inner = [i for i, p in enumerate(skew.xyz) if 0 < p[0] < 0.7 and 0 < p[1] < 0.5]
+    skew.xyz[inner, :2] += rng.uniform(-0.02, 0.02, size=(len(inner), 2))
+    tilt = meshes.flat_plate(6, 6, Lx=0.6, Ly=0.6)
+    c, s = np.cos(0.6), np.sin(0.6)
+    R = np.array([[1, 0, 0], [0, c, -s], [0, s, c]]) @ np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    tilt.xyz = tilt.xyz @ R.T + np.array([0.5, -1.0, 2.0])
+    warp = meshes.flat_plate(6, 4, Lx=0.6, Ly=0.4)
+    warp.xyz[:, 2] = rng.uniform(-2e-4, 2e-4, size=len(warp.xyz))
+    return {"skew": skew, "tilt": tilt, "warp": warp, "box": meshes.wing_box(12, 4, nrhs=1)}
+
+
+@pytest.mark.parametrize("name", ["skew", "tilt", "warp", "box"])
+def test_element_and_assembled_kg(ctx, oracle, name):
+    from jf_2026_05_25_amd import buckling, solver
+    m = kg_models()[name]
+    u = np.random.default_rng(5).normal(size=m.ndof) * 1e-4
+    asm = solver.assemble_stiffness(m, ctx)
+    kg = buckling.element_kg(asm.mesh, u)
+    exp = np.array([oracle.q4_kg(m.xyz[q], m.quad_props[e], np.concatenate([u[6 * n:6 * n + 6] for n in q]))
+                    for e, q in enumerate(m.quads)])
+    scale = np.abs(exp).max()
+    assert np.abs(kg - exp).max() <= 1e-12 * scale, np.abs(kg - exp).max() / scale
+    G = buckling.GeometricStiffness(asm, u)
+    Gd = G.to_scipy()
+    Go = oracle.kg_assemble(m, u)
+    diff = abs(Gd - Go).max()
+    assert diff <= 1e-12 * abs(Go).max(), diff / abs(Go).max()
+    G.free()
+
+
+def test_warped_cell_rejected(ctx):
+    from jf_2026_05_25_amd import _lib, buckling, solver
+    m = meshes.flat_plate(3, 3, Lx=0.3, Ly=0.3)
+    corner = [i for i, p in enumerate(m.xyz) if np.isclose(p[0], 0.1) and np.isclose(p[1], 0.1)][0]
+    m.xyz[corner, 2] = 0.01                     # 0.0025 out of plane on a 0.14 diagonal
+    asm = solver.assemble_stiffness(m, ctx)
+    with pytest.raises(_lib.JfemError) as e:
+        buckling.element_kg(asm.mesh, np.zeros(m.ndof))
+    assert e.value.code == 4
+
+
+def test_shift_and_block_products(ctx):
+    from jf_2026_05_25_amd import buckling, solver
+    m = meshes.flat_plate(8, 6)
+    rng = np.random.default_rng(9)
+    u = rng.normal(size=m.ndof) * 1e-4
+    asm = solver.assemble_stiffness(m, ctx)
+    G = buckling.GeometricStiffness(asm, u)
+    A = buckling.ShiftedMatrix(asm.K, -2.5e3, G)
+    K, Gs, As = asm.K.to_scipy(), G.to_scipy(), A.to_scipy()
+    # one rounding (or two, unfused) per entry
+    assert abs(As - (K - 2.5e3 * Gs)).max() <= 4.5e-16 * (abs(K) + abs(2.5e3 * Gs)).max()
+    fixed = (rng.random(m.ndof) < 0.2).astype(np.uint8)
+    n, p, q = m.ndof, 5, 3
+    B = buckling._Blocks(ctx, n, p, fixed)
+    try:
+        X = rng.normal(size=(n, p))
+        B.bufs["X"].upload(np.ascontiguousarray(X.T))
+        B.spmv(G, "X", "W", p)
+        W = B.download("W", p)
+        f = fixed == 0
+        Gf = Gs.multiply(np.outer(f, f)).tocsr()
+        Xf = np.where(f[:, None], X, 0.0)
+        assert np.abs(W - Gf @ Xf).max() <= 1e-13 * np.abs(Gf @ Xf).max()
+        C = B.gram("X", "W", p, p)
+        assert np.abs(C - X.T @ W).max() <= 1e-12 * np.abs(X.T @ W).max()
+        Q = rng.normal(size=(p, q))
+        B.combine("X", Q, "Y")
+        Y = B.download("Y", p)[:, :q]
+        assert np.abs(Y - X @ Q).max() <= 1e-13 * np.abs(X @ Q).max()
+    finally:
+        B.free()
+        A.free()
+        G.free()
+
+
+def _mac(a, b):
+    return (a @ b) ** 2 / ((a @ a) * (b @ b))
+
+
+def test_recorded_plate(ctx, oracle):
+    g = golden()
+    dk = D.read_deck(g["deck"])
+    res = D.solve_buckling_deck(dk, ctx=ctx)
+    r, u, diag = res[2]
+    assert r.diagnostics["free_dofs"] == g["free_dofs"]
+    us = np.array(g["static_displacements"]["values"]).ravel()
+    assert np.linalg.norm(u - us) <= 1e-6 * np.linalg.norm(us)
+    ref = np.array(g["eigenvalues"])
+    assert len(r.eigenvalues) == len(ref)             # all roots in (0, 1e9] (NDP = 12 > 7)
+    # oracle: dense eigensolution of the same pencil
+    m = dk.model(1)
+    m.loads = oracle.load_vector(m, **dk.load_cards(1))[:, None]
+    K, fixed, uo, _, _ = oracle.sol101(m, refine=10)
+    lam, modes = oracle.buckling_dense(K, oracle.kg_assemble(m, uo[:, 0]), fixed)
+    lam_pos = lam[lam > 0]
+    np.testing.assert_allclose(r.eigenvalues, lam_pos[:len(ref)], rtol=1e-9)
+    assert np.all(np.abs(r.eigenvalues / ref - 1) <= 0.012)
+    for k, md in enumerate(g["modes"]):
+        phi = np.array(md["values"]).ravel()
+        assert _mac(phi, r.modes[:, k]) >= 0.9995, k
+        assert np.isclose(np.abs(r.modes[:, k]).max(), 1.0)
+    assert max(r.diagnostics["mode_residual"]) <= 1e-8
+    # the same roots from the factor of K + sigma Kg, sigma half the first root
+    res2 = D.solve_buckling_deck(dk, ctx=ctx, shift=0.5 * ref[0])
+    np.testing.assert_allclose(res2[2][0].eigenvalues, r.eigenvalues, rtol=1e-9)
+
+
+def test_plate_16_vs_dense(ctx, oracle):
+    import sys
+    sys.path.insert(0, HERE)
+    from test_oracle_buckling import buckle, ss_plate
+    from jf_2026_05_25_amd import buckling
+    m = ss_plate(16, oracle)
+    lam, _, _, _ = buckle(m, oracle)
+    r, u, diag = buckling.solve_buckling(m, buckling.EigbRequest(1, "INV", 0.0, 10 * lam[5], 2, 6), ctx=ctx)
+    np.testing.assert_allclose(r.eigenvalues, lam[:6], rtol=1e-9)
