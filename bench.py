@@ -35,7 +35,7 @@ MI355X_FP32_TFLOPS = 157.3
 MI355X_FP64_TFLOPS = 78.6
 METRIC = "solves/s + p50/p99 latency @1e-10 residual: FP32-factor+refine vs FP64"
 FAMILIES = ["element", "assembly", "factor", "solve", "spmv", "factor_update", "factor_panel", "extend_add",
-            "fwd_flow", "bwd_flow"]
+            "fwd_flow", "bwd_flow", "factor_schur"]
 # algorithmic bytes of one element's generation: read its grid points' coordinates,
 # its connectivity and its section record, write its matrix once
 ELEM_READ = {"CQUAD4": 4 * 24 + 4 * 4 + 8 * 8, "CTRIA3": 3 * 24 + 3 * 4 + 8 * 8, "CTETRA10": 10 * 24 + 10 * 4 + 2 * 8}
@@ -258,15 +258,17 @@ def solve_rooflines(fam, info, nrhs, precision):
         lines[fname] = (f_ms, {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": MI355X_HBM_GBS,
                                "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS, "traffic": None,
                                "bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": f_n})
-    upd_ms, upd_n = fam["factor_update"]
-    if upd_n:
-        per_launch_flops = info["update_flops"] / upd_n
-        avg_ms = upd_ms / upd_n
+    for fname, kname, fkey in (("factor_update", f"update_kernel<{tname}>", "update_flops"),
+                               ("factor_schur", f"schur_kernel<{tname}>", "schur_flops")):
+        k_ms, k_n = fam[fname]
+        if not k_n:
+            continue
+        per_launch_flops = info[fkey] / k_n
+        avg_ms = k_ms / k_n
         achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
-        lines["factor_update"] = (upd_ms, {"bound": "mfma", "kernel": f"update_kernel<{tname}>", "achieved": achieved,
-                                           "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                                           "flops_per_launch": per_launch_flops, "avg_launch_ms": avg_ms,
-                                           "launches": upd_n})
+        lines[fname] = (k_ms, {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": peak,
+                               "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                               "flops_per_launch": per_launch_flops, "avg_launch_ms": avg_ms, "launches": k_n})
     return lines
 
 

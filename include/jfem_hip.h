@@ -67,7 +67,7 @@ int jfem_device_memory(jfem_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes)
  * 1 assembly, 2 numeric factor, 3 triangular solves, 4 residual SpMV; with timing
  * level 2 also per launch: 5 factor update kernel, 6 factor panel kernels,
  * 7 extend-add kernel, 8 forward dataflow solve kernel, 9 backward dataflow
- * solve kernel.
+ * solve kernel, 10 update-matrix (Schur complement) kernel.
  * jfem_ctx_enable_timing(ctx, 0 | 1 | 2). */
 int jfem_ctx_kernel_time(jfem_ctx *ctx, int family, double *ms, int64_t *launches);
 int jfem_ctx_reset_kernel_time(jfem_ctx *ctx);
@@ -256,13 +256,17 @@ typedef struct {
     double flops;            /* sum_j (eta_j + 1)^2 over columns of L (counted per column) */
     int64_t max_front;       /* largest front order */
     int64_t update_entries;  /* stored update-matrix entries */
-    double update_flops;     /* multiply-adds x 2 of the trailing-update tiles (lower triangles) */
+    double update_flops;     /* multiply-adds x 2 of the per-step trailing-update tiles (pivot
+                                columns of the fronts, lower triangles) */
     int64_t update_launches; /* update-kernel launches per factorization */
     int64_t inv_entries;      /* entries of the panels' inverse diagonal blocks (solves) */
     int64_t upd_rows;         /* update-vector rows over all supernodes */
     int64_t flow_fwd_tasks, flow_bwd_tasks;   /* workgroups of the two dataflow solve kernels */
     int64_t solve_entries;    /* factor entries one triangular solve reads: per panel its
                                  inverse diagonal block and the rows below that block */
+    double schur_flops;       /* multiply-adds x 2 of the update-matrix tiles, each taken once
+                                 with the full rank of its front (lower triangles) */
+    int64_t schur_launches;   /* update-matrix launches per factorization (one per level) */
 } jfem_symbolic_info;
 
 /* Fill-reducing ordering (geometric nested dissection over grid points),

@@ -53,7 +53,8 @@ class SymbolicInfo(C.Structure):
                 ("nnz_L", C.c_int64), ("flops", C.c_double), ("max_front", C.c_int64),
                 ("update_entries", C.c_int64), ("update_flops", C.c_double), ("update_launches", C.c_int64),
                 ("inv_entries", C.c_int64), ("upd_rows", C.c_int64), ("flow_fwd_tasks", C.c_int64),
-                ("flow_bwd_tasks", C.c_int64), ("solve_entries", C.c_int64)]
+                ("flow_bwd_tasks", C.c_int64), ("solve_entries", C.c_int64), ("schur_flops", C.c_double),
+                ("schur_launches", C.c_int64)]
 
 
 _lib = None

@@ -35,6 +35,7 @@ struct jfem_symbolic {
     FlowTask *d_flow_f = nullptr, *d_flow_b = nullptr;
     FlowChild *d_flow_gb = nullptr;
     UpdateTask *d_utasks = nullptr;
+    UpdateTask *d_stasks = nullptr;
     EaTask *d_etasks = nullptr;
     int32_t *d_ea_bounds = nullptr;
     int64_t *d_ea_boff = nullptr;
@@ -642,6 +643,97 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
     const int bad = potrf_dinv(a, reinterpret_cast<char *>(&B[0][0]), lane, wave, bn, L + lofs[s] + c0 + ms * c0, ms,
                                Linv + iofs[s] + (int64_t)kPanel * c0);
     if (bad < bn && tid == 0) report_min(status, blockIdx.y * vs.col + col0[s] + c0 + bad, JFEM_ERR_NOT_SPD);
+}
+
+// Update matrix of a front, after its last panel step: the tile (r0, c0) of U
+// (rows and columns >= ncol) takes -L[r0.., 0:ncol] L[c0.., 0:ncol]' (lower
+// triangle of the tile) with the whole rank ncol at once.  Chunks of kPanel
+// columns are staged through LDS; the next chunk's loads are issued before the
+// current chunk's products.  MFMA operand layout and tile split over the four
+// wavefronts as in update_kernel; the products accumulate over the chunks in
+// order, so the result is reproducible bit for bit.
+template <class T>
+__global__ void __launch_bounds__(kBS) schur_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
+                                                    const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
+                                                    const int64_t *__restrict__ uofs, const T *__restrict__ L,
+                                                    T *__restrict__ U, VarStride vs) {
+    typedef typename Mfma<T>::v4 v4;
+    L += blockIdx.y * vs.L;
+    U += blockIdx.y * vs.U;
+    __shared__ T A[kPanel][kTile + 1];   // A[p][i] = L[r0 + i, k + p]
+    __shared__ T B[kPanel][kTile + 1];   // B[p][j] = L[c0 + j, k + p]
+    const UpdateTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int64_t ns = ncol[s], ms = mrow[s];
+    const int64_t r0 = t.r0, c0 = t.c0;
+    const int nr = (int)min<int64_t>(kTile, ms - r0), nc = (int)min<int64_t>(kTile, ms - c0);
+    const T *Ls = L + lofs[s];
+    T *Cs = U + uofs[s];
+    const int64_t ld = ms - ns;
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    T cpre[2][2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = min(wi + 16 * x + l16, nr - 1), j = min(wj + 16 * y + Mfma<T>::drow(l4, v), nc - 1);
+                cpre[x][y][v] = Cs[(r0 + i - ns) + ld * (c0 + j - ns)];
+            }
+    T av[16], bv[16];
+    auto load = [&](int64_t k) {
+        const int b = (int)min<int64_t>(kPanel, ns - k);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = tid + kBS * u, i = q % kTile, p = min(q / kTile, b - 1);
+            av[u] = Ls[(r0 + min(i, nr - 1)) + ms * (k + p)];
+            bv[u] = Ls[(c0 + min(i, nc - 1)) + ms * (k + p)];
+        }
+    };
+    v4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = v4{T(0), T(0), T(0), T(0)};
+    load(0);
+    for (int64_t k = 0; k < ns; k += kPanel) {
+        const int b = (int)min<int64_t>(kPanel, ns - k);
+        const int b4 = (b + 3) & ~3;
+        if (k) __syncthreads();                  // the previous chunk's products are done with A and B
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = tid + kBS * u, i = q % kTile, p = q / kTile;
+            A[p][i] = (p < b && i < nr) ? av[u] : T(0);
+            B[p][i] = (p < b && i < nc) ? bv[u] : T(0);
+        }
+        __syncthreads();
+        if (k + kPanel < ns) load(k + kPanel);  // in flight during the products below
+        for (int p = 0; p < b4; p += 4) {
+            T aj[2], bi[2];
+#pragma unroll
+            for (int y = 0; y < 2; ++y) aj[y] = B[p + l4][wj + 16 * y + l16];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) bi[x] = A[p + l4][wi + 16 * x + l16];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) acc[x][y] = Mfma<T>::mac(aj[y], bi[x], acc[x][y]);
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = wi + 16 * x + l16, j = wj + 16 * y + Mfma<T>::drow(l4, v);
+                if (i < nr && j < nc && r0 + i >= c0 + j)
+                    Cs[(r0 + i - ns) + ld * (c0 + j - ns)] = cpre[x][y][v] - acc[x][y][v];
+            }
 }
 
 // chain products of the triangular solves, one workgroup per panel boundary q
@@ -2004,6 +2096,14 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
                 if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_UPDATE, 1); }
             }
         }
+        // the level's update matrices, with the full rank of each front
+        const int64_t nsch = H.sstep_off[l + 1] - H.sstep_off[l];
+        if (nsch) {
+            if (detail) (void)hipEventRecord(ctx->ev2, st);
+            hipLaunchKernelGGL(schur_kernel<T>, dim3((unsigned)nsch, V), dim3(kBS), 0, st, S->d_stasks + H.sstep_off[l],
+                               S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, (const T *)L, U, vs);
+            if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_SCHUR, 1); }
+        }
     }
     // inverse diagonal blocks and chain products for the triangular solves
     if (!H.flow_b.empty()) {
@@ -2133,6 +2233,7 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_amap_dst, H.amap_dst, st);
     if (e == hipSuccess) e = up(S, &S->d_ptasks, H.ptasks, st);
     if (e == hipSuccess) e = up(S, &S->d_utasks, H.utasks, st);
+    if (e == hipSuccess) e = up(S, &S->d_stasks, H.stasks, st);
     if (e == hipSuccess) e = up(S, &S->d_etasks, H.etasks, st);
     if (e == hipSuccess) e = up(S, &S->d_ea_bounds, H.ea_bounds, st);
     if (e == hipSuccess) e = up(S, &S->d_ea_boff, H.ea_boff, st);
@@ -2154,6 +2255,10 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
         info->update_entries = H.U_entries;
         info->update_flops = H.update_flops;
         info->update_launches = (int64_t)H.ustep_off.size() - 1;
+        info->schur_flops = H.schur_flops;
+        int64_t nsl = 0;
+        for (size_t l = 0; l + 1 < H.sstep_off.size(); ++l) nsl += H.sstep_off[l + 1] > H.sstep_off[l];
+        info->schur_launches = nsl;
         info->inv_entries = H.inv_entries;
         info->upd_rows = H.upd_rows;
         info->flow_fwd_tasks = (int64_t)H.flow_f.size();

@@ -645,10 +645,14 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 if (k >= S.ncol[s]) continue;
                 int64_t b = std::min<int64_t>(kPanel, S.ncol[s] - k);
                 for (int64_t r0 = k + kTile; r0 < S.m[s]; r0 += kTile) S.ptasks.push_back({s, (int32_t)k, (int32_t)r0, 0});
+                // tile origins of the trailing columns of the front (pivot columns, then
+                // update rows); per step only the tiles of pivot columns: the update
+                // matrix takes its whole rank-ncol product once, after the last step
                 std::vector<int32_t> bnd;
                 for (int64_t c = k + b; c < S.ncol[s]; c += kTile) bnd.push_back((int32_t)c);
+                const size_t npiv = bnd.size();
                 for (int64_t c = std::max<int64_t>(k + b, S.ncol[s]); c < S.m[s]; c += kTile) bnd.push_back((int32_t)c);
-                for (size_t ci = 0; ci < bnd.size(); ++ci)
+                for (size_t ci = 0; ci < npiv; ++ci)
                     for (size_t ri = ci; ri < bnd.size(); ++ri) {
                         S.utasks.push_back({s, (int32_t)k, bnd[ri], bnd[ci]});
                         int64_t r0 = bnd[ri], c0 = bnd[ci];
@@ -661,6 +665,27 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
             }
         }
     }
+    // per level: the update-matrix tiles of its fronts (lower triangles), taken
+    // after the level's last panel step with the full rank ncol of each front
+    S.stasks.clear();
+    S.sstep_off.assign(nlev + 1, 0);
+    S.schur_flops = 0;
+    for (int32_t l = 0; l < nlev; ++l) {
+        S.sstep_off[l] = (int64_t)S.stasks.size();
+        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+            const int32_t s = S.level_sn[q];
+            const int64_t ns = S.ncol[s], ms = S.m[s];
+            for (int64_t c0 = ns; c0 < ms; c0 += kTile)
+                for (int64_t r0 = c0; r0 < ms; r0 += kTile) {
+                    S.stasks.push_back({s, 0, (int32_t)r0, (int32_t)c0});
+                    const int64_t r1 = std::min<int64_t>(r0 + kTile, ms), c1 = std::min<int64_t>(c0 + kTile, ms);
+                    double cnt = 0;
+                    for (int64_t c = c0; c < c1; ++c) cnt += (double)std::max<int64_t>(0, r1 - std::max(r0, c));
+                    S.schur_flops += 2.0 * (double)ns * cnt;
+                }
+        }
+    }
+    S.sstep_off[nlev] = (int64_t)S.stasks.size();
     S.step_ptr[nlev] = (int64_t)S.pstep_off.size();
     S.pstep_off.push_back((int64_t)S.ptasks.size());
     S.ustep_off.push_back((int64_t)S.utasks.size());

@@ -61,6 +61,8 @@ struct SymbolicHost {
     // per-level task lists
     std::vector<PanelTask> ptasks;
     std::vector<UpdateTask> utasks;
+    std::vector<UpdateTask> stasks;         // update-matrix tiles (k unused), per level
+    std::vector<int64_t> sstep_off;         // per level: offset into stasks (size levels+1)
     std::vector<EaTask> etasks;
     // ptask/utask ranges per (level, step): offsets into the arrays
     std::vector<int64_t> step_ptr;          // per level: first step index (size levels+1)
@@ -89,7 +91,8 @@ struct SymbolicHost {
     int64_t inv_entries = 0;        // diagonal-block inverses, then the chain products (bofs)
     int64_t L_entries = 0, U_entries = 0, upd_rows = 0;
     double flops = 0;
-    double update_flops = 0;   // 2 x multiply-adds of the update tiles (lower triangles)
+    double update_flops = 0;   // 2 x multiply-adds of the per-step tiles (lower triangles)
+    double schur_flops = 0;    // 2 x multiply-adds of the update-matrix tiles (lower triangles)
     int64_t max_front = 0;
 };
 

@@ -110,6 +110,21 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                     if (int e = potrf(s, t.c0)) return e;
             }
         }
+        // after the level's last step: the update matrices, full rank per front
+        for (int64_t q = S.sstep_off[l]; q < S.sstep_off[l + 1]; ++q) {
+            UpdateTask t = S.stasks[q];
+            int s = t.s;
+            int64_t ns = S.ncol[s], ms = S.m[s];
+            if (t.r0 < ns || t.c0 < ns || t.r0 < t.c0 || t.r0 >= ms) return 12;
+            int64_t r1 = std::min<int64_t>(t.r0 + kTile, ms), c1 = std::min<int64_t>(t.c0 + kTile, ms);
+            const double *Ls = L.data() + S.lofs[s];
+            for (int64_t c = t.c0; c < c1; ++c)
+                for (int64_t r = std::max<int64_t>(t.r0, c); r < r1; ++r) {
+                    double acc = 0;
+                    for (int64_t p = 0; p < ns; ++p) acc += Ls[r + ms * p] * Ls[c + ms * p];
+                    U[S.uofs[s] + (r - ns) + (ms - ns) * (c - ns)] -= acc;
+                }
+        }
     }
     // solves, interleaved right-hand sides (v[i * nrhs + r])
     const int64_t n = S.n;
