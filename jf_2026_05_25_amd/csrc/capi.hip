@@ -15,6 +15,37 @@ int fail(jfem_ctx *ctx, int code, const std::string &msg) {
     return code;
 }
 
+void tl_begin(jfem_ctx *ctx) {
+    ctx->tl_n = 0;
+    tl_mark(ctx, -1);
+}
+
+void tl_mark(jfem_ctx *ctx, int fam) {
+    if (ctx->tl_n == ctx->tl_ev.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        ctx->tl_ev.push_back(e);
+        ctx->tl_fam.push_back(-1);
+    }
+    (void)hipEventRecord(ctx->tl_ev[ctx->tl_n], ctx->stream);
+    ctx->tl_fam[ctx->tl_n] = fam;
+    ++ctx->tl_n;
+}
+
+void tl_flush(jfem_ctx *ctx) {
+    if (ctx->tl_n == 0) return;
+    (void)hipEventSynchronize(ctx->tl_ev[ctx->tl_n - 1]);
+    for (size_t i = 1; i < ctx->tl_n; ++i) {
+        const int f = ctx->tl_fam[i];
+        if (f < 0 || f >= FAM_COUNT) continue;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ctx->tl_ev[i - 1], ctx->tl_ev[i]);
+        ctx->ms[f] += ms;
+        ctx->launches[f] += 1;
+    }
+    ctx->tl_n = 0;
+}
+
 void timer_start(jfem_ctx *ctx) {
     if (ctx->timing) (void)hipEventRecord(ctx->ev0, ctx->stream);
 }
@@ -74,6 +105,7 @@ int jfem_ctx_destroy(jfem_ctx *ctx) {
     (void)hipEventDestroy(ctx->ev1);
     (void)hipEventDestroy(ctx->ev2);
     (void)hipEventDestroy(ctx->ev3);
+    for (hipEvent_t e : ctx->tl_ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return JFEM_OK;

@@ -2044,13 +2044,10 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
     hipStream_t st = ctx->stream;
     const SymbolicHost &H = S->H;
     const bool detail = ctx->timing_level >= 2;
-    auto lap = [&](int fam, int64_t launches) {
-        (void)hipEventSynchronize(ctx->ev3);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, ctx->ev2, ctx->ev3);
-        ctx->ms[fam] += ms;
-        ctx->launches[fam] += launches;
+    auto lap = [&](int fam) {
+        if (detail) tl_mark(ctx, fam);
     };
+    if (detail) tl_begin(ctx);
     const int64_t nLv = V > 1 ? vs.L * (V - 1) + H.L_entries : H.L_entries;
     const int64_t nUv = V > 1 ? vs.U * (V - 1) + H.U_entries : H.U_entries;
     hipLaunchKernelGGL(zero16_kernel<T>, dim3(grid_for(nLv * (int64_t)sizeof(T) / 16)), dim3(kBS), 0, st, L, nLv);
@@ -2060,49 +2057,47 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
     if (na)
         hipLaunchKernelGGL(scatter_kernel<T>, dim3(grid_for(na), V), dim3(kBS), 0, st, na, S->d_amap_src, S->d_amap_dst,
                            Kval, L, vs);
+    lap(-1);
     const int64_t nlev = (int64_t)H.level_ptr.size() - 1;
     for (int64_t l = 0; l < nlev; ++l) {
         int64_t ne = H.ea_off[l + 1] - H.ea_off[l];
         if (ne) {
-            if (detail) (void)hipEventRecord(ctx->ev2, st);
             hipLaunchKernelGGL(extend_add_kernel<T>, dim3((unsigned)ne, V), dim3(kBS), 0, st, S->d_etasks + H.ea_off[l],
                                S->d_ea_boff + H.ea_off[l], S->d_ea_bounds, S->d_ncol, S->d_m, S->d_lofs, S->d_uofs,
                                S->d_relofs, S->d_relmap, S->d_child_ptr, S->d_child, L, U, vs);
-            if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_EA, 1); }
+            lap(FAM_EA);
         }
         if (H.step_ptr[l + 1] > H.step_ptr[l]) {
             // diagonal blocks of the level's first panels
             const int64_t stp = H.step_ptr[l];
             const int64_t nd = H.pstep_mid[stp] - H.pstep_off[stp];
-            if (detail) (void)hipEventRecord(ctx->ev2, st);
-            if (nd)
+            if (nd) {
                 hipLaunchKernelGGL(diag_factor_kernel<T>, dim3((unsigned)nd, V), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
                                    S->d_ncol, S->d_m, S->d_lofs, S->d_iofs, S->d_col0, L, Linv, ctx->d_status, vs);
-            if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, nd ? 1 : 0); }
+                lap(FAM_PANEL);
+            }
         }
         for (int64_t stp = H.step_ptr[l]; stp < H.step_ptr[l + 1]; ++stp) {
             const int64_t np = H.pstep_off[stp + 1] - H.pstep_off[stp];
             const int64_t nu = H.ustep_off[stp + 1] - H.ustep_off[stp];
-            if (detail) (void)hipEventRecord(ctx->ev2, st);
-            if (np)
+            if (np) {
                 hipLaunchKernelGGL(tile_solve_kernel<T>, dim3((unsigned)np, V), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
                                    S->d_ncol, S->d_m, S->d_lofs, S->d_iofs, L, (const T *)Linv, vs);
-            if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_PANEL, np ? 1 : 0); }
+                lap(FAM_PANEL);
+            }
             if (nu) {
-                if (detail) (void)hipEventRecord(ctx->ev2, st);
                 hipLaunchKernelGGL(update_kernel<T>, dim3((unsigned)nu, V), dim3(kBS), 0, st, S->d_utasks + H.ustep_off[stp],
                                    S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, S->d_iofs, L, U, Linv, 1, S->d_col0,
                                    ctx->d_status, vs);
-                if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_UPDATE, 1); }
+                lap(FAM_UPDATE);
             }
         }
         // the level's update matrices, with the full rank of each front
         const int64_t nsch = H.sstep_off[l + 1] - H.sstep_off[l];
         if (nsch) {
-            if (detail) (void)hipEventRecord(ctx->ev2, st);
             hipLaunchKernelGGL(schur_kernel<T>, dim3((unsigned)nsch, V), dim3(kBS), 0, st, S->d_stasks + H.sstep_off[l],
                                S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, (const T *)L, U, vs);
-            if (detail) { (void)hipEventRecord(ctx->ev3, st); lap(FAM_SCHUR, 1); }
+            lap(FAM_SCHUR);
         }
     }
     // inverse diagonal blocks and chain products for the triangular solves
@@ -2111,6 +2106,10 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
                            (const T *)L, Linv, vs);
         hipLaunchKernelGGL(chain_product_kernel<T>, dim3((unsigned)H.flow_b.size(), V), dim3(kBS), 0, st, S->d_flow_b,
                            (const T *)L, Linv, vs);
+    }
+    if (detail) {
+        lap(-1);
+        tl_flush(ctx);
     }
     return hipGetLastError();
 }
@@ -2127,24 +2126,18 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
     const T *L = (const T *)F->d_L, *Linv = (const T *)F->d_Linv;
     const int64_t nv = H.n * nrhs;
     const int64_t nwv = H.upd_rows * nrhs;     // update vectors: polled by the parents
+    // timing level 2: the kernels on the context's event timeline
+    const bool detail = ctx->timing_level >= 2;
+    if (detail) tl_begin(ctx);
     hipLaunchKernelGGL(solve_marks_kernel<T>, dim3(grid_for(std::max(nv, nwv))), dim3(kBS), 0, st, Z, X, nv, W, nwv,
                        pending_mark<T>(), cnt + nsn + 1);
-    // timing level 2: each kernel between its own pair of events on this stream
-    const bool detail = ctx->timing_level >= 2;
-    auto lap = [&](int fam) {
-        (void)hipEventRecord(ctx->ev3, st);
-        (void)hipEventSynchronize(ctx->ev3);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, ctx->ev2, ctx->ev3);
-        ctx->ms[fam] += ms;
-        ctx->launches[fam] += 1;
-    };
+    auto lap = [&](int fam) { tl_mark(ctx, fam); };
+    if (detail) lap(-1);
     // forward: the bottom subtrees' tasks (the head of the list) one wavefront
     // each, then the rest; backward: the rest, then the bottom subtrees (the tail)
     const int64_t nfl = H.flow_f_leaf;
     const int64_t nbl = H.flow_b_leaf, nbm = (int64_t)H.flow_b.size() - nbl;
     if (!H.flow_f.empty()) {
-        if (detail) (void)hipEventRecord(ctx->ev2, st);
         const int64_t nleaf = kLeafQuads<T, NR> ? nfl : 0;
         const int64_t grid = (nleaf + 3) / 4 + (int64_t)H.flow_f.size() - nleaf;
         hipLaunchKernelGGL((fwd_flow_kernel<T, NR>), dim3((unsigned)grid), dim3(kBS), 0, st, S->d_flow_f, S->d_flow_gb,
@@ -2152,13 +2145,13 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
         if (detail) lap(FAM_FLOW_FWD);
     }
     if (!H.flow_b.empty()) {
-        if (detail) (void)hipEventRecord(ctx->ev2, st);
         const int64_t nmain = kLeafPairs<T, NR> ? nbm : (int64_t)H.flow_b.size(), nleaf = (int64_t)H.flow_b.size() - nmain;
         const int64_t grid = nmain + (nleaf + 1) / 2;
         hipLaunchKernelGGL((bwd_flow_kernel<T, NR>), dim3((unsigned)grid), dim3(kBS), 0, st, S->d_flow_b, S->d_rows, L,
                            Linv, (const T *)Z, X, nrhs, cnt, nsn, nmain, nleaf);
         if (detail) lap(FAM_FLOW_BWD);
     }
+    if (detail) tl_flush(ctx);
     return hipGetLastError();
 }
 

@@ -33,6 +33,12 @@ struct jfem_ctx {
     double ms[jfem::FAM_COUNT] = {0};
     int64_t launches[jfem::FAM_COUNT] = {0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // timing level 2: a timeline of events, one recorded after each kernel of a
+    // launch sequence and resolved once the sequence has been issued, so that no
+    // host wait (and no launch latency) falls inside a measured interval
+    std::vector<hipEvent_t> tl_ev;
+    std::vector<int> tl_fam;
+    size_t tl_n = 0;
     // device status word shared by kernels that report element / pivot failures
     int64_t *d_status = nullptr;   // [0] = first bad index (INT64_MAX if none), [1] = code
     // scratch for solves
@@ -111,6 +117,12 @@ inline int64_t jfem_stride(int64_t n, int64_t es) {
 
 namespace jfem {
 int fail(jfem_ctx *ctx, int code, const std::string &msg);
+// timeline of timing level 2: tl_begin starts a sequence; tl_mark records an event
+// after the kernel just issued, charged to family fam (fam < 0: not charged);
+// tl_flush waits for the last event and adds each interval to its family
+void tl_begin(jfem_ctx *ctx);
+void tl_mark(jfem_ctx *ctx, int fam);
+void tl_flush(jfem_ctx *ctx);
 void timer_start(jfem_ctx *ctx);
 void timer_stop(jfem_ctx *ctx, int family, int64_t launches);
 
