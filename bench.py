@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import dataclasses
 import json
+import math
 import os
 import platform
 import sys
@@ -480,6 +481,7 @@ def main():
                                                 "launch_batch") if k in v}
         cfg["C2_wingbox"] = {"fp32_vs_fp64": out.get("fp32_vs_fp64_speedup"),
                              "fp32_vs_fp64_ceiling": out.get("fp32_vs_fp64_ceiling")}
+        cfg["sol105_plate_128"] = buckling_line(ctx, 128)
         out["configs"] = cfg
     if args.workload == "wingbox" and not args.no_serving and args.only == "both":
         out["serving"] = run_serving_both(args, as_section=True)
@@ -489,6 +491,53 @@ def main():
         out["cpu_baseline"] = cb
     ctx.close()
     print(json.dumps(out))
+
+
+def buckling_line(ctx, n: int = 128, roots: int = 4, reps: int = 5):
+    """SOL 105 on the simply supported n x n plate under unit uniaxial compression
+    (meshes.ss_plate_compression): the static preload (FP64 factor, refined), the
+    differential stiffness on K's pattern (per-element generation and assembly,
+    timed over ``reps`` calls), and the first ``roots`` buckling roots by subspace
+    inverse iteration on the static factor (EIGB 0 < lambda, NDP = roots).  The
+    first root is compared with Timoshenko-Gere's sigma_cr = 4 pi^2 D / (b^2 h)."""
+    from jf_2026_05_25_amd import buckling, meshes, solver
+    m = meshes.ss_plate_compression(n)
+    asm = solver.assemble_stiffness(m, ctx)
+    cache = solver.LinearSolveCache()
+    t0 = time.perf_counter()
+    u, fixed, _, sdiag = solver.apply_bc_and_solve(asm, m, precision=64, linear_cache=cache)
+    t_static = time.perf_counter() - t0
+    u = u.reshape(m.ndof, -1)[:, 0]
+    fx, S, F = next(iter(cache.entries.values()))[:3]
+    G = buckling.GeometricStiffness(asm, u, ctx)
+    G.free()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        G = buckling.GeometricStiffness(asm, u, ctx)
+        G.free()
+    t_kg = (time.perf_counter() - t0) / reps
+    G = buckling.GeometricStiffness(asm, u, ctx)
+    t0 = time.perf_counter()
+    res = buckling.buckling_roots(asm.K, G, fx, F, (0.0, math.inf, roots))
+    t_roots = time.perf_counter() - t0
+    G.free(); F.free(); S.free()
+    D = 210e9 * 0.01 ** 3 / (12 * (1 - 0.3 ** 2))
+    scr = 4 * math.pi ** 2 * D / 0.01
+    nq = len(m.quads)
+    kg_bytes = nq * (4 * 24 + 16 + 64 + 4 * 48 + 4608) + 12 * 576 * nq + 16 * asm.pattern.nnz
+    out = {"workload": f"SOL 105: simply supported {n}x{n} CQUAD4 plate under uniaxial compression, "
+                       f"first {roots} roots",
+           "dofs": m.ndof, "free_dofs": res.diagnostics["free_dofs"], "roots": [float(x) for x in res.eigenvalues],
+           "first_root_vs_timoshenko": float(res.eigenvalues[0] / scr),
+           "static_preload_ms": 1000 * t_static, "kg_generation_assembly_ms": 1000 * t_kg,
+           "kg_algorithmic_bytes": kg_bytes, "kg_achieved_GBps": kg_bytes / t_kg / 1e9,
+           "roots_ms": 1000 * t_roots, "subspace_iterations": res.diagnostics["iterations"],
+           "block": res.diagnostics["block"], "triangular_solves": res.diagnostics["triangular_solves"],
+           "max_mode_residual": float(max(res.diagnostics["mode_residual"])),
+           "note": "Kg timed with the preload upload and a stream synchronization per call; roots: host "
+                   "projected eigenproblem, device products and refined solves"}
+    asm.K.free(); asm.pattern.free(); asm.mesh.free()
+    return out
 
 
 def run_variants(args, count=None, streams: int = 4):

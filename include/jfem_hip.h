@@ -239,13 +239,14 @@ int jfem_matrix_shift(jfem_ctx *ctx, const jfem_matrix *K, double sigma, const j
  *   jfem_matrix_spmv:   y = A_ff x (fixed: host mask of ndof bytes, 1 = fixed, or
  *                       NULL; fixed rows of y are zero, fixed columns skipped);
  *   jfem_block_gram:    c_host (p x q column-major) = x' y over n rows;
- *   jfem_block_combine: x = y Q for a host p x q matrix Q (column-major). */
+ *   jfem_block_combine: x = y Q for a host p x q matrix Q (column-major), or
+ *                       x += y Q when accumulate != 0. */
 int jfem_matrix_spmv(jfem_ctx *ctx, const jfem_matrix *A, const uint8_t *fixed, int ncol,
                      const double *x_dev, double *y_dev);
 int jfem_block_gram(jfem_ctx *ctx, int64_t n, int p, const double *x_dev, int q,
                     const double *y_dev, double *c_host);
 int jfem_block_combine(jfem_ctx *ctx, int64_t n, int p, const double *y_dev, int q,
-                       const double *Q_host, double *x_dev);
+                       const double *Q_host, double *x_dev, int accumulate);
 
 /* ---- factor / solve -------------------------------------------------------- */
 typedef struct {

@@ -92,7 +92,7 @@ def load():
         "jfem_matrix_shift": ([vp, vp, dbl, vp, P(vp)], C.c_int),
         "jfem_matrix_spmv": ([vp, vp, vp, C.c_int, vp, vp], C.c_int),
         "jfem_block_gram": ([vp, i64, C.c_int, vp, C.c_int, vp, vp], C.c_int),
-        "jfem_block_combine": ([vp, i64, C.c_int, vp, C.c_int, vp, vp], C.c_int),
+        "jfem_block_combine": ([vp, i64, C.c_int, vp, C.c_int, vp, vp, C.c_int], C.c_int),
         "jfem_mesh_create": ([vp, i64, vp, vp, vp, i64, vp, vp, i64, vp, vp, i64, vp, vp, dbl, P(vp)], C.c_int),
         "jfem_mesh_update_props": ([vp, vp, vp, vp, vp], C.c_int),
         "jfem_mesh_free": ([vp], C.c_int),

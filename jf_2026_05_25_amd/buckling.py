@@ -18,9 +18,9 @@ inverse iteration: X <- (K + sigma Kg)^{-1} (Kg X) with the device factor and
 refined solves, Rayleigh-Ritz on the projected pencil (X'KX, -X'KgX) whose
 O(ndof) products run on the device (jfem_matrix_spmv, jfem_block_gram,
 jfem_block_combine); only the p x p projected eigenproblem is solved on the
-host.  The iteration stops when the wanted Ritz values change by less than
-``tol`` relative and their modes' residuals ||K x + lambda Kg x|| / ||K x|| are
-below ``res_tol``.
+host.  The iteration stops when the wanted modes' residuals
+||K x + lambda Kg x|| / ||K x|| (formed on the device from K Y and Kg Y) are below
+``res_tol`` and the wanted Ritz values have settled.
 """
 from __future__ import annotations
 
@@ -143,11 +143,11 @@ class _Blocks:
                                                      ptr(out)))
         return out.reshape(q, p).T
 
-    def combine(self, src, Q, dst):
+    def combine(self, src, Q, dst, accumulate=False):
         p, q = Q.shape
         Qc = np.ascontiguousarray(Q.T).ravel()       # column-major p x q
         check(self.ctx.h, self.ctx.L.jfem_block_combine(self.ctx.h, self.n, p, self.bufs[src].ptr, q, ptr(Qc),
-                                                        self.bufs[dst].ptr))
+                                                        self.bufs[dst].ptr, int(bool(accumulate))))
 
     def solve(self, F: solver.Factor, A, src, dst, ncol, tol, max_refine):
         iters = 0
@@ -193,13 +193,16 @@ def _ritz(Kr, Br):
 
 def buckling_roots(K: solver.DeviceMatrix, Kg: solver.DeviceMatrix, fixed: np.ndarray, factor: solver.Factor,
                    window: tuple, *, factor_matrix: solver.DeviceMatrix | None = None, block: int | None = None,
-                   tol: float = 1e-12, res_tol: float = 1e-8, max_iter: int = 500, solve_tol: float = 1e-13,
+                   tol: float = 1e-10, res_tol: float = 1e-9, max_iter: int = 500, solve_tol: float = 1e-13,
                    seed: int = 20260525) -> BucklingResult:
     """Positive roots of K_ff phi = lambda (-Kg_ff) phi in (lower, upper], at most
     ``quota`` (window = (lower, upper, quota)).  ``factor`` factors
     ``factor_matrix`` (default K), i.e. K + sigma Kg for a shift sigma below the
     first root, on the free set ``fixed == 0``; refined solves run against
-    ``factor_matrix``."""
+    ``factor_matrix``.  Converged when every wanted mode's residual
+    ||K x + lambda Kg x|| / ||K x|| is at most ``res_tol`` and the wanted Ritz
+    values, with the first one past the window or the quota, change by at most
+    ``tol`` relative between iterations."""
     t0 = time.perf_counter()
     ctx = K.ctx
     A = factor_matrix or K
@@ -215,6 +218,7 @@ def buckling_roots(K: solver.DeviceMatrix, Kg: solver.DeviceMatrix, fixed: np.nd
     X0 = rng.standard_normal((n, p))
     X0[~free] = 0.0
     B = _Blocks(ctx, n, p, fixed)
+    res = np.zeros(0)
     try:
         B.bufs["X"].upload(np.ascontiguousarray(X0.T))
         lam_prev = None
@@ -229,30 +233,30 @@ def buckling_roots(K: solver.DeviceMatrix, Kg: solver.DeviceMatrix, fixed: np.nd
             Br = -B.gram("Y", "GY", p, p)
             lam, mu, Q = _ritz(Kr, Br)
             B.combine("Y", Q, "X")                               # X'KX = I
-            inwin = np.where((lam > lower) & (lam <= upper))[0]
-            m = min(len(inwin), want)
-            # converged when the wanted values settle and the first Ritz value above
-            # the window (or beyond the quota) has settled as well
+            m = min(int(np.sum((lam > lower) & (lam <= upper))), want)
+            # residuals of the wanted Ritz pairs: K x = (K Y) q, Kg x = (Kg Y) q
+            if m:
+                Qm = Q[:, :m]
+                B.combine("KY", Qm, "W")
+                kk = np.diag(B.gram("W", "W", m, m)).copy()
+                B.combine("GY", Qm * lam[:m], "W", accumulate=True)
+                rr = np.diag(B.gram("W", "W", m, m))
+                res = np.sqrt(np.maximum(rr, 0.0) / np.maximum(kk, 1e-300))
+            else:
+                res = np.zeros(0)
+            # the wanted values and the first Ritz value past the window or the quota
             k_chk = min(m + 1, p - 1) if m < p else m
             if lam_prev is not None and k_chk > 0:
                 rel = np.abs(lam[:k_chk] - lam_prev[:k_chk]) / np.maximum(np.abs(lam[:k_chk]), 1e-300)
                 finite = np.isfinite(rel)
-                if np.all(rel[finite] <= tol):
+                if np.all(rel[finite] <= tol) and np.all(res <= res_tol):
                     converged = True
                     break
             lam_prev = lam
-        # residuals of the returned modes (device products of the final block)
-        m = min(int(np.sum((lam > lower) & (lam <= upper))), want)
-        B.spmv(K, "X", "KY", p)
-        B.spmv(Kg, "X", "GY", p)
-        KX = B.download("KY", p)[:, :m]
-        GX = B.download("GY", p)[:, :m]
         X = B.download("X", p)[:, :m]
     finally:
         B.free()
     vals = lam[:m].copy()
-    res = np.array([np.linalg.norm(KX[:, k] + vals[k] * GX[:, k]) / max(np.linalg.norm(KX[:, k]), 1e-300)
-                    for k in range(m)])
     # NORM = MAX over the analysis (free) set, largest component positive
     modes = X.copy()
     for k in range(m):
@@ -268,8 +272,8 @@ def buckling_roots(K: solver.DeviceMatrix, Kg: solver.DeviceMatrix, fixed: np.nd
             "ritz_values": [float(v) for v in lam],
             "complete": bool(np.any(lam[m:] > upper)) or m == want,
             "time_s": time.perf_counter() - t0}
-    if not converged or np.any(res > res_tol):
-        raise JfemError(8, f"buckling roots did not converge (residuals {res.tolist()})")
+    if not converged:
+        raise JfemError(8, f"buckling roots did not converge in {max_iter} iterations (residuals {res.tolist()})")
     return BucklingResult(vals, modes, diag)
 
 

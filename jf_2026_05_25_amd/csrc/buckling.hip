@@ -3,7 +3,7 @@
 // the device in FULL DOF numbering (ndof x p, column-major):
 //   Y = A X restricted to the free DOFs (fixed rows zero, fixed columns skipped),
 //   C = X' Y (p x q, summed in a fixed order, copied to the host),
-//   X = Y Q for a small host matrix Q (p x q).
+//   X = Y Q (or X += Y Q) for a small host matrix Q (p x q).
 // The reference extracts EIGB roots with a shifted sparse factorization and an
 // eigensolver on (K_ff, -Kg_ff) (_solve_eigb_buckling, src/solver/helpers.jl:
 // 280-420; solve_buckling, src/solver/solve_case.jl:2803); here the factor and
@@ -71,9 +71,11 @@ __global__ void __launch_bounds__(kBlkBS) gram_sum_kernel(int64_t nchunk, int pa
     C[e] = s;
 }
 
-// X[:, c] = sum_a Y[:, a] Q[a, c] (a in order)
+// X[:, c] = (accumulate ? X[:, c] : 0) + sum_a Y[:, a] Q[a, c] (a in order, the sum
+// formed first and added once)
 __global__ void __launch_bounds__(kBlkBS) block_combine_kernel(int64_t n, int p, int q, const double *__restrict__ Y,
-                                                               const double *__restrict__ Q, double *__restrict__ X) {
+                                                               const double *__restrict__ Q, double *__restrict__ X,
+                                                               int accumulate) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double y[kMaxCols];
@@ -81,7 +83,7 @@ __global__ void __launch_bounds__(kBlkBS) block_combine_kernel(int64_t n, int p,
     for (int c = 0; c < q; ++c) {
         double s = 0.0;
         for (int a = 0; a < p; ++a) s += y[a] * Q[a + p * c];
-        X[i + n * c] = s;
+        X[i + n * c] = accumulate ? X[i + n * c] + s : s;
     }
 }
 
@@ -136,7 +138,7 @@ int jfem_block_gram(jfem_ctx *ctx, int64_t n, int p, const double *x_dev, int q,
 }
 
 int jfem_block_combine(jfem_ctx *ctx, int64_t n, int p, const double *y_dev, int q, const double *Q_host,
-                       double *x_dev) {
+                       double *x_dev, int accumulate) {
     if (!ctx || !y_dev || !Q_host || !x_dev) return jfem::fail(ctx, JFEM_ERR_BAD_ARG, "null argument");
     if (n < 1 || p < 1 || q < 1 || p > kMaxCols || q > kMaxCols) return jfem::fail(ctx, JFEM_ERR_BAD_ARG, "block shape");
     if (x_dev == y_dev) return jfem::fail(ctx, JFEM_ERR_BAD_ARG, "x and y: distinct device blocks");
@@ -145,7 +147,8 @@ int jfem_block_combine(jfem_ctx *ctx, int64_t n, int p, const double *y_dev, int
     JFEM_CHECK_HIP(ctx, hipMalloc((void **)&d_Q, (size_t)(p * q) * sizeof(double)));
     hipError_t e = hipMemcpyAsync(d_Q, Q_host, (size_t)(p * q) * sizeof(double), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(block_combine_kernel, dim3(blocks(n)), dim3(kBlkBS), 0, s, n, p, q, y_dev, d_Q, x_dev);
+        hipLaunchKernelGGL(block_combine_kernel, dim3(blocks(n)), dim3(kBlkBS), 0, s, n, p, q, y_dev, d_Q, x_dev,
+                           accumulate);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -168,6 +168,32 @@ def flat_plate(nx: int = 40, ny: int = 40, Lx: float = 0.5, Ly: float = 0.5,
                  {"config": f"{nx}x{ny} CQUAD4 plate {Lx}x{Ly} m, t={t}, p={pressure} Pa"})
 
 
+def ss_plate_compression(n: int, size: float = 1.0, t: float = 0.01, E: float = 210e9, nu: float = 0.3,
+                         rigid_shear: bool = False) -> Model:
+    """Simply supported square plate (size x size x t, n x n CQUAD4) under a uniform
+    uniaxial compression sigma_x = -1 Pa: w = 0 on the four edges, u = 0 on x = 0,
+    v = 0 at the origin, consistent edge forces on x = size (the SOL 105 preload
+    of a Timoshenko-Gere buckling plate; first root 4 pi^2 D / (size^2 t))."""
+    m = flat_plate(n, n, Lx=size, Ly=size, E=E, nu=nu, t=t, name="ss_plate_compression")
+    xy = m.xyz
+    spc = np.zeros(m.ndof, np.uint8)
+    x0, x1 = np.isclose(xy[:, 0], 0.0), np.isclose(xy[:, 0], size)
+    y0, y1 = np.isclose(xy[:, 1], 0.0), np.isclose(xy[:, 1], size)
+    spc[6 * np.where(x0 | x1 | y0 | y1)[0] + 2] = 1
+    spc[6 * np.where(x0)[0]] = 1
+    spc[6 * np.where(x0 & y0)[0] + 1] = 1
+    F = np.zeros(m.ndof)
+    right = np.where(x1)[0]
+    w = np.where(y0[right] | y1[right], 0.5, 1.0) * (size / n)
+    F[6 * right] = -t * w
+    m.spc = spc
+    m.loads = F[:, None]
+    if rigid_shear:
+        m.quad_props[:, SP_RIGID] = 1.0
+    m.meta["config"] = f"simply supported {n}x{n} CQUAD4 plate {size}x{size} m, t={t}, sigma_x = -1 Pa"
+    return m
+
+
 # ---------------------------------------------------------------------------
 # C2: wing box (closed rectangular tube of four CQUAD4 faces)
 # ---------------------------------------------------------------------------

@@ -194,22 +194,10 @@ def tg_model(n=None):
     return dk, None
 
 
-def ss_plate(n, oracle):
+def ss_plate(n, oracle=None):
     """The 1 x 1 x 0.01 plate of the recorded case on an n x n mesh: w = 0 on the
     edges, u = 0 on x = 0, v = 0 at the origin, unit compression on x = 1."""
-    m = meshes.flat_plate(n, n, Lx=1.0, Ly=1.0, E=E, nu=NU, t=H)
-    xy = m.xyz
-    spc = np.zeros(m.ndof, np.uint8)
-    edge = (np.isclose(xy[:, 0], 0) | np.isclose(xy[:, 0], 1) | np.isclose(xy[:, 1], 0) | np.isclose(xy[:, 1], 1))
-    spc[6 * np.where(edge)[0] + 2] = 1
-    spc[6 * np.where(np.isclose(xy[:, 0], 0))[0]] = 1
-    spc[6 * np.where(np.isclose(xy[:, 0], 0) & np.isclose(xy[:, 1], 0))[0] + 1] = 1
-    F = np.zeros(m.ndof)
-    right = np.where(np.isclose(xy[:, 0], 1))[0]
-    w = np.where(np.isclose(xy[right, 1], 0) | np.isclose(xy[right, 1], 1), 0.5, 1.0) / n
-    F[6 * right] = -H * w                     # sigma_x = -1 Pa
-    m.spc = spc
-    m.loads = F[:, None]
+    m = meshes.ss_plate_compression(n, size=1.0, t=H, E=E, nu=NU)
     m.k6rot = 100.0
     return m
 
