@@ -234,6 +234,22 @@ int jfem_assemble_kg(jfem_ctx *ctx, const jfem_mesh *mesh, const jfem_pattern *p
  * a new matrix; otherwise *out (same pattern) is overwritten. */
 int jfem_matrix_shift(jfem_ctx *ctx, const jfem_matrix *K, double sigma, const jfem_matrix *Kg,
                       jfem_matrix **out);
+/* Inertia of A_ff on the symbolic analysis S of K (A on K's pattern, e.g. from
+ * jfem_matrix_shift or jfem_matrix_shift_scaled): the unpivoted signed
+ * factorization A_ff = L J L' (FP64, same launch sequence as the Cholesky),
+ * the number of negative pivots (the eigenvalues of A_ff below zero, by
+ * Sylvester's law of inertia) and the smallest and largest pivot magnitude.
+ * A zero or non-finite pivot returns JFEM_ERR_NOT_SPD with bad_column.  The
+ * reference counts buckling roots in an interval this way, with LDL' of the
+ * diagonally scaled shifted matrices (_eigb_interval_count,
+ * src/solver/helpers.jl:105-143). */
+int jfem_factor_inertia(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *A, int64_t *negative,
+                        double *min_pivot, double *max_pivot, int64_t *bad_column);
+/* out = D^{-1/2} (K + sigma Kg) D^{-1/2} with D the diagonal of K (entries with
+ * K_ii <= 0 scaled by 1): the scaled shifted matrix of that interval count.
+ * *out == NULL allocates; otherwise *out (same pattern) is overwritten. */
+int jfem_matrix_shift_scaled(jfem_ctx *ctx, const jfem_matrix *K, double sigma, const jfem_matrix *Kg,
+                             jfem_matrix **out);
 /* Block products of the eigen-extraction, device blocks in full DOF numbering
  * (ndof x ncol column-major, at most 32 columns), sums in a fixed order:
  *   jfem_matrix_spmv:   y = A_ff x (fixed: host mask of ndof bytes, 1 = fixed, or

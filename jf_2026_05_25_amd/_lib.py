@@ -38,7 +38,7 @@ EXPORTS = [
     "jfem_batch_create", "jfem_batch_update_props", "jfem_batch_assemble", "jfem_batch_matrix", "jfem_batch_factor",
     "jfem_batch_factor_view", "jfem_batch_free",
     "jfem_q4_kg_batch", "jfem_assemble_kg", "jfem_matrix_shift", "jfem_matrix_spmv", "jfem_block_gram",
-    "jfem_block_combine",
+    "jfem_block_combine", "jfem_factor_inertia", "jfem_matrix_shift_scaled",
 ]
 
 
@@ -93,6 +93,8 @@ def load():
         "jfem_matrix_spmv": ([vp, vp, vp, C.c_int, vp, vp], C.c_int),
         "jfem_block_gram": ([vp, i64, C.c_int, vp, C.c_int, vp, vp], C.c_int),
         "jfem_block_combine": ([vp, i64, C.c_int, vp, C.c_int, vp, vp, C.c_int], C.c_int),
+        "jfem_factor_inertia": ([vp, vp, vp, P(i64), P(dbl), P(dbl), P(i64)], C.c_int),
+        "jfem_matrix_shift_scaled": ([vp, vp, dbl, vp, P(vp)], C.c_int),
         "jfem_mesh_create": ([vp, i64, vp, vp, vp, i64, vp, vp, i64, vp, vp, i64, vp, vp, dbl, P(vp)], C.c_int),
         "jfem_mesh_update_props": ([vp, vp, vp, vp, vp], C.c_int),
         "jfem_mesh_free": ([vp], C.c_int),

@@ -13,7 +13,10 @@ unit largest free component (NORM = MAX).
 
 Here Kg is generated and assembled on the device on K's pattern
 (jfem_assemble_kg), K + sigma Kg is factored on K's symbolic analysis
-(jfem_matrix_shift + jfem_factor_create), and the roots come from subspace
+(jfem_matrix_shift + jfem_factor_create), the number of roots in a finite
+window comes from the inertia of the scaled shifted matrices at its ends
+(jfem_matrix_shift_scaled + jfem_factor_inertia: an unpivoted signed
+factorization on the same symbolic analysis), and the roots come from subspace
 inverse iteration: X <- (K + sigma Kg)^{-1} (Kg X) with the device factor and
 refined solves, Rayleigh-Ritz on the projected pencil (X'KX, -X'KgX) whose
 O(ndof) products run on the device (jfem_matrix_spmv, jfem_block_gram,
@@ -94,19 +97,77 @@ class GeometricStiffness(solver.DeviceMatrix):
 
 
 class ShiftedMatrix(solver.DeviceMatrix):
-    """K + sigma Kg on K's pattern (jfem_matrix_shift); ``shift`` overwrites it."""
+    """K + sigma Kg on K's pattern (jfem_matrix_shift), or with ``scaled`` its
+    diagonal scaling D^-1/2 (K + sigma Kg) D^-1/2, D = diag(K)
+    (jfem_matrix_shift_scaled); ``shift`` overwrites it."""
 
-    def __init__(self, K: solver.DeviceMatrix, sigma: float, Kg: solver.DeviceMatrix):
+    def __init__(self, K: solver.DeviceMatrix, sigma: float, Kg: solver.DeviceMatrix, scaled: bool = False):
         self.ctx = K.ctx
         self.pattern = K.pattern
         self.h = None
+        self.scaled = scaled
         self.shift(K, sigma, Kg)
 
     def shift(self, K, sigma, Kg):
         h = C.c_void_p(self.h.value if self.h is not None else None)
-        check(self.ctx.h, self.ctx.L.jfem_matrix_shift(self.ctx.h, K.h, float(sigma), Kg.h, C.byref(h)))
+        fn = self.ctx.L.jfem_matrix_shift_scaled if self.scaled else self.ctx.L.jfem_matrix_shift
+        check(self.ctx.h, fn(self.ctx.h, K.h, float(sigma), Kg.h, C.byref(h)))
         self.h = h
         self.sigma = float(sigma)
+
+
+def inertia(S: solver.Symbolic, A: solver.DeviceMatrix):
+    """(negative pivots, smallest, largest pivot magnitude) of the unpivoted signed
+    factorization of A_ff on the symbolic analysis S (jfem_factor_inertia)."""
+    ctx = A.ctx
+    neg = C.c_int64(0); mn = C.c_double(0); mx = C.c_double(0); bad = C.c_int64(-1)
+    check(ctx.h, ctx.L.jfem_factor_inertia(ctx.h, S.h, A.h, C.byref(neg), C.byref(mn), C.byref(mx), C.byref(bad)))
+    return int(neg.value), float(mn.value), float(mx.value)
+
+
+def interval_count(K: solver.DeviceMatrix, Kg: solver.DeviceMatrix, S: solver.Symbolic, lower: float,
+                   upper: float) -> int:
+    """Roots of K_ff phi = lambda (-Kg_ff) phi in (lower, upper], 0 <= lower <
+    upper < inf, by inertia: the negative pivots of D^-1/2 (K + sigma Kg) D^-1/2 at
+    the padded ends, at two paddings that must agree (_eigb_interval_count,
+    src/solver/helpers.jl:105-143).  Raises ValueError where the count is not
+    established (pivots within 64 eps of zero relative to the largest, unequal
+    counts at the two paddings)."""
+    if not (0.0 <= lower < upper < math.inf):
+        raise ValueError("the interval count requires finite 0 <= lower < upper")
+    scale = max(abs(lower), abs(upper), np.finfo(float).tiny)
+    pad = 128 * np.finfo(float).eps * scale
+    eps = np.finfo(float).eps
+    A = None
+
+    def at(sigma):
+        nonlocal A
+        if A is None:
+            A = ShiftedMatrix(K, sigma, Kg, scaled=True)
+        else:
+            A.shift(K, sigma, Kg)
+        neg, mn, mx = inertia(S, A)
+        if mn <= 64 * eps * mx:
+            raise ValueError("interval end within rounding of a root: unpivoted LDL' pivots are unsafe; "
+                             "move L1/L2 away from a root")
+        return neg
+
+    try:
+        counts = []
+        for mult in (1.0, 8.0):
+            a = max(0.0, lower - mult * pad)
+            b = upper + mult * pad
+            na = 0 if a == 0.0 else at(a)
+            nb = at(b)
+            if nb < na:
+                raise ValueError("positive-branch inertia count is inconsistent")
+            counts.append(nb - na)
+        if counts[0] != counts[1]:
+            raise ValueError("interval count is unstable near an endpoint; move L1/L2 away from a root")
+        return counts[0]
+    finally:
+        if A is not None:
+            A.free()
 
 
 def element_kg(mesh: solver.DeviceMesh, u) -> np.ndarray:
@@ -297,12 +358,26 @@ def solve_buckling(model, request: EigbRequest, F_static=None, *, ctx: solver.Co
     win = request.positive_window
     if win is None:
         raise NotImplementedError("EIGB windows without a positive branch (L2 <= 0) are not on this path")
+    lower, upper, quota = win
     Ash, Fsh = None, Fct
+    available = None
     try:
-        if shift != 0.0:
-            Ash = ShiftedMatrix(asm.K, shift, Kg)
-            Fsh = solver.Factor(S, Ash, solver.FP64, ctx)
-        res = buckling_roots(asm.K, Kg, fixed_e, Fsh, win, factor_matrix=Ash, **kw)
+        if math.isfinite(upper):
+            # the roots in the window by inertia; the target is NDP of them at most
+            available = interval_count(asm.K, Kg, S, lower, upper)
+            quota = available if quota is None else min(quota, available)
+        t_count = time.perf_counter()
+        if quota == 0:
+            res = BucklingResult(np.zeros(0), np.zeros((model.ndof, 0)),
+                                 {"free_dofs": int((fixed_e == 0).sum()), "iterations": 0})
+        else:
+            if shift != 0.0:
+                Ash = ShiftedMatrix(asm.K, shift, Kg)
+                Fsh = solver.Factor(S, Ash, solver.FP64, ctx)
+            res = buckling_roots(asm.K, Kg, fixed_e, Fsh, (lower, upper, quota), factor_matrix=Ash, **kw)
+            if available is not None and len(res.eigenvalues) != quota:
+                raise JfemError(8, f"EIGB SID={request.sid}: {len(res.eigenvalues)} of {quota} required roots "
+                                   f"recovered (interval inertia {available})")
     finally:
         if Fsh is not Fct:
             Fsh.free()
@@ -310,7 +385,9 @@ def solve_buckling(model, request: EigbRequest, F_static=None, *, ctx: solver.Co
             Ash.free()
         Kg.free()
     t3 = time.perf_counter()
+    res.diagnostics["interval_count"] = available
+    res.diagnostics["target_count"] = quota
     diag = {"static": sdiag, "phase_timings": {"static_reference_solve": t1 - t0, "kg_assembly": t2 - t1,
-                                               "buckling_eigensolve": t3 - t2},
+                                               "interval_count": t_count - t2, "buckling_eigensolve": t3 - t_count},
             "eigenvalue_extraction_method": request.method, "shift": shift}
     return res, u, diag

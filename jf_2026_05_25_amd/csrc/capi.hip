@@ -484,6 +484,38 @@ int jfem_matrix_shift(jfem_ctx *ctx, const jfem_matrix *K, double sigma, const j
     return JFEM_OK;
 }
 
+int jfem_matrix_shift_scaled(jfem_ctx *ctx, const jfem_matrix *K, double sigma, const jfem_matrix *G,
+                             jfem_matrix **out) {
+    if (!ctx || !K || !G || !out) return fail(ctx, JFEM_ERR_BAD_ARG, "null argument");
+    if (K->P != G->P || (*out && (*out)->P != K->P)) return fail(ctx, JFEM_ERR_BAD_ARG, "matrices of different patterns");
+    if (!std::isfinite(sigma)) return fail(ctx, JFEM_ERR_BAD_ARG, "shift is not finite");
+    const jfem_pattern *P = K->P;
+    jfem_matrix *A = *out;
+    if (!A) {
+        A = new jfem_matrix();
+        A->device = ctx->device;
+        A->P = P;
+        if (hipMalloc((void **)&A->d_val, std::max<int64_t>(P->nnz, 1) * sizeof(double)) != hipSuccess) {
+            jfem_matrix_free(A);
+            return fail(ctx, JFEM_ERR_OOM, "shifted matrix");
+        }
+    }
+    int32_t *d_dp = nullptr;
+    double *d_sd = nullptr;
+    hipError_t e = upload(&d_dp, P->diagpos.data(), P->diagpos.size(), ctx->stream);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_sd, std::max<int64_t>(P->ndof, 1) * sizeof(double));
+    if (e == hipSuccess) e = launch_csr_shift_scaled(ctx->stream, P, d_dp, d_sd, K->d_val, sigma, G->d_val, A->d_val);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (d_dp) (void)hipFree(d_dp);
+    if (d_sd) (void)hipFree(d_sd);
+    if (e != hipSuccess) {
+        if (!*out) jfem_matrix_free(A);
+        return fail(ctx, JFEM_ERR_HIP, hipGetErrorString(e));
+    }
+    *out = A;
+    return JFEM_OK;
+}
+
 // ------------------------------------------------------------------------- pattern
 int jfem_pattern_create(jfem_ctx *ctx, const jfem_mesh *m, jfem_pattern **out, int64_t *nnz_out) {
     *out = nullptr;

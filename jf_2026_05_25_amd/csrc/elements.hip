@@ -2091,6 +2091,26 @@ __global__ void csr_shift_kernel(int64_t n, const double *__restrict__ a, double
     if (i < n) c[i] = a[i] + sigma * b[i];
 }
 
+// the scale 1/sqrt(K_ii) per row (1 where the diagonal is absent or not positive)
+__global__ void diag_scale_kernel(int64_t n, const int32_t *__restrict__ diagpos, const double *__restrict__ a,
+                                  double *__restrict__ sd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t p = diagpos[i];
+    const double d = p >= 0 ? a[p] : 0.0;
+    sd[i] = d > 0.0 ? 1.0 / sqrt(d) : 1.0;
+}
+
+// c = s_i (a + sigma b) s_j over CSR rows, s the diagonal scale of a
+__global__ void csr_shift_scaled_kernel(int64_t n, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                        const double *__restrict__ a, double sigma, const double *__restrict__ b,
+                                        const double *__restrict__ sd, double *__restrict__ c) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double si = sd[i];
+    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) c[p] = (si * (a[p] + sigma * b[p])) * sd[col[p]];
+}
+
 // one thread per CQUAD4: its global Kg at Ke + 576 e (the layout the assembly sums)
 __global__ void __launch_bounds__(64)
 q4_kg_kernel(int64_t nq, const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
@@ -2171,6 +2191,17 @@ hipError_t launch_csr_shift(hipStream_t s, int64_t n, const double *a, double si
     if (n == 0) return hipSuccess;
     const int bs = 256;
     hipLaunchKernelGGL(csr_shift_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, n, a, sigma, b, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_csr_shift_scaled(hipStream_t s, const jfem_pattern *P, const int32_t *d_diagpos, double *d_sd,
+                                   const double *a, double sigma, const double *b, double *c) {
+    const int64_t n = P->ndof;
+    if (n == 0) return hipSuccess;
+    const int bs = 256;
+    hipLaunchKernelGGL(diag_scale_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, n, d_diagpos, a, d_sd);
+    hipLaunchKernelGGL(csr_shift_scaled_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, n, P->d_rowptr,
+                       P->d_col, a, sigma, b, d_sd, c);
     return hipGetLastError();
 }
 

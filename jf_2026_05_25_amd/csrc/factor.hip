@@ -350,7 +350,7 @@ __device__ inline double rsq_hw<double>(double x) { return rsqrt(x); }
 template <class T>
 struct PanelLds {
     static constexpr int kRow = 16 + 16 / (int)sizeof(T);          // padded panel row (elements)
-    static constexpr int bytes = 2 * 64 * kRow * (int)sizeof(T) + 16;
+    static constexpr int bytes = 2 * 64 * kRow * (int)sizeof(T) + 16 + 2 * 16 * (int)sizeof(T);
 };
 
 // A = L L' in place for the block (rows/columns >= b: identity).  lds: PanelLds<T>::bytes
@@ -358,35 +358,56 @@ struct PanelLds {
 // dimension ld, rows/columns < b) and the lower triangles of D_w^{-1} at their
 // positions in the b x b column-major block inverse at inv.  Returns the first
 // column with a non-positive pivot, b if none (uniform).
-template <class T>
-__device__ __forceinline__ int potrf_dinv(T (&a)[16], char *lds, int lane, int w, int b, T *Lblk, int64_t ld, T *inv) {
+// SG (signed): A = L J L' with J = diag(+-1) and L_jj = sqrt|d_j| > 0, J_j the sign
+// of the pivot d_j (the unpivoted LDL' of an indefinite matrix, D = J L_jj^2):
+// multipliers L_ij = J_j a_ij / L_jj, updates a_ic -= L_ij J_j L_cj; J of the
+// block's columns is written to Jblk[0..b).  A pivot of 0 or not finite is bad.
+template <class T, bool SG = false>
+__device__ __forceinline__ int potrf_dinv(T (&a)[16], char *lds, int lane, int w, int b, T *Lblk, int64_t ld, T *inv,
+                                          T *Jblk = nullptr) {
     constexpr int PS = PanelLds<T>::kRow;
     T (*P)[64][PS] = reinterpret_cast<T (*)[64][PS]>(lds);
     int *badw = reinterpret_cast<int *>(lds + 2 * 64 * PS * sizeof(T));
+    T (*Jp)[16] = reinterpret_cast<T (*)[16]>(lds + 2 * 64 * PS * sizeof(T) + 16);
     int bad = b;
     const int nb = (b + 15) / 16;
     for (int jb = 0; jb < nb; ++jb) {
         if (w == jb) {
+            T sg[16];
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) {
                 const int j = 16 * jb + jj;
                 const T d = bcast(a[jj], j);
-                const bool ok = d > T(0);
+                const bool ok = SG ? (d != T(0) && isfinite(d)) : d > T(0);
                 bad = (!ok && j < b && bad == b) ? j : bad;
-                const T dd = ok ? d : T(1);
+                const T sj = SG ? (d < T(0) ? T(-1) : T(1)) : T(1);
+                const T dd = ok ? (SG ? fabs(d) : d) : T(1);
                 const T r = rsq_hw(dd);
-                a[jj] = lane > j ? a[jj] * r : (lane == j ? dd * r : a[jj]);
+                sg[jj] = sj;
+                a[jj] = lane > j ? (SG ? a[jj] * r * sj : a[jj] * r) : (lane == j ? dd * r : a[jj]);
 #pragma unroll
-                for (int cc = jj + 1; cc < 16; ++cc) a[cc] -= a[jj] * bcast(a[jj], 16 * jb + cc);
+                for (int cc = jj + 1; cc < 16; ++cc)
+                    a[cc] -= (SG ? a[jj] * sj : a[jj]) * bcast(a[jj], 16 * jb + cc);
             }
 #pragma unroll
             for (int jj = 0; jj < 16; ++jj) P[jb & 1][lane][jj] = a[jj];
+            if (SG && lane < 16) {
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj)
+                    if (lane == jj) Jp[jb & 1][jj] = sg[jj];
+                if (16 * jb + lane < b) {
+                    T v = sg[0];
+#pragma unroll
+                    for (int jj = 1; jj < 16; ++jj) v = lane == jj ? sg[jj] : v;
+                    Jblk[16 * jb + lane] = v;
+                }
+            }
         }
         __syncthreads();
         if (w > jb) {
             T pl[16];
 #pragma unroll
-            for (int jj = 0; jj < 16; ++jj) pl[jj] = P[jb & 1][lane][jj];
+            for (int jj = 0; jj < 16; ++jj) pl[jj] = SG ? P[jb & 1][lane][jj] * Jp[jb & 1][jj] : P[jb & 1][lane][jj];
 #pragma unroll
             for (int cc = 0; cc < 16; ++cc) {
                 const T *pc = P[jb & 1][16 * w + cc];
@@ -436,12 +457,12 @@ __device__ __forceinline__ int potrf_dinv(T (&a)[16], char *lds, int lane, int w
 
 // factor the diagonal blocks of the first panels of a level's supernodes (one
 // workgroup per task; tasks are the level's step-0 diagonal panel tasks)
-template <class T>
+template <class T, bool SG = false>
 __global__ void __launch_bounds__(kBS) diag_factor_kernel(const PanelTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                           const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
                                                           const int64_t *__restrict__ iofs, const int64_t *__restrict__ col0,
                                                           T *__restrict__ L, T *__restrict__ Linv,
-                                                          int64_t *__restrict__ status, VarStride vs) {
+                                                          int64_t *__restrict__ status, VarStride vs, T *__restrict__ J) {
     __shared__ __attribute__((aligned(16))) char lds[PanelLds<T>::bytes];
     L += blockIdx.y * vs.L;
     Linv += blockIdx.y * vs.inv;
@@ -459,7 +480,9 @@ __global__ void __launch_bounds__(kBS) diag_factor_kernel(const PanelTask *__res
         const T v = Ls[(k + li) + ms * (k + min(c, b - 1))];
         a[cc] = (c <= lane && lane < b) ? v : (c == lane ? T(1) : T(0));
     }
-    const int bad = potrf_dinv(a, lds, lane, w, b, Ls + k + ms * k, ms, Linv + iofs[s] + (int64_t)kPanel * kPanel * (k / kPanel));
+    const int bad = potrf_dinv<T, SG>(a, lds, lane, w, b, Ls + k + ms * k, ms,
+                                      Linv + iofs[s] + (int64_t)kPanel * kPanel * (k / kPanel),
+                                      SG ? J + blockIdx.y * vs.col + col0[s] + k : nullptr);
     if (bad < b && tid == 0) report_min(status, blockIdx.y * vs.col + col0[s] + k + bad, JFEM_ERR_NOT_SPD);
 }
 
@@ -545,13 +568,15 @@ __global__ void __launch_bounds__(kBS) tile_solve_kernel(const PanelTask *__rest
 // (r0 == c0 == k + b < ns) also factors that block after its update (staged
 // through LDS, blocked over the four wavefronts), so the next step's tile solves
 // can start at once.
-template <class T>
+// SG (signed factorization): C -= W J W' with the panel's signs J on the column
+// operand, and the next diagonal block factored with signs.
+template <class T, bool SG = false>
 __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                      const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
                                                      const int64_t *__restrict__ uofs, const int64_t *__restrict__ iofs,
                                                      T *__restrict__ L, T *__restrict__ U, T *__restrict__ Linv,
                                                      int factor_next, const int64_t *__restrict__ col0,
-                                                     int64_t *__restrict__ status, VarStride vs) {
+                                                     int64_t *__restrict__ status, VarStride vs, T *__restrict__ J) {
     typedef typename Mfma<T>::v4 v4;
     L += blockIdx.y * vs.L;
     U += blockIdx.y * vs.U;
@@ -598,6 +623,14 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
                 const int i = min(wi + 16 * x + l16, nr - 1), j = min(wj + 16 * y + Mfma<T>::drow(l4, v), nc - 1);
                 cpre[x][y][v] = Cs[(r0 + i - off) + ld * (c0 + j - off)];
             }
+    if (SG) {
+        const T *Jk = J + blockIdx.y * vs.col + col0[s] + k;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int p = min((tid + kBS * u) / kTile, b - 1);
+            bv[u] *= Jk[p];
+        }
+    }
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
         const int q = tid + kBS * u, i = q % kTile, p = q / kTile;
@@ -640,8 +673,9 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
         const int c = 16 * wave + cc;
         a[cc] = (c <= lane && lane < bn) ? A[c][lane] : (c == lane ? T(1) : T(0));
     }
-    const int bad = potrf_dinv(a, reinterpret_cast<char *>(&B[0][0]), lane, wave, bn, L + lofs[s] + c0 + ms * c0, ms,
-                               Linv + iofs[s] + (int64_t)kPanel * c0);
+    const int bad = potrf_dinv<T, SG>(a, reinterpret_cast<char *>(&B[0][0]), lane, wave, bn, L + lofs[s] + c0 + ms * c0,
+                                      ms, Linv + iofs[s] + (int64_t)kPanel * c0,
+                                      SG ? J + blockIdx.y * vs.col + col0[s] + c0 : nullptr);
     if (bad < bn && tid == 0) report_min(status, blockIdx.y * vs.col + col0[s] + c0 + bad, JFEM_ERR_NOT_SPD);
 }
 
@@ -652,11 +686,12 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
 // current chunk's products.  MFMA operand layout and tile split over the four
 // wavefronts as in update_kernel; the products accumulate over the chunks in
 // order, so the result is reproducible bit for bit.
-template <class T>
+template <class T, bool SG = false>
 __global__ void __launch_bounds__(kBS) schur_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
                                                     const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
                                                     const int64_t *__restrict__ uofs, const T *__restrict__ L,
-                                                    T *__restrict__ U, VarStride vs) {
+                                                    T *__restrict__ U, VarStride vs, const int64_t *__restrict__ col0,
+                                                    const T *__restrict__ J) {
     typedef typename Mfma<T>::v4 v4;
     L += blockIdx.y * vs.L;
     U += blockIdx.y * vs.U;
@@ -685,6 +720,7 @@ __global__ void __launch_bounds__(kBS) schur_kernel(const UpdateTask *__restrict
                 cpre[x][y][v] = Cs[(r0 + i - ns) + ld * (c0 + j - ns)];
             }
     T av[16], bv[16];
+    const T *Js = SG ? J + blockIdx.y * vs.col + col0[s] : nullptr;
     auto load = [&](int64_t k) {
         const int b = (int)min<int64_t>(kPanel, ns - k);
 #pragma unroll
@@ -692,6 +728,7 @@ __global__ void __launch_bounds__(kBS) schur_kernel(const UpdateTask *__restrict
             const int q = tid + kBS * u, i = q % kTile, p = min(q / kTile, b - 1);
             av[u] = Ls[(r0 + min(i, nr - 1)) + ms * (k + p)];
             bv[u] = Ls[(c0 + min(i, nc - 1)) + ms * (k + p)];
+            if (SG) bv[u] *= Js[k + p];
         }
     };
     v4 acc[2][2];
@@ -2034,13 +2071,47 @@ __global__ void diag_ratio_kernel(int64_t n, const int32_t *__restrict__ perm, c
     }
 }
 
+// inertia of a signed factorization: the number of negative signs and the
+// smallest and largest pivot magnitude D_kk = L_kk^2, one workgroup, fixed
+// reduction tree (out: count, min, max)
+__global__ void __launch_bounds__(1024) inertia_kernel(int64_t n, const int32_t *__restrict__ pos2sn,
+                                                       const int64_t *__restrict__ col0, const int64_t *__restrict__ mrow,
+                                                       const int64_t *__restrict__ lofs, const double *__restrict__ L,
+                                                       const double *__restrict__ J, double *__restrict__ out) {
+    __shared__ double sc[1024], smin[1024], smax[1024];
+    double cnt = 0.0, mn = INFINITY, mx = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) {
+        const int s = pos2sn[i];
+        const int64_t j = i - col0[s];
+        const double l = L[lofs[s] + j + mrow[s] * j];
+        const double d = l * l;
+        cnt += J[i] < 0.0 ? 1.0 : 0.0;
+        mn = fmin(mn, d);
+        mx = fmax(mx, d);
+    }
+    sc[threadIdx.x] = cnt; smin[threadIdx.x] = mn; smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sc[threadIdx.x] += sc[threadIdx.x + w];
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = sc[0]; out[1] = smin[0]; out[2] = smax[0]; }
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBS - 1) / kBS, 65536)); }
 
 // numeric factorization of V matrices sharing S (K values, factors and update
 // matrices of variant v at v times the strides in vs), in one launch sequence
-template <class T>
+// SG: the signed factorization A = L J L' (J into Jv, one sign per elimination
+// position); solve_blocks: also the inverse blocks and chain products the
+// triangular solves need
+template <class T, bool SG = false>
 hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T *L, T *U, T *Linv, int V,
-                   VarStride vs) {
+                   VarStride vs, T *Jv = nullptr, bool solve_blocks = true) {
     hipStream_t st = ctx->stream;
     const SymbolicHost &H = S->H;
     const bool detail = ctx->timing_level >= 2;
@@ -2072,8 +2143,9 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
             const int64_t stp = H.step_ptr[l];
             const int64_t nd = H.pstep_mid[stp] - H.pstep_off[stp];
             if (nd) {
-                hipLaunchKernelGGL(diag_factor_kernel<T>, dim3((unsigned)nd, V), dim3(kBS), 0, st, S->d_ptasks + H.pstep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, S->d_iofs, S->d_col0, L, Linv, ctx->d_status, vs);
+                hipLaunchKernelGGL((diag_factor_kernel<T, SG>), dim3((unsigned)nd, V), dim3(kBS), 0, st,
+                                   S->d_ptasks + H.pstep_off[stp], S->d_ncol, S->d_m, S->d_lofs, S->d_iofs, S->d_col0, L,
+                                   Linv, ctx->d_status, vs, Jv);
                 lap(FAM_PANEL);
             }
         }
@@ -2086,22 +2158,23 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
                 lap(FAM_PANEL);
             }
             if (nu) {
-                hipLaunchKernelGGL(update_kernel<T>, dim3((unsigned)nu, V), dim3(kBS), 0, st, S->d_utasks + H.ustep_off[stp],
-                                   S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, S->d_iofs, L, U, Linv, 1, S->d_col0,
-                                   ctx->d_status, vs);
+                hipLaunchKernelGGL((update_kernel<T, SG>), dim3((unsigned)nu, V), dim3(kBS), 0, st,
+                                   S->d_utasks + H.ustep_off[stp], S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, S->d_iofs, L,
+                                   U, Linv, 1, S->d_col0, ctx->d_status, vs, Jv);
                 lap(FAM_UPDATE);
             }
         }
         // the level's update matrices, with the full rank of each front
         const int64_t nsch = H.sstep_off[l + 1] - H.sstep_off[l];
         if (nsch) {
-            hipLaunchKernelGGL(schur_kernel<T>, dim3((unsigned)nsch, V), dim3(kBS), 0, st, S->d_stasks + H.sstep_off[l],
-                               S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, (const T *)L, U, vs);
+            hipLaunchKernelGGL((schur_kernel<T, SG>), dim3((unsigned)nsch, V), dim3(kBS), 0, st,
+                               S->d_stasks + H.sstep_off[l], S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, (const T *)L, U, vs,
+                               S->d_col0, (const T *)Jv);
             lap(FAM_SCHUR);
         }
     }
     // inverse diagonal blocks and chain products for the triangular solves
-    if (!H.flow_b.empty()) {
+    if (solve_blocks && !H.flow_b.empty()) {
         hipLaunchKernelGGL(diag_inverse_kernel<T>, dim3((unsigned)H.flow_b.size(), V), dim3(64), 0, st, S->d_flow_b,
                            (const T *)L, Linv, vs);
         hipLaunchKernelGGL(chain_product_kernel<T>, dim3((unsigned)H.flow_b.size(), V), dim3(kBS), 0, st, S->d_flow_b,
@@ -2459,6 +2532,72 @@ int jfem_factor_small_pivots(jfem_ctx *ctx, const jfem_factor *F, const jfem_mat
     return JFEM_OK;
 }
 
+// Inertia of A_ff (A on the pattern of S, typically K + sigma Kg): the unpivoted
+// signed factorization A_ff = L J L' in FP64 on the symbolic analysis S (the same
+// launch sequence as the Cholesky, with the pivots' signs carried through the
+// updates), the number of negative pivots, and the smallest and largest pivot
+// magnitude.  The reference's interval count factors its (diagonally scaled)
+// shifted matrices with an unpivoted LDL' and counts negative pivots
+// (_eigb_interval_count, src/solver/helpers.jl:105-143).
+int jfem_factor_inertia(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *A, int64_t *negative,
+                        double *min_pivot, double *max_pivot, int64_t *bad_column) {
+    if (bad_column) *bad_column = -1;
+    if (!ctx || !S || !A || !negative) return fail(ctx, JFEM_ERR_BAD_ARG, "null argument");
+    if (A->P != S->P) return fail(ctx, JFEM_ERR_BAD_ARG, "matrix and symbolic use different patterns");
+    const SymbolicHost &H = S->H;
+    *negative = 0;
+    if (H.n == 0) {
+        if (min_pivot) *min_pivot = 0.0;
+        if (max_pivot) *max_pivot = 0.0;
+        return JFEM_OK;
+    }
+    std::vector<int32_t> pos2sn(H.n);
+    for (int64_t s = 0; s < H.ns; ++s)
+        for (int64_t j = 0; j < H.ncol[s]; ++j) pos2sn[H.col0[s] + j] = (int32_t)s;
+    double *dL = nullptr, *dU = nullptr, *dI = nullptr, *dJ = nullptr, *dOut = nullptr;
+    int32_t *dP = nullptr;
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipMalloc((void **)&dL, std::max<int64_t>(H.L_entries, 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&dU, std::max<int64_t>(H.U_entries, 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&dI, std::max<int64_t>(H.inv_entries, 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&dJ, H.n * sizeof(double) + 3 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&dP, H.n * sizeof(int32_t));
+    auto release = [&]() {
+        for (void *p : {(void *)dL, (void *)dU, (void *)dI, (void *)dJ, (void *)dP})
+            if (p) (void)hipFree(p);
+    };
+    if (e != hipSuccess) {
+        release();
+        return fail(ctx, JFEM_ERR_OOM, "inertia factor storage");
+    }
+    dOut = dJ + H.n;
+    e = hipMemcpyAsync(dP, pos2sn.data(), H.n * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_reset_status(st, ctx->d_status);
+    if (e == hipSuccess) e = numeric<double, true>(ctx, S, A->d_val, dL, dU, dI, 1, VarStride{}, dJ, false);
+    int64_t stw[2] = {0, 0};
+    double out[3] = {0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(stw, ctx->d_status, sizeof(stw), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && stw[1] == 0) {
+        hipLaunchKernelGGL(inertia_kernel, dim3(1), dim3(1024), 0, st, H.n, dP, S->d_col0, S->d_m, S->d_lofs, dL, dJ,
+                           dOut);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(out, dOut, sizeof(out), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    release();
+    if (e != hipSuccess) return fail(ctx, JFEM_ERR_HIP, hipGetErrorString(e));
+    if (stw[1] != 0) {
+        const int64_t g = (stw[0] >= 0 && stw[0] < H.n) ? H.perm[stw[0]] : -1;
+        if (bad_column) *bad_column = g;
+        return fail(ctx, JFEM_ERR_NOT_SPD, "zero or non-finite pivot at DOF " + std::to_string(g));
+    }
+    *negative = (int64_t)out[0];
+    if (min_pivot) *min_pivot = out[1];
+    if (max_pivot) *max_pivot = out[2];
+    return JFEM_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -2807,10 +2946,10 @@ int jfem_selftest_update(jfem_ctx *ctx, int precision, double *max_rel_diff) {
     }
     dim3 g((unsigned)tasks.size());
     if (precision == 64) {
-        hipLaunchKernelGGL(update_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, d_meta + 2, (double *)dL[0], (double *)dU[0], (double *)nullptr, 0, d_meta, ctx->d_status, VarStride{});
+        hipLaunchKernelGGL(update_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, d_meta + 2, (double *)dL[0], (double *)dU[0], (double *)nullptr, 0, d_meta, ctx->d_status, VarStride{}, (double *)nullptr);
         hipLaunchKernelGGL(update_ref_kernel<double>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (double *)dL[1], (double *)dU[1]);
     } else {
-        hipLaunchKernelGGL(update_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, d_meta + 2, (float *)dL[0], (float *)dU[0], (float *)nullptr, 0, d_meta, ctx->d_status, VarStride{});
+        hipLaunchKernelGGL(update_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, d_meta + 2, (float *)dL[0], (float *)dU[0], (float *)nullptr, 0, d_meta, ctx->d_status, VarStride{}, (float *)nullptr);
         hipLaunchKernelGGL(update_ref_kernel<float>, g, dim3(kBS), 0, ctx->stream, d_tasks, d_meta, d_meta + 1, d_meta + 2, d_meta + 3, (float *)dL[1], (float *)dU[1]);
     }
     JFEM_CHECK_HIP(ctx, hipGetLastError());

@@ -15,6 +15,11 @@ buckling results.
   recorded eigenvalues, and modes of MAC >= 0.9995 with the recorded ones; the
   same roots from a factorization of K + sigma Kg shifted toward them.
 * A 16 x 16 plate: the first six roots within 1e-9 of the dense eigensolution.
+* Inertia (jfem_factor_inertia, the unpivoted signed factorization on K's
+  symbolic analysis): the number of negative pivots of the scaled K + sigma Kg
+  equals the number of negative eigenvalues of the dense K_ff + sigma Kg_ff
+  (Sylvester's law) for shifts below the first root, between roots and past the
+  seventh; the EIGB interval count of the recorded plate is its seven roots.
 """
 import json
 import os
@@ -138,6 +143,7 @@ def test_recorded_plate(ctx, oracle):
     assert np.linalg.norm(u - us) <= 1e-6 * np.linalg.norm(us)
     ref = np.array(g["eigenvalues"])
     assert len(r.eigenvalues) == len(ref)             # all roots in (0, 1e9] (NDP = 12 > 7)
+    assert r.diagnostics["interval_count"] == len(ref) and r.diagnostics["target_count"] == len(ref)
     # oracle: dense eigensolution of the same pencil
     m = dk.model(1)
     m.loads = oracle.load_vector(m, **dk.load_cards(1))[:, None]
@@ -173,3 +179,31 @@ def test_plate_16_vs_dense(ctx, oracle):
     lam, _, _, _ = buckle(m, oracle)
     r, u, diag = buckling.solve_buckling(m, buckling.EigbRequest(1, "INV", 0.0, 10 * lam[5], 2, 6), ctx=ctx)
     np.testing.assert_allclose(r.eigenvalues, lam[:6], rtol=1e-9)
+
+
+def test_inertia_matches_dense(ctx, oracle):
+    import scipy.linalg as sl
+    from jf_2026_05_25_amd import buckling, solver
+    from test_oracle_buckling import buckle, ss_plate
+    m = ss_plate(8)
+    lam, _, u, fixed = buckle(m, oracle)
+    asm = solver.assemble_stiffness(m, ctx)
+    fx = solver.bc_partition(asm.K, m.spc)
+    assert np.array_equal(fx, fixed)
+    S = solver.Symbolic(asm.K, asm.mesh, fx, ctx)
+    G = buckling.GeometricStiffness(asm, u)
+    K, Gs = asm.K.to_scipy(), G.to_scipy()
+    f = np.where(fx == 0)[0]
+    Kf, Gf = K[f][:, f].toarray(), Gs[f][:, f].toarray()
+    for sigma in (0.0, 0.5 * lam[0], 0.5 * (lam[0] + lam[1]), 0.5 * (lam[3] + lam[4]), 1.5 * lam[6]):
+        A = buckling.ShiftedMatrix(asm.K, sigma, G, scaled=True)
+        neg, mn, mx = buckling.inertia(S, A)
+        A.free()
+        M = Kf + sigma * Gf
+        exp = int(np.sum(sl.eigvalsh(0.5 * (M + M.T)) < 0))
+        assert neg == exp == int(np.sum(lam < sigma)), (sigma, neg, exp)
+        assert 0 < mn <= mx and np.isfinite(mx)
+    # the interval count of (0, between roots 5 and 6]
+    assert buckling.interval_count(asm.K, G, S, 0.0, 0.5 * (lam[4] + lam[5])) == 5
+    assert buckling.interval_count(asm.K, G, S, 0.5 * (lam[0] + lam[1]), 0.5 * (lam[4] + lam[5])) == 4
+    G.free(); S.free()
