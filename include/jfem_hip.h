@@ -30,7 +30,9 @@ typedef enum {
     JFEM_ERR_HIP = 1,             /* HIP runtime error */
     JFEM_ERR_DEGENERATE = 2,      /* degenerate/folded shell: ArgumentError, src/FEMKernels.jl:2862-2867, 4244-4251 */
     JFEM_ERR_INVERTED = 3,        /* non-positive solid Jacobian: DomainError, src/tetra10.jl:31-35 */
-    JFEM_ERR_WARPED = 4,          /* reserved (warped CQUAD4s are handled by the finite-warp congruence) */
+    JFEM_ERR_WARPED = 4,          /* CQUAD4 outside the flat-shell differential-stiffness operator (corner
+                                     deviation >= 1e-2 of the longer diagonal, src/solver/assembly.jl:6575-6594);
+                                     the stiffness path handles warped cells by the finite-warp congruence */
     JFEM_ERR_BAD_ARG = 5,         /* bad sizes / material: ArgumentError, src/tetra10.jl:55-58 */
     JFEM_ERR_NOT_SPD = 6,         /* factorization hit a non-positive pivot: PosDefException */
     JFEM_ERR_OOM = 7,             /* device allocation failed */
