@@ -212,21 +212,25 @@ int jfem_shell_recover(jfem_ctx *ctx, const jfem_mesh *mesh, const double *u, in
                        double *q4_out, double *t3_out, int64_t *bad_element);
 
 /* ---- SOL 105: differential stiffness, shifted matrices, block products ------ */
-/* Differential (geometric) stiffness of the CQUAD4s of a mesh under the membrane
- * state of the preload displacements u (host, ndof doubles, global DOF order):
- * per element the default operator of a flat isotropic CQUAD4,
+/* Differential (geometric) stiffness of the shells of a mesh under the membrane
+ * state of the preload displacements u (host, ndof doubles, global DOF order).
+ * CQUAD4: the default operator of a flat isotropic CQUAD4,
  * geometric_stiffness_quad4_reference_kdjj_iso (src/FEMKernels.jl:8810-8942),
  * with the assembly's preload map, warp congruence and element-frame transform
  * (src/solver/assembly.jl:6985-6998, 8063-8075, 8264-8320).  A cell whose corners
  * leave the plane of its diagonals by 1e-2 of the longer diagonal or more is not
  * on this operator (assembly.jl:6575-6594): JFEM_ERR_WARPED, bad_element its
- * index.  Meshes with CTRIA3 or CTETRA10 elements return JFEM_ERR_BAD_ARG.
- * jfem_q4_kg_batch: per element the global 24x24 matrix (column-major, nq x 576
- * doubles, host).  jfem_assemble_kg: the assembled Kg on the pattern of K
- * (assemble_geometric_stiffness, src/solver/assembly.jl:6106; duplicates summed
- * in element order), a jfem_matrix like K's. */
-int jfem_q4_kg_batch(jfem_ctx *ctx, const jfem_mesh *mesh, const double *u, double *kg_out,
-                     int64_t *bad_element);
+ * index.  CTRIA3: geometric_stiffness_tria3 in the default principal-transverse
+ * mode (src/FEMKernels.jl:11142-11288, 8205-8338) under the membrane resultants
+ * of stress_strain_tria3 (src/FEMKernels.jl:9859-9937), transformed to global
+ * (src/solver/assembly.jl:8343-8498).  Meshes with CTETRA10 elements return
+ * JFEM_ERR_BAD_ARG.  jfem_shell_kg_batch: per element the global matrix
+ * (column-major, host): nq x 576 doubles of the CQUAD4s, then nt x 324 of the
+ * CTRIA3s; bad_element counts in that order.  jfem_assemble_kg: the assembled Kg
+ * on the pattern of K (assemble_geometric_stiffness, src/solver/assembly.jl:6106;
+ * duplicates summed in element order), a jfem_matrix like K's. */
+int jfem_shell_kg_batch(jfem_ctx *ctx, const jfem_mesh *mesh, const double *u, double *kg_out,
+                        int64_t *bad_element);
 int jfem_assemble_kg(jfem_ctx *ctx, const jfem_mesh *mesh, const jfem_pattern *pattern,
                      const double *u, jfem_matrix **out, int64_t *bad_element);
 /* out = K + sigma * Kg (values of two matrices on one pattern): the shifted

@@ -37,7 +37,7 @@ EXPORTS = [
     "jfem_loadset_create", "jfem_loadset_set_gravity", "jfem_load_vector", "jfem_load_vector_dev", "jfem_loadset_free",
     "jfem_batch_create", "jfem_batch_update_props", "jfem_batch_assemble", "jfem_batch_matrix", "jfem_batch_factor",
     "jfem_batch_factor_view", "jfem_batch_free",
-    "jfem_q4_kg_batch", "jfem_assemble_kg", "jfem_matrix_shift", "jfem_matrix_spmv", "jfem_block_gram",
+    "jfem_shell_kg_batch", "jfem_assemble_kg", "jfem_matrix_shift", "jfem_matrix_spmv", "jfem_block_gram",
     "jfem_block_combine", "jfem_factor_inertia", "jfem_matrix_shift_scaled",
 ]
 
@@ -87,7 +87,7 @@ def load():
         "jfem_t3_local_batch": ([vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, P(i64)], C.c_int),
         "jfem_tet10_batch": ([vp, i64, vp, vp, vp, vp, P(i64)], C.c_int),
         "jfem_shell_recover": ([vp, vp, vp, C.c_int, vp, vp, P(i64)], C.c_int),
-        "jfem_q4_kg_batch": ([vp, vp, vp, vp, P(i64)], C.c_int),
+        "jfem_shell_kg_batch": ([vp, vp, vp, vp, P(i64)], C.c_int),
         "jfem_assemble_kg": ([vp, vp, vp, vp, P(vp), P(i64)], C.c_int),
         "jfem_matrix_shift": ([vp, vp, dbl, vp, P(vp)], C.c_int),
         "jfem_matrix_spmv": ([vp, vp, vp, C.c_int, vp, vp], C.c_int),

@@ -81,7 +81,7 @@ class EigbRequest:
 
 
 class GeometricStiffness(solver.DeviceMatrix):
-    """Kg of the CQUAD4s for a preload u (device CSR values on K's pattern)."""
+    """Kg of the shells (CQUAD4, CTRIA3) for a preload u (device CSR values on K's pattern)."""
 
     def __init__(self, asm: solver.AssembledStiffness, u, ctx: solver.Context | None = None):
         self.ctx = ctx or asm.K.ctx
@@ -170,17 +170,20 @@ def interval_count(K: solver.DeviceMatrix, Kg: solver.DeviceMatrix, S: solver.Sy
             A.free()
 
 
-def element_kg(mesh: solver.DeviceMesh, u) -> np.ndarray:
-    """Per-CQUAD4 global differential stiffness (n_quad4, 24, 24) for displacements u."""
+def element_kg(mesh: solver.DeviceMesh, u) -> tuple[np.ndarray, np.ndarray]:
+    """Per-element global differential stiffness for displacements u:
+    (n_quad4, 24, 24) of the CQUAD4s and (n_tria3, 18, 18) of the CTRIA3s."""
     ctx = mesh.ctx
-    nq = mesh.model.quads.shape[0]
-    out = np.zeros((max(nq, 1), 576))
+    nq, nt = mesh.model.quads.shape[0], mesh.model.trias.shape[0]
+    out = np.zeros(max(576 * nq + 324 * nt, 1))
     u = np.ascontiguousarray(np.asarray(u, float).reshape(-1))
     bad = C.c_int64(-1)
-    code = ctx.L.jfem_q4_kg_batch(ctx.h, mesh.h, ptr(u), ptr(out), C.byref(bad))
+    code = ctx.L.jfem_shell_kg_batch(ctx.h, mesh.h, ptr(u), ptr(out), C.byref(bad))
     if code:
         check(ctx.h, code)
-    return out[:nq].reshape(nq, 24, 24).transpose(0, 2, 1).copy()
+    kq = out[:576 * nq].reshape(nq, 24, 24).transpose(0, 2, 1).copy()
+    kt = out[576 * nq:576 * nq + 324 * nt].reshape(nt, 18, 18).transpose(0, 2, 1).copy()
+    return kq, kt
 
 
 class _Blocks:

@@ -389,15 +389,15 @@ int jfem_shell_recover(jfem_ctx *ctx, const jfem_mesh *m, const double *u, int m
 
 // ------------------------------------------------------------------ geometric stiffness
 namespace {
-// Kg of every CQUAD4 into Ke (device, nq x 576) for host displacements u
+// Kg of every shell into Ke (device, nq x 576 then nt x 324) for host displacements u
 int run_kg(jfem_ctx *ctx, const jfem_mesh *m, const double *u, double *Ke, int64_t *bad) {
     if (bad) *bad = -1;
-    if (m->nt || m->ntet) return fail(ctx, JFEM_ERR_BAD_ARG, "geometric stiffness covers CQUAD4 meshes only");
+    if (m->ntet) return fail(ctx, JFEM_ERR_BAD_ARG, "geometric stiffness covers shell meshes only");
     hipStream_t s = ctx->stream;
     double *d_u = nullptr;
     hipError_t e = upload(&d_u, u, (size_t)(6 * m->n_nodes), s);
     if (e == hipSuccess) e = launch_reset_status(s, ctx->d_status);
-    if (e == hipSuccess) e = launch_q4_kg(s, m, d_u, Ke, ctx->d_status);
+    if (e == hipSuccess) e = launch_shell_kg(s, m, d_u, Ke, ctx->d_status);
     int64_t st[2] = {0, 0};
     if (e == hipSuccess) e = hipMemcpyAsync(st, ctx->d_status, sizeof(st), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -411,17 +411,18 @@ int run_kg(jfem_ctx *ctx, const jfem_mesh *m, const double *u, double *Ke, int64
 }
 }  // namespace
 
-int jfem_q4_kg_batch(jfem_ctx *ctx, const jfem_mesh *m, const double *u, double *kg_out, int64_t *bad) {
+int jfem_shell_kg_batch(jfem_ctx *ctx, const jfem_mesh *m, const double *u, double *kg_out, int64_t *bad) {
     if (bad) *bad = -1;
-    if (!ctx || !m || !u || (m->nq && !kg_out)) return fail(ctx, JFEM_ERR_BAD_ARG, "null argument");
-    if (m->nq == 0) return m->nt || m->ntet ? fail(ctx, JFEM_ERR_BAD_ARG, "geometric stiffness covers CQUAD4 meshes only")
-                                            : JFEM_OK;
+    const size_t count = (size_t)m->nq * 576 + (size_t)m->nt * 324;
+    if (!ctx || !m || !u || (count && !kg_out)) return fail(ctx, JFEM_ERR_BAD_ARG, "null argument");
+    if (m->ntet) return fail(ctx, JFEM_ERR_BAD_ARG, "geometric stiffness covers shell meshes only");
+    if (count == 0) return JFEM_OK;
     double *d_kg = nullptr;
-    if (hipMalloc((void **)&d_kg, (size_t)m->nq * 576 * sizeof(double)) != hipSuccess)
+    if (hipMalloc((void **)&d_kg, count * sizeof(double)) != hipSuccess)
         return fail(ctx, JFEM_ERR_OOM, "element matrices");
     int st = run_kg(ctx, m, u, d_kg, bad);
     if (st == JFEM_OK) {
-        hipError_t e = hipMemcpyAsync(kg_out, d_kg, (size_t)m->nq * 576 * sizeof(double), hipMemcpyDeviceToHost,
+        hipError_t e = hipMemcpyAsync(kg_out, d_kg, count * sizeof(double), hipMemcpyDeviceToHost,
                                       ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) st = fail(ctx, JFEM_ERR_HIP, hipGetErrorString(e));

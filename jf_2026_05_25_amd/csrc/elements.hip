@@ -2006,6 +2006,28 @@ __device__ void kg_warp_map(const double *lc, const double *zl, double *Tg) {
     }
 }
 
+// out = R' blk R for every 3 x 3 block of the N x N element-frame matrix Kl (R rows
+// = v1, v2, v3: the frame axes in global coordinates)
+template <int N>
+__device__ void kg_rotate_blocks(const double *Kl, const double *R, double *out) {
+    for (int B = 0; B < N / 3; ++B)
+        for (int A = 0; A < N / 3; ++A) {
+            double t[9];
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    double s = 0;
+                    for (int l = 0; l < 3; ++l) s += Kl[(3 * A + i) + N * (3 * B + l)] * R[l * 3 + j];
+                    t[i + 3 * j] = s;
+                }
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    double s = 0;
+                    for (int l = 0; l < 3; ++l) s += R[l * 3 + i] * t[l + 3 * j];
+                    out[(3 * A + i) + N * (3 * B + j)] = s;
+                }
+        }
+}
+
 // global 24 x 24 Kg of one CQUAD4 (column-major into out); returns a status code
 __device__ int q4_kg(const double *p, const double *pr, const double *ug, double *out) {
     {   // flat-element gate: corner deviation below 1e-2 of the longer diagonal
@@ -2063,24 +2085,91 @@ __device__ int q4_kg(const double *p, const double *pr, const double *ug, double
                 Kl[j + 24 * i] = s;
             }
     }
-    // R' * blk * R per 3 x 3 block, R rows = (v1, v2, v3)
-    const double *R = P.R;
-    for (int B = 0; B < 8; ++B)
-        for (int A = 0; A < 8; ++A) {
-            double t[9];
-            for (int j = 0; j < 3; ++j)
-                for (int i = 0; i < 3; ++i) {
-                    double s = 0;
-                    for (int l = 0; l < 3; ++l) s += Kl[(3 * A + i) + 24 * (3 * B + l)] * R[l * 3 + j];
-                    t[i + 3 * j] = s;
-                }
-            for (int j = 0; j < 3; ++j)
-                for (int i = 0; i < 3; ++i) {
-                    double s = 0;
-                    for (int l = 0; l < 3; ++l) s += R[l * 3 + i] * t[l + 3 * j];
-                    out[(3 * A + i) + 24 * (3 * B + j)] = s;
+    kg_rotate_blocks<24>(Kl, P.R, out);
+    return 0;
+}
+
+// principal stresses of (sxx, syy, sxy) and their directions (c, s), the second
+// direction (-s1, c1): principal_stress_2d_components (src/FEMKernels.jl:8090-8101)
+__device__ void kg_principal(double sxx, double syy, double sxy, double *lam, double *dc, double *ds) {
+    const double mean_s = 0.5 * (sxx + syy), half_d = 0.5 * (sxx - syy);
+    const double radius = sqrt(half_d * half_d + sxy * sxy);
+    if (radius <= 1e-30) {
+        lam[0] = mean_s; dc[0] = 1.0; ds[0] = 0.0;
+        lam[1] = mean_s; dc[1] = 0.0; ds[1] = 1.0;
+        return;
+    }
+    const double theta = 0.5 * atan2(2.0 * sxy, sxx - syy);
+    const double c = cos(theta), s = sin(theta);
+    lam[0] = mean_s + radius; dc[0] = c; ds[0] = s;
+    lam[1] = mean_s - radius; dc[1] = -s; ds[1] = c;
+}
+
+// element-frame CTRIA3 Kg (18 x 18, column-major) for membrane stresses sig on a
+// triangle of local corners lc: per node pair, the in-plane DOFs take the principal
+// directions of each stress component alone and w takes those of the whole stress
+// (geometric_stiffness_tria3 in principal-transverse mode, src/FEMKernels.jl:
+// 11142-11288 with 8205-8338); drill rows stay zero
+__device__ void t3_kg_local(const double *lc, const double *sig, double h, double *Kl) {
+    for (int q = 0; q < 324; ++q) Kl[q] = 0.0;
+    if (h < 1e-30) return;
+    const double x1 = lc[0], y1 = lc[1], x2 = lc[2], y2 = lc[3], x3 = lc[4], y3 = lc[5];
+    const double A = 0.5 * fabs(x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2));
+    if (A < 1e-12) return;
+    const double bv[3] = {(y2 - y3) / (2 * A), (y3 - y1) / (2 * A), (y1 - y2) / (2 * A)};
+    const double cv[3] = {(x3 - x2) / (2 * A), (x1 - x3) / (2 * A), (x2 - x1) / (2 * A)};
+    const double sc = h * A;
+    // four stress states: sxx, syy, sxy alone (in-plane), the whole stress (w)
+    const double st[4][3] = {{sig[0], 0.0, 0.0}, {0.0, sig[1], 0.0}, {0.0, 0.0, sig[2]}, {sig[0], sig[1], sig[2]}};
+    double lam[4][2], dc[4][2], ds[4][2];
+    for (int k = 0; k < 4; ++k) kg_principal(st[k][0], st[k][1], st[k][2], lam[k], dc[k], ds[k]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int r0 = 6 * i, c0 = 6 * j;
+            for (int k = 0; k < 4; ++k)
+                for (int d = 0; d < 2; ++d) {
+                    if (fabs(lam[k][d]) <= 1e-30) continue;
+                    const double c = dc[k][d], s = ds[k][d];
+                    const double gi = c * bv[i] + s * cv[i], gj = c * bv[j] + s * cv[j];
+                    const double val = sc * lam[k][d] * gi * gj;
+                    if (k < 3) {
+                        const double p11 = s * s, p22 = 1.0 * c * c, p12 = 1.0 * -c * s;
+                        Kl[r0 + 18 * c0] += val * p11;
+                        Kl[r0 + 18 * (c0 + 1)] += val * p12;
+                        Kl[r0 + 1 + 18 * c0] += val * p12;
+                        Kl[r0 + 1 + 18 * (c0 + 1)] += val * p22;
+                    } else {
+                        Kl[r0 + 2 + 18 * (c0 + 2)] += 1.0 * val;
+                    }
                 }
         }
+}
+
+// global 18 x 18 Kg of one CTRIA3 (column-major into out) under the membrane
+// resultants that stress_strain_tria3 gives for ug; returns a status code
+__device__ int t3_kg(const double *p, const double *pr, const double *ug, double *out) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6;
+    double v1[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]}, e13[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+    double v2[3], v3[3];
+    normalize3(v1);
+    cross3(v1, e13, v3);
+    normalize3(v3);
+    cross3(v3, v1, v2);
+    double c[3], lc[6];
+    for (int k = 0; k < 3; ++k) c[k] = ((p1[k] + p2[k]) + p3[k]) / 3.0;
+    for (int k = 0; k < 3; ++k) {
+        const double d[3] = {p[3 * k] - c[0], p[3 * k + 1] - c[1], p[3 * k + 2] - c[2]};
+        lc[2 * k] = dot3(d, v1); lc[2 * k + 1] = dot3(d, v2);
+    }
+    double rec[20];
+    const int st = t3_recover(p, pr, ug, rec);
+    if (st != 0) return st;
+    const double h = pr[JFEM_SP_H];
+    const double sig[3] = {rec[0] / h, rec[1] / h, rec[2] / h};
+    double Kl[324];
+    t3_kg_local(lc, sig, h, Kl);
+    const double R[9] = {v1[0], v1[1], v1[2], v2[0], v2[1], v2[2], v3[0], v3[1], v3[2]};
+    kg_rotate_blocks<18>(Kl, R, out);
     return 0;
 }
 
@@ -2127,6 +2216,25 @@ q4_kg_kernel(int64_t nq, const double *__restrict__ x, const double *__restrict_
     for (int c = 0; c < 8; ++c) pr[c] = qprop[c * nq + e];
     const int st = q4_kg(p, pr, ug, Ke + 576 * e);
     if (st != 0) report(status, e, st);
+}
+
+// one thread per CTRIA3: its global Kg at Ke + 324 e (Ke: the CTRIA3 part of the
+// triplet layout); failures reported with the index nq + e
+__global__ void __launch_bounds__(64)
+t3_kg_kernel(int64_t nq, int64_t nt, const double *__restrict__ x, const double *__restrict__ y,
+             const double *__restrict__ z, const int32_t *__restrict__ tconn, const double *__restrict__ tprop,
+             const double *__restrict__ u, double *__restrict__ Ke, int64_t *status) {
+    const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (e >= nt) return;
+    double p[9], pr[8], ug[18];
+    for (int k = 0; k < 3; ++k) {
+        const int64_t nd = tconn[k * nt + e];
+        p[3 * k] = x[nd]; p[3 * k + 1] = y[nd]; p[3 * k + 2] = z[nd];
+        for (int c = 0; c < 6; ++c) ug[6 * k + c] = u[6 * nd + c];
+    }
+    for (int c = 0; c < 8; ++c) pr[c] = tprop[c * nt + e];
+    const int st = t3_kg(p, pr, ug, Ke + 324 * e);
+    if (st != 0) report(status, nq + e, st);
 }
 
 
@@ -2205,10 +2313,13 @@ hipError_t launch_csr_shift_scaled(hipStream_t s, const jfem_pattern *P, const i
     return hipGetLastError();
 }
 
-hipError_t launch_q4_kg(hipStream_t s, const jfem_mesh *m, const double *u, double *Ke, int64_t *status) {
-    if (m->nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(q4_kg_kernel, dim3((unsigned)((m->nq + 63) / 64)), dim3(64), 0, s, m->nq, m->d_x, m->d_y,
-                       m->d_z, m->d_q, m->d_qp, u, Ke, status);
+hipError_t launch_shell_kg(hipStream_t s, const jfem_mesh *m, const double *u, double *Ke, int64_t *status) {
+    if (m->nq)
+        hipLaunchKernelGGL(q4_kg_kernel, dim3((unsigned)((m->nq + 63) / 64)), dim3(64), 0, s, m->nq, m->d_x, m->d_y,
+                           m->d_z, m->d_q, m->d_qp, u, Ke, status);
+    if (m->nt)
+        hipLaunchKernelGGL(t3_kg_kernel, dim3((unsigned)((m->nt + 63) / 64)), dim3(64), 0, s, m->nq, m->nt, m->d_x,
+                           m->d_y, m->d_z, m->d_t, m->d_tp, u, Ke + 576 * m->nq, status);
     return hipGetLastError();
 }
 

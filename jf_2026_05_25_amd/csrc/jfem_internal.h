@@ -135,12 +135,15 @@ hipError_t launch_q4_local(hipStream_t s, int64_t n, const double *lc, const dou
                            double *Ke, int64_t *status);
 hipError_t launch_t3_local(hipStream_t s, int64_t n, const double *lc, const double *Dm, const double *Db,
                            const double *Ds, const double *k6, double *Ke, int64_t *status);
-// shell force / stress / strain recovery of a mesh's CQUAD4s (44 doubles each) and
-// CTRIA3s (20 each) from device displacements u (6 per node)
-hipError_t launch_q4_kg(hipStream_t s, const jfem_mesh *m, const double *u, double *Ke, int64_t *status);
+// differential stiffness of a mesh's shells for device displacements u (6 per node):
+// CQUAD4s at Ke (576 each), then CTRIA3s (324 each)
+hipError_t launch_shell_kg(hipStream_t s, const jfem_mesh *m, const double *u, double *Ke, int64_t *status);
+// c = a + sigma b; and its diagonally scaled form (scale factors of a into d_sd)
 hipError_t launch_csr_shift(hipStream_t s, int64_t n, const double *a, double sigma, const double *b, double *c);
 hipError_t launch_csr_shift_scaled(hipStream_t s, const jfem_pattern *P, const int32_t *d_diagpos, double *d_sd,
                                    const double *a, double sigma, const double *b, double *c);
+// shell force / stress / strain recovery of a mesh's CQUAD4s (44 doubles each) and
+// CTRIA3s (20 each) from device displacements u (6 per node)
 hipError_t launch_shell_recover(hipStream_t s, const jfem_mesh *m, const double *u, int mean_shear, double *qout,
                                 double *tout, int64_t *status);
 hipError_t launch_tet10_direct(hipStream_t s, int64_t n, const double *X, const double *prop, double *Ke,

@@ -169,12 +169,14 @@ def flat_plate(nx: int = 40, ny: int = 40, Lx: float = 0.5, Ly: float = 0.5,
 
 
 def ss_plate_compression(n: int, size: float = 1.0, t: float = 0.01, E: float = 210e9, nu: float = 0.3,
-                         rigid_shear: bool = False) -> Model:
-    """Simply supported square plate (size x size x t, n x n CQUAD4) under a uniform
+                         rigid_shear: bool = False, tria_fraction: float = 0.0) -> Model:
+    """Simply supported square plate (size x size x t, n x n cells) under a uniform
     uniaxial compression sigma_x = -1 Pa: w = 0 on the four edges, u = 0 on x = 0,
     v = 0 at the origin, consistent edge forces on x = size (the SOL 105 preload
-    of a Timoshenko-Gere buckling plate; first root 4 pi^2 D / (size^2 t))."""
-    m = flat_plate(n, n, Lx=size, Ly=size, E=E, nu=nu, t=t, name="ss_plate_compression")
+    of a Timoshenko-Gere buckling plate; first root 4 pi^2 D / (size^2 t)).
+    ``tria_fraction`` splits that share of cells into two CTRIA3."""
+    m = flat_plate(n, n, Lx=size, Ly=size, E=E, nu=nu, t=t, tria_fraction=tria_fraction,
+                   name="ss_plate_compression")
     xy = m.xyz
     spc = np.zeros(m.ndof, np.uint8)
     x0, x1 = np.isclose(xy[:, 0], 0.0), np.isclose(xy[:, 0], size)
@@ -190,7 +192,8 @@ def ss_plate_compression(n: int, size: float = 1.0, t: float = 0.01, E: float = 
     m.loads = F[:, None]
     if rigid_shear:
         m.quad_props[:, SP_RIGID] = 1.0
-    m.meta["config"] = f"simply supported {n}x{n} CQUAD4 plate {size}x{size} m, t={t}, sigma_x = -1 Pa"
+    kind = "CQUAD4" if tria_fraction <= 0 else f"shell ({tria_fraction:g} of cells as CTRIA3 pairs)"
+    m.meta["config"] = f"simply supported {n}x{n} {kind} plate {size}x{size} m, t={t}, sigma_x = -1 Pa"
     return m
 
 

@@ -1396,6 +1396,89 @@ int or_q4_kg(const double p[12], const double prop[8], const double ug[24], doub
     return OR_OK;
 }
 
+/* principal stresses and directions of (sxx, syy, sxy): principal_stress_2d_components
+ * (src/FEMKernels.jl:8090-8101) */
+static void kg_principal(double sxx, double syy, double sxy, double *l1, double *c1, double *s1, double *l2,
+                         double *c2, double *s2) {
+    double mean_s = 0.5 * (sxx + syy), half_d = 0.5 * (sxx - syy);
+    double radius = sqrt(half_d * half_d + sxy * sxy);
+    if (radius <= 1e-30) { *l1 = mean_s; *c1 = 1.0; *s1 = 0.0; *l2 = mean_s; *c2 = 0.0; *s2 = 1.0; return; }
+    double theta = 0.5 * atan2(2.0 * sxy, sxx - syy);
+    double c = cos(theta), s = sin(theta);
+    *l1 = mean_s + radius; *c1 = c; *s1 = s;
+    *l2 = mean_s - radius; *c2 = -s; *s2 = c;
+}
+
+/* one principal direction's transverse pair block (add_geometric_principal_transverse_pair_direction!,
+ * src/FEMKernels.jl:8205-8238, unit shear factors) into Kg (n x n column-major) */
+static void kg_pt_dir(double *Kg, int n, int r0, int c0, double dix, double diy, double djx, double djy, double scale,
+                      double lambda, double c, double s, double uf, double vf, double uvf, double wf) {
+    if (fabs(lambda) <= 1e-30) return;
+    double gi = c * dix + s * diy, gj = c * djx + s * djy;
+    double val = scale * lambda * gi * gj;
+    double p11 = uf * s * s, p22 = vf * 1.0 * c * c, p12 = uvf * 1.0 * -c * s;
+    M(Kg, r0, c0, n) += val * p11;
+    M(Kg, r0, c0 + 1, n) += val * p12;
+    M(Kg, r0 + 1, c0, n) += val * p12;
+    M(Kg, r0 + 1, c0 + 1, n) += val * p22;
+    M(Kg, r0 + 2, c0 + 2, n) += wf * 1.0 * val;
+}
+
+/* add_geometric_principal_transverse_pair! (src/FEMKernels.jl:8240-8295): both principal directions */
+static void kg_pt_pair(double *Kg, int n, int r0, int c0, double dix, double diy, double djx, double djy, double scale,
+                       double sxx, double syy, double sxy, double uf, double vf, double uvf, double wf) {
+    double l1, c1, s1, l2, c2, s2;
+    kg_principal(sxx, syy, sxy, &l1, &c1, &s1, &l2, &c2, &s2);
+    kg_pt_dir(Kg, n, r0, c0, dix, diy, djx, djy, scale, l1, c1, s1, uf, vf, uvf, wf);
+    kg_pt_dir(Kg, n, r0, c0, dix, diy, djx, djy, scale, l2, c2, s2, uf, vf, uvf, wf);
+}
+
+int or_t3_kg_local(const double lc[6], const double sig[3], double h, double Kg[324]) {
+    for (int q = 0; q < 324; ++q) Kg[q] = 0.0;
+    if (h < 1e-30) return OR_OK;
+    const double x1 = lc[0], y1 = lc[1], x2 = lc[2], y2 = lc[3], x3 = lc[4], y3 = lc[5];
+    double A2 = x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2);
+    double A = 0.5 * fabs(A2);
+    if (A < 1e-12) return OR_OK;
+    const double bv[3] = {(y2 - y3) / (2 * A), (y3 - y1) / (2 * A), (y1 - y2) / (2 * A)};
+    const double cv[3] = {(x3 - x2) / (2 * A), (x1 - x3) / (2 * A), (x2 - x1) / (2 * A)};
+    const double sxx = sig[0], syy = sig[1], sxy = sig[2], sc = h * A;
+    /* add_geometric_linear_principal_transverse_pair! (:8297-8338): the in-plane work
+     * per stress component, the transverse work with the whole stress */
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            int r0 = 6 * i, c0 = 6 * j;
+            kg_pt_pair(Kg, 18, r0, c0, bv[i], cv[i], bv[j], cv[j], sc, sxx, 0.0, 0.0, 1.0, 1.0, 1.0, 0.0);
+            kg_pt_pair(Kg, 18, r0, c0, bv[i], cv[i], bv[j], cv[j], sc, 0.0, syy, 0.0, 1.0, 1.0, 1.0, 0.0);
+            kg_pt_pair(Kg, 18, r0, c0, bv[i], cv[i], bv[j], cv[j], sc, 0.0, 0.0, sxy, 1.0, 1.0, 1.0, 0.0);
+            kg_pt_pair(Kg, 18, r0, c0, bv[i], cv[i], bv[j], cv[j], sc, sxx, syy, sxy, 0.0, 0.0, 0.0, 1.0);
+        }
+    return OR_OK;
+}
+
+int or_t3_kg(const double p[9], const double prop[8], const double ug[18], double Kg[324]) {
+    const double *p1 = p, *p2 = p + 3, *p3 = p + 6;
+    double v1[3], v2[3], v3[3], e13[3];
+    v3sub(p2, p1, v1); v3normalize(v1);
+    v3sub(p3, p1, e13); v3cross(v1, e13, v3); v3normalize(v3);
+    v3cross(v3, v1, v2);
+    double c[3], lc[6];
+    for (int k = 0; k < 3; ++k) c[k] = ((p1[k] + p2[k]) + p3[k]) / 3.0;
+    for (int k = 0; k < 3; ++k) { double d[3]; v3sub(p + 3 * k, c, d); lc[2 * k] = v3dot(d, v1); lc[2 * k + 1] = v3dot(d, v2); }
+    /* membrane resultants of the preload: stress_strain_tria3 (the recovery's N) */
+    double rec[OR_T3R_N];
+    int st = or_t3_recover(p, prop, ug, rec);
+    if (st != OR_OK) return st;
+    const double h = prop[OR_SP_H];
+    double sig[3] = {rec[OR_T3R_N0] / h, rec[OR_T3R_N0 + 1] / h, rec[OR_T3R_N0 + 2] / h};
+    double Kl[324];
+    st = or_t3_kg_local(lc, sig, h, Kl);
+    if (st != OR_OK) return st;
+    double R[9] = {v1[0], v1[1], v1[2], v2[0], v2[1], v2[2], v3[0], v3[1], v3[2]};
+    transform_blocks(Kl, 18, R, Kg);
+    return OR_OK;
+}
+
 /* ------------------------------------------------------------------------- */
 /* CTETRA10, src/tetra10.jl:6-79; iso_3d_constitutive src/FEMKernels.jl:12056 */
 /* ------------------------------------------------------------------------- */

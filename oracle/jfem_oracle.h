@@ -89,6 +89,15 @@ int or_q4_kg(const double p[12], const double prop[8], const double ug[24], doub
 int or_q4_kg_local(const double lc[8], const double ue[24], double E, double nu, double h, int assumed_transverse,
                    double Kg[576]);
 
+/* CTRIA3 differential stiffness (geometric_stiffness_tria3, principal-transverse
+ * mode, src/FEMKernels.jl:11142-11288, 8090-8338) under the membrane resultants
+ * of stress_strain_tria3 for the global displacements ug (the recovery's N),
+ * assembled per src/solver/assembly.jl:8343-8498: global 18x18 Kg (column-major).
+ * or_t3_kg_local: the element-frame kernel on local coordinates lc and membrane
+ * stresses sig = N / h. */
+int or_t3_kg(const double p[9], const double prop[8], const double ug[18], double Kg[324]);
+int or_t3_kg_local(const double lc[6], const double sig[3], double h, double Kg[324]);
+
 /* ---------- assembly with sparse(I,J,V) semantics ---------- */
 /* Triplet order: all CQUAD4 (element order, column-major 24x24), then CTRIA3, then
  * CTETRA10.  Duplicates summed in triplet order.  Output CSR (row_ptr n+1, col, val)

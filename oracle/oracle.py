@@ -45,6 +45,8 @@ def lib():
         L.or_t3_recover.argtypes = [dp, dp, dp, dp]
         L.or_q4_kg.argtypes = [dp, dp, dp, dp]
         L.or_q4_kg_local.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp]
+        L.or_t3_kg.argtypes = [dp, dp, dp, dp]
+        L.or_t3_kg_local.argtypes = [dp, dp, C.c_double, dp]
         L.or_assemble.argtypes = [C.c_int64, dp, dp, dp, C.c_int64, C.c_void_p, C.c_void_p,
                                   C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                   C.c_double, C.POINTER(C.c_int64), C.POINTER(C.POINTER(C.c_int64)),
@@ -219,21 +221,43 @@ def q4_kg_local(lc, ue, E, nu, h, assumed_transverse=True):
     return out.reshape(24, 24, order="F")
 
 
+def t3_kg(p, prop, ug):
+    """Global 18 x 18 differential stiffness of one CTRIA3 (or_t3_kg)."""
+    out = np.zeros(324)
+    st = lib().or_t3_kg(_f(np.asarray(p, float).reshape(3, 3).ravel()), _f(prop),
+                        _f(np.asarray(ug, float).ravel()), out)
+    if st:
+        raise OracleError(st)
+    return out.reshape(18, 18, order="F")
+
+
+def t3_kg_local(lc, sig, h):
+    """The CTRIA3 element-frame kernel (or_t3_kg_local) for membrane stresses sig."""
+    out = np.zeros(324)
+    st = lib().or_t3_kg_local(_f(np.asarray(lc, float).reshape(3, 2).ravel()), _f(np.asarray(sig, float)),
+                              float(h), out)
+    if st:
+        raise OracleError(st)
+    return out.reshape(18, 18, order="F")
+
+
 def kg_assemble(model, u):
-    """Global differential stiffness (scipy CSR, ndof x ndof) of a CQUAD4 model for
-    the displacement vector u: element matrices summed in element order."""
+    """Global differential stiffness (scipy CSR, ndof x ndof) of a shell model
+    (CQUAD4, CTRIA3) for the displacement vector u: element matrices summed in
+    triplet order (CQUAD4s, then CTRIA3s)."""
     u = np.asarray(u, float).reshape(-1)
-    nq = len(model.quads)
     rows, cols, vals = [], [], []
-    for e, nodes in enumerate(np.asarray(model.quads)):
-        ug = np.concatenate([u[6 * n:6 * n + 6] for n in nodes])
-        kg = q4_kg(model.xyz[nodes], model.quad_props[e], ug)
-        dofs = np.concatenate([np.arange(6 * n, 6 * n + 6) for n in nodes])
-        rows.append(np.repeat(dofs, 24)); cols.append(np.tile(dofs, 24)); vals.append(kg.ravel())
-    if len(model.trias) or len(model.tets):
+    for kind, conn, props, fn, nn in (("q", model.quads, model.quad_props, q4_kg, 4),
+                                      ("t", model.trias, model.tria_props, t3_kg, 3)):
+        for e, nodes in enumerate(np.asarray(conn)):
+            ug = np.concatenate([u[6 * n:6 * n + 6] for n in nodes])
+            kg = fn(model.xyz[nodes], props[e], ug)
+            dofs = np.concatenate([np.arange(6 * n, 6 * n + 6) for n in nodes])
+            rows.append(np.repeat(dofs, 6 * nn)); cols.append(np.tile(dofs, 6 * nn)); vals.append(kg.ravel())
+    if len(model.tets):
         raise OracleError(5)
     n = model.ndof
-    if nq == 0:
+    if not vals:
         return sp.csr_matrix((n, n))
     return sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n)).tocsr()
 

@@ -24,6 +24,14 @@ or_q4_kg restates geometric_stiffness_quad4_reference_kdjj_iso
   this path's elastic K, while the recorded modes are eigenvectors of this
   path's pencil to MAC 0.99999998 (mode 1), so the gap sits in the reference's
   SOL 105 stiffness calibration rather than in Kg.
+
+or_t3_kg restates geometric_stiffness_tria3 in the default principal-transverse
+mode (src/FEMKernels.jl:11142-11288, 8090-8338; kg_shell_trans_mode,
+src/solver/assembly.jl:8484) under the membrane resultants of
+stress_strain_tria3.  It is pinned by the constant-strain closed forms (the
+transverse block h A g_i' sigma g_j, the in-plane blocks by stress component),
+the membrane state of a linear field, symmetry, linearity, rigid translations,
+rotation covariance, and the Timoshenko-Gere plate meshed with CTRIA3 pairs.
 """
 import json
 import os
@@ -241,6 +249,122 @@ def test_timoshenko_convergence(oracle):
         m.quad_props[:, meshes.SP_RIGID] = rigid
         lam, _, _, _ = buckle(m, oracle)
         assert lo <= lam[0] / scr - 1 <= hi, (rigid, n, lam[0] / scr - 1)
+
+
+# ------------------------------------------------------------------ CTRIA3
+TRIS = {
+    "right": np.array([[0, 0, 0], [0.2, 0, 0], [0, 0.1, 0]], float),
+    "obtuse": np.array([[0, 0, 0], [0.3, 0.05, 0], [-0.1, 0.12, 0]], float),
+    "sliver": np.array([[0, 0, 0], [0.4, 0.0, 0], [0.21, 0.02, 0]], float),
+}
+TRIS["spatial"] = TRIS["obtuse"] @ rotation([1.0, 2.0, -0.5], 0.9).T + np.array([1.0, 2.0, 3.0])
+
+
+def tri_grads(lc):
+    """Constant shape-function gradients and area of a 3-node triangle (lc: 3 x 2)."""
+    P = np.column_stack([np.ones(3), lc])
+    G = np.linalg.inv(P)[1:]                   # rows: d/dx, d/dy of N1..N3
+    return G[0], G[1], 0.5 * abs(np.linalg.det(P))
+
+
+@pytest.mark.parametrize("sig", [(-1.0, 0.0, 0.0), (0.0, 2.0, 0.0), (0.0, 0.0, 1.5), (-1.0, 0.3, 0.7),
+                                 (2.0, 2.0, 0.0), (0.4, -1.2, -0.9)])
+@pytest.mark.parametrize("name", ["right", "obtuse", "sliver"])
+def test_tria_closed_form(oracle, name, sig):
+    """Constant-strain triangle under a uniform membrane stress: the transverse block
+    is h A g_i' sigma g_j (the principal directions of the whole stress recombine to
+    sigma); the in-plane blocks take each stress component on its own, which gives
+    v-v: sxx gx gx', u-u: syy gy gy', and under shear u-u = v-v = sxy sym / 2,
+    u-v = v-u = -sxy (gx gx' + gy gy') / 2, sym = gx gy' + gy gx'; drilling,
+    bending rotations and u/v-w coupling carry nothing."""
+    lc = TRIS[name][:, :2] - TRIS[name][:, :2].mean(0)
+    sxx, syy, sxy = sig
+    Kg = oracle.t3_kg_local(lc, sig, H)
+    gx, gy, A = tri_grads(lc)
+    w = H * A
+    sym = np.outer(gx, gy) + np.outer(gy, gx)
+    XX, YY = np.outer(gx, gx), np.outer(gy, gy)
+    uu, vv, uv, ww = Kg[0::6, 0::6], Kg[1::6, 1::6], Kg[0::6, 1::6], Kg[2::6, 2::6]
+    scale = w * max(abs(np.array(sig))) * max(np.abs(XX).max(), np.abs(YY).max())
+    tol = dict(rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(ww, w * (sxx * XX + syy * YY + sxy * sym), **tol)
+    np.testing.assert_allclose(vv, w * (sxx * XX + sxy * sym / 2), **tol)
+    np.testing.assert_allclose(uu, w * (syy * YY + sxy * sym / 2), **tol)
+    np.testing.assert_allclose(uv, -w * sxy * (XX + YY) / 2, **tol)
+    np.testing.assert_allclose(Kg[1::6, 0::6], uv, **tol)
+    zero = np.concatenate([np.arange(c, 18, 6) for c in (3, 4, 5)])
+    assert np.all(Kg[zero] == 0) and np.all(Kg[:, zero] == 0)
+    for a, b in ((0, 2), (1, 2)):
+        assert np.abs(Kg[a::6, b::6]).max() <= 1e-12 * scale
+
+
+def test_tria_membrane_state(oracle):
+    """The global element under a linear displacement field takes the membrane
+    stress sigma = E / (1 - nu^2) [[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]] eps
+    of that field (frame v1 along the first edge): or_t3_kg = or_t3_kg_local at
+    that stress; a rigid motion (zero membrane strain) gives zero."""
+    p = TRIS["obtuse"].copy()
+    ex, ey, g = 1e-3, -4e-4, 6e-4
+    ug = np.zeros(18)
+    for k, (x, y, _) in enumerate(p):
+        ug[6 * k] = ex * x + 0.5 * g * y
+        ug[6 * k + 1] = ey * y + 0.5 * g * x
+    # local frame: v1 along p2 - p1, v3 = +z
+    v1 = (p[1] - p[0]) / np.linalg.norm(p[1] - p[0])
+    R = np.array([v1, np.cross([0, 0, 1.0], v1), [0, 0, 1.0]])
+    eps_g = np.array([[ex, g / 2], [g / 2, ey]])
+    el = R[:2, :2] @ eps_g @ R[:2, :2].T
+    Dm = E / (1 - NU ** 2) * np.array([[1, NU, 0], [NU, 1, 0], [0, 0, (1 - NU) / 2]])
+    sig = Dm @ np.array([el[0, 0], el[1, 1], 2 * el[0, 1]])
+    lc = (p - p.mean(0)) @ R[:2].T
+    T = np.kron(np.eye(6), R)
+    exp = T.T @ oracle.t3_kg_local(lc, sig, H) @ T
+    got = oracle.t3_kg(p, prop(), ug)
+    np.testing.assert_allclose(got, exp, rtol=0, atol=1e-11 * np.abs(exp).max())
+    u3 = np.zeros(18)
+    for k in range(3):
+        u3[6 * k:6 * k + 3] = np.array([1e-3, -2e-3, 5e-4]) + np.cross([2e-4, -1e-4, 3e-4], p[k] - p[0])
+        u3[6 * k + 3:6 * k + 6] = [2e-4, -1e-4, 3e-4]
+    assert np.abs(oracle.t3_kg(p, prop(), u3)).max() <= 1e-9 * np.abs(exp).max()
+
+
+@pytest.mark.parametrize("name", list(TRIS))
+def test_tria_invariants(oracle, name):
+    """Symmetry, linearity in the preload, zero action on rigid translations,
+    covariance under rotations of the element in space."""
+    p = TRIS[name]
+    rng = np.random.default_rng(3)
+    u1, u2 = rng.normal(size=18) * 1e-4, rng.normal(size=18) * 1e-4
+    K1, K2 = oracle.t3_kg(p, prop(), u1), oracle.t3_kg(p, prop(), u2)
+    K12 = oracle.t3_kg(p, prop(), u1 + 2.5 * u2)
+    s = np.abs(K12).max()
+    assert np.abs(K1 - K1.T).max() <= 1e-14 * np.abs(K1).max()
+    np.testing.assert_allclose(K12, K1 + 2.5 * K2, rtol=0, atol=1e-12 * s)
+    for d in range(3):
+        t = np.zeros(18)
+        t[d::6] = 1.0
+        assert np.abs(K1 @ t).max() <= 1e-12 * np.abs(K1).max()
+    Q = rotation([0.3, -0.5, 0.8], 0.7)
+    T = np.kron(np.eye(6), Q)
+    Kq = oracle.t3_kg(p @ Q.T + np.array([3.0, -1.0, 2.0]), prop(), T @ u1)
+    np.testing.assert_allclose(Kq, T @ K1 @ T.T, rtol=0, atol=1e-11 * np.abs(K1).max())
+
+
+def test_tria_plate_timoshenko(oracle):
+    """The simply supported square of CTRIA3 pairs (every cell split) and of a
+    half-split mixed mesh: the first root within 1 % of Timoshenko and Gere's
+    4 pi^2 D / (b^2 h) on 12 x 12 and 16 x 16 cells (the plate's own shear
+    flexibility puts the limit about 0.7 % low), the second within 2 % of its
+    closed form (m = 2 half-waves: k = (2 + 1/2)^2 against 4)."""
+    scr = 4 * np.pi ** 2 * E * H ** 2 / (12 * (1 - NU ** 2))
+    for frac, n in ((1.0, 12), (1.0, 16), (0.5, 16)):
+        m = meshes.ss_plate_compression(n, size=1.0, t=H, E=E, nu=NU, tria_fraction=frac)
+        m.k6rot = 100.0
+        assert len(m.trias) > 0
+        lam, _, _, _ = buckle(m, oracle)
+        assert abs(lam[0] / scr - 1) <= 1e-2, (frac, n, lam[0] / scr - 1)
+        if n == 16:
+            assert abs(lam[1] / (scr * 6.25 / 4) - 1) <= 2e-2, (frac, n, lam[1] / scr)
 
 
 def test_deck_eigb_request():
