@@ -36,7 +36,7 @@ MI355X_FP32_TFLOPS = 157.3
 MI355X_FP64_TFLOPS = 78.6
 METRIC = "solves/s + p50/p99 latency @1e-10 residual: FP32-factor+refine vs FP64"
 FAMILIES = ["element", "assembly", "factor", "solve", "spmv", "factor_update", "factor_panel", "extend_add",
-            "fwd_flow", "bwd_flow", "factor_schur"]
+            "fwd_flow", "bwd_flow", "factor_schur", "factor_inverse"]
 # algorithmic bytes of one element's generation: read its grid points' coordinates,
 # its connectivity and its section record, write its matrix once
 ELEM_READ = {"CQUAD4": 4 * 24 + 4 * 4 + 8 * 8, "CTRIA3": 3 * 24 + 3 * 4 + 8 * 8, "CTETRA10": 10 * 24 + 10 * 4 + 2 * 8}

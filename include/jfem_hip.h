@@ -69,7 +69,9 @@ int jfem_device_memory(jfem_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes)
  * 1 assembly, 2 numeric factor, 3 triangular solves, 4 residual SpMV; with timing
  * level 2 also per launch: 5 factor update kernel, 6 factor panel kernels,
  * 7 extend-add kernel, 8 forward dataflow solve kernel, 9 backward dataflow
- * solve kernel, 10 update-matrix (Schur complement) kernel.
+ * solve kernel, 10 update-matrix (Schur complement) kernel, 11 diagonal-block
+ * inverses of the solves (the 64 x 64 inverses and X = L_11^{-1} of the
+ * supernodes with two or more panels, one interval for the whole sequence).
  * jfem_ctx_enable_timing(ctx, 0 | 1 | 2). */
 int jfem_ctx_kernel_time(jfem_ctx *ctx, int family, double *ms, int64_t *launches);
 int jfem_ctx_reset_kernel_time(jfem_ctx *ctx);

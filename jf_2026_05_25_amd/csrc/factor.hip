@@ -36,6 +36,7 @@ struct jfem_symbolic {
     FlowChild *d_flow_gb = nullptr;
     UpdateTask *d_utasks = nullptr;
     UpdateTask *d_stasks = nullptr;
+    InvTask *d_itasks = nullptr;
     EaTask *d_etasks = nullptr;
     int32_t *d_ea_bounds = nullptr;
     int64_t *d_ea_boff = nullptr;
@@ -49,7 +50,7 @@ struct jfem_factor {
     int precision = 64;
     void *d_L = nullptr;
     void *d_U = nullptr;
-    void *d_Linv = nullptr;         // inverse diagonal blocks of the panels (solves)
+    void *d_Linv = nullptr;         // inverse diagonal blocks of the panels, X = L_11^{-1} of the multi-panel supernodes (solves)
     int64_t bytes = 0;
     bool view = false;              // a variant of a jfem_batch: storage owned by the batch
 };
@@ -79,17 +80,19 @@ __device__ inline void report_min(int64_t *status, int64_t idx, int code) {
 }
 
 // ----------------------------------------------------------------- fill / scatter
-// before a dataflow solve: Z and X (nv entries each) and W (nw) to the pending
-// mark, the two launch tickets to zero
+// before a dataflow solve: Z, X, V and Tb (nv entries each) and W (nw) to the
+// pending mark, the two launch tickets to zero
 template <class T>
-__global__ void solve_marks_kernel(T *__restrict__ Z, T *__restrict__ X, int64_t nv, T *__restrict__ W, int64_t nw,
-                                   T mark, int32_t *__restrict__ tickets) {
+__global__ void solve_marks_kernel(T *__restrict__ Z, T *__restrict__ X, T *__restrict__ V, T *__restrict__ Tb,
+                                   int64_t nv, T *__restrict__ W, int64_t nw, T mark, int32_t *__restrict__ tickets) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i0 < 2) tickets[i0] = 0;
     for (int64_t i = i0; i < nv; i += stride) {
         Z[i] = mark;
         X[i] = mark;
+        V[i] = mark;
+        Tb[i] = mark;
     }
     for (int64_t i = i0; i < nw; i += stride) W[i] = mark;
 }
@@ -214,7 +217,8 @@ __device__ inline double bcast<double>(double v, int lane) {
 // inverse of each panel's diagonal block, for the triangular solves (one
 // wavefront per panel, after the factorization): lane c computes column c of
 // L_pp^{-1} by forward substitution in FP64, L_pp read as broadcasts from LDS;
-// stored column-major with leading dimension b
+// stored column-major with leading dimension b, and for a supernode with two or
+// more panels also as diagonal block p of its X = L_11^{-1} (leading dimension ns)
 template <class T>
 __global__ void __launch_bounds__(64) diag_inverse_kernel(const FlowTask *__restrict__ tasks, const T *__restrict__ L,
                                                           T *__restrict__ Linv, VarStride vs) {
@@ -248,6 +252,10 @@ __global__ void __launch_bounds__(64) diag_inverse_kernel(const FlowTask *__rest
     T *dst = Linv + t.iofs;
     if (lane < b)
         for (int c = 0; c < b; ++c) dst[lane + (int64_t)b * c] = D[lane][c];
+    if (t.xofs >= 0 && lane < b) {
+        T *xd = Linv + t.xofs + k + t.ns * k;
+        for (int c = 0; c < b; ++c) xd[lane + t.ns * c] = D[lane][c];
+    }
 }
 
 // ----------------------------------------------------------------- update kernel (reference)
@@ -773,42 +781,94 @@ __global__ void __launch_bounds__(kBS) schur_kernel(const UpdateTask *__restrict
             }
 }
 
-// chain products of the triangular solves, one workgroup per panel boundary q
-// (backward task (s, q) with q < np-1), after the diagonal inverses:
-//   M_{q+1} = L_{q+1,q+1}^{-1} A  and  G_q = (A L_qq^{-1})^T,  A = L[q+1 rows, q cols],
-// both 64 x 64 column-major (zero beyond the b_{q+1} rows of panel q+1).  With
-// them a forward chain step is z_p = u - M_p z_{p-1} and a backward one
-// x_p = u - G_p x_{p+1}, u formed before the awaited value arrives.
-template <class T>
-__global__ void __launch_bounds__(kBS) chain_product_kernel(const FlowTask *__restrict__ tasks, const T *__restrict__ L,
-                                                            T *__restrict__ Linv, VarStride vs) {
-    __shared__ T A[kPanel][kPanel + 1];
+// Off-diagonal blocks of X = L_11^{-1} for the supernodes with two or more
+// panels, one workgroup per 64 x 64 output tile (InvTask, after the diagonal
+// inverses).  C(r, c) = sum over the chunks k in [k0, k1) of
+// sum_kk A[kk][r] B[kk][c], with A[kk][r] = Aop(r, kk) and B[kk][c] = Bop(kk, c)
+// staged through LDS chunk by chunk (the next chunk's loads in flight during the
+// products), the MFMA operand layout and split over the four wavefronts of
+// schur_kernel, the chunks added in order:
+//   PH 0: Aop = L[rows of panel i, columns of panel k], Bop = X[rows of panel k,
+//         columns of panel j]; T = C stored transposed in the upper block
+//         (rows of panel j, columns of panel i) of X;
+//   PH 1: Aop = X[rows of panel i, columns of panel k], Bop = T of block (k, j)
+//         (read back from the upper block); X[rows of panel i, columns of panel j] = -C.
+// Panel i may be the last, partial one (nr < 64 rows), and so may chunk k = i in
+// PH 1; entries beyond ns are masked (their indices clamped into the matrix).
+template <class T, int PH>
+__global__ void __launch_bounds__(kBS) inv_tile_kernel(const InvTask *__restrict__ tasks, const T *__restrict__ L,
+                                                       T *__restrict__ Linv, VarStride vs) {
+    typedef typename Mfma<T>::v4 v4;
     L += blockIdx.y * vs.L;
     Linv += blockIdx.y * vs.inv;
-    const FlowTask t = tasks[blockIdx.x];
-    if (t.bofs < 0) return;                            // last panel: no boundary below it
-    const int b1 = (int)min<int64_t>(kPanel, t.ns - (int64_t)kPanel * (t.p + 1));
-    const int64_t row1 = (int64_t)kPanel * (t.p + 1), col0 = (int64_t)kPanel * t.p;
+    __shared__ T A[kPanel][kTile + 1];
+    __shared__ T B[kPanel][kTile + 1];
+    const InvTask t = tasks[blockIdx.x];
+    const int64_t ns = t.ns, ms = t.ms;
+    T *X = Linv + t.xofs;
     const T *Ls = L + t.lofs;
-    for (int q = threadIdx.x; q < kPanel * kPanel; q += kBS) {
-        const int k = q % kPanel, m = q / kPanel;
-        A[k][m] = k < b1 ? Ls[(row1 + min(k, b1 - 1)) + t.ms * (col0 + m)] : T(0);
+    const int64_t r0 = (int64_t)kPanel * t.i, c0 = (int64_t)kPanel * t.j;
+    const int nr = (int)min<int64_t>(kPanel, ns - r0);
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    T av[16], bv[16];
+    // thread q of pass u: x = q % 64 (contiguous in memory), y = q / 64
+    auto load = [&](int k) {
+        const int64_t kc = (int64_t)kPanel * k;
+        const int kb = (int)min<int64_t>(kPanel, ns - kc);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = tid + kBS * u, x = q % kTile, y = q / kTile;
+            const int64_t ra = r0 + min(x, nr - 1), ka = kc + min(y, kb - 1);
+            av[u] = PH == 0 ? Ls[ra + ms * ka] : X[ra + ns * ka];
+            bv[u] = PH == 0 ? X[(kc + min(x, kb - 1)) + ns * (c0 + y)] : X[(c0 + x) + ns * ka];
+        }
+    };
+    v4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = v4{T(0), T(0), T(0), T(0)};
+    load(t.k0);
+    for (int k = t.k0; k < t.k1; ++k) {
+        const int kb = (int)min<int64_t>(kPanel, ns - (int64_t)kPanel * k);
+        const int b4 = (kb + 3) & ~3;
+        if (k > t.k0) __syncthreads();           // the previous chunk's products are done with A and B
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = tid + kBS * u, x = q % kTile, y = q / kTile;
+            A[y][x] = (x < nr && y < kb) ? av[u] : T(0);
+            if (PH == 0) B[x][y] = x < kb ? bv[u] : T(0);
+            else B[y][x] = y < kb ? bv[u] : T(0);
+        }
+        __syncthreads();
+        if (k + 1 < t.k1) load(k + 1);           // in flight during the products below
+        for (int p = 0; p < b4; p += 4) {
+            T aj[2], bi[2];
+#pragma unroll
+            for (int y = 0; y < 2; ++y) aj[y] = B[p + l4][wj + 16 * y + l16];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) bi[x] = A[p + l4][wi + 16 * x + l16];
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) acc[x][y] = Mfma<T>::mac(aj[y], bi[x], acc[x][y]);
+        }
     }
-    __syncthreads();
-    const T *Li0 = Linv + t.iofs;                      // L_qq^{-1}, 64 x 64 (q < np-1)
-    const T *Li1 = Linv + t.iofs + kPanel * kPanel;    // L_{q+1,q+1}^{-1}, b1 x b1
-    T *M = Linv + t.bofs - kPanel * kPanel, *G = Linv + t.bofs;
-    // accumulated in FP64 and rounded once (the product of the factor's own entries)
-    for (int q = threadIdx.x; q < kPanel * kPanel; q += kBS) {
-        const int i = q % kPanel, c = q / kPanel;
-        double sm = 0.0, sg = 0.0;
-        if (i < b1)
-            for (int m = 0; m <= i; ++m) sm = fma((double)Li1[i + (int64_t)b1 * m], (double)A[m][c], sm);
-        if (c < b1)
-            for (int m = i; m < kPanel; ++m) sg = fma((double)A[c][m], (double)Li0[m + (int64_t)kPanel * i], sg);
-        M[i + kPanel * c] = (T)sm;                     // M(i, c)
-        G[i + kPanel * c] = (T)sg;                     // G(i, c) = (A L_qq^{-1})(c, i)
-    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = wi + 16 * x + l16, j = wj + 16 * y + Mfma<T>::drow(l4, v);
+                if (i < nr) {
+                    if (PH == 0) X[(c0 + j) + ns * (r0 + i)] = acc[x][y][v];
+                    else X[(r0 + i) + ns * (c0 + j)] = -acc[x][y][v];
+                }
+            }
 }
 
 // ---------------------------------------------------------------- dataflow solves
@@ -1098,18 +1158,22 @@ __device__ __forceinline__ bool gather_children(T (*vs)[NR], const FlowChild *__
     return ok;
 }
 
-// forward: pivot task (s, p): z_p = L_pp^{-1} (y_p + children's contributions
-// - sum_{k<p} L[p rows, k] z_k), written to Z; update task: w = children's
+// forward: pivot task (s, p) of a supernode with one panel: z = L_11^{-1}
+// (y + children's contributions); of a supernode with np >= 2 panels: v_p = y_p +
+// children's contributions to the rows of panel p is published to V, then
+// z_p = sum_{k<p} X[p,k] v_k + X_pp v_p with X = L_11^{-1} and the v_k of the
+// panels before it polled from V -- the panels of s run side by side, with one
+// exchange of v and no chain along the front; update task: w = children's
 // contributions - sum_k L[rows, k] z_k, written to s's update vector in W;
 // whole-front task: both for a supernode with one panel (rows of the front
 // <= kWholeRows), the update rows in blocks of 64.
-// The k loop runs without LDS or barriers: every lane polls the z_k values it
-// needs straight into its registers.
+// The k loops run without LDS or barriers: every lane polls the values it needs
+// straight into its registers.
 template <class T, int NR>
 __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T (*vx)[NR], int &failed,
                                               const FlowChild *__restrict__ gch, const int32_t *__restrict__ relmap,
                                               const T *__restrict__ L, const T *__restrict__ Linv,
-                                              const T *__restrict__ Y, T *W, T *Z, int nrhs, int32_t *cnt,
+                                              const T *__restrict__ Y, T *W, T *Z, T *V, int nrhs, int32_t *cnt,
                                               int64_t nsn) {
     int32_t *abort_flag = cnt + nsn;
     const FlowChild first = t.nch > 0 ? gch[t.gb] : FlowChild{};   // (in flight with the task's first loads)
@@ -1117,12 +1181,14 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
     const int i = 16 * w + j;                          // row of the 64-row block
     const int rq = g16_r0<NR>(cg);
     const bool pivot = t.p >= 0, whole = t.kind == 1;
-    const bool chain = pivot && t.bofs >= 0;           // p >= 1: panel p-1 enters through M_p
-    const int kend = pivot ? (chain ? t.p - 1 : t.p) : t.np;
-    const T *Ls = L + t.lofs + t.r0;                   // front row r0, column 0
-    T lm[16], im[16], mm[16];
-    if (chain) g16_load(mm, Linv + t.bofs, kPanel, t.nr, kPanel, i, cg);
-    if (kend > 0) g16_load(lm, Ls, t.ms, t.nr, (int)min<int64_t>(kPanel, t.ns), i, cg);
+    const bool multi = pivot && t.xofs >= 0;           // np >= 2: through X and the v of the earlier panels
+    const int kend = pivot ? t.p : t.np;
+    // operand of the k loop: update task L[rows, k] (ld ms), multi-panel pivot X[p rows, k] (ld ns)
+    const T *Kb = multi ? Linv + t.xofs + t.r0 : L + t.lofs + t.r0;
+    const int64_t kld = multi ? t.ns : t.ms;
+    const T *Kv = multi ? V : Z;                       // the vector the k loop polls
+    T lm[16], im[16];
+    if (kend > 0) g16_load(lm, Kb, kld, t.nr, (int)min<int64_t>(kPanel, t.ns), i, cg);
     else if (whole && t.ms > t.ns) g16_load(lm, L + t.lofs + t.ns, t.ms, (int)(t.ms - t.ns), t.nr, i, cg);
     if (pivot) g16_load(im, Linv + t.iofs, t.nr, t.nr, t.nr, i, cg);
 #pragma unroll
@@ -1138,6 +1204,13 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
         failed = 1;
     __syncthreads();
     if (failed) return;
+    if (multi && i < t.nr) {                           // v_p for the later panels of s
+#pragma unroll
+        for (int rr = 0; rr < NR / 4; ++rr) {
+            const int r = rq + rr;
+            if (r < nrhs) st_dev(V + (t.c0 + t.r0 + i) * nrhs + r, vs[i][r]);
+        }
+    }
     T acc[NR], xr[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
@@ -1178,17 +1251,31 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
     bool ok = true;
     for (int k = 0; k < kend; ++k) {
         const int64_t col = (int64_t)kPanel * k;
-        const T *zk = Z + (t.c0 + col) * nrhs;
+        const T *zk = Kv + (t.c0 + col) * nrhs;
         ok = g16_load_wait<T, NR>(xr, [&](int c) { return zk + (int64_t)c * nrhs; },
                                   (int)min<int64_t>(kPanel, t.ns - col), nrhs, cg, j, abort_flag) && ok;
         T cur[16];
 #pragma unroll
         for (int c = 0; c < 16; ++c) cur[c] = lm[c];
         if (k + 1 < kend)
-            g16_load(lm, Ls + t.ms * (col + kPanel), t.ms, t.nr, (int)min<int64_t>(kPanel, t.ns - col - kPanel), i, cg);
+            g16_load(lm, Kb + kld * (col + kPanel), kld, t.nr, (int)min<int64_t>(kPanel, t.ns - col - kPanel), i, cg);
         g16_mac<T, NR>(acc, cur, xr);
     }
     if (!ok) failed = 1;
+    if (multi) {
+        // + X_pp v_p (v_p from LDS), one reduction for the whole sum
+        g16_from_lds<T, NR>(xr, vs, t.nr, cg, j);
+        g16_mac<T, NR>(acc, im, xr);
+        g16_reduce<T, NR>(acc);
+        if (i < t.nr && ok) {
+#pragma unroll
+            for (int rr = 0; rr < NR / 4; ++rr) {
+                const int r = rq + rr;
+                if (r < nrhs) st_dev(Z + (t.c0 + t.r0 + i) * nrhs + r, acc[r]);
+            }
+        }
+        return;
+    }
     g16_reduce<T, NR>(acc);
     if (!pivot) {
         if (i < t.nr) {
@@ -1200,30 +1287,12 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
         }
         return;
     }
-#pragma unroll
-    for (int rr = 0; rr < NR / 4; ++rr) {
-        const int r = rq + rr;
-        vx[i][r] = i < t.nr ? vs[i][r] - acc[r] : T(0);
-    }
-    __syncthreads();
-    if (failed) return;
-    g16_from_lds<T, NR>(xr, vx, t.nr, cg, j);
+    // one-panel supernode (kend = 0): z = L_11^{-1} v
+    g16_from_lds<T, NR>(xr, vs, t.nr, cg, j);
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
     g16_mac<T, NR>(acc, im, xr);
     g16_reduce<T, NR>(acc);
-    if (chain) {
-        // z_p = u - M_p z_{p-1}: the only product left once z_{p-1} arrives
-        const T *zk = Z + (t.c0 + t.r0 - kPanel) * nrhs;
-        (void)g16_load_wait<T, NR>(xr, [&](int c) { return zk + (int64_t)c * nrhs; }, kPanel, nrhs, cg, j, abort_flag);
-        T a2[NR];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) a2[r] = T(0);
-        g16_mac<T, NR>(a2, mm, xr);
-        g16_reduce<T, NR>(a2);
-#pragma unroll
-        for (int r = 0; r < NR; ++r) acc[r] -= a2[r];
-    }
     if (i < t.nr) {
 #pragma unroll
         for (int rr = 0; rr < NR / 4; ++rr) {
@@ -1233,31 +1302,36 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
     }
 }
 
-// backward: task (s, p): x_p = L_pp^{-T} (z_p - sum over the rows i below the
-// panel of L[i, p]' x_i): first the update rows (ancestors' values), tile by tile
-// from the last -- the order in which the ancestors' panels publish them -- then
-// the later panels of s from the last, each value awaited as it is published.
+// backward: task (s, p): t_p = z_p - sum over the update rows i of the front of
+// L[i, p]' x_i (ancestors' values), tile by tile from the last -- the order in
+// which the ancestors' panels publish them; then x_p = L_pp^{-T} t_p for a
+// supernode with one panel, and for np >= 2 panels t_p is published to Tb and
+// x_p = sum_{q>p} X[q,p]' t_q + X_pp' t_p (from the last panel, X = L_11^{-1}),
+// the t_q of the later panels polled from Tb: one exchange, no chain.
 // The products are transposed (lane row = panel column): each tile is loaded
 // coalesced, staged through LDS (double-buffered in FP32, so one barrier per
-// tile), and read back transposed before its x values are awaited.
+// tile), and read back transposed before its values are awaited.
 template <class T>
 constexpr int kBwdStage = sizeof(T) == 4 ? 2 : 1;      // staging buffers of the four-wave task (LDS budget)
 template <class T, int NR>
 __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][kTile + 1], T (*vx)[NR], int &failed,
                                               const int32_t *__restrict__ rows, const T *__restrict__ L,
-                                              const T *__restrict__ Linv, const T *__restrict__ Z, T *X, int nrhs,
-                                              int32_t *cnt, int64_t nsn) {
+                                              const T *__restrict__ Linv, const T *__restrict__ Z, T *X, T *Tb,
+                                              int nrhs, int32_t *cnt, int64_t nsn) {
     constexpr int NB = kBwdStage<T>;
     int32_t *abort_flag = cnt + nsn;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, j = lane & 15, cg = lane >> 4;
     const int c = 16 * w + j;                          // panel column of this lane's output
     const int rq = g16_r0<NR>(cg);
     const int nc = t.nr;
+    const bool multi = t.xofs >= 0;
     const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;
-    const int ntile = nchunk + (t.np - 1 - t.p);
+    const int ntile = nchunk + (multi ? t.np - 1 - t.p : 0);
     const T *Lc = L + t.lofs + t.ms * t.r0;            // front row 0, column r0
+    const T *Xc = multi ? Linv + t.xofs + t.ns * t.r0 : Lc;   // X row 0, column r0
     const int32_t *rw = rows + t.rofs;
-    // tile e: update rows [u0 + 64(nchunk-1-e), ...) for e < nchunk, else panel q = np-1-(e-nchunk)
+    // tile e: update rows [u0 + 64(nchunk-1-e), ...) of L for e < nchunk, else the
+    // rows of panel q = np-1-(e-nchunk) of X
     auto tile_row0 = [&](int e) -> int64_t {
         return e < nchunk ? t.u0 + (int64_t)kTile * (nchunk - 1 - e) : (int64_t)kPanel * (t.np - 1 - (e - nchunk));
     };
@@ -1265,30 +1339,48 @@ __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][
         const int64_t r0 = tile_row0(e);
         return (int)min<int64_t>(kTile, (e < nchunk ? (int64_t)t.u1 : t.ns) - r0);
     };
-    const bool chain = t.bofs >= 0;                    // p < np-1: panel p+1 enters through G_p
-    const int nloop = chain ? ntile - 1 : ntile;       // tiles other than panel p+1 (the last one)
-    T lv[16], im[16], gm[16], zv[NR / 4], acc[NR], xr[NR];
+    auto load_tile = [&](T (&v)[16], int e) {
+        if (e < nchunk) load_cols16(v, Lc + tile_row0(e), t.ms, tile_rows(e), nc, lane, w);
+        else load_cols16(v, Xc + tile_row0(e), t.ns, tile_rows(e), nc, lane, w);
+    };
+    T lv[16], im[16], zv[NR / 4], acc[NR], xr[NR];
     g16_load_t(im, Linv + t.iofs, nc, nc, nc, c, cg); // row c of L_pp^{-T}
-    if (chain) g16_load(gm, Linv + t.bofs, kPanel, nc, kPanel, c, cg);
     {
         const T *zp = Z + (t.c0 + t.r0 + min(c, nc - 1)) * nrhs;
 #pragma unroll
         for (int rr = 0; rr < NR / 4; ++rr) zv[rr] = zp[min(rq + rr, nrhs - 1)];
     }
     if (tid == 0) failed = 0;
-    if (nloop > 0) load_cols16(lv, Lc + tile_row0(0), t.ms, tile_rows(0), nc, lane, w);
+    if (ntile > 0) load_tile(lv, 0);
     int32_t ridx[NR];                                  // row indices of the next update-row tile
     if (nchunk > 0) g16_rows<NR>(ridx, rw + tile_row0(0), tile_rows(0), cg, j);
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
+    // t_p = z_p - (update-row sum) into vx, published for the earlier panels of s;
+    // the accumulator restarts for the later panels' sum
+    bool formed = false;
+    auto form_t = [&]() {
+        g16_reduce<T, NR>(acc);
+#pragma unroll
+        for (int rr = 0; rr < NR / 4; ++rr) {
+            const int r = rq + rr;
+            const T tv = (c < nc && r < nrhs) ? zv[rr] - acc[r] : T(0);
+            vx[c][r] = tv;
+            if (multi && c < nc && r < nrhs) st_dev(Tb + (t.c0 + t.r0 + c) * nrhs + r, tv);
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] = T(0);
+        formed = true;
+    };
     bool ok = true;
-    for (int e = 0; e < nloop; ++e) {
+    for (int e = 0; e < ntile; ++e) {
+        if (e == nchunk) form_t();
         const int nre = tile_rows(e);
         T (*S)[kTile + 1] = Lt[e % NB];
 #pragma unroll
         for (int k = 0; k < 16; ++k) S[lane][16 * w + k] = (lane < nre && 16 * w + k < nc) ? lv[k] : T(0);
         __syncthreads();
-        if (e + 1 < nloop) load_cols16(lv, Lc + tile_row0(e + 1), t.ms, tile_rows(e + 1), nc, lane, w);
+        if (e + 1 < ntile) load_tile(lv, e + 1);
         T m[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) m[k] = S[16 * cg + k][c];
@@ -1298,39 +1390,19 @@ __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][
             const int en = min(e + 1, nchunk - 1);
             g16_rows<NR>(ridx, rw + tile_row0(en), tile_rows(en), cg, j);
         } else {
-            const T *xq = X + (t.c0 + tile_row0(e)) * nrhs;
-            ok = g16_load_wait<T, NR>(xr, [&](int k) { return xq + (int64_t)k * nrhs; }, nre, nrhs, cg, j,
+            const T *tq = Tb + (t.c0 + tile_row0(e)) * nrhs;
+            ok = g16_load_wait<T, NR>(xr, [&](int k) { return tq + (int64_t)k * nrhs; }, nre, nrhs, cg, j,
                                       abort_flag) && ok;
         }
         g16_mac<T, NR>(acc, m, xr);
     }
     if (!ok) failed = 1;
-    g16_reduce<T, NR>(acc);
-#pragma unroll
-    for (int rr = 0; rr < NR / 4; ++rr) {
-        const int r = rq + rr;
-        vx[c][r] = (c < nc && r < nrhs) ? zv[rr] - acc[r] : T(0);
-    }
+    if (!formed) form_t();                             // no later panels
     __syncthreads();
     if (failed) return;
     g16_from_lds<T, NR>(xr, vx, nc, cg, j);
-#pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = T(0);
     g16_mac<T, NR>(acc, im, xr);
     g16_reduce<T, NR>(acc);
-    if (chain) {
-        // x_p = u - G_p x_{p+1}: the only product left once x_{p+1} arrives
-        const T *xq = X + (t.c0 + t.r0 + kPanel) * nrhs;
-        const int b1 = (int)min<int64_t>(kPanel, t.ns - t.r0 - kPanel);
-        (void)g16_load_wait<T, NR>(xr, [&](int k) { return xq + (int64_t)k * nrhs; }, b1, nrhs, cg, j, abort_flag);
-        T a2[NR];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) a2[r] = T(0);
-        g16_mac<T, NR>(a2, gm, xr);
-        g16_reduce<T, NR>(a2);
-#pragma unroll
-        for (int r = 0; r < NR; ++r) acc[r] -= a2[r];
-    }
     if (c < nc) {
 #pragma unroll
         for (int rr = 0; rr < NR / 4; ++rr) {
@@ -1541,7 +1613,7 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
                                                        const FlowChild *__restrict__ gch,
                                                        const int32_t *__restrict__ relmap, const T *__restrict__ L,
                                                        const T *__restrict__ Linv, const T *__restrict__ Y, T *W, T *Z,
-                                                       int nrhs, int32_t *cnt, int64_t nsn, int64_t nleaf) {
+                                                       T *V, int nrhs, int32_t *cnt, int64_t nsn, int64_t nleaf) {
     constexpr int NQ = kLeafQuads<T, NR> ? 4 : 1;
     __shared__ __attribute__((aligned(16))) T lds[NQ][kWholeRows + kTile][NR];
     __shared__ int failed_one;                         // (with quads: in the main path's unused quarter)
@@ -1562,7 +1634,7 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
     int *failed;
     if constexpr (NQ > 1) failed = (int *)&lds[1][0][0];
     else failed = &failed_one;
-    fwd_main_task<T, NR>(t, lds[0], lds[0] + kWholeRows, *failed, gch, relmap, L, Linv, Y, W, Z, nrhs, cnt, nsn);
+    fwd_main_task<T, NR>(t, lds[0], lds[0] + kWholeRows, *failed, gch, relmap, L, Linv, Y, W, Z, V, nrhs, cnt, nsn);
 }
 
 // backward task of a one-panel supernode: x = L_11^{-T} (z - L[update rows, :]' x_u),
@@ -1695,8 +1767,8 @@ constexpr bool kLeafPairs = sizeof(T) * NR <= 64;     // (FP64 with 16 right-han
 template <class T, int NR>
 __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restrict__ tasks, const int32_t *__restrict__ rows,
                                                        const T *__restrict__ L, const T *__restrict__ Linv,
-                                                       const T *__restrict__ Z, T *X, int nrhs, int32_t *cnt, int64_t nsn,
-                                                       int64_t nmain, int64_t nleaf) {
+                                                       const T *__restrict__ Z, T *X, T *Tb, int nrhs, int32_t *cnt,
+                                                       int64_t nsn, int64_t nmain, int64_t nleaf) {
     constexpr int NB = kBwdStage<T>;
     constexpr int kMain = NB * kTile * (kTile + 1) + kTile * NR;
     constexpr int kPair = 2 * kTile * 17 + kTile * NR;
@@ -1716,7 +1788,7 @@ __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restric
         return;
     }
     bwd_main_task<T, NR>(tasks[b], (T (*)[kTile][kTile + 1])lds, (T (*)[NR])(lds + NB * kTile * (kTile + 1)),
-                         flags[0], rows, L, Linv, Z, X, nrhs, cnt, nsn);
+                         flags[0], rows, L, Linv, Z, X, Tb, nrhs, cnt, nsn);
 }
 
 // ----------------------------------------------------------------- refinement helpers
@@ -2173,12 +2245,22 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
             lap(FAM_SCHUR);
         }
     }
-    // inverse diagonal blocks and chain products for the triangular solves
+    // inverse diagonal blocks and the X = L_11^{-1} of the multi-panel supernodes
+    // (doubling levels, two phases each) for the triangular solves
     if (solve_blocks && !H.flow_b.empty()) {
         hipLaunchKernelGGL(diag_inverse_kernel<T>, dim3((unsigned)H.flow_b.size(), V), dim3(64), 0, st, S->d_flow_b,
                            (const T *)L, Linv, vs);
-        hipLaunchKernelGGL(chain_product_kernel<T>, dim3((unsigned)H.flow_b.size(), V), dim3(kBS), 0, st, S->d_flow_b,
-                           (const T *)L, Linv, vs);
+        for (size_t q = 0; q + 1 < H.istep_off.size(); ++q) {
+            const int64_t ni = H.istep_off[q + 1] - H.istep_off[q];
+            if (!ni) continue;
+            if (q % 2 == 0)
+                hipLaunchKernelGGL((inv_tile_kernel<T, 0>), dim3((unsigned)ni, V), dim3(kBS), 0, st,
+                                   S->d_itasks + H.istep_off[q], (const T *)L, Linv, vs);
+            else
+                hipLaunchKernelGGL((inv_tile_kernel<T, 1>), dim3((unsigned)ni, V), dim3(kBS), 0, st,
+                                   S->d_itasks + H.istep_off[q], (const T *)L, Linv, vs);
+        }
+        lap(FAM_INV);
     }
     if (detail) {
         lap(-1);
@@ -2189,10 +2271,11 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
 
 // y = (L L')^{-1} y by the dataflow kernels; cnt: ns + 3 words: ns unused, the
 // abort flag (cleared once per refinement call), the two launch tickets (cleared
-// before every solve)
+// before every solve); V and Tb: the v and t values the panels of a multi-panel
+// supernode exchange (forward and backward)
 template <class T, int NR>
 hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_factor *F, const T *Y, T *W, T *Z, T *X,
-                         int nrhs, int32_t *cnt) {
+                         T *V, T *Tb, int nrhs, int32_t *cnt) {
     const SymbolicHost &H = S->H;
     hipStream_t st = ctx->stream;
     const int64_t nsn = H.ns;
@@ -2202,8 +2285,8 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
     // timing level 2: the kernels on the context's event timeline
     const bool detail = ctx->timing_level >= 2;
     if (detail) tl_begin(ctx);
-    hipLaunchKernelGGL(solve_marks_kernel<T>, dim3(grid_for(std::max(nv, nwv))), dim3(kBS), 0, st, Z, X, nv, W, nwv,
-                       pending_mark<T>(), cnt + nsn + 1);
+    hipLaunchKernelGGL(solve_marks_kernel<T>, dim3(grid_for(std::max(nv, nwv))), dim3(kBS), 0, st, Z, X, V, Tb, nv, W,
+                       nwv, pending_mark<T>(), cnt + nsn + 1);
     auto lap = [&](int fam) { tl_mark(ctx, fam); };
     if (detail) lap(-1);
     // forward: the bottom subtrees' tasks (the head of the list) one wavefront
@@ -2214,14 +2297,14 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
         const int64_t nleaf = kLeafQuads<T, NR> ? nfl : 0;
         const int64_t grid = (nleaf + 3) / 4 + (int64_t)H.flow_f.size() - nleaf;
         hipLaunchKernelGGL((fwd_flow_kernel<T, NR>), dim3((unsigned)grid), dim3(kBS), 0, st, S->d_flow_f, S->d_flow_gb,
-                           S->d_relmap, L, Linv, Y, W, Z, nrhs, cnt, nsn, nleaf);
+                           S->d_relmap, L, Linv, Y, W, Z, V, nrhs, cnt, nsn, nleaf);
         if (detail) lap(FAM_FLOW_FWD);
     }
     if (!H.flow_b.empty()) {
         const int64_t nmain = kLeafPairs<T, NR> ? nbm : (int64_t)H.flow_b.size(), nleaf = (int64_t)H.flow_b.size() - nmain;
         const int64_t grid = nmain + (nleaf + 1) / 2;
         hipLaunchKernelGGL((bwd_flow_kernel<T, NR>), dim3((unsigned)grid), dim3(kBS), 0, st, S->d_flow_b, S->d_rows, L,
-                           Linv, (const T *)Z, X, nrhs, cnt, nsn, nmain, nleaf);
+                           Linv, (const T *)Z, X, Tb, nrhs, cnt, nsn, nmain, nleaf);
         if (detail) lap(FAM_FLOW_BWD);
     }
     if (detail) tl_flush(ctx);
@@ -2230,10 +2313,10 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
 
 template <class T>
 hipError_t flow_solve(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_factor *F, const T *Y, T *W, T *Z, T *X,
-                      int nrhs, int32_t *cnt) {
-    if (nrhs <= 4) return flow_solve_nr<T, 4>(ctx, S, F, Y, W, Z, X, nrhs, cnt);
-    if (nrhs <= 8) return flow_solve_nr<T, 8>(ctx, S, F, Y, W, Z, X, nrhs, cnt);
-    return flow_solve_nr<T, 16>(ctx, S, F, Y, W, Z, X, nrhs, cnt);
+                      T *V, T *Tb, int nrhs, int32_t *cnt) {
+    if (nrhs <= 4) return flow_solve_nr<T, 4>(ctx, S, F, Y, W, Z, X, V, Tb, nrhs, cnt);
+    if (nrhs <= 8) return flow_solve_nr<T, 8>(ctx, S, F, Y, W, Z, X, V, Tb, nrhs, cnt);
+    return flow_solve_nr<T, 16>(ctx, S, F, Y, W, Z, X, V, Tb, nrhs, cnt);
 }
 
 template <class V>
@@ -2300,6 +2383,7 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_ptasks, H.ptasks, st);
     if (e == hipSuccess) e = up(S, &S->d_utasks, H.utasks, st);
     if (e == hipSuccess) e = up(S, &S->d_stasks, H.stasks, st);
+    if (e == hipSuccess) e = up(S, &S->d_itasks, H.itasks, st);
     if (e == hipSuccess) e = up(S, &S->d_etasks, H.etasks, st);
     if (e == hipSuccess) e = up(S, &S->d_ea_bounds, H.ea_bounds, st);
     if (e == hipSuccess) e = up(S, &S->d_ea_boff, H.ea_boff, st);
@@ -2603,17 +2687,17 @@ int jfem_factor_inertia(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix
 namespace {
 
 // Solve scratch of one refinement call (in the context's scratch buffer): the
-// factor-precision vectors of the dataflow solves (Y, W, Z, X), their wait words,
+// factor-precision vectors of the dataflow solves (Y, W, Z, X, V, Tb), their wait words,
 // the FP64 iterate, residual and CG vectors (interleaved, n x NR), the global
 // iterate for the SpMV (ndof x NR) and the reduction partials.
 struct SolveLayout {
-    size_t oY, oW, oZs, oXs, oCnt, oXh, oXl, oR, oB, oD, oSv, oPv, oQv, oZv, oXgh, oXgl, oPg;
+    size_t oY, oW, oZs, oXs, oVs, oTs, oCnt, oXh, oXl, oR, oB, oD, oSv, oPv, oQv, oZv, oXgh, oXgl, oPg;
     size_t oPart, oNrm, oRho, oNum, oAl, oBe, oActO, oActI, oTicket, bytes = 0;
     SolveLayout(const SymbolicHost &H, int64_t ndof, int NR, size_t es) {
         const size_t nv = (size_t)H.n * NR, ng = (size_t)ndof * NR;
         auto take = [&](size_t b) { size_t o = bytes; bytes += (b + 255) & ~(size_t)255; return o; };
         oY = take(nv * es); oW = take((size_t)std::max<int64_t>(H.upd_rows, 1) * NR * es);
-        oZs = take(nv * es); oXs = take(nv * es);
+        oZs = take(nv * es); oXs = take(nv * es); oVs = take(nv * es); oTs = take(nv * es);
         oCnt = take((H.ns + 3) * sizeof(int32_t));
         oXh = take(nv * 8); oXl = take(nv * 8); oR = take(nv * 8); oB = take(nv * 8); oD = take(nv * 8);
         oSv = take(nv * 8); oPv = take(nv * 8); oQv = take(nv * 8); oZv = take(nv * 8);
@@ -2665,6 +2749,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
         base = (char *)ctx->scratch;
     }
     T *Y = (T *)(base + lay.oY), *W = (T *)(base + lay.oW), *Zs = (T *)(base + lay.oZs), *Xs = (T *)(base + lay.oXs);
+    T *Vs = (T *)(base + lay.oVs), *Ts = (T *)(base + lay.oTs);
     int32_t *cnt = (int32_t *)(base + lay.oCnt);
     double *Xh = (double *)(base + lay.oXh), *Xl = (double *)(base + lay.oXl), *R = (double *)(base + lay.oR);
     double *Bp = (double *)(base + lay.oB), *Dd = (double *)(base + lay.oD), *Sv = (double *)(base + lay.oSv);
@@ -2703,7 +2788,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
         auto prec = [&](const double *sv, double *z) -> hipError_t {
             timer_start(ctx);
             hipLaunchKernelGGL(to_t_kernel<T>, gv, blk, 0, st, nv, sv, Y);
-            hipError_t e = flow_solve<T>(ctx, S, F, Y, W, Zs, Xs, NR, cnt);
+            hipError_t e = flow_solve<T>(ctx, S, F, Y, W, Zs, Xs, Vs, Ts, NR, cnt);
             hipLaunchKernelGGL(from_t_kernel<T>, gv, blk, 0, st, nv, (const T *)Xs, z);
             timer_stop(ctx, FAM_SOLVE, 1);
             return e;

@@ -186,9 +186,11 @@ static void order_flow_lists(SymbolicHost &S, const std::vector<int32_t> &snode_
                 key[i] = ready;
                 out_fin[t.s] = ready + base + tile * (double)(t.ms - t.ns) / kTile;
             } else if (t.p >= 0) {
+                // the panels of s run side by side: each publishes its v, then
+                // forms z_p from the v of the panels before it (one exchange)
                 if (t.p == 0) chain[t.s] = ready;
                 key[i] = chain[t.s] + eps * t.p;
-                last_fin[t.s] = (t.p == 0 ? ready : last_fin[t.s] + lat) + base;
+                last_fin[t.s] = ready + (t.np > 1 ? lat : 0.0) + base;
             } else {
                 key[i] = chain[t.s] + eps * (t.np + 1);
                 out_fin[t.s] = std::max({out_fin[t.s], ready + base + tile * t.np, last_fin[t.s] + lat + tile});
@@ -719,17 +721,48 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         }
         S.iofs[k + 1] = S.iofs[k] + e;
     }
-    // chain products after the inverses: per supernode, boundary q between
-    // panels q and q+1 holds [M_{q+1}, G_q], 2 x 64 x 64 entries
-    std::vector<int64_t> bofs(nsn, -1);
-    int64_t nbnd = S.iofs[nsn];
+    // X = L_11^{-1} of every supernode with two or more panels, after the
+    // diagonal-block inverses (ns x ns each, 64-entry aligned)
+    std::vector<int64_t> xofs(nsn, -1);
+    int64_t nx = (S.iofs[nsn] + 63) & ~(int64_t)63;
+    int32_t max_np = 0;
     for (int64_t k = 0; k < nsn; ++k) {
-        const int64_t np = (S.ncol[k] + kPanel - 1) / kPanel;
+        const int32_t np = npan(k);
+        max_np = std::max(max_np, np);
         if (np < 2) continue;
-        bofs[k] = nbnd;
-        nbnd += (np - 1) * 2 * kPanel * kPanel;
+        xofs[k] = nx;
+        nx += (S.ncol[k] * S.ncol[k] + 63) & ~(int64_t)63;
     }
-    S.inv_entries = nbnd;
+    S.inv_entries = nx;
+    // inversion tasks by doubling (InvTask); within a launch the longest k ranges first
+    S.itasks.clear();
+    S.istep_off.assign(1, 0);
+    S.inv_flops = 0;
+    for (int32_t B = 1; B < max_np; B *= 2)
+        for (int ph = 0; ph < 2; ++ph) {
+            const size_t first = S.itasks.size();
+            for (int64_t s = 0; s < nsn; ++s) {
+                const int32_t np = npan(s);
+                if (np <= B) continue;
+                for (int32_t a = 0; a + B < np; a += 2 * B) {
+                    const int32_t c1 = std::min(a + 2 * B, np);
+                    for (int32_t i = a + B; i < c1; ++i)
+                        for (int32_t j = a; j < a + B; ++j) {
+                            InvTask t;
+                            t.lofs = S.lofs[s]; t.xofs = xofs[s];
+                            t.ns = (int32_t)S.ncol[s]; t.ms = (int32_t)S.m[s];
+                            t.i = i; t.j = j;
+                            t.k0 = ph == 0 ? j : a + B;
+                            t.k1 = ph == 0 ? a + B : i + 1;
+                            S.itasks.push_back(t);
+                            S.inv_flops += 2.0 * kPanel * kPanel * kPanel * (t.k1 - t.k0);
+                        }
+                }
+            }
+            std::stable_sort(S.itasks.begin() + first, S.itasks.end(),
+                             [](const InvTask &x, const InvTask &y) { return x.k1 - x.k0 > y.k1 - y.k0; });
+            S.istep_off.push_back((int64_t)S.itasks.size());
+        }
     auto whole = [&](int64_t s) { return S.ncol[s] <= kPanel && S.m[s] <= kWholeRows; };
     std::vector<int32_t> need(nsn, 0);
     for (int64_t k = 0; k < nsn; ++k)
@@ -768,7 +801,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
             const int32_t s = S.level_sn[q];
             FlowTask t = base(s);
-            t.bofs = -1;
+            t.xofs = -1;
             if (whole(s)) {
                 t.kind = 1;
                 t.p = 0; t.r0 = 0; t.nr = (int32_t)S.ncol[s];
@@ -781,11 +814,11 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 t.r0 = p * kPanel;
                 t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
                 t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
-                t.bofs = p >= 1 ? bofs[s] + 2 * (int64_t)kPanel * kPanel * (p - 1) : -1;     // M_p
+                t.xofs = xofs[s];
                 add_fwd(t, (int64_t)t.r0 + t.nr);
             }
             t.iofs = 0;
-            t.bofs = -1;
+            t.xofs = -1;
             for (int32_t u = 0; u < nupd(s); ++u) {
                 t.p = -1 - u;
                 t.r0 = (int32_t)(S.ncol[s] + (int64_t)u * kTile);
@@ -804,7 +837,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 t.r0 = p * kPanel;
                 t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
                 t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
-                t.bofs = p + 1 < t.np ? bofs[s] + 2 * (int64_t)kPanel * kPanel * p + kPanel * kPanel : -1;   // G_p
+                t.xofs = xofs[s];
                 S.flow_b.push_back(t);
             }
         }

@@ -29,12 +29,10 @@ struct FlowTask {
     int32_t u0, u1;                 // backward: update rows [u0, u1) of the front
     int64_t ns, ms, lofs, c0, relofs, rofs;
     int64_t iofs;                   // inverse of the diagonal block of panel p
-    // chain product of the task's last awaited panel (-1 if none): forward pivot
-    // task, p >= 1: M_p = L_pp^{-1} L[p rows, p-1 cols] (64 x 64, column-major);
-    // backward task, p < np-1: G_p = (L[p+1 rows, p cols] L_pp^{-1})^T (64 x 64,
-    // column-major).  Both live in the inverse array after the diagonal inverses:
-    // per supernode and panel boundary q, [M_{q+1}, G_q].
-    int64_t bofs;
+    // supernode with np >= 2 panels (pivot and backward tasks; -1 otherwise): the
+    // offset in the inverse array of X = L_11^{-1}, ns x ns column-major, of which
+    // the solves read the lower blocks (the upper ones are the inversion's scratch)
+    int64_t xofs;
     int64_t gb;                     // forward: first FlowChild entry
 };
 // per (forward task, child of its supernode), in child order: the child's update
@@ -42,6 +40,18 @@ struct FlowTask {
 struct FlowChild {
     int64_t relofs;
     int32_t i0, i1;
+};
+// Inversion of L_11 for a supernode with np >= 2 panels, by doubling: at block
+// size B panels, for each pair of consecutive diagonal superblocks A = [a, a+B),
+// C = [a+B, min(a+2B, np)) (X_AA and X_CC already inverted),
+//   phase 0: T = L_CA X_AA,  T[i,j] = sum_{k=j}^{a+B-1} L[i,k] X[k,j],
+//   phase 1: X_CA = -X_CC T, X[i,j] = -sum_{k=a+B}^{i} X[i,k] T[k,j],
+// one task per 64 x 64 output tile (i in C, j in A, panel indices), k over
+// [k0, k1).  T[i,j] is kept transposed in the upper blocks of X (at rows of
+// panel j, columns of panel i), which the solves never read.
+struct InvTask {
+    int64_t lofs, xofs;
+    int32_t ns, ms, i, j, k0, k1;
 };
 
 struct SymbolicHost {
@@ -88,7 +98,12 @@ struct SymbolicHost {
     int64_t flow_f_leaf = 0, flow_b_leaf = 0;
     std::vector<FlowChild> flow_gb;
     std::vector<int64_t> iofs;
-    int64_t inv_entries = 0;        // diagonal-block inverses, then the chain products (bofs)
+    int64_t inv_entries = 0;        // diagonal-block inverses, then the X of the multi-panel supernodes (xofs)
+    // inversion tasks: per doubling level, phase 0 then phase 1 (istep_off: one
+    // offset per (level, phase), plus the end)
+    std::vector<InvTask> itasks;
+    std::vector<int64_t> istep_off;
+    double inv_flops = 0;      // 2 x multiply-adds of the inversion tiles (full 64 x 64 x 64 products)
     int64_t L_entries = 0, U_entries = 0, upd_rows = 0;
     double flops = 0;
     double update_flops = 0;   // 2 x multiply-adds of the per-step tiles (lower triangles)

@@ -139,6 +139,15 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
     // land inside the task's rows (8).
     {
         std::vector<double> Linv(std::max<int64_t>(S.inv_entries, 1), 0.0);
+        // X = L_11^{-1} regions of the multi-panel supernodes: in range, disjoint
+        // from the diagonal inverses; per block the launch that wrote it (lower
+        // blocks: X, upper blocks: the transposed T of the inversion)
+        std::vector<std::vector<int>> xw(S.ns);
+        for (const FlowTask &t : S.flow_b) {
+            if ((t.xofs >= 0) != (t.np >= 2)) return 10;
+            if (t.xofs >= 0 && (t.xofs < S.iofs[S.ns] || t.xofs + t.ns * t.ns > S.inv_entries)) return 10;
+            if (t.xofs >= 0 && xw[t.s].empty()) xw[t.s].assign((size_t)t.np * t.np, -1);
+        }
         for (const FlowTask &t : S.flow_b) {
             const int b = t.nr;
             const int64_t k = t.r0, ms = t.ms;
@@ -151,35 +160,88 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                     for (int j = 0; j < i; ++j) v -= L[t.lofs + (k + i) + ms * (k + j)] * xc[j];
                     xc[i] = v / L[t.lofs + (k + i) + ms * (k + i)];
                 }
-                for (int i = 0; i < b; ++i) Linv[t.iofs + i + (int64_t)b * c] = xc[i];
+                for (int i = 0; i < b; ++i) {
+                    Linv[t.iofs + i + (int64_t)b * c] = xc[i];
+                    if (t.xofs >= 0) Linv[t.xofs + (k + i) + t.ns * (k + c)] = xc[i];
+                }
             }
+            if (t.xofs >= 0) xw[t.s][(size_t)t.p * t.np + t.p] = 0;
         }
-        // chain products: boundary q of s (backward task (s, q), q < np-1):
-        // M_{q+1} = L_{q+1,q+1}^{-1} A at bofs - 4096, G_q = (A L_qq^{-1})^T at bofs,
-        // A = L[q+1 rows, q cols], both 64 x 64 column-major
-        const int64_t PP = (int64_t)kPanel * kPanel;
-        for (const FlowTask &t : S.flow_b) {
-            if ((t.bofs >= 0) != (t.p + 1 < t.np)) return 10;
-            if (t.bofs < 0) continue;
-            if (t.bofs - PP < 0 || t.bofs + PP > S.inv_entries || t.nr != kPanel) return 10;
-            const int b1 = (int)std::min<int64_t>(kPanel, t.ns - (int64_t)kPanel * (t.p + 1));
-            if (t.iofs + PP + (int64_t)b1 * b1 > S.inv_entries) return 10;
-            auto A = [&](int k, int m) { return k < b1 ? L[t.lofs + (kPanel * (t.p + 1) + k) + t.ms * (kPanel * t.p + m)] : 0.0; };
-            for (int c = 0; c < kPanel; ++c)
-                for (int i = 0; i < kPanel; ++i) {
-                    double sm = 0, sg = 0;
-                    if (i < b1)
-                        for (int m = 0; m <= i; ++m) sm += Linv[t.iofs + PP + i + (int64_t)b1 * m] * A(m, c);
-                    if (c < b1)
-                        for (int m = i; m < kPanel; ++m) sg += A(c, m) * Linv[t.iofs + m + (int64_t)kPanel * i];
-                    Linv[t.bofs - PP + i + kPanel * c] = sm;
-                    Linv[t.bofs + i + kPanel * c] = sg;
+        // the inversion launches (InvTask), sequentially, each tile as the kernel
+        // forms it: loads clamped into the matrix, entries beyond ns masked; a
+        // tile may read only blocks written by an earlier launch (9)
+        {
+            if (S.istep_off.empty() || (S.istep_off.size() - 1) % 2 != 0) return 10;
+            std::vector<int64_t> snode_of_lofs;
+            for (size_t q = 0; q + 1 < S.istep_off.size(); ++q) {
+                const int ph = (int)(q % 2), launch = (int)q + 1;
+                for (int64_t e = S.istep_off[q]; e < S.istep_off[q + 1]; ++e) {
+                    const InvTask t = S.itasks[e];
+                    int64_t s = -1;
+                    for (int64_t c = 0; c < S.ns; ++c)
+                        if (S.lofs[c] == t.lofs && (int64_t)t.ns == S.ncol[c]) { s = c; break; }
+                    if (s < 0 || xw[s].empty()) return 10;
+                    const int np = (int)((t.ns + kPanel - 1) / kPanel);
+                    if (t.i <= t.j || t.i >= np || t.j < 0 || t.k0 >= t.k1 || t.k0 < 0 || t.k1 > np) return 10;
+                    const int64_t ns = t.ns, ms = t.ms, r0 = (int64_t)kPanel * t.i, c0 = (int64_t)kPanel * t.j;
+                    const int nr = (int)std::min<int64_t>(kPanel, ns - r0);
+                    if (c0 + kPanel > ns) return 10;                 // panel j is never the partial one
+                    std::vector<double> C((size_t)kTile * kTile, 0.0);
+                    for (int k = t.k0; k < t.k1; ++k) {
+                        const int64_t kc = (int64_t)kPanel * k;
+                        const int kb = (int)std::min<int64_t>(kPanel, ns - kc);
+                        if (ph == 0) {
+                            if (k < t.j || xw[s][(size_t)k * np + t.j] < 0 || xw[s][(size_t)k * np + t.j] >= launch) return 9;
+                        } else {
+                            if (k > t.i || xw[s][(size_t)t.i * np + k] < 0 || xw[s][(size_t)t.i * np + k] >= launch) return 9;
+                            if (xw[s][(size_t)t.j * np + k] < 0 || xw[s][(size_t)t.j * np + k] >= launch) return 9;
+                        }
+                        for (int y = 0; y < kTile; ++y)         // every load the kernel forms
+                            for (int x = 0; x < kTile; ++x) {
+                                const int64_t ra = r0 + std::min(x, nr - 1), ka = kc + std::min(y, kb - 1);
+                                const int64_t ia = ph == 0 ? t.lofs + ra + ms * ka : t.xofs + ra + ns * ka;
+                                const int64_t ib = ph == 0 ? t.xofs + (kc + std::min(x, kb - 1)) + ns * (c0 + y)
+                                                           : t.xofs + (c0 + x) + ns * ka;
+                                if (ph == 0 && (ia < t.lofs || ia >= t.lofs + ms * ns || ia >= S.L_entries)) return 10;
+                                if (ph == 1 && (ia < t.xofs || ia >= t.xofs + ns * ns)) return 10;
+                                if (ib < t.xofs || ib >= t.xofs + ns * ns) return 10;
+                            }
+                        for (int c = 0; c < kTile; ++c)
+                            for (int r = 0; r < nr; ++r) {
+                                double a = 0;
+                                for (int kk = 0; kk < kb; ++kk) {
+                                    const double av = ph == 0 ? L[t.lofs + (r0 + r) + ms * (kc + kk)]
+                                                              : Linv[t.xofs + (r0 + r) + ns * (kc + kk)];
+                                    const double bv = ph == 0 ? Linv[t.xofs + (kc + kk) + ns * (c0 + c)]
+                                                              : Linv[t.xofs + (c0 + c) + ns * (kc + kk)];
+                                    a += av * bv;
+                                }
+                                C[r + (size_t)kTile * c] += a;
+                            }
+                    }
+                    for (int c = 0; c < kTile; ++c)
+                        for (int r = 0; r < nr; ++r) {
+                            if (ph == 0) Linv[t.xofs + (c0 + c) + ns * (r0 + r)] = C[r + (size_t)kTile * c];
+                            else Linv[t.xofs + (r0 + r) + ns * (c0 + c)] = -C[r + (size_t)kTile * c];
+                        }
+                    int &f = ph == 0 ? xw[s][(size_t)t.j * np + t.i] : xw[s][(size_t)t.i * np + t.j];
+                    if (f >= 0 && ph == 1) return 11;                // each X block written once
+                    f = launch;
+                }
+            }
+            for (int64_t s = 0; s < S.ns; ++s)                   // every lower block of every X written
+                if (!xw[s].empty()) {
+                    const int np = (int)((S.ncol[s] + kPanel - 1) / kPanel);
+                    for (int i = 0; i < np; ++i)
+                        for (int j = 0; j <= i; ++j)
+                            if (xw[s][(size_t)i * np + j] < 0) return 11;
                 }
         }
         const int64_t nsn = S.ns;
-        std::vector<int32_t> ready(nsn, 0), zdone(nsn, 0), xdone(nsn, 0);
+        std::vector<int32_t> ready(nsn, 0), zdone(nsn, 0), xdone(nsn, 0), vdone(nsn, 0);
         std::vector<double> Y2(n * nrhs), W2(nw * nrhs, 0.0), Z2(n * nrhs, 0.0), X2(n * nrhs, 0.0);
-        std::vector<char> wW2(nw, 0), wZ2(n, 0), wX2(n, 0);
+        std::vector<double> V2(n * nrhs, 0.0), T2(n * nrhs, 0.0);
+        std::vector<char> wW2(nw, 0), wZ2(n, 0), wX2(n, 0), wV2(n, 0), wT2(n, 0);
         for (int64_t i = 0; i < n; ++i)
             for (int r = 0; r < nrhs; ++r) Y2[i * nrhs + r] = b[S.perm[i] + ndof * r];
         // an L index formed for block (s, rows [r0, r0+nr), cols [c, c+nc)), clamped as load_cols16 does
@@ -207,7 +269,7 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             for (int64_t e = 0; e < (int64_t)S.flow_b.size(); ++e) {
                 const FlowTask &t = S.flow_b[e];
                 if ((e >= b0) != (leaf[t.s] != 0)) return 12;
-                if (e >= b0 && (t.np != 1 || t.p != 0 || t.bofs >= 0 || t.u0 != t.ns || t.u1 != t.ms)) return 12;
+                if (e >= b0 && (t.np != 1 || t.p != 0 || t.xofs >= 0 || t.u0 != t.ns || t.u1 != t.ms)) return 12;
             }
             for (int64_t s = 0; s < nsn; ++s)
                 if (leaf[s])
@@ -217,10 +279,9 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
         for (size_t e = 0; e < S.flow_f.size(); ++e) {
             const FlowTask &t = S.flow_f[e];
             const bool pivot = t.p >= 0;
-            const bool chain = pivot && t.bofs >= 0;
-            if (chain != (pivot && t.kind != 1 && t.p >= 1)) return 10;
-            if (chain && (t.bofs < 0 || t.bofs + PP > S.inv_entries)) return 10;
-            const int kend = pivot ? (chain ? t.p - 1 : t.p) : t.np;
+            const bool multi = pivot && t.xofs >= 0;     // z_p through X and the v of the earlier panels
+            if (multi != (pivot && t.kind != 1 && t.np >= 2)) return 10;
+            const int kend = pivot ? t.p : t.np;
             const int nr = t.nr;
             if (t.s < 0 || t.s >= nsn || nr < 1 || nr > kTile) return 10;
             if (pivot && (t.r0 != kPanel * t.p || t.r0 + nr > t.ns)) return 10;
@@ -278,20 +339,60 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 if (t.parent >= 0) ready[t.parent] += 1;
                 continue;
             }
+            if (multi) {                                 // v_p published for the later panels of s
+                if (vdone[t.s] != t.p) return 9;
+                for (int i = 0; i < nr; ++i) {
+                    const int64_t vi = t.c0 + t.r0 + i;
+                    if (wV2[vi]) return 11;
+                    wV2[vi] = 1;
+                    for (int r = 0; r < nrhs; ++r) V2[vi * nrhs + r] = vs[i * nrhs + r];
+                }
+                vdone[t.s] = t.p + 1;
+            }
             std::vector<double> acc(4 * kTile * nrhs, 0.0);
             for (int k = 0; k < kend; ++k) {
-                if (zdone[t.s] < k + 1) return 9;
                 const int64_t col = (int64_t)kPanel * k;
                 const int bk = (int)std::min<int64_t>(kPanel, t.ns - col);
-                if (!Lblock_ok(t, t.r0, nr, col, bk)) return 10;
+                if (multi) {
+                    if (vdone[t.s] < k + 1) return 9;
+                    const int64_t lo = t.xofs + t.r0 + t.ns * col, hi = lo + (nr - 1) + t.ns * (bk - 1);
+                    if (lo < t.xofs || hi >= t.xofs + t.ns * t.ns || k >= t.p) return 10;
+                } else {
+                    if (zdone[t.s] < k + 1) return 9;
+                    if (!Lblock_ok(t, t.r0, nr, col, bk)) return 10;
+                }
+                const std::vector<double> &src = multi ? V2 : Z2;
                 for (int c = 0; c < bk; ++c)
-                    if (!wZ2[t.c0 + col + c]) return 11;
+                    if (!(multi ? wV2 : wZ2)[t.c0 + col + c]) return 11;
                 for (int w = 0; w < 4; ++w)
                     for (int i = 0; i < nr; ++i)
-                        for (int c = 16 * w; c < std::min(16 * w + 16, bk); ++c)
+                        for (int c = 16 * w; c < std::min(16 * w + 16, bk); ++c) {
+                            const double m = multi ? Linv[t.xofs + (t.r0 + i) + t.ns * (col + c)]
+                                                   : L[t.lofs + (t.r0 + i) + t.ms * (col + c)];
                             for (int r = 0; r < nrhs; ++r)
-                                acc[(w * kTile + i) * nrhs + r] +=
-                                    L[t.lofs + (t.r0 + i) + t.ms * (col + c)] * Z2[(t.c0 + col + c) * nrhs + r];
+                                acc[(w * kTile + i) * nrhs + r] += m * src[(t.c0 + col + c) * nrhs + r];
+                        }
+            }
+            if (pivot && (t.iofs < 0 || t.iofs + (int64_t)nr * nr > S.inv_entries)) return 10;
+            if (multi) {
+                // + X_pp v_p, one reduction for the whole sum
+                for (int w = 0; w < 4; ++w)
+                    for (int i = 0; i < nr; ++i)
+                        for (int c = 16 * w; c < std::min(16 * w + 16, nr); ++c)
+                            for (int r = 0; r < nrhs; ++r)
+                                acc[(w * kTile + i) * nrhs + r] += Linv[t.iofs + i + (int64_t)nr * c] * vs[c * nrhs + r];
+                for (int i = 0; i < nr; ++i) {
+                    const int64_t zi = t.c0 + t.r0 + i;
+                    if (wZ2[zi]) return 11;
+                    wZ2[zi] = 1;
+                    for (int r = 0; r < nrhs; ++r) {
+                        const double *a = &acc[i * nrhs + r];
+                        const int64_t st = (int64_t)kTile * nrhs;
+                        Z2[zi * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                    }
+                }
+                zdone[t.s] += 1;
+                continue;
             }
             std::vector<double> v(kTile * nrhs, 0.0);
             for (int i = 0; i < nr; ++i)
@@ -310,8 +411,8 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 ready[t.parent] += 1;
                 continue;
             }
-            if (t.iofs < 0 || t.iofs + (int64_t)nr * nr > S.inv_entries) return 10;
-            if (zdone[t.s] != t.p) return 9;
+            // one-panel supernode: z = L_11^{-1} v
+            if (t.np != 1 || t.p != 0) return 10;
             std::fill(acc.begin(), acc.end(), 0.0);
             for (int w = 0; w < 4; ++w)
                 for (int i = 0; i < nr; ++i)
@@ -325,17 +426,10 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 for (int r = 0; r < nrhs; ++r) {
                     const double *a = &acc[i * nrhs + r];
                     const int64_t st = (int64_t)kTile * nrhs;
-                    double z = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
-                    if (chain)      // z_p = u - M_p z_{p-1}
-                        for (int c = 0; c < kPanel; ++c) {
-                            const int64_t zc = t.c0 + t.r0 - kPanel + c;
-                            if (!wZ2[zc]) return 11;
-                            z -= Linv[t.bofs + i + kPanel * c] * Z2[zc * nrhs + r];
-                        }
-                    Z2[zi * nrhs + r] = z;
+                    Z2[zi * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
                 }
             }
-            zdone[t.s] = t.p + 1;
+            zdone[t.s] = 1;
         }
         for (int64_t i = 0; i < n; ++i)
             if (!wZ2[i]) return 11;
@@ -344,24 +438,19 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             const int nc = t.nr;
             if (t.parent >= 0 && t.pnp != (int32_t)((S.ncol[t.parent] + kPanel - 1) / kPanel)) return 10;
             if (t.u0 != t.ns || t.u1 != t.ms) return 10;
+            const bool multi = t.xofs >= 0;              // x_p through X and the t of the later panels
+            if (multi != (t.np >= 2)) return 10;
+            if (xdone[t.s] != t.np - 1 - t.p) return 9;
             std::vector<double> acc(4 * kTile * nrhs, 0.0);
             const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;
-            const int ntile = nchunk + (t.np - 1 - t.p);
-            const bool chain = t.bofs >= 0;              // panel p+1 (the last tile) through G_p
-            for (int e2 = 0; e2 < (chain ? ntile - 1 : ntile); ++e2) {
-                const bool upd = e2 < nchunk;
-                const int q = upd ? -1 : t.np - 1 - (e2 - nchunk);
-                const int64_t row0 = upd ? t.u0 + (int64_t)kTile * (nchunk - 1 - e2) : (int64_t)kPanel * q;
-                const int nre = (int)std::min<int64_t>(kTile, (upd ? (int64_t)t.u1 : t.ns) - row0);
-                if (!upd && xdone[t.s] < t.np - q) return 9;
-                if (!Lblock_ok(t, row0, nre, t.r0, nc)) {
-                    // update rows lie below the pivot columns: check against the whole front
-                    const int64_t lo = t.lofs + row0 + t.ms * t.r0, hi = lo + (nre - 1) + t.ms * (nc - 1);
-                    if (!(row0 >= 0 && row0 + nre <= t.ms && hi < t.lofs + t.ms * t.ns)) return 10;
-                }
+            for (int e2 = 0; e2 < nchunk; ++e2) {      // update rows (ancestors' values), from the last tile
+                const int64_t row0 = t.u0 + (int64_t)kTile * (nchunk - 1 - e2);
+                const int nre = (int)std::min<int64_t>(kTile, (int64_t)t.u1 - row0);
+                const int64_t lo = t.lofs + row0 + t.ms * t.r0, hi = lo + (nre - 1) + t.ms * (nc - 1);
+                if (!(row0 >= 0 && row0 + nre <= t.ms && hi < t.lofs + t.ms * t.ns)) return 10;
                 for (int i = 0; i < nre; ++i) {
-                    if (upd && (t.rofs + row0 + i >= (int64_t)S.rows.size())) return 10;
-                    const int64_t g = upd ? S.rows[t.rofs + row0 + i] : t.c0 + row0 + i;
+                    if (t.rofs + row0 + i >= (int64_t)S.rows.size()) return 10;
+                    const int64_t g = S.rows[t.rofs + row0 + i];
                     if (g < 0 || g >= n || !wX2[g]) return 11;
                     for (int c = 0; c < nc; ++c) {
                         const int w = i / 16;
@@ -370,20 +459,40 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                     }
                 }
             }
-            std::vector<double> v(kTile * nrhs, 0.0);
+            std::vector<double> v(kTile * nrhs, 0.0);     // t_p
             for (int c = 0; c < nc; ++c)
                 for (int r = 0; r < nrhs; ++r) {
                     const double *a = &acc[c * nrhs + r];
                     const int64_t st = (int64_t)kTile * nrhs;
                     v[c * nrhs + r] = Z2[(t.c0 + t.r0 + c) * nrhs + r] - (((a[0] + a[st]) + a[2 * st]) + a[3 * st]);
                 }
+            if (multi)
+                for (int c = 0; c < nc; ++c) {
+                    const int64_t ti = t.c0 + t.r0 + c;
+                    if (wT2[ti]) return 11;
+                    wT2[ti] = 1;
+                    for (int r = 0; r < nrhs; ++r) T2[ti * nrhs + r] = v[c * nrhs + r];
+                }
             std::fill(acc.begin(), acc.end(), 0.0);
+            if (multi)
+                for (int q = t.np - 1; q > t.p; --q) {   // later panels of X, from the last
+                    const int64_t row0 = (int64_t)kPanel * q;
+                    const int nre = (int)std::min<int64_t>(kTile, t.ns - row0);
+                    const int64_t lo = t.xofs + row0 + t.ns * t.r0, hi = lo + (nre - 1) + t.ns * (nc - 1);
+                    if (lo < t.xofs || hi >= t.xofs + t.ns * t.ns) return 10;
+                    for (int i = 0; i < nre; ++i) {
+                        const int64_t g = t.c0 + row0 + i;
+                        if (!wT2[g]) return 11;
+                        for (int c = 0; c < nc; ++c)
+                            for (int r = 0; r < nrhs; ++r)
+                                acc[((i / 16) * kTile + c) * nrhs + r] +=
+                                    Linv[t.xofs + (row0 + i) + t.ns * (t.r0 + c)] * T2[g * nrhs + r];
+                    }
+                }
             for (int i = 0; i < nc; ++i)
                 for (int c = 0; c < nc; ++c)
                     for (int r = 0; r < nrhs; ++r)
                         acc[((i / 16) * kTile + c) * nrhs + r] += Linv[t.iofs + i + (int64_t)nc * c] * v[i * nrhs + r];
-            if (xdone[t.s] != t.np - 1 - t.p) return 9;
-            const int b1 = chain ? (int)std::min<int64_t>(kPanel, t.ns - t.r0 - kPanel) : 0;
             for (int c = 0; c < nc; ++c) {
                 const int64_t xi = t.c0 + t.r0 + c;
                 if (wX2[xi]) return 11;
@@ -391,13 +500,7 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 for (int r = 0; r < nrhs; ++r) {
                     const double *a = &acc[c * nrhs + r];
                     const int64_t st = (int64_t)kTile * nrhs;
-                    double x = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
-                    for (int k = 0; k < b1; ++k) {     // x_p = u - G_p x_{p+1}
-                        const int64_t xk = t.c0 + t.r0 + kPanel + k;
-                        if (!wX2[xk]) return 11;
-                        x -= Linv[t.bofs + c + kPanel * k] * X2[xk * nrhs + r];
-                    }
-                    X2[xi * nrhs + r] = x;
+                    X2[xi * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
                 }
             }
             xdone[t.s] += 1;

@@ -48,6 +48,8 @@ MODELS = {
     "box": lambda: meshes.wing_box(24, 4, tria_fraction=0.1, nrhs=3),
     "block": lambda: meshes.solid_block(3, nrhs=2),
     "stiffened": lambda: meshes.stiffened_plate(12, n_stiff=2, blade_cells=2, nrhs=2),
+    # a six-panel separator: inversion by doubling at block sizes 1, 2 and 4 (uneven last pair)
+    "block5": lambda: meshes.solid_block(5, nrhs=2),
 }
 
 
