@@ -335,7 +335,7 @@ def main():
                          "variants of a 128x128 stiffened plate, 4 RHS, one symbolic analysis; serving: Poisson "
                          "stream of shell requests (2k-200k DOFs, 10% CTRIA3, 1-16 RHS)")
     ap.add_argument("--requests", type=int, default=512, help="serving: number of requests")
-    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent request slots (HIP streams)")
+    ap.add_argument("--slots", type=int, default=8, help="serving: concurrent request slots (HIP streams)")
     ap.add_argument("--utilization", type=float, default=0.7, help="serving: offered load / measured service rate")
     ap.add_argument("--variants", type=int, default=64, help="variants: number of design variants")
     ap.add_argument("--nx", type=int, default=256, help="spanwise cells (256 = the benchmark workload)")

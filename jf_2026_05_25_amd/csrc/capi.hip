@@ -85,7 +85,8 @@ int jfem_ctx_create(int device, jfem_ctx **out) {
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
-        hipMalloc((void **)&c->d_status, 2 * sizeof(int64_t)) != hipSuccess) {
+        hipMalloc((void **)&c->d_status, 2 * sizeof(int64_t)) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_red, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         delete c;
         return JFEM_ERR_HIP;
     }
@@ -100,6 +101,7 @@ int jfem_ctx_destroy(jfem_ctx *ctx) {
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->bc_dev) (void)hipFree(ctx->bc_dev);
     if (ctx->bc_host) (void)hipHostFree(ctx->bc_host);
+    if (ctx->h_red) (void)hipHostFree(ctx->h_red);
     (void)hipFree(ctx->d_status);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);

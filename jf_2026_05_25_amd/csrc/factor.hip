@@ -1919,7 +1919,7 @@ __global__ void __launch_bounds__(kBS) residual_kernel(int64_t n, const int32_t 
 // sh: the block's reduction buffer (free once its partials are stored).
 template <int NR>
 __device__ __forceinline__ void reduce_last_block(double (*sh)[kBS], double *part, double *out,
-                                                  unsigned *ticket) {
+                                                  unsigned *ticket, double *host_out = nullptr) {
     __shared__ bool last;
     if (threadIdx.x < NR) {
         st_dev(part + (int64_t)threadIdx.x * gridDim.x + blockIdx.x, sh[threadIdx.x][0]);
@@ -1941,14 +1941,17 @@ __device__ __forceinline__ void reduce_last_block(double (*sh)[kBS], double *par
         __syncthreads();
     }
     if (threadIdx.x < NR) out[threadIdx.x] = sh[threadIdx.x][0];
+    if (host_out && threadIdx.x < NR) host_out[threadIdx.x] = sh[threadIdx.x][0];
     if (threadIdx.x == 0) st_dev(ticket, 0u);
 }
 
 // per-block partial sums of squares, per RHS, in tree order; the last block adds
-// the partials (reduce_last_block) into out.  Launched with kBS blocks.
+// the partials (reduce_last_block) into out and into the pinned host array
+// host_out, which the host reads after a stream synchronization (no copy
+// launch).  Launched with kBS blocks.
 template <int NR>
 __global__ void __launch_bounds__(kBS) sumsq_partial_kernel(int64_t n, const double *__restrict__ v, double *__restrict__ part,
-                                                            double *__restrict__ out, unsigned *ticket) {
+                                                            double *__restrict__ out, unsigned *ticket, double *host_out) {
     __shared__ double sh[NR][kBS];
     double a[NR];
 #pragma unroll
@@ -1970,7 +1973,7 @@ __global__ void __launch_bounds__(kBS) sumsq_partial_kernel(int64_t n, const dou
         }
         __syncthreads();
     }
-    reduce_last_block<NR>(sh, part, out, ticket);
+    reduce_last_block<NR>(sh, part, out, ticket, host_out);
 }
 
 // q = K_ff p (plain FP64), p given in the global interleaved layout (zero on fixed
@@ -2776,10 +2779,12 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
         hipLaunchKernelGGL(fill_kernel<double>, gv, blk, 0, st, Xl, nv);
         hipLaunchKernelGGL(gather_rhs_kernel, gv, blk, 0, st, n, NR, ndof, S->d_perm, b_dev, Bp);
         hipLaunchKernelGGL(to_t_kernel<double>, gv, blk, 0, st, nv, Bp, R);
+        // squared norms per right-hand side, written by the reduction's last block
+        // into the context's pinned host array
         auto norms = [&](const double *v, double *outn) -> hipError_t {
-            hipLaunchKernelGGL(sumsq_partial_kernel<NR>, dim3(nblk), blk, 0, st, n, v, part, nrm, ticket);
-            hipError_t e = hipMemcpyAsync(outn, nrm, NR * 8, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            hipLaunchKernelGGL(sumsq_partial_kernel<NR>, dim3(nblk), blk, 0, st, n, v, part, nrm, ticket, ctx->h_red);
+            const hipError_t e = hipStreamSynchronize(st);
+            if (e == hipSuccess) std::memcpy(outn, ctx->h_red, NR * sizeof(double));
             return e;
         };
         auto dot = [&](const double *a, const double *b2, double *out_dev) {

@@ -40,6 +40,8 @@ struct jfem_ctx {
     std::vector<hipEvent_t> tl_ev;
     std::vector<int> tl_fam;
     size_t tl_n = 0;
+    // pinned host array (16 doubles) the refinement's norm reductions write into
+    double *h_red = nullptr;
     // device status word shared by kernels that report element / pivot failures
     int64_t *d_status = nullptr;   // [0] = first bad index (INT64_MAX if none), [1] = code
     // scratch for solves
