@@ -35,7 +35,7 @@ MI355X_HBM_GBS = 8000.0
 MI355X_FP32_TFLOPS = 157.3
 MI355X_FP64_TFLOPS = 78.6
 METRIC = "solves/s + p50/p99 latency @1e-10 residual: FP32-factor+refine vs FP64"
-FAMILIES = ["element", "assembly", "factor", "solve", "spmv", "factor_update", "factor_panel", "extend_add",
+FAMILIES = ["element", "assembly", "factor", "solve", "spmv", "factor_update", "factor_panel", "unused7",
             "fwd_flow", "bwd_flow", "factor_schur", "factor_inverse"]
 # algorithmic bytes of one element's generation: read its grid points' coordinates,
 # its connectivity and its section record, write its matrix once
@@ -185,7 +185,7 @@ def request_summary(r, nrhs):
             "p99_latency_ms": float(np.percentile(r["lat"], 99) * 1000),
             "max_relative_residual": float(np.max(r["relres"])), "converged": bool(r["converged"]),
             "triangular_solves_per_rhs": [int(x) for x in r["iters"]],
-            "kernel_ms": {k: round(v[0], 4) for k, v in r["fam"].items()}}
+            "kernel_ms": {k: round(v[0], 4) for k, v in r["fam"].items() if v[1]}}
 
 
 def ratio_ceiling(r32, r64):
@@ -440,14 +440,14 @@ def main():
         u32, u64 = r["u"], r64["u"]
         out["fp32_refined_error_vs_fp64"] = float(np.max(np.linalg.norm(u32 - u64, axis=0) /
                                                          np.linalg.norm(u64, axis=0)))
-        out["fp64_kernel_ms"] = {k: round(v[0], 4) for k, v in r64["fam"].items()}
+        out["fp64_kernel_ms"] = {k: round(v[0], 4) for k, v in r64["fam"].items() if v[1]}
     # kernel-family breakdown; roofline of the dominant kernel of the headline path
     # (largest total time in the instrumented step) among the factor's trailing
     # update (matrix cores) and the two dataflow triangular-solve kernels (HBM:
     # per launch they stream the factor entries one solve needs plus the vectors);
     # element generation and assembly beside them
     fam = r["fam"]
-    out["kernel_ms"] = {k: round(v[0], 4) for k, v in fam.items()}
+    out["kernel_ms"] = {k: round(v[0], 4) for k, v in fam.items() if v[1]}
     lines = solve_rooflines(fam, info, nrhs, main_prec)
     counters = args.workload == "wingbox" and args.nx == 256 and args.nw == 64   # the workload the counters were read on
     if lines:

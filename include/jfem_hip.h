@@ -68,8 +68,9 @@ int jfem_device_memory(jfem_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes)
  * (timed with HIP events on the context's stream); family: 0 element generation,
  * 1 assembly, 2 numeric factor, 3 triangular solves, 4 residual SpMV; with timing
  * level 2 also per launch: 5 factor update kernel, 6 factor panel kernels,
- * 7 extend-add kernel, 8 forward dataflow solve kernel, 9 backward dataflow
- * solve kernel, 10 update-matrix (Schur complement) kernel, 11 diagonal-block
+ * 7 (unused), 8 forward dataflow solve kernel, 9 backward dataflow solve
+ * kernel, 10 update-matrix kernel (Schur complement pushed into the parent
+ * fronts), 11 diagonal-block
  * inverses of the solves (the 64 x 64 inverses and X = L_11^{-1} of the
  * supernodes with two or more panels, one interval for the whole sequence).
  * jfem_ctx_enable_timing(ctx, 0 | 1 | 2). */

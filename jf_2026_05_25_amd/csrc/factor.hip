@@ -35,11 +35,8 @@ struct jfem_symbolic {
     FlowTask *d_flow_f = nullptr, *d_flow_b = nullptr;
     FlowChild *d_flow_gb = nullptr;
     UpdateTask *d_utasks = nullptr;
-    UpdateTask *d_stasks = nullptr;
+    PushTask *d_stasks = nullptr;
     InvTask *d_itasks = nullptr;
-    EaTask *d_etasks = nullptr;
-    int32_t *d_ea_bounds = nullptr;
-    int64_t *d_ea_boff = nullptr;
     std::vector<void *> allocs;
     int64_t bytes = 0;              // device bytes behind allocs
 };
@@ -123,79 +120,6 @@ __global__ void scatter_kernel(int64_t n, const int32_t *__restrict__ src, const
     L += blockIdx.y * vs.L;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         L[dst[i]] = (T)val[src[i]];
-}
-
-// ----------------------------------------------------------------- extend-add
-// One workgroup per (parent, 64 front columns, 512 front rows); the children are
-// added in child order.  The child entries that land in the task's block are the
-// update columns [j0, j1) x update rows [max(j, i0), i1), precomputed per
-// (task, child).  Each thread handles 8 independent entries per pass so their
-// loads overlap.
-template <class T>
-__global__ void __launch_bounds__(kBS) extend_add_kernel(const EaTask *__restrict__ tasks, const int64_t *__restrict__ boff,
-                                                         const int32_t *__restrict__ bounds, const int64_t *__restrict__ ncol,
-                                                         const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
-                                                         const int64_t *__restrict__ uofs, const int64_t *__restrict__ relofs,
-                                                         const int32_t *__restrict__ relmap, const int32_t *__restrict__ child_ptr,
-                                                         const int32_t *__restrict__ child, T *__restrict__ L, T *__restrict__ U,
-                                                         VarStride vs) {
-    L += blockIdx.y * vs.L;
-    U += blockIdx.y * vs.U;
-    constexpr int kU = 8;
-    const EaTask t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s], mu = ms - ns;
-    T *Ls = L + lofs[s];
-    T *Us = U + uofs[s];
-    const int32_t *bd = bounds + 4 * boff[blockIdx.x];
-    for (int q = child_ptr[s]; q < child_ptr[s + 1]; ++q, bd += 4) {
-        const int c = child[q];
-        const int64_t nu = mrow[c] - ncol[c];
-        const int32_t *rel = relmap + relofs[c];
-        const T *Uc = U + uofs[c];
-        const int64_t j0 = bd[0], j1 = bd[1], i1 = bd[3];
-        const int64_t ib = max<int64_t>(bd[2], j0);
-        const int64_t nrow = i1 - ib, nj = j1 - j0;
-        if (nrow > 0 && nj > 0) {
-            // block of at most 64 columns x kEaRows rows: 32-bit indices
-            const int total = (int)(nj * nrow), nr32 = (int)nrow;
-            for (int base = threadIdx.x; base < total; base += kU * kBS) {
-                int64_t jj[kU], ii[kU];
-                bool ok[kU];
-#pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    const int qq = base + u * kBS;
-                    const int qc = qq < total ? qq : total - 1;
-                    const int jq = qc / nr32;
-                    jj[u] = j0 + jq;
-                    ii[u] = ib + (qc - jq * nr32);
-                    ok[u] = qq < total && ii[u] >= jj[u];
-                }
-                // all loads of the pass before the first wait: indices and child
-                // entries, then the parent entries (valid addresses also for the
-                // entries that are skipped)
-                int32_t pr[kU], pc[kU];
-                T v[kU];
-#pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    pr[u] = rel[ii[u]];
-                    pc[u] = rel[jj[u]];
-                    v[u] = Uc[ii[u] + nu * jj[u]];
-                }
-                T *dst[kU];
-                T dv[kU];
-#pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    dst[u] = pc[u] < ns ? (Ls + pr[u] + ms * (int64_t)pc[u]) : (Us + (pr[u] - ns) + mu * (int64_t)(pc[u] - ns));
-                    dv[u] = *dst[u];
-                }
-#pragma unroll
-                for (int u = 0; u < kU; ++u)
-                    if (ok[u]) *dst[u] = dv[u] + v[u];
-            }
-        }
-        __syncthreads();
-    }
 }
 
 // ----------------------------------------------------------------- broadcasts
@@ -687,37 +611,55 @@ __global__ void __launch_bounds__(kBS) update_kernel(const UpdateTask *__restric
     if (bad < bn && tid == 0) report_min(status, blockIdx.y * vs.col + col0[s] + c0 + bad, JFEM_ERR_NOT_SPD);
 }
 
-// Update matrix of a front, after its last panel step: the tile (r0, c0) of U
-// (rows and columns >= ncol) takes -L[r0.., 0:ncol] L[c0.., 0:ncol]' (lower
-// triangle of the tile) with the whole rank ncol at once.  Chunks of kPanel
-// columns are staged through LDS; the next chunk's loads are issued before the
-// current chunk's products.  MFMA operand layout and tile split over the four
-// wavefronts as in update_kernel; the products accumulate over the chunks in
-// order, so the result is reproducible bit for bit.
+// Update matrix of a front, after its last panel step, pushed into the parent's
+// front (PushTask): the tile (r0, c0) of U_s (rows and columns >= ncol) becomes
+// U_s - L[r0.., 0:ncol] L[c0.., 0:ncol]' (lower triangle of the tile) with the
+// whole rank ncol at once, and each entry is added to the parent's front at the
+// relmap positions of its row and column (a pivot column of the parent lands in
+// its L, the rest in its U): the update matrix is never stored, and no
+// extend-add pass reads it back.  U_s itself holds what s's own children pushed.
+// Chunks of kPanel columns are staged through LDS; the next chunk's loads are
+// issued before the current chunk's products; the parent's entries are loaded
+// with the tile's, so their latency hides behind the products.  MFMA operand
+// layout and tile split over the four wavefronts as in update_kernel; the
+// products accumulate over the chunks in order, and each parent entry receives
+// t = U_s - product, rounded, then parent + t: results are reproducible bit for
+// bit.  The launch holds at most one child of each parent, so no two tiles add
+// into the same parent entry.
 template <class T, bool SG = false>
-__global__ void __launch_bounds__(kBS) schur_kernel(const UpdateTask *__restrict__ tasks, const int64_t *__restrict__ ncol,
-                                                    const int64_t *__restrict__ mrow, const int64_t *__restrict__ lofs,
-                                                    const int64_t *__restrict__ uofs, const T *__restrict__ L,
-                                                    T *__restrict__ U, VarStride vs, const int64_t *__restrict__ col0,
-                                                    const T *__restrict__ J) {
+__global__ void __launch_bounds__(kBS) schur_kernel(const PushTask *__restrict__ tasks, const int32_t *__restrict__ relmap,
+                                                    T *__restrict__ L, T *__restrict__ U, VarStride vs,
+                                                    const int64_t *__restrict__ col0, const T *__restrict__ J) {
     typedef typename Mfma<T>::v4 v4;
     L += blockIdx.y * vs.L;
     U += blockIdx.y * vs.U;
     __shared__ T A[kPanel][kTile + 1];   // A[p][i] = L[r0 + i, k + p]
     __shared__ T B[kPanel][kTile + 1];   // B[p][j] = L[c0 + j, k + p]
-    const UpdateTask t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int64_t ns = ncol[s], ms = mrow[s];
+    const PushTask t = tasks[blockIdx.x];
+    const int64_t ns = t.ns, ms = t.ms, nsp = t.nsp, msp = t.msp;
     const int64_t r0 = t.r0, c0 = t.c0;
     const int nr = (int)min<int64_t>(kTile, ms - r0), nc = (int)min<int64_t>(kTile, ms - c0);
-    const T *Ls = L + lofs[s];
-    T *Cs = U + uofs[s];
-    const int64_t ld = ms - ns;
+    const T *Ls = L + t.lofs;
+    const T *Cs = U + t.uofs;
+    const int32_t *rel = relmap + t.relofs - ns;       // rel[front row >= ns] = parent front row
+    T *Lp = L + t.lofsp, *Up = U + t.uofsp;
+    const int64_t ld = ms - ns, ldp = msp - nsp;
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
     const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
     const int l16 = lane & 15, l4 = lane >> 4;
-    T cpre[2][2][4];
+    // this lane's rows (x) and columns (y, v) of the tile, clamped, and their parent positions
+    int32_t prow[2], pcol[2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) prow[x] = rel[r0 + min(wi + 16 * x + l16, nr - 1)];
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) pcol[y][v] = rel[c0 + min(wj + 16 * y + Mfma<T>::drow(l4, v), nc - 1)];
+    auto pidx = [&](int32_t pr, int32_t pc) -> T * {
+        return pc < nsp ? Lp + pr + msp * (int64_t)pc : Up + (pr - nsp) + ldp * (int64_t)(pc - nsp);
+    };
+    T cpre[2][2][4], ppre[2][2][4];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -726,9 +668,12 @@ __global__ void __launch_bounds__(kBS) schur_kernel(const UpdateTask *__restrict
             for (int v = 0; v < 4; ++v) {
                 const int i = min(wi + 16 * x + l16, nr - 1), j = min(wj + 16 * y + Mfma<T>::drow(l4, v), nc - 1);
                 cpre[x][y][v] = Cs[(r0 + i - ns) + ld * (c0 + j - ns)];
+                // (a clamped entry above the diagonal reads the mirrored parent entry; it is not stored)
+                const int32_t pr = max(prow[x], pcol[y][v]), pc = min(prow[x], pcol[y][v]);
+                ppre[x][y][v] = *pidx(pr, pc);
             }
     T av[16], bv[16];
-    const T *Js = SG ? J + blockIdx.y * vs.col + col0[s] : nullptr;
+    const T *Js = SG ? J + blockIdx.y * vs.col + col0[t.s] : nullptr;
     auto load = [&](int64_t k) {
         const int b = (int)min<int64_t>(kPanel, ns - k);
 #pragma unroll
@@ -776,8 +721,10 @@ __global__ void __launch_bounds__(kBS) schur_kernel(const UpdateTask *__restrict
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = wi + 16 * x + l16, j = wj + 16 * y + Mfma<T>::drow(l4, v);
-                if (i < nr && j < nc && r0 + i >= c0 + j)
-                    Cs[(r0 + i - ns) + ld * (c0 + j - ns)] = cpre[x][y][v] - acc[x][y][v];
+                if (i < nr && j < nc && r0 + i >= c0 + j) {
+                    const T tv = cpre[x][y][v] - acc[x][y][v];
+                    *pidx(prow[x], pcol[y][v]) = ppre[x][y][v] + tv;
+                }
             }
 }
 
@@ -2206,13 +2153,6 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
     lap(-1);
     const int64_t nlev = (int64_t)H.level_ptr.size() - 1;
     for (int64_t l = 0; l < nlev; ++l) {
-        int64_t ne = H.ea_off[l + 1] - H.ea_off[l];
-        if (ne) {
-            hipLaunchKernelGGL(extend_add_kernel<T>, dim3((unsigned)ne, V), dim3(kBS), 0, st, S->d_etasks + H.ea_off[l],
-                               S->d_ea_boff + H.ea_off[l], S->d_ea_bounds, S->d_ncol, S->d_m, S->d_lofs, S->d_uofs,
-                               S->d_relofs, S->d_relmap, S->d_child_ptr, S->d_child, L, U, vs);
-            lap(FAM_EA);
-        }
         if (H.step_ptr[l + 1] > H.step_ptr[l]) {
             // diagonal blocks of the level's first panels
             const int64_t stp = H.step_ptr[l];
@@ -2239,12 +2179,13 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
                 lap(FAM_UPDATE);
             }
         }
-        // the level's update matrices, with the full rank of each front
-        const int64_t nsch = H.sstep_off[l + 1] - H.sstep_off[l];
-        if (nsch) {
+        // the level's update matrices, with the full rank of each front, pushed into
+        // the parents' fronts (one launch per sibling rank)
+        for (int64_t q = H.slev_ptr[l]; q < H.slev_ptr[l + 1]; ++q) {
+            const int64_t nsch = H.spush_off[q + 1] - H.spush_off[q];
+            if (!nsch) continue;
             hipLaunchKernelGGL((schur_kernel<T, SG>), dim3((unsigned)nsch, V), dim3(kBS), 0, st,
-                               S->d_stasks + H.sstep_off[l], S->d_ncol, S->d_m, S->d_lofs, S->d_uofs, (const T *)L, U, vs,
-                               S->d_col0, (const T *)Jv);
+                               S->d_stasks + H.spush_off[q], S->d_relmap, L, U, vs, S->d_col0, (const T *)Jv);
             lap(FAM_SCHUR);
         }
     }
@@ -2387,9 +2328,6 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
     if (e == hipSuccess) e = up(S, &S->d_utasks, H.utasks, st);
     if (e == hipSuccess) e = up(S, &S->d_stasks, H.stasks, st);
     if (e == hipSuccess) e = up(S, &S->d_itasks, H.itasks, st);
-    if (e == hipSuccess) e = up(S, &S->d_etasks, H.etasks, st);
-    if (e == hipSuccess) e = up(S, &S->d_ea_bounds, H.ea_bounds, st);
-    if (e == hipSuccess) e = up(S, &S->d_ea_boff, H.ea_boff, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_f, H.flow_f, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_b, H.flow_b, st);
     if (e == hipSuccess) e = up(S, &S->d_flow_gb, H.flow_gb, st);
@@ -2410,7 +2348,7 @@ int jfem_symbolic_create(jfem_ctx *ctx, const jfem_pattern *P, const jfem_mesh *
         info->update_launches = (int64_t)H.ustep_off.size() - 1;
         info->schur_flops = H.schur_flops;
         int64_t nsl = 0;
-        for (size_t l = 0; l + 1 < H.sstep_off.size(); ++l) nsl += H.sstep_off[l + 1] > H.sstep_off[l];
+        for (size_t q = 0; q + 1 < H.spush_off.size(); ++q) nsl += H.spush_off[q + 1] > H.spush_off[q];
         info->schur_launches = nsl;
         info->inv_entries = H.inv_entries;
         info->upd_rows = H.upd_rows;

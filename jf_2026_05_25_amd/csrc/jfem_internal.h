@@ -19,7 +19,7 @@
 namespace jfem {
 
 enum Family { FAM_ELEM = 0, FAM_ASSEMBLY = 1, FAM_FACTOR = 2, FAM_SOLVE = 3, FAM_SPMV = 4, FAM_UPDATE = 5,
-              FAM_PANEL = 6, FAM_EA = 7, FAM_FLOW_FWD = 8, FAM_FLOW_BWD = 9, FAM_SCHUR = 10,
+              FAM_PANEL = 6, FAM_UNUSED7 = 7, FAM_FLOW_FWD = 8, FAM_FLOW_BWD = 9, FAM_SCHUR = 10,
               FAM_INV = 11, FAM_COUNT = 12 };
 
 }  // namespace jfem

@@ -237,7 +237,8 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         S.step_ptr.assign(1, 0);
         S.pstep_off.assign(1, 0);
         S.ustep_off.assign(1, 0);
-        S.ea_off.assign(1, 0);
+        S.slev_ptr.assign(1, 0);
+        S.spush_off.assign(1, 0);
         S.iofs.assign(1, 0);
         S.iperm.assign(ndof, -1);
         return 0;
@@ -613,17 +614,11 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
     // task lists
     S.step_ptr.assign(nlev + 1, 0);
     S.pstep_off.clear(); S.ustep_off.clear(); S.pstep_mid.clear(); S.pstep_bw.clear();
-    S.ea_off.assign(nlev + 1, 0);
     for (int32_t l = 0; l < nlev; ++l) {
-        S.ea_off[l] = (int64_t)S.etasks.size();
         int64_t maxsteps = 0;
         for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
             int32_t s = S.level_sn[q];
             maxsteps = std::max<int64_t>(maxsteps, (S.ncol[s] + kPanel - 1) / kPanel);
-            if (S.child_ptr[s + 1] > S.child_ptr[s])
-                for (int64_t c0 = 0; c0 < S.m[s]; c0 += kTile)
-                    for (int64_t r0 = c0; r0 < S.m[s]; r0 += kEaRows)
-                        S.etasks.push_back({s, (int32_t)c0, (int32_t)std::min<int64_t>(c0 + kTile, S.m[s]), (int32_t)r0});
         }
         S.step_ptr[l] = (int64_t)S.pstep_off.size();
         for (int64_t t = 0; t < maxsteps; ++t) {
@@ -668,47 +663,52 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         }
     }
     // per level: the update-matrix tiles of its fronts (lower triangles), taken
-    // after the level's last panel step with the full rank ncol of each front
+    // after the level's last panel step with the full rank ncol of each front and
+    // added into the parents' fronts; one launch per sibling rank
     S.stasks.clear();
-    S.sstep_off.assign(nlev + 1, 0);
+    S.spush_off.assign(1, 0);
+    S.slev_ptr.assign(nlev + 1, 0);
     S.schur_flops = 0;
-    for (int32_t l = 0; l < nlev; ++l) {
-        S.sstep_off[l] = (int64_t)S.stasks.size();
-        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-            const int32_t s = S.level_sn[q];
-            const int64_t ns = S.ncol[s], ms = S.m[s];
-            for (int64_t c0 = ns; c0 < ms; c0 += kTile)
-                for (int64_t r0 = c0; r0 < ms; r0 += kTile) {
-                    S.stasks.push_back({s, 0, (int32_t)r0, (int32_t)c0});
-                    const int64_t r1 = std::min<int64_t>(r0 + kTile, ms), c1 = std::min<int64_t>(c0 + kTile, ms);
-                    double cnt = 0;
-                    for (int64_t c = c0; c < c1; ++c) cnt += (double)std::max<int64_t>(0, r1 - std::max(r0, c));
-                    S.schur_flops += 2.0 * (double)ns * cnt;
+    {
+        std::vector<int32_t> rank(nsn, 0), seen(nsn, -1), cnt_at(nsn, 0);
+        for (int32_t l = 0; l < nlev; ++l) {
+            S.slev_ptr[l] = (int64_t)S.spush_off.size() - 1;
+            int32_t maxrank = -1;
+            for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+                const int32_t s = S.level_sn[q], p = S.parent[s];
+                if (p < 0 || S.m[s] <= S.ncol[s]) continue;
+                if (seen[p] != l) { seen[p] = l; cnt_at[p] = 0; }
+                rank[s] = cnt_at[p]++;
+                maxrank = std::max(maxrank, rank[s]);
+            }
+            for (int32_t r = 0; r <= maxrank; ++r) {
+                for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+                    const int32_t s = S.level_sn[q], p = S.parent[s];
+                    if (p < 0 || S.m[s] <= S.ncol[s] || rank[s] != r) continue;
+                    const int64_t ns = S.ncol[s], ms = S.m[s];
+                    for (int64_t c0 = ns; c0 < ms; c0 += kTile)
+                        for (int64_t r0 = c0; r0 < ms; r0 += kTile) {
+                            PushTask t{};
+                            t.s = s; t.r0 = (int32_t)r0; t.c0 = (int32_t)c0;
+                            t.ns = (int32_t)ns; t.ms = (int32_t)ms;
+                            t.nsp = (int32_t)S.ncol[p]; t.msp = (int32_t)S.m[p];
+                            t.lofs = S.lofs[s]; t.uofs = S.uofs[s]; t.relofs = S.relofs[s];
+                            t.lofsp = S.lofs[p]; t.uofsp = S.uofs[p];
+                            S.stasks.push_back(t);
+                            const int64_t r1 = std::min<int64_t>(r0 + kTile, ms), c1 = std::min<int64_t>(c0 + kTile, ms);
+                            double cnt = 0;
+                            for (int64_t c = c0; c < c1; ++c) cnt += (double)std::max<int64_t>(0, r1 - std::max(r0, c));
+                            S.schur_flops += 2.0 * (double)ns * cnt;
+                        }
                 }
+                S.spush_off.push_back((int64_t)S.stasks.size());
+            }
         }
+        S.slev_ptr[nlev] = (int64_t)S.spush_off.size() - 1;
     }
-    S.sstep_off[nlev] = (int64_t)S.stasks.size();
     S.step_ptr[nlev] = (int64_t)S.pstep_off.size();
     S.pstep_off.push_back((int64_t)S.ptasks.size());
     S.ustep_off.push_back((int64_t)S.utasks.size());
-    S.ea_off[nlev] = (int64_t)S.etasks.size();
-    S.ea_bounds.clear();
-    S.ea_boff.assign(S.etasks.size() + 1, 0);
-    for (size_t e = 0; e < S.etasks.size(); ++e) {
-        const EaTask &t = S.etasks[e];
-        S.ea_boff[e] = (int64_t)(S.ea_bounds.size() / 4);
-        for (int32_t qc = S.child_ptr[t.s]; qc < S.child_ptr[t.s + 1]; ++qc) {
-            const int32_t c = S.child[qc];
-            const int32_t *rel = S.relmap.data() + S.relofs[c];
-            const int64_t nu = S.m[c] - S.ncol[c];
-            auto lb = [&](int64_t key) { return (int32_t)(std::lower_bound(rel, rel + nu, key) - rel); };
-            S.ea_bounds.push_back(lb(t.c0));
-            S.ea_bounds.push_back(lb(t.c1));
-            S.ea_bounds.push_back(lb(t.r0));
-            S.ea_bounds.push_back(lb((int64_t)t.r0 + kEaRows));
-        }
-    }
-    S.ea_boff[S.etasks.size()] = (int64_t)(S.ea_bounds.size() / 4);
     // dataflow solve lists
     auto npan = [&](int64_t s) { return (int32_t)((S.ncol[s] + kPanel - 1) / kPanel); };
     auto nupd = [&](int64_t s) { return (int32_t)((S.m[s] - S.ncol[s] + kTile - 1) / kTile); };

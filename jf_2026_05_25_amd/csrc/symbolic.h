@@ -8,11 +8,17 @@ namespace jfem {
 
 constexpr int kPanel = 64;   // panel width of the blocked partial factorization
 constexpr int kTile = 64;    // tile edge of panel/update tasks
-constexpr int kEaRows = 512; // front rows per extend-add task
 
 struct PanelTask { int32_t s, k, r0, pad; };          // supernode, panel column, first row
 struct UpdateTask { int32_t s, k, r0, c0; };          // supernode, panel column, tile origin
-struct EaTask { int32_t s, c0, c1, r0; };
+// Tile (r0, c0) of the update matrix U_s (front rows / columns >= ns) pushed into
+// the parent's front: U_s - L[r0.., 0:ns] L[c0.., 0:ns]' added to the parent's
+// entries at the relmap positions (pivot columns of the parent in its L, the rest
+// in its U).  The supernode's and its parent's shapes and offsets are inlined.
+struct PushTask {
+    int32_t s, r0, c0, ns, ms, nsp, msp, pad;
+    int64_t lofs, uofs, relofs, lofsp, uofsp;
+};
 // Dataflow solve task: one workgroup each; a list is in dependency order, so a
 // task waits only on tasks listed before it.  The supernode's metadata is
 // inlined so that a task starts without dependent loads.
@@ -71,20 +77,19 @@ struct SymbolicHost {
     // per-level task lists
     std::vector<PanelTask> ptasks;
     std::vector<UpdateTask> utasks;
-    std::vector<UpdateTask> stasks;         // update-matrix tiles (k unused), per level
-    std::vector<int64_t> sstep_off;         // per level: offset into stasks (size levels+1)
-    std::vector<EaTask> etasks;
+    // update matrices pushed into the parents after each level's last panel step:
+    // the level's supernodes in launches by sibling rank (the r-th child of each
+    // parent among the level's supernodes goes to launch r), so no two tasks of a
+    // launch add into the same parent entry; every entry of a front receives its
+    // children's updates in a fixed order (child level, then rank)
+    std::vector<PushTask> stasks;
+    std::vector<int64_t> spush_off;         // per push launch: offset into stasks (size launches+1)
+    std::vector<int64_t> slev_ptr;          // per level: first push launch (size levels+1)
     // ptask/utask ranges per (level, step): offsets into the arrays
     std::vector<int64_t> step_ptr;          // per level: first step index (size levels+1)
     std::vector<int64_t> pstep_off, ustep_off;   // per step: offset (size steps+1)
     std::vector<int64_t> pstep_mid;              // per step: first non-diagonal panel tile
     std::vector<int32_t> pstep_bw;               // per panel step: widest diagonal block
-    std::vector<int64_t> ea_off;            // per level: offset into etasks (size levels+1)
-    // per extend-add task, per child of its parent (child order): the child's
-    // update columns [j0, j1) landing in front columns [c0, c1) and update rows
-    // [i0, i1) landing in front rows [r0, r0 + kEaRows)
-    std::vector<int32_t> ea_bounds;         // 4 per (task, child)
-    std::vector<int64_t> ea_boff;           // per task: first (task, child) entry
     // dataflow solves: forward list (per supernode its whole front, or its panels
     // and then its update blocks of kTile rows), backward list (per supernode its
     // panels from the last), both ordered by estimated start; per supernode the

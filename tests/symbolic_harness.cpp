@@ -22,31 +22,13 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
     std::string err;
     if (build_symbolic(ndof, rowptr, col, xyz, fixed, S, err)) return 1;
     std::vector<double> L(S.L_entries + 1, 0.0), U(S.U_entries + 1, 0.0);
+    std::vector<int64_t> Lstamp(S.L_entries + 1, 0), Ustamp(S.U_entries + 1, 0);   // last push launch per entry
     for (size_t i = 0; i < S.amap_src.size(); ++i) L[S.amap_dst[i]] = val[S.amap_src[i]];
     const int64_t nlev = (int64_t)S.level_ptr.size() - 1;
+    std::vector<int64_t> lev_of(S.ns, -1);
+    for (int64_t l = 0; l < nlev; ++l)
+        for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) lev_of[S.level_sn[q]] = l;
     for (int64_t l = 0; l < nlev; ++l) {
-        for (int64_t q = S.ea_off[l]; q < S.ea_off[l + 1]; ++q) {
-            EaTask t = S.etasks[q];
-            int s = t.s;
-            int64_t ns = S.ncol[s], ms = S.m[s], mu = ms - ns;
-            for (int32_t qc = S.child_ptr[s]; qc < S.child_ptr[s + 1]; ++qc) {
-                int c = S.child[qc];
-                const int32_t *rel = S.relmap.data() + S.relofs[c];
-                const int32_t *bd = S.ea_bounds.data() + 4 * (S.ea_boff[q] + (qc - S.child_ptr[s]));
-                for (int64_t j = bd[0]; j < bd[1]; ++j) {
-                    int64_t pc = rel[j];
-                    if (pc < t.c0 || pc >= t.c1) return 6;
-                    for (int64_t i = std::max<int64_t>(j, bd[2]); i < bd[3]; ++i) {
-                        int64_t pr = rel[i];
-                        if (pr < t.r0 || pr >= t.r0 + kEaRows) return 7;
-                        int64_t nu = S.m[c] - S.ncol[c];
-                        double v = U[S.uofs[c] + i + nu * j];
-                        if (pc < ns) L[S.lofs[s] + pr + ms * pc] += v;
-                        else U[S.uofs[s] + (pr - ns) + mu * (pc - ns)] += v;
-                    }
-                }
-            }
-        }
         // factor the b x b diagonal block at column k of supernode s in place
         auto potrf = [&](int s, int64_t k) -> int {
             int64_t ns = S.ncol[s], ms = S.m[s];
@@ -110,20 +92,41 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                     if (int e = potrf(s, t.c0)) return e;
             }
         }
-        // after the level's last step: the update matrices, full rank per front
-        for (int64_t q = S.sstep_off[l]; q < S.sstep_off[l + 1]; ++q) {
-            UpdateTask t = S.stasks[q];
-            int s = t.s;
-            int64_t ns = S.ncol[s], ms = S.m[s];
-            if (t.r0 < ns || t.c0 < ns || t.r0 < t.c0 || t.r0 >= ms) return 12;
-            int64_t r1 = std::min<int64_t>(t.r0 + kTile, ms), c1 = std::min<int64_t>(t.c0 + kTile, ms);
-            const double *Ls = L.data() + S.lofs[s];
-            for (int64_t c = t.c0; c < c1; ++c)
-                for (int64_t r = std::max<int64_t>(t.r0, c); r < r1; ++r) {
-                    double acc = 0;
-                    for (int64_t p = 0; p < ns; ++p) acc += Ls[r + ms * p] * Ls[c + ms * p];
-                    U[S.uofs[s] + (r - ns) + (ms - ns) * (c - ns)] -= acc;
-                }
+        // after the level's last step: the update matrices, full rank per front,
+        // pushed into the parents' fronts, one launch per sibling rank: a task's
+        // fields match the symbolic arrays (10), its parent lies on a later level
+        // (14), and no parent entry is written twice within one launch (13)
+        if (S.slev_ptr.size() != (size_t)nlev + 1 || S.slev_ptr[l] > S.slev_ptr[l + 1]) return 10;
+        for (int64_t q = S.slev_ptr[l]; q < S.slev_ptr[l + 1]; ++q) {
+            const int64_t launch = q + 1;
+            for (int64_t e = S.spush_off[q]; e < S.spush_off[q + 1]; ++e) {
+                const PushTask t = S.stasks[e];
+                const int s = t.s, p = S.parent[s];
+                const int64_t ns = S.ncol[s], ms = S.m[s];
+                if (p < 0 || t.ns != ns || t.ms != ms || t.nsp != S.ncol[p] || t.msp != S.m[p] || t.lofs != S.lofs[s] ||
+                    t.uofs != S.uofs[s] || t.relofs != S.relofs[s] || t.lofsp != S.lofs[p] || t.uofsp != S.uofs[p])
+                    return 10;
+                if (t.r0 < ns || t.c0 < ns || t.r0 < t.c0 || t.r0 >= ms) return 12;
+                if (lev_of[p] <= l || lev_of[s] != l) return 14;
+                const int64_t r1 = std::min<int64_t>(t.r0 + kTile, ms), c1 = std::min<int64_t>(t.c0 + kTile, ms);
+                const double *Ls = L.data() + S.lofs[s];
+                const int32_t *rel = S.relmap.data() + S.relofs[s] - ns;
+                const int64_t nsp = S.ncol[p], msp = S.m[p];
+                for (int64_t c = t.c0; c < c1; ++c)
+                    for (int64_t r = std::max<int64_t>(t.r0, c); r < r1; ++r) {
+                        double acc = 0;
+                        for (int64_t k = 0; k < ns; ++k) acc += Ls[r + ms * k] * Ls[c + ms * k];
+                        const double tv = U[S.uofs[s] + (r - ns) + (ms - ns) * (c - ns)] - acc;
+                        const int64_t pr = rel[r], pc = rel[c];
+                        if (pr < pc || pr >= msp || pc < 0) return 10;
+                        const bool inL = pc < nsp;
+                        const int64_t at = inL ? S.lofs[p] + pr + msp * pc : S.uofs[p] + (pr - nsp) + (msp - nsp) * (pc - nsp);
+                        int64_t &stamp = inL ? Lstamp[at] : Ustamp[at];
+                        if (stamp == launch) return 13;
+                        stamp = launch;
+                        (inL ? L : U)[at] += tv;
+                    }
+            }
         }
     }
     // solves, interleaved right-hand sides (v[i * nrhs + r])
