@@ -2,13 +2,15 @@
 // triangular solves, the FP64 residual SpMV and the iterative-refinement driver.
 //
 // Storage per supernode s (ncol n, front order m): L_s is m x n column-major
-// (rows = the front's row list), the update matrix U_s is (m-n) x (m-n).
+// (rows = the front's row list), the update-matrix storage U_s is (m-n) x (m-n).
 // Factorization, level by level from the leaves:
-//   scatter K entries into L (once, all supernodes), extend-add children's U into
-//   the parent's L/U (deterministic: children in order), then for each 64-column
+//   scatter K entries into L (once, all supernodes), then for each 64-column
 //   panel: factor the diagonal block + triangular solve of the panel rows
-//   (panel kernel), and a rank-64 update of the trailing front (update kernel,
-//   MFMA tiles).  All supernodes of a level are batched into the same launches.
+//   (panel kernels), and a rank-64 update of the trailing pivot columns (update
+//   kernel, MFMA tiles); after the level's last panel the update matrix of each
+//   front, with its full rank, added straight into the parent's front (Schur
+//   kernel, one launch per sibling rank: deterministic).  All supernodes of a
+//   level are batched into the same launches.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -2129,8 +2131,8 @@ inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std:
 // numeric factorization of V matrices sharing S (K values, factors and update
 // matrices of variant v at v times the strides in vs), in one launch sequence
 // SG: the signed factorization A = L J L' (J into Jv, one sign per elimination
-// position); solve_blocks: also the inverse blocks and chain products the
-// triangular solves need
+// position); solve_blocks: also the diagonal-block inverses and the X = L_11^{-1}
+// of the multi-panel supernodes the triangular solves need
 template <class T, bool SG = false>
 hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T *L, T *U, T *Linv, int V,
                    VarStride vs, T *Jv = nullptr, bool solve_blocks = true) {

@@ -12,7 +12,7 @@
 //     scatter map (upper triangle of K in original numbering, as
 //     cholesky(Symmetric(K_ff)) reads it, src/solver/boundary_conditions.jl:1660),
 //     child -> parent relative maps, level schedule and the task lists of the
-//     batched panel/update/extend-add/solve kernels.
+//     batched panel/update/update-matrix/inversion/solve kernels.
 #include <algorithm>
 #include <cstdint>
 #include <cstdint>

@@ -1,10 +1,11 @@
 // CPU harness for the symbolic analysis (test infrastructure).
 //
 // Runs jfem::build_symbolic and then executes, sequentially on the CPU, the same
-// task decomposition the device kernels execute (scatter, extend-add tasks, panel
-// tasks in two phases per step, update tiles, and the dataflow solve lists in
-// their order), so the structures and task lists can be checked against a dense
-// solve without a GPU.
+// task decomposition the device kernels execute (scatter, panel tasks, update
+// tiles, the update matrices pushed into the parents launch by launch, the
+// inversion tiles of X = L_11^{-1}, and the dataflow solve lists in their order),
+// so the structures and task lists can be checked against a dense solve without
+// a GPU.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
