@@ -79,11 +79,12 @@ __device__ inline void report_min(int64_t *status, int64_t idx, int code) {
 }
 
 // ----------------------------------------------------------------- fill / scatter
-// before a dataflow solve: Z, X, V and Tb (nv entries each) and W (nw) to the
-// pending mark, the two launch tickets to zero
+// before a dataflow solve: Z, X, V and Tb (nv entries each), W (nw) and the
+// partial sums P (np) to the pending mark, the two launch tickets to zero
 template <class T>
 __global__ void solve_marks_kernel(T *__restrict__ Z, T *__restrict__ X, T *__restrict__ V, T *__restrict__ Tb,
-                                   int64_t nv, T *__restrict__ W, int64_t nw, T mark, int32_t *__restrict__ tickets) {
+                                   int64_t nv, T *__restrict__ W, int64_t nw, T *__restrict__ P, int64_t np, T mark,
+                                   int32_t *__restrict__ tickets) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i0 < 2) tickets[i0] = 0;
@@ -94,6 +95,7 @@ __global__ void solve_marks_kernel(T *__restrict__ Z, T *__restrict__ X, T *__re
         Tb[i] = mark;
     }
     for (int64_t i = i0; i < nw; i += stride) W[i] = mark;
+    for (int64_t i = i0; i < np; i += stride) P[i] = mark;
 }
 
 template <class T>
@@ -1060,6 +1062,50 @@ __device__ __forceinline__ bool g16_load_wait_rows(T (&xr)[NR], const T *X, cons
 template <int NR>
 __device__ __forceinline__ int g16_r0(int cg) { return cg * (NR / 4); }
 
+// acc[r] += P_u[i][r] for the npart partial sums u = 0, 1, ... (kTile rows x nrhs
+// each, consecutive from P) of output row i, in that order; called after the
+// reduction, so every lane of a row adds the same values.  The loads of several
+// partials go out together (one round trip per batch).  False on abort / time-out.
+template <class T, int NR>
+__device__ __forceinline__ bool add_partials(T (&acc)[NR], const T *P, int npart, int i, int nr, int nrhs,
+                                             int32_t *abort_flag) {
+    constexpr int PB = NR >= 16 ? 1 : 16 / NR;
+    bool ok = true;
+    for (int u0 = 0; u0 < npart; u0 += PB) {
+        T v[PB * NR];
+        const T *pp[PB * NR];
+        bool live[PB * NR];
+#pragma unroll
+        for (int u = 0; u < PB; ++u)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const int q = u * NR + r;
+                pp[q] = P + ((int64_t)min(u0 + u, npart - 1) * kTile + min(i, nr - 1)) * nrhs + min(r, nrhs - 1);
+                live[q] = u0 + u < npart && i < nr && r < nrhs;
+                v[q] = ld_dev(pp[q]);
+            }
+        ok = await_values<T, PB * NR>(v, pp, live, abort_flag) && ok;
+#pragma unroll
+        for (int u = 0; u < PB; ++u)
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (u0 + u < npart) acc[r] += live[u * NR + r] ? v[u * NR + r] : T(0);
+    }
+    return ok;
+}
+// a partial sum task's result: rows i < nr of this lane's share of right-hand
+// sides into slot P (kTile rows x nrhs)
+template <class T, int NR>
+__device__ __forceinline__ void store_partial(T *P, const T (&acc)[NR], int i, int nr, int nrhs, int rq) {
+    if (i < nr) {
+#pragma unroll
+        for (int rr = 0; rr < NR / 4; ++rr) {
+            const int r = rq + rr;
+            if (r < nrhs) st_dev(P + (int64_t)i * nrhs + r, acc[r]);
+        }
+    }
+}
+
 // Gather of the children's update-vector entries landing in rows [r0, r0+nrows)
 // of a front into vs (LDS), nt threads, NR slots per row (right-hand sides
 // beyond nrhs are skipped): the (index, value) loads of 8 entries per thread go
@@ -1117,28 +1163,59 @@ __device__ __forceinline__ bool gather_children(T (*vs)[NR], const FlowChild *__
 // whole-front task: both for a supernode with one panel (rows of the front
 // <= kWholeRows), the update rows in blocks of 64.
 // The k loops run without LDS or barriers: every lane polls the values it needs
-// straight into its registers.
+// straight into its registers.  The k loop covers [k0, k1); the partial sums of
+// the blocks before k0 (kind 2 tasks) are added after the reduction.
 template <class T, int NR>
 __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T (*vx)[NR], int &failed,
                                               const FlowChild *__restrict__ gch, const int32_t *__restrict__ relmap,
                                               const T *__restrict__ L, const T *__restrict__ Linv,
-                                              const T *__restrict__ Y, T *W, T *Z, T *V, int nrhs, int32_t *cnt,
-                                              int64_t nsn) {
+                                              const T *__restrict__ Y, T *W, T *Z, T *V, T *P, int nrhs,
+                                              int32_t *cnt, int64_t nsn) {
     int32_t *abort_flag = cnt + nsn;
     const FlowChild first = t.nch > 0 ? gch[t.gb] : FlowChild{};   // (in flight with the task's first loads)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, j = lane & 15, cg = lane >> 4;
     const int i = 16 * w + j;                          // row of the 64-row block
     const int rq = g16_r0<NR>(cg);
-    const bool pivot = t.p >= 0, whole = t.kind == 1;
+    const bool pivot = t.p >= 0, whole = t.kind == 1, part = t.kind == 2;
     const bool multi = pivot && t.xofs >= 0;           // np >= 2: through X and the v of the earlier panels
-    const int kend = pivot ? t.p : t.np;
+    const int k0 = t.k0, k1 = t.k1;
     // operand of the k loop: update task L[rows, k] (ld ms), multi-panel pivot X[p rows, k] (ld ns)
     const T *Kb = multi ? Linv + t.xofs + t.r0 : L + t.lofs + t.r0;
     const int64_t kld = multi ? t.ns : t.ms;
     const T *Kv = multi ? V : Z;                       // the vector the k loop polls
     T lm[16], im[16];
-    if (kend > 0) g16_load(lm, Kb, kld, t.nr, (int)min<int64_t>(kPanel, t.ns), i, cg);
-    else if (whole && t.ms > t.ns) g16_load(lm, L + t.lofs + t.ns, t.ms, (int)(t.ms - t.ns), t.nr, i, cg);
+    if (k1 > k0) {
+        const int64_t col = (int64_t)kPanel * k0;
+        g16_load(lm, Kb + kld * col, kld, t.nr, (int)min<int64_t>(kPanel, t.ns - col), i, cg);
+    } else if (whole && t.ms > t.ns) {
+        g16_load(lm, L + t.lofs + t.ns, t.ms, (int)(t.ms - t.ns), t.nr, i, cg);
+    }
+    T acc[NR], xr[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = T(0);
+    auto kloop = [&]() -> bool {
+        bool ok = true;
+        for (int k = k0; k < k1; ++k) {
+            const int64_t col = (int64_t)kPanel * k;
+            const T *zk = Kv + (t.c0 + col) * nrhs;
+            ok = g16_load_wait<T, NR>(xr, [&](int c) { return zk + (int64_t)c * nrhs; },
+                                      (int)min<int64_t>(kPanel, t.ns - col), nrhs, cg, j, abort_flag) && ok;
+            T cur[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) cur[c] = lm[c];
+            if (k + 1 < k1)
+                g16_load(lm, Kb + kld * (col + kPanel), kld, t.nr, (int)min<int64_t>(kPanel, t.ns - col - kPanel), i, cg);
+            g16_mac<T, NR>(acc, cur, xr);
+        }
+        return ok;
+    };
+    if (part) {                                        // partial sum over [k0, k1) into slot pofs
+        const bool ok = kloop();
+        g16_reduce<T, NR>(acc);
+        if (ok) store_partial<T, NR>(P + (int64_t)t.pofs * kTile * nrhs, acc, i, t.nr, nrhs, rq);
+        return;
+    }
+    const T *Pc = P + (int64_t)t.pofs * kTile * nrhs;  // the partial sums of [0, k0)
     if (pivot) g16_load(im, Linv + t.iofs, t.nr, t.nr, t.nr, i, cg);
 #pragma unroll
     for (int q2 = 0; q2 < kWholeRows * NR / kBS; ++q2) {
@@ -1160,9 +1237,6 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
             if (r < nrhs) st_dev(V + (t.c0 + t.r0 + i) * nrhs + r, vs[i][r]);
         }
     }
-    T acc[NR], xr[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = T(0);
     if (whole) {
         // z = L_11^{-1} v[0:ns], then w = v - L[update rows, :] z in blocks of 64 rows
         g16_from_lds<T, NR>(xr, vs, t.nr, cg, j);      // update rows of the front are not part of v
@@ -1197,25 +1271,14 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
         }
         return;
     }
-    bool ok = true;
-    for (int k = 0; k < kend; ++k) {
-        const int64_t col = (int64_t)kPanel * k;
-        const T *zk = Kv + (t.c0 + col) * nrhs;
-        ok = g16_load_wait<T, NR>(xr, [&](int c) { return zk + (int64_t)c * nrhs; },
-                                  (int)min<int64_t>(kPanel, t.ns - col), nrhs, cg, j, abort_flag) && ok;
-        T cur[16];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) cur[c] = lm[c];
-        if (k + 1 < kend)
-            g16_load(lm, Kb + kld * (col + kPanel), kld, t.nr, (int)min<int64_t>(kPanel, t.ns - col - kPanel), i, cg);
-        g16_mac<T, NR>(acc, cur, xr);
-    }
+    bool ok = kloop();
     if (!ok) failed = 1;
     if (multi) {
-        // + X_pp v_p (v_p from LDS), one reduction for the whole sum
+        // + X_pp v_p (v_p from LDS), one reduction for the whole sum, then the partials
         g16_from_lds<T, NR>(xr, vs, t.nr, cg, j);
         g16_mac<T, NR>(acc, im, xr);
         g16_reduce<T, NR>(acc);
+        if (t.npart > 0) ok = add_partials<T, NR>(acc, Pc, t.npart, i, t.nr, nrhs, abort_flag) && ok;
         if (i < t.nr && ok) {
 #pragma unroll
             for (int rr = 0; rr < NR / 4; ++rr) {
@@ -1227,6 +1290,7 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
     }
     g16_reduce<T, NR>(acc);
     if (!pivot) {
+        if (t.npart > 0 && !add_partials<T, NR>(acc, Pc, t.npart, i, t.nr, nrhs, abort_flag)) failed = 1;
         if (i < t.nr) {
 #pragma unroll
             for (int rr = 0; rr < NR / 4; ++rr) {
@@ -1260,13 +1324,19 @@ __device__ __forceinline__ void fwd_main_task(const FlowTask &t, T (*vs)[NR], T 
 // The products are transposed (lane row = panel column): each tile is loaded
 // coalesced, staged through LDS (double-buffered in FP32, so one barrier per
 // tile), and read back transposed before its values are awaited.
+// Long sums are split over several tasks: kind 2 sums L[rows, p]' x_rows over
+// update rows [u0, u1), or X[q,p]' t_q over panels q in [k0, k1), into a partial
+// slot; a panel task adds the partials of its far update rows before forming
+// t_p; a panel with many later panels has kind 3 form and publish t_p and kind 4
+// form x_p from X_pp' t_p (t_p polled from Tb), its own q in [k0, k1) and the
+// partials of the panels above k1.
 template <class T>
 constexpr int kBwdStage = sizeof(T) == 4 ? 2 : 1;      // staging buffers of the four-wave task (LDS budget)
 template <class T, int NR>
 __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][kTile + 1], T (*vx)[NR], int &failed,
                                               const int32_t *__restrict__ rows, const T *__restrict__ L,
                                               const T *__restrict__ Linv, const T *__restrict__ Z, T *X, T *Tb,
-                                              int nrhs, int32_t *cnt, int64_t nsn) {
+                                              T *P, int nrhs, int32_t *cnt, int64_t nsn) {
     constexpr int NB = kBwdStage<T>;
     int32_t *abort_flag = cnt + nsn;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, j = lane & 15, cg = lane >> 4;
@@ -1274,15 +1344,17 @@ __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][
     const int rq = g16_r0<NR>(cg);
     const int nc = t.nr;
     const bool multi = t.xofs >= 0;
-    const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;
-    const int ntile = nchunk + (multi ? t.np - 1 - t.p : 0);
+    const bool tphase = t.kind == 0 || t.kind == 3;    // forms t_p from the update rows
+    const bool xout = t.kind == 0 || t.kind == 4;      // forms x_p
+    const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;   // (none for the X tasks)
+    const int ntile = nchunk + (multi ? t.k1 - t.k0 : 0);
     const T *Lc = L + t.lofs + t.ms * t.r0;            // front row 0, column r0
     const T *Xc = multi ? Linv + t.xofs + t.ns * t.r0 : Lc;   // X row 0, column r0
     const int32_t *rw = rows + t.rofs;
     // tile e: update rows [u0 + 64(nchunk-1-e), ...) of L for e < nchunk, else the
-    // rows of panel q = np-1-(e-nchunk) of X
+    // rows of panel q = k1-1-(e-nchunk) of X
     auto tile_row0 = [&](int e) -> int64_t {
-        return e < nchunk ? t.u0 + (int64_t)kTile * (nchunk - 1 - e) : (int64_t)kPanel * (t.np - 1 - (e - nchunk));
+        return e < nchunk ? t.u0 + (int64_t)kTile * (nchunk - 1 - e) : (int64_t)kPanel * (t.k1 - 1 - (e - nchunk));
     };
     auto tile_rows = [&](int e) -> int {
         const int64_t r0 = tile_row0(e);
@@ -1293,8 +1365,8 @@ __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][
         else load_cols16(v, Xc + tile_row0(e), t.ns, tile_rows(e), nc, lane, w);
     };
     T lv[16], im[16], zv[NR / 4], acc[NR], xr[NR];
-    g16_load_t(im, Linv + t.iofs, nc, nc, nc, c, cg); // row c of L_pp^{-T}
-    {
+    if (xout) g16_load_t(im, Linv + t.iofs, nc, nc, nc, c, cg); // row c of L_pp^{-T}
+    if (tphase) {
         const T *zp = Z + (t.c0 + t.r0 + min(c, nc - 1)) * nrhs;
 #pragma unroll
         for (int rr = 0; rr < NR / 4; ++rr) zv[rr] = zp[min(rq + rr, nrhs - 1)];
@@ -1305,11 +1377,15 @@ __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][
     if (nchunk > 0) g16_rows<NR>(ridx, rw + tile_row0(0), tile_rows(0), cg, j);
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = T(0);
-    // t_p = z_p - (update-row sum) into vx, published for the earlier panels of s;
-    // the accumulator restarts for the later panels' sum
+    // t_p = z_p - (update-row sum, then the partial sums of the far update rows)
+    // into vx, published for the earlier panels of s; the accumulator restarts
+    // for the later panels' sum
     bool formed = false;
+    bool ok = true;
     auto form_t = [&]() {
         g16_reduce<T, NR>(acc);
+        if (t.npart > 0)
+            ok = add_partials<T, NR>(acc, P + (int64_t)t.pofs * kTile * nrhs, t.npart, c, nc, nrhs, abort_flag) && ok;
 #pragma unroll
         for (int rr = 0; rr < NR / 4; ++rr) {
             const int r = rq + rr;
@@ -1321,9 +1397,8 @@ __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][
         for (int r = 0; r < NR; ++r) acc[r] = T(0);
         formed = true;
     };
-    bool ok = true;
     for (int e = 0; e < ntile; ++e) {
-        if (e == nchunk) form_t();
+        if (tphase && e == nchunk) form_t();
         const int nre = tile_rows(e);
         T (*S)[kTile + 1] = Lt[e % NB];
 #pragma unroll
@@ -1346,12 +1421,26 @@ __device__ __forceinline__ void bwd_main_task(const FlowTask &t, T (*Lt)[kTile][
         g16_mac<T, NR>(acc, m, xr);
     }
     if (!ok) failed = 1;
-    if (!formed) form_t();                             // no later panels
+    if (tphase && !formed) form_t();                   // no later panels
+    if (t.kind == 3) return;
+    if (t.kind == 2) {
+        g16_reduce<T, NR>(acc);
+        if (ok) store_partial<T, NR>(P + (int64_t)t.pofs * kTile * nrhs, acc, c, nc, nrhs, rq);
+        return;
+    }
+    if (t.kind == 4) {                                 // t_p, published by the panel's kind 3 task
+        const T *tp = Tb + (t.c0 + t.r0) * nrhs;
+        if (!g16_load_wait<T, NR>(xr, [&](int k) { return tp + (int64_t)k * nrhs; }, nc, nrhs, cg, j, abort_flag))
+            failed = 1;
+    }
     __syncthreads();
     if (failed) return;
-    g16_from_lds<T, NR>(xr, vx, nc, cg, j);
+    if (t.kind == 0) g16_from_lds<T, NR>(xr, vx, nc, cg, j);
     g16_mac<T, NR>(acc, im, xr);
     g16_reduce<T, NR>(acc);
+    if (t.kind == 4 && t.npart > 0 &&
+        !add_partials<T, NR>(acc, P + (int64_t)t.pofs * kTile * nrhs, t.npart, c, nc, nrhs, abort_flag))
+        return;
     if (c < nc) {
 #pragma unroll
         for (int rr = 0; rr < NR / 4; ++rr) {
@@ -1562,7 +1651,7 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
                                                        const FlowChild *__restrict__ gch,
                                                        const int32_t *__restrict__ relmap, const T *__restrict__ L,
                                                        const T *__restrict__ Linv, const T *__restrict__ Y, T *W, T *Z,
-                                                       T *V, int nrhs, int32_t *cnt, int64_t nsn, int64_t nleaf) {
+                                                       T *V, T *P, int nrhs, int32_t *cnt, int64_t nsn, int64_t nleaf) {
     constexpr int NQ = kLeafQuads<T, NR> ? 4 : 1;
     __shared__ __attribute__((aligned(16))) T lds[NQ][kWholeRows + kTile][NR];
     __shared__ int failed_one;                         // (with quads: in the main path's unused quarter)
@@ -1583,7 +1672,7 @@ __global__ void __launch_bounds__(kBS) fwd_flow_kernel(const FlowTask *__restric
     int *failed;
     if constexpr (NQ > 1) failed = (int *)&lds[1][0][0];
     else failed = &failed_one;
-    fwd_main_task<T, NR>(t, lds[0], lds[0] + kWholeRows, *failed, gch, relmap, L, Linv, Y, W, Z, V, nrhs, cnt, nsn);
+    fwd_main_task<T, NR>(t, lds[0], lds[0] + kWholeRows, *failed, gch, relmap, L, Linv, Y, W, Z, V, P, nrhs, cnt, nsn);
 }
 
 // backward task of a one-panel supernode: x = L_11^{-T} (z - L[update rows, :]' x_u),
@@ -1716,8 +1805,8 @@ constexpr bool kLeafPairs = sizeof(T) * NR <= 64;     // (FP64 with 16 right-han
 template <class T, int NR>
 __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restrict__ tasks, const int32_t *__restrict__ rows,
                                                        const T *__restrict__ L, const T *__restrict__ Linv,
-                                                       const T *__restrict__ Z, T *X, T *Tb, int nrhs, int32_t *cnt,
-                                                       int64_t nsn, int64_t nmain, int64_t nleaf) {
+                                                       const T *__restrict__ Z, T *X, T *Tb, T *P, int nrhs,
+                                                       int32_t *cnt, int64_t nsn, int64_t nmain, int64_t nleaf) {
     constexpr int NB = kBwdStage<T>;
     constexpr int kMain = NB * kTile * (kTile + 1) + kTile * NR;
     constexpr int kPair = 2 * kTile * 17 + kTile * NR;
@@ -1737,7 +1826,7 @@ __global__ void __launch_bounds__(kBS) bwd_flow_kernel(const FlowTask *__restric
         return;
     }
     bwd_main_task<T, NR>(tasks[b], (T (*)[kTile][kTile + 1])lds, (T (*)[NR])(lds + NB * kTile * (kTile + 1)),
-                         flags[0], rows, L, Linv, Z, X, Tb, nrhs, cnt, nsn);
+                         flags[0], rows, L, Linv, Z, X, Tb, P, nrhs, cnt, nsn);
 }
 
 // ----------------------------------------------------------------- refinement helpers
@@ -2218,10 +2307,11 @@ hipError_t numeric(jfem_ctx *ctx, const jfem_symbolic *S, const double *Kval, T 
 // y = (L L')^{-1} y by the dataflow kernels; cnt: ns + 3 words: ns unused, the
 // abort flag (cleared once per refinement call), the two launch tickets (cleared
 // before every solve); V and Tb: the v and t values the panels of a multi-panel
-// supernode exchange (forward and backward)
+// supernode exchange (forward and backward); P: the partial-sum slots of the
+// forward list, then those of the backward list (kTile x nrhs entries each)
 template <class T, int NR>
 hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_factor *F, const T *Y, T *W, T *Z, T *X,
-                         T *V, T *Tb, int nrhs, int32_t *cnt) {
+                         T *V, T *Tb, T *P, int nrhs, int32_t *cnt) {
     const SymbolicHost &H = S->H;
     hipStream_t st = ctx->stream;
     const int64_t nsn = H.ns;
@@ -2231,8 +2321,9 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
     // timing level 2: the kernels on the context's event timeline
     const bool detail = ctx->timing_level >= 2;
     if (detail) tl_begin(ctx);
-    hipLaunchKernelGGL(solve_marks_kernel<T>, dim3(grid_for(std::max(nv, nwv))), dim3(kBS), 0, st, Z, X, V, Tb, nv, W,
-                       nwv, pending_mark<T>(), cnt + nsn + 1);
+    const int64_t npf = H.flow_f_parts * kTile * nrhs, npv = npf + H.flow_b_parts * kTile * nrhs;
+    hipLaunchKernelGGL(solve_marks_kernel<T>, dim3(grid_for(std::max({nv, nwv, npv}))), dim3(kBS), 0, st, Z, X, V, Tb,
+                       nv, W, nwv, P, npv, pending_mark<T>(), cnt + nsn + 1);
     auto lap = [&](int fam) { tl_mark(ctx, fam); };
     if (detail) lap(-1);
     // forward: the bottom subtrees' tasks (the head of the list) one wavefront
@@ -2243,14 +2334,14 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
         const int64_t nleaf = kLeafQuads<T, NR> ? nfl : 0;
         const int64_t grid = (nleaf + 3) / 4 + (int64_t)H.flow_f.size() - nleaf;
         hipLaunchKernelGGL((fwd_flow_kernel<T, NR>), dim3((unsigned)grid), dim3(kBS), 0, st, S->d_flow_f, S->d_flow_gb,
-                           S->d_relmap, L, Linv, Y, W, Z, V, nrhs, cnt, nsn, nleaf);
+                           S->d_relmap, L, Linv, Y, W, Z, V, P, nrhs, cnt, nsn, nleaf);
         if (detail) lap(FAM_FLOW_FWD);
     }
     if (!H.flow_b.empty()) {
         const int64_t nmain = kLeafPairs<T, NR> ? nbm : (int64_t)H.flow_b.size(), nleaf = (int64_t)H.flow_b.size() - nmain;
         const int64_t grid = nmain + (nleaf + 1) / 2;
         hipLaunchKernelGGL((bwd_flow_kernel<T, NR>), dim3((unsigned)grid), dim3(kBS), 0, st, S->d_flow_b, S->d_rows, L,
-                           Linv, (const T *)Z, X, Tb, nrhs, cnt, nsn, nmain, nleaf);
+                           Linv, (const T *)Z, X, Tb, P + npf, nrhs, cnt, nsn, nmain, nleaf);
         if (detail) lap(FAM_FLOW_BWD);
     }
     if (detail) tl_flush(ctx);
@@ -2259,10 +2350,10 @@ hipError_t flow_solve_nr(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_facto
 
 template <class T>
 hipError_t flow_solve(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_factor *F, const T *Y, T *W, T *Z, T *X,
-                      T *V, T *Tb, int nrhs, int32_t *cnt) {
-    if (nrhs <= 4) return flow_solve_nr<T, 4>(ctx, S, F, Y, W, Z, X, V, Tb, nrhs, cnt);
-    if (nrhs <= 8) return flow_solve_nr<T, 8>(ctx, S, F, Y, W, Z, X, V, Tb, nrhs, cnt);
-    return flow_solve_nr<T, 16>(ctx, S, F, Y, W, Z, X, V, Tb, nrhs, cnt);
+                      T *V, T *Tb, T *P, int nrhs, int32_t *cnt) {
+    if (nrhs <= 4) return flow_solve_nr<T, 4>(ctx, S, F, Y, W, Z, X, V, Tb, P, nrhs, cnt);
+    if (nrhs <= 8) return flow_solve_nr<T, 8>(ctx, S, F, Y, W, Z, X, V, Tb, P, nrhs, cnt);
+    return flow_solve_nr<T, 16>(ctx, S, F, Y, W, Z, X, V, Tb, P, nrhs, cnt);
 }
 
 template <class V>
@@ -2630,17 +2721,19 @@ int jfem_factor_inertia(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix
 namespace {
 
 // Solve scratch of one refinement call (in the context's scratch buffer): the
-// factor-precision vectors of the dataflow solves (Y, W, Z, X, V, Tb), their wait words,
+// factor-precision vectors of the dataflow solves (Y, W, Z, X, V, Tb and the
+// partial-sum slots), their wait words,
 // the FP64 iterate, residual and CG vectors (interleaved, n x NR), the global
 // iterate for the SpMV (ndof x NR) and the reduction partials.
 struct SolveLayout {
-    size_t oY, oW, oZs, oXs, oVs, oTs, oCnt, oXh, oXl, oR, oB, oD, oSv, oPv, oQv, oZv, oXgh, oXgl, oPg;
+    size_t oY, oW, oZs, oXs, oVs, oTs, oPs, oCnt, oXh, oXl, oR, oB, oD, oSv, oPv, oQv, oZv, oXgh, oXgl, oPg;
     size_t oPart, oNrm, oRho, oNum, oAl, oBe, oActO, oActI, oTicket, bytes = 0;
     SolveLayout(const SymbolicHost &H, int64_t ndof, int NR, size_t es) {
         const size_t nv = (size_t)H.n * NR, ng = (size_t)ndof * NR;
         auto take = [&](size_t b) { size_t o = bytes; bytes += (b + 255) & ~(size_t)255; return o; };
         oY = take(nv * es); oW = take((size_t)std::max<int64_t>(H.upd_rows, 1) * NR * es);
         oZs = take(nv * es); oXs = take(nv * es); oVs = take(nv * es); oTs = take(nv * es);
+        oPs = take((size_t)std::max<int64_t>(H.flow_f_parts + H.flow_b_parts, 1) * kTile * NR * es);
         oCnt = take((H.ns + 3) * sizeof(int32_t));
         oXh = take(nv * 8); oXl = take(nv * 8); oR = take(nv * 8); oB = take(nv * 8); oD = take(nv * 8);
         oSv = take(nv * 8); oPv = take(nv * 8); oQv = take(nv * 8); oZv = take(nv * 8);
@@ -2692,7 +2785,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
         base = (char *)ctx->scratch;
     }
     T *Y = (T *)(base + lay.oY), *W = (T *)(base + lay.oW), *Zs = (T *)(base + lay.oZs), *Xs = (T *)(base + lay.oXs);
-    T *Vs = (T *)(base + lay.oVs), *Ts = (T *)(base + lay.oTs);
+    T *Vs = (T *)(base + lay.oVs), *Ts = (T *)(base + lay.oTs), *Ps = (T *)(base + lay.oPs);
     int32_t *cnt = (int32_t *)(base + lay.oCnt);
     double *Xh = (double *)(base + lay.oXh), *Xl = (double *)(base + lay.oXl), *R = (double *)(base + lay.oR);
     double *Bp = (double *)(base + lay.oB), *Dd = (double *)(base + lay.oD), *Sv = (double *)(base + lay.oSv);
@@ -2733,7 +2826,7 @@ int solve_refine(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, cons
         auto prec = [&](const double *sv, double *z) -> hipError_t {
             timer_start(ctx);
             hipLaunchKernelGGL(to_t_kernel<T>, gv, blk, 0, st, nv, sv, Y);
-            hipError_t e = flow_solve<T>(ctx, S, F, Y, W, Zs, Xs, Vs, Ts, NR, cnt);
+            hipError_t e = flow_solve<T>(ctx, S, F, Y, W, Zs, Xs, Vs, Ts, Ps, NR, cnt);
             hipLaunchKernelGGL(from_t_kernel<T>, gv, blk, 0, st, nv, (const T *)Xs, z);
             timer_stop(ctx, FAM_SOLVE, 1);
             return e;

@@ -179,6 +179,7 @@ static void order_flow_lists(SymbolicHost &S, const std::vector<int32_t> &snode_
         std::vector<double> out_fin(nsn, 0.0), chain(nsn, 0.0), last_fin(nsn, 0.0), key(S.flow_f.size());
         for (size_t i = 0; i < S.flow_f.size(); ++i) {
             const FlowTask &t = S.flow_f[i];
+            if (t.kind == 2) continue;                 // keyed with the task it precedes, below
             double ready = 0.0;
             for (int32_t q = S.child_ptr[t.s]; q < S.child_ptr[t.s + 1]; ++q) ready = std::max(ready, out_fin[S.child[q]]);
             if (t.nch) ready += lat;
@@ -193,9 +194,12 @@ static void order_flow_lists(SymbolicHost &S, const std::vector<int32_t> &snode_
                 last_fin[t.s] = ready + (t.np > 1 ? lat : 0.0) + base;
             } else {
                 key[i] = chain[t.s] + eps * (t.np + 1);
-                out_fin[t.s] = std::max({out_fin[t.s], ready + base + tile * t.np, last_fin[t.s] + lat + tile});
+                const double stream = tile * (t.k1 - t.k0 + (t.npart > 0 ? kSolveChunk : 0));
+                out_fin[t.s] = std::max({out_fin[t.s], ready + base + stream, last_fin[t.s] + lat + tile});
             }
         }
+        for (size_t i = S.flow_f.size(); i-- > 0;)
+            if (S.flow_f[i].kind == 2) key[i] = key[i + 1];   // (a partial task never ends the list)
         stable_by(S.flow_f, key);
     }
     // backward: a supernode's panel tasks read its update rows' values, which its
@@ -205,14 +209,23 @@ static void order_flow_lists(SymbolicHost &S, const std::vector<int32_t> &snode_
     {
         std::vector<double> first_fin(nsn, 0.0), key(S.flow_b.size());
         std::vector<double> chain_key(nsn, 0.0);
+        std::vector<int32_t> xtask(nsn, 0);
+        auto tpart = [](const FlowTask &t) { return t.kind == 2 && t.u1 > t.u0; };
         for (size_t i = 0; i < S.flow_b.size(); ++i) {
             const FlowTask &t = S.flow_b[i];
+            if (tpart(t)) continue;                    // keyed with the t task it precedes, below
+            if (t.kind == 2 || t.kind == 4) {          // after the panels' t tasks, in list order
+                key[i] = chain_key[t.s] + eps * (t.np + xtask[t.s]++);
+                continue;
+            }
             if (t.p == t.np - 1) {
                 chain_key[t.s] = t.parent >= 0 ? first_fin[t.parent] : 0.0;
-                first_fin[t.s] = chain_key[t.s] + base + tile * (double)(t.u1 - t.u0) / kTile;
+                first_fin[t.s] = chain_key[t.s] + base + tile * (double)std::min<int64_t>(t.ms - t.ns, 2 * kTile * kSolveChunk) / kTile;
             }
             key[i] = chain_key[t.s] + eps * (t.np - 1 - t.p);
         }
+        for (size_t i = S.flow_b.size(); i-- > 0;)
+            if (tpart(S.flow_b[i])) key[i] = key[i + 1];   // (a partial task never ends the list)
         stable_by(S.flow_b, key);
     }
 }
@@ -797,6 +810,30 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
         }
         S.flow_f.push_back(t);
     };
+    // a forward task summing blocks k in [0, kend): its own sum over the last
+    // kSolveChunk, the partial tasks over the rest listed just before it
+    S.flow_f_parts = S.flow_b_parts = 0;
+    auto add_chunked = [&](FlowTask t, int32_t kend, int64_t rows_end) {
+        const int32_t own0 = solve_own_first(kend, kSolveSplit);
+        t.npart = 0;
+        t.pofs = (int32_t)S.flow_f_parts;
+        for (int32_t k = 0; k < own0; k += kSolveChunk) {
+            FlowTask q = t;
+            q.kind = 2;
+            q.k0 = k;
+            q.k1 = std::min(k + kSolveChunk, own0);
+            q.need = 0;
+            q.nch = 0;
+            q.gb = 0;
+            q.npart = 0;
+            q.pofs = (int32_t)S.flow_f_parts++;
+            S.flow_f.push_back(q);
+            ++t.npart;
+        }
+        t.k0 = own0;
+        t.k1 = kend;
+        add_fwd(t, rows_end);
+    };
     for (int32_t l = 0; l < nlev; ++l)
         for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
             const int32_t s = S.level_sn[q];
@@ -806,6 +843,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 t.kind = 1;
                 t.p = 0; t.r0 = 0; t.nr = (int32_t)S.ncol[s];
                 t.iofs = S.iofs[s];
+                t.k0 = t.k1 = t.npart = t.pofs = 0;
                 add_fwd(t, S.m[s]);
                 continue;
             }
@@ -815,7 +853,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
                 t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
                 t.xofs = xofs[s];
-                add_fwd(t, (int64_t)t.r0 + t.nr);
+                add_chunked(t, p, (int64_t)t.r0 + t.nr);
             }
             t.iofs = 0;
             t.xofs = -1;
@@ -823,7 +861,7 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
                 t.p = -1 - u;
                 t.r0 = (int32_t)(S.ncol[s] + (int64_t)u * kTile);
                 t.nr = (int32_t)std::min<int64_t>(kTile, S.m[s] - t.r0);
-                add_fwd(t, (int64_t)t.r0 + t.nr);
+                add_chunked(t, t.np, (int64_t)t.r0 + t.nr);
             }
         }
     for (int32_t l = nlev - 1; l >= 0; --l)
@@ -832,12 +870,61 @@ int build_symbolic(int64_t ndof, const int64_t *rowptr, const int32_t *col, cons
             FlowTask t = base(s);
             t.u0 = (int32_t)S.ncol[s];
             t.u1 = (int32_t)S.m[s];
-            for (int32_t p = t.np - 1; p >= 0; --p) {
+            auto panel = [&](int32_t p) {
                 t.p = p;
                 t.r0 = p * kPanel;
                 t.nr = (int32_t)std::min<int64_t>(kPanel, S.ncol[s] - t.r0);
                 t.iofs = S.iofs[s] + (int64_t)kPanel * kPanel * p;
                 t.xofs = xofs[s];
+                t.npart = 0;
+                t.pofs = 0;
+            };
+            // per panel the partial sums of its far update rows and its t (and x,
+            // up to 2 kSolveChunk later panels); then the split panels' partial
+            // sums over X and x
+            const int32_t nut = (int32_t)((S.m[s] - S.ncol[s] + kTile - 1) / kTile);   // update-row tiles
+            const int32_t urow1 = (int32_t)(S.ncol[s] + (int64_t)kTile * kSolveChunk);
+            const bool usplit = !whole(s) && solve_own_first(nut, kSolveSplitRows) > 0;   // (bottom-subtree tasks never split)
+            for (int32_t p = t.np - 1; p >= 0; --p) {
+                panel(p);
+                const int32_t first = (int32_t)S.flow_b_parts;
+                if (usplit)
+                    for (int64_t r0 = urow1; r0 < S.m[s]; r0 += (int64_t)kTile * kSolveChunk) {
+                        t.kind = 2;
+                        t.k0 = t.k1 = 0;
+                        t.u0 = (int32_t)r0;
+                        t.u1 = (int32_t)std::min<int64_t>(r0 + (int64_t)kTile * kSolveChunk, S.m[s]);
+                        t.pofs = (int32_t)S.flow_b_parts++;
+                        S.flow_b.push_back(t);
+                    }
+                const bool split = solve_own_first(t.np - 1 - p, kSolveSplit) > 0;
+                t.kind = split ? 3 : 0;
+                t.k0 = split ? 0 : p + 1;
+                t.k1 = split ? 0 : t.np;
+                t.u0 = (int32_t)S.ncol[s];
+                t.u1 = usplit ? urow1 : (int32_t)S.m[s];
+                t.npart = (int32_t)S.flow_b_parts - first;
+                t.pofs = first;
+                S.flow_b.push_back(t);
+            }
+            t.u0 = t.u1 = (int32_t)S.ncol[s];                 // (the X tasks read no update rows)
+            for (int32_t p = t.np - 1; p >= 0; --p) {
+                if (solve_own_first(t.np - 1 - p, kSolveSplit) == 0) continue;
+                panel(p);
+                const int32_t own1 = p + 1 + kSolveChunk;
+                const int32_t first = (int32_t)S.flow_b_parts;
+                for (int32_t q0 = own1; q0 < t.np; q0 += kSolveChunk) {
+                    t.kind = 2;
+                    t.k0 = q0;
+                    t.k1 = std::min(q0 + kSolveChunk, t.np);
+                    t.pofs = (int32_t)S.flow_b_parts++;
+                    S.flow_b.push_back(t);
+                }
+                t.kind = 4;
+                t.k0 = p + 1;
+                t.k1 = own1;
+                t.npart = (int32_t)S.flow_b_parts - first;
+                t.pofs = first;
                 S.flow_b.push_back(t);
             }
         }

@@ -26,13 +26,46 @@ struct PushTask {
 //   p < 0: update rows [r0, r0+nr).  Kind 1: the whole front of a supernode with
 //   one panel and at most kWholeRows rows (pivot and update rows in one task).
 // Backward: columns [r0, r0+nr) (panel p) of s; update rows [u0, u1) = [ns, ms).
+// A block sum over more than kSolveSplit panels (kSolveSplitRows 64-row tiles of
+// update rows) is cut into chunks of at most kSolveChunk blocks: a task streams
+// its own chunk and adds the npart partial sums (64 rows x nrhs each, partial
+// slots [pofs, pofs + npart)) that the partial tasks publish, so the blocks are
+// streamed side by side and the consumer waits one exchange for the rest.
+// Forward: kind 0 pivot (p >= 0) sums X[p, k] v_k, update (p < 0) L[rows, k] z_k,
+//   over k in [k0, k1) plus the partials of [0, k0); kind 2: the partial sum over
+//   [k0, k1) of the task that follows, into slot pofs.
+// Backward: kind 0 panel task: t_p over the update rows [u0, u1) plus the
+//   partials of [u1, ms), then x_p over the later panels q in [k0, k1) (= [p+1, np));
+//   a panel with more than kSolveSplit later panels is split: kind 3 forms and
+//   publishes t_p only, kind 4 forms x_p from X_pp t_p, q in [k0, k1) and the
+//   partials of [k1, np).  Kind 2: a partial sum, over update rows [u0, u1) (of a
+//   kind 0 or 3 task, listed just before it) or over panels q in [k0, k1) (of a
+//   kind 4 task).
 constexpr int kWholeRows = 256;  // front rows of a whole-front forward task
+// (the host-side task-list tests also build with smaller values)
+#ifndef JFEM_SOLVE_CHUNK
+#define JFEM_SOLVE_CHUNK 4
+#endif
+#ifndef JFEM_SOLVE_SPLIT
+#define JFEM_SOLVE_SPLIT 4
+#endif
+#ifndef JFEM_SOLVE_SPLIT_ROWS
+#define JFEM_SOLVE_SPLIT_ROWS 16
+#endif
+constexpr int kSolveChunk = JFEM_SOLVE_CHUNK;
+constexpr int kSolveSplit = JFEM_SOLVE_SPLIT;
+constexpr int kSolveSplitRows = JFEM_SOLVE_SPLIT_ROWS;
+// first block of a sum over [0, nb) that its own task streams (the rest in
+// partials): nb - kSolveChunk if nb exceeds the split threshold, else 0
+inline int32_t solve_own_first(int32_t nb, int32_t split) { return nb > split ? nb - kSolveChunk : 0; }
 struct FlowTask {
     int32_t s, p, r0, nr;
     int32_t np, need;               // panels of s; update tasks of s's children (forward)
     int32_t parent, pnp;            // parent supernode (-1 for a root) and its panels
     int32_t nch, kind;              // children of s; task kind
     int32_t u0, u1;                 // backward: update rows [u0, u1) of the front
+    int32_t k0, k1;                 // block range of the task's own sum
+    int32_t npart, pofs;            // partial sums added (consumer) / its slot (kind 2)
     int64_t ns, ms, lofs, c0, relofs, rofs;
     int64_t iofs;                   // inverse of the diagonal block of panel p
     // supernode with np >= 2 panels (pivot and backward tasks; -1 otherwise): the
@@ -101,6 +134,7 @@ struct SymbolicHost {
     // flow_f and the last flow_b_leaf entries of flow_b, run by the one-wavefront
     // leaf kernels (forward before, backward after the main launch)
     int64_t flow_f_leaf = 0, flow_b_leaf = 0;
+    int64_t flow_f_parts = 0, flow_b_parts = 0;   // partial-sum slots of the two lists
     std::vector<FlowChild> flow_gb;
     std::vector<int64_t> iofs;
     int64_t inv_entries = 0;        // diagonal-block inverses, then the X of the multi-panel supernodes (xofs)

@@ -153,6 +153,7 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             if (t.xofs >= 0 && xw[t.s].empty()) xw[t.s].assign((size_t)t.np * t.np, -1);
         }
         for (const FlowTask &t : S.flow_b) {
+            if (t.kind != 0 && t.kind != 3) continue;      // one task per panel forms t_p
             const int b = t.nr;
             const int64_t k = t.r0, ms = t.ms;
             if (b < 1 || b > kPanel || t.iofs < 0 || t.iofs + (int64_t)b * b > S.inv_entries) return 10;
@@ -242,10 +243,47 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 }
         }
         const int64_t nsn = S.ns;
-        std::vector<int32_t> ready(nsn, 0), zdone(nsn, 0), xdone(nsn, 0), vdone(nsn, 0);
+        std::vector<int32_t> ready(nsn, 0), zdone(nsn, 0), vdone(nsn, 0);
         std::vector<double> Y2(n * nrhs), W2(nw * nrhs, 0.0), Z2(n * nrhs, 0.0), X2(n * nrhs, 0.0);
         std::vector<double> V2(n * nrhs, 0.0), T2(n * nrhs, 0.0);
         std::vector<char> wW2(nw, 0), wZ2(n, 0), wX2(n, 0), wV2(n, 0), wT2(n, 0);
+        // partial sums of the chunked block sums: value, written flag, (s, r0, k0, k1)
+        const int64_t npf = std::max<int64_t>(S.flow_f_parts, 1), npb = std::max<int64_t>(S.flow_b_parts, 1);
+        std::vector<double> PF2(npf * kTile * nrhs, 0.0), PB2(npb * kTile * nrhs, 0.0);
+        std::vector<char> wPF(npf, 0), wPB(npb, 0);
+        std::vector<int64_t> idPF(4 * npf, -1), idPB(4 * npb, -1);
+        // a partial task's result into its slot (written once)
+        auto put_part = [&](const FlowTask &t, int64_t nparts, std::vector<double> &P2, std::vector<char> &wP,
+                            std::vector<int64_t> &id, const std::vector<double> &acc) -> int {
+            if (t.pofs < 0 || t.pofs >= nparts || wP[t.pofs]) return 11;
+            wP[t.pofs] = 1;
+            const bool rows = t.u1 > t.u0;                // backward: over update rows, else over blocks
+            id[4 * t.pofs] = t.s; id[4 * t.pofs + 1] = t.r0;
+            id[4 * t.pofs + 2] = rows ? t.u0 : t.k0; id[4 * t.pofs + 3] = rows ? t.u1 : t.k1;
+            const int64_t st = (int64_t)kTile * nrhs;
+            for (int i = 0; i < t.nr; ++i)
+                for (int r = 0; r < nrhs; ++r) {
+                    const double *a = &acc[i * nrhs + r];
+                    P2[(t.pofs * kTile + i) * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                }
+            return 0;
+        };
+        // a consumer's partials: slots [pofs, pofs + npart), written, of its rows,
+        // covering the blocks [lo, hi) in order; their sums added to out (nr x nrhs)
+        auto add_parts = [&](const FlowTask &t, int64_t nparts, const std::vector<double> &P2, const std::vector<char> &wP,
+                             const std::vector<int64_t> &id, int32_t lo, int32_t hi, std::vector<double> &out) -> int {
+            if (t.npart < 0 || (t.npart > 0 && (t.pofs < 0 || t.pofs + t.npart > nparts))) return 10;
+            int64_t at = lo;
+            for (int u = 0; u < t.npart; ++u) {
+                const int64_t q = t.pofs + u;
+                if (!wP[q]) return 9;
+                if (id[4 * q] != t.s || id[4 * q + 1] != t.r0 || id[4 * q + 2] != at) return 10;
+                at = id[4 * q + 3];
+                for (int i = 0; i < t.nr; ++i)
+                    for (int r = 0; r < nrhs; ++r) out[i * nrhs + r] += P2[(q * kTile + i) * nrhs + r];
+            }
+            return at == hi ? 0 : 10;
+        };
         for (int64_t i = 0; i < n; ++i)
             for (int r = 0; r < nrhs; ++r) Y2[i * nrhs + r] = b[S.perm[i] + ndof * r];
         // an L index formed for block (s, rows [r0, r0+nr), cols [c, c+nc)), clamped as load_cols16 does
@@ -286,6 +324,12 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
             const bool multi = pivot && t.xofs >= 0;     // z_p through X and the v of the earlier panels
             if (multi != (pivot && t.kind != 1 && t.np >= 2)) return 10;
             const int kend = pivot ? t.p : t.np;
+            const bool part = t.kind == 2;
+            if (t.kind < 0 || t.kind > 2) return 10;
+            if (part && !(t.k0 >= 0 && t.k0 < t.k1 && t.k1 - t.k0 <= kSolveChunk && t.k1 <= kend - kSolveChunk))
+                return 10;
+            if (t.kind == 0 && (t.k1 != kend || t.k0 != solve_own_first(kend, kSolveSplit))) return 10;
+            if (part && (t.need != 0 || t.nch != 0)) return 10;
             const int nr = t.nr;
             if (t.s < 0 || t.s >= nsn || nr < 1 || nr > kTile) return 10;
             if (pivot && (t.r0 != kPanel * t.p || t.r0 + nr > t.ns)) return 10;
@@ -343,7 +387,7 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 if (t.parent >= 0) ready[t.parent] += 1;
                 continue;
             }
-            if (multi) {                                 // v_p published for the later panels of s
+            if (multi && !part) {                        // v_p published for the later panels of s
                 if (vdone[t.s] != t.p) return 9;
                 for (int i = 0; i < nr; ++i) {
                     const int64_t vi = t.c0 + t.r0 + i;
@@ -354,7 +398,7 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                 vdone[t.s] = t.p + 1;
             }
             std::vector<double> acc(4 * kTile * nrhs, 0.0);
-            for (int k = 0; k < kend; ++k) {
+            for (int k = t.k0; k < t.k1; ++k) {
                 const int64_t col = (int64_t)kPanel * k;
                 const int bk = (int)std::min<int64_t>(kPanel, t.ns - col);
                 if (multi) {
@@ -377,34 +421,38 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                                 acc[(w * kTile + i) * nrhs + r] += m * src[(t.c0 + col + c) * nrhs + r];
                         }
             }
+            if (part) {
+                if (const int rc = put_part(t, S.flow_f_parts, PF2, wPF, idPF, acc)) return rc;
+                continue;
+            }
             if (pivot && (t.iofs < 0 || t.iofs + (int64_t)nr * nr > S.inv_entries)) return 10;
-            if (multi) {
-                // + X_pp v_p, one reduction for the whole sum
+            std::vector<double> sum(kTile * nrhs, 0.0);   // the block sum, then the partials of [0, k0)
+            if (multi)                                   // + X_pp v_p, one reduction for the whole sum
                 for (int w = 0; w < 4; ++w)
                     for (int i = 0; i < nr; ++i)
                         for (int c = 16 * w; c < std::min(16 * w + 16, nr); ++c)
                             for (int r = 0; r < nrhs; ++r)
                                 acc[(w * kTile + i) * nrhs + r] += Linv[t.iofs + i + (int64_t)nr * c] * vs[c * nrhs + r];
+            for (int i = 0; i < nr; ++i)
+                for (int r = 0; r < nrhs; ++r) {
+                    const double *a = &acc[i * nrhs + r];
+                    const int64_t st = (int64_t)kTile * nrhs;
+                    sum[i * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                }
+            if (const int rc = add_parts(t, S.flow_f_parts, PF2, wPF, idPF, 0, t.k0, sum)) return rc;
+            if (multi) {
                 for (int i = 0; i < nr; ++i) {
                     const int64_t zi = t.c0 + t.r0 + i;
                     if (wZ2[zi]) return 11;
                     wZ2[zi] = 1;
-                    for (int r = 0; r < nrhs; ++r) {
-                        const double *a = &acc[i * nrhs + r];
-                        const int64_t st = (int64_t)kTile * nrhs;
-                        Z2[zi * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
-                    }
+                    for (int r = 0; r < nrhs; ++r) Z2[zi * nrhs + r] = sum[i * nrhs + r];
                 }
                 zdone[t.s] += 1;
                 continue;
             }
             std::vector<double> v(kTile * nrhs, 0.0);
             for (int i = 0; i < nr; ++i)
-                for (int r = 0; r < nrhs; ++r) {
-                    const double *a = &acc[i * nrhs + r];
-                    const int64_t st = (int64_t)kTile * nrhs;
-                    v[i * nrhs + r] = vs[i * nrhs + r] - (((a[0] + a[st]) + a[2 * st]) + a[3 * st]);
-                }
+                for (int r = 0; r < nrhs; ++r) v[i * nrhs + r] = vs[i * nrhs + r] - sum[i * nrhs + r];
             if (!pivot) {
                 for (int i = 0; i < nr; ++i) {
                     const int64_t wi = t.relofs + t.r0 - t.ns + i;
@@ -437,49 +485,87 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
         }
         for (int64_t i = 0; i < n; ++i)
             if (!wZ2[i]) return 11;
+        std::vector<int32_t> tdone(nsn, 0);
         for (size_t e = 0; e < S.flow_b.size(); ++e) {
             const FlowTask &t = S.flow_b[e];
             const int nc = t.nr;
             if (t.parent >= 0 && t.pnp != (int32_t)((S.ncol[t.parent] + kPanel - 1) / kPanel)) return 10;
-            if (t.u0 != t.ns || t.u1 != t.ms) return 10;
             const bool multi = t.xofs >= 0;              // x_p through X and the t of the later panels
             if (multi != (t.np >= 2)) return 10;
-            if (xdone[t.s] != t.np - 1 - t.p) return 9;
+            const bool tphase = t.kind == 0 || t.kind == 3;
+            const bool tpart = t.kind == 2 && t.u1 > t.u0;   // partial sum over far update rows
+            const int later = t.np - 1 - t.p;            // panels after p
+            const int nut = (int)((t.ms - t.ns + kTile - 1) / kTile);
+            const bool whole = t.ns <= kPanel && t.ms <= kWholeRows;   // never split
+            const int64_t urow1 = !whole && solve_own_first(nut, kSolveSplitRows) > 0 ? t.ns + (int64_t)kTile * kSolveChunk : t.ms;
+            if (tphase && (t.u0 != t.ns || t.u1 != urow1)) return 10;
+            if (!tphase && !tpart && t.u0 != t.u1) return 10;
+            if (tpart && !(t.u0 >= urow1 && t.u1 <= t.ms && t.u1 - t.u0 <= kTile * kSolveChunk && t.k0 == t.k1 &&
+                           (t.u0 - t.ns) % kTile == 0))
+                return 10;
+            if (t.kind == 0 && (solve_own_first(later, kSolveSplit) > 0 || t.k0 != t.p + 1 || t.k1 != t.np)) return 10;
+            if (t.kind == 3 && (solve_own_first(later, kSolveSplit) == 0 || !multi || t.k0 != t.k1)) return 10;
+            if (t.kind == 2 && !tpart &&
+                !(multi && t.k0 >= t.p + 1 + kSolveChunk && t.k0 < t.k1 && t.k1 <= t.np && t.k1 - t.k0 <= kSolveChunk))
+                return 10;
+            if (t.kind == 4 && (solve_own_first(later, kSolveSplit) == 0 || t.k0 != t.p + 1 || t.k1 != t.p + 1 + kSolveChunk)) return 10;
+            if (t.kind == 1 || t.kind < 0 || t.kind > 4) return 10;
             std::vector<double> acc(4 * kTile * nrhs, 0.0);
-            const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;
-            for (int e2 = 0; e2 < nchunk; ++e2) {      // update rows (ancestors' values), from the last tile
-                const int64_t row0 = t.u0 + (int64_t)kTile * (nchunk - 1 - e2);
-                const int nre = (int)std::min<int64_t>(kTile, (int64_t)t.u1 - row0);
-                const int64_t lo = t.lofs + row0 + t.ms * t.r0, hi = lo + (nre - 1) + t.ms * (nc - 1);
-                if (!(row0 >= 0 && row0 + nre <= t.ms && hi < t.lofs + t.ms * t.ns)) return 10;
-                for (int i = 0; i < nre; ++i) {
-                    if (t.rofs + row0 + i >= (int64_t)S.rows.size()) return 10;
-                    const int64_t g = S.rows[t.rofs + row0 + i];
-                    if (g < 0 || g >= n || !wX2[g]) return 11;
-                    for (int c = 0; c < nc; ++c) {
-                        const int w = i / 16;
-                        for (int r = 0; r < nrhs; ++r)
-                            acc[(w * kTile + c) * nrhs + r] += L[t.lofs + (row0 + i) + t.ms * (t.r0 + c)] * X2[g * nrhs + r];
+            std::vector<double> v(kTile * nrhs, 0.0);     // t_p
+            if (tphase || tpart) {
+                if (tphase) {
+                    if (tdone[t.s] != later) return 9;   // the panels' t tasks from the last panel
+                    tdone[t.s] += 1;
+                }
+                const int nchunk = (t.u1 - t.u0 + kTile - 1) / kTile;
+                for (int e2 = 0; e2 < nchunk; ++e2) {  // update rows (ancestors' values), from the last tile
+                    const int64_t row0 = t.u0 + (int64_t)kTile * (nchunk - 1 - e2);
+                    const int nre = (int)std::min<int64_t>(kTile, (int64_t)t.u1 - row0);
+                    const int64_t lo = t.lofs + row0 + t.ms * t.r0, hi = lo + (nre - 1) + t.ms * (nc - 1);
+                    if (!(row0 >= 0 && row0 + nre <= t.ms && hi < t.lofs + t.ms * t.ns)) return 10;
+                    for (int i = 0; i < nre; ++i) {
+                        if (t.rofs + row0 + i >= (int64_t)S.rows.size()) return 10;
+                        const int64_t g = S.rows[t.rofs + row0 + i];
+                        if (g < 0 || g >= n || !wX2[g]) return 11;
+                        for (int c = 0; c < nc; ++c) {
+                            const int w = i / 16;
+                            for (int r = 0; r < nrhs; ++r)
+                                acc[(w * kTile + c) * nrhs + r] += L[t.lofs + (row0 + i) + t.ms * (t.r0 + c)] * X2[g * nrhs + r];
+                        }
                     }
                 }
-            }
-            std::vector<double> v(kTile * nrhs, 0.0);     // t_p
-            for (int c = 0; c < nc; ++c)
-                for (int r = 0; r < nrhs; ++r) {
-                    const double *a = &acc[c * nrhs + r];
-                    const int64_t st = (int64_t)kTile * nrhs;
-                    v[c * nrhs + r] = Z2[(t.c0 + t.r0 + c) * nrhs + r] - (((a[0] + a[st]) + a[2 * st]) + a[3 * st]);
+                if (tpart) {
+                    if (const int rc = put_part(t, S.flow_b_parts, PB2, wPB, idPB, acc)) return rc;
+                    continue;
                 }
-            if (multi)
+                std::vector<double> us(kTile * nrhs, 0.0);   // update-row sum, then the far rows' partials
+                for (int c = 0; c < nc; ++c)
+                    for (int r = 0; r < nrhs; ++r) {
+                        const double *a = &acc[c * nrhs + r];
+                        const int64_t st = (int64_t)kTile * nrhs;
+                        us[c * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                    }
+                if (const int rc = add_parts(t, S.flow_b_parts, PB2, wPB, idPB, t.u1, (int32_t)t.ms, us)) return rc;
+                for (int c = 0; c < nc; ++c)
+                    for (int r = 0; r < nrhs; ++r) v[c * nrhs + r] = Z2[(t.c0 + t.r0 + c) * nrhs + r] - us[c * nrhs + r];
+                if (multi)
+                    for (int c = 0; c < nc; ++c) {
+                        const int64_t ti = t.c0 + t.r0 + c;
+                        if (wT2[ti]) return 11;
+                        wT2[ti] = 1;
+                        for (int r = 0; r < nrhs; ++r) T2[ti * nrhs + r] = v[c * nrhs + r];
+                    }
+                if (t.kind == 3) continue;
+                std::fill(acc.begin(), acc.end(), 0.0);
+            } else if (t.kind == 4) {                    // t_p as the panel's t task published it
                 for (int c = 0; c < nc; ++c) {
                     const int64_t ti = t.c0 + t.r0 + c;
-                    if (wT2[ti]) return 11;
-                    wT2[ti] = 1;
-                    for (int r = 0; r < nrhs; ++r) T2[ti * nrhs + r] = v[c * nrhs + r];
+                    if (!wT2[ti]) return 11;
+                    for (int r = 0; r < nrhs; ++r) v[c * nrhs + r] = T2[ti * nrhs + r];
                 }
-            std::fill(acc.begin(), acc.end(), 0.0);
+            }
             if (multi)
-                for (int q = t.np - 1; q > t.p; --q) {   // later panels of X, from the last
+                for (int q = t.k1 - 1; q >= t.k0; --q) { // later panels of X, from the last
                     const int64_t row0 = (int64_t)kPanel * q;
                     const int nre = (int)std::min<int64_t>(kTile, t.ns - row0);
                     const int64_t lo = t.xofs + row0 + t.ns * t.r0, hi = lo + (nre - 1) + t.ns * (nc - 1);
@@ -493,21 +579,29 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
                                     Linv[t.xofs + (row0 + i) + t.ns * (t.r0 + c)] * T2[g * nrhs + r];
                     }
                 }
+            if (t.kind == 2) {
+                if (const int rc = put_part(t, S.flow_b_parts, PB2, wPB, idPB, acc)) return rc;
+                continue;
+            }
             for (int i = 0; i < nc; ++i)
                 for (int c = 0; c < nc; ++c)
                     for (int r = 0; r < nrhs; ++r)
                         acc[((i / 16) * kTile + c) * nrhs + r] += Linv[t.iofs + i + (int64_t)nc * c] * v[i * nrhs + r];
+            std::vector<double> sum(kTile * nrhs, 0.0);
+            for (int c = 0; c < nc; ++c)
+                for (int r = 0; r < nrhs; ++r) {
+                    const double *a = &acc[c * nrhs + r];
+                    const int64_t st = (int64_t)kTile * nrhs;
+                    sum[c * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
+                }
+            if (t.kind == 4)
+                if (const int rc = add_parts(t, S.flow_b_parts, PB2, wPB, idPB, t.k1, t.np, sum)) return rc;
             for (int c = 0; c < nc; ++c) {
                 const int64_t xi = t.c0 + t.r0 + c;
                 if (wX2[xi]) return 11;
                 wX2[xi] = 1;
-                for (int r = 0; r < nrhs; ++r) {
-                    const double *a = &acc[c * nrhs + r];
-                    const int64_t st = (int64_t)kTile * nrhs;
-                    X2[xi * nrhs + r] = ((a[0] + a[st]) + a[2 * st]) + a[3 * st];
-                }
+                for (int r = 0; r < nrhs; ++r) X2[xi * nrhs + r] = sum[c * nrhs + r];
             }
-            xdone[t.s] += 1;
         }
         for (int64_t i = 0; i < n; ++i)
             if (!wX2[i]) return 11;
@@ -524,5 +618,7 @@ extern "C" int harness_solve(int64_t ndof, const int64_t *rowptr, const int32_t 
     info[5] = (int64_t)S.utasks.size();
     info[6] = S.max_front;
     info[7] = S.flow_f_leaf;
+    info[8] = S.flow_f_parts;
+    info[9] = S.flow_b_parts;
     return 0;
 }

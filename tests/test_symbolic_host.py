@@ -14,10 +14,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 
 
-@pytest.fixture(scope="module")
-def harness(tmp_path_factory):
-    out = str(tmp_path_factory.mktemp("harness") / "libharness.so")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", out,
+def _build_harness(tmp_path_factory, chunk=None):
+    # chunk None: the library's own split of the solve sums; else partial sums of
+    # `chunk` blocks for sums over more than 2 chunk blocks
+    out = str(tmp_path_factory.mktemp("harness") / f"libharness{chunk or ''}.so")
+    defs = [] if chunk is None else [f"-DJFEM_SOLVE_CHUNK={chunk}", f"-DJFEM_SOLVE_SPLIT={2 * chunk}",
+                                     f"-DJFEM_SOLVE_SPLIT_ROWS={2 * chunk}"]
+    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", *defs, "-o", out,
                     os.path.join(HERE, "symbolic_harness.cpp"),
                     os.path.join(ROOT, "jf_2026_05_25_amd", "csrc", "symbolic.cpp")], check=True)
     L = ctypes.CDLL(out)
@@ -27,13 +30,25 @@ def harness(tmp_path_factory):
     return L
 
 
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    return _build_harness(tmp_path_factory)
+
+
+@pytest.fixture(scope="module")
+def harness_chunk1(tmp_path_factory):
+    # block sums over more than two blocks split into one-block partial tasks, so
+    # that the small models run every kind of split task
+    return _build_harness(tmp_path_factory, 1)
+
+
 def run(L, K, xyz, fixed, B):
     K = K.tocsr(); K.sort_indices()
     n = K.shape[0]
     B = np.asarray(B, float).reshape(n, -1)
     nrhs = B.shape[1]
     Bc = np.ascontiguousarray(B.T).ravel(); X = np.zeros_like(Bc)
-    info = np.zeros(8, np.int64)
+    info = np.zeros(10, np.int64)
     rp = K.indptr.astype(np.int64); ci = K.indices.astype(np.int32); v = np.ascontiguousarray(K.data)
     xyzc = np.ascontiguousarray(xyz, float).ravel(); fx = np.ascontiguousarray(fixed, np.uint8)
     st = L.harness_solve(n, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, xyzc.ctypes.data,
@@ -53,12 +68,13 @@ MODELS = {
 }
 
 
+@pytest.mark.parametrize("chunk", [4, 1])
 @pytest.mark.parametrize("name", list(MODELS))
-def test_task_decomposition_solves(harness, oracle, name):
+def test_task_decomposition_solves(harness, harness_chunk1, oracle, name, chunk):
     m = MODELS[name]()
     K = oracle.assemble(m)
     fixed = oracle.bc_partition(K, m.spc)
-    x, info = run(harness, K, m.xyz, fixed, m.loads)
+    x, info = run(harness if chunk == 4 else harness_chunk1, K, m.xyz, fixed, m.loads)
     free = np.where(fixed == 0)[0]
     Kd = K.toarray()[np.ix_(free, free)]
     Ku = np.triu(Kd); Ks = Ku + np.triu(Ku, 1).T
@@ -68,6 +84,8 @@ def test_task_decomposition_solves(harness, oracle, name):
     ns, nlev, nL, nU, npt, nut, maxf, nleaf = info[:8]
     assert ns > 1 and nlev > 1 and nL >= np.count_nonzero(np.tril(Ks))
     assert 0 < nleaf < ns            # bottom subtrees (one wavefront per task) and the rest
+    if chunk == 1 and name in ("box", "block5"):
+        assert info[8] > 0 and info[9] > 0   # partial-sum tasks in both lists
 
 
 def test_fill_is_nested_dissection_quality(harness, oracle):
