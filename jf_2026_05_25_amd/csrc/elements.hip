@@ -2241,6 +2241,7 @@ t3_kg_kernel(int64_t nq, int64_t nt, const double *__restrict__ x, const double 
 namespace jfem {
 
 hipError_t launch_reset_status(hipStream_t s, int64_t *status) {
+    clear_stale_hip_error();                           // (the first launch of an element or assembly call)
     hipLaunchKernelGGL(reset_status_kernel, dim3(1), dim3(1), 0, s, status);
     return hipGetLastError();
 }

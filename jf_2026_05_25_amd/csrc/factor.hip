@@ -2490,6 +2490,7 @@ static int do_numeric(jfem_ctx *ctx, jfem_factor *F, const jfem_matrix *K, int64
 
 int jfem_factor_create(jfem_ctx *ctx, const jfem_symbolic *S, const jfem_matrix *K, int precision, jfem_factor **out,
                        int64_t *bad_column) {
+    clear_stale_hip_error();
     *out = nullptr;
     if (precision != 64 && precision != 32) return fail(ctx, JFEM_ERR_BAD_ARG, "precision must be 32 or 64");
     jfem_factor *F = new jfem_factor();
@@ -3001,6 +3002,7 @@ static int check_solve_args(jfem_ctx *ctx, const jfem_factor *F, const jfem_matr
 int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_dev,
                    double *x_dev, double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev,
                    double *relres_out) {
+    clear_stale_hip_error();
     const int rc = check_solve_args(ctx, F, K, nrhs, max_refine);
     if (rc != JFEM_OK) return rc;
     return solve_dev_impl(ctx, F, K, nrhs, b_dev, x_dev, tol, max_refine, iters, relres, xlo_dev, relres_out, nullptr);
@@ -3009,6 +3011,7 @@ int jfem_solve_dev(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, in
 int jfem_solve_dev_static(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_dev,
                           double *x_dev, double tol, int max_refine, int32_t *iters, double *relres, double *xlo_dev,
                           double *relres_out) {
+    clear_stale_hip_error();
     int rc = check_solve_args(ctx, F, K, nrhs, max_refine);
     if (rc != JFEM_OK) return rc;
     const int64_t need = jfem_solve_scratch_bytes(F->S, K->P->ndof, nrhs, F->precision);
@@ -3104,6 +3107,7 @@ int jfem_selftest_update(jfem_ctx *ctx, int precision, double *max_rel_diff) {
 
 int jfem_solve(jfem_ctx *ctx, const jfem_factor *F, const jfem_matrix *K, int nrhs, const double *b_host, double *x_host,
                double tol, int max_refine, int32_t *iters, double *relres, double *xlo_host, double *relres_out) {
+    clear_stale_hip_error();
     const int64_t ndof = K->P->ndof;
     const int64_t cnt = std::max<int64_t>(ndof * nrhs, 1);
     double *d_b = nullptr, *d_x = nullptr, *d_xl = nullptr;

@@ -113,6 +113,12 @@ struct jfem_batch {
 
 // per-variant strides rounded up to 256 bytes, so every variant's storage starts
 // as aligned as a single allocation
+// A HIP call whose status is not taken (a free on a release path) leaves its
+// error in the calling thread's last-error slot, where the next launch check
+// (hipGetLastError after a launch) would report it as its own: the entry points
+// that check launches that way start from a cleared slot.
+inline void clear_stale_hip_error() { (void)hipGetLastError(); }
+
 inline int64_t jfem_stride(int64_t n, int64_t es) {
     const int64_t q = 256 / es;
     return ((n + q - 1) / q) * q;
